@@ -1,0 +1,168 @@
+"""Residual-path benchmark on one MI355X.
+
+A step is one FP64 residual evaluation R(W) over the whole mesh (boundary
+ghosts, gradients + limiters, face fluxes), on a synthetic state resident in
+HBM.  Prints one JSON line: whole-job cell-residual evaluations per second,
+the roofline of the dominant kernel from HIP events recorded live around each
+kernel, and a CPU baseline (the CPU restatement on a bounded sample).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plate]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def make_case(name, scale=1):
+    from freecfd_amd import cases
+    if name == "plate":
+        return cases.plate_case(512 // scale, 128 // scale, 64 // scale, noise=0.01)
+    if name == "bump":
+        return cases.bump_case(256 // scale, 96 // scale, 64 // scale, noise=0.01)
+    if name == "tetbox":
+        return cases.tetbox_case(64 // scale)
+    if name == "cylinder":
+        return cases.cylinder_case(256 // scale, 128 // scale, noise=0.01)
+    if name == "wedge":
+        return cases.wedge_case(96 // scale, 48 // scale)
+    raise SystemExit("unknown config " + name)
+
+
+WORKLOADS = {
+    "plate": "flat plate 512x128x64 hex, top 20% layers split into prisms, Mach 0.3, laminar viscous Roe, "
+             "2nd-order curvilinear gradients + Venkatakrishnan limiter, 5 equations/cell",
+    "bump": "3-D bump channel 256x96x64 hex, Mach 0.85, inviscid Roe, curvilinear gradients + Venkatakrishnan",
+    "tetbox": "unit box 64^3 hexes x 6 tets, Mach 0.5, viscous Roe, lsqr gradients + Venkatakrishnan",
+    "cylinder": "cylinder O-grid 256x128x1, Mach 0.2 Re 40, AUSM+up, Sutherland, lsqr gradients",
+    "wedge": "Mach 2 compression ramp 96x48x1, Roe, lsqr gradients + minmod",
+}
+
+
+def model_bytes(info, n_grad, has_phi, sw=False):
+    """Algorithmic (compulsory) HBM bytes per launch of each kernel."""
+    N, B, F, H = info.n_cells, info.n_ghosts, info.n_faces, info.n_halffaces
+    face_flux = F * (12 + 24 + 24 + 8 + 24 + 16 + (16 if sw else 0))   # topology+bc, n, t1, area, centroid, weights
+    flux = (face_flux + H * 4 + (N + 1) * 4 + (N + B) * 5 * 8 + N * 15 * 8 + (N * 5 * 8 if has_phi else 0)
+            + (N + B) * 24 + N * 5 * 8)
+    grad = ((N + B) * 5 * 8 + n_grad * (4 + 24) + (N + 1) * 4 + H * 4 + F * (8 + 24) + N * 24
+            + (N + B) * 15 * 8 + (N * 5 * 8 if has_phi else 0))
+    ghost = B * (4 + 4 + 4 + 24 + 24 + 8 + 5 * 8 + 5 * 8)
+    return {"ghost": ghost, "grad_lim": grad, "flux": flux}
+
+
+def cpu_baseline(name, target_s=12.0):
+    """The CPU restatement (one core) on a bounded sample of the same workload
+    family: the same configuration at 1/8 of the mesh size per direction."""
+    from oracle.oracle import Oracle
+    cs = make_case(name, scale=8)
+    o = Oracle(cs.raw, cs.opt)
+    o.residual(cs.W0)   # warm
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        o.residual(cs.W0)
+        n += 1
+        if time.perf_counter() - t0 > target_s:
+            break
+    dt = (time.perf_counter() - t0) / n
+    return {"value": cs.n_cells / dt, "unit": "cell-residual evals/s", "cores": 1, "kind": "port",
+            "sample": "%s at 1/8 scale per direction (%d cells), %d residual evaluations (update_boundaries + "
+                      "calc_cell_grads + calc_limiter + residual face loop), %.1f s" % (name, cs.n_cells, n, n * dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="plate", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the residual path is measured on one MI355X); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from freecfd_amd import _abi as A
+    from freecfd_amd._lib import lib
+    from freecfd_amd.ns import NavierStokes
+
+    t0 = time.perf_counter()
+    cs = make_case(args.config)
+    print("bench: mesh %s, %d cells (%.1f s)" % (args.config, cs.n_cells, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ns = NavierStokes(cs.raw, cs.opt)
+    print("bench: set-up %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    ns.set_state(cs.W0)
+    info = ns.info
+    for _ in range(args.warmup):
+        ns.assemble_residual()
+    torch.cuda.synchronize()
+    ns.status()
+
+    L = lib()
+    L.fcfd_profile(ns.h, 1, None, None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ns.assemble_residual()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stage = (C.c_double * 3)()
+    calls = C.c_longlong()
+    L.fcfd_profile(ns.h, 0, stage, C.byref(calls))
+    ns.status()
+    R = ns.R[:, :info.n_cells]
+    if not bool(torch.isfinite(R).all()):
+        raise SystemExit("non-finite residual")
+
+    ms_step = 1e3 * elapsed / args.steps
+    evals_s = args.steps / elapsed
+    names = ["ghost", "grad_lim", "flux"]
+    avg_ms = {names[k]: stage[k] / max(calls.value, 1) for k in range(3)}
+    has_phi = cs.opt.limiter != A.LIMITER_NONE and cs.opt.order == A.ORDER_SECOND
+    mb = model_bytes(info, info.n_gradmap, has_phi, sw=cs.opt.convective_flux == A.FLUX_SW)
+    dom = max(avg_ms, key=lambda k: avg_ms[k])
+    achieved = mb[dom] / (avg_ms[dom] * 1e-3) / 1e9
+    total_bytes = sum(mb.values())
+    line = {
+        "metric": "cell-residual evals/s",
+        "value": cs.n_cells * evals_s,
+        "unit": "cell-residual evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS[args.config], "cells": info.n_cells, "faces": info.n_faces,
+                   "boundary_faces": info.n_ghosts, "half_faces": info.n_halffaces,
+                   "residual_evals_per_s": evals_s,
+                   "stage_ms": {k: round(v, 5) for k, v in avg_ms.items()},
+                   "model_bytes_per_eval": total_bytes,
+                   "residual_hbm_gbs_model": total_bytes / (ms_step * 1e-3) / 1e9},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": mb[dom], "avg_launch_ms": avg_ms[dom]},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.config)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,254 @@
+"""The five benchmark configurations as (raw mesh, options, initial state).
+
+Inputs mirror what a Free CFD input deck sets (src/read_inputs.cc defaults,
+src/navier_stokes/ns_set_bcs.cc for BC values); every state array is N x 5
+primitive (p, u, v, w, T) in raw (= reference) cell order.
+"""
+import numpy as np
+
+from . import _abi as A
+from . import meshgen
+
+
+def gas_R(Mw):
+    return A.UNIV_GAS_CONST / Mw
+
+
+# ---------------------------------------------------------------------------
+# boundary conditions (what NavierStokes::set_bcs derives from an input deck)
+# ---------------------------------------------------------------------------
+
+def bc_symmetry():
+    return dict(type=A.BC_SYMMETRY, kind=A.KIND_NONE, specified=A.SPEC_NONE, thermal=A.THERMAL_ADIABATIC)
+
+
+def bc_wall(slip=False, T=None, qdot=None):
+    d = dict(type=A.BC_WALL, kind=A.KIND_SLIP if slip else A.KIND_NONE, specified=A.SPEC_NONE,
+             thermal=A.THERMAL_ADIABATIC)
+    if not slip:
+        d.update(V_fixed=1, V=(0.0, 0.0, 0.0))
+    if T is not None:
+        d.update(thermal=A.THERMAL_FIXED_T, T=T, T_fixed=1, specified=A.SPEC_T)
+    if qdot is not None:
+        d.update(thermal=A.THERMAL_FIXED_Q, qdot=qdot, qdot_fixed=1)
+    return d
+
+
+def bc_inlet_state(p, T, V, R):
+    """velocity inlet with p and T given: specified = BC_STATE (ns_set_bcs.cc:52-74)."""
+    return dict(type=A.BC_INLET, kind=A.KIND_VELOCITY, specified=A.SPEC_STATE, thermal=A.THERMAL_FIXED_T,
+                p=p, T=T, rho=p / (R * T), V=tuple(V), p_fixed=1, T_fixed=1, rho_fixed=1, V_fixed=1)
+
+
+def bc_inlet_T(T, V):
+    """velocity inlet with only T given: specified = BC_T (ns_set_bcs.cc:86-97)."""
+    return dict(type=A.BC_INLET, kind=A.KIND_VELOCITY, specified=A.SPEC_T, thermal=A.THERMAL_FIXED_T,
+                T=T, V=tuple(V), T_fixed=1, V_fixed=1)
+
+
+def bc_stagnation(p_total, T_total):
+    return dict(type=A.BC_INLET, kind=A.KIND_STAGNATION, specified=A.SPEC_NONE, thermal=A.THERMAL_ADIABATIC,
+                p_total=p_total, T_total=T_total)
+
+
+def bc_outlet_p(p):
+    return dict(type=A.BC_OUTLET, kind=A.KIND_NONE, specified=A.SPEC_P, thermal=A.THERMAL_ADIABATIC,
+                p=p, p_fixed=1)
+
+
+def bc_outlet_extrapolate():
+    return dict(type=A.BC_OUTLET, kind=A.KIND_NONE, specified=A.SPEC_NONE, thermal=A.THERMAL_ADIABATIC)
+
+
+def make_options(bcs, gamma=1.4, Mw=28.97, Pref=0.0, Tref=0.0, mu=0.0, lam=0.0, sutherland=None, Pr=None,
+                 order=A.ORDER_SECOND, jac_order=A.ORDER_FIRST, limiter=A.LIMITER_VK,
+                 flux=A.FLUX_AUSM_PLUS_UP, limiter_threshold=0.0, Minf=1.0, wdiss=0.3, bl_height=0.0,
+                 grad=(A.GRAD_CURVILINEAR, A.GRAD_CURVILINEAR, A.GRAD_LSQR), interp=A.INTERP_WTLI,
+                 stencil=2, loads_bcs=(), moment_center=(0.0, 0.0, 0.0)):
+    """Defaults are the reference's (src/read_inputs.cc:47-54,70-75,125-134)."""
+    o = A.Options()
+    o.gamma, o.Mw, o.Pref, o.Tref = gamma, Mw, Pref, Tref
+    if sutherland is not None:
+        o.visc_model = A.VISC_SUTHERLAND
+        o.sut_mu_ref, o.sut_T_ref, o.sut_S = sutherland
+    else:
+        o.visc_model, o.mu = A.VISC_CONSTANT, mu
+    if Pr is not None:
+        o.cond_model, o.Pr = A.COND_PRANDTL, Pr
+    else:
+        o.cond_model, o.lambda_ = A.COND_CONSTANT, lam
+    o.order, o.jac_order, o.limiter, o.convective_flux = order, jac_order, limiter, flux
+    o.limiter_threshold, o.Minf, o.wdiss, o.bl_height = limiter_threshold, Minf, wdiss, bl_height
+    o.grad_hex, o.grad_prism, o.grad_other = grad
+    o.interp_method, o.stencil_size, o.skewness_tol = interp, stencil, 0.99
+    for q in range(3):
+        o.moment_center[q] = moment_center[q]
+    o.n_bc = len(bcs)
+    for b, d in enumerate(bcs):
+        bc = o.bc[b]
+        bc.type, bc.kind, bc.specified, bc.thermal = d["type"], d["kind"], d["specified"], d["thermal"]
+        for key in ("p_fixed", "T_fixed", "rho_fixed", "V_fixed", "qdot_fixed"):
+            setattr(bc, key, int(d.get(key, 0)))
+        for key in ("p", "T", "rho", "p_total", "T_total", "mdot", "qdot"):
+            setattr(bc, key, float(d.get(key, 0.0)))
+        V = d.get("V", (0.0, 0.0, 0.0))
+        for q in range(3):
+            bc.V[q] = V[q]
+        bc.include_in_loads = int(b in loads_bcs)
+    return o
+
+
+def freestream_state(n, p, T, V):
+    W = np.empty((n, 5))
+    W[:, 0] = p
+    W[:, 1:4] = np.asarray(V)[None, :]
+    W[:, 4] = T
+    return W
+
+
+class Case:
+    def __init__(self, name, raw, opt, W0, meta):
+        self.name, self.raw, self.opt, self.W0, self.meta = name, raw, opt, W0, meta
+
+    @property
+    def n_cells(self):
+        return len(self.raw["cell_ptr"]) - 1
+
+
+# ---------------------------------------------------------------------------
+# configurations
+# ---------------------------------------------------------------------------
+
+def wedge_case(ni=96, nj=48, seed=1, flux=A.FLUX_ROE):
+    """Inviscid Euler, gamma 1.4, Mach 2 over a 15 degree ramp; Roe, second-order
+    lsqr gradients, minmod limiter; freestream with seeded +-1% uniform noise on
+    rho, u, p."""
+    raw = meshgen.wedge(ni, nj)
+    Mw, g = 28.97, 1.4
+    R = gas_R(Mw)
+    p, T = 101325.0, 300.0
+    U = 2.0 * np.sqrt(g * R * T)
+    bcs = [bc_inlet_state(p, T, (U, 0.0, 0.0), R), bc_outlet_extrapolate(), bc_wall(slip=True),
+           bc_wall(slip=True), bc_symmetry(), bc_symmetry()]
+    opt = make_options(bcs, gamma=g, Mw=Mw, flux=flux, limiter=A.LIMITER_MINMOD,
+                       grad=(A.GRAD_LSQR, A.GRAD_LSQR, A.GRAD_LSQR), Minf=2.0, loads_bcs=(2,))
+    n = len(raw["cell_ptr"]) - 1
+    rng = np.random.default_rng(seed)
+    rho = p / (R * T) * (1.0 + rng.uniform(-0.01, 0.01, n))
+    u = U * (1.0 + rng.uniform(-0.01, 0.01, n))
+    pp = p * (1.0 + rng.uniform(-0.01, 0.01, n))
+    W = freestream_state(n, p, T, (U, 0.0, 0.0))
+    W[:, 0], W[:, 1], W[:, 4] = pp, u, pp / (rho * R)
+    return Case("wedge", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=2.0))
+
+
+def cylinder_case(nt=256, nr=128, seed=2, noise=0.0):
+    """Laminar compressible NS past a unit cylinder, Mach 0.2, Re_D = 40, Pr 0.72,
+    Sutherland viscosity; AUSM+up, lsqr gradients for every cell type."""
+    raw = meshgen.cylinder_ogrid(nt, nr, seed=seed)
+    Mw, g = 28.97, 1.4
+    R = gas_R(Mw)
+    p, T = 101325.0, 300.0
+    U = 0.2 * np.sqrt(g * R * T)
+    rho = p / (R * T)
+    mu_inf = rho * U * 1.0 / 40.0
+    bcs = [bc_wall(slip=False), bc_inlet_T(T, (U, 0.0, 0.0)), bc_outlet_p(p), bc_symmetry(), bc_symmetry()]
+    opt = make_options(bcs, gamma=g, Mw=Mw, sutherland=(mu_inf, T, 110.4), Pr=0.72, flux=A.FLUX_AUSM_PLUS_UP,
+                       grad=(A.GRAD_LSQR, A.GRAD_LSQR, A.GRAD_LSQR), Minf=0.2, loads_bcs=(0,))
+    n = len(raw["cell_ptr"]) - 1
+    W = freestream_state(n, p, T, (U, 0.0, 0.0))
+    if noise:
+        rng = np.random.default_rng(seed + 100)
+        W[:, 0] *= 1.0 + rng.uniform(-noise, noise, n)
+        W[:, 1] *= 1.0 + rng.uniform(-noise, noise, n)
+        W[:, 4] *= 1.0 + rng.uniform(-noise, noise, n)
+    return Case("cylinder", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.2, mu=mu_inf))
+
+
+def smooth_perturbation(xyz, seed, amp=0.05, modes=8):
+    """Sum of `modes` random-phase Fourier modes (wave numbers 1..3 per axis),
+    normalised to max |.| = 1, one field per variable; returns (n, 5)."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((len(xyz), 5))
+    for v in range(5):
+        f = np.zeros(len(xyz))
+        for _ in range(modes):
+            kv = rng.integers(1, 4, size=3)
+            ph = rng.uniform(0.0, 2.0 * np.pi, size=3)
+            f += np.prod(np.sin(2.0 * np.pi * kv[None, :] * xyz + ph[None, :]), axis=1)
+        out[:, v] = amp * f / max(np.abs(f).max(), 1e-300)
+    return out
+
+
+def tetbox_case(n=64, seed=3, viscous=True):
+    """Differentiation check: [0,1]^3 of n^3 hexes x 6 tets, Mach 0.5 freestream
+    plus a seeded smooth 5% perturbation in rho, u, v, w, p; Roe, limited lsqr,
+    viscous terms on."""
+    raw = meshgen.tet_box(n)
+    Mw, g = 28.97, 1.4
+    R = gas_R(Mw)
+    p, T = 101325.0, 300.0
+    a = np.sqrt(g * R * T)
+    U = 0.5 * a
+    rho = p / (R * T)
+    mu = rho * U * 1.0 / 1000.0 if viscous else 0.0
+    bcs = [bc_inlet_state(p, T, (U, 0.0, 0.0), R), bc_outlet_p(p), bc_symmetry(), bc_symmetry(),
+           bc_symmetry(), bc_symmetry()]
+    opt = make_options(bcs, gamma=g, Mw=Mw, mu=mu, Pr=0.72 if viscous else None, flux=A.FLUX_ROE,
+                       limiter=A.LIMITER_VK, limiter_threshold=5.0,
+                       grad=(A.GRAD_LSQR, A.GRAD_LSQR, A.GRAD_LSQR), Minf=0.5)
+    nc = len(raw["cell_ptr"]) - 1
+    xc = meshgen.cell_centroids(raw)
+    d = smooth_perturbation(xc, seed)
+    rho_c = rho * (1.0 + d[:, 0])
+    W = np.empty((nc, 5))
+    W[:, 1] = U * (1.0 + d[:, 1])
+    W[:, 2] = U * d[:, 2]
+    W[:, 3] = U * d[:, 3]
+    W[:, 0] = p * (1.0 + d[:, 4])
+    W[:, 4] = W[:, 0] / (rho_c * R)
+    return Case("tetbox", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.5, mu=mu))
+
+
+def bump_case(ni=256, nj=96, nk=64, seed=4, noise=0.0):
+    """Transonic inviscid channel flow, Mach 0.85, 10% circular-arc bump."""
+    raw = meshgen.bump_channel(ni, nj, nk)
+    Mw, g = 28.97, 1.4
+    R = gas_R(Mw)
+    p, T = 101325.0, 300.0
+    U = 0.85 * np.sqrt(g * R * T)
+    bcs = [bc_inlet_T(T, (U, 0.0, 0.0)), bc_outlet_p(p), bc_wall(slip=True), bc_wall(slip=True),
+           bc_symmetry(), bc_symmetry()]
+    opt = make_options(bcs, gamma=g, Mw=Mw, flux=A.FLUX_ROE, limiter=A.LIMITER_VK, Minf=0.85, loads_bcs=(2,))
+    n = len(raw["cell_ptr"]) - 1
+    W = freestream_state(n, p, T, (U, 0.0, 0.0))
+    if noise:
+        rng = np.random.default_rng(seed)
+        W[:, 0] *= 1.0 + rng.uniform(-noise, noise, n)
+        W[:, 1] *= 1.0 + rng.uniform(-noise, noise, n)
+        W[:, 4] *= 1.0 + rng.uniform(-noise, noise, n)
+    return Case("bump", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.85))
+
+
+def plate_case(ni=512, nj=128, nk=64, seed=5, noise=0.0, first=5.5e-6):
+    """Turbulent flat plate geometry (Mach 0.3, Re_L = 5e6), hex layers with the
+    top 20% split into prisms; laminar viscous Roe until the SA model lands."""
+    raw = meshgen.flat_plate(ni, nj, nk, first=first)
+    Mw, g = 28.97, 1.4
+    R = gas_R(Mw)
+    p, T = 101325.0, 300.0
+    U = 0.3 * np.sqrt(g * R * T)
+    rho = p / (R * T)
+    mu = rho * U * 1.0 / 5.0e6
+    bcs = [bc_inlet_T(T, (U, 0.0, 0.0)), bc_outlet_p(p), bc_wall(slip=False), bc_outlet_p(p),
+           bc_symmetry(), bc_symmetry()]
+    opt = make_options(bcs, gamma=g, Mw=Mw, mu=mu, Pr=0.72, flux=A.FLUX_ROE, limiter=A.LIMITER_VK,
+                       Minf=0.3, loads_bcs=(2,))
+    n = len(raw["cell_ptr"]) - 1
+    W = freestream_state(n, p, T, (U, 0.0, 0.0))
+    if noise:
+        rng = np.random.default_rng(seed)
+        W[:, 0] *= 1.0 + rng.uniform(-noise, noise, n)
+        W[:, 1] *= 1.0 + rng.uniform(-noise, noise, n)
+        W[:, 4] *= 1.0 + rng.uniform(-noise, noise, n)
+    return Case("plate", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.3, mu=mu))

@@ -1,0 +1,447 @@
+// C-ABI entry points of libfcfd (declared in include/fcfd_gpu.h): context
+// creation (host set-up + device upload), layout helpers and the residual.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fcfd_device.h"
+#include "fcfd_internal.h"
+
+using namespace fcfd;
+
+struct fcfd_ctx {
+    HostGrid hg;
+    fcfd_options opt;
+    Params P;
+    std::vector<BcDev> bch;
+    bool device = false;
+    DevGrid dg{};
+    std::vector<void*> allocs;
+    double *G = nullptr, *PHI = nullptr, *dref = nullptr;
+    int* status = nullptr;
+    std::string err;
+    // optional per-stage timing: a ring of event quadruples recorded on the
+    // launch stream around the three kernels, folded in lazily so that
+    // timing adds no host synchronisation to the call sequence
+    static const int kRing = 512;
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;       // 4 * kRing
+    int ring_head = 0, ring_count = 0;
+    double stage_ms[3] = {0.0, 0.0, 0.0};
+    long long stage_calls = 0;
+};
+
+namespace {
+
+int fail(fcfd_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(ctx, FCFD_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int upload(fcfd_ctx* c, const std::vector<T>& h, const T** out, size_t pad_to = 0) {
+    size_t n = std::max(h.size(), pad_to);
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    HIPCHK(c, hipMalloc(&p, n * sizeof(T)));
+    c->allocs.push_back(p);
+    if (n > h.size()) HIPCHK(c, hipMemset(p, 0, n * sizeof(T)));
+    if (!h.empty()) HIPCHK(c, hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = (const T*)p;
+    return FCFD_OK;
+}
+
+template <typename T>
+int dalloc(fcfd_ctx* c, size_t n, T** out) {
+    void* p = nullptr;
+    HIPCHK(c, hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    c->allocs.push_back(p);
+    HIPCHK(c, hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    *out = (T*)p;
+    return FCFD_OK;
+}
+
+int validate_options(fcfd_ctx* c) {
+    const fcfd_options& o = c->opt;
+    if (o.gamma <= 1.0 || o.Mw <= 0.0) return fail(c, FCFD_EINVAL, "material: gamma must exceed 1 and Mw be positive");
+    for (int b = 0; b < o.n_bc; ++b) {
+        const fcfd_bc& bc = o.bc[b];
+        if (bc.type < FCFD_BC_SYMMETRY || bc.type > FCFD_BC_OUTLET) return fail(c, FCFD_EBC, "BC_" + std::to_string(b + 1) + ": unknown type");
+        if (bc.type == FCFD_BC_OUTLET && bc.specified == FCFD_SPEC_STATE)   // ns_apply_bcs.cc:165-167
+            return fail(c, FCFD_EBC, "You can't specify complete thermodynamic state at outlet BC_" + std::to_string(b + 1));
+        if (bc.type == FCFD_BC_INLET && (bc.kind == FCFD_KIND_VELOCITY || bc.kind == FCFD_KIND_MDOT) &&
+            bc.specified == FCFD_SPEC_NONE)   // ns_apply_bcs.cc:78-81,123-126
+            return fail(c, FCFD_EBC, "At least one of p,T,rho needs to be specified at Inlet at BC_" + std::to_string(b + 1));
+    }
+    if (o.grad_hex == FCFD_GRAD_GREENGAUSS || o.grad_prism == FCFD_GRAD_GREENGAUSS || o.grad_other == FCFD_GRAD_GREENGAUSS)
+        return fail(c, FCFD_ENOTSUP, "Green-Gauss gradients are not supported by this build");
+    return FCFD_OK;
+}
+
+void set_params(fcfd_ctx* c) {
+    const fcfd_options& o = c->opt;
+    Params& P = c->P;
+    P.gamma = o.gamma;
+    P.R = 8314.47215 / o.Mw;
+    P.Cp = o.gamma * P.R / (o.gamma - 1.);
+    P.Pref = o.Pref; P.Tref = o.Tref;
+    P.visc_model = o.visc_model; P.mu = o.mu; P.sut_mu_ref = o.sut_mu_ref; P.sut_T_ref = o.sut_T_ref; P.sut_S = o.sut_S;
+    P.cond_model = o.cond_model; P.lambda = o.lambda; P.Pr = o.Pr;
+    P.order = o.order; P.limiter = o.limiter; P.flux = o.convective_flux;
+    P.limiter_threshold = o.limiter_threshold; P.Minf = o.Minf; P.wdiss = o.wdiss; P.bl_height = o.bl_height;
+    P.glscale = c->hg.glscale;
+    bool inviscid = (o.visc_model == FCFD_VISC_CONSTANT && o.mu == 0.0) &&
+                    (o.cond_model == FCFD_COND_CONSTANT && o.lambda == 0.0);
+    bool fixedq = false;
+    for (int b = 0; b < o.n_bc; ++b) fixedq |= (o.bc[b].thermal == FCFD_THERMAL_FIXED_Q && o.bc[b].qdot != 0.0);
+    P.viscous = (inviscid && !fixedq) ? 0 : 1;
+    c->bch.assign(std::max(o.n_bc, 1), BcDev{});
+    for (int b = 0; b < o.n_bc; ++b) {
+        const fcfd_bc& s = o.bc[b];
+        BcDev& d = c->bch[b];
+        d.type = s.type; d.kind = s.kind; d.specified = s.specified; d.thermal = s.thermal;
+        d.p = s.p; d.T = s.T; d.rho = s.rho; d.V[0] = s.V[0]; d.V[1] = s.V[1]; d.V[2] = s.V[2];
+        d.p_total = s.p_total; d.T_total = s.T_total; d.mdot = s.mdot; d.qdot = s.qdot;
+        d.total_area = b < (int)c->hg.bc_area.size() ? c->hg.bc_area[b] : 0.0;
+    }
+}
+
+int build_device(fcfd_ctx* c) {
+    HostGrid& g = c->hg;
+    const int N = g.N, B = g.B, F = g.F, T = N + B;
+    DevGrid& d = c->dg;
+    d.N = N; d.B = B; d.F = F;
+    d.ld = ((T + 63) / 64) * 64;
+    const int ld = d.ld;
+    d.ref_first = g.perm[0];
+    d.ref_last = g.perm[N - 1];
+    // cells
+    {
+        std::vector<double> x(ld, 0.0), y(ld, 0.0), z(ld, 0.0), v(ld, 0.0), l(ld, 0.0);
+        for (int i = 0; i < T; ++i) {
+            int r = g.iperm[i];
+            x[i] = g.cc[3 * r]; y[i] = g.cc[3 * r + 1]; z[i] = g.cc[3 * r + 2];
+            v[i] = g.vol[r]; l[i] = g.lscale[r];
+        }
+        int rc;
+        if ((rc = upload(c, x, &d.ccx)) || (rc = upload(c, y, &d.ccy)) || (rc = upload(c, z, &d.ccz)) ||
+            (rc = upload(c, v, &d.vol)) || (rc = upload(c, l, &d.lscale)))
+            return rc;
+    }
+    // gradient maps, sliced ELL with 64-cell slices
+    {
+        int ns = (N + 63) / 64;
+        std::vector<int> gs(ns + 1, 0), gi;
+        std::vector<double> wx, wy, wz;
+        for (int s = 0; s < ns; ++s) {
+            int width = 0;
+            for (int lane = 0; lane < 64; ++lane) {
+                int i = s * 64 + lane;
+                if (i < N) { int r = g.iperm[i]; width = std::max(width, g.gptr[r + 1] - g.gptr[r]); }
+            }
+            size_t base = gi.size();
+            gi.resize(base + (size_t)width * 64);
+            wx.resize(gi.size(), 0.0); wy.resize(gi.size(), 0.0); wz.resize(gi.size(), 0.0);
+            for (int lane = 0; lane < 64; ++lane) {
+                int i = s * 64 + lane;
+                int r = i < N ? g.iperm[i] : -1;
+                int cnt = r >= 0 ? g.gptr[r + 1] - g.gptr[r] : 0;
+                for (int k = 0; k < width; ++k) {
+                    size_t e = base + (size_t)k * 64 + lane;
+                    if (k < cnt) {
+                        int ent = g.gptr[r] + k;
+                        gi[e] = g.perm[g.gidx[ent]];
+                        wx[e] = g.gw[3 * ent]; wy[e] = g.gw[3 * ent + 1]; wz[e] = g.gw[3 * ent + 2];
+                    } else {
+                        gi[e] = i < N ? i : 0;   // zero-weight pad
+                    }
+                }
+            }
+            gs[s + 1] = (int)gi.size();
+        }
+        int rc;
+        if ((rc = upload(c, gs, &d.gslice)) || (rc = upload(c, gi, &d.gidx)) || (rc = upload(c, wx, &d.gwx)) ||
+            (rc = upload(c, wy, &d.gwy)) || (rc = upload(c, wz, &d.gwz)))
+            return rc;
+    }
+    // half-faces and flux blocks
+    {
+        std::vector<int> hp(N + 1, 0), hf;
+        hf.reserve(g.cfaces.size());
+        for (int i = 0; i < N; ++i) {
+            int r = g.iperm[i];
+            for (int k = g.cfptr[r]; k < g.cfptr[r + 1]; ++k) {
+                int f = g.cfaces[k];
+                int fi = g.fperm[f];
+                hf.push_back(g.fparent[f] == r ? fi : ~fi);
+            }
+            hp[i + 1] = (int)hf.size();
+        }
+        std::vector<int> blk{0};
+        int start = 0;
+        for (int i = 0; i < N; ++i) {
+            if (hp[i + 1] - hp[start] > 256) { blk.push_back(i); start = i; }
+        }
+        blk.push_back(N);
+        d.nhf = (int)hf.size();
+        d.nblk = (int)blk.size() - 1;
+        int rc;
+        if ((rc = upload(c, hp, &d.hf_ptr)) || (rc = upload(c, hf, &d.hf_face)) || (rc = upload(c, blk, &d.blk_cell))) return rc;
+    }
+    // faces
+    {
+        std::vector<int> par(F), nei(F), bc(F);
+        std::vector<double> nx(F), ny(F), nz(F), tx(F), ty(F), tz(F), ar(F), cx(F), cy(F), cz(F), wp(F, 0.5), wn(F, 0.5), wd(F), ds(F);
+        for (int i = 0; i < F; ++i) {
+            int f = g.ifperm[i];
+            par[i] = g.perm[g.fparent[f]];
+            nei[i] = g.perm[g.fneigh[f]];
+            bc[i] = g.fbc[f];
+            nx[i] = g.fn[3 * f]; ny[i] = g.fn[3 * f + 1]; nz[i] = g.fn[3 * f + 2];
+            tx[i] = g.ft1[3 * f]; ty[i] = g.ft1[3 * f + 1]; tz[i] = g.ft1[3 * f + 2];
+            ar[i] = g.farea[f];
+            cx[i] = g.fc[3 * f]; cy[i] = g.fc[3 * f + 1]; cz[i] = g.fc[3 * f + 2];
+            wd[i] = g.fwd[f]; ds[i] = g.fdiss[f];
+            if (g.fbc[f] < 0) {
+                if (g.aptr[f + 1] - g.aptr[f] != 2 || g.aidx[g.aptr[f]] != g.fparent[f] || g.aidx[g.aptr[f] + 1] != g.fneigh[f])
+                    return fail(c, FCFD_ENOTSUP, "device path supports two-cell face-average stencils only (WTLI stencil 2 or SIMPLE)");
+                wp[i] = g.aw[g.aptr[f]];
+                wn[i] = g.aw[g.aptr[f] + 1];
+            }
+        }
+        std::vector<int> gh(B);
+        for (int i = 0; i < B; ++i) gh[i] = g.fperm[g.ghost_face[g.iperm[N + i] - N]];
+        int rc;
+        if ((rc = upload(c, par, &d.fpar)) || (rc = upload(c, nei, &d.fnei)) || (rc = upload(c, bc, &d.fbc)) ||
+            (rc = upload(c, nx, &d.fnx)) || (rc = upload(c, ny, &d.fny)) || (rc = upload(c, nz, &d.fnz)) ||
+            (rc = upload(c, tx, &d.t1x)) || (rc = upload(c, ty, &d.t1y)) || (rc = upload(c, tz, &d.t1z)) ||
+            (rc = upload(c, ar, &d.farea)) || (rc = upload(c, cx, &d.fcx)) || (rc = upload(c, cy, &d.fcy)) ||
+            (rc = upload(c, cz, &d.fcz)) || (rc = upload(c, wp, &d.fwp)) || (rc = upload(c, wn, &d.fwn)) ||
+            (rc = upload(c, wd, &d.fwd)) || (rc = upload(c, ds, &d.fdiss)) || (rc = upload(c, gh, &d.gh_face)))
+            return rc;
+    }
+    {
+        int rc;
+        if ((rc = upload(c, c->bch, &d.bc))) return rc;
+        if ((rc = dalloc(c, (size_t)15 * ld, &c->G)) || (rc = dalloc(c, (size_t)5 * ld, &c->PHI)) ||
+            (rc = dalloc(c, 8, &c->dref)) || (rc = dalloc(c, 4, &c->status)))
+            return rc;
+    }
+    c->device = true;
+    return FCFD_OK;
+}
+
+thread_local std::string g_create_err;
+
+int fold_slot(fcfd_ctx* c, int slot) {
+    hipEvent_t* e = &c->ev[4 * slot];
+    HIPCHK(c, hipEventSynchronize(e[3]));
+    for (int k = 0; k < 3; ++k) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, e[k], e[k + 1]));
+        c->stage_ms[k] += ms;
+    }
+    c->stage_calls++;
+    return FCFD_OK;
+}
+
+int create_impl(const fcfd_mesh* mesh, const fcfd_options* opt, fcfd_ctx** out, bool device) {
+    if (!out) return FCFD_EINVAL;
+    *out = nullptr;
+    fcfd_ctx* c = new (std::nothrow) fcfd_ctx();
+    if (!c) return FCFD_ENOMEM;
+    if (!mesh || !opt) { g_create_err = "null mesh or options"; delete c; return FCFD_EINVAL; }
+    c->opt = *opt;
+    int rc = validate_options(c);
+    if (!rc) {
+        long long nw = 0;
+        for (int b = 0; b < opt->n_bc; ++b) nw += (opt->bc[b].type == FCFD_BC_WALL && opt->bc[b].kind != FCFD_KIND_SLIP);
+        bool host_wd = (long long)mesh->n_cells * (nw ? 1 : 0) < 2000000LL;   // brute force only for small meshes
+        rc = build_host_grid(mesh, opt, c->hg, c->err, host_wd || nw == 0);
+        if (!rc && !host_wd && opt->convective_flux == FCFD_FLUX_SW)
+            rc = fail(c, FCFD_ENOTSUP, "Steger-Warming flux needs wall distances; not computed for meshes this large");
+    }
+    if (!rc) {
+        renumber(c->hg);
+        set_params(c);
+        if (device) rc = build_device(c);
+    }
+    if (rc) {
+        g_create_err = c->err;
+        for (void* p : c->allocs) (void)hipFree(p);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return FCFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fcfd_create(const fcfd_mesh* mesh, const fcfd_options* opt, fcfd_ctx** out) { return create_impl(mesh, opt, out, true); }
+int fcfd_create_host(const fcfd_mesh* mesh, const fcfd_options* opt, fcfd_ctx** out) { return create_impl(mesh, opt, out, false); }
+
+void fcfd_destroy(fcfd_ctx* c) {
+    if (!c) return;
+    for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    for (void* p : c->allocs) (void)hipFree(p);
+    delete c;
+}
+
+const char* fcfd_last_error(const fcfd_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int fcfd_get_info(const fcfd_ctx* c, fcfd_info* info) {
+    if (!c || !info) return FCFD_EINVAL;
+    const HostGrid& g = c->hg;
+    std::memset(info, 0, sizeof *info);
+    info->n_nodes = g.nnode; info->n_cells = g.N; info->n_ghosts = g.B; info->n_faces = g.F; info->n_bfaces = g.B;
+    info->ld_cell = ((g.N + g.B + 63) / 64) * 64;
+    info->ld_face = g.F;
+    info->n_gradmap = (int)g.gidx.size();
+    info->n_faceavg = (int)g.aidx.size();
+    info->n_nbr = (int)g.nb.size();
+    info->n_halffaces = (int)g.cfaces.size();
+    info->nv = 5;
+    return FCFD_OK;
+}
+
+int fcfd_metrics(const fcfd_ctx* c, fcfd_metrics_out* m) {
+    if (!c || !m) return FCFD_EINVAL;
+    const HostGrid& g = c->hg;
+    const int F = g.F, N = g.N, T = g.N + g.B;
+    auto cp_i = [](int* dst, const std::vector<int>& src) { if (dst && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(int)); };
+    auto cp_d = [](double* dst, const std::vector<double>& src, size_t n) { if (dst && n) std::memcpy(dst, src.data(), n * sizeof(double)); };
+    cp_i(m->face_parent, g.fparent); cp_i(m->face_neighbor, g.fneigh); cp_i(m->face_bc, g.fbc);
+    cp_i(m->face_node_ptr, g.fnptr); cp_i(m->face_nodes, g.fnodes);
+    cp_d(m->face_centroid, g.fc, 3 * (size_t)F); cp_d(m->face_normal, g.fn, 3 * (size_t)F);
+    cp_d(m->face_area, g.farea, F); cp_d(m->face_tangent1, g.ft1, 3 * (size_t)F);
+    cp_d(m->cell_centroid, g.cc, 3 * (size_t)T); cp_d(m->cell_volume, g.vol, T);
+    cp_d(m->cell_lengthscale, g.lscale, N); cp_d(m->wall_distance, g.wdist, T);
+    cp_i(m->cell_face_ptr, g.cfptr); cp_i(m->cell_faces, g.cfaces);
+    cp_i(m->nbr_ptr, g.nbptr); cp_i(m->nbr, g.nb);
+    cp_i(m->grad_ptr, g.gptr); cp_i(m->grad_idx, g.gidx); cp_d(m->grad_w, g.gw, g.gw.size());
+    cp_i(m->avg_ptr, g.aptr); cp_i(m->avg_idx, g.aidx); cp_d(m->avg_w, g.aw, g.aw.size());
+    cp_i(m->cell_perm, g.perm);
+    if (m->grid_lengthscale) m->grid_lengthscale[0] = g.glscale;
+    return FCFD_OK;
+}
+
+int fcfd_state_to_device(fcfd_ctx* c, const double* W_host, double* W_dev) {
+    if (!c || !W_host || !W_dev) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data (created with fcfd_create_host)");
+    const int N = c->hg.N, ld = c->dg.ld;
+    std::vector<double> soa((size_t)5 * ld, 0.0);
+    for (int i = 0; i < N; ++i) {
+        int r = c->hg.iperm[i];
+        for (int v = 0; v < 5; ++v) soa[(size_t)v * ld + i] = W_host[5 * (size_t)r + v];
+    }
+    HIPCHK(c, hipMemcpy(W_dev, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+    return FCFD_OK;
+}
+
+int fcfd_cells_to_host(fcfd_ctx* c, const double* X_dev, int nvar, int include_ghosts, double* X_host) {
+    if (!c || !X_dev || !X_host || nvar <= 0) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    const int N = c->hg.N, B = c->hg.B, ld = c->dg.ld;
+    const int n = include_ghosts ? N + B : N;
+    std::vector<double> soa((size_t)nvar * ld);
+    HIPCHK(c, hipMemcpy(soa.data(), X_dev, soa.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+        int r = c->hg.iperm[i];
+        for (int v = 0; v < nvar; ++v) X_host[(size_t)nvar * r + v] = soa[(size_t)v * ld + i];
+    }
+    return FCFD_OK;
+}
+
+int fcfd_residual(fcfd_ctx* c, double* W_dev, double* R_dev, void* stream) {
+    if (!c || !W_dev || !R_dev) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t* e = nullptr;
+    if (c->profiling) {
+        if (c->ring_count == fcfd_ctx::kRing) {   // fold the oldest slot (long finished)
+            int old = (c->ring_head + fcfd_ctx::kRing - c->ring_count) % fcfd_ctx::kRing;
+            int rc = fold_slot(c, old);
+            if (rc) return rc;
+            c->ring_count--;
+        }
+        e = &c->ev[4 * c->ring_head];
+        HIPCHK(c, hipEventRecord(e[0], s));
+    }
+    launch_ghosts(c->dg, c->P, W_dev, c->status, s);
+    if (e) HIPCHK(c, hipEventRecord(e[1], s));
+    launch_grad_lim(c->dg, c->P, W_dev, c->G, c->PHI, c->dref, s);
+    if (e) HIPCHK(c, hipEventRecord(e[2], s));
+    launch_flux(c->dg, c->P, W_dev, c->G, c->PHI, R_dev, c->status, s);
+    if (e) {
+        HIPCHK(c, hipEventRecord(e[3], s));
+        c->ring_head = (c->ring_head + 1) % fcfd_ctx::kRing;
+        c->ring_count++;
+    }
+    HIPCHK(c, hipGetLastError());
+    return FCFD_OK;
+}
+
+int fcfd_profile(fcfd_ctx* c, int enable, double* stage_ms, long long* calls) {
+    if (!c || !c->device) return FCFD_EINVAL;
+    while (c->ring_count > 0) {
+        int old = (c->ring_head + fcfd_ctx::kRing - c->ring_count) % fcfd_ctx::kRing;
+        int rc = fold_slot(c, old);
+        if (rc) return rc;
+        c->ring_count--;
+    }
+    if (stage_ms) for (int k = 0; k < 3; ++k) stage_ms[k] = c->stage_ms[k];
+    if (calls) *calls = c->stage_calls;
+    if (enable >= 0) {
+        if (enable && c->ev.empty()) {
+            c->ev.resize(4 * fcfd_ctx::kRing, nullptr);
+            for (auto& ev : c->ev) HIPCHK(c, hipEventCreate(&ev));
+        }
+        c->profiling = enable != 0;
+        for (int k = 0; k < 3; ++k) c->stage_ms[k] = 0.0;
+        c->stage_calls = 0;
+        c->ring_head = 0;
+    }
+    return FCFD_OK;
+}
+
+int fcfd_internal_buffers(fcfd_ctx* c, const double** grad, const double** phi) {
+    if (!c || !c->device) return FCFD_EINVAL;
+    if (grad) *grad = c->G;
+    if (phi) *phi = c->PHI;
+    return FCFD_OK;
+}
+
+int fcfd_status(fcfd_ctx* c, int clear) {
+    if (!c || !c->device) return FCFD_EINVAL;
+    int h = 0;
+    HIPCHK(c, hipMemcpy(&h, c->status, sizeof(int), hipMemcpyDeviceToHost));
+    if (clear) HIPCHK(c, hipMemset(c->status, 0, sizeof(int)));
+    if (h & 1) return fail(c, FCFD_EBC, "an inlet became supersonic; the thermodynamic state must be specified (ns_apply_bcs.cc:54-59)");
+    return FCFD_OK;
+}
+
+int fcfd_time_step(fcfd_ctx* c, const double* W_dev, int dt_type, double cfl, double* dt_dev, double* max_cfl, void* stream) {
+    if (!c || !W_dev || !dt_dev) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    if (dt_type != FCFD_DT_CFL_LOCAL) return fail(c, FCFD_ENOTSUP, "only CFL_LOCAL time stepping is implemented");
+    launch_dt_local(c->dg, c->P, W_dev, cfl, dt_dev, (hipStream_t)stream);
+    HIPCHK(c, hipGetLastError());
+    if (max_cfl) *max_cfl = cfl;
+    return FCFD_OK;
+}
+
+}  // extern "C"

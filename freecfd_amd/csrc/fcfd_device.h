@@ -1,0 +1,36 @@
+// Device-side mesh layout (internal numbering) and kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fcfd_internal.h"
+
+namespace fcfd {
+
+// All arrays are device pointers; cells and faces are in internal (RCM)
+// order.  Cell arrays are padded to `ld` entries; interior cells [0,N), ghosts
+// [N, N+B).  A half-face entry hf_face[h] is the internal face id f when the
+// owning cell is the face's parent and ~f when it is the neighbour; the
+// half-faces of a cell are in ascending reference face order, so summing them
+// in order reproduces the reference's VecSetValues(ADD) sequence.
+struct DevGrid {
+    int N, B, F, ld, nhf, nblk;
+    int ref_first, ref_last;            // internal ids of reference cells 0 and N-1
+    const double *ccx, *ccy, *ccz, *vol, *lscale;
+    const int* gslice;                  // N/64+1 entry offsets of 64-wide gradient slices
+    const int* gidx;
+    const double *gwx, *gwy, *gwz;
+    const int *hf_ptr, *hf_face;        // N+1, nhf
+    const int* blk_cell;                // nblk+1 first cell of each flux block
+    const int *fpar, *fnei, *fbc;
+    const double *fnx, *fny, *fnz, *t1x, *t1y, *t1z, *farea, *fcx, *fcy, *fcz, *fwp, *fwn, *fwd, *fdiss;
+    const int* gh_face;                 // B: boundary face of ghost N+i
+    const BcDev* bc;
+};
+
+void launch_ghosts(const DevGrid& g, const Params& P, double* W, int* status, hipStream_t s);
+void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s);
+void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* R,
+                 int* status, hipStream_t s);
+void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s);
+
+}  // namespace fcfd

@@ -1,0 +1,62 @@
+// Internal declarations shared by the host set-up, the device kernels and the
+// C-ABI entry points of libfcfd.  Not part of the public interface.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fcfd_gpu.h"
+
+namespace fcfd {
+
+// Mesh + metrics in reference numbering, built on the host by fcfd_setup.cpp.
+struct HostGrid {
+    int dim = 3;
+    int nnode = 0, N = 0, B = 0, F = 0;
+    std::vector<double> node;                 // 3*nnode (local node order)
+    std::vector<int> cell_nptr, cell_nodes;   // N+1, nodes per interior cell (local ids)
+    std::vector<int> fparent, fneigh, fbc;    // F
+    std::vector<int> fnptr, fnodes;           // F+1, face node lists
+    std::vector<double> fc, fn, ft1, farea;   // 3F, 3F, 3F, F
+    std::vector<int> fsym, face2bc;           // F
+    std::vector<double> fwd, fdiss;           // F: closest wall distance, SW dissipation factor
+    std::vector<double> cc, vol, lscale, wdist;   // 3(N+B), N+B, N+B, N+B
+    std::vector<int> cbc;                     // N+B (ghost's BC, -1 interior)
+    std::vector<int> ghost_face;              // B: the boundary face of each ghost
+    std::vector<int> cfptr, cfaces;           // N+1, faces of interior cells, ascending
+    std::vector<int> nbptr, nb;               // N+B+1, neighborCells
+    std::vector<int> gptr, gidx;              // N+1, gradient map keys (ascending)
+    std::vector<double> gw;                   // 3*entries
+    std::vector<int> aptr, aidx;              // F+1, face-average keys (ascending)
+    std::vector<double> aw;
+    std::vector<double> bc_area;              // per BC total area
+    double total_volume = 0.0, glscale = 0.0;
+    // internal numbering
+    std::vector<int> perm;                    // N+B: internal id of reference cell
+    std::vector<int> iperm;                   // N+B: reference id of internal cell
+    std::vector<int> fperm;                   // F: internal face id of reference face
+    std::vector<int> ifperm;                  // F: reference face of internal face
+};
+
+int build_host_grid(const fcfd_mesh* m, const fcfd_options* o, HostGrid& g, std::string& err,
+                    bool wall_distance_on_host);
+void renumber(HostGrid& g);
+
+// Material/numerics constants passed by value to kernels.
+struct Params {
+    double gamma, R, Cp, Pref, Tref;
+    int visc_model; double mu, sut_mu_ref, sut_T_ref, sut_S;
+    int cond_model; double lambda, Pr;
+    int order, limiter, flux;
+    double limiter_threshold, Minf, wdiss, bl_height;
+    double glscale;
+    int viscous;     // 0: mu and lambda identically zero -> diffusive flux is exactly 0
+};
+
+// BC table entry on the device (uniform BC values).
+struct BcDev {
+    int type, kind, specified, thermal;
+    double p, T, rho, V[3], p_total, T_total, mdot, qdot, total_area;
+};
+
+}  // namespace fcfd

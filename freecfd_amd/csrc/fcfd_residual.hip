@@ -1,0 +1,389 @@
+// Residual kernels for gfx950 (FP64).
+//
+// Three launches per residual evaluation, all deterministic and atomic-free:
+//   1. ghost_kernel      one thread per boundary face: boundary ghost state
+//   2. grad_lim_kernel   one thread per cell: gradient-map gather (sliced
+//                        ELL, 64-wide slices matching the wavefront) fused
+//                        with the limiter; also copies gradients to ghosts
+//   3. flux_kernel       one thread per half-face (cell, face) pair: each
+//                        block owns whole cells, computes the face fluxes of
+//                        their faces, stages them in LDS and sums every cell's
+//                        contributions in reference face order.  An interior
+//                        face is evaluated by both of its cells with
+//                        identical inputs and operation order, so both sides
+//                        see the bitwise-same flux and R is conservative.
+#include <hip/hip_runtime.h>
+
+#include "fcfd_device.h"
+#include "fcfd_physics.h"
+
+namespace fcfd {
+
+__device__ inline void load_geo(const DevGrid& g, int f, FaceGeo& F) {
+    F.n = mk<double>(g.fnx[f], g.fny[f], g.fnz[f]);
+    F.t1 = mk<double>(g.t1x[f], g.t1y[f], g.t1z[f]);
+    F.t2 = vnorm(vcross(F.n, F.t1));
+    F.area = g.farea[f];
+    F.bc = g.fbc[f];
+}
+
+__device__ inline Vec<double> centroid(const DevGrid& g, int c) { return mk<double>(g.ccx[c], g.ccy[c], g.ccz[c]); }
+
+// --------------------------------------------------------------------------
+// 1. boundary ghosts: apply_bcs with the parent's cell-centre state as left and
+//    right pre-set to a copy of left; ghost = right.{p,V,T}_center
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) ghost_kernel(DevGrid g, Params P, double* __restrict__ W, int* __restrict__ status) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.B) return;
+    int f = g.gh_face[i];
+    int c = g.fpar[f];
+    FaceGeo F;
+    load_geo(g, f, F);
+    const size_t ld = g.ld;
+    CState<double> L;
+    L.p = L.p_center = W[c];
+    L.V = L.V_center = mk<double>(W[ld + c], W[2 * ld + c], W[3 * ld + c]);
+    L.T = L.T_center = W[4 * ld + c];
+    L.rho = m_rho(P, L.p, L.T);
+    finish_state(P, L, F);
+    CState<double> R = L;
+    int flag = apply_bcs(P, g.bc[F.bc], L, R, F);
+    if (flag) atomicOr(status, 1);
+    int gi = g.N + i;
+    W[gi] = R.p_center;
+    W[ld + gi] = R.V_center.x;
+    W[2 * ld + gi] = R.V_center.y;
+    W[3 * ld + gi] = R.V_center.z;
+    W[4 * ld + gi] = R.T_center;
+}
+
+// --------------------------------------------------------------------------
+// 2. gradients (calc_cell_grads, ns.cc:301-336 + variable.h:193-216) fused
+//    with the limiter (ns_limiters.cc:26-248)
+// --------------------------------------------------------------------------
+template <int LIM>
+__global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, const double* __restrict__ W,
+                                                       double* __restrict__ G, double* __restrict__ PHI,
+                                                       const double* __restrict__ dref) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld;
+    double gr[15];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) gr[k] = 0.0;
+    {
+        int s = c >> 6, lane = c & 63;
+        int e0 = g.gslice[s], e1 = g.gslice[s + 1];
+        for (int e = e0 + lane; e < e1; e += 64) {
+            int j = g.gidx[e];
+            double wx = g.gwx[e], wy = g.gwy[e], wz = g.gwz[e];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                double q = W[v * ld + j];
+                gr[3 * v] += wx * q;
+                gr[3 * v + 1] += wy * q;
+                gr[3 * v + 2] += wz * q;
+            }
+        }
+    }
+    const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) G[k * ld + c] = gr[k];
+    double q0[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) q0[v] = W[v * ld + c];
+    double phi[5] = {1., 1., 1., 1., 1.};
+    Vec<double> xc = centroid(g, c);
+    if (LIM == FCFD_LIMITER_MINMOD) {
+        const double small = 1.e-12;
+        for (int h = h0; h < h1; ++h) {
+            int fs = g.hf_face[h];
+            int f = fs >= 0 ? fs : ~fs;
+            int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
+            Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                double dU = W[v * ld + nb] - q0[v];
+                double dUg = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
+                double mm = (dU * dUg < 0.) ? 0. : (dU > 0.) ? smin(dU, dUg) : smax(dU, dUg);
+                double sU = (double)((mm > 0) - (mm < 0)), sG = (double)((dUg > 0) - (dUg < 0));
+                phi[v] = smin(phi[v], (mm + sU * small) / (dUg + sG * small));
+            }
+        }
+    } else if (LIM == FCFD_LIMITER_BJ || LIM == FCFD_LIMITER_VK) {
+        double umax[5], umin[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) umax[v] = umin[v] = q0[v];
+        for (int h = h0; h < h1; ++h) {
+            int fs = g.hf_face[h];
+            int f = fs >= 0 ? fs : ~fs;
+            int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                double q = W[v * ld + nb];
+                umax[v] = smax(umax[v], q);
+                umin[v] = smin(umin[v], q);
+            }
+        }
+        for (int h = h0; h < h1; ++h) {
+            int fs = g.hf_face[h];
+            int f = fs >= 0 ? fs : ~fs;
+            Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                double dM = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
+                double dP = q0[v];
+                if (LIM == FCFD_LIMITER_BJ) {
+                    if (dM > 0.) dP = umax[v] - dP;
+                    else if (dM < 0.) dP = umin[v] - dP;
+                    else { dP = 1.; dM = 1.; }
+                    phi[v] = smin(phi[v], dP / dM);
+                } else {
+                    double eps = dref[v] / P.glscale;
+                    if (dM > 0.) dP = umax[v] - dP;
+                    else if (dM < 0.) dP = umin[v] - dP;
+                    double dP2 = dP * dP, dM2 = dM * dM;
+                    double e = 1.e-6 * P.limiter_threshold * eps;
+                    double eps2 = e * e;
+                    if (dM != 0.) phi[v] = smin(phi[v], ((dP2 + eps2) + 2. * dM * dP) / (dP2 + 2. * dM2 + dM * dP + eps2));
+                }
+            }
+        }
+    }
+    if (LIM != FCFD_LIMITER_NONE) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) PHI[v * ld + c] = phi[v];
+    }
+    // boundary ghosts take their parent's gradient (ns.cc:320-333)
+    for (int h = h0; h < h1; ++h) {
+        int fs = g.hf_face[h];
+        if (fs < 0) continue;
+        int gi = g.fnei[fs];
+        if (gi < g.N) continue;
+#pragma unroll
+        for (int k = 0; k < 15; ++k) G[k * ld + gi] = gr[k];
+    }
+}
+
+// find_min_max (ns.cc:400-435) with the reference's index slips, for VK's
+// reference range: one block, fixed-order tree reduction (deterministic)
+// The reference loop sets qmax[3] = max(qmax[4], w_c) and then
+// qmax[4] = max(qmax[4], p_c), with qmax[4] seeded by T of cell 0.  After the
+// loop: slot 4 = max(T_0, max_c p_c); slot 3 = max(max(T_0, max_{c<last} p_c), w_last)
+// where `last` is the last cell in reference order.  Min likewise.
+// `first`/`last` are the internal indices of reference cells 0 and N-1.
+__global__ void __launch_bounds__(1024) minmax_kernel(int N, size_t ld, int first, int last, const double* __restrict__ W,
+                                                      double* __restrict__ dref) {
+    __shared__ double sh[8][1024];
+    int t = threadIdx.x;
+    // 0..2: max p,u,v   3: max p excluding `last`   4..6: min p,u,v   7: min p excluding `last`
+    double r[8];
+    r[0] = r[4] = W[first]; r[1] = r[5] = W[ld + first]; r[2] = r[6] = W[2 * ld + first];
+    r[3] = r[7] = W[4 * ld + first];   // T_0 seeds the p-tracking slots (never excluded)
+    for (int c = t; c < N; c += blockDim.x) {
+        double p = W[c], u = W[ld + c], v = W[2 * ld + c];
+        r[0] = smax(r[0], p); r[1] = smax(r[1], u); r[2] = smax(r[2], v);
+        r[4] = smin(r[4], p); r[5] = smin(r[5], u); r[6] = smin(r[6], v);
+        if (c != last) { r[3] = smax(r[3], p); r[7] = smin(r[7], p); }
+    }
+    for (int k = 0; k < 8; ++k) sh[k][t] = r[k];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            for (int k = 0; k < 4; ++k) sh[k][t] = smax(sh[k][t], sh[k][t + s]);
+            for (int k = 4; k < 8; ++k) sh[k][t] = smin(sh[k][t], sh[k][t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        double T0 = W[4 * ld + first];
+        double pl = W[last], wl = W[3 * ld + last];
+        double q4max = smax(sh[3][0], pl), q4min = smin(sh[7][0], pl);
+        q4max = smax(q4max, T0); q4min = smin(q4min, T0);
+        double q3max = smax(sh[3][0], wl), q3min = smin(sh[7][0], wl);
+        dref[0] = fabs(sh[0][0] - sh[4][0]);
+        dref[1] = fabs(sh[1][0] - sh[5][0]);
+        dref[2] = fabs(sh[2][0] - sh[6][0]);
+        dref[3] = fabs(q3max - q3min);
+        dref[4] = fabs(q4max - q4min);
+    }
+}
+
+// --------------------------------------------------------------------------
+// 3. face fluxes + per-cell ordered accumulation (ns_assemble_linear_system.cc:75-140)
+// --------------------------------------------------------------------------
+__device__ inline void load_cell(const DevGrid& g, const double* __restrict__ W, const double* __restrict__ G,
+                                 const double* __restrict__ PHI, int c, bool has_phi, double phi_const, double q[5],
+                                 Vec<double> gr[5], double ph[5]) {
+    const size_t ld = g.ld;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        q[v] = W[v * ld + c];
+        gr[v] = mk<double>(G[(3 * v) * ld + c], G[(3 * v + 1) * ld + c], G[(3 * v + 2) * ld + c]);
+        ph[v] = has_phi ? PHI[v * ld + c] : phi_const;
+    }
+}
+
+template <int FLUX, int VISC>
+__device__ inline void face_flux(const DevGrid& g, const Params& P, const double* __restrict__ W, const double* __restrict__ G,
+                                 const double* __restrict__ PHI, bool has_phi, double phi_const, int f, double out[5],
+                                 int* status) {
+    FaceGeo F;
+    load_geo(g, f, F);
+    const int p = g.fpar[f], n = g.fnei[f];
+    Vec<double> xf = mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]);
+    Vec<double> xp = centroid(g, p), xn = centroid(g, n);
+    F.l2r = vsub(xn, xp);
+    double q[5], ph[5];
+    Vec<double> gp[5];
+    load_cell(g, W, G, PHI, p, has_phi, phi_const, q, gp, ph);
+    CState<double> L, R;
+    recon(P, L, q, gp, ph, vsub(xf, xp));
+    finish_state(P, L, F);
+    Vec<double> gn[5];
+    if (F.bc >= 0) {
+        const size_t ld = g.ld;
+        R.p_center = W[n];
+        R.V_center = mk<double>(W[ld + n], W[2 * ld + n], W[3 * ld + n]);
+        R.T_center = W[4 * ld + n];
+        if (apply_bcs(P, g.bc[F.bc], L, R, F)) atomicOr(status, 1);
+    } else {
+        double qn[5], phn[5];
+        load_cell(g, W, G, PHI, n, has_phi, phi_const, qn, gn, phn);
+        recon(P, R, qn, gn, phn, vsub(xf, xn));
+    }
+    finish_state(P, R, F);
+    double C[5];
+    convective_flux<FLUX>(P, L, R, F, VISC >= 0 && FLUX == FCFD_FLUX_SW ? g.fdiss[f] : 0.0,
+                          FLUX == FCFD_FLUX_SW ? g.fwd[f] : 0.0, C);
+    if (VISC) {
+        FState<double> fs;
+        if (F.bc >= 0) {
+            fs.gu = gp[1]; fs.gv = gp[2]; fs.gw = gp[3]; fs.gT = gp[4];
+        } else {   // face average (variable.h:134-147), two-entry stencil in key order
+            double wp = g.fwp[f], wn = g.fwn[f];
+            fs.gu = vadd(vsc(wp, gp[1]), vsc(wn, gn[1]));
+            fs.gv = vadd(vsc(wp, gp[2]), vsc(wn, gn[2]));
+            fs.gw = vadd(vsc(wp, gp[3]), vsc(wn, gn[3]));
+            fs.gT = vadd(vsc(wp, gp[4]), vsc(wn, gn[4]));
+        }
+        face_state(P, L, R, F, fs);
+        double D[5];
+        diffusive_flux(fs, F, g.bc, D);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) out[v] = D[v] - C[v] + 0.;
+    } else {
+        out[0] = 0. - C[0] + 0.;
+#pragma unroll
+        for (int v = 1; v < 5; ++v) out[v] = 0. - C[v] + 0.;
+    }
+}
+
+template <int FLUX, int VISC>
+__global__ void __launch_bounds__(256) flux_kernel(DevGrid g, Params P, const double* __restrict__ W,
+                                                   const double* __restrict__ G, const double* __restrict__ PHI,
+                                                   int has_phi, double phi_const, double* __restrict__ Rout,
+                                                   int* __restrict__ status) {
+    __shared__ double lds[5][256];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int c0 = g.blk_cell[b], c1 = g.blk_cell[b + 1];
+    const int h0 = g.hf_ptr[c0], nh = g.hf_ptr[c1] - h0;
+    if (t < nh) {
+        int fs = g.hf_face[h0 + t];
+        int f = fs >= 0 ? fs : ~fs;
+        double o[5];
+        face_flux<FLUX, VISC>(g, P, W, G, PHI, has_phi != 0, phi_const, f, o, status);
+        if (fs >= 0) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) lds[v][t] = o[v];
+        } else {   // neighbour side: -1*(D-C)+0 (ns_assemble_linear_system.cc:137)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) lds[v][t] = -1. * (o[v] - 0.) + 0.;
+        }
+    }
+    __syncthreads();
+    if (t < c1 - c0) {
+        int c = c0 + t;
+        int a0 = g.hf_ptr[c] - h0, a1 = g.hf_ptr[c + 1] - h0;
+        const size_t ld = g.ld;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            double s = 0.0;
+            for (int k = a0; k < a1; ++k) s += lds[v][k];
+            Rout[v * ld + c] = s;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// local time step (time_step.cc:193-220) for CFL_LOCAL; CFL_MAX uses a
+// deterministic min-reduction
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) dt_local_kernel(DevGrid g, Params P, const double* __restrict__ W, double cfl,
+                                                       double* __restrict__ dt) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld;
+    double p = W[c], T = W[4 * ld + c];
+    Vec<double> V = mk<double>(W[ld + c], W[2 * ld + c], W[3 * ld + c]);
+    double a = m_a(P, p, T);
+    dt[c] = cfl * g.lscale[c] / (vmag(V) + a);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+template <int FLUX, int VISC>
+static void launch_flux_t(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI,
+                          int has_phi, double phi_const, double* R, int* status, hipStream_t s) {
+    hipLaunchKernelGGL((flux_kernel<FLUX, VISC>), dim3(g.nblk), dim3(256), 0, s, g, P, W, G, PHI, has_phi, phi_const, R, status);
+}
+
+template <int FLUX>
+static void launch_flux_v(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI,
+                          int has_phi, double phi_const, double* R, int* status, hipStream_t s) {
+    if (P.viscous) launch_flux_t<FLUX, 1>(g, P, W, G, PHI, has_phi, phi_const, R, status, s);
+    else launch_flux_t<FLUX, 0>(g, P, W, G, PHI, has_phi, phi_const, R, status, s);
+}
+
+void launch_ghosts(const DevGrid& g, const Params& P, double* W, int* status, hipStream_t s) {
+    if (g.B > 0) hipLaunchKernelGGL(ghost_kernel, dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, status);
+}
+
+void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s) {
+    int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (lim == FCFD_LIMITER_VK) {
+        hipLaunchKernelGGL(minmax_kernel, dim3(1), dim3(1024), 0, s, g.N, (size_t)g.ld, g.ref_first, g.ref_last, W, dref);
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_VK>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+    } else if (lim == FCFD_LIMITER_BJ) {
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_BJ>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+    } else if (lim == FCFD_LIMITER_MINMOD) {
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_MINMOD>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+    } else {
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_NONE>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+    }
+}
+
+void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* R,
+                 int* status, hipStream_t s) {
+    // limiter buffer is used only when a limiter runs; otherwise phi is the
+    // constant initial value (ns.cc:282-287): 1 for second order, 0 for first
+    int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    int has_phi = lim != FCFD_LIMITER_NONE;
+    double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
+    switch (P.flux) {
+    case FCFD_FLUX_ROE: launch_flux_v<FCFD_FLUX_ROE>(g, P, W, G, PHI, has_phi, phi_const, R, status, s); break;
+    case FCFD_FLUX_VAN_LEER: launch_flux_v<FCFD_FLUX_VAN_LEER>(g, P, W, G, PHI, has_phi, phi_const, R, status, s); break;
+    case FCFD_FLUX_SD_SLAU: launch_flux_v<FCFD_FLUX_SD_SLAU>(g, P, W, G, PHI, has_phi, phi_const, R, status, s); break;
+    case FCFD_FLUX_SW: launch_flux_v<FCFD_FLUX_SW>(g, P, W, G, PHI, has_phi, phi_const, R, status, s); break;
+    default: launch_flux_v<FCFD_FLUX_AUSM_PLUS_UP>(g, P, W, G, PHI, has_phi, phi_const, R, status, s); break;
+    }
+}
+
+void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s) {
+    hipLaunchKernelGGL(dt_local_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, P, W, cfl, dt);
+}
+
+}  // namespace fcfd

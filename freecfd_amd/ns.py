@@ -1,0 +1,125 @@
+"""Host-side mirror of the reference's NavierStokes/Grid entry points for the
+residual path, over the libfcfd C-ABI.
+
+    grid = Grid(raw, opt)                 # Grid::setup .. gradient_maps on the host
+    ns = NavierStokes(raw, opt)           # + device upload (fcfd_create)
+    ns.set_state(W)                       # p, u, v, w, T per cell (reference order)
+    ns.assemble_residual()                # update_boundaries, calc_cell_grads,
+                                          # calc_limiter, face loop -> R on the GPU
+    R = ns.residual_host()                # N x 5, reference order
+
+Device memory is allocated with torch (plumbing only); every computation runs
+in libfcfd's HIP kernels.  Nothing here falls back to the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from ._lib import lib
+
+
+def _check(h, rc):
+    if rc != 0:
+        msg = lib().fcfd_last_error(h)
+        raise RuntimeError("libfcfd error %d: %s" % (rc, msg.decode() if msg else ""))
+
+
+class Grid:
+    """Host-only set-up (no GPU needed): connectivity, metrics, maps."""
+
+    _device = False
+
+    def __init__(self, raw, opt):
+        self._mesh = A.make_mesh(raw)
+        self._opt = opt
+        h = C.c_void_p()
+        fn = lib().fcfd_create if self._device else lib().fcfd_create_host
+        rc = fn(C.byref(self._mesh), C.byref(opt), C.byref(h))
+        if rc != 0:
+            raise RuntimeError("fcfd_create failed (%d): %s" % (rc, lib().fcfd_last_error(None).decode()))
+        self.h = h
+        self.info = A.Info()
+        _check(self.h, lib().fcfd_get_info(self.h, C.byref(self.info)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fcfd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def metrics(self):
+        m, arrs = A.alloc_metrics(self.info)
+        _check(self.h, lib().fcfd_metrics(self.h, C.byref(m)))
+        arrs["face_nodes"] = arrs["face_nodes"][:arrs["face_node_ptr"][self.info.n_faces]]
+        return arrs
+
+
+class NavierStokes(Grid):
+    """One grid on one GPU: state, residual, gradients, limiters."""
+
+    _device = True
+
+    def __init__(self, raw, opt, device="cuda:0"):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("NavierStokes needs a GPU (libfcfd has no CPU path)")
+        self.torch = torch
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        super().__init__(raw, opt)
+        ld = self.info.ld_cell
+        self.W = torch.zeros((5, ld), dtype=torch.float64, device=self.device)
+        self.R = torch.zeros((5, ld), dtype=torch.float64, device=self.device)
+
+    @property
+    def ld(self):
+        return self.info.ld_cell
+
+    def set_state(self, W_host):
+        W_host = np.ascontiguousarray(W_host, dtype=np.float64)
+        assert W_host.shape == (self.info.n_cells, 5)
+        self.torch.cuda.synchronize()
+        _check(self.h, lib().fcfd_state_to_device(self.h, W_host.ctypes.data, self.W.data_ptr()))
+
+    def assemble_residual(self, W=None, R=None, stream=None):
+        """R(W) on the GPU; W, R are [5][ld] float64 device tensors."""
+        W = self.W if W is None else W
+        R = self.R if R is None else R
+        s = self.torch.cuda.current_stream().cuda_stream if stream is None else stream
+        _check(self.h, lib().fcfd_residual(self.h, W.data_ptr(), R.data_ptr(), C.c_void_p(s)))
+        return R
+
+    def to_host(self, X, nvar, include_ghosts=False):
+        self.torch.cuda.synchronize()
+        n = self.info.n_cells + (self.info.n_ghosts if include_ghosts else 0)
+        out = np.zeros((n, nvar))
+        _check(self.h, lib().fcfd_cells_to_host(self.h, X.data_ptr(), nvar, int(include_ghosts), out.ctypes.data))
+        return out
+
+    def residual_host(self):
+        return self.to_host(self.R, 5)
+
+    def state_host(self, include_ghosts=True):
+        return self.to_host(self.W, 5, include_ghosts)
+
+    def gradients_limiters_host(self):
+        """Gradients ((N+B) x 15) and limiters (N x 5) of the last residual."""
+        g = C.c_void_p()
+        p = C.c_void_p()
+        _check(self.h, lib().fcfd_internal_buffers(self.h, C.byref(g), C.byref(p)))
+        self.torch.cuda.synchronize()
+        n = self.info.n_cells
+        G = np.zeros((n + self.info.n_ghosts, 15))
+        PHI = np.zeros((n, 5))
+        _check(self.h, lib().fcfd_cells_to_host(self.h, g, 15, 1, G.ctypes.data))
+        _check(self.h, lib().fcfd_cells_to_host(self.h, p, 5, 0, PHI.ctypes.data))
+        return G, PHI
+
+    def status(self, clear=True):
+        _check(self.h, lib().fcfd_status(self.h, int(clear)))

@@ -1,0 +1,237 @@
+/*
+ * fcfd_gpu.h -- C-ABI of the MI355X (gfx950) finite-volume residual path.
+ *
+ * Plain C: opaque context, plain pointers and sizes, int status codes.
+ * Every entry point returns FCFD_OK (0) on success or a negative FCFD_E*
+ * code; the text of the last error is available from fcfd_last_error().
+ * Nothing here calls exit() or aborts (the reference exits on input errors,
+ * src/navier_stokes/ns_apply_bcs.cc:55-58, and aborts on divergence,
+ * src/navier_stokes/ns.cc:350-355; both become return codes).
+ *
+ * Reference interface replaced (Free CFD 2.0, jackxiang/freecfd):
+ *   mesh set-up     Grid::setup (src/grid/grid.cc:55), set_bcs (src/set_bcs.cc:35),
+ *                   set_lengthScales (src/main.cc:456), face_interpolation_weights
+ *                   (src/face_interpolation_weights.cc:32), gradient_maps
+ *                   (src/gradient_maps.cc:39)                     -> fcfd_create
+ *   per step        NavierStokes::update_boundaries (declared src/navier_stokes/ns.h:165),
+ *                   calc_cell_grads (ns.cc:301), calc_limiter (ns_limiters.cc:26),
+ *                   assemble_linear_system residual part (ns_assemble_linear_system.cc:75-140)
+ *                                                                  -> fcfd_residual
+ *                   get_jacobians + MatSetValues (ns_assemble_linear_system.cc:144-267),
+ *                   time_terms (ns_time_terms.cc:107)              -> fcfd_fd_jacobian
+ *                   update_time_step (src/time_step.cc:143)       -> fcfd_time_step
+ *                   petsc_solve + update_variables (ns_petsc_functions.cc:85, ns.cc:338)
+ *                                                                  -> fcfd_newton_step
+ *
+ * Numbering.  Host arrays passed to or returned by fcfd_* in "reference
+ * numbering" use the cell, face and node order the reference's Grid::setup
+ * produces on one rank (cells in input order, faces in creation order of
+ * src/grid/grid_create_elements.cc:184-442, boundary ghosts appended per BC
+ * region in face order, grid_create_elements.cc:702-743).  Device arrays use
+ * the context's internal (reverse Cuthill-McKee) numbering and a
+ * structure-of-arrays layout: variable v of internal cell i lives at
+ * ptr[v*ld + i], where ld = fcfd_info.ld_cell (a multiple of 64).
+ */
+#ifndef FCFD_GPU_H
+#define FCFD_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define FCFD_OK            0
+#define FCFD_EINVAL       -1   /* bad argument / inconsistent input            */
+#define FCFD_EMESH        -2   /* mesh cannot be processed (unsupported cell)   */
+#define FCFD_EBC          -3   /* boundary-condition input error                */
+#define FCFD_EHIP         -4   /* HIP runtime error                              */
+#define FCFD_EDIVERGED    -5   /* NaN/Inf detected in an update (ns.cc:350-355) */
+#define FCFD_ENOTSUP      -6   /* option not supported by this build             */
+#define FCFD_ENOMEM       -7
+
+/* ---- option values: same numbers as the reference headers ---- */
+/* BC types/kinds/specification (src/bc.h:28-51) */
+#define FCFD_BC_SYMMETRY 1
+#define FCFD_BC_WALL     2
+#define FCFD_BC_INLET    3
+#define FCFD_BC_OUTLET   4
+#define FCFD_KIND_NONE        -1
+#define FCFD_KIND_NO_REVERSE   1
+#define FCFD_KIND_DAMP_REVERSE 2
+#define FCFD_KIND_VELOCITY     3
+#define FCFD_KIND_MDOT         4
+#define FCFD_KIND_SLIP         5
+#define FCFD_KIND_STAGNATION   6
+#define FCFD_KIND_FORCE_SUPERSONIC 7
+#define FCFD_THERMAL_FIXED_T   1
+#define FCFD_THERMAL_FIXED_Q   2
+#define FCFD_THERMAL_ADIABATIC 3
+#define FCFD_SPEC_NONE  0
+#define FCFD_SPEC_RHO   1
+#define FCFD_SPEC_P     2
+#define FCFD_SPEC_T     3
+#define FCFD_SPEC_STATE 4
+/* numerics (src/navier_stokes/ns.h:40-55) */
+#define FCFD_LIMITER_NONE   -1
+#define FCFD_LIMITER_VK      1
+#define FCFD_LIMITER_BJ      2
+#define FCFD_LIMITER_MINMOD  3
+#define FCFD_ORDER_FIRST     1
+#define FCFD_ORDER_SECOND    2
+#define FCFD_FLUX_ROE        1
+#define FCFD_FLUX_VAN_LEER   2
+#define FCFD_FLUX_AUSM_PLUS_UP 3
+#define FCFD_FLUX_SD_SLAU    4
+#define FCFD_FLUX_SW         5
+/* gradient methods (src/gradient_maps.cc:26-28) */
+#define FCFD_GRAD_CURVILINEAR 1
+#define FCFD_GRAD_LSQR        2
+#define FCFD_GRAD_GREENGAUSS  3
+/* interpolation (src/interpolate/interpolate.h) */
+#define FCFD_INTERP_WTLI   1
+#define FCFD_INTERP_IDW    2
+#define FCFD_INTERP_SIMPLE 3
+/* material models (src/material/material.h:28-35) */
+#define FCFD_VISC_CONSTANT    1
+#define FCFD_VISC_SUTHERLAND  2
+#define FCFD_COND_CONSTANT    1
+#define FCFD_COND_PRANDTL     2
+/* time step type (src/time_step.h) */
+#define FCFD_DT_FIXED     1
+#define FCFD_DT_CFL_MAX   2
+#define FCFD_DT_CFL_LOCAL 3
+
+#define FCFD_MAX_BC 32
+
+/* Raw unstructured mesh, the content of the reference's GridRawData for
+ * cell-based input (src/grid/grid.h:61-78).  Cells use CGNS node order for
+ * TETRA_4 / PYRA_5 / PENTA_6 / HEXA_8.  bc node lists are raw node ids; each
+ * list is a set (order and duplicates do not matter). */
+typedef struct fcfd_mesh {
+    int n_nodes;
+    const double* xyz;          /* 3*n_nodes, x,y,z interleaved               */
+    int n_cells;
+    const int* cell_ptr;        /* n_cells+1 offsets into cell_nodes          */
+    const int* cell_nodes;      /* raw node ids                               */
+    int n_bc;                   /* number of BC regions (node sets)           */
+    const int* bc_ptr;          /* n_bc+1 offsets into bc_nodes               */
+    const int* bc_nodes;        /* raw node ids                               */
+    int dimension;              /* 2 or 3 (grid_N{dimension=...})             */
+} fcfd_mesh;
+
+/* One boundary region; fields mirror BCregion (src/bc.h:54-63) and the
+ * per-BC values set in NavierStokes::set_bcs (ns_set_bcs.cc:25-150). */
+typedef struct fcfd_bc {
+    int type;         /* FCFD_BC_*                                  */
+    int kind;         /* FCFD_KIND_*                                */
+    int specified;    /* FCFD_SPEC_*                                */
+    int thermal;      /* FCFD_THERMAL_*                             */
+    int p_fixed, T_fixed, rho_fixed, V_fixed, qdot_fixed;  /* Variable::fixedonBC */
+    double p, T, rho, V[3], p_total, T_total, mdot, qdot;
+    int include_in_loads;
+} fcfd_bc;
+
+typedef struct fcfd_options {
+    /* material (src/material/material.cc:29-106) */
+    double gamma, Mw, Pref, Tref;
+    int visc_model;  double mu, sut_mu_ref, sut_T_ref, sut_S;
+    int cond_model;  double lambda, Pr;
+    /* navierstokes section (src/read_inputs.cc:125-134; ns.cc:33-86) */
+    int order, jac_order, limiter, convective_flux;
+    double limiter_threshold, Minf, wdiss, bl_height;
+    /* gradients and interpolation sections (read_inputs.cc:47-54) */
+    int grad_hex, grad_prism, grad_other;
+    int interp_method, stencil_size;
+    double skewness_tol;
+    /* loads */
+    double moment_center[3];
+    int n_bc;
+    fcfd_bc bc[FCFD_MAX_BC];
+} fcfd_options;
+
+typedef struct fcfd_info {
+    int n_nodes, n_cells, n_ghosts, n_faces, n_bfaces;
+    int ld_cell;         /* padded leading dimension of cell SoA arrays (>= n_cells+n_ghosts) */
+    int ld_face;
+    int n_gradmap;       /* total gradient-map entries over interior cells */
+    int n_faceavg;       /* total face-average entries over faces          */
+    int n_nbr;           /* total neighborCells entries over all cells     */
+    int n_halffaces;     /* sum over interior cells of faces per cell      */
+    int nv;              /* equations per cell (5)                         */
+} fcfd_info;
+
+/* Metrics in reference numbering for bit-exact checks of the set-up.  Every
+ * pointer may be NULL (skipped); sizes come from fcfd_info. */
+typedef struct fcfd_metrics_out {
+    int* face_parent; int* face_neighbor; int* face_bc;   /* n_faces          */
+    int* face_node_ptr; int* face_nodes;                  /* n_faces+1, <=4*n_faces */
+    double* face_centroid; double* face_normal; double* face_area; /* 3F,3F,F */
+    double* face_tangent1;                                /* 3F (ns_assemble_linear_system.cc:337-341) */
+    double* cell_centroid; double* cell_volume;           /* 3(N+B), N+B      */
+    double* cell_lengthscale;                             /* N                */
+    double* wall_distance;                                /* N+B              */
+    int* cell_face_ptr; int* cell_faces;                  /* N+1, halffaces   */
+    int* nbr_ptr; int* nbr;                               /* N+B+1, n_nbr     */
+    int* grad_ptr; int* grad_idx; double* grad_w;         /* N+1, G, 3G       */
+    int* avg_ptr; int* avg_idx; double* avg_w;            /* F+1, A, A        */
+    int* cell_perm;                                       /* N+B: internal index of reference cell */
+    double* grid_lengthscale;                             /* 1                */
+} fcfd_metrics_out;
+
+typedef struct fcfd_ctx fcfd_ctx;
+
+/* Build the context: host set-up (connectivity, geometry, ghosts, BCs,
+ * length scales, interpolation weights, gradient maps, renumbering) and
+ * device upload.  Replaces Grid::setup .. gradient_maps (main.cc:181-189). */
+int  fcfd_create(const fcfd_mesh* mesh, const fcfd_options* opt, fcfd_ctx** out);
+/* Host-only set-up (no GPU needed): same as fcfd_create without device
+ * allocation; the context then only serves fcfd_info/fcfd_metrics. */
+int  fcfd_create_host(const fcfd_mesh* mesh, const fcfd_options* opt, fcfd_ctx** out);
+void fcfd_destroy(fcfd_ctx* ctx);
+const char* fcfd_last_error(const fcfd_ctx* ctx);
+int  fcfd_get_info(const fcfd_ctx* ctx, fcfd_info* info);
+int  fcfd_metrics(const fcfd_ctx* ctx, fcfd_metrics_out* out);
+
+/* Layout helpers.  W_host is N x 5 in reference numbering, AoS
+ * (p,u,v,w,T per cell, the order of Variable p, V, T in ns.h:104-106).
+ * W_dev is a device SoA array [5][ld_cell]. */
+int  fcfd_state_to_device(fcfd_ctx* ctx, const double* W_host, double* W_dev);
+int  fcfd_cells_to_host(fcfd_ctx* ctx, const double* X_dev, int nvar, int include_ghosts, double* X_host);
+
+/* FP64 residual R(W): ghost update (update_boundaries, decided semantics),
+ * gradients (calc_cell_grads), limiters (calc_limiter) and the face-flux
+ * loop (ns_assemble_linear_system.cc:75-140).  W_dev is [5][ld_cell]; its
+ * interior slots [0,N) are read and its boundary-ghost slots [N,N+B) are
+ * written with the ghost state (the reference keeps ghosts in the same
+ * Variable arrays, variable.h:84-89).  R_dev is [5][ld_cell], interior cells:
+ * R_c = sum over the faces of c, in reference face order, of +-(D - C).
+ * Stream-ordered on `stream` (a hipStream_t, or NULL for the null stream);
+ * no host synchronisation. */
+int  fcfd_residual(fcfd_ctx* ctx, double* W_dev, double* R_dev, void* stream);
+
+/* Gradients [15][ld_cell] (dp/dx,dp/dy,dp/dz, du/.., dv/.., dw/.., dT/..,
+ * ghosts carry their parent's) and limiters [5][ld_cell] of the last
+ * fcfd_residual call; device pointers owned by the context. */
+int  fcfd_internal_buffers(fcfd_ctx* ctx, const double** grad, const double** phi);
+
+/* Per-stage timing of fcfd_residual with HIP events recorded on the launch
+ * stream around its three kernels (ghosts, gradients+limiters, fluxes).
+ * Returns the accumulated milliseconds per stage and the number of timed
+ * calls; enable = 1 starts (and resets), 0 stops (and resets), -1 only
+ * reads.  No counterpart in the reference (its only timer is the wall clock
+ * of main.cc:445-448). */
+int  fcfd_profile(fcfd_ctx* ctx, int enable, double* stage_ms, long long* calls);
+
+/* Synchronises and reports run-time BC errors flagged by kernels since the
+ * last clear (velocity inlet turned supersonic without a complete state,
+ * ns_apply_bcs.cc:54-59 -> FCFD_EBC). */
+int  fcfd_status(fcfd_ctx* ctx, int clear);
+
+/* Local time step (update_time_step, src/time_step.cc:143-224) for
+ * CFL_LOCAL or CFL_MAX; dt_dev is [ld_cell]; *max_cfl as the reference. */
+int  fcfd_time_step(fcfd_ctx* ctx, const double* W_dev, int dt_type, double cfl, double* dt_dev, double* max_cfl, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FCFD_GPU_H */

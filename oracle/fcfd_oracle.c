@@ -1,0 +1,1661 @@
+/*
+ * fcfd_oracle.c -- TEST INFRASTRUCTURE ONLY (the parity checker).
+ *
+ * A line-by-line CPU restatement, in plain C and FP64, of Free CFD 2.0's
+ * unstructured finite-volume residual and FD-Jacobian path.  Every function
+ * names the reference file:line it follows (paths relative to the reference
+ * repository root).  Built with gcc -O2 -ffp-contract=off so that the
+ * arithmetic sequence is the reference's (x86-64 GCC -O3 emits no FMA).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+ * this code; the product library never links it.
+ *
+ * Decisions where the reference is undefined (documented in DESIGN.md):
+ *  D1  update_boundaries() has no source (declared ns.h:165).  Ghost values
+ *      are set per boundary face by apply_bcs() with the parent cell-centre
+ *      state as `left` and `right` pre-set to a copy of `left`; the ghost
+ *      receives right.{p,V,T}_center.
+ *  D2  outlet() "extrapolate everything" branch (ns_apply_bcs.cc:189-194)
+ *      leaves right.rho at a stale value from the previous face; here
+ *      right.rho = EOS(right.p,right.T) (== left.rho).
+ *  D3  curvilinear_grad_map() erases std::set elements it is iterating
+ *      (curvilinear_grad_map.cc:57-78); here the intended greedy pairing is
+ *      used: take the lowest remaining face slot, pair it with the most
+ *      opposite remaining later slot (first minimum wins), remove both.
+ *  D4  degenerate cells (repeated nodes) are rejected (FCFD_EMESH).
+ *  D5  WTLI with stencil size > 2 (tetra/tri fits) is rejected (FCFD_ENOTSUP).
+ */
+#include "fcfd_oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* Vec3D (src/vec3d/vec3d.cc:29-211): same operation order            */
+/* ------------------------------------------------------------------ */
+typedef struct { double c[3]; } V3;
+static V3 v3(double x, double y, double z) { V3 r; r.c[0] = x; r.c[1] = y; r.c[2] = z; return r; }
+static V3 vadd(V3 a, V3 b) { return v3(a.c[0] + b.c[0], a.c[1] + b.c[1], a.c[2] + b.c[2]); }
+static V3 vsub(V3 a, V3 b) { return v3(a.c[0] - b.c[0], a.c[1] - b.c[1], a.c[2] - b.c[2]); }
+static V3 vsc(double s, V3 a) { return v3(s * a.c[0], s * a.c[1], s * a.c[2]); }     /* operator*(double,Vec3D) */
+static V3 vmul(V3 a, double s) { return v3(a.c[0] * s, a.c[1] * s, a.c[2] * s); }     /* Vec3D*double */
+static V3 vdiv(V3 a, double s) { return v3(a.c[0] / s, a.c[1] / s, a.c[2] / s); }
+static double vdot(V3 a, V3 b) { return a.c[0] * b.c[0] + a.c[1] * b.c[1] + a.c[2] * b.c[2]; } /* vec3d.cc:35 */
+static double vmag(V3 a) { return sqrt(a.c[0] * a.c[0] + a.c[1] * a.c[1] + a.c[2] * a.c[2]); } /* vec3d.cc:39 */
+static V3 vnorm(V3 a) { return vdiv(a, vmag(a)); }                                   /* vec3d.cc:51 */
+static V3 vcross(V3 a, V3 b) {                                                        /* vec3d.cc:43 */
+    return v3(a.c[1] * b.c[2] - a.c[2] * b.c[1], -a.c[0] * b.c[2] + a.c[2] * b.c[0], a.c[0] * b.c[1] - a.c[1] * b.c[0]);
+}
+static V3 vzero(void) { return v3(0., 0., 0.); }
+
+static double dmin(double a, double b) { return (b < a) ? b : a; }   /* std::min */
+static double dmax(double a, double b) { return (a < b) ? b : a; }   /* std::max */
+
+/* growable int list */
+typedef struct { int n, cap; int* a; } ivec;
+static void iv_push(ivec* v, int x) {
+    if (v->n == v->cap) { v->cap = v->cap ? 2 * v->cap : 8; v->a = (int*)realloc(v->a, sizeof(int) * v->cap); }
+    v->a[v->n++] = x;
+}
+static int iv_has(const ivec* v, int x) { for (int i = 0; i < v->n; ++i) if (v->a[i] == x) return 1; return 0; }
+static void iv_free(ivec* v) { free(v->a); v->a = NULL; v->n = v->cap = 0; }
+
+/* ordered map int -> V3 (std::map<int,Vec3D> semantics) */
+#define MAPMAX 512
+typedef struct { int n; int key[MAPMAX]; V3 val[MAPMAX]; } vmap;
+static int vm_find(const vmap* m, int k) { for (int i = 0; i < m->n; ++i) if (m->key[i] == k) return i; return -1; }
+static int vm_insert(vmap* m, int k, V3 v) { /* std::map::insert: no-op if present */
+    if (vm_find(m, k) >= 0) return 0;
+    if (m->n >= MAPMAX) return -1;
+    int i = m->n;
+    while (i > 0 && m->key[i - 1] > k) { m->key[i] = m->key[i - 1]; m->val[i] = m->val[i - 1]; --i; }
+    m->key[i] = k; m->val[i] = v; m->n++;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* context                                                            */
+/* ------------------------------------------------------------------ */
+typedef struct {   /* NS_Cell_State (ns_state_cache.h:26-48) */
+    double p, p_center, T, T_center, rho, a, H, volume;
+    V3 V, V_center, Vn;
+    double update[5];
+} cstate;
+typedef struct {   /* NS_Face_State (ns_state_cache.h:50-59) */
+    int index, parent, neighbor;
+    double T, mu, lambda;
+    V3 V, gradu, gradv, gradw, gradT, normal, tangent1, tangent2, left2right;
+    double area;
+    int bc;
+} fstate;
+
+struct orc_ctx {
+    fcfd_options opt;
+    int dim;
+    double R;               /* gas constant UNIV_GAS_CONST/Mw (material.h:26) */
+    double Cp;              /* ideal gas gamma*R/(gamma-1) (material.cc:142-145) */
+    /* nodes */
+    int nnode; V3* node; ivec* node_cells;
+    /* cells: [0,ncell) interior, [ncell,ntot) boundary ghosts */
+    int ncell, ntot, ncap;
+    ivec* cnodes; ivec* cfaces; ivec* cnbr;
+    V3* cc; double* vol; double* lscale; double* wdist; int* cbc;
+    /* faces */
+    int nface, fcap;
+    int *fparent, *fneigh, *fbc, *fsym, *face2bc;
+    ivec* fnodes;
+    V3 *fc, *fn;
+    double *farea, *fwd, *fdiss;
+    double glscale, total_volume;
+    /* BC node flags and areas */
+    int nbc; unsigned char** bcflag; double* bc_area; double* bc_total_area;
+    /* interpolation + gradient maps (CSR, ascending keys) */
+    int *avg_ptr, *avg_idx; double* avg_w;
+    int *g_ptr, *g_idx; V3* g_w;
+    /* state */
+    double *p, *u, *v, *w, *T;      /* ntot */
+    V3 *gp, *gu, *gv, *gw, *gT;     /* ntot */
+    double* phi;                    /* ntot*5 */
+    double* upd;                    /* ncell*5 */
+    double *Rs, *Rabs;              /* ncell*5 */
+    double *JL, *JR, *Dg;           /* F*25, F*25, N*25 */
+    double* loads;                  /* nbc*6 */
+    int err; char msg[256];
+};
+
+static char g_err[256];
+static void seterr(orc_ctx* o, int code, const char* m) {
+    if (o) { o->err = code; snprintf(o->msg, sizeof o->msg, "%s", m); }
+    snprintf(g_err, sizeof g_err, "%s", m);
+}
+const char* orc_last_error(const orc_ctx* o) { return o ? o->msg : g_err; }
+
+/* ------------------------------------------------------------------ */
+/* material (src/material/material.cc:108-153)                        */
+/* ------------------------------------------------------------------ */
+static double m_rho(const orc_ctx* o, double p, double T) { return (p + o->opt.Pref) / (o->R * (T + o->opt.Tref)); }
+static double m_p(const orc_ctx* o, double rho, double T) { return rho * o->R * (T + o->opt.Tref) - o->opt.Pref; }
+static double m_T(const orc_ctx* o, double p, double rho) { return (p + o->opt.Pref) / (o->R * rho) - o->opt.Tref; }
+static double m_a(const orc_ctx* o, double p, double T) { return sqrt(o->opt.gamma * (p + o->opt.Pref) / m_rho(o, p, T)); }
+static double m_visc(const orc_ctx* o, double T) {
+    if (o->opt.visc_model == FCFD_VISC_SUTHERLAND)
+        return o->opt.sut_mu_ref * pow((T + o->opt.Tref) / o->opt.sut_T_ref, 1.5) * (o->opt.sut_T_ref + o->opt.sut_S) / (T + o->opt.Tref + o->opt.sut_S);
+    return o->opt.mu;
+}
+static double m_cond(const orc_ctx* o, double T) {
+    if (o->opt.cond_model == FCFD_COND_PRANDTL) return o->Cp * m_visc(o, T) / o->opt.Pr;
+    return o->opt.lambda;
+}
+
+/* ------------------------------------------------------------------ */
+/* Grid::create_nodes_cells (src/grid/grid_create_elements.cc:27-182) */
+/* ------------------------------------------------------------------ */
+static int create_nodes_cells(orc_ctx* o, const fcfd_mesh* m) {
+    int* g2l = (int*)malloc(sizeof(int) * (size_t)m->n_nodes);
+    for (int i = 0; i < m->n_nodes; ++i) g2l[i] = -1;
+    o->ncell = m->n_cells;
+    o->ncap = m->n_cells + 1;
+    o->cnodes = (ivec*)calloc(o->ncap, sizeof(ivec));
+    o->cfaces = (ivec*)calloc(o->ncap, sizeof(ivec));
+    o->node = (V3*)malloc(sizeof(V3) * (size_t)m->n_nodes);
+    o->nnode = 0;
+    for (int c = 0; c < m->n_cells; ++c) {                       /* :35-103 */
+        int nn = m->cell_ptr[c + 1] - m->cell_ptr[c];
+        if (nn != 4 && nn != 5 && nn != 6 && nn != 8) { free(g2l); seterr(o, FCFD_EMESH, "unsupported cell node count"); return FCFD_EMESH; }
+        for (int n = 0; n < nn; ++n) {
+            int ng = m->cell_nodes[m->cell_ptr[c] + n];
+            if (ng < 0 || ng >= m->n_nodes) { free(g2l); seterr(o, FCFD_EINVAL, "node id out of range"); return FCFD_EINVAL; }
+            if (g2l[ng] < 0) {
+                g2l[ng] = o->nnode;
+                o->node[o->nnode] = v3(m->xyz[3 * ng], m->xyz[3 * ng + 1], m->xyz[3 * ng + 2]);
+                o->nnode++;
+            }
+            iv_push(&o->cnodes[c], g2l[ng]);
+        }
+        int nf = (nn == 4) ? 4 : (nn == 5) ? 5 : (nn == 6) ? 5 : 6; /* :68-81 */
+        for (int i = 0; i < nf; ++i) iv_push(&o->cfaces[c], -1);
+    }
+    /* node -> cells (:126-142) */
+    o->node_cells = (ivec*)calloc(o->nnode, sizeof(ivec));
+    for (int c = 0; c < o->ncell; ++c)
+        for (int cn = 0; cn < o->cnodes[c].n; ++cn) {
+            int n = o->cnodes[c].a[cn];
+            if (!iv_has(&o->node_cells[n], c)) iv_push(&o->node_cells[n], c);
+        }
+    /* cell -> node-sharing neighbours incl. self (:146-164) */
+    o->cnbr = (ivec*)calloc(o->ncap, sizeof(ivec));
+    for (int c = 0; c < o->ncell; ++c)
+        for (int cn = 0; cn < o->cnodes[c].n; ++cn) {
+            int n = o->cnodes[c].a[cn];
+            for (int nc = 0; nc < o->node_cells[n].n; ++nc) {
+                int c2 = o->node_cells[n].a[nc];
+                if (!iv_has(&o->cnbr[c], c2)) iv_push(&o->cnbr[c], c2);
+            }
+        }
+    /* BC node sets in local numbering (:168-178) */
+    o->nbc = m->n_bc;
+    o->bcflag = (unsigned char**)calloc(o->nbc > 0 ? o->nbc : 1, sizeof(unsigned char*));
+    for (int b = 0; b < o->nbc; ++b) {
+        o->bcflag[b] = (unsigned char*)calloc(o->nnode > 0 ? o->nnode : 1, 1);
+        for (int k = m->bc_ptr[b]; k < m->bc_ptr[b + 1]; ++k) {
+            int ng = m->bc_nodes[k];
+            if (ng >= 0 && ng < m->n_nodes && g2l[ng] >= 0) o->bcflag[b][g2l[ng]] = 1;
+        }
+    }
+    free(g2l);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Grid::create_faces (grid_create_elements.cc:184-442)               */
+/* ------------------------------------------------------------------ */
+static const int hexaFaces[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {1, 2, 6, 5}, {0, 4, 7, 3}, {1, 5, 4, 0}, {2, 3, 7, 6}};
+static const int prismFaces[5][4] = {{0, 2, 1, 0}, {3, 4, 5, 0}, {0, 3, 5, 2}, {1, 2, 5, 4}, {0, 1, 4, 3}};
+static const int pyraFaces[5][4] = {{0, 3, 2, 1}, {0, 1, 4, 0}, {1, 2, 4, 0}, {3, 4, 2, 0}, {0, 4, 3, 0}};
+static const int tetraFaces[4][3] = {{0, 2, 1}, {1, 2, 3}, {0, 3, 2}, {0, 1, 3}};
+
+static int have_nodes(const orc_ctx* o, int c, int nl, const int* list) {   /* :978-986 */
+    for (int i = 0; i < nl; ++i) {
+        int match = 0;
+        for (int j = 0; j < o->cnodes[c].n; ++j) if (list[i] == o->cnodes[c].a[j]) { match = 1; break; }
+        if (!match) return 0;
+    }
+    return 1;
+}
+
+static void face_grow(orc_ctx* o) {
+    if (o->nface < o->fcap) return;
+    int nc = o->fcap ? 2 * o->fcap : 1024;
+    o->fparent = (int*)realloc(o->fparent, sizeof(int) * nc);
+    o->fneigh = (int*)realloc(o->fneigh, sizeof(int) * nc);
+    o->fbc = (int*)realloc(o->fbc, sizeof(int) * nc);
+    o->fnodes = (ivec*)realloc(o->fnodes, sizeof(ivec) * nc);
+    memset(o->fnodes + o->fcap, 0, sizeof(ivec) * (nc - o->fcap));
+    o->fcap = nc;
+}
+
+static int create_faces(orc_ctx* o) {
+    o->nface = 0;
+    for (int c = 0; c < o->ncell; ++c) {
+        int nn = o->cnodes[c].n;
+        for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
+            int fnn;
+            switch (nn) {
+            case 4: fnn = 3; break;
+            case 5: fnn = (cf < 1) ? 4 : 3; break;
+            case 6: fnn = (cf < 2) ? 3 : 4; break;
+            default: fnn = 4; break;
+            }
+            int tn[4];
+            for (int fn = 0; fn < fnn; ++fn) {
+                switch (nn) {
+                case 4: tn[fn] = o->cnodes[c].a[tetraFaces[cf][fn]]; break;
+                case 5: tn[fn] = o->cnodes[c].a[pyraFaces[cf][fn]]; break;
+                case 6: tn[fn] = o->cnodes[c].a[prismFaces[cf][fn]]; break;
+                default: tn[fn] = o->cnodes[c].a[hexaFaces[cf][fn]]; break;
+                }
+            }
+            for (int fn = 1; fn < fnn; ++fn)                         /* :274-294 */
+                for (int i = 0; i < fn; ++i)
+                    if (tn[fn] == tn[i]) { seterr(o, FCFD_EMESH, "degenerate cell (repeated nodes) not supported"); return FCFD_EMESH; }
+            int internal = 0, unique = 1, neighbor = -1;              /* :296-314 */
+            int n0 = tn[0];
+            for (int nc = 0; nc < o->node_cells[n0].n; ++nc) {
+                int i = o->node_cells[n0].a[nc];
+                if (i != c && i < o->ncell && have_nodes(o, i, fnn, tn)) {
+                    if (i > c) { neighbor = i; internal = 1; }
+                    else unique = 0;
+                }
+            }
+            if (!unique) continue;
+            int bcv = -1;  /* INTERNAL_FACE */
+            if (!internal) {                                          /* :318-363 */
+                bcv = -2;  /* UNASSIGNED_FACE */
+                int matched[FCFD_MAX_BC], nmatched = 0, cell_matched = -1;
+                for (int nbc = 0; nbc < o->nbc; ++nbc) {
+                    int match = 1;
+                    for (int i = 0; i < fnn; ++i) if (!o->bcflag[nbc][tn[i]]) { match = 0; break; }
+                    if (match && nmatched < FCFD_MAX_BC) matched[nmatched++] = nbc;
+                    if (cell_matched == -1) {
+                        match = 1;
+                        for (int i = 0; i < nn; ++i) if (!o->bcflag[nbc][o->cnodes[c].a[i]]) { match = 0; break; }
+                        if (match) cell_matched = nbc;
+                    }
+                }
+                if (nmatched > 1) {
+                    for (int k = 0; k < nmatched; ++k) if (matched[k] != cell_matched) { bcv = matched[k]; break; }
+                } else if (nmatched == 1) bcv = matched[0];
+            }
+            face_grow(o);
+            int f = o->nface;
+            for (int i = 0; i < o->cfaces[c].n; ++i) if (o->cfaces[c].a[i] < 0) { o->cfaces[c].a[i] = f; break; }  /* :364-369 */
+            if (internal)
+                for (int i = 0; i < o->cfaces[neighbor].n; ++i) if (o->cfaces[neighbor].a[i] < 0) { o->cfaces[neighbor].a[i] = f; break; }
+            o->fparent[f] = c; o->fneigh[f] = neighbor; o->fbc[f] = bcv;
+            o->fnodes[f].n = 0;
+            for (int i = 0; i < fnn; ++i) iv_push(&o->fnodes[f], tn[i]);
+            o->nface++;
+        }
+    }
+    for (int f = 0; f < o->nface; ++f) if (o->fbc[f] == -2) { seterr(o, FCFD_EBC, "boundary condition could not be found for a face"); return FCFD_EBC; } /* set_bcs.cc:147-150 */
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Grid::areas_volumes (src/grid/grid.cc:123-178)                      */
+/* ------------------------------------------------------------------ */
+static void areas_volumes(orc_ctx* o) {
+    o->fc = (V3*)malloc(sizeof(V3) * o->nface);
+    o->fn = (V3*)malloc(sizeof(V3) * o->nface);
+    o->farea = (double*)malloc(sizeof(double) * o->nface);
+    for (int f = 0; f < o->nface; ++f) {
+        ivec* fnod = &o->fnodes[f];
+        if (fnod->n == 4) {
+            V3 d1 = vsub(o->node[fnod->a[2]], o->node[fnod->a[0]]);
+            V3 d2 = vsub(o->node[fnod->a[3]], o->node[fnod->a[1]]);
+            V3 nrm = vcross(d1, d2);
+            o->farea[f] = 0.5 * vmag(nrm);
+            o->fn[f] = vnorm(nrm);
+        } else {
+            V3 areaVec = vzero();
+            V3 cen = o->node[fnod->a[0]];
+            for (int n = 1; n < fnod->n; ++n) {
+                int next = n + 1;
+                if (next == fnod->n) next = 0;
+                V3 patch = vsc(0.5, vcross(vsub(o->node[fnod->a[n]], cen), vsub(o->node[fnod->a[next]], cen)));
+                areaVec = vadd(areaVec, patch);
+            }
+            o->farea[f] = vmag(areaVec);
+            o->fn[f] = vnorm(areaVec);
+        }
+        V3 s = vzero();
+        for (int n = 0; n < fnod->n; ++n) s = vadd(s, o->node[fnod->a[n]]);
+        o->fc[f] = vdiv(s, (double)fnod->n);
+    }
+    o->cc = (V3*)malloc(sizeof(V3) * o->ncap);
+    o->vol = (double*)malloc(sizeof(double) * o->ncap);
+    double total = 0.;
+    for (int c = 0; c < o->ncell; ++c) {
+        V3 s = vzero();
+        for (int cn = 0; cn < o->cnodes[c].n; ++cn) s = vadd(s, o->node[o->cnodes[c].a[cn]]);
+        o->cc[c] = vdiv(s, (double)o->cnodes[c].n);
+        o->vol[c] = 0.;
+        for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
+            int f = o->cfaces[c].a[cf];
+            V3 base = vsc(o->farea[f], o->fn[f]);
+            if (o->fparent[f] == c) base = vmul(base, -1.);
+            V3 bn = vnorm(base);
+            V3 height = vsc(vdot(vsub(o->cc[c], o->fc[f]), bn), bn);
+            o->vol[c] += vdot(base, height) / 3.;
+        }
+        total += o->vol[c];
+    }
+    o->total_volume = total;
+}
+
+/* ------------------------------------------------------------------ */
+/* Grid::create_boundary_ghosts (grid_create_elements.cc:702-772)      */
+/* ------------------------------------------------------------------ */
+static void create_boundary_ghosts(orc_ctx* o) {
+    int nb = 0;
+    for (int f = 0; f < o->nface; ++f) if (o->fbc[f] >= 0) nb++;
+    int ntot = o->ncell + nb;
+    o->ncap = ntot + 1;
+    o->cnodes = (ivec*)realloc(o->cnodes, sizeof(ivec) * o->ncap);
+    o->cfaces = (ivec*)realloc(o->cfaces, sizeof(ivec) * o->ncap);
+    o->cnbr = (ivec*)realloc(o->cnbr, sizeof(ivec) * o->ncap);
+    memset(o->cnodes + o->ncell, 0, sizeof(ivec) * (o->ncap - o->ncell));
+    memset(o->cfaces + o->ncell, 0, sizeof(ivec) * (o->ncap - o->ncell));
+    memset(o->cnbr + o->ncell, 0, sizeof(ivec) * (o->ncap - o->ncell));
+    o->cc = (V3*)realloc(o->cc, sizeof(V3) * o->ncap);
+    o->vol = (double*)realloc(o->vol, sizeof(double) * o->ncap);
+    o->cbc = (int*)malloc(sizeof(int) * o->ncap);
+    for (int c = 0; c < o->ncell; ++c) o->cbc[c] = -1;
+    int g = o->ncell;
+    for (int b = 0; b < o->nbc; ++b)
+        for (int f = 0; f < o->nface; ++f)
+            if (o->fbc[f] == b) {
+                int par = o->fparent[f];
+                o->cbc[g] = b;
+                o->vol[g] = o->vol[par];
+                o->cc[g] = vadd(o->cc[par], vsc(2. * vdot(vsub(o->fc[f], o->cc[par]), o->fn[f]), o->fn[f])); /* :725-726 */
+                o->fneigh[f] = g;
+                for (int fn = 0; fn < o->fnodes[f].n; ++fn) iv_push(&o->node_cells[o->fnodes[f].a[fn]], g);
+                g++;
+            }
+    o->ntot = g;
+    for (int c = 0; c < o->ncell; ++c)                                /* :746-769 */
+        for (int cn = 0; cn < o->cnodes[c].n; ++cn) {
+            int n = o->cnodes[c].a[cn];
+            for (int ng = 0; ng < o->node_cells[n].n; ++ng) {
+                int gg = o->node_cells[n].a[ng];
+                if (gg >= o->ncell && !iv_has(&o->cnbr[c], gg)) {
+                    iv_push(&o->cnbr[c], gg);
+                    iv_push(&o->cnbr[gg], c);
+                }
+            }
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* set_bcs (src/set_bcs.cc:35-260), BC regions = mesh node sets       */
+/* ------------------------------------------------------------------ */
+static void set_bcs_grid(orc_ctx* o) {
+    o->fsym = (int*)calloc(o->nface, sizeof(int));
+    o->face2bc = (int*)calloc(o->nface, sizeof(int));
+    o->bc_area = (double*)calloc(o->nbc + 1, sizeof(double));
+    o->bc_total_area = (double*)calloc(o->nbc + 1, sizeof(double));
+    for (int b = 0; b < o->nbc; ++b) {                                /* :86-98 */
+        int counter = 0;
+        double area = 0.;
+        for (int f = 0; f < o->nface; ++f)
+            if (o->fbc[f] == b) {
+                o->fsym[f] = 0;
+                o->face2bc[f] = counter++;
+                area += o->farea[f];
+                if (o->opt.bc[b].type == FCFD_BC_SYMMETRY) o->fsym[f] = 1;
+            }
+        o->bc_area[b] = area;
+        o->bc_total_area[b] = area;
+    }
+    for (int f = 0; f < o->nface; ++f) {                              /* :146-161 */
+        int b = o->fbc[f];
+        if (b >= 0 && (o->opt.bc[b].type == FCFD_BC_INLET || o->opt.bc[b].type == FCFD_BC_OUTLET)) {
+            int nb = o->fneigh[f], par = o->fparent[f];
+            o->cc[nb] = vadd(o->cc[par], vsc(2., vsub(o->fc[f], o->cc[par])));
+        }
+    }
+    /* closest wall distance, brute force (:178-249) */
+    int nsf = 0;
+    for (int f = 0; f < o->nface; ++f) {
+        int b = o->fbc[f];
+        if (b >= 0 && o->opt.bc[b].type == FCFD_BC_WALL && o->opt.bc[b].kind != FCFD_KIND_SLIP) nsf++;
+    }
+    V3* wc = (V3*)malloc(sizeof(V3) * (nsf + 1));
+    V3* wn = (V3*)malloc(sizeof(V3) * (nsf + 1));
+    int k = 0;
+    for (int f = 0; f < o->nface; ++f) {
+        int b = o->fbc[f];
+        if (b >= 0 && o->opt.bc[b].type == FCFD_BC_WALL && o->opt.bc[b].kind != FCFD_KIND_SLIP) { wc[k] = o->fc[f]; wn[k] = o->fn[f]; k++; }
+    }
+    o->wdist = (double*)malloc(sizeof(double) * o->ncap);
+    for (int c = 0; c < o->ntot; ++c) {
+        o->wdist[c] = 1.e20;
+        for (int s = 0; s < nsf; ++s) {
+            double d = vmag(vsub(o->cc[c], wc[s]));
+            if (o->wdist[c] > d) o->wdist[c] = d;
+        }
+    }
+    o->fwd = (double*)malloc(sizeof(double) * o->nface);
+    o->fdiss = (double*)malloc(sizeof(double) * o->nface);
+    for (int f = 0; f < o->nface; ++f) {
+        int b = o->fbc[f];
+        o->fwd[f] = 1.e20;
+        if (b < 0 || o->opt.bc[b].type != FCFD_BC_WALL || o->opt.bc[b].kind == FCFD_KIND_SLIP) {
+            V3 fN = vzero();
+            for (int s = 0; s < nsf; ++s) {
+                double d = vmag(vsub(o->fc[f], wc[s]));
+                if (o->fwd[f] > d) { o->fwd[f] = d; fN = wn[s]; }
+            }
+            o->fdiss[f] = 1. - fabs(fN.c[0] * o->fn[f].c[0] + fN.c[1] * o->fn[f].c[1] + fN.c[2] * o->fn[f].c[2]);
+        } else { o->fwd[f] = 0.; o->fdiss[f] = 0.; }
+    }
+    free(wc); free(wn);
+}
+
+/* set_lengthScales (src/main.cc:456-491) */
+static void set_length_scales(orc_ctx* o) {
+    o->lscale = (double*)malloc(sizeof(double) * o->ncap);
+    for (int c = 0; c < o->ncell; ++c) {
+        o->lscale[c] = 1.e20;
+        for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
+            int f = o->cfaces[c].a[cf];
+            double h = fabs(vdot(o->fn[f], vsub(o->fc[f], o->cc[c])));
+            int skip = (o->fbc[f] >= 0 && o->opt.bc[o->fbc[f]].type == FCFD_BC_SYMMETRY);
+            if (!skip) o->lscale[c] = dmin(o->lscale[c], h);
+        }
+    }
+    for (int c = o->ncell; c < o->ntot; ++c) o->lscale[c] = 0.;
+    if (o->dim == 3) o->glscale = pow(o->total_volume, 1. / 3.);
+    else {
+        o->glscale = 0.;
+        for (int b = 0; b < o->nbc; ++b)
+            if (o->opt.bc[b].type == FCFD_BC_SYMMETRY) o->glscale = dmax(o->glscale, sqrt(0.5 * o->bc_total_area[b]));
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* face_interpolation_weights (src/face_interpolation_weights.cc:32-138)
+ * + Interpolate::calculate_weights/sort_stencil/interpolate_line
+ * (src/interpolate/interpolate.cc:36-121,320-343)                      */
+/* ------------------------------------------------------------------ */
+static int cmp_int(const void* a, const void* b) { int x = *(const int*)a, y = *(const int*)b; return (x > y) - (x < y); }
+
+static int face_interpolation_weights(orc_ctx* o) {
+    int method = o->opt.interp_method;
+    int maxs = o->opt.stencil_size;
+    if (maxs < 1) maxs = (o->dim == 3) ? 12 : (o->dim == 2 ? 6 : 2);
+    if (method == FCFD_INTERP_IDW) maxs = 99;
+    if (method == FCFD_INTERP_WTLI && maxs > 2) { seterr(o, FCFD_ENOTSUP, "WTLI with stencil size > 2 is not supported"); return FCFD_ENOTSUP; }
+    o->avg_ptr = (int*)malloc(sizeof(int) * (o->nface + 1));
+    int cap = o->nface * 2 + 16, nent = 0;
+    o->avg_idx = (int*)malloc(sizeof(int) * cap);
+    o->avg_w = (double*)malloc(sizeof(double) * cap);
+    int* sidx = (int*)malloc(sizeof(int) * 4096);
+    V3* spts = (V3*)malloc(sizeof(V3) * 4096);
+    int* setbuf = (int*)malloc(sizeof(int) * 4096);
+    for (int f = 0; f < o->nface; ++f) {
+        o->avg_ptr[f] = nent;
+        V3 point = o->fc[f];
+        int c = o->fparent[f], n = o->fneigh[f];
+        int ns = 0;
+        sidx[ns] = c; spts[ns++] = o->cc[c];
+        sidx[ns] = n; spts[ns++] = o->cc[n];
+        if (ns < maxs) {                                               /* :76-93 */
+            int nset = 0;
+            for (int k = 0; k < o->cnbr[c].n; ++k) setbuf[nset++] = o->cnbr[c].a[k];
+            if (o->fbc[f] == -1) for (int k = 0; k < o->cnbr[n].n; ++k) setbuf[nset++] = o->cnbr[n].a[k];
+            qsort(setbuf, nset, sizeof(int), cmp_int);
+            int prev = -999999;
+            for (int k = 0; k < nset; ++k) {
+                int x = setbuf[k];
+                if (x == prev) continue;
+                prev = x;
+                if (x == c || x == n) continue;
+                sidx[ns] = x; spts[ns++] = o->cc[x];
+            }
+        }
+        double wts[4096];
+        int nw = 0;
+        if (method == FCFD_INTERP_WTLI) {
+            /* sort_stencil with is_internal=true keeps the first two; with
+             * max stencil 2 the stencil is [parent, neighbour] */
+            V3 edge1 = vsub(spts[1], spts[0]);
+            V3 basis1 = vnorm(edge1);
+            V3 pp = vsub(point, spts[0]);
+            pp = vsc(vdot(pp, basis1), basis1);
+            wts[1] = vdot(pp, basis1) / vmag(edge1);
+            wts[0] = 1. - wts[1];
+            nw = 2;
+        } else if (method == FCFD_INTERP_IDW) {                        /* :109-118 */
+            double sum = 0.;
+            for (int s = 0; s < ns; ++s) { wts[s] = vmag(vsub(o->fc[f], spts[s])); sum += wts[s]; }
+            for (int s = 0; s < ns; ++s) wts[s] /= sum;
+            nw = ns;
+        } else {                                                       /* :120-124 SIMPLE */
+            wts[0] = 0.5; wts[1] = 0.5; nw = 2;
+        }
+        /* std::map insert order -> ascending keys */
+        if (nent + nw > cap) {
+            cap = 2 * cap + nw;
+            o->avg_idx = (int*)realloc(o->avg_idx, sizeof(int) * cap);
+            o->avg_w = (double*)realloc(o->avg_w, sizeof(double) * cap);
+        }
+        int base = nent;
+        for (int s = 0; s < nw; ++s) {
+            int key = sidx[s], dup = 0;
+            for (int q = base; q < nent; ++q) if (o->avg_idx[q] == key) { dup = 1; break; }
+            if (dup) continue;
+            int q = nent;
+            while (q > base && o->avg_idx[q - 1] > key) { o->avg_idx[q] = o->avg_idx[q - 1]; o->avg_w[q] = o->avg_w[q - 1]; --q; }
+            o->avg_idx[q] = key; o->avg_w[q] = wts[s];
+            nent++;
+        }
+    }
+    o->avg_ptr[o->nface] = nent;
+    free(sidx); free(spts); free(setbuf);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* gradient maps (src/gradient_maps.cc:39-78)                          */
+/* ------------------------------------------------------------------ */
+static void lsqr_grad_map(orc_ctx* o, int ci, vmap* gm) {      /* src/lsqr_grad_map.cc:32-99 */
+    static int st[MAPMAX]; static V3 dist[MAPMAX]; static double w[MAPMAX];
+    int ns = 0;
+    for (int i = 0; i < o->cnbr[ci].n; ++i) { int nb = o->cnbr[ci].a[i]; if (nb != ci && ns < MAPMAX) st[ns++] = nb; }
+    double a = 0., b = 0., c = 0., d = 0., e = 0., f = 0.;
+    for (int k = 0; k < ns; ++k) {
+        dist[k] = vsub(o->cc[st[k]], o->cc[ci]);
+        w[k] = vdot(dist[k], dist[k]);
+        a += w[k] * dist[k].c[0] * dist[k].c[0];
+        b += w[k] * dist[k].c[0] * dist[k].c[1];
+        c += w[k] * dist[k].c[0] * dist[k].c[2];
+        d += w[k] * dist[k].c[1] * dist[k].c[1];
+        e += w[k] * dist[k].c[1] * dist[k].c[2];
+        f += w[k] * dist[k].c[2] * dist[k].c[2];
+    }
+    for (int cf = 0; cf < o->cfaces[ci].n; ++cf) {                      /* :66-81 */
+        int fi = o->cfaces[ci].a[cf];
+        if (o->fsym[fi] && ns < MAPMAX) {
+            st[ns] = ci;
+            V3 point = vadd(o->cc[ci], vsc(vdot(vsub(o->fc[fi], o->cc[ci]), o->fn[fi]), o->fn[fi]));
+            dist[ns] = vsub(point, o->cc[ci]);
+            int k = ns;
+            w[k] = vdot(dist[k], dist[k]);
+            a += w[k] * dist[k].c[0] * dist[k].c[0];
+            b += w[k] * dist[k].c[0] * dist[k].c[1];
+            c += w[k] * dist[k].c[0] * dist[k].c[2];
+            d += w[k] * dist[k].c[1] * dist[k].c[1];
+            e += w[k] * dist[k].c[1] * dist[k].c[2];
+            f += w[k] * dist[k].c[2] * dist[k].c[2];
+            ns++;
+        }
+    }
+    double det = a * d * f - a * e * e - b * b * f - c * c * d + 2. * b * c * e;
+    gm->n = 0;
+    for (int k = 0; k < ns; ++k) {                                      /* :87-96 */
+        V3 wt;
+        wt.c[0] = w[k] * (dist[k].c[0] * (d * f - e * e) + dist[k].c[1] * (c * e - b * f) + dist[k].c[2] * (b * e - c * d));
+        wt.c[1] = w[k] * (dist[k].c[0] * (c * e - b * f) + dist[k].c[1] * (a * f - c * c) + dist[k].c[2] * (b * c - a * e));
+        wt.c[2] = w[k] * (dist[k].c[0] * (b * e - c * d) + dist[k].c[1] * (b * c - a * e) + dist[k].c[2] * (a * d - b * b));
+        wt = vdiv(wt, det);
+        vm_insert(gm, st[k], wt);
+        int s = vm_find(gm, ci);
+        if (s < 0) vm_insert(gm, ci, vmul(wt, -1.));
+        else gm->val[s] = vsub(gm->val[s], wt);
+    }
+}
+
+static int other_cell(const orc_ctx* o, int f, int c) { int x = o->fneigh[f]; if (x == c) x = o->fparent[f]; return x; }
+
+static void curvilinear_grad_map(orc_ctx* o, int c, vmap* gm) { /* src/curvilinear_grad_map.cc:28-228 */
+    int face_pairs[6] = {-1, -1, -1, -1, -1, -1};
+    int cell_pairs[6] = {-1, -1, -1, -1, -1, -1};
+    V3 Jac[3] = {vzero(), vzero(), vzero()};
+    int nf = o->cfaces[c].n;
+    if (o->cnodes[c].n == 8) {
+        /* D3: greedy pairing over the remaining face slots in ascending order */
+        int rem[8], nrem = nf;
+        for (int i = 0; i < nf; ++i) rem[i] = i;
+        int counter = 0;
+        while (nrem > 0 && counter < 3) {
+            int s1 = rem[0];
+            double min_product = 1.e20;
+            int f1 = o->cfaces[c].a[s1];
+            V3 my = o->fn[f1];
+            if (o->fparent[f1] != c) my = vmul(my, -1.);
+            face_pairs[2 * counter + 1] = s1;
+            for (int k = 1; k < nrem; ++k) {
+                int f2 = o->cfaces[c].a[rem[k]];
+                V3 other = o->fn[f2];
+                if (o->fparent[f2] != c) other = vmul(other, -1.);
+                double prod = vdot(my, other);
+                if (prod < min_product) { min_product = prod; face_pairs[2 * counter] = rem[k]; }
+            }
+            int nr = 0;
+            for (int k = 0; k < nrem; ++k) if (rem[k] != face_pairs[2 * counter] && rem[k] != face_pairs[2 * counter + 1]) rem[nr++] = rem[k];
+            nrem = nr;
+            counter++;
+        }
+        for (int i = 0; i < 3; ++i) {                                    /* :81-91 */
+            int cp = other_cell(o, o->cfaces[c].a[face_pairs[2 * i + 1]], c);
+            cell_pairs[2 * i + 1] = cp;
+            Jac[i] = o->cc[cp];
+            int cm = other_cell(o, o->cfaces[c].a[face_pairs[2 * i]], c);
+            cell_pairs[2 * i] = cm;
+            Jac[i] = vsub(Jac[i], o->cc[cm]);
+        }
+    } else {                                                             /* :93-155 */
+        int st[8], ns = nf;
+        for (int i = 0; i < nf; ++i) st[i] = o->cfaces[c].a[i];
+        qsort(st, ns, sizeof(int), cmp_int);
+        double max_det = 0.;
+        int cpt[6];
+        V3 Jt[3];
+        for (int i1 = 0; i1 < ns; ++i1)
+            for (int i2 = i1; i2 < ns; ++i2) {
+                cpt[1] = other_cell(o, st[i2], c); Jt[0] = o->cc[cpt[1]];
+                cpt[0] = other_cell(o, st[i1], c); Jt[0] = vsub(Jt[0], o->cc[cpt[0]]);
+                for (int i3 = i1; i3 < ns; ++i3)
+                    for (int i4 = i3; i4 < ns; ++i4) {
+                        cpt[3] = other_cell(o, st[i4], c); Jt[1] = o->cc[cpt[3]];
+                        cpt[2] = other_cell(o, st[i3], c); Jt[1] = vsub(Jt[1], o->cc[cpt[2]]);
+                        for (int i5 = i2; i5 < ns; ++i5)
+                            for (int i6 = i5; i6 < ns; ++i6) {
+                                cpt[5] = other_cell(o, st[i6], c); Jt[2] = o->cc[cpt[5]];
+                                cpt[4] = other_cell(o, st[i5], c); Jt[2] = vsub(Jt[2], o->cc[cpt[4]]);
+                                double det = vdot(Jt[0], vcross(Jt[1], Jt[2]));
+                                if (det > max_det) {
+                                    max_det = det;
+                                    for (int q = 0; q < 6; ++q) cell_pairs[q] = cpt[q];
+                                    Jac[0] = Jt[0]; Jac[1] = Jt[1]; Jac[2] = Jt[2];
+                                }
+                            }
+                    }
+            }
+    }
+    double det = vdot(Jac[0], vcross(Jac[1], Jac[2]));                  /* :160-172 */
+    V3 JiT[3];
+    JiT[0].c[0] = (Jac[1].c[1] * Jac[2].c[2] - Jac[1].c[2] * Jac[2].c[1]) / det;
+    JiT[1].c[0] = (Jac[0].c[2] * Jac[2].c[1] - Jac[0].c[1] * Jac[2].c[2]) / det;
+    JiT[2].c[0] = (Jac[0].c[1] * Jac[1].c[2] - Jac[0].c[2] * Jac[1].c[1]) / det;
+    JiT[0].c[1] = (Jac[1].c[2] * Jac[2].c[0] - Jac[1].c[0] * Jac[2].c[2]) / det;
+    JiT[1].c[1] = (Jac[0].c[0] * Jac[2].c[2] - Jac[0].c[2] * Jac[2].c[0]) / det;
+    JiT[2].c[1] = (Jac[0].c[2] * Jac[1].c[0] - Jac[0].c[0] * Jac[1].c[2]) / det;
+    JiT[0].c[2] = (Jac[1].c[0] * Jac[2].c[1] - Jac[1].c[1] * Jac[2].c[0]) / det;
+    JiT[1].c[2] = (Jac[0].c[1] * Jac[2].c[0] - Jac[0].c[0] * Jac[2].c[1]) / det;
+    JiT[2].c[2] = (Jac[0].c[0] * Jac[1].c[1] - Jac[0].c[1] * Jac[1].c[0]) / det;
+    gm->n = 0;                                                           /* :203-220 */
+    vm_insert(gm, cell_pairs[0], vmul(JiT[0], -1.));
+    for (int q = 1; q < 6; ++q) {
+        int dir = q / 2;
+        int plus = q & 1;
+        int s = vm_find(gm, cell_pairs[q]);
+        if (s < 0) vm_insert(gm, cell_pairs[q], plus ? JiT[dir] : vsc(-1., JiT[dir]));
+        else gm->val[s] = plus ? vadd(gm->val[s], JiT[dir]) : vsub(gm->val[s], JiT[dir]);
+    }
+}
+
+static int gradient_maps(orc_ctx* o) {
+    o->g_ptr = (int*)malloc(sizeof(int) * (o->ncell + 1));
+    int cap = o->ncell * 8 + 64, n = 0;
+    o->g_idx = (int*)malloc(sizeof(int) * cap);
+    o->g_w = (V3*)malloc(sizeof(V3) * cap);
+    vmap* gm = (vmap*)malloc(sizeof(vmap));
+    for (int c = 0; c < o->ncell; ++c) {
+        o->g_ptr[c] = n;
+        int nn = o->cnodes[c].n;
+        int meth = (nn == 8) ? o->opt.grad_hex : (nn == 6) ? o->opt.grad_prism : o->opt.grad_other;
+        gm->n = 0;
+        if (meth == FCFD_GRAD_CURVILINEAR) curvilinear_grad_map(o, c, gm);
+        else if (meth == FCFD_GRAD_LSQR) lsqr_grad_map(o, c, gm);
+        if (n + gm->n > cap) {
+            cap = 2 * cap + gm->n;
+            o->g_idx = (int*)realloc(o->g_idx, sizeof(int) * cap);
+            o->g_w = (V3*)realloc(o->g_w, sizeof(V3) * cap);
+        }
+        for (int k = 0; k < gm->n; ++k) { o->g_idx[n] = gm->key[k]; o->g_w[n] = gm->val[k]; n++; }
+    }
+    o->g_ptr[o->ncell] = n;
+    free(gm);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* public set-up                                                      */
+/* ------------------------------------------------------------------ */
+orc_ctx* orc_create(const fcfd_mesh* mesh, const fcfd_options* opt) {
+    if (!mesh || !opt || mesh->n_cells <= 0 || mesh->n_bc > FCFD_MAX_BC || opt->n_bc != mesh->n_bc) {
+        seterr(NULL, FCFD_EINVAL, "invalid mesh/options");
+        return NULL;
+    }
+    orc_ctx* o = (orc_ctx*)calloc(1, sizeof(orc_ctx));
+    o->opt = *opt;
+    o->dim = mesh->dimension;
+    o->R = 8314.47215 / opt->Mw;
+    o->Cp = opt->gamma * o->R / (opt->gamma - 1.);
+    int rc = create_nodes_cells(o, mesh);
+    if (!rc) rc = create_faces(o);
+    if (rc) { snprintf(g_err, sizeof g_err, "%s", o->msg); orc_destroy(o); return NULL; }
+    areas_volumes(o);
+    create_boundary_ghosts(o);
+    set_bcs_grid(o);
+    set_length_scales(o);
+    rc = face_interpolation_weights(o);
+    if (!rc) rc = gradient_maps(o);
+    if (rc) { snprintf(g_err, sizeof g_err, "%s", o->msg); orc_destroy(o); return NULL; }
+    int nt = o->ntot;
+    o->p = (double*)calloc(nt, sizeof(double)); o->u = (double*)calloc(nt, sizeof(double));
+    o->v = (double*)calloc(nt, sizeof(double)); o->w = (double*)calloc(nt, sizeof(double));
+    o->T = (double*)calloc(nt, sizeof(double));
+    o->gp = (V3*)calloc(nt, sizeof(V3)); o->gu = (V3*)calloc(nt, sizeof(V3)); o->gv = (V3*)calloc(nt, sizeof(V3));
+    o->gw = (V3*)calloc(nt, sizeof(V3)); o->gT = (V3*)calloc(nt, sizeof(V3));
+    o->phi = (double*)malloc(sizeof(double) * nt * 5);
+    for (int i = 0; i < nt * 5; ++i) o->phi[i] = (opt->order == FCFD_ORDER_FIRST) ? 0. : 1.;   /* ns.cc:282-287 */
+    o->upd = (double*)calloc((size_t)o->ncell * 5, sizeof(double));
+    o->Rs = (double*)calloc((size_t)o->ncell * 5, sizeof(double));
+    o->Rabs = (double*)calloc((size_t)o->ncell * 5, sizeof(double));
+    o->loads = (double*)calloc((size_t)(o->nbc + 1) * 6, sizeof(double));
+    return o;
+}
+
+void orc_destroy(orc_ctx* o) {
+    if (!o) return;
+    for (int c = 0; c < o->ncap && o->cnodes; ++c) { iv_free(&o->cnodes[c]); iv_free(&o->cfaces[c]); if (o->cnbr) iv_free(&o->cnbr[c]); }
+    for (int n = 0; n < o->nnode && o->node_cells; ++n) iv_free(&o->node_cells[n]);
+    for (int f = 0; f < o->nface && o->fnodes; ++f) iv_free(&o->fnodes[f]);
+    for (int b = 0; b < o->nbc && o->bcflag; ++b) free(o->bcflag[b]);
+    free(o->bcflag); free(o->cnodes); free(o->cfaces); free(o->cnbr); free(o->node_cells); free(o->node);
+    free(o->cc); free(o->vol); free(o->lscale); free(o->wdist); free(o->cbc);
+    free(o->fparent); free(o->fneigh); free(o->fbc); free(o->fsym); free(o->face2bc); free(o->fnodes);
+    free(o->fc); free(o->fn); free(o->farea); free(o->fwd); free(o->fdiss);
+    free(o->bc_area); free(o->bc_total_area);
+    free(o->avg_ptr); free(o->avg_idx); free(o->avg_w); free(o->g_ptr); free(o->g_idx); free(o->g_w);
+    free(o->p); free(o->u); free(o->v); free(o->w); free(o->T);
+    free(o->gp); free(o->gu); free(o->gv); free(o->gw); free(o->gT);
+    free(o->phi); free(o->upd); free(o->Rs); free(o->Rabs); free(o->JL); free(o->JR); free(o->Dg); free(o->loads);
+    free(o);
+}
+
+int orc_info(const orc_ctx* o, fcfd_info* info) {
+    memset(info, 0, sizeof *info);
+    info->n_nodes = o->nnode; info->n_cells = o->ncell; info->n_ghosts = o->ntot - o->ncell;
+    info->n_faces = o->nface; info->n_bfaces = o->ntot - o->ncell;
+    info->ld_cell = o->ntot; info->ld_face = o->nface;
+    info->n_gradmap = o->g_ptr[o->ncell]; info->n_faceavg = o->avg_ptr[o->nface];
+    int nn = 0; for (int c = 0; c < o->ntot; ++c) nn += o->cnbr[c].n;
+    info->n_nbr = nn;
+    int hf = 0; for (int c = 0; c < o->ncell; ++c) hf += o->cfaces[c].n;
+    info->n_halffaces = hf;
+    info->nv = 5;
+    return 0;
+}
+
+/* face tangent1 (ns_assemble_linear_system.cc:337-341) */
+static V3 face_t1(const orc_ctx* o, int f) {
+    const ivec* fn = &o->fnodes[f];
+    if (fn->n == 4)
+        return vnorm(vsub(vadd(o->node[fn->a[0]], o->node[fn->a[1]]), vadd(o->node[fn->a[2]], o->node[fn->a[3]])));
+    return vnorm(vsub(vsc(0.5, vadd(o->node[fn->a[0]], o->node[fn->a[1]])), o->fc[f]));
+}
+
+int orc_metrics(const orc_ctx* o, fcfd_metrics_out* m) {
+    int F = o->nface, N = o->ncell, T = o->ntot;
+    for (int f = 0; f < F; ++f) {
+        if (m->face_parent) m->face_parent[f] = o->fparent[f];
+        if (m->face_neighbor) m->face_neighbor[f] = o->fneigh[f];
+        if (m->face_bc) m->face_bc[f] = o->fbc[f];
+        for (int k = 0; k < 3; ++k) {
+            if (m->face_centroid) m->face_centroid[3 * f + k] = o->fc[f].c[k];
+            if (m->face_normal) m->face_normal[3 * f + k] = o->fn[f].c[k];
+        }
+        if (m->face_tangent1) { V3 t = face_t1(o, f); for (int k = 0; k < 3; ++k) m->face_tangent1[3 * f + k] = t.c[k]; }
+        if (m->face_area) m->face_area[f] = o->farea[f];
+    }
+    if (m->face_node_ptr) {
+        int n = 0;
+        for (int f = 0; f < F; ++f) { m->face_node_ptr[f] = n; for (int k = 0; k < o->fnodes[f].n; ++k) { if (m->face_nodes) m->face_nodes[n] = o->fnodes[f].a[k]; n++; } }
+        m->face_node_ptr[F] = n;
+    }
+    for (int c = 0; c < T; ++c) {
+        for (int k = 0; k < 3; ++k) if (m->cell_centroid) m->cell_centroid[3 * c + k] = o->cc[c].c[k];
+        if (m->cell_volume) m->cell_volume[c] = o->vol[c];
+        if (m->wall_distance) m->wall_distance[c] = o->wdist[c];
+        if (m->cell_perm) m->cell_perm[c] = c;
+    }
+    for (int c = 0; c < N; ++c) if (m->cell_lengthscale) m->cell_lengthscale[c] = o->lscale[c];
+    if (m->cell_face_ptr) {
+        int n = 0;
+        for (int c = 0; c < N; ++c) { m->cell_face_ptr[c] = n; for (int k = 0; k < o->cfaces[c].n; ++k) { if (m->cell_faces) m->cell_faces[n] = o->cfaces[c].a[k]; n++; } }
+        m->cell_face_ptr[N] = n;
+    }
+    if (m->nbr_ptr) {
+        int n = 0;
+        for (int c = 0; c < T; ++c) { m->nbr_ptr[c] = n; for (int k = 0; k < o->cnbr[c].n; ++k) { if (m->nbr) m->nbr[n] = o->cnbr[c].a[k]; n++; } }
+        m->nbr_ptr[T] = n;
+    }
+    if (m->grad_ptr) {
+        for (int c = 0; c <= N; ++c) m->grad_ptr[c] = o->g_ptr[c];
+        for (int k = 0; k < o->g_ptr[N]; ++k) {
+            if (m->grad_idx) m->grad_idx[k] = o->g_idx[k];
+            if (m->grad_w) for (int q = 0; q < 3; ++q) m->grad_w[3 * k + q] = o->g_w[k].c[q];
+        }
+    }
+    if (m->avg_ptr) {
+        for (int f = 0; f <= F; ++f) m->avg_ptr[f] = o->avg_ptr[f];
+        for (int k = 0; k < o->avg_ptr[F]; ++k) { if (m->avg_idx) m->avg_idx[k] = o->avg_idx[k]; if (m->avg_w) m->avg_w[k] = o->avg_w[k]; }
+    }
+    if (m->grid_lengthscale) m->grid_lengthscale[0] = o->glscale;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* state                                                              */
+/* ------------------------------------------------------------------ */
+int orc_set_state(orc_ctx* o, const double* W) {
+    for (int c = 0; c < o->ncell; ++c) {
+        o->p[c] = W[5 * c]; o->u[c] = W[5 * c + 1]; o->v[c] = W[5 * c + 2]; o->w[c] = W[5 * c + 3]; o->T[c] = W[5 * c + 4];
+    }
+    return 0;
+}
+int orc_set_update(orc_ctx* o, const double* dW) { memcpy(o->upd, dW, sizeof(double) * 5 * (size_t)o->ncell); return 0; }
+
+static V3 cellV(const orc_ctx* o, int c) { return v3(o->u[c], o->v[c], o->w[c]); }
+
+/* ------------------------------------------------------------------ */
+/* boundary conditions (src/navier_stokes/ns_apply_bcs.cc:25-258)      */
+/* ------------------------------------------------------------------ */
+static int apply_bcs(orc_ctx* o, cstate* L, cstate* R, const fstate* F) {
+    const fcfd_bc* bc = &o->opt.bc[F->bc];
+    double g = o->opt.gamma, Pref = o->opt.Pref, Tref = o->opt.Tref;
+    if (bc->type == FCFD_BC_INLET) {
+        if (bc->kind == FCFD_KIND_VELOCITY) {                          /* :45-88 */
+            R->V = v3(bc->V[0], bc->V[1], bc->V[2]);
+            double uNL = vdot(L->V, F->normal);
+            double uNR = vdot(R->V, F->normal);
+            R->a = L->a + 0.5 * (uNL - uNR) * (g - 1.);
+            double MachL = -uNL / L->a;
+            if (MachL >= 1. && bc->specified != FCFD_SPEC_STATE) { seterr(o, FCFD_EBC, "inlet became supersonic, thermodynamic state needed"); return FCFD_EBC; }
+            if (bc->specified == FCFD_SPEC_STATE) { R->p = bc->p; R->T = bc->T; R->rho = bc->rho; }
+            else if (bc->specified == FCFD_SPEC_P) { R->p = bc->p; R->rho = g * (R->p + Pref) / (R->a * R->a); R->T = m_T(o, R->p, R->rho); }
+            else if (bc->specified == FCFD_SPEC_T) { R->T = bc->T; R->rho = L->rho * pow((L->T + Tref) / (R->T + Tref), 1. / (1. - g)); R->p = m_p(o, R->rho, R->T); }
+            else if (bc->specified == FCFD_SPEC_RHO) { R->rho = bc->rho; R->p = R->a * R->a * R->rho / g - Pref; R->T = m_T(o, R->p, R->rho); }
+            else { seterr(o, FCFD_EBC, "inlet needs one of p,T,rho"); return FCFD_EBC; }
+            R->p_center = R->p; R->V_center = R->V; R->T_center = R->T;
+        } else if (bc->kind == FCFD_KIND_MDOT) {                       /* :90-135 */
+            double uNL = vdot(L->V, F->normal);
+            double mdotNR = -1. * bc->mdot / o->bc_total_area[F->bc];
+            if (bc->specified == FCFD_SPEC_STATE) { R->p = bc->p; R->T = bc->T; R->rho = bc->rho; }
+            else if (bc->specified == FCFD_SPEC_P) { R->p = bc->p; R->T = L->T; R->rho = m_rho(o, R->p, R->T); }
+            else if (bc->specified == FCFD_SPEC_T) { R->T = bc->T; R->p = L->p; R->rho = m_rho(o, R->p, R->T); }
+            else if (bc->specified == FCFD_SPEC_RHO) {
+                R->rho = bc->rho;
+                R->a = 0.5 * (g - 1.) * (uNL + 2. * L->a / (g - 1.) - mdotNR / R->rho);
+                R->p = R->a * R->a * R->rho / g - Pref;
+                R->T = m_T(o, R->p, R->rho);
+            } else { seterr(o, FCFD_EBC, "inlet needs one of p,T,rho"); return FCFD_EBC; }
+            R->V = vsc(mdotNR / R->rho, F->normal);
+            R->p_center = R->p; R->V_center = R->V; R->T_center = R->T;
+        } else if (bc->kind == FCFD_KIND_STAGNATION) {                 /* :137-155 */
+            if (vdot(L->V, F->normal) > 0.) R->V = vsc(-1., L->V_center);
+            else R->V = L->V_center;
+            R->T = bc->T_total - 0.5 * (g - 1.) / (g * o->R) * vdot(R->V, R->V);
+            R->p = (bc->p_total + Pref) / (1. + 0.5 * vdot(R->V, R->V) / (o->R * (R->T + Tref))) - Pref;
+            R->rho = m_rho(o, R->p, R->T);
+            R->p_center = R->p; R->V_center = R->V; R->T_center = R->T;
+        }
+    } else if (bc->type == FCFD_BC_OUTLET) {                           /* :157-215 */
+        double uNL = vdot(L->V, F->normal);
+        double MachL = uNL / L->a;
+        int supersonic = (MachL > 1.);
+        if (bc->specified == FCFD_SPEC_STATE) { seterr(o, FCFD_EBC, "complete thermodynamic state at an outlet"); return FCFD_EBC; }
+        else if (bc->specified == FCFD_SPEC_P && !supersonic) {
+            R->p = R->p_center = bc->p;
+            R->rho = L->rho * pow((R->p + Pref), 1. / g) / pow((L->p + Pref), 1. / g);
+            R->a = sqrt(g * (R->p + Pref) / R->rho);
+            R->T = m_T(o, R->p, R->rho);
+            R->V = L->V;
+        } else if (bc->specified == FCFD_SPEC_T && !supersonic) {
+            R->T = R->T_center = bc->T;
+            R->a = sqrt(g * o->R * (R->T + Tref));
+            R->rho = pow(g * (L->p + Pref) / (R->a * R->a * pow(L->rho, g)), 1. - g);
+            R->p = m_p(o, R->rho, R->T);
+            R->V = L->V;
+        } else if (bc->specified == FCFD_SPEC_RHO && !supersonic) {
+            R->rho = bc->rho;
+            R->p = (L->p + Pref) * pow(R->rho / L->rho, g) - Pref;
+            R->T = m_T(o, R->p, R->rho);
+            R->a = sqrt(g * (R->p + Pref) / R->rho);
+            R->V = L->V;
+        } else {
+            R->p = L->p; R->T = L->T; R->V = L->V;
+            R->rho = m_rho(o, R->p, R->T);      /* D2 */
+        }
+        if (MachL < 0.) {
+            if (bc->kind == FCFD_KIND_DAMP_REVERSE) {
+                R->p -= 0.5 * L->rho * uNL * uNL;
+                R->rho = L->rho * pow((R->p + Pref), 1. / g) / pow((L->p + Pref), 1. / g);
+                R->a = sqrt(g * (R->p + Pref) / R->rho);
+                R->T = m_T(o, R->p, R->rho);
+            }
+            if (bc->kind == FCFD_KIND_NO_REVERSE) R->V = vsc(-1., L->V);
+        }
+        if (MachL < 1. && bc->kind == FCFD_KIND_FORCE_SUPERSONIC) {
+            R->p -= 0.5 * L->rho * pow(L->a - uNL, 2.);
+            R->p = dmin(1.e-8, R->p + Pref) - Pref;
+            R->rho = L->rho * pow((R->p + Pref), 1. / g) / pow((L->p + Pref), 1. / g);
+            R->a = sqrt(g * (R->p + Pref) / R->rho);
+            R->T = m_T(o, R->p, R->rho);
+        }
+    } else if (bc->type == FCFD_BC_WALL) {                             /* :217-244 */
+        if (bc->thermal == FCFD_THERMAL_FIXED_T) {
+            R->T = R->T_center = bc->T;
+            R->p = L->p;
+            R->rho = m_rho(o, R->p, R->T);
+        } else {
+            R->p = L->p; R->rho = L->rho; R->T = L->T; R->T_center = L->T_center;
+        }
+        if (bc->kind == FCFD_KIND_SLIP) {
+            R->V = vsub(L->V, vsc(2. * vdot(L->V, F->normal), F->normal));
+            R->V_center = vsub(L->V_center, vsc(2. * vdot(L->V_center, F->normal), F->normal));
+        } else {
+            R->V = vsc(-1., L->V);
+            R->V_center = vsc(-1., L->V_center);
+        }
+        R->p_center = L->p_center;
+    } else if (bc->type == FCFD_BC_SYMMETRY) {                         /* :246-258 */
+        R->p = L->p;
+        R->V = vsub(L->V, vsc(2. * vdot(L->V, F->normal), F->normal));
+        R->T = L->T;
+        R->rho = L->rho;
+        R->p_center = L->p_center;
+        R->T_center = L->T_center;
+        R->V_center = vsub(L->V_center, vsc(2. * vdot(L->V_center, F->normal), F->normal));
+    }
+    return 0;
+}
+
+/* face_geom_update (ns_assemble_linear_system.cc:332-351) */
+static void face_geom_update(const orc_ctx* o, fstate* F, int f) {
+    F->index = f;
+    F->parent = o->fparent[f];
+    F->neighbor = o->fneigh[f];
+    F->normal = o->fn[f];
+    F->tangent1 = face_t1(o, f);
+    F->tangent2 = vnorm(vcross(F->normal, F->tangent1));
+    F->area = o->farea[f];
+    F->bc = o->fbc[f];
+    F->left2right = vsub(o->cc[F->neighbor], o->cc[F->parent]);
+}
+
+static void finish_state(const orc_ctx* o, cstate* S, const fstate* F) {   /* :286-290, :323-327 */
+    S->a = m_a(o, S->p, S->T);
+    S->H = S->a * S->a / (o->opt.gamma - 1.) + 0.5 * vdot(S->V, S->V);
+    S->Vn = v3(vdot(S->V, F->normal), vdot(S->V, F->tangent1), vdot(S->V, F->tangent2));
+}
+
+/* left_state_update / interior part of right_state_update (:269-294, :305-320) */
+static void recon_state(const orc_ctx* o, cstate* S, const fstate* F, int c) {
+    V3 c2f = vsub(vsc(1., o->fc[F->index]), o->cc[c]);    /* order_factor == 1 in the face loop */
+    const double* ph = &o->phi[5 * c];
+    S->p_center = o->p[c];
+    S->p = S->p_center + ph[0] * vdot(c2f, o->gp[c]);
+    V3 dV = v3(ph[1] * vdot(c2f, o->gu[c]), ph[2] * vdot(c2f, o->gv[c]), ph[3] * vdot(c2f, o->gw[c]));
+    S->V_center = cellV(o, c);
+    S->V = vadd(S->V_center, dV);
+    S->T_center = o->T[c];
+    S->T = S->T_center + ph[4] * vdot(c2f, o->gT[c]);
+    S->rho = m_rho(o, S->p, S->T);
+    for (int i = 0; i < 5; ++i) S->update[i] = (c < o->ncell) ? o->upd[5 * c + i] : 0.;
+    S->volume = o->vol[c];
+}
+
+static int right_state_update(orc_ctx* o, cstate* L, cstate* R, const fstate* F) {   /* :296-330 */
+    if (F->bc >= 0) {
+        for (int i = 0; i < 5; ++i) R->update[i] = 0.;
+        int g = F->neighbor;
+        R->p_center = o->p[g]; R->T_center = o->T[g]; R->V_center = cellV(o, g);
+        int rc = apply_bcs(o, L, R, F);
+        if (rc) return rc;
+    } else {
+        recon_state(o, R, F, F->neighbor);
+    }
+    finish_state(o, R, F);
+    return 0;
+}
+
+/* face_state_update (:353-415) */
+static void face_state_update(const orc_ctx* o, const cstate* L, const cstate* R, fstate* F) {
+    if (F->bc >= 0) {
+        F->gradu = o->gu[F->parent]; F->gradv = o->gv[F->parent]; F->gradw = o->gw[F->parent]; F->gradT = o->gT[F->parent];
+    } else {   /* Variable<Vec3D>::face_calculate (variable.h:134-147) */
+        V3 su = vzero(), sv = vzero(), sw = vzero(), sT = vzero();
+        for (int k = o->avg_ptr[F->index]; k < o->avg_ptr[F->index + 1]; ++k) {
+            int c = o->avg_idx[k];
+            double wt = o->avg_w[k];
+            su = vadd(su, vsc(wt, o->gu[c])); sv = vadd(sv, vsc(wt, o->gv[c]));
+            sw = vadd(sw, vsc(wt, o->gw[c])); sT = vadd(sT, vsc(wt, o->gT[c]));
+        }
+        F->gradu = su; F->gradv = sv; F->gradw = sw; F->gradT = sT;
+    }
+    V3 l2rn = vnorm(F->left2right);
+    double l2rmag = vmag(F->left2right);
+    F->gradu = vsub(F->gradu, vsc(vdot(F->gradu, l2rn), l2rn));
+    F->gradu = vadd(F->gradu, vsc((R->V_center.c[0] - L->V_center.c[0]) / l2rmag, l2rn));
+    F->gradv = vsub(F->gradv, vsc(vdot(F->gradv, l2rn), l2rn));
+    F->gradv = vadd(F->gradv, vsc((R->V_center.c[1] - L->V_center.c[1]) / l2rmag, l2rn));
+    F->gradw = vsub(F->gradw, vsc(vdot(F->gradw, l2rn), l2rn));
+    F->gradw = vadd(F->gradw, vsc((R->V_center.c[2] - L->V_center.c[2]) / l2rmag, l2rn));
+    F->gradT = vsub(F->gradT, vsc(vdot(F->gradT, l2rn), l2rn));
+    F->gradT = vadd(F->gradT, vsc((R->T_center - L->T_center) / l2rmag, l2rn));
+    F->V = vsc(0.5, vadd(L->V, R->V));
+    F->T = 0.5 * (L->T + R->T);
+    F->mu = m_visc(o, F->T);
+    F->lambda = m_cond(o, F->T);
+}
+
+/* face_state_adjust (:460-500), including the case-1 fall-through */
+static void face_state_adjust(const orc_ctx* o, const cstate* L, const cstate* R, fstate* F, int var) {
+    V3 l2rn = vnorm(F->left2right);
+    double l2rmag = vmag(F->left2right);
+    switch (var) {
+    case 0: break;
+    case 1:
+        F->V.c[0] = 0.5 * (L->V.c[0] + R->V.c[0]);
+        F->gradu = vsub(F->gradu, vsc(vdot(F->gradu, l2rn), l2rn));
+        F->gradu = vadd(F->gradu, vsc((R->V_center.c[0] - L->V_center.c[0]) / l2rmag, l2rn));
+        /* fall through */
+    case 2:
+        F->V.c[1] = 0.5 * (L->V.c[1] + R->V.c[1]);
+        F->gradv = vsub(F->gradv, vsc(vdot(F->gradv, l2rn), l2rn));
+        F->gradv = vadd(F->gradv, vsc((R->V_center.c[1] - L->V_center.c[1]) / l2rmag, l2rn));
+        break;
+    case 3:
+        F->V.c[2] = 0.5 * (L->V.c[2] + R->V.c[2]);
+        F->gradw = vsub(F->gradw, vsc(vdot(F->gradw, l2rn), l2rn));
+        F->gradw = vadd(F->gradw, vsc((R->V_center.c[2] - L->V_center.c[2]) / l2rmag, l2rn));
+        break;
+    case 4:
+        F->T = 0.5 * (L->T + R->T);
+        F->gradT = vsub(F->gradT, vsc(vdot(F->gradT, l2rn), l2rn));
+        F->gradT = vadd(F->gradT, vsc((R->T_center - L->T_center) / l2rmag, l2rn));
+        F->mu = m_visc(o, F->T);
+        F->lambda = m_cond(o, F->T);
+        break;
+    }
+}
+
+/* state_perturb (:417-458) */
+static void state_perturb(const orc_ctx* o, cstate* S, const fstate* F, int var, double eps) {
+    switch (var) {
+    case 0: S->p += eps; S->rho = m_rho(o, S->p, S->T); S->a = m_a(o, S->p, S->T); break;
+    case 1: case 2: case 3:
+        S->V.c[var - 1] += eps; S->V_center.c[var - 1] += eps;
+        S->Vn = v3(vdot(S->V, F->normal), vdot(S->V, F->tangent1), vdot(S->V, F->tangent2));
+        break;
+    case 4: S->T += eps; S->T_center += eps; S->rho = m_rho(o, S->p, S->T); S->a = m_a(o, S->p, S->T); break;
+    }
+    S->H = S->a * S->a / (o->opt.gamma - 1.) + 0.5 * vdot(S->V, S->V);
+}
+
+/* ------------------------------------------------------------------ */
+/* convective fluxes                                                  */
+/* ------------------------------------------------------------------ */
+static void roe_flux(const cstate* L, const cstate* R, double fn[5], double Gamma) {   /* ns_roe.cc:25-98 */
+    double rho = sqrt(R->rho / L->rho);
+    double u = (L->Vn.c[0] + rho * R->Vn.c[0]) / (1. + rho);
+    double v = (L->Vn.c[1] + rho * R->Vn.c[1]) / (1. + rho);
+    double w = (L->Vn.c[2] + rho * R->Vn.c[2]) / (1. + rho);
+    double H = (L->H + rho * R->H) / (1. + rho);
+    double a = sqrt((Gamma - 1.) * (H - 0.5 * (u * u + v * v + w * w)));
+    rho *= L->rho;
+    double Du = R->Vn.c[0] - L->Vn.c[0];
+    double Dp = R->p - L->p;
+    double deltaV, lambda, Dlambda, mdot, product;
+    if (u >= 0.) {
+        deltaV = 0.5 * (Dp - rho * a * Du) / (a * a);
+        lambda = u - a;
+        Dlambda = 2. * (dmin(a, dmax(0., 2. * (L->a - R->a + Du))));
+        if (lambda <= (-0.5 * Dlambda)) { ; }
+        else if (lambda < (0.5 * Dlambda)) lambda = -0.5 * (lambda - 0.5 * Dlambda) * (lambda - 0.5 * Dlambda) / Dlambda;
+        else lambda = 0.;
+        mdot = L->rho * L->Vn.c[0];
+        product = lambda * deltaV;
+        fn[0] = mdot + product;
+        fn[1] = mdot * L->Vn.c[0] + L->p + product * (u - a);
+        fn[2] = mdot * L->Vn.c[1] + product * v;
+        fn[3] = mdot * L->Vn.c[2] + product * w;
+        fn[4] = mdot * L->H + product * (H - u * a);
+    } else {
+        deltaV = 0.5 * (Dp + rho * a * Du) / (a * a);
+        lambda = u + a;
+        Dlambda = 2. * (dmin(a, dmax(0., 2. * (R->a - L->a + Du))));
+        if (lambda >= (0.5 * Dlambda)) { ; }
+        else if (lambda > (-0.5 * Dlambda)) lambda = 0.5 * (lambda + 0.5 * Dlambda) * (lambda + 0.5 * Dlambda) / Dlambda;
+        else lambda = 0.;
+        mdot = R->rho * R->Vn.c[0];
+        product = lambda * deltaV;
+        fn[0] = mdot - product;
+        fn[1] = mdot * R->Vn.c[0] + R->p - product * (u + a);
+        fn[2] = mdot * R->Vn.c[1] - product * v;
+        fn[3] = mdot * R->Vn.c[2] - product * w;
+        fn[4] = mdot * R->H - product * (H + u * a);
+    }
+}
+
+/* AUSM+-up (ns_ausm_plus_up.cc:35-152) */
+static const double ausm_beta = 0.125;
+static double msp2(double M) { return 0.25 * (M + 1.) * (M + 1.); }
+static double msm2(double M) { return -0.25 * (M - 1.) * (M - 1.); }
+static double msp4(double M) { return (fabs(M) >= 1.) ? 0.5 * (M + fabs(M)) : msp2(M) * (1. - 16. * ausm_beta * msm2(M)); }
+static double msm4(double M) { return (fabs(M) >= 1.) ? 0.5 * (M - fabs(M)) : msm2(M) * (1. + 16. * ausm_beta * msp2(M)); }
+static double psp5(double M, double alpha) { return (fabs(M) >= 1.) ? 0.5 * (M + fabs(M)) / M : msp2(M) * ((2. - M) - 16. * alpha * M * msm2(M)); }
+static double psm5(double M, double alpha) { return (fabs(M) >= 1.) ? 0.5 * (M - fabs(M)) / M : msm2(M) * ((-2. - M) + 16. * alpha * M * msp2(M)); }
+
+static void ausm_flux(const cstate* L, const cstate* R, double fn[5], double Pref, double Minf, double* alpha_io) {
+    double Kp = 0.25, Ku = 0.75, sigma = 1.;
+    double aL = L->a, aR = R->a;
+    double aLh = aL * aL / dmax(aL, L->Vn.c[0]);
+    double aRh = aR * aR / dmax(aR, -1. * R->Vn.c[0]);
+    double a = dmin(aLh, aRh);
+    double rho = 0.5 * (L->rho + R->rho);
+    double ML = L->Vn.c[0] / a, MR = R->Vn.c[0] / a;
+    double Mbar2 = 0.5 * (ML * ML + MR * MR);
+    double Mref = dmax(Minf, sqrt(Mbar2));
+    Mref = dmin(Mref, 1.);
+    double fa;
+    if (Mbar2 >= 1.) fa = 1.;
+    else { double Mo = sqrt(dmin(1., dmax(Mbar2, Mref * Mref))); fa = Mo * (2. - Mo); }
+    /* the global `alpha` (ns_ausm_plus_up.cc:26) still holds the previous
+     * call's value while msp4/msm4 run; p_split_5 uses the updated one */
+    double M = msp4(ML) + msm4(MR) - Kp / fa * dmax(1. - sigma * Mbar2, 0.) * (R->p - L->p) / (rho * a * a);
+    double mdot = (M > 0) ? a * M * L->rho : a * M * R->rho;
+    double alpha = 3. / 16. * (-4. + 5. * fa * fa);
+    *alpha_io = alpha;
+    double p = psp5(ML, alpha) * (L->p + Pref) + psm5(MR, alpha) * (R->p + Pref)
+             - Ku * psp5(ML, alpha) * psm5(MR, alpha) * (L->rho + R->rho) * fa * a * (R->Vn.c[0] - L->Vn.c[0]);
+    p -= Pref;
+    fn[0] = mdot;
+    if (mdot > 0.) {
+        fn[1] = mdot * L->Vn.c[0] + p; fn[2] = mdot * L->Vn.c[1]; fn[3] = mdot * L->Vn.c[2]; fn[4] = mdot * L->H;
+    } else {
+        fn[1] = mdot * R->Vn.c[0] + p; fn[2] = mdot * R->Vn.c[1]; fn[3] = mdot * R->Vn.c[2]; fn[4] = mdot * R->H;
+    }
+}
+
+static void vanleer_flux(const cstate* L, const cstate* R, double fn[5], double Gamma, double Pref) {   /* ns_vanleer.cc:25-66 */
+    double ML = L->Vn.c[0] / L->a, MR = R->Vn.c[0] / R->a;
+    double M = 0.5 * (ML + MR);
+    if (M <= -1.) {
+        double mdot = R->rho * R->Vn.c[0];
+        fn[0] = mdot; fn[1] = mdot * R->Vn.c[0] + R->p; fn[2] = mdot * R->Vn.c[1]; fn[3] = mdot * R->Vn.c[2]; fn[4] = mdot * R->H;
+    } else if (M >= 1.) {
+        double mdot = L->rho * L->Vn.c[0];
+        fn[0] = mdot; fn[1] = mdot * L->Vn.c[0] + L->p; fn[2] = mdot * L->Vn.c[1]; fn[3] = mdot * L->Vn.c[2]; fn[4] = mdot * L->H;
+    } else {
+        double fplus = 0.25 * L->rho * L->a * (ML + 1.) * (ML + 1.);
+        double fminus = -0.25 * R->rho * R->a * (1. - MR) * (1. - MR);
+        fn[0] = fplus + fminus;
+        double termL = (2. * L->a / Gamma) * (0.5 * (Gamma - 1) * ML + 1.);
+        double termR = (2. * R->a / Gamma) * (0.5 * (Gamma - 1) * MR - 1.);
+        fn[1] = fplus * termL + fminus * termR - Pref;
+        fn[2] = fplus * L->Vn.c[1] + fminus * R->Vn.c[1];
+        fn[3] = fplus * L->Vn.c[2] + fminus * R->Vn.c[2];
+        termL = 0.5 * Gamma * Gamma / (Gamma * Gamma - 1.) * termL * termL + 0.5 * (L->Vn.c[1] * L->Vn.c[1] + L->Vn.c[2] * L->Vn.c[2]);
+        termR = 0.5 * Gamma * Gamma / (Gamma * Gamma - 1.) * termR * termR + 0.5 * (R->Vn.c[1] * R->Vn.c[1] + R->Vn.c[2] * R->Vn.c[2]);
+        fn[4] = fplus * termL + fminus * termR;
+    }
+}
+
+static double sgn(double n) { return (n < 0) ? -1. : (n > 0) ? 1. : 0.; }
+static void slau_flux(const cstate* L, const cstate* R, double fn[5], double Pref) {   /* ns_sd_slau.cc:37-94 */
+    const double Csd1 = 0.1, Csd2 = 10.;
+    double a = 0.5 * (L->a + R->a);
+    double Mhat = dmin(1., sqrt(0.5 * (vdot(L->V, L->V) + vdot(R->V, R->V)) / a));
+    double chi = (1. - Mhat) * (1. - Mhat);
+    double Mp = L->Vn.c[0] / a, Mm = R->Vn.c[0] / a;
+    double alpha = 0.;
+    double Bp, Bm;
+    if (fabs(Mp) < 1) Bp = 0.25 * (2. - Mp) * (Mp + 1.) * (Mp + 1.) + alpha * Mp * (Mp - 1.) * (Mp - 1.);
+    else Bp = 0.5 * (1. + sgn(Mp));
+    if (fabs(Mm) < 1) Bm = 0.25 * (2. + Mm) * (Mm - 1.) * (Mm - 1.) - alpha * Mm * (Mm - 1.) * (Mm - 1.);
+    else Bm = 0.5 * (1. + sgn(-Mm));
+    double ave_p = 0.5 * (L->p + R->p) + Pref;
+    double delta_p = R->p - L->p;
+    double p = ave_p - 0.5 * (Bp - Bm) * delta_p + (1. - chi) * (Bp + Bm - 1.) * ave_p;
+    double max_delta_p = fabs(delta_p);
+    double gg = -dmax(dmin(Mp, 0.), -1.) * dmin(dmax(Mm, 0.), 1.);
+    double Vn = (L->rho * fabs(L->Vn.c[0]) + R->rho * fabs(R->Vn.c[0])) / (L->rho + R->rho);
+    double Vnp = (1. - gg) * Vn + gg * fabs(L->Vn.c[0]);
+    double Vnm = (1. - gg) * Vn + gg * fabs(R->Vn.c[0]);
+    double theta = (Csd2 * fabs(delta_p) / ave_p + Csd1) / (max_delta_p / ave_p + Csd1);
+    theta = dmin(1., theta * theta);
+    double mdot = 0.5 * (L->rho * (L->Vn.c[0] + Vnp) + R->rho * (R->Vn.c[0] - Vnm) - theta * dmax(0., (1. - Vn / a)) * delta_p / a);
+    p -= Pref;
+    fn[0] = mdot;
+    if (mdot > 0.) { fn[1] = mdot * L->Vn.c[0] + p; fn[2] = mdot * L->Vn.c[1]; fn[3] = mdot * L->Vn.c[2]; fn[4] = mdot * L->H; }
+    else { fn[1] = mdot * R->Vn.c[0] + p; fn[2] = mdot * R->Vn.c[1]; fn[3] = mdot * R->Vn.c[2]; fn[4] = mdot * R->H; }
+}
+
+/* modified Steger-Warming (ns_stegger_warming.cc:23-145) */
+static void sw_flux(const orc_ctx* o, const cstate* L, const cstate* R, double diss, double cwd, double fn[5]) {
+    double alpha = 6.;
+    double RW = 8314.47215 / o->opt.Mw;
+    double gm = o->opt.gamma;
+    double wdiss = o->opt.wdiss, blh = o->opt.bl_height;
+    double Lm[5][5], Rm[5][5], Jac[5][5], Lam[5], Q[5];
+    for (int i = 0; i < 5; ++i) fn[i] = 0.;
+    double deltap = fabs(L->p - R->p) / dmin(L->p, R->p);
+    double w = 0.5 / (alpha * alpha * deltap * deltap + 1.);
+    for (int h = 0; h <= 1; h++) {
+        double signal = pow(-1., h);
+        double wL = (signal > 0.) ? 1. - w : w;
+        double wR = 1. - wL;
+        double p = wL * L->p + wR * R->p;
+        V3 V = vadd(vsc(wL, L->Vn), vsc(wR, R->Vn));
+        double V2 = vmag(V); V2 *= V2;
+        double T = wL * L->T + wR * R->T;
+        double rho = wL * L->rho + wR * R->rho;
+        double Cp = o->Cp;
+        double beta = gm * RW / Cp;
+        double a = sqrt((1. + beta) * p / rho);
+        double a2 = a * a;
+        double H = 0.5 * V2 + a2 / (gm - 1.);
+        double gs = RW * T + beta * (0.5 * V2 - Cp * T / gm);
+        double eta = (beta * V2 - gs) / (beta * a2);
+        for (int i = 0; i < 5; ++i) { Lam[i] = 0.; for (int j = 0; j < 5; ++j) { Lm[i][j] = 0.; Rm[i][j] = 0.; } Jac[i][i] = 0.; }
+        Lm[0][0] = 1.0 / a2;
+        Lm[0][3] = Lm[0][4] = 0.5 / a2;
+        Lm[1][0] = V.c[0] / a2; Lm[2][0] = V.c[1] / a2; Lm[3][0] = V.c[2] / a2;
+        Lm[1][3] = 0.5 * (V.c[0] + a) / a2;
+        Lm[1][4] = 0.5 * (V.c[0] - a) / a2;
+        Lm[2][1] = 1.;
+        Lm[2][3] = Lm[2][4] = 0.5 * V.c[1] / a2;
+        Lm[3][2] = 1.;
+        Lm[3][3] = Lm[3][4] = 0.5 * V.c[2] / a2;
+        Lm[4][0] = eta; Lm[4][1] = V.c[1]; Lm[4][2] = V.c[2];
+        Lm[4][3] = 0.5 * (H + a * V.c[0]) / a2;
+        Lm[4][4] = 0.5 * (H - a * V.c[0]) / a2;
+        double eps = (cwd > blh) ? wdiss * (a + vmag(V)) : wdiss * diss * (a + fabs(V.c[0]));
+        Lam[0] = Lam[1] = Lam[2] = 0.5 * (V.c[0] + signal * sqrt(V.c[0] * V.c[0] + eps * eps));
+        Lam[3] = 0.5 * (V.c[0] + a + signal * sqrt((V.c[0] + a) * (V.c[0] + a) + eps * eps));
+        Lam[4] = 0.5 * (V.c[0] - a + signal * sqrt((V.c[0] - a) * (V.c[0] - a) + eps * eps));
+        for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) Rm[i][j] += Lm[i][j] * Lam[j];   /* mtx_vec_mult into zeroed R */
+        for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) Lm[i][j] = Rm[i][j];
+        for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) Rm[i][j] = 0.;
+        Rm[0][0] = a2 - gs; Rm[0][1] = beta * V.c[0]; Rm[0][2] = beta * V.c[1]; Rm[0][3] = beta * V.c[2]; Rm[0][4] = -beta;
+        Rm[1][0] = -V.c[1]; Rm[1][2] = 1.;
+        Rm[2][0] = -V.c[2]; Rm[2][3] = 1.;
+        Rm[3][0] = gs - V.c[0] * a; Rm[3][1] = a - beta * V.c[0]; Rm[3][2] = -beta * V.c[1]; Rm[3][3] = -beta * V.c[2];
+        Rm[4][0] = gs + V.c[0] * a; Rm[4][1] = -a - beta * V.c[0]; Rm[4][2] = -beta * V.c[1]; Rm[4][3] = -beta * V.c[2];
+        Rm[3][4] = Rm[4][4] = beta;
+        for (int i = 0; i < 5; i++) for (int j = 0; j < 5; j++) { Jac[i][j] = 0.; for (int k = 0; k < 5; k++) Jac[i][j] += Lm[i][k] * Rm[k][j]; }
+        const cstate* S = (signal > 0) ? L : R;
+        Q[0] = S->rho; Q[1] = S->rho * S->Vn.c[0]; Q[2] = S->rho * S->Vn.c[1]; Q[3] = S->rho * S->Vn.c[2]; Q[4] = S->rho * S->H - S->p;
+        for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) fn[i] += Jac[i][j] * Q[j];
+    }
+}
+
+static double g_ausm_alpha = 0.;
+/* convective_face_flux (ns_convective_face_flux.cc:32-59) */
+static void convective_face_flux(orc_ctx* o, const cstate* L, const cstate* R, const fstate* F, double flux[5]) {
+    double fn[5];
+    switch (o->opt.convective_flux) {
+    case FCFD_FLUX_ROE: roe_flux(L, R, fn, o->opt.gamma); break;
+    case FCFD_FLUX_VAN_LEER: vanleer_flux(L, R, fn, o->opt.gamma, o->opt.Pref); break;
+    case FCFD_FLUX_SD_SLAU: slau_flux(L, R, fn, o->opt.Pref); break;
+    case FCFD_FLUX_SW: sw_flux(o, L, R, o->fdiss[F->index], o->fwd[F->index], fn); break;
+    default: ausm_flux(L, R, fn, o->opt.Pref, o->opt.Minf, &g_ausm_alpha); break;
+    }
+    flux[0] = fn[0] * F->area;
+    flux[1] = (fn[1] * F->normal.c[0] + fn[2] * F->tangent1.c[0] + fn[3] * F->tangent2.c[0]) * F->area;
+    flux[2] = (fn[1] * F->normal.c[1] + fn[2] * F->tangent1.c[1] + fn[3] * F->tangent2.c[1]) * F->area;
+    flux[3] = (fn[1] * F->normal.c[2] + fn[2] * F->tangent1.c[2] + fn[3] * F->tangent2.c[2]) * F->area;
+    flux[4] = fn[4] * F->area;
+}
+
+/* diffusive_face_flux (ns_diffusive_face_flux.cc:28-64), laminar */
+static void diffusive_face_flux(const orc_ctx* o, const fstate* F, double flux[5]) {
+    V3 aV = vmul(F->normal, F->area);
+    V3 tx, ty, tz;
+    tx.c[0] = 2. / 3. * (2. * F->gradu.c[0] - F->gradv.c[1] - F->gradw.c[2]);
+    tx.c[1] = F->gradu.c[1] + F->gradv.c[0];
+    tx.c[2] = F->gradu.c[2] + F->gradw.c[0];
+    ty.c[0] = tx.c[1];
+    ty.c[1] = 2. / 3. * (2. * F->gradv.c[1] - F->gradu.c[0] - F->gradw.c[2]);
+    ty.c[2] = F->gradv.c[2] + F->gradw.c[1];
+    tz.c[0] = tx.c[2];
+    tz.c[1] = ty.c[2];
+    tz.c[2] = 2. / 3. * (2. * F->gradw.c[2] - F->gradu.c[0] - F->gradv.c[1]);
+    double mu = F->mu + 0.;
+    flux[1] = mu * vdot(tx, aV);
+    flux[2] = mu * vdot(ty, aV);
+    flux[3] = mu * vdot(tz, aV);
+    flux[4] = mu * (vdot(tx, F->V) * aV.c[0] + vdot(ty, F->V) * aV.c[1] + vdot(tz, F->V) * aV.c[2]);
+    double qq = (F->lambda + 0.) * vdot(F->gradT, aV);
+    if (F->bc >= 0 && o->opt.bc[F->bc].thermal == FCFD_THERMAL_FIXED_Q) qq = o->opt.bc[F->bc].qdot * F->area;
+    flux[4] += qq;
+}
+
+/* ------------------------------------------------------------------ */
+/* update_boundaries: decision D1                                     */
+/* ------------------------------------------------------------------ */
+int orc_update_boundaries(orc_ctx* o) {
+    for (int f = 0; f < o->nface; ++f) {
+        if (o->fbc[f] < 0) continue;
+        fstate F;
+        face_geom_update(o, &F, f);
+        int c = F.parent;
+        cstate L;
+        L.p = L.p_center = o->p[c];
+        L.V = L.V_center = cellV(o, c);
+        L.T = L.T_center = o->T[c];
+        L.rho = m_rho(o, L.p, L.T);
+        L.volume = o->vol[c];
+        for (int i = 0; i < 5; ++i) L.update[i] = 0.;
+        finish_state(o, &L, &F);
+        cstate R = L;
+        int rc = apply_bcs(o, &L, &R, &F);
+        if (rc) return rc;
+        int g = F.neighbor;
+        o->p[g] = R.p_center; o->u[g] = R.V_center.c[0]; o->v[g] = R.V_center.c[1]; o->w[g] = R.V_center.c[2]; o->T[g] = R.T_center;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* calc_cell_grads (src/navier_stokes/ns.cc:301-336) with
+ * Variable::cell_gradient (variable.h:193-216)                        */
+/* ------------------------------------------------------------------ */
+static double face_value_scalar(const orc_ctx* o, const double* q, int f, int which) {
+    int b = o->fbc[f];
+    if (b >= 0) {
+        const fcfd_bc* bc = &o->opt.bc[b];
+        if (which == 0 && bc->p_fixed) return bc->p;
+        if (which == 4 && bc->T_fixed) return bc->T;
+        if (which >= 1 && which <= 3 && bc->V_fixed) return bc->V[which - 1];
+    }
+    double t = 0.;
+    for (int k = o->avg_ptr[f]; k < o->avg_ptr[f + 1]; ++k) t += o->avg_w[k] * q[o->avg_idx[k]];
+    return t;
+}
+
+int orc_calc_cell_grads(orc_ctx* o) {
+    const double* q[5] = {o->p, o->u, o->v, o->w, o->T};
+    for (int c = 0; c < o->ncell; ++c) {
+        V3 g[5];
+        for (int k = 0; k < 5; ++k) g[k] = vzero();
+        if (o->g_ptr[c + 1] > o->g_ptr[c]) {
+            for (int e = o->g_ptr[c]; e < o->g_ptr[c + 1]; ++e) {
+                int j = o->g_idx[e];
+                for (int k = 0; k < 5; ++k)
+                    for (int i = 0; i < 3; ++i) g[k].c[i] += o->g_w[e].c[i] * q[k][j];
+            }
+        } else {   /* Green-Gauss fallback */
+            for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
+                int f = o->cfaces[c].a[cf];
+                V3 aV = vdiv(vmul(o->fn[f], o->farea[f]), o->vol[c]);
+                if (o->fparent[f] != c) aV = vmul(aV, -1.);
+                for (int k = 0; k < 5; ++k) {
+                    double fv = face_value_scalar(o, q[k], f, k);
+                    for (int i = 0; i < 3; ++i) g[k].c[i] += fv * aV.c[i];
+                }
+            }
+        }
+        o->gp[c] = g[0]; o->gu[c] = g[1]; o->gv[c] = g[2]; o->gw[c] = g[3]; o->gT[c] = g[4];
+    }
+    for (int f = 0; f < o->nface; ++f)
+        if (o->fbc[f] >= 0) {
+            int par = o->fparent[f], nb = o->fneigh[f];
+            o->gp[nb] = o->gp[par]; o->gu[nb] = o->gu[par]; o->gv[nb] = o->gv[par]; o->gw[nb] = o->gw[par]; o->gT[nb] = o->gT[par];
+        }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* limiters (src/navier_stokes/ns_limiters.cc:26-248, ns.cc:400-435)  */
+/* ------------------------------------------------------------------ */
+static double lim_minmod(double a1, double a2) { if (a1 * a2 < 0.) return 0.; else if (a1 > 0.) return dmin(a1, a2); else return dmax(a1, a2); }
+static double isgn(double x) { return (double)((x > 0) - (x < 0)); }
+
+int orc_calc_limiter(orc_ctx* o) {
+    int lf = o->opt.limiter;
+    if (lf == FCFD_LIMITER_NONE || o->opt.order == FCFD_ORDER_FIRST) return 0;
+    const double* q[5] = {o->p, o->u, o->v, o->w, o->T};
+    V3* gr[5] = {o->gp, o->gu, o->gv, o->gw, o->gT};
+    double deltaRef[5] = {0, 0, 0, 0, 0};
+    double Lref = o->glscale, K = o->opt.limiter_threshold;
+    if (lf == FCFD_LIMITER_VK) {        /* find_min_max with its index slips */
+        double qmax[5], qmin[5];
+        qmax[0] = qmin[0] = o->p[0]; qmax[1] = qmin[1] = o->u[0]; qmax[2] = qmin[2] = o->v[0]; qmax[3] = qmin[3] = o->w[0]; qmax[4] = qmin[4] = o->T[0];
+        for (int c = 0; c < o->ncell; ++c) {
+            qmax[0] = dmax(qmax[0], o->p[c]); qmax[1] = dmax(qmax[1], o->u[c]); qmax[2] = dmax(qmax[2], o->v[c]);
+            qmax[3] = dmax(qmax[4], o->w[c]); qmax[4] = dmax(qmax[4], o->p[c]);
+            qmin[0] = dmin(qmin[0], o->p[c]); qmin[1] = dmin(qmin[1], o->u[c]); qmin[2] = dmin(qmin[2], o->v[c]);
+            qmin[3] = dmin(qmin[4], o->w[c]); qmin[4] = dmin(qmin[4], o->p[c]);
+        }
+        for (int i = 0; i < 5; ++i) deltaRef[i] = fabs(qmax[i] - qmin[i]);
+    }
+    for (int c = 0; c < o->ncell; ++c) {
+        double phi[5] = {1., 1., 1., 1., 1.};
+        if (lf == FCFD_LIMITER_MINMOD) {
+            const double small = 1.e-12;
+            for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
+                int f = o->cfaces[c].a[cf];
+                V3 c2f = vsub(o->fc[f], o->cc[c]);
+                int nb = (c == o->fparent[f]) ? o->fneigh[f] : o->fparent[f];
+                for (int var = 0; var < 5; ++var) {
+                    double dU = q[var][nb] - q[var][c];
+                    double dUg = vdot(gr[var][c], c2f);
+                    dU = lim_minmod(dU, dUg);
+                    phi[var] = dmin(phi[var], (dU + isgn(dU) * small) / (dUg + isgn(dUg) * small));
+                }
+            }
+        } else {
+            double umax[5], umin[5];
+            for (int i = 0; i < 5; ++i) umax[i] = umin[i] = q[i][c];
+            for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
+                int f = o->cfaces[c].a[cf];
+                int nb = (c == o->fparent[f]) ? o->fneigh[f] : o->fparent[f];
+                for (int i = 0; i < 5; ++i) { umax[i] = dmax(umax[i], q[i][nb]); umin[i] = dmin(umin[i], q[i][nb]); }
+            }
+            for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
+                int f = o->cfaces[c].a[cf];
+                V3 c2f = vsub(o->fc[f], o->cc[c]);
+                for (int var = 0; var < 5; ++var) {
+                    double dM = vdot(gr[var][c], c2f);
+                    double dP = q[var][c];
+                    if (lf == FCFD_LIMITER_BJ) {
+                        if (dM > 0.) dP = umax[var] - dP;
+                        else if (dM < 0.) dP = umin[var] - dP;
+                        else { dP = 1.; dM = 1.; }
+                        phi[var] = dmin(phi[var], dP / dM);
+                    } else {   /* VK */
+                        double eps = deltaRef[var] / Lref;
+                        if (dM > 0.) dP = umax[var] - dP;
+                        else if (dM < 0.) dP = umin[var] - dP;
+                        double dP2 = dP * dP, dM2 = dM * dM;
+                        double eps2 = pow(1.e-6 * K * eps, 2);
+                        if (dM != 0.) phi[var] = dmin(phi[var], ((dP2 + eps2) + 2. * dM * dP) / (dP2 + 2. * dM2 + dM * dP + eps2));
+                    }
+                }
+            }
+        }
+        for (int i = 0; i < 5; ++i) o->phi[5 * c + i] = phi[i];
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* assemble_linear_system (ns_assemble_linear_system.cc:34-267)        */
+/* ------------------------------------------------------------------ */
+int orc_assemble(orc_ctx* o, int with_jac) {
+    int N = o->ncell;
+    for (int i = 0; i < 5 * N; ++i) { o->Rs[i] = 0.; o->Rabs[i] = 0.; }
+    for (int i = 0; i < 6 * o->nbc; ++i) o->loads[i] = 0.;
+    if (with_jac) {
+        if (!o->JL) { o->JL = (double*)malloc(sizeof(double) * 25 * o->nface); o->JR = (double*)malloc(sizeof(double) * 25 * o->nface); o->Dg = (double*)malloc(sizeof(double) * 25 * (size_t)N); }
+        memset(o->JL, 0, sizeof(double) * 25 * o->nface);
+        memset(o->JR, 0, sizeof(double) * 25 * o->nface);
+        memset(o->Dg, 0, sizeof(double) * 25 * (size_t)N);
+    }
+    const double small_number = 10. * sqrt(2.220446049250313e-16);
+    const double factor = 0.001;
+    double fc[5], fd[5], fcp[5], fdp[5];
+    for (int f = 0; f < o->nface; ++f) {
+        fstate F;
+        cstate L, R;
+        face_geom_update(o, &F, f);
+        recon_state(o, &L, &F, F.parent);
+        finish_state(o, &L, &F);
+        int rc = right_state_update(o, &L, &R, &F);
+        if (rc) return rc;
+        face_state_update(o, &L, &R, &F);
+        convective_face_flux(o, &L, &R, &F, fc);
+        fd[0] = 0.;
+        diffusive_face_flux(o, &F, fd);
+        if (F.bc >= 0 && o->opt.bc[F.bc].include_in_loads) {          /* :109-119 */
+            V3 t = v3(fc[1] - fd[1], fc[2] - fd[2], fc[3] - fd[3]);
+            double* ld = &o->loads[6 * F.bc];
+            ld[0] += t.c[0]; ld[1] += t.c[1]; ld[2] += t.c[2];
+            V3 mc = v3(o->opt.moment_center[0], o->opt.moment_center[1], o->opt.moment_center[2]);
+            V3 mm = vcross(vsub(o->fc[f], mc), t);
+            ld[3] += mm.c[0]; ld[4] += mm.c[1]; ld[5] += mm.c[2];
+        }
+        for (int i = 0; i < 5; ++i) {                                  /* :131-140 */
+            double val = fd[i] - fc[i] + 0.;
+            o->Rs[5 * F.parent + i] += val;
+            o->Rabs[5 * F.parent + i] += fabs(val);
+            if (F.bc == -1) {
+                double vn = -1. * (fd[i] - fc[i]) + 0.;
+                o->Rs[5 * F.neighbor + i] += vn;
+                o->Rabs[5 * F.neighbor + i] += fabs(vn);
+            }
+        }
+        if (!with_jac) continue;
+        for (int var = 0; var < 5; ++var) {                            /* get_jacobians (:197-267) */
+            double jl[5], jr[5] = {0, 0, 0, 0, 0};
+            cstate Lp = L, Rp = R;
+            double eps;
+            if (L.update[var] > small_number) eps = dmax(small_number, factor * L.update[var]);
+            else if (L.update[var] < -small_number) eps = dmin(-small_number, factor * L.update[var]);
+            else eps = small_number;
+            state_perturb(o, &Lp, &F, var, eps);
+            if (F.bc >= 0) { rc = right_state_update(o, &Lp, &Rp, &F); if (rc) return rc; }
+            convective_face_flux(o, &Lp, &Rp, &F, fcp);
+            face_state_adjust(o, &Lp, &Rp, &F, var);
+            fdp[0] = 0.;
+            diffusive_face_flux(o, &F, fdp);
+            for (int j = 0; j < 5; ++j) jl[j] = (fdp[j] - fd[j] - fcp[j] + fc[j]) / eps;
+            if (F.bc == -1) {
+                if (R.update[var] > small_number) eps = dmax(small_number, factor * R.update[var]);
+                else if (R.update[var] < -small_number) eps = dmin(-small_number, factor * R.update[var]);
+                else eps = small_number;
+                state_perturb(o, &Rp, &F, var, eps);
+                convective_face_flux(o, &L, &Rp, &F, fcp);
+                face_state_adjust(o, &L, &Rp, &F, var);
+                fdp[0] = 0.;
+                diffusive_face_flux(o, &F, fdp);
+                for (int j = 0; j < 5; ++j) jr[j] = (fdp[j] - fd[j] - fcp[j] + fc[j]) / eps;
+            }
+            face_state_adjust(o, &L, &R, &F, var);
+            for (int j = 0; j < 5; ++j) {
+                o->JL[25 * f + 5 * var + j] = jl[j];
+                o->JR[25 * f + 5 * var + j] = jr[j];
+            }
+            /* MatSetValues(ADD) of the diagonal blocks (:154-177) */
+            for (int j = 0; j < 5; ++j) {
+                o->Dg[25 * F.parent + 5 * j + var] += -1. * jl[j];
+                if (F.bc == -1) o->Dg[25 * F.neighbor + 5 * j + var] += jr[j];
+            }
+        }
+    }
+    return 0;
+}
+
+int orc_get_residual(const orc_ctx* o, double* R, double* Rabs) {
+    if (R) memcpy(R, o->Rs, sizeof(double) * 5 * (size_t)o->ncell);
+    if (Rabs) memcpy(Rabs, o->Rabs, sizeof(double) * 5 * (size_t)o->ncell);
+    return 0;
+}
+
+int orc_get_fields(const orc_ctx* o, double* Wf, double* G, double* phi) {
+    for (int c = 0; c < o->ntot; ++c) {
+        if (Wf) { Wf[5 * c] = o->p[c]; Wf[5 * c + 1] = o->u[c]; Wf[5 * c + 2] = o->v[c]; Wf[5 * c + 3] = o->w[c]; Wf[5 * c + 4] = o->T[c]; }
+        if (G) {
+            const V3* g[5] = {&o->gp[c], &o->gu[c], &o->gv[c], &o->gw[c], &o->gT[c]};
+            for (int k = 0; k < 5; ++k) for (int i = 0; i < 3; ++i) G[15 * c + 3 * k + i] = g[k]->c[i];
+        }
+    }
+    if (phi) for (int c = 0; c < o->ncell; ++c) for (int i = 0; i < 5; ++i) phi[5 * c + i] = o->phi[5 * c + i];
+    return 0;
+}
+
+int orc_get_jacobian(const orc_ctx* o, double* JL, double* JR, double* Dg) {
+    if (!o->JL) return FCFD_EINVAL;
+    if (JL) memcpy(JL, o->JL, sizeof(double) * 25 * o->nface);
+    if (JR) memcpy(JR, o->JR, sizeof(double) * 25 * o->nface);
+    if (Dg) memcpy(Dg, o->Dg, sizeof(double) * 25 * (size_t)o->ncell);
+    return 0;
+}
+
+int orc_get_loads(const orc_ctx* o, double* loads) { memcpy(loads, o->loads, sizeof(double) * 6 * o->nbc); return 0; }
+
+/* update_time_step (src/time_step.cc:143-224), no ramp / schedule */
+int orc_time_step(orc_ctx* o, int dt_type, double cfl, double* dt, double* max_cfl) {
+    double cur = 1.;
+    if (dt_type == FCFD_DT_CFL_MAX) {
+        cur = 1.e20;
+        for (int c = 0; c < o->ncell; ++c) {
+            double a = m_a(o, o->p[c], o->T[c]);
+            cur = dmin(cur, cfl * o->lscale[c] / (vmag(cellV(o, c)) + a));
+        }
+        for (int c = 0; c < o->ncell; ++c) dt[c] = cur;
+    } else {
+        for (int c = 0; c < o->ncell; ++c) {
+            double a = m_a(o, o->p[c], o->T[c]);
+            dt[c] = cfl * o->lscale[c] / (vmag(cellV(o, c)) + a);
+        }
+    }
+    double mc = -1.;
+    for (int c = 0; c < o->ncell; ++c) {
+        double a = m_a(o, o->p[c], o->T[c]);
+        mc = dmax(mc, (vmag(cellV(o, c)) + a) * cur / o->lscale[c]);
+    }
+    if (dt_type == FCFD_DT_CFL_LOCAL) mc = cfl;
+    *max_cfl = mc;
+    return 0;
+}
+
+/* time_terms: cons2prim(c)*V/dt (ns_time_terms.cc:26-56,147-163) */
+int orc_time_terms(orc_ctx* o, const double* dt, double* blocks) {
+    for (int c = 0; c < o->ncell; ++c) {
+        double P[5][5] = {{0}};
+        double rho = m_rho(o, o->p[c], o->T[c]);
+        double pressure = o->p[c] + o->opt.Pref, temperature = o->T[c] + o->opt.Tref;
+        double drho_dT = -1. * rho / temperature, drho_dp = rho / pressure;
+        double c_p = o->Cp;
+        V3 V = cellV(o, c);
+        double H = c_p * temperature + 0.5 * vdot(V, V);
+        P[0][0] = drho_dp; P[0][4] = drho_dT;
+        P[1][0] = drho_dp * V.c[0]; P[1][1] = rho; P[1][4] = drho_dT * V.c[0];
+        P[2][0] = drho_dp * V.c[1]; P[2][2] = rho; P[2][4] = drho_dT * V.c[1];
+        P[3][0] = drho_dp * V.c[2]; P[3][3] = rho; P[3][4] = drho_dT * V.c[2];
+        P[4][0] = drho_dp * H - 1.; P[4][1] = rho * V.c[0]; P[4][2] = rho * V.c[1]; P[4][3] = rho * V.c[2];
+        P[4][4] = drho_dT * H + rho * c_p;
+        for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) blocks[25 * c + 5 * i + j] = P[i][j] * o->vol[c] / dt[c];
+    }
+    return 0;
+}

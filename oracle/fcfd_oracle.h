@@ -1,0 +1,54 @@
+/*
+ * fcfd_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement (plain C, FP64, no FMA contraction) of Free CFD's
+ * residual/Jacobian path, used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the parity checker.  Nothing in the
+ * product (freecfd_amd/, include/) links or calls it.
+ */
+#ifndef FCFD_ORACLE_H
+#define FCFD_ORACLE_H
+#include "../include/fcfd_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+/* set-up: Grid::setup (create_nodes_cells, create_faces, areas_volumes,
+ * create_boundary_ghosts), set_bcs, set_lengthScales,
+ * face_interpolation_weights, gradient_maps.  Returns NULL on error; the
+ * message is then in orc_last_error(NULL). */
+orc_ctx* orc_create(const fcfd_mesh* mesh, const fcfd_options* opt);
+void orc_destroy(orc_ctx* o);
+const char* orc_last_error(const orc_ctx* o);
+int orc_info(const orc_ctx* o, fcfd_info* info);
+int orc_metrics(const orc_ctx* o, fcfd_metrics_out* m);
+
+/* state W: N x 5 (p,u,v,w,T), reference numbering */
+int orc_set_state(orc_ctx* o, const double* W);
+int orc_set_update(orc_ctx* o, const double* dW);   /* previous Newton update (update[i].cell) */
+int orc_update_boundaries(orc_ctx* o);
+int orc_calc_cell_grads(orc_ctx* o);
+int orc_calc_limiter(orc_ctx* o);
+/* face loop; with_jacobian adds the FD Jacobian (get_jacobians) */
+int orc_assemble(orc_ctx* o, int with_jacobian);
+/* R, sum|contrib| : N x 5 */
+int orc_get_residual(const orc_ctx* o, double* R, double* Rabs);
+/* full W incl. ghosts (N+B) x 5, grads (N+B) x 15, limiter N x 5 */
+int orc_get_fields(const orc_ctx* o, double* Wfull, double* G, double* phi);
+/* FD Jacobian: per face JL,JR [F][5 var][5 eq]; diag blocks [N][5 row][5 col]
+ * (face contributions only, before time terms) */
+int orc_get_jacobian(const orc_ctx* o, double* JL, double* JR, double* Dg);
+/* loads per BC: force xyz, moment xyz */
+int orc_get_loads(const orc_ctx* o, double* loads);
+/* local time step, CFL_LOCAL or CFL_MAX (time_step.cc:143-224) */
+int orc_time_step(orc_ctx* o, int dt_type, double cfl, double* dt, double* max_cfl);
+/* time_terms P*V/dt blocks [N][5][5] (ns_time_terms.cc:26-56,147-163) */
+int orc_time_terms(orc_ctx* o, const double* dt, double* blocks);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,146 @@
+"""ctypes wrapper of the CPU restatement (TEST INFRASTRUCTURE ONLY).
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the parity checker; the product package never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from freecfd_amd import _abi as A
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(A.Mesh), C.POINTER(A.Options)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_last_error.argtypes = [C.c_void_p]
+        for name in ("orc_set_state", "orc_set_update"):
+            getattr(L, name).argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        for name in ("orc_update_boundaries", "orc_calc_cell_grads", "orc_calc_limiter"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.orc_assemble.argtypes = [C.c_void_p, C.c_int]
+        L.orc_info.argtypes = [C.c_void_p, C.POINTER(A.Info)]
+        L.orc_metrics.argtypes = [C.c_void_p, C.POINTER(A.MetricsOut)]
+        L.orc_get_residual.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_get_fields.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_get_jacobian.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_get_loads.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_time_step.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.POINTER(C.c_double)]
+        L.orc_time_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """Reference-numbered CPU restatement of one grid + NavierStokes solver."""
+
+    def __init__(self, raw, opt):
+        L = lib()
+        self._mesh = A.make_mesh(raw)
+        self._opt = opt
+        h = L.orc_create(C.byref(self._mesh), C.byref(opt))
+        if not h:
+            raise RuntimeError("oracle set-up failed: " + L.orc_last_error(None).decode())
+        self.h = h
+        self.info = A.Info()
+        L.orc_info(h, C.byref(self.info))
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError("oracle error %d: %s" % (rc, lib().orc_last_error(self.h).decode()))
+
+    def metrics(self):
+        m, arrs = A.alloc_metrics(self.info)
+        self._check(lib().orc_metrics(self.h, C.byref(m)))
+        F = self.info.n_faces
+        arrs["face_nodes"] = arrs["face_nodes"][:arrs["face_node_ptr"][F]]
+        return arrs
+
+    def set_state(self, W, dW=None):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        self._check(lib().orc_set_state(self.h, W.ctypes.data_as(C.POINTER(C.c_double))))
+        if dW is not None:
+            dW = np.ascontiguousarray(dW, dtype=np.float64)
+            self._check(lib().orc_set_update(self.h, dW.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def prepare(self):
+        """update_boundaries -> calc_cell_grads -> calc_limiter (ns.cc:112-116)."""
+        L = lib()
+        self._check(L.orc_update_boundaries(self.h))
+        self._check(L.orc_calc_cell_grads(self.h))
+        self._check(L.orc_calc_limiter(self.h))
+
+    def residual(self, W, with_jacobian=False, dW=None):
+        self.set_state(W, dW)
+        self.prepare()
+        self._check(lib().orc_assemble(self.h, int(with_jacobian)))
+        N = self.info.n_cells
+        R = np.zeros((N, 5))
+        Rabs = np.zeros((N, 5))
+        self._check(lib().orc_get_residual(self.h, _p(R), _p(Rabs)))
+        return R, Rabs
+
+    def fields(self):
+        T = self.info.n_cells + self.info.n_ghosts
+        W = np.zeros((T, 5))
+        G = np.zeros((T, 15))
+        phi = np.zeros((self.info.n_cells, 5))
+        self._check(lib().orc_get_fields(self.h, _p(W), _p(G), _p(phi)))
+        return W, G, phi
+
+    def jacobian(self):
+        F, N = self.info.n_faces, self.info.n_cells
+        JL = np.zeros((F, 5, 5))
+        JR = np.zeros((F, 5, 5))
+        Dg = np.zeros((N, 5, 5))
+        self._check(lib().orc_get_jacobian(self.h, _p(JL), _p(JR), _p(Dg)))
+        return JL, JR, Dg
+
+    def loads(self):
+        out = np.zeros((self._opt.n_bc, 6))
+        self._check(lib().orc_get_loads(self.h, _p(out)))
+        return out
+
+    def time_step(self, dt_type, cfl):
+        dt = np.zeros(self.info.n_cells)
+        mc = C.c_double(0.0)
+        self._check(lib().orc_time_step(self.h, dt_type, cfl, _p(dt), C.byref(mc)))
+        return dt, mc.value
+
+    def time_terms(self, dt):
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        out = np.zeros((self.info.n_cells, 5, 5))
+        self._check(lib().orc_time_terms(self.h, _p(dt), _p(out)))
+        return out

@@ -1,0 +1,24 @@
+"""The C-ABI library loads on the CPU and exports every entry point that
+include/fcfd_gpu.h declares (no compute calls without a GPU)."""
+import ctypes
+
+from freecfd_amd import _lib
+
+
+def test_library_exports_header_symbols():
+    L = _lib.lib()
+    names = _lib.exported_symbols()
+    assert len(names) >= 10
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_create_host_reports_errors_without_exit():
+    from freecfd_amd import _abi as A, cases
+    cs = cases.wedge_case(4, 2)
+    cs.opt.bc[1].specified = A.SPEC_STATE      # outlet with a complete state: input error
+    m = A.make_mesh(cs.raw)
+    h = ctypes.c_void_p()
+    rc = _lib.lib().fcfd_create_host(ctypes.byref(m), ctypes.byref(cs.opt), ctypes.byref(h))
+    assert rc == A.EBC
+    assert b"outlet" in _lib.lib().fcfd_last_error(None)

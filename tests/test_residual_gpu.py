@@ -1,0 +1,99 @@
+"""GPU parity of the FP64 residual path against the CPU restatement.
+
+Bar (north star): per cell and variable, |R_gpu - R_oracle| <= 1e-12 * S,
+where S = sum over the cell's faces of |face contribution| (the summation
+scale; |R| itself cancels to ~0 near steady state).  Ghost states must match
+to 1e-14 relative, gradients to 1e-11 of their summation scale.
+"""
+import numpy as np
+import pytest
+
+from freecfd_amd import _abi as A
+from freecfd_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+RTOL_R = 1e-12
+
+
+def _compare(cs, W=None, label=""):
+    from oracle.oracle import Oracle
+    from freecfd_amd.ns import NavierStokes
+    W = cs.W0 if W is None else W
+    print("case", cs.name, label, "N", cs.n_cells, flush=True)
+    o = Oracle(cs.raw, cs.opt)
+    Ro, Rabs = o.residual(W)
+    Wo, Go, phio = o.fields()
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(W)
+    ns.assemble_residual()
+    ns.status()
+    Rg = ns.residual_host()
+    Wg = ns.state_host(include_ghosts=True)
+    Gg, phig = ns.gradients_limiters_host()
+    N = cs.n_cells
+    # ghosts
+    gscale = np.abs(Wo).max(axis=0)
+    assert np.all(np.abs(Wg - Wo) <= 1e-14 * gscale + 1e-300), "ghost state mismatch"
+    # gradients: scale = max |G| of the variable over the mesh
+    gs = np.abs(Go).max(axis=0) + 1e-300
+    gerr = (np.abs(Gg - Go) / gs).max()
+    assert gerr <= 1e-11, "gradient mismatch %g" % gerr
+    if cs.opt.limiter != A.LIMITER_NONE and cs.opt.order == A.ORDER_SECOND:
+        perr = np.abs(phig - phio).max()
+        assert perr <= 1e-9, "limiter mismatch %g" % perr
+    err = np.abs(Rg - Ro) / (Rabs + 1e-300)
+    worst = float(err.max())
+    print("  max |dR|/S = %.3e  max|R| = %.3e" % (worst, np.abs(Ro).max()), flush=True)
+    assert worst <= RTOL_R, "residual mismatch %g at %s" % (worst, np.unravel_index(err.argmax(), err.shape))
+    ns.close()
+    o.close()
+    return worst
+
+
+@pytest.mark.parametrize("flux", [A.FLUX_ROE, A.FLUX_AUSM_PLUS_UP, A.FLUX_VAN_LEER, A.FLUX_SD_SLAU])
+def test_wedge_fluxes(flux):
+    _compare(cases.wedge_case(24, 12, flux=flux), label="flux=%d" % flux)
+
+
+def test_wedge_full_config():
+    _compare(cases.wedge_case(96, 48), label="96x48")
+
+
+@pytest.mark.parametrize("lim", [A.LIMITER_BJ, A.LIMITER_VK, A.LIMITER_NONE])
+def test_wedge_limiters(lim):
+    cs = cases.wedge_case(24, 12)
+    cs.opt.limiter = lim
+    _compare(cs, label="lim=%d" % lim)
+
+
+def test_first_order():
+    cs = cases.wedge_case(24, 12)
+    cs.opt.order = A.ORDER_FIRST
+    _compare(cs, label="first order")
+
+
+def test_cylinder_viscous_small():
+    _compare(cases.cylinder_case(32, 16, noise=0.01), label="32x16")
+
+
+def test_cylinder_full_config():
+    _compare(cases.cylinder_case(256, 128, noise=0.01), label="256x128")
+
+
+def test_tetbox_viscous_lsqr():
+    _compare(cases.tetbox_case(4), label="n=4")
+
+
+def test_bump_curvilinear():
+    _compare(cases.bump_case(16, 8, 4, noise=0.01), label="16x8x4")
+
+
+def test_plate_prisms_viscous():
+    _compare(cases.plate_case(16, 10, 4, noise=0.01, first=1e-3), label="16x10x4")
+
+
+def test_plate_steger_warming():
+    cs = cases.plate_case(12, 8, 3, noise=0.01, first=1e-3)
+    cs.opt.convective_flux = A.FLUX_SW
+    _compare(cs, label="SW")
