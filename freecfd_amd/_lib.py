@@ -33,6 +33,9 @@ def lib():
         "fcfd_internal_buffers": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
         "fcfd_status": (C.c_int, [_vp, C.c_int]),
         "fcfd_profile": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+        "fcfd_fd_jacobian": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "fcfd_jacobian_buffers": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+        "fcfd_jacobian_to_host": (C.c_int, [_vp, _vp, _vp, _vp]),
         "fcfd_time_step": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _vp, C.POINTER(C.c_double), _vp]),
     }
     for name, (res, args) in sig.items():

@@ -11,6 +11,7 @@
 
 #include "fcfd_device.h"
 #include "fcfd_internal.h"
+#include "fcfd_jacobian.h"
 
 using namespace fcfd;
 
@@ -24,6 +25,7 @@ struct fcfd_ctx {
     std::vector<void*> allocs;
     double *G = nullptr, *PHI = nullptr, *dref = nullptr;
     int* status = nullptr;
+    double *JL = nullptr, *JR = nullptr, *Dg = nullptr, *zero_upd = nullptr;   // FD Jacobian (lazy)
     std::string err;
     // optional per-stage timing: a ring of event quadruples recorded on the
     // launch stream around the three kernels, folded in lazily so that
@@ -441,6 +443,59 @@ int fcfd_time_step(fcfd_ctx* c, const double* W_dev, int dt_type, double cfl, do
     launch_dt_local(c->dg, c->P, W_dev, cfl, dt_dev, (hipStream_t)stream);
     HIPCHK(c, hipGetLastError());
     if (max_cfl) *max_cfl = cfl;
+    return FCFD_OK;
+}
+
+int fcfd_fd_jacobian(fcfd_ctx* c, double* W_dev, const double* dWprev_dev, const double* dt_dev, void* stream) {
+    if (!c || !W_dev) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    const DevGrid& g = c->dg;
+    if (!c->JL) {
+        int rc;
+        if ((rc = dalloc(c, (size_t)25 * g.F, &c->JL)) || (rc = dalloc(c, (size_t)25 * g.F, &c->JR)) ||
+            (rc = dalloc(c, (size_t)25 * g.ld, &c->Dg)) || (rc = dalloc(c, (size_t)5 * g.ld, &c->zero_upd)))
+            return rc;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    launch_ghosts(g, c->P, W_dev, c->status, s);
+    launch_grad_lim(g, c->P, W_dev, c->G, c->PHI, c->dref, s);
+    launch_fd_jacobian(g, c->P, W_dev, c->G, c->PHI, dWprev_dev ? dWprev_dev : c->zero_upd, dt_dev, c->JL, c->JR, c->Dg,
+                       c->status, s);
+    HIPCHK(c, hipGetLastError());
+    return FCFD_OK;
+}
+
+int fcfd_jacobian_buffers(fcfd_ctx* c, const double** JL, const double** JR, const double** D) {
+    if (!c || !c->device || !c->JL) return fail(c, FCFD_EINVAL, "no Jacobian has been assembled");
+    if (JL) *JL = c->JL;
+    if (JR) *JR = c->JR;
+    if (D) *D = c->Dg;
+    return FCFD_OK;
+}
+
+int fcfd_jacobian_to_host(fcfd_ctx* c, double* JL_host, double* JR_host, double* D_host) {
+    if (!c || !c->device || !c->JL) return fail(c, FCFD_EINVAL, "no Jacobian has been assembled");
+    HIPCHK(c, hipDeviceSynchronize());
+    const HostGrid& hg = c->hg;
+    const int F = hg.F, N = hg.N, ld = c->dg.ld;
+    std::vector<double> tmp((size_t)25 * std::max(F, ld));
+    double* outs[2] = {JL_host, JR_host};
+    const double* srcs[2] = {c->JL, c->JR};
+    for (int k = 0; k < 2; ++k) {
+        if (!outs[k]) continue;
+        HIPCHK(c, hipMemcpy(tmp.data(), srcs[k], (size_t)25 * F * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < F; ++i) {
+            int f = hg.ifperm[i];
+            for (int e = 0; e < 25; ++e) outs[k][(size_t)25 * f + e] = tmp[(size_t)e * F + i];
+        }
+    }
+    if (D_host) {
+        HIPCHK(c, hipMemcpy(tmp.data(), c->Dg, (size_t)25 * ld * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < N; ++i) {
+            int r = hg.iperm[i];
+            for (int e = 0; e < 25; ++e) D_host[(size_t)25 * r + e] = tmp[(size_t)e * ld + i];
+        }
+    }
     return FCFD_OK;
 }
 

@@ -1,0 +1,101 @@
+// Per-face state assembly shared by the residual, Jacobian and tangent
+// kernels: loads the face geometry and both cell states, reconstructs the
+// face states (MUSCL), applies the boundary condition, and evaluates the
+// convective and diffusive fluxes, exactly as one iteration of the face loop
+// of ns_assemble_linear_system.cc:75-93.
+#pragma once
+#include "fcfd_device.h"
+#include "fcfd_physics.h"
+
+namespace fcfd {
+
+__device__ inline void load_geo(const DevGrid& g, int f, FaceGeo& F) {
+    F.n = mk<double>(g.fnx[f], g.fny[f], g.fnz[f]);
+    F.t1 = mk<double>(g.t1x[f], g.t1y[f], g.t1z[f]);
+    F.t2 = vnorm(vcross(F.n, F.t1));
+    F.area = g.farea[f];
+    F.bc = g.fbc[f];
+}
+
+__device__ inline Vec<double> centroid(const DevGrid& g, int c) { return mk<double>(g.ccx[c], g.ccy[c], g.ccz[c]); }
+
+__device__ inline void load_cell(const DevGrid& g, const double* __restrict__ W, const double* __restrict__ G,
+                                 const double* __restrict__ PHI, int c, bool has_phi, double phi_const, double q[5],
+                                 Vec<double> gr[5], double ph[5]) {
+    const size_t ld = g.ld;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        q[v] = W[v * ld + c];
+        gr[v] = mk<double>(G[(3 * v) * ld + c], G[(3 * v + 1) * ld + c], G[(3 * v + 2) * ld + c]);
+        ph[v] = has_phi ? PHI[v * ld + c] : phi_const;
+    }
+}
+
+// everything one face-loop iteration computes before the residual insertion
+struct FaceCtx {
+    FaceGeo F;
+    CState<double> L, R;
+    FState<double> fs;
+    double C[5], D[5];
+    int p, n;
+};
+
+template <int FLUX, int VISC>
+__device__ inline void face_base(const DevGrid& g, const Params& P, const double* __restrict__ W, const double* __restrict__ G,
+                                 const double* __restrict__ PHI, bool has_phi, double phi_const, int f, FaceCtx& X,
+                                 int* status) {
+    FaceGeo& F = X.F;
+    load_geo(g, f, F);
+    const int p = g.fpar[f], n = g.fnei[f];
+    X.p = p; X.n = n;
+    Vec<double> xf = mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]);
+    Vec<double> xp = centroid(g, p), xn = centroid(g, n);
+    F.l2r = vsub(xn, xp);
+    double q[5], ph[5];
+    Vec<double> gp[5];
+    load_cell(g, W, G, PHI, p, has_phi, phi_const, q, gp, ph);
+    recon(P, X.L, q, gp, ph, vsub(xf, xp));
+    finish_state(P, X.L, F);
+    Vec<double> gn[5];
+    if (F.bc >= 0) {   // right_state_update for a boundary face (:298-303)
+        const size_t ld = g.ld;
+        X.R.p_center = W[n];
+        X.R.V_center = mk<double>(W[ld + n], W[2 * ld + n], W[3 * ld + n]);
+        X.R.T_center = W[4 * ld + n];
+        if (apply_bcs(P, g.bc[F.bc], X.L, X.R, F)) atomicOr(status, 1);
+    } else {
+        double qn[5], phn[5];
+        load_cell(g, W, G, PHI, n, has_phi, phi_const, qn, gn, phn);
+        recon(P, X.R, qn, gn, phn, vsub(xf, xn));
+    }
+    finish_state(P, X.R, F);
+    convective_flux<FLUX>(P, X.L, X.R, F, FLUX == FCFD_FLUX_SW ? g.fdiss[f] : 0.0, FLUX == FCFD_FLUX_SW ? g.fwd[f] : 0.0, X.C);
+    if (VISC) {
+        if (F.bc >= 0) {
+            X.fs.gu = gp[1]; X.fs.gv = gp[2]; X.fs.gw = gp[3]; X.fs.gT = gp[4];
+        } else {   // face average (variable.h:134-147), two-entry stencil in key order
+            double wp = g.fwp[f], wn = g.fwn[f];
+            X.fs.gu = vadd(vsc(wp, gp[1]), vsc(wn, gn[1]));
+            X.fs.gv = vadd(vsc(wp, gp[2]), vsc(wn, gn[2]));
+            X.fs.gw = vadd(vsc(wp, gp[3]), vsc(wn, gn[3]));
+            X.fs.gT = vadd(vsc(wp, gp[4]), vsc(wn, gn[4]));
+        }
+        face_state(P, X.L, X.R, F, X.fs);
+        diffusive_flux(X.fs, F, g.bc, X.D);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) X.D[v] = 0.0;
+    }
+}
+
+template <int FLUX, int VISC>
+__device__ inline void face_flux(const DevGrid& g, const Params& P, const double* __restrict__ W, const double* __restrict__ G,
+                                 const double* __restrict__ PHI, bool has_phi, double phi_const, int f, double out[5],
+                                 int* status) {
+    FaceCtx X;
+    face_base<FLUX, VISC>(g, P, W, G, PHI, has_phi, phi_const, f, X, status);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = X.D[v] - X.C[v] + 0.;
+}
+
+}  // namespace fcfd

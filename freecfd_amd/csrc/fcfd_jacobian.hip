@@ -1,0 +1,206 @@
+// Reference-equivalent finite-difference Jacobian (FP64) and its assembly.
+//
+//   fdjac_kernel     one thread per face: the 1 + 10 flux evaluations of
+//                    get_jacobians (ns_assemble_linear_system.cc:197-267) with
+//                    the same epsilon policy, BC re-application on left
+//                    perturbations, face-state adjustment (including the
+//                    case-1 fall-through and the face-state drift between
+//                    perturbed variables), writing JL[var][eq] and JR[var][eq]
+//                    per face ([25][F] each, coalesced).
+//   diag_kernel      one thread per cell: diagonal block = sum over the cell's
+//                    faces in reference face order of -JL (parent side) or
+//                    +JR (neighbour side), then + P V / dt (time_terms,
+//                    ns_time_terms.cc:147-163) -- the MatSetValues(ADD) order.
+//   The off-diagonal block (row P, col N) is -JR^T and (row N, col P) is +JL^T
+//   of the face; each is touched by exactly one face, so it stays per face.
+#include <hip/hip_runtime.h>
+
+#include "fcfd_face.h"
+#include "fcfd_jacobian.h"
+
+namespace fcfd {
+
+// state_perturb (:417-458)
+__device__ inline void state_perturb(const Params& P, CState<double>& s, const FaceGeo& F, int var, double eps) {
+    switch (var) {
+    case 0: s.p += eps; s.rho = m_rho(P, s.p, s.T); s.a = m_a(P, s.p, s.T); break;
+    case 1: s.V.x += eps; s.V_center.x += eps; s.Vn = mk<double>(vdot(s.V, F.n), vdot(s.V, F.t1), vdot(s.V, F.t2)); break;
+    case 2: s.V.y += eps; s.V_center.y += eps; s.Vn = mk<double>(vdot(s.V, F.n), vdot(s.V, F.t1), vdot(s.V, F.t2)); break;
+    case 3: s.V.z += eps; s.V_center.z += eps; s.Vn = mk<double>(vdot(s.V, F.n), vdot(s.V, F.t1), vdot(s.V, F.t2)); break;
+    default: s.T += eps; s.T_center += eps; s.rho = m_rho(P, s.p, s.T); s.a = m_a(P, s.p, s.T); break;
+    }
+    s.H = s.a * s.a / (P.gamma - 1.) + 0.5 * vdot(s.V, s.V);
+}
+
+// face_state_adjust (:460-500); case 1 falls through into case 2
+__device__ inline void face_state_adjust(const Params& P, const CState<double>& L, const CState<double>& R, const FaceGeo& F,
+                                         FState<double>& f, int var) {
+    Vec<double> n = vnorm(F.l2r);
+    double m = vmag(F.l2r);
+    if (var == 1) {
+        f.V.x = 0.5 * (L.V.x + R.V.x);
+        f.gu = correct(f.gu, n, m, R.V_center.x - L.V_center.x);
+    }
+    if (var == 1 || var == 2) {
+        f.V.y = 0.5 * (L.V.y + R.V.y);
+        f.gv = correct(f.gv, n, m, R.V_center.y - L.V_center.y);
+    } else if (var == 3) {
+        f.V.z = 0.5 * (L.V.z + R.V.z);
+        f.gw = correct(f.gw, n, m, R.V_center.z - L.V_center.z);
+    } else if (var == 4) {
+        f.T = 0.5 * (L.T + R.T);
+        f.gT = correct(f.gT, n, m, R.T_center - L.T_center);
+        f.mu = m_visc(P, f.T);
+        f.lambda = m_cond(P, f.T);
+    }
+}
+
+__device__ inline double fd_eps(double upd) {   // (:48, :217-219)
+    const double small_number = 10. * 1.4901161193847656e-08;   // 10*sqrt(DBL_EPSILON)
+    const double factor = 0.001;
+    if (upd > small_number) return smax(small_number, factor * upd);
+    if (upd < -small_number) return smin(-small_number, factor * upd);
+    return small_number;
+}
+
+template <int FLUX, int VISC>
+__global__ void __launch_bounds__(128) fdjac_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ G,
+                                                    const double* __restrict__ PHI, int has_phi, double phi_const,
+                                                    const double* __restrict__ UPD, double* __restrict__ JL,
+                                                    double* __restrict__ JR, int* __restrict__ status) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= g.F) return;
+    FaceCtx X;
+    face_base<FLUX, VISC>(g, P, W, G, PHI, has_phi != 0, phi_const, f, X, status);
+    const FaceGeo& F = X.F;
+    const size_t ld = g.ld;
+    const size_t Fs = g.F;
+    const bool interior = F.bc < 0;
+    const double diss = FLUX == FCFD_FLUX_SW ? g.fdiss[f] : 0.0, cwd = FLUX == FCFD_FLUX_SW ? g.fwd[f] : 0.0;
+    for (int var = 0; var < 5; ++var) {
+        double Cp[5], Dp[5];
+        CState<double> Lp = X.L, Rp = X.R;
+        double eps = fd_eps(UPD[var * ld + X.p]);
+        state_perturb(P, Lp, F, var, eps);
+        if (!interior) {   // right_state_update(leftPlus, rightPlus) (:224)
+            Rp.p_center = W[X.n];
+            Rp.V_center = mk<double>(W[ld + X.n], W[2 * ld + X.n], W[3 * ld + X.n]);
+            Rp.T_center = W[4 * ld + X.n];
+            if (apply_bcs(P, g.bc[F.bc], Lp, Rp, F)) atomicOr(status, 1);
+            finish_state(P, Rp, F);
+        }
+        convective_flux<FLUX>(P, Lp, Rp, F, diss, cwd, Cp);
+        if (VISC) {
+            face_state_adjust(P, Lp, Rp, F, X.fs, var);
+            diffusive_flux(X.fs, F, g.bc, Dp);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 5; ++j) Dp[j] = 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 5; ++j) JL[(5 * var + j) * Fs + f] = (Dp[j] - X.D[j] - Cp[j] + X.C[j]) / eps;
+        if (interior) {
+            Rp = X.R;
+            double eps2 = fd_eps(UPD[var * ld + X.n]);
+            state_perturb(P, Rp, F, var, eps2);
+            convective_flux<FLUX>(P, X.L, Rp, F, diss, cwd, Cp);
+            if (VISC) {
+                face_state_adjust(P, X.L, Rp, F, X.fs, var);
+                diffusive_flux(X.fs, F, g.bc, Dp);
+            }
+#pragma unroll
+            for (int j = 0; j < 5; ++j) JR[(5 * var + j) * Fs + f] = (Dp[j] - X.D[j] - Cp[j] + X.C[j]) / eps2;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 5; ++j) JR[(5 * var + j) * Fs + f] = 0.0;
+        }
+        if (VISC) face_state_adjust(P, X.L, X.R, F, X.fs, var);   // (:265)
+    }
+}
+
+// cons2prim (ns_time_terms.cc:26-56)
+__device__ inline void cons2prim(const Params& P, double p, double u, double v, double w, double T, double Pm[5][5]) {
+    double rho = m_rho(P, p, T);
+    double pressure = p + P.Pref, temperature = T + P.Tref;
+    double drho_dT = -1. * rho / temperature, drho_dp = rho / pressure;
+    double c_p = P.Cp;
+    double H = c_p * temperature + 0.5 * (u * u + v * v + w * w);
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) Pm[i][j] = 0.0;
+    Pm[0][0] = drho_dp; Pm[0][4] = drho_dT;
+    Pm[1][0] = drho_dp * u; Pm[1][1] = rho; Pm[1][4] = drho_dT * u;
+    Pm[2][0] = drho_dp * v; Pm[2][2] = rho; Pm[2][4] = drho_dT * v;
+    Pm[3][0] = drho_dp * w; Pm[3][3] = rho; Pm[3][4] = drho_dT * w;
+    Pm[4][0] = drho_dp * H - 1.; Pm[4][1] = rho * u; Pm[4][2] = rho * v; Pm[4][3] = rho * w;
+    Pm[4][4] = drho_dT * H + rho * c_p;
+}
+
+__global__ void __launch_bounds__(256) diag_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ JL,
+                                                   const double* __restrict__ JR, const double* __restrict__ dt,
+                                                   double* __restrict__ D) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld, Fs = g.F;
+    double blk[25];
+#pragma unroll
+    for (int k = 0; k < 25; ++k) blk[k] = 0.0;
+    for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
+        int fs = g.hf_face[h];
+        if (fs >= 0) {
+#pragma unroll
+            for (int var = 0; var < 5; ++var)
+#pragma unroll
+                for (int j = 0; j < 5; ++j) blk[5 * j + var] += -1. * JL[(5 * var + j) * Fs + fs];
+        } else {
+            int f = ~fs;
+#pragma unroll
+            for (int var = 0; var < 5; ++var)
+#pragma unroll
+                for (int j = 0; j < 5; ++j) blk[5 * j + var] += JR[(5 * var + j) * Fs + f];
+        }
+    }
+    if (dt) {
+        double Pm[5][5];
+        cons2prim(P, W[c], W[ld + c], W[2 * ld + c], W[3 * ld + c], W[4 * ld + c], Pm);
+        double vol = g.vol[c], dtc = dt[c];
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) blk[5 * i + j] += Pm[i][j] * vol / dtc;
+    }
+#pragma unroll
+    for (int k = 0; k < 25; ++k) D[k * ld + c] = blk[k];
+}
+
+template <int FLUX, int VISC>
+static void launch_fdjac_t(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, int has_phi,
+                           double phi_const, const double* UPD, double* JL, double* JR, int* status, hipStream_t s) {
+    hipLaunchKernelGGL((fdjac_kernel<FLUX, VISC>), dim3((g.F + 127) / 128), dim3(128), 0, s, g, P, W, G, PHI, has_phi,
+                       phi_const, UPD, JL, JR, status);
+}
+
+template <int FLUX>
+static void launch_fdjac_v(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, int has_phi,
+                           double phi_const, const double* UPD, double* JL, double* JR, int* status, hipStream_t s) {
+    if (P.viscous) launch_fdjac_t<FLUX, 1>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s);
+    else launch_fdjac_t<FLUX, 0>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s);
+}
+
+void launch_fd_jacobian(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI,
+                        const double* UPD, const double* dt, double* JL, double* JR, double* D, int* status, hipStream_t s) {
+    int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    int has_phi = lim != FCFD_LIMITER_NONE;
+    double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
+    switch (P.flux) {
+    case FCFD_FLUX_ROE: launch_fdjac_v<FCFD_FLUX_ROE>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s); break;
+    case FCFD_FLUX_VAN_LEER: launch_fdjac_v<FCFD_FLUX_VAN_LEER>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s); break;
+    case FCFD_FLUX_SD_SLAU: launch_fdjac_v<FCFD_FLUX_SD_SLAU>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s); break;
+    case FCFD_FLUX_SW: launch_fdjac_v<FCFD_FLUX_SW>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s); break;
+    default: launch_fdjac_v<FCFD_FLUX_AUSM_PLUS_UP>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s); break;
+    }
+    hipLaunchKernelGGL(diag_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, P, W, JL, JR, dt, D);
+}
+
+}  // namespace fcfd

@@ -15,19 +15,9 @@
 #include <hip/hip_runtime.h>
 
 #include "fcfd_device.h"
-#include "fcfd_physics.h"
+#include "fcfd_face.h"
 
 namespace fcfd {
-
-__device__ inline void load_geo(const DevGrid& g, int f, FaceGeo& F) {
-    F.n = mk<double>(g.fnx[f], g.fny[f], g.fnz[f]);
-    F.t1 = mk<double>(g.t1x[f], g.t1y[f], g.t1z[f]);
-    F.t2 = vnorm(vcross(F.n, F.t1));
-    F.area = g.farea[f];
-    F.bc = g.fbc[f];
-}
-
-__device__ inline Vec<double> centroid(const DevGrid& g, int c) { return mk<double>(g.ccx[c], g.ccy[c], g.ccz[c]); }
 
 // --------------------------------------------------------------------------
 // 1. boundary ghosts: apply_bcs with the parent's cell-centre state as left and
@@ -213,73 +203,6 @@ __global__ void __launch_bounds__(1024) minmax_kernel(int N, size_t ld, int firs
 // --------------------------------------------------------------------------
 // 3. face fluxes + per-cell ordered accumulation (ns_assemble_linear_system.cc:75-140)
 // --------------------------------------------------------------------------
-__device__ inline void load_cell(const DevGrid& g, const double* __restrict__ W, const double* __restrict__ G,
-                                 const double* __restrict__ PHI, int c, bool has_phi, double phi_const, double q[5],
-                                 Vec<double> gr[5], double ph[5]) {
-    const size_t ld = g.ld;
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        q[v] = W[v * ld + c];
-        gr[v] = mk<double>(G[(3 * v) * ld + c], G[(3 * v + 1) * ld + c], G[(3 * v + 2) * ld + c]);
-        ph[v] = has_phi ? PHI[v * ld + c] : phi_const;
-    }
-}
-
-template <int FLUX, int VISC>
-__device__ inline void face_flux(const DevGrid& g, const Params& P, const double* __restrict__ W, const double* __restrict__ G,
-                                 const double* __restrict__ PHI, bool has_phi, double phi_const, int f, double out[5],
-                                 int* status) {
-    FaceGeo F;
-    load_geo(g, f, F);
-    const int p = g.fpar[f], n = g.fnei[f];
-    Vec<double> xf = mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]);
-    Vec<double> xp = centroid(g, p), xn = centroid(g, n);
-    F.l2r = vsub(xn, xp);
-    double q[5], ph[5];
-    Vec<double> gp[5];
-    load_cell(g, W, G, PHI, p, has_phi, phi_const, q, gp, ph);
-    CState<double> L, R;
-    recon(P, L, q, gp, ph, vsub(xf, xp));
-    finish_state(P, L, F);
-    Vec<double> gn[5];
-    if (F.bc >= 0) {
-        const size_t ld = g.ld;
-        R.p_center = W[n];
-        R.V_center = mk<double>(W[ld + n], W[2 * ld + n], W[3 * ld + n]);
-        R.T_center = W[4 * ld + n];
-        if (apply_bcs(P, g.bc[F.bc], L, R, F)) atomicOr(status, 1);
-    } else {
-        double qn[5], phn[5];
-        load_cell(g, W, G, PHI, n, has_phi, phi_const, qn, gn, phn);
-        recon(P, R, qn, gn, phn, vsub(xf, xn));
-    }
-    finish_state(P, R, F);
-    double C[5];
-    convective_flux<FLUX>(P, L, R, F, VISC >= 0 && FLUX == FCFD_FLUX_SW ? g.fdiss[f] : 0.0,
-                          FLUX == FCFD_FLUX_SW ? g.fwd[f] : 0.0, C);
-    if (VISC) {
-        FState<double> fs;
-        if (F.bc >= 0) {
-            fs.gu = gp[1]; fs.gv = gp[2]; fs.gw = gp[3]; fs.gT = gp[4];
-        } else {   // face average (variable.h:134-147), two-entry stencil in key order
-            double wp = g.fwp[f], wn = g.fwn[f];
-            fs.gu = vadd(vsc(wp, gp[1]), vsc(wn, gn[1]));
-            fs.gv = vadd(vsc(wp, gp[2]), vsc(wn, gn[2]));
-            fs.gw = vadd(vsc(wp, gp[3]), vsc(wn, gn[3]));
-            fs.gT = vadd(vsc(wp, gp[4]), vsc(wn, gn[4]));
-        }
-        face_state(P, L, R, F, fs);
-        double D[5];
-        diffusive_flux(fs, F, g.bc, D);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) out[v] = D[v] - C[v] + 0.;
-    } else {
-        out[0] = 0. - C[0] + 0.;
-#pragma unroll
-        for (int v = 1; v < 5; ++v) out[v] = 0. - C[v] + 0.;
-    }
-}
-
 template <int FLUX, int VISC>
 __global__ void __launch_bounds__(256) flux_kernel(DevGrid g, Params P, const double* __restrict__ W,
                                                    const double* __restrict__ G, const double* __restrict__ PHI,

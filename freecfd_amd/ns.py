@@ -121,5 +121,20 @@ class NavierStokes(Grid):
         _check(self.h, lib().fcfd_cells_to_host(self.h, p, 5, 0, PHI.ctypes.data))
         return G, PHI
 
+    def fd_jacobian(self, W=None, dW_prev=None, dt=None, stream=None):
+        """Reference FD Jacobian (+ time terms if dt is given) on the GPU."""
+        W = self.W if W is None else W
+        s = self.torch.cuda.current_stream().cuda_stream if stream is None else stream
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        _check(self.h, lib().fcfd_fd_jacobian(self.h, W.data_ptr(), ptr(dW_prev), ptr(dt), C.c_void_p(s)))
+
+    def jacobian_host(self):
+        F, N = self.info.n_faces, self.info.n_cells
+        JL = np.zeros((F, 5, 5))
+        JR = np.zeros((F, 5, 5))
+        D = np.zeros((N, 5, 5))
+        _check(self.h, lib().fcfd_jacobian_to_host(self.h, JL.ctypes.data, JR.ctypes.data, D.ctypes.data))
+        return JL, JR, D
+
     def status(self, clear=True):
         _check(self.h, lib().fcfd_status(self.h, int(clear)))

@@ -227,6 +227,22 @@ int  fcfd_profile(fcfd_ctx* ctx, int enable, double* stage_ms, long long* calls)
  * ns_apply_bcs.cc:54-59 -> FCFD_EBC). */
 int  fcfd_status(fcfd_ctx* ctx, int clear);
 
+/* Reference-equivalent FP64 finite-difference Jacobian of the face loop
+ * (get_jacobians + MatSetValues, ns_assemble_linear_system.cc:144-267) and,
+ * when dt_dev is not NULL, the time terms P*V/dt (ns_time_terms.cc:147-163).
+ * Recomputes ghosts, gradients and limiters of W_dev first.  dWprev_dev
+ * ([5][ld_cell], the previous Newton update that sets the FD step,
+ * :217-219) may be NULL (zero update).  The result stays in the context:
+ * per face JL and JR ([25][n_faces], entry (var,eq) at (5*var+eq)*F + f, in
+ * internal face order) and diagonal blocks D ([25][ld_cell], row-major 5x5).
+ * The operator row P, column N block is -JR^T of the face (P,N); row N,
+ * column P is +JL^T. */
+int  fcfd_fd_jacobian(fcfd_ctx* ctx, double* W_dev, const double* dWprev_dev, const double* dt_dev, void* stream);
+int  fcfd_jacobian_buffers(fcfd_ctx* ctx, const double** JL, const double** JR, const double** D);
+/* Copies the last Jacobian to the host in reference numbering: JL, JR as
+ * [n_faces][5 var][5 eq], D as [n_cells][5 row][5 col].  Any may be NULL. */
+int  fcfd_jacobian_to_host(fcfd_ctx* ctx, double* JL_host, double* JR_host, double* D_host);
+
 /* Local time step (update_time_step, src/time_step.cc:143-224) for
  * CFL_LOCAL or CFL_MAX; dt_dev is [ld_cell]; *max_cfl as the reference. */
 int  fcfd_time_step(fcfd_ctx* ctx, const double* W_dev, int dt_type, double cfl, double* dt_dev, double* max_cfl, void* stream);

@@ -870,6 +870,12 @@ int orc_set_state(orc_ctx* o, const double* W) {
     }
     return 0;
 }
+int orc_set_full_state(orc_ctx* o, const double* W) {
+    for (int c = 0; c < o->ntot; ++c) {
+        o->p[c] = W[5 * c]; o->u[c] = W[5 * c + 1]; o->v[c] = W[5 * c + 2]; o->w[c] = W[5 * c + 3]; o->T[c] = W[5 * c + 4];
+    }
+    return 0;
+}
 int orc_set_update(orc_ctx* o, const double* dW) { memcpy(o->upd, dW, sizeof(double) * 5 * (size_t)o->ncell); return 0; }
 
 static V3 cellV(const orc_ctx* o, int c) { return v3(o->u[c], o->v[c], o->w[c]); }
@@ -1657,5 +1663,39 @@ int orc_time_terms(orc_ctx* o, const double* dt, double* blocks) {
         P[4][4] = drho_dT * H + rho * c_p;
         for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) blocks[25 * c + 5 * i + j] = P[i][j] * o->vol[c] / dt[c];
     }
+    return 0;
+}
+
+/* probe: convective flux between two primitive states across a face with
+ * normal n and tangent t1 (t2 = norm(n x t1)), as convective_face_flux. */
+int orc_flux_probe(orc_ctx* o, int flux_type, const double* WL, const double* WR, const double* n, const double* t1,
+                   double area, double* out) {
+    fstate F;
+    memset(&F, 0, sizeof F);
+    F.normal = v3(n[0], n[1], n[2]);
+    F.tangent1 = v3(t1[0], t1[1], t1[2]);
+    F.tangent2 = vnorm(vcross(F.normal, F.tangent1));
+    F.area = area;
+    F.index = 0;
+    F.bc = -1;
+    cstate L, R;
+    const double* src[2] = {WL, WR};
+    cstate* dst[2] = {&L, &R};
+    for (int k = 0; k < 2; ++k) {
+        cstate* S = dst[k];
+        S->p = S->p_center = src[k][0];
+        S->V = S->V_center = v3(src[k][1], src[k][2], src[k][3]);
+        S->T = S->T_center = src[k][4];
+        S->rho = m_rho(o, S->p, S->T);
+        finish_state(o, S, &F);
+    }
+    int saved = o->opt.convective_flux;
+    o->opt.convective_flux = flux_type;
+    double diss0 = o->fdiss ? o->fdiss[0] : 0.;
+    double wd0 = o->fwd ? o->fwd[0] : 0.;
+    if (o->fdiss) { o->fdiss[0] = 1.; o->fwd[0] = 1.e20; }
+    convective_face_flux(o, &L, &R, &F, out);
+    if (o->fdiss) { o->fdiss[0] = diss0; o->fwd[0] = wd0; }
+    o->opt.convective_flux = saved;
     return 0;
 }

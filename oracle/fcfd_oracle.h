@@ -29,6 +29,9 @@ int orc_metrics(const orc_ctx* o, fcfd_metrics_out* m);
 /* state W: N x 5 (p,u,v,w,T), reference numbering */
 int orc_set_state(orc_ctx* o, const double* W);
 int orc_set_update(orc_ctx* o, const double* dW);   /* previous Newton update (update[i].cell) */
+int orc_set_full_state(orc_ctx* o, const double* W);  /* (N+B) x 5 incl. ghosts */
+int orc_flux_probe(orc_ctx* o, int flux_type, const double* WL, const double* WR, const double* n, const double* t1,
+                   double area, double* out);
 int orc_update_boundaries(orc_ctx* o);
 int orc_calc_cell_grads(orc_ctx* o);
 int orc_calc_limiter(orc_ctx* o);

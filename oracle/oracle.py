@@ -31,7 +31,7 @@ def lib():
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_last_error.restype = C.c_char_p
         L.orc_last_error.argtypes = [C.c_void_p]
-        for name in ("orc_set_state", "orc_set_update"):
+        for name in ("orc_set_state", "orc_set_update", "orc_set_full_state"):
             getattr(L, name).argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         for name in ("orc_update_boundaries", "orc_calc_cell_grads", "orc_calc_limiter"):
             getattr(L, name).argtypes = [C.c_void_p]
@@ -44,6 +44,8 @@ def lib():
         L.orc_get_loads.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_time_step.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.POINTER(C.c_double)]
         L.orc_time_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_flux_probe.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_double, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -94,6 +96,25 @@ class Oracle:
         if dW is not None:
             dW = np.ascontiguousarray(dW, dtype=np.float64)
             self._check(lib().orc_set_update(self.h, dW.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def set_full_state(self, W):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        self._check(lib().orc_set_full_state(self.h, W.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def grads(self):
+        self._check(lib().orc_calc_cell_grads(self.h))
+        return self.fields()[1]
+
+    def limiter(self):
+        self._check(lib().orc_calc_limiter(self.h))
+        return self.fields()[2]
+
+    def flux_probe(self, flux_type, WL, WR, n, t1, area=1.0):
+        out = np.zeros(5)
+        arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (WL, WR, n, t1)]
+        self._check(lib().orc_flux_probe(self.h, flux_type, _p(arr[0]), _p(arr[1]), _p(arr[2]), _p(arr[3]),
+                                         float(area), _p(out)))
+        return out
 
     def prepare(self):
         """update_boundaries -> calc_cell_grads -> calc_limiter (ns.cc:112-116)."""

@@ -1,0 +1,65 @@
+"""GPU parity of the reference-equivalent FD Jacobian (get_jacobians +
+assembly) against the CPU restatement.
+
+The FD quotient divides flux differences by the step (1.5e-7 absolute for a
+zero previous update), so one-ulp differences of the underlying fluxes (pow,
+sqrt implementations) are amplified by ~|F|/eps.  Entries are compared to
+1e-9 of the largest entry of the same 5x5 block, which the bitwise-identical
+operation order (no FMA contraction on either side) keeps far inside.
+"""
+import numpy as np
+import pytest
+
+from freecfd_amd import _abi as A
+from freecfd_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _blockwise_err(a, b):
+    scale = np.abs(b).reshape(len(b), -1).max(axis=1) + 1e-300
+    return (np.abs(a - b).reshape(len(b), -1).max(axis=1) / scale).max()
+
+
+def _compare(cs, dW=None, label=""):
+    import torch
+    from oracle.oracle import Oracle
+    from freecfd_amd.ns import NavierStokes
+    print("case", cs.name, label, flush=True)
+    o = Oracle(cs.raw, cs.opt)
+    o.residual(cs.W0, with_jacobian=True, dW=dW)
+    JLo, JRo, Do = o.jacobian()
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    dWd = None
+    if dW is not None:
+        dWd = torch.zeros_like(ns.W)
+        from freecfd_amd._lib import lib
+        lib().fcfd_state_to_device(ns.h, np.ascontiguousarray(dW).ctypes.data, dWd.data_ptr())
+    ns.fd_jacobian(dW_prev=dWd)
+    ns.status()
+    JLg, JRg, Dg = ns.jacobian_host()
+    e = (_blockwise_err(JLg, JLo), _blockwise_err(JRg[np.abs(JRo).reshape(len(JRo), -1).max(1) > 0],
+                                                  JRo[np.abs(JRo).reshape(len(JRo), -1).max(1) > 0]),
+         _blockwise_err(Dg, Do))
+    print("  JL %.2e  JR %.2e  D %.2e" % e, flush=True)
+    assert max(e) <= 1e-9, e
+
+
+def test_fdjac_wedge_roe():
+    _compare(cases.wedge_case(24, 12), label="roe")
+
+
+def test_fdjac_wedge_prev_update():
+    cs = cases.wedge_case(16, 8, flux=A.FLUX_AUSM_PLUS_UP)
+    rng = np.random.default_rng(3)
+    dW = rng.normal(size=cs.W0.shape) * np.array([50.0, 1.0, 1.0, 0.1, 0.2])
+    _compare(cs, dW=dW, label="ausm, nonzero previous update")
+
+
+def test_fdjac_cylinder_viscous():
+    _compare(cases.cylinder_case(32, 16, noise=0.01), label="viscous ausm")
+
+
+def test_fdjac_plate_viscous_prisms():
+    _compare(cases.plate_case(12, 8, 3, noise=0.01, first=1e-3), label="viscous roe prisms")
