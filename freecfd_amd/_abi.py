@@ -61,6 +61,28 @@ class Info(C.Structure):
                 ("n_faceavg", C.c_int), ("n_nbr", C.c_int), ("n_halffaces", C.c_int), ("nv", C.c_int)]
 
 
+JAC_FD, JAC_JVP_FROZEN, JAC_JVP_FULL = 1, 2, 3
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [("dt_type", C.c_int), ("cfl", C.c_double), ("jacobian", C.c_int), ("krylov_fp32", C.c_int),
+                ("restart", C.c_int), ("max_iters", C.c_int), ("rtol", C.c_double), ("abstol", C.c_double),
+                ("knoll_guess", C.c_int)]
+
+
+class NewtonStats(C.Structure):
+    _fields_ = [("res_norm", C.c_double), ("update_norms", C.c_double * 3), ("krylov_iters", C.c_int),
+                ("krylov_converged", C.c_int), ("krylov_rnorm", C.c_double), ("krylov_r0norm", C.c_double),
+                ("ms_residual", C.c_double), ("ms_jacobian", C.c_double), ("ms_krylov", C.c_double),
+                ("ms_update", C.c_double)]
+
+
+def newton_opts(cfl=10.0, jacobian=JAC_FD, fp32=False, restart=30, max_iters=10, rtol=1e-6, abstol=1e-12, knoll=True):
+    """Defaults follow the reference (read_inputs.cc:125-127, ns_petsc_functions.cc:74-79)."""
+    return NewtonOpts(dt_type=DT_CFL_LOCAL, cfl=cfl, jacobian=jacobian, krylov_fp32=int(fp32), restart=restart,
+                      max_iters=max_iters, rtol=rtol, abstol=abstol, knoll_guess=int(knoll))
+
+
 class MetricsOut(C.Structure):
     _fields_ = [("face_parent", _ip), ("face_neighbor", _ip), ("face_bc", _ip),
                 ("face_node_ptr", _ip), ("face_nodes", _ip),

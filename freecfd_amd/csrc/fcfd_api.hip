@@ -12,6 +12,9 @@
 #include "fcfd_device.h"
 #include "fcfd_internal.h"
 #include "fcfd_jacobian.h"
+#include "fcfd_krylov.h"
+
+#include <chrono>
 
 using namespace fcfd;
 
@@ -26,6 +29,13 @@ struct fcfd_ctx {
     double *G = nullptr, *PHI = nullptr, *dref = nullptr;
     int* status = nullptr;
     double *JL = nullptr, *JR = nullptr, *Dg = nullptr, *zero_upd = nullptr;   // FD Jacobian (lazy)
+    // Newton-Krylov workspace (lazy)
+    double *R = nullptr, *dt = nullptr, *dW = nullptr, *upd = nullptr, *red_part = nullptr, *red_out = nullptr;
+    int gm_restart = 0, gm_fp32 = -1;
+    GmresWork<double> w64;
+    GmresWork<float> w32;
+    double* Dinv64 = nullptr;
+    float *Dinv32 = nullptr, *D32 = nullptr, *JL32 = nullptr, *JR32 = nullptr, *b32 = nullptr, *x32 = nullptr;
     std::string err;
     // optional per-stage timing: a ring of event quadruples recorded on the
     // launch stream around the three kernels, folded in lazily so that
@@ -496,6 +506,108 @@ int fcfd_jacobian_to_host(fcfd_ctx* c, double* JL_host, double* JR_host, double*
             for (int e = 0; e < 25; ++e) D_host[(size_t)25 * r + e] = tmp[(size_t)e * ld + i];
         }
     }
+    return FCFD_OK;
+}
+
+int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd_newton_stats* st, void* stream) {
+    if (!c || !W_dev || !o) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    if (o->dt_type != FCFD_DT_CFL_LOCAL) return fail(c, FCFD_ENOTSUP, "only CFL_LOCAL time stepping is implemented");
+    if (o->jacobian != FCFD_JAC_FD) return fail(c, FCFD_ENOTSUP, "matrix-free tangent operator not available in this build");
+    if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad Krylov options");
+    const DevGrid& g = c->dg;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n5 = (size_t)5 * g.ld;
+    int rc;
+    if (!c->R) {
+        if ((rc = dalloc(c, n5, &c->R)) || (rc = dalloc(c, (size_t)g.ld, &c->dt)) || (rc = dalloc(c, n5, &c->dW)) ||
+            (rc = dalloc(c, n5, &c->upd)) || (rc = dalloc(c, (size_t)512 * 8, &c->red_part)) || (rc = dalloc(c, 64, &c->red_out)))
+            return rc;
+    }
+    if (!c->JL) {
+        if ((rc = dalloc(c, (size_t)25 * g.F, &c->JL)) || (rc = dalloc(c, (size_t)25 * g.F, &c->JR)) ||
+            (rc = dalloc(c, (size_t)25 * g.ld, &c->Dg)) || (rc = dalloc(c, n5, &c->zero_upd)))
+            return rc;
+    }
+    const int fp32 = o->krylov_fp32 ? 1 : 0;
+    if (c->gm_restart != o->restart || c->gm_fp32 != fp32) {   // (re)allocate the Krylov basis
+        size_t m1 = (size_t)o->restart + 1;
+        if (fp32) {
+            if ((rc = dalloc(c, m1 * n5, &c->w32.V)) || (rc = dalloc(c, n5, &c->w32.z)) || (rc = dalloc(c, n5, &c->w32.r)) ||
+                (rc = dalloc(c, m1 * 512, &c->w32.part)) || (rc = dalloc(c, m1, &c->w32.out_dev)) || (rc = dalloc(c, m1, &c->w32.coef_dev)))
+                return rc;
+            if (!c->Dinv32) {
+                if ((rc = dalloc(c, (size_t)25 * g.ld, &c->Dinv32)) || (rc = dalloc(c, (size_t)25 * g.ld, &c->D32)) ||
+                    (rc = dalloc(c, (size_t)25 * g.F, &c->JL32)) || (rc = dalloc(c, (size_t)25 * g.F, &c->JR32)) ||
+                    (rc = dalloc(c, n5, &c->b32)) || (rc = dalloc(c, n5, &c->x32)))
+                    return rc;
+            }
+        } else {
+            if ((rc = dalloc(c, m1 * n5, &c->w64.V)) || (rc = dalloc(c, n5, &c->w64.z)) || (rc = dalloc(c, n5, &c->w64.r)) ||
+                (rc = dalloc(c, m1 * 512, &c->w64.part)) || (rc = dalloc(c, m1, &c->w64.out_dev)) || (rc = dalloc(c, m1, &c->w64.coef_dev)))
+                return rc;
+            if (!c->Dinv64 && (rc = dalloc(c, (size_t)25 * g.ld, &c->Dinv64))) return rc;
+        }
+        c->gm_restart = o->restart;
+        c->gm_fp32 = fp32;
+    }
+    using clk = std::chrono::steady_clock;
+    auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    fcfd_newton_stats S{};
+    auto t0 = clk::now();
+    // 1. residual and its norm
+    launch_ghosts(g, c->P, W_dev, c->status, s);
+    launch_grad_lim(g, c->P, W_dev, c->G, c->PHI, c->dref, s);
+    launch_flux(g, c->P, W_dev, c->G, c->PHI, c->R, c->status, s);
+    double rr = 0.0;
+    HIPCHK(c, dev_mdot(g, 1, c->R, n5, c->R, c->red_part, c->red_out, &rr, s));
+    S.res_norm = std::sqrt(rr);
+    auto t1 = clk::now();
+    // 2. local time step, FD Jacobian + time terms, block-Jacobi inverse
+    launch_dt_local(g, c->P, W_dev, o->cfl, c->dt, s);
+    launch_fd_jacobian(g, c->P, W_dev, c->G, c->PHI, c->upd, c->dt, c->JL, c->JR, c->Dg, c->status, s);
+    if (fp32) {
+        dev_bj_invert<float>(g, c->Dg, c->Dinv32, c->status, s);
+        dev_convert<double, float>((size_t)25 * g.ld, c->Dg, c->D32, s);
+        dev_convert<double, float>((size_t)25 * g.F, c->JL, c->JL32, s);
+        dev_convert<double, float>((size_t)25 * g.F, c->JR, c->JR32, s);
+        dev_convert<double, float>(n5, c->R, c->b32, s);
+    } else {
+        dev_bj_invert<double>(g, c->Dg, c->Dinv64, c->status, s);
+    }
+    HIPCHK(c, hipStreamSynchronize(s));
+    auto t2 = clk::now();
+    // 3. GMRES
+    GmresOpts go;
+    go.restart = o->restart; go.max_iters = o->max_iters; go.rtol = o->rtol; go.abstol = o->abstol; go.knoll = o->knoll_guess;
+    GmresStats gs;
+    if (fp32) {
+        LinOp<float> A{[&](const float* x, float* y, hipStream_t ss) { dev_bsr_matvec<float>(g, c->D32, c->JL32, c->JR32, x, y, ss); }};
+        LinOp<float> M{[&](const float* x, float* y, hipStream_t ss) { dev_bj_apply<float>(g, c->Dinv32, x, y, ss); }};
+        rc = gmres<float>(g, A, M, c->b32, c->x32, go, c->w32, gs, s);
+        if (!rc) dev_convert<float, double>(n5, c->x32, c->dW, s);
+    } else {
+        LinOp<double> A{[&](const double* x, double* y, hipStream_t ss) { dev_bsr_matvec<double>(g, c->Dg, c->JL, c->JR, x, y, ss); }};
+        LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) { dev_bj_apply<double>(g, c->Dinv64, x, y, ss); }};
+        rc = gmres<double>(g, A, M, c->R, c->dW, go, c->w64, gs, s);
+    }
+    if (rc) return fail(c, rc, "GMRES failed (HIP error)");
+    HIPCHK(c, hipStreamSynchronize(s));
+    auto t3 = clk::now();
+    // 4. update W, norms, NaN check; keep dW as the previous update
+    double out[4];
+    HIPCHK(c, dev_update(g, W_dev, c->dW, c->dt, c->red_part, c->red_out, out, s));
+    HIPCHK(c, hipMemcpyAsync(c->upd, c->dW, n5 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    auto t4 = clk::now();
+    S.update_norms[0] = std::sqrt(out[0]); S.update_norms[1] = std::sqrt(out[1]); S.update_norms[2] = std::sqrt(out[2]);
+    S.krylov_iters = gs.iters; S.krylov_converged = gs.converged; S.krylov_rnorm = gs.rnorm; S.krylov_r0norm = gs.r0norm;
+    S.ms_residual = ms(t0, t1); S.ms_jacobian = ms(t1, t2); S.ms_krylov = ms(t2, t3); S.ms_update = ms(t3, t4);
+    if (st) *st = S;
+    if (out[3] > 0.0) return fail(c, FCFD_EDIVERGED, "Divergence detected: NaN/Inf in the update (ns.cc:350-355)");
+    int bad = 0;
+    HIPCHK(c, hipMemcpy(&bad, c->status, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad & 2) return fail(c, FCFD_EDIVERGED, "singular diagonal block in the block-Jacobi preconditioner");
     return FCFD_OK;
 }
 

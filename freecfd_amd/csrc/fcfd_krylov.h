@@ -1,0 +1,46 @@
+// Krylov solver and implicit-operator building blocks (fcfd_krylov.hip).
+#pragma once
+#include <functional>
+
+#include "fcfd_device.h"
+
+namespace fcfd {
+
+template <typename T>
+struct LinOp {   // y = Op(x) on [5][ld] device vectors, stream-ordered
+    std::function<void(const T*, T*, hipStream_t)> apply;
+};
+
+struct GmresOpts {
+    int restart = 30, max_iters = 30, knoll = 1;
+    double rtol = 1e-6, abstol = 1e-12;
+};
+struct GmresStats {
+    int iters = 0, converged = 0;
+    double rnorm = 0.0, r0norm = 0.0, bnorm = 0.0;
+};
+template <typename T>
+struct GmresWork {
+    T *V = nullptr, *z = nullptr, *r = nullptr;   // (restart+1) x 5 ld, 5 ld, 5 ld
+    double *part = nullptr, *out_dev = nullptr, *coef_dev = nullptr;
+};
+
+template <typename T> void dev_axpy(size_t n, T a, const T* x, T* y, hipStream_t s);
+template <typename T> void dev_scale_copy(size_t n, T a, const T* x, T* y, hipStream_t s);
+template <typename A, typename B> void dev_convert(size_t n, const A* x, B* y, hipStream_t s);
+template <typename T> void dev_xmy(size_t n, const T* b, T* y, hipStream_t s);
+template <typename T>
+hipError_t dev_mdot(const DevGrid& g, int k, const T* V, size_t stride, const T* w, double* part, double* out_dev,
+                    double* out_host, hipStream_t s);
+template <typename T> void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s);
+template <typename T> void dev_bj_invert(const DevGrid& g, const double* D, T* Dinv, int* status, hipStream_t s);
+template <typename T> void dev_bj_apply(const DevGrid& g, const T* Dinv, const T* x, T* y, hipStream_t s);
+hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
+                      double out_host[4], hipStream_t s);
+template <typename T>
+int gmres(const DevGrid& g, const LinOp<T>& A, const LinOp<T>& M, const T* b, T* x, const GmresOpts& o, GmresWork<T>& w,
+          GmresStats& st, hipStream_t s);
+
+constexpr int kRedPartials = 512 * 8;   // size of GmresWork::part per multi-dot row budget (k <= 8 rows at once)
+
+}  // namespace fcfd

@@ -1,0 +1,412 @@
+// Linear algebra of the implicit step on the GPU: operator matvec with the
+// per-face Jacobian blocks, batched 5x5 block-Jacobi, deterministic
+// reductions and a restarted right-preconditioned GMRES, in FP64 or FP32.
+//
+// Replaces the reference's PETSc MPIAIJ matrix + FGMRES(300) KSP
+// (ns_petsc_functions.cc:25-110).  The operator is the reference's implicit
+// operator A = P V/dt - dR/dW stored as: diagonal blocks D ([25][ld]) and per
+// face JL, JR ([25][F]); row P, col N block = -JR^T, row N, col P = +JL^T.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "fcfd_krylov.h"
+
+namespace fcfd {
+
+constexpr int kRedBlocks = 512;   // fixed -> deterministic reductions
+constexpr int kRedThreads = 256;
+
+// ---------------------------------------------------------------------------
+// element-wise kernels on [5][ld] vectors (only the first N entries of each
+// variable row are meaningful; the loops cover 5*ld for simplicity)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void axpy_kernel(size_t n, T a, const T* __restrict__ x, T* __restrict__ y) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] += a * x[i];
+}
+template <typename T>
+__global__ void scale_copy_kernel(size_t n, T a, const T* __restrict__ x, T* __restrict__ y) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = a * x[i];
+}
+template <typename A, typename B>
+__global__ void convert_kernel(size_t n, const A* __restrict__ x, B* __restrict__ y) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = (B)x[i];
+}
+template <typename T>
+__global__ void xmy_kernel(size_t n, const T* __restrict__ b, T* __restrict__ y) {   // y = b - y
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = b[i] - y[i];
+}
+// y -= sum_k h[k] * V_k  (V_k at V + k*stride)
+template <typename T>
+__global__ void mgs_update_kernel(size_t n, int k, const T* __restrict__ V, size_t stride, const double* __restrict__ h,
+                                  T* __restrict__ y) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        T acc = y[i];
+        for (int q = 0; q < k; ++q) acc -= (T)h[q] * V[q * stride + i];
+        y[i] = acc;
+    }
+}
+// x += sum_k c[k] * V_k
+template <typename T>
+__global__ void lincomb_kernel(size_t n, int k, const T* __restrict__ V, size_t stride, const double* __restrict__ c,
+                               T* __restrict__ x) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        T acc = x[i];
+        for (int q = 0; q < k; ++q) acc += (T)c[q] * V[q * stride + i];
+        x[i] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// deterministic multi-dot: out[q] = <V_q, w> for q < k over interior cells,
+// accumulated in FP64; partials [k][kRedBlocks] then a fixed-order tree
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) mdot_partial_kernel(int N, size_t ld, int k, const T* __restrict__ V, size_t stride,
+                                                                   const T* __restrict__ w, double* __restrict__ part) {
+    __shared__ double sh[kRedThreads];
+    const int q = blockIdx.y;
+    const T* v = V + q * stride;
+    double acc = 0.0;
+    for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < N; i += kRedBlocks * kRedThreads) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) acc += (double)v[r * ld + i] * (double)w[r * ld + i];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[q * kRedBlocks + blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(kRedThreads) reduce_final_kernel(const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double sh[kRedThreads];
+    const int q = blockIdx.x;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) acc += part[q * kRedBlocks + i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = sh[0];
+}
+
+// Newton update W += dW over interior cells, with the reference's update norms
+// (ns.cc:349-381): sums of dp^2/dt^2, |dV|^2/dt^2, dT^2/dt^2, plus a NaN/Inf
+// count and ||dW||^2 -- partial sums per block, fixed-order tree.
+__global__ void __launch_bounds__(kRedThreads) update_kernel(int N, size_t ld, double* __restrict__ W, const double* __restrict__ dW,
+                                                             const double* __restrict__ dt, double* __restrict__ part) {
+    __shared__ double sh[4][kRedThreads];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = blockIdx.x * kRedThreads + threadIdx.x; c < N; c += kRedBlocks * kRedThreads) {
+        double d[5];
+        bool bad = false;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) { d[v] = dW[v * ld + c]; bad |= !isfinite(d[v]); }
+        if (bad) { acc[3] += 1.0; continue; }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) W[v * ld + c] += d[v];
+        double dt2 = dt[c] * dt[c];
+        acc[0] += d[0] * d[0] / dt2;
+        acc[1] += d[1] * d[1] / dt2 + d[2] * d[2] / dt2 + d[3] * d[3] / dt2;
+        acc[2] += d[4] * d[4] / dt2;
+    }
+    for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) part[q * kRedBlocks + blockIdx.x] = sh[q][0];
+}
+
+// ---------------------------------------------------------------------------
+// operator: y = D x + sum_faces (-JR^T x_N | +JL^T x_P)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __restrict__ D, const T* __restrict__ JL,
+                                                         const T* __restrict__ JR, const T* __restrict__ x, T* __restrict__ y) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld, Fs = g.F;
+    T xc[5], acc[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) xc[i] = x[i * ld + c];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        T s = 0;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) s += D[(5 * j + i) * ld + c] * xc[i];
+        acc[j] = s;
+    }
+    for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
+        int fs = g.hf_face[h];
+        int f = fs >= 0 ? fs : ~fs;
+        int o = fs >= 0 ? g.fnei[f] : g.fpar[f];
+        if (o >= g.N) continue;   // boundary ghost: no column (ghost coupling is in JL)
+        T xo[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) xo[i] = x[i * ld + o];
+        const T* J = fs >= 0 ? JR : JL;
+        const T sg = fs >= 0 ? T(-1) : T(1);
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) acc[j] += sg * J[(5 * i + j) * Fs + f] * xo[i];
+    }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) y[j * ld + c] = acc[j];
+}
+
+// batched 5x5 inverse by Gauss-Jordan with partial pivoting (block-Jacobi)
+template <typename T>
+__global__ void __launch_bounds__(256) bj_invert_kernel(int N, size_t ld, const double* __restrict__ D, T* __restrict__ Dinv,
+                                                        int* __restrict__ status) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    T a[5][10];
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) { a[i][j] = (T)D[(5 * i + j) * ld + c]; a[i][5 + j] = (i == j) ? T(1) : T(0); }
+    bool singular = false;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        int piv = k;
+        T best = fabs(a[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < 5; ++i) if (fabs(a[i][k]) > best) { best = fabs(a[i][k]); piv = i; }
+        if (best == T(0)) { singular = true; break; }
+        if (piv != k) {
+#pragma unroll
+            for (int j = 0; j < 10; ++j) { T t = a[k][j]; a[k][j] = a[piv][j]; a[piv][j] = t; }
+        }
+        T inv = T(1) / a[k][k];
+#pragma unroll
+        for (int j = 0; j < 10; ++j) a[k][j] *= inv;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            if (i == k) continue;
+            T m = a[i][k];
+#pragma unroll
+            for (int j = 0; j < 10; ++j) a[i][j] -= m * a[k][j];
+        }
+    }
+    if (singular) atomicOr(status, 2);
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) Dinv[(5 * i + j) * ld + c] = singular ? ((i == j) ? T(1) : T(0)) : a[i][5 + j];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) bj_apply_kernel(int N, size_t ld, const T* __restrict__ Dinv, const T* __restrict__ x,
+                                                       T* __restrict__ y) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    T xc[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) xc[i] = x[i * ld + c];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        T s = 0;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) s += Dinv[(5 * j + i) * ld + c] * xc[i];
+        y[j * ld + c] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static inline dim3 ew_grid(size_t n) { return dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)); }
+
+template <typename T> void dev_axpy(size_t n, T a, const T* x, T* y, hipStream_t s) {
+    hipLaunchKernelGGL(axpy_kernel<T>, ew_grid(n), dim3(256), 0, s, n, a, x, y);
+}
+template <typename T> void dev_scale_copy(size_t n, T a, const T* x, T* y, hipStream_t s) {
+    hipLaunchKernelGGL(scale_copy_kernel<T>, ew_grid(n), dim3(256), 0, s, n, a, x, y);
+}
+template <typename A, typename B> void dev_convert(size_t n, const A* x, B* y, hipStream_t s) {
+    hipLaunchKernelGGL((convert_kernel<A, B>), ew_grid(n), dim3(256), 0, s, n, x, y);
+}
+template <typename T> void dev_xmy(size_t n, const T* b, T* y, hipStream_t s) {
+    hipLaunchKernelGGL(xmy_kernel<T>, ew_grid(n), dim3(256), 0, s, n, b, y);
+}
+
+template void dev_axpy<double>(size_t, double, const double*, double*, hipStream_t);
+template void dev_axpy<float>(size_t, float, const float*, float*, hipStream_t);
+template void dev_scale_copy<double>(size_t, double, const double*, double*, hipStream_t);
+template void dev_scale_copy<float>(size_t, float, const float*, float*, hipStream_t);
+template void dev_convert<double, float>(size_t, const double*, float*, hipStream_t);
+template void dev_convert<float, double>(size_t, const float*, double*, hipStream_t);
+template void dev_convert<double, double>(size_t, const double*, double*, hipStream_t);
+template void dev_xmy<double>(size_t, const double*, double*, hipStream_t);
+template void dev_xmy<float>(size_t, const float*, float*, hipStream_t);
+
+template <typename T>
+hipError_t dev_mdot(const DevGrid& g, int k, const T* V, size_t stride, const T* w, double* part, double* out_dev,
+                    double* out_host, hipStream_t s) {
+    hipLaunchKernelGGL(mdot_partial_kernel<T>, dim3(kRedBlocks, k), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, k, V, stride, w, part);
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(k), dim3(kRedThreads), 0, s, part, out_dev);
+    hipError_t e = hipMemcpyAsync(out_host, out_dev, k * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+template hipError_t dev_mdot<double>(const DevGrid&, int, const double*, size_t, const double*, double*, double*, double*, hipStream_t);
+template hipError_t dev_mdot<float>(const DevGrid&, int, const float*, size_t, const float*, double*, double*, double*, hipStream_t);
+
+template <typename T>
+void dev_mgs_update(const DevGrid& g, int k, const T* V, size_t stride, const double* h_dev, T* y, hipStream_t s) {
+    size_t n = (size_t)5 * g.ld;
+    hipLaunchKernelGGL(mgs_update_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, h_dev, y);
+}
+template <typename T>
+void dev_lincomb(const DevGrid& g, int k, const T* V, size_t stride, const double* c_dev, T* x, hipStream_t s) {
+    size_t n = (size_t)5 * g.ld;
+    hipLaunchKernelGGL(lincomb_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, c_dev, x);
+}
+template void dev_mgs_update<double>(const DevGrid&, int, const double*, size_t, const double*, double*, hipStream_t);
+template void dev_mgs_update<float>(const DevGrid&, int, const float*, size_t, const double*, float*, hipStream_t);
+template void dev_lincomb<double>(const DevGrid&, int, const double*, size_t, const double*, double*, hipStream_t);
+template void dev_lincomb<float>(const DevGrid&, int, const float*, size_t, const double*, float*, hipStream_t);
+
+template <typename T>
+void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s) {
+    hipLaunchKernelGGL(bsr_matvec_kernel<T>, dim3((g.N + 255) / 256), dim3(256), 0, s, g, D, JL, JR, x, y);
+}
+template void dev_bsr_matvec<double>(const DevGrid&, const double*, const double*, const double*, const double*, double*, hipStream_t);
+template void dev_bsr_matvec<float>(const DevGrid&, const float*, const float*, const float*, const float*, float*, hipStream_t);
+
+template <typename T>
+void dev_bj_invert(const DevGrid& g, const double* D, T* Dinv, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(bj_invert_kernel<T>, dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
+}
+template <typename T>
+void dev_bj_apply(const DevGrid& g, const T* Dinv, const T* x, T* y, hipStream_t s) {
+    hipLaunchKernelGGL(bj_apply_kernel<T>, dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
+}
+template void dev_bj_invert<double>(const DevGrid&, const double*, double*, int*, hipStream_t);
+template void dev_bj_invert<float>(const DevGrid&, const double*, float*, int*, hipStream_t);
+template void dev_bj_apply<double>(const DevGrid&, const double*, const double*, double*, hipStream_t);
+template void dev_bj_apply<float>(const DevGrid&, const float*, const float*, float*, hipStream_t);
+
+hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
+                      double out_host[4], hipStream_t s) {
+    hipLaunchKernelGGL(update_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, W, dW, dt, part);
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(4), dim3(kRedThreads), 0, s, part, out_dev);
+    hipError_t e = hipMemcpyAsync(out_host, out_dev, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+// ---------------------------------------------------------------------------
+// restarted GMRES, right preconditioned: A M^{-1} u = b, x = M^{-1} u
+// ---------------------------------------------------------------------------
+template <typename T>
+int gmres(const DevGrid& g, const LinOp<T>& A, const LinOp<T>& M, const T* b, T* x, const GmresOpts& o, GmresWork<T>& w,
+          GmresStats& st, hipStream_t s) {
+    const size_t n = (size_t)5 * g.ld;
+    const int m = o.restart;
+    const size_t stride = n;
+    T* V = w.V;       // (m+1) vectors
+    T* z = w.z;
+    T* r = w.r;
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), gv(m + 1, 0.0), h(m + 1, 0.0), y(m, 0.0);
+    double bnorm2 = 0.0;
+    if (dev_mdot(g, 1, b, stride, b, w.part, w.out_dev, &bnorm2, s) != hipSuccess) return FCFD_EHIP;
+    const double bnorm = std::sqrt(bnorm2);
+    // initial guess: Knoll (x0 = M^{-1} b) or zero
+    if (o.knoll) M.apply(b, x, s);
+    else hipMemsetAsync(x, 0, n * sizeof(T), s);
+    int it = 0;
+    double rnorm = bnorm, r0norm = -1.0;
+    st.converged = 0;
+    while (it < o.max_iters) {
+        // r = b - A x
+        A.apply(x, r, s);
+        dev_xmy(n, b, r, s);
+        double rr = 0.0;
+        if (dev_mdot(g, 1, r, stride, r, w.part, w.out_dev, &rr, s) != hipSuccess) return FCFD_EHIP;
+        rnorm = std::sqrt(rr);
+        if (r0norm < 0.0) r0norm = rnorm;
+        double target = std::max(o.rtol * r0norm, o.abstol);
+        if (rnorm <= target || rnorm == 0.0) { st.converged = 1; break; }
+        dev_scale_copy(n, (T)(1.0 / rnorm), r, V, s);
+        std::fill(gv.begin(), gv.end(), 0.0);
+        gv[0] = rnorm;
+        int j = 0;
+        for (; j < m && it < o.max_iters; ++j, ++it) {
+            T* vj = V + (size_t)j * stride;
+            T* vn = V + (size_t)(j + 1) * stride;
+            M.apply(vj, z, s);
+            A.apply(z, vn, s);
+            // classical Gram-Schmidt, twice (CGS2): one multi-dot per pass
+            for (int pass = 0; pass < 2; ++pass) {
+                if (dev_mdot(g, j + 1, V, stride, vn, w.part, w.out_dev, h.data(), s) != hipSuccess) return FCFD_EHIP;
+                if (hipMemcpyAsync(w.coef_dev, h.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+                    hipStreamSynchronize(s) != hipSuccess)
+                    return FCFD_EHIP;
+                dev_mgs_update(g, j + 1, V, stride, w.coef_dev, vn, s);
+                for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] += h[q];
+            }
+            double hn2 = 0.0;
+            if (dev_mdot(g, 1, vn, stride, vn, w.part, w.out_dev, &hn2, s) != hipSuccess) return FCFD_EHIP;
+            double hn = std::sqrt(hn2);
+            H[(size_t)(j + 1) * m + j] = hn;
+            if (hn > 0.0) dev_scale_copy(n, (T)(1.0 / hn), vn, vn, s);
+            // Givens rotations
+            for (int q = 0; q < j; ++q) {
+                double a = H[(size_t)q * m + j], bq = H[(size_t)(q + 1) * m + j];
+                H[(size_t)q * m + j] = cs[q] * a + sn[q] * bq;
+                H[(size_t)(q + 1) * m + j] = -sn[q] * a + cs[q] * bq;
+            }
+            double a = H[(size_t)j * m + j], bq = H[(size_t)(j + 1) * m + j];
+            double den = std::sqrt(a * a + bq * bq);
+            cs[j] = den > 0.0 ? a / den : 1.0;
+            sn[j] = den > 0.0 ? bq / den : 0.0;
+            H[(size_t)j * m + j] = den;
+            H[(size_t)(j + 1) * m + j] = 0.0;
+            gv[j + 1] = -sn[j] * gv[j];
+            gv[j] = cs[j] * gv[j];
+            rnorm = std::fabs(gv[j + 1]);
+            if (rnorm <= std::max(o.rtol * r0norm, o.abstol)) { ++j; ++it; st.converged = 1; break; }
+        }
+        // solve H y = g (upper triangular j x j), x += M^{-1} (V y)
+        for (int q = j - 1; q >= 0; --q) {
+            double acc = gv[q];
+            for (int k = q + 1; k < j; ++k) acc -= H[(size_t)q * m + k] * y[k];
+            y[q] = H[(size_t)q * m + q] != 0.0 ? acc / H[(size_t)q * m + q] : 0.0;
+        }
+        if (hipMemcpyAsync(w.coef_dev, y.data(), j * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return FCFD_EHIP;
+        hipMemsetAsync(r, 0, n * sizeof(T), s);
+        dev_lincomb(g, j, V, stride, w.coef_dev, r, s);
+        M.apply(r, z, s);
+        dev_axpy(n, (T)1, z, x, s);
+        std::fill(H.begin(), H.end(), 0.0);
+        if (st.converged) break;
+    }
+    st.iters = it;
+    st.rnorm = rnorm;
+    st.r0norm = r0norm < 0 ? bnorm : r0norm;
+    st.bnorm = bnorm;
+    hipError_t e = hipStreamSynchronize(s);
+    return e == hipSuccess ? FCFD_OK : FCFD_EHIP;
+}
+
+template int gmres<double>(const DevGrid&, const LinOp<double>&, const LinOp<double>&, const double*, double*, const GmresOpts&,
+                           GmresWork<double>&, GmresStats&, hipStream_t);
+template int gmres<float>(const DevGrid&, const LinOp<float>&, const LinOp<float>&, const float*, float*, const GmresOpts&,
+                          GmresWork<float>&, GmresStats&, hipStream_t);
+
+}  // namespace fcfd

@@ -136,5 +136,24 @@ class NavierStokes(Grid):
         _check(self.h, lib().fcfd_jacobian_to_host(self.h, JL.ctypes.data, JR.ctypes.data, D.ctypes.data))
         return JL, JR, D
 
+    def newton_step(self, opts=None, W=None, stream=None, check=True):
+        """One implicit step (NavierStokes::solve, ns.cc:104-118); returns NewtonStats."""
+        opts = A.newton_opts() if opts is None else opts
+        W = self.W if W is None else W
+        st = A.NewtonStats()
+        s = self.torch.cuda.current_stream().cuda_stream if stream is None else stream
+        rc = lib().fcfd_newton_step(self.h, W.data_ptr(), C.byref(opts), C.byref(st), C.c_void_p(s))
+        if check:
+            _check(self.h, rc)
+        return st if check else (rc, st)
+
+    def time_step_host(self, cfl):
+        dt = self.torch.zeros(self.ld, dtype=self.torch.float64, device=self.device)
+        mc = C.c_double()
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_time_step(self.h, self.W.data_ptr(), A.DT_CFL_LOCAL, cfl, dt.data_ptr(), C.byref(mc),
+                                           C.c_void_p(s)))
+        return self.to_host(dt.reshape(1, -1), 1)[:, 0]
+
     def status(self, clear=True):
         _check(self.h, lib().fcfd_status(self.h, int(clear)))

@@ -243,6 +243,38 @@ int  fcfd_jacobian_buffers(fcfd_ctx* ctx, const double** JL, const double** JR, 
  * [n_faces][5 var][5 eq], D as [n_cells][5 row][5 col].  Any may be NULL. */
 int  fcfd_jacobian_to_host(fcfd_ctx* ctx, double* JL_host, double* JR_host, double* D_host);
 
+/* ---- implicit Newton-Krylov step ---- */
+#define FCFD_JAC_FD          1   /* reference FD Jacobian, assembled per face        */
+#define FCFD_JAC_JVP_FROZEN  2   /* matrix-free exact tangent, gradients/limiters frozen */
+#define FCFD_JAC_JVP_FULL    3   /* matrix-free exact tangent of the full residual    */
+
+typedef struct fcfd_newton_opts {
+    int dt_type;          /* FCFD_DT_CFL_LOCAL (update_time_step, time_step.cc:193-200) */
+    double cfl;
+    int jacobian;         /* FCFD_JAC_*                                                  */
+    int krylov_fp32;      /* 0: FP64 GMRES + FP64 block-Jacobi; 1: FP32 for both         */
+    int restart;          /* GMRES restart length (the reference uses FGMRES(300))       */
+    int max_iters;        /* maximumiterations (default 10, read_inputs.cc:127)          */
+    double rtol, abstol;  /* relativetolerance / absolutetolerance (read_inputs.cc:125-126) */
+    int knoll_guess;      /* initial guess = M^{-1} b (KSPSetInitialGuessKnoll, ns_petsc_functions.cc:75) */
+} fcfd_newton_opts;
+
+typedef struct fcfd_newton_stats {
+    double res_norm;          /* ||R(W)||_2 over interior cells and the 5 equations, before the update */
+    double update_norms[3];   /* sqrt(sum dp^2/dt^2), sqrt(sum |dV|^2/dt^2), sqrt(sum dT^2/dt^2) (ns.cc:377-394) */
+    int krylov_iters, krylov_converged;
+    double krylov_rnorm, krylov_r0norm;
+    double ms_residual, ms_jacobian, ms_krylov, ms_update;
+} fcfd_newton_stats;
+
+/* One implicit step (NavierStokes::solve without the gradient refresh,
+ * ns.cc:104-118): residual, local time step, Jacobian (or matrix-free
+ * tangent), time terms, GMRES with block-Jacobi, W += dW with the NaN/Inf
+ * check (FCFD_EDIVERGED, ns.cc:350-355).  The update is kept in the context
+ * as the previous update that sets the FD step of the next call.  W_dev is
+ * [5][ld_cell], updated in place. */
+int  fcfd_newton_step(fcfd_ctx* ctx, double* W_dev, const fcfd_newton_opts* opts, fcfd_newton_stats* stats, void* stream);
+
 /* Local time step (update_time_step, src/time_step.cc:143-224) for
  * CFL_LOCAL or CFL_MAX; dt_dev is [ld_cell]; *max_cfl as the reference. */
 int  fcfd_time_step(fcfd_ctx* ctx, const double* W_dev, int dt_type, double cfl, double* dt_dev, double* max_cfl, void* stream);

@@ -2,10 +2,17 @@
 assembly) against the CPU restatement.
 
 The FD quotient divides flux differences by the step (1.5e-7 absolute for a
-zero previous update), so one-ulp differences of the underlying fluxes (pow,
-sqrt implementations) are amplified by ~|F|/eps.  Entries are compared to
-1e-9 of the largest entry of the same 5x5 block, which the bitwise-identical
-operation order (no FMA contraction on either side) keeps far inside.
+zero previous update), so one-ulp differences of the underlying fluxes are
+amplified by ~|F|/eps.  Both sides use the same operation order with no FMA
+contraction, and IEEE-exact +,-,*,/,sqrt on both, so for constant-property
+gases the blocks agree to 1e-9 of the largest entry of each 5x5 block.
+pow() is not correctly rounded on either side and differs by an ulp between
+the device library and glibc: with Sutherland's law mu(T) = ...pow(T/Tref,1.5)
+perturbed by eps/T ~ 5e-10, one ulp (1.1e-16) becomes ~2e-7 of the viscous
+derivative, so those cases are held to 1e-6.  The same holds for faces on
+outlets with a given pressure, whose entropy extrapolation uses
+pow(p, 1/gamma) (ns_apply_bcs.cc:168-174) and is re-applied on every left
+perturbation.
 """
 import numpy as np
 import pytest
@@ -21,7 +28,7 @@ def _blockwise_err(a, b):
     return (np.abs(a - b).reshape(len(b), -1).max(axis=1) / scale).max()
 
 
-def _compare(cs, dW=None, label=""):
+def _compare(cs, dW=None, label="", tol=1e-9):
     import torch
     from oracle.oracle import Oracle
     from freecfd_amd.ns import NavierStokes
@@ -43,7 +50,7 @@ def _compare(cs, dW=None, label=""):
                                                   JRo[np.abs(JRo).reshape(len(JRo), -1).max(1) > 0]),
          _blockwise_err(Dg, Do))
     print("  JL %.2e  JR %.2e  D %.2e" % e, flush=True)
-    assert max(e) <= 1e-9, e
+    assert max(e) <= tol, e
 
 
 def test_fdjac_wedge_roe():
@@ -58,8 +65,8 @@ def test_fdjac_wedge_prev_update():
 
 
 def test_fdjac_cylinder_viscous():
-    _compare(cases.cylinder_case(32, 16, noise=0.01), label="viscous ausm")
+    _compare(cases.cylinder_case(32, 16, noise=0.01), label="viscous ausm, Sutherland", tol=1e-6)
 
 
 def test_fdjac_plate_viscous_prisms():
-    _compare(cases.plate_case(12, 8, 3, noise=0.01, first=1e-3), label="viscous roe prisms")
+    _compare(cases.plate_case(12, 8, 3, noise=0.01, first=1e-3), label="viscous roe prisms, p-outlets", tol=1e-6)
