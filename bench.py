@@ -49,15 +49,17 @@ WORKLOADS = {
 
 
 def model_bytes(info, n_grad, has_phi, sw=False):
-    """Algorithmic (compulsory) HBM bytes per launch of each kernel."""
+    """Algorithmic (compulsory) HBM bytes per launch of each kernel: every
+    array element the kernel needs, read or written once."""
     N, B, F, H = info.n_cells, info.n_ghosts, info.n_faces, info.n_halffaces
-    face_flux = F * (12 + 24 + 24 + 8 + 24 + 16 + (16 if sw else 0))   # topology+bc, n, t1, area, centroid, weights
-    flux = (face_flux + H * 4 + (N + 1) * 4 + (N + B) * 5 * 8 + N * 15 * 8 + (N * 5 * 8 if has_phi else 0)
-            + (N + B) * 24 + N * 5 * 8)
+    face_arrays = F * (12 + 24 + 24 + 8 + 24 + 16 + (16 if sw else 0))   # parent/neighbour/bc, n, t1, area, centroid, weights
+    face_flux = (face_arrays + (N + B) * 5 * 8 + N * 15 * 8 + (N * 5 * 8 if has_phi else 0) + (N + B) * 24
+                 + F * 5 * 8)                                              # + write D-C per face
+    cell_sum = H * 4 + (N + 1) * 4 + F * 5 * 8 + N * 5 * 8                 # half-faces, read D-C, write R
     grad = ((N + B) * 5 * 8 + n_grad * (4 + 24) + (N + 1) * 4 + H * 4 + F * (8 + 24) + N * 24
             + (N + B) * 15 * 8 + (N * 5 * 8 if has_phi else 0))
     ghost = B * (4 + 4 + 4 + 24 + 24 + 8 + 5 * 8 + 5 * 8)
-    return {"ghost": ghost, "grad_lim": grad, "flux": flux}
+    return {"ghost": ghost, "grad_lim": grad, "face_flux": face_flux, "cell_sum": cell_sum}
 
 
 def cpu_baseline(name, target_s=12.0):
@@ -119,7 +121,7 @@ def main():
         ns.assemble_residual()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    stage = (C.c_double * 3)()
+    stage = (C.c_double * 4)()
     calls = C.c_longlong()
     L.fcfd_profile(ns.h, 0, stage, C.byref(calls))
     ns.status()
@@ -129,8 +131,8 @@ def main():
 
     ms_step = 1e3 * elapsed / args.steps
     evals_s = args.steps / elapsed
-    names = ["ghost", "grad_lim", "flux"]
-    avg_ms = {names[k]: stage[k] / max(calls.value, 1) for k in range(3)}
+    names = ["ghost", "grad_lim", "face_flux", "cell_sum"]
+    avg_ms = {names[k]: stage[k] / max(calls.value, 1) for k in range(4)}
     has_phi = cs.opt.limiter != A.LIMITER_NONE and cs.opt.order == A.ORDER_SECOND
     mb = model_bytes(info, info.n_gradmap, has_phi, sw=cs.opt.convective_flux == A.FLUX_SW)
     dom = max(avg_ms, key=lambda k: avg_ms[k])

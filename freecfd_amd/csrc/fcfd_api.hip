@@ -26,7 +26,7 @@ struct fcfd_ctx {
     bool device = false;
     DevGrid dg{};
     std::vector<void*> allocs;
-    double *G = nullptr, *PHI = nullptr, *dref = nullptr;
+    double *G = nullptr, *PHI = nullptr, *dref = nullptr, *FX = nullptr;
     int* status = nullptr;
     double *JL = nullptr, *JR = nullptr, *Dg = nullptr, *zero_upd = nullptr;   // FD Jacobian (lazy)
     // Newton-Krylov workspace (lazy)
@@ -42,9 +42,10 @@ struct fcfd_ctx {
     // timing adds no host synchronisation to the call sequence
     static const int kRing = 512;
     bool profiling = false;
-    std::vector<hipEvent_t> ev;       // 4 * kRing
+    static const int kStages = 4;     // ghosts, gradients+limiters, face fluxes, cell sums
+    std::vector<hipEvent_t> ev;       // (kStages+1) * kRing
     int ring_head = 0, ring_count = 0;
-    double stage_ms[3] = {0.0, 0.0, 0.0};
+    double stage_ms[kStages] = {0.0, 0.0, 0.0, 0.0};
     long long stage_calls = 0;
 };
 
@@ -247,7 +248,8 @@ int build_device(fcfd_ctx* c) {
         int rc;
         if ((rc = upload(c, c->bch, &d.bc))) return rc;
         if ((rc = dalloc(c, (size_t)15 * ld, &c->G)) || (rc = dalloc(c, (size_t)5 * ld, &c->PHI)) ||
-            (rc = dalloc(c, 8, &c->dref)) || (rc = dalloc(c, 4, &c->status)))
+            (rc = dalloc(c, (size_t)kDrefSize, &c->dref)) || (rc = dalloc(c, 4, &c->status)) ||
+            (rc = dalloc(c, (size_t)5 * g.F, &c->FX)))
             return rc;
     }
     c->device = true;
@@ -257,9 +259,9 @@ int build_device(fcfd_ctx* c) {
 thread_local std::string g_create_err;
 
 int fold_slot(fcfd_ctx* c, int slot) {
-    hipEvent_t* e = &c->ev[4 * slot];
-    HIPCHK(c, hipEventSynchronize(e[3]));
-    for (int k = 0; k < 3; ++k) {
+    hipEvent_t* e = &c->ev[(fcfd_ctx::kStages + 1) * slot];
+    HIPCHK(c, hipEventSynchronize(e[fcfd_ctx::kStages]));
+    for (int k = 0; k < fcfd_ctx::kStages; ++k) {
         float ms = 0.f;
         HIPCHK(c, hipEventElapsedTime(&ms, e[k], e[k + 1]));
         c->stage_ms[k] += ms;
@@ -390,16 +392,18 @@ int fcfd_residual(fcfd_ctx* c, double* W_dev, double* R_dev, void* stream) {
             if (rc) return rc;
             c->ring_count--;
         }
-        e = &c->ev[4 * c->ring_head];
+        e = &c->ev[(fcfd_ctx::kStages + 1) * c->ring_head];
         HIPCHK(c, hipEventRecord(e[0], s));
     }
     launch_ghosts(c->dg, c->P, W_dev, c->status, s);
     if (e) HIPCHK(c, hipEventRecord(e[1], s));
     launch_grad_lim(c->dg, c->P, W_dev, c->G, c->PHI, c->dref, s);
     if (e) HIPCHK(c, hipEventRecord(e[2], s));
-    launch_flux(c->dg, c->P, W_dev, c->G, c->PHI, R_dev, c->status, s);
+    launch_flux(c->dg, c->P, W_dev, c->G, c->PHI, c->FX, R_dev, c->status, s);
+    if (e) HIPCHK(c, hipEventRecord(e[3], s));
+    launch_cell_sum(c->dg, c->FX, R_dev, s);
     if (e) {
-        HIPCHK(c, hipEventRecord(e[3], s));
+        HIPCHK(c, hipEventRecord(e[4], s));
         c->ring_head = (c->ring_head + 1) % fcfd_ctx::kRing;
         c->ring_count++;
     }
@@ -415,15 +419,15 @@ int fcfd_profile(fcfd_ctx* c, int enable, double* stage_ms, long long* calls) {
         if (rc) return rc;
         c->ring_count--;
     }
-    if (stage_ms) for (int k = 0; k < 3; ++k) stage_ms[k] = c->stage_ms[k];
+    if (stage_ms) for (int k = 0; k < fcfd_ctx::kStages; ++k) stage_ms[k] = c->stage_ms[k];
     if (calls) *calls = c->stage_calls;
     if (enable >= 0) {
         if (enable && c->ev.empty()) {
-            c->ev.resize(4 * fcfd_ctx::kRing, nullptr);
+            c->ev.resize((fcfd_ctx::kStages + 1) * fcfd_ctx::kRing, nullptr);
             for (auto& ev : c->ev) HIPCHK(c, hipEventCreate(&ev));
         }
         c->profiling = enable != 0;
-        for (int k = 0; k < 3; ++k) c->stage_ms[k] = 0.0;
+        for (int k = 0; k < fcfd_ctx::kStages; ++k) c->stage_ms[k] = 0.0;
         c->stage_calls = 0;
         c->ring_head = 0;
     }
@@ -558,7 +562,8 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     // 1. residual and its norm
     launch_ghosts(g, c->P, W_dev, c->status, s);
     launch_grad_lim(g, c->P, W_dev, c->G, c->PHI, c->dref, s);
-    launch_flux(g, c->P, W_dev, c->G, c->PHI, c->R, c->status, s);
+    launch_flux(g, c->P, W_dev, c->G, c->PHI, c->FX, c->R, c->status, s);
+    launch_cell_sum(g, c->FX, c->R, s);
     double rr = 0.0;
     HIPCHK(c, dev_mdot(g, 1, c->R, n5, c->R, c->red_part, c->red_out, &rr, s));
     S.res_norm = std::sqrt(rr);

@@ -29,8 +29,12 @@ struct DevGrid {
 
 void launch_ghosts(const DevGrid& g, const Params& P, double* W, int* status, hipStream_t s);
 void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s);
-void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* R,
-                 int* status, hipStream_t s);
+// face fluxes D - C per face into FX ([5][F]) ...
+void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* FX,
+                 double* R, int* status, hipStream_t s);
+// ... and their ordered per-cell sums into R
+void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s);
+constexpr int kDrefSize = 8 + 8 * 1024;   // dref[5] + VK min/max partials
 void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s);
 
 }  // namespace fcfd

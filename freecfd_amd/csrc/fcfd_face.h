@@ -9,7 +9,7 @@
 
 namespace fcfd {
 
-__device__ inline void load_geo(const DevGrid& g, int f, FaceGeo& F) {
+__device__ __forceinline__ void load_geo(const DevGrid& g, int f, FaceGeo& F) {
     F.n = mk<double>(g.fnx[f], g.fny[f], g.fnz[f]);
     F.t1 = mk<double>(g.t1x[f], g.t1y[f], g.t1z[f]);
     F.t2 = vnorm(vcross(F.n, F.t1));
@@ -17,9 +17,9 @@ __device__ inline void load_geo(const DevGrid& g, int f, FaceGeo& F) {
     F.bc = g.fbc[f];
 }
 
-__device__ inline Vec<double> centroid(const DevGrid& g, int c) { return mk<double>(g.ccx[c], g.ccy[c], g.ccz[c]); }
+__device__ __forceinline__ Vec<double> centroid(const DevGrid& g, int c) { return mk<double>(g.ccx[c], g.ccy[c], g.ccz[c]); }
 
-__device__ inline void load_cell(const DevGrid& g, const double* __restrict__ W, const double* __restrict__ G,
+__device__ __forceinline__ void load_cell(const DevGrid& g, const double* __restrict__ W, const double* __restrict__ G,
                                  const double* __restrict__ PHI, int c, bool has_phi, double phi_const, double q[5],
                                  Vec<double> gr[5], double ph[5]) {
     const size_t ld = g.ld;
@@ -41,7 +41,7 @@ struct FaceCtx {
 };
 
 template <int FLUX, int VISC>
-__device__ inline void face_base(const DevGrid& g, const Params& P, const double* __restrict__ W, const double* __restrict__ G,
+__device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, const double* __restrict__ W, const double* __restrict__ G,
                                  const double* __restrict__ PHI, bool has_phi, double phi_const, int f, FaceCtx& X,
                                  int* status) {
     FaceGeo& F = X.F;
@@ -57,16 +57,23 @@ __device__ inline void face_base(const DevGrid& g, const Params& P, const double
     recon(P, X.L, q, gp, ph, vsub(xf, xp));
     finish_state(P, X.L, F);
     Vec<double> gn[5];
-    if (F.bc >= 0) {   // right_state_update for a boundary face (:298-303)
-        const size_t ld = g.ld;
-        X.R.p_center = W[n];
-        X.R.V_center = mk<double>(W[ld + n], W[2 * ld + n], W[3 * ld + n]);
-        X.R.T_center = W[4 * ld + n];
-        if (apply_bcs(P, g.bc[F.bc], X.L, X.R, F)) atomicOr(status, 1);
-    } else {
+    // right state: boundary (right_state_update :298-303, apply_bcs) or
+    // interior reconstruction; both branches produce a whole state value
+    {
         double qn[5], phn[5];
-        load_cell(g, W, G, PHI, n, has_phi, phi_const, qn, gn, phn);
-        recon(P, X.R, qn, gn, phn, vsub(xf, xn));
+        const bool bnd = F.bc >= 0;
+        load_cell(g, W, G, PHI, n, has_phi && !bnd, bnd ? 0.0 : phi_const, qn, gn, phn);
+        CState<double> Rb;
+        if (bnd) {
+            Rb = X.L;
+            Rb.p_center = qn[0];
+            Rb.V_center = mk<double>(qn[1], qn[2], qn[3]);
+            Rb.T_center = qn[4];
+            if (apply_bcs(P, g.bc[F.bc], X.L, Rb, F)) atomicOr(status, 1);
+        } else {
+            recon(P, Rb, qn, gn, phn, vsub(xf, xn));
+        }
+        X.R = Rb;
     }
     finish_state(P, X.R, F);
     convective_flux<FLUX>(P, X.L, X.R, F, FLUX == FCFD_FLUX_SW ? g.fdiss[f] : 0.0, FLUX == FCFD_FLUX_SW ? g.fwd[f] : 0.0, X.C);
@@ -89,7 +96,7 @@ __device__ inline void face_base(const DevGrid& g, const Params& P, const double
 }
 
 template <int FLUX, int VISC>
-__device__ inline void face_flux(const DevGrid& g, const Params& P, const double* __restrict__ W, const double* __restrict__ G,
+__device__ __forceinline__ void face_flux(const DevGrid& g, const Params& P, const double* __restrict__ W, const double* __restrict__ G,
                                  const double* __restrict__ PHI, bool has_phi, double phi_const, int f, double out[5],
                                  int* status) {
     FaceCtx X;

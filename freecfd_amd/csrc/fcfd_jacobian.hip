@@ -21,7 +21,7 @@
 namespace fcfd {
 
 // state_perturb (:417-458)
-__device__ inline void state_perturb(const Params& P, CState<double>& s, const FaceGeo& F, int var, double eps) {
+__device__ __forceinline__ void state_perturb(const Params& P, CState<double>& s, const FaceGeo& F, int var, double eps) {
     switch (var) {
     case 0: s.p += eps; s.rho = m_rho(P, s.p, s.T); s.a = m_a(P, s.p, s.T); break;
     case 1: s.V.x += eps; s.V_center.x += eps; s.Vn = mk<double>(vdot(s.V, F.n), vdot(s.V, F.t1), vdot(s.V, F.t2)); break;
@@ -33,7 +33,7 @@ __device__ inline void state_perturb(const Params& P, CState<double>& s, const F
 }
 
 // face_state_adjust (:460-500); case 1 falls through into case 2
-__device__ inline void face_state_adjust(const Params& P, const CState<double>& L, const CState<double>& R, const FaceGeo& F,
+__device__ __forceinline__ void face_state_adjust(const Params& P, const CState<double>& L, const CState<double>& R, const FaceGeo& F,
                                          FState<double>& f, int var) {
     Vec<double> n = vnorm(F.l2r);
     double m = vmag(F.l2r);
@@ -55,7 +55,7 @@ __device__ inline void face_state_adjust(const Params& P, const CState<double>& 
     }
 }
 
-__device__ inline double fd_eps(double upd) {   // (:48, :217-219)
+__device__ __forceinline__ double fd_eps(double upd) {   // (:48, :217-219)
     const double small_number = 10. * 1.4901161193847656e-08;   // 10*sqrt(DBL_EPSILON)
     const double factor = 0.001;
     if (upd > small_number) return smax(small_number, factor * upd);
@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(128) fdjac_kernel(DevGrid g, Params P, const d
 }
 
 // cons2prim (ns_time_terms.cc:26-56)
-__device__ inline void cons2prim(const Params& P, double p, double u, double v, double w, double T, double Pm[5][5]) {
+__device__ __forceinline__ void cons2prim(const Params& P, double p, double u, double v, double w, double T, double Pm[5][5]) {
     double rho = m_rho(P, p, T);
     double pressure = p + P.Pref, temperature = T + P.Tref;
     double drho_dT = -1. * rho / temperature, drho_dp = rho / pressure;

@@ -11,25 +11,25 @@
 namespace fcfd {
 
 // ---- scalar helpers ---------------------------------------------------------
-__host__ __device__ inline double val(double x) { return x; }
-__device__ inline double sq_root(double x) { return sqrt(x); }
-__device__ inline double powr(double x, double y) { return pow(x, y); }
-__device__ inline double absv(double x) { return fabs(x); }
-template <typename S> __device__ inline S smin(const S& a, const S& b) { return (val(b) < val(a)) ? b : a; }   // std::min
-template <typename S> __device__ inline S smax(const S& a, const S& b) { return (val(a) < val(b)) ? b : a; }   // std::max
+__host__ __device__ __forceinline__ double val(double x) { return x; }
+__device__ __forceinline__ double sq_root(double x) { return sqrt(x); }
+__device__ __forceinline__ double powr(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ double absv(double x) { return fabs(x); }
+template <typename S> __device__ __forceinline__ S smin(const S& a, const S& b) { return (val(b) < val(a)) ? b : a; }   // std::min
+template <typename S> __device__ __forceinline__ S smax(const S& a, const S& b) { return (val(a) < val(b)) ? b : a; }   // std::max
 
 template <typename S> struct Vec { S x, y, z; };
-template <typename S> __device__ inline Vec<S> mk(S a, S b, S c) { Vec<S> r; r.x = a; r.y = b; r.z = c; return r; }
-template <typename S> __device__ inline Vec<S> vadd(const Vec<S>& a, const Vec<S>& b) { return mk<S>(a.x + b.x, a.y + b.y, a.z + b.z); }
-template <typename S> __device__ inline Vec<S> vsub(const Vec<S>& a, const Vec<S>& b) { return mk<S>(a.x - b.x, a.y - b.y, a.z - b.z); }
-template <typename S, typename T> __device__ inline Vec<S> vsc(const T& s, const Vec<S>& a) { return mk<S>(s * a.x, s * a.y, s * a.z); }
-template <typename S, typename T> __device__ inline S vdot(const Vec<S>& a, const Vec<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-template <typename S> __device__ inline S vmag(const Vec<S>& a) { return sq_root(a.x * a.x + a.y * a.y + a.z * a.z); }
-template <typename S> __device__ inline Vec<S> vnorm(const Vec<S>& a) { S m = vmag(a); return mk<S>(a.x / m, a.y / m, a.z / m); }
-template <typename S> __device__ inline Vec<S> vcross(const Vec<S>& a, const Vec<S>& b) {
+template <typename S> __device__ __forceinline__ Vec<S> mk(S a, S b, S c) { Vec<S> r; r.x = a; r.y = b; r.z = c; return r; }
+template <typename S> __device__ __forceinline__ Vec<S> vadd(const Vec<S>& a, const Vec<S>& b) { return mk<S>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <typename S> __device__ __forceinline__ Vec<S> vsub(const Vec<S>& a, const Vec<S>& b) { return mk<S>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <typename S, typename T> __device__ __forceinline__ Vec<S> vsc(const T& s, const Vec<S>& a) { return mk<S>(s * a.x, s * a.y, s * a.z); }
+template <typename S, typename T> __device__ __forceinline__ S vdot(const Vec<S>& a, const Vec<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename S> __device__ __forceinline__ S vmag(const Vec<S>& a) { return sq_root(a.x * a.x + a.y * a.y + a.z * a.z); }
+template <typename S> __device__ __forceinline__ Vec<S> vnorm(const Vec<S>& a) { S m = vmag(a); return mk<S>(a.x / m, a.y / m, a.z / m); }
+template <typename S> __device__ __forceinline__ Vec<S> vcross(const Vec<S>& a, const Vec<S>& b) {
     return mk<S>(a.y * b.z - a.z * b.y, -a.x * b.z + a.z * b.x, a.x * b.y - a.y * b.x);
 }
-template <typename S> __device__ inline Vec<S> lift(const Vec<double>& a) { return mk<S>(S(a.x), S(a.y), S(a.z)); }
+template <typename S> __device__ __forceinline__ Vec<S> lift(const Vec<double>& a) { return mk<S>(S(a.x), S(a.y), S(a.z)); }
 
 // ---- face geometry (double only) -----------------------------------------
 struct FaceGeo {
@@ -39,16 +39,16 @@ struct FaceGeo {
 };
 
 // ---- material (src/material/material.cc:108-153) ---------------------------
-template <typename S> __device__ inline S m_rho(const Params& P, const S& p, const S& T) { return (p + P.Pref) / (P.R * (T + P.Tref)); }
-template <typename S> __device__ inline S m_p(const Params& P, const S& rho, const S& T) { return rho * P.R * (T + P.Tref) - P.Pref; }
-template <typename S> __device__ inline S m_T(const Params& P, const S& p, const S& rho) { return (p + P.Pref) / (P.R * rho) - P.Tref; }
-template <typename S> __device__ inline S m_a(const Params& P, const S& p, const S& T) { return sq_root(P.gamma * (p + P.Pref) / m_rho(P, p, T)); }
-template <typename S> __device__ inline S m_visc(const Params& P, const S& T) {
+template <typename S> __device__ __forceinline__ S m_rho(const Params& P, const S& p, const S& T) { return (p + P.Pref) / (P.R * (T + P.Tref)); }
+template <typename S> __device__ __forceinline__ S m_p(const Params& P, const S& rho, const S& T) { return rho * P.R * (T + P.Tref) - P.Pref; }
+template <typename S> __device__ __forceinline__ S m_T(const Params& P, const S& p, const S& rho) { return (p + P.Pref) / (P.R * rho) - P.Tref; }
+template <typename S> __device__ __forceinline__ S m_a(const Params& P, const S& p, const S& T) { return sq_root(P.gamma * (p + P.Pref) / m_rho(P, p, T)); }
+template <typename S> __device__ __forceinline__ S m_visc(const Params& P, const S& T) {
     if (P.visc_model == FCFD_VISC_SUTHERLAND)
         return P.sut_mu_ref * powr((T + P.Tref) / P.sut_T_ref, 1.5) * (P.sut_T_ref + P.sut_S) / (T + P.Tref + P.sut_S);
     return S(P.mu);
 }
-template <typename S> __device__ inline S m_cond(const Params& P, const S& T) {
+template <typename S> __device__ __forceinline__ S m_cond(const Params& P, const S& T) {
     if (P.cond_model == FCFD_COND_PRANDTL) return P.Cp * m_visc(P, T) / P.Pr;
     return S(P.lambda);
 }
@@ -63,7 +63,7 @@ template <typename S> struct FState {
     Vec<S> V, gu, gv, gw, gT;
 };
 
-template <typename S> __device__ inline void finish_state(const Params& P, CState<S>& s, const FaceGeo& F) {
+template <typename S> __device__ __forceinline__ void finish_state(const Params& P, CState<S>& s, const FaceGeo& F) {
     s.a = m_a(P, s.p, s.T);
     s.H = s.a * s.a / (P.gamma - 1.) + 0.5 * vdot(s.V, s.V);
     s.Vn = mk<S>(vdot(s.V, F.n), vdot(s.V, F.t1), vdot(s.V, F.t2));
@@ -71,7 +71,7 @@ template <typename S> __device__ inline void finish_state(const Params& P, CStat
 
 // MUSCL reconstruction (ns_assemble_linear_system.cc:269-294,305-320):
 // q_f = q_c + phi * (grad q . (x_f - x_c))
-template <typename S> __device__ inline void recon(const Params& P, CState<S>& s, const S q[5], const Vec<S> g[5],
+template <typename S> __device__ __forceinline__ void recon(const Params& P, CState<S>& s, const S q[5], const Vec<S> g[5],
                                                    const S phi[5], const Vec<double>& c2f) {
     s.p_center = q[0];
     s.p = s.p_center + phi[0] * vdot(g[0], c2f);
@@ -84,122 +84,129 @@ template <typename S> __device__ inline void recon(const Params& P, CState<S>& s
 
 // Boundary conditions (src/navier_stokes/ns_apply_bcs.cc:25-258).  Returns a
 // nonzero flag where the reference would exit (supersonic velocity inlet
-// without a complete state).
-template <typename S> __device__ inline int apply_bcs(const Params& P, const BcDev& bc, const CState<S>& L,
-                                                      CState<S>& R, const FaceGeo& F) {
+// without a complete state).  The right state is carried in SSA locals and
+// written back whole, so that no branch stores to an individual field of the
+// state struct (divergent field stores would pin the struct in scratch).
+template <typename S> __device__ __forceinline__ int apply_bcs(const Params& P, const BcDev& bc, const CState<S>& L,
+                                                      CState<S>& Rout, const FaceGeo& F) {
     const double g = P.gamma, Pref = P.Pref, Tref = P.Tref;
     int flag = 0;
+    const Vec<S> n = lift<S>(F.n);
+    S p = Rout.p, pc = Rout.p_center, T = Rout.T, Tc = Rout.T_center, rho = Rout.rho, a = Rout.a;
+    Vec<S> V = Rout.V, Vc = Rout.V_center;
     if (bc.type == FCFD_BC_INLET) {
         if (bc.kind == FCFD_KIND_VELOCITY) {
-            R.V = mk<S>(S(bc.V[0]), S(bc.V[1]), S(bc.V[2]));
+            V = mk<S>(S(bc.V[0]), S(bc.V[1]), S(bc.V[2]));
             S uNL = vdot(L.V, F.n);
-            S uNR = vdot(R.V, F.n);
-            R.a = L.a + 0.5 * (uNL - uNR) * (g - 1.);
+            S uNR = vdot(V, F.n);
+            a = L.a + 0.5 * (uNL - uNR) * (g - 1.);
             S MachL = -uNL / L.a;
             if (val(MachL) >= 1. && bc.specified != FCFD_SPEC_STATE) flag = 1;
-            if (bc.specified == FCFD_SPEC_STATE) { R.p = S(bc.p); R.T = S(bc.T); R.rho = S(bc.rho); }
-            else if (bc.specified == FCFD_SPEC_P) { R.p = S(bc.p); R.rho = g * (R.p + Pref) / (R.a * R.a); R.T = m_T(P, R.p, R.rho); }
-            else if (bc.specified == FCFD_SPEC_T) { R.T = S(bc.T); R.rho = L.rho * powr((L.T + Tref) / (R.T + Tref), 1. / (1. - g)); R.p = m_p(P, R.rho, R.T); }
-            else { R.rho = S(bc.rho); R.p = R.a * R.a * R.rho / g - Pref; R.T = m_T(P, R.p, R.rho); }
-            R.p_center = R.p; R.V_center = R.V; R.T_center = R.T;
+            if (bc.specified == FCFD_SPEC_STATE) { p = S(bc.p); T = S(bc.T); rho = S(bc.rho); }
+            else if (bc.specified == FCFD_SPEC_P) { p = S(bc.p); rho = g * (p + Pref) / (a * a); T = m_T(P, p, rho); }
+            else if (bc.specified == FCFD_SPEC_T) { T = S(bc.T); rho = L.rho * powr((L.T + Tref) / (T + Tref), 1. / (1. - g)); p = m_p(P, rho, T); }
+            else { rho = S(bc.rho); p = a * a * rho / g - Pref; T = m_T(P, p, rho); }
+            pc = p; Vc = V; Tc = T;
         } else if (bc.kind == FCFD_KIND_MDOT) {
             S uNL = vdot(L.V, F.n);
             double mdotNR = -1. * bc.mdot / bc.total_area;
-            if (bc.specified == FCFD_SPEC_STATE) { R.p = S(bc.p); R.T = S(bc.T); R.rho = S(bc.rho); }
-            else if (bc.specified == FCFD_SPEC_P) { R.p = S(bc.p); R.T = L.T; R.rho = m_rho(P, R.p, R.T); }
-            else if (bc.specified == FCFD_SPEC_T) { R.T = S(bc.T); R.p = L.p; R.rho = m_rho(P, R.p, R.T); }
+            if (bc.specified == FCFD_SPEC_STATE) { p = S(bc.p); T = S(bc.T); rho = S(bc.rho); }
+            else if (bc.specified == FCFD_SPEC_P) { p = S(bc.p); T = L.T; rho = m_rho(P, p, T); }
+            else if (bc.specified == FCFD_SPEC_T) { T = S(bc.T); p = L.p; rho = m_rho(P, p, T); }
             else {
-                R.rho = S(bc.rho);
-                R.a = 0.5 * (g - 1.) * (uNL + 2. * L.a / (g - 1.) - mdotNR / R.rho);
-                R.p = R.a * R.a * R.rho / g - Pref;
-                R.T = m_T(P, R.p, R.rho);
+                rho = S(bc.rho);
+                a = 0.5 * (g - 1.) * (uNL + 2. * L.a / (g - 1.) - mdotNR / rho);
+                p = a * a * rho / g - Pref;
+                T = m_T(P, p, rho);
             }
-            R.V = vsc(mdotNR / R.rho, lift<S>(F.n));
-            R.p_center = R.p; R.V_center = R.V; R.T_center = R.T;
+            V = vsc(mdotNR / rho, n);
+            pc = p; Vc = V; Tc = T;
         } else if (bc.kind == FCFD_KIND_STAGNATION) {
-            if (val(vdot(L.V, F.n)) > 0.) R.V = vsc(-1., L.V_center);
-            else R.V = L.V_center;
-            R.T = bc.T_total - 0.5 * (g - 1.) / (g * P.R) * vdot(R.V, R.V);
-            R.p = (bc.p_total + Pref) / (1. + 0.5 * vdot(R.V, R.V) / (P.R * (R.T + Tref))) - Pref;
-            R.rho = m_rho(P, R.p, R.T);
-            R.p_center = R.p; R.V_center = R.V; R.T_center = R.T;
+            if (val(vdot(L.V, F.n)) > 0.) V = vsc(-1., L.V_center);
+            else V = L.V_center;
+            T = bc.T_total - 0.5 * (g - 1.) / (g * P.R) * vdot(V, V);
+            p = (bc.p_total + Pref) / (1. + 0.5 * vdot(V, V) / (P.R * (T + Tref))) - Pref;
+            rho = m_rho(P, p, T);
+            pc = p; Vc = V; Tc = T;
         }
     } else if (bc.type == FCFD_BC_OUTLET) {
         S uNL = vdot(L.V, F.n);
         S MachL = uNL / L.a;
         bool supersonic = val(MachL) > 1.;
         if (bc.specified == FCFD_SPEC_P && !supersonic) {
-            R.p = S(bc.p); R.p_center = R.p;
-            R.rho = L.rho * powr((R.p + Pref), 1. / g) / powr((L.p + Pref), 1. / g);
-            R.a = sq_root(g * (R.p + Pref) / R.rho);
-            R.T = m_T(P, R.p, R.rho);
-            R.V = L.V;
+            p = S(bc.p); pc = p;
+            rho = L.rho * powr((p + Pref), 1. / g) / powr((L.p + Pref), 1. / g);
+            a = sq_root(g * (p + Pref) / rho);
+            T = m_T(P, p, rho);
+            V = L.V;
         } else if (bc.specified == FCFD_SPEC_T && !supersonic) {
-            R.T = S(bc.T); R.T_center = R.T;
-            R.a = sq_root(g * P.R * (R.T + Tref));
-            R.rho = powr(g * (L.p + Pref) / (R.a * R.a * powr(L.rho, g)), 1. - g);
-            R.p = m_p(P, R.rho, R.T);
-            R.V = L.V;
+            T = S(bc.T); Tc = T;
+            a = sq_root(g * P.R * (T + Tref));
+            rho = powr(g * (L.p + Pref) / (a * a * powr(L.rho, g)), 1. - g);
+            p = m_p(P, rho, T);
+            V = L.V;
         } else if (bc.specified == FCFD_SPEC_RHO && !supersonic) {
-            R.rho = S(bc.rho);
-            R.p = (L.p + Pref) * powr(R.rho / L.rho, g) - Pref;
-            R.T = m_T(P, R.p, R.rho);
-            R.a = sq_root(g * (R.p + Pref) / R.rho);
-            R.V = L.V;
+            rho = S(bc.rho);
+            p = (L.p + Pref) * powr(rho / L.rho, g) - Pref;
+            T = m_T(P, p, rho);
+            a = sq_root(g * (p + Pref) / rho);
+            V = L.V;
         } else {
             // extrapolate everything; the reference leaves rho stale here, the
             // density of the extrapolated p, T is used instead
-            R.p = L.p; R.T = L.T; R.V = L.V;
-            R.rho = m_rho(P, R.p, R.T);
+            p = L.p; T = L.T; V = L.V;
+            rho = m_rho(P, p, T);
         }
         if (val(MachL) < 0.) {
             if (bc.kind == FCFD_KIND_DAMP_REVERSE) {
-                R.p = R.p - 0.5 * L.rho * uNL * uNL;
-                R.rho = L.rho * powr((R.p + Pref), 1. / g) / powr((L.p + Pref), 1. / g);
-                R.a = sq_root(g * (R.p + Pref) / R.rho);
-                R.T = m_T(P, R.p, R.rho);
+                p = p - 0.5 * L.rho * uNL * uNL;
+                rho = L.rho * powr((p + Pref), 1. / g) / powr((L.p + Pref), 1. / g);
+                a = sq_root(g * (p + Pref) / rho);
+                T = m_T(P, p, rho);
             }
-            if (bc.kind == FCFD_KIND_NO_REVERSE) R.V = vsc(-1., L.V);
+            if (bc.kind == FCFD_KIND_NO_REVERSE) V = vsc(-1., L.V);
         }
         if (val(MachL) < 1. && bc.kind == FCFD_KIND_FORCE_SUPERSONIC) {
             S d = L.a - uNL;
-            R.p = R.p - 0.5 * L.rho * (d * d);
-            R.p = smin(S(1.e-8), R.p + Pref) - Pref;
-            R.rho = L.rho * powr((R.p + Pref), 1. / g) / powr((L.p + Pref), 1. / g);
-            R.a = sq_root(g * (R.p + Pref) / R.rho);
-            R.T = m_T(P, R.p, R.rho);
+            p = p - 0.5 * L.rho * (d * d);
+            p = smin(S(1.e-8), p + Pref) - Pref;
+            rho = L.rho * powr((p + Pref), 1. / g) / powr((L.p + Pref), 1. / g);
+            a = sq_root(g * (p + Pref) / rho);
+            T = m_T(P, p, rho);
         }
     } else if (bc.type == FCFD_BC_WALL) {
         if (bc.thermal == FCFD_THERMAL_FIXED_T) {
-            R.T = S(bc.T); R.T_center = R.T;
-            R.p = L.p;
-            R.rho = m_rho(P, R.p, R.T);
+            T = S(bc.T); Tc = T;
+            p = L.p;
+            rho = m_rho(P, p, T);
         } else {
-            R.p = L.p; R.rho = L.rho; R.T = L.T; R.T_center = L.T_center;
+            p = L.p; rho = L.rho; T = L.T; Tc = L.T_center;
         }
         if (bc.kind == FCFD_KIND_SLIP) {
-            R.V = vsub(L.V, vsc(2. * vdot(L.V, F.n), lift<S>(F.n)));
-            R.V_center = vsub(L.V_center, vsc(2. * vdot(L.V_center, F.n), lift<S>(F.n)));
+            V = vsub(L.V, vsc(2. * vdot(L.V, F.n), n));
+            Vc = vsub(L.V_center, vsc(2. * vdot(L.V_center, F.n), n));
         } else {
-            R.V = vsc(-1., L.V);
-            R.V_center = vsc(-1., L.V_center);
+            V = vsc(-1., L.V);
+            Vc = vsc(-1., L.V_center);
         }
-        R.p_center = L.p_center;
+        pc = L.p_center;
     } else if (bc.type == FCFD_BC_SYMMETRY) {
-        R.p = L.p;
-        R.V = vsub(L.V, vsc(2. * vdot(L.V, F.n), lift<S>(F.n)));
-        R.T = L.T;
-        R.rho = L.rho;
-        R.p_center = L.p_center;
-        R.T_center = L.T_center;
-        R.V_center = vsub(L.V_center, vsc(2. * vdot(L.V_center, F.n), lift<S>(F.n)));
+        p = L.p;
+        V = vsub(L.V, vsc(2. * vdot(L.V, F.n), n));
+        T = L.T;
+        rho = L.rho;
+        pc = L.p_center;
+        Tc = L.T_center;
+        Vc = vsub(L.V_center, vsc(2. * vdot(L.V_center, F.n), n));
     }
+    Rout.p = p; Rout.p_center = pc; Rout.T = T; Rout.T_center = Tc; Rout.rho = rho; Rout.a = a;
+    Rout.V = V; Rout.V_center = Vc;
     return flag;
 }
 
 // ---- convective fluxes ----------------------------------------------------
 // Roe with entropy fix (ns_roe.cc:25-98)
-template <typename S> __device__ inline void roe_flux(const CState<S>& L, const CState<S>& R, S fn[5], double G) {
+template <typename S> __device__ __forceinline__ void roe_flux(const CState<S>& L, const CState<S>& R, S fn[5], double G) {
     S rho = sq_root(R.rho / L.rho);
     S u = (L.Vn.x + rho * R.Vn.x) / (1. + rho);
     S v = (L.Vn.y + rho * R.Vn.y) / (1. + rho);
@@ -247,14 +254,14 @@ template <typename S> __device__ inline void roe_flux(const CState<S>& L, const 
 }
 
 // AUSM+-up (ns_ausm_plus_up.cc:35-152)
-template <typename S> __device__ inline S ms2p(const S& M) { return 0.25 * (M + 1.) * (M + 1.); }
-template <typename S> __device__ inline S ms2m(const S& M) { return -0.25 * (M - 1.) * (M - 1.); }
-template <typename S> __device__ inline S ms4p(const S& M) { return (val(absv(M)) >= 1.) ? S(0.5 * (M + absv(M))) : S(ms2p(M) * (1. - 16. * 0.125 * ms2m(M))); }
-template <typename S> __device__ inline S ms4m(const S& M) { return (val(absv(M)) >= 1.) ? S(0.5 * (M - absv(M))) : S(ms2m(M) * (1. + 16. * 0.125 * ms2p(M))); }
-template <typename S> __device__ inline S ps5p(const S& M, const S& al) { return (val(absv(M)) >= 1.) ? S(0.5 * (M + absv(M)) / M) : S(ms2p(M) * ((2. - M) - 16. * al * M * ms2m(M))); }
-template <typename S> __device__ inline S ps5m(const S& M, const S& al) { return (val(absv(M)) >= 1.) ? S(0.5 * (M - absv(M)) / M) : S(ms2m(M) * ((-2. - M) + 16. * al * M * ms2p(M))); }
+template <typename S> __device__ __forceinline__ S ms2p(const S& M) { return 0.25 * (M + 1.) * (M + 1.); }
+template <typename S> __device__ __forceinline__ S ms2m(const S& M) { return -0.25 * (M - 1.) * (M - 1.); }
+template <typename S> __device__ __forceinline__ S ms4p(const S& M) { return (val(absv(M)) >= 1.) ? S(0.5 * (M + absv(M))) : S(ms2p(M) * (1. - 16. * 0.125 * ms2m(M))); }
+template <typename S> __device__ __forceinline__ S ms4m(const S& M) { return (val(absv(M)) >= 1.) ? S(0.5 * (M - absv(M))) : S(ms2m(M) * (1. + 16. * 0.125 * ms2p(M))); }
+template <typename S> __device__ __forceinline__ S ps5p(const S& M, const S& al) { return (val(absv(M)) >= 1.) ? S(0.5 * (M + absv(M)) / M) : S(ms2p(M) * ((2. - M) - 16. * al * M * ms2m(M))); }
+template <typename S> __device__ __forceinline__ S ps5m(const S& M, const S& al) { return (val(absv(M)) >= 1.) ? S(0.5 * (M - absv(M)) / M) : S(ms2m(M) * ((-2. - M) + 16. * al * M * ms2p(M))); }
 
-template <typename S> __device__ inline void ausm_flux(const CState<S>& L, const CState<S>& R, S fn[5], double Pref, double Minf) {
+template <typename S> __device__ __forceinline__ void ausm_flux(const CState<S>& L, const CState<S>& R, S fn[5], double Pref, double Minf) {
     const double Kp = 0.25, Ku = 0.75, sigma = 1.;
     S aLh = L.a * L.a / smax(L.a, L.Vn.x);
     S aRh = R.a * R.a / smax(R.a, -1. * R.Vn.x);
@@ -279,7 +286,7 @@ template <typename S> __device__ inline void ausm_flux(const CState<S>& L, const
 }
 
 // van Leer (ns_vanleer.cc:25-66)
-template <typename S> __device__ inline void vanleer_flux(const CState<S>& L, const CState<S>& R, S fn[5], double G, double Pref) {
+template <typename S> __device__ __forceinline__ void vanleer_flux(const CState<S>& L, const CState<S>& R, S fn[5], double G, double Pref) {
     S ML = L.Vn.x / L.a, MR = R.Vn.x / R.a;
     S M = 0.5 * (ML + MR);
     if (val(M) <= -1.) {
@@ -304,8 +311,8 @@ template <typename S> __device__ inline void vanleer_flux(const CState<S>& L, co
 }
 
 // SD-SLAU (ns_sd_slau.cc:37-94)
-template <typename S> __device__ inline S sgn(const S& x) { return (val(x) < 0) ? S(-1.) : (val(x) > 0) ? S(1.) : S(0.); }
-template <typename S> __device__ inline void slau_flux(const CState<S>& L, const CState<S>& R, S fn[5], double Pref) {
+template <typename S> __device__ __forceinline__ S sgn(const S& x) { return (val(x) < 0) ? S(-1.) : (val(x) > 0) ? S(1.) : S(0.); }
+template <typename S> __device__ __forceinline__ void slau_flux(const CState<S>& L, const CState<S>& R, S fn[5], double Pref) {
     const double Csd1 = 0.1, Csd2 = 10.;
     S a = 0.5 * (L.a + R.a);
     S Mhat = smin(S(1.), sq_root(0.5 * (vdot(L.V, L.V) + vdot(R.V, R.V)) / a));
@@ -335,7 +342,7 @@ template <typename S> __device__ inline void slau_flux(const CState<S>& L, const
 }
 
 // modified Steger-Warming (ns_stegger_warming.cc:23-145)
-template <typename S> __device__ inline void sw_flux(const Params& P, const CState<S>& L, const CState<S>& R, double diss,
+template <typename S> __device__ __forceinline__ void sw_flux(const Params& P, const CState<S>& L, const CState<S>& R, double diss,
                                                      double cwd, S fn[5]) {
     const double alpha = 6.;
     const double RW = P.R, gm = P.gamma;
@@ -385,8 +392,9 @@ template <typename S> __device__ inline void sw_flux(const Params& P, const CSta
         Rm[3][0] = gs - V.x * a; Rm[3][1] = a - beta * V.x; Rm[3][2] = -beta * V.y; Rm[3][3] = -beta * V.z;
         Rm[4][0] = gs + V.x * a; Rm[4][1] = -1. * a - beta * V.x; Rm[4][2] = -beta * V.y; Rm[4][3] = -beta * V.z;
         Rm[3][4] = Rm[4][4] = S(beta);
-        const CState<S>& Sd = (signal > 0) ? L : R;
-        S Q[5] = {Sd.rho, Sd.rho * Sd.Vn.x, Sd.rho * Sd.Vn.y, Sd.rho * Sd.Vn.z, Sd.rho * Sd.H - Sd.p};
+        S Q[5];
+        if (signal > 0) { Q[0] = L.rho; Q[1] = L.rho * L.Vn.x; Q[2] = L.rho * L.Vn.y; Q[3] = L.rho * L.Vn.z; Q[4] = L.rho * L.H - L.p; }
+        else { Q[0] = R.rho; Q[1] = R.rho * R.Vn.x; Q[2] = R.rho * R.Vn.y; Q[3] = R.rho * R.Vn.z; Q[4] = R.rho * R.H - R.p; }
         for (int i = 0; i < 5; ++i) {
             for (int j = 0; j < 5; ++j) {
                 S J = S(0.);
@@ -398,7 +406,7 @@ template <typename S> __device__ inline void sw_flux(const Params& P, const CSta
 }
 
 // dispatch + rotation back to xyz (ns_convective_face_flux.cc:32-59)
-template <int FLUX, typename S> __device__ inline void convective_flux(const Params& P, const CState<S>& L, const CState<S>& R,
+template <int FLUX, typename S> __device__ __forceinline__ void convective_flux(const Params& P, const CState<S>& L, const CState<S>& R,
                                                                        const FaceGeo& F, double diss, double cwd, S flux[5]) {
     S fn[5];
     if (FLUX == FCFD_FLUX_ROE) roe_flux(L, R, fn, P.gamma);
@@ -414,14 +422,14 @@ template <int FLUX, typename S> __device__ inline void convective_flux(const Par
 }
 
 // deferred correction of one face gradient along l2r (ns_assemble_linear_system.cc:394-404)
-template <typename S> __device__ inline Vec<S> correct(const Vec<S>& g, const Vec<double>& l2rn, double l2rmag, const S& dq) {
+template <typename S> __device__ __forceinline__ Vec<S> correct(const Vec<S>& g, const Vec<double>& l2rn, double l2rmag, const S& dq) {
     Vec<S> r = vsub(g, vsc(vdot(g, l2rn), lift<S>(l2rn)));
     return vadd(r, vsc(dq / l2rmag, lift<S>(l2rn)));
 }
 
 // face_state_update (ns_assemble_linear_system.cc:353-415), given the face
 // gradients before correction
-template <typename S> __device__ inline void face_state(const Params& P, const CState<S>& L, const CState<S>& R,
+template <typename S> __device__ __forceinline__ void face_state(const Params& P, const CState<S>& L, const CState<S>& R,
                                                         const FaceGeo& F, FState<S>& fs) {
     Vec<double> l2rn = vnorm(F.l2r);
     double l2rmag = vmag(F.l2r);
@@ -436,7 +444,7 @@ template <typename S> __device__ inline void face_state(const Params& P, const C
 }
 
 // diffusive_face_flux (ns_diffusive_face_flux.cc:28-64), laminar
-template <typename S> __device__ inline void diffusive_flux(const FState<S>& f, const FaceGeo& F, const BcDev* bc, S flux[5]) {
+template <typename S> __device__ __forceinline__ void diffusive_flux(const FState<S>& f, const FaceGeo& F, const BcDev* bc, S flux[5]) {
     Vec<double> aV = mk<double>(F.n.x * F.area, F.n.y * F.area, F.n.z * F.area);
     Vec<S> tx, ty, tz;
     tx.x = 2. / 3. * (2. * f.gu.x - f.gv.y - f.gw.z);

@@ -157,21 +157,23 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
 }
 
 // find_min_max (ns.cc:400-435) with the reference's index slips, for VK's
-// reference range: one block, fixed-order tree reduction (deterministic)
-// The reference loop sets qmax[3] = max(qmax[4], w_c) and then
-// qmax[4] = max(qmax[4], p_c), with qmax[4] seeded by T of cell 0.  After the
-// loop: slot 4 = max(T_0, max_c p_c); slot 3 = max(max(T_0, max_{c<last} p_c), w_last)
-// where `last` is the last cell in reference order.  Min likewise.
-// `first`/`last` are the internal indices of reference cells 0 and N-1.
-__global__ void __launch_bounds__(1024) minmax_kernel(int N, size_t ld, int first, int last, const double* __restrict__ W,
-                                                      double* __restrict__ dref) {
-    __shared__ double sh[8][1024];
+// reference range.  The reference loop sets qmax[3] = max(qmax[4], w_c) and
+// then qmax[4] = max(qmax[4], p_c), with qmax[4] seeded by T of cell 0.  After
+// the loop: slot 4 = max(T_0, max_c p_c); slot 3 = max(max(T_0, max_{c<last}
+// p_c), w_last) where `last` is the last cell in reference order.  Min
+// likewise.  `first`/`last` are the internal indices of reference cells 0 and
+// N-1.  Two passes: fixed-size grid of partial min/max, then one block (max and
+// min are exact, so the result is independent of the order anyway).
+constexpr int kMMBlocks = 1024;
+__global__ void __launch_bounds__(256) minmax_partial_kernel(int N, size_t ld, int first, int last, const double* __restrict__ W,
+                                                             double* __restrict__ part) {
+    __shared__ double sh[8][256];
     int t = threadIdx.x;
     // 0..2: max p,u,v   3: max p excluding `last`   4..6: min p,u,v   7: min p excluding `last`
     double r[8];
     r[0] = r[4] = W[first]; r[1] = r[5] = W[ld + first]; r[2] = r[6] = W[2 * ld + first];
-    r[3] = r[7] = W[4 * ld + first];   // T_0 seeds the p-tracking slots (never excluded)
-    for (int c = t; c < N; c += blockDim.x) {
+    r[3] = r[7] = W[4 * ld + first];   // T_0 seeds the p-tracking slots
+    for (int c = blockIdx.x * 256 + t; c < N; c += kMMBlocks * 256) {
         double p = W[c], u = W[ld + c], v = W[2 * ld + c];
         r[0] = smax(r[0], p); r[1] = smax(r[1], u); r[2] = smax(r[2], v);
         r[4] = smin(r[4], p); r[5] = smin(r[5], u); r[6] = smin(r[6], v);
@@ -179,7 +181,27 @@ __global__ void __launch_bounds__(1024) minmax_kernel(int N, size_t ld, int firs
     }
     for (int k = 0; k < 8; ++k) sh[k][t] = r[k];
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            for (int k = 0; k < 4; ++k) sh[k][t] = smax(sh[k][t], sh[k][t + s]);
+            for (int k = 4; k < 8; ++k) sh[k][t] = smin(sh[k][t], sh[k][t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) for (int k = 0; k < 8; ++k) part[k * kMMBlocks + blockIdx.x] = sh[k][0];
+}
+
+__global__ void __launch_bounds__(256) minmax_final_kernel(size_t ld, int first, int last, const double* __restrict__ W,
+                                                           const double* __restrict__ part, double* __restrict__ dref) {
+    __shared__ double sh[8][256];
+    int t = threadIdx.x;
+    double r[8];
+    for (int k = 0; k < 8; ++k) r[k] = part[k * kMMBlocks + t];
+    for (int i = t + 256; i < kMMBlocks; i += 256)
+        for (int k = 0; k < 8; ++k) r[k] = k < 4 ? smax(r[k], part[k * kMMBlocks + i]) : smin(r[k], part[k * kMMBlocks + i]);
+    for (int k = 0; k < 8; ++k) sh[k][t] = r[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
         if (t < s) {
             for (int k = 0; k < 4; ++k) sh[k][t] = smax(sh[k][t], sh[k][t + s]);
             for (int k = 4; k < 8; ++k) sh[k][t] = smin(sh[k][t], sh[k][t + s]);
@@ -189,8 +211,7 @@ __global__ void __launch_bounds__(1024) minmax_kernel(int N, size_t ld, int firs
     if (t == 0) {
         double T0 = W[4 * ld + first];
         double pl = W[last], wl = W[3 * ld + last];
-        double q4max = smax(sh[3][0], pl), q4min = smin(sh[7][0], pl);
-        q4max = smax(q4max, T0); q4min = smin(q4min, T0);
+        double q4max = smax(smax(sh[3][0], pl), T0), q4min = smin(smin(sh[7][0], pl), T0);
         double q3max = smax(sh[3][0], wl), q3min = smin(sh[7][0], wl);
         dref[0] = fabs(sh[0][0] - sh[4][0]);
         dref[1] = fabs(sh[1][0] - sh[5][0]);
@@ -204,39 +225,40 @@ __global__ void __launch_bounds__(1024) minmax_kernel(int N, size_t ld, int firs
 // 3. face fluxes + per-cell ordered accumulation (ns_assemble_linear_system.cc:75-140)
 // --------------------------------------------------------------------------
 template <int FLUX, int VISC>
-__global__ void __launch_bounds__(256) flux_kernel(DevGrid g, Params P, const double* __restrict__ W,
-                                                   const double* __restrict__ G, const double* __restrict__ PHI,
-                                                   int has_phi, double phi_const, double* __restrict__ Rout,
-                                                   int* __restrict__ status) {
-    __shared__ double lds[5][256];
-    const int b = blockIdx.x, t = threadIdx.x;
-    const int c0 = g.blk_cell[b], c1 = g.blk_cell[b + 1];
-    const int h0 = g.hf_ptr[c0], nh = g.hf_ptr[c1] - h0;
-    if (t < nh) {
-        int fs = g.hf_face[h0 + t];
-        int f = fs >= 0 ? fs : ~fs;
-        double o[5];
-        face_flux<FLUX, VISC>(g, P, W, G, PHI, has_phi != 0, phi_const, f, o, status);
+__global__ void __launch_bounds__(256) face_flux_kernel(DevGrid g, Params P, const double* __restrict__ W,
+                                                        const double* __restrict__ G, const double* __restrict__ PHI,
+                                                        int has_phi, double phi_const, double* __restrict__ FX,
+                                                        int* __restrict__ status) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= g.F) return;
+    FaceCtx X;
+    face_base<FLUX, VISC>(g, P, W, G, PHI, has_phi != 0, phi_const, f, X, status);
+    const size_t Fs = g.F;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) FX[v * Fs + f] = X.D[v] - X.C[v];
+}
+
+// R_c = sum over the faces of c in reference face order of (D-C)+0 (parent)
+// or -1*(D-C)+0 (neighbour) -- the VecSetValues(ADD) sequence of
+// ns_assemble_linear_system.cc:131-140 starting from VecSet(rhs, 0)
+__global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, const double* __restrict__ FX, double* __restrict__ Rout) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld, Fs = g.F;
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
+        int fs = g.hf_face[h];
         if (fs >= 0) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) lds[v][t] = o[v];
-        } else {   // neighbour side: -1*(D-C)+0 (ns_assemble_linear_system.cc:137)
+            for (int v = 0; v < 5; ++v) s[v] += FX[v * Fs + fs] + 0.;
+        } else {
+            int f = ~fs;
 #pragma unroll
-            for (int v = 0; v < 5; ++v) lds[v][t] = -1. * (o[v] - 0.) + 0.;
+            for (int v = 0; v < 5; ++v) s[v] += -1. * FX[v * Fs + f] + 0.;
         }
     }
-    __syncthreads();
-    if (t < c1 - c0) {
-        int c = c0 + t;
-        int a0 = g.hf_ptr[c] - h0, a1 = g.hf_ptr[c + 1] - h0;
-        const size_t ld = g.ld;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            double s = 0.0;
-            for (int k = a0; k < a1; ++k) s += lds[v][k];
-            Rout[v * ld + c] = s;
-        }
-    }
+    for (int v = 0; v < 5; ++v) Rout[v * ld + c] = s[v];
 }
 
 // --------------------------------------------------------------------------
@@ -259,15 +281,16 @@ __global__ void __launch_bounds__(256) dt_local_kernel(DevGrid g, Params P, cons
 // ---------------------------------------------------------------------------
 template <int FLUX, int VISC>
 static void launch_flux_t(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI,
-                          int has_phi, double phi_const, double* R, int* status, hipStream_t s) {
-    hipLaunchKernelGGL((flux_kernel<FLUX, VISC>), dim3(g.nblk), dim3(256), 0, s, g, P, W, G, PHI, has_phi, phi_const, R, status);
+                          int has_phi, double phi_const, double* FX, int* status, hipStream_t s) {
+    hipLaunchKernelGGL((face_flux_kernel<FLUX, VISC>), dim3((g.F + 255) / 256), dim3(256), 0, s, g, P, W, G, PHI, has_phi,
+                       phi_const, FX, status);
 }
 
 template <int FLUX>
 static void launch_flux_v(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI,
-                          int has_phi, double phi_const, double* R, int* status, hipStream_t s) {
-    if (P.viscous) launch_flux_t<FLUX, 1>(g, P, W, G, PHI, has_phi, phi_const, R, status, s);
-    else launch_flux_t<FLUX, 0>(g, P, W, G, PHI, has_phi, phi_const, R, status, s);
+                          int has_phi, double phi_const, double* FX, int* status, hipStream_t s) {
+    if (P.viscous) launch_flux_t<FLUX, 1>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);
+    else launch_flux_t<FLUX, 0>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);
 }
 
 void launch_ghosts(const DevGrid& g, const Params& P, double* W, int* status, hipStream_t s) {
@@ -278,7 +301,13 @@ void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double*
     int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
     dim3 grid((g.N + 255) / 256), blk(256);
     if (lim == FCFD_LIMITER_VK) {
-        hipLaunchKernelGGL(minmax_kernel, dim3(1), dim3(1024), 0, s, g.N, (size_t)g.ld, g.ref_first, g.ref_last, W, dref);
+        // eps^2 = (1e-6 K dref/L)^2 vanishes for K = 0 (the default threshold,
+        // read_inputs.cc:129) whatever dref is, provided L > 0: skip the reduction
+        if (!(P.limiter_threshold == 0.0 && P.glscale > 0.0)) {
+            hipLaunchKernelGGL(minmax_partial_kernel, dim3(kMMBlocks), dim3(256), 0, s, g.N, (size_t)g.ld, g.ref_first, g.ref_last, W,
+                               dref + 8);
+            hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(256), 0, s, (size_t)g.ld, g.ref_first, g.ref_last, W, dref + 8, dref);
+        }
         hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_VK>), grid, blk, 0, s, g, P, W, G, PHI, dref);
     } else if (lim == FCFD_LIMITER_BJ) {
         hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_BJ>), grid, blk, 0, s, g, P, W, G, PHI, dref);
@@ -289,20 +318,24 @@ void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double*
     }
 }
 
-void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* R,
-                 int* status, hipStream_t s) {
+void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* FX,
+                 double* R, int* status, hipStream_t s) {
     // limiter buffer is used only when a limiter runs; otherwise phi is the
     // constant initial value (ns.cc:282-287): 1 for second order, 0 for first
     int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
     int has_phi = lim != FCFD_LIMITER_NONE;
     double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
     switch (P.flux) {
-    case FCFD_FLUX_ROE: launch_flux_v<FCFD_FLUX_ROE>(g, P, W, G, PHI, has_phi, phi_const, R, status, s); break;
-    case FCFD_FLUX_VAN_LEER: launch_flux_v<FCFD_FLUX_VAN_LEER>(g, P, W, G, PHI, has_phi, phi_const, R, status, s); break;
-    case FCFD_FLUX_SD_SLAU: launch_flux_v<FCFD_FLUX_SD_SLAU>(g, P, W, G, PHI, has_phi, phi_const, R, status, s); break;
-    case FCFD_FLUX_SW: launch_flux_v<FCFD_FLUX_SW>(g, P, W, G, PHI, has_phi, phi_const, R, status, s); break;
-    default: launch_flux_v<FCFD_FLUX_AUSM_PLUS_UP>(g, P, W, G, PHI, has_phi, phi_const, R, status, s); break;
+    case FCFD_FLUX_ROE: launch_flux_v<FCFD_FLUX_ROE>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s); break;
+    case FCFD_FLUX_VAN_LEER: launch_flux_v<FCFD_FLUX_VAN_LEER>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s); break;
+    case FCFD_FLUX_SD_SLAU: launch_flux_v<FCFD_FLUX_SD_SLAU>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s); break;
+    case FCFD_FLUX_SW: launch_flux_v<FCFD_FLUX_SW>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s); break;
+    default: launch_flux_v<FCFD_FLUX_AUSM_PLUS_UP>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s); break;
     }
+}
+
+void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s) {
+    hipLaunchKernelGGL(cell_sum_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, FX, R);
 }
 
 void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s) {

@@ -215,7 +215,8 @@ int  fcfd_residual(fcfd_ctx* ctx, double* W_dev, double* R_dev, void* stream);
 int  fcfd_internal_buffers(fcfd_ctx* ctx, const double** grad, const double** phi);
 
 /* Per-stage timing of fcfd_residual with HIP events recorded on the launch
- * stream around its three kernels (ghosts, gradients+limiters, fluxes).
+ * stream around its four stages (ghosts, gradients+limiters, face fluxes,
+ * ordered cell sums): stage_ms must hold 4 values.
  * Returns the accumulated milliseconds per stage and the number of timed
  * calls; enable = 1 starts (and resets), 0 stops (and resets), -1 only
  * reads.  No counterpart in the reference (its only timer is the wall clock
