@@ -36,6 +36,7 @@ def lib():
         "fcfd_fd_jacobian": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "fcfd_jacobian_buffers": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
         "fcfd_jacobian_to_host": (C.c_int, [_vp, _vp, _vp, _vp]),
+        "fcfd_jvp": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
         "fcfd_newton_step": (C.c_int, [_vp, _vp, C.POINTER(A.NewtonOpts), C.POINTER(A.NewtonStats), _vp]),
         "fcfd_time_step": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _vp, C.POINTER(C.c_double), _vp]),
     }

@@ -13,6 +13,7 @@
 #include "fcfd_internal.h"
 #include "fcfd_jacobian.h"
 #include "fcfd_krylov.h"
+#include "fcfd_tangent.h"
 
 #include <chrono>
 
@@ -36,6 +37,8 @@ struct fcfd_ctx {
     GmresWork<float> w32;
     double* Dinv64 = nullptr;
     float *Dinv32 = nullptr, *D32 = nullptr, *JL32 = nullptr, *JR32 = nullptr, *b32 = nullptr, *x32 = nullptr;
+    // tangent workspace (lazy)
+    double *dG = nullptr, *dPHI = nullptr, *dFX = nullptr, *jv_in = nullptr, *jv_out = nullptr, *jv_R = nullptr;
     std::string err;
     // optional per-stage timing: a ring of event quadruples recorded on the
     // launch stream around the three kernels, folded in lazily so that
@@ -513,11 +516,38 @@ int fcfd_jacobian_to_host(fcfd_ctx* c, double* JL_host, double* JR_host, double*
     return FCFD_OK;
 }
 
+static int ensure_tangent(fcfd_ctx* c) {
+    if (c->dG) return FCFD_OK;
+    const DevGrid& g = c->dg;
+    int rc;
+    if ((rc = dalloc(c, (size_t)15 * g.ld, &c->dG)) || (rc = dalloc(c, (size_t)5 * g.ld, &c->dPHI)) ||
+        (rc = dalloc(c, (size_t)5 * g.F, &c->dFX)) || (rc = dalloc(c, (size_t)5 * g.ld, &c->jv_in)) ||
+        (rc = dalloc(c, (size_t)5 * g.ld, &c->jv_out)) || (rc = dalloc(c, (size_t)5 * g.ld, &c->jv_R)))
+        return rc;
+    return FCFD_OK;
+}
+
+int fcfd_jvp(fcfd_ctx* c, double* W_dev, double* dW_dev, double* dR_dev, int mode, void* stream) {
+    if (!c || !W_dev || !dW_dev || !dR_dev) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    if (mode != FCFD_JAC_JVP_FULL && mode != FCFD_JAC_JVP_FROZEN) return fail(c, FCFD_EINVAL, "mode must be FCFD_JAC_JVP_FULL or _FROZEN");
+    int rc = ensure_tangent(c);
+    if (rc) return rc;
+    launch_jvp(c->dg, c->P, W_dev, dW_dev, c->G, c->dG, c->PHI, c->dPHI, c->dref, c->dFX, dR_dev, mode == FCFD_JAC_JVP_FULL,
+               c->status, (hipStream_t)stream);
+    HIPCHK(c, hipGetLastError());
+    return FCFD_OK;
+}
+
 int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd_newton_stats* st, void* stream) {
     if (!c || !W_dev || !o) return FCFD_EINVAL;
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    int rc0 = 0;
     if (o->dt_type != FCFD_DT_CFL_LOCAL) return fail(c, FCFD_ENOTSUP, "only CFL_LOCAL time stepping is implemented");
-    if (o->jacobian != FCFD_JAC_FD) return fail(c, FCFD_ENOTSUP, "matrix-free tangent operator not available in this build");
+    if (o->jacobian != FCFD_JAC_FD && o->jacobian != FCFD_JAC_JVP_FULL && o->jacobian != FCFD_JAC_JVP_FROZEN)
+        return fail(c, FCFD_EINVAL, "unknown Jacobian mode");
+    const bool mfree = o->jacobian != FCFD_JAC_FD;
+    if (mfree && (rc0 = ensure_tangent(c))) return rc0;
     if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad Krylov options");
     const DevGrid& g = c->dg;
     hipStream_t s = (hipStream_t)stream;
@@ -573,9 +603,11 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     launch_fd_jacobian(g, c->P, W_dev, c->G, c->PHI, c->upd, c->dt, c->JL, c->JR, c->Dg, c->status, s);
     if (fp32) {
         dev_bj_invert<float>(g, c->Dg, c->Dinv32, c->status, s);
-        dev_convert<double, float>((size_t)25 * g.ld, c->Dg, c->D32, s);
-        dev_convert<double, float>((size_t)25 * g.F, c->JL, c->JL32, s);
-        dev_convert<double, float>((size_t)25 * g.F, c->JR, c->JR32, s);
+        if (!mfree) {
+            dev_convert<double, float>((size_t)25 * g.ld, c->Dg, c->D32, s);
+            dev_convert<double, float>((size_t)25 * g.F, c->JL, c->JL32, s);
+            dev_convert<double, float>((size_t)25 * g.F, c->JR, c->JR32, s);
+        }
         dev_convert<double, float>(n5, c->R, c->b32, s);
     } else {
         dev_bj_invert<double>(g, c->Dg, c->Dinv64, c->status, s);
@@ -586,7 +618,28 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     GmresOpts go;
     go.restart = o->restart; go.max_iters = o->max_iters; go.rtol = o->rtol; go.abstol = o->abstol; go.knoll = o->knoll_guess;
     GmresStats gs;
-    if (fp32) {
+    // matrix-free operator: y = (P V/dt) x - J x with the tangent of W (the
+    // ghost slots of W are rewritten with the same values by each tangent call)
+    auto jvp_op = [&](const double* x, double* y, hipStream_t ss) {
+        hipMemcpyAsync(c->jv_in, x, n5 * sizeof(double), hipMemcpyDeviceToDevice, ss);
+        launch_jvp(g, c->P, W_dev, c->jv_in, c->G, c->dG, c->PHI, c->dPHI, c->dref, c->dFX, c->jv_R,
+                   o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss);
+        launch_newton_op(g, c->P, W_dev, c->dt, x, c->jv_R, y, ss);
+    };
+    if (fp32 && mfree) {
+        LinOp<float> A{[&](const float* x, float* y, hipStream_t ss) {
+            dev_convert<float, double>(n5, x, c->jv_out, ss);
+            jvp_op(c->jv_out, c->jv_out, ss);
+            dev_convert<double, float>(n5, c->jv_out, y, ss);
+        }};
+        LinOp<float> M{[&](const float* x, float* y, hipStream_t ss) { dev_bj_apply<float>(g, c->Dinv32, x, y, ss); }};
+        rc = gmres<float>(g, A, M, c->b32, c->x32, go, c->w32, gs, s);
+        if (!rc) dev_convert<float, double>(n5, c->x32, c->dW, s);
+    } else if (mfree) {
+        LinOp<double> A{[&](const double* x, double* y, hipStream_t ss) { jvp_op(x, y, ss); }};
+        LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) { dev_bj_apply<double>(g, c->Dinv64, x, y, ss); }};
+        rc = gmres<double>(g, A, M, c->R, c->dW, go, c->w64, gs, s);
+    } else if (fp32) {
         LinOp<float> A{[&](const float* x, float* y, hipStream_t ss) { dev_bsr_matvec<float>(g, c->D32, c->JL32, c->JR32, x, y, ss); }};
         LinOp<float> M{[&](const float* x, float* y, hipStream_t ss) { dev_bj_apply<float>(g, c->Dinv32, x, y, ss); }};
         rc = gmres<float>(g, A, M, c->b32, c->x32, go, c->w32, gs, s);

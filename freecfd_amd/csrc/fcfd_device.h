@@ -34,6 +34,8 @@ void launch_flux(const DevGrid& g, const Params& P, const double* W, const doubl
                  double* R, int* status, hipStream_t s);
 // ... and their ordered per-cell sums into R
 void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s);
+// VK reference range only (find_min_max), when the limiter threshold needs it
+void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double* dref, hipStream_t s);
 constexpr int kDrefSize = 8 + 8 * 1024;   // dref[5] + VK min/max partials
 void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s);
 

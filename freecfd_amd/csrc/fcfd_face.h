@@ -2,9 +2,12 @@
 // kernels: loads the face geometry and both cell states, reconstructs the
 // face states (MUSCL), applies the boundary condition, and evaluates the
 // convective and diffusive fluxes, exactly as one iteration of the face loop
-// of ns_assemble_linear_system.cc:75-93.
+// of ns_assemble_linear_system.cc:75-93.  Templated on the scalar type S and
+// on a cell loader, so that the tangent kernels run the same code on dual
+// numbers.
 #pragma once
 #include "fcfd_device.h"
+#include "fcfd_dual.h"
 #include "fcfd_physics.h"
 
 namespace fcfd {
@@ -19,31 +22,67 @@ __device__ __forceinline__ void load_geo(const DevGrid& g, int f, FaceGeo& F) {
 
 __device__ __forceinline__ Vec<double> centroid(const DevGrid& g, int c) { return mk<double>(g.ccx[c], g.ccy[c], g.ccz[c]); }
 
-__device__ __forceinline__ void load_cell(const DevGrid& g, const double* __restrict__ W, const double* __restrict__ G,
-                                 const double* __restrict__ PHI, int c, bool has_phi, double phi_const, double q[5],
-                                 Vec<double> gr[5], double ph[5]) {
-    const size_t ld = g.ld;
+// primal cell loader: W [5][ld], gradients [15][ld], limiters [5][ld]
+struct CellLoader {
+    const double* __restrict__ W;
+    const double* __restrict__ G;
+    const double* __restrict__ PHI;
+    bool has_phi;
+    double phi_const;
+    size_t ld;
+    __device__ __forceinline__ void load(int c, bool ghost, double q[5], Vec<double> gr[5], double ph[5]) const {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        q[v] = W[v * ld + c];
-        gr[v] = mk<double>(G[(3 * v) * ld + c], G[(3 * v + 1) * ld + c], G[(3 * v + 2) * ld + c]);
-        ph[v] = has_phi ? PHI[v * ld + c] : phi_const;
+        for (int v = 0; v < 5; ++v) {
+            q[v] = W[v * ld + c];
+            gr[v] = mk<double>(G[(3 * v) * ld + c], G[(3 * v + 1) * ld + c], G[(3 * v + 2) * ld + c]);
+            ph[v] = (has_phi && !ghost) ? PHI[v * ld + c] : (ghost ? 0.0 : phi_const);
+        }
     }
-}
+};
+
+// tangent cell loader: value = primal arrays, derivative = tangent arrays.
+// FULL = false: gradients, limiters and ghost states carry no tangent (the
+// reference FD Jacobian's frozen-gradient linearisation).
+template <bool FULL>
+struct DualLoader {
+    CellLoader p;
+    const double* __restrict__ dW;
+    const double* __restrict__ dG;
+    const double* __restrict__ dPHI;
+    int N;
+    __device__ __forceinline__ void load(int c, bool ghost, Dual q[5], Vec<Dual> gr[5], Dual ph[5]) const {
+        double qv[5], phv[5];
+        Vec<double> gv[5];
+        p.load(c, ghost, qv, gv, phv);
+        const size_t ld = p.ld;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            double dq = (FULL || c < N) ? dW[v * ld + c] : 0.0;
+            q[v] = Dual(qv[v], dq);
+            if (FULL) {
+                gr[v] = mk<Dual>(Dual(gv[v].x, dG[(3 * v) * ld + c]), Dual(gv[v].y, dG[(3 * v + 1) * ld + c]),
+                                 Dual(gv[v].z, dG[(3 * v + 2) * ld + c]));
+                ph[v] = Dual(phv[v], (p.has_phi && !ghost) ? dPHI[v * ld + c] : 0.0);
+            } else {
+                gr[v] = mk<Dual>(Dual(gv[v].x), Dual(gv[v].y), Dual(gv[v].z));
+                ph[v] = Dual(phv[v]);
+            }
+        }
+    }
+};
 
 // everything one face-loop iteration computes before the residual insertion
+template <typename S>
 struct FaceCtx {
     FaceGeo F;
-    CState<double> L, R;
-    FState<double> fs;
-    double C[5], D[5];
+    CState<S> L, R;
+    FState<S> fs;
+    S C[5], D[5];
     int p, n;
 };
 
-template <int FLUX, int VISC>
-__device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, const double* __restrict__ W, const double* __restrict__ G,
-                                 const double* __restrict__ PHI, bool has_phi, double phi_const, int f, FaceCtx& X,
-                                 int* status) {
+template <int FLUX, int VISC, typename S, typename Loader>
+__device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, const Loader& ld_, int f, FaceCtx<S>& X, int* status) {
     FaceGeo& F = X.F;
     load_geo(g, f, F);
     const int p = g.fpar[f], n = g.fnei[f];
@@ -51,23 +90,23 @@ __device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, con
     Vec<double> xf = mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]);
     Vec<double> xp = centroid(g, p), xn = centroid(g, n);
     F.l2r = vsub(xn, xp);
-    double q[5], ph[5];
-    Vec<double> gp[5];
-    load_cell(g, W, G, PHI, p, has_phi, phi_const, q, gp, ph);
+    S q[5], ph[5];
+    Vec<S> gp[5];
+    ld_.load(p, false, q, gp, ph);
     recon(P, X.L, q, gp, ph, vsub(xf, xp));
     finish_state(P, X.L, F);
-    Vec<double> gn[5];
+    Vec<S> gn[5];
     // right state: boundary (right_state_update :298-303, apply_bcs) or
     // interior reconstruction; both branches produce a whole state value
     {
-        double qn[5], phn[5];
+        S qn[5], phn[5];
         const bool bnd = F.bc >= 0;
-        load_cell(g, W, G, PHI, n, has_phi && !bnd, bnd ? 0.0 : phi_const, qn, gn, phn);
-        CState<double> Rb;
+        ld_.load(n, bnd, qn, gn, phn);
+        CState<S> Rb;
         if (bnd) {
             Rb = X.L;
             Rb.p_center = qn[0];
-            Rb.V_center = mk<double>(qn[1], qn[2], qn[3]);
+            Rb.V_center = mk<S>(qn[1], qn[2], qn[3]);
             Rb.T_center = qn[4];
             if (apply_bcs(P, g.bc[F.bc], X.L, Rb, F)) atomicOr(status, 1);
         } else {
@@ -91,18 +130,8 @@ __device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, con
         diffusive_flux(X.fs, F, g.bc, X.D);
     } else {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) X.D[v] = 0.0;
+        for (int v = 0; v < 5; ++v) X.D[v] = S(0.0);
     }
-}
-
-template <int FLUX, int VISC>
-__device__ __forceinline__ void face_flux(const DevGrid& g, const Params& P, const double* __restrict__ W, const double* __restrict__ G,
-                                 const double* __restrict__ PHI, bool has_phi, double phi_const, int f, double out[5],
-                                 int* status) {
-    FaceCtx X;
-    face_base<FLUX, VISC>(g, P, W, G, PHI, has_phi, phi_const, f, X, status);
-#pragma unroll
-    for (int v = 0; v < 5; ++v) out[v] = X.D[v] - X.C[v] + 0.;
 }
 
 }  // namespace fcfd

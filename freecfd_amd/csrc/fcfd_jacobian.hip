@@ -70,8 +70,9 @@ __global__ void __launch_bounds__(128) fdjac_kernel(DevGrid g, Params P, const d
                                                     double* __restrict__ JR, int* __restrict__ status) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
-    FaceCtx X;
-    face_base<FLUX, VISC>(g, P, W, G, PHI, has_phi != 0, phi_const, f, X, status);
+    FaceCtx<double> X;
+    CellLoader ld_{W, G, PHI, has_phi != 0, phi_const, (size_t)g.ld};
+    face_base<FLUX, VISC>(g, P, ld_, f, X, status);
     const FaceGeo& F = X.F;
     const size_t ld = g.ld;
     const size_t Fs = g.F;

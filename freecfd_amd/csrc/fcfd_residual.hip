@@ -231,8 +231,9 @@ __global__ void __launch_bounds__(256) face_flux_kernel(DevGrid g, Params P, con
                                                         int* __restrict__ status) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
-    FaceCtx X;
-    face_base<FLUX, VISC>(g, P, W, G, PHI, has_phi != 0, phi_const, f, X, status);
+    FaceCtx<double> X;
+    CellLoader ld_{W, G, PHI, has_phi != 0, phi_const, (size_t)g.ld};
+    face_base<FLUX, VISC>(g, P, ld_, f, X, status);
     const size_t Fs = g.F;
 #pragma unroll
     for (int v = 0; v < 5; ++v) FX[v * Fs + f] = X.D[v] - X.C[v];
@@ -316,6 +317,13 @@ void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double*
     } else {
         hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_NONE>), grid, blk, 0, s, g, P, W, G, PHI, dref);
     }
+}
+
+void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double* dref, hipStream_t s) {
+    int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    if (lim != FCFD_LIMITER_VK || (P.limiter_threshold == 0.0 && P.glscale > 0.0)) return;
+    hipLaunchKernelGGL(minmax_partial_kernel, dim3(kMMBlocks), dim3(256), 0, s, g.N, (size_t)g.ld, g.ref_first, g.ref_last, W, dref + 8);
+    hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(256), 0, s, (size_t)g.ld, g.ref_first, g.ref_last, W, dref + 8, dref);
 }
 
 void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* FX,

@@ -1,0 +1,248 @@
+// Tangent (JVP) of the residual: dR = (dR/dW) dW, by forward-mode dual
+// numbers through the same code as the FP64 residual.
+//
+//   FULL   ghosts (update_boundaries) -> gradients -> limiters -> face
+//          fluxes are all differentiated: the exact Jacobian-vector product of
+//          the second-order residual.
+//   FROZEN only the cell-centre states of the two cells of each face carry a
+//          tangent; gradients, limiters and ghost centres are frozen.  This is
+//          the linearisation the reference FD Jacobian approximates
+//          (get_jacobians perturbs the reconstructed face states only,
+//          ns_assemble_linear_system.cc:197-267), i.e. dR = sum_f JL dW_P +
+//          JR dW_N with exact derivatives instead of one-sided differences.
+// The VK global reference range (find_min_max) is held constant; with the
+// default threshold K = 0 it does not enter the limiter at all.
+#include <hip/hip_runtime.h>
+
+#include "fcfd_face.h"
+#include "fcfd_tangent.h"
+
+namespace fcfd {
+
+__global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, double* __restrict__ W, double* __restrict__ dW,
+                                                      int* __restrict__ status) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.B) return;
+    int f = g.gh_face[i];
+    int c = g.fpar[f];
+    FaceGeo F;
+    load_geo(g, f, F);
+    const size_t ld = g.ld;
+    CState<Dual> L;
+    L.p = L.p_center = Dual(W[c], dW[c]);
+    L.V = L.V_center = mk<Dual>(Dual(W[ld + c], dW[ld + c]), Dual(W[2 * ld + c], dW[2 * ld + c]), Dual(W[3 * ld + c], dW[3 * ld + c]));
+    L.T = L.T_center = Dual(W[4 * ld + c], dW[4 * ld + c]);
+    L.rho = m_rho(P, L.p, L.T);
+    finish_state(P, L, F);
+    CState<Dual> R = L;
+    if (apply_bcs(P, g.bc[F.bc], L, R, F)) atomicOr(status, 1);
+    int gi = g.N + i;
+    W[gi] = R.p_center.v;          dW[gi] = R.p_center.d;
+    W[ld + gi] = R.V_center.x.v;   dW[ld + gi] = R.V_center.x.d;
+    W[2 * ld + gi] = R.V_center.y.v; dW[2 * ld + gi] = R.V_center.y.d;
+    W[3 * ld + gi] = R.V_center.z.v; dW[3 * ld + gi] = R.V_center.z.d;
+    W[4 * ld + gi] = R.T_center.v; dW[4 * ld + gi] = R.T_center.d;
+}
+
+template <int LIM>
+__global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, const double* __restrict__ W,
+                                                         const double* __restrict__ dW, double* __restrict__ G,
+                                                         double* __restrict__ dG, double* __restrict__ PHI,
+                                                         double* __restrict__ dPHI, const double* __restrict__ dref) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld;
+    Dual gr[15];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) gr[k] = Dual(0.0);
+    {
+        int s = c >> 6, lane = c & 63;
+        int e0 = g.gslice[s], e1 = g.gslice[s + 1];
+        for (int e = e0 + lane; e < e1; e += 64) {
+            int j = g.gidx[e];
+            double wx = g.gwx[e], wy = g.gwy[e], wz = g.gwz[e];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                Dual q(W[v * ld + j], dW[v * ld + j]);
+                gr[3 * v] += wx * q;
+                gr[3 * v + 1] += wy * q;
+                gr[3 * v + 2] += wz * q;
+            }
+        }
+    }
+    const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) { G[k * ld + c] = gr[k].v; dG[k * ld + c] = gr[k].d; }
+    Dual q0[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) q0[v] = Dual(W[v * ld + c], dW[v * ld + c]);
+    Dual phi[5] = {Dual(1.), Dual(1.), Dual(1.), Dual(1.), Dual(1.)};
+    Vec<double> xc = centroid(g, c);
+    if (LIM == FCFD_LIMITER_MINMOD) {
+        const double small = 1.e-12;
+        for (int h = h0; h < h1; ++h) {
+            int fs = g.hf_face[h];
+            int f = fs >= 0 ? fs : ~fs;
+            int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
+            Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                Dual dU = Dual(W[v * ld + nb], dW[v * ld + nb]) - q0[v];
+                Dual dUg = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
+                Dual mm = (dU.v * dUg.v < 0.) ? Dual(0.) : (dU.v > 0.) ? smin(dU, dUg) : smax(dU, dUg);
+                double sU = (double)((mm.v > 0) - (mm.v < 0)), sG = (double)((dUg.v > 0) - (dUg.v < 0));
+                phi[v] = smin(phi[v], (mm + sU * small) / (dUg + sG * small));
+            }
+        }
+    } else if (LIM == FCFD_LIMITER_BJ || LIM == FCFD_LIMITER_VK) {
+        Dual umax[5], umin[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) umax[v] = umin[v] = q0[v];
+        for (int h = h0; h < h1; ++h) {
+            int fs = g.hf_face[h];
+            int f = fs >= 0 ? fs : ~fs;
+            int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                Dual q(W[v * ld + nb], dW[v * ld + nb]);
+                umax[v] = smax(umax[v], q);
+                umin[v] = smin(umin[v], q);
+            }
+        }
+        for (int h = h0; h < h1; ++h) {
+            int fs = g.hf_face[h];
+            int f = fs >= 0 ? fs : ~fs;
+            Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                Dual dM = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
+                Dual dP = q0[v];
+                if (LIM == FCFD_LIMITER_BJ) {
+                    if (dM.v > 0.) dP = umax[v] - dP;
+                    else if (dM.v < 0.) dP = umin[v] - dP;
+                    else { dP = Dual(1.); dM = Dual(1.); }
+                    phi[v] = smin(phi[v], dP / dM);
+                } else {
+                    double eps = dref[v] / P.glscale;
+                    if (dM.v > 0.) dP = umax[v] - dP;
+                    else if (dM.v < 0.) dP = umin[v] - dP;
+                    Dual dP2 = dP * dP, dM2 = dM * dM;
+                    double e = 1.e-6 * P.limiter_threshold * eps;
+                    double eps2 = e * e;
+                    if (dM.v != 0.) phi[v] = smin(phi[v], ((dP2 + eps2) + 2. * dM * dP) / (dP2 + 2. * dM2 + dM * dP + eps2));
+                }
+            }
+        }
+    }
+    if (LIM != FCFD_LIMITER_NONE) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) { PHI[v * ld + c] = phi[v].v; dPHI[v * ld + c] = phi[v].d; }
+    }
+    for (int h = h0; h < h1; ++h) {
+        int fs = g.hf_face[h];
+        if (fs < 0) continue;
+        int gi = g.fnei[fs];
+        if (gi < g.N) continue;
+#pragma unroll
+        for (int k = 0; k < 15; ++k) { G[k * ld + gi] = gr[k].v; dG[k * ld + gi] = gr[k].d; }
+    }
+}
+
+template <int FLUX, int VISC, bool FULL>
+__global__ void __launch_bounds__(256) face_flux_t_kernel(DevGrid g, Params P, DualLoader<FULL> L, double* __restrict__ dFX,
+                                                          int* __restrict__ status) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= g.F) return;
+    FaceCtx<Dual> X;
+    face_base<FLUX, VISC>(g, P, L, f, X, status);
+    const size_t Fs = g.F;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) dFX[v * Fs + f] = (X.D[v] - X.C[v]).d;
+}
+
+// y = (P V/dt) x - dR  (the implicit operator applied matrix-free,
+// time_terms ns_time_terms.cc:26-56,147-163 minus the tangent)
+__global__ void __launch_bounds__(256) newton_op_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ dt,
+                                                        const double* __restrict__ x, const double* __restrict__ dR,
+                                                        double* __restrict__ y) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld;
+    double p = W[c], u = W[ld + c], v = W[2 * ld + c], w = W[3 * ld + c], T = W[4 * ld + c];
+    double rho = m_rho(P, p, T);
+    double pressure = p + P.Pref, temperature = T + P.Tref;
+    double drho_dT = -1. * rho / temperature, drho_dp = rho / pressure;
+    double H = P.Cp * temperature + 0.5 * (u * u + v * v + w * w);
+    double xv[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) xv[k] = x[k * ld + c];
+    double s = g.vol[c] / dt[c];
+    double r0 = drho_dp * xv[0] + drho_dT * xv[4];
+    double r1 = drho_dp * u * xv[0] + rho * xv[1] + drho_dT * u * xv[4];
+    double r2 = drho_dp * v * xv[0] + rho * xv[2] + drho_dT * v * xv[4];
+    double r3 = drho_dp * w * xv[0] + rho * xv[3] + drho_dT * w * xv[4];
+    double r4 = (drho_dp * H - 1.) * xv[0] + rho * u * xv[1] + rho * v * xv[2] + rho * w * xv[3] + (drho_dT * H + rho * P.Cp) * xv[4];
+    y[c] = s * r0 - dR[c];
+    y[ld + c] = s * r1 - dR[ld + c];
+    y[2 * ld + c] = s * r2 - dR[2 * ld + c];
+    y[3 * ld + c] = s * r3 - dR[3 * ld + c];
+    y[4 * ld + c] = s * r4 - dR[4 * ld + c];
+}
+
+// ---------------------------------------------------------------------------
+template <int FLUX, int VISC, bool FULL>
+static void launch_ft(const DevGrid& g, const Params& P, const DualLoader<FULL>& L, double* dFX, int* status, hipStream_t s) {
+    hipLaunchKernelGGL((face_flux_t_kernel<FLUX, VISC, FULL>), dim3((g.F + 255) / 256), dim3(256), 0, s, g, P, L, dFX, status);
+}
+template <int FLUX, bool FULL>
+static void launch_fv(const DevGrid& g, const Params& P, const DualLoader<FULL>& L, double* dFX, int* status, hipStream_t s) {
+    if (P.viscous) launch_ft<FLUX, 1, FULL>(g, P, L, dFX, status, s);
+    else launch_ft<FLUX, 0, FULL>(g, P, L, dFX, status, s);
+}
+template <bool FULL>
+static void launch_ff(const DevGrid& g, const Params& P, const DualLoader<FULL>& L, double* dFX, int* status, hipStream_t s) {
+    switch (P.flux) {
+    case FCFD_FLUX_ROE: launch_fv<FCFD_FLUX_ROE, FULL>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_VAN_LEER: launch_fv<FCFD_FLUX_VAN_LEER, FULL>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_SD_SLAU: launch_fv<FCFD_FLUX_SD_SLAU, FULL>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_SW: launch_fv<FCFD_FLUX_SW, FULL>(g, P, L, dFX, status, s); break;
+    default: launch_fv<FCFD_FLUX_AUSM_PLUS_UP, FULL>(g, P, L, dFX, status, s); break;
+    }
+}
+
+void launch_jvp(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
+                double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s) {
+    const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    const bool has_phi = lim != FCFD_LIMITER_NONE;
+    const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
+    CellLoader cl{W, G, PHI, has_phi, phi_const, (size_t)g.ld};
+    if (full) {
+        if (g.B > 0) hipLaunchKernelGGL(ghost_t_kernel, dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
+        dim3 grid((g.N + 255) / 256), blk(256);
+        if (lim == FCFD_LIMITER_VK) {
+            launch_vk_range(g, P, W, dref, s);   // primal reference range (constant in the tangent)
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_VK>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        } else if (lim == FCFD_LIMITER_BJ) {
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_BJ>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        } else if (lim == FCFD_LIMITER_MINMOD) {
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_MINMOD>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        } else {
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_NONE>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        }
+        DualLoader<true> L{cl, dW, dG, dPHI, g.N};
+        launch_ff<true>(g, P, L, dFX, status, s);
+    } else {
+        launch_ghosts(g, P, W, status, s);
+        launch_grad_lim(g, P, W, G, PHI, dref, s);
+        DualLoader<false> L{cl, dW, nullptr, nullptr, g.N};
+        launch_ff<false>(g, P, L, dFX, status, s);
+    }
+    launch_cell_sum(g, dFX, dR, s);
+}
+
+void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* dR,
+                      double* y, hipStream_t s) {
+    hipLaunchKernelGGL(newton_op_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, P, W, dt, x, dR, y);
+}
+
+}  // namespace fcfd

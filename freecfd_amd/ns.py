@@ -136,6 +136,22 @@ class NavierStokes(Grid):
         _check(self.h, lib().fcfd_jacobian_to_host(self.h, JL.ctypes.data, JR.ctypes.data, D.ctypes.data))
         return JL, JR, D
 
+    def jvp(self, dW, mode=A.JAC_JVP_FULL, W=None, out=None):
+        """dR = (dR/dW) dW on the GPU (dW a [5][ld] float64 device tensor)."""
+        W = self.W if W is None else W
+        out = self.torch.zeros_like(self.W) if out is None else out
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_jvp(self.h, W.data_ptr(), dW.data_ptr(), out.data_ptr(), int(mode), C.c_void_p(s)))
+        return out
+
+    def to_device(self, X_host):
+        """N x 5 host array (reference order) -> [5][ld] device tensor."""
+        X_host = np.ascontiguousarray(X_host, dtype=np.float64)
+        t = self.torch.zeros_like(self.W)
+        self.torch.cuda.synchronize()
+        _check(self.h, lib().fcfd_state_to_device(self.h, X_host.ctypes.data, t.data_ptr()))
+        return t
+
     def newton_step(self, opts=None, W=None, stream=None, check=True):
         """One implicit step (NavierStokes::solve, ns.cc:104-118); returns NewtonStats."""
         opts = A.newton_opts() if opts is None else opts

@@ -252,7 +252,10 @@ int  fcfd_jacobian_to_host(fcfd_ctx* ctx, double* JL_host, double* JR_host, doub
 typedef struct fcfd_newton_opts {
     int dt_type;          /* FCFD_DT_CFL_LOCAL (update_time_step, time_step.cc:193-200) */
     double cfl;
-    int jacobian;         /* FCFD_JAC_*                                                  */
+    int jacobian;         /* FCFD_JAC_FD: assembled reference FD Jacobian; FCFD_JAC_JVP_*:
+                             matrix-free tangent operator (P V/dt) x - J x.  In every mode the
+                             block-Jacobi preconditioner inverts the diagonal blocks of the
+                             FD Jacobian + time terms.                                 */
     int krylov_fp32;      /* 0: FP64 GMRES + FP64 block-Jacobi; 1: FP32 for both         */
     int restart;          /* GMRES restart length (the reference uses FGMRES(300))       */
     int max_iters;        /* maximumiterations (default 10, read_inputs.cc:127)          */
@@ -267,6 +270,16 @@ typedef struct fcfd_newton_stats {
     double krylov_rnorm, krylov_r0norm;
     double ms_residual, ms_jacobian, ms_krylov, ms_update;
 } fcfd_newton_stats;
+
+/* Tangent (Jacobian-vector product) of the residual: dR = (dR/dW) dW, exact
+ * derivatives by forward-mode dual numbers through the residual kernels.
+ * mode FCFD_JAC_JVP_FULL differentiates ghosts, gradients and limiters (the
+ * Jacobian of the second-order residual; the ghost slots [N,N+B) of W_dev
+ * and dW_dev are written); FCFD_JAC_JVP_FROZEN holds gradients, limiters and
+ * ghost centres fixed -- the linearisation the reference FD Jacobian
+ * approximates.  Arrays [5][ld_cell].  No counterpart in the reference,
+ * which only builds the FD Jacobian. */
+int  fcfd_jvp(fcfd_ctx* ctx, double* W_dev, double* dW_dev, double* dR_dev, int mode, void* stream);
 
 /* One implicit step (NavierStokes::solve without the gradient refresh,
  * ns.cc:104-118): residual, local time step, Jacobian (or matrix-free

@@ -1,0 +1,175 @@
+"""Tangent (JVP) kernels on the GPU.
+
+- FULL tangent vs central finite differences of the GPU residual itself
+  (second-order residual with gradients, limiters and ghosts); the direction
+  is scaled per variable so that h*v is ~1e-7 of the state, as in the
+  differentiation-check configuration;
+- FROZEN tangent vs the assembled reference FD Jacobian of the CPU
+  restatement applied to the same direction (agreement to the FD truncation);
+- exact linearity: J(2u) == 2 J(u) bitwise (every derivative operation is
+  linear in the tangents and doubling is exact), J(u+v) = Ju + Jv to rounding;
+- first-order inviscid: FULL == FROZEN (no gradients or ghost centres enter).
+"""
+import numpy as np
+import pytest
+
+from freecfd_amd import _abi as A
+from freecfd_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+SCALES = np.array([1e5, 100.0, 100.0, 100.0, 300.0])
+
+
+def _ns(cs):
+    from freecfd_amd.ns import NavierStokes
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    return ns
+
+
+def _direction(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=(n, 5)) * SCALES
+
+
+def _residual_at(ns, W_host):
+    ns.set_state(W_host)
+    ns.assemble_residual()
+    return ns.residual_host()
+
+
+def _smooth(cs, seed=21, amp=0.02):
+    """freestream of the case times a seeded smooth perturbation (8 Fourier
+    modes), so that limiter switches (local extrema) are rare."""
+    from freecfd_amd import meshgen
+    xc = meshgen.cell_centroids(cs.raw)
+    lo, hi = xc.min(axis=0), xc.max(axis=0)
+    d = cases.smooth_perturbation((xc - lo) / np.maximum(hi - lo, 1e-30), seed, amp=amp)
+    W = cs.W0.mean(axis=0)[None, :] * np.ones((cs.n_cells, 1))
+    W[:, 0] *= 1.0 + d[:, 0]
+    W[:, 1] *= 1.0 + d[:, 1]
+    W[:, 2] += cs.meta["U"] * d[:, 2]
+    W[:, 3] += cs.meta["U"] * d[:, 3]
+    W[:, 4] *= 1.0 + d[:, 4]
+    cs.W0 = W
+    return cs
+
+
+def _fd_check(cs, label):
+    print("case", cs.name, label, flush=True)
+    ns = _ns(cs)
+    v = _direction(cs.n_cells, 4)
+    Jv = ns.to_host(ns.jvp(ns.to_device(v)), 5)
+    h = 1e-7
+    Rp = _residual_at(ns, cs.W0 + h * v)
+    Rm = _residual_at(ns, cs.W0 - h * v)
+    fd = (Rp - Rm) / (2 * h)
+    scale = np.abs(fd).max(axis=0) + 1e-300
+    rel = np.abs(Jv - fd) / scale
+    print("  per-variable max |Jv - FD| / max|FD|:", " ".join("%.2e" % e for e in rel.max(axis=0)), flush=True)
+    return rel
+
+
+@pytest.mark.parametrize("name,make", [
+    ("wedge", lambda: cases.wedge_case(24, 12)),
+    ("cylinder_viscous", lambda: cases.cylinder_case(32, 16)),
+    ("tetbox_viscous", lambda: cases.tetbox_case(4)),
+    ("bump", lambda: cases.bump_case(12, 6, 4)),
+    ("plate_prisms", lambda: cases.plate_case(12, 8, 3, first=1e-3)),
+])
+def test_full_jvp_vs_central_fd_unlimited(name, make):
+    """Second-order residual without a limiter: smooth, so the exact tangent
+    and central differences (h = 1e-7 of the state) must agree to the FD
+    truncation/round-off level."""
+    cs = _smooth(make())
+    cs.opt.limiter = A.LIMITER_NONE
+    rel = _fd_check(cs, "no limiter")
+    assert rel.max() < 1e-6
+
+
+@pytest.mark.parametrize("name,make", [
+    ("tetbox_vk", lambda: cases.tetbox_case(5)),
+    ("bump_vk", lambda: cases.bump_case(12, 6, 4)),
+])
+def test_full_jvp_vs_central_fd_limited(name, make):
+    """With a limiter the residual is only piecewise smooth (min over faces,
+    umax/umin switching at local extrema): the tangent takes the active branch
+    and central differences straddling a switch differ there by O(1).  The
+    Venkatakrishnan function also has a large curvature where dM -> 0, which
+    costs central differences a few 1e-5.  Required: no O(1e-4) disagreement
+    on more than 2% of the cells, and a median below 1e-7.
+
+    Minmod on one-cell-thick meshes is not used here: the lsqr z-gradient is
+    rounding noise there, whose sign flips phi between 0 and 1 (the
+    reference's minmod, ns_limiters.cc:39-43,82-83), a discontinuity rather
+    than a kink."""
+    cs = _smooth(make())
+    rel = _fd_check(cs, "limited")
+    per_cell = rel.max(axis=1)
+    bad = (per_cell > 1e-4).mean()
+    print("  fraction of cells with a limiter switch inside +-h: %.4f, median %.2e" % (bad, np.median(per_cell)), flush=True)
+    assert bad < 0.02 and np.median(per_cell) < 1e-7
+
+
+def test_frozen_jvp_vs_reference_fd_jacobian():
+    from oracle.oracle import Oracle
+    cs = cases.cylinder_case(32, 16, noise=0.01)
+    o = Oracle(cs.raw, cs.opt)
+    o.residual(cs.W0, with_jacobian=True)
+    JL, JR, _ = o.jacobian()
+    m = o.metrics()
+    N = cs.n_cells
+    v = _direction(N, 9)
+    ref = np.zeros((N, 5))
+    for f in range(o.info.n_faces):
+        p, n = m["face_parent"][f], m["face_neighbor"][f]
+        d = JL[f].T @ v[p]
+        if m["face_bc"][f] < 0:
+            d = d + JR[f].T @ v[n]
+            ref[n] -= d
+        ref[p] += d
+    ns = _ns(cs)
+    Jv = ns.to_host(ns.jvp(ns.to_device(v), mode=A.JAC_JVP_FROZEN), 5)
+    scale = np.abs(ref).max(axis=0)
+    err = (np.abs(Jv - ref) / scale).max()
+    print("  frozen JVP vs FD Jacobian matvec: %.2e" % err, flush=True)
+    # the reference's absolute FD step (1.5e-7) is ~1e-12 of p ~ 1e5, so the
+    # FD Jacobian itself carries ~|F| eps_mach / 1.5e-7 ~ 1e-4 relative noise
+    assert err < 1e-3
+
+
+def test_jvp_linearity():
+    cs = cases.tetbox_case(4)
+    ns = _ns(cs)
+    u = ns.to_device(_direction(cs.n_cells, 1))
+    w = ns.to_device(_direction(cs.n_cells, 2))
+    Ju = ns.jvp(u.clone()).clone()
+    J2u = ns.jvp(2.0 * u).clone()
+    Jw = ns.jvp(w.clone()).clone()
+    Juw = ns.jvp(u + w).clone()
+    N = cs.n_cells
+    assert bool((J2u[:, :N] == 2.0 * Ju[:, :N]).all())
+    s = (Ju[:, :N].abs() + Jw[:, :N].abs()).max()
+    assert float(((Juw - Ju - Jw)[:, :N].abs().max() / s)) < 1e-12
+
+
+def test_first_order_inviscid_full_equals_frozen():
+    cs = cases.wedge_case(16, 8)
+    cs.opt.order = A.ORDER_FIRST
+    ns = _ns(cs)
+    v = ns.to_device(_direction(cs.n_cells, 3))
+    a = ns.jvp(v.clone(), mode=A.JAC_JVP_FULL).clone()
+    b = ns.jvp(v.clone(), mode=A.JAC_JVP_FROZEN).clone()
+    N = cs.n_cells
+    assert float((a - b)[:, :N].abs().max() / b[:, :N].abs().max()) < 1e-13
+
+
+@pytest.mark.parametrize("mode,fp32", [(A.JAC_JVP_FULL, False), (A.JAC_JVP_FULL, True), (A.JAC_JVP_FROZEN, True)])
+def test_matrix_free_newton_steps(mode, fp32):
+    cs = cases.wedge_case(48, 24)
+    ns = _ns(cs)
+    opts = A.newton_opts(cfl=10.0, jacobian=mode, fp32=fp32, restart=30, max_iters=30)
+    hist = [ns.newton_step(opts).res_norm for _ in range(10)]
+    print("  ||R||:", " ".join("%.3e" % h for h in hist), flush=True)
+    assert np.all(np.isfinite(hist)) and hist[-1] < hist[0]
