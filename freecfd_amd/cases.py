@@ -180,6 +180,23 @@ def smooth_perturbation(xyz, seed, amp=0.05, modes=8):
     return out
 
 
+def smooth_state(cs, seed=21, amp=0.02):
+    """Replace cs.W0 by the case's mean state times a seeded smooth
+    perturbation (8 Fourier modes per variable over the bounding box), so that
+    limiter switches (local extrema) are rare.  Returns cs."""
+    xc = meshgen.cell_centroids(cs.raw)
+    lo, hi = xc.min(axis=0), xc.max(axis=0)
+    d = smooth_perturbation((xc - lo) / np.maximum(hi - lo, 1e-30), seed, amp=amp)
+    W = cs.W0.mean(axis=0)[None, :] * np.ones((cs.n_cells, 1))
+    W[:, 0] *= 1.0 + d[:, 0]
+    W[:, 1] *= 1.0 + d[:, 1]
+    W[:, 2] += cs.meta["U"] * d[:, 2]
+    W[:, 3] += cs.meta["U"] * d[:, 3]
+    W[:, 4] *= 1.0 + d[:, 4]
+    cs.W0 = W
+    return cs
+
+
 def tetbox_case(n=64, seed=3, viscous=True):
     """Differentiation check: [0,1]^3 of n^3 hexes x 6 tets, Mach 0.5 freestream
     plus a seeded smooth 5% perturbation in rho, u, v, w, p; Roe, limited lsqr,

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "fcfd_adjoint.h"
 #include "fcfd_device.h"
 #include "fcfd_internal.h"
 #include "fcfd_jacobian.h"
@@ -39,6 +40,12 @@ struct fcfd_ctx {
     float *Dinv32 = nullptr, *D32 = nullptr, *JL32 = nullptr, *JR32 = nullptr, *b32 = nullptr, *x32 = nullptr;
     // tangent workspace (lazy)
     double *dG = nullptr, *dPHI = nullptr, *dFX = nullptr, *jv_in = nullptr, *jv_out = nullptr, *jv_R = nullptr;
+    // adjoint and loads (lazy)
+    bool adj_ready = false;
+    AdjGrid ag{};
+    AdjWork aw{};
+    int n_loadfaces = 0;
+    double *loads_tmp = nullptr, *loads_out = nullptr;
     std::string err;
     // optional per-stage timing: a ring of event quadruples recorded on the
     // launch stream around the three kernels, folded in lazily so that
@@ -133,6 +140,25 @@ void set_params(fcfd_ctx* c) {
     }
 }
 
+// Half-face lists in internal numbering: the faces of internal cell i are
+// hf[hp[i] .. hp[i+1]), in ascending reference face order, each the internal
+// face id when i is the face's parent and its complement ~id otherwise.
+void half_faces(const HostGrid& g, std::vector<int>& hp, std::vector<int>& hf) {
+    const int N = g.N;
+    hp.assign(N + 1, 0);
+    hf.clear();
+    hf.reserve(g.cfaces.size());
+    for (int i = 0; i < N; ++i) {
+        int r = g.iperm[i];
+        for (int k = g.cfptr[r]; k < g.cfptr[r + 1]; ++k) {
+            int f = g.cfaces[k];
+            int fi = g.fperm[f];
+            hf.push_back(g.fparent[f] == r ? fi : ~fi);
+        }
+        hp[i + 1] = (int)hf.size();
+    }
+}
+
 int build_device(fcfd_ctx* c) {
     HostGrid& g = c->hg;
     const int N = g.N, B = g.B, F = g.F, T = N + B;
@@ -193,17 +219,8 @@ int build_device(fcfd_ctx* c) {
     }
     // half-faces and flux blocks
     {
-        std::vector<int> hp(N + 1, 0), hf;
-        hf.reserve(g.cfaces.size());
-        for (int i = 0; i < N; ++i) {
-            int r = g.iperm[i];
-            for (int k = g.cfptr[r]; k < g.cfptr[r + 1]; ++k) {
-                int f = g.cfaces[k];
-                int fi = g.fperm[f];
-                hf.push_back(g.fparent[f] == r ? fi : ~fi);
-            }
-            hp[i + 1] = (int)hf.size();
-        }
+        std::vector<int> hp, hf;
+        half_faces(g, hp, hf);
         std::vector<int> blk{0};
         int start = 0;
         for (int i = 0; i < N; ++i) {
@@ -524,6 +541,103 @@ static int ensure_tangent(fcfd_ctx* c) {
         (rc = dalloc(c, (size_t)5 * g.F, &c->dFX)) || (rc = dalloc(c, (size_t)5 * g.ld, &c->jv_in)) ||
         (rc = dalloc(c, (size_t)5 * g.ld, &c->jv_out)) || (rc = dalloc(c, (size_t)5 * g.ld, &c->jv_R)))
         return rc;
+    return FCFD_OK;
+}
+
+// Reverse-sweep connectivity (half-face twins, ghost half-faces, transposed
+// gradient map), the per-BC load face lists and the adjoint workspace.
+static int ensure_adjoint(fcfd_ctx* c) {
+    if (c->adj_ready) return FCFD_OK;
+    const HostGrid& g = c->hg;
+    const DevGrid& d = c->dg;
+    const int N = g.N, B = g.B, F = g.F, T = N + B;
+    int rc;
+    std::vector<int> hp, hf;
+    half_faces(g, hp, hf);
+    const int nhf = (int)hf.size();
+    {
+        std::vector<int> par_h(F, -1), nei_h(F, -1);
+        for (int h = 0; h < nhf; ++h) {
+            if (hf[h] >= 0) par_h[hf[h]] = h;
+            else nei_h[~hf[h]] = h;
+        }
+        std::vector<int> twin(nhf), ghh(B);
+        for (int h = 0; h < nhf; ++h) twin[h] = hf[h] >= 0 ? nei_h[hf[h]] : par_h[~hf[h]];
+        for (int i = 0; i < B; ++i) ghh[i] = par_h[g.fperm[g.ghost_face[g.iperm[N + i] - N]]];
+        if ((rc = upload(c, twin, &c->ag.hf_twin)) || (rc = upload(c, ghh, &c->ag.gh_hf))) return rc;
+    }
+    {
+        std::vector<int> tp(T + 1, 0);
+        for (int i = 0; i < N; ++i) {
+            int r = g.iperm[i];
+            for (int e = g.gptr[r]; e < g.gptr[r + 1]; ++e) tp[g.perm[g.gidx[e]] + 1]++;
+        }
+        for (int j = 0; j < T; ++j) tp[j + 1] += tp[j];
+        std::vector<int> next(tp.begin(), tp.end() - 1), tc(tp[T]);
+        std::vector<double> wx(tp[T]), wy(tp[T]), wz(tp[T]);
+        for (int i = 0; i < N; ++i) {   // ascending internal cell: each list ends up in ascending c
+            int r = g.iperm[i];
+            for (int e = g.gptr[r]; e < g.gptr[r + 1]; ++e) {
+                int pos = next[g.perm[g.gidx[e]]]++;
+                tc[pos] = i;
+                wx[pos] = g.gw[3 * e]; wy[pos] = g.gw[3 * e + 1]; wz[pos] = g.gw[3 * e + 2];
+            }
+        }
+        if ((rc = upload(c, tp, &c->ag.tg_ptr)) || (rc = upload(c, tc, &c->ag.tg_cell)) || (rc = upload(c, wx, &c->ag.tgx)) ||
+            (rc = upload(c, wy, &c->ag.tgy)) || (rc = upload(c, wz, &c->ag.tgz)))
+            return rc;
+    }
+    {
+        const int nbc = c->opt.n_bc;
+        std::vector<int> lp(nbc + 1, 0), lf;
+        for (int b = 0; b < nbc; ++b) {
+            if (c->opt.bc[b].include_in_loads)
+                for (int f = 0; f < F; ++f)
+                    if (g.fbc[f] == b) lf.push_back(g.fperm[f]);
+            lp[b + 1] = (int)lf.size();
+        }
+        c->n_loadfaces = (int)lf.size();
+        c->ag.n_bc = nbc;
+        if ((rc = upload(c, lp, &c->ag.lf_ptr)) || (rc = upload(c, lf, &c->ag.lf_face)) ||
+            (rc = dalloc(c, (size_t)6 * std::max(c->n_loadfaces, 1), &c->loads_tmp)) ||
+            (rc = dalloc(c, (size_t)6 * std::max(nbc, 1), &c->loads_out)))
+            return rc;
+    }
+    const size_t nz = c->P.viscous ? 32 : 20;
+    if ((rc = dalloc(c, nz * F, &c->aw.Z)) || (rc = dalloc(c, (size_t)5 * d.ld, &c->aw.Wb1)) ||
+        (rc = dalloc(c, (size_t)15 * d.ld, &c->aw.Gb)) || (rc = dalloc(c, (size_t)5 * nhf, &c->aw.LH)) ||
+        (rc = dalloc(c, (size_t)5 * d.ld, &c->aw.Wb2)))
+        return rc;
+    c->adj_ready = true;
+    return FCFD_OK;
+}
+
+int fcfd_vjp(fcfd_ctx* c, double* W_dev, const double* Rbar_dev, const double* fx_weight_dev, double* Wbar_dev, void* stream) {
+    if (!c || !W_dev || !Rbar_dev || !Wbar_dev) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    int rc = ensure_adjoint(c);
+    if (rc) return rc;
+    launch_vjp(c->dg, c->ag, c->P, W_dev, c->G, c->PHI, c->dref, Rbar_dev, fx_weight_dev, c->aw, Wbar_dev, c->status,
+               (hipStream_t)stream);
+    HIPCHK(c, hipGetLastError());
+    return FCFD_OK;
+}
+
+int fcfd_loads(fcfd_ctx* c, double* W_dev, double* loads_host, void* stream) {
+    if (!c || !W_dev || !loads_host) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    int rc = ensure_adjoint(c);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    launch_ghosts(c->dg, c->P, W_dev, c->status, s);
+    launch_grad_lim(c->dg, c->P, W_dev, c->G, c->PHI, c->dref, s);
+    launch_loads(c->dg, c->ag, c->P, W_dev, c->G, c->PHI, c->opt.moment_center[0], c->opt.moment_center[1],
+                 c->opt.moment_center[2], c->loads_tmp, c->n_loadfaces, c->loads_out, c->status, s);
+    HIPCHK(c, hipGetLastError());
+    if (c->opt.n_bc > 0) {
+        HIPCHK(c, hipMemcpyAsync(loads_host, c->loads_out, sizeof(double) * 6 * c->opt.n_bc, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+    }
     return FCFD_OK;
 }
 

@@ -42,9 +42,13 @@ __device__ __forceinline__ Dual& operator+=(Dual& a, const Dual& b) { a = a + b;
 __device__ __forceinline__ Dual& operator-=(Dual& a, const Dual& b) { a = a - b; return a; }
 __device__ __forceinline__ Dual& operator*=(Dual& a, const Dual& b) { a = a * b; return a; }
 
+// At x = 0 the derivative of sqrt is unbounded; the code takes sqrt(0) only
+// for a magnitude |V| of a vector that is exactly zero (the no-slip wall face
+// of Steger-Warming), where x = V.V has a zero tangent: the tangent 0 is then
+// the exact derivative of |V|^2 and the zero subgradient of |V|.
 __device__ __forceinline__ Dual sq_root(const Dual& x) {
     double s = sqrt(x.v);
-    return Dual(s, x.d / (2.0 * s));
+    return Dual(s, s > 0.0 ? x.d / (2.0 * s) : 0.0);
 }
 __device__ __forceinline__ Dual powr(const Dual& x, double y) {   // y constant
     double p = pow(x.v, y);

@@ -81,8 +81,12 @@ struct FaceCtx {
     int p, n;
 };
 
-template <int FLUX, int VISC, typename S, typename Loader>
-__device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, const Loader& ld_, int f, FaceCtx<S>& X, int* status) {
+// Stage 1 of a face-loop iteration: geometry, the two cell loads and the
+// reconstruction.  Leaves X.L reconstructed (rho set, not finished); X.R
+// reconstructed for an interior face, or holding only the ghost's centre
+// values for a boundary face; X.fs.g* the face gradients when VISC.
+template <int VISC, typename S, typename Loader>
+__device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_, const Params& P, int f, FaceCtx<S>& X) {
     FaceGeo& F = X.F;
     load_geo(g, f, F);
     const int p = g.fpar[f], n = g.fnei[f];
@@ -94,28 +98,21 @@ __device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, con
     Vec<S> gp[5];
     ld_.load(p, false, q, gp, ph);
     recon(P, X.L, q, gp, ph, vsub(xf, xp));
-    finish_state(P, X.L, F);
     Vec<S> gn[5];
-    // right state: boundary (right_state_update :298-303, apply_bcs) or
-    // interior reconstruction; both branches produce a whole state value
     {
         S qn[5], phn[5];
         const bool bnd = F.bc >= 0;
         ld_.load(n, bnd, qn, gn, phn);
         CState<S> Rb;
         if (bnd) {
-            Rb = X.L;
             Rb.p_center = qn[0];
             Rb.V_center = mk<S>(qn[1], qn[2], qn[3]);
             Rb.T_center = qn[4];
-            if (apply_bcs(P, g.bc[F.bc], X.L, Rb, F)) atomicOr(status, 1);
         } else {
             recon(P, Rb, qn, gn, phn, vsub(xf, xn));
         }
         X.R = Rb;
     }
-    finish_state(P, X.R, F);
-    convective_flux<FLUX>(P, X.L, X.R, F, FLUX == FCFD_FLUX_SW ? g.fdiss[f] : 0.0, FLUX == FCFD_FLUX_SW ? g.fwd[f] : 0.0, X.C);
     if (VISC) {
         if (F.bc >= 0) {
             X.fs.gu = gp[1]; X.fs.gv = gp[2]; X.fs.gw = gp[3]; X.fs.gT = gp[4];
@@ -126,12 +123,45 @@ __device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, con
             X.fs.gw = vadd(vsc(wp, gp[3]), vsc(wn, gn[3]));
             X.fs.gT = vadd(vsc(wp, gp[4]), vsc(wn, gn[4]));
         }
+    }
+}
+
+// Stage 2: finish the states, apply the boundary condition (right state of
+// a boundary face = a copy of the finished left state with the ghost's
+// centre values, then apply_bcs; right_state_update :298-303) and evaluate
+// the convective and diffusive fluxes.
+template <int FLUX, int VISC, typename S>
+__device__ __forceinline__ void face_tail(const DevGrid& g, const Params& P, int f, FaceCtx<S>& X, int* status) {
+    FaceGeo& F = X.F;
+    finish_state(P, X.L, F);
+    {
+        CState<S> Rb;
+        if (F.bc >= 0) {
+            Rb = X.L;
+            Rb.p_center = X.R.p_center;
+            Rb.V_center = X.R.V_center;
+            Rb.T_center = X.R.T_center;
+            if (apply_bcs(P, g.bc[F.bc], X.L, Rb, F)) atomicOr(status, 1);
+        } else {
+            Rb = X.R;
+        }
+        X.R = Rb;
+    }
+    finish_state(P, X.R, F);
+    convective_flux<FLUX>(P, X.L, X.R, F, FLUX == FCFD_FLUX_SW ? g.fdiss[f] : 0.0, FLUX == FCFD_FLUX_SW ? g.fwd[f] : 0.0, X.C);
+    if (VISC) {
         face_state(P, X.L, X.R, F, X.fs);
         diffusive_flux(X.fs, F, g.bc, X.D);
     } else {
 #pragma unroll
         for (int v = 0; v < 5; ++v) X.D[v] = S(0.0);
     }
+}
+
+template <int FLUX, int VISC, typename S, typename Loader>
+__device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, const Loader& ld_, int f, FaceCtx<S>& X, int* status) {
+    face_inputs<VISC>(g, ld_, P, f, X);
+    face_tail<FLUX, VISC>(g, P, f, X, status);
 }
 
 }  // namespace fcfd

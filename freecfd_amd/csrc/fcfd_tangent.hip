@@ -15,9 +15,18 @@
 #include <hip/hip_runtime.h>
 
 #include "fcfd_face.h"
+#include "fcfd_limiter.h"
 #include "fcfd_tangent.h"
 
 namespace fcfd {
+
+// neighbour states with their tangents, for limiter_phi
+struct TangentNb {
+    const double* __restrict__ W;
+    const double* __restrict__ dW;
+    size_t ld;
+    __device__ __forceinline__ Dual nb(int, int j, int v) const { return Dual(W[v * ld + j], dW[v * ld + j]); }
+};
 
 __global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, double* __restrict__ W, double* __restrict__ dW,
                                                       int* __restrict__ status) {
@@ -76,64 +85,8 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
     Dual q0[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) q0[v] = Dual(W[v * ld + c], dW[v * ld + c]);
-    Dual phi[5] = {Dual(1.), Dual(1.), Dual(1.), Dual(1.), Dual(1.)};
-    Vec<double> xc = centroid(g, c);
-    if (LIM == FCFD_LIMITER_MINMOD) {
-        const double small = 1.e-12;
-        for (int h = h0; h < h1; ++h) {
-            int fs = g.hf_face[h];
-            int f = fs >= 0 ? fs : ~fs;
-            int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
-            Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                Dual dU = Dual(W[v * ld + nb], dW[v * ld + nb]) - q0[v];
-                Dual dUg = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
-                Dual mm = (dU.v * dUg.v < 0.) ? Dual(0.) : (dU.v > 0.) ? smin(dU, dUg) : smax(dU, dUg);
-                double sU = (double)((mm.v > 0) - (mm.v < 0)), sG = (double)((dUg.v > 0) - (dUg.v < 0));
-                phi[v] = smin(phi[v], (mm + sU * small) / (dUg + sG * small));
-            }
-        }
-    } else if (LIM == FCFD_LIMITER_BJ || LIM == FCFD_LIMITER_VK) {
-        Dual umax[5], umin[5];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) umax[v] = umin[v] = q0[v];
-        for (int h = h0; h < h1; ++h) {
-            int fs = g.hf_face[h];
-            int f = fs >= 0 ? fs : ~fs;
-            int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                Dual q(W[v * ld + nb], dW[v * ld + nb]);
-                umax[v] = smax(umax[v], q);
-                umin[v] = smin(umin[v], q);
-            }
-        }
-        for (int h = h0; h < h1; ++h) {
-            int fs = g.hf_face[h];
-            int f = fs >= 0 ? fs : ~fs;
-            Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                Dual dM = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
-                Dual dP = q0[v];
-                if (LIM == FCFD_LIMITER_BJ) {
-                    if (dM.v > 0.) dP = umax[v] - dP;
-                    else if (dM.v < 0.) dP = umin[v] - dP;
-                    else { dP = Dual(1.); dM = Dual(1.); }
-                    phi[v] = smin(phi[v], dP / dM);
-                } else {
-                    double eps = dref[v] / P.glscale;
-                    if (dM.v > 0.) dP = umax[v] - dP;
-                    else if (dM.v < 0.) dP = umin[v] - dP;
-                    Dual dP2 = dP * dP, dM2 = dM * dM;
-                    double e = 1.e-6 * P.limiter_threshold * eps;
-                    double eps2 = e * e;
-                    if (dM.v != 0.) phi[v] = smin(phi[v], ((dP2 + eps2) + 2. * dM * dP) / (dP2 + 2. * dM2 + dM * dP + eps2));
-                }
-            }
-        }
-    }
+    Dual phi[5];
+    limiter_phi<LIM>(g, P, TangentNb{W, dW, ld}, c, q0, gr, dref, phi);
     if (LIM != FCFD_LIMITER_NONE) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) { PHI[v * ld + c] = phi[v].v; dPHI[v * ld + c] = phi[v].d; }

@@ -144,6 +144,30 @@ class NavierStokes(Grid):
         _check(self.h, lib().fcfd_jvp(self.h, W.data_ptr(), dW.data_ptr(), out.data_ptr(), int(mode), C.c_void_p(s)))
         return out
 
+    def vjp(self, Rbar, fx_weight=None, W=None, out=None):
+        """Wbar = (dR/dW)^T Rbar [+ d/dW sum_f fx_weight[bc(f)] . (D - C)_f] on
+        the GPU; Rbar a [5][ld] float64 device tensor, fx_weight an optional
+        (n_bc, 5) array (host or device)."""
+        W = self.W if W is None else W
+        out = self.torch.zeros_like(self.W) if out is None else out
+        fw = None
+        if fx_weight is not None:
+            fw = self.torch.as_tensor(np.asarray(fx_weight, dtype=np.float64).reshape(-1), device=self.W.device).contiguous()
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_vjp(self.h, W.data_ptr(), Rbar.data_ptr(), None if fw is None else fw.data_ptr(),
+                                      out.data_ptr(), C.c_void_p(s)))
+        return out
+
+    def loads(self, W=None):
+        """(n_bc, 6) surface loads (Fx Fy Fz Mx My Mz) of the state, reference
+        face-order sums over the BCs with include_in_loads."""
+        W = self.W if W is None else W
+        n_bc = max(int(self._opt.n_bc), 1)
+        out = np.zeros(6 * n_bc)
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_loads(self.h, W.data_ptr(), out.ctypes.data, C.c_void_p(s)))
+        return out.reshape(n_bc, 6)[: int(self._opt.n_bc)]
+
     def to_device(self, X_host):
         """N x 5 host array (reference order) -> [5][ld] device tensor."""
         X_host = np.ascontiguousarray(X_host, dtype=np.float64)

@@ -22,6 +22,9 @@
  *                   update_time_step (src/time_step.cc:143)       -> fcfd_time_step
  *                   petsc_solve + update_variables (ns_petsc_functions.cc:85, ns.cc:338)
  *                                                                  -> fcfd_newton_step
+ *                   surface loads of the face loop (ns_assemble_linear_system.cc:109-119)
+ *                                                                  -> fcfd_loads
+ *   build-only      tangent and adjoint of the residual           -> fcfd_jvp, fcfd_vjp
  *
  * Numbering.  Host arrays passed to or returned by fcfd_* in "reference
  * numbering" use the cell, face and node order the reference's Grid::setup
@@ -292,6 +295,27 @@ int  fcfd_newton_step(fcfd_ctx* ctx, double* W_dev, const fcfd_newton_opts* opts
 /* Local time step (update_time_step, src/time_step.cc:143-224) for
  * CFL_LOCAL or CFL_MAX; dt_dev is [ld_cell]; *max_cfl as the reference. */
 int  fcfd_time_step(fcfd_ctx* ctx, const double* W_dev, int dt_type, double cfl, double* dt_dev, double* max_cfl, void* stream);
+
+/* Adjoint (vector-Jacobian product) of the residual, the exact transpose of
+ * fcfd_jvp's FULL mode:
+ *   Wbar = (dR/dW)^T Rbar + d/dW sum_{boundary faces f} fx_weight[bc(f)] . (D - C)_f
+ * where D - C is the face's contribution to its parent cell's residual.  The
+ * second term gives the state adjoint of a surface-load functional: the
+ * force on BC b is -sum_f (D - C)_f[1..3], so a drag direction e on a wall
+ * BC is fx_weight[b] = (0, -e_x, -e_y, -e_z, 0).  Rbar_dev and Wbar_dev are
+ * [5][ld_cell] device arrays (only interior cells of Rbar are read);
+ * fx_weight_dev is a device [n_bc][5] array or NULL.  The ghost slots of
+ * W_dev are written.  Deterministic (no atomics).  No counterpart in the
+ * reference, which has no adjoint. */
+int  fcfd_vjp(fcfd_ctx* ctx, double* W_dev, const double* Rbar_dev, const double* fx_weight_dev, double* Wbar_dev, void* stream);
+
+/* Surface loads of the state W_dev (ns_assemble_linear_system.cc:109-119,
+ * loads.h:33-39): for every BC with include_in_loads, force = sum over its
+ * faces of (C - D)[1..3] and moment = sum (x_f - moment_center) x force_f,
+ * summed in reference face order; BCs not included report zeros.  Writes
+ * 6 * n_bc doubles (Fx Fy Fz Mx My Mz per BC) to loads_host after
+ * synchronising the stream.  The ghost slots of W_dev are written. */
+int  fcfd_loads(fcfd_ctx* ctx, double* W_dev, double* loads_host, void* stream);
 
 #ifdef __cplusplus
 }
