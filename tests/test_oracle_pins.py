@@ -24,7 +24,11 @@ reference's own source text fixes:
   - FD Jacobian: first-order inviscid diagonal blocks = -dR_P/dW_P by
     independent finite differences of the whole residual;
   - time terms: P = dQ/dW against finite differences of the conservative
-    variables; local time step closed form.
+    variables; local time step closed form;
+  - surface loads: a gas at rest on the 15 degree ramp gives the closed-form
+    pressure force and moment on the ramp wall (the face-centroid sums are
+    exact for piecewise-linear walls), and the loads over every boundary of
+    the closed domain sum to zero.
 """
 import numpy as np
 import pytest
@@ -421,3 +425,46 @@ def test_local_time_step_closed_form(oracle_lib):
     a = np.sqrt(g * R * W[:, 4])
     assert np.allclose(dt, 10.0 * ls / (np.linalg.norm(W[:, 1:4], axis=1) + a), rtol=1e-14)
     assert mc == 10.0
+
+
+# ---------------------------------------------------------------------------
+# surface loads (ns_assemble_linear_system.cc:109-119)
+# ---------------------------------------------------------------------------
+
+def _wedge_at_rest(loads_bcs, flux=A.FLUX_ROE):
+    """Gas at rest on the wedge (0.2 is a grid line, so the ramp kink is a
+    node), inlet/outlet carrying the same state."""
+    ni, nj, dz = 40, 10, 0.01
+    raw = meshgen.wedge(ni, nj, dz=dz)
+    R = cases.gas_R(28.97)
+    p0, T0 = 101325.0, 300.0
+    bcs = [cases.bc_inlet_state(p0, T0, (0.0, 0.0, 0.0), R), cases.bc_outlet_extrapolate(), cases.bc_wall(slip=True),
+           cases.bc_wall(slip=True), cases.bc_symmetry(), cases.bc_symmetry()]
+    opt = cases.make_options(bcs, flux=flux, limiter=A.LIMITER_MINMOD, grad=(A.GRAD_LSQR,) * 3, loads_bcs=loads_bcs)
+    o = _oracle(raw, opt)
+    W = cases.freestream_state(o.info.n_cells, p0, T0, (0.0, 0.0, 0.0))
+    o.residual(W)
+    return o.loads(), p0, dz
+
+
+@pytest.mark.parametrize("flux", [A.FLUX_ROE, A.FLUX_AUSM_PLUS_UP, A.FLUX_VAN_LEER])
+def test_loads_pressure_on_ramp_closed_form(flux, oracle_lib):
+    L, p0, dz = _wedge_at_rest((2,), flux)
+    t = np.tan(np.radians(15.0))
+    # force = p0 * sum(n A) over the bottom wall: flat 0.2 long (n = -y), ramp
+    # 0.8 long in x rising 0.8 tan15 (n A = (0.8 tan15, -0.8) dz)
+    F = p0 * dz * np.array([0.8 * t, -1.0, 0.0])
+    # moment about the origin of the face forces at the face centroids (z = dz/2)
+    Mz = p0 * dz * (-0.02 - 0.48 - 0.32 * t * t)
+    M = np.array([-(dz / 2) * F[1], (dz / 2) * F[0], Mz])
+    ref = np.concatenate([F, M])
+    assert np.abs(L[2] - ref).max() <= 1e-12 * np.abs(ref).max()
+    others = np.delete(L, 2, axis=0)
+    assert np.all(others == 0.0)   # BCs not in the loads report nothing
+
+
+def test_loads_closed_domain_sum_zero(oracle_lib):
+    L, p0, dz = _wedge_at_rest(tuple(range(6)))
+    F = L[:, :3].sum(axis=0)
+    scale = np.abs(L[:, :3]).sum()
+    assert scale > 0 and np.abs(F).max() <= 1e-13 * scale
