@@ -13,6 +13,7 @@ SPEC_NONE, SPEC_RHO, SPEC_P, SPEC_T, SPEC_STATE = 0, 1, 2, 3, 4
 LIMITER_NONE, LIMITER_VK, LIMITER_BJ, LIMITER_MINMOD = -1, 1, 2, 3
 ORDER_FIRST, ORDER_SECOND = 1, 2
 FLUX_ROE, FLUX_VAN_LEER, FLUX_AUSM_PLUS_UP, FLUX_SD_SLAU, FLUX_SW = 1, 2, 3, 4, 5
+TURB_NONE, TURB_SA = 0, 1
 GRAD_CURVILINEAR, GRAD_LSQR, GRAD_GREENGAUSS = 1, 2, 3
 INTERP_WTLI, INTERP_IDW, INTERP_SIMPLE = 1, 2, 3
 VISC_CONSTANT, VISC_SUTHERLAND = 1, 2
@@ -38,7 +39,7 @@ class BC(C.Structure):
                 ("qdot_fixed", C.c_int),
                 ("p", C.c_double), ("T", C.c_double), ("rho", C.c_double), ("V", C.c_double * 3),
                 ("p_total", C.c_double), ("T_total", C.c_double), ("mdot", C.c_double), ("qdot", C.c_double),
-                ("include_in_loads", C.c_int)]
+                ("include_in_loads", C.c_int), ("nut", C.c_double)]
 
 
 class Options(C.Structure):
@@ -52,7 +53,7 @@ class Options(C.Structure):
                 ("grad_hex", C.c_int), ("grad_prism", C.c_int), ("grad_other", C.c_int),
                 ("interp_method", C.c_int), ("stencil_size", C.c_int), ("skewness_tol", C.c_double),
                 ("moment_center", C.c_double * 3),
-                ("n_bc", C.c_int), ("bc", BC * MAX_BC)]
+                ("n_bc", C.c_int), ("bc", BC * MAX_BC), ("turbulence", C.c_int), ("Pr_t", C.c_double)]
 
 
 class Info(C.Structure):
@@ -74,7 +75,7 @@ class NewtonStats(C.Structure):
     _fields_ = [("res_norm", C.c_double), ("update_norms", C.c_double * 3), ("krylov_iters", C.c_int),
                 ("krylov_converged", C.c_int), ("krylov_rnorm", C.c_double), ("krylov_r0norm", C.c_double),
                 ("ms_residual", C.c_double), ("ms_jacobian", C.c_double), ("ms_krylov", C.c_double),
-                ("ms_update", C.c_double)]
+                ("ms_update", C.c_double), ("nut_norm", C.c_double)]
 
 
 def newton_opts(cfl=10.0, jacobian=JAC_FD, fp32=False, restart=30, max_iters=10, rtol=1e-6, abstol=1e-12, knoll=True):

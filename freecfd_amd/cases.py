@@ -64,7 +64,7 @@ def make_options(bcs, gamma=1.4, Mw=28.97, Pref=0.0, Tref=0.0, mu=0.0, lam=0.0, 
                  order=A.ORDER_SECOND, jac_order=A.ORDER_FIRST, limiter=A.LIMITER_VK,
                  flux=A.FLUX_AUSM_PLUS_UP, limiter_threshold=0.0, Minf=1.0, wdiss=0.3, bl_height=0.0,
                  grad=(A.GRAD_CURVILINEAR, A.GRAD_CURVILINEAR, A.GRAD_LSQR), interp=A.INTERP_WTLI,
-                 stencil=2, loads_bcs=(), moment_center=(0.0, 0.0, 0.0)):
+                 stencil=2, loads_bcs=(), moment_center=(0.0, 0.0, 0.0), turbulence=A.TURB_NONE, Pr_t=0.9):
     """Defaults are the reference's (src/read_inputs.cc:47-54,70-75,125-134)."""
     o = A.Options()
     o.gamma, o.Mw, o.Pref, o.Tref = gamma, Mw, Pref, Tref
@@ -95,6 +95,8 @@ def make_options(bcs, gamma=1.4, Mw=28.97, Pref=0.0, Tref=0.0, mu=0.0, lam=0.0, 
         for q in range(3):
             bc.V[q] = V[q]
         bc.include_in_loads = int(b in loads_bcs)
+        bc.nut = float(d.get("nut", 0.0))
+    o.turbulence, o.Pr_t = turbulence, Pr_t
     return o
 
 
@@ -193,6 +195,8 @@ def smooth_state(cs, seed=21, amp=0.02):
     W[:, 2] += cs.meta["U"] * d[:, 2]
     W[:, 3] += cs.meta["U"] * d[:, 3]
     W[:, 4] *= 1.0 + d[:, 4]
+    if W.shape[1] == 6:   # SA nu~
+        W[:, 5] *= 1.0 + smooth_perturbation((xc - lo) / np.maximum(hi - lo, 1e-30), seed + 7, amp=amp)[:, 0]
     cs.W0 = W
     return cs
 
@@ -247,9 +251,10 @@ def bump_case(ni=256, nj=96, nk=64, seed=4, noise=0.0):
     return Case("bump", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.85))
 
 
-def plate_case(ni=512, nj=128, nk=64, seed=5, noise=0.0, first=5.5e-6):
-    """Turbulent flat plate geometry (Mach 0.3, Re_L = 5e6), hex layers with the
-    top 20% split into prisms; laminar viscous Roe until the SA model lands."""
+def plate_case(ni=512, nj=128, nk=64, seed=5, noise=0.0, first=5.5e-6, sa=False):
+    """Turbulent flat plate (Mach 0.3, Re_L = 5e6), hex layers with the top 20%
+    split into prisms, viscous Roe; sa=True adds the Spalart-Allmaras variable
+    nu~ (freestream and inlet nu~ = 3 nu) as the sixth equation."""
     raw = meshgen.flat_plate(ni, nj, nk, first=first)
     Mw, g = 28.97, 1.4
     R = gas_R(Mw)
@@ -257,15 +262,21 @@ def plate_case(ni=512, nj=128, nk=64, seed=5, noise=0.0, first=5.5e-6):
     U = 0.3 * np.sqrt(g * R * T)
     rho = p / (R * T)
     mu = rho * U * 1.0 / 5.0e6
-    bcs = [bc_inlet_T(T, (U, 0.0, 0.0)), bc_outlet_p(p), bc_wall(slip=False), bc_outlet_p(p),
-           bc_symmetry(), bc_symmetry()]
+    nut_inf = 3.0 * mu / rho
+    inlet = bc_inlet_T(T, (U, 0.0, 0.0))
+    inlet["nut"] = nut_inf
+    bcs = [inlet, bc_outlet_p(p), bc_wall(slip=False), bc_outlet_p(p), bc_symmetry(), bc_symmetry()]
     opt = make_options(bcs, gamma=g, Mw=Mw, mu=mu, Pr=0.72, flux=A.FLUX_ROE, limiter=A.LIMITER_VK,
-                       Minf=0.3, loads_bcs=(2,))
+                       Minf=0.3, loads_bcs=(2,), turbulence=A.TURB_SA if sa else A.TURB_NONE)
     n = len(raw["cell_ptr"]) - 1
     W = freestream_state(n, p, T, (U, 0.0, 0.0))
+    if sa:
+        W = np.concatenate([W, np.full((n, 1), nut_inf)], axis=1)
     if noise:
         rng = np.random.default_rng(seed)
         W[:, 0] *= 1.0 + rng.uniform(-noise, noise, n)
         W[:, 1] *= 1.0 + rng.uniform(-noise, noise, n)
         W[:, 4] *= 1.0 + rng.uniform(-noise, noise, n)
-    return Case("plate", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.3, mu=mu))
+        if sa:
+            W[:, 5] *= 1.0 + rng.uniform(-noise, noise, n)
+    return Case("plate_sa" if sa else "plate", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.3, mu=mu, nut=nut_inf))

@@ -104,6 +104,10 @@ extern "C" {
 #define FCFD_DT_CFL_MAX   2
 #define FCFD_DT_CFL_LOCAL 3
 
+/* turbulence model (build-only) */
+#define FCFD_TURB_NONE 0
+#define FCFD_TURB_SA   1
+
 #define FCFD_MAX_BC 32
 
 /* Raw unstructured mesh, the content of the reference's GridRawData for
@@ -132,6 +136,7 @@ typedef struct fcfd_bc {
     int p_fixed, T_fixed, rho_fixed, V_fixed, qdot_fixed;  /* Variable::fixedonBC */
     double p, T, rho, V[3], p_total, T_total, mdot, qdot;
     int include_in_loads;
+    double nut;       /* SA working variable nu~ at an inlet (m^2/s); build-only */
 } fcfd_bc;
 
 typedef struct fcfd_options {
@@ -150,6 +155,10 @@ typedef struct fcfd_options {
     double moment_center[3];
     int n_bc;
     fcfd_bc bc[FCFD_MAX_BC];
+    /* turbulence (build-only; the reference's own RANS path is segregated
+     * k-omega/SST): FCFD_TURB_SA adds nu~ as a sixth variable, coupled */
+    int turbulence;
+    double Pr_t;      /* turbulent Prandtl number (the reference's turbulentPr) */
 } fcfd_options;
 
 typedef struct fcfd_info {
@@ -272,6 +281,7 @@ typedef struct fcfd_newton_stats {
     int krylov_iters, krylov_converged;
     double krylov_rnorm, krylov_r0norm;
     double ms_residual, ms_jacobian, ms_krylov, ms_update;
+    double nut_norm;          /* sqrt(sum dnu~^2/dt^2) with the SA model, else 0 */
 } fcfd_newton_stats;
 
 /* Tangent (Jacobian-vector product) of the residual: dR = (dR/dW) dW, exact

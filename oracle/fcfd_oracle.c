@@ -77,17 +77,21 @@ static int vm_insert(vmap* m, int k, V3 v) { /* std::map::insert: no-op if prese
 /* ------------------------------------------------------------------ */
 /* context                                                            */
 /* ------------------------------------------------------------------ */
-typedef struct {   /* NS_Cell_State (ns_state_cache.h:26-48) */
+#define NVM 6   /* internal stride of per-cell variable arrays: p,u,v,w,T (+ nu~ with SA) */
+typedef struct {   /* NS_Cell_State (ns_state_cache.h:26-48) + SA nu~ */
     double p, p_center, T, T_center, rho, a, H, volume;
     V3 V, V_center, Vn;
-    double update[5];
+    double update[NVM];
+    double nut, nut_center;
 } cstate;
-typedef struct {   /* NS_Face_State (ns_state_cache.h:50-59) */
+typedef struct {   /* NS_Face_State (ns_state_cache.h:50-59) + SA face quantities */
     int index, parent, neighbor;
     double T, mu, lambda;
     V3 V, gradu, gradv, gradw, gradT, normal, tangent1, tangent2, left2right;
     double area;
     int bc;
+    V3 gradn;
+    double nut, rho, mu_t;
 } fstate;
 
 struct orc_ctx {
@@ -114,12 +118,15 @@ struct orc_ctx {
     int *avg_ptr, *avg_idx; double* avg_w;
     int *g_ptr, *g_idx; V3* g_w;
     /* state */
+    int nv;                         /* 5, or 6 with the SA model */
     double *p, *u, *v, *w, *T;      /* ntot */
+    double* nut;                    /* ntot (SA) */
     V3 *gp, *gu, *gv, *gw, *gT;     /* ntot */
-    double* phi;                    /* ntot*5 */
-    double* upd;                    /* ncell*5 */
-    double *Rs, *Rabs;              /* ncell*5 */
-    double *JL, *JR, *Dg;           /* F*25, F*25, N*25 */
+    V3* gn;                         /* ntot (SA) */
+    double* phi;                    /* ntot*NVM */
+    double* upd;                    /* ncell*NVM */
+    double *Rs, *Rabs;              /* ncell*NVM */
+    double *JL, *JR, *Dg;           /* F*NVM^2 [var][eq], F*NVM^2, N*NVM^2 [row][col] */
     double* loads;                  /* nbc*6 */
     int err; char msg[256];
 };
@@ -146,6 +153,71 @@ static double m_visc(const orc_ctx* o, double T) {
 static double m_cond(const orc_ctx* o, double T) {
     if (o->opt.cond_model == FCFD_COND_PRANDTL) return o->Cp * m_visc(o, T) / o->opt.Pr;
     return o->opt.lambda;
+}
+
+/* ------------------------------------------------------------------ */
+/* Spalart-Allmaras model (build-only; the reference has none).        */
+/* SA without ft2, with the S~ modification and the negative-nu~ branch */
+/* of Allmaras, Johnson & Spalart (ICCFD7-1902, 2012), compressible    */
+/* form without the grad(rho).grad(nu~) term:                           */
+/*   d(rho nu~)/dt + div(rho V nu~) = rho (P - D)                       */
+/*       + (1/sigma) [div((mu + rho nu~ fn) grad nu~) + cb2 rho |grad nu~|^2] */
+/* ------------------------------------------------------------------ */
+static const double sa_cb1 = 0.1355, sa_sigma = 2. / 3., sa_cb2 = 0.622, sa_kappa = 0.41, sa_cw2 = 0.3, sa_cw3 = 2.,
+                    sa_cv1 = 7.1, sa_ct3 = 1.2, sa_cv2 = 0.7, sa_cv3 = 0.9, sa_cn1 = 16.;
+/* scale of nu~ for the FD step: the largest |nu~| prescribed at a BC, 1 if none */
+static double sa_nut_scale(const orc_ctx* o) {
+    double s = 0.;
+    for (int b = 0; b < o->nbc; ++b) s = dmax(s, fabs(o->opt.bc[b].nut));
+    return s > 0. ? s : 1.;
+}
+static double sa_cw1(void) { return sa_cb1 / (sa_kappa * sa_kappa) + (1. + sa_cb2) / sa_sigma; }
+/* fv1 = chi^3 / (chi^3 + cv1^3) */
+static double sa_fv1(double chi) { double c3 = chi * chi * chi; return c3 / (c3 + sa_cv1 * sa_cv1 * sa_cv1); }
+/* eddy viscosity mu_t = rho nu~ fv1 (nu~ >= 0), 0 otherwise */
+static double sa_mut(double rho, double nut, double mu) {
+    if (!(nut > 0.)) return 0.;
+    double chi = rho * nut / mu;
+    return rho * nut * sa_fv1(chi);
+}
+/* diffusion coefficient factor fn (1 for nu~ >= 0) */
+static double sa_fn(double rho, double nut, double mu) {
+    if (nut >= 0.) return 1.;
+    double chi = rho * nut / mu;
+    double c3 = chi * chi * chi;
+    return (sa_cn1 + c3) / (sa_cn1 - c3);
+}
+/* source per unit volume: rho (P - D) + (cb2/sigma) rho |grad nu~|^2, from
+ * the cell state, the vorticity magnitude of the cell velocity gradient and
+ * the wall distance d */
+static double sa_source(double rho, double mu, double nut, V3 gu, V3 gv, V3 gw, V3 gn, double d, double* prod, double* dest) {
+    double wx = gw.c[1] - gv.c[2], wy = gu.c[2] - gw.c[0], wz = gv.c[0] - gu.c[1];
+    double Om = sqrt(wx * wx + wy * wy + wz * wz);
+    double P, D;
+    if (nut >= 0.) {
+        double chi = rho * nut / mu;
+        double fv1 = sa_fv1(chi);
+        double fv2 = 1. - chi / (1. + chi * fv1);
+        double k2d2 = sa_kappa * sa_kappa * d * d;
+        double Sbar = nut * fv2 / k2d2;
+        double St;
+        if (Sbar >= -sa_cv2 * Om) St = Om + Sbar;
+        else St = Om + Om * (sa_cv2 * sa_cv2 * Om + sa_cv3 * Sbar) / ((sa_cv3 - 2. * sa_cv2) * Om - Sbar);
+        double r = (St > 0.) ? dmin(nut / (St * k2d2), 10.) : 10.;
+        double r2 = r * r;
+        double g = r + sa_cw2 * (r2 * r2 * r2 - r);
+        double g2 = g * g, c2 = sa_cw3 * sa_cw3;
+        double c6 = c2 * c2 * c2;
+        double fw = g * pow((1. + c6) / (g2 * g2 * g2 + c6), 1. / 6.);
+        P = sa_cb1 * St * nut;
+        D = sa_cw1() * fw * (nut / d) * (nut / d);
+    } else {
+        P = sa_cb1 * (1. - sa_ct3) * Om * nut;
+        D = -1. * sa_cw1() * (nut / d) * (nut / d);
+    }
+    if (prod) *prod = P;
+    if (dest) *dest = D;
+    return rho * (P - D) + sa_cb2 / sa_sigma * rho * vdot(gn, gn);
 }
 
 /* ------------------------------------------------------------------ */
@@ -762,11 +834,14 @@ orc_ctx* orc_create(const fcfd_mesh* mesh, const fcfd_options* opt) {
     o->T = (double*)calloc(nt, sizeof(double));
     o->gp = (V3*)calloc(nt, sizeof(V3)); o->gu = (V3*)calloc(nt, sizeof(V3)); o->gv = (V3*)calloc(nt, sizeof(V3));
     o->gw = (V3*)calloc(nt, sizeof(V3)); o->gT = (V3*)calloc(nt, sizeof(V3));
-    o->phi = (double*)malloc(sizeof(double) * nt * 5);
-    for (int i = 0; i < nt * 5; ++i) o->phi[i] = (opt->order == FCFD_ORDER_FIRST) ? 0. : 1.;   /* ns.cc:282-287 */
-    o->upd = (double*)calloc((size_t)o->ncell * 5, sizeof(double));
-    o->Rs = (double*)calloc((size_t)o->ncell * 5, sizeof(double));
-    o->Rabs = (double*)calloc((size_t)o->ncell * 5, sizeof(double));
+    o->nv = (opt->turbulence == FCFD_TURB_SA) ? 6 : 5;
+    o->nut = (double*)calloc(nt, sizeof(double));
+    o->gn = (V3*)calloc(nt, sizeof(V3));
+    o->phi = (double*)malloc(sizeof(double) * nt * NVM);
+    for (int i = 0; i < nt * NVM; ++i) o->phi[i] = (opt->order == FCFD_ORDER_FIRST) ? 0. : 1.;   /* ns.cc:282-287 */
+    o->upd = (double*)calloc((size_t)o->ncell * NVM, sizeof(double));
+    o->Rs = (double*)calloc((size_t)o->ncell * NVM, sizeof(double));
+    o->Rabs = (double*)calloc((size_t)o->ncell * NVM, sizeof(double));
     o->loads = (double*)calloc((size_t)(o->nbc + 1) * 6, sizeof(double));
     return o;
 }
@@ -784,7 +859,7 @@ void orc_destroy(orc_ctx* o) {
     free(o->bc_area); free(o->bc_total_area);
     free(o->avg_ptr); free(o->avg_idx); free(o->avg_w); free(o->g_ptr); free(o->g_idx); free(o->g_w);
     free(o->p); free(o->u); free(o->v); free(o->w); free(o->T);
-    free(o->gp); free(o->gu); free(o->gv); free(o->gw); free(o->gT);
+    free(o->gp); free(o->gu); free(o->gv); free(o->gw); free(o->gT); free(o->nut); free(o->gn);
     free(o->phi); free(o->upd); free(o->Rs); free(o->Rabs); free(o->JL); free(o->JR); free(o->Dg); free(o->loads);
     free(o);
 }
@@ -799,7 +874,7 @@ int orc_info(const orc_ctx* o, fcfd_info* info) {
     info->n_nbr = nn;
     int hf = 0; for (int c = 0; c < o->ncell; ++c) hf += o->cfaces[c].n;
     info->n_halffaces = hf;
-    info->nv = 5;
+    info->nv = o->nv;
     return 0;
 }
 
@@ -864,19 +939,19 @@ int orc_metrics(const orc_ctx* o, fcfd_metrics_out* m) {
 /* ------------------------------------------------------------------ */
 /* state                                                              */
 /* ------------------------------------------------------------------ */
-int orc_set_state(orc_ctx* o, const double* W) {
-    for (int c = 0; c < o->ncell; ++c) {
-        o->p[c] = W[5 * c]; o->u[c] = W[5 * c + 1]; o->v[c] = W[5 * c + 2]; o->w[c] = W[5 * c + 3]; o->T[c] = W[5 * c + 4];
+static void load_state(orc_ctx* o, const double* W, int n) {
+    const int nv = o->nv;
+    for (int c = 0; c < n; ++c) {
+        o->p[c] = W[nv * c]; o->u[c] = W[nv * c + 1]; o->v[c] = W[nv * c + 2]; o->w[c] = W[nv * c + 3]; o->T[c] = W[nv * c + 4];
+        if (nv == 6) o->nut[c] = W[nv * c + 5];
     }
+}
+int orc_set_state(orc_ctx* o, const double* W) { load_state(o, W, o->ncell); return 0; }
+int orc_set_full_state(orc_ctx* o, const double* W) { load_state(o, W, o->ntot); return 0; }
+int orc_set_update(orc_ctx* o, const double* dW) {
+    for (int c = 0; c < o->ncell; ++c) for (int i = 0; i < o->nv; ++i) o->upd[NVM * c + i] = dW[o->nv * c + i];
     return 0;
 }
-int orc_set_full_state(orc_ctx* o, const double* W) {
-    for (int c = 0; c < o->ntot; ++c) {
-        o->p[c] = W[5 * c]; o->u[c] = W[5 * c + 1]; o->v[c] = W[5 * c + 2]; o->w[c] = W[5 * c + 3]; o->T[c] = W[5 * c + 4];
-    }
-    return 0;
-}
-int orc_set_update(orc_ctx* o, const double* dW) { memcpy(o->upd, dW, sizeof(double) * 5 * (size_t)o->ncell); return 0; }
 
 static V3 cellV(const orc_ctx* o, int c) { return v3(o->u[c], o->v[c], o->w[c]); }
 
@@ -990,6 +1065,12 @@ static int apply_bcs(orc_ctx* o, cstate* L, cstate* R, const fstate* F) {
         R->T_center = L->T_center;
         R->V_center = vsub(L->V_center, vsc(2. * vdot(L->V_center, F->normal), F->normal));
     }
+    /* SA nu~ (build-only): inlet value; zero on a no-slip wall (the ghost
+     * mirrors to -nu~ so the face average vanishes); copied otherwise */
+    if (bc->type == FCFD_BC_INLET) { R->nut = bc->nut; R->nut_center = bc->nut; }
+    else if (bc->type == FCFD_BC_WALL && bc->kind != FCFD_KIND_SLIP) { R->nut = -1. * L->nut; R->nut_center = -1. * L->nut_center; }
+    else if (bc->type == FCFD_BC_OUTLET) { R->nut = L->nut; }
+    else { R->nut = L->nut; R->nut_center = L->nut_center; }
     return 0;
 }
 
@@ -1015,7 +1096,7 @@ static void finish_state(const orc_ctx* o, cstate* S, const fstate* F) {   /* :2
 /* left_state_update / interior part of right_state_update (:269-294, :305-320) */
 static void recon_state(const orc_ctx* o, cstate* S, const fstate* F, int c) {
     V3 c2f = vsub(vsc(1., o->fc[F->index]), o->cc[c]);    /* order_factor == 1 in the face loop */
-    const double* ph = &o->phi[5 * c];
+    const double* ph = &o->phi[NVM * c];
     S->p_center = o->p[c];
     S->p = S->p_center + ph[0] * vdot(c2f, o->gp[c]);
     V3 dV = v3(ph[1] * vdot(c2f, o->gu[c]), ph[2] * vdot(c2f, o->gv[c]), ph[3] * vdot(c2f, o->gw[c]));
@@ -1024,15 +1105,18 @@ static void recon_state(const orc_ctx* o, cstate* S, const fstate* F, int c) {
     S->T_center = o->T[c];
     S->T = S->T_center + ph[4] * vdot(c2f, o->gT[c]);
     S->rho = m_rho(o, S->p, S->T);
-    for (int i = 0; i < 5; ++i) S->update[i] = (c < o->ncell) ? o->upd[5 * c + i] : 0.;
+    S->nut_center = o->nut[c];
+    S->nut = (o->nv == 6) ? S->nut_center + ph[5] * vdot(c2f, o->gn[c]) : 0.;
+    for (int i = 0; i < NVM; ++i) S->update[i] = (c < o->ncell) ? o->upd[NVM * c + i] : 0.;
     S->volume = o->vol[c];
 }
 
 static int right_state_update(orc_ctx* o, cstate* L, cstate* R, const fstate* F) {   /* :296-330 */
     if (F->bc >= 0) {
-        for (int i = 0; i < 5; ++i) R->update[i] = 0.;
+        for (int i = 0; i < NVM; ++i) R->update[i] = 0.;
         int g = F->neighbor;
         R->p_center = o->p[g]; R->T_center = o->T[g]; R->V_center = cellV(o, g);
+        R->nut_center = o->nut[g];
         int rc = apply_bcs(o, L, R, F);
         if (rc) return rc;
     } else {
@@ -1040,6 +1124,27 @@ static int right_state_update(orc_ctx* o, cstate* L, cstate* R, const fstate* F)
     }
     finish_state(o, R, F);
     return 0;
+}
+
+/* SA face quantities: face gradient of nu~ (face average + the l2r
+ * correction of face_state_update), face nu~ and rho (averages of the two
+ * face states) and mu_t(face).  with_grad = 0 keeps the gradient. */
+static void sa_face_update(const orc_ctx* o, const cstate* L, const cstate* R, fstate* F, int with_grad) {
+    if (with_grad) {
+        if (F->bc >= 0) F->gradn = o->gn[F->parent];
+        else {
+            V3 sn = vzero();
+            for (int k = o->avg_ptr[F->index]; k < o->avg_ptr[F->index + 1]; ++k) sn = vadd(sn, vsc(o->avg_w[k], o->gn[o->avg_idx[k]]));
+            F->gradn = sn;
+        }
+        V3 l2rn = vnorm(F->left2right);
+        double l2rmag = vmag(F->left2right);
+        F->gradn = vsub(F->gradn, vsc(vdot(F->gradn, l2rn), l2rn));
+        F->gradn = vadd(F->gradn, vsc((R->nut_center - L->nut_center) / l2rmag, l2rn));
+    }
+    F->nut = 0.5 * (L->nut + R->nut);
+    F->rho = 0.5 * (L->rho + R->rho);
+    F->mu_t = sa_mut(F->rho, F->nut, F->mu);
 }
 
 /* face_state_update (:353-415) */
@@ -1070,6 +1175,8 @@ static void face_state_update(const orc_ctx* o, const cstate* L, const cstate* R
     F->T = 0.5 * (L->T + R->T);
     F->mu = m_visc(o, F->T);
     F->lambda = m_cond(o, F->T);
+    F->gradn = vzero(); F->nut = 0.; F->rho = 0.; F->mu_t = 0.;
+    if (o->nv == 6) sa_face_update(o, L, R, F, 1);
 }
 
 /* face_state_adjust (:460-500), including the case-1 fall-through */
@@ -1101,6 +1208,15 @@ static void face_state_adjust(const orc_ctx* o, const cstate* L, const cstate* R
         F->lambda = m_cond(o, F->T);
         break;
     }
+    if (o->nv == 6) {   /* SA face quantities follow the perturbed states */
+        if (var == 5) {
+            V3 l2rn2 = vnorm(F->left2right);
+            double l2rmag2 = vmag(F->left2right);
+            F->gradn = vsub(F->gradn, vsc(vdot(F->gradn, l2rn2), l2rn2));
+            F->gradn = vadd(F->gradn, vsc((R->nut_center - L->nut_center) / l2rmag2, l2rn2));
+        }
+        sa_face_update(o, L, R, F, 0);
+    }
 }
 
 /* state_perturb (:417-458) */
@@ -1112,6 +1228,7 @@ static void state_perturb(const orc_ctx* o, cstate* S, const fstate* F, int var,
         S->Vn = v3(vdot(S->V, F->normal), vdot(S->V, F->tangent1), vdot(S->V, F->tangent2));
         break;
     case 4: S->T += eps; S->T_center += eps; S->rho = m_rho(o, S->p, S->T); S->a = m_a(o, S->p, S->T); break;
+    case 5: S->nut += eps; S->nut_center += eps; break;
     }
     S->H = S->a * S->a / (o->opt.gamma - 1.) + 0.5 * vdot(S->V, S->V);
 }
@@ -1316,7 +1433,7 @@ static void sw_flux(const orc_ctx* o, const cstate* L, const cstate* R, double d
 
 static double g_ausm_alpha = 0.;
 /* convective_face_flux (ns_convective_face_flux.cc:32-59) */
-static void convective_face_flux(orc_ctx* o, const cstate* L, const cstate* R, const fstate* F, double flux[5]) {
+static void convective_face_flux(orc_ctx* o, const cstate* L, const cstate* R, const fstate* F, double flux[NVM]) {
     double fn[5];
     switch (o->opt.convective_flux) {
     case FCFD_FLUX_ROE: roe_flux(L, R, fn, o->opt.gamma); break;
@@ -1330,10 +1447,12 @@ static void convective_face_flux(orc_ctx* o, const cstate* L, const cstate* R, c
     flux[2] = (fn[1] * F->normal.c[1] + fn[2] * F->tangent1.c[1] + fn[3] * F->tangent2.c[1]) * F->area;
     flux[3] = (fn[1] * F->normal.c[2] + fn[2] * F->tangent1.c[2] + fn[3] * F->tangent2.c[2]) * F->area;
     flux[4] = fn[4] * F->area;
+    /* SA: nu~ carried by the face mass flux, upwind face state */
+    flux[5] = (o->nv == 6) ? flux[0] * ((flux[0] >= 0.) ? L->nut : R->nut) : 0.;
 }
 
 /* diffusive_face_flux (ns_diffusive_face_flux.cc:28-64), laminar */
-static void diffusive_face_flux(const orc_ctx* o, const fstate* F, double flux[5]) {
+static void diffusive_face_flux(const orc_ctx* o, const fstate* F, double flux[NVM]) {
     V3 aV = vmul(F->normal, F->area);
     V3 tx, ty, tz;
     tx.c[0] = 2. / 3. * (2. * F->gradu.c[0] - F->gradv.c[1] - F->gradw.c[2]);
@@ -1345,14 +1464,16 @@ static void diffusive_face_flux(const orc_ctx* o, const fstate* F, double flux[5
     tz.c[0] = tx.c[2];
     tz.c[1] = ty.c[2];
     tz.c[2] = 2. / 3. * (2. * F->gradw.c[2] - F->gradu.c[0] - F->gradv.c[1]);
-    double mu = F->mu + 0.;
+    double mu = F->mu + F->mu_t;   /* face.mu + turb_visc (:36-39) */
     flux[1] = mu * vdot(tx, aV);
     flux[2] = mu * vdot(ty, aV);
     flux[3] = mu * vdot(tz, aV);
     flux[4] = mu * (vdot(tx, F->V) * aV.c[0] + vdot(ty, F->V) * aV.c[1] + vdot(tz, F->V) * aV.c[2]);
-    double qq = (F->lambda + 0.) * vdot(F->gradT, aV);
+    double turb_cond = (o->nv == 6) ? o->Cp * F->mu_t / o->opt.Pr_t : 0.;   /* Cp(T_f) mu_t / Pr_t (:38) */
+    double qq = (F->lambda + turb_cond) * vdot(F->gradT, aV);
     if (F->bc >= 0 && o->opt.bc[F->bc].thermal == FCFD_THERMAL_FIXED_Q) qq = o->opt.bc[F->bc].qdot * F->area;
     flux[4] += qq;
+    flux[5] = (o->nv == 6) ? (F->mu + F->rho * F->nut * sa_fn(F->rho, F->nut, F->mu)) * vdot(F->gradn, aV) / sa_sigma : 0.;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1368,15 +1489,17 @@ int orc_update_boundaries(orc_ctx* o) {
         L.p = L.p_center = o->p[c];
         L.V = L.V_center = cellV(o, c);
         L.T = L.T_center = o->T[c];
+        L.nut = L.nut_center = o->nut[c];
         L.rho = m_rho(o, L.p, L.T);
         L.volume = o->vol[c];
-        for (int i = 0; i < 5; ++i) L.update[i] = 0.;
+        for (int i = 0; i < NVM; ++i) L.update[i] = 0.;
         finish_state(o, &L, &F);
         cstate R = L;
         int rc = apply_bcs(o, &L, &R, &F);
         if (rc) return rc;
         int g = F.neighbor;
         o->p[g] = R.p_center; o->u[g] = R.V_center.c[0]; o->v[g] = R.V_center.c[1]; o->w[g] = R.V_center.c[2]; o->T[g] = R.T_center;
+        o->nut[g] = R.nut_center;
     }
     return 0;
 }
@@ -1399,14 +1522,15 @@ static double face_value_scalar(const orc_ctx* o, const double* q, int f, int wh
 }
 
 int orc_calc_cell_grads(orc_ctx* o) {
-    const double* q[5] = {o->p, o->u, o->v, o->w, o->T};
+    const double* q[6] = {o->p, o->u, o->v, o->w, o->T, o->nut};
+    const int nv = o->nv;
     for (int c = 0; c < o->ncell; ++c) {
-        V3 g[5];
-        for (int k = 0; k < 5; ++k) g[k] = vzero();
+        V3 g[6];
+        for (int k = 0; k < 6; ++k) g[k] = vzero();
         if (o->g_ptr[c + 1] > o->g_ptr[c]) {
             for (int e = o->g_ptr[c]; e < o->g_ptr[c + 1]; ++e) {
                 int j = o->g_idx[e];
-                for (int k = 0; k < 5; ++k)
+                for (int k = 0; k < nv; ++k)
                     for (int i = 0; i < 3; ++i) g[k].c[i] += o->g_w[e].c[i] * q[k][j];
             }
         } else {   /* Green-Gauss fallback */
@@ -1414,18 +1538,20 @@ int orc_calc_cell_grads(orc_ctx* o) {
                 int f = o->cfaces[c].a[cf];
                 V3 aV = vdiv(vmul(o->fn[f], o->farea[f]), o->vol[c]);
                 if (o->fparent[f] != c) aV = vmul(aV, -1.);
-                for (int k = 0; k < 5; ++k) {
+                for (int k = 0; k < nv; ++k) {
                     double fv = face_value_scalar(o, q[k], f, k);
                     for (int i = 0; i < 3; ++i) g[k].c[i] += fv * aV.c[i];
                 }
             }
         }
         o->gp[c] = g[0]; o->gu[c] = g[1]; o->gv[c] = g[2]; o->gw[c] = g[3]; o->gT[c] = g[4];
+        o->gn[c] = g[5];
     }
     for (int f = 0; f < o->nface; ++f)
         if (o->fbc[f] >= 0) {
             int par = o->fparent[f], nb = o->fneigh[f];
             o->gp[nb] = o->gp[par]; o->gu[nb] = o->gu[par]; o->gv[nb] = o->gv[par]; o->gw[nb] = o->gw[par]; o->gT[nb] = o->gT[par];
+            o->gn[nb] = o->gn[par];
         }
     return 0;
 }
@@ -1439,9 +1565,10 @@ static double isgn(double x) { return (double)((x > 0) - (x < 0)); }
 int orc_calc_limiter(orc_ctx* o) {
     int lf = o->opt.limiter;
     if (lf == FCFD_LIMITER_NONE || o->opt.order == FCFD_ORDER_FIRST) return 0;
-    const double* q[5] = {o->p, o->u, o->v, o->w, o->T};
-    V3* gr[5] = {o->gp, o->gu, o->gv, o->gw, o->gT};
-    double deltaRef[5] = {0, 0, 0, 0, 0};
+    const double* q[6] = {o->p, o->u, o->v, o->w, o->T, o->nut};
+    V3* gr[6] = {o->gp, o->gu, o->gv, o->gw, o->gT, o->gn};
+    const int nv = o->nv;
+    double deltaRef[6] = {0, 0, 0, 0, 0, 0};
     double Lref = o->glscale, K = o->opt.limiter_threshold;
     if (lf == FCFD_LIMITER_VK) {        /* find_min_max with its index slips */
         double qmax[5], qmin[5];
@@ -1453,16 +1580,21 @@ int orc_calc_limiter(orc_ctx* o) {
             qmin[3] = dmin(qmin[4], o->w[c]); qmin[4] = dmin(qmin[4], o->p[c]);
         }
         for (int i = 0; i < 5; ++i) deltaRef[i] = fabs(qmax[i] - qmin[i]);
+        if (nv == 6) {   /* SA nu~: plain range (build-only variable, no reference slot) */
+            double nmax = o->nut[0], nmin = o->nut[0];
+            for (int c = 0; c < o->ncell; ++c) { nmax = dmax(nmax, o->nut[c]); nmin = dmin(nmin, o->nut[c]); }
+            deltaRef[5] = fabs(nmax - nmin);
+        }
     }
     for (int c = 0; c < o->ncell; ++c) {
-        double phi[5] = {1., 1., 1., 1., 1.};
+        double phi[6] = {1., 1., 1., 1., 1., 1.};
         if (lf == FCFD_LIMITER_MINMOD) {
             const double small = 1.e-12;
             for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
                 int f = o->cfaces[c].a[cf];
                 V3 c2f = vsub(o->fc[f], o->cc[c]);
                 int nb = (c == o->fparent[f]) ? o->fneigh[f] : o->fparent[f];
-                for (int var = 0; var < 5; ++var) {
+                for (int var = 0; var < nv; ++var) {
                     double dU = q[var][nb] - q[var][c];
                     double dUg = vdot(gr[var][c], c2f);
                     dU = lim_minmod(dU, dUg);
@@ -1470,17 +1602,17 @@ int orc_calc_limiter(orc_ctx* o) {
                 }
             }
         } else {
-            double umax[5], umin[5];
-            for (int i = 0; i < 5; ++i) umax[i] = umin[i] = q[i][c];
+            double umax[6], umin[6];
+            for (int i = 0; i < nv; ++i) umax[i] = umin[i] = q[i][c];
             for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
                 int f = o->cfaces[c].a[cf];
                 int nb = (c == o->fparent[f]) ? o->fneigh[f] : o->fparent[f];
-                for (int i = 0; i < 5; ++i) { umax[i] = dmax(umax[i], q[i][nb]); umin[i] = dmin(umin[i], q[i][nb]); }
+                for (int i = 0; i < nv; ++i) { umax[i] = dmax(umax[i], q[i][nb]); umin[i] = dmin(umin[i], q[i][nb]); }
             }
             for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
                 int f = o->cfaces[c].a[cf];
                 V3 c2f = vsub(o->fc[f], o->cc[c]);
-                for (int var = 0; var < 5; ++var) {
+                for (int var = 0; var < nv; ++var) {
                     double dM = vdot(gr[var][c], c2f);
                     double dP = q[var][c];
                     if (lf == FCFD_LIMITER_BJ) {
@@ -1499,7 +1631,7 @@ int orc_calc_limiter(orc_ctx* o) {
                 }
             }
         }
-        for (int i = 0; i < 5; ++i) o->phi[5 * c + i] = phi[i];
+        for (int i = 0; i < nv; ++i) o->phi[NVM * c + i] = phi[i];
     }
     return 0;
 }
@@ -1509,17 +1641,24 @@ int orc_calc_limiter(orc_ctx* o) {
 /* ------------------------------------------------------------------ */
 int orc_assemble(orc_ctx* o, int with_jac) {
     int N = o->ncell;
-    for (int i = 0; i < 5 * N; ++i) { o->Rs[i] = 0.; o->Rabs[i] = 0.; }
+    const int nv = o->nv, B2 = NVM * NVM;
+    for (int i = 0; i < NVM * N; ++i) { o->Rs[i] = 0.; o->Rabs[i] = 0.; }
     for (int i = 0; i < 6 * o->nbc; ++i) o->loads[i] = 0.;
     if (with_jac) {
-        if (!o->JL) { o->JL = (double*)malloc(sizeof(double) * 25 * o->nface); o->JR = (double*)malloc(sizeof(double) * 25 * o->nface); o->Dg = (double*)malloc(sizeof(double) * 25 * (size_t)N); }
-        memset(o->JL, 0, sizeof(double) * 25 * o->nface);
-        memset(o->JR, 0, sizeof(double) * 25 * o->nface);
-        memset(o->Dg, 0, sizeof(double) * 25 * (size_t)N);
+        if (!o->JL) { o->JL = (double*)malloc(sizeof(double) * B2 * o->nface); o->JR = (double*)malloc(sizeof(double) * B2 * o->nface); o->Dg = (double*)malloc(sizeof(double) * B2 * (size_t)N); }
+        memset(o->JL, 0, sizeof(double) * B2 * o->nface);
+        memset(o->JR, 0, sizeof(double) * B2 * o->nface);
+        memset(o->Dg, 0, sizeof(double) * B2 * (size_t)N);
     }
     const double small_number = 10. * sqrt(2.220446049250313e-16);
     const double factor = 0.001;
-    double fc[5], fd[5], fcp[5], fdp[5];
+    /* FD step floor per variable: the reference's absolute 10 sqrt(eps) for
+     * p,u,v,w,T; for SA nu~ (build-only) the same number relative to the
+     * largest inlet nu~, since nu~ itself is O(1e-5) */
+    double small_v[NVM];
+    for (int i = 0; i < 5; ++i) small_v[i] = small_number;
+    small_v[5] = small_number * sa_nut_scale(o);
+    double fc[NVM], fd[NVM], fcp[NVM], fdp[NVM];
     for (int f = 0; f < o->nface; ++f) {
         fstate F;
         cstate L, R;
@@ -1540,51 +1679,75 @@ int orc_assemble(orc_ctx* o, int with_jac) {
             V3 mm = vcross(vsub(o->fc[f], mc), t);
             ld[3] += mm.c[0]; ld[4] += mm.c[1]; ld[5] += mm.c[2];
         }
-        for (int i = 0; i < 5; ++i) {                                  /* :131-140 */
+        for (int i = 0; i < nv; ++i) {                                 /* :131-140 */
             double val = fd[i] - fc[i] + 0.;
-            o->Rs[5 * F.parent + i] += val;
-            o->Rabs[5 * F.parent + i] += fabs(val);
+            o->Rs[NVM * F.parent + i] += val;
+            o->Rabs[NVM * F.parent + i] += fabs(val);
             if (F.bc == -1) {
                 double vn = -1. * (fd[i] - fc[i]) + 0.;
-                o->Rs[5 * F.neighbor + i] += vn;
-                o->Rabs[5 * F.neighbor + i] += fabs(vn);
+                o->Rs[NVM * F.neighbor + i] += vn;
+                o->Rabs[NVM * F.neighbor + i] += fabs(vn);
             }
         }
         if (!with_jac) continue;
-        for (int var = 0; var < 5; ++var) {                            /* get_jacobians (:197-267) */
-            double jl[5], jr[5] = {0, 0, 0, 0, 0};
+        for (int var = 0; var < nv; ++var) {                           /* get_jacobians (:197-267) */
+            double jl[NVM], jr[NVM] = {0, 0, 0, 0, 0, 0};
             cstate Lp = L, Rp = R;
             double eps;
-            if (L.update[var] > small_number) eps = dmax(small_number, factor * L.update[var]);
-            else if (L.update[var] < -small_number) eps = dmin(-small_number, factor * L.update[var]);
-            else eps = small_number;
+            const double sn = small_v[var];
+            if (L.update[var] > sn) eps = dmax(sn, factor * L.update[var]);
+            else if (L.update[var] < -sn) eps = dmin(-sn, factor * L.update[var]);
+            else eps = sn;
             state_perturb(o, &Lp, &F, var, eps);
             if (F.bc >= 0) { rc = right_state_update(o, &Lp, &Rp, &F); if (rc) return rc; }
             convective_face_flux(o, &Lp, &Rp, &F, fcp);
             face_state_adjust(o, &Lp, &Rp, &F, var);
             fdp[0] = 0.;
             diffusive_face_flux(o, &F, fdp);
-            for (int j = 0; j < 5; ++j) jl[j] = (fdp[j] - fd[j] - fcp[j] + fc[j]) / eps;
+            for (int j = 0; j < nv; ++j) jl[j] = (fdp[j] - fd[j] - fcp[j] + fc[j]) / eps;
             if (F.bc == -1) {
-                if (R.update[var] > small_number) eps = dmax(small_number, factor * R.update[var]);
-                else if (R.update[var] < -small_number) eps = dmin(-small_number, factor * R.update[var]);
-                else eps = small_number;
+                if (R.update[var] > sn) eps = dmax(sn, factor * R.update[var]);
+                else if (R.update[var] < -sn) eps = dmin(-sn, factor * R.update[var]);
+                else eps = sn;
                 state_perturb(o, &Rp, &F, var, eps);
                 convective_face_flux(o, &L, &Rp, &F, fcp);
                 face_state_adjust(o, &L, &Rp, &F, var);
                 fdp[0] = 0.;
                 diffusive_face_flux(o, &F, fdp);
-                for (int j = 0; j < 5; ++j) jr[j] = (fdp[j] - fd[j] - fcp[j] + fc[j]) / eps;
+                for (int j = 0; j < nv; ++j) jr[j] = (fdp[j] - fd[j] - fcp[j] + fc[j]) / eps;
             }
             face_state_adjust(o, &L, &R, &F, var);
-            for (int j = 0; j < 5; ++j) {
-                o->JL[25 * f + 5 * var + j] = jl[j];
-                o->JR[25 * f + 5 * var + j] = jr[j];
+            for (int j = 0; j < nv; ++j) {
+                o->JL[B2 * f + NVM * var + j] = jl[j];
+                o->JR[B2 * f + NVM * var + j] = jr[j];
             }
             /* MatSetValues(ADD) of the diagonal blocks (:154-177) */
-            for (int j = 0; j < 5; ++j) {
-                o->Dg[25 * F.parent + 5 * j + var] += -1. * jl[j];
-                if (F.bc == -1) o->Dg[25 * F.neighbor + 5 * j + var] += jr[j];
+            for (int j = 0; j < nv; ++j) {
+                o->Dg[B2 * F.parent + NVM * j + var] += -1. * jl[j];
+                if (F.bc == -1) o->Dg[B2 * F.neighbor + NVM * j + var] += jr[j];
+            }
+        }
+    }
+    if (nv == 6) {   /* SA cell source, added after the face sums */
+        for (int c = 0; c < N; ++c) {
+            double rho = m_rho(o, o->p[c], o->T[c]);
+            double S = o->vol[c] * sa_source(rho, m_visc(o, o->T[c]), o->nut[c], o->gu[c], o->gv[c], o->gw[c], o->gn[c], o->wdist[c], NULL, NULL);
+            o->Rs[NVM * c + 5] += S;
+            o->Rabs[NVM * c + 5] += fabs(S);
+            if (!with_jac) continue;
+            /* its diagonal block: one-sided differences in the cell state
+             * (gradients frozen) with the face perturbation policy */
+            for (int var = 0; var < nv; ++var) {
+                double u = o->upd[NVM * c + var], eps;
+                const double sn = small_v[var];
+                if (u > sn) eps = dmax(sn, factor * u);
+                else if (u < -sn) eps = dmin(-sn, factor * u);
+                else eps = sn;
+                double q[6] = {o->p[c], o->u[c], o->v[c], o->w[c], o->T[c], o->nut[c]};
+                q[var] += eps;
+                double rp = m_rho(o, q[0], q[4]);
+                double Sp = o->vol[c] * sa_source(rp, m_visc(o, q[4]), q[5], o->gu[c], o->gv[c], o->gw[c], o->gn[c], o->wdist[c], NULL, NULL);
+                o->Dg[B2 * c + NVM * 5 + var] += -1. * ((Sp - S) / eps);
             }
         }
     }
@@ -1592,28 +1755,44 @@ int orc_assemble(orc_ctx* o, int with_jac) {
 }
 
 int orc_get_residual(const orc_ctx* o, double* R, double* Rabs) {
-    if (R) memcpy(R, o->Rs, sizeof(double) * 5 * (size_t)o->ncell);
-    if (Rabs) memcpy(Rabs, o->Rabs, sizeof(double) * 5 * (size_t)o->ncell);
+    const int nv = o->nv;
+    for (int c = 0; c < o->ncell; ++c)
+        for (int i = 0; i < nv; ++i) {
+            if (R) R[nv * c + i] = o->Rs[NVM * c + i];
+            if (Rabs) Rabs[nv * c + i] = o->Rabs[NVM * c + i];
+        }
     return 0;
 }
 
 int orc_get_fields(const orc_ctx* o, double* Wf, double* G, double* phi) {
+    const int nv = o->nv;
     for (int c = 0; c < o->ntot; ++c) {
-        if (Wf) { Wf[5 * c] = o->p[c]; Wf[5 * c + 1] = o->u[c]; Wf[5 * c + 2] = o->v[c]; Wf[5 * c + 3] = o->w[c]; Wf[5 * c + 4] = o->T[c]; }
+        if (Wf) {
+            Wf[nv * c] = o->p[c]; Wf[nv * c + 1] = o->u[c]; Wf[nv * c + 2] = o->v[c]; Wf[nv * c + 3] = o->w[c]; Wf[nv * c + 4] = o->T[c];
+            if (nv == 6) Wf[nv * c + 5] = o->nut[c];
+        }
         if (G) {
-            const V3* g[5] = {&o->gp[c], &o->gu[c], &o->gv[c], &o->gw[c], &o->gT[c]};
-            for (int k = 0; k < 5; ++k) for (int i = 0; i < 3; ++i) G[15 * c + 3 * k + i] = g[k]->c[i];
+            const V3* g[6] = {&o->gp[c], &o->gu[c], &o->gv[c], &o->gw[c], &o->gT[c], &o->gn[c]};
+            for (int k = 0; k < nv; ++k) for (int i = 0; i < 3; ++i) G[3 * nv * c + 3 * k + i] = g[k]->c[i];
         }
     }
-    if (phi) for (int c = 0; c < o->ncell; ++c) for (int i = 0; i < 5; ++i) phi[5 * c + i] = o->phi[5 * c + i];
+    if (phi) for (int c = 0; c < o->ncell; ++c) for (int i = 0; i < nv; ++i) phi[nv * c + i] = o->phi[NVM * c + i];
     return 0;
 }
 
 int orc_get_jacobian(const orc_ctx* o, double* JL, double* JR, double* Dg) {
     if (!o->JL) return FCFD_EINVAL;
-    if (JL) memcpy(JL, o->JL, sizeof(double) * 25 * o->nface);
-    if (JR) memcpy(JR, o->JR, sizeof(double) * 25 * o->nface);
-    if (Dg) memcpy(Dg, o->Dg, sizeof(double) * 25 * (size_t)o->ncell);
+    const int nv = o->nv, B2 = NVM * NVM;
+    for (int f = 0; f < o->nface; ++f)
+        for (int a = 0; a < nv; ++a)
+            for (int b = 0; b < nv; ++b) {
+                if (JL) JL[nv * nv * f + nv * a + b] = o->JL[B2 * f + NVM * a + b];
+                if (JR) JR[nv * nv * f + nv * a + b] = o->JR[B2 * f + NVM * a + b];
+            }
+    if (Dg)
+        for (int c = 0; c < o->ncell; ++c)
+            for (int a = 0; a < nv; ++a)
+                for (int b = 0; b < nv; ++b) Dg[nv * nv * c + nv * a + b] = o->Dg[B2 * c + NVM * a + b];
     return 0;
 }
 
@@ -1647,8 +1826,9 @@ int orc_time_step(orc_ctx* o, int dt_type, double cfl, double* dt, double* max_c
 
 /* time_terms: cons2prim(c)*V/dt (ns_time_terms.cc:26-56,147-163) */
 int orc_time_terms(orc_ctx* o, const double* dt, double* blocks) {
+    const int nv = o->nv;
     for (int c = 0; c < o->ncell; ++c) {
-        double P[5][5] = {{0}};
+        double P[6][6] = {{0}};
         double rho = m_rho(o, o->p[c], o->T[c]);
         double pressure = o->p[c] + o->opt.Pref, temperature = o->T[c] + o->opt.Tref;
         double drho_dT = -1. * rho / temperature, drho_dp = rho / pressure;
@@ -1661,7 +1841,10 @@ int orc_time_terms(orc_ctx* o, const double* dt, double* blocks) {
         P[3][0] = drho_dp * V.c[2]; P[3][3] = rho; P[3][4] = drho_dT * V.c[2];
         P[4][0] = drho_dp * H - 1.; P[4][1] = rho * V.c[0]; P[4][2] = rho * V.c[1]; P[4][3] = rho * V.c[2];
         P[4][4] = drho_dT * H + rho * c_p;
-        for (int i = 0; i < 5; ++i) for (int j = 0; j < 5; ++j) blocks[25 * c + 5 * i + j] = P[i][j] * o->vol[c] / dt[c];
+        if (nv == 6) {   /* Q6 = rho nu~ */
+            P[5][0] = drho_dp * o->nut[c]; P[5][4] = drho_dT * o->nut[c]; P[5][5] = rho;
+        }
+        for (int i = 0; i < nv; ++i) for (int j = 0; j < nv; ++j) blocks[nv * nv * c + nv * i + j] = P[i][j] * o->vol[c] / dt[c];
     }
     return 0;
 }
@@ -1686,6 +1869,7 @@ int orc_flux_probe(orc_ctx* o, int flux_type, const double* WL, const double* WR
         S->p = S->p_center = src[k][0];
         S->V = S->V_center = v3(src[k][1], src[k][2], src[k][3]);
         S->T = S->T_center = src[k][4];
+        S->nut = S->nut_center = 0.;
         S->rho = m_rho(o, S->p, S->T);
         finish_state(o, S, &F);
     }
@@ -1694,8 +1878,22 @@ int orc_flux_probe(orc_ctx* o, int flux_type, const double* WL, const double* WR
     double diss0 = o->fdiss ? o->fdiss[0] : 0.;
     double wd0 = o->fwd ? o->fwd[0] : 0.;
     if (o->fdiss) { o->fdiss[0] = 1.; o->fwd[0] = 1.e20; }
-    convective_face_flux(o, &L, &R, &F, out);
+    double fl[NVM];
+    convective_face_flux(o, &L, &R, &F, fl);
+    for (int i = 0; i < 5; ++i) out[i] = fl[i];
     if (o->fdiss) { o->fdiss[0] = diss0; o->fwd[0] = wd0; }
     o->opt.convective_flux = saved;
+    return 0;
+}
+
+/* SA model pieces at one point (probe for the pin tests): out = source per
+ * unit volume, P, D, mu_t, fn; gvel = du, dv, dw gradients (9) */
+int orc_sa_probe(double rho, double mu, double nut, const double* gvel, const double* gn, double d, double* out) {
+    V3 gu = v3(gvel[0], gvel[1], gvel[2]), gv = v3(gvel[3], gvel[4], gvel[5]), gw = v3(gvel[6], gvel[7], gvel[8]);
+    double P, D;
+    out[0] = sa_source(rho, mu, nut, gu, gv, gw, v3(gn[0], gn[1], gn[2]), d, &P, &D);
+    out[1] = P; out[2] = D;
+    out[3] = sa_mut(rho, nut, mu);
+    out[4] = sa_fn(rho, nut, mu);
     return 0;
 }

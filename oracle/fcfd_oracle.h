@@ -26,7 +26,8 @@ const char* orc_last_error(const orc_ctx* o);
 int orc_info(const orc_ctx* o, fcfd_info* info);
 int orc_metrics(const orc_ctx* o, fcfd_metrics_out* m);
 
-/* state W: N x 5 (p,u,v,w,T), reference numbering */
+/* state W: N x nv (p,u,v,w,T[,nu~]; nv = 6 with the SA model), reference
+ * numbering; every per-cell array below uses the same nv */
 int orc_set_state(orc_ctx* o, const double* W);
 int orc_set_update(orc_ctx* o, const double* dW);   /* previous Newton update (update[i].cell) */
 int orc_set_full_state(orc_ctx* o, const double* W);  /* (N+B) x 5 incl. ghosts */
@@ -50,6 +51,8 @@ int orc_get_loads(const orc_ctx* o, double* loads);
 int orc_time_step(orc_ctx* o, int dt_type, double cfl, double* dt, double* max_cfl);
 /* time_terms P*V/dt blocks [N][5][5] (ns_time_terms.cc:26-56,147-163) */
 int orc_time_terms(orc_ctx* o, const double* dt, double* blocks);
+/* SA model pieces at one point: out = source/volume, P, D, mu_t, fn */
+int orc_sa_probe(double rho, double mu, double nut, const double* gvel, const double* gn, double d, double* out);
 
 #ifdef __cplusplus
 }

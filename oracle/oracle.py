@@ -46,6 +46,7 @@ def lib():
         L.orc_time_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_flux_probe.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_double, C.c_void_p]
+        L.orc_sa_probe.argtypes = [C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -127,25 +128,26 @@ class Oracle:
         self.set_state(W, dW)
         self.prepare()
         self._check(lib().orc_assemble(self.h, int(with_jacobian)))
-        N = self.info.n_cells
-        R = np.zeros((N, 5))
-        Rabs = np.zeros((N, 5))
+        N, nv = self.info.n_cells, self.info.nv
+        R = np.zeros((N, nv))
+        Rabs = np.zeros((N, nv))
         self._check(lib().orc_get_residual(self.h, _p(R), _p(Rabs)))
         return R, Rabs
 
     def fields(self):
         T = self.info.n_cells + self.info.n_ghosts
-        W = np.zeros((T, 5))
-        G = np.zeros((T, 15))
-        phi = np.zeros((self.info.n_cells, 5))
+        nv = self.info.nv
+        W = np.zeros((T, nv))
+        G = np.zeros((T, 3 * nv))
+        phi = np.zeros((self.info.n_cells, nv))
         self._check(lib().orc_get_fields(self.h, _p(W), _p(G), _p(phi)))
         return W, G, phi
 
     def jacobian(self):
-        F, N = self.info.n_faces, self.info.n_cells
-        JL = np.zeros((F, 5, 5))
-        JR = np.zeros((F, 5, 5))
-        Dg = np.zeros((N, 5, 5))
+        F, N, nv = self.info.n_faces, self.info.n_cells, self.info.nv
+        JL = np.zeros((F, nv, nv))
+        JR = np.zeros((F, nv, nv))
+        Dg = np.zeros((N, nv, nv))
         self._check(lib().orc_get_jacobian(self.h, _p(JL), _p(JR), _p(Dg)))
         return JL, JR, Dg
 
@@ -162,6 +164,18 @@ class Oracle:
 
     def time_terms(self, dt):
         dt = np.ascontiguousarray(dt, dtype=np.float64)
-        out = np.zeros((self.info.n_cells, 5, 5))
+        nv = self.info.nv
+        out = np.zeros((self.info.n_cells, nv, nv))
         self._check(lib().orc_time_terms(self.h, _p(dt), _p(out)))
         return out
+
+
+def sa_probe(rho, mu, nut, grad_vel, grad_nut, d):
+    """SA model pieces at one point: grad_vel is the 3x3 velocity gradient
+    (rows du, dv, dw), grad_nut the nu~ gradient.  Returns a dict with the
+    source per unit volume, P, D (per unit mass), mu_t and fn."""
+    gv = np.ascontiguousarray(grad_vel, dtype=np.float64).reshape(9)
+    gn = np.ascontiguousarray(grad_nut, dtype=np.float64).reshape(3)
+    out = np.zeros(5)
+    lib().orc_sa_probe(float(rho), float(mu), float(nut), _p(gv), _p(gn), float(d), _p(out))
+    return dict(source=out[0], P=out[1], D=out[2], mu_t=out[3], fn=out[4])
