@@ -36,70 +36,86 @@ namespace {
 
 __device__ __forceinline__ Dual seed(double v, int k, int idx) { return Dual(v, k == idx ? 1.0 : 0.0); }
 
-// Z rows: 0-4 left face state (p,u,v,w,T), 5-9 left centre, 10-14 right face
-// state (interior faces), 15-19 right centre (ghost centre on a boundary
-// face), 20-31 face gradients of u,v,w,T (viscous)
-constexpr int kZL = 0, kZLc = 5, kZR = 10, kZRc = 15, kZG = 20;
+// Z rows, NV variables per state: [0, NV) left face state, [NV, 2NV) left
+// centre, [2NV, 3NV) right face state (interior faces), [3NV, 4NV) right
+// centre (ghost centre on a boundary face), then the face gradients of
+// u, v, w, T (and nu~) when viscous
+template <int NV> struct ZRows {
+    static constexpr int L = 0, Lc = NV, R = 2 * NV, Rc = 3 * NV, G = 4 * NV;
+    static constexpr int ngrad = NV == 6 ? 15 : 12;
+};
+template <int VISC, int NV> constexpr int zcount() { return 4 * NV + (VISC ? ZRows<NV>::ngrad : 0); }
 
-template <int FLUX, int VISC>
-__global__ void __launch_bounds__(256) face_adj_kernel(DevGrid g, Params P, CellLoader cl, const double* __restrict__ Rbar,
+template <int FLUX, int VISC, int NV>
+__global__ void __launch_bounds__(256) face_adj_kernel(DevGrid g, Params P, CellLoader<NV> cl, const double* __restrict__ Rbar,
                                                        const double* __restrict__ fxw, double* __restrict__ Z,
                                                        int* __restrict__ status) {
+    using Zr = ZRows<NV>;
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
     const size_t Fs = g.F, ld = g.ld;
     const int p = g.fpar[f], n = g.fnei[f], bc = g.fbc[f];
     const bool bnd = bc >= 0;
-    constexpr int nz = VISC ? 32 : 20;
-    double fb[5];
+    constexpr int nz = zcount<VISC, NV>();
+    double fb[NV];
     bool any = false;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
+    for (int v = 0; v < NV; ++v) {
         fb[v] = Rbar[v * ld + p] - (bnd ? 0.0 : Rbar[v * ld + n]);
-        if (bnd && fxw) fb[v] += fxw[5 * bc + v];
+        if (bnd && fxw && v < 5) fb[v] += fxw[5 * bc + v];
         any |= fb[v] != 0.0;
     }
     if (!any) {
         for (int k = 0; k < nz; ++k) Z[k * Fs + f] = 0.0;
         return;
     }
-    FaceCtx<double> X0;
-    face_inputs<VISC>(g, cl, P, f, X0);
+    FaceCtx<double, NV> X0;
+    face_inputs<VISC, NV>(g, cl, P, f, X0);
 #pragma unroll 1
     for (int k = 0; k < nz; ++k) {
-        if (bnd && k >= kZR && k < kZRc) { Z[k * Fs + f] = 0.0; continue; }
-        FaceCtx<Dual> X;
+        if (bnd && k >= Zr::R && k < Zr::Rc) { Z[k * Fs + f] = 0.0; continue; }
+        FaceCtx<Dual, NV> X;
         X.F = X0.F; X.p = p; X.n = n;
-        X.L.p = seed(X0.L.p, k, kZL);
-        X.L.V = mk<Dual>(seed(X0.L.V.x, k, kZL + 1), seed(X0.L.V.y, k, kZL + 2), seed(X0.L.V.z, k, kZL + 3));
-        X.L.T = seed(X0.L.T, k, kZL + 4);
-        X.L.p_center = seed(X0.L.p_center, k, kZLc);
-        X.L.V_center = mk<Dual>(seed(X0.L.V_center.x, k, kZLc + 1), seed(X0.L.V_center.y, k, kZLc + 2), seed(X0.L.V_center.z, k, kZLc + 3));
-        X.L.T_center = seed(X0.L.T_center, k, kZLc + 4);
+        X.L.p = seed(X0.L.p, k, Zr::L);
+        X.L.V = mk<Dual>(seed(X0.L.V.x, k, Zr::L + 1), seed(X0.L.V.y, k, Zr::L + 2), seed(X0.L.V.z, k, Zr::L + 3));
+        X.L.T = seed(X0.L.T, k, Zr::L + 4);
+        X.L.p_center = seed(X0.L.p_center, k, Zr::Lc);
+        X.L.V_center = mk<Dual>(seed(X0.L.V_center.x, k, Zr::Lc + 1), seed(X0.L.V_center.y, k, Zr::Lc + 2),
+                                seed(X0.L.V_center.z, k, Zr::Lc + 3));
+        X.L.T_center = seed(X0.L.T_center, k, Zr::Lc + 4);
+        if constexpr (NV == 6) {
+            X.L.nut = seed(X0.L.nut, k, Zr::L + 5);
+            X.L.nut_center = seed(X0.L.nut_center, k, Zr::Lc + 5);
+        }
         X.L.rho = m_rho(P, X.L.p, X.L.T);
         {
             CState<Dual> Rs;
             if (!bnd) {
-                Rs.p = seed(X0.R.p, k, kZR);
-                Rs.V = mk<Dual>(seed(X0.R.V.x, k, kZR + 1), seed(X0.R.V.y, k, kZR + 2), seed(X0.R.V.z, k, kZR + 3));
-                Rs.T = seed(X0.R.T, k, kZR + 4);
+                Rs.p = seed(X0.R.p, k, Zr::R);
+                Rs.V = mk<Dual>(seed(X0.R.V.x, k, Zr::R + 1), seed(X0.R.V.y, k, Zr::R + 2), seed(X0.R.V.z, k, Zr::R + 3));
+                Rs.T = seed(X0.R.T, k, Zr::R + 4);
+                if constexpr (NV == 6) Rs.nut = seed(X0.R.nut, k, Zr::R + 5);
                 Rs.rho = m_rho(P, Rs.p, Rs.T);
             }
-            Rs.p_center = seed(X0.R.p_center, k, kZRc);
-            Rs.V_center = mk<Dual>(seed(X0.R.V_center.x, k, kZRc + 1), seed(X0.R.V_center.y, k, kZRc + 2), seed(X0.R.V_center.z, k, kZRc + 3));
-            Rs.T_center = seed(X0.R.T_center, k, kZRc + 4);
+            Rs.p_center = seed(X0.R.p_center, k, Zr::Rc);
+            Rs.V_center = mk<Dual>(seed(X0.R.V_center.x, k, Zr::Rc + 1), seed(X0.R.V_center.y, k, Zr::Rc + 2),
+                                   seed(X0.R.V_center.z, k, Zr::Rc + 3));
+            Rs.T_center = seed(X0.R.T_center, k, Zr::Rc + 4);
+            if constexpr (NV == 6) Rs.nut_center = seed(X0.R.nut_center, k, Zr::Rc + 5);
             X.R = Rs;
         }
         if (VISC) {
-            X.fs.gu = mk<Dual>(seed(X0.fs.gu.x, k, kZG + 0), seed(X0.fs.gu.y, k, kZG + 1), seed(X0.fs.gu.z, k, kZG + 2));
-            X.fs.gv = mk<Dual>(seed(X0.fs.gv.x, k, kZG + 3), seed(X0.fs.gv.y, k, kZG + 4), seed(X0.fs.gv.z, k, kZG + 5));
-            X.fs.gw = mk<Dual>(seed(X0.fs.gw.x, k, kZG + 6), seed(X0.fs.gw.y, k, kZG + 7), seed(X0.fs.gw.z, k, kZG + 8));
-            X.fs.gT = mk<Dual>(seed(X0.fs.gT.x, k, kZG + 9), seed(X0.fs.gT.y, k, kZG + 10), seed(X0.fs.gT.z, k, kZG + 11));
+            X.fs.gu = mk<Dual>(seed(X0.fs.gu.x, k, Zr::G + 0), seed(X0.fs.gu.y, k, Zr::G + 1), seed(X0.fs.gu.z, k, Zr::G + 2));
+            X.fs.gv = mk<Dual>(seed(X0.fs.gv.x, k, Zr::G + 3), seed(X0.fs.gv.y, k, Zr::G + 4), seed(X0.fs.gv.z, k, Zr::G + 5));
+            X.fs.gw = mk<Dual>(seed(X0.fs.gw.x, k, Zr::G + 6), seed(X0.fs.gw.y, k, Zr::G + 7), seed(X0.fs.gw.z, k, Zr::G + 8));
+            X.fs.gT = mk<Dual>(seed(X0.fs.gT.x, k, Zr::G + 9), seed(X0.fs.gT.y, k, Zr::G + 10), seed(X0.fs.gT.z, k, Zr::G + 11));
+            if constexpr (NV == 6)
+                X.fs.gn = mk<Dual>(seed(X0.fs.gn.x, k, Zr::G + 12), seed(X0.fs.gn.y, k, Zr::G + 13), seed(X0.fs.gn.z, k, Zr::G + 14));
         }
-        face_tail<FLUX, VISC>(g, P, f, X, status);
+        face_tail<FLUX, VISC, NV>(g, P, f, X, status);
         double zb = 0.0;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) zb += fb[v] * (X.D[v].d - X.C[v].d);
+        for (int v = 0; v < NV; ++v) zb += fb[v] * (X.D[v].d - X.C[v].d);
         Z[k * Fs + f] = zb;
     }
 }
@@ -112,37 +128,38 @@ struct AdjNb {
     __device__ __forceinline__ Dual nb(int h, int j, int v) const { return Dual(W[v * ld + j], h == dir ? 1.0 : 0.0); }
 };
 
-template <int LIM, int VISC>
+template <int LIM, int VISC, int NV>
 __global__ void __launch_bounds__(256) cell_adj_kernel(DevGrid g, Params P, const double* __restrict__ W,
                                                        const double* __restrict__ G, const double* __restrict__ PHI,
                                                        double phi_const, const double* __restrict__ dref,
-                                                       const double* __restrict__ Z, double* __restrict__ Wb1,
-                                                       double* __restrict__ Gb, double* __restrict__ LH) {
+                                                       const double* __restrict__ Rbar, const double* __restrict__ Z,
+                                                       double* __restrict__ Wb1, double* __restrict__ Gb, double* __restrict__ LH) {
+    using Zr = ZRows<NV>;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N + g.B) return;
     const size_t Fs = g.F, ld = g.ld, nhf = g.nhf;
     if (c >= g.N) {   // a ghost's centre state enters its face only
         const int f = g.gh_face[c - g.N];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) Wb1[v * ld + c] = Z[(kZRc + v) * Fs + f];
+        for (int v = 0; v < NV; ++v) Wb1[v * ld + c] = Z[(Zr::Rc + v) * Fs + f];
         return;
     }
     constexpr bool has_phi = LIM != FCFD_LIMITER_NONE;
-    double qb[5], gb[15], pb[5], gc[15], ph[5];
+    double qb[NV], gb[3 * NV], pb[NV], gc[3 * NV], ph[NV];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) { gb[k] = 0.0; gc[k] = G[k * ld + c]; }
+    for (int k = 0; k < 3 * NV; ++k) { gb[k] = 0.0; gc[k] = G[k * ld + c]; }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) { qb[v] = 0.0; pb[v] = 0.0; ph[v] = has_phi ? PHI[v * ld + c] : phi_const; }
+    for (int v = 0; v < NV; ++v) { qb[v] = 0.0; pb[v] = 0.0; ph[v] = has_phi ? PHI[v * ld + c] : phi_const; }
     const Vec<double> xc = centroid(g, c);
     const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
     for (int h = h0; h < h1; ++h) {
         const int fs = g.hf_face[h];
         const bool left = fs >= 0;
         const int f = left ? fs : ~fs;
-        const int of = left ? kZL : kZR, oc = left ? kZLc : kZRc;
+        const int of = left ? Zr::L : Zr::R, oc = left ? Zr::Lc : Zr::Rc;
         const Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
+        for (int v = 0; v < NV; ++v) {
             // q_f = q_c + phi (grad q . c2f)  (recon) and the centre value itself
             const double zf = Z[(of + v) * Fs + f], zc = Z[(oc + v) * Fs + f];
             qb[v] += zf;
@@ -156,64 +173,97 @@ __global__ void __launch_bounds__(256) cell_adj_kernel(DevGrid g, Params P, cons
         if (VISC) {   // face gradient = w_P grad_P + w_N grad_N (interior), grad_P (boundary)
             const double w = g.fbc[f] >= 0 ? 1.0 : (left ? g.fwp[f] : g.fwn[f]);
 #pragma unroll
-            for (int m = 0; m < 12; ++m) gb[3 + m] += w * Z[(kZG + m) * Fs + f];
+            for (int m = 0; m < 12; ++m) gb[3 + m] += w * Z[(Zr::G + m) * Fs + f];
+            if constexpr (NV == 6) {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) gb[15 + m] += w * Z[(Zr::G + 12 + m) * Fs + f];
+            }
+        }
+    }
+    if constexpr (NV == 6) {   // SA source V S(p, T, nu~, grad u, grad v, grad w, grad nu~)
+        const double rb = Rbar[5 * ld + c] * g.vol[c];
+        if (rb != 0.0) {
+            const double d = g.wd[c];
+            // directions 0..2: p, T, nu~; 3..14: gradient components of u, v, w, nu~
+            // (unrolled: the accumulation targets are then compile-time registers)
+#pragma unroll
+            for (int k = 0; k < 15; ++k) {
+                Dual p(W[c], k == 0 ? 1.0 : 0.0), T(W[4 * ld + c], k == 1 ? 1.0 : 0.0), nut(W[5 * ld + c], k == 2 ? 1.0 : 0.0);
+                auto gvec = [&](int v, int base) {
+                    Vec<Dual> r;
+                    r.x = Dual(gc[3 * v], k == base ? 1.0 : 0.0);
+                    r.y = Dual(gc[3 * v + 1], k == base + 1 ? 1.0 : 0.0);
+                    r.z = Dual(gc[3 * v + 2], k == base + 2 ? 1.0 : 0.0);
+                    return r;
+                };
+                Dual S = sa_source<Dual>(m_rho(P, p, T), m_visc(P, T), nut, gvec(1, 3), gvec(2, 6), gvec(3, 9), gvec(5, 12), d);
+                const double a = rb * S.d;
+                if (k == 0) qb[0] += a;
+                else if (k == 1) qb[4] += a;
+                else if (k == 2) qb[5] += a;
+                else if (k < 12) gb[3 + (k - 3)] += a;
+                else gb[15 + (k - 12)] += a;
+            }
         }
     }
     if (has_phi) {
-        const bool any = pb[0] != 0.0 || pb[1] != 0.0 || pb[2] != 0.0 || pb[3] != 0.0 || pb[4] != 0.0;
+        bool any = false;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) any |= pb[v] != 0.0;
         if (!any) {
             for (int h = h0; h < h1; ++h)
 #pragma unroll
-                for (int v = 0; v < 5; ++v) LH[v * nhf + h] = 0.0;
+                for (int v = 0; v < NV; ++v) LH[v * nhf + h] = 0.0;
         } else {
             // directions: -1 own state, -2..-4 own gradient x/y/z, h >= h0 the
             // neighbour across half-face h
 #pragma unroll 1
             for (int dir = -4; dir < h1; dir = (dir == -1) ? h0 : dir + 1) {
-                Dual q0[5], gr[15], phi[5];
+                Dual q0[NV], gr[3 * NV], phi[NV];
 #pragma unroll
-                for (int v = 0; v < 5; ++v) {
+                for (int v = 0; v < NV; ++v) {
                     q0[v] = Dual(W[v * ld + c], dir == -1 ? 1.0 : 0.0);
                     gr[3 * v] = Dual(gc[3 * v], dir == -2 ? 1.0 : 0.0);
                     gr[3 * v + 1] = Dual(gc[3 * v + 1], dir == -3 ? 1.0 : 0.0);
                     gr[3 * v + 2] = Dual(gc[3 * v + 2], dir == -4 ? 1.0 : 0.0);
                 }
-                limiter_phi<LIM>(g, P, AdjNb{W, ld, dir}, c, q0, gr, dref, phi);
+                limiter_phi<LIM, NV>(g, P, AdjNb{W, ld, dir}, c, q0, gr, dref, phi);
                 if (dir == -1) {
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) qb[v] += pb[v] * phi[v].d;
+                    for (int v = 0; v < NV; ++v) qb[v] += pb[v] * phi[v].d;
                 } else if (dir < -1) {
                     const int k = -2 - dir;
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) gb[3 * v + k] += pb[v] * phi[v].d;
+                    for (int v = 0; v < NV; ++v) gb[3 * v + k] += pb[v] * phi[v].d;
                 } else {
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) LH[v * nhf + dir] = pb[v] * phi[v].d;
+                    for (int v = 0; v < NV; ++v) LH[v * nhf + dir] = pb[v] * phi[v].d;
                 }
             }
         }
     }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Wb1[v * ld + c] = qb[v];
+    for (int v = 0; v < NV; ++v) Wb1[v * ld + c] = qb[v];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) Gb[k * ld + c] = gb[k];
+    for (int k = 0; k < 3 * NV; ++k) Gb[k * ld + c] = gb[k];
 }
 
 // transposed gradient map + the neighbours' limiter contributions
+template <int NV>
 __global__ void __launch_bounds__(256) gradt_kernel(DevGrid g, AdjGrid a, int has_lim, const double* __restrict__ Wb1,
                                                     const double* __restrict__ Gb, const double* __restrict__ LH,
                                                     double* __restrict__ Wb2) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= g.N + g.B) return;
     const size_t ld = g.ld, nhf = g.nhf;
-    double acc[5];
+    double acc[NV];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) acc[v] = Wb1[v * ld + j];
+    for (int v = 0; v < NV; ++v) acc[v] = Wb1[v * ld + j];
     for (int e = a.tg_ptr[j]; e < a.tg_ptr[j + 1]; ++e) {
         const int c = a.tg_cell[e];
         const double wx = a.tgx[e], wy = a.tgy[e], wz = a.tgz[e];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) acc[v] += wx * Gb[(3 * v) * ld + c] + wy * Gb[(3 * v + 1) * ld + c] + wz * Gb[(3 * v + 2) * ld + c];
+        for (int v = 0; v < NV; ++v) acc[v] += wx * Gb[(3 * v) * ld + c] + wy * Gb[(3 * v + 1) * ld + c] + wz * Gb[(3 * v + 2) * ld + c];
     }
     if (has_lim) {
         if (j < g.N) {
@@ -221,22 +271,23 @@ __global__ void __launch_bounds__(256) gradt_kernel(DevGrid g, AdjGrid a, int ha
                 const int t = a.hf_twin[h];
                 if (t < 0) continue;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) acc[v] += LH[v * nhf + t];
+                for (int v = 0; v < NV; ++v) acc[v] += LH[v * nhf + t];
             }
         } else {
             const int t = a.gh_hf[j - g.N];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) acc[v] += LH[v * nhf + t];
+            for (int v = 0; v < NV; ++v) acc[v] += LH[v * nhf + t];
         }
     }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Wb2[v * ld + j] = acc[v];
+    for (int v = 0; v < NV; ++v) Wb2[v * ld + j] = acc[v];
 }
 
 // ghost state of boundary face f from its parent c (update_boundaries as in
 // ghost_kernel), with a unit tangent on parent variable m
+template <int NV>
 __device__ __forceinline__ void ghost_dual(const DevGrid& g, const Params& P, int f, int c, const double* __restrict__ W, int m,
-                                           Dual out[5], int* status) {
+                                           Dual out[NV], int* status) {
     FaceGeo F;
     load_geo(g, f, F);
     const size_t ld = g.ld;
@@ -245,6 +296,7 @@ __device__ __forceinline__ void ghost_dual(const DevGrid& g, const Params& P, in
     L.V = L.V_center = mk<Dual>(Dual(W[ld + c], m == 1 ? 1.0 : 0.0), Dual(W[2 * ld + c], m == 2 ? 1.0 : 0.0),
                                 Dual(W[3 * ld + c], m == 3 ? 1.0 : 0.0));
     L.T = L.T_center = Dual(W[4 * ld + c], m == 4 ? 1.0 : 0.0);
+    if constexpr (NV == 6) L.nut = L.nut_center = Dual(W[5 * ld + c], m == 5 ? 1.0 : 0.0);
     L.rho = m_rho(P, L.p, L.T);
     finish_state(P, L, F);
     CState<Dual> R = L;
@@ -254,49 +306,51 @@ __device__ __forceinline__ void ghost_dual(const DevGrid& g, const Params& P, in
     out[2] = R.V_center.y;
     out[3] = R.V_center.z;
     out[4] = R.T_center;
+    if constexpr (NV == 6) out[5] = R.nut_center;
 }
 
+template <int NV>
 __global__ void __launch_bounds__(256) ghost_adj_kernel(DevGrid g, Params P, const double* __restrict__ W,
                                                         const double* __restrict__ Wb2, double* __restrict__ Wbar,
                                                         int* __restrict__ status) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
-    double acc[5];
+    double acc[NV];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) acc[v] = Wb2[v * ld + c];
+    for (int v = 0; v < NV; ++v) acc[v] = Wb2[v * ld + c];
     for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
         const int f = g.hf_face[h];
         if (f < 0 || g.fbc[f] < 0) continue;
         const int gi = g.fnei[f];
-        double wg[5];
+        double wg[NV];
         bool any = false;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) { wg[v] = Wb2[v * ld + gi]; any |= wg[v] != 0.0; }
+        for (int v = 0; v < NV; ++v) { wg[v] = Wb2[v * ld + gi]; any |= wg[v] != 0.0; }
         if (!any) continue;
 #pragma unroll 1
-        for (int m = 0; m < 5; ++m) {
-            Dual o[5];
-            ghost_dual(g, P, f, c, W, m, o, status);
+        for (int m = 0; m < NV; ++m) {
+            Dual o[NV];
+            ghost_dual<NV>(g, P, f, c, W, m, o, status);
             double s = 0.0;
 #pragma unroll
-            for (int v = 0; v < 5; ++v) s += wg[v] * o[v].d;
+            for (int v = 0; v < NV; ++v) s += wg[v] * o[v].d;
             acc[m] += s;
         }
     }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Wbar[v * ld + c] = acc[v];
+    for (int v = 0; v < NV; ++v) Wbar[v * ld + c] = acc[v];
 }
 
 // ---- surface loads -----------------------------------------------------------
-template <int FLUX, int VISC>
-__global__ void __launch_bounds__(256) loads_face_kernel(DevGrid g, AdjGrid a, Params P, CellLoader cl, double mcx, double mcy,
+template <int FLUX, int VISC, int NV>
+__global__ void __launch_bounds__(256) loads_face_kernel(DevGrid g, AdjGrid a, Params P, CellLoader<NV> cl, double mcx, double mcy,
                                                          double mcz, double* __restrict__ tmp, int nlf, int* __restrict__ status) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nlf) return;
     const int f = a.lf_face[i];
-    FaceCtx<double> X;
-    face_base<FLUX, VISC>(g, P, cl, f, X, status);
+    FaceCtx<double, NV> X;
+    face_base<FLUX, VISC, NV>(g, P, cl, f, X, status);
     // force += (C - D)[1..3], moment += (x_f - x_m) x force (:109-119)
     Vec<double> t = mk<double>(X.C[1] - X.D[1], X.C[2] - X.D[2], X.C[3] - X.D[3]);
     Vec<double> r = mk<double>(g.fcx[f] - mcx, g.fcy[f] - mcy, g.fcz[f] - mcz);
@@ -317,74 +371,95 @@ __global__ void __launch_bounds__(64) loads_sum_kernel(AdjGrid a, const double* 
 }
 
 // ---- dispatch -----------------------------------------------------------------
-template <int FLUX, int VISC>
-void launch_face_adj(const DevGrid& g, const Params& P, const CellLoader& cl, const double* Rbar, const double* fxw, double* Z,
+template <int NV>
+struct AdjLaunch {
+    template <int FLUX, int VISC>
+    static void face(const DevGrid& g, const Params& P, const CellLoader<NV>& cl, const double* Rbar, const double* fxw, double* Z,
                      int* status, hipStream_t s) {
-    hipLaunchKernelGGL((face_adj_kernel<FLUX, VISC>), dim3((g.F + 255) / 256), dim3(256), 0, s, g, P, cl, Rbar, fxw, Z, status);
-}
-template <int FLUX>
-void launch_face_adj_v(const DevGrid& g, const Params& P, const CellLoader& cl, const double* Rbar, const double* fxw, double* Z,
+        hipLaunchKernelGGL((face_adj_kernel<FLUX, VISC, NV>), dim3((g.F + 255) / 256), dim3(256), 0, s, g, P, cl, Rbar, fxw, Z, status);
+    }
+    template <int FLUX>
+    static void face_v(const DevGrid& g, const Params& P, const CellLoader<NV>& cl, const double* Rbar, const double* fxw, double* Z,
                        int* status, hipStream_t s) {
-    if (P.viscous) launch_face_adj<FLUX, 1>(g, P, cl, Rbar, fxw, Z, status, s);
-    else launch_face_adj<FLUX, 0>(g, P, cl, Rbar, fxw, Z, status, s);
-}
-template <int LIM>
-void launch_cell_adj(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double phi_const,
-                     const double* dref, const AdjWork& w, hipStream_t s) {
-    dim3 grid((g.N + g.B + 255) / 256), blk(256);
-    if (P.viscous)
-        hipLaunchKernelGGL((cell_adj_kernel<LIM, 1>), grid, blk, 0, s, g, P, W, G, PHI, phi_const, dref, w.Z, w.Wb1, w.Gb, w.LH);
-    else
-        hipLaunchKernelGGL((cell_adj_kernel<LIM, 0>), grid, blk, 0, s, g, P, W, G, PHI, phi_const, dref, w.Z, w.Wb1, w.Gb, w.LH);
-}
-template <int FLUX>
-void launch_loads_v(const DevGrid& g, const AdjGrid& a, const Params& P, const CellLoader& cl, double mcx, double mcy, double mcz,
-                    double* tmp, int nlf, int* status, hipStream_t s) {
-    dim3 grid((nlf + 255) / 256), blk(256);
-    if (P.viscous) hipLaunchKernelGGL((loads_face_kernel<FLUX, 1>), grid, blk, 0, s, g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status);
-    else hipLaunchKernelGGL((loads_face_kernel<FLUX, 0>), grid, blk, 0, s, g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status);
-}
+        if constexpr (NV == 6) face<FLUX, 1>(g, P, cl, Rbar, fxw, Z, status, s);
+        else if (P.viscous) face<FLUX, 1>(g, P, cl, Rbar, fxw, Z, status, s);
+        else face<FLUX, 0>(g, P, cl, Rbar, fxw, Z, status, s);
+    }
+    template <int LIM>
+    static void cell(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double phi_const,
+                     const double* dref, const double* Rbar, const AdjWork& w, hipStream_t s) {
+        dim3 grid((g.N + g.B + 255) / 256), blk(256);
+        if constexpr (NV == 6)
+            hipLaunchKernelGGL((cell_adj_kernel<LIM, 1, NV>), grid, blk, 0, s, g, P, W, G, PHI, phi_const, dref, Rbar, w.Z, w.Wb1, w.Gb, w.LH);
+        else if (P.viscous)
+            hipLaunchKernelGGL((cell_adj_kernel<LIM, 1, NV>), grid, blk, 0, s, g, P, W, G, PHI, phi_const, dref, Rbar, w.Z, w.Wb1, w.Gb, w.LH);
+        else
+            hipLaunchKernelGGL((cell_adj_kernel<LIM, 0, NV>), grid, blk, 0, s, g, P, W, G, PHI, phi_const, dref, Rbar, w.Z, w.Wb1, w.Gb, w.LH);
+    }
+    template <int FLUX>
+    static void loads_v(const DevGrid& g, const AdjGrid& a, const Params& P, const CellLoader<NV>& cl, double mcx, double mcy, double mcz,
+                        double* tmp, int nlf, int* status, hipStream_t s) {
+        dim3 grid((nlf + 255) / 256), blk(256);
+        if constexpr (NV == 6)
+            hipLaunchKernelGGL((loads_face_kernel<FLUX, 1, NV>), grid, blk, 0, s, g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status);
+        else if (P.viscous)
+            hipLaunchKernelGGL((loads_face_kernel<FLUX, 1, NV>), grid, blk, 0, s, g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status);
+        else
+            hipLaunchKernelGGL((loads_face_kernel<FLUX, 0, NV>), grid, blk, 0, s, g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status);
+    }
+
+    static void vjp(const DevGrid& g, const AdjGrid& a, const Params& P, double* W, double* G, double* PHI, double* dref,
+                    const double* Rbar, const double* fxw, const AdjWork& w, double* Wbar, int* status, hipStream_t s) {
+        const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+        const bool has_phi = lim != FCFD_LIMITER_NONE;
+        const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
+        launch_ghosts(g, P, W, status, s);
+        launch_grad_lim(g, P, W, G, PHI, dref, s);
+        CellLoader<NV> cl{W, G, PHI, has_phi, phi_const, (size_t)g.ld};
+        switch (P.flux) {
+        case FCFD_FLUX_ROE: face_v<FCFD_FLUX_ROE>(g, P, cl, Rbar, fxw, w.Z, status, s); break;
+        case FCFD_FLUX_VAN_LEER: face_v<FCFD_FLUX_VAN_LEER>(g, P, cl, Rbar, fxw, w.Z, status, s); break;
+        case FCFD_FLUX_SD_SLAU: face_v<FCFD_FLUX_SD_SLAU>(g, P, cl, Rbar, fxw, w.Z, status, s); break;
+        case FCFD_FLUX_SW: face_v<FCFD_FLUX_SW>(g, P, cl, Rbar, fxw, w.Z, status, s); break;
+        default: face_v<FCFD_FLUX_AUSM_PLUS_UP>(g, P, cl, Rbar, fxw, w.Z, status, s); break;
+        }
+        switch (lim) {
+        case FCFD_LIMITER_VK: cell<FCFD_LIMITER_VK>(g, P, W, G, PHI, phi_const, dref, Rbar, w, s); break;
+        case FCFD_LIMITER_BJ: cell<FCFD_LIMITER_BJ>(g, P, W, G, PHI, phi_const, dref, Rbar, w, s); break;
+        case FCFD_LIMITER_MINMOD: cell<FCFD_LIMITER_MINMOD>(g, P, W, G, PHI, phi_const, dref, Rbar, w, s); break;
+        default: cell<FCFD_LIMITER_NONE>(g, P, W, G, PHI, phi_const, dref, Rbar, w, s); break;
+        }
+        hipLaunchKernelGGL(gradt_kernel<NV>, dim3((g.N + g.B + 255) / 256), dim3(256), 0, s, g, a, has_phi ? 1 : 0, w.Wb1, w.Gb, w.LH, w.Wb2);
+        hipLaunchKernelGGL(ghost_adj_kernel<NV>, dim3((g.N + 255) / 256), dim3(256), 0, s, g, P, W, w.Wb2, Wbar, status);
+    }
+
+    static void loads(const DevGrid& g, const AdjGrid& a, const Params& P, const double* W, const double* G, const double* PHI,
+                      double mcx, double mcy, double mcz, double* tmp, int nlf, int* status, hipStream_t s) {
+        const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+        CellLoader<NV> cl{W, G, PHI, lim != FCFD_LIMITER_NONE, (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0, (size_t)g.ld};
+        if (nlf <= 0) return;
+        switch (P.flux) {
+        case FCFD_FLUX_ROE: loads_v<FCFD_FLUX_ROE>(g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status, s); break;
+        case FCFD_FLUX_VAN_LEER: loads_v<FCFD_FLUX_VAN_LEER>(g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status, s); break;
+        case FCFD_FLUX_SD_SLAU: loads_v<FCFD_FLUX_SD_SLAU>(g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status, s); break;
+        case FCFD_FLUX_SW: loads_v<FCFD_FLUX_SW>(g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status, s); break;
+        default: loads_v<FCFD_FLUX_AUSM_PLUS_UP>(g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status, s); break;
+        }
+    }
+};
 
 }  // namespace
 
 void launch_vjp(const DevGrid& g, const AdjGrid& a, const Params& P, double* W, double* G, double* PHI, double* dref,
                 const double* Rbar, const double* fxw, const AdjWork& w, double* Wbar, int* status, hipStream_t s) {
-    const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
-    const bool has_phi = lim != FCFD_LIMITER_NONE;
-    const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
-    launch_ghosts(g, P, W, status, s);
-    launch_grad_lim(g, P, W, G, PHI, dref, s);
-    CellLoader cl{W, G, PHI, has_phi, phi_const, (size_t)g.ld};
-    switch (P.flux) {
-    case FCFD_FLUX_ROE: launch_face_adj_v<FCFD_FLUX_ROE>(g, P, cl, Rbar, fxw, w.Z, status, s); break;
-    case FCFD_FLUX_VAN_LEER: launch_face_adj_v<FCFD_FLUX_VAN_LEER>(g, P, cl, Rbar, fxw, w.Z, status, s); break;
-    case FCFD_FLUX_SD_SLAU: launch_face_adj_v<FCFD_FLUX_SD_SLAU>(g, P, cl, Rbar, fxw, w.Z, status, s); break;
-    case FCFD_FLUX_SW: launch_face_adj_v<FCFD_FLUX_SW>(g, P, cl, Rbar, fxw, w.Z, status, s); break;
-    default: launch_face_adj_v<FCFD_FLUX_AUSM_PLUS_UP>(g, P, cl, Rbar, fxw, w.Z, status, s); break;
-    }
-    switch (lim) {
-    case FCFD_LIMITER_VK: launch_cell_adj<FCFD_LIMITER_VK>(g, P, W, G, PHI, phi_const, dref, w, s); break;
-    case FCFD_LIMITER_BJ: launch_cell_adj<FCFD_LIMITER_BJ>(g, P, W, G, PHI, phi_const, dref, w, s); break;
-    case FCFD_LIMITER_MINMOD: launch_cell_adj<FCFD_LIMITER_MINMOD>(g, P, W, G, PHI, phi_const, dref, w, s); break;
-    default: launch_cell_adj<FCFD_LIMITER_NONE>(g, P, W, G, PHI, phi_const, dref, w, s); break;
-    }
-    hipLaunchKernelGGL(gradt_kernel, dim3((g.N + g.B + 255) / 256), dim3(256), 0, s, g, a, has_phi ? 1 : 0, w.Wb1, w.Gb, w.LH, w.Wb2);
-    hipLaunchKernelGGL(ghost_adj_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, P, W, w.Wb2, Wbar, status);
+    if (g.nv == 6) AdjLaunch<6>::vjp(g, a, P, W, G, PHI, dref, Rbar, fxw, w, Wbar, status, s);
+    else AdjLaunch<5>::vjp(g, a, P, W, G, PHI, dref, Rbar, fxw, w, Wbar, status, s);
 }
 
 void launch_loads(const DevGrid& g, const AdjGrid& a, const Params& P, const double* W, const double* G, const double* PHI,
                   double mcx, double mcy, double mcz, double* tmp, int nlf, double* out, int* status, hipStream_t s) {
-    const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
-    CellLoader cl{W, G, PHI, lim != FCFD_LIMITER_NONE, (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0, (size_t)g.ld};
-    if (nlf > 0) {
-        switch (P.flux) {
-        case FCFD_FLUX_ROE: launch_loads_v<FCFD_FLUX_ROE>(g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status, s); break;
-        case FCFD_FLUX_VAN_LEER: launch_loads_v<FCFD_FLUX_VAN_LEER>(g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status, s); break;
-        case FCFD_FLUX_SD_SLAU: launch_loads_v<FCFD_FLUX_SD_SLAU>(g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status, s); break;
-        case FCFD_FLUX_SW: launch_loads_v<FCFD_FLUX_SW>(g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status, s); break;
-        default: launch_loads_v<FCFD_FLUX_AUSM_PLUS_UP>(g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status, s); break;
-        }
-    }
+    if (g.nv == 6) AdjLaunch<6>::loads(g, a, P, W, G, PHI, mcx, mcy, mcz, tmp, nlf, status, s);
+    else AdjLaunch<5>::loads(g, a, P, W, G, PHI, mcx, mcy, mcz, tmp, nlf, status, s);
     if (a.n_bc > 0) hipLaunchKernelGGL(loads_sum_kernel, dim3((6 * a.n_bc + 63) / 64), dim3(64), 0, s, a, tmp, nlf, out);
 }
 

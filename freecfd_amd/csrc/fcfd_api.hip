@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -23,6 +25,7 @@ using namespace fcfd;
 struct fcfd_ctx {
     HostGrid hg;
     fcfd_options opt;
+    bool gpu_wall_distance = false;   // wall distances left to the device set-up (large meshes)
     Params P;
     std::vector<BcDev> bch;
     bool device = false;
@@ -109,6 +112,12 @@ int validate_options(fcfd_ctx* c) {
     }
     if (o.grad_hex == FCFD_GRAD_GREENGAUSS || o.grad_prism == FCFD_GRAD_GREENGAUSS || o.grad_other == FCFD_GRAD_GREENGAUSS)
         return fail(c, FCFD_ENOTSUP, "Green-Gauss gradients are not supported by this build");
+    if (o.turbulence != FCFD_TURB_NONE && o.turbulence != FCFD_TURB_SA) return fail(c, FCFD_EINVAL, "unknown turbulence model");
+    if (o.turbulence == FCFD_TURB_SA) {
+        if (!(o.visc_model == FCFD_VISC_SUTHERLAND || o.mu > 0.0))
+            return fail(c, FCFD_EINVAL, "the Spalart-Allmaras model needs a positive molecular viscosity");
+        if (!(o.Pr_t > 0.0)) return fail(c, FCFD_EINVAL, "the Spalart-Allmaras model needs a positive turbulent Prandtl number");
+    }
     return FCFD_OK;
 }
 
@@ -129,6 +138,11 @@ void set_params(fcfd_ctx* c) {
     bool fixedq = false;
     for (int b = 0; b < o.n_bc; ++b) fixedq |= (o.bc[b].thermal == FCFD_THERMAL_FIXED_Q && o.bc[b].qdot != 0.0);
     P.viscous = (inviscid && !fixedq) ? 0 : 1;
+    P.nv = (o.turbulence == FCFD_TURB_SA) ? 6 : 5;
+    P.Pr_t = o.Pr_t;
+    double nut_scale = 0.0;
+    for (int b = 0; b < o.n_bc; ++b) nut_scale = std::max(nut_scale, std::fabs(o.bc[b].nut));
+    P.nut_eps = 10. * 1.4901161193847656e-08 * (nut_scale > 0.0 ? nut_scale : 1.0);
     c->bch.assign(std::max(o.n_bc, 1), BcDev{});
     for (int b = 0; b < o.n_bc; ++b) {
         const fcfd_bc& s = o.bc[b];
@@ -137,6 +151,7 @@ void set_params(fcfd_ctx* c) {
         d.p = s.p; d.T = s.T; d.rho = s.rho; d.V[0] = s.V[0]; d.V[1] = s.V[1]; d.V[2] = s.V[2];
         d.p_total = s.p_total; d.T_total = s.T_total; d.mdot = s.mdot; d.qdot = s.qdot;
         d.total_area = b < (int)c->hg.bc_area.size() ? c->hg.bc_area[b] : 0.0;
+        d.nut = s.nut;
     }
 }
 
@@ -164,21 +179,22 @@ int build_device(fcfd_ctx* c) {
     const int N = g.N, B = g.B, F = g.F, T = N + B;
     DevGrid& d = c->dg;
     d.N = N; d.B = B; d.F = F;
+    d.nv = c->P.nv;
     d.ld = ((T + 63) / 64) * 64;
     const int ld = d.ld;
     d.ref_first = g.perm[0];
     d.ref_last = g.perm[N - 1];
     // cells
     {
-        std::vector<double> x(ld, 0.0), y(ld, 0.0), z(ld, 0.0), v(ld, 0.0), l(ld, 0.0);
+        std::vector<double> x(ld, 0.0), y(ld, 0.0), z(ld, 0.0), v(ld, 0.0), l(ld, 0.0), w(ld, 1.e20);
         for (int i = 0; i < T; ++i) {
             int r = g.iperm[i];
             x[i] = g.cc[3 * r]; y[i] = g.cc[3 * r + 1]; z[i] = g.cc[3 * r + 2];
-            v[i] = g.vol[r]; l[i] = g.lscale[r];
+            v[i] = g.vol[r]; l[i] = g.lscale[r]; w[i] = g.wdist[r];
         }
         int rc;
         if ((rc = upload(c, x, &d.ccx)) || (rc = upload(c, y, &d.ccy)) || (rc = upload(c, z, &d.ccz)) ||
-            (rc = upload(c, v, &d.vol)) || (rc = upload(c, l, &d.lscale)))
+            (rc = upload(c, v, &d.vol)) || (rc = upload(c, l, &d.lscale)) || (rc = upload(c, w, &d.wd)))
             return rc;
     }
     // gradient maps, sliced ELL with 64-cell slices
@@ -267,10 +283,41 @@ int build_device(fcfd_ctx* c) {
     {
         int rc;
         if ((rc = upload(c, c->bch, &d.bc))) return rc;
-        if ((rc = dalloc(c, (size_t)15 * ld, &c->G)) || (rc = dalloc(c, (size_t)5 * ld, &c->PHI)) ||
+        const size_t nv = d.nv;
+        if ((rc = dalloc(c, 3 * nv * ld, &c->G)) || (rc = dalloc(c, nv * ld, &c->PHI)) ||
             (rc = dalloc(c, (size_t)kDrefSize, &c->dref)) || (rc = dalloc(c, 4, &c->status)) ||
-            (rc = dalloc(c, (size_t)5 * g.F, &c->FX)))
+            (rc = dalloc(c, nv * g.F, &c->FX)))
             return rc;
+    }
+    if (c->gpu_wall_distance) {   // brute force on the device, then mirrored into the host metrics
+        std::vector<double> wx, wy, wz, wnx, wny, wnz;
+        std::vector<int> is_wall(F, 0);
+        for (int f = 0; f < F; ++f) {   // reference face order (first closest face wins ties)
+            int b = g.fbc[f];
+            if (b >= 0 && c->opt.bc[b].type == FCFD_BC_WALL && c->opt.bc[b].kind != FCFD_KIND_SLIP) {
+                wx.push_back(g.fc[3 * f]); wy.push_back(g.fc[3 * f + 1]); wz.push_back(g.fc[3 * f + 2]);
+                wnx.push_back(g.fn[3 * f]); wny.push_back(g.fn[3 * f + 1]); wnz.push_back(g.fn[3 * f + 2]);
+                is_wall[g.fperm[f]] = 1;
+            }
+        }
+        const int nw = (int)wx.size();
+        const double *dwx, *dwy, *dwz, *dwnx, *dwny, *dwnz;
+        const int* dis;
+        int rc;
+        if ((rc = upload(c, wx, &dwx)) || (rc = upload(c, wy, &dwy)) || (rc = upload(c, wz, &dwz)) || (rc = upload(c, wnx, &dwnx)) ||
+            (rc = upload(c, wny, &dwny)) || (rc = upload(c, wnz, &dwnz)) || (rc = upload(c, is_wall, &dis)))
+            return rc;
+        launch_wall_distance(T, d.ccx, d.ccy, d.ccz, nw, dwx, dwy, dwz, const_cast<double*>(d.wd), nullptr);
+        launch_face_wall(F, d.fcx, d.fcy, d.fcz, d.fnx, d.fny, d.fnz, dis, nw, dwx, dwy, dwz, dwnx, dwny, dwnz,
+                         const_cast<double*>(d.fwd), const_cast<double*>(d.fdiss), nullptr);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipDeviceSynchronize());
+        std::vector<double> tw(ld), tf(F), td(F);
+        HIPCHK(c, hipMemcpy(tw.data(), d.wd, ld * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(tf.data(), d.fwd, F * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(td.data(), d.fdiss, F * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < T; ++i) g.wdist[g.iperm[i]] = tw[i];
+        for (int i = 0; i < F; ++i) { g.fwd[g.ifperm[i]] = tf[i]; g.fdiss[g.ifperm[i]] = td[i]; }
     }
     c->device = true;
     return FCFD_OK;
@@ -301,10 +348,14 @@ int create_impl(const fcfd_mesh* mesh, const fcfd_options* opt, fcfd_ctx** out, 
     if (!rc) {
         long long nw = 0;
         for (int b = 0; b < opt->n_bc; ++b) nw += (opt->bc[b].type == FCFD_BC_WALL && opt->bc[b].kind != FCFD_KIND_SLIP);
-        bool host_wd = (long long)mesh->n_cells * (nw ? 1 : 0) < 2000000LL;   // brute force only for small meshes
-        rc = build_host_grid(mesh, opt, c->hg, c->err, host_wd || nw == 0);
-        if (!rc && !host_wd && opt->convective_flux == FCFD_FLUX_SW)
-            rc = fail(c, FCFD_ENOTSUP, "Steger-Warming flux needs wall distances; not computed for meshes this large");
+        // brute force on the host for small meshes; the device computes them
+        // for large ones (the same exact minimum, set_bcs.cc:178-249)
+        // (a host-only context of a large mesh leaves them unset: 1e20)
+        long long host_max = 2000000LL;   // FCFD_HOST_WALL_DISTANCE_MAX_CELLS overrides (tests force the device path)
+        if (const char* e = std::getenv("FCFD_HOST_WALL_DISTANCE_MAX_CELLS")) host_max = std::atoll(e);
+        bool host_wd = nw == 0 || (long long)mesh->n_cells < host_max;
+        rc = build_host_grid(mesh, opt, c->hg, c->err, host_wd);
+        c->gpu_wall_distance = !host_wd && device;
     }
     if (!rc) {
         renumber(c->hg);
@@ -348,7 +399,7 @@ int fcfd_get_info(const fcfd_ctx* c, fcfd_info* info) {
     info->n_faceavg = (int)g.aidx.size();
     info->n_nbr = (int)g.nb.size();
     info->n_halffaces = (int)g.cfaces.size();
-    info->nv = 5;
+    info->nv = (c->opt.turbulence == FCFD_TURB_SA) ? 6 : 5;
     return FCFD_OK;
 }
 
@@ -377,10 +428,11 @@ int fcfd_state_to_device(fcfd_ctx* c, const double* W_host, double* W_dev) {
     if (!c || !W_host || !W_dev) return FCFD_EINVAL;
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data (created with fcfd_create_host)");
     const int N = c->hg.N, ld = c->dg.ld;
-    std::vector<double> soa((size_t)5 * ld, 0.0);
+    const int nv = c->P.nv;
+    std::vector<double> soa((size_t)nv * ld, 0.0);
     for (int i = 0; i < N; ++i) {
         int r = c->hg.iperm[i];
-        for (int v = 0; v < 5; ++v) soa[(size_t)v * ld + i] = W_host[5 * (size_t)r + v];
+        for (int v = 0; v < nv; ++v) soa[(size_t)v * ld + i] = W_host[(size_t)nv * r + v];
     }
     HIPCHK(c, hipMemcpy(W_dev, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
     return FCFD_OK;
@@ -421,7 +473,7 @@ int fcfd_residual(fcfd_ctx* c, double* W_dev, double* R_dev, void* stream) {
     if (e) HIPCHK(c, hipEventRecord(e[2], s));
     launch_flux(c->dg, c->P, W_dev, c->G, c->PHI, c->FX, R_dev, c->status, s);
     if (e) HIPCHK(c, hipEventRecord(e[3], s));
-    launch_cell_sum(c->dg, c->FX, R_dev, s);
+    launch_residual_sum(c->dg, c->P, W_dev, c->G, c->FX, R_dev, s);
     if (e) {
         HIPCHK(c, hipEventRecord(e[4], s));
         c->ring_head = (c->ring_head + 1) % fcfd_ctx::kRing;
@@ -486,8 +538,9 @@ int fcfd_fd_jacobian(fcfd_ctx* c, double* W_dev, const double* dWprev_dev, const
     const DevGrid& g = c->dg;
     if (!c->JL) {
         int rc;
-        if ((rc = dalloc(c, (size_t)25 * g.F, &c->JL)) || (rc = dalloc(c, (size_t)25 * g.F, &c->JR)) ||
-            (rc = dalloc(c, (size_t)25 * g.ld, &c->Dg)) || (rc = dalloc(c, (size_t)5 * g.ld, &c->zero_upd)))
+        const size_t b2 = (size_t)g.nv * g.nv;
+        if ((rc = dalloc(c, b2 * g.F, &c->JL)) || (rc = dalloc(c, b2 * g.F, &c->JR)) ||
+            (rc = dalloc(c, b2 * g.ld, &c->Dg)) || (rc = dalloc(c, (size_t)g.nv * g.ld, &c->zero_upd)))
             return rc;
     }
     hipStream_t s = (hipStream_t)stream;
@@ -512,22 +565,23 @@ int fcfd_jacobian_to_host(fcfd_ctx* c, double* JL_host, double* JR_host, double*
     HIPCHK(c, hipDeviceSynchronize());
     const HostGrid& hg = c->hg;
     const int F = hg.F, N = hg.N, ld = c->dg.ld;
-    std::vector<double> tmp((size_t)25 * std::max(F, ld));
+    const int b2 = c->dg.nv * c->dg.nv;
+    std::vector<double> tmp((size_t)b2 * std::max(F, ld));
     double* outs[2] = {JL_host, JR_host};
     const double* srcs[2] = {c->JL, c->JR};
     for (int k = 0; k < 2; ++k) {
         if (!outs[k]) continue;
-        HIPCHK(c, hipMemcpy(tmp.data(), srcs[k], (size_t)25 * F * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(tmp.data(), srcs[k], (size_t)b2 * F * sizeof(double), hipMemcpyDeviceToHost));
         for (int i = 0; i < F; ++i) {
             int f = hg.ifperm[i];
-            for (int e = 0; e < 25; ++e) outs[k][(size_t)25 * f + e] = tmp[(size_t)e * F + i];
+            for (int e = 0; e < b2; ++e) outs[k][(size_t)b2 * f + e] = tmp[(size_t)e * F + i];
         }
     }
     if (D_host) {
-        HIPCHK(c, hipMemcpy(tmp.data(), c->Dg, (size_t)25 * ld * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(tmp.data(), c->Dg, (size_t)b2 * ld * sizeof(double), hipMemcpyDeviceToHost));
         for (int i = 0; i < N; ++i) {
             int r = hg.iperm[i];
-            for (int e = 0; e < 25; ++e) D_host[(size_t)25 * r + e] = tmp[(size_t)e * ld + i];
+            for (int e = 0; e < b2; ++e) D_host[(size_t)b2 * r + e] = tmp[(size_t)e * ld + i];
         }
     }
     return FCFD_OK;
@@ -537,9 +591,10 @@ static int ensure_tangent(fcfd_ctx* c) {
     if (c->dG) return FCFD_OK;
     const DevGrid& g = c->dg;
     int rc;
-    if ((rc = dalloc(c, (size_t)15 * g.ld, &c->dG)) || (rc = dalloc(c, (size_t)5 * g.ld, &c->dPHI)) ||
-        (rc = dalloc(c, (size_t)5 * g.F, &c->dFX)) || (rc = dalloc(c, (size_t)5 * g.ld, &c->jv_in)) ||
-        (rc = dalloc(c, (size_t)5 * g.ld, &c->jv_out)) || (rc = dalloc(c, (size_t)5 * g.ld, &c->jv_R)))
+    const size_t nv = g.nv;
+    if ((rc = dalloc(c, 3 * nv * g.ld, &c->dG)) || (rc = dalloc(c, nv * g.ld, &c->dPHI)) ||
+        (rc = dalloc(c, nv * g.F, &c->dFX)) || (rc = dalloc(c, nv * g.ld, &c->jv_in)) ||
+        (rc = dalloc(c, nv * g.ld, &c->jv_out)) || (rc = dalloc(c, nv * g.ld, &c->jv_R)))
         return rc;
     return FCFD_OK;
 }
@@ -603,10 +658,11 @@ static int ensure_adjoint(fcfd_ctx* c) {
             (rc = dalloc(c, (size_t)6 * std::max(nbc, 1), &c->loads_out)))
             return rc;
     }
-    const size_t nz = c->P.viscous ? 32 : 20;
-    if ((rc = dalloc(c, nz * F, &c->aw.Z)) || (rc = dalloc(c, (size_t)5 * d.ld, &c->aw.Wb1)) ||
-        (rc = dalloc(c, (size_t)15 * d.ld, &c->aw.Gb)) || (rc = dalloc(c, (size_t)5 * nhf, &c->aw.LH)) ||
-        (rc = dalloc(c, (size_t)5 * d.ld, &c->aw.Wb2)))
+    const size_t nv = d.nv;
+    const size_t nz = 4 * nv + ((c->P.viscous || nv == 6) ? (nv == 6 ? 15 : 12) : 0);   // face inputs per face
+    if ((rc = dalloc(c, nz * F, &c->aw.Z)) || (rc = dalloc(c, nv * d.ld, &c->aw.Wb1)) ||
+        (rc = dalloc(c, 3 * nv * d.ld, &c->aw.Gb)) || (rc = dalloc(c, nv * nhf, &c->aw.LH)) ||
+        (rc = dalloc(c, nv * d.ld, &c->aw.Wb2)))
         return rc;
     c->adj_ready = true;
     return FCFD_OK;
@@ -665,7 +721,8 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad Krylov options");
     const DevGrid& g = c->dg;
     hipStream_t s = (hipStream_t)stream;
-    const size_t n5 = (size_t)5 * g.ld;
+    const size_t n5 = (size_t)g.nv * g.ld;      // one [nv][ld] vector
+    const size_t b2 = (size_t)g.nv * g.nv;     // entries of one block
     int rc;
     if (!c->R) {
         if ((rc = dalloc(c, n5, &c->R)) || (rc = dalloc(c, (size_t)g.ld, &c->dt)) || (rc = dalloc(c, n5, &c->dW)) ||
@@ -673,8 +730,8 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
             return rc;
     }
     if (!c->JL) {
-        if ((rc = dalloc(c, (size_t)25 * g.F, &c->JL)) || (rc = dalloc(c, (size_t)25 * g.F, &c->JR)) ||
-            (rc = dalloc(c, (size_t)25 * g.ld, &c->Dg)) || (rc = dalloc(c, n5, &c->zero_upd)))
+        if ((rc = dalloc(c, b2 * g.F, &c->JL)) || (rc = dalloc(c, b2 * g.F, &c->JR)) ||
+            (rc = dalloc(c, b2 * g.ld, &c->Dg)) || (rc = dalloc(c, n5, &c->zero_upd)))
             return rc;
     }
     const int fp32 = o->krylov_fp32 ? 1 : 0;
@@ -685,8 +742,8 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
                 (rc = dalloc(c, m1 * 512, &c->w32.part)) || (rc = dalloc(c, m1, &c->w32.out_dev)) || (rc = dalloc(c, m1, &c->w32.coef_dev)))
                 return rc;
             if (!c->Dinv32) {
-                if ((rc = dalloc(c, (size_t)25 * g.ld, &c->Dinv32)) || (rc = dalloc(c, (size_t)25 * g.ld, &c->D32)) ||
-                    (rc = dalloc(c, (size_t)25 * g.F, &c->JL32)) || (rc = dalloc(c, (size_t)25 * g.F, &c->JR32)) ||
+                if ((rc = dalloc(c, b2 * g.ld, &c->Dinv32)) || (rc = dalloc(c, b2 * g.ld, &c->D32)) ||
+                    (rc = dalloc(c, b2 * g.F, &c->JL32)) || (rc = dalloc(c, b2 * g.F, &c->JR32)) ||
                     (rc = dalloc(c, n5, &c->b32)) || (rc = dalloc(c, n5, &c->x32)))
                     return rc;
             }
@@ -694,7 +751,7 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
             if ((rc = dalloc(c, m1 * n5, &c->w64.V)) || (rc = dalloc(c, n5, &c->w64.z)) || (rc = dalloc(c, n5, &c->w64.r)) ||
                 (rc = dalloc(c, m1 * 512, &c->w64.part)) || (rc = dalloc(c, m1, &c->w64.out_dev)) || (rc = dalloc(c, m1, &c->w64.coef_dev)))
                 return rc;
-            if (!c->Dinv64 && (rc = dalloc(c, (size_t)25 * g.ld, &c->Dinv64))) return rc;
+            if (!c->Dinv64 && (rc = dalloc(c, b2 * g.ld, &c->Dinv64))) return rc;
         }
         c->gm_restart = o->restart;
         c->gm_fp32 = fp32;
@@ -707,7 +764,7 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     launch_ghosts(g, c->P, W_dev, c->status, s);
     launch_grad_lim(g, c->P, W_dev, c->G, c->PHI, c->dref, s);
     launch_flux(g, c->P, W_dev, c->G, c->PHI, c->FX, c->R, c->status, s);
-    launch_cell_sum(g, c->FX, c->R, s);
+    launch_residual_sum(g, c->P, W_dev, c->G, c->FX, c->R, s);
     double rr = 0.0;
     HIPCHK(c, dev_mdot(g, 1, c->R, n5, c->R, c->red_part, c->red_out, &rr, s));
     S.res_norm = std::sqrt(rr);
@@ -718,9 +775,9 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     if (fp32) {
         dev_bj_invert<float>(g, c->Dg, c->Dinv32, c->status, s);
         if (!mfree) {
-            dev_convert<double, float>((size_t)25 * g.ld, c->Dg, c->D32, s);
-            dev_convert<double, float>((size_t)25 * g.F, c->JL, c->JL32, s);
-            dev_convert<double, float>((size_t)25 * g.F, c->JR, c->JR32, s);
+            dev_convert<double, float>(b2 * g.ld, c->Dg, c->D32, s);
+            dev_convert<double, float>(b2 * g.F, c->JL, c->JL32, s);
+            dev_convert<double, float>(b2 * g.F, c->JR, c->JR32, s);
         }
         dev_convert<double, float>(n5, c->R, c->b32, s);
     } else {
@@ -735,7 +792,7 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     // matrix-free operator: y = (P V/dt) x - J x with the tangent of W (the
     // ghost slots of W are rewritten with the same values by each tangent call)
     auto jvp_op = [&](const double* x, double* y, hipStream_t ss) {
-        hipMemcpyAsync(c->jv_in, x, n5 * sizeof(double), hipMemcpyDeviceToDevice, ss);
+        (void)hipMemcpyAsync(c->jv_in, x, n5 * sizeof(double), hipMemcpyDeviceToDevice, ss);
         launch_jvp(g, c->P, W_dev, c->jv_in, c->G, c->dG, c->PHI, c->dPHI, c->dref, c->dFX, c->jv_R,
                    o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss);
         launch_newton_op(g, c->P, W_dev, c->dt, x, c->jv_R, y, ss);
@@ -767,12 +824,13 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     HIPCHK(c, hipStreamSynchronize(s));
     auto t3 = clk::now();
     // 4. update W, norms, NaN check; keep dW as the previous update
-    double out[4];
+    double out[5];
     HIPCHK(c, dev_update(g, W_dev, c->dW, c->dt, c->red_part, c->red_out, out, s));
     HIPCHK(c, hipMemcpyAsync(c->upd, c->dW, n5 * sizeof(double), hipMemcpyDeviceToDevice, s));
     HIPCHK(c, hipStreamSynchronize(s));
     auto t4 = clk::now();
     S.update_norms[0] = std::sqrt(out[0]); S.update_norms[1] = std::sqrt(out[1]); S.update_norms[2] = std::sqrt(out[2]);
+    S.nut_norm = std::sqrt(out[4]);
     S.krylov_iters = gs.iters; S.krylov_converged = gs.converged; S.krylov_rnorm = gs.rnorm; S.krylov_r0norm = gs.r0norm;
     S.ms_residual = ms(t0, t1); S.ms_jacobian = ms(t1, t2); S.ms_krylov = ms(t2, t3); S.ms_update = ms(t3, t4);
     if (st) *st = S;

@@ -8,12 +8,16 @@ namespace fcfd {
 
 // All arrays are device pointers; cells and faces are in internal (RCM)
 // order.  Cell arrays are padded to `ld` entries; interior cells [0,N), ghosts
-// [N, N+B).  A half-face entry hf_face[h] is the internal face id f when the
+// [N, N+B).  Per-cell variable arrays are [nv][ld] (p, u, v, w, T, and nu~
+// with the SA model), gradients [3 nv][ld], limiters [nv][ld], face fluxes
+// [nv][F].  A half-face entry hf_face[h] is the internal face id f when the
 // owning cell is the face's parent and ~f when it is the neighbour; the
 // half-faces of a cell are in ascending reference face order, so summing them
 // in order reproduces the reference's VecSetValues(ADD) sequence.
 struct DevGrid {
     int N, B, F, ld, nhf, nblk;
+    int nv;                             // variables per cell (5, or 6 with SA)
+    const double* wd;                   // [ld] closest wall distance of each cell (SA)
     int ref_first, ref_last;            // internal ids of reference cells 0 and N-1
     const double *ccx, *ccy, *ccz, *vol, *lscale;
     const int* gslice;                  // N/64+1 entry offsets of 64-wide gradient slices
@@ -32,11 +36,20 @@ void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double*
 // face fluxes D - C per face into FX ([5][F]) ...
 void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* FX,
                  double* R, int* status, hipStream_t s);
-// ... and their ordered per-cell sums into R
+// ... and their ordered per-cell sums into R (no source: tangents)
 void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s);
+// the residual's per-cell sums, plus V S in the nu~ row with the SA model
+void launch_residual_sum(const DevGrid& g, const Params& P, const double* W, const double* G, const double* FX, double* R,
+                         hipStream_t s);
 // VK reference range only (find_min_max), when the limiter threshold needs it
 void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double* dref, hipStream_t s);
-constexpr int kDrefSize = 8 + 8 * 1024;   // dref[5] + VK min/max partials
+constexpr int kDrefSize = 8 + 10 * 1024;   // dref[nv] + VK min/max partials
 void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s);
+// brute-force wall distances (cells: T centroids; faces: with the SW dissipation factor)
+void launch_wall_distance(int T, const double* ccx, const double* ccy, const double* ccz, int nw, const double* wx, const double* wy,
+                          const double* wz, double* wd, hipStream_t s);
+void launch_face_wall(int F, const double* fcx, const double* fcy, const double* fcz, const double* fnx, const double* fny,
+                      const double* fnz, const int* is_wall, int nw, const double* wx, const double* wy, const double* wz,
+                      const double* wnx, const double* wny, const double* wnz, double* fwd, double* fdiss, hipStream_t s);
 
 }  // namespace fcfd

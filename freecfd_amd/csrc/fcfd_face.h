@@ -22,7 +22,8 @@ __device__ __forceinline__ void load_geo(const DevGrid& g, int f, FaceGeo& F) {
 
 __device__ __forceinline__ Vec<double> centroid(const DevGrid& g, int c) { return mk<double>(g.ccx[c], g.ccy[c], g.ccz[c]); }
 
-// primal cell loader: W [5][ld], gradients [15][ld], limiters [5][ld]
+// primal cell loader: W [NV][ld], gradients [3 NV][ld], limiters [NV][ld]
+template <int NV>
 struct CellLoader {
     const double* __restrict__ W;
     const double* __restrict__ G;
@@ -30,9 +31,9 @@ struct CellLoader {
     bool has_phi;
     double phi_const;
     size_t ld;
-    __device__ __forceinline__ void load(int c, bool ghost, double q[5], Vec<double> gr[5], double ph[5]) const {
+    __device__ __forceinline__ void load(int c, bool ghost, double q[NV], Vec<double> gr[NV], double ph[NV]) const {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
+        for (int v = 0; v < NV; ++v) {
             q[v] = W[v * ld + c];
             gr[v] = mk<double>(G[(3 * v) * ld + c], G[(3 * v + 1) * ld + c], G[(3 * v + 2) * ld + c]);
             ph[v] = (has_phi && !ghost) ? PHI[v * ld + c] : (ghost ? 0.0 : phi_const);
@@ -43,20 +44,20 @@ struct CellLoader {
 // tangent cell loader: value = primal arrays, derivative = tangent arrays.
 // FULL = false: gradients, limiters and ghost states carry no tangent (the
 // reference FD Jacobian's frozen-gradient linearisation).
-template <bool FULL>
+template <int NV, bool FULL>
 struct DualLoader {
-    CellLoader p;
+    CellLoader<NV> p;
     const double* __restrict__ dW;
     const double* __restrict__ dG;
     const double* __restrict__ dPHI;
     int N;
-    __device__ __forceinline__ void load(int c, bool ghost, Dual q[5], Vec<Dual> gr[5], Dual ph[5]) const {
-        double qv[5], phv[5];
-        Vec<double> gv[5];
+    __device__ __forceinline__ void load(int c, bool ghost, Dual q[NV], Vec<Dual> gr[NV], Dual ph[NV]) const {
+        double qv[NV], phv[NV];
+        Vec<double> gv[NV];
         p.load(c, ghost, qv, gv, phv);
         const size_t ld = p.ld;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
+        for (int v = 0; v < NV; ++v) {
             double dq = (FULL || c < N) ? dW[v * ld + c] : 0.0;
             q[v] = Dual(qv[v], dq);
             if (FULL) {
@@ -72,12 +73,12 @@ struct DualLoader {
 };
 
 // everything one face-loop iteration computes before the residual insertion
-template <typename S>
+template <typename S, int NV>
 struct FaceCtx {
     FaceGeo F;
     CState<S> L, R;
     FState<S> fs;
-    S C[5], D[5];
+    S C[NV], D[NV];
     int p, n;
 };
 
@@ -85,8 +86,8 @@ struct FaceCtx {
 // reconstruction.  Leaves X.L reconstructed (rho set, not finished); X.R
 // reconstructed for an interior face, or holding only the ghost's centre
 // values for a boundary face; X.fs.g* the face gradients when VISC.
-template <int VISC, typename S, typename Loader>
-__device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_, const Params& P, int f, FaceCtx<S>& X) {
+template <int VISC, int NV, typename S, typename Loader>
+__device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_, const Params& P, int f, FaceCtx<S, NV>& X) {
     FaceGeo& F = X.F;
     load_geo(g, f, F);
     const int p = g.fpar[f], n = g.fnei[f];
@@ -94,13 +95,13 @@ __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_,
     Vec<double> xf = mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]);
     Vec<double> xp = centroid(g, p), xn = centroid(g, n);
     F.l2r = vsub(xn, xp);
-    S q[5], ph[5];
-    Vec<S> gp[5];
+    S q[NV], ph[NV];
+    Vec<S> gp[NV];
     ld_.load(p, false, q, gp, ph);
-    recon(P, X.L, q, gp, ph, vsub(xf, xp));
-    Vec<S> gn[5];
+    recon<NV>(P, X.L, q, gp, ph, vsub(xf, xp));
+    Vec<S> gn[NV];
     {
-        S qn[5], phn[5];
+        S qn[NV], phn[NV];
         const bool bnd = F.bc >= 0;
         ld_.load(n, bnd, qn, gn, phn);
         CState<S> Rb;
@@ -108,8 +109,10 @@ __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_,
             Rb.p_center = qn[0];
             Rb.V_center = mk<S>(qn[1], qn[2], qn[3]);
             Rb.T_center = qn[4];
+            if constexpr (NV == 6) Rb.nut_center = qn[5];
+            else Rb.nut_center = S(0.);
         } else {
-            recon(P, Rb, qn, gn, phn, vsub(xf, xn));
+            recon<NV>(P, Rb, qn, gn, phn, vsub(xf, xn));
         }
         X.R = Rb;
     }
@@ -123,6 +126,7 @@ __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_,
             X.fs.gw = vadd(vsc(wp, gp[3]), vsc(wn, gn[3]));
             X.fs.gT = vadd(vsc(wp, gp[4]), vsc(wn, gn[4]));
         }
+        if constexpr (NV == 6) X.fs.gn = (F.bc >= 0) ? gp[5] : vadd(vsc(g.fwp[f], gp[5]), vsc(g.fwn[f], gn[5]));
     }
 }
 
@@ -130,8 +134,8 @@ __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_,
 // a boundary face = a copy of the finished left state with the ghost's
 // centre values, then apply_bcs; right_state_update :298-303) and evaluate
 // the convective and diffusive fluxes.
-template <int FLUX, int VISC, typename S>
-__device__ __forceinline__ void face_tail(const DevGrid& g, const Params& P, int f, FaceCtx<S>& X, int* status) {
+template <int FLUX, int VISC, int NV, typename S>
+__device__ __forceinline__ void face_tail(const DevGrid& g, const Params& P, int f, FaceCtx<S, NV>& X, int* status) {
     FaceGeo& F = X.F;
     finish_state(P, X.L, F);
     {
@@ -141,6 +145,7 @@ __device__ __forceinline__ void face_tail(const DevGrid& g, const Params& P, int
             Rb.p_center = X.R.p_center;
             Rb.V_center = X.R.V_center;
             Rb.T_center = X.R.T_center;
+            Rb.nut_center = X.R.nut_center;
             if (apply_bcs(P, g.bc[F.bc], X.L, Rb, F)) atomicOr(status, 1);
         } else {
             Rb = X.R;
@@ -148,20 +153,20 @@ __device__ __forceinline__ void face_tail(const DevGrid& g, const Params& P, int
         X.R = Rb;
     }
     finish_state(P, X.R, F);
-    convective_flux<FLUX>(P, X.L, X.R, F, FLUX == FCFD_FLUX_SW ? g.fdiss[f] : 0.0, FLUX == FCFD_FLUX_SW ? g.fwd[f] : 0.0, X.C);
+    convective_flux<FLUX, NV>(P, X.L, X.R, F, FLUX == FCFD_FLUX_SW ? g.fdiss[f] : 0.0, FLUX == FCFD_FLUX_SW ? g.fwd[f] : 0.0, X.C);
     if (VISC) {
-        face_state(P, X.L, X.R, F, X.fs);
-        diffusive_flux(X.fs, F, g.bc, X.D);
+        face_state<NV>(P, X.L, X.R, F, X.fs);
+        diffusive_flux<NV>(P, X.fs, F, g.bc, X.D);
     } else {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) X.D[v] = S(0.0);
+        for (int v = 0; v < NV; ++v) X.D[v] = S(0.0);
     }
 }
 
-template <int FLUX, int VISC, typename S, typename Loader>
-__device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, const Loader& ld_, int f, FaceCtx<S>& X, int* status) {
-    face_inputs<VISC>(g, ld_, P, f, X);
-    face_tail<FLUX, VISC>(g, P, f, X, status);
+template <int FLUX, int VISC, int NV, typename S, typename Loader>
+__device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, const Loader& ld_, int f, FaceCtx<S, NV>& X, int* status) {
+    face_inputs<VISC, NV>(g, ld_, P, f, X);
+    face_tail<FLUX, VISC, NV>(g, P, f, X, status);
 }
 
 }  // namespace fcfd

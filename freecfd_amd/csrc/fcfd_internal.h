@@ -51,12 +51,16 @@ struct Params {
     double limiter_threshold, Minf, wdiss, bl_height;
     double glscale;
     int viscous;     // 0: mu and lambda identically zero -> diffusive flux is exactly 0
+    int nv;          // variables per cell: 5, or 6 with the SA model (nu~ last)
+    double Pr_t;     // turbulent Prandtl number (SA)
+    double nut_eps;  // FD step floor for nu~ (10 sqrt(eps) times the largest inlet nu~)
 };
 
 // BC table entry on the device (uniform BC values).
 struct BcDev {
     int type, kind, specified, thermal;
     double p, T, rho, V[3], p_total, T_total, mdot, qdot, total_area;
+    double nut;      // SA nu~ prescribed at an inlet
 };
 
 }  // namespace fcfd

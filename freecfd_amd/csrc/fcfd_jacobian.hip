@@ -1,18 +1,23 @@
 // Reference-equivalent finite-difference Jacobian (FP64) and its assembly.
 //
-//   fdjac_kernel     one thread per face: the 1 + 10 flux evaluations of
+//   fdjac_kernel     one thread per face: the 1 + 2 NV flux evaluations of
 //                    get_jacobians (ns_assemble_linear_system.cc:197-267) with
 //                    the same epsilon policy, BC re-application on left
 //                    perturbations, face-state adjustment (including the
 //                    case-1 fall-through and the face-state drift between
 //                    perturbed variables), writing JL[var][eq] and JR[var][eq]
-//                    per face ([25][F] each, coalesced).
+//                    per face ([NV*NV][F] each, coalesced).
 //   diag_kernel      one thread per cell: diagonal block = sum over the cell's
 //                    faces in reference face order of -JL (parent side) or
-//                    +JR (neighbour side), then + P V / dt (time_terms,
-//                    ns_time_terms.cc:147-163) -- the MatSetValues(ADD) order.
+//                    +JR (neighbour side), then (SA model) the source block by
+//                    one-sided differences in the cell state, then + P V / dt
+//                    (time_terms, ns_time_terms.cc:147-163) -- the
+//                    MatSetValues(ADD) order.
 //   The off-diagonal block (row P, col N) is -JR^T and (row N, col P) is +JL^T
 //   of the face; each is touched by exactly one face, so it stays per face.
+// With the SA model (NV = 6, no reference counterpart) nu~ is perturbed like
+// T (face and centre value), the SA face quantities follow every perturbation,
+// and the nu~ step floor is 10 sqrt(eps) times the largest inlet nu~.
 #include <hip/hip_runtime.h>
 
 #include "fcfd_face.h"
@@ -27,14 +32,16 @@ __device__ __forceinline__ void state_perturb(const Params& P, CState<double>& s
     case 1: s.V.x += eps; s.V_center.x += eps; s.Vn = mk<double>(vdot(s.V, F.n), vdot(s.V, F.t1), vdot(s.V, F.t2)); break;
     case 2: s.V.y += eps; s.V_center.y += eps; s.Vn = mk<double>(vdot(s.V, F.n), vdot(s.V, F.t1), vdot(s.V, F.t2)); break;
     case 3: s.V.z += eps; s.V_center.z += eps; s.Vn = mk<double>(vdot(s.V, F.n), vdot(s.V, F.t1), vdot(s.V, F.t2)); break;
-    default: s.T += eps; s.T_center += eps; s.rho = m_rho(P, s.p, s.T); s.a = m_a(P, s.p, s.T); break;
+    case 4: s.T += eps; s.T_center += eps; s.rho = m_rho(P, s.p, s.T); s.a = m_a(P, s.p, s.T); break;
+    default: s.nut += eps; s.nut_center += eps; break;
     }
     s.H = s.a * s.a / (P.gamma - 1.) + 0.5 * vdot(s.V, s.V);
 }
 
 // face_state_adjust (:460-500); case 1 falls through into case 2
+template <int NV>
 __device__ __forceinline__ void face_state_adjust(const Params& P, const CState<double>& L, const CState<double>& R, const FaceGeo& F,
-                                         FState<double>& f, int var) {
+                                                  FState<double>& f, int var) {
     Vec<double> n = vnorm(F.l2r);
     double m = vmag(F.l2r);
     if (var == 1) {
@@ -53,140 +60,174 @@ __device__ __forceinline__ void face_state_adjust(const Params& P, const CState<
         f.mu = m_visc(P, f.T);
         f.lambda = m_cond(P, f.T);
     }
+    if constexpr (NV == 6) {   // SA face quantities follow the perturbed states
+        if (var == 5) f.gn = correct(f.gn, n, m, R.nut_center - L.nut_center);
+        sa_face(L, R, f);
+    }
 }
 
-__device__ __forceinline__ double fd_eps(double upd) {   // (:48, :217-219)
-    const double small_number = 10. * 1.4901161193847656e-08;   // 10*sqrt(DBL_EPSILON)
+__device__ __forceinline__ double fd_eps(double upd, double small_number) {   // (:48, :217-219)
     const double factor = 0.001;
     if (upd > small_number) return smax(small_number, factor * upd);
     if (upd < -small_number) return smin(-small_number, factor * upd);
     return small_number;
 }
+__device__ __forceinline__ double fd_floor(const Params& P, int var) {
+    return var == 5 ? P.nut_eps : 10. * 1.4901161193847656e-08;   // 10*sqrt(DBL_EPSILON)
+}
 
-template <int FLUX, int VISC>
+template <int FLUX, int VISC, int NV>
 __global__ void __launch_bounds__(128) fdjac_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ G,
                                                     const double* __restrict__ PHI, int has_phi, double phi_const,
                                                     const double* __restrict__ UPD, double* __restrict__ JL,
                                                     double* __restrict__ JR, int* __restrict__ status) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
-    FaceCtx<double> X;
-    CellLoader ld_{W, G, PHI, has_phi != 0, phi_const, (size_t)g.ld};
-    face_base<FLUX, VISC>(g, P, ld_, f, X, status);
+    FaceCtx<double, NV> X;
+    CellLoader<NV> ld_{W, G, PHI, has_phi != 0, phi_const, (size_t)g.ld};
+    face_base<FLUX, VISC, NV>(g, P, ld_, f, X, status);
     const FaceGeo& F = X.F;
     const size_t ld = g.ld;
     const size_t Fs = g.F;
     const bool interior = F.bc < 0;
     const double diss = FLUX == FCFD_FLUX_SW ? g.fdiss[f] : 0.0, cwd = FLUX == FCFD_FLUX_SW ? g.fwd[f] : 0.0;
-    for (int var = 0; var < 5; ++var) {
-        double Cp[5], Dp[5];
+    for (int var = 0; var < NV; ++var) {
+        double Cp[NV], Dp[NV];
         CState<double> Lp = X.L, Rp = X.R;
-        double eps = fd_eps(UPD[var * ld + X.p]);
+        const double sn = fd_floor(P, var);
+        double eps = fd_eps(UPD[var * ld + X.p], sn);
         state_perturb(P, Lp, F, var, eps);
         if (!interior) {   // right_state_update(leftPlus, rightPlus) (:224)
             Rp.p_center = W[X.n];
             Rp.V_center = mk<double>(W[ld + X.n], W[2 * ld + X.n], W[3 * ld + X.n]);
             Rp.T_center = W[4 * ld + X.n];
+            if constexpr (NV == 6) Rp.nut_center = W[5 * ld + X.n];
             if (apply_bcs(P, g.bc[F.bc], Lp, Rp, F)) atomicOr(status, 1);
             finish_state(P, Rp, F);
         }
-        convective_flux<FLUX>(P, Lp, Rp, F, diss, cwd, Cp);
+        convective_flux<FLUX, NV>(P, Lp, Rp, F, diss, cwd, Cp);
         if (VISC) {
-            face_state_adjust(P, Lp, Rp, F, X.fs, var);
-            diffusive_flux(X.fs, F, g.bc, Dp);
+            face_state_adjust<NV>(P, Lp, Rp, F, X.fs, var);
+            diffusive_flux<NV>(P, X.fs, F, g.bc, Dp);
         } else {
 #pragma unroll
-            for (int j = 0; j < 5; ++j) Dp[j] = 0.0;
+            for (int j = 0; j < NV; ++j) Dp[j] = 0.0;
         }
 #pragma unroll
-        for (int j = 0; j < 5; ++j) JL[(5 * var + j) * Fs + f] = (Dp[j] - X.D[j] - Cp[j] + X.C[j]) / eps;
+        for (int j = 0; j < NV; ++j) JL[(NV * var + j) * Fs + f] = (Dp[j] - X.D[j] - Cp[j] + X.C[j]) / eps;
         if (interior) {
             Rp = X.R;
-            double eps2 = fd_eps(UPD[var * ld + X.n]);
+            double eps2 = fd_eps(UPD[var * ld + X.n], sn);
             state_perturb(P, Rp, F, var, eps2);
-            convective_flux<FLUX>(P, X.L, Rp, F, diss, cwd, Cp);
+            convective_flux<FLUX, NV>(P, X.L, Rp, F, diss, cwd, Cp);
             if (VISC) {
-                face_state_adjust(P, X.L, Rp, F, X.fs, var);
-                diffusive_flux(X.fs, F, g.bc, Dp);
+                face_state_adjust<NV>(P, X.L, Rp, F, X.fs, var);
+                diffusive_flux<NV>(P, X.fs, F, g.bc, Dp);
             }
 #pragma unroll
-            for (int j = 0; j < 5; ++j) JR[(5 * var + j) * Fs + f] = (Dp[j] - X.D[j] - Cp[j] + X.C[j]) / eps2;
+            for (int j = 0; j < NV; ++j) JR[(NV * var + j) * Fs + f] = (Dp[j] - X.D[j] - Cp[j] + X.C[j]) / eps2;
         } else {
 #pragma unroll
-            for (int j = 0; j < 5; ++j) JR[(5 * var + j) * Fs + f] = 0.0;
+            for (int j = 0; j < NV; ++j) JR[(NV * var + j) * Fs + f] = 0.0;
         }
-        if (VISC) face_state_adjust(P, X.L, X.R, F, X.fs, var);   // (:265)
+        if (VISC) face_state_adjust<NV>(P, X.L, X.R, F, X.fs, var);   // (:265)
     }
 }
 
-// cons2prim (ns_time_terms.cc:26-56)
-__device__ __forceinline__ void cons2prim(const Params& P, double p, double u, double v, double w, double T, double Pm[5][5]) {
+// cons2prim (ns_time_terms.cc:26-56); with the SA model the row of rho nu~
+template <int NV>
+__device__ __forceinline__ void cons2prim(const Params& P, const double* q, double Pm[NV][NV]) {
+    const double p = q[0], u = q[1], v = q[2], w = q[3], T = q[4];
     double rho = m_rho(P, p, T);
     double pressure = p + P.Pref, temperature = T + P.Tref;
     double drho_dT = -1. * rho / temperature, drho_dp = rho / pressure;
     double c_p = P.Cp;
     double H = c_p * temperature + 0.5 * (u * u + v * v + w * w);
 #pragma unroll
-    for (int i = 0; i < 5; ++i)
+    for (int i = 0; i < NV; ++i)
 #pragma unroll
-        for (int j = 0; j < 5; ++j) Pm[i][j] = 0.0;
+        for (int j = 0; j < NV; ++j) Pm[i][j] = 0.0;
     Pm[0][0] = drho_dp; Pm[0][4] = drho_dT;
     Pm[1][0] = drho_dp * u; Pm[1][1] = rho; Pm[1][4] = drho_dT * u;
     Pm[2][0] = drho_dp * v; Pm[2][2] = rho; Pm[2][4] = drho_dT * v;
     Pm[3][0] = drho_dp * w; Pm[3][3] = rho; Pm[3][4] = drho_dT * w;
     Pm[4][0] = drho_dp * H - 1.; Pm[4][1] = rho * u; Pm[4][2] = rho * v; Pm[4][3] = rho * w;
     Pm[4][4] = drho_dT * H + rho * c_p;
+    if constexpr (NV == 6) { Pm[5][0] = drho_dp * q[5]; Pm[5][4] = drho_dT * q[5]; Pm[5][5] = rho; }
 }
 
-__global__ void __launch_bounds__(256) diag_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ JL,
+template <int NV>
+__global__ void __launch_bounds__(256) diag_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ G,
+                                                   const double* __restrict__ UPD, const double* __restrict__ JL,
                                                    const double* __restrict__ JR, const double* __restrict__ dt,
                                                    double* __restrict__ D) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld, Fs = g.F;
-    double blk[25];
+    constexpr int B2 = NV * NV;
+    double blk[B2];
 #pragma unroll
-    for (int k = 0; k < 25; ++k) blk[k] = 0.0;
+    for (int k = 0; k < B2; ++k) blk[k] = 0.0;
     for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
         int fs = g.hf_face[h];
         if (fs >= 0) {
 #pragma unroll
-            for (int var = 0; var < 5; ++var)
+            for (int var = 0; var < NV; ++var)
 #pragma unroll
-                for (int j = 0; j < 5; ++j) blk[5 * j + var] += -1. * JL[(5 * var + j) * Fs + fs];
+                for (int j = 0; j < NV; ++j) blk[NV * j + var] += -1. * JL[(NV * var + j) * Fs + fs];
         } else {
             int f = ~fs;
 #pragma unroll
-            for (int var = 0; var < 5; ++var)
+            for (int var = 0; var < NV; ++var)
 #pragma unroll
-                for (int j = 0; j < 5; ++j) blk[5 * j + var] += JR[(5 * var + j) * Fs + f];
+                for (int j = 0; j < NV; ++j) blk[NV * j + var] += JR[(NV * var + j) * Fs + f];
+        }
+    }
+    double q[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) q[v] = W[v * ld + c];
+    if constexpr (NV == 6) {   // SA source block, gradients frozen
+        auto gvec = [&](int v) { return mk<double>(G[(3 * v) * ld + c], G[(3 * v + 1) * ld + c], G[(3 * v + 2) * ld + c]); };
+        const Vec<double> gu = gvec(1), gv = gvec(2), gw = gvec(3), gn = gvec(5);
+        const double vol = g.vol[c], d = g.wd[c];
+        const double S0 = vol * sa_source<double>(m_rho(P, q[0], q[4]), m_visc(P, q[4]), q[5], gu, gv, gw, gn, d);
+#pragma unroll
+        for (int var = 0; var < NV; ++var) {
+            double eps = fd_eps(UPD[var * ld + c], fd_floor(P, var));
+            double qp[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qp[v] = q[v];
+            qp[var] += eps;
+            double Sp = vol * sa_source<double>(m_rho(P, qp[0], qp[4]), m_visc(P, qp[4]), qp[5], gu, gv, gw, gn, d);
+            blk[NV * 5 + var] += -1. * ((Sp - S0) / eps);
         }
     }
     if (dt) {
-        double Pm[5][5];
-        cons2prim(P, W[c], W[ld + c], W[2 * ld + c], W[3 * ld + c], W[4 * ld + c], Pm);
+        double Pm[NV][NV];
+        cons2prim<NV>(P, q, Pm);
         double vol = g.vol[c], dtc = dt[c];
 #pragma unroll
-        for (int i = 0; i < 5; ++i)
+        for (int i = 0; i < NV; ++i)
 #pragma unroll
-            for (int j = 0; j < 5; ++j) blk[5 * i + j] += Pm[i][j] * vol / dtc;
+            for (int j = 0; j < NV; ++j) blk[NV * i + j] += Pm[i][j] * vol / dtc;
     }
 #pragma unroll
-    for (int k = 0; k < 25; ++k) D[k * ld + c] = blk[k];
+    for (int k = 0; k < B2; ++k) D[k * ld + c] = blk[k];
 }
 
-template <int FLUX, int VISC>
+template <int FLUX, int VISC, int NV>
 static void launch_fdjac_t(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, int has_phi,
                            double phi_const, const double* UPD, double* JL, double* JR, int* status, hipStream_t s) {
-    hipLaunchKernelGGL((fdjac_kernel<FLUX, VISC>), dim3((g.F + 127) / 128), dim3(128), 0, s, g, P, W, G, PHI, has_phi,
+    hipLaunchKernelGGL((fdjac_kernel<FLUX, VISC, NV>), dim3((g.F + 127) / 128), dim3(128), 0, s, g, P, W, G, PHI, has_phi,
                        phi_const, UPD, JL, JR, status);
 }
 
 template <int FLUX>
 static void launch_fdjac_v(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, int has_phi,
                            double phi_const, const double* UPD, double* JL, double* JR, int* status, hipStream_t s) {
-    if (P.viscous) launch_fdjac_t<FLUX, 1>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s);
-    else launch_fdjac_t<FLUX, 0>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s);
+    if (g.nv == 6) launch_fdjac_t<FLUX, 1, 6>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s);
+    else if (P.viscous) launch_fdjac_t<FLUX, 1, 5>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s);
+    else launch_fdjac_t<FLUX, 0, 5>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s);
 }
 
 void launch_fd_jacobian(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI,
@@ -201,7 +242,9 @@ void launch_fd_jacobian(const DevGrid& g, const Params& P, const double* W, cons
     case FCFD_FLUX_SW: launch_fdjac_v<FCFD_FLUX_SW>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s); break;
     default: launch_fdjac_v<FCFD_FLUX_AUSM_PLUS_UP>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s); break;
     }
-    hipLaunchKernelGGL(diag_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, P, W, JL, JR, dt, D);
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL(diag_kernel<6>, grid, blk, 0, s, g, P, W, G, UPD, JL, JR, dt, D);
+    else hipLaunchKernelGGL(diag_kernel<5>, grid, blk, 0, s, g, P, W, G, UPD, JL, JR, dt, D);
 }
 
 }  // namespace fcfd

@@ -7,7 +7,7 @@
 namespace fcfd {
 
 template <typename T>
-struct LinOp {   // y = Op(x) on [5][ld] device vectors, stream-ordered
+struct LinOp {   // y = Op(x) on [nv][ld] device vectors, stream-ordered
     std::function<void(const T*, T*, hipStream_t)> apply;
 };
 
@@ -21,7 +21,7 @@ struct GmresStats {
 };
 template <typename T>
 struct GmresWork {
-    T *V = nullptr, *z = nullptr, *r = nullptr;   // (restart+1) x 5 ld, 5 ld, 5 ld
+    T *V = nullptr, *z = nullptr, *r = nullptr;   // (restart+1) x nv ld, nv ld, nv ld
     double *part = nullptr, *out_dev = nullptr, *coef_dev = nullptr;
 };
 
@@ -35,8 +35,9 @@ hipError_t dev_mdot(const DevGrid& g, int k, const T* V, size_t stride, const T*
 template <typename T> void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s);
 template <typename T> void dev_bj_invert(const DevGrid& g, const double* D, T* Dinv, int* status, hipStream_t s);
 template <typename T> void dev_bj_apply(const DevGrid& g, const T* Dinv, const T* x, T* y, hipStream_t s);
+// out_host: sum dp^2/dt^2, sum |dV|^2/dt^2, sum dT^2/dt^2, NaN/Inf count, sum dnu~^2/dt^2
 hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
-                      double out_host[4], hipStream_t s);
+                      double out_host[5], hipStream_t s);
 template <typename T>
 int gmres(const DevGrid& g, const LinOp<T>& A, const LinOp<T>& M, const T* b, T* x, const GmresOpts& o, GmresWork<T>& w,
           GmresStats& st, hipStream_t s);

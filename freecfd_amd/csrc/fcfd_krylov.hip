@@ -1,11 +1,13 @@
 // Linear algebra of the implicit step on the GPU: operator matvec with the
-// per-face Jacobian blocks, batched 5x5 block-Jacobi, deterministic
-// reductions and a restarted right-preconditioned GMRES, in FP64 or FP32.
+// per-face Jacobian blocks, batched NV x NV block-Jacobi (5x5, or 6x6 with
+// the SA model), deterministic reductions and a restarted right-preconditioned
+// GMRES, in FP64 or FP32.
 //
 // Replaces the reference's PETSc MPIAIJ matrix + FGMRES(300) KSP
 // (ns_petsc_functions.cc:25-110).  The operator is the reference's implicit
-// operator A = P V/dt - dR/dW stored as: diagonal blocks D ([25][ld]) and per
-// face JL, JR ([25][F]); row P, col N block = -JR^T, row N, col P = +JL^T.
+// operator A = P V/dt - dR/dW stored as: diagonal blocks D ([NV*NV][ld]) and
+// per face JL, JR ([NV*NV][F]); row P, col N block = -JR^T, row N, col P =
+// +JL^T.  Vectors are [NV][ld].
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -19,8 +21,8 @@ constexpr int kRedBlocks = 512;   // fixed -> deterministic reductions
 constexpr int kRedThreads = 256;
 
 // ---------------------------------------------------------------------------
-// element-wise kernels on [5][ld] vectors (only the first N entries of each
-// variable row are meaningful; the loops cover 5*ld for simplicity)
+// element-wise kernels on [NV][ld] vectors (only the first N entries of each
+// variable row are meaningful; the loops cover NV*ld for simplicity)
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void axpy_kernel(size_t n, T a, const T* __restrict__ x, T* __restrict__ y) {
@@ -63,7 +65,7 @@ __global__ void lincomb_kernel(size_t n, int k, const T* __restrict__ V, size_t 
 // deterministic multi-dot: out[q] = <V_q, w> for q < k over interior cells,
 // accumulated in FP64; partials [k][kRedBlocks] then a fixed-order tree
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int NV>
 __global__ void __launch_bounds__(kRedThreads) mdot_partial_kernel(int N, size_t ld, int k, const T* __restrict__ V, size_t stride,
                                                                    const T* __restrict__ w, double* __restrict__ part) {
     __shared__ double sh[kRedThreads];
@@ -72,7 +74,7 @@ __global__ void __launch_bounds__(kRedThreads) mdot_partial_kernel(int N, size_t
     double acc = 0.0;
     for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < N; i += kRedBlocks * kRedThreads) {
 #pragma unroll
-        for (int r = 0; r < 5; ++r) acc += (double)v[r * ld + i] * (double)w[r * ld + i];
+        for (int r = 0; r < NV; ++r) acc += (double)v[r * ld + i] * (double)w[r * ld + i];
     }
     sh[threadIdx.x] = acc;
     __syncthreads();
@@ -98,53 +100,55 @@ __global__ void __launch_bounds__(kRedThreads) reduce_final_kernel(const double*
 }
 
 // Newton update W += dW over interior cells, with the reference's update norms
-// (ns.cc:349-381): sums of dp^2/dt^2, |dV|^2/dt^2, dT^2/dt^2, plus a NaN/Inf
-// count and ||dW||^2 -- partial sums per block, fixed-order tree.
+// (ns.cc:349-381): sums of dp^2/dt^2, |dV|^2/dt^2, dT^2/dt^2, a NaN/Inf count
+// and (SA model) dnu~^2/dt^2 -- partial sums per block, fixed-order tree.
+template <int NV>
 __global__ void __launch_bounds__(kRedThreads) update_kernel(int N, size_t ld, double* __restrict__ W, const double* __restrict__ dW,
                                                              const double* __restrict__ dt, double* __restrict__ part) {
-    __shared__ double sh[4][kRedThreads];
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    __shared__ double sh[5][kRedThreads];
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int c = blockIdx.x * kRedThreads + threadIdx.x; c < N; c += kRedBlocks * kRedThreads) {
-        double d[5];
+        double d[NV];
         bool bad = false;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) { d[v] = dW[v * ld + c]; bad |= !isfinite(d[v]); }
+        for (int v = 0; v < NV; ++v) { d[v] = dW[v * ld + c]; bad |= !isfinite(d[v]); }
         if (bad) { acc[3] += 1.0; continue; }
 #pragma unroll
-        for (int v = 0; v < 5; ++v) W[v * ld + c] += d[v];
+        for (int v = 0; v < NV; ++v) W[v * ld + c] += d[v];
         double dt2 = dt[c] * dt[c];
         acc[0] += d[0] * d[0] / dt2;
         acc[1] += d[1] * d[1] / dt2 + d[2] * d[2] / dt2 + d[3] * d[3] / dt2;
         acc[2] += d[4] * d[4] / dt2;
+        if constexpr (NV == 6) acc[4] += d[5] * d[5] / dt2;
     }
-    for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = acc[q];
+    for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] = acc[q];
     __syncthreads();
     for (int s = kRedThreads / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s)
-            for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+            for (int q = 0; q < 5; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 4; ++q) part[q * kRedBlocks + blockIdx.x] = sh[q][0];
+        for (int q = 0; q < 5; ++q) part[q * kRedBlocks + blockIdx.x] = sh[q][0];
 }
 
 // ---------------------------------------------------------------------------
 // operator: y = D x + sum_faces (-JR^T x_N | +JL^T x_P)
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int NV>
 __global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __restrict__ D, const T* __restrict__ JL,
                                                          const T* __restrict__ JR, const T* __restrict__ x, T* __restrict__ y) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld, Fs = g.F;
-    T xc[5], acc[5];
+    T xc[NV], acc[NV];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) xc[i] = x[i * ld + c];
+    for (int i = 0; i < NV; ++i) xc[i] = x[i * ld + c];
 #pragma unroll
-    for (int j = 0; j < 5; ++j) {
+    for (int j = 0; j < NV; ++j) {
         T s = 0;
 #pragma unroll
-        for (int i = 0; i < 5; ++i) s += D[(5 * j + i) * ld + c] * xc[i];
+        for (int i = 0; i < NV; ++i) s += D[(NV * j + i) * ld + c] * xc[i];
         acc[j] = s;
     }
     for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
@@ -152,74 +156,79 @@ __global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __r
         int f = fs >= 0 ? fs : ~fs;
         int o = fs >= 0 ? g.fnei[f] : g.fpar[f];
         if (o >= g.N) continue;   // boundary ghost: no column (ghost coupling is in JL)
-        T xo[5];
+        T xo[NV];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) xo[i] = x[i * ld + o];
+        for (int i = 0; i < NV; ++i) xo[i] = x[i * ld + o];
         const T* J = fs >= 0 ? JR : JL;
         const T sg = fs >= 0 ? T(-1) : T(1);
 #pragma unroll
-        for (int i = 0; i < 5; ++i)
+        for (int i = 0; i < NV; ++i)
 #pragma unroll
-            for (int j = 0; j < 5; ++j) acc[j] += sg * J[(5 * i + j) * Fs + f] * xo[i];
+            for (int j = 0; j < NV; ++j) acc[j] += sg * J[(NV * i + j) * Fs + f] * xo[i];
     }
 #pragma unroll
-    for (int j = 0; j < 5; ++j) y[j * ld + c] = acc[j];
+    for (int j = 0; j < NV; ++j) y[j * ld + c] = acc[j];
 }
 
-// batched 5x5 inverse by Gauss-Jordan with partial pivoting (block-Jacobi)
-template <typename T>
+// batched NV x NV inverse by Gauss-Jordan with partial pivoting (block-Jacobi)
+template <typename T, int NV>
 __global__ void __launch_bounds__(256) bj_invert_kernel(int N, size_t ld, const double* __restrict__ D, T* __restrict__ Dinv,
                                                         int* __restrict__ status) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    T a[5][10];
+    T a[NV][2 * NV];
 #pragma unroll
-    for (int i = 0; i < 5; ++i)
+    for (int i = 0; i < NV; ++i)
 #pragma unroll
-        for (int j = 0; j < 5; ++j) { a[i][j] = (T)D[(5 * i + j) * ld + c]; a[i][5 + j] = (i == j) ? T(1) : T(0); }
+        for (int j = 0; j < NV; ++j) { a[i][j] = (T)D[(NV * i + j) * ld + c]; a[i][NV + j] = (i == j) ? T(1) : T(0); }
     bool singular = false;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < NV; ++k) {
         int piv = k;
         T best = fabs(a[k][k]);
 #pragma unroll
-        for (int i = k + 1; i < 5; ++i) if (fabs(a[i][k]) > best) { best = fabs(a[i][k]); piv = i; }
+        for (int i = k + 1; i < NV; ++i) if (fabs(a[i][k]) > best) { best = fabs(a[i][k]); piv = i; }
         if (best == T(0)) { singular = true; break; }
-        if (piv != k) {
+        // row swap with compile-time indices only (a runtime row index would
+        // put the tableau in scratch memory)
 #pragma unroll
-            for (int j = 0; j < 10; ++j) { T t = a[k][j]; a[k][j] = a[piv][j]; a[piv][j] = t; }
+        for (int i = k + 1; i < NV; ++i) {
+            if (i == piv) {
+#pragma unroll
+                for (int j = 0; j < 2 * NV; ++j) { T t = a[k][j]; a[k][j] = a[i][j]; a[i][j] = t; }
+            }
         }
         T inv = T(1) / a[k][k];
 #pragma unroll
-        for (int j = 0; j < 10; ++j) a[k][j] *= inv;
+        for (int j = 0; j < 2 * NV; ++j) a[k][j] *= inv;
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
+        for (int i = 0; i < NV; ++i) {
             if (i == k) continue;
             T m = a[i][k];
 #pragma unroll
-            for (int j = 0; j < 10; ++j) a[i][j] -= m * a[k][j];
+            for (int j = 0; j < 2 * NV; ++j) a[i][j] -= m * a[k][j];
         }
     }
     if (singular) atomicOr(status, 2);
 #pragma unroll
-    for (int i = 0; i < 5; ++i)
+    for (int i = 0; i < NV; ++i)
 #pragma unroll
-        for (int j = 0; j < 5; ++j) Dinv[(5 * i + j) * ld + c] = singular ? ((i == j) ? T(1) : T(0)) : a[i][5 + j];
+        for (int j = 0; j < NV; ++j) Dinv[(NV * i + j) * ld + c] = singular ? ((i == j) ? T(1) : T(0)) : a[i][NV + j];
 }
 
-template <typename T>
+template <typename T, int NV>
 __global__ void __launch_bounds__(256) bj_apply_kernel(int N, size_t ld, const T* __restrict__ Dinv, const T* __restrict__ x,
                                                        T* __restrict__ y) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    T xc[5];
+    T xc[NV];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) xc[i] = x[i * ld + c];
+    for (int i = 0; i < NV; ++i) xc[i] = x[i * ld + c];
 #pragma unroll
-    for (int j = 0; j < 5; ++j) {
+    for (int j = 0; j < NV; ++j) {
         T s = 0;
 #pragma unroll
-        for (int i = 0; i < 5; ++i) s += Dinv[(5 * j + i) * ld + c] * xc[i];
+        for (int i = 0; i < NV; ++i) s += Dinv[(NV * j + i) * ld + c] * xc[i];
         y[j * ld + c] = s;
     }
 }
@@ -255,7 +264,10 @@ template void dev_xmy<float>(size_t, const float*, float*, hipStream_t);
 template <typename T>
 hipError_t dev_mdot(const DevGrid& g, int k, const T* V, size_t stride, const T* w, double* part, double* out_dev,
                     double* out_host, hipStream_t s) {
-    hipLaunchKernelGGL(mdot_partial_kernel<T>, dim3(kRedBlocks, k), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, k, V, stride, w, part);
+    if (g.nv == 6)
+        hipLaunchKernelGGL((mdot_partial_kernel<T, 6>), dim3(kRedBlocks, k), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, k, V, stride, w, part);
+    else
+        hipLaunchKernelGGL((mdot_partial_kernel<T, 5>), dim3(kRedBlocks, k), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, k, V, stride, w, part);
     hipLaunchKernelGGL(reduce_final_kernel, dim3(k), dim3(kRedThreads), 0, s, part, out_dev);
     hipError_t e = hipMemcpyAsync(out_host, out_dev, k * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
@@ -266,12 +278,12 @@ template hipError_t dev_mdot<float>(const DevGrid&, int, const float*, size_t, c
 
 template <typename T>
 void dev_mgs_update(const DevGrid& g, int k, const T* V, size_t stride, const double* h_dev, T* y, hipStream_t s) {
-    size_t n = (size_t)5 * g.ld;
+    size_t n = (size_t)g.nv * g.ld;
     hipLaunchKernelGGL(mgs_update_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, h_dev, y);
 }
 template <typename T>
 void dev_lincomb(const DevGrid& g, int k, const T* V, size_t stride, const double* c_dev, T* x, hipStream_t s) {
-    size_t n = (size_t)5 * g.ld;
+    size_t n = (size_t)g.nv * g.ld;
     hipLaunchKernelGGL(lincomb_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, c_dev, x);
 }
 template void dev_mgs_update<double>(const DevGrid&, int, const double*, size_t, const double*, double*, hipStream_t);
@@ -281,18 +293,21 @@ template void dev_lincomb<float>(const DevGrid&, int, const float*, size_t, cons
 
 template <typename T>
 void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s) {
-    hipLaunchKernelGGL(bsr_matvec_kernel<T>, dim3((g.N + 255) / 256), dim3(256), 0, s, g, D, JL, JR, x, y);
+    if (g.nv == 6) hipLaunchKernelGGL((bsr_matvec_kernel<T, 6>), dim3((g.N + 255) / 256), dim3(256), 0, s, g, D, JL, JR, x, y);
+    else hipLaunchKernelGGL((bsr_matvec_kernel<T, 5>), dim3((g.N + 255) / 256), dim3(256), 0, s, g, D, JL, JR, x, y);
 }
 template void dev_bsr_matvec<double>(const DevGrid&, const double*, const double*, const double*, const double*, double*, hipStream_t);
 template void dev_bsr_matvec<float>(const DevGrid&, const float*, const float*, const float*, const float*, float*, hipStream_t);
 
 template <typename T>
 void dev_bj_invert(const DevGrid& g, const double* D, T* Dinv, int* status, hipStream_t s) {
-    hipLaunchKernelGGL(bj_invert_kernel<T>, dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
+    if (g.nv == 6) hipLaunchKernelGGL((bj_invert_kernel<T, 6>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
+    else hipLaunchKernelGGL((bj_invert_kernel<T, 5>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
 }
 template <typename T>
 void dev_bj_apply(const DevGrid& g, const T* Dinv, const T* x, T* y, hipStream_t s) {
-    hipLaunchKernelGGL(bj_apply_kernel<T>, dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
+    if (g.nv == 6) hipLaunchKernelGGL((bj_apply_kernel<T, 6>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
+    else hipLaunchKernelGGL((bj_apply_kernel<T, 5>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
 }
 template void dev_bj_invert<double>(const DevGrid&, const double*, double*, int*, hipStream_t);
 template void dev_bj_invert<float>(const DevGrid&, const double*, float*, int*, hipStream_t);
@@ -300,10 +315,11 @@ template void dev_bj_apply<double>(const DevGrid&, const double*, const double*,
 template void dev_bj_apply<float>(const DevGrid&, const float*, const float*, float*, hipStream_t);
 
 hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
-                      double out_host[4], hipStream_t s) {
-    hipLaunchKernelGGL(update_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, W, dW, dt, part);
-    hipLaunchKernelGGL(reduce_final_kernel, dim3(4), dim3(kRedThreads), 0, s, part, out_dev);
-    hipError_t e = hipMemcpyAsync(out_host, out_dev, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
+                      double out_host[5], hipStream_t s) {
+    if (g.nv == 6) hipLaunchKernelGGL(update_kernel<6>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, W, dW, dt, part);
+    else hipLaunchKernelGGL(update_kernel<5>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, W, dW, dt, part);
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(5), dim3(kRedThreads), 0, s, part, out_dev);
+    hipError_t e = hipMemcpyAsync(out_host, out_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(s);
 }
@@ -314,7 +330,7 @@ hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const doubl
 template <typename T>
 int gmres(const DevGrid& g, const LinOp<T>& A, const LinOp<T>& M, const T* b, T* x, const GmresOpts& o, GmresWork<T>& w,
           GmresStats& st, hipStream_t s) {
-    const size_t n = (size_t)5 * g.ld;
+    const size_t n = (size_t)g.nv * g.ld;
     const int m = o.restart;
     const size_t stride = n;
     T* V = w.V;       // (m+1) vectors
@@ -326,7 +342,7 @@ int gmres(const DevGrid& g, const LinOp<T>& A, const LinOp<T>& M, const T* b, T*
     const double bnorm = std::sqrt(bnorm2);
     // initial guess: Knoll (x0 = M^{-1} b) or zero
     if (o.knoll) M.apply(b, x, s);
-    else hipMemsetAsync(x, 0, n * sizeof(T), s);
+    else if (hipMemsetAsync(x, 0, n * sizeof(T), s) != hipSuccess) return FCFD_EHIP;
     int it = 0;
     double rnorm = bnorm, r0norm = -1.0;
     st.converged = 0;
@@ -389,7 +405,7 @@ int gmres(const DevGrid& g, const LinOp<T>& A, const LinOp<T>& M, const T* b, T*
         if (hipMemcpyAsync(w.coef_dev, y.data(), j * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
             return FCFD_EHIP;
-        hipMemsetAsync(r, 0, n * sizeof(T), s);
+        if (hipMemsetAsync(r, 0, n * sizeof(T), s) != hipSuccess) return FCFD_EHIP;
         dev_lincomb(g, j, V, stride, w.coef_dev, r, s);
         M.apply(r, z, s);
         dev_axpy(n, (T)1, z, x, s);

@@ -13,11 +13,11 @@
 
 namespace fcfd {
 
-template <int LIM, typename S, typename Src>
-__device__ __forceinline__ void limiter_phi(const DevGrid& g, const Params& P, const Src& src, int c, const S q0[5],
-                                            const S gr[15], const double* __restrict__ dref, S phi[5]) {
+template <int LIM, int NV, typename S, typename Src>
+__device__ __forceinline__ void limiter_phi(const DevGrid& g, const Params& P, const Src& src, int c, const S q0[NV],
+                                            const S gr[3 * NV], const double* __restrict__ dref, S phi[NV]) {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) phi[v] = S(1.);
+    for (int v = 0; v < NV; ++v) phi[v] = S(1.);
     const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
     Vec<double> xc = centroid(g, c);
     if (LIM == FCFD_LIMITER_MINMOD) {
@@ -28,7 +28,7 @@ __device__ __forceinline__ void limiter_phi(const DevGrid& g, const Params& P, c
             int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
             Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 S dU = src.nb(h, nb, v) - q0[v];
                 S dUg = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
                 S mm = (val(dU) * val(dUg) < 0.) ? S(0.) : (val(dU) > 0.) ? smin(dU, dUg) : smax(dU, dUg);
@@ -37,15 +37,15 @@ __device__ __forceinline__ void limiter_phi(const DevGrid& g, const Params& P, c
             }
         }
     } else if (LIM == FCFD_LIMITER_BJ || LIM == FCFD_LIMITER_VK) {
-        S umax[5], umin[5];
+        S umax[NV], umin[NV];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) umax[v] = umin[v] = q0[v];
+        for (int v = 0; v < NV; ++v) umax[v] = umin[v] = q0[v];
         for (int h = h0; h < h1; ++h) {
             int fs = g.hf_face[h];
             int f = fs >= 0 ? fs : ~fs;
             int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 S q = src.nb(h, nb, v);
                 umax[v] = smax(umax[v], q);
                 umin[v] = smin(umin[v], q);
@@ -56,7 +56,7 @@ __device__ __forceinline__ void limiter_phi(const DevGrid& g, const Params& P, c
             int f = fs >= 0 ? fs : ~fs;
             Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 S dM = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
                 S dP = q0[v];
                 if (LIM == FCFD_LIMITER_BJ) {

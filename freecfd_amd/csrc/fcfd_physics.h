@@ -53,14 +53,77 @@ template <typename S> __device__ __forceinline__ S m_cond(const Params& P, const
     return S(P.lambda);
 }
 
+// ---- Spalart-Allmaras model (no reference counterpart) ----------------------
+// SA without ft2, with the S~ modification and the negative-nu~ branch of
+// Allmaras, Johnson & Spalart (ICCFD7-1902, 2012); compressible form without
+// the grad(rho).grad(nu~) term:
+//   d(rho nu~)/dt + div(rho V nu~) = rho (P - D)
+//       + (1/sigma) [div((mu + rho nu~ fn) grad nu~) + cb2 rho |grad nu~|^2]
+constexpr double sa_cb1 = 0.1355, sa_sigma = 2. / 3., sa_cb2 = 0.622, sa_kappa = 0.41, sa_cw2 = 0.3, sa_cw3 = 2.,
+                 sa_cv1 = 7.1, sa_ct3 = 1.2, sa_cv2 = 0.7, sa_cv3 = 0.9, sa_cn1 = 16.;
+constexpr double sa_cw1 = sa_cb1 / (sa_kappa * sa_kappa) + (1. + sa_cb2) / sa_sigma;
+
+template <typename S> __device__ __forceinline__ S sa_fv1(const S& chi) {
+    S c3 = chi * chi * chi;
+    return c3 / (c3 + sa_cv1 * sa_cv1 * sa_cv1);
+}
+// eddy viscosity rho nu~ fv1 (nu~ > 0), else 0
+template <typename S> __device__ __forceinline__ S sa_mut(const S& rho, const S& nut, const S& mu) {
+    if (!(val(nut) > 0.)) return S(0.);
+    S chi = rho * nut / mu;
+    return rho * nut * sa_fv1(chi);
+}
+// diffusion factor fn (1 for nu~ >= 0)
+template <typename S> __device__ __forceinline__ S sa_fn(const S& rho, const S& nut, const S& mu) {
+    if (val(nut) >= 0.) return S(1.);
+    S chi = rho * nut / mu;
+    S c3 = chi * chi * chi;
+    return (sa_cn1 + c3) / (sa_cn1 - c3);
+}
+// source per unit volume: rho (P - D) + (cb2/sigma) rho |grad nu~|^2, from the
+// cell state, its velocity gradient (vorticity magnitude) and wall distance d
+template <typename S>
+__device__ __forceinline__ S sa_source(const S& rho, const S& mu, const S& nut, const Vec<S>& gu, const Vec<S>& gv, const Vec<S>& gw,
+                                       const Vec<S>& gn, double d) {
+    S wx = gw.y - gv.z, wy = gu.z - gw.x, wz = gv.x - gu.y;
+    S Om = sq_root(wx * wx + wy * wy + wz * wz);
+    S Pr, De;
+    if (val(nut) >= 0.) {
+        S chi = rho * nut / mu;
+        S fv1 = sa_fv1(chi);
+        S fv2 = 1. - chi / (1. + chi * fv1);
+        double k2d2 = sa_kappa * sa_kappa * d * d;
+        S Sbar = nut * fv2 / k2d2;
+        S St;
+        if (val(Sbar) >= val(-sa_cv2 * Om)) St = Om + Sbar;
+        else St = Om + Om * (sa_cv2 * sa_cv2 * Om + sa_cv3 * Sbar) / ((sa_cv3 - 2. * sa_cv2) * Om - Sbar);
+        S r = (val(St) > 0.) ? smin(nut / (St * k2d2), S(10.)) : S(10.);
+        S r2 = r * r;
+        S g = r + sa_cw2 * (r2 * r2 * r2 - r);
+        S g2 = g * g;
+        constexpr double c2 = sa_cw3 * sa_cw3;
+        constexpr double c6 = c2 * c2 * c2;
+        S fw = g * powr((1. + c6) / (g2 * g2 * g2 + c6), 1. / 6.);
+        Pr = sa_cb1 * St * nut;
+        De = sa_cw1 * fw * (nut / d) * (nut / d);
+    } else {
+        Pr = sa_cb1 * (1. - sa_ct3) * Om * nut;
+        De = -1. * sa_cw1 * (nut / d) * (nut / d);
+    }
+    return rho * (Pr - De) + sa_cb2 / sa_sigma * rho * vdot(gn, gn);
+}
+
 // ---- cell/face state caches (ns_state_cache.h:26-59) -----------------------
 template <typename S> struct CState {
     S p, p_center, T, T_center, rho, a, H;
     Vec<S> V, V_center, Vn;
+    S nut, nut_center;   // SA nu~ (zero without the model)
 };
 template <typename S> struct FState {
     S T, mu, lambda;
     Vec<S> V, gu, gv, gw, gT;
+    Vec<S> gn;           // SA: face gradient of nu~
+    S nut, rho, mu_t;    // SA: face nu~, rho, eddy viscosity (mu_t = 0 without the model)
 };
 
 template <typename S> __device__ __forceinline__ void finish_state(const Params& P, CState<S>& s, const FaceGeo& F) {
@@ -70,9 +133,9 @@ template <typename S> __device__ __forceinline__ void finish_state(const Params&
 }
 
 // MUSCL reconstruction (ns_assemble_linear_system.cc:269-294,305-320):
-// q_f = q_c + phi * (grad q . (x_f - x_c))
-template <typename S> __device__ __forceinline__ void recon(const Params& P, CState<S>& s, const S q[5], const Vec<S> g[5],
-                                                   const S phi[5], const Vec<double>& c2f) {
+// q_f = q_c + phi * (grad q . (x_f - x_c)); nu~ the same way with the SA model
+template <int NV, typename S> __device__ __forceinline__ void recon(const Params& P, CState<S>& s, const S q[NV], const Vec<S> g[NV],
+                                                           const S phi[NV], const Vec<double>& c2f) {
     s.p_center = q[0];
     s.p = s.p_center + phi[0] * vdot(g[0], c2f);
     s.V_center = mk<S>(q[1], q[2], q[3]);
@@ -80,6 +143,12 @@ template <typename S> __device__ __forceinline__ void recon(const Params& P, CSt
     s.T_center = q[4];
     s.T = s.T_center + phi[4] * vdot(g[4], c2f);
     s.rho = m_rho(P, s.p, s.T);
+    if constexpr (NV == 6) {
+        s.nut_center = q[5];
+        s.nut = s.nut_center + phi[5] * vdot(g[5], c2f);
+    } else {
+        s.nut = s.nut_center = S(0.);
+    }
 }
 
 // Boundary conditions (src/navier_stokes/ns_apply_bcs.cc:25-258).  Returns a
@@ -94,6 +163,7 @@ template <typename S> __device__ __forceinline__ int apply_bcs(const Params& P, 
     const Vec<S> n = lift<S>(F.n);
     S p = Rout.p, pc = Rout.p_center, T = Rout.T, Tc = Rout.T_center, rho = Rout.rho, a = Rout.a;
     Vec<S> V = Rout.V, Vc = Rout.V_center;
+    S nt = Rout.nut, ntc = Rout.nut_center;
     if (bc.type == FCFD_BC_INLET) {
         if (bc.kind == FCFD_KIND_VELOCITY) {
             V = mk<S>(S(bc.V[0]), S(bc.V[1]), S(bc.V[2]));
@@ -199,8 +269,15 @@ template <typename S> __device__ __forceinline__ int apply_bcs(const Params& P, 
         Tc = L.T_center;
         Vc = vsub(L.V_center, vsc(2. * vdot(L.V_center, F.n), n));
     }
+    // SA nu~: inlet value; zero on a no-slip wall (the ghost mirrors to -nu~);
+    // copied otherwise (an outlet keeps the ghost's centre value)
+    if (bc.type == FCFD_BC_INLET) { nt = S(bc.nut); ntc = S(bc.nut); }
+    else if (bc.type == FCFD_BC_WALL && bc.kind != FCFD_KIND_SLIP) { nt = -1. * L.nut; ntc = -1. * L.nut_center; }
+    else if (bc.type == FCFD_BC_OUTLET) { nt = L.nut; }
+    else { nt = L.nut; ntc = L.nut_center; }
     Rout.p = p; Rout.p_center = pc; Rout.T = T; Rout.T_center = Tc; Rout.rho = rho; Rout.a = a;
     Rout.V = V; Rout.V_center = Vc;
+    Rout.nut = nt; Rout.nut_center = ntc;
     return flag;
 }
 
@@ -406,8 +483,9 @@ template <typename S> __device__ __forceinline__ void sw_flux(const Params& P, c
 }
 
 // dispatch + rotation back to xyz (ns_convective_face_flux.cc:32-59)
-template <int FLUX, typename S> __device__ __forceinline__ void convective_flux(const Params& P, const CState<S>& L, const CState<S>& R,
-                                                                       const FaceGeo& F, double diss, double cwd, S flux[5]) {
+template <int FLUX, int NV, typename S>
+__device__ __forceinline__ void convective_flux(const Params& P, const CState<S>& L, const CState<S>& R, const FaceGeo& F, double diss,
+                                                double cwd, S flux[NV]) {
     S fn[5];
     if (FLUX == FCFD_FLUX_ROE) roe_flux(L, R, fn, P.gamma);
     else if (FLUX == FCFD_FLUX_VAN_LEER) vanleer_flux(L, R, fn, P.gamma, P.Pref);
@@ -419,6 +497,14 @@ template <int FLUX, typename S> __device__ __forceinline__ void convective_flux(
     flux[2] = (fn[1] * F.n.y + fn[2] * F.t1.y + fn[3] * F.t2.y) * F.area;
     flux[3] = (fn[1] * F.n.z + fn[2] * F.t1.z + fn[3] * F.t2.z) * F.area;
     flux[4] = fn[4] * F.area;
+    if constexpr (NV == 6) flux[5] = flux[0] * ((val(flux[0]) >= 0.) ? L.nut : R.nut);   // SA: nu~ upwind with the mass flux
+}
+
+// SA face quantities: nu~ and rho averages of the two face states, mu_t
+template <typename S> __device__ __forceinline__ void sa_face(const CState<S>& L, const CState<S>& R, FState<S>& fs) {
+    fs.nut = 0.5 * (L.nut + R.nut);
+    fs.rho = 0.5 * (L.rho + R.rho);
+    fs.mu_t = sa_mut(fs.rho, fs.nut, fs.mu);
 }
 
 // deferred correction of one face gradient along l2r (ns_assemble_linear_system.cc:394-404)
@@ -429,8 +515,8 @@ template <typename S> __device__ __forceinline__ Vec<S> correct(const Vec<S>& g,
 
 // face_state_update (ns_assemble_linear_system.cc:353-415), given the face
 // gradients before correction
-template <typename S> __device__ __forceinline__ void face_state(const Params& P, const CState<S>& L, const CState<S>& R,
-                                                        const FaceGeo& F, FState<S>& fs) {
+template <int NV, typename S>
+__device__ __forceinline__ void face_state(const Params& P, const CState<S>& L, const CState<S>& R, const FaceGeo& F, FState<S>& fs) {
     Vec<double> l2rn = vnorm(F.l2r);
     double l2rmag = vmag(F.l2r);
     fs.gu = correct(fs.gu, l2rn, l2rmag, S(R.V_center.x - L.V_center.x));
@@ -441,10 +527,19 @@ template <typename S> __device__ __forceinline__ void face_state(const Params& P
     fs.T = 0.5 * (L.T + R.T);
     fs.mu = m_visc(P, fs.T);
     fs.lambda = m_cond(P, fs.T);
+    if constexpr (NV == 6) {
+        fs.gn = correct(fs.gn, l2rn, l2rmag, S(R.nut_center - L.nut_center));
+        sa_face(L, R, fs);
+    } else {
+        fs.mu_t = S(0.);
+    }
 }
 
-// diffusive_face_flux (ns_diffusive_face_flux.cc:28-64), laminar
-template <typename S> __device__ __forceinline__ void diffusive_flux(const FState<S>& f, const FaceGeo& F, const BcDev* bc, S flux[5]) {
+// diffusive_face_flux (ns_diffusive_face_flux.cc:28-64): mu + turb_visc and
+// lambda + Cp turb_visc / Pr_t (:36-39), turb_visc = mu_t(face) of the SA
+// model (0 without it); plus the SA diffusion of nu~
+template <int NV, typename S>
+__device__ __forceinline__ void diffusive_flux(const Params& P, const FState<S>& f, const FaceGeo& F, const BcDev* bc, S flux[NV]) {
     Vec<double> aV = mk<double>(F.n.x * F.area, F.n.y * F.area, F.n.z * F.area);
     Vec<S> tx, ty, tz;
     tx.x = 2. / 3. * (2. * f.gu.x - f.gv.y - f.gw.z);
@@ -456,15 +551,18 @@ template <typename S> __device__ __forceinline__ void diffusive_flux(const FStat
     tz.x = tx.z;
     tz.y = ty.z;
     tz.z = 2. / 3. * (2. * f.gw.z - f.gu.x - f.gv.y);
-    S mu = f.mu + 0.;
+    S mu = f.mu + f.mu_t;
     flux[0] = S(0.);
     flux[1] = mu * vdot(tx, aV);
     flux[2] = mu * vdot(ty, aV);
     flux[3] = mu * vdot(tz, aV);
     flux[4] = mu * (vdot(tx, f.V) * aV.x + vdot(ty, f.V) * aV.y + vdot(tz, f.V) * aV.z);
-    S qq = (f.lambda + 0.) * vdot(f.gT, aV);
+    S tc = S(0.);
+    if constexpr (NV == 6) tc = P.Cp * f.mu_t / P.Pr_t;
+    S qq = (f.lambda + tc) * vdot(f.gT, aV);
     if (F.bc >= 0 && bc[F.bc].thermal == FCFD_THERMAL_FIXED_Q) qq = S(bc[F.bc].qdot * F.area);
     flux[4] = flux[4] + qq;
+    if constexpr (NV == 6) flux[5] = (f.mu + f.rho * f.nut * sa_fn(f.rho, f.nut, f.mu)) * vdot(f.gn, aV) / sa_sigma;
 }
 
 }  // namespace fcfd

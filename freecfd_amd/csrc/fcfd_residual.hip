@@ -1,6 +1,8 @@
 // Residual kernels for gfx950 (FP64).
 //
-// Three launches per residual evaluation, all deterministic and atomic-free:
+// Launches per residual evaluation, all deterministic and atomic-free (every
+// kernel is templated on the number of variables per cell, NV = 5, or 6 with
+// the SA model):
 //   1. ghost_kernel      one thread per boundary face: boundary ghost state
 //   2. grad_lim_kernel   one thread per cell: gradient-map gather (sliced
 //                        ELL, 64-wide slices matching the wavefront) fused
@@ -23,6 +25,7 @@ namespace fcfd {
 // 1. boundary ghosts: apply_bcs with the parent's cell-centre state as left and
 //    right pre-set to a copy of left; ghost = right.{p,V,T}_center
 // --------------------------------------------------------------------------
+template <int NV>
 __global__ void __launch_bounds__(256) ghost_kernel(DevGrid g, Params P, double* __restrict__ W, int* __restrict__ status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.B) return;
@@ -35,6 +38,8 @@ __global__ void __launch_bounds__(256) ghost_kernel(DevGrid g, Params P, double*
     L.p = L.p_center = W[c];
     L.V = L.V_center = mk<double>(W[ld + c], W[2 * ld + c], W[3 * ld + c]);
     L.T = L.T_center = W[4 * ld + c];
+    if constexpr (NV == 6) L.nut = L.nut_center = W[5 * ld + c];
+    else L.nut = L.nut_center = 0.0;
     L.rho = m_rho(P, L.p, L.T);
     finish_state(P, L, F);
     CState<double> R = L;
@@ -46,22 +51,23 @@ __global__ void __launch_bounds__(256) ghost_kernel(DevGrid g, Params P, double*
     W[2 * ld + gi] = R.V_center.y;
     W[3 * ld + gi] = R.V_center.z;
     W[4 * ld + gi] = R.T_center;
+    if constexpr (NV == 6) W[5 * ld + gi] = R.nut_center;
 }
 
 // --------------------------------------------------------------------------
 // 2. gradients (calc_cell_grads, ns.cc:301-336 + variable.h:193-216) fused
 //    with the limiter (ns_limiters.cc:26-248)
 // --------------------------------------------------------------------------
-template <int LIM>
+template <int LIM, int NV>
 __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, const double* __restrict__ W,
                                                        double* __restrict__ G, double* __restrict__ PHI,
                                                        const double* __restrict__ dref) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
-    double gr[15];
+    double gr[3 * NV];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) gr[k] = 0.0;
+    for (int k = 0; k < 3 * NV; ++k) gr[k] = 0.0;
     {
         int s = c >> 6, lane = c & 63;
         int e0 = g.gslice[s], e1 = g.gslice[s + 1];
@@ -69,7 +75,7 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
             int j = g.gidx[e];
             double wx = g.gwx[e], wy = g.gwy[e], wz = g.gwz[e];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 double q = W[v * ld + j];
                 gr[3 * v] += wx * q;
                 gr[3 * v + 1] += wy * q;
@@ -79,11 +85,10 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
     }
     const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) G[k * ld + c] = gr[k];
-    double q0[5];
+    for (int k = 0; k < 3 * NV; ++k) G[k * ld + c] = gr[k];
+    double q0[NV], phi[NV];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) q0[v] = W[v * ld + c];
-    double phi[5] = {1., 1., 1., 1., 1.};
+    for (int v = 0; v < NV; ++v) { q0[v] = W[v * ld + c]; phi[v] = 1.; }
     Vec<double> xc = centroid(g, c);
     if (LIM == FCFD_LIMITER_MINMOD) {
         const double small = 1.e-12;
@@ -93,7 +98,7 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
             int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
             Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 double dU = W[v * ld + nb] - q0[v];
                 double dUg = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
                 double mm = (dU * dUg < 0.) ? 0. : (dU > 0.) ? smin(dU, dUg) : smax(dU, dUg);
@@ -102,15 +107,15 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
             }
         }
     } else if (LIM == FCFD_LIMITER_BJ || LIM == FCFD_LIMITER_VK) {
-        double umax[5], umin[5];
+        double umax[NV], umin[NV];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) umax[v] = umin[v] = q0[v];
+        for (int v = 0; v < NV; ++v) umax[v] = umin[v] = q0[v];
         for (int h = h0; h < h1; ++h) {
             int fs = g.hf_face[h];
             int f = fs >= 0 ? fs : ~fs;
             int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 double q = W[v * ld + nb];
                 umax[v] = smax(umax[v], q);
                 umin[v] = smin(umin[v], q);
@@ -121,7 +126,7 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
             int f = fs >= 0 ? fs : ~fs;
             Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 double dM = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
                 double dP = q0[v];
                 if (LIM == FCFD_LIMITER_BJ) {
@@ -143,7 +148,7 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
     }
     if (LIM != FCFD_LIMITER_NONE) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) PHI[v * ld + c] = phi[v];
+        for (int v = 0; v < NV; ++v) PHI[v * ld + c] = phi[v];
     }
     // boundary ghosts take their parent's gradient (ns.cc:320-333)
     for (int h = h0; h < h1; ++h) {
@@ -152,7 +157,7 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
         int gi = g.fnei[fs];
         if (gi < g.N) continue;
 #pragma unroll
-        for (int k = 0; k < 15; ++k) G[k * ld + gi] = gr[k];
+        for (int k = 0; k < 3 * NV; ++k) G[k * ld + gi] = gr[k];
     }
 }
 
@@ -165,46 +170,53 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
 // N-1.  Two passes: fixed-size grid of partial min/max, then one block (max and
 // min are exact, so the result is independent of the order anyway).
 constexpr int kMMBlocks = 1024;
+// slots: 0..2 max p,u,v   3: max p excluding `last`   4..6: min p,u,v   7: min
+// p excluding `last`; with the SA model 8: max nu~, 9: min nu~ (a plain range:
+// the build-only variable has no reference slot)
+template <int NV>
 __global__ void __launch_bounds__(256) minmax_partial_kernel(int N, size_t ld, int first, int last, const double* __restrict__ W,
                                                              double* __restrict__ part) {
-    __shared__ double sh[8][256];
+    constexpr int K = NV == 6 ? 10 : 8;
+    __shared__ double sh[K][256];
     int t = threadIdx.x;
-    // 0..2: max p,u,v   3: max p excluding `last`   4..6: min p,u,v   7: min p excluding `last`
-    double r[8];
+    double r[K];
     r[0] = r[4] = W[first]; r[1] = r[5] = W[ld + first]; r[2] = r[6] = W[2 * ld + first];
     r[3] = r[7] = W[4 * ld + first];   // T_0 seeds the p-tracking slots
+    if constexpr (NV == 6) r[8] = r[9] = W[5 * ld + first];
     for (int c = blockIdx.x * 256 + t; c < N; c += kMMBlocks * 256) {
         double p = W[c], u = W[ld + c], v = W[2 * ld + c];
         r[0] = smax(r[0], p); r[1] = smax(r[1], u); r[2] = smax(r[2], v);
         r[4] = smin(r[4], p); r[5] = smin(r[5], u); r[6] = smin(r[6], v);
         if (c != last) { r[3] = smax(r[3], p); r[7] = smin(r[7], p); }
+        if constexpr (NV == 6) { double n = W[5 * ld + c]; r[8] = smax(r[8], n); r[9] = smin(r[9], n); }
     }
-    for (int k = 0; k < 8; ++k) sh[k][t] = r[k];
+    for (int k = 0; k < K; ++k) sh[k][t] = r[k];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (t < s) {
-            for (int k = 0; k < 4; ++k) sh[k][t] = smax(sh[k][t], sh[k][t + s]);
-            for (int k = 4; k < 8; ++k) sh[k][t] = smin(sh[k][t], sh[k][t + s]);
+            for (int k = 0; k < K; ++k) sh[k][t] = (k < 4 || k == 8) ? smax(sh[k][t], sh[k][t + s]) : smin(sh[k][t], sh[k][t + s]);
         }
         __syncthreads();
     }
-    if (t == 0) for (int k = 0; k < 8; ++k) part[k * kMMBlocks + blockIdx.x] = sh[k][0];
+    if (t == 0) for (int k = 0; k < K; ++k) part[k * kMMBlocks + blockIdx.x] = sh[k][0];
 }
 
+template <int NV>
 __global__ void __launch_bounds__(256) minmax_final_kernel(size_t ld, int first, int last, const double* __restrict__ W,
                                                            const double* __restrict__ part, double* __restrict__ dref) {
-    __shared__ double sh[8][256];
+    constexpr int K = NV == 6 ? 10 : 8;
+    __shared__ double sh[K][256];
     int t = threadIdx.x;
-    double r[8];
-    for (int k = 0; k < 8; ++k) r[k] = part[k * kMMBlocks + t];
+    double r[K];
+    for (int k = 0; k < K; ++k) r[k] = part[k * kMMBlocks + t];
     for (int i = t + 256; i < kMMBlocks; i += 256)
-        for (int k = 0; k < 8; ++k) r[k] = k < 4 ? smax(r[k], part[k * kMMBlocks + i]) : smin(r[k], part[k * kMMBlocks + i]);
-    for (int k = 0; k < 8; ++k) sh[k][t] = r[k];
+        for (int k = 0; k < K; ++k)
+            r[k] = (k < 4 || k == 8) ? smax(r[k], part[k * kMMBlocks + i]) : smin(r[k], part[k * kMMBlocks + i]);
+    for (int k = 0; k < K; ++k) sh[k][t] = r[k];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (t < s) {
-            for (int k = 0; k < 4; ++k) sh[k][t] = smax(sh[k][t], sh[k][t + s]);
-            for (int k = 4; k < 8; ++k) sh[k][t] = smin(sh[k][t], sh[k][t + s]);
+            for (int k = 0; k < K; ++k) sh[k][t] = (k < 4 || k == 8) ? smax(sh[k][t], sh[k][t + s]) : smin(sh[k][t], sh[k][t + s]);
         }
         __syncthreads();
     }
@@ -218,48 +230,65 @@ __global__ void __launch_bounds__(256) minmax_final_kernel(size_t ld, int first,
         dref[2] = fabs(sh[2][0] - sh[6][0]);
         dref[3] = fabs(q3max - q3min);
         dref[4] = fabs(q4max - q4min);
+        if constexpr (NV == 6) dref[5] = fabs(sh[8][0] - sh[9][0]);
     }
 }
 
 // --------------------------------------------------------------------------
-// 3. face fluxes + per-cell ordered accumulation (ns_assemble_linear_system.cc:75-140)
+// 3. face fluxes (ns_assemble_linear_system.cc:75-130): D - C per face
 // --------------------------------------------------------------------------
-template <int FLUX, int VISC>
+template <int FLUX, int VISC, int NV>
 __global__ void __launch_bounds__(256) face_flux_kernel(DevGrid g, Params P, const double* __restrict__ W,
                                                         const double* __restrict__ G, const double* __restrict__ PHI,
                                                         int has_phi, double phi_const, double* __restrict__ FX,
                                                         int* __restrict__ status) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
-    FaceCtx<double> X;
-    CellLoader ld_{W, G, PHI, has_phi != 0, phi_const, (size_t)g.ld};
-    face_base<FLUX, VISC>(g, P, ld_, f, X, status);
+    FaceCtx<double, NV> X;
+    CellLoader<NV> ld_{W, G, PHI, has_phi != 0, phi_const, (size_t)g.ld};
+    face_base<FLUX, VISC, NV>(g, P, ld_, f, X, status);
     const size_t Fs = g.F;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) FX[v * Fs + f] = X.D[v] - X.C[v];
+    for (int v = 0; v < NV; ++v) FX[v * Fs + f] = X.D[v] - X.C[v];
+}
+
+// SA source of cell c per unit volume, from the cell state, its gradients and
+// its wall distance
+__device__ __forceinline__ double cell_sa_source(const DevGrid& g, const Params& P, const double* __restrict__ W,
+                                                 const double* __restrict__ G, int c) {
+    const size_t ld = g.ld;
+    double p = W[c], T = W[4 * ld + c], nut = W[5 * ld + c];
+    auto gvec = [&](int v) { return mk<double>(G[(3 * v) * ld + c], G[(3 * v + 1) * ld + c], G[(3 * v + 2) * ld + c]); };
+    return sa_source<double>(m_rho(P, p, T), m_visc(P, T), nut, gvec(1), gvec(2), gvec(3), gvec(5), g.wd[c]);
 }
 
 // R_c = sum over the faces of c in reference face order of (D-C)+0 (parent)
 // or -1*(D-C)+0 (neighbour) -- the VecSetValues(ADD) sequence of
-// ns_assemble_linear_system.cc:131-140 starting from VecSet(rhs, 0)
-__global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, const double* __restrict__ FX, double* __restrict__ Rout) {
+// ns_assemble_linear_system.cc:131-140 starting from VecSet(rhs, 0) -- then,
+// with SRC (SA model), + V_c S_c in the nu~ row
+template <int NV, bool SRC>
+__global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ G,
+                                                       const double* __restrict__ FX, double* __restrict__ Rout) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld, Fs = g.F;
-    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double s[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s[v] = 0.0;
     for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
         int fs = g.hf_face[h];
         if (fs >= 0) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) s[v] += FX[v * Fs + fs] + 0.;
+            for (int v = 0; v < NV; ++v) s[v] += FX[v * Fs + fs] + 0.;
         } else {
             int f = ~fs;
 #pragma unroll
-            for (int v = 0; v < 5; ++v) s[v] += -1. * FX[v * Fs + f] + 0.;
+            for (int v = 0; v < NV; ++v) s[v] += -1. * FX[v * Fs + f] + 0.;
         }
     }
+    if constexpr (SRC && NV == 6) s[5] += g.vol[c] * cell_sa_source(g, P, W, G, c);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) Rout[v * ld + c] = s[v];
+    for (int v = 0; v < NV; ++v) Rout[v * ld + c] = s[v];
 }
 
 // --------------------------------------------------------------------------
@@ -277,53 +306,115 @@ __global__ void __launch_bounds__(256) dt_local_kernel(DevGrid g, Params P, cons
     dt[c] = cfl * g.lscale[c] / (vmag(V) + a);
 }
 
+// closest wall distance of every cell (set_bcs.cc:178-210 brute force): the
+// minimum over the no-slip wall face centroids of |x_c - x_f|, exact and so
+// independent of the order of the search; wall centroids staged in LDS tiles
+__global__ void __launch_bounds__(256) wall_distance_kernel(int T, const double* __restrict__ ccx, const double* __restrict__ ccy,
+                                                            const double* __restrict__ ccz, int nw, const double* __restrict__ wx,
+                                                            const double* __restrict__ wy, const double* __restrict__ wz,
+                                                            double* __restrict__ wd) {
+    __shared__ double tx[256], ty[256], tz[256];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = c < T;
+    Vec<double> x = active ? mk<double>(ccx[c], ccy[c], ccz[c]) : mk<double>(0., 0., 0.);
+    double best = 1.e20;
+    for (int t0 = 0; t0 < nw; t0 += 256) {
+        int k = t0 + threadIdx.x;
+        if (k < nw) { tx[threadIdx.x] = wx[k]; ty[threadIdx.x] = wy[k]; tz[threadIdx.x] = wz[k]; }
+        __syncthreads();
+        const int m = min(256, nw - t0);
+        for (int q = 0; q < m; ++q) {
+            double d = vmag(vsub(x, mk<double>(tx[q], ty[q], tz[q])));
+            if (best > d) best = d;
+        }
+        __syncthreads();
+    }
+    if (active) wd[c] = best;
+}
+
+// closest no-slip wall face (centroid distance) of every non-wall face and the
+// Steger-Warming dissipation factor 1 - |n_wall . n_f| of the first closest
+// wall face in reference face order (set_bcs.cc:211-249); wall faces get 0, 0
+__global__ void __launch_bounds__(256) face_wall_kernel(int F, const double* __restrict__ fcx, const double* __restrict__ fcy,
+                                                        const double* __restrict__ fcz, const double* __restrict__ fnx,
+                                                        const double* __restrict__ fny, const double* __restrict__ fnz,
+                                                        const int* __restrict__ is_wall, int nw, const double* __restrict__ wx,
+                                                        const double* __restrict__ wy, const double* __restrict__ wz,
+                                                        const double* __restrict__ wnx, const double* __restrict__ wny,
+                                                        const double* __restrict__ wnz, double* __restrict__ fwd,
+                                                        double* __restrict__ fdiss) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    if (is_wall[f]) { fwd[f] = 0.; fdiss[f] = 0.; return; }
+    Vec<double> x = mk<double>(fcx[f], fcy[f], fcz[f]);
+    double best = 1.e20, nx = 0., ny = 0., nz = 0.;
+    for (int q = 0; q < nw; ++q) {
+        double d = vmag(vsub(x, mk<double>(wx[q], wy[q], wz[q])));
+        if (best > d) { best = d; nx = wnx[q]; ny = wny[q]; nz = wnz[q]; }
+    }
+    fwd[f] = best;
+    fdiss[f] = 1. - fabs(nx * fnx[f] + ny * fny[f] + nz * fnz[f]);
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-template <int FLUX, int VISC>
+template <int FLUX, int VISC, int NV>
 static void launch_flux_t(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI,
                           int has_phi, double phi_const, double* FX, int* status, hipStream_t s) {
-    hipLaunchKernelGGL((face_flux_kernel<FLUX, VISC>), dim3((g.F + 255) / 256), dim3(256), 0, s, g, P, W, G, PHI, has_phi,
+    hipLaunchKernelGGL((face_flux_kernel<FLUX, VISC, NV>), dim3((g.F + 255) / 256), dim3(256), 0, s, g, P, W, G, PHI, has_phi,
                        phi_const, FX, status);
 }
 
 template <int FLUX>
 static void launch_flux_v(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI,
                           int has_phi, double phi_const, double* FX, int* status, hipStream_t s) {
-    if (P.viscous) launch_flux_t<FLUX, 1>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);
-    else launch_flux_t<FLUX, 0>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);
+    if (g.nv == 6) launch_flux_t<FLUX, 1, 6>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);   // SA is always viscous
+    else if (P.viscous) launch_flux_t<FLUX, 1, 5>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);
+    else launch_flux_t<FLUX, 0, 5>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);
 }
 
 void launch_ghosts(const DevGrid& g, const Params& P, double* W, int* status, hipStream_t s) {
-    if (g.B > 0) hipLaunchKernelGGL(ghost_kernel, dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, status);
+    if (g.B == 0) return;
+    if (g.nv == 6) hipLaunchKernelGGL(ghost_kernel<6>, dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, status);
+    else hipLaunchKernelGGL(ghost_kernel<5>, dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, status);
 }
 
-void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s) {
-    int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
-    dim3 grid((g.N + 255) / 256), blk(256);
-    if (lim == FCFD_LIMITER_VK) {
-        // eps^2 = (1e-6 K dref/L)^2 vanishes for K = 0 (the default threshold,
-        // read_inputs.cc:129) whatever dref is, provided L > 0: skip the reduction
-        if (!(P.limiter_threshold == 0.0 && P.glscale > 0.0)) {
-            hipLaunchKernelGGL(minmax_partial_kernel, dim3(kMMBlocks), dim3(256), 0, s, g.N, (size_t)g.ld, g.ref_first, g.ref_last, W,
-                               dref + 8);
-            hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(256), 0, s, (size_t)g.ld, g.ref_first, g.ref_last, W, dref + 8, dref);
-        }
-        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_VK>), grid, blk, 0, s, g, P, W, G, PHI, dref);
-    } else if (lim == FCFD_LIMITER_BJ) {
-        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_BJ>), grid, blk, 0, s, g, P, W, G, PHI, dref);
-    } else if (lim == FCFD_LIMITER_MINMOD) {
-        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_MINMOD>), grid, blk, 0, s, g, P, W, G, PHI, dref);
-    } else {
-        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_NONE>), grid, blk, 0, s, g, P, W, G, PHI, dref);
-    }
+template <int NV>
+static void launch_vk_range_t(const DevGrid& g, const double* W, double* dref, hipStream_t s) {
+    hipLaunchKernelGGL(minmax_partial_kernel<NV>, dim3(kMMBlocks), dim3(256), 0, s, g.N, (size_t)g.ld, g.ref_first, g.ref_last, W,
+                       dref + 8);
+    hipLaunchKernelGGL(minmax_final_kernel<NV>, dim3(1), dim3(256), 0, s, (size_t)g.ld, g.ref_first, g.ref_last, W, dref + 8, dref);
 }
 
 void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double* dref, hipStream_t s) {
     int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    // eps^2 = (1e-6 K dref/L)^2 vanishes for K = 0 (the default threshold,
+    // read_inputs.cc:129) whatever dref is, provided L > 0: skip the reduction
     if (lim != FCFD_LIMITER_VK || (P.limiter_threshold == 0.0 && P.glscale > 0.0)) return;
-    hipLaunchKernelGGL(minmax_partial_kernel, dim3(kMMBlocks), dim3(256), 0, s, g.N, (size_t)g.ld, g.ref_first, g.ref_last, W, dref + 8);
-    hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(256), 0, s, (size_t)g.ld, g.ref_first, g.ref_last, W, dref + 8, dref);
+    if (g.nv == 6) launch_vk_range_t<6>(g, W, dref, s);
+    else launch_vk_range_t<5>(g, W, dref, s);
+}
+
+template <int NV>
+static void launch_grad_lim_t(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s) {
+    int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (lim == FCFD_LIMITER_VK) {
+        launch_vk_range(g, P, W, dref, s);
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_VK, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+    } else if (lim == FCFD_LIMITER_BJ) {
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_BJ, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+    } else if (lim == FCFD_LIMITER_MINMOD) {
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_MINMOD, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+    } else {
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_NONE, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+    }
+}
+
+void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s) {
+    if (g.nv == 6) launch_grad_lim_t<6>(g, P, W, G, PHI, dref, s);
+    else launch_grad_lim_t<5>(g, P, W, G, PHI, dref, s);
 }
 
 void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* FX,
@@ -343,11 +434,34 @@ void launch_flux(const DevGrid& g, const Params& P, const double* W, const doubl
 }
 
 void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s) {
-    hipLaunchKernelGGL(cell_sum_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, FX, R);
+    dim3 grid((g.N + 255) / 256), blk(256);
+    Params P{};
+    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, false>), grid, blk, 0, s, g, P, nullptr, nullptr, FX, R);
+    else hipLaunchKernelGGL((cell_sum_kernel<5, false>), grid, blk, 0, s, g, P, nullptr, nullptr, FX, R);
+}
+
+void launch_residual_sum(const DevGrid& g, const Params& P, const double* W, const double* G, const double* FX, double* R,
+                         hipStream_t s) {
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, true>), grid, blk, 0, s, g, P, W, G, FX, R);
+    else hipLaunchKernelGGL((cell_sum_kernel<5, false>), grid, blk, 0, s, g, P, W, G, FX, R);
 }
 
 void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s) {
     hipLaunchKernelGGL(dt_local_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, P, W, cfl, dt);
+}
+
+void launch_wall_distance(int T, const double* ccx, const double* ccy, const double* ccz, int nw, const double* wx, const double* wy,
+                          const double* wz, double* wd, hipStream_t s) {
+    if (T > 0) hipLaunchKernelGGL(wall_distance_kernel, dim3((T + 255) / 256), dim3(256), 0, s, T, ccx, ccy, ccz, nw, wx, wy, wz, wd);
+}
+
+void launch_face_wall(int F, const double* fcx, const double* fcy, const double* fcz, const double* fnx, const double* fny,
+                      const double* fnz, const int* is_wall, int nw, const double* wx, const double* wy, const double* wz,
+                      const double* wnx, const double* wny, const double* wnz, double* fwd, double* fdiss, hipStream_t s) {
+    if (F > 0)
+        hipLaunchKernelGGL(face_wall_kernel, dim3((F + 255) / 256), dim3(256), 0, s, F, fcx, fcy, fcz, fnx, fny, fnz, is_wall, nw,
+                           wx, wy, wz, wnx, wny, wnz, fwd, fdiss);
 }
 
 }  // namespace fcfd

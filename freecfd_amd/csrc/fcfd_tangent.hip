@@ -2,16 +2,18 @@
 // numbers through the same code as the FP64 residual.
 //
 //   FULL   ghosts (update_boundaries) -> gradients -> limiters -> face
-//          fluxes are all differentiated: the exact Jacobian-vector product of
-//          the second-order residual.
-//   FROZEN only the cell-centre states of the two cells of each face carry a
-//          tangent; gradients, limiters and ghost centres are frozen.  This is
-//          the linearisation the reference FD Jacobian approximates
-//          (get_jacobians perturbs the reconstructed face states only,
-//          ns_assemble_linear_system.cc:197-267), i.e. dR = sum_f JL dW_P +
-//          JR dW_N with exact derivatives instead of one-sided differences.
+//          fluxes (-> SA source) are all differentiated: the exact
+//          Jacobian-vector product of the second-order residual.
+//   FROZEN only the cell-centre states of the two cells of each face (and of
+//          the cell itself in the SA source) carry a tangent; gradients,
+//          limiters and ghost centres are frozen.  This is the linearisation
+//          the reference FD Jacobian approximates (get_jacobians perturbs the
+//          reconstructed face states only, ns_assemble_linear_system.cc:197-267),
+//          i.e. dR = sum_f JL dW_P + JR dW_N with exact derivatives instead of
+//          one-sided differences.
 // The VK global reference range (find_min_max) is held constant; with the
-// default threshold K = 0 it does not enter the limiter at all.
+// default threshold K = 0 it does not enter the limiter at all.  Every kernel
+// is templated on NV (5, or 6 with the SA model).
 #include <hip/hip_runtime.h>
 
 #include "fcfd_face.h"
@@ -28,6 +30,7 @@ struct TangentNb {
     __device__ __forceinline__ Dual nb(int, int j, int v) const { return Dual(W[v * ld + j], dW[v * ld + j]); }
 };
 
+template <int NV>
 __global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, double* __restrict__ W, double* __restrict__ dW,
                                                       int* __restrict__ status) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -41,6 +44,7 @@ __global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, doubl
     L.p = L.p_center = Dual(W[c], dW[c]);
     L.V = L.V_center = mk<Dual>(Dual(W[ld + c], dW[ld + c]), Dual(W[2 * ld + c], dW[2 * ld + c]), Dual(W[3 * ld + c], dW[3 * ld + c]));
     L.T = L.T_center = Dual(W[4 * ld + c], dW[4 * ld + c]);
+    if constexpr (NV == 6) L.nut = L.nut_center = Dual(W[5 * ld + c], dW[5 * ld + c]);
     L.rho = m_rho(P, L.p, L.T);
     finish_state(P, L, F);
     CState<Dual> R = L;
@@ -51,9 +55,10 @@ __global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, doubl
     W[2 * ld + gi] = R.V_center.y.v; dW[2 * ld + gi] = R.V_center.y.d;
     W[3 * ld + gi] = R.V_center.z.v; dW[3 * ld + gi] = R.V_center.z.d;
     W[4 * ld + gi] = R.T_center.v; dW[4 * ld + gi] = R.T_center.d;
+    if constexpr (NV == 6) { W[5 * ld + gi] = R.nut_center.v; dW[5 * ld + gi] = R.nut_center.d; }
 }
 
-template <int LIM>
+template <int LIM, int NV>
 __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, const double* __restrict__ W,
                                                          const double* __restrict__ dW, double* __restrict__ G,
                                                          double* __restrict__ dG, double* __restrict__ PHI,
@@ -61,9 +66,9 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
-    Dual gr[15];
+    Dual gr[3 * NV];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) gr[k] = Dual(0.0);
+    for (int k = 0; k < 3 * NV; ++k) gr[k] = Dual(0.0);
     {
         int s = c >> 6, lane = c & 63;
         int e0 = g.gslice[s], e1 = g.gslice[s + 1];
@@ -71,7 +76,7 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
             int j = g.gidx[e];
             double wx = g.gwx[e], wy = g.gwy[e], wz = g.gwz[e];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
+            for (int v = 0; v < NV; ++v) {
                 Dual q(W[v * ld + j], dW[v * ld + j]);
                 gr[3 * v] += wx * q;
                 gr[3 * v + 1] += wy * q;
@@ -81,15 +86,15 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
     }
     const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) { G[k * ld + c] = gr[k].v; dG[k * ld + c] = gr[k].d; }
-    Dual q0[5];
+    for (int k = 0; k < 3 * NV; ++k) { G[k * ld + c] = gr[k].v; dG[k * ld + c] = gr[k].d; }
+    Dual q0[NV];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) q0[v] = Dual(W[v * ld + c], dW[v * ld + c]);
-    Dual phi[5];
-    limiter_phi<LIM>(g, P, TangentNb{W, dW, ld}, c, q0, gr, dref, phi);
+    for (int v = 0; v < NV; ++v) q0[v] = Dual(W[v * ld + c], dW[v * ld + c]);
+    Dual phi[NV];
+    limiter_phi<LIM, NV>(g, P, TangentNb{W, dW, ld}, c, q0, gr, dref, phi);
     if (LIM != FCFD_LIMITER_NONE) {
 #pragma unroll
-        for (int v = 0; v < 5; ++v) { PHI[v * ld + c] = phi[v].v; dPHI[v * ld + c] = phi[v].d; }
+        for (int v = 0; v < NV; ++v) { PHI[v * ld + c] = phi[v].v; dPHI[v * ld + c] = phi[v].d; }
     }
     for (int h = h0; h < h1; ++h) {
         int fs = g.hf_face[h];
@@ -97,24 +102,46 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
         int gi = g.fnei[fs];
         if (gi < g.N) continue;
 #pragma unroll
-        for (int k = 0; k < 15; ++k) { G[k * ld + gi] = gr[k].v; dG[k * ld + gi] = gr[k].d; }
+        for (int k = 0; k < 3 * NV; ++k) { G[k * ld + gi] = gr[k].v; dG[k * ld + gi] = gr[k].d; }
     }
 }
 
-template <int FLUX, int VISC, bool FULL>
-__global__ void __launch_bounds__(256) face_flux_t_kernel(DevGrid g, Params P, DualLoader<FULL> L, double* __restrict__ dFX,
+template <int FLUX, int VISC, bool FULL, int NV>
+__global__ void __launch_bounds__(256) face_flux_t_kernel(DevGrid g, Params P, DualLoader<NV, FULL> L, double* __restrict__ dFX,
                                                           int* __restrict__ status) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
-    FaceCtx<Dual> X;
-    face_base<FLUX, VISC>(g, P, L, f, X, status);
+    FaceCtx<Dual, NV> X;
+    face_base<FLUX, VISC, NV>(g, P, L, f, X, status);
     const size_t Fs = g.F;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) dFX[v * Fs + f] = (X.D[v] - X.C[v]).d;
+    for (int v = 0; v < NV; ++v) dFX[v * Fs + f] = (X.D[v] - X.C[v]).d;
+}
+
+// tangent of V_c S_c (SA source) added to the nu~ row; FULL carries the
+// gradient tangents, FROZEN only the cell state's
+template <bool FULL>
+__global__ void __launch_bounds__(256) sa_source_t_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ dW,
+                                                          const double* __restrict__ G, const double* __restrict__ dG,
+                                                          double* __restrict__ dR) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld;
+    Dual p(W[c], dW[c]), T(W[4 * ld + c], dW[4 * ld + c]), nut(W[5 * ld + c], dW[5 * ld + c]);
+    auto gvec = [&](int v) {
+        Vec<Dual> r;
+        r.x = Dual(G[(3 * v) * ld + c], FULL ? dG[(3 * v) * ld + c] : 0.0);
+        r.y = Dual(G[(3 * v + 1) * ld + c], FULL ? dG[(3 * v + 1) * ld + c] : 0.0);
+        r.z = Dual(G[(3 * v + 2) * ld + c], FULL ? dG[(3 * v + 2) * ld + c] : 0.0);
+        return r;
+    };
+    Dual S = sa_source<Dual>(m_rho(P, p, T), m_visc(P, T), nut, gvec(1), gvec(2), gvec(3), gvec(5), g.wd[c]);
+    dR[5 * ld + c] += g.vol[c] * S.d;
 }
 
 // y = (P V/dt) x - dR  (the implicit operator applied matrix-free,
 // time_terms ns_time_terms.cc:26-56,147-163 minus the tangent)
+template <int NV>
 __global__ void __launch_bounds__(256) newton_op_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ dt,
                                                         const double* __restrict__ x, const double* __restrict__ dR,
                                                         double* __restrict__ y) {
@@ -126,9 +153,9 @@ __global__ void __launch_bounds__(256) newton_op_kernel(DevGrid g, Params P, con
     double pressure = p + P.Pref, temperature = T + P.Tref;
     double drho_dT = -1. * rho / temperature, drho_dp = rho / pressure;
     double H = P.Cp * temperature + 0.5 * (u * u + v * v + w * w);
-    double xv[5];
+    double xv[NV];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) xv[k] = x[k * ld + c];
+    for (int k = 0; k < NV; ++k) xv[k] = x[k * ld + c];
     double s = g.vol[c] / dt[c];
     double r0 = drho_dp * xv[0] + drho_dT * xv[4];
     double r1 = drho_dp * u * xv[0] + rho * xv[1] + drho_dT * u * xv[4];
@@ -140,62 +167,81 @@ __global__ void __launch_bounds__(256) newton_op_kernel(DevGrid g, Params P, con
     y[2 * ld + c] = s * r2 - dR[2 * ld + c];
     y[3 * ld + c] = s * r3 - dR[3 * ld + c];
     y[4 * ld + c] = s * r4 - dR[4 * ld + c];
+    if constexpr (NV == 6) {
+        double nut = W[5 * ld + c];
+        double r5 = drho_dp * nut * xv[0] + drho_dT * nut * xv[4] + rho * xv[5];
+        y[5 * ld + c] = s * r5 - dR[5 * ld + c];
+    }
 }
 
 // ---------------------------------------------------------------------------
-template <int FLUX, int VISC, bool FULL>
-static void launch_ft(const DevGrid& g, const Params& P, const DualLoader<FULL>& L, double* dFX, int* status, hipStream_t s) {
-    hipLaunchKernelGGL((face_flux_t_kernel<FLUX, VISC, FULL>), dim3((g.F + 255) / 256), dim3(256), 0, s, g, P, L, dFX, status);
+template <int FLUX, int VISC, bool FULL, int NV>
+static void launch_ft(const DevGrid& g, const Params& P, const DualLoader<NV, FULL>& L, double* dFX, int* status, hipStream_t s) {
+    hipLaunchKernelGGL((face_flux_t_kernel<FLUX, VISC, FULL, NV>), dim3((g.F + 255) / 256), dim3(256), 0, s, g, P, L, dFX, status);
 }
-template <int FLUX, bool FULL>
-static void launch_fv(const DevGrid& g, const Params& P, const DualLoader<FULL>& L, double* dFX, int* status, hipStream_t s) {
-    if (P.viscous) launch_ft<FLUX, 1, FULL>(g, P, L, dFX, status, s);
-    else launch_ft<FLUX, 0, FULL>(g, P, L, dFX, status, s);
+template <int FLUX, bool FULL, int NV>
+static void launch_fv(const DevGrid& g, const Params& P, const DualLoader<NV, FULL>& L, double* dFX, int* status, hipStream_t s) {
+    if constexpr (NV == 6) launch_ft<FLUX, 1, FULL, 6>(g, P, L, dFX, status, s);   // SA is always viscous
+    else if (P.viscous) launch_ft<FLUX, 1, FULL, 5>(g, P, L, dFX, status, s);
+    else launch_ft<FLUX, 0, FULL, 5>(g, P, L, dFX, status, s);
 }
-template <bool FULL>
-static void launch_ff(const DevGrid& g, const Params& P, const DualLoader<FULL>& L, double* dFX, int* status, hipStream_t s) {
+template <bool FULL, int NV>
+static void launch_ff(const DevGrid& g, const Params& P, const DualLoader<NV, FULL>& L, double* dFX, int* status, hipStream_t s) {
     switch (P.flux) {
-    case FCFD_FLUX_ROE: launch_fv<FCFD_FLUX_ROE, FULL>(g, P, L, dFX, status, s); break;
-    case FCFD_FLUX_VAN_LEER: launch_fv<FCFD_FLUX_VAN_LEER, FULL>(g, P, L, dFX, status, s); break;
-    case FCFD_FLUX_SD_SLAU: launch_fv<FCFD_FLUX_SD_SLAU, FULL>(g, P, L, dFX, status, s); break;
-    case FCFD_FLUX_SW: launch_fv<FCFD_FLUX_SW, FULL>(g, P, L, dFX, status, s); break;
-    default: launch_fv<FCFD_FLUX_AUSM_PLUS_UP, FULL>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_ROE: launch_fv<FCFD_FLUX_ROE, FULL, NV>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_VAN_LEER: launch_fv<FCFD_FLUX_VAN_LEER, FULL, NV>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_SD_SLAU: launch_fv<FCFD_FLUX_SD_SLAU, FULL, NV>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_SW: launch_fv<FCFD_FLUX_SW, FULL, NV>(g, P, L, dFX, status, s); break;
+    default: launch_fv<FCFD_FLUX_AUSM_PLUS_UP, FULL, NV>(g, P, L, dFX, status, s); break;
+    }
+}
+
+template <int NV>
+static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
+                         double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s) {
+    const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    const bool has_phi = lim != FCFD_LIMITER_NONE;
+    const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
+    CellLoader<NV> cl{W, G, PHI, has_phi, phi_const, (size_t)g.ld};
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (full) {
+        if (g.B > 0) hipLaunchKernelGGL(ghost_t_kernel<NV>, dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
+        if (lim == FCFD_LIMITER_VK) {
+            launch_vk_range(g, P, W, dref, s);   // primal reference range (constant in the tangent)
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_VK, NV>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        } else if (lim == FCFD_LIMITER_BJ) {
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_BJ, NV>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        } else if (lim == FCFD_LIMITER_MINMOD) {
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_MINMOD, NV>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        } else {
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_NONE, NV>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        }
+        DualLoader<NV, true> L{cl, dW, dG, dPHI, g.N};
+        launch_ff<true, NV>(g, P, L, dFX, status, s);
+    } else {
+        launch_ghosts(g, P, W, status, s);
+        launch_grad_lim(g, P, W, G, PHI, dref, s);
+        DualLoader<NV, false> L{cl, dW, nullptr, nullptr, g.N};
+        launch_ff<false, NV>(g, P, L, dFX, status, s);
+    }
+    launch_cell_sum(g, dFX, dR, s);
+    if constexpr (NV == 6) {
+        if (full) hipLaunchKernelGGL(sa_source_t_kernel<true>, grid, blk, 0, s, g, P, W, dW, G, dG, dR);
+        else hipLaunchKernelGGL(sa_source_t_kernel<false>, grid, blk, 0, s, g, P, W, dW, G, dG, dR);
     }
 }
 
 void launch_jvp(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
                 double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s) {
-    const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
-    const bool has_phi = lim != FCFD_LIMITER_NONE;
-    const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
-    CellLoader cl{W, G, PHI, has_phi, phi_const, (size_t)g.ld};
-    if (full) {
-        if (g.B > 0) hipLaunchKernelGGL(ghost_t_kernel, dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
-        dim3 grid((g.N + 255) / 256), blk(256);
-        if (lim == FCFD_LIMITER_VK) {
-            launch_vk_range(g, P, W, dref, s);   // primal reference range (constant in the tangent)
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_VK>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
-        } else if (lim == FCFD_LIMITER_BJ) {
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_BJ>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
-        } else if (lim == FCFD_LIMITER_MINMOD) {
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_MINMOD>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
-        } else {
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_NONE>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
-        }
-        DualLoader<true> L{cl, dW, dG, dPHI, g.N};
-        launch_ff<true>(g, P, L, dFX, status, s);
-    } else {
-        launch_ghosts(g, P, W, status, s);
-        launch_grad_lim(g, P, W, G, PHI, dref, s);
-        DualLoader<false> L{cl, dW, nullptr, nullptr, g.N};
-        launch_ff<false>(g, P, L, dFX, status, s);
-    }
-    launch_cell_sum(g, dFX, dR, s);
+    if (g.nv == 6) launch_jvp_t<6>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
+    else launch_jvp_t<5>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
 }
 
 void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* dR,
                       double* y, hipStream_t s) {
-    hipLaunchKernelGGL(newton_op_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, P, W, dt, x, dR, y);
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL(newton_op_kernel<6>, grid, blk, 0, s, g, P, W, dt, x, dR, y);
+    else hipLaunchKernelGGL(newton_op_kernel<5>, grid, blk, 0, s, g, P, W, dt, x, dR, y);
 }
 
 }  // namespace fcfd

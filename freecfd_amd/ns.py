@@ -73,22 +73,27 @@ class NavierStokes(Grid):
         self.device = torch.device(device)
         torch.cuda.set_device(self.device)
         super().__init__(raw, opt)
-        ld = self.info.ld_cell
-        self.W = torch.zeros((5, ld), dtype=torch.float64, device=self.device)
-        self.R = torch.zeros((5, ld), dtype=torch.float64, device=self.device)
+        ld, nv = self.info.ld_cell, self.info.nv
+        self.W = torch.zeros((nv, ld), dtype=torch.float64, device=self.device)
+        self.R = torch.zeros((nv, ld), dtype=torch.float64, device=self.device)
 
     @property
     def ld(self):
         return self.info.ld_cell
 
+    @property
+    def nv(self):
+        """variables per cell: 5 (p, u, v, w, T), 6 with the SA model (+ nu~)"""
+        return self.info.nv
+
     def set_state(self, W_host):
         W_host = np.ascontiguousarray(W_host, dtype=np.float64)
-        assert W_host.shape == (self.info.n_cells, 5)
+        assert W_host.shape == (self.info.n_cells, self.nv)
         self.torch.cuda.synchronize()
         _check(self.h, lib().fcfd_state_to_device(self.h, W_host.ctypes.data, self.W.data_ptr()))
 
     def assemble_residual(self, W=None, R=None, stream=None):
-        """R(W) on the GPU; W, R are [5][ld] float64 device tensors."""
+        """R(W) on the GPU; W, R are [nv][ld] float64 device tensors."""
         W = self.W if W is None else W
         R = self.R if R is None else R
         s = self.torch.cuda.current_stream().cuda_stream if stream is None else stream
@@ -103,22 +108,23 @@ class NavierStokes(Grid):
         return out
 
     def residual_host(self):
-        return self.to_host(self.R, 5)
+        return self.to_host(self.R, self.nv)
 
     def state_host(self, include_ghosts=True):
-        return self.to_host(self.W, 5, include_ghosts)
+        return self.to_host(self.W, self.nv, include_ghosts)
 
     def gradients_limiters_host(self):
-        """Gradients ((N+B) x 15) and limiters (N x 5) of the last residual."""
+        """Gradients ((N+B) x 3nv) and limiters (N x nv) of the last residual."""
         g = C.c_void_p()
         p = C.c_void_p()
         _check(self.h, lib().fcfd_internal_buffers(self.h, C.byref(g), C.byref(p)))
         self.torch.cuda.synchronize()
         n = self.info.n_cells
-        G = np.zeros((n + self.info.n_ghosts, 15))
-        PHI = np.zeros((n, 5))
-        _check(self.h, lib().fcfd_cells_to_host(self.h, g, 15, 1, G.ctypes.data))
-        _check(self.h, lib().fcfd_cells_to_host(self.h, p, 5, 0, PHI.ctypes.data))
+        nv = self.nv
+        G = np.zeros((n + self.info.n_ghosts, 3 * nv))
+        PHI = np.zeros((n, nv))
+        _check(self.h, lib().fcfd_cells_to_host(self.h, g, 3 * nv, 1, G.ctypes.data))
+        _check(self.h, lib().fcfd_cells_to_host(self.h, p, nv, 0, PHI.ctypes.data))
         return G, PHI
 
     def fd_jacobian(self, W=None, dW_prev=None, dt=None, stream=None):
@@ -129,15 +135,15 @@ class NavierStokes(Grid):
         _check(self.h, lib().fcfd_fd_jacobian(self.h, W.data_ptr(), ptr(dW_prev), ptr(dt), C.c_void_p(s)))
 
     def jacobian_host(self):
-        F, N = self.info.n_faces, self.info.n_cells
-        JL = np.zeros((F, 5, 5))
-        JR = np.zeros((F, 5, 5))
-        D = np.zeros((N, 5, 5))
+        F, N, nv = self.info.n_faces, self.info.n_cells, self.nv
+        JL = np.zeros((F, nv, nv))
+        JR = np.zeros((F, nv, nv))
+        D = np.zeros((N, nv, nv))
         _check(self.h, lib().fcfd_jacobian_to_host(self.h, JL.ctypes.data, JR.ctypes.data, D.ctypes.data))
         return JL, JR, D
 
     def jvp(self, dW, mode=A.JAC_JVP_FULL, W=None, out=None):
-        """dR = (dR/dW) dW on the GPU (dW a [5][ld] float64 device tensor)."""
+        """dR = (dR/dW) dW on the GPU (dW a [nv][ld] float64 device tensor)."""
         W = self.W if W is None else W
         out = self.torch.zeros_like(self.W) if out is None else out
         s = self.torch.cuda.current_stream().cuda_stream
@@ -169,7 +175,7 @@ class NavierStokes(Grid):
         return out.reshape(n_bc, 6)[: int(self._opt.n_bc)]
 
     def to_device(self, X_host):
-        """N x 5 host array (reference order) -> [5][ld] device tensor."""
+        """N x nv host array (reference order) -> [nv][ld] device tensor."""
         X_host = np.ascontiguousarray(X_host, dtype=np.float64)
         t = self.torch.zeros_like(self.W)
         self.torch.cuda.synchronize()

@@ -19,7 +19,7 @@ from freecfd_amd import cases
 
 pytestmark = pytest.mark.gpu
 
-SCALES = np.array([1e5, 100.0, 100.0, 100.0, 300.0])
+SCALES = np.array([1e5, 100.0, 100.0, 100.0, 300.0, 1e-4])   # p, u, v, w, T, nu~
 
 
 def _ns(cs):
@@ -31,12 +31,12 @@ def _ns(cs):
 
 def _dot_check(cs, seed=11):
     ns = _ns(cs)
-    N = cs.n_cells
+    N, nv = cs.n_cells, ns.nv
     rng = np.random.default_rng(seed)
-    u = rng.normal(size=(N, 5)) * SCALES
-    v = rng.normal(size=(N, 5))
-    Ju = ns.to_host(ns.jvp(ns.to_device(u), mode=A.JAC_JVP_FULL), 5)
-    JTv = ns.to_host(ns.vjp(ns.to_device(v)), 5)
+    u = rng.normal(size=(N, nv)) * SCALES[:nv]
+    v = rng.normal(size=(N, nv))
+    Ju = ns.to_host(ns.jvp(ns.to_device(u), mode=A.JAC_JVP_FULL), nv)
+    JTv = ns.to_host(ns.vjp(ns.to_device(v)), nv)
     lhs = float(np.sum(v * Ju))
     rhs = float(np.sum(JTv * u))
     mag = float(np.sum(np.abs(v * Ju)))
