@@ -1,12 +1,14 @@
 """Residual-path benchmark on one MI355X.
 
 A step is one FP64 residual evaluation R(W) over the whole mesh (boundary
-ghosts, gradients + limiters, face fluxes), on a synthetic state resident in
-HBM.  Prints one JSON line: whole-job cell-residual evaluations per second,
-the roofline of the dominant kernel from HIP events recorded live around each
-kernel, and a CPU baseline (the CPU restatement on a bounded sample).
+ghosts, gradients + limiters, face fluxes, ordered cell sums with the SA
+source), on a synthetic state resident in HBM.  Prints one JSON line:
+whole-job cell-residual evaluations per second, the roofline of the dominant
+kernel from HIP events recorded live around each kernel, and a CPU baseline
+(the CPU restatement on a bounded sample).  The default workload is the
+SA-RANS flat plate (6 equations per cell).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plate]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plate_sa]
 """
 import argparse
 import ctypes as C
@@ -25,6 +27,8 @@ HBM_PEAK_GBS = 8000.0
 
 def make_case(name, scale=1):
     from freecfd_amd import cases
+    if name == "plate_sa":
+        return cases.plate_case(512 // scale, 128 // scale, 64 // scale, noise=0.01, sa=True)
     if name == "plate":
         return cases.plate_case(512 // scale, 128 // scale, 64 // scale, noise=0.01)
     if name == "bump":
@@ -39,6 +43,9 @@ def make_case(name, scale=1):
 
 
 WORKLOADS = {
+    "plate_sa": "SA-RANS flat plate 512x128x64 hex, top 20% layers split into prisms, Mach 0.3, Re_L 5e6, nu~ = 3 nu, "
+                "viscous Roe + Spalart-Allmaras (coupled), 2nd-order curvilinear gradients + Venkatakrishnan limiter, "
+                "6 equations/cell",
     "plate": "flat plate 512x128x64 hex, top 20% layers split into prisms, Mach 0.3, laminar viscous Roe, "
              "2nd-order curvilinear gradients + Venkatakrishnan limiter, 5 equations/cell",
     "bump": "3-D bump channel 256x96x64 hex, Mach 0.85, inviscid Roe, curvilinear gradients + Venkatakrishnan",
@@ -50,15 +57,18 @@ WORKLOADS = {
 
 def model_bytes(info, n_grad, has_phi, sw=False):
     """Algorithmic (compulsory) HBM bytes per launch of each kernel: every
-    array element the kernel needs, read or written once."""
-    N, B, F, H = info.n_cells, info.n_ghosts, info.n_faces, info.n_halffaces
+    array element the kernel needs, read or written once (nv = 5, or 6 with
+    the SA model)."""
+    N, B, F, H, nv = info.n_cells, info.n_ghosts, info.n_faces, info.n_halffaces, info.nv
     face_arrays = F * (12 + 24 + 24 + 8 + 24 + 16 + (16 if sw else 0))   # parent/neighbour/bc, n, t1, area, centroid, weights
-    face_flux = (face_arrays + (N + B) * 5 * 8 + N * 15 * 8 + (N * 5 * 8 if has_phi else 0) + (N + B) * 24
-                 + F * 5 * 8)                                              # + write D-C per face
-    cell_sum = H * 4 + (N + 1) * 4 + F * 5 * 8 + N * 5 * 8                 # half-faces, read D-C, write R
-    grad = ((N + B) * 5 * 8 + n_grad * (4 + 24) + (N + 1) * 4 + H * 4 + F * (8 + 24) + N * 24
-            + (N + B) * 15 * 8 + (N * 5 * 8 if has_phi else 0))
-    ghost = B * (4 + 4 + 4 + 24 + 24 + 8 + 5 * 8 + 5 * 8)
+    face_flux = (face_arrays + (N + B) * nv * 8 + N * 3 * nv * 8 + (N * nv * 8 if has_phi else 0) + (N + B) * 24
+                 + F * nv * 8)                                             # + write D-C per face
+    cell_sum = H * 4 + (N + 1) * 4 + F * nv * 8 + N * nv * 8               # half-faces, read D-C, write R
+    if nv == 6:                                                            # SA source: p, T, nu~, 12 gradients, V, d
+        cell_sum += N * (3 + 12 + 2) * 8
+    grad = ((N + B) * nv * 8 + n_grad * (4 + 24) + (N + 1) * 4 + H * 4 + F * (8 + 24) + N * 24
+            + (N + B) * 3 * nv * 8 + (N * nv * 8 if has_phi else 0))
+    ghost = B * (4 + 4 + 4 + 24 + 24 + 8 + nv * 8 + nv * 8)
     return {"ghost": ghost, "grad_lim": grad, "face_flux": face_flux, "cell_sum": cell_sum}
 
 
@@ -82,13 +92,48 @@ def cpu_baseline(name, target_s=12.0):
                       "calc_cell_grads + calc_limiter + residual face loop), %.1f s" % (name, cs.n_cells, n, n * dt)}
 
 
+def newton_compare(ns, cs, steps, cfl=10.0):
+    """Implicit steps from the same initial state with the reference-style FD
+    Jacobian and FP64 GMRES(30), and with the exact tangent (matrix-free) and
+    FP32 GMRES(30); both precondition with the FD diagonal blocks (block
+    Jacobi) and do at most 30 Krylov iterations per step.  Wall time per step
+    and its split, Krylov iterations and ||R||_2 before each step."""
+    import torch
+    from freecfd_amd import _abi as A
+    out = {}
+    for label, jac, fp32 in (("fd_fp64", A.JAC_FD, False), ("jvp_fp32", A.JAC_JVP_FULL, True)):
+        opts = A.newton_opts(cfl=cfl, jacobian=jac, fp32=fp32, restart=30, max_iters=30)
+        ns.set_state(cs.W0)
+        ns.newton_step(opts)          # allocation of the Krylov workspace, untimed
+        ns.set_state(cs.W0)
+        ms, parts, its, hist = [], [], [], []
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = ns.newton_step(opts)
+            torch.cuda.synchronize()
+            ms.append(1e3 * (time.perf_counter() - t0))
+            parts.append([st.ms_residual, st.ms_jacobian, st.ms_krylov, st.ms_update])
+            its.append(st.krylov_iters)
+            hist.append(st.res_norm)
+        p = np.median(np.array(parts), axis=0)
+        out[label] = {"ms_per_step": float(np.median(ms)), "ms_residual": float(p[0]), "ms_jacobian_or_precond": float(p[1]),
+                      "ms_krylov": float(p[2]), "ms_update": float(p[3]), "krylov_iters": its,
+                      "res_norm_before_step": [float(h) for h in hist]}
+        print("bench: newton %s %.1f ms/step, ||R|| %s" % (label, np.median(ms), " ".join("%.3e" % h for h in hist)),
+              file=sys.stderr, flush=True)
+    return {"cfl": cfl, "steps": steps, "restart": 30, "max_krylov_iters": 30, "modes": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="plate", choices=sorted(WORKLOADS))
+    ap.add_argument("--config", default="plate_sa", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--newton-steps", type=int, default=4,
+                    help="implicit steps timed per Jacobian mode (FD/FP64 and JVP/FP32) after the residual; 0 skips")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the residual path is measured on one MI355X); "
@@ -161,6 +206,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": mb[dom], "avg_launch_ms": avg_ms[dom]},
     }
+    if args.newton_steps > 0:
+        line["config"]["newton"] = newton_compare(ns, cs, args.newton_steps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.config)
     print(json.dumps(line), flush=True)
