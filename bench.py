@@ -72,6 +72,21 @@ def model_bytes(info, n_grad, has_phi, sw=False):
     return {"ghost": ghost, "grad_lim": grad, "face_flux": face_flux, "cell_sum": cell_sum}
 
 
+def measured_traffic(config, cells, kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE runs (scaled by a 1 GiB copy, see
+    profiles/residual_<config>_hbm_traffic.md), or None if not measured."""
+    path = os.path.join(ROOT, "profiles", "residual_%s_traffic.json" % config)
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != config or d.get("cells") != cells or kernel not in d.get("kernels", {}):
+        return None
+    return d["kernels"][kernel]["traffic_bytes"]
+
+
 def cpu_baseline(name, target_s=12.0):
     """The CPU restatement (one core) on a bounded sample of the same workload
     family: the same configuration at 1/8 of the mesh size per direction."""
@@ -132,6 +147,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="plate_sa", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (a known-size kernel for HBM counter calibration)")
     ap.add_argument("--newton-steps", type=int, default=4,
                     help="implicit steps timed per Jacobian mode (FD/FP64 and JVP/FP32) after the residual; 0 skips")
     args = ap.parse_args()
@@ -145,6 +162,13 @@ def main():
     from freecfd_amd._lib import lib
     from freecfd_amd.ns import NavierStokes
 
+    if args.copy_calibration:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print("bench: copied 1 GiB device to device (counter calibration)", file=sys.stderr, flush=True)
+        del a, b
     t0 = time.perf_counter()
     cs = make_case(args.config)
     print("bench: mesh %s, %d cells (%.1f s)" % (args.config, cs.n_cells, time.perf_counter() - t0), file=sys.stderr, flush=True)
@@ -203,7 +227,7 @@ def main():
                    "model_bytes_per_eval": total_bytes,
                    "residual_hbm_gbs_model": total_bytes / (ms_step * 1e-3) / 1e9},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, info.n_cells, dom),
                      "algorithmic_bytes_per_launch": mb[dom], "avg_launch_ms": avg_ms[dom]},
     }
     if args.newton_steps > 0:

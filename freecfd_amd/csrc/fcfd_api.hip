@@ -308,8 +308,11 @@ int build_device(fcfd_ctx* c) {
             (rc = upload(c, wny, &dwny)) || (rc = upload(c, wnz, &dwnz)) || (rc = upload(c, is_wall, &dis)))
             return rc;
         launch_wall_distance(T, d.ccx, d.ccy, d.ccz, nw, dwx, dwy, dwz, const_cast<double*>(d.wd), nullptr);
-        launch_face_wall(F, d.fcx, d.fcy, d.fcz, d.fnx, d.fny, d.fnz, dis, nw, dwx, dwy, dwz, dwnx, dwny, dwnz,
-                         const_cast<double*>(d.fwd), const_cast<double*>(d.fdiss), nullptr);
+        // face distances feed only the Steger-Warming dissipation factor
+        const bool faces = c->opt.convective_flux == FCFD_FLUX_SW;
+        if (faces)
+            launch_face_wall(F, d.fcx, d.fcy, d.fcz, d.fnx, d.fny, d.fnz, dis, nw, dwx, dwy, dwz, dwnx, dwny, dwnz,
+                             const_cast<double*>(d.fwd), const_cast<double*>(d.fdiss), nullptr);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipDeviceSynchronize());
         std::vector<double> tw(ld), tf(F), td(F);
@@ -317,7 +320,8 @@ int build_device(fcfd_ctx* c) {
         HIPCHK(c, hipMemcpy(tf.data(), d.fwd, F * sizeof(double), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(td.data(), d.fdiss, F * sizeof(double), hipMemcpyDeviceToHost));
         for (int i = 0; i < T; ++i) g.wdist[g.iperm[i]] = tw[i];
-        for (int i = 0; i < F; ++i) { g.fwd[g.ifperm[i]] = tf[i]; g.fdiss[g.ifperm[i]] = td[i]; }
+        if (faces)
+            for (int i = 0; i < F; ++i) { g.fwd[g.ifperm[i]] = tf[i]; g.fdiss[g.ifperm[i]] = td[i]; }
     }
     c->device = true;
     return FCFD_OK;
@@ -510,6 +514,23 @@ int fcfd_internal_buffers(fcfd_ctx* c, const double** grad, const double** phi) 
     if (!c || !c->device) return FCFD_EINVAL;
     if (grad) *grad = c->G;
     if (phi) *phi = c->PHI;
+    return FCFD_OK;
+}
+
+int fcfd_gradients_to_host(fcfd_ctx* c, double* G_host, double* PHI_host) {
+    if (!c || !c->device) return FCFD_EINVAL;
+    const int N = c->hg.N, B = c->hg.B, ld = c->dg.ld, nv = c->dg.nv;
+    std::vector<double> g((size_t)3 * nv * ld), p((size_t)nv * ld);
+    HIPCHK(c, hipDeviceSynchronize());
+    HIPCHK(c, hipMemcpy(g.data(), c->G, g.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(p.data(), c->PHI, p.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < N + B; ++i) {
+        const int r = c->hg.iperm[i];
+        if (G_host)
+            for (int k = 0; k < 3 * nv; ++k) G_host[(size_t)3 * nv * r + k] = g[(size_t)k * ld + i];
+        if (PHI_host && i < N)
+            for (int v = 0; v < nv; ++v) PHI_host[(size_t)nv * r + v] = p[(size_t)v * ld + i];
+    }
     return FCFD_OK;
 }
 

@@ -10,7 +10,9 @@ namespace fcfd {
 // order.  Cell arrays are padded to `ld` entries; interior cells [0,N), ghosts
 // [N, N+B).  Per-cell variable arrays are [nv][ld] (p, u, v, w, T, and nu~
 // with the SA model), gradients [3 nv][ld], limiters [nv][ld], face fluxes
-// [nv][F].  A half-face entry hf_face[h] is the internal face id f when the
+// [nv][F].  (Structure of arrays: consecutive faces have consecutive parent
+// cells, so a wavefront's parent-side gathers coalesce; per-cell records and
+// face-major flux rows were measured slower.)  A half-face entry hf_face[h] is the internal face id f when the
 // owning cell is the face's parent and ~f when it is the neighbour; the
 // half-faces of a cell are in ascending reference face order, so summing them
 // in order reproduces the reference's VecSetValues(ADD) sequence.
@@ -33,7 +35,7 @@ struct DevGrid {
 
 void launch_ghosts(const DevGrid& g, const Params& P, double* W, int* status, hipStream_t s);
 void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s);
-// face fluxes D - C per face into FX ([5][F]) ...
+// face fluxes D - C per face into FX ([nv][F]) ...
 void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* FX,
                  double* R, int* status, hipStream_t s);
 // ... and their ordered per-cell sums into R (no source: tangents)

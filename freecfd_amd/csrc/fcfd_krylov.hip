@@ -40,14 +40,14 @@ template <typename T>
 __global__ void xmy_kernel(size_t n, const T* __restrict__ b, T* __restrict__ y) {   // y = b - y
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = b[i] - y[i];
 }
-// y -= sum_k h[k] * V_k  (V_k at V + k*stride)
+// y = (y - sum_k h[k] * V_k) * scale  (V_k at V + k*stride)
 template <typename T>
 __global__ void mgs_update_kernel(size_t n, int k, const T* __restrict__ V, size_t stride, const double* __restrict__ h,
-                                  T* __restrict__ y) {
+                                  T scale, T* __restrict__ y) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         T acc = y[i];
         for (int q = 0; q < k; ++q) acc -= (T)h[q] * V[q * stride + i];
-        y[i] = acc;
+        y[i] = acc * scale;
     }
 }
 // x += sum_k c[k] * V_k
@@ -277,17 +277,17 @@ template hipError_t dev_mdot<double>(const DevGrid&, int, const double*, size_t,
 template hipError_t dev_mdot<float>(const DevGrid&, int, const float*, size_t, const float*, double*, double*, double*, hipStream_t);
 
 template <typename T>
-void dev_mgs_update(const DevGrid& g, int k, const T* V, size_t stride, const double* h_dev, T* y, hipStream_t s) {
+void dev_mgs_update(const DevGrid& g, int k, const T* V, size_t stride, const double* h_dev, T scale, T* y, hipStream_t s) {
     size_t n = (size_t)g.nv * g.ld;
-    hipLaunchKernelGGL(mgs_update_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, h_dev, y);
+    hipLaunchKernelGGL(mgs_update_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, h_dev, scale, y);
 }
 template <typename T>
 void dev_lincomb(const DevGrid& g, int k, const T* V, size_t stride, const double* c_dev, T* x, hipStream_t s) {
     size_t n = (size_t)g.nv * g.ld;
     hipLaunchKernelGGL(lincomb_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, c_dev, x);
 }
-template void dev_mgs_update<double>(const DevGrid&, int, const double*, size_t, const double*, double*, hipStream_t);
-template void dev_mgs_update<float>(const DevGrid&, int, const float*, size_t, const double*, float*, hipStream_t);
+template void dev_mgs_update<double>(const DevGrid&, int, const double*, size_t, const double*, double, double*, hipStream_t);
+template void dev_mgs_update<float>(const DevGrid&, int, const float*, size_t, const double*, float, float*, hipStream_t);
 template void dev_lincomb<double>(const DevGrid&, int, const double*, size_t, const double*, double*, hipStream_t);
 template void dev_lincomb<float>(const DevGrid&, int, const float*, size_t, const double*, float*, hipStream_t);
 
@@ -365,20 +365,29 @@ int gmres(const DevGrid& g, const LinOp<T>& A, const LinOp<T>& M, const T* b, T*
             T* vn = V + (size_t)(j + 1) * stride;
             M.apply(vj, z, s);
             A.apply(z, vn, s);
-            // classical Gram-Schmidt, twice (CGS2): one multi-dot per pass
+            // classical Gram-Schmidt with selective reorthogonalisation (DGKS,
+            // eta = 1/sqrt(2)): one multi-dot per pass returns <V_q, w> and,
+            // as its last row, ||w||^2 (w is V_{j+1}); the norm after the
+            // projection is ||w||^2 - sum h_q^2, so the normalisation rides on
+            // the update kernel.  A second pass runs only when the projection
+            // removed more than half of ||w||^2.
+            double hn = 0.0;
             for (int pass = 0; pass < 2; ++pass) {
-                if (dev_mdot(g, j + 1, V, stride, vn, w.part, w.out_dev, h.data(), s) != hipSuccess) return FCFD_EHIP;
+                if (dev_mdot(g, j + 2, V, stride, vn, w.part, w.out_dev, h.data(), s) != hipSuccess) return FCFD_EHIP;
+                const double nn = h[j + 1];
+                double hs = 0.0;
+                for (int q = 0; q <= j; ++q) hs += h[q] * h[q];
+                const double est = nn - hs;
+                const bool again = pass == 0 && !(est > 0.5 * nn);
+                hn = (!again && est > 0.0) ? std::sqrt(est) : 0.0;
                 if (hipMemcpyAsync(w.coef_dev, h.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
                     hipStreamSynchronize(s) != hipSuccess)
                     return FCFD_EHIP;
-                dev_mgs_update(g, j + 1, V, stride, w.coef_dev, vn, s);
+                dev_mgs_update(g, j + 1, V, stride, w.coef_dev, (T)(hn > 0.0 ? 1.0 / hn : 1.0), vn, s);
                 for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] += h[q];
+                if (!again) break;
             }
-            double hn2 = 0.0;
-            if (dev_mdot(g, 1, vn, stride, vn, w.part, w.out_dev, &hn2, s) != hipSuccess) return FCFD_EHIP;
-            double hn = std::sqrt(hn2);
             H[(size_t)(j + 1) * m + j] = hn;
-            if (hn > 0.0) dev_scale_copy(n, (T)(1.0 / hn), vn, vn, s);
             // Givens rotations
             for (int q = 0; q < j; ++q) {
                 double a = H[(size_t)q * m + j], bq = H[(size_t)(q + 1) * m + j];

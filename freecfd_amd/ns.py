@@ -115,16 +115,10 @@ class NavierStokes(Grid):
 
     def gradients_limiters_host(self):
         """Gradients ((N+B) x 3nv) and limiters (N x nv) of the last residual."""
-        g = C.c_void_p()
-        p = C.c_void_p()
-        _check(self.h, lib().fcfd_internal_buffers(self.h, C.byref(g), C.byref(p)))
-        self.torch.cuda.synchronize()
-        n = self.info.n_cells
-        nv = self.nv
+        n, nv = self.info.n_cells, self.nv
         G = np.zeros((n + self.info.n_ghosts, 3 * nv))
         PHI = np.zeros((n, nv))
-        _check(self.h, lib().fcfd_cells_to_host(self.h, g, 3 * nv, 1, G.ctypes.data))
-        _check(self.h, lib().fcfd_cells_to_host(self.h, p, nv, 0, PHI.ctypes.data))
+        _check(self.h, lib().fcfd_gradients_to_host(self.h, G.ctypes.data, PHI.ctypes.data))
         return G, PHI
 
     def fd_jacobian(self, W=None, dW_prev=None, dt=None, stream=None):

@@ -212,19 +212,23 @@ int  fcfd_cells_to_host(fcfd_ctx* ctx, const double* X_dev, int nvar, int includ
 
 /* FP64 residual R(W): ghost update (update_boundaries, decided semantics),
  * gradients (calc_cell_grads), limiters (calc_limiter) and the face-flux
- * loop (ns_assemble_linear_system.cc:75-140).  W_dev is [5][ld_cell]; its
+ * loop (ns_assemble_linear_system.cc:75-140).  W_dev is [nv][ld_cell]; its
  * interior slots [0,N) are read and its boundary-ghost slots [N,N+B) are
  * written with the ghost state (the reference keeps ghosts in the same
- * Variable arrays, variable.h:84-89).  R_dev is [5][ld_cell], interior cells:
+ * Variable arrays, variable.h:84-89).  R_dev is [nv][ld_cell], interior cells:
  * R_c = sum over the faces of c, in reference face order, of +-(D - C).
  * Stream-ordered on `stream` (a hipStream_t, or NULL for the null stream);
  * no host synchronisation. */
 int  fcfd_residual(fcfd_ctx* ctx, double* W_dev, double* R_dev, void* stream);
 
-/* Gradients [15][ld_cell] (dp/dx,dp/dy,dp/dz, du/.., dv/.., dw/.., dT/..,
- * ghosts carry their parent's) and limiters [5][ld_cell] of the last
- * fcfd_residual call; device pointers owned by the context. */
+/* Gradients [3nv][ld_cell] (dp/dx,dp/dy,dp/dz, du/.., dv/.., dw/.., dT/..
+ * [, dnu~/..]; ghosts carry their parent's) and limiters [nv][ld_cell] of the
+ * last fcfd_residual call; device pointers owned by the context. */
 int  fcfd_internal_buffers(fcfd_ctx* ctx, const double** grad, const double** phi);
+
+/* The same, unpacked to the host in reference numbering: G_host (N+B) x 3nv,
+ * PHI_host N x nv (either may be NULL).  Synchronises the device. */
+int  fcfd_gradients_to_host(fcfd_ctx* ctx, double* G_host, double* PHI_host);
 
 /* Per-stage timing of fcfd_residual with HIP events recorded on the launch
  * stream around its four stages (ghosts, gradients+limiters, face fluxes,
