@@ -140,6 +140,61 @@ def newton_compare(ns, cs, steps, cfl=10.0):
     return {"cfl": cfl, "steps": steps, "restart": 30, "max_krylov_iters": 30, "modes": out}
 
 
+RACES = {
+    "cylinder64": ("laminar cylinder O-grid 64x32x1 (Mach 0.2, Re_D 40, AUSM+up, Sutherland, lsqr)",
+                   lambda: __import__("freecfd_amd.cases", fromlist=["x"]).cylinder_case(64, 32)),
+    "cylinder128": ("laminar cylinder O-grid 128x64x1 (Mach 0.2, Re_D 40, AUSM+up, Sutherland, lsqr)",
+                    lambda: __import__("freecfd_amd.cases", fromlist=["x"]).cylinder_case(128, 64)),
+}
+
+
+def race(name, cfl=1000.0, drop=1e-10, max_steps=200):
+    """Implicit solve to a `drop` residual reduction from the freestream with
+    (a) the reference-style FD Jacobian and FP64 GMRES and (b) the exact
+    tangent, matrix-free, with FP32 GMRES; both preconditioned by one
+    multicolour block-SGS sweep on the FD Jacobian blocks, GMRES(30) capped at
+    100 iterations per step, constant local CFL.  Reports wall time to the drop
+    and the surface loads of both converged states against the CPU restatement
+    evaluated on the JVP/FP32 state."""
+    import torch
+    from freecfd_amd import _abi as A
+    from freecfd_amd.ns import NavierStokes
+    from oracle.oracle import Oracle
+    label, make = RACES[name]
+    cs = make()
+    ns = NavierStokes(cs.raw, cs.opt)
+    modes = {}
+    states = {}
+    for mode, jac, fp32 in (("fd_fp64", A.JAC_FD, False), ("jvp_fp32", A.JAC_JVP_FULL, True)):
+        ns.set_state(cs.W0)
+        opts = A.newton_opts(cfl=cfl, jacobian=jac, fp32=fp32, restart=30, max_iters=100, rtol=1e-4,
+                             precond=A.PC_BLOCK_SGS, sweeps=1)
+        r = ns.solve(opts, max_steps=max_steps, drop=drop)
+        loads = ns.loads()
+        states[mode] = ns.state_host(include_ghosts=False)
+        modes[mode] = {"status": r["status"], "steps": r["steps"], "seconds": r["total_ms"] / 1e3, "drop": r["drop"],
+                       "krylov_iters": int(np.sum(r["krylov_iters"])), "loads": loads.tolist()}
+        print("bench: race %s %s: %s in %d steps, %.3f s, drop %.2e" % (name, mode, r["status"], r["steps"],
+                                                                          r["total_ms"] / 1e3, r["drop"]),
+              file=sys.stderr, flush=True)
+    o = Oracle(cs.raw, cs.opt)
+    o.residual(states["jvp_fp32"])
+    ref = o.loads()
+    wall = int(np.argmax(np.abs(ref).sum(axis=1)))
+    for mode in modes:
+        got = np.array(modes[mode]["loads"])
+        modes[mode]["load_err_vs_oracle"] = float(np.abs(got[wall, :3] - ref[wall, :3]).max() / np.abs(ref[wall, :3]).max())
+    fd, jv = modes["fd_fp64"], modes["jvp_fp32"]
+    both = fd["status"] == "converged" and jv["status"] == "converged"
+    return {"metric": "wall time to a 1e-10 residual drop", "value": jv["seconds"], "unit": "s", "n_gpus": 1,
+            "higher_is_better": False, "vs_baseline": None, "dtype": "f64 residual/update, f32 Krylov (jvp_fp32)",
+            "data": "synthetic", "config": {"workload": label, "cells": cs.n_cells, "cfl": cfl, "modes": modes,
+                                            "speedup_fd_over_jvp": (fd["seconds"] / jv["seconds"]) if both else None,
+                                            "drag_lift_fd_vs_jvp": float(np.abs(np.array(fd["loads"])[wall, :3] -
+                                                                                np.array(jv["loads"])[wall, :3]).max() /
+                                                                         np.abs(np.array(jv["loads"])[wall, :3]).max())}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
@@ -149,6 +204,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy first (a known-size kernel for HBM counter calibration)")
+    ap.add_argument("--race", default=None, choices=sorted(RACES),
+                    help="instead of the residual benchmark: Newton to a 1e-10 residual drop with FD/FP64 and "
+                         "JVP/FP32 on this case, one JSON line")
     ap.add_argument("--newton-steps", type=int, default=4,
                     help="implicit steps timed per Jacobian mode (FD/FP64 and JVP/FP32) after the residual; 0 skips")
     args = ap.parse_args()
@@ -161,6 +219,10 @@ def main():
     from freecfd_amd import _abi as A
     from freecfd_amd._lib import lib
     from freecfd_amd.ns import NavierStokes
+
+    if args.race:
+        print(json.dumps(race(args.race)), flush=True)
+        return
 
     if args.copy_calibration:
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB

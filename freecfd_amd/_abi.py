@@ -68,7 +68,7 @@ JAC_FD, JAC_JVP_FROZEN, JAC_JVP_FULL = 1, 2, 3
 class NewtonOpts(C.Structure):
     _fields_ = [("dt_type", C.c_int), ("cfl", C.c_double), ("jacobian", C.c_int), ("krylov_fp32", C.c_int),
                 ("restart", C.c_int), ("max_iters", C.c_int), ("rtol", C.c_double), ("abstol", C.c_double),
-                ("knoll_guess", C.c_int)]
+                ("knoll_guess", C.c_int), ("precond", C.c_int), ("sweeps", C.c_int)]
 
 
 class NewtonStats(C.Structure):
@@ -78,10 +78,14 @@ class NewtonStats(C.Structure):
                 ("ms_update", C.c_double), ("nut_norm", C.c_double)]
 
 
-def newton_opts(cfl=10.0, jacobian=JAC_FD, fp32=False, restart=30, max_iters=10, rtol=1e-6, abstol=1e-12, knoll=True):
+PC_BLOCK_JACOBI, PC_BLOCK_SGS = 0, 1
+
+
+def newton_opts(cfl=10.0, jacobian=JAC_FD, fp32=False, restart=30, max_iters=10, rtol=1e-6, abstol=1e-12, knoll=True,
+                precond=PC_BLOCK_JACOBI, sweeps=1):
     """Defaults follow the reference (read_inputs.cc:125-127, ns_petsc_functions.cc:74-79)."""
     return NewtonOpts(dt_type=DT_CFL_LOCAL, cfl=cfl, jacobian=jacobian, krylov_fp32=int(fp32), restart=restart,
-                      max_iters=max_iters, rtol=rtol, abstol=abstol, knoll_guess=int(knoll))
+                      max_iters=max_iters, rtol=rtol, abstol=abstol, knoll_guess=int(knoll), precond=precond, sweeps=sweeps)
 
 
 class MetricsOut(C.Structure):

@@ -43,6 +43,8 @@ struct fcfd_ctx {
     float *Dinv32 = nullptr, *D32 = nullptr, *JL32 = nullptr, *JR32 = nullptr, *b32 = nullptr, *x32 = nullptr;
     // tangent workspace (lazy)
     double *dG = nullptr, *dPHI = nullptr, *dFX = nullptr, *jv_in = nullptr, *jv_out = nullptr, *jv_R = nullptr;
+    // cell colouring for the block Gauss-Seidel preconditioner (lazy)
+    Colors colors;
     // adjoint and loads (lazy)
     bool adj_ready = false;
     AdjGrid ag{};
@@ -730,6 +732,48 @@ int fcfd_jvp(fcfd_ctx* c, double* W_dev, double* dW_dev, double* dR_dev, int mod
     return FCFD_OK;
 }
 
+// Greedy colouring of the interior cells on the face graph, in internal (RCM)
+// order: each cell takes the smallest colour none of its face neighbours has.
+static int ensure_colors(fcfd_ctx* c) {
+    if (c->colors.n > 0) return FCFD_OK;
+    const HostGrid& g = c->hg;
+    const int N = g.N;
+    std::vector<int> hp, hf;
+    half_faces(g, hp, hf);
+    std::vector<int> par(g.F), nei(g.F);
+    for (int i = 0; i < g.F; ++i) {
+        int f = g.ifperm[i];
+        par[i] = g.perm[g.fparent[f]];
+        nei[i] = g.perm[g.fneigh[f]];
+    }
+    std::vector<int> col(N, -1);
+    int ncol = 0;
+    std::vector<char> used;
+    for (int i = 0; i < N; ++i) {
+        used.assign(ncol + 1, 0);
+        for (int h = hp[i]; h < hp[i + 1]; ++h) {
+            int fs = hf[h];
+            int f = fs >= 0 ? fs : ~fs;
+            int o = fs >= 0 ? nei[f] : par[f];
+            if (o < N && col[o] >= 0) used[col[o]] = 1;
+        }
+        int k = 0;
+        while (k < ncol && used[k]) ++k;
+        col[i] = k;
+        if (k == ncol) ++ncol;
+    }
+    std::vector<int> ptr(ncol + 1, 0), cells(N);
+    for (int i = 0; i < N; ++i) ptr[col[i] + 1]++;
+    for (int k = 0; k < ncol; ++k) ptr[k + 1] += ptr[k];
+    std::vector<int> next(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < N; ++i) cells[next[col[i]]++] = i;
+    int rc = upload(c, cells, &c->colors.cells);
+    if (rc) return rc;
+    c->colors.ptr = ptr;
+    c->colors.n = ncol;
+    return FCFD_OK;
+}
+
 int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd_newton_stats* st, void* stream) {
     if (!c || !W_dev || !o) return FCFD_EINVAL;
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
@@ -740,6 +784,10 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     const bool mfree = o->jacobian != FCFD_JAC_FD;
     if (mfree && (rc0 = ensure_tangent(c))) return rc0;
     if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad Krylov options");
+    if (o->precond != FCFD_PC_BLOCK_JACOBI && o->precond != FCFD_PC_BLOCK_SGS) return fail(c, FCFD_EINVAL, "unknown preconditioner");
+    const bool sgs = o->precond == FCFD_PC_BLOCK_SGS;
+    const int sweeps = std::max(o->sweeps, 1);
+    if (sgs && (rc0 = ensure_colors(c))) return rc0;
     const DevGrid& g = c->dg;
     hipStream_t s = (hipStream_t)stream;
     const size_t n5 = (size_t)g.nv * g.ld;      // one [nv][ld] vector
@@ -795,7 +843,7 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     launch_fd_jacobian(g, c->P, W_dev, c->G, c->PHI, c->upd, c->dt, c->JL, c->JR, c->Dg, c->status, s);
     if (fp32) {
         dev_bj_invert<float>(g, c->Dg, c->Dinv32, c->status, s);
-        if (!mfree) {
+        if (!mfree || sgs) {
             dev_convert<double, float>(b2 * g.ld, c->Dg, c->D32, s);
             dev_convert<double, float>(b2 * g.F, c->JL, c->JL32, s);
             dev_convert<double, float>(b2 * g.F, c->JR, c->JR32, s);
@@ -824,21 +872,33 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
             jvp_op(c->jv_out, c->jv_out, ss);
             dev_convert<double, float>(n5, c->jv_out, y, ss);
         }};
-        LinOp<float> M{[&](const float* x, float* y, hipStream_t ss) { dev_bj_apply<float>(g, c->Dinv32, x, y, ss); }};
+        LinOp<float> M{[&](const float* x, float* y, hipStream_t ss) {
+            if (sgs) dev_bsgs_apply<float>(g, c->colors, c->Dinv32, c->JL32, c->JR32, x, y, sweeps, ss);
+            else dev_bj_apply<float>(g, c->Dinv32, x, y, ss);
+        }};
         rc = gmres<float>(g, A, M, c->b32, c->x32, go, c->w32, gs, s);
         if (!rc) dev_convert<float, double>(n5, c->x32, c->dW, s);
     } else if (mfree) {
         LinOp<double> A{[&](const double* x, double* y, hipStream_t ss) { jvp_op(x, y, ss); }};
-        LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) { dev_bj_apply<double>(g, c->Dinv64, x, y, ss); }};
+        LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) {
+            if (sgs) dev_bsgs_apply<double>(g, c->colors, c->Dinv64, c->JL, c->JR, x, y, sweeps, ss);
+            else dev_bj_apply<double>(g, c->Dinv64, x, y, ss);
+        }};
         rc = gmres<double>(g, A, M, c->R, c->dW, go, c->w64, gs, s);
     } else if (fp32) {
         LinOp<float> A{[&](const float* x, float* y, hipStream_t ss) { dev_bsr_matvec<float>(g, c->D32, c->JL32, c->JR32, x, y, ss); }};
-        LinOp<float> M{[&](const float* x, float* y, hipStream_t ss) { dev_bj_apply<float>(g, c->Dinv32, x, y, ss); }};
+        LinOp<float> M{[&](const float* x, float* y, hipStream_t ss) {
+            if (sgs) dev_bsgs_apply<float>(g, c->colors, c->Dinv32, c->JL32, c->JR32, x, y, sweeps, ss);
+            else dev_bj_apply<float>(g, c->Dinv32, x, y, ss);
+        }};
         rc = gmres<float>(g, A, M, c->b32, c->x32, go, c->w32, gs, s);
         if (!rc) dev_convert<float, double>(n5, c->x32, c->dW, s);
     } else {
         LinOp<double> A{[&](const double* x, double* y, hipStream_t ss) { dev_bsr_matvec<double>(g, c->Dg, c->JL, c->JR, x, y, ss); }};
-        LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) { dev_bj_apply<double>(g, c->Dinv64, x, y, ss); }};
+        LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) {
+            if (sgs) dev_bsgs_apply<double>(g, c->colors, c->Dinv64, c->JL, c->JR, x, y, sweeps, ss);
+            else dev_bj_apply<double>(g, c->Dinv64, x, y, ss);
+        }};
         rc = gmres<double>(g, A, M, c->R, c->dW, go, c->w64, gs, s);
     }
     if (rc) return fail(c, rc, "GMRES failed (HIP error)");

@@ -1,6 +1,7 @@
 // Krylov solver and implicit-operator building blocks (fcfd_krylov.hip).
 #pragma once
 #include <functional>
+#include <vector>
 
 #include "fcfd_device.h"
 
@@ -35,6 +36,16 @@ hipError_t dev_mdot(const DevGrid& g, int k, const T* V, size_t stride, const T*
 template <typename T> void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s);
 template <typename T> void dev_bj_invert(const DevGrid& g, const double* D, T* Dinv, int* status, hipStream_t s);
 template <typename T> void dev_bj_apply(const DevGrid& g, const T* Dinv, const T* x, T* y, hipStream_t s);
+// cell colouring of the face graph: colour k holds cells[ptr[k] .. ptr[k+1]) (device), ptr on the host
+struct Colors {
+    int n = 0;
+    std::vector<int> ptr;
+    const int* cells = nullptr;
+};
+// x = M^{-1} b by `sweeps` symmetric multicolour block Gauss-Seidel sweeps from x = 0
+template <typename T>
+void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T* JL, const T* JR, const T* b, T* x, int sweeps,
+                    hipStream_t s);
 // out_host: sum dp^2/dt^2, sum |dV|^2/dt^2, sum dT^2/dt^2, NaN/Inf count, sum dnu~^2/dt^2
 hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
                       double out_host[5], hipStream_t s);

@@ -233,6 +233,45 @@ __global__ void __launch_bounds__(256) bj_apply_kernel(int N, size_t ld, const T
     }
 }
 
+// one colour of a multicolour block Gauss-Seidel sweep: for every cell of the
+// colour, x_c = D_c^{-1} (b_c - sum over its faces of the off-diagonal block
+// times x of the neighbour), with the latest x of the other colours (cells of
+// one colour share no face, so the update is race-free)
+template <typename T, int NV>
+__global__ void __launch_bounds__(256) bsgs_color_kernel(DevGrid g, const T* __restrict__ Dinv, const T* __restrict__ JL,
+                                                         const T* __restrict__ JR, const T* __restrict__ b, T* __restrict__ x,
+                                                         const int* __restrict__ cells, int count) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int c = cells[t];
+    const size_t ld = g.ld, Fs = g.F;
+    T acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = b[j * ld + c];
+    for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
+        int fs = g.hf_face[h];
+        int f = fs >= 0 ? fs : ~fs;
+        int o = fs >= 0 ? g.fnei[f] : g.fpar[f];
+        if (o >= g.N) continue;
+        T xo[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) xo[i] = x[i * ld + o];
+        const T* J = fs >= 0 ? JR : JL;
+        const T sg = fs >= 0 ? T(-1) : T(1);
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int j = 0; j < NV; ++j) acc[j] -= sg * J[(NV * i + j) * Fs + f] * xo[i];
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        T s = 0;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) s += Dinv[(NV * j + i) * ld + c] * acc[i];
+        x[j * ld + c] = s;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -313,6 +352,30 @@ template void dev_bj_invert<double>(const DevGrid&, const double*, double*, int*
 template void dev_bj_invert<float>(const DevGrid&, const double*, float*, int*, hipStream_t);
 template void dev_bj_apply<double>(const DevGrid&, const double*, const double*, double*, hipStream_t);
 template void dev_bj_apply<float>(const DevGrid&, const float*, const float*, float*, hipStream_t);
+
+template <typename T>
+void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T* JL, const T* JR, const T* b, T* x, int sweeps,
+                    hipStream_t s) {
+    const size_t n = (size_t)g.nv * g.ld;
+    hipLaunchKernelGGL(scale_copy_kernel<T>, ew_grid(n), dim3(256), 0, s, n, T(0), b, x);   // x = 0
+    auto one = [&](int k) {
+        const int cnt = col.ptr[k + 1] - col.ptr[k];
+        if (cnt <= 0) return;
+        const int* cells = col.cells + col.ptr[k];
+        if (g.nv == 6)
+            hipLaunchKernelGGL((bsgs_color_kernel<T, 6>), dim3((cnt + 255) / 256), dim3(256), 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
+        else
+            hipLaunchKernelGGL((bsgs_color_kernel<T, 5>), dim3((cnt + 255) / 256), dim3(256), 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
+    };
+    for (int sw = 0; sw < sweeps; ++sw) {
+        for (int k = 0; k < col.n; ++k) one(k);        // forward
+        for (int k = col.n - 1; k >= 0; --k) one(k);   // backward
+    }
+}
+template void dev_bsgs_apply<double>(const DevGrid&, const Colors&, const double*, const double*, const double*, const double*, double*,
+                                     int, hipStream_t);
+template void dev_bsgs_apply<float>(const DevGrid&, const Colors&, const float*, const float*, const float*, const float*, float*, int,
+                                    hipStream_t);
 
 hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
                       double out_host[5], hipStream_t s) {

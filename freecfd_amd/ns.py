@@ -187,6 +187,42 @@ class NavierStokes(Grid):
             _check(self.h, rc)
         return st if check else (rc, st)
 
+    def solve(self, opts=None, max_steps=500, drop=1e-10, cfl_initial=None, cfl_growth=1.0, cfl_target=None,
+              W=None, callback=None):
+        """Implicit pseudo-time iterations to a residual drop (the step loop of
+        main.cc/NavierStokes::solve with update_time_step's CFL ramp,
+        time_step.cc:143-158: CFL_1 = cfl_initial, CFL_k = min(CFL_{k-1} *
+        growth, target)).  Stops when ||R_k||_2 <= drop * ||R_0||_2 (R before
+        step k), on divergence, or after max_steps.  Returns a dict with the
+        residual history, per-step wall times (ms) and the final drop."""
+        import time as _time
+        opts = A.newton_opts() if opts is None else opts
+        cfl = opts.cfl if cfl_initial is None else cfl_initial
+        target = cfl if cfl_target is None else cfl_target
+        hist, ms, iters = [], [], []
+        status = "max_steps"
+        for k in range(max_steps):
+            if k > 0:
+                cfl = min(cfl * cfl_growth, target)
+            opts.cfl = cfl
+            self.torch.cuda.synchronize()
+            t0 = _time.perf_counter()
+            rc, st = self.newton_step(opts, W=W, check=False)
+            self.torch.cuda.synchronize()
+            ms.append(1e3 * (_time.perf_counter() - t0))
+            hist.append(st.res_norm)
+            iters.append(st.krylov_iters)
+            if callback is not None:
+                callback(k, st, cfl)
+            if rc != 0 or not np.isfinite(st.res_norm):
+                status = "diverged"
+                break
+            if st.res_norm <= drop * hist[0]:
+                status = "converged"
+                break
+        return dict(status=status, res_norm=hist, ms=ms, krylov_iters=iters, steps=len(hist),
+                    drop=hist[-1] / hist[0] if hist else 1.0, total_ms=float(np.sum(ms)))
+
     def time_step_host(self, cfl):
         dt = self.torch.zeros(self.ld, dtype=self.torch.float64, device=self.device)
         mc = C.c_double()
