@@ -64,10 +64,11 @@ def model_bytes(info, n_grad, has_phi, sw=False):
     face_flux = (face_arrays + (N + B) * nv * 8 + N * 3 * nv * 8 + (N * nv * 8 if has_phi else 0) + (N + B) * 24
                  + F * nv * 8)                                             # + write D-C per face
     cell_sum = H * 4 + (N + 1) * 4 + F * nv * 8 + N * nv * 8               # half-faces, read D-C, write R
-    if nv == 6:                                                            # SA source: p, T, nu~, 12 gradients, V, d
-        cell_sum += N * (3 + 12 + 2) * 8
     grad = ((N + B) * nv * 8 + n_grad * (4 + 24) + (N + 1) * 4 + H * 4 + F * (8 + 24) + N * 24
             + (N + B) * 3 * nv * 8 + (N * nv * 8 if has_phi else 0))
+    if nv == 6:   # SA source V S: grad_lim reads V and d and writes it, cell_sum reads it
+        grad += N * 3 * 8
+        cell_sum += N * 8
     ghost = B * (4 + 4 + 4 + 24 + 24 + 8 + nv * 8 + nv * 8)
     return {"ghost": ghost, "grad_lim": grad, "face_flux": face_flux, "cell_sum": cell_sum}
 

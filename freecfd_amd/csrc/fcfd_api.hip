@@ -32,6 +32,7 @@ struct fcfd_ctx {
     DevGrid dg{};
     std::vector<void*> allocs;
     double *G = nullptr, *PHI = nullptr, *dref = nullptr, *FX = nullptr;
+    double* SRC = nullptr;   // [ld] V S of the SA source (nv = 6), from grad_lim
     int* status = nullptr;
     double *JL = nullptr, *JR = nullptr, *Dg = nullptr, *zero_upd = nullptr;   // FD Jacobian (lazy)
     // Newton-Krylov workspace (lazy)
@@ -290,6 +291,7 @@ int build_device(fcfd_ctx* c) {
             (rc = dalloc(c, (size_t)kDrefSize, &c->dref)) || (rc = dalloc(c, 4, &c->status)) ||
             (rc = dalloc(c, nv * g.F, &c->FX)))
             return rc;
+        if (nv == 6 && (rc = dalloc(c, ld, &c->SRC))) return rc;
     }
     if (c->gpu_wall_distance) {   // brute force on the device, then mirrored into the host metrics
         std::vector<double> wx, wy, wz, wnx, wny, wnz;
@@ -364,7 +366,11 @@ int create_impl(const fcfd_mesh* mesh, const fcfd_options* opt, fcfd_ctx** out, 
         c->gpu_wall_distance = !host_wd && device;
     }
     if (!rc) {
-        renumber(c->hg);
+        // internal cell order: space-filling curve (default) or reverse
+        // Cuthill-McKee (FCFD_CELL_ORDER=rcm); results do not depend on it
+        int order = kOrderSfc;
+        if (const char* e = std::getenv("FCFD_CELL_ORDER")) order = std::strcmp(e, "rcm") == 0 ? kOrderRcm : kOrderSfc;
+        renumber(c->hg, order);
         set_params(c);
         if (device) rc = build_device(c);
     }
@@ -475,11 +481,11 @@ int fcfd_residual(fcfd_ctx* c, double* W_dev, double* R_dev, void* stream) {
     }
     launch_ghosts(c->dg, c->P, W_dev, c->status, s);
     if (e) HIPCHK(c, hipEventRecord(e[1], s));
-    launch_grad_lim(c->dg, c->P, W_dev, c->G, c->PHI, c->dref, s);
+    launch_grad_lim(c->dg, c->P, W_dev, c->G, c->PHI, c->dref, s, c->SRC);
     if (e) HIPCHK(c, hipEventRecord(e[2], s));
     launch_flux(c->dg, c->P, W_dev, c->G, c->PHI, c->FX, R_dev, c->status, s);
     if (e) HIPCHK(c, hipEventRecord(e[3], s));
-    launch_residual_sum(c->dg, c->P, W_dev, c->G, c->FX, R_dev, s);
+    launch_residual_sum(c->dg, c->SRC, c->FX, R_dev, s);
     if (e) {
         HIPCHK(c, hipEventRecord(e[4], s));
         c->ring_head = (c->ring_head + 1) % fcfd_ctx::kRing;
@@ -831,9 +837,9 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     auto t0 = clk::now();
     // 1. residual and its norm
     launch_ghosts(g, c->P, W_dev, c->status, s);
-    launch_grad_lim(g, c->P, W_dev, c->G, c->PHI, c->dref, s);
+    launch_grad_lim(g, c->P, W_dev, c->G, c->PHI, c->dref, s, c->SRC);
     launch_flux(g, c->P, W_dev, c->G, c->PHI, c->FX, c->R, c->status, s);
-    launch_residual_sum(g, c->P, W_dev, c->G, c->FX, c->R, s);
+    launch_residual_sum(g, c->SRC, c->FX, c->R, s);
     double rr = 0.0;
     HIPCHK(c, dev_mdot(g, 1, c->R, n5, c->R, c->red_part, c->red_out, &rr, s));
     S.res_norm = std::sqrt(rr);

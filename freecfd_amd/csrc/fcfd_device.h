@@ -33,16 +33,32 @@ struct DevGrid {
     const BcDev* bc;
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2: workgroup b runs on XCD b % 8.  xcd_block() renumbers them so that XCD x
+// takes the contiguous range [x q + min(x, r), ...) of logical blocks (q, r =
+// gridDim / 8, gridDim % 8): faces and cells that are close in the (RCM)
+// numbering, and so share gathered cell data, then share an L2.  A bijection
+// of [0, gridDim) for every grid size.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+    const int n = gridDim.x, b = blockIdx.x;
+    const int q = n / kXcds, r = n % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 void launch_ghosts(const DevGrid& g, const Params& P, double* W, int* status, hipStream_t s);
-void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s);
+// gradients and limiters; with the SA model and src != nullptr also V_c S_c
+// (the nu~ source of each interior cell, from the fresh gradients) into src[ld]
+void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s,
+                     double* src = nullptr);
 // face fluxes D - C per face into FX ([nv][F]) ...
 void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* FX,
                  double* R, int* status, hipStream_t s);
 // ... and their ordered per-cell sums into R (no source: tangents)
 void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s);
-// the residual's per-cell sums, plus V S in the nu~ row with the SA model
-void launch_residual_sum(const DevGrid& g, const Params& P, const double* W, const double* G, const double* FX, double* R,
-                         hipStream_t s);
+// the residual's per-cell sums, plus V S (src, from launch_grad_lim) in the
+// nu~ row with the SA model
+void launch_residual_sum(const DevGrid& g, const double* src, const double* FX, double* R, hipStream_t s);
 // VK reference range only (find_min_max), when the limiter threshold needs it
 void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double* dref, hipStream_t s);
 constexpr int kDrefSize = 8 + 10 * 1024;   // dref[nv] + VK min/max partials

@@ -40,7 +40,9 @@ struct HostGrid {
 
 int build_host_grid(const fcfd_mesh* m, const fcfd_options* o, HostGrid& g, std::string& err,
                     bool wall_distance_on_host);
-void renumber(HostGrid& g);
+// internal cell numbering: reverse Cuthill-McKee or space-filling curve
+constexpr int kOrderRcm = 0, kOrderSfc = 1;
+void renumber(HostGrid& g, int cell_order);
 
 // Material/numerics constants passed by value to kernels.
 struct Params {

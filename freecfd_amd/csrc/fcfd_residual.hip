@@ -27,7 +27,7 @@ namespace fcfd {
 // --------------------------------------------------------------------------
 template <int NV>
 __global__ void __launch_bounds__(256) ghost_kernel(DevGrid g, Params P, double* __restrict__ W, int* __restrict__ status) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= g.B) return;
     int f = g.gh_face[i];
     int c = g.fpar[f];
@@ -61,8 +61,8 @@ __global__ void __launch_bounds__(256) ghost_kernel(DevGrid g, Params P, double*
 template <int LIM, int NV>
 __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, const double* __restrict__ W,
                                                        double* __restrict__ G, double* __restrict__ PHI,
-                                                       const double* __restrict__ dref) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+                                                       const double* __restrict__ dref, double* __restrict__ SRC) {
+    int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
     double gr[3 * NV];
@@ -89,6 +89,14 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
     double q0[NV], phi[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) { q0[v] = W[v * ld + c]; phi[v] = 1.; }
+    if constexpr (NV == 6) {
+        // V_c S_c of the SA source (ns_sources.cc:25-30 has none; build-only model)
+        if (SRC) {
+            auto gv = [&](int v) { return mk<double>(gr[3 * v], gr[3 * v + 1], gr[3 * v + 2]); };
+            SRC[c] = g.vol[c] * sa_source<double>(m_rho(P, q0[0], q0[4]), m_visc(P, q0[4]), q0[5], gv(1), gv(2), gv(3), gv(5),
+                                                  g.wd[c]);
+        }
+    }
     Vec<double> xc = centroid(g, c);
     if (LIM == FCFD_LIMITER_MINMOD) {
         const double small = 1.e-12;
@@ -242,7 +250,7 @@ __global__ void __launch_bounds__(256) face_flux_kernel(DevGrid g, Params P, con
                                                         const double* __restrict__ G, const double* __restrict__ PHI,
                                                         int has_phi, double phi_const, double* __restrict__ FX,
                                                         int* __restrict__ status) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
     FaceCtx<double, NV> X;
     CellLoader<NV> ld_{W, G, PHI, has_phi != 0, phi_const, (size_t)g.ld};
@@ -252,24 +260,15 @@ __global__ void __launch_bounds__(256) face_flux_kernel(DevGrid g, Params P, con
     for (int v = 0; v < NV; ++v) FX[v * Fs + f] = X.D[v] - X.C[v];
 }
 
-// SA source of cell c per unit volume, from the cell state, its gradients and
-// its wall distance
-__device__ __forceinline__ double cell_sa_source(const DevGrid& g, const Params& P, const double* __restrict__ W,
-                                                 const double* __restrict__ G, int c) {
-    const size_t ld = g.ld;
-    double p = W[c], T = W[4 * ld + c], nut = W[5 * ld + c];
-    auto gvec = [&](int v) { return mk<double>(G[(3 * v) * ld + c], G[(3 * v + 1) * ld + c], G[(3 * v + 2) * ld + c]); };
-    return sa_source<double>(m_rho(P, p, T), m_visc(P, T), nut, gvec(1), gvec(2), gvec(3), gvec(5), g.wd[c]);
-}
-
 // R_c = sum over the faces of c in reference face order of (D-C)+0 (parent)
 // or -1*(D-C)+0 (neighbour) -- the VecSetValues(ADD) sequence of
 // ns_assemble_linear_system.cc:131-140 starting from VecSet(rhs, 0) -- then,
-// with SRC (SA model), + V_c S_c in the nu~ row
+// with SRC (SA model), + V_c S_c (src[c], written by grad_lim_kernel) in the
+// nu~ row
 template <int NV, bool SRC>
-__global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ G,
-                                                       const double* __restrict__ FX, double* __restrict__ Rout) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, const double* __restrict__ src, const double* __restrict__ FX,
+                                                       double* __restrict__ Rout) {
+    const int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld, Fs = g.F;
     double s[NV];
@@ -286,7 +285,7 @@ __global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, Params P, cons
             for (int v = 0; v < NV; ++v) s[v] += -1. * FX[v * Fs + f] + 0.;
         }
     }
-    if constexpr (SRC && NV == 6) s[5] += g.vol[c] * cell_sa_source(g, P, W, G, c);
+    if constexpr (SRC && NV == 6) s[5] += src[c];
 #pragma unroll
     for (int v = 0; v < NV; ++v) Rout[v * ld + c] = s[v];
 }
@@ -397,24 +396,26 @@ void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double*
 }
 
 template <int NV>
-static void launch_grad_lim_t(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s) {
+static void launch_grad_lim_t(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, double* src,
+                              hipStream_t s) {
     int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
     dim3 grid((g.N + 255) / 256), blk(256);
     if (lim == FCFD_LIMITER_VK) {
         launch_vk_range(g, P, W, dref, s);
-        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_VK, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_VK, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref, src);
     } else if (lim == FCFD_LIMITER_BJ) {
-        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_BJ, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_BJ, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref, src);
     } else if (lim == FCFD_LIMITER_MINMOD) {
-        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_MINMOD, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_MINMOD, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref, src);
     } else {
-        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_NONE, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref);
+        hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_NONE, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref, src);
     }
 }
 
-void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s) {
-    if (g.nv == 6) launch_grad_lim_t<6>(g, P, W, G, PHI, dref, s);
-    else launch_grad_lim_t<5>(g, P, W, G, PHI, dref, s);
+void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s,
+                     double* src) {
+    if (g.nv == 6) launch_grad_lim_t<6>(g, P, W, G, PHI, dref, src, s);
+    else launch_grad_lim_t<5>(g, P, W, G, PHI, dref, nullptr, s);
 }
 
 void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* FX,
@@ -435,16 +436,14 @@ void launch_flux(const DevGrid& g, const Params& P, const double* W, const doubl
 
 void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s) {
     dim3 grid((g.N + 255) / 256), blk(256);
-    Params P{};
-    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, false>), grid, blk, 0, s, g, P, nullptr, nullptr, FX, R);
-    else hipLaunchKernelGGL((cell_sum_kernel<5, false>), grid, blk, 0, s, g, P, nullptr, nullptr, FX, R);
+    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, false>), grid, blk, 0, s, g, nullptr, FX, R);
+    else hipLaunchKernelGGL((cell_sum_kernel<5, false>), grid, blk, 0, s, g, nullptr, FX, R);
 }
 
-void launch_residual_sum(const DevGrid& g, const Params& P, const double* W, const double* G, const double* FX, double* R,
-                         hipStream_t s) {
+void launch_residual_sum(const DevGrid& g, const double* src, const double* FX, double* R, hipStream_t s) {
     dim3 grid((g.N + 255) / 256), blk(256);
-    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, true>), grid, blk, 0, s, g, P, W, G, FX, R);
-    else hipLaunchKernelGGL((cell_sum_kernel<5, false>), grid, blk, 0, s, g, P, W, G, FX, R);
+    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, true>), grid, blk, 0, s, g, src, FX, R);
+    else hipLaunchKernelGGL((cell_sum_kernel<5, false>), grid, blk, 0, s, g, nullptr, FX, R);
 }
 
 void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s) {
