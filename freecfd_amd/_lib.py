@@ -42,6 +42,8 @@ def lib():
         "fcfd_vjp": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "fcfd_gradients_to_host": (C.c_int, [_vp, _vp, _vp]),
         "fcfd_loads": (C.c_int, [_vp, _vp, _vp, _vp]),
+        "fcfd_adjoint_step": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(A.NewtonOpts), C.POINTER(A.NewtonStats), _vp]),
+        "fcfd_set_bc": (C.c_int, [_vp, C.c_int, C.POINTER(A.BC)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -251,17 +251,18 @@ def bump_case(ni=256, nj=96, nk=64, seed=4, noise=0.0):
     return Case("bump", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.85))
 
 
-def plate_case(ni=512, nj=128, nk=64, seed=5, noise=0.0, first=5.5e-6, sa=False):
-    """Turbulent flat plate (Mach 0.3, Re_L = 5e6), hex layers with the top 20%
-    split into prisms, viscous Roe; sa=True adds the Spalart-Allmaras variable
-    nu~ (freestream and inlet nu~ = 3 nu) as the sixth equation."""
-    raw = meshgen.flat_plate(ni, nj, nk, first=first)
+def plate_case(ni=512, nj=128, nk=64, seed=5, noise=0.0, first=5.5e-6, sa=False, re=5.0e6, prism_frac=0.2):
+    """Turbulent flat plate (Mach 0.3, Re_L = 5e6, or `re`), hex layers with
+    the top 20% split into prisms, viscous Roe; sa=True adds the
+    Spalart-Allmaras variable nu~ (freestream and inlet nu~ = 3 nu) as the
+    sixth equation."""
+    raw = meshgen.flat_plate(ni, nj, nk, first=first, prism_frac=prism_frac)
     Mw, g = 28.97, 1.4
     R = gas_R(Mw)
     p, T = 101325.0, 300.0
     U = 0.3 * np.sqrt(g * R * T)
     rho = p / (R * T)
-    mu = rho * U * 1.0 / 5.0e6
+    mu = rho * U * 1.0 / re
     nut_inf = 3.0 * mu / rho
     inlet = bc_inlet_T(T, (U, 0.0, 0.0))
     inlet["nut"] = nut_inf

@@ -33,11 +33,12 @@ struct fcfd_ctx {
     std::vector<void*> allocs;
     double *G = nullptr, *PHI = nullptr, *dref = nullptr, *FX = nullptr;
     double* SRC = nullptr;   // [ld] V S of the SA source (nv = 6), from grad_lim
+    double* adj_v = nullptr; // [nv][ld] J^T x inside the adjoint step's operator
     int* status = nullptr;
     double *JL = nullptr, *JR = nullptr, *Dg = nullptr, *zero_upd = nullptr;   // FD Jacobian (lazy)
     // Newton-Krylov workspace (lazy)
     double *R = nullptr, *dt = nullptr, *dW = nullptr, *upd = nullptr, *red_part = nullptr, *red_out = nullptr;
-    int gm_restart = 0, gm_fp32 = -1;
+    int gm_restart32 = 0, gm_restart64 = 0;   // restart length of the allocated FP32 / FP64 Krylov bases
     GmresWork<double> w64;
     GmresWork<float> w32;
     double* Dinv64 = nullptr;
@@ -89,6 +90,16 @@ int upload(fcfd_ctx* c, const std::vector<T>& h, const T** out, size_t pad_to = 
     if (!h.empty()) HIPCHK(c, hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
     *out = (const T*)p;
     return FCFD_OK;
+}
+
+// release one allocation of the context (nullptr: nothing)
+void dfree(fcfd_ctx* c, void* p) {
+    if (!p) return;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+    if (it != c->allocs.end()) {
+        (void)hipFree(p);
+        c->allocs.erase(it);
+    }
 }
 
 template <typename T>
@@ -740,6 +751,53 @@ int fcfd_jvp(fcfd_ctx* c, double* W_dev, double* dW_dev, double* dR_dev, int mod
 
 // Greedy colouring of the interior cells on the face graph, in internal (RCM)
 // order: each cell takes the smallest colour none of its face neighbours has.
+// buffers of the implicit step: residual, dt, updates, FD blocks, the
+// Krylov basis of the requested restart and precision
+static int ensure_newton(fcfd_ctx* c, const fcfd_newton_opts* o) {
+    const DevGrid& g = c->dg;
+    const size_t n5 = (size_t)g.nv * g.ld;
+    const size_t b2 = (size_t)g.nv * g.nv;
+    int rc;
+    if (!c->R) {
+        if ((rc = dalloc(c, n5, &c->R)) || (rc = dalloc(c, (size_t)g.ld, &c->dt)) || (rc = dalloc(c, n5, &c->dW)) ||
+            (rc = dalloc(c, n5, &c->upd)) || (rc = dalloc(c, (size_t)512 * 8, &c->red_part)) || (rc = dalloc(c, 64, &c->red_out)))
+            return rc;
+    }
+    if (!c->JL) {
+        if ((rc = dalloc(c, b2 * g.F, &c->JL)) || (rc = dalloc(c, b2 * g.F, &c->JR)) ||
+            (rc = dalloc(c, b2 * g.ld, &c->Dg)) || (rc = dalloc(c, n5, &c->zero_upd)))
+            return rc;
+    }
+    const int fp32 = o->krylov_fp32 ? 1 : 0;
+    const size_t m1 = (size_t)o->restart + 1;
+    if (fp32 && c->gm_restart32 != o->restart) {   // (re)allocate the FP32 Krylov basis
+        for (void* p : {(void*)c->w32.V, (void*)c->w32.z, (void*)c->w32.r, (void*)c->w32.part, (void*)c->w32.out_dev,
+                        (void*)c->w32.coef_dev})
+            dfree(c, p);
+        if ((rc = dalloc(c, m1 * n5, &c->w32.V)) || (rc = dalloc(c, n5, &c->w32.z)) || (rc = dalloc(c, n5, &c->w32.r)) ||
+            (rc = dalloc(c, m1 * 512, &c->w32.part)) || (rc = dalloc(c, m1, &c->w32.out_dev)) || (rc = dalloc(c, m1, &c->w32.coef_dev)))
+            return rc;
+        c->gm_restart32 = o->restart;
+    }
+    if (fp32 && !c->Dinv32) {
+        if ((rc = dalloc(c, b2 * g.ld, &c->Dinv32)) || (rc = dalloc(c, b2 * g.ld, &c->D32)) ||
+            (rc = dalloc(c, b2 * g.F, &c->JL32)) || (rc = dalloc(c, b2 * g.F, &c->JR32)) ||
+            (rc = dalloc(c, n5, &c->b32)) || (rc = dalloc(c, n5, &c->x32)))
+            return rc;
+    }
+    if (!fp32 && c->gm_restart64 != o->restart) {   // (re)allocate the FP64 Krylov basis
+        for (void* p : {(void*)c->w64.V, (void*)c->w64.z, (void*)c->w64.r, (void*)c->w64.part, (void*)c->w64.out_dev,
+                        (void*)c->w64.coef_dev})
+            dfree(c, p);
+        if ((rc = dalloc(c, m1 * n5, &c->w64.V)) || (rc = dalloc(c, n5, &c->w64.z)) || (rc = dalloc(c, n5, &c->w64.r)) ||
+            (rc = dalloc(c, m1 * 512, &c->w64.part)) || (rc = dalloc(c, m1, &c->w64.out_dev)) || (rc = dalloc(c, m1, &c->w64.coef_dev)))
+            return rc;
+        c->gm_restart64 = o->restart;
+    }
+    if (!fp32 && !c->Dinv64 && (rc = dalloc(c, b2 * g.ld, &c->Dinv64))) return rc;
+    return FCFD_OK;
+}
+
 static int ensure_colors(fcfd_ctx* c) {
     if (c->colors.n > 0) return FCFD_OK;
     const HostGrid& g = c->hg;
@@ -798,39 +856,9 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     hipStream_t s = (hipStream_t)stream;
     const size_t n5 = (size_t)g.nv * g.ld;      // one [nv][ld] vector
     const size_t b2 = (size_t)g.nv * g.nv;     // entries of one block
-    int rc;
-    if (!c->R) {
-        if ((rc = dalloc(c, n5, &c->R)) || (rc = dalloc(c, (size_t)g.ld, &c->dt)) || (rc = dalloc(c, n5, &c->dW)) ||
-            (rc = dalloc(c, n5, &c->upd)) || (rc = dalloc(c, (size_t)512 * 8, &c->red_part)) || (rc = dalloc(c, 64, &c->red_out)))
-            return rc;
-    }
-    if (!c->JL) {
-        if ((rc = dalloc(c, b2 * g.F, &c->JL)) || (rc = dalloc(c, b2 * g.F, &c->JR)) ||
-            (rc = dalloc(c, b2 * g.ld, &c->Dg)) || (rc = dalloc(c, n5, &c->zero_upd)))
-            return rc;
-    }
+    int rc = ensure_newton(c, o);
+    if (rc) return rc;
     const int fp32 = o->krylov_fp32 ? 1 : 0;
-    if (c->gm_restart != o->restart || c->gm_fp32 != fp32) {   // (re)allocate the Krylov basis
-        size_t m1 = (size_t)o->restart + 1;
-        if (fp32) {
-            if ((rc = dalloc(c, m1 * n5, &c->w32.V)) || (rc = dalloc(c, n5, &c->w32.z)) || (rc = dalloc(c, n5, &c->w32.r)) ||
-                (rc = dalloc(c, m1 * 512, &c->w32.part)) || (rc = dalloc(c, m1, &c->w32.out_dev)) || (rc = dalloc(c, m1, &c->w32.coef_dev)))
-                return rc;
-            if (!c->Dinv32) {
-                if ((rc = dalloc(c, b2 * g.ld, &c->Dinv32)) || (rc = dalloc(c, b2 * g.ld, &c->D32)) ||
-                    (rc = dalloc(c, b2 * g.F, &c->JL32)) || (rc = dalloc(c, b2 * g.F, &c->JR32)) ||
-                    (rc = dalloc(c, n5, &c->b32)) || (rc = dalloc(c, n5, &c->x32)))
-                    return rc;
-            }
-        } else {
-            if ((rc = dalloc(c, m1 * n5, &c->w64.V)) || (rc = dalloc(c, n5, &c->w64.z)) || (rc = dalloc(c, n5, &c->w64.r)) ||
-                (rc = dalloc(c, m1 * 512, &c->w64.part)) || (rc = dalloc(c, m1, &c->w64.out_dev)) || (rc = dalloc(c, m1, &c->w64.coef_dev)))
-                return rc;
-            if (!c->Dinv64 && (rc = dalloc(c, b2 * g.ld, &c->Dinv64))) return rc;
-        }
-        c->gm_restart = o->restart;
-        c->gm_fp32 = fp32;
-    }
     using clk = std::chrono::steady_clock;
     auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
     fcfd_newton_stats S{};
@@ -925,6 +953,90 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     int bad = 0;
     HIPCHK(c, hipMemcpy(&bad, c->status, sizeof(int), hipMemcpyDeviceToHost));
     if (bad & 2) return fail(c, FCFD_EDIVERGED, "singular diagonal block in the block-Jacobi preconditioner");
+    return FCFD_OK;
+}
+
+int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* lam_dev, const fcfd_newton_opts* o,
+                      fcfd_newton_stats* st, void* stream) {
+    if (!c || !W_dev || !g_dev || !lam_dev || !o) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    if (o->dt_type != FCFD_DT_CFL_LOCAL) return fail(c, FCFD_ENOTSUP, "only CFL_LOCAL time stepping is implemented");
+    if (o->krylov_fp32) return fail(c, FCFD_ENOTSUP, "the adjoint step solves in FP64 (krylov_fp32 must be 0)");
+    if (o->precond != FCFD_PC_BLOCK_JACOBI) return fail(c, FCFD_ENOTSUP, "the adjoint step preconditions with block Jacobi");
+    if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad Krylov options");
+    int rc = ensure_adjoint(c);
+    if (!rc) rc = ensure_newton(c, o);
+    if (rc) return rc;
+    const DevGrid& g = c->dg;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n5 = (size_t)g.nv * g.ld;
+    if (!c->adj_v && (rc = dalloc(c, n5, &c->adj_v))) return rc;
+    using clk = std::chrono::steady_clock;
+    auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    fcfd_newton_stats S{};
+    auto t0 = clk::now();
+    // 1. adjoint residual rho = J^T lam + g (the VJP also refreshes ghosts, G and PHI of W)
+    launch_vjp(g, c->ag, c->P, W_dev, c->G, c->PHI, c->dref, lam_dev, nullptr, c->aw, c->R, c->status, s);
+    dev_axpy<double>(n5, 1.0, g_dev, c->R, s);
+    double rr = 0.0;
+    HIPCHK(c, dev_mdot(g, 1, c->R, n5, c->R, c->red_part, c->red_out, &rr, s));
+    S.res_norm = std::sqrt(rr);
+    auto t1 = clk::now();
+    if (!std::isfinite(S.res_norm)) {
+        if (st) *st = S;
+        return fail(c, FCFD_EDIVERGED, "Divergence detected: NaN/Inf in the adjoint residual");
+    }
+    // 2. local time step, FD blocks with the floor step, transposed block Jacobi
+    launch_dt_local(g, c->P, W_dev, o->cfl, c->dt, s);
+    launch_fd_jacobian(g, c->P, W_dev, c->G, c->PHI, c->zero_upd, c->dt, c->JL, c->JR, c->Dg, c->status, s);
+    dev_bj_invert<double>(g, c->Dg, c->Dinv64, c->status, s);
+    HIPCHK(c, hipStreamSynchronize(s));
+    auto t2 = clk::now();
+    // 3. GMRES on ((P V/dt) - J)^T delta = rho
+    GmresOpts go;
+    go.restart = o->restart; go.max_iters = o->max_iters; go.rtol = o->rtol; go.abstol = o->abstol; go.knoll = o->knoll_guess;
+    GmresStats gs;
+    LinOp<double> A{[&](const double* x, double* y, hipStream_t ss) {
+        launch_vjp(g, c->ag, c->P, W_dev, c->G, c->PHI, c->dref, x, nullptr, c->aw, c->adj_v, c->status, ss);
+        launch_newton_op_t(g, c->P, W_dev, c->dt, x, c->adj_v, y, ss);
+    }};
+    LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) { dev_bj_apply_t(g, c->Dinv64, x, y, ss); }};
+    rc = gmres<double>(g, A, M, c->R, c->dW, go, c->w64, gs, s);
+    if (rc) return fail(c, rc, "GMRES failed (HIP error)");
+    HIPCHK(c, hipStreamSynchronize(s));
+    auto t3 = clk::now();
+    // 4. lam += delta
+    dev_axpy<double>(n5, 1.0, c->dW, lam_dev, s);
+    HIPCHK(c, hipStreamSynchronize(s));
+    auto t4 = clk::now();
+    S.krylov_iters = gs.iters; S.krylov_converged = gs.converged; S.krylov_rnorm = gs.rnorm; S.krylov_r0norm = gs.r0norm;
+    S.ms_residual = ms(t0, t1); S.ms_jacobian = ms(t1, t2); S.ms_krylov = ms(t2, t3); S.ms_update = ms(t3, t4);
+    if (st) *st = S;
+    int bad = 0;
+    HIPCHK(c, hipMemcpy(&bad, c->status, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad & 2) return fail(c, FCFD_EDIVERGED, "singular diagonal block in the block-Jacobi preconditioner");
+    return FCFD_OK;
+}
+
+int fcfd_set_bc(fcfd_ctx* c, int index, const fcfd_bc* bc) {
+    if (!c || !bc) return FCFD_EINVAL;
+    if (index < 0 || index >= c->opt.n_bc) return fail(c, FCFD_EINVAL, "BC index out of range");
+    const fcfd_bc& old = c->opt.bc[index];
+    if (bc->type != old.type || bc->kind != old.kind || bc->specified != old.specified || bc->thermal != old.thermal)
+        return fail(c, FCFD_EBC, "fcfd_set_bc changes BC values only: type, kind, specified and thermal are fixed at create");
+    const fcfd_options saved = c->opt;
+    c->opt.bc[index] = *bc;
+    int rc = validate_options(c);
+    if (rc) {
+        c->opt = saved;
+        return rc;
+    }
+    set_params(c);
+    if (c->device) {
+        // the table is read by every kernel: replace it with no work in flight
+        HIPCHK(c, hipDeviceSynchronize());
+        HIPCHK(c, hipMemcpy((void*)c->dg.bc, c->bch.data(), sizeof(BcDev) * c->bch.size(), hipMemcpyHostToDevice));
+    }
     return FCFD_OK;
 }
 

@@ -36,6 +36,7 @@ hipError_t dev_mdot(const DevGrid& g, int k, const T* V, size_t stride, const T*
 template <typename T> void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s);
 template <typename T> void dev_bj_invert(const DevGrid& g, const double* D, T* Dinv, int* status, hipStream_t s);
 template <typename T> void dev_bj_apply(const DevGrid& g, const T* Dinv, const T* x, T* y, hipStream_t s);
+void dev_bj_apply_t(const DevGrid& g, const double* Dinv, const double* x, double* y, hipStream_t s);   // y = D^{-T} x
 // cell colouring of the face graph: colour k holds cells[ptr[k] .. ptr[k+1]) (device), ptr on the host
 struct Colors {
     int n = 0;

@@ -216,7 +216,9 @@ __global__ void __launch_bounds__(256) bj_invert_kernel(int N, size_t ld, const 
         for (int j = 0; j < NV; ++j) Dinv[(NV * i + j) * ld + c] = singular ? ((i == j) ? T(1) : T(0)) : a[i][NV + j];
 }
 
-template <typename T, int NV>
+// y_c = D_c^{-1} x_c, or with TRANS y_c = D_c^{-T} x_c (the adjoint's
+// preconditioner)
+template <typename T, int NV, bool TRANS = false>
 __global__ void __launch_bounds__(256) bj_apply_kernel(int N, size_t ld, const T* __restrict__ Dinv, const T* __restrict__ x,
                                                        T* __restrict__ y) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -228,7 +230,7 @@ __global__ void __launch_bounds__(256) bj_apply_kernel(int N, size_t ld, const T
     for (int j = 0; j < NV; ++j) {
         T s = 0;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) s += Dinv[(NV * j + i) * ld + c] * xc[i];
+        for (int i = 0; i < NV; ++i) s += Dinv[(TRANS ? NV * i + j : NV * j + i) * ld + c] * xc[i];
         y[j * ld + c] = s;
     }
 }
@@ -347,6 +349,12 @@ template <typename T>
 void dev_bj_apply(const DevGrid& g, const T* Dinv, const T* x, T* y, hipStream_t s) {
     if (g.nv == 6) hipLaunchKernelGGL((bj_apply_kernel<T, 6>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
     else hipLaunchKernelGGL((bj_apply_kernel<T, 5>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
+}
+void dev_bj_apply_t(const DevGrid& g, const double* Dinv, const double* x, double* y, hipStream_t s) {
+    if (g.nv == 6)
+        hipLaunchKernelGGL((bj_apply_kernel<double, 6, true>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
+    else
+        hipLaunchKernelGGL((bj_apply_kernel<double, 5, true>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
 }
 template void dev_bj_invert<double>(const DevGrid&, const double*, double*, int*, hipStream_t);
 template void dev_bj_invert<float>(const DevGrid&, const double*, float*, int*, hipStream_t);

@@ -11,4 +11,7 @@ void launch_jvp(const DevGrid& g, const Params& P, double* W, double* dW, double
 // y = (P V / dt) x - dR
 void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* dR,
                       double* y, hipStream_t s);
+// y = (P V / dt)^T x - Jt   (Jt = (dR/dW)^T x)
+void launch_newton_op_t(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* Jt,
+                        double* y, hipStream_t s);
 }

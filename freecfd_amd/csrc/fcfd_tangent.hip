@@ -174,6 +174,42 @@ __global__ void __launch_bounds__(256) newton_op_kernel(DevGrid g, Params P, con
     }
 }
 
+// y = (P V/dt)^T x - Jt  (the transposed implicit operator for the adjoint
+// pseudo-time step; Jt = J^T x from the adjoint kernels).  The time-term block
+// is the one newton_op_kernel applies, transposed.
+template <int NV>
+__global__ void __launch_bounds__(256) newton_op_t_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ dt,
+                                                          const double* __restrict__ x, const double* __restrict__ Jt,
+                                                          double* __restrict__ y) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld;
+    double p = W[c], u = W[ld + c], v = W[2 * ld + c], w = W[3 * ld + c], T = W[4 * ld + c];
+    double rho = m_rho(P, p, T);
+    double pressure = p + P.Pref, temperature = T + P.Tref;
+    double drho_dT = -1. * rho / temperature, drho_dp = rho / pressure;
+    double H = P.Cp * temperature + 0.5 * (u * u + v * v + w * w);
+    double xv[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) xv[k] = x[k * ld + c];
+    double s = g.vol[c] / dt[c];
+    double nut = 0., x5 = 0.;
+    if constexpr (NV == 6) { nut = W[5 * ld + c]; x5 = xv[5]; }
+    double c0 = drho_dp * xv[0] + drho_dp * u * xv[1] + drho_dp * v * xv[2] + drho_dp * w * xv[3] + (drho_dp * H - 1.) * xv[4] +
+                drho_dp * nut * x5;
+    double c1 = rho * xv[1] + rho * u * xv[4];
+    double c2 = rho * xv[2] + rho * v * xv[4];
+    double c3 = rho * xv[3] + rho * w * xv[4];
+    double c4 = drho_dT * xv[0] + drho_dT * u * xv[1] + drho_dT * v * xv[2] + drho_dT * w * xv[3] + (drho_dT * H + rho * P.Cp) * xv[4] +
+                drho_dT * nut * x5;
+    y[c] = s * c0 - Jt[c];
+    y[ld + c] = s * c1 - Jt[ld + c];
+    y[2 * ld + c] = s * c2 - Jt[2 * ld + c];
+    y[3 * ld + c] = s * c3 - Jt[3 * ld + c];
+    y[4 * ld + c] = s * c4 - Jt[4 * ld + c];
+    if constexpr (NV == 6) y[5 * ld + c] = s * (rho * x5) - Jt[5 * ld + c];
+}
+
 // ---------------------------------------------------------------------------
 template <int FLUX, int VISC, bool FULL, int NV>
 static void launch_ft(const DevGrid& g, const Params& P, const DualLoader<NV, FULL>& L, double* dFX, int* status, hipStream_t s) {
@@ -235,6 +271,13 @@ void launch_jvp(const DevGrid& g, const Params& P, double* W, double* dW, double
                 double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s) {
     if (g.nv == 6) launch_jvp_t<6>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
     else launch_jvp_t<5>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
+}
+
+void launch_newton_op_t(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* Jt,
+                        double* y, hipStream_t s) {
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL(newton_op_t_kernel<6>, grid, blk, 0, s, g, P, W, dt, x, Jt, y);
+    else hipLaunchKernelGGL(newton_op_t_kernel<5>, grid, blk, 0, s, g, P, W, dt, x, Jt, y);
 }
 
 void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* dR,

@@ -223,6 +223,57 @@ class NavierStokes(Grid):
         return dict(status=status, res_norm=hist, ms=ms, krylov_iters=iters, steps=len(hist),
                     drop=hist[-1] / hist[0] if hist else 1.0, total_ms=float(np.sum(ms)))
 
+    def set_bc(self, index, **values):
+        """Replace values of BC `index` (p, T, rho, V, p_total, T_total, mdot,
+        qdot, nut) in the live context; the mesh set-up is unchanged."""
+        bc = A.BC.from_buffer_copy(self._opt.bc[index])
+        for k, v in values.items():
+            if k == "V":
+                for i in range(3):
+                    bc.V[i] = float(v[i])
+            else:
+                setattr(bc, k, float(v))
+        self.torch.cuda.synchronize()
+        _check(self.h, lib().fcfd_set_bc(self.h, int(index), C.byref(bc)))
+        self._opt.bc[index] = bc
+
+    def adjoint_step(self, g, lam, opts=None, W=None):
+        """One pseudo-time step of (dR/dW)^T lam = -g (g, lam [nv][ld] device
+        tensors, lam updated in place); returns NewtonStats with res_norm =
+        ||(dR/dW)^T lam + g||_2 before the step."""
+        opts = A.newton_opts(fp32=False) if opts is None else opts
+        W = self.W if W is None else W
+        st = A.NewtonStats()
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_adjoint_step(self.h, W.data_ptr(), g.data_ptr(), lam.data_ptr(), C.byref(opts), C.byref(st),
+                                               C.c_void_p(s)))
+        return st
+
+    def adjoint_solve(self, g, opts=None, max_steps=200, drop=1e-10, cfl_initial=None, cfl_growth=1.0, cfl_target=None,
+                      lam=None, W=None):
+        """Adjoint pseudo-time iterations from lam = 0 (or `lam`) with the same
+        CFL ramp as solve(), until ||(dR/dW)^T lam + g|| <= drop * ||g||.
+        Returns (lam, dict(status, res_norm history, steps, drop))."""
+        opts = A.newton_opts(fp32=False) if opts is None else opts
+        lam = self.torch.zeros_like(self.W) if lam is None else lam
+        cfl = opts.cfl if cfl_initial is None else cfl_initial
+        target = cfl if cfl_target is None else cfl_target
+        g0 = float(self.torch.linalg.vector_norm(g))
+        hist, status = [], "max_steps"
+        for k in range(max_steps):
+            if k > 0:
+                cfl = min(cfl * cfl_growth, target)
+            opts.cfl = cfl
+            st = self.adjoint_step(g, lam, opts, W=W)
+            hist.append(st.res_norm)
+            if not np.isfinite(st.res_norm):
+                status = "diverged"
+                break
+            if st.res_norm <= drop * g0:
+                status = "converged"
+                break
+        return lam, dict(status=status, res_norm=hist, steps=len(hist), drop=(hist[-1] / g0) if hist and g0 > 0 else 0.0)
+
     def time_step_host(self, cfl):
         dt = self.torch.zeros(self.ld, dtype=self.torch.float64, device=self.device)
         mc = C.c_double()

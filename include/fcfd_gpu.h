@@ -25,6 +25,7 @@
  *                   surface loads of the face loop (ns_assemble_linear_system.cc:109-119)
  *                                                                  -> fcfd_loads
  *   build-only      tangent and adjoint of the residual           -> fcfd_jvp, fcfd_vjp
+ *                   steady adjoint solve, BC parameter updates    -> fcfd_adjoint_step, fcfd_set_bc
  *
  * Numbering.  Host arrays passed to or returned by fcfd_* in "reference
  * numbering" use the cell, face and node order the reference's Grid::setup
@@ -323,7 +324,7 @@ int  fcfd_time_step(fcfd_ctx* ctx, const double* W_dev, int dt_type, double cfl,
  * second term gives the state adjoint of a surface-load functional: the
  * force on BC b is -sum_f (D - C)_f[1..3], so a drag direction e on a wall
  * BC is fx_weight[b] = (0, -e_x, -e_y, -e_z, 0).  Rbar_dev and Wbar_dev are
- * [5][ld_cell] device arrays (only interior cells of Rbar are read);
+ * [nv][ld_cell] device arrays (only interior cells of Rbar are read);
  * fx_weight_dev is a device [n_bc][5] array or NULL.  The ghost slots of
  * W_dev are written.  Deterministic (no atomics).  No counterpart in the
  * reference, which has no adjoint. */
@@ -336,6 +337,29 @@ int  fcfd_vjp(fcfd_ctx* ctx, double* W_dev, const double* Rbar_dev, const double
  * 6 * n_bc doubles (Fx Fy Fz Mx My Mz per BC) to loads_host after
  * synchronising the stream.  The ghost slots of W_dev are written. */
 int  fcfd_loads(fcfd_ctx* ctx, double* W_dev, double* loads_host, void* stream);
+
+/* One pseudo-time step of the steady adjoint (dR/dW)^T lam = -g at the state
+ * W_dev (g = dJ/dW of an output J, e.g. from fcfd_vjp with a load weight and
+ * Rbar = 0): with rho = (dR/dW)^T lam + g, solves ((P V/dt) - dR/dW)^T delta
+ * = rho by FP64 GMRES with the exact transpose (fcfd_vjp) as the operator and
+ * the transposed FD block-Jacobi of the same step as the preconditioner, then
+ * lam += delta.  As CFL grows it tends to the Newton solve of the adjoint
+ * system, which is linear: lam converges at the rate of the Krylov solves.
+ * opts: cfl, restart, max_iters, rtol, abstol, knoll_guess; krylov_fp32 must
+ * be 0 and precond FCFD_PC_BLOCK_JACOBI.  stats->res_norm = ||rho||_2 before
+ * the step.  g_dev and lam_dev are [nv][ld_cell] (lam updated in place);
+ * the ghost slots of W_dev are written.  No counterpart in the reference. */
+int  fcfd_adjoint_step(fcfd_ctx* ctx, double* W_dev, const double* g_dev, double* lam_dev, const fcfd_newton_opts* opts,
+                       fcfd_newton_stats* stats, void* stream);
+
+/* Replace the values of BC `index` (p, T, rho, V, p_total, T_total, mdot,
+ * qdot, nut) in a live context, the mesh and its set-up unchanged: type,
+ * kind, specified and thermal must equal the ones given at create
+ * (FCFD_EBC otherwise).  Synchronises the device first, so no work of the
+ * context may be in flight on another thread.  For parameter studies and
+ * the parameter derivatives of an adjoint; the reference reads BC values
+ * once from its input file (src/set_bcs.cc:35, ns_set_bcs.cc:25-180). */
+int  fcfd_set_bc(fcfd_ctx* ctx, int index, const fcfd_bc* bc);
 
 #ifdef __cplusplus
 }

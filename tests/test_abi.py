@@ -22,3 +22,26 @@ def test_create_host_reports_errors_without_exit():
     rc = _lib.lib().fcfd_create_host(ctypes.byref(m), ctypes.byref(cs.opt), ctypes.byref(h))
     assert rc == A.EBC
     assert b"outlet" in _lib.lib().fcfd_last_error(None)
+
+
+def test_set_bc_validates_on_a_host_context():
+    """fcfd_set_bc replaces BC values in a live (host-only) context; it refuses
+    a change of BC type and an out-of-range index."""
+    from freecfd_amd import _abi as A, cases
+    cs = cases.wedge_case(4, 2)
+    m = A.make_mesh(cs.raw)
+    h = ctypes.c_void_p()
+    L = _lib.lib()
+    assert L.fcfd_create_host(ctypes.byref(m), ctypes.byref(cs.opt), ctypes.byref(h)) == 0
+    try:
+        bc = A.BC.from_buffer_copy(cs.opt.bc[0])
+        bc.V[0] *= 1.01
+        assert L.fcfd_set_bc(h, 0, ctypes.byref(bc)) == 0
+        bad = A.BC.from_buffer_copy(cs.opt.bc[0])
+        bad.type = A.BC_WALL
+        assert L.fcfd_set_bc(h, 0, ctypes.byref(bad)) == A.EBC
+        assert b"type" in L.fcfd_last_error(h)
+        assert L.fcfd_set_bc(h, 99, ctypes.byref(bc)) == A.EINVAL
+        assert L.fcfd_set_bc(h, -1, ctypes.byref(bc)) == A.EINVAL
+    finally:
+        L.fcfd_destroy(h)
