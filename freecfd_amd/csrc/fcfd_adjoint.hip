@@ -54,7 +54,8 @@ __global__ void __launch_bounds__(256) face_adj_kernel(DevGrid g, Params P, Cell
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
     const size_t Fs = g.F, ld = g.ld;
-    const int p = g.fpar[f], n = g.fnei[f], bc = g.fbc[f];
+    const int4 cr = g.fcell[f];
+    const int p = cr.x, n = cr.y, bc = cr.z;
     const bool bnd = bc >= 0;
     constexpr int nz = zcount<VISC, NV>();
     double fb[NV];
@@ -146,10 +147,12 @@ __global__ void __launch_bounds__(256) cell_adj_kernel(DevGrid g, Params P, cons
     }
     constexpr bool has_phi = LIM != FCFD_LIMITER_NONE;
     double qb[NV], gb[3 * NV], pb[NV], gc[3 * NV], ph[NV];
+    load_pairs<3 * NV>(G, ld, c, gc);
 #pragma unroll
-    for (int k = 0; k < 3 * NV; ++k) { gb[k] = 0.0; gc[k] = G[k * ld + c]; }
+    for (int k = 0; k < 3 * NV; ++k) gb[k] = 0.0;
+    if (has_phi) load_pairs<NV>(PHI, ld, c, ph);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) { qb[v] = 0.0; pb[v] = 0.0; ph[v] = has_phi ? PHI[v * ld + c] : phi_const; }
+    for (int v = 0; v < NV; ++v) { qb[v] = 0.0; pb[v] = 0.0; if (!has_phi) ph[v] = phi_const; }
     const Vec<double> xc = centroid(g, c);
     const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
     for (int h = h0; h < h1; ++h) {
@@ -157,7 +160,7 @@ __global__ void __launch_bounds__(256) cell_adj_kernel(DevGrid g, Params P, cons
         const bool left = fs >= 0;
         const int f = left ? fs : ~fs;
         const int of = left ? Zr::L : Zr::R, oc = left ? Zr::Lc : Zr::Rc;
-        const Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
+        const Vec<double> c2f = vsub(face_centroid(g, f), xc);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             // q_f = q_c + phi (grad q . c2f)  (recon) and the centre value itself
@@ -171,7 +174,8 @@ __global__ void __launch_bounds__(256) cell_adj_kernel(DevGrid g, Params P, cons
             gb[3 * v + 2] += s * c2f.z;
         }
         if (VISC) {   // face gradient = w_P grad_P + w_N grad_N (interior), grad_P (boundary)
-            const double w = g.fbc[f] >= 0 ? 1.0 : (left ? g.fwp[f] : g.fwn[f]);
+            const double2 wpn = face_pair(g, 5, f);
+            const double w = g.fbc[f] >= 0 ? 1.0 : (left ? wpn.x : wpn.y);
 #pragma unroll
             for (int m = 0; m < 12; ++m) gb[3 + m] += w * Z[(Zr::G + m) * Fs + f];
             if constexpr (NV == 6) {
@@ -353,7 +357,8 @@ __global__ void __launch_bounds__(256) loads_face_kernel(DevGrid g, AdjGrid a, P
     face_base<FLUX, VISC, NV>(g, P, cl, f, X, status);
     // force += (C - D)[1..3], moment += (x_f - x_m) x force (:109-119)
     Vec<double> t = mk<double>(X.C[1] - X.D[1], X.C[2] - X.D[2], X.C[3] - X.D[3]);
-    Vec<double> r = mk<double>(g.fcx[f] - mcx, g.fcy[f] - mcy, g.fcz[f] - mcz);
+    const Vec<double> xf = face_centroid(g, f);
+    Vec<double> r = mk<double>(xf.x - mcx, xf.y - mcy, xf.z - mcz);
     Vec<double> mm = vcross(r, t);
     const size_t s = nlf;
     tmp[i] = t.x; tmp[s + i] = t.y; tmp[2 * s + i] = t.z;

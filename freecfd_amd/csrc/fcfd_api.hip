@@ -210,6 +210,12 @@ int build_device(fcfd_ctx* c) {
         if ((rc = upload(c, x, &d.ccx)) || (rc = upload(c, y, &d.ccy)) || (rc = upload(c, z, &d.ccz)) ||
             (rc = upload(c, v, &d.vol)) || (rc = upload(c, l, &d.lscale)) || (rc = upload(c, w, &d.wd)))
             return rc;
+        std::vector<double2> c2((size_t)2 * ld);
+        for (int i = 0; i < ld; ++i) {
+            c2[i] = make_double2(x[i], y[i]);
+            c2[(size_t)ld + i] = make_double2(z[i], v[i]);
+        }
+        if ((rc = upload(c, c2, &d.cc2))) return rc;
     }
     // gradient maps, sliced ELL with 64-cell slices
     {
@@ -242,10 +248,16 @@ int build_device(fcfd_ctx* c) {
             }
             gs[s + 1] = (int)gi.size();
         }
+        std::vector<double2> ga(gi.size()), gb(gi.size());
+        for (size_t e = 0; e < gi.size(); ++e) {
+            long long bits = gi[e];
+            double key;
+            std::memcpy(&key, &bits, sizeof(key));
+            ga[e] = make_double2(wx[e], wy[e]);
+            gb[e] = make_double2(wz[e], key);
+        }
         int rc;
-        if ((rc = upload(c, gs, &d.gslice)) || (rc = upload(c, gi, &d.gidx)) || (rc = upload(c, wx, &d.gwx)) ||
-            (rc = upload(c, wy, &d.gwy)) || (rc = upload(c, wz, &d.gwz)))
-            return rc;
+        if ((rc = upload(c, gs, &d.gslice)) || (rc = upload(c, ga, &d.gwa)) || (rc = upload(c, gb, &d.gwb))) return rc;
     }
     // half-faces and flux blocks
     {
@@ -293,12 +305,25 @@ int build_device(fcfd_ctx* c) {
             (rc = upload(c, cz, &d.fcz)) || (rc = upload(c, wp, &d.fwp)) || (rc = upload(c, wn, &d.fwn)) ||
             (rc = upload(c, wd, &d.fwd)) || (rc = upload(c, ds, &d.fdiss)) || (rc = upload(c, gh, &d.gh_face)))
             return rc;
+        std::vector<int4> cell(F);
+        std::vector<double2> geo((size_t)6 * F);
+        for (int i = 0; i < F; ++i) {
+            cell[i] = make_int4(par[i], nei[i], bc[i], 0);
+            geo[i] = make_double2(nx[i], ny[i]);
+            geo[(size_t)F + i] = make_double2(nz[i], tx[i]);
+            geo[(size_t)2 * F + i] = make_double2(ty[i], tz[i]);
+            geo[(size_t)3 * F + i] = make_double2(ar[i], cx[i]);
+            geo[(size_t)4 * F + i] = make_double2(cy[i], cz[i]);
+            geo[(size_t)5 * F + i] = make_double2(wp[i], wn[i]);
+        }
+        if ((rc = upload(c, cell, &d.fcell)) || (rc = upload(c, geo, &d.fgeo))) return rc;
     }
     {
         int rc;
         if ((rc = upload(c, c->bch, &d.bc))) return rc;
         const size_t nv = d.nv;
-        if ((rc = dalloc(c, 3 * nv * ld, &c->G)) || (rc = dalloc(c, nv * ld, &c->PHI)) ||
+        const size_t grows = 2 * pair_groups(3 * (int)nv), prows = 2 * pair_groups((int)nv);   // row pairs
+        if ((rc = dalloc(c, grows * ld, &c->G)) || (rc = dalloc(c, prows * ld, &c->PHI)) ||
             (rc = dalloc(c, (size_t)kDrefSize, &c->dref)) || (rc = dalloc(c, 4, &c->status)) ||
             (rc = dalloc(c, nv * g.F, &c->FX)))
             return rc;
@@ -539,16 +564,16 @@ int fcfd_internal_buffers(fcfd_ctx* c, const double** grad, const double** phi) 
 int fcfd_gradients_to_host(fcfd_ctx* c, double* G_host, double* PHI_host) {
     if (!c || !c->device) return FCFD_EINVAL;
     const int N = c->hg.N, B = c->hg.B, ld = c->dg.ld, nv = c->dg.nv;
-    std::vector<double> g((size_t)3 * nv * ld), p((size_t)nv * ld);
+    std::vector<double> g((size_t)2 * pair_groups(3 * nv) * ld), p((size_t)2 * pair_groups(nv) * ld);
     HIPCHK(c, hipDeviceSynchronize());
     HIPCHK(c, hipMemcpy(g.data(), c->G, g.size() * sizeof(double), hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(p.data(), c->PHI, p.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < N + B; ++i) {
         const int r = c->hg.iperm[i];
         if (G_host)
-            for (int k = 0; k < 3 * nv; ++k) G_host[(size_t)3 * nv * r + k] = g[(size_t)k * ld + i];
+            for (int k = 0; k < 3 * nv; ++k) G_host[(size_t)3 * nv * r + k] = g[pidx(ld, k, i)];
         if (PHI_host && i < N)
-            for (int v = 0; v < nv; ++v) PHI_host[(size_t)nv * r + v] = p[(size_t)v * ld + i];
+            for (int v = 0; v < nv; ++v) PHI_host[(size_t)nv * r + v] = p[pidx(ld, v, i)];
     }
     return FCFD_OK;
 }
@@ -632,7 +657,7 @@ static int ensure_tangent(fcfd_ctx* c) {
     const DevGrid& g = c->dg;
     int rc;
     const size_t nv = g.nv;
-    if ((rc = dalloc(c, 3 * nv * g.ld, &c->dG)) || (rc = dalloc(c, nv * g.ld, &c->dPHI)) ||
+    if ((rc = dalloc(c, 2 * pair_groups(3 * (int)nv) * g.ld, &c->dG)) || (rc = dalloc(c, 2 * pair_groups((int)nv) * g.ld, &c->dPHI)) ||
         (rc = dalloc(c, nv * g.F, &c->dFX)) || (rc = dalloc(c, nv * g.ld, &c->jv_in)) ||
         (rc = dalloc(c, nv * g.ld, &c->jv_out)) || (rc = dalloc(c, nv * g.ld, &c->jv_R)))
         return rc;

@@ -9,8 +9,8 @@ namespace fcfd {
 // All arrays are device pointers; cells and faces are in internal (RCM)
 // order.  Cell arrays are padded to `ld` entries; interior cells [0,N), ghosts
 // [N, N+B).  Per-cell variable arrays are [nv][ld] (p, u, v, w, T, and nu~
-// with the SA model), gradients [3 nv][ld], limiters [nv][ld], face fluxes
-// [nv][F].  (Structure of arrays: consecutive faces have consecutive parent
+// with the SA model), gradients 3 nv and limiters nv components per cell in
+// row pairs (pidx below), face fluxes [nv][F].  (Structure of arrays: consecutive faces have consecutive parent
 // cells, so a wavefront's parent-side gathers coalesce; per-cell records and
 // face-major flux rows were measured slower.)  A half-face entry hf_face[h] is the internal face id f when the
 // owning cell is the face's parent and ~f when it is the neighbour; the
@@ -23,15 +23,46 @@ struct DevGrid {
     int ref_first, ref_last;            // internal ids of reference cells 0 and N-1
     const double *ccx, *ccy, *ccz, *vol, *lscale;
     const int* gslice;                  // N/64+1 entry offsets of 64-wide gradient slices
-    const int* gidx;
-    const double *gwx, *gwy, *gwz;
+    const double2 *gwa, *gwb;           // per entry (w_x, w_y) and (w_z, neighbour index as int64 bits)
     const int *hf_ptr, *hf_face;        // N+1, nhf
     const int* blk_cell;                // nblk+1 first cell of each flux block
     const int *fpar, *fnei, *fbc;
     const double *fnx, *fny, *fnz, *t1x, *t1y, *t1z, *farea, *fcx, *fcy, *fcz, *fwp, *fwn, *fwd, *fdiss;
     const int* gh_face;                 // B: boundary face of ghost N+i
     const BcDev* bc;
+    // the same face data packed for the face kernels (one 16-byte load per
+    // pair): fcell[f] = (parent, neighbour, bc, 0); fgeo [6][F] row pairs
+    // (n.x, n.y) (n.z, t1.x) (t1.y, t1.z) (area, c.x) (c.y, c.z) (w_parent, w_neighbour)
+    const int4* fcell;
+    const double2* fgeo;
+    const double2* cc2;                 // [2][ld] cell centroid and volume packed: (x, y), (z, V)
 };
+
+// Gradients ([3 nv] components per cell) and limiters ([nv]) are stored in
+// row pairs: component k of cell i at ((k / 2) ld + i) 2 + k % 2, the row
+// count rounded up to even (the pad row holds 0).  A lane then loads two
+// components of its cell with one 16-byte load while the lanes of a
+// wavefront still read adjacent cells, which halves the load instructions of
+// the gather kernels (bound by their memory pipeline, not by bytes).
+__host__ __device__ constexpr int pair_groups(int rows) { return (rows + 1) / 2; }
+__host__ __device__ __forceinline__ size_t pidx(size_t ld, int k, size_t i) { return ((size_t)(k >> 1) * ld + i) * 2 + (k & 1); }
+
+template <int K>
+__device__ __forceinline__ void store_pairs(double* __restrict__ A, size_t ld, size_t i, const double (&v)[K]) {
+    double2* A2 = reinterpret_cast<double2*>(A);
+#pragma unroll
+    for (int j = 0; j < pair_groups(K); ++j) A2[(size_t)j * ld + i] = make_double2(v[2 * j], 2 * j + 1 < K ? v[2 * j + 1] : 0.0);
+}
+template <int K>
+__device__ __forceinline__ void load_pairs(const double* __restrict__ A, size_t ld, size_t i, double (&v)[K]) {
+    const double2* A2 = reinterpret_cast<const double2*>(A);
+#pragma unroll
+    for (int j = 0; j < pair_groups(K); ++j) {
+        const double2 t = A2[(size_t)j * ld + i];
+        v[2 * j] = t.x;
+        if (2 * j + 1 < K) v[2 * j + 1] = t.y;
+    }
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2: workgroup b runs on XCD b % 8.  xcd_block() renumbers them so that XCD x

@@ -12,17 +12,28 @@
 
 namespace fcfd {
 
-__device__ __forceinline__ void load_geo(const DevGrid& g, int f, FaceGeo& F) {
-    F.n = mk<double>(g.fnx[f], g.fny[f], g.fnz[f]);
-    F.t1 = mk<double>(g.t1x[f], g.t1y[f], g.t1z[f]);
-    F.t2 = vnorm(vcross(F.n, F.t1));
-    F.area = g.farea[f];
-    F.bc = g.fbc[f];
+__device__ __forceinline__ double2 face_pair(const DevGrid& g, int row, int f) { return g.fgeo[(size_t)row * g.F + f]; }
+
+__device__ __forceinline__ Vec<double> face_centroid(const DevGrid& g, int f) {
+    const double2 a = face_pair(g, 3, f), b = face_pair(g, 4, f);
+    return mk<double>(a.y, b.x, b.y);
 }
 
-__device__ __forceinline__ Vec<double> centroid(const DevGrid& g, int c) { return mk<double>(g.ccx[c], g.ccy[c], g.ccz[c]); }
+__device__ __forceinline__ void load_geo(const DevGrid& g, int f, FaceGeo& F) {
+    const double2 a0 = face_pair(g, 0, f), a1 = face_pair(g, 1, f), a2 = face_pair(g, 2, f), a3 = face_pair(g, 3, f);
+    F.n = mk<double>(a0.x, a0.y, a1.x);
+    F.t1 = mk<double>(a1.y, a2.x, a2.y);
+    F.t2 = vnorm(vcross(F.n, F.t1));
+    F.area = a3.x;
+    F.bc = g.fcell[f].z;
+}
 
-// primal cell loader: W [NV][ld], gradients [3 NV][ld], limiters [NV][ld]
+__device__ __forceinline__ Vec<double> centroid(const DevGrid& g, int c) {
+    const double2 a = g.cc2[c], b = g.cc2[(size_t)g.ld + c];
+    return mk<double>(a.x, a.y, b.x);
+}
+
+// primal cell loader: W [NV][ld], gradients and limiters in row pairs
 template <int NV>
 struct CellLoader {
     const double* __restrict__ W;
@@ -32,11 +43,14 @@ struct CellLoader {
     double phi_const;
     size_t ld;
     __device__ __forceinline__ void load(int c, bool ghost, double q[NV], Vec<double> gr[NV], double ph[NV]) const {
+        double g[3 * NV], pl[NV];
+        load_pairs<3 * NV>(G, ld, c, g);
+        if (has_phi && !ghost) load_pairs<NV>(PHI, ld, c, pl);
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             q[v] = W[v * ld + c];
-            gr[v] = mk<double>(G[(3 * v) * ld + c], G[(3 * v + 1) * ld + c], G[(3 * v + 2) * ld + c]);
-            ph[v] = (has_phi && !ghost) ? PHI[v * ld + c] : (ghost ? 0.0 : phi_const);
+            gr[v] = mk<double>(g[3 * v], g[3 * v + 1], g[3 * v + 2]);
+            ph[v] = (has_phi && !ghost) ? pl[v] : (ghost ? 0.0 : phi_const);
         }
     }
 };
@@ -56,14 +70,18 @@ struct DualLoader {
         Vec<double> gv[NV];
         p.load(c, ghost, qv, gv, phv);
         const size_t ld = p.ld;
+        double dg[3 * NV], dp[NV];
+        if (FULL) {
+            load_pairs<3 * NV>(dG, ld, c, dg);
+            if (p.has_phi && !ghost) load_pairs<NV>(dPHI, ld, c, dp);
+        }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
             double dq = (FULL || c < N) ? dW[v * ld + c] : 0.0;
             q[v] = Dual(qv[v], dq);
             if (FULL) {
-                gr[v] = mk<Dual>(Dual(gv[v].x, dG[(3 * v) * ld + c]), Dual(gv[v].y, dG[(3 * v + 1) * ld + c]),
-                                 Dual(gv[v].z, dG[(3 * v + 2) * ld + c]));
-                ph[v] = Dual(phv[v], (p.has_phi && !ghost) ? dPHI[v * ld + c] : 0.0);
+                gr[v] = mk<Dual>(Dual(gv[v].x, dg[3 * v]), Dual(gv[v].y, dg[3 * v + 1]), Dual(gv[v].z, dg[3 * v + 2]));
+                ph[v] = Dual(phv[v], (p.has_phi && !ghost) ? dp[v] : 0.0);
             } else {
                 gr[v] = mk<Dual>(Dual(gv[v].x), Dual(gv[v].y), Dual(gv[v].z));
                 ph[v] = Dual(phv[v]);
@@ -90,9 +108,10 @@ template <int VISC, int NV, typename S, typename Loader>
 __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_, const Params& P, int f, FaceCtx<S, NV>& X) {
     FaceGeo& F = X.F;
     load_geo(g, f, F);
-    const int p = g.fpar[f], n = g.fnei[f];
+    const int4 cr = g.fcell[f];
+    const int p = cr.x, n = cr.y;
     X.p = p; X.n = n;
-    Vec<double> xf = mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]);
+    Vec<double> xf = face_centroid(g, f);
     Vec<double> xp = centroid(g, p), xn = centroid(g, n);
     F.l2r = vsub(xn, xp);
     S q[NV], ph[NV];
@@ -120,13 +139,20 @@ __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_,
         if (F.bc >= 0) {
             X.fs.gu = gp[1]; X.fs.gv = gp[2]; X.fs.gw = gp[3]; X.fs.gT = gp[4];
         } else {   // face average (variable.h:134-147), two-entry stencil in key order
-            double wp = g.fwp[f], wn = g.fwn[f];
+            const double2 w = face_pair(g, 5, f);
+            const double wp = w.x, wn = w.y;
             X.fs.gu = vadd(vsc(wp, gp[1]), vsc(wn, gn[1]));
             X.fs.gv = vadd(vsc(wp, gp[2]), vsc(wn, gn[2]));
             X.fs.gw = vadd(vsc(wp, gp[3]), vsc(wn, gn[3]));
             X.fs.gT = vadd(vsc(wp, gp[4]), vsc(wn, gn[4]));
         }
-        if constexpr (NV == 6) X.fs.gn = (F.bc >= 0) ? gp[5] : vadd(vsc(g.fwp[f], gp[5]), vsc(g.fwn[f], gn[5]));
+        if constexpr (NV == 6) {
+            if (F.bc >= 0) X.fs.gn = gp[5];
+            else {
+                const double2 w = face_pair(g, 5, f);
+                X.fs.gn = vadd(vsc(w.x, gp[5]), vsc(w.y, gn[5]));
+            }
+        }
     }
 }
 

@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256) diag_kernel(DevGrid g, Params P, const do
 #pragma unroll
     for (int v = 0; v < NV; ++v) q[v] = W[v * ld + c];
     if constexpr (NV == 6) {   // SA source block, gradients frozen
-        auto gvec = [&](int v) { return mk<double>(G[(3 * v) * ld + c], G[(3 * v + 1) * ld + c], G[(3 * v + 2) * ld + c]); };
+        auto gvec = [&](int v) { return mk<double>(G[pidx(ld, 3 * v, c)], G[pidx(ld, 3 * v + 1, c)], G[pidx(ld, 3 * v + 2, c)]); };
         const Vec<double> gu = gvec(1), gv = gvec(2), gw = gvec(3), gn = gvec(5);
         const double vol = g.vol[c], d = g.wd[c];
         const double S0 = vol * sa_source<double>(m_rho(P, q[0], q[4]), m_visc(P, q[4]), q[5], gu, gv, gw, gn, d);

@@ -26,7 +26,7 @@ __device__ __forceinline__ void limiter_phi(const DevGrid& g, const Params& P, c
             int fs = g.hf_face[h];
             int f = fs >= 0 ? fs : ~fs;
             int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
-            Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
+            Vec<double> c2f = vsub(face_centroid(g, f), xc);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 S dU = src.nb(h, nb, v) - q0[v];
@@ -54,7 +54,7 @@ __device__ __forceinline__ void limiter_phi(const DevGrid& g, const Params& P, c
         for (int h = h0; h < h1; ++h) {
             int fs = g.hf_face[h];
             int f = fs >= 0 ? fs : ~fs;
-            Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
+            Vec<double> c2f = vsub(face_centroid(g, f), xc);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 S dM = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;

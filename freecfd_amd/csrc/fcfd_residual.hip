@@ -72,8 +72,9 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
         int s = c >> 6, lane = c & 63;
         int e0 = g.gslice[s], e1 = g.gslice[s + 1];
         for (int e = e0 + lane; e < e1; e += 64) {
-            int j = g.gidx[e];
-            double wx = g.gwx[e], wy = g.gwy[e], wz = g.gwz[e];
+            const double2 a = g.gwa[e], b = g.gwb[e];
+            const int j = (int)__double_as_longlong(b.y);
+            const double wx = a.x, wy = a.y, wz = b.x;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 double q = W[v * ld + j];
@@ -84,8 +85,7 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
         }
     }
     const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
-#pragma unroll
-    for (int k = 0; k < 3 * NV; ++k) G[k * ld + c] = gr[k];
+    store_pairs<3 * NV>(G, ld, c, gr);
     double q0[NV], phi[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) { q0[v] = W[v * ld + c]; phi[v] = 1.; }
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
             int fs = g.hf_face[h];
             int f = fs >= 0 ? fs : ~fs;
             int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
-            Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
+            Vec<double> c2f = vsub(face_centroid(g, f), xc);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 double dU = W[v * ld + nb] - q0[v];
@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
         for (int h = h0; h < h1; ++h) {
             int fs = g.hf_face[h];
             int f = fs >= 0 ? fs : ~fs;
-            Vec<double> c2f = vsub(mk<double>(g.fcx[f], g.fcy[f], g.fcz[f]), xc);
+            Vec<double> c2f = vsub(face_centroid(g, f), xc);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 double dM = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
@@ -155,8 +155,7 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
         }
     }
     if (LIM != FCFD_LIMITER_NONE) {
-#pragma unroll
-        for (int v = 0; v < NV; ++v) PHI[v * ld + c] = phi[v];
+        store_pairs<NV>(PHI, ld, c, phi);
     }
     // boundary ghosts take their parent's gradient (ns.cc:320-333)
     for (int h = h0; h < h1; ++h) {
@@ -164,8 +163,7 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
         if (fs < 0) continue;
         int gi = g.fnei[fs];
         if (gi < g.N) continue;
-#pragma unroll
-        for (int k = 0; k < 3 * NV; ++k) G[k * ld + gi] = gr[k];
+        store_pairs<3 * NV>(G, ld, gi, gr);
     }
 }
 

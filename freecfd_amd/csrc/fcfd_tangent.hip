@@ -73,8 +73,9 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
         int s = c >> 6, lane = c & 63;
         int e0 = g.gslice[s], e1 = g.gslice[s + 1];
         for (int e = e0 + lane; e < e1; e += 64) {
-            int j = g.gidx[e];
-            double wx = g.gwx[e], wy = g.gwy[e], wz = g.gwz[e];
+            const double2 a = g.gwa[e], b = g.gwb[e];
+            const int j = (int)__double_as_longlong(b.y);
+            const double wx = a.x, wy = a.y, wz = b.x;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 Dual q(W[v * ld + j], dW[v * ld + j]);
@@ -85,24 +86,30 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
         }
     }
     const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
+    double gv[3 * NV], gd[3 * NV];
 #pragma unroll
-    for (int k = 0; k < 3 * NV; ++k) { G[k * ld + c] = gr[k].v; dG[k * ld + c] = gr[k].d; }
+    for (int k = 0; k < 3 * NV; ++k) { gv[k] = gr[k].v; gd[k] = gr[k].d; }
+    store_pairs<3 * NV>(G, ld, c, gv);
+    store_pairs<3 * NV>(dG, ld, c, gd);
     Dual q0[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) q0[v] = Dual(W[v * ld + c], dW[v * ld + c]);
     Dual phi[NV];
     limiter_phi<LIM, NV>(g, P, TangentNb{W, dW, ld}, c, q0, gr, dref, phi);
     if (LIM != FCFD_LIMITER_NONE) {
+        double pv[NV], pd[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) { PHI[v * ld + c] = phi[v].v; dPHI[v * ld + c] = phi[v].d; }
+        for (int v = 0; v < NV; ++v) { pv[v] = phi[v].v; pd[v] = phi[v].d; }
+        store_pairs<NV>(PHI, ld, c, pv);
+        store_pairs<NV>(dPHI, ld, c, pd);
     }
     for (int h = h0; h < h1; ++h) {
         int fs = g.hf_face[h];
         if (fs < 0) continue;
         int gi = g.fnei[fs];
         if (gi < g.N) continue;
-#pragma unroll
-        for (int k = 0; k < 3 * NV; ++k) { G[k * ld + gi] = gr[k].v; dG[k * ld + gi] = gr[k].d; }
+        store_pairs<3 * NV>(G, ld, gi, gv);
+        store_pairs<3 * NV>(dG, ld, gi, gd);
     }
 }
 
@@ -130,9 +137,9 @@ __global__ void __launch_bounds__(256) sa_source_t_kernel(DevGrid g, Params P, c
     Dual p(W[c], dW[c]), T(W[4 * ld + c], dW[4 * ld + c]), nut(W[5 * ld + c], dW[5 * ld + c]);
     auto gvec = [&](int v) {
         Vec<Dual> r;
-        r.x = Dual(G[(3 * v) * ld + c], FULL ? dG[(3 * v) * ld + c] : 0.0);
-        r.y = Dual(G[(3 * v + 1) * ld + c], FULL ? dG[(3 * v + 1) * ld + c] : 0.0);
-        r.z = Dual(G[(3 * v + 2) * ld + c], FULL ? dG[(3 * v + 2) * ld + c] : 0.0);
+        r.x = Dual(G[pidx(ld, 3 * v, c)], FULL ? dG[pidx(ld, 3 * v, c)] : 0.0);
+        r.y = Dual(G[pidx(ld, 3 * v + 1, c)], FULL ? dG[pidx(ld, 3 * v + 1, c)] : 0.0);
+        r.z = Dual(G[pidx(ld, 3 * v + 2, c)], FULL ? dG[pidx(ld, 3 * v + 2, c)] : 0.0);
         return r;
     };
     Dual S = sa_source<Dual>(m_rho(P, p, T), m_visc(P, T), nut, gvec(1), gvec(2), gvec(3), gvec(5), g.wd[c]);

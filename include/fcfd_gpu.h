@@ -222,9 +222,11 @@ int  fcfd_cells_to_host(fcfd_ctx* ctx, const double* X_dev, int nvar, int includ
  * no host synchronisation. */
 int  fcfd_residual(fcfd_ctx* ctx, double* W_dev, double* R_dev, void* stream);
 
-/* Gradients [3nv][ld_cell] (dp/dx,dp/dy,dp/dz, du/.., dv/.., dw/.., dT/..
- * [, dnu~/..]; ghosts carry their parent's) and limiters [nv][ld_cell] of the
- * last fcfd_residual call; device pointers owned by the context. */
+/* Gradients (3nv components per cell: dp/dx,dp/dy,dp/dz, du/.., dv/.., dw/..,
+ * dT/.. [, dnu~/..]; ghosts carry their parent's) and limiters (nv per cell)
+ * of the last fcfd_residual call; device pointers owned by the context.
+ * Stored in row pairs: component k of cell i is at ((k/2)*ld_cell + i)*2 +
+ * k%2 (fcfd_gradients_to_host returns them per cell in reference order). */
 int  fcfd_internal_buffers(fcfd_ctx* ctx, const double** grad, const double** phi);
 
 /* The same, unpacked to the host in reference numbering: G_host (N+B) x 3nv,
