@@ -78,7 +78,7 @@ class NewtonStats(C.Structure):
                 ("ms_update", C.c_double), ("nut_norm", C.c_double)]
 
 
-PC_BLOCK_JACOBI, PC_BLOCK_SGS = 0, 1
+PC_BLOCK_JACOBI, PC_BLOCK_SGS, PC_MULTIGRID = 0, 1, 2
 
 
 def newton_opts(cfl=10.0, jacobian=JAC_FD, fp32=False, restart=30, max_iters=10, rtol=1e-6, abstol=1e-12, knoll=True,

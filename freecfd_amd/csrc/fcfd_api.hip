@@ -16,6 +16,7 @@
 #include "fcfd_internal.h"
 #include "fcfd_jacobian.h"
 #include "fcfd_krylov.h"
+#include "fcfd_mg.h"
 #include "fcfd_tangent.h"
 
 #include <chrono>
@@ -34,6 +35,8 @@ struct fcfd_ctx {
     double *G = nullptr, *PHI = nullptr, *dref = nullptr, *FX = nullptr;
     double* SRC = nullptr;   // [ld] V S of the SA source (nv = 6), from grad_lim
     double* adj_v = nullptr; // [nv][ld] J^T x inside the adjoint step's operator
+    Multigrid mg;            // aggregation multigrid hierarchy (built on first use)
+    bool mg_tried = false;
     int* status = nullptr;
     double *JL = nullptr, *JR = nullptr, *Dg = nullptr, *zero_upd = nullptr;   // FD Jacobian (lazy)
     // Newton-Krylov workspace (lazy)
@@ -431,6 +434,7 @@ void fcfd_destroy(fcfd_ctx* c) {
     if (!c) return;
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
+    mg_free(c->mg);
     delete c;
 }
 
@@ -823,6 +827,27 @@ static int ensure_newton(fcfd_ctx* c, const fcfd_newton_opts* o) {
     return FCFD_OK;
 }
 
+// aggregation multigrid hierarchy on the interior face graph (host, once)
+static int ensure_mg(fcfd_ctx* c) {
+    if (c->mg.ready) return FCFD_OK;
+    if (c->mg_tried) return fail(c, FCFD_ENOTSUP, "the mesh is too small for a multigrid hierarchy");
+    c->mg_tried = true;
+    const HostGrid& g = c->hg;
+    std::vector<int> par(g.F), nei(g.F);
+    for (int i = 0; i < g.F; ++i) {
+        int f = g.ifperm[i];
+        par[i] = g.perm[g.fparent[f]];
+        nei[i] = g.perm[g.fneigh[f]];
+    }
+    int rc = mg_build(c->dg, par, nei, c->mg, 12, 256);
+    if (rc) {
+        mg_free(c->mg);
+        return fail(c, rc, "multigrid set-up failed (HIP error)");
+    }
+    if (!c->mg.ready) return fail(c, FCFD_ENOTSUP, "the mesh is too small for a multigrid hierarchy");
+    return FCFD_OK;
+}
+
 static int ensure_colors(fcfd_ctx* c) {
     if (c->colors.n > 0) return FCFD_OK;
     const HostGrid& g = c->hg;
@@ -873,10 +898,13 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     const bool mfree = o->jacobian != FCFD_JAC_FD;
     if (mfree && (rc0 = ensure_tangent(c))) return rc0;
     if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad Krylov options");
-    if (o->precond != FCFD_PC_BLOCK_JACOBI && o->precond != FCFD_PC_BLOCK_SGS) return fail(c, FCFD_EINVAL, "unknown preconditioner");
+    if (o->precond != FCFD_PC_BLOCK_JACOBI && o->precond != FCFD_PC_BLOCK_SGS && o->precond != FCFD_PC_MULTIGRID)
+        return fail(c, FCFD_EINVAL, "unknown preconditioner");
+    const bool mgp = o->precond == FCFD_PC_MULTIGRID;
     const bool sgs = o->precond == FCFD_PC_BLOCK_SGS;
     const int sweeps = std::max(o->sweeps, 1);
-    if (sgs && (rc0 = ensure_colors(c))) return rc0;
+    if ((sgs || mgp) && (rc0 = ensure_colors(c))) return rc0;
+    if (mgp && (rc0 = ensure_mg(c))) return rc0;
     const DevGrid& g = c->dg;
     hipStream_t s = (hipStream_t)stream;
     const size_t n5 = (size_t)g.nv * g.ld;      // one [nv][ld] vector
@@ -908,8 +936,12 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
             dev_convert<double, float>(b2 * g.F, c->JR, c->JR32, s);
         }
         dev_convert<double, float>(n5, c->R, c->b32, s);
-    } else {
+    } else if (!mgp) {
         dev_bj_invert<double>(g, c->Dg, c->Dinv64, c->status, s);
+    }
+    if (mgp) {   // FP64 V-cycle on the FD blocks: fine inverse + coarse levels
+        if (!c->Dinv64 && (rc = dalloc(c, b2 * g.ld, &c->Dinv64))) return rc;
+        mg_setup(g, c->mg, c->Dg, c->JL, c->JR, c->Dinv64, c->status, s);
     }
     HIPCHK(c, hipStreamSynchronize(s));
     auto t2 = clk::now();
@@ -925,39 +957,45 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
                    o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss);
         launch_newton_op(g, c->P, W_dev, c->dt, x, c->jv_R, y, ss);
     };
+    // preconditioners: block Jacobi, block SGS, or the FP64 multigrid V-cycle
+    // (FP32 Krylov vectors converted at its boundary)
+    auto prec64 = [&](const double* x, double* y, hipStream_t ss) {
+        if (mgp) mg_vcycle(g, c->colors, c->mg, c->Dg, c->JL, c->JR, c->Dinv64, x, y, sweeps, ss);
+        else if (sgs) dev_bsgs_apply<double>(g, c->colors, c->Dinv64, c->JL, c->JR, x, y, sweeps, ss);
+        else dev_bj_apply<double>(g, c->Dinv64, x, y, ss);
+    };
+    auto prec32 = [&](const float* x, float* y, hipStream_t ss) {
+        if (mgp) {
+            dev_convert<float, double>(n5, x, c->mg.fine_in, ss);
+            mg_vcycle(g, c->colors, c->mg, c->Dg, c->JL, c->JR, c->Dinv64, c->mg.fine_in, c->mg.fine_out, sweeps, ss);
+            dev_convert<double, float>(n5, c->mg.fine_out, y, ss);
+        } else if (sgs) {
+            dev_bsgs_apply<float>(g, c->colors, c->Dinv32, c->JL32, c->JR32, x, y, sweeps, ss);
+        } else {
+            dev_bj_apply<float>(g, c->Dinv32, x, y, ss);
+        }
+    };
     if (fp32 && mfree) {
         LinOp<float> A{[&](const float* x, float* y, hipStream_t ss) {
             dev_convert<float, double>(n5, x, c->jv_out, ss);
             jvp_op(c->jv_out, c->jv_out, ss);
             dev_convert<double, float>(n5, c->jv_out, y, ss);
         }};
-        LinOp<float> M{[&](const float* x, float* y, hipStream_t ss) {
-            if (sgs) dev_bsgs_apply<float>(g, c->colors, c->Dinv32, c->JL32, c->JR32, x, y, sweeps, ss);
-            else dev_bj_apply<float>(g, c->Dinv32, x, y, ss);
-        }};
+        LinOp<float> M{prec32};
         rc = gmres<float>(g, A, M, c->b32, c->x32, go, c->w32, gs, s);
         if (!rc) dev_convert<float, double>(n5, c->x32, c->dW, s);
     } else if (mfree) {
         LinOp<double> A{[&](const double* x, double* y, hipStream_t ss) { jvp_op(x, y, ss); }};
-        LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) {
-            if (sgs) dev_bsgs_apply<double>(g, c->colors, c->Dinv64, c->JL, c->JR, x, y, sweeps, ss);
-            else dev_bj_apply<double>(g, c->Dinv64, x, y, ss);
-        }};
+        LinOp<double> M{prec64};
         rc = gmres<double>(g, A, M, c->R, c->dW, go, c->w64, gs, s);
     } else if (fp32) {
         LinOp<float> A{[&](const float* x, float* y, hipStream_t ss) { dev_bsr_matvec<float>(g, c->D32, c->JL32, c->JR32, x, y, ss); }};
-        LinOp<float> M{[&](const float* x, float* y, hipStream_t ss) {
-            if (sgs) dev_bsgs_apply<float>(g, c->colors, c->Dinv32, c->JL32, c->JR32, x, y, sweeps, ss);
-            else dev_bj_apply<float>(g, c->Dinv32, x, y, ss);
-        }};
+        LinOp<float> M{prec32};
         rc = gmres<float>(g, A, M, c->b32, c->x32, go, c->w32, gs, s);
         if (!rc) dev_convert<float, double>(n5, c->x32, c->dW, s);
     } else {
         LinOp<double> A{[&](const double* x, double* y, hipStream_t ss) { dev_bsr_matvec<double>(g, c->Dg, c->JL, c->JR, x, y, ss); }};
-        LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) {
-            if (sgs) dev_bsgs_apply<double>(g, c->colors, c->Dinv64, c->JL, c->JR, x, y, sweeps, ss);
-            else dev_bj_apply<double>(g, c->Dinv64, x, y, ss);
-        }};
+        LinOp<double> M{prec64};
         rc = gmres<double>(g, A, M, c->R, c->dW, go, c->w64, gs, s);
     }
     if (rc) return fail(c, rc, "GMRES failed (HIP error)");

@@ -43,10 +43,13 @@ struct Colors {
     std::vector<int> ptr;
     const int* cells = nullptr;
 };
-// x = M^{-1} b by `sweeps` symmetric multicolour block Gauss-Seidel sweeps from x = 0
+// x = M^{-1} b by `sweeps` symmetric multicolour block Gauss-Seidel sweeps
+// from x = 0 (zero_init) or from the x given
 template <typename T>
 void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T* JL, const T* JR, const T* b, T* x, int sweeps,
-                    hipStream_t s);
+                    hipStream_t s, bool zero_init = true);
+// batched nv x nv inverses of n row-major blocks stored [nv^2][ld]
+void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, int* status, hipStream_t s);
 // out_host: sum dp^2/dt^2, sum |dV|^2/dt^2, sum dT^2/dt^2, NaN/Inf count, sum dnu~^2/dt^2
 hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
                       double out_host[5], hipStream_t s);

@@ -361,11 +361,17 @@ template void dev_bj_invert<float>(const DevGrid&, const double*, float*, int*, 
 template void dev_bj_apply<double>(const DevGrid&, const double*, const double*, double*, hipStream_t);
 template void dev_bj_apply<float>(const DevGrid&, const float*, const float*, float*, hipStream_t);
 
+void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, int* status, hipStream_t s) {
+    if (n <= 0) return;
+    if (nv == 6) hipLaunchKernelGGL((bj_invert_kernel<double, 6>), dim3((n + 255) / 256), dim3(256), 0, s, n, ld, D, Dinv, status);
+    else hipLaunchKernelGGL((bj_invert_kernel<double, 5>), dim3((n + 255) / 256), dim3(256), 0, s, n, ld, D, Dinv, status);
+}
+
 template <typename T>
 void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T* JL, const T* JR, const T* b, T* x, int sweeps,
-                    hipStream_t s) {
+                    hipStream_t s, bool zero_init) {
     const size_t n = (size_t)g.nv * g.ld;
-    hipLaunchKernelGGL(scale_copy_kernel<T>, ew_grid(n), dim3(256), 0, s, n, T(0), b, x);   // x = 0
+    if (zero_init) hipLaunchKernelGGL(scale_copy_kernel<T>, ew_grid(n), dim3(256), 0, s, n, T(0), b, x);   // x = 0
     auto one = [&](int k) {
         const int cnt = col.ptr[k + 1] - col.ptr[k];
         if (cnt <= 0) return;
@@ -381,9 +387,9 @@ void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T*
     }
 }
 template void dev_bsgs_apply<double>(const DevGrid&, const Colors&, const double*, const double*, const double*, const double*, double*,
-                                     int, hipStream_t);
+                                     int, hipStream_t, bool);
 template void dev_bsgs_apply<float>(const DevGrid&, const Colors&, const float*, const float*, const float*, const float*, float*, int,
-                                    hipStream_t);
+                                    hipStream_t, bool);
 
 hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
                       double out_host[5], hipStream_t s) {

@@ -1,0 +1,451 @@
+// Aggregation multigrid preconditioner (see fcfd_mg.h).
+//
+// Block conventions (those of the FD Jacobian and bsr_matvec_kernel): the
+// operator row of cell c is D_c x_c + sum over its faces of a block times the
+// other cell's x, the block being -JR_f^T when c is the face's parent and
+// +JL_f^T when c is its neighbour (JL/JR stored [(nv i + j) F + f] for row
+// j, column i).  Diagonal and coarse blocks are row-major, entry (j, i) at
+// [(nv j + i) nblk + k].  A Galerkin source code is (index << 2) | kind with
+// kind 0 = a diagonal block of the finer level, 1 = -JR_f^T (parent row,
+// neighbour column), 2 = +JL_f^T (neighbour row, parent column), 3 = an
+// off-diagonal block of a coarse finer level.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "fcfd_mg.h"
+
+namespace fcfd {
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <int NV, bool FINE>
+__global__ void __launch_bounds__(256) mg_assemble_kernel(int nblk, const int* __restrict__ ptr, const int* __restrict__ src,
+                                                          double* __restrict__ out, const double* __restrict__ D0, size_t ld0,
+                                                          const double* __restrict__ JL, const double* __restrict__ JR, size_t F,
+                                                          const double* __restrict__ Dl, size_t nl, const double* __restrict__ Al,
+                                                          size_t nnzl) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nblk) return;
+    double acc[NV * NV];
+#pragma unroll
+    for (int q = 0; q < NV * NV; ++q) acc[q] = 0.0;
+    for (int e = ptr[k]; e < ptr[k + 1]; ++e) {
+        const int code = src[e];
+        const size_t idx = (size_t)(code >> 2);
+        const int kind = code & 3;
+        if (FINE) {
+            if (kind == 0) {
+#pragma unroll
+                for (int q = 0; q < NV * NV; ++q) acc[q] += D0[q * ld0 + idx];
+            } else if (kind == 1) {
+#pragma unroll
+                for (int j = 0; j < NV; ++j)
+#pragma unroll
+                    for (int i = 0; i < NV; ++i) acc[NV * j + i] -= JR[(NV * i + j) * F + idx];
+            } else {
+#pragma unroll
+                for (int j = 0; j < NV; ++j)
+#pragma unroll
+                    for (int i = 0; i < NV; ++i) acc[NV * j + i] += JL[(NV * i + j) * F + idx];
+            }
+        } else {
+            if (kind == 0) {
+#pragma unroll
+                for (int q = 0; q < NV * NV; ++q) acc[q] += Dl[q * nl + idx];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV * NV; ++q) acc[q] += Al[q * nnzl + idx];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NV * NV; ++q) out[q * (size_t)nblk + k] = acc[q];
+}
+
+// one colour of a block Gauss-Seidel sweep on a coarse level
+template <int NV>
+__global__ void __launch_bounds__(256) mg_sgs_color_kernel(const int* __restrict__ cells, int count, size_t ld, size_t n, size_t nnz,
+                                                           const int* __restrict__ rptr, const int* __restrict__ col,
+                                                           const double* __restrict__ A, const double* __restrict__ Dinv,
+                                                           const double* __restrict__ b, double* __restrict__ x) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int I = cells[t];
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = b[j * ld + I];
+    for (int k = rptr[I]; k < rptr[I + 1]; ++k) {
+        const int J = col[k];
+        double xj[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) xj[i] = x[i * ld + J];
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) acc[j] -= A[(NV * j + i) * nnz + k] * xj[i];
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) s += Dinv[(NV * j + i) * n + I] * acc[i];
+        x[j * ld + I] = s;
+    }
+}
+
+// r = b - A x on a coarse level
+template <int NV>
+__global__ void __launch_bounds__(256) mg_residual_kernel(int n, size_t ld, size_t nnz, const int* __restrict__ rptr,
+                                                          const int* __restrict__ col, const double* __restrict__ D,
+                                                          const double* __restrict__ A, const double* __restrict__ b,
+                                                          const double* __restrict__ x, double* __restrict__ r) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n) return;
+    double acc[NV], xi[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) { acc[j] = b[j * ld + I]; xi[j] = x[j * ld + I]; }
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[j] -= D[(NV * j + i) * (size_t)n + I] * xi[i];
+    for (int k = rptr[I]; k < rptr[I + 1]; ++k) {
+        const int J = col[k];
+        double xj[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) xj[i] = x[i * ld + J];
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) acc[j] -= A[(NV * j + i) * nnz + k] * xj[i];
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) r[j * ld + I] = acc[j];
+}
+
+// bc[I] = sum of rf over the members of aggregate I (member order fixed)
+template <int NV>
+__global__ void __launch_bounds__(256) mg_restrict_kernel(int n, const int* __restrict__ mptr, const int* __restrict__ mem,
+                                                          const double* __restrict__ rf, size_t ldf, double* __restrict__ bc,
+                                                          size_t ldc) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n) return;
+    double s[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) s[j] = 0.0;
+    for (int e = mptr[I]; e < mptr[I + 1]; ++e) {
+        const int i = mem[e];
+#pragma unroll
+        for (int j = 0; j < NV; ++j) s[j] += rf[j * ldf + i];
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) bc[j * ldc + I] = s[j];
+}
+
+// xf[i] += xc[agg[i]]
+template <int NV>
+__global__ void __launch_bounds__(256) mg_prolong_kernel(int nf, const int* __restrict__ agg, const double* __restrict__ xc, size_t ldc,
+                                                         double* __restrict__ xf, size_t ldf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nf) return;
+    const int I = agg[i];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) xf[j * ldf + i] += xc[j * ldc + I];
+}
+
+__global__ void __launch_bounds__(256) mg_zero_kernel(size_t n, double* __restrict__ x) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// host: hierarchy
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+int mg_upload(Multigrid& mg, const std::vector<T>& h, const T** out) {
+    void* p = nullptr;
+    size_t n = std::max<size_t>(h.size(), 1);
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return FCFD_EHIP;
+    mg.allocs.push_back(p);
+    if (!h.empty() && hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return FCFD_EHIP;
+    *out = (const T*)p;
+    return FCFD_OK;
+}
+
+int mg_alloc(Multigrid& mg, size_t n, double** out) {
+    void* p = nullptr;
+    n = std::max<size_t>(n, 1);
+    if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) return FCFD_EHIP;
+    mg.allocs.push_back(p);
+    if (hipMemset(p, 0, n * sizeof(double)) != hipSuccess) return FCFD_EHIP;
+    *out = (double*)p;
+    return FCFD_OK;
+}
+
+// plain aggregation of an undirected graph (CSR, no self loops): a free cell
+// whose neighbours are all free seeds an aggregate with them; then every
+// leftover cell joins the first-phase aggregate of its first aggregated
+// neighbour; isolated leftovers form their own aggregates
+int aggregate(int n, const std::vector<int>& ptr, const std::vector<int>& adj, std::vector<int>& agg) {
+    agg.assign(n, -1);
+    int na = 0;
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        bool free_nb = true;
+        for (int k = ptr[i]; k < ptr[i + 1] && free_nb; ++k) free_nb = agg[adj[k]] < 0;
+        if (!free_nb) continue;
+        agg[i] = na;
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k) agg[adj[k]] = na;
+        ++na;
+    }
+    std::vector<int> first(agg);
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+            if (first[adj[k]] >= 0) { agg[i] = first[adj[k]]; break; }
+    }
+    for (int i = 0; i < n; ++i)
+        if (agg[i] < 0) agg[i] = na++;
+    return na;
+}
+
+void greedy_colors(int n, const std::vector<int>& ptr, const std::vector<int>& adj, std::vector<int>& cptr, std::vector<int>& cells,
+                   int& ncol) {
+    std::vector<int> col(n, -1);
+    ncol = 0;
+    std::vector<char> used;
+    for (int i = 0; i < n; ++i) {
+        used.assign(ncol + 1, 0);
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+            if (col[adj[k]] >= 0) used[col[adj[k]]] = 1;
+        int c = 0;
+        while (c < ncol && used[c]) ++c;
+        col[i] = c;
+        if (c == ncol) ++ncol;
+    }
+    cptr.assign(ncol + 1, 0);
+    for (int i = 0; i < n; ++i) cptr[col[i] + 1]++;
+    for (int c = 0; c < ncol; ++c) cptr[c + 1] += cptr[c];
+    std::vector<int> next(cptr.begin(), cptr.end() - 1);
+    cells.assign(n, 0);
+    for (int i = 0; i < n; ++i) cells[next[col[i]]++] = i;
+}
+
+}  // namespace
+
+int mg_build(const DevGrid& g, const std::vector<int>& par, const std::vector<int>& nei, Multigrid& mg, int max_levels,
+             int coarse_cells) {
+    const int N = g.N, F = g.F, nv = g.nv;
+    mg.nv = nv;
+    mg.lv.clear();
+    // level 0 adjacency: interior faces (both cells interior)
+    std::vector<int> ptr(N + 1, 0), adj;
+    for (int f = 0; f < F; ++f)
+        if (nei[f] < N) { ptr[par[f] + 1]++; ptr[nei[f] + 1]++; }
+    for (int i = 0; i < N; ++i) ptr[i + 1] += ptr[i];
+    adj.resize(ptr[N]);
+    {
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (int f = 0; f < F; ++f)
+            if (nei[f] < N) { adj[fill[par[f]]++] = nei[f]; adj[fill[nei[f]]++] = par[f]; }
+    }
+    MgLevel l0;
+    l0.n = N;
+    l0.ld = g.ld;
+    mg.lv.push_back(l0);
+    int rc;
+    if ((rc = mg_alloc(mg, (size_t)nv * g.ld, &mg.fine_r)) || (rc = mg_alloc(mg, (size_t)nv * g.ld, &mg.fine_in)) ||
+        (rc = mg_alloc(mg, (size_t)nv * g.ld, &mg.fine_out)))
+        return rc;
+    // the finer level's blocks as (row, col) pairs with their source codes,
+    // diagonal sources per row
+    std::vector<int> erow, ecol, ecode;   // off-diagonal blocks of the finer level
+    for (int f = 0; f < F; ++f) {
+        if (nei[f] >= N) continue;
+        erow.push_back(par[f]); ecol.push_back(nei[f]); ecode.push_back((f << 2) | 1);
+        erow.push_back(nei[f]); ecol.push_back(par[f]); ecode.push_back((f << 2) | 2);
+    }
+    int nf = N;
+    for (int level = 1; level < max_levels && nf > coarse_cells; ++level) {
+        std::vector<int> agg;
+        const int nc = aggregate(nf, ptr, adj, agg);
+        if (nc >= nf || nc < 1) break;
+        MgLevel L;
+        L.n = nc;
+        L.ld = ((nc + 63) / 64) * 64;
+        // members
+        std::vector<int> mptr(nc + 1, 0), mem(nf);
+        for (int i = 0; i < nf; ++i) mptr[agg[i] + 1]++;
+        for (int I = 0; I < nc; ++I) mptr[I + 1] += mptr[I];
+        {
+            std::vector<int> next(mptr.begin(), mptr.end() - 1);
+            for (int i = 0; i < nf; ++i) mem[next[agg[i]]++] = i;
+        }
+        // coarse adjacency (sorted rows) from the finer off-diagonal blocks
+        std::vector<std::vector<int>> rows(nc);
+        for (size_t e = 0; e < erow.size(); ++e) {
+            int I = agg[erow[e]], J = agg[ecol[e]];
+            if (I != J) rows[I].push_back(J);
+        }
+        std::vector<int> rptr(nc + 1, 0), col;
+        for (int I = 0; I < nc; ++I) {
+            auto& r = rows[I];
+            std::sort(r.begin(), r.end());
+            r.erase(std::unique(r.begin(), r.end()), r.end());
+            rptr[I + 1] = rptr[I] + (int)r.size();
+            col.insert(col.end(), r.begin(), r.end());
+        }
+        L.nnz = (int)col.size();
+        // Galerkin source lists: diagonal = member diagonals, then internal
+        // off-diagonal blocks (finer-level order); off-diagonal (I,J) = the
+        // finer blocks from I to J (finer-level order)
+        const int kdiag = 0, koff = level == 1 ? -1 : 3;
+        std::vector<std::vector<int>> dl(nc), ol(L.nnz);
+        for (int I = 0; I < nc; ++I)
+            for (int e = mptr[I]; e < mptr[I + 1]; ++e) dl[I].push_back((mem[e] << 2) | kdiag);
+        for (size_t e = 0; e < erow.size(); ++e) {
+            const int I = agg[erow[e]], J = agg[ecol[e]];
+            const int code = koff < 0 ? ecode[e] : (((int)e) << 2) | koff;
+            if (I == J) {
+                dl[I].push_back(code);
+            } else {
+                const int k = (int)(std::lower_bound(col.begin() + rptr[I], col.begin() + rptr[I + 1], J) - col.begin());
+                ol[k].push_back(code);
+            }
+        }
+        std::vector<int> dptr(nc + 1, 0), dsrc, optr(L.nnz + 1, 0), osrc;
+        for (int I = 0; I < nc; ++I) { dptr[I + 1] = dptr[I] + (int)dl[I].size(); dsrc.insert(dsrc.end(), dl[I].begin(), dl[I].end()); }
+        for (int k = 0; k < L.nnz; ++k) { optr[k + 1] = optr[k] + (int)ol[k].size(); osrc.insert(osrc.end(), ol[k].begin(), ol[k].end()); }
+        // colours
+        std::vector<int> cptr, cells;
+        int ncol = 0;
+        greedy_colors(nc, rptr, col, cptr, cells, ncol);
+        L.rptr_h = rptr;
+        if ((rc = mg_upload(mg, agg, &L.agg)) || (rc = mg_upload(mg, mptr, &L.mptr)) || (rc = mg_upload(mg, mem, &L.mem)) ||
+            (rc = mg_upload(mg, rptr, &L.rptr)) || (rc = mg_upload(mg, col, &L.col)) || (rc = mg_upload(mg, dptr, &L.dptr)) ||
+            (rc = mg_upload(mg, dsrc, &L.dsrc)) || (rc = mg_upload(mg, optr, &L.optr)) || (rc = mg_upload(mg, osrc, &L.osrc)) ||
+            (rc = mg_upload(mg, cells, &L.colors.cells)))
+            return rc;
+        L.colors.n = ncol;
+        L.colors.ptr = cptr;
+        const size_t b2 = (size_t)nv * nv;
+        if ((rc = mg_alloc(mg, b2 * nc, &L.D)) || (rc = mg_alloc(mg, b2 * std::max(L.nnz, 1), &L.A)) ||
+            (rc = mg_alloc(mg, b2 * nc, &L.Dinv)) || (rc = mg_alloc(mg, (size_t)nv * L.ld, &L.b)) ||
+            (rc = mg_alloc(mg, (size_t)nv * L.ld, &L.x)) || (rc = mg_alloc(mg, (size_t)nv * L.ld, &L.r)))
+            return rc;
+        mg.lv.push_back(L);
+        // next level's graph and off-diagonal block list
+        ptr = rptr;
+        adj = col;
+        erow.assign(L.nnz, 0);
+        ecol = col;
+        ecode.assign(L.nnz, 0);
+        for (int I = 0; I < nc; ++I)
+            for (int k = rptr[I]; k < rptr[I + 1]; ++k) erow[k] = I;
+        nf = nc;
+    }
+    mg.ready = mg.lv.size() > 1;
+    return FCFD_OK;
+}
+
+void mg_free(Multigrid& mg) {
+    for (void* p : mg.allocs) (void)hipFree(p);
+    mg.allocs.clear();
+    mg.lv.clear();
+    mg.ready = false;
+}
+
+// ---------------------------------------------------------------------------
+// host: set-up and V-cycle
+// ---------------------------------------------------------------------------
+template <int NV>
+static void setup_t(const DevGrid& g, Multigrid& mg, const double* D, const double* JL, const double* JR, int* status, hipStream_t s) {
+    for (size_t l = 1; l < mg.lv.size(); ++l) {
+        MgLevel& L = mg.lv[l];
+        if (l == 1) {
+            hipLaunchKernelGGL((mg_assemble_kernel<NV, true>), dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.dptr, L.dsrc, L.D, D,
+                               (size_t)g.ld, JL, JR, (size_t)g.F, nullptr, (size_t)0, nullptr, (size_t)0);
+            if (L.nnz > 0)
+                hipLaunchKernelGGL((mg_assemble_kernel<NV, true>), dim3((L.nnz + 255) / 256), dim3(256), 0, s, L.nnz, L.optr, L.osrc, L.A,
+                                   D, (size_t)g.ld, JL, JR, (size_t)g.F, nullptr, (size_t)0, nullptr, (size_t)0);
+        } else {
+            const MgLevel& P = mg.lv[l - 1];
+            hipLaunchKernelGGL((mg_assemble_kernel<NV, false>), dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.dptr, L.dsrc, L.D,
+                               nullptr, (size_t)0, nullptr, nullptr, (size_t)0, P.D, (size_t)P.n, P.A, (size_t)std::max(P.nnz, 1));
+            if (L.nnz > 0)
+                hipLaunchKernelGGL((mg_assemble_kernel<NV, false>), dim3((L.nnz + 255) / 256), dim3(256), 0, s, L.nnz, L.optr, L.osrc,
+                                   L.A, nullptr, (size_t)0, nullptr, nullptr, (size_t)0, P.D, (size_t)P.n, P.A,
+                                   (size_t)std::max(P.nnz, 1));
+        }
+        dev_block_invert(NV, L.n, (size_t)L.n, L.D, L.Dinv, status, s);
+    }
+}
+
+void mg_setup(const DevGrid& g, Multigrid& mg, const double* D, const double* JL, const double* JR, double* fine_Dinv, int* status,
+              hipStream_t s) {
+    dev_block_invert(g.nv, g.N, (size_t)g.ld, D, fine_Dinv, status, s);
+    if (g.nv == 6) setup_t<6>(g, mg, D, JL, JR, status, s);
+    else setup_t<5>(g, mg, D, JL, JR, status, s);
+}
+
+template <int NV>
+static void coarse_sgs(const MgLevel& L, int sweeps, hipStream_t s) {
+    auto one = [&](int c) {
+        const int cnt = L.colors.ptr[c + 1] - L.colors.ptr[c];
+        if (cnt <= 0) return;
+        hipLaunchKernelGGL(mg_sgs_color_kernel<NV>, dim3((cnt + 255) / 256), dim3(256), 0, s, L.colors.cells + L.colors.ptr[c], cnt,
+                           (size_t)L.ld, (size_t)L.n, (size_t)std::max(L.nnz, 1), L.rptr, L.col, L.A, L.Dinv, L.b, L.x);
+    };
+    for (int sw = 0; sw < sweeps; ++sw) {
+        for (int c = 0; c < L.colors.n; ++c) one(c);
+        for (int c = L.colors.n - 1; c >= 0; --c) one(c);
+    }
+}
+
+template <int NV>
+static void level_cycle(Multigrid& mg, size_t l, int sweeps, hipStream_t s) {
+    MgLevel& L = mg.lv[l];
+    const size_t nvec = (size_t)NV * L.ld;
+    hipLaunchKernelGGL(mg_zero_kernel, dim3((unsigned)std::min<size_t>((nvec + 255) / 256, 4096)), dim3(256), 0, s, nvec, L.x);
+    if (l + 1 == mg.lv.size()) {   // coarsest: smoothing only
+        coarse_sgs<NV>(L, 4 * sweeps, s);
+        return;
+    }
+    coarse_sgs<NV>(L, sweeps, s);
+    hipLaunchKernelGGL(mg_residual_kernel<NV>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, (size_t)L.ld, (size_t)std::max(L.nnz, 1),
+                       L.rptr, L.col, L.D, L.A, L.b, L.x, L.r);
+    MgLevel& C = mg.lv[l + 1];
+    hipLaunchKernelGGL(mg_restrict_kernel<NV>, dim3((C.n + 255) / 256), dim3(256), 0, s, C.n, C.mptr, C.mem, L.r, (size_t)L.ld, C.b,
+                       (size_t)C.ld);
+    level_cycle<NV>(mg, l + 1, sweeps, s);
+    hipLaunchKernelGGL(mg_prolong_kernel<NV>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, C.agg, C.x, (size_t)C.ld, L.x,
+                       (size_t)L.ld);
+    coarse_sgs<NV>(L, sweeps, s);
+}
+
+template <int NV>
+static void vcycle_t(const DevGrid& g, const Colors& fc, Multigrid& mg, const double* D, const double* JL, const double* JR,
+                     const double* Dinv, const double* b, double* x, int sweeps, hipStream_t s) {
+    const size_t n5 = (size_t)NV * g.ld;
+    dev_bsgs_apply<double>(g, fc, Dinv, JL, JR, b, x, sweeps, s, true);
+    dev_bsr_matvec<double>(g, D, JL, JR, x, mg.fine_r, s);
+    dev_xmy<double>(n5, b, mg.fine_r, s);   // r = b - A x
+    MgLevel& C = mg.lv[1];
+    hipLaunchKernelGGL(mg_restrict_kernel<NV>, dim3((C.n + 255) / 256), dim3(256), 0, s, C.n, C.mptr, C.mem, mg.fine_r, (size_t)g.ld,
+                       C.b, (size_t)C.ld);
+    level_cycle<NV>(mg, 1, sweeps, s);
+    hipLaunchKernelGGL(mg_prolong_kernel<NV>, dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, C.agg, C.x, (size_t)C.ld, x, (size_t)g.ld);
+    dev_bsgs_apply<double>(g, fc, Dinv, JL, JR, b, x, sweeps, s, false);
+}
+
+void mg_vcycle(const DevGrid& g, const Colors& fine_colors, Multigrid& mg, const double* D, const double* JL, const double* JR,
+               const double* fine_Dinv, const double* b, double* x, int sweeps, hipStream_t s) {
+    if (g.nv == 6) vcycle_t<6>(g, fine_colors, mg, D, JL, JR, fine_Dinv, b, x, sweeps, s);
+    else vcycle_t<5>(g, fine_colors, mg, D, JL, JR, fine_Dinv, b, x, sweeps, s);
+}
+
+}  // namespace fcfd

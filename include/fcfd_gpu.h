@@ -280,13 +280,16 @@ typedef struct fcfd_newton_opts {
     int max_iters;        /* maximumiterations (default 10, read_inputs.cc:127)          */
     double rtol, abstol;  /* relativetolerance / absolutetolerance (read_inputs.cc:125-126) */
     int knoll_guess;      /* initial guess = M^{-1} b (KSPSetInitialGuessKnoll, ns_petsc_functions.cc:75) */
-    int precond;          /* FCFD_PC_BLOCK_JACOBI (default) or FCFD_PC_BLOCK_SGS: multicolour block
-                             symmetric Gauss-Seidel with the FD Jacobian blocks (build-only; the
-                             reference leaves the PC to PETSc options)                      */
-    int sweeps;           /* SGS sweeps per application (>= 1)                                */
+    int precond;          /* FCFD_PC_BLOCK_JACOBI (default); FCFD_PC_BLOCK_SGS: multicolour block
+                             symmetric Gauss-Seidel with the FD Jacobian blocks; FCFD_PC_MULTIGRID:
+                             one FP64 V-cycle of aggregation multigrid on the same blocks (SGS
+                             smoothing, Galerkin coarse levels).  Build-only; the reference leaves
+                             the PC to PETSc options                                         */
+    int sweeps;           /* SGS sweeps per application / per level and V-cycle side (>= 1)  */
 } fcfd_newton_opts;
 #define FCFD_PC_BLOCK_JACOBI 0
 #define FCFD_PC_BLOCK_SGS    1
+#define FCFD_PC_MULTIGRID    2
 
 typedef struct fcfd_newton_stats {
     double res_norm;          /* ||R(W)||_2 over interior cells and the 5 equations, before the update */

@@ -1,7 +1,11 @@
-"""GPU implicit step: the GMRES/block-Jacobi solve of the FD-Jacobian
-system against a direct sparse solve of the same system assembled from the
-CPU restatement's Jacobian blocks and time terms; and convergence of the
-wedge case (config 1 numerics: Roe, lsqr, minmod, CFL 10)."""
+"""GPU implicit step: the GMRES solve of the FD-Jacobian system, with each
+preconditioner (block Jacobi, block SGS, aggregation multigrid), against a
+direct sparse solve of the same system assembled from the CPU restatement's
+Jacobian blocks and time terms (the preconditioner changes the iterations,
+not the solution); convergence of the wedge case (config 1 numerics: Roe,
+lsqr, minmod, CFL 10); and the implicit solve of the laminar cylinder to a
+1e-10 residual drop with the multigrid preconditioner, FD/FP64 and
+exact-tangent/FP32."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -37,12 +41,13 @@ def _assemble(o, dt):
     return sp.csr_matrix((vals, (rows, cols)), shape=(5 * N, 5 * N))
 
 
-@pytest.mark.parametrize("fp32", [False, True])
-def test_newton_linear_solve_matches_direct(fp32):
+@pytest.mark.parametrize("fp32,precond", [(False, A.PC_BLOCK_JACOBI), (True, A.PC_BLOCK_JACOBI), (False, A.PC_BLOCK_SGS),
+                                           (False, A.PC_MULTIGRID), (True, A.PC_MULTIGRID)])
+def test_newton_linear_solve_matches_direct(fp32, precond):
     from oracle.oracle import Oracle
     from freecfd_amd.ns import NavierStokes
     cs = cases.wedge_case(24, 12)
-    print("case wedge 24x12 fp32=%s" % fp32, flush=True)
+    print("case wedge 24x12 fp32=%s precond=%d" % (fp32, precond), flush=True)
     o = Oracle(cs.raw, cs.opt)
     R, _ = o.residual(cs.W0, with_jacobian=True)
     dt, _ = o.time_step(A.DT_CFL_LOCAL, 10.0)
@@ -50,7 +55,8 @@ def test_newton_linear_solve_matches_direct(fp32):
     dW_ref = spla.spsolve(Amat.tocsc(), R.reshape(-1)).reshape(-1, 5)
     ns = NavierStokes(cs.raw, cs.opt)
     ns.set_state(cs.W0)
-    opts = A.newton_opts(cfl=10.0, fp32=fp32, restart=60, max_iters=400, rtol=1e-12 if not fp32 else 1e-6, abstol=0.0)
+    opts = A.newton_opts(cfl=10.0, fp32=fp32, restart=60, max_iters=400, rtol=1e-12 if not fp32 else 1e-6, abstol=0.0,
+                         precond=precond)
     st = ns.newton_step(opts)
     W1 = ns.state_host(include_ghosts=False)
     dW = W1 - cs.W0
@@ -74,3 +80,17 @@ def test_wedge_newton_converges():
     print("  ||R|| history", " ".join("%.3e" % h for h in hist[::5]), flush=True)
     assert np.all(np.isfinite(hist))
     assert hist[-1] < 0.5 * hist[0]
+
+
+@pytest.mark.parametrize("jac,fp32", [(A.JAC_FD, False), (A.JAC_JVP_FULL, True)])
+def test_cylinder_multigrid_newton_reaches_1e10(jac, fp32):
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.cylinder_case(64, 32)
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    opts = A.newton_opts(cfl=1000.0, jacobian=jac, fp32=fp32, restart=30, max_iters=60, rtol=1e-4, precond=A.PC_MULTIGRID,
+                         sweeps=1)
+    r = ns.solve(opts, max_steps=80, drop=1e-10)
+    print("  cylinder 64x32 jac=%d fp32=%s: %s in %d steps, drop %.2e" % (jac, fp32, r["status"], r["steps"], r["drop"]),
+          flush=True)
+    assert r["status"] == "converged"
