@@ -146,17 +146,20 @@ RACES = {
                    lambda: __import__("freecfd_amd.cases", fromlist=["x"]).cylinder_case(64, 32)),
     "cylinder128": ("laminar cylinder O-grid 128x64x1 (Mach 0.2, Re_D 40, AUSM+up, Sutherland, lsqr)",
                     lambda: __import__("freecfd_amd.cases", fromlist=["x"]).cylinder_case(128, 64)),
+    "cylinder256": ("laminar cylinder O-grid 256x128x1 (Mach 0.2, Re_D 40, AUSM+up, Sutherland, lsqr)",
+                    lambda: __import__("freecfd_amd.cases", fromlist=["x"]).cylinder_case(256, 128)),
 }
 
 
 def race(name, cfl=1000.0, drop=1e-10, max_steps=200):
     """Implicit solve to a `drop` residual reduction from the freestream with
     (a) the reference-style FD Jacobian and FP64 GMRES and (b) the exact
-    tangent, matrix-free, with FP32 GMRES; both preconditioned by one
-    multicolour block-SGS sweep on the FD Jacobian blocks, GMRES(30) capped at
-    100 iterations per step, constant local CFL.  Reports wall time to the drop
-    and the surface loads of both converged states against the CPU restatement
-    evaluated on the JVP/FP32 state."""
+    tangent, matrix-free, with FP32 GMRES; both preconditioned by one V-cycle
+    of aggregation multigrid on the FD Jacobian blocks (one block-SGS sweep
+    per level and side), GMRES(30) capped at 60 iterations per step, constant
+    local CFL.  Reports wall time to the drop and the surface loads of both
+    converged states against the CPU restatement evaluated on the JVP/FP32
+    state."""
     import torch
     from freecfd_amd import _abi as A
     from freecfd_amd.ns import NavierStokes
@@ -168,8 +171,8 @@ def race(name, cfl=1000.0, drop=1e-10, max_steps=200):
     states = {}
     for mode, jac, fp32 in (("fd_fp64", A.JAC_FD, False), ("jvp_fp32", A.JAC_JVP_FULL, True)):
         ns.set_state(cs.W0)
-        opts = A.newton_opts(cfl=cfl, jacobian=jac, fp32=fp32, restart=30, max_iters=100, rtol=1e-4,
-                             precond=A.PC_BLOCK_SGS, sweeps=1)
+        opts = A.newton_opts(cfl=cfl, jacobian=jac, fp32=fp32, restart=30, max_iters=60, rtol=1e-4,
+                             precond=A.PC_MULTIGRID, sweeps=1)
         r = ns.solve(opts, max_steps=max_steps, drop=drop)
         loads = ns.loads()
         states[mode] = ns.state_host(include_ghosts=False)

@@ -68,24 +68,25 @@ JAC_FD, JAC_JVP_FROZEN, JAC_JVP_FULL = 1, 2, 3
 class NewtonOpts(C.Structure):
     _fields_ = [("dt_type", C.c_int), ("cfl", C.c_double), ("jacobian", C.c_int), ("krylov_fp32", C.c_int),
                 ("restart", C.c_int), ("max_iters", C.c_int), ("rtol", C.c_double), ("abstol", C.c_double),
-                ("knoll_guess", C.c_int), ("precond", C.c_int), ("sweeps", C.c_int)]
+                ("knoll_guess", C.c_int), ("precond", C.c_int), ("sweeps", C.c_int), ("max_rel_update", C.c_double)]
 
 
 class NewtonStats(C.Structure):
     _fields_ = [("res_norm", C.c_double), ("update_norms", C.c_double * 3), ("krylov_iters", C.c_int),
                 ("krylov_converged", C.c_int), ("krylov_rnorm", C.c_double), ("krylov_r0norm", C.c_double),
                 ("ms_residual", C.c_double), ("ms_jacobian", C.c_double), ("ms_krylov", C.c_double),
-                ("ms_update", C.c_double), ("nut_norm", C.c_double)]
+                ("ms_update", C.c_double), ("nut_norm", C.c_double), ("relax", C.c_double)]
 
 
 PC_BLOCK_JACOBI, PC_BLOCK_SGS, PC_MULTIGRID = 0, 1, 2
 
 
 def newton_opts(cfl=10.0, jacobian=JAC_FD, fp32=False, restart=30, max_iters=10, rtol=1e-6, abstol=1e-12, knoll=True,
-                precond=PC_BLOCK_JACOBI, sweeps=1):
+                precond=PC_BLOCK_JACOBI, sweeps=1, max_rel_update=0.0):
     """Defaults follow the reference (read_inputs.cc:125-127, ns_petsc_functions.cc:74-79)."""
     return NewtonOpts(dt_type=DT_CFL_LOCAL, cfl=cfl, jacobian=jacobian, krylov_fp32=int(fp32), restart=restart,
-                      max_iters=max_iters, rtol=rtol, abstol=abstol, knoll_guess=int(knoll), precond=precond, sweeps=sweeps)
+                      max_iters=max_iters, rtol=rtol, abstol=abstol, knoll_guess=int(knoll), precond=precond, sweeps=sweeps,
+                      max_rel_update=max_rel_update)
 
 
 class MetricsOut(C.Structure):

@@ -231,14 +231,26 @@ def tetbox_case(n=64, seed=3, viscous=True):
     return Case("tetbox", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.5, mu=mu))
 
 
-def bump_case(ni=256, nj=96, nk=64, seed=4, noise=0.0):
-    """Transonic inviscid channel flow, Mach 0.85, 10% circular-arc bump."""
+def bump_case(ni=256, nj=96, nk=64, seed=4, noise=0.0, inlet="velocity"):
+    """Transonic inviscid channel flow, Mach 0.85, 10% circular-arc bump.
+    inlet="velocity": velocity inlet (V and T of the Mach 0.85 stream), the
+    form used for the residual and tangent tests; the 10% bump chokes this
+    channel (throat/inlet area 0.9 < A*/A = 0.991 at Mach 0.85), so with V
+    fixed at the inlet there is no steady state and the inlet turns
+    supersonic in pseudo-time.  inlet="stagnation": p0, T0 of the Mach 0.85
+    stream, the well-posed choked-bump problem (the inlet Mach settles where
+    the area ratio puts it) used for implicit solves to convergence."""
     raw = meshgen.bump_channel(ni, nj, nk)
     Mw, g = 28.97, 1.4
     R = gas_R(Mw)
     p, T = 101325.0, 300.0
     U = 0.85 * np.sqrt(g * R * T)
-    bcs = [bc_inlet_T(T, (U, 0.0, 0.0)), bc_outlet_p(p), bc_wall(slip=True), bc_wall(slip=True),
+    if inlet == "stagnation":
+        f = 1.0 + 0.5 * (g - 1.0) * 0.85 ** 2
+        inl = bc_stagnation(p * f ** (g / (g - 1.0)), T * f)
+    else:
+        inl = bc_inlet_T(T, (U, 0.0, 0.0))
+    bcs = [inl, bc_outlet_p(p), bc_wall(slip=True), bc_wall(slip=True),
            bc_symmetry(), bc_symmetry()]
     opt = make_options(bcs, gamma=g, Mw=Mw, flux=A.FLUX_ROE, limiter=A.LIMITER_VK, Minf=0.85, loads_bcs=(2,))
     n = len(raw["cell_ptr"]) - 1

@@ -37,6 +37,12 @@ struct fcfd_ctx {
     double* adj_v = nullptr; // [nv][ld] J^T x inside the adjoint step's operator
     Multigrid mg;            // aggregation multigrid hierarchy (built on first use)
     bool mg_tried = false;
+    // the V-cycle as a HIP graph (fine_in -> fine_out), captured once on a
+    // private stream and replayed on the caller's; FCFD_NO_GRAPHS=1 launches
+    // the same kernels one by one instead
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t mg_exec = nullptr;
+    int mg_exec_sweeps = -1;
     int* status = nullptr;
     double *JL = nullptr, *JR = nullptr, *Dg = nullptr, *zero_upd = nullptr;   // FD Jacobian (lazy)
     // Newton-Krylov workspace (lazy)
@@ -433,6 +439,8 @@ int fcfd_create_host(const fcfd_mesh* mesh, const fcfd_options* opt, fcfd_ctx** 
 void fcfd_destroy(fcfd_ctx* c) {
     if (!c) return;
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+    if (c->mg_exec) (void)hipGraphExecDestroy(c->mg_exec);
+    if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
     for (void* p : c->allocs) (void)hipFree(p);
     mg_free(c->mg);
     delete c;
@@ -848,6 +856,30 @@ static int ensure_mg(fcfd_ctx* c) {
     return FCFD_OK;
 }
 
+static bool graphs_enabled() {
+    const char* e = std::getenv("FCFD_NO_GRAPHS");
+    return !(e && std::strcmp(e, "1") == 0);
+}
+
+// capture one V-cycle (mg.fine_in -> mg.fine_out) with `sweeps` into a graph
+static int ensure_mg_graph(fcfd_ctx* c, int sweeps) {
+    if (c->mg_exec && c->mg_exec_sweeps == sweeps) return FCFD_OK;
+    if (c->mg_exec) {
+        HIPCHK(c, hipGraphExecDestroy(c->mg_exec));
+        c->mg_exec = nullptr;
+    }
+    if (!c->cap_stream) HIPCHK(c, hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+    HIPCHK(c, hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeRelaxed));
+    mg_vcycle(c->dg, c->colors, c->mg, c->Dg, c->JL, c->JR, c->Dinv64, c->mg.fine_in, c->mg.fine_out, sweeps, c->cap_stream);
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamEndCapture(c->cap_stream, &graph));
+    hipError_t e = hipGraphInstantiate(&c->mg_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(c, e);
+    c->mg_exec_sweeps = sweeps;
+    return FCFD_OK;
+}
+
 static int ensure_colors(fcfd_ctx* c) {
     if (c->colors.n > 0) return FCFD_OK;
     const HostGrid& g = c->hg;
@@ -939,9 +971,11 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     } else if (!mgp) {
         dev_bj_invert<double>(g, c->Dg, c->Dinv64, c->status, s);
     }
+    const bool use_graph = mgp && graphs_enabled();
     if (mgp) {   // FP64 V-cycle on the FD blocks: fine inverse + coarse levels
         if (!c->Dinv64 && (rc = dalloc(c, b2 * g.ld, &c->Dinv64))) return rc;
         mg_setup(g, c->mg, c->Dg, c->JL, c->JR, c->Dinv64, c->status, s);
+        if (use_graph && (rc = ensure_mg_graph(c, sweeps))) return rc;
     }
     HIPCHK(c, hipStreamSynchronize(s));
     auto t2 = clk::now();
@@ -960,14 +994,19 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     // preconditioners: block Jacobi, block SGS, or the FP64 multigrid V-cycle
     // (FP32 Krylov vectors converted at its boundary)
     auto prec64 = [&](const double* x, double* y, hipStream_t ss) {
-        if (mgp) mg_vcycle(g, c->colors, c->mg, c->Dg, c->JL, c->JR, c->Dinv64, x, y, sweeps, ss);
+        if (use_graph) {
+            (void)hipMemcpyAsync(c->mg.fine_in, x, n5 * sizeof(double), hipMemcpyDeviceToDevice, ss);
+            (void)hipGraphLaunch(c->mg_exec, ss);
+            (void)hipMemcpyAsync(y, c->mg.fine_out, n5 * sizeof(double), hipMemcpyDeviceToDevice, ss);
+        } else if (mgp) mg_vcycle(g, c->colors, c->mg, c->Dg, c->JL, c->JR, c->Dinv64, x, y, sweeps, ss);
         else if (sgs) dev_bsgs_apply<double>(g, c->colors, c->Dinv64, c->JL, c->JR, x, y, sweeps, ss);
         else dev_bj_apply<double>(g, c->Dinv64, x, y, ss);
     };
     auto prec32 = [&](const float* x, float* y, hipStream_t ss) {
         if (mgp) {
             dev_convert<float, double>(n5, x, c->mg.fine_in, ss);
-            mg_vcycle(g, c->colors, c->mg, c->Dg, c->JL, c->JR, c->Dinv64, c->mg.fine_in, c->mg.fine_out, sweeps, ss);
+            if (use_graph) (void)hipGraphLaunch(c->mg_exec, ss);
+            else mg_vcycle(g, c->colors, c->mg, c->Dg, c->JL, c->JR, c->Dinv64, c->mg.fine_in, c->mg.fine_out, sweeps, ss);
             dev_convert<double, float>(n5, c->mg.fine_out, y, ss);
         } else if (sgs) {
             dev_bsgs_apply<float>(g, c->colors, c->Dinv32, c->JL32, c->JR32, x, y, sweeps, ss);
@@ -1001,7 +1040,17 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     if (rc) return fail(c, rc, "GMRES failed (HIP error)");
     HIPCHK(c, hipStreamSynchronize(s));
     auto t3 = clk::now();
-    // 4. update W, norms, NaN check; keep dW as the previous update
+    // 4. update W (relaxed if it would change p or T by more than
+    //    max_rel_update), norms, NaN check; keep dW as the previous update
+    S.relax = 1.0;
+    if (o->max_rel_update > 0.0) {
+        double rmax = 0.0;
+        HIPCHK(c, dev_relmax(g, c->P.Pref, c->P.Tref, W_dev, c->dW, c->red_part, &rmax, s));
+        if (rmax > o->max_rel_update) {
+            S.relax = o->max_rel_update / rmax;
+            dev_scale_copy<double>(n5, S.relax, c->dW, c->dW, s);
+        }
+    }
     double out[5];
     HIPCHK(c, dev_update(g, W_dev, c->dW, c->dt, c->red_part, c->red_out, out, s));
     HIPCHK(c, hipMemcpyAsync(c->upd, c->dW, n5 * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1016,6 +1065,8 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     int bad = 0;
     HIPCHK(c, hipMemcpy(&bad, c->status, sizeof(int), hipMemcpyDeviceToHost));
     if (bad & 2) return fail(c, FCFD_EDIVERGED, "singular diagonal block in the block-Jacobi preconditioner");
+    if (bad & 1)   // the reference exits here (ns_apply_bcs.cc:54-59)
+        return fail(c, FCFD_EBC, "an inlet became supersonic; the thermodynamic state must be specified (ns_apply_bcs.cc:54-59)");
     return FCFD_OK;
 }
 
@@ -1070,6 +1121,7 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
     auto t3 = clk::now();
     // 4. lam += delta
     dev_axpy<double>(n5, 1.0, c->dW, lam_dev, s);
+    S.relax = 1.0;
     HIPCHK(c, hipStreamSynchronize(s));
     auto t4 = clk::now();
     S.krylov_iters = gs.iters; S.krylov_converged = gs.converged; S.krylov_rnorm = gs.rnorm; S.krylov_r0norm = gs.r0norm;

@@ -50,6 +50,9 @@ void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T*
                     hipStream_t s, bool zero_init = true);
 // batched nv x nv inverses of n row-major blocks stored [nv^2][ld]
 void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, int* status, hipStream_t s);
+// max over interior cells of |dp|/(p+Pref) and |dT|/(T+Tref) for the update dW
+hipError_t dev_relmax(const DevGrid& g, double Pref, double Tref, const double* W, const double* dW, double* part, double* out_host,
+                      hipStream_t s);
 // out_host: sum dp^2/dt^2, sum |dV|^2/dt^2, sum dT^2/dt^2, NaN/Inf count, sum dnu~^2/dt^2
 hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
                       double out_host[5], hipStream_t s);

@@ -99,6 +99,26 @@ __global__ void __launch_bounds__(kRedThreads) reduce_final_kernel(const double*
     if (threadIdx.x == 0) out[q] = sh[0];
 }
 
+// largest relative change of the absolute pressure and temperature that the
+// update dW would make: max over interior cells of |dp|/(p+Pref), |dT|/(T+Tref)
+// (a max, so the result does not depend on the reduction order)
+__global__ void __launch_bounds__(kRedThreads) relmax_kernel(int N, size_t ld, double Pref, double Tref, const double* __restrict__ W,
+                                                             const double* __restrict__ dW, double* __restrict__ part) {
+    __shared__ double sh[kRedThreads];
+    double m = 0.0;
+    for (int c = blockIdx.x * kRedThreads + threadIdx.x; c < N; c += kRedBlocks * kRedThreads) {
+        const double rp = fabs(dW[c]) / fabs(W[c] + Pref), rT = fabs(dW[4 * ld + c]) / fabs(W[4 * ld + c] + Tref);
+        m = fmax(m, fmax(rp, rT));   // fmax drops a NaN operand: NaNs are caught by the update's own check
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 // Newton update W += dW over interior cells, with the reference's update norms
 // (ns.cc:349-381): sums of dp^2/dt^2, |dV|^2/dt^2, dT^2/dt^2, a NaN/Inf count
 // and (SA model) dnu~^2/dt^2 -- partial sums per block, fixed-order tree.
@@ -390,6 +410,19 @@ template void dev_bsgs_apply<double>(const DevGrid&, const Colors&, const double
                                      int, hipStream_t, bool);
 template void dev_bsgs_apply<float>(const DevGrid&, const Colors&, const float*, const float*, const float*, const float*, float*, int,
                                     hipStream_t, bool);
+
+hipError_t dev_relmax(const DevGrid& g, double Pref, double Tref, const double* W, const double* dW, double* part, double* out_host,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(relmax_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, Pref, Tref, W, dW, part);
+    std::vector<double> h(kRedBlocks);
+    hipError_t e = hipMemcpyAsync(h.data(), part, kRedBlocks * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    double m = 0.0;
+    for (double v : h) m = std::max(m, v);
+    *out_host = m;
+    return hipSuccess;
+}
 
 hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
                       double out_host[5], hipStream_t s) {

@@ -286,6 +286,9 @@ typedef struct fcfd_newton_opts {
                              smoothing, Galerkin coarse levels).  Build-only; the reference leaves
                              the PC to PETSc options                                         */
     int sweeps;           /* SGS sweeps per application / per level and V-cycle side (>= 1)  */
+    double max_rel_update;/* > 0: scale the whole update so that no cell's absolute p or T
+                             changes by more than this fraction (a step-length limit for
+                             high CFL; build-only); 0: the reference's full update          */
 } fcfd_newton_opts;
 #define FCFD_PC_BLOCK_JACOBI 0
 #define FCFD_PC_BLOCK_SGS    1
@@ -298,6 +301,7 @@ typedef struct fcfd_newton_stats {
     double krylov_rnorm, krylov_r0norm;
     double ms_residual, ms_jacobian, ms_krylov, ms_update;
     double nut_norm;          /* sqrt(sum dnu~^2/dt^2) with the SA model, else 0 */
+    double relax;             /* factor applied to the update (1 unless max_rel_update limited it) */
 } fcfd_newton_stats;
 
 /* Tangent (Jacobian-vector product) of the residual: dR = (dR/dW) dW, exact

@@ -94,3 +94,58 @@ def test_cylinder_multigrid_newton_reaches_1e10(jac, fp32):
     print("  cylinder 64x32 jac=%d fp32=%s: %s in %d steps, drop %.2e" % (jac, fp32, r["status"], r["steps"], r["drop"]),
           flush=True)
     assert r["status"] == "converged"
+
+
+def test_multigrid_graph_replays_match_eager_launches():
+    """The V-cycle replayed from its HIP graph (captured once, replayed by
+    every GMRES iteration with the eager Krylov kernels in between, over three
+    implicit steps that re-assemble the coarse levels) gives bit-identical
+    states to the same kernels launched one by one (FCFD_NO_GRAPHS=1)."""
+    import os
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.cylinder_case(64, 32)
+    out = {}
+    for mode in ("eager", "graph"):
+        if mode == "eager":
+            os.environ["FCFD_NO_GRAPHS"] = "1"
+        else:
+            os.environ.pop("FCFD_NO_GRAPHS", None)
+        ns = NavierStokes(cs.raw, cs.opt)
+        ns.set_state(cs.W0)
+        for fp32, jac in ((False, A.JAC_FD), (True, A.JAC_JVP_FULL), (False, A.JAC_FD)):
+            opts = A.newton_opts(cfl=100.0, jacobian=jac, fp32=fp32, restart=20, max_iters=40, rtol=1e-8,
+                                 precond=A.PC_MULTIGRID, sweeps=1)
+            ns.newton_step(opts)
+        out[mode] = ns.state_host(include_ghosts=False)
+        ns.close()
+    os.environ.pop("FCFD_NO_GRAPHS", None)
+    diff = np.abs(out["graph"] - out["eager"]).max()
+    print("  max |W_graph - W_eager| = %.3e" % diff, flush=True)
+    assert np.array_equal(out["graph"], out["eager"])
+
+
+def test_relative_update_limit_scales_the_step():
+    """max_rel_update scales the whole Newton update by relax < 1 when the
+    full step would change p or T by more than the limit: the limited step
+    equals relax times the full one, and relax is the limit over the largest
+    relative change."""
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.cylinder_case(32, 16)
+    full = None
+    for limit in (0.0, 1e-5):
+        # a fresh context each time: the FD step of an implicit step depends on
+        # the context's previous update
+        ns = NavierStokes(cs.raw, cs.opt)
+        ns.set_state(cs.W0)
+        opts = A.newton_opts(cfl=50.0, restart=30, max_iters=60, rtol=1e-10, precond=A.PC_BLOCK_SGS, max_rel_update=limit)
+        st = ns.newton_step(opts)
+        dW = ns.state_host(include_ghosts=False) - cs.W0
+        if limit == 0.0:
+            full = dW
+            assert st.relax == 1.0
+            rel = max(np.max(np.abs(dW[:, 0]) / np.abs(cs.W0[:, 0])), np.max(np.abs(dW[:, 4]) / np.abs(cs.W0[:, 4])))
+        else:
+            print("  relax %.6e, limit/max-rel %.6e" % (st.relax, limit / rel), flush=True)
+            assert np.isclose(st.relax, limit / rel, rtol=1e-12)
+            assert np.allclose(dW, st.relax * full, rtol=1e-9, atol=1e-12 * np.abs(cs.W0).max())
+        ns.close()
