@@ -947,6 +947,8 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     using clk = std::chrono::steady_clock;
     auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
     fcfd_newton_stats S{};
+    // the step reports the BC and pivot flags its own kernels raise
+    HIPCHK(c, hipMemsetAsync(c->status, 0, sizeof(int), s));
     auto t0 = clk::now();
     // 1. residual and its norm
     launch_ghosts(g, c->P, W_dev, c->status, s);
