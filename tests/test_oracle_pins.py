@@ -13,8 +13,11 @@ reference's own source text fixes:
     (set_bcs.cc:153-159);
   - gradients: the reference's own "gradient test" (ns.cc:161-185): exact
     for linear fields with lsqr and curvilinear maps; curvilinear = central
-    differences on Cartesian hexes;
-  - interpolation: line weights on a stretched line (closed form);
+    differences on Cartesian hexes; for other cell types the face pairs are a
+    maximum-determinant choice (brute force over every admissible choice on
+    jittered prisms);
+  - interpolation: line weights on a stretched line (closed form); the "idw"
+    face averages are distance-proportional over the extended stencil;
   - limiters: phi = 1 on linear data (Barth-Jespersen, minmod);
   - fluxes: consistency F(U,U) = Euler flux for Roe, AUSM+up, van Leer,
     SD-SLAU and Steger-Warming (a wrong sign in the 5x5 eigen-products breaks
@@ -468,3 +471,112 @@ def test_loads_closed_domain_sum_zero(oracle_lib):
     F = L[:, :3].sum(axis=0)
     scale = np.abs(L[:, :3]).sum()
     assert scale > 0 and np.abs(F).max() <= 1e-13 * scale
+
+
+def _jittered_plate(seed=9, amp=0.08):
+    """A small flat-plate mesh whose top rows are prisms, interior nodes moved
+    by up to amp of the local spacing (seeded) so that no two face-pair
+    choices of the curvilinear search tie."""
+    raw = meshgen.flat_plate(4, 5, 2, first=0.05, ratio=1.2, prism_frac=0.4)
+    xyz = np.array(raw["xyz"], dtype=np.float64).reshape(-1, 3)
+    lo, hi = xyz.min(axis=0), xyz.max(axis=0)
+    inner = np.all((xyz > lo + 1e-9) & (xyz < hi - 1e-9), axis=1)
+    rng = np.random.default_rng(seed)
+    xyz[inner] += rng.uniform(-amp, amp, size=(int(inner.sum()), 3)) * np.array([0.25, 0.02, 0.125])
+    raw = dict(raw)
+    raw["xyz"] = xyz.reshape(-1)
+    return raw
+
+
+def test_curvilinear_nonhex_map_is_the_max_det_choice(oracle_lib):
+    """For cells that are not hexahedra, curvilinear_grad_map.cc:93-158 picks
+    three (minus, plus) face pairs from the cell's faces (in std::set order,
+    later pairs starting no earlier than the first) that maximise
+    det(Jac) = J0 . (J1 x J2), J_i = centroid(plus_i) - centroid(minus_i),
+    and builds the map from Jac^{-T}.  Brute force over every admissible
+    choice on jittered prisms: the oracle's map must be the map of a choice
+    whose determinant is the maximum (to rounding), with Jac^{-T} computed
+    here by numpy's general inverse."""
+    raw = _jittered_plate()
+    bcs = [cases.bc_inlet_T(300.0, (100.0, 0.0, 0.0)), cases.bc_outlet_p(1e5), cases.bc_wall(), cases.bc_outlet_p(1e5),
+           cases.bc_symmetry(), cases.bc_symmetry()]
+    opt = cases.make_options(bcs, grad=(A.GRAD_CURVILINEAR,) * 3)
+    o = _oracle(raw, opt)
+    m = o.metrics()
+    N = o.info.n_cells
+    cc = m["cell_centroid"].reshape(-1, 3)
+    fpar, fnei = m["face_parent"], m["face_neighbor"]
+    cptr = np.asarray(raw["cell_ptr"])
+    checked = 0
+    for c in range(N):
+        if cptr[c + 1] - cptr[c] == 8:
+            continue
+        faces = sorted(m["cell_faces"][m["cell_face_ptr"][c]:m["cell_face_ptr"][c + 1]])
+        other = [int(fnei[f]) if fnei[f] != c else int(fpar[f]) for f in faces]
+        nf = len(faces)
+        best, combos = 0.0, []
+        for i1 in range(nf):
+            for i2 in range(i1, nf):
+                J0 = cc[other[i2]] - cc[other[i1]]
+                for i3 in range(i1, nf):
+                    for i4 in range(i3, nf):
+                        J1 = cc[other[i4]] - cc[other[i3]]
+                        for i5 in range(i2, nf):
+                            for i6 in range(i5, nf):
+                                J2 = cc[other[i6]] - cc[other[i5]]
+                                d = float(np.dot(J0, np.cross(J1, J2)))
+                                if d > 0.0:
+                                    combos.append((d, (i1, i2, i3, i4, i5, i6), np.array([J0, J1, J2])))
+                                    best = max(best, d)
+        got = {int(k): m["grad_w"][3 * e:3 * e + 3] for e, k in
+               zip(range(m["grad_ptr"][c], m["grad_ptr"][c + 1]), m["grad_idx"][m["grad_ptr"][c]:m["grad_ptr"][c + 1]])}
+        match = False
+        for d, idx, J in combos:
+            if d < best * (1.0 - 1e-12):
+                continue
+            JiT = np.linalg.inv(J).T
+            want = {}
+            for q in range(3):
+                mi, pl = other[idx[2 * q]], other[idx[2 * q + 1]]
+                want[mi] = want.get(mi, 0.0) - JiT[q]
+                want[pl] = want.get(pl, 0.0) + JiT[q]
+            if set(want) == set(got):
+                scale = max(np.abs(w).max() for w in want.values())
+                if all(np.abs(want[k] - got[k]).max() <= 1e-12 * scale for k in want):
+                    match = True
+                    break
+        assert match, "cell %d: the oracle's map is not that of a maximum-determinant choice" % c
+        checked += 1
+    assert checked >= 8
+
+
+def test_idw_weights_are_distance_proportional(oracle_lib):
+    """The reference's "idw" face average (face_interpolation_weights.cc:52,
+    64-90, 109-118) takes the parent and the neighbour, then the sorted union
+    of both cells' node-sharing neighbour sets without them (stencil size 99,
+    so always extended), and weights each cell by its distance to the face
+    centroid over the sum of the distances (distance-proportional, despite
+    the name).  Checked face by face from the oracle's own neighbour sets and
+    centroids (both pinned above): keys, sum = 1, and w_s = d_s / sum d."""
+    raw = _jittered_plate(seed=4)
+    bcs = [cases.bc_inlet_T(300.0, (100.0, 0.0, 0.0)), cases.bc_outlet_p(1e5), cases.bc_wall(), cases.bc_outlet_p(1e5),
+           cases.bc_symmetry(), cases.bc_symmetry()]
+    opt = cases.make_options(bcs, interp=A.INTERP_IDW)
+    o = _oracle(raw, opt)
+    m = o.metrics()
+    cc = m["cell_centroid"].reshape(-1, 3)
+    fc = m["face_centroid"].reshape(-1, 3)
+    nb = lambda c: set(m["nbr"][m["nbr_ptr"][c]:m["nbr_ptr"][c + 1]].tolist())
+    for f in range(o.info.n_faces):
+        p, n = int(m["face_parent"][f]), int(m["face_neighbor"][f])
+        ext = nb(p) | (nb(n) if m["face_bc"][f] < 0 else set())
+        ext -= {p, n}
+        stencil = [p, n] + sorted(ext)
+        d = np.array([np.linalg.norm(fc[f] - cc[s]) for s in stencil])
+        w = d / d.sum()
+        keys = m["avg_idx"][m["avg_ptr"][f]:m["avg_ptr"][f + 1]]
+        vals = m["avg_w"][m["avg_ptr"][f]:m["avg_ptr"][f + 1]]
+        assert list(keys) == sorted(stencil)
+        want = dict(zip(stencil, w))
+        assert np.allclose(vals, [want[int(k)] for k in keys], rtol=1e-13, atol=0.0)
+        assert abs(vals.sum() - 1.0) < 1e-14
