@@ -615,7 +615,7 @@ int fcfd_fd_jacobian(fcfd_ctx* c, double* W_dev, const double* dWprev_dev, const
     const DevGrid& g = c->dg;
     if (!c->JL) {
         int rc;
-        const size_t b2 = (size_t)g.nv * g.nv;
+        const size_t b2 = 2 * (size_t)pair_groups(g.nv * g.nv);   // block entries in row pairs
         if ((rc = dalloc(c, b2 * g.F, &c->JL)) || (rc = dalloc(c, b2 * g.F, &c->JR)) ||
             (rc = dalloc(c, b2 * g.ld, &c->Dg)) || (rc = dalloc(c, (size_t)g.nv * g.ld, &c->zero_upd)))
             return rc;
@@ -643,22 +643,23 @@ int fcfd_jacobian_to_host(fcfd_ctx* c, double* JL_host, double* JR_host, double*
     const HostGrid& hg = c->hg;
     const int F = hg.F, N = hg.N, ld = c->dg.ld;
     const int b2 = c->dg.nv * c->dg.nv;
-    std::vector<double> tmp((size_t)b2 * std::max(F, ld));
+    const size_t rows = 2 * (size_t)pair_groups(b2);   // stored in row pairs
+    std::vector<double> tmp(rows * std::max(F, ld));
     double* outs[2] = {JL_host, JR_host};
     const double* srcs[2] = {c->JL, c->JR};
     for (int k = 0; k < 2; ++k) {
         if (!outs[k]) continue;
-        HIPCHK(c, hipMemcpy(tmp.data(), srcs[k], (size_t)b2 * F * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(tmp.data(), srcs[k], rows * F * sizeof(double), hipMemcpyDeviceToHost));
         for (int i = 0; i < F; ++i) {
             int f = hg.ifperm[i];
-            for (int e = 0; e < b2; ++e) outs[k][(size_t)b2 * f + e] = tmp[(size_t)e * F + i];
+            for (int e = 0; e < b2; ++e) outs[k][(size_t)b2 * f + e] = tmp[pidx((size_t)F, e, i)];
         }
     }
     if (D_host) {
-        HIPCHK(c, hipMemcpy(tmp.data(), c->Dg, (size_t)b2 * ld * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(tmp.data(), c->Dg, rows * ld * sizeof(double), hipMemcpyDeviceToHost));
         for (int i = 0; i < N; ++i) {
             int r = hg.iperm[i];
-            for (int e = 0; e < b2; ++e) D_host[(size_t)b2 * r + e] = tmp[(size_t)e * ld + i];
+            for (int e = 0; e < b2; ++e) D_host[(size_t)b2 * r + e] = tmp[pidx((size_t)ld, e, i)];
         }
     }
     return FCFD_OK;
@@ -793,7 +794,7 @@ int fcfd_jvp(fcfd_ctx* c, double* W_dev, double* dW_dev, double* dR_dev, int mod
 static int ensure_newton(fcfd_ctx* c, const fcfd_newton_opts* o) {
     const DevGrid& g = c->dg;
     const size_t n5 = (size_t)g.nv * g.ld;
-    const size_t b2 = (size_t)g.nv * g.nv;
+    const size_t b2 = 2 * (size_t)pair_groups(g.nv * g.nv);   // block entries in row pairs
     int rc;
     if (!c->R) {
         if ((rc = dalloc(c, n5, &c->R)) || (rc = dalloc(c, (size_t)g.ld, &c->dt)) || (rc = dalloc(c, n5, &c->dW)) ||
@@ -940,7 +941,7 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     const DevGrid& g = c->dg;
     hipStream_t s = (hipStream_t)stream;
     const size_t n5 = (size_t)g.nv * g.ld;      // one [nv][ld] vector
-    const size_t b2 = (size_t)g.nv * g.nv;     // entries of one block
+    const size_t b2 = 2 * (size_t)pair_groups(g.nv * g.nv);   // entries of one block in row pairs (padded)
     int rc = ensure_newton(c, o);
     if (rc) return rc;
     const int fp32 = o->krylov_fp32 ? 1 : 0;

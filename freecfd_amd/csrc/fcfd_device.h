@@ -38,7 +38,8 @@ struct DevGrid {
     const double2* cc2;                 // [2][ld] cell centroid and volume packed: (x, y), (z, V)
 };
 
-// Gradients ([3 nv] components per cell) and limiters ([nv]) are stored in
+// Gradients ([3 nv] components per cell), limiters ([nv]) and the FD
+// Jacobian blocks (nv^2 entries per face or cell, FP64 and FP32) are stored in
 // row pairs: component k of cell i at ((k / 2) ld + i) 2 + k % 2, the row
 // count rounded up to even (the pad row holds 0).  A lane then loads two
 // components of its cell with one 16-byte load while the lanes of a
@@ -47,18 +48,29 @@ struct DevGrid {
 __host__ __device__ constexpr int pair_groups(int rows) { return (rows + 1) / 2; }
 __host__ __device__ __forceinline__ size_t pidx(size_t ld, int k, size_t i) { return ((size_t)(k >> 1) * ld + i) * 2 + (k & 1); }
 
-template <int K>
-__device__ __forceinline__ void store_pairs(double* __restrict__ A, size_t ld, size_t i, const double (&v)[K]) {
-    double2* A2 = reinterpret_cast<double2*>(A);
-#pragma unroll
-    for (int j = 0; j < pair_groups(K); ++j) A2[(size_t)j * ld + i] = make_double2(v[2 * j], 2 * j + 1 < K ? v[2 * j + 1] : 0.0);
-}
-template <int K>
-__device__ __forceinline__ void load_pairs(const double* __restrict__ A, size_t ld, size_t i, double (&v)[K]) {
-    const double2* A2 = reinterpret_cast<const double2*>(A);
+template <typename T> struct PairOf;
+template <> struct PairOf<double> { using type = double2; };
+template <> struct PairOf<float> { using type = float2; };
+
+template <int K, typename T>
+__device__ __forceinline__ void store_pairs(T* __restrict__ A, size_t ld, size_t i, const T (&v)[K]) {
+    using P2 = typename PairOf<T>::type;
+    P2* A2 = reinterpret_cast<P2*>(A);
 #pragma unroll
     for (int j = 0; j < pair_groups(K); ++j) {
-        const double2 t = A2[(size_t)j * ld + i];
+        P2 t;
+        t.x = v[2 * j];
+        t.y = 2 * j + 1 < K ? v[2 * j + 1] : T(0);
+        A2[(size_t)j * ld + i] = t;
+    }
+}
+template <int K, typename T>
+__device__ __forceinline__ void load_pairs(const T* __restrict__ A, size_t ld, size_t i, T (&v)[K]) {
+    using P2 = typename PairOf<T>::type;
+    const P2* A2 = reinterpret_cast<const P2*>(A);
+#pragma unroll
+    for (int j = 0; j < pair_groups(K); ++j) {
+        const P2 t = A2[(size_t)j * ld + i];
         v[2 * j] = t.x;
         if (2 * j + 1 < K) v[2 * j + 1] = t.y;
     }

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(128) fdjac_kernel(DevGrid g, Params P, const d
             for (int j = 0; j < NV; ++j) Dp[j] = 0.0;
         }
 #pragma unroll
-        for (int j = 0; j < NV; ++j) JL[(NV * var + j) * Fs + f] = (Dp[j] - X.D[j] - Cp[j] + X.C[j]) / eps;
+        for (int j = 0; j < NV; ++j) JL[pidx(Fs, NV * var + j, f)] = (Dp[j] - X.D[j] - Cp[j] + X.C[j]) / eps;
         if (interior) {
             Rp = X.R;
             double eps2 = fd_eps(UPD[var * ld + X.n], sn);
@@ -125,10 +125,10 @@ __global__ void __launch_bounds__(128) fdjac_kernel(DevGrid g, Params P, const d
                 diffusive_flux<NV>(P, X.fs, F, g.bc, Dp);
             }
 #pragma unroll
-            for (int j = 0; j < NV; ++j) JR[(NV * var + j) * Fs + f] = (Dp[j] - X.D[j] - Cp[j] + X.C[j]) / eps2;
+            for (int j = 0; j < NV; ++j) JR[pidx(Fs, NV * var + j, f)] = (Dp[j] - X.D[j] - Cp[j] + X.C[j]) / eps2;
         } else {
 #pragma unroll
-            for (int j = 0; j < NV; ++j) JR[(NV * var + j) * Fs + f] = 0.0;
+            for (int j = 0; j < NV; ++j) JR[pidx(Fs, NV * var + j, f)] = 0.0;
         }
         if (VISC) face_state_adjust<NV>(P, X.L, X.R, F, X.fs, var);   // (:265)
     }
@@ -174,13 +174,13 @@ __global__ void __launch_bounds__(256) diag_kernel(DevGrid g, Params P, const do
 #pragma unroll
             for (int var = 0; var < NV; ++var)
 #pragma unroll
-                for (int j = 0; j < NV; ++j) blk[NV * j + var] += -1. * JL[(NV * var + j) * Fs + fs];
+                for (int j = 0; j < NV; ++j) blk[NV * j + var] += -1. * JL[pidx(Fs, NV * var + j, fs)];
         } else {
             int f = ~fs;
 #pragma unroll
             for (int var = 0; var < NV; ++var)
 #pragma unroll
-                for (int j = 0; j < NV; ++j) blk[NV * j + var] += JR[(NV * var + j) * Fs + f];
+                for (int j = 0; j < NV; ++j) blk[NV * j + var] += JR[pidx(Fs, NV * var + j, f)];
         }
     }
     double q[NV];
@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(256) diag_kernel(DevGrid g, Params P, const do
             for (int j = 0; j < NV; ++j) blk[NV * i + j] += Pm[i][j] * vol / dtc;
     }
 #pragma unroll
-    for (int k = 0; k < B2; ++k) D[k * ld + c] = blk[k];
+    for (int k = 0; k < B2; ++k) D[pidx(ld, k, c)] = blk[k];
 }
 
 template <int FLUX, int VISC, int NV>

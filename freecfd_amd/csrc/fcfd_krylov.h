@@ -48,8 +48,9 @@ struct Colors {
 template <typename T>
 void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T* JL, const T* JR, const T* b, T* x, int sweeps,
                     hipStream_t s, bool zero_init = true);
-// batched nv x nv inverses of n row-major blocks stored [nv^2][ld]
-void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, int* status, hipStream_t s);
+// batched nv x nv inverses of n row-major blocks, stored in row pairs
+// (paired, the mesh level's layout) or plainly as [nv^2][ld] (coarse levels)
+void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, int* status, hipStream_t s, bool paired);
 // max over interior cells of |dp|/(p+Pref) and |dT|/(T+Tref) for the update dW
 hipError_t dev_relmax(const DevGrid& g, double Pref, double Tref, const double* W, const double* dW, double* part, double* out_host,
                       hipStream_t s);

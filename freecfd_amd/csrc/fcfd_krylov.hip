@@ -161,14 +161,15 @@ __global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __r
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld, Fs = g.F;
-    T xc[NV], acc[NV];
+    T xc[NV], acc[NV], blk[NV * NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) xc[i] = x[i * ld + c];
+    load_pairs<NV * NV>(D, ld, c, blk);
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         T s = 0;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) s += D[(NV * j + i) * ld + c] * xc[i];
+        for (int i = 0; i < NV; ++i) s += blk[NV * j + i] * xc[i];
         acc[j] = s;
     }
     for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
@@ -179,19 +180,19 @@ __global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __r
         T xo[NV];
 #pragma unroll
         for (int i = 0; i < NV; ++i) xo[i] = x[i * ld + o];
-        const T* J = fs >= 0 ? JR : JL;
+        load_pairs<NV * NV>(fs >= 0 ? JR : JL, Fs, f, blk);
         const T sg = fs >= 0 ? T(-1) : T(1);
 #pragma unroll
         for (int i = 0; i < NV; ++i)
 #pragma unroll
-            for (int j = 0; j < NV; ++j) acc[j] += sg * J[(NV * i + j) * Fs + f] * xo[i];
+            for (int j = 0; j < NV; ++j) acc[j] += sg * blk[NV * i + j] * xo[i];
     }
 #pragma unroll
     for (int j = 0; j < NV; ++j) y[j * ld + c] = acc[j];
 }
 
 // batched NV x NV inverse by Gauss-Jordan with partial pivoting (block-Jacobi)
-template <typename T, int NV>
+template <typename T, int NV, bool PAIRED = true>
 __global__ void __launch_bounds__(256) bj_invert_kernel(int N, size_t ld, const double* __restrict__ D, T* __restrict__ Dinv,
                                                         int* __restrict__ status) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,7 +201,10 @@ __global__ void __launch_bounds__(256) bj_invert_kernel(int N, size_t ld, const 
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
-        for (int j = 0; j < NV; ++j) { a[i][j] = (T)D[(NV * i + j) * ld + c]; a[i][NV + j] = (i == j) ? T(1) : T(0); }
+        for (int j = 0; j < NV; ++j) {
+            a[i][j] = (T)D[PAIRED ? pidx(ld, NV * i + j, c) : (NV * i + j) * ld + c];
+            a[i][NV + j] = (i == j) ? T(1) : T(0);
+        }
     bool singular = false;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
@@ -233,7 +237,8 @@ __global__ void __launch_bounds__(256) bj_invert_kernel(int N, size_t ld, const 
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
-        for (int j = 0; j < NV; ++j) Dinv[(NV * i + j) * ld + c] = singular ? ((i == j) ? T(1) : T(0)) : a[i][NV + j];
+        for (int j = 0; j < NV; ++j)
+            Dinv[PAIRED ? pidx(ld, NV * i + j, c) : (NV * i + j) * ld + c] = singular ? ((i == j) ? T(1) : T(0)) : a[i][NV + j];
 }
 
 // y_c = D_c^{-1} x_c, or with TRANS y_c = D_c^{-T} x_c (the adjoint's
@@ -243,14 +248,15 @@ __global__ void __launch_bounds__(256) bj_apply_kernel(int N, size_t ld, const T
                                                        T* __restrict__ y) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    T xc[NV];
+    T xc[NV], blk[NV * NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) xc[i] = x[i * ld + c];
+    load_pairs<NV * NV>(Dinv, ld, c, blk);
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         T s = 0;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) s += Dinv[(TRANS ? NV * i + j : NV * j + i) * ld + c] * xc[i];
+        for (int i = 0; i < NV; ++i) s += blk[TRANS ? NV * i + j : NV * j + i] * xc[i];
         y[j * ld + c] = s;
     }
 }
@@ -267,7 +273,7 @@ __global__ void __launch_bounds__(256) bsgs_color_kernel(DevGrid g, const T* __r
     if (t >= count) return;
     const int c = cells[t];
     const size_t ld = g.ld, Fs = g.F;
-    T acc[NV];
+    T acc[NV], blk[NV * NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = b[j * ld + c];
     for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
@@ -278,18 +284,19 @@ __global__ void __launch_bounds__(256) bsgs_color_kernel(DevGrid g, const T* __r
         T xo[NV];
 #pragma unroll
         for (int i = 0; i < NV; ++i) xo[i] = x[i * ld + o];
-        const T* J = fs >= 0 ? JR : JL;
+        load_pairs<NV * NV>(fs >= 0 ? JR : JL, Fs, f, blk);
         const T sg = fs >= 0 ? T(-1) : T(1);
 #pragma unroll
         for (int i = 0; i < NV; ++i)
 #pragma unroll
-            for (int j = 0; j < NV; ++j) acc[j] -= sg * J[(NV * i + j) * Fs + f] * xo[i];
+            for (int j = 0; j < NV; ++j) acc[j] -= sg * blk[NV * i + j] * xo[i];
     }
+    load_pairs<NV * NV>(Dinv, ld, c, blk);
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         T s = 0;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) s += Dinv[(NV * j + i) * ld + c] * acc[i];
+        for (int i = 0; i < NV; ++i) s += blk[NV * j + i] * acc[i];
         x[j * ld + c] = s;
     }
 }
@@ -381,10 +388,16 @@ template void dev_bj_invert<float>(const DevGrid&, const double*, float*, int*, 
 template void dev_bj_apply<double>(const DevGrid&, const double*, const double*, double*, hipStream_t);
 template void dev_bj_apply<float>(const DevGrid&, const float*, const float*, float*, hipStream_t);
 
-void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, int* status, hipStream_t s) {
+void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, int* status, hipStream_t s, bool paired) {
     if (n <= 0) return;
-    if (nv == 6) hipLaunchKernelGGL((bj_invert_kernel<double, 6>), dim3((n + 255) / 256), dim3(256), 0, s, n, ld, D, Dinv, status);
-    else hipLaunchKernelGGL((bj_invert_kernel<double, 5>), dim3((n + 255) / 256), dim3(256), 0, s, n, ld, D, Dinv, status);
+    dim3 grid((n + 255) / 256), blk(256);
+    if (paired) {
+        if (nv == 6) hipLaunchKernelGGL((bj_invert_kernel<double, 6, true>), grid, blk, 0, s, n, ld, D, Dinv, status);
+        else hipLaunchKernelGGL((bj_invert_kernel<double, 5, true>), grid, blk, 0, s, n, ld, D, Dinv, status);
+    } else {
+        if (nv == 6) hipLaunchKernelGGL((bj_invert_kernel<double, 6, false>), grid, blk, 0, s, n, ld, D, Dinv, status);
+        else hipLaunchKernelGGL((bj_invert_kernel<double, 5, false>), grid, blk, 0, s, n, ld, D, Dinv, status);
+    }
 }
 
 template <typename T>

@@ -38,19 +38,23 @@ __global__ void __launch_bounds__(256) mg_assemble_kernel(int nblk, const int* _
         const size_t idx = (size_t)(code >> 2);
         const int kind = code & 3;
         if (FINE) {
+            double blk[NV * NV];   // the mesh level's blocks are stored in row pairs
             if (kind == 0) {
+                load_pairs<NV * NV>(D0, ld0, idx, blk);
 #pragma unroll
-                for (int q = 0; q < NV * NV; ++q) acc[q] += D0[q * ld0 + idx];
+                for (int q = 0; q < NV * NV; ++q) acc[q] += blk[q];
             } else if (kind == 1) {
+                load_pairs<NV * NV>(JR, F, idx, blk);
 #pragma unroll
                 for (int j = 0; j < NV; ++j)
 #pragma unroll
-                    for (int i = 0; i < NV; ++i) acc[NV * j + i] -= JR[(NV * i + j) * F + idx];
+                    for (int i = 0; i < NV; ++i) acc[NV * j + i] -= blk[NV * i + j];
             } else {
+                load_pairs<NV * NV>(JL, F, idx, blk);
 #pragma unroll
                 for (int j = 0; j < NV; ++j)
 #pragma unroll
-                    for (int i = 0; i < NV; ++i) acc[NV * j + i] += JL[(NV * i + j) * F + idx];
+                    for (int i = 0; i < NV; ++i) acc[NV * j + i] += blk[NV * i + j];
             }
         } else {
             if (kind == 0) {
@@ -381,13 +385,13 @@ static void setup_t(const DevGrid& g, Multigrid& mg, const double* D, const doub
                                    L.A, nullptr, (size_t)0, nullptr, nullptr, (size_t)0, P.D, (size_t)P.n, P.A,
                                    (size_t)std::max(P.nnz, 1));
         }
-        dev_block_invert(NV, L.n, (size_t)L.n, L.D, L.Dinv, status, s);
+        dev_block_invert(NV, L.n, (size_t)L.n, L.D, L.Dinv, status, s, false);
     }
 }
 
 void mg_setup(const DevGrid& g, Multigrid& mg, const double* D, const double* JL, const double* JR, double* fine_Dinv, int* status,
               hipStream_t s) {
-    dev_block_invert(g.nv, g.N, (size_t)g.ld, D, fine_Dinv, status, s);
+    dev_block_invert(g.nv, g.N, (size_t)g.ld, D, fine_Dinv, status, s, true);
     if (g.nv == 6) setup_t<6>(g, mg, D, JL, JR, status, s);
     else setup_t<5>(g, mg, D, JL, JR, status, s);
 }

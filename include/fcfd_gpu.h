@@ -253,10 +253,12 @@ int  fcfd_status(fcfd_ctx* ctx, int clear);
  * Recomputes ghosts, gradients and limiters of W_dev first.  dWprev_dev
  * ([5][ld_cell], the previous Newton update that sets the FD step,
  * :217-219) may be NULL (zero update).  The result stays in the context:
- * per face JL and JR ([25][n_faces], entry (var,eq) at (5*var+eq)*F + f, in
- * internal face order) and diagonal blocks D ([25][ld_cell], row-major 5x5).
- * The operator row P, column N block is -JR^T of the face (P,N); row N,
- * column P is +JL^T. */
+ * per face JL and JR (nv x nv entries per face: entry k = nv*var+eq of face f
+ * at ((k/2)*F + f)*2 + k%2, in internal face order, entries stored in row
+ * pairs) and diagonal blocks D (row-major nv x nv, entry k of cell i at
+ * ((k/2)*ld_cell + i)*2 + k%2); fcfd_jacobian_to_host returns them per block
+ * in reference order.  The operator row P, column N block is -JR^T of the
+ * face (P,N); row N, column P is +JL^T. */
 int  fcfd_fd_jacobian(fcfd_ctx* ctx, double* W_dev, const double* dWprev_dev, const double* dt_dev, void* stream);
 int  fcfd_jacobian_buffers(fcfd_ctx* ctx, const double** JL, const double** JR, const double** D);
 /* Copies the last Jacobian to the host in reference numbering: JL, JR as
