@@ -43,11 +43,12 @@ struct Colors {
     std::vector<int> ptr;
     const int* cells = nullptr;
 };
-// x = M^{-1} b by `sweeps` symmetric multicolour block Gauss-Seidel sweeps
-// from x = 0 (zero_init) or from the x given
+// x = M^{-1} b by `sweeps` multicolour block Gauss-Seidel sweeps from x = 0
+// (zero_init) or from the x given; dir 0: symmetric (colours forward then
+// backward), 1: forward only, 2: backward only
 template <typename T>
 void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T* JL, const T* JR, const T* b, T* x, int sweeps,
-                    hipStream_t s, bool zero_init = true);
+                    hipStream_t s, bool zero_init = true, int dir = 0);
 // batched nv x nv inverses of n row-major blocks, stored in row pairs
 // (paired, the mesh level's layout) or plainly as [nv^2][ld] (coarse levels)
 void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, int* status, hipStream_t s, bool paired);

@@ -402,7 +402,7 @@ void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, i
 
 template <typename T>
 void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T* JL, const T* JR, const T* b, T* x, int sweeps,
-                    hipStream_t s, bool zero_init) {
+                    hipStream_t s, bool zero_init, int dir) {
     const size_t n = (size_t)g.nv * g.ld;
     if (zero_init) hipLaunchKernelGGL(scale_copy_kernel<T>, ew_grid(n), dim3(256), 0, s, n, T(0), b, x);   // x = 0
     auto one = [&](int k) {
@@ -415,14 +415,16 @@ void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T*
             hipLaunchKernelGGL((bsgs_color_kernel<T, 5>), dim3((cnt + 255) / 256), dim3(256), 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
     };
     for (int sw = 0; sw < sweeps; ++sw) {
-        for (int k = 0; k < col.n; ++k) one(k);        // forward
-        for (int k = col.n - 1; k >= 0; --k) one(k);   // backward
+        if (dir != 2)
+            for (int k = 0; k < col.n; ++k) one(k);        // forward
+        if (dir != 1)
+            for (int k = col.n - 1; k >= 0; --k) one(k);   // backward
     }
 }
 template void dev_bsgs_apply<double>(const DevGrid&, const Colors&, const double*, const double*, const double*, const double*, double*,
-                                     int, hipStream_t, bool);
+                                     int, hipStream_t, bool, int);
 template void dev_bsgs_apply<float>(const DevGrid&, const Colors&, const float*, const float*, const float*, const float*, float*, int,
-                                    hipStream_t, bool);
+                                    hipStream_t, bool, int);
 
 hipError_t dev_relmax(const DevGrid& g, double Pref, double Tref, const double* W, const double* dW, double* part, double* out_host,
                       hipStream_t s) {

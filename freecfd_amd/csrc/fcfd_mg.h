@@ -11,9 +11,10 @@
 // j in J of A_l[i][j]: each coarse block is a fixed, host-ordered list of
 // finer blocks, summed by one thread (deterministic), re-assembled on every
 // implicit step from the new FD blocks.  The V-cycle runs in FP64: multicolour
-// block SGS pre- and post-smoothing, residual, restriction (sum over the
-// aggregate), coarse correction, prolongation (injection); SGS sweeps alone
-// on the coarsest level.  A fixed linear operator, so GMRES may use it as a
+// block Gauss-Seidel pre-smoothing (colours in forward order) and
+// post-smoothing (backward order), residual, restriction (sum over the
+// aggregate), coarse correction, prolongation (injection); symmetric sweeps
+// alone on the coarsest level.  A fixed linear operator, so GMRES may use it as a
 // right preconditioner.
 #pragma once
 #include <vector>

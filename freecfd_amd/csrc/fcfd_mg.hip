@@ -397,7 +397,7 @@ void mg_setup(const DevGrid& g, Multigrid& mg, const double* D, const double* JL
 }
 
 template <int NV>
-static void coarse_sgs(const MgLevel& L, int sweeps, hipStream_t s) {
+static void coarse_sgs(const MgLevel& L, int sweeps, hipStream_t s, int dir) {
     auto one = [&](int c) {
         const int cnt = L.colors.ptr[c + 1] - L.colors.ptr[c];
         if (cnt <= 0) return;
@@ -405,8 +405,10 @@ static void coarse_sgs(const MgLevel& L, int sweeps, hipStream_t s) {
                            (size_t)L.ld, (size_t)L.n, (size_t)std::max(L.nnz, 1), L.rptr, L.col, L.A, L.Dinv, L.b, L.x);
     };
     for (int sw = 0; sw < sweeps; ++sw) {
-        for (int c = 0; c < L.colors.n; ++c) one(c);
-        for (int c = L.colors.n - 1; c >= 0; --c) one(c);
+        if (dir != 2)
+            for (int c = 0; c < L.colors.n; ++c) one(c);
+        if (dir != 1)
+            for (int c = L.colors.n - 1; c >= 0; --c) one(c);
     }
 }
 
@@ -416,10 +418,10 @@ static void level_cycle(Multigrid& mg, size_t l, int sweeps, hipStream_t s) {
     const size_t nvec = (size_t)NV * L.ld;
     hipLaunchKernelGGL(mg_zero_kernel, dim3((unsigned)std::min<size_t>((nvec + 255) / 256, 4096)), dim3(256), 0, s, nvec, L.x);
     if (l + 1 == mg.lv.size()) {   // coarsest: smoothing only
-        coarse_sgs<NV>(L, 4 * sweeps, s);
+        coarse_sgs<NV>(L, 4 * sweeps, s, 0);
         return;
     }
-    coarse_sgs<NV>(L, sweeps, s);
+    coarse_sgs<NV>(L, sweeps, s, 1);   // pre-smoothing: forward sweeps
     hipLaunchKernelGGL(mg_residual_kernel<NV>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, (size_t)L.ld, (size_t)std::max(L.nnz, 1),
                        L.rptr, L.col, L.D, L.A, L.b, L.x, L.r);
     MgLevel& C = mg.lv[l + 1];
@@ -428,14 +430,14 @@ static void level_cycle(Multigrid& mg, size_t l, int sweeps, hipStream_t s) {
     level_cycle<NV>(mg, l + 1, sweeps, s);
     hipLaunchKernelGGL(mg_prolong_kernel<NV>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, C.agg, C.x, (size_t)C.ld, L.x,
                        (size_t)L.ld);
-    coarse_sgs<NV>(L, sweeps, s);
+    coarse_sgs<NV>(L, sweeps, s, 2);   // post-smoothing: backward sweeps
 }
 
 template <int NV>
 static void vcycle_t(const DevGrid& g, const Colors& fc, Multigrid& mg, const double* D, const double* JL, const double* JR,
                      const double* Dinv, const double* b, double* x, int sweeps, hipStream_t s) {
     const size_t n5 = (size_t)NV * g.ld;
-    dev_bsgs_apply<double>(g, fc, Dinv, JL, JR, b, x, sweeps, s, true);
+    dev_bsgs_apply<double>(g, fc, Dinv, JL, JR, b, x, sweeps, s, true, 1);   // forward pre-smoothing
     dev_bsr_matvec<double>(g, D, JL, JR, x, mg.fine_r, s);
     dev_xmy<double>(n5, b, mg.fine_r, s);   // r = b - A x
     MgLevel& C = mg.lv[1];
@@ -443,7 +445,7 @@ static void vcycle_t(const DevGrid& g, const Colors& fc, Multigrid& mg, const do
                        C.b, (size_t)C.ld);
     level_cycle<NV>(mg, 1, sweeps, s);
     hipLaunchKernelGGL(mg_prolong_kernel<NV>, dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, C.agg, C.x, (size_t)C.ld, x, (size_t)g.ld);
-    dev_bsgs_apply<double>(g, fc, Dinv, JL, JR, b, x, sweeps, s, false);
+    dev_bsgs_apply<double>(g, fc, Dinv, JL, JR, b, x, sweeps, s, false, 2);   // backward post-smoothing
 }
 
 void mg_vcycle(const DevGrid& g, const Colors& fine_colors, Multigrid& mg, const double* D, const double* JL, const double* JR,
