@@ -893,27 +893,20 @@ static int ensure_colors(fcfd_ctx* c) {
         par[i] = g.perm[g.fparent[f]];
         nei[i] = g.perm[g.fneigh[f]];
     }
-    std::vector<int> col(N, -1);
-    int ncol = 0;
-    std::vector<char> used;
+    // interior face graph (ghost neighbours excluded), coloured greedily
+    std::vector<int> aptr(N + 1, 0), adj;
     for (int i = 0; i < N; ++i) {
-        used.assign(ncol + 1, 0);
         for (int h = hp[i]; h < hp[i + 1]; ++h) {
             int fs = hf[h];
             int f = fs >= 0 ? fs : ~fs;
             int o = fs >= 0 ? nei[f] : par[f];
-            if (o < N && col[o] >= 0) used[col[o]] = 1;
+            if (o < N) adj.push_back(o);
         }
-        int k = 0;
-        while (k < ncol && used[k]) ++k;
-        col[i] = k;
-        if (k == ncol) ++ncol;
+        aptr[i + 1] = (int)adj.size();
     }
-    std::vector<int> ptr(ncol + 1, 0), cells(N);
-    for (int i = 0; i < N; ++i) ptr[col[i] + 1]++;
-    for (int k = 0; k < ncol; ++k) ptr[k + 1] += ptr[k];
-    std::vector<int> next(ptr.begin(), ptr.end() - 1);
-    for (int i = 0; i < N; ++i) cells[next[col[i]]++] = i;
+    std::vector<int> ptr, cells;
+    int ncol = 0;
+    color_graph(N, aptr, adj, ptr, cells, ncol);
     int rc = upload(c, cells, &c->colors.cells);
     if (rc) return rc;
     c->colors.ptr = ptr;

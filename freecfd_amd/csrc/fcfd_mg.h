@@ -49,6 +49,10 @@ struct Multigrid {
     std::vector<void*> allocs;
 };
 
+// greedy colouring (given or breadth-first vertex order, whichever needs fewer
+// colours): colour k holds cells[cptr[k] .. cptr[k+1])
+void color_graph(int n, const std::vector<int>& ptr, const std::vector<int>& adj, std::vector<int>& cptr, std::vector<int>& cells,
+                 int& ncol);
 // build the hierarchy for the interior face graph of the mesh (host, once);
 // allocations are recorded in mg.allocs
 int mg_build(const DevGrid& g, const std::vector<int>& par, const std::vector<int>& nei, Multigrid& mg, int max_levels,

@@ -217,20 +217,53 @@ int aggregate(int n, const std::vector<int>& ptr, const std::vector<int>& adj, s
     return na;
 }
 
-void greedy_colors(int n, const std::vector<int>& ptr, const std::vector<int>& adj, std::vector<int>& cptr, std::vector<int>& cells,
-                   int& ncol) {
-    std::vector<int> col(n, -1);
-    ncol = 0;
-    std::vector<char> used;
-    for (int i = 0; i < n; ++i) {
-        used.assign(ncol + 1, 0);
-        for (int k = ptr[i]; k < ptr[i + 1]; ++k)
-            if (col[adj[k]] >= 0) used[col[adj[k]]] = 1;
-        int c = 0;
-        while (c < ncol && used[c]) ++c;
-        col[i] = c;
-        if (c == ncol) ++ncol;
+}  // namespace
+
+// Greedy colouring of an undirected graph (CSR, no self loops), tried in the
+// given vertex order and in breadth-first order (per component, from its
+// lowest vertex); the order with fewer colours is kept (the given order on a
+// tie).  Breadth-first greedy colours a bipartite graph -- the face graph of a
+// hexahedral or quadrilateral mesh -- with two colours.  Each colour is a
+// dependent launch of a Gauss-Seidel sweep, so fewer colours mean fewer
+// launches.
+void color_graph(int n, const std::vector<int>& ptr, const std::vector<int>& adj, std::vector<int>& cptr, std::vector<int>& cells,
+                 int& ncol) {
+    auto greedy = [&](const std::vector<int>& order, std::vector<int>& col) {
+        col.assign(n, -1);
+        int nc = 0;
+        std::vector<char> used;
+        for (int i : order) {
+            used.assign(nc + 1, 0);
+            for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+                if (col[adj[k]] >= 0) used[col[adj[k]]] = 1;
+            int c = 0;
+            while (c < nc && used[c]) ++c;
+            col[i] = c;
+            if (c == nc) ++nc;
+        }
+        return nc;
+    };
+    std::vector<int> natural(n), bfs;
+    for (int i = 0; i < n; ++i) natural[i] = i;
+    bfs.reserve(n);
+    {
+        std::vector<char> seen(n, 0);
+        for (int s0 = 0; s0 < n; ++s0) {
+            if (seen[s0]) continue;
+            size_t head = bfs.size();
+            bfs.push_back(s0);
+            seen[s0] = 1;
+            while (head < bfs.size()) {
+                const int v = bfs[head++];
+                for (int k = ptr[v]; k < ptr[v + 1]; ++k)
+                    if (!seen[adj[k]]) { seen[adj[k]] = 1; bfs.push_back(adj[k]); }
+            }
+        }
     }
+    std::vector<int> c1, c2;
+    const int n1 = greedy(natural, c1), n2 = greedy(bfs, c2);
+    const std::vector<int>& col = n2 < n1 ? c2 : c1;
+    ncol = n2 < n1 ? n2 : n1;
     cptr.assign(ncol + 1, 0);
     for (int i = 0; i < n; ++i) cptr[col[i] + 1]++;
     for (int c = 0; c < ncol; ++c) cptr[c + 1] += cptr[c];
@@ -238,8 +271,6 @@ void greedy_colors(int n, const std::vector<int>& ptr, const std::vector<int>& a
     cells.assign(n, 0);
     for (int i = 0; i < n; ++i) cells[next[col[i]]++] = i;
 }
-
-}  // namespace
 
 int mg_build(const DevGrid& g, const std::vector<int>& par, const std::vector<int>& nei, Multigrid& mg, int max_levels,
              int coarse_cells) {
@@ -327,7 +358,7 @@ int mg_build(const DevGrid& g, const std::vector<int>& par, const std::vector<in
         // colours
         std::vector<int> cptr, cells;
         int ncol = 0;
-        greedy_colors(nc, rptr, col, cptr, cells, ncol);
+        color_graph(nc, rptr, col, cptr, cells, ncol);
         L.rptr_h = rptr;
         if ((rc = mg_upload(mg, agg, &L.agg)) || (rc = mg_upload(mg, mptr, &L.mptr)) || (rc = mg_upload(mg, mem, &L.mem)) ||
             (rc = mg_upload(mg, rptr, &L.rptr)) || (rc = mg_upload(mg, col, &L.col)) || (rc = mg_upload(mg, dptr, &L.dptr)) ||
