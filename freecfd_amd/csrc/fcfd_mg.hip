@@ -449,7 +449,7 @@ static void level_cycle(Multigrid& mg, size_t l, int sweeps, hipStream_t s) {
     const size_t nvec = (size_t)NV * L.ld;
     hipLaunchKernelGGL(mg_zero_kernel, dim3((unsigned)std::min<size_t>((nvec + 255) / 256, 4096)), dim3(256), 0, s, nvec, L.x);
     if (l + 1 == mg.lv.size()) {   // coarsest: smoothing only
-        coarse_sgs<NV>(L, 4 * sweeps, s, 0);
+        coarse_sgs<NV>(L, 2 * sweeps, s, 0);
         return;
     }
     coarse_sgs<NV>(L, sweeps, s, 1);   // pre-smoothing: forward sweeps
