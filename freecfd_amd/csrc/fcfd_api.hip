@@ -280,8 +280,18 @@ int build_device(fcfd_ctx* c) {
         blk.push_back(N);
         d.nhf = (int)hf.size();
         d.nblk = (int)blk.size() - 1;
+        std::vector<int> hn(hf.size());
+        for (int i = 0; i < N; ++i) {
+            if (hp[i + 1] - hp[i] > kMaxCellFaces || hp[i + 1] == hp[i]) return fail(c, FCFD_EMESH, "cell with no or more than 6 faces");
+            for (int h = hp[i]; h < hp[i + 1]; ++h) {
+                const int fs = hf[h], fr = g.ifperm[fs >= 0 ? fs : ~fs];
+                hn[h] = g.perm[fs >= 0 ? g.fneigh[fr] : g.fparent[fr]];
+            }
+        }
         int rc;
-        if ((rc = upload(c, hp, &d.hf_ptr)) || (rc = upload(c, hf, &d.hf_face)) || (rc = upload(c, blk, &d.blk_cell))) return rc;
+        if ((rc = upload(c, hp, &d.hf_ptr)) || (rc = upload(c, hf, &d.hf_face)) || (rc = upload(c, hn, &d.hf_nb)) ||
+            (rc = upload(c, blk, &d.blk_cell)))
+            return rc;
     }
     // faces
     {

@@ -25,6 +25,7 @@ struct DevGrid {
     const int* gslice;                  // N/64+1 entry offsets of 64-wide gradient slices
     const double2 *gwa, *gwb;           // per entry (w_x, w_y) and (w_z, neighbour index as int64 bits)
     const int *hf_ptr, *hf_face;        // N+1, nhf
+    const int* hf_nb;                   // nhf: the cell across each half-face (neighbour or parent; a ghost for a boundary face)
     const int* blk_cell;                // nblk+1 first cell of each flux block
     const int *fpar, *fnei, *fbc;
     const double *fnx, *fny, *fnz, *t1x, *t1y, *t1z, *farea, *fcx, *fcy, *fcz, *fwp, *fwn, *fwd, *fdiss;
@@ -82,6 +83,10 @@ __device__ __forceinline__ void load_pairs(const T* __restrict__ A, size_t ld, s
 // gridDim / 8, gridDim % 8): faces and cells that are close in the (RCM)
 // numbering, and so share gathered cell data, then share an L2.  A bijection
 // of [0, gridDim) for every grid size.
+// faces of a cell: at most 6 for the supported cell types (tetrahedron,
+// pyramid, prism, hexahedron); the set-up rejects meshes with more
+constexpr int kMaxCellFaces = 6;
+
 constexpr int kXcds = 8;
 __device__ __forceinline__ int xcd_block() {
     const int n = gridDim.x, b = blockIdx.x;

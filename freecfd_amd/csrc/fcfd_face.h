@@ -90,6 +90,21 @@ struct DualLoader {
     }
 };
 
+// A face's packed records, all issued before any is used: the cell loads
+// depend on the parent/neighbour ids, and issuing the geometry with them
+// (rather than after) keeps one round of load latency per face
+template <int VISC>
+struct FaceRec {
+    int4 cr;
+    double2 a[6];
+    __device__ __forceinline__ void load(const DevGrid& g, int f) {
+        cr = g.fcell[f];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) a[r] = face_pair(g, r, f);
+        a[5] = VISC ? face_pair(g, 5, f) : make_double2(0., 0.);
+    }
+};
+
 // everything one face-loop iteration computes before the residual insertion
 template <typename S, int NV>
 struct FaceCtx {
@@ -106,12 +121,17 @@ struct FaceCtx {
 // values for a boundary face; X.fs.g* the face gradients when VISC.
 template <int VISC, int NV, typename S, typename Loader>
 __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_, const Params& P, int f, FaceCtx<S, NV>& X) {
+    FaceRec<VISC> rec;
+    rec.load(g, f);
     FaceGeo& F = X.F;
-    load_geo(g, f, F);
-    const int4 cr = g.fcell[f];
-    const int p = cr.x, n = cr.y;
+    F.n = mk<double>(rec.a[0].x, rec.a[0].y, rec.a[1].x);
+    F.t1 = mk<double>(rec.a[1].y, rec.a[2].x, rec.a[2].y);
+    F.t2 = vnorm(vcross(F.n, F.t1));
+    F.area = rec.a[3].x;
+    F.bc = rec.cr.z;
+    const int p = rec.cr.x, n = rec.cr.y;
     X.p = p; X.n = n;
-    Vec<double> xf = face_centroid(g, f);
+    Vec<double> xf = mk<double>(rec.a[3].y, rec.a[4].x, rec.a[4].y);
     Vec<double> xp = centroid(g, p), xn = centroid(g, n);
     F.l2r = vsub(xn, xp);
     S q[NV], ph[NV];
@@ -136,11 +156,10 @@ __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_,
         X.R = Rb;
     }
     if (VISC) {
+        const double wp = rec.a[5].x, wn = rec.a[5].y;
         if (F.bc >= 0) {
             X.fs.gu = gp[1]; X.fs.gv = gp[2]; X.fs.gw = gp[3]; X.fs.gT = gp[4];
         } else {   // face average (variable.h:134-147), two-entry stencil in key order
-            const double2 w = face_pair(g, 5, f);
-            const double wp = w.x, wn = w.y;
             X.fs.gu = vadd(vsc(wp, gp[1]), vsc(wn, gn[1]));
             X.fs.gv = vadd(vsc(wp, gp[2]), vsc(wn, gn[2]));
             X.fs.gw = vadd(vsc(wp, gp[3]), vsc(wn, gn[3]));
@@ -148,10 +167,7 @@ __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_,
         }
         if constexpr (NV == 6) {
             if (F.bc >= 0) X.fs.gn = gp[5];
-            else {
-                const double2 w = face_pair(g, 5, f);
-                X.fs.gn = vadd(vsc(w.x, gp[5]), vsc(w.y, gn[5]));
-            }
+            else X.fs.gn = vadd(vsc(wp, gp[5]), vsc(wn, gn[5]));
         }
     }
 }

@@ -62,33 +62,53 @@ template <int LIM, int NV>
 __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, const double* __restrict__ W,
                                                        double* __restrict__ G, double* __restrict__ PHI,
                                                        const double* __restrict__ dref, double* __restrict__ SRC) {
+    // The loads are issued in rounds (all map entries of a chunk, then all
+    // their states; all half-faces, then all neighbour states and face
+    // centroids) rather than entry by entry: the kernel is bound by chains of
+    // dependent loads.  The arithmetic is the reference's, in its order.
+    constexpr int kChunk = 6;   // gradient-map entries per round
     int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
+    double q0[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) q0[v] = W[v * ld + c];
+    const int h0 = g.hf_ptr[c], nh = g.hf_ptr[c + 1] - h0;
     double gr[3 * NV];
 #pragma unroll
     for (int k = 0; k < 3 * NV; ++k) gr[k] = 0.0;
-    {
-        int s = c >> 6, lane = c & 63;
-        int e0 = g.gslice[s], e1 = g.gslice[s + 1];
-        for (int e = e0 + lane; e < e1; e += 64) {
-            const double2 a = g.gwa[e], b = g.gwb[e];
-            const int j = (int)__double_as_longlong(b.y);
-            const double wx = a.x, wy = a.y, wz = b.x;
+    {   // entry k of cell `lane` of slice s at e0 + 64 k + lane (K entries each, zero-weight pads included)
+        const int s = c >> 6, lane = c & 63;
+        const int e0 = g.gslice[s], K = (g.gslice[s + 1] - e0) >> 6;
+        for (int k0 = 0; k0 < K; k0 += kChunk) {
+            double2 a[kChunk], b[kChunk];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                double q = W[v * ld + j];
-                gr[3 * v] += wx * q;
-                gr[3 * v + 1] += wy * q;
-                gr[3 * v + 2] += wz * q;
+            for (int u = 0; u < kChunk; ++u) {
+                const int e = e0 + 64 * min(k0 + u, K - 1) + lane;
+                a[u] = g.gwa[e];
+                b[u] = g.gwb[e];
+            }
+            double q[kChunk][NV];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const int j = (int)__double_as_longlong(b[u].y);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) q[u][v] = W[v * ld + j];
+            }
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                if (k0 + u < K) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        gr[3 * v] += a[u].x * q[u][v];
+                        gr[3 * v + 1] += a[u].y * q[u][v];
+                        gr[3 * v + 2] += b[u].x * q[u][v];
+                    }
+                }
             }
         }
     }
-    const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
     store_pairs<3 * NV>(G, ld, c, gr);
-    double q0[NV], phi[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) { q0[v] = W[v * ld + c]; phi[v] = 1.; }
     if constexpr (NV == 6) {
         // V_c S_c of the SA source (ns_sources.cc:25-30 has none; build-only model)
         if (SRC) {
@@ -97,73 +117,85 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
                                                   g.wd[c]);
         }
     }
-    Vec<double> xc = centroid(g, c);
-    if (LIM == FCFD_LIMITER_MINMOD) {
-        const double small = 1.e-12;
-        for (int h = h0; h < h1; ++h) {
-            int fs = g.hf_face[h];
-            int f = fs >= 0 ? fs : ~fs;
-            int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
-            Vec<double> c2f = vsub(face_centroid(g, f), xc);
+    // the cell's half-faces (at most kMaxCellFaces; slots past nh repeat the
+    // last one and are masked)
+    int fs[kMaxCellFaces], nb[kMaxCellFaces];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                double dU = W[v * ld + nb] - q0[v];
-                double dUg = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
-                double mm = (dU * dUg < 0.) ? 0. : (dU > 0.) ? smin(dU, dUg) : smax(dU, dUg);
-                double sU = (double)((mm > 0) - (mm < 0)), sG = (double)((dUg > 0) - (dUg < 0));
-                phi[v] = smin(phi[v], (mm + sU * small) / (dUg + sG * small));
-            }
+    for (int k = 0; k < kMaxCellFaces; ++k) {
+        const int h = h0 + min(k, nh - 1);
+        fs[k] = g.hf_face[h];
+        nb[k] = g.hf_nb[h];
+    }
+    double phi[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) phi[v] = 1.;
+    if (LIM != FCFD_LIMITER_NONE) {
+        const Vec<double> xc = centroid(g, c);
+        double qn[kMaxCellFaces][NV];
+        Vec<double> c2f[kMaxCellFaces];
+#pragma unroll
+        for (int k = 0; k < kMaxCellFaces; ++k) {
+            c2f[k] = vsub(face_centroid(g, fs[k] >= 0 ? fs[k] : ~fs[k]), xc);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qn[k][v] = W[v * ld + nb[k]];
         }
-    } else if (LIM == FCFD_LIMITER_BJ || LIM == FCFD_LIMITER_VK) {
-        double umax[NV], umin[NV];
+        if (LIM == FCFD_LIMITER_MINMOD) {
+            const double small = 1.e-12;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) umax[v] = umin[v] = q0[v];
-        for (int h = h0; h < h1; ++h) {
-            int fs = g.hf_face[h];
-            int f = fs >= 0 ? fs : ~fs;
-            int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
+            for (int k = 0; k < kMaxCellFaces; ++k) {
+                if (k >= nh) break;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                double q = W[v * ld + nb];
-                umax[v] = smax(umax[v], q);
-                umin[v] = smin(umin[v], q);
+                for (int v = 0; v < NV; ++v) {
+                    double dU = qn[k][v] - q0[v];
+                    double dUg = gr[3 * v] * c2f[k].x + gr[3 * v + 1] * c2f[k].y + gr[3 * v + 2] * c2f[k].z;
+                    double mm = (dU * dUg < 0.) ? 0. : (dU > 0.) ? smin(dU, dUg) : smax(dU, dUg);
+                    double sU = (double)((mm > 0) - (mm < 0)), sG = (double)((dUg > 0) - (dUg < 0));
+                    phi[v] = smin(phi[v], (mm + sU * small) / (dUg + sG * small));
+                }
             }
-        }
-        for (int h = h0; h < h1; ++h) {
-            int fs = g.hf_face[h];
-            int f = fs >= 0 ? fs : ~fs;
-            Vec<double> c2f = vsub(face_centroid(g, f), xc);
+        } else if (LIM == FCFD_LIMITER_BJ || LIM == FCFD_LIMITER_VK) {
+            double umax[NV], umin[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                double dM = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
-                double dP = q0[v];
-                if (LIM == FCFD_LIMITER_BJ) {
-                    if (dM > 0.) dP = umax[v] - dP;
-                    else if (dM < 0.) dP = umin[v] - dP;
-                    else { dP = 1.; dM = 1.; }
-                    phi[v] = smin(phi[v], dP / dM);
-                } else {
-                    double eps = dref[v] / P.glscale;
-                    if (dM > 0.) dP = umax[v] - dP;
-                    else if (dM < 0.) dP = umin[v] - dP;
-                    double dP2 = dP * dP, dM2 = dM * dM;
-                    double e = 1.e-6 * P.limiter_threshold * eps;
-                    double eps2 = e * e;
-                    if (dM != 0.) phi[v] = smin(phi[v], ((dP2 + eps2) + 2. * dM * dP) / (dP2 + 2. * dM2 + dM * dP + eps2));
+            for (int v = 0; v < NV; ++v) umax[v] = umin[v] = q0[v];
+#pragma unroll
+            for (int k = 0; k < kMaxCellFaces; ++k) {
+                if (k >= nh) break;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    umax[v] = smax(umax[v], qn[k][v]);
+                    umin[v] = smin(umin[v], qn[k][v]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kMaxCellFaces; ++k) {
+                if (k >= nh) break;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    double dM = gr[3 * v] * c2f[k].x + gr[3 * v + 1] * c2f[k].y + gr[3 * v + 2] * c2f[k].z;
+                    double dP = q0[v];
+                    if (LIM == FCFD_LIMITER_BJ) {
+                        if (dM > 0.) dP = umax[v] - dP;
+                        else if (dM < 0.) dP = umin[v] - dP;
+                        else { dP = 1.; dM = 1.; }
+                        phi[v] = smin(phi[v], dP / dM);
+                    } else {
+                        double eps = dref[v] / P.glscale;
+                        if (dM > 0.) dP = umax[v] - dP;
+                        else if (dM < 0.) dP = umin[v] - dP;
+                        double dP2 = dP * dP, dM2 = dM * dM;
+                        double e = 1.e-6 * P.limiter_threshold * eps;
+                        double eps2 = e * e;
+                        if (dM != 0.) phi[v] = smin(phi[v], ((dP2 + eps2) + 2. * dM * dP) / (dP2 + 2. * dM2 + dM * dP + eps2));
+                    }
                 }
             }
         }
-    }
-    if (LIM != FCFD_LIMITER_NONE) {
         store_pairs<NV>(PHI, ld, c, phi);
     }
     // boundary ghosts take their parent's gradient (ns.cc:320-333)
-    for (int h = h0; h < h1; ++h) {
-        int fs = g.hf_face[h];
-        if (fs < 0) continue;
-        int gi = g.fnei[fs];
-        if (gi < g.N) continue;
-        store_pairs<3 * NV>(G, ld, gi, gr);
+#pragma unroll
+    for (int k = 0; k < kMaxCellFaces; ++k) {
+        if (k < nh && fs[k] >= 0 && nb[k] >= g.N) store_pairs<3 * NV>(G, ld, nb[k], gr);
     }
 }
 
