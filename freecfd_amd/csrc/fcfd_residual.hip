@@ -58,8 +58,10 @@ __global__ void __launch_bounds__(256) ghost_kernel(DevGrid g, Params P, double*
 // 2. gradients (calc_cell_grads, ns.cc:301-336 + variable.h:193-216) fused
 //    with the limiter (ns_limiters.cc:26-248)
 // --------------------------------------------------------------------------
+// (3 waves per SIMD: 168 VGPRs with a 12-byte spill; the compiler's own
+// choice, 174 VGPRs at 2 waves, is 6% slower)
 template <int LIM, int NV>
-__global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, const double* __restrict__ W,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) grad_lim_kernel(DevGrid g, Params P, const double* __restrict__ W,
                                                        double* __restrict__ G, double* __restrict__ PHI,
                                                        const double* __restrict__ dref, double* __restrict__ SRC) {
     // The loads are issued in rounds (all map entries of a chunk, then all
@@ -131,15 +133,18 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
     for (int v = 0; v < NV; ++v) phi[v] = 1.;
     if (LIM != FCFD_LIMITER_NONE) {
         const Vec<double> xc = centroid(g, c);
-        double qn[kMaxCellFaces][NV];
-        Vec<double> c2f[kMaxCellFaces];
-#pragma unroll
-        for (int k = 0; k < kMaxCellFaces; ++k) {
-            c2f[k] = vsub(face_centroid(g, fs[k] >= 0 ? fs[k] : ~fs[k]), xc);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) qn[k][v] = W[v * ld + nb[k]];
-        }
+        // the neighbour states (min/max, or minmod's differences), then the
+        // face centroids (so both sets are never live at once)
         if (LIM == FCFD_LIMITER_MINMOD) {
+            double qn[kMaxCellFaces][NV];
+#pragma unroll
+            for (int k = 0; k < kMaxCellFaces; ++k) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qn[k][v] = W[v * ld + nb[k]];
+            }
+            Vec<double> c2f[kMaxCellFaces];
+#pragma unroll
+            for (int k = 0; k < kMaxCellFaces; ++k) c2f[k] = vsub(face_centroid(g, fs[k] >= 0 ? fs[k] : ~fs[k]), xc);
             const double small = 1.e-12;
 #pragma unroll
             for (int k = 0; k < kMaxCellFaces; ++k) {
@@ -157,15 +162,31 @@ __global__ void __launch_bounds__(256) grad_lim_kernel(DevGrid g, Params P, cons
             double umax[NV], umin[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) umax[v] = umin[v] = q0[v];
+            // two rounds of half the faces (max and min are exact: the
+            // grouping does not change them)
+            constexpr int kHalf = kMaxCellFaces / 2;
 #pragma unroll
-            for (int k = 0; k < kMaxCellFaces; ++k) {
-                if (k >= nh) break;
+            for (int k0 = 0; k0 < kMaxCellFaces; k0 += kHalf) {
+                double qn[kHalf][NV];
 #pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    umax[v] = smax(umax[v], qn[k][v]);
-                    umin[v] = smin(umin[v], qn[k][v]);
+                for (int u = 0; u < kHalf; ++u) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) qn[u][v] = W[v * ld + nb[k0 + u]];
+                }
+#pragma unroll
+                for (int u = 0; u < kHalf; ++u) {
+                    if (k0 + u < nh) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) {
+                            umax[v] = smax(umax[v], qn[u][v]);
+                            umin[v] = smin(umin[v], qn[u][v]);
+                        }
+                    }
                 }
             }
+            Vec<double> c2f[kMaxCellFaces];
+#pragma unroll
+            for (int k = 0; k < kMaxCellFaces; ++k) c2f[k] = vsub(face_centroid(g, fs[k] >= 0 ? fs[k] : ~fs[k]), xc);
 #pragma unroll
             for (int k = 0; k < kMaxCellFaces; ++k) {
                 if (k >= nh) break;
