@@ -319,21 +319,31 @@ __global__ void __launch_bounds__(256) face_flux_kernel(DevGrid g, Params P, con
 template <int NV, bool SRC>
 __global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, const double* __restrict__ src, const double* __restrict__ FX,
                                                        double* __restrict__ Rout) {
+    // all half-face entries in one round, then all their flux rows in one
+    // round (entries past the cell's count repeat its last one and are
+    // masked), then the sum in order
     const int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld, Fs = g.F;
+    const int h0 = g.hf_ptr[c], nh = g.hf_ptr[c + 1] - h0;
+    int fs[kMaxCellFaces];
+#pragma unroll
+    for (int k = 0; k < kMaxCellFaces; ++k) fs[k] = g.hf_face[h0 + min(k, nh - 1)];
+    double x[kMaxCellFaces][NV];
+#pragma unroll
+    for (int k = 0; k < kMaxCellFaces; ++k) {
+        const int f = fs[k] >= 0 ? fs[k] : ~fs[k];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) x[k][v] = FX[v * Fs + f];
+    }
     double s[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) s[v] = 0.0;
-    for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
-        int fs = g.hf_face[h];
-        if (fs >= 0) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) s[v] += FX[v * Fs + fs] + 0.;
-        } else {
-            int f = ~fs;
+    for (int k = 0; k < kMaxCellFaces; ++k) {
+        if (k < nh) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) s[v] += -1. * FX[v * Fs + f] + 0.;
+            for (int v = 0; v < NV; ++v) s[v] += (fs[k] >= 0 ? x[k][v] : -1. * x[k][v]) + 0.;
         }
     }
     if constexpr (SRC && NV == 6) s[5] += src[c];
