@@ -155,12 +155,51 @@ __global__ void __launch_bounds__(kRedThreads) update_kernel(int N, size_t ld, d
 // ---------------------------------------------------------------------------
 // operator: y = D x + sum_faces (-JR^T x_N | +JL^T x_P)
 // ---------------------------------------------------------------------------
+// acc[j] (+ or, SUB, -)= sg B[i][j] x_o[i] over the cell's half-faces in
+// order, B = JR (sg = -1) where the cell is the face's parent and JL (sg = +1)
+// where it is the neighbour, o the cell across; faces to a boundary ghost
+// carry no column (the ghost coupling is in JL).  The half-face entries are
+// loaded in one round; each face's loads use valid (clamped) indices and only
+// the accumulation is masked, so the loads of several faces can be in flight
+// together.  x: [nv][ld] (ghost rows exist, read but unused).
+template <typename T, int NV, bool SUB>
+__device__ __forceinline__ void offdiag_accumulate(const DevGrid& g, const T* __restrict__ JL, const T* __restrict__ JR,
+                                                   const T* __restrict__ x, int c, T (&acc)[NV]) {
+    const size_t ld = g.ld, Fs = g.F;
+    const int h0 = g.hf_ptr[c], nh = g.hf_ptr[c + 1] - h0;
+    int fs[kMaxCellFaces], nb[kMaxCellFaces];
+#pragma unroll
+    for (int k = 0; k < kMaxCellFaces; ++k) {
+        const int h = h0 + min(k, nh - 1);
+        fs[k] = g.hf_face[h];
+        nb[k] = g.hf_nb[h];
+    }
+#pragma unroll
+    for (int k = 0; k < kMaxCellFaces; ++k) {
+        const int f = fs[k] >= 0 ? fs[k] : ~fs[k], o = nb[k];
+        T xo[NV], blk[NV * NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) xo[i] = x[i * ld + o];
+        load_pairs<NV * NV>(fs[k] >= 0 ? JR : JL, Fs, f, blk);
+        if (k < nh && o < g.N) {
+            const T sg = fs[k] >= 0 ? T(-1) : T(1);
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    if (SUB) acc[j] -= sg * blk[NV * i + j] * xo[i];
+                    else acc[j] += sg * blk[NV * i + j] * xo[i];
+                }
+        }
+    }
+}
+
 template <typename T, int NV>
 __global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __restrict__ D, const T* __restrict__ JL,
                                                          const T* __restrict__ JR, const T* __restrict__ x, T* __restrict__ y) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
-    const size_t ld = g.ld, Fs = g.F;
+    const size_t ld = g.ld;
     T xc[NV], acc[NV], blk[NV * NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) xc[i] = x[i * ld + c];
@@ -172,21 +211,7 @@ __global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __r
         for (int i = 0; i < NV; ++i) s += blk[NV * j + i] * xc[i];
         acc[j] = s;
     }
-    for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
-        int fs = g.hf_face[h];
-        int f = fs >= 0 ? fs : ~fs;
-        int o = fs >= 0 ? g.fnei[f] : g.fpar[f];
-        if (o >= g.N) continue;   // boundary ghost: no column (ghost coupling is in JL)
-        T xo[NV];
-#pragma unroll
-        for (int i = 0; i < NV; ++i) xo[i] = x[i * ld + o];
-        load_pairs<NV * NV>(fs >= 0 ? JR : JL, Fs, f, blk);
-        const T sg = fs >= 0 ? T(-1) : T(1);
-#pragma unroll
-        for (int i = 0; i < NV; ++i)
-#pragma unroll
-            for (int j = 0; j < NV; ++j) acc[j] += sg * blk[NV * i + j] * xo[i];
-    }
+    offdiag_accumulate<T, NV, false>(g, JL, JR, x, c, acc);
 #pragma unroll
     for (int j = 0; j < NV; ++j) y[j * ld + c] = acc[j];
 }
@@ -272,25 +297,11 @@ __global__ void __launch_bounds__(256) bsgs_color_kernel(DevGrid g, const T* __r
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const int c = cells[t];
-    const size_t ld = g.ld, Fs = g.F;
+    const size_t ld = g.ld;
     T acc[NV], blk[NV * NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = b[j * ld + c];
-    for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
-        int fs = g.hf_face[h];
-        int f = fs >= 0 ? fs : ~fs;
-        int o = fs >= 0 ? g.fnei[f] : g.fpar[f];
-        if (o >= g.N) continue;
-        T xo[NV];
-#pragma unroll
-        for (int i = 0; i < NV; ++i) xo[i] = x[i * ld + o];
-        load_pairs<NV * NV>(fs >= 0 ? JR : JL, Fs, f, blk);
-        const T sg = fs >= 0 ? T(-1) : T(1);
-#pragma unroll
-        for (int i = 0; i < NV; ++i)
-#pragma unroll
-            for (int j = 0; j < NV; ++j) acc[j] -= sg * blk[NV * i + j] * xo[i];
-    }
+    offdiag_accumulate<T, NV, true>(g, JL, JR, x, c, acc);
     load_pairs<NV * NV>(Dinv, ld, c, blk);
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
