@@ -7,7 +7,9 @@
 // `src.nb(h, nb, v)` returns variable v of the neighbour `nb` across
 // half-face h (the only place neighbour states enter); q0 and gr are the
 // cell's own state and gradient; dref is the VK reference range (held
-// constant).
+// constant).  Loop-based (the neighbour loads of one half-face at a time):
+// on dual numbers the round-based load schedule of the FP64 kernel needs
+// more registers than it saves latency.
 #pragma once
 #include "fcfd_face.h"
 
@@ -25,7 +27,7 @@ __device__ __forceinline__ void limiter_phi(const DevGrid& g, const Params& P, c
         for (int h = h0; h < h1; ++h) {
             int fs = g.hf_face[h];
             int f = fs >= 0 ? fs : ~fs;
-            int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
+            int nb = g.hf_nb[h];
             Vec<double> c2f = vsub(face_centroid(g, f), xc);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
@@ -41,9 +43,7 @@ __device__ __forceinline__ void limiter_phi(const DevGrid& g, const Params& P, c
 #pragma unroll
         for (int v = 0; v < NV; ++v) umax[v] = umin[v] = q0[v];
         for (int h = h0; h < h1; ++h) {
-            int fs = g.hf_face[h];
-            int f = fs >= 0 ? fs : ~fs;
-            int nb = fs >= 0 ? g.fnei[f] : g.fpar[f];
+            int nb = g.hf_nb[h];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 S q = src.nb(h, nb, v);

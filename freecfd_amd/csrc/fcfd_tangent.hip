@@ -69,23 +69,39 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
     Dual gr[3 * NV];
 #pragma unroll
     for (int k = 0; k < 3 * NV; ++k) gr[k] = Dual(0.0);
-    {
-        int s = c >> 6, lane = c & 63;
-        int e0 = g.gslice[s], e1 = g.gslice[s + 1];
-        for (int e = e0 + lane; e < e1; e += 64) {
-            const double2 a = g.gwa[e], b = g.gwb[e];
-            const int j = (int)__double_as_longlong(b.y);
-            const double wx = a.x, wy = a.y, wz = b.x;
+    {   // map entries in chunks: their weights in one round, their states in the next
+        constexpr int kChunk = 3;
+        const int s = c >> 6, lane = c & 63;
+        const int e0 = g.gslice[s], K = (g.gslice[s + 1] - e0) >> 6;
+        for (int k0 = 0; k0 < K; k0 += kChunk) {
+            double2 a[kChunk], b[kChunk];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                Dual q(W[v * ld + j], dW[v * ld + j]);
-                gr[3 * v] += wx * q;
-                gr[3 * v + 1] += wy * q;
-                gr[3 * v + 2] += wz * q;
+            for (int u = 0; u < kChunk; ++u) {
+                const int e = e0 + 64 * min(k0 + u, K - 1) + lane;
+                a[u] = g.gwa[e];
+                b[u] = g.gwb[e];
+            }
+            Dual q[kChunk][NV];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const int j = (int)__double_as_longlong(b[u].y);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) q[u][v] = Dual(W[v * ld + j], dW[v * ld + j]);
+            }
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                if (k0 + u < K) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        gr[3 * v] += a[u].x * q[u][v];
+                        gr[3 * v + 1] += a[u].y * q[u][v];
+                        gr[3 * v + 2] += b[u].x * q[u][v];
+                    }
+                }
             }
         }
     }
-    const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
+    const int h0 = g.hf_ptr[c], nh = g.hf_ptr[c + 1] - h0;
     double gv[3 * NV], gd[3 * NV];
 #pragma unroll
     for (int k = 0; k < 3 * NV; ++k) { gv[k] = gr[k].v; gd[k] = gr[k].d; }
@@ -103,13 +119,14 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
         store_pairs<NV>(PHI, ld, c, pv);
         store_pairs<NV>(dPHI, ld, c, pd);
     }
-    for (int h = h0; h < h1; ++h) {
-        int fs = g.hf_face[h];
-        if (fs < 0) continue;
-        int gi = g.fnei[fs];
-        if (gi < g.N) continue;
-        store_pairs<3 * NV>(G, ld, gi, gv);
-        store_pairs<3 * NV>(dG, ld, gi, gd);
+    // boundary ghosts take their parent's gradient
+#pragma unroll
+    for (int k = 0; k < kMaxCellFaces; ++k) {
+        const int h = h0 + min(k, nh - 1), fs = g.hf_face[h], gi = g.hf_nb[h];
+        if (k < nh && fs >= 0 && gi >= g.N) {
+            store_pairs<3 * NV>(G, ld, gi, gv);
+            store_pairs<3 * NV>(dG, ld, gi, gd);
+        }
     }
 }
 
