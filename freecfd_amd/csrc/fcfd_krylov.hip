@@ -63,26 +63,43 @@ __global__ void lincomb_kernel(size_t n, int k, const T* __restrict__ V, size_t 
 
 // ---------------------------------------------------------------------------
 // deterministic multi-dot: out[q] = <V_q, w> for q < k over interior cells,
-// accumulated in FP64; partials [k][kRedBlocks] then a fixed-order tree
+// accumulated in FP64; partials [k][kRedBlocks] then a fixed-order tree.
+// Each block row takes kDotGroup of the V_q, so w is read once per group
+// rather than once per vector; every partial keeps its own order.
 // ---------------------------------------------------------------------------
+constexpr int kDotGroup = 8;
 template <typename T, int NV>
 __global__ void __launch_bounds__(kRedThreads) mdot_partial_kernel(int N, size_t ld, int k, const T* __restrict__ V, size_t stride,
                                                                    const T* __restrict__ w, double* __restrict__ part) {
-    __shared__ double sh[kRedThreads];
-    const int q = blockIdx.y;
-    const T* v = V + q * stride;
-    double acc = 0.0;
-    for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < N; i += kRedBlocks * kRedThreads) {
+    __shared__ double sh[kDotGroup][kRedThreads];
+    const int q0 = blockIdx.y * kDotGroup, nq = min(kDotGroup, k - q0);
+    double acc[kDotGroup];
 #pragma unroll
-        for (int r = 0; r < NV; ++r) acc += (double)v[r * ld + i] * (double)w[r * ld + i];
+    for (int u = 0; u < kDotGroup; ++u) acc[u] = 0.0;
+    for (int i = blockIdx.x * kRedThreads + threadIdx.x; i < N; i += kRedBlocks * kRedThreads) {
+        T wv[NV];
+#pragma unroll
+        for (int r = 0; r < NV; ++r) wv[r] = w[r * ld + i];
+#pragma unroll
+        for (int u = 0; u < kDotGroup; ++u) {
+            if (u < nq) {
+                const T* v = V + (size_t)(q0 + u) * stride;
+#pragma unroll
+                for (int r = 0; r < NV; ++r) acc[u] += (double)v[r * ld + i] * (double)wv[r];
+            }
+        }
     }
-    sh[threadIdx.x] = acc;
+#pragma unroll
+    for (int u = 0; u < kDotGroup; ++u) sh[u][threadIdx.x] = acc[u];
     __syncthreads();
     for (int s = kRedThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        if (threadIdx.x < s) {
+#pragma unroll
+            for (int u = 0; u < kDotGroup; ++u) sh[u][threadIdx.x] += sh[u][threadIdx.x + s];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[q * kRedBlocks + blockIdx.x] = sh[0];
+    if (threadIdx.x < nq) part[(q0 + threadIdx.x) * kRedBlocks + blockIdx.x] = sh[threadIdx.x][0];
 }
 
 __global__ void __launch_bounds__(kRedThreads) reduce_final_kernel(const double* __restrict__ part, double* __restrict__ out) {
@@ -344,9 +361,9 @@ template <typename T>
 hipError_t dev_mdot(const DevGrid& g, int k, const T* V, size_t stride, const T* w, double* part, double* out_dev,
                     double* out_host, hipStream_t s) {
     if (g.nv == 6)
-        hipLaunchKernelGGL((mdot_partial_kernel<T, 6>), dim3(kRedBlocks, k), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, k, V, stride, w, part);
+        hipLaunchKernelGGL((mdot_partial_kernel<T, 6>), dim3(kRedBlocks, (k + kDotGroup - 1) / kDotGroup), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, k, V, stride, w, part);
     else
-        hipLaunchKernelGGL((mdot_partial_kernel<T, 5>), dim3(kRedBlocks, k), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, k, V, stride, w, part);
+        hipLaunchKernelGGL((mdot_partial_kernel<T, 5>), dim3(kRedBlocks, (k + kDotGroup - 1) / kDotGroup), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, k, V, stride, w, part);
     hipLaunchKernelGGL(reduce_final_kernel, dim3(k), dim3(kRedThreads), 0, s, part, out_dev);
     hipError_t e = hipMemcpyAsync(out_host, out_dev, k * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
