@@ -130,8 +130,9 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
     }
 }
 
+// one wavefront per workgroup, as the primal face kernel (3% faster here)
 template <int FLUX, int VISC, bool FULL, int NV>
-__global__ void __launch_bounds__(256) face_flux_t_kernel(DevGrid g, Params P, DualLoader<NV, FULL> L, double* __restrict__ dFX,
+__global__ void __launch_bounds__(64) face_flux_t_kernel(DevGrid g, Params P, DualLoader<NV, FULL> L, double* __restrict__ dFX,
                                                           int* __restrict__ status) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
@@ -237,7 +238,7 @@ __global__ void __launch_bounds__(256) newton_op_t_kernel(DevGrid g, Params P, c
 // ---------------------------------------------------------------------------
 template <int FLUX, int VISC, bool FULL, int NV>
 static void launch_ft(const DevGrid& g, const Params& P, const DualLoader<NV, FULL>& L, double* dFX, int* status, hipStream_t s) {
-    hipLaunchKernelGGL((face_flux_t_kernel<FLUX, VISC, FULL, NV>), dim3((g.F + 255) / 256), dim3(256), 0, s, g, P, L, dFX, status);
+    hipLaunchKernelGGL((face_flux_t_kernel<FLUX, VISC, FULL, NV>), dim3((g.F + 63) / 64), dim3(64), 0, s, g, P, L, dFX, status);
 }
 template <int FLUX, bool FULL, int NV>
 static void launch_fv(const DevGrid& g, const Params& P, const DualLoader<NV, FULL>& L, double* dFX, int* status, hipStream_t s) {
