@@ -49,6 +49,10 @@ struct DevGrid {
 __host__ __device__ constexpr int pair_groups(int rows) { return (rows + 1) / 2; }
 __host__ __device__ __forceinline__ size_t pidx(size_t ld, int k, size_t i) { return ((size_t)(k >> 1) * ld + i) * 2 + (k & 1); }
 
+// entries between consecutive contiguous nv x nv blocks: nv^2 rounded up to
+// even, so that every block starts on a 16-byte boundary
+__host__ __device__ constexpr int block_stride(int nv) { return 2 * pair_groups(nv * nv); }
+
 template <typename T> struct PairOf;
 template <> struct PairOf<double> { using type = double2; };
 template <> struct PairOf<float> { using type = float2; };

@@ -50,7 +50,8 @@ template <typename T>
 void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T* JL, const T* JR, const T* b, T* x, int sweeps,
                     hipStream_t s, bool zero_init = true, int dir = 0);
 // batched nv x nv inverses of n row-major blocks, stored in row pairs
-// (paired, the mesh level's layout) or plainly as [nv^2][ld] (coarse levels)
+// (paired, the mesh level's layout, stride ld) or each block contiguous,
+// block_stride(nv) entries apart (coarse levels; ld unused)
 void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, int* status, hipStream_t s, bool paired);
 // max over interior cells of |dp|/(p+Pref) and |dT|/(T+Tref) for the update dW
 hipError_t dev_relmax(const DevGrid& g, double Pref, double Tref, const double* W, const double* dW, double* part, double* out_host,

@@ -233,7 +233,9 @@ __global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __r
     for (int j = 0; j < NV; ++j) y[j * ld + c] = acc[j];
 }
 
-// batched NV x NV inverse by Gauss-Jordan with partial pivoting (block-Jacobi)
+// batched NV x NV inverse by Gauss-Jordan with partial pivoting (block-Jacobi);
+// PAIRED: blocks in row pairs with stride ld (the mesh level), else each
+// block contiguous, entry q of block c at c * block_stride(NV) + q (coarse levels)
 template <typename T, int NV, bool PAIRED = true>
 __global__ void __launch_bounds__(256) bj_invert_kernel(int N, size_t ld, const double* __restrict__ D, T* __restrict__ Dinv,
                                                         int* __restrict__ status) {
@@ -244,7 +246,7 @@ __global__ void __launch_bounds__(256) bj_invert_kernel(int N, size_t ld, const 
     for (int i = 0; i < NV; ++i)
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
-            a[i][j] = (T)D[PAIRED ? pidx(ld, NV * i + j, c) : (NV * i + j) * ld + c];
+            a[i][j] = (T)D[PAIRED ? pidx(ld, NV * i + j, c) : (size_t)c * block_stride(NV) + NV * i + j];
             a[i][NV + j] = (i == j) ? T(1) : T(0);
         }
     bool singular = false;
@@ -280,7 +282,8 @@ __global__ void __launch_bounds__(256) bj_invert_kernel(int N, size_t ld, const 
     for (int i = 0; i < NV; ++i)
 #pragma unroll
         for (int j = 0; j < NV; ++j)
-            Dinv[PAIRED ? pidx(ld, NV * i + j, c) : (NV * i + j) * ld + c] = singular ? ((i == j) ? T(1) : T(0)) : a[i][NV + j];
+            Dinv[PAIRED ? pidx(ld, NV * i + j, c) : (size_t)c * block_stride(NV) + NV * i + j] =
+                singular ? ((i == j) ? T(1) : T(0)) : a[i][NV + j];
 }
 
 // y_c = D_c^{-1} x_c, or with TRANS y_c = D_c^{-T} x_c (the adjoint's

@@ -35,7 +35,7 @@ struct MgLevel {
     // Galerkin assembly lists: diagonal block I sums dsrc[dptr[I] .. dptr[I+1]),
     // off-diagonal block k sums osrc[optr[k] .. optr[k+1]) (codes: see fcfd_mg.hip)
     const int *dptr = nullptr, *dsrc = nullptr, *optr = nullptr, *osrc = nullptr;
-    double *D = nullptr, *A = nullptr, *Dinv = nullptr;   // [nv^2][n], [nv^2][nnz], [nv^2][n]
+    double *D = nullptr, *A = nullptr, *Dinv = nullptr;   // n, nnz, n contiguous blocks of block_stride(nv) entries
     double *b = nullptr, *x = nullptr, *r = nullptr;      // [nv][ld]
     Colors colors;
 };

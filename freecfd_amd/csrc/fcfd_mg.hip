@@ -3,9 +3,11 @@
 // Block conventions (those of the FD Jacobian and bsr_matvec_kernel): the
 // operator row of cell c is D_c x_c + sum over its faces of a block times the
 // other cell's x, the block being -JR_f^T when c is the face's parent and
-// +JL_f^T when c is its neighbour (JL/JR stored [(nv i + j) F + f] for row
-// j, column i).  Diagonal and coarse blocks are row-major, entry (j, i) at
-// [(nv j + i) nblk + k].  A Galerkin source code is (index << 2) | kind with
+// +JL_f^T when c is its neighbour (JL/JR in row pairs, entry nv i + j for row
+// j, column i).  Coarse-level blocks are row-major and each block contiguous:
+// entry (j, i) of block k at [k B + nv j + i], B = block_stride(nv), so a
+// thread that walks a row loads each block with a few 16-byte loads from two
+// or three cache lines.  A Galerkin source code is (index << 2) | kind with
 // kind 0 = a diagonal block of the finer level, 1 = -JR_f^T (parent row,
 // neighbour column), 2 = +JL_f^T (neighbour row, parent column), 3 = an
 // off-diagonal block of a coarse finer level.
@@ -22,12 +24,29 @@ namespace fcfd {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void load_block(const double* __restrict__ A, size_t k, double (&b)[NV * NV]) {
+    const double2* A2 = reinterpret_cast<const double2*>(A + k * block_stride(NV));
+#pragma unroll
+    for (int q = 0; q < block_stride(NV) / 2; ++q) {
+        const double2 t = A2[q];
+        b[2 * q] = t.x;
+        if (2 * q + 1 < NV * NV) b[2 * q + 1] = t.y;
+    }
+}
+template <int NV>
+__device__ __forceinline__ void store_block(double* __restrict__ A, size_t k, const double (&b)[NV * NV]) {
+    double2* A2 = reinterpret_cast<double2*>(A + k * block_stride(NV));
+#pragma unroll
+    for (int q = 0; q < block_stride(NV) / 2; ++q)
+        A2[q] = make_double2(b[2 * q], 2 * q + 1 < NV * NV ? b[2 * q + 1] : 0.0);
+}
+
 template <int NV, bool FINE>
 __global__ void __launch_bounds__(256) mg_assemble_kernel(int nblk, const int* __restrict__ ptr, const int* __restrict__ src,
                                                           double* __restrict__ out, const double* __restrict__ D0, size_t ld0,
                                                           const double* __restrict__ JL, const double* __restrict__ JR, size_t F,
-                                                          const double* __restrict__ Dl, size_t nl, const double* __restrict__ Al,
-                                                          size_t nnzl) {
+                                                          const double* __restrict__ Dl, const double* __restrict__ Al) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nblk) return;
     double acc[NV * NV];
@@ -57,22 +76,18 @@ __global__ void __launch_bounds__(256) mg_assemble_kernel(int nblk, const int* _
                     for (int i = 0; i < NV; ++i) acc[NV * j + i] += blk[NV * i + j];
             }
         } else {
-            if (kind == 0) {
+            double blk[NV * NV];
+            load_block<NV>(kind == 0 ? Dl : Al, idx, blk);
 #pragma unroll
-                for (int q = 0; q < NV * NV; ++q) acc[q] += Dl[q * nl + idx];
-            } else {
-#pragma unroll
-                for (int q = 0; q < NV * NV; ++q) acc[q] += Al[q * nnzl + idx];
-            }
+            for (int q = 0; q < NV * NV; ++q) acc[q] += blk[q];
         }
     }
-#pragma unroll
-    for (int q = 0; q < NV * NV; ++q) out[q * (size_t)nblk + k] = acc[q];
+    store_block<NV>(out, k, acc);
 }
 
 // one colour of a block Gauss-Seidel sweep on a coarse level
 template <int NV>
-__global__ void __launch_bounds__(256) mg_sgs_color_kernel(const int* __restrict__ cells, int count, size_t ld, size_t n, size_t nnz,
+__global__ void __launch_bounds__(256) mg_sgs_color_kernel(const int* __restrict__ cells, int count, size_t ld,
                                                            const int* __restrict__ rptr, const int* __restrict__ col,
                                                            const double* __restrict__ A, const double* __restrict__ Dinv,
                                                            const double* __restrict__ b, double* __restrict__ x) {
@@ -84,47 +99,53 @@ __global__ void __launch_bounds__(256) mg_sgs_color_kernel(const int* __restrict
     for (int j = 0; j < NV; ++j) acc[j] = b[j * ld + I];
     for (int k = rptr[I]; k < rptr[I + 1]; ++k) {
         const int J = col[k];
-        double xj[NV];
+        double xj[NV], a[NV * NV];
+        load_block<NV>(A, k, a);
 #pragma unroll
         for (int i = 0; i < NV; ++i) xj[i] = x[i * ld + J];
 #pragma unroll
         for (int j = 0; j < NV; ++j)
 #pragma unroll
-            for (int i = 0; i < NV; ++i) acc[j] -= A[(NV * j + i) * nnz + k] * xj[i];
+            for (int i = 0; i < NV; ++i) acc[j] -= a[NV * j + i] * xj[i];
     }
+    double di[NV * NV];
+    load_block<NV>(Dinv, I, di);
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         double s = 0.0;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) s += Dinv[(NV * j + i) * n + I] * acc[i];
+        for (int i = 0; i < NV; ++i) s += di[NV * j + i] * acc[i];
         x[j * ld + I] = s;
     }
 }
 
 // r = b - A x on a coarse level
 template <int NV>
-__global__ void __launch_bounds__(256) mg_residual_kernel(int n, size_t ld, size_t nnz, const int* __restrict__ rptr,
+__global__ void __launch_bounds__(256) mg_residual_kernel(int n, size_t ld, const int* __restrict__ rptr,
                                                           const int* __restrict__ col, const double* __restrict__ D,
                                                           const double* __restrict__ A, const double* __restrict__ b,
                                                           const double* __restrict__ x, double* __restrict__ r) {
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= n) return;
     double acc[NV], xi[NV];
+    double d[NV * NV];
+    load_block<NV>(D, I, d);
 #pragma unroll
     for (int j = 0; j < NV; ++j) { acc[j] = b[j * ld + I]; xi[j] = x[j * ld + I]; }
 #pragma unroll
     for (int j = 0; j < NV; ++j)
 #pragma unroll
-        for (int i = 0; i < NV; ++i) acc[j] -= D[(NV * j + i) * (size_t)n + I] * xi[i];
+        for (int i = 0; i < NV; ++i) acc[j] -= d[NV * j + i] * xi[i];
     for (int k = rptr[I]; k < rptr[I + 1]; ++k) {
         const int J = col[k];
-        double xj[NV];
+        double xj[NV], a[NV * NV];
+        load_block<NV>(A, k, a);
 #pragma unroll
         for (int i = 0; i < NV; ++i) xj[i] = x[i * ld + J];
 #pragma unroll
         for (int j = 0; j < NV; ++j)
 #pragma unroll
-            for (int i = 0; i < NV; ++i) acc[j] -= A[(NV * j + i) * nnz + k] * xj[i];
+            for (int i = 0; i < NV; ++i) acc[j] -= a[NV * j + i] * xj[i];
     }
 #pragma unroll
     for (int j = 0; j < NV; ++j) r[j * ld + I] = acc[j];
@@ -367,7 +388,7 @@ int mg_build(const DevGrid& g, const std::vector<int>& par, const std::vector<in
             return rc;
         L.colors.n = ncol;
         L.colors.ptr = cptr;
-        const size_t b2 = (size_t)nv * nv;
+        const size_t b2 = (size_t)block_stride(nv);   // entries per stored block (nv^2 and a pad entry when odd)
         if ((rc = mg_alloc(mg, b2 * nc, &L.D)) || (rc = mg_alloc(mg, b2 * std::max(L.nnz, 1), &L.A)) ||
             (rc = mg_alloc(mg, b2 * nc, &L.Dinv)) || (rc = mg_alloc(mg, (size_t)nv * L.ld, &L.b)) ||
             (rc = mg_alloc(mg, (size_t)nv * L.ld, &L.x)) || (rc = mg_alloc(mg, (size_t)nv * L.ld, &L.r)))
@@ -403,20 +424,19 @@ static void setup_t(const DevGrid& g, Multigrid& mg, const double* D, const doub
         MgLevel& L = mg.lv[l];
         if (l == 1) {
             hipLaunchKernelGGL((mg_assemble_kernel<NV, true>), dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.dptr, L.dsrc, L.D, D,
-                               (size_t)g.ld, JL, JR, (size_t)g.F, nullptr, (size_t)0, nullptr, (size_t)0);
+                               (size_t)g.ld, JL, JR, (size_t)g.F, nullptr, nullptr);
             if (L.nnz > 0)
                 hipLaunchKernelGGL((mg_assemble_kernel<NV, true>), dim3((L.nnz + 255) / 256), dim3(256), 0, s, L.nnz, L.optr, L.osrc, L.A,
-                                   D, (size_t)g.ld, JL, JR, (size_t)g.F, nullptr, (size_t)0, nullptr, (size_t)0);
+                                   D, (size_t)g.ld, JL, JR, (size_t)g.F, nullptr, nullptr);
         } else {
             const MgLevel& P = mg.lv[l - 1];
             hipLaunchKernelGGL((mg_assemble_kernel<NV, false>), dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.dptr, L.dsrc, L.D,
-                               nullptr, (size_t)0, nullptr, nullptr, (size_t)0, P.D, (size_t)P.n, P.A, (size_t)std::max(P.nnz, 1));
+                               nullptr, (size_t)0, nullptr, nullptr, (size_t)0, P.D, P.A);
             if (L.nnz > 0)
                 hipLaunchKernelGGL((mg_assemble_kernel<NV, false>), dim3((L.nnz + 255) / 256), dim3(256), 0, s, L.nnz, L.optr, L.osrc,
-                                   L.A, nullptr, (size_t)0, nullptr, nullptr, (size_t)0, P.D, (size_t)P.n, P.A,
-                                   (size_t)std::max(P.nnz, 1));
+                                   L.A, nullptr, (size_t)0, nullptr, nullptr, (size_t)0, P.D, P.A);
         }
-        dev_block_invert(NV, L.n, (size_t)L.n, L.D, L.Dinv, status, s, false);
+        dev_block_invert(NV, L.n, (size_t)0, L.D, L.Dinv, status, s, false);
     }
 }
 
@@ -433,7 +453,7 @@ static void coarse_sgs(const MgLevel& L, int sweeps, hipStream_t s, int dir) {
         const int cnt = L.colors.ptr[c + 1] - L.colors.ptr[c];
         if (cnt <= 0) return;
         hipLaunchKernelGGL(mg_sgs_color_kernel<NV>, dim3((cnt + 255) / 256), dim3(256), 0, s, L.colors.cells + L.colors.ptr[c], cnt,
-                           (size_t)L.ld, (size_t)L.n, (size_t)std::max(L.nnz, 1), L.rptr, L.col, L.A, L.Dinv, L.b, L.x);
+                           (size_t)L.ld, L.rptr, L.col, L.A, L.Dinv, L.b, L.x);
     };
     for (int sw = 0; sw < sweeps; ++sw) {
         if (dir != 2)
@@ -453,8 +473,8 @@ static void level_cycle(Multigrid& mg, size_t l, int sweeps, hipStream_t s) {
         return;
     }
     coarse_sgs<NV>(L, sweeps, s, 1);   // pre-smoothing: forward sweeps
-    hipLaunchKernelGGL(mg_residual_kernel<NV>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, (size_t)L.ld, (size_t)std::max(L.nnz, 1),
-                       L.rptr, L.col, L.D, L.A, L.b, L.x, L.r);
+    hipLaunchKernelGGL(mg_residual_kernel<NV>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, (size_t)L.ld, L.rptr, L.col, L.D, L.A,
+                       L.b, L.x, L.r);
     MgLevel& C = mg.lv[l + 1];
     hipLaunchKernelGGL(mg_restrict_kernel<NV>, dim3((C.n + 255) / 256), dim3(256), 0, s, C.n, C.mptr, C.mem, L.r, (size_t)L.ld, C.b,
                        (size_t)C.ld);
