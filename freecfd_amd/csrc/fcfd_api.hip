@@ -797,8 +797,6 @@ int fcfd_jvp(fcfd_ctx* c, double* W_dev, double* dW_dev, double* dR_dev, int mod
     return FCFD_OK;
 }
 
-// Greedy colouring of the interior cells on the face graph, in internal (RCM)
-// order: each cell takes the smallest colour none of its face neighbours has.
 // buffers of the implicit step: residual, dt, updates, FD blocks, the
 // Krylov basis of the requested restart and precision
 static int ensure_newton(fcfd_ctx* c, const fcfd_newton_opts* o) {
@@ -891,6 +889,9 @@ static int ensure_mg_graph(fcfd_ctx* c, int sweeps) {
     return FCFD_OK;
 }
 
+// colouring of the interior cells on the face graph (color_graph: greedy in
+// internal or breadth-first order, whichever needs fewer colours), for the
+// multicolour block Gauss-Seidel sweeps
 static int ensure_colors(fcfd_ctx* c) {
     if (c->colors.n > 0) return FCFD_OK;
     const HostGrid& g = c->hg;

@@ -6,7 +6,7 @@
 
 namespace fcfd {
 
-// All arrays are device pointers; cells and faces are in internal (RCM)
+// All arrays are device pointers; cells and faces are in internal (space-filling-curve or RCM)
 // order.  Cell arrays are padded to `ld` entries; interior cells [0,N), ghosts
 // [N, N+B).  Per-cell variable arrays are [nv][ld] (p, u, v, w, T, and nu~
 // with the SA model), gradients 3 nv and limiters nv components per cell in
@@ -84,7 +84,7 @@ __device__ __forceinline__ void load_pairs(const T* __restrict__ A, size_t ld, s
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2: workgroup b runs on XCD b % 8.  xcd_block() renumbers them so that XCD x
 // takes the contiguous range [x q + min(x, r), ...) of logical blocks (q, r =
-// gridDim / 8, gridDim % 8): faces and cells that are close in the (RCM)
+// gridDim / 8, gridDim % 8): faces and cells that are close in the internal
 // numbering, and so share gathered cell data, then share an L2.  A bijection
 // of [0, gridDim) for every grid size.
 // faces of a cell: at most 6 for the supported cell types (tetrahedron,

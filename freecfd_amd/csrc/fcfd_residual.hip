@@ -6,14 +6,20 @@
 //   1. ghost_kernel      one thread per boundary face: boundary ghost state
 //   2. grad_lim_kernel   one thread per cell: gradient-map gather (sliced
 //                        ELL, 64-wide slices matching the wavefront) fused
-//                        with the limiter; also copies gradients to ghosts
-//   3. flux_kernel       one thread per half-face (cell, face) pair: each
-//                        block owns whole cells, computes the face fluxes of
-//                        their faces, stages them in LDS and sums every cell's
-//                        contributions in reference face order.  An interior
-//                        face is evaluated by both of its cells with
-//                        identical inputs and operation order, so both sides
-//                        see the bitwise-same flux and R is conservative.
+//                        with the limiter (and, with SA, the cell's V S
+//                        source); also copies gradients to the ghosts
+//   3. face_flux_kernel  one thread per face, one wavefront per workgroup:
+//                        reconstruction, boundary condition, convective and
+//                        diffusive flux; D - C per face into FX [nv][F]
+//   4. cell_sum_kernel   one thread per cell: the cell's face contributions
+//                        in reference face order (the reference's
+//                        VecSetValues(ADD) sequence), so R rounds as the
+//                        reference's does
+// The gather kernels are bound by chains of dependent loads (an index, then
+// what it points at), so each issues its loads in as few dependent rounds as
+// it can: all of a cell's half-face entries at once (at most kMaxCellFaces,
+// indices past the count clamped to a valid slot and the arithmetic masked),
+// a face's packed records together before the cell loads they index.
 #include <hip/hip_runtime.h>
 
 #include "fcfd_device.h"
