@@ -1083,10 +1083,14 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
     if (o->dt_type != FCFD_DT_CFL_LOCAL) return fail(c, FCFD_ENOTSUP, "only CFL_LOCAL time stepping is implemented");
     if (o->krylov_fp32) return fail(c, FCFD_ENOTSUP, "the adjoint step solves in FP64 (krylov_fp32 must be 0)");
-    if (o->precond != FCFD_PC_BLOCK_JACOBI) return fail(c, FCFD_ENOTSUP, "the adjoint step preconditions with block Jacobi");
+    if (o->precond != FCFD_PC_BLOCK_JACOBI && o->precond != FCFD_PC_BLOCK_SGS)
+        return fail(c, FCFD_ENOTSUP, "the adjoint step preconditions with block Jacobi or block SGS");
+    if (o->precond == FCFD_PC_BLOCK_SGS && o->sweeps < 1) return fail(c, FCFD_EINVAL, "block SGS needs sweeps >= 1");
     if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad Krylov options");
+    const bool sgs = o->precond == FCFD_PC_BLOCK_SGS;
     int rc = ensure_adjoint(c);
     if (!rc) rc = ensure_newton(c, o);
+    if (!rc && sgs) rc = ensure_colors(c);
     if (rc) return rc;
     const DevGrid& g = c->dg;
     hipStream_t s = (hipStream_t)stream;
@@ -1107,7 +1111,8 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
         if (st) *st = S;
         return fail(c, FCFD_EDIVERGED, "Divergence detected: NaN/Inf in the adjoint residual");
     }
-    // 2. local time step, FD blocks with the floor step, transposed block Jacobi
+    // 2. local time step, FD blocks with the floor step, their diagonal
+    //    inverses (transposed block Jacobi or transposed block SGS)
     launch_dt_local(g, c->P, W_dev, o->cfl, c->dt, s);
     launch_fd_jacobian(g, c->P, W_dev, c->G, c->PHI, c->zero_upd, c->dt, c->JL, c->JR, c->Dg, c->status, s);
     dev_bj_invert<double>(g, c->Dg, c->Dinv64, c->status, s);
@@ -1121,7 +1126,10 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
         launch_vjp(g, c->ag, c->P, W_dev, c->G, c->PHI, c->dref, x, nullptr, c->aw, c->adj_v, c->status, ss);
         launch_newton_op_t(g, c->P, W_dev, c->dt, x, c->adj_v, y, ss);
     }};
-    LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) { dev_bj_apply_t(g, c->Dinv64, x, y, ss); }};
+    LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) {
+        if (sgs) dev_bsgs_apply<double>(g, c->colors, c->Dinv64, c->JL, c->JR, x, y, o->sweeps, ss, true, 0, true);
+        else dev_bj_apply_t(g, c->Dinv64, x, y, ss);
+    }};
     rc = gmres<double>(g, A, M, c->R, c->dW, go, c->w64, gs, s);
     if (rc) return fail(c, rc, "GMRES failed (HIP error)");
     HIPCHK(c, hipStreamSynchronize(s));

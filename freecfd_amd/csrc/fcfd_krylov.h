@@ -45,10 +45,11 @@ struct Colors {
 };
 // x = M^{-1} b by `sweeps` multicolour block Gauss-Seidel sweeps from x = 0
 // (zero_init) or from the x given; dir 0: symmetric (colours forward then
-// backward), 1: forward only, 2: backward only
+// backward), 1: forward only, 2: backward only; trans: sweeps on the
+// transposed operator (the adjoint's preconditioner)
 template <typename T>
 void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T* JL, const T* JR, const T* b, T* x, int sweeps,
-                    hipStream_t s, bool zero_init = true, int dir = 0);
+                    hipStream_t s, bool zero_init = true, int dir = 0, bool trans = false);
 // batched nv x nv inverses of n row-major blocks, stored in row pairs
 // (paired, the mesh level's layout, stride ld) or each block contiguous,
 // block_stride(nv) entries apart (coarse levels; ld unused)

@@ -175,11 +175,13 @@ __global__ void __launch_bounds__(kRedThreads) update_kernel(int N, size_t ld, d
 // acc[j] (+ or, SUB, -)= sg B[i][j] x_o[i] over the cell's half-faces in
 // order, B = JR (sg = -1) where the cell is the face's parent and JL (sg = +1)
 // where it is the neighbour, o the cell across; faces to a boundary ghost
-// carry no column (the ghost coupling is in JL).  The half-face entries are
+// carry no column (the ghost coupling is in JL).  TRANS: the same row of the
+// transposed operator, acc[j] (+/-)= sg B[j][i] x_o[i] with B = JL (sg = +1)
+// on the parent side and JR (sg = -1) on the neighbour side.  The half-face entries are
 // loaded in one round; each face's loads use valid (clamped) indices and only
 // the accumulation is masked, so the loads of several faces can be in flight
 // together.  x: [nv][ld] (ghost rows exist, read but unused).
-template <typename T, int NV, bool SUB>
+template <typename T, int NV, bool SUB, bool TRANS = false>
 __device__ __forceinline__ void offdiag_accumulate(const DevGrid& g, const T* __restrict__ JL, const T* __restrict__ JR,
                                                    const T* __restrict__ x, int c, T (&acc)[NV]) {
     const size_t ld = g.ld, Fs = g.F;
@@ -197,15 +199,16 @@ __device__ __forceinline__ void offdiag_accumulate(const DevGrid& g, const T* __
         T xo[NV], blk[NV * NV];
 #pragma unroll
         for (int i = 0; i < NV; ++i) xo[i] = x[i * ld + o];
-        load_pairs<NV * NV>(fs[k] >= 0 ? JR : JL, Fs, f, blk);
+        load_pairs<NV * NV>((fs[k] >= 0) != TRANS ? JR : JL, Fs, f, blk);
         if (k < nh && o < g.N) {
-            const T sg = fs[k] >= 0 ? T(-1) : T(1);
+            const T sg = (fs[k] >= 0) != TRANS ? T(-1) : T(1);
 #pragma unroll
             for (int i = 0; i < NV; ++i)
 #pragma unroll
                 for (int j = 0; j < NV; ++j) {
-                    if (SUB) acc[j] -= sg * blk[NV * i + j] * xo[i];
-                    else acc[j] += sg * blk[NV * i + j] * xo[i];
+                    const T bij = TRANS ? blk[NV * j + i] : blk[NV * i + j];
+                    if (SUB) acc[j] -= sg * bij * xo[i];
+                    else acc[j] += sg * bij * xo[i];
                 }
         }
     }
@@ -309,8 +312,9 @@ __global__ void __launch_bounds__(256) bj_apply_kernel(int N, size_t ld, const T
 // one colour of a multicolour block Gauss-Seidel sweep: for every cell of the
 // colour, x_c = D_c^{-1} (b_c - sum over its faces of the off-diagonal block
 // times x of the neighbour), with the latest x of the other colours (cells of
-// one colour share no face, so the update is race-free)
-template <typename T, int NV>
+// one colour share no face, so the update is race-free); TRANS: the sweep on
+// the transposed operator (transposed off-diagonal blocks, D_c^{-T})
+template <typename T, int NV, bool TRANS = false>
 __global__ void __launch_bounds__(256) bsgs_color_kernel(DevGrid g, const T* __restrict__ Dinv, const T* __restrict__ JL,
                                                          const T* __restrict__ JR, const T* __restrict__ b, T* __restrict__ x,
                                                          const int* __restrict__ cells, int count) {
@@ -321,13 +325,13 @@ __global__ void __launch_bounds__(256) bsgs_color_kernel(DevGrid g, const T* __r
     T acc[NV], blk[NV * NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = b[j * ld + c];
-    offdiag_accumulate<T, NV, true>(g, JL, JR, x, c, acc);
+    offdiag_accumulate<T, NV, true, TRANS>(g, JL, JR, x, c, acc);
     load_pairs<NV * NV>(Dinv, ld, c, blk);
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         T s = 0;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) s += blk[NV * j + i] * acc[i];
+        for (int i = 0; i < NV; ++i) s += (TRANS ? blk[NV * i + j] : blk[NV * j + i]) * acc[i];
         x[j * ld + c] = s;
     }
 }
@@ -433,17 +437,18 @@ void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, i
 
 template <typename T>
 void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T* JL, const T* JR, const T* b, T* x, int sweeps,
-                    hipStream_t s, bool zero_init, int dir) {
+                    hipStream_t s, bool zero_init, int dir, bool trans) {
     const size_t n = (size_t)g.nv * g.ld;
     if (zero_init) hipLaunchKernelGGL(scale_copy_kernel<T>, ew_grid(n), dim3(256), 0, s, n, T(0), b, x);   // x = 0
     auto one = [&](int k) {
         const int cnt = col.ptr[k + 1] - col.ptr[k];
         if (cnt <= 0) return;
         const int* cells = col.cells + col.ptr[k];
-        if (g.nv == 6)
-            hipLaunchKernelGGL((bsgs_color_kernel<T, 6>), dim3((cnt + 255) / 256), dim3(256), 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
-        else
-            hipLaunchKernelGGL((bsgs_color_kernel<T, 5>), dim3((cnt + 255) / 256), dim3(256), 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
+        const dim3 grid((cnt + 255) / 256), blk(256);
+        if (g.nv == 6 && trans) hipLaunchKernelGGL((bsgs_color_kernel<T, 6, true>), grid, blk, 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
+        else if (g.nv == 6) hipLaunchKernelGGL((bsgs_color_kernel<T, 6>), grid, blk, 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
+        else if (trans) hipLaunchKernelGGL((bsgs_color_kernel<T, 5, true>), grid, blk, 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
+        else hipLaunchKernelGGL((bsgs_color_kernel<T, 5>), grid, blk, 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
     };
     for (int sw = 0; sw < sweeps; ++sw) {
         if (dir != 2)
@@ -453,9 +458,9 @@ void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T*
     }
 }
 template void dev_bsgs_apply<double>(const DevGrid&, const Colors&, const double*, const double*, const double*, const double*, double*,
-                                     int, hipStream_t, bool, int);
+                                     int, hipStream_t, bool, int, bool);
 template void dev_bsgs_apply<float>(const DevGrid&, const Colors&, const float*, const float*, const float*, const float*, float*, int,
-                                    hipStream_t, bool, int);
+                                    hipStream_t, bool, int, bool);
 
 hipError_t dev_relmax(const DevGrid& g, double Pref, double Tref, const double* W, const double* dW, double* part, double* out_host,
                       hipStream_t s) {

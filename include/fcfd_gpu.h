@@ -32,7 +32,9 @@
  * produces on one rank (cells in input order, faces in creation order of
  * src/grid/grid_create_elements.cc:184-442, boundary ghosts appended per BC
  * region in face order, grid_create_elements.cc:702-743).  Device arrays use
- * the context's internal (reverse Cuthill-McKee) numbering and a
+ * the context's internal numbering (cells along a space-filling curve, or in
+ * reverse Cuthill-McKee order with the environment variable
+ * FCFD_CELL_ORDER=rcm; results do not depend on it) and a
  * structure-of-arrays layout: variable v of internal cell i lives at
  * ptr[v*ld + i], where ld = fcfd_info.ld_cell (a multiple of 64).
  */
@@ -352,12 +354,14 @@ int  fcfd_loads(fcfd_ctx* ctx, double* W_dev, double* loads_host, void* stream);
 /* One pseudo-time step of the steady adjoint (dR/dW)^T lam = -g at the state
  * W_dev (g = dJ/dW of an output J, e.g. from fcfd_vjp with a load weight and
  * Rbar = 0): with rho = (dR/dW)^T lam + g, solves ((P V/dt) - dR/dW)^T delta
- * = rho by FP64 GMRES with the exact transpose (fcfd_vjp) as the operator and
- * the transposed FD block-Jacobi of the same step as the preconditioner, then
+ * = rho by FP64 GMRES with the exact transpose (fcfd_vjp) as the operator and,
+ * as the preconditioner, the transposed FD block Jacobi (FCFD_PC_BLOCK_JACOBI)
+ * or `sweeps` symmetric multicolour block Gauss-Seidel sweeps on the
+ * transposed FD operator (FCFD_PC_BLOCK_SGS) of the same step, then
  * lam += delta.  As CFL grows it tends to the Newton solve of the adjoint
  * system, which is linear: lam converges at the rate of the Krylov solves.
- * opts: cfl, restart, max_iters, rtol, abstol, knoll_guess; krylov_fp32 must
- * be 0 and precond FCFD_PC_BLOCK_JACOBI.  stats->res_norm = ||rho||_2 before
+ * opts: cfl, restart, max_iters, rtol, abstol, knoll_guess, precond, sweeps;
+ * krylov_fp32 must be 0.  stats->res_norm = ||rho||_2 before
  * the step.  g_dev and lam_dev are [nv][ld_cell] (lam updated in place);
  * the ghost slots of W_dev are written.  No counterpart in the reference. */
 int  fcfd_adjoint_step(fcfd_ctx* ctx, double* W_dev, const double* g_dev, double* lam_dev, const fcfd_newton_opts* opts,

@@ -75,6 +75,34 @@ def test_adjoint_converges_and_matches_fd(converged):
         assert rel < 1e-4
 
 
+def test_adjoint_step_block_sgs_matches_block_jacobi(converged):
+    """One adjoint step from lam = 0 preconditioned by the transposed block SGS
+    sweep reaches the same update as with the transposed block Jacobi (the
+    preconditioner changes the Krylov iterations, not the step's linear
+    system), in fewer iterations: a sweep over the wrong transpose would still
+    converge, but no faster than block Jacobi."""
+    ns, cs, q, params, vals = converged
+    torch = ns.torch
+    area = S.wall_area(ns, WALL)
+    g = ns.vjp(torch.zeros_like(ns.W), fx_weight=S.drag_weight(int(ns._opt.n_bc), WALL, q, area)).clone()
+    out = {}
+    for pc in (A.PC_BLOCK_JACOBI, A.PC_BLOCK_SGS):
+        lam = torch.zeros_like(ns.W)
+        opts = A.newton_opts(cfl=20.0, jacobian=A.JAC_FD, fp32=False, restart=120, max_iters=480, rtol=1e-12, abstol=0.0,
+                             knoll=False, precond=pc, sweeps=1)
+        st = ns.adjoint_step(g, lam, opts)
+        out[pc] = (lam.clone(), st.krylov_iters, st.krylov_converged)
+        print("  precond %d: %d Krylov iterations, converged %d, rnorm %.3e/%.3e" % (pc, st.krylov_iters, st.krylov_converged,
+                                                                                 st.krylov_rnorm, st.krylov_r0norm), flush=True)
+    lj, ij, cj = out[A.PC_BLOCK_JACOBI]
+    ls, isgs, csgs = out[A.PC_BLOCK_SGS]
+    err = float((ls - lj).abs().max() / lj.abs().max())
+    print("  max |lam_SGS - lam_BJ| / max |lam_BJ| = %.3e" % err, flush=True)
+    assert cj and csgs
+    assert err < 1e-6
+    assert isgs < ij
+
+
 def test_set_bc_roundtrip(converged):
     """fcfd_set_bc changes the residual through the BC table and restores it
     exactly when the old values come back."""
