@@ -38,6 +38,7 @@ struct MgLevel {
     double *D = nullptr, *A = nullptr, *Dinv = nullptr;   // n, nnz, n contiguous blocks of block_stride(nv) entries
     double *b = nullptr, *x = nullptr, *r = nullptr;      // [nv][ld]
     Colors colors;
+    const int* cptr_dev = nullptr;       // colours.ptr on the device (small levels' one-launch sweeps)
 };
 
 struct Multigrid {

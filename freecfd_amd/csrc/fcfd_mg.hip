@@ -85,15 +85,12 @@ __global__ void __launch_bounds__(256) mg_assemble_kernel(int nblk, const int* _
     store_block<NV>(out, k, acc);
 }
 
-// one colour of a block Gauss-Seidel sweep on a coarse level
+// block Gauss-Seidel update of coarse row I: x_I = D_I^{-1} (b_I - sum of
+// the row's off-diagonal blocks times x)
 template <int NV>
-__global__ void __launch_bounds__(256) mg_sgs_color_kernel(const int* __restrict__ cells, int count, size_t ld,
-                                                           const int* __restrict__ rptr, const int* __restrict__ col,
-                                                           const double* __restrict__ A, const double* __restrict__ Dinv,
-                                                           const double* __restrict__ b, double* __restrict__ x) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const int I = cells[t];
+__device__ __forceinline__ void sgs_row(int I, size_t ld, const int* __restrict__ rptr, const int* __restrict__ col,
+                                        const double* __restrict__ A, const double* __restrict__ Dinv, const double* __restrict__ b,
+                                        double* __restrict__ x) {
     double acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = b[j * ld + I];
@@ -116,6 +113,41 @@ __global__ void __launch_bounds__(256) mg_sgs_color_kernel(const int* __restrict
 #pragma unroll
         for (int i = 0; i < NV; ++i) s += di[NV * j + i] * acc[i];
         x[j * ld + I] = s;
+    }
+}
+
+// one colour of a block Gauss-Seidel sweep on a coarse level
+template <int NV>
+__global__ void __launch_bounds__(256) mg_sgs_color_kernel(const int* __restrict__ cells, int count, size_t ld,
+                                                           const int* __restrict__ rptr, const int* __restrict__ col,
+                                                           const double* __restrict__ A, const double* __restrict__ Dinv,
+                                                           const double* __restrict__ b, double* __restrict__ x) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    sgs_row<NV>(cells[t], ld, rptr, col, A, Dinv, b, x);
+}
+
+// all the colours of `sweeps` sweeps on a level of at most kSmallLevel cells
+// in one workgroup, a workgroup barrier between colours (the x a colour
+// writes is visible to the next one: the workgroup shares one CU's cache);
+// the same updates in the same colour order as the per-colour launches, in
+// one launch instead of one per colour.  dir as in coarse_sgs.
+constexpr int kSmallLevel = 1024;
+template <int NV>
+__global__ void __launch_bounds__(kSmallLevel) mg_sgs_level_kernel(int ncol, const int* __restrict__ cptr, const int* __restrict__ cells,
+                                                                   size_t ld, const int* __restrict__ rptr, const int* __restrict__ col,
+                                                                   const double* __restrict__ A, const double* __restrict__ Dinv,
+                                                                   const double* __restrict__ b, double* __restrict__ x, int sweeps,
+                                                                   int dir) {
+    for (int sw = 0; sw < sweeps; ++sw) {
+        for (int pass = 0; pass < 2; ++pass) {
+            if ((pass == 0 && dir == 2) || (pass == 1 && dir == 1)) continue;
+            for (int q = 0; q < ncol; ++q) {
+                const int k = pass == 0 ? q : ncol - 1 - q;
+                for (int t = cptr[k] + threadIdx.x; t < cptr[k + 1]; t += blockDim.x) sgs_row<NV>(cells[t], ld, rptr, col, A, Dinv, b, x);
+                __syncthreads();
+            }
+        }
     }
 }
 
@@ -384,7 +416,7 @@ int mg_build(const DevGrid& g, const std::vector<int>& par, const std::vector<in
         if ((rc = mg_upload(mg, agg, &L.agg)) || (rc = mg_upload(mg, mptr, &L.mptr)) || (rc = mg_upload(mg, mem, &L.mem)) ||
             (rc = mg_upload(mg, rptr, &L.rptr)) || (rc = mg_upload(mg, col, &L.col)) || (rc = mg_upload(mg, dptr, &L.dptr)) ||
             (rc = mg_upload(mg, dsrc, &L.dsrc)) || (rc = mg_upload(mg, optr, &L.optr)) || (rc = mg_upload(mg, osrc, &L.osrc)) ||
-            (rc = mg_upload(mg, cells, &L.colors.cells)))
+            (rc = mg_upload(mg, cells, &L.colors.cells)) || (rc = mg_upload(mg, cptr, &L.cptr_dev)))
             return rc;
         L.colors.n = ncol;
         L.colors.ptr = cptr;
@@ -449,6 +481,11 @@ void mg_setup(const DevGrid& g, Multigrid& mg, const double* D, const double* JL
 
 template <int NV>
 static void coarse_sgs(const MgLevel& L, int sweeps, hipStream_t s, int dir) {
+    if (L.n <= kSmallLevel) {   // small level: every colour in one workgroup launch
+        hipLaunchKernelGGL(mg_sgs_level_kernel<NV>, dim3(1), dim3(kSmallLevel), 0, s, L.colors.n, L.cptr_dev, L.colors.cells,
+                           (size_t)L.ld, L.rptr, L.col, L.A, L.Dinv, L.b, L.x, sweeps, dir);
+        return;
+    }
     auto one = [&](int c) {
         const int cnt = L.colors.ptr[c + 1] - L.colors.ptr[c];
         if (cnt <= 0) return;
