@@ -39,6 +39,7 @@ struct MgLevel {
     double *b = nullptr, *x = nullptr, *r = nullptr;      // [nv][ld]
     Colors colors;
     const int* cptr_dev = nullptr;       // colours.ptr on the device (small levels' one-launch sweeps)
+    bool one_launch = false;             // sweeps of every colour in one workgroup launch (small levels)
 };
 
 struct Multigrid {

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -420,6 +421,10 @@ int mg_build(const DevGrid& g, const std::vector<int>& par, const std::vector<in
             return rc;
         L.colors.n = ncol;
         L.colors.ptr = cptr;
+        // FCFD_MG_NO_LEVEL_LAUNCH=1: one launch per colour on every level (the
+        // same updates; for checking the one-launch sweeps against them)
+        const char* e = std::getenv("FCFD_MG_NO_LEVEL_LAUNCH");
+        L.one_launch = nc <= kSmallLevel && !(e && std::strcmp(e, "1") == 0);
         const size_t b2 = (size_t)block_stride(nv);   // entries per stored block (nv^2 and a pad entry when odd)
         if ((rc = mg_alloc(mg, b2 * nc, &L.D)) || (rc = mg_alloc(mg, b2 * std::max(L.nnz, 1), &L.A)) ||
             (rc = mg_alloc(mg, b2 * nc, &L.Dinv)) || (rc = mg_alloc(mg, (size_t)nv * L.ld, &L.b)) ||
@@ -481,7 +486,7 @@ void mg_setup(const DevGrid& g, Multigrid& mg, const double* D, const double* JL
 
 template <int NV>
 static void coarse_sgs(const MgLevel& L, int sweeps, hipStream_t s, int dir) {
-    if (L.n <= kSmallLevel) {   // small level: every colour in one workgroup launch
+    if (L.one_launch) {   // small level: every colour in one workgroup launch
         hipLaunchKernelGGL(mg_sgs_level_kernel<NV>, dim3(1), dim3(kSmallLevel), 0, s, L.colors.n, L.cptr_dev, L.colors.cells,
                            (size_t)L.ld, L.rptr, L.col, L.A, L.Dinv, L.b, L.x, sweeps, dir);
         return;

@@ -67,6 +67,63 @@ def test_newton_linear_solve_matches_direct(fp32, precond):
     assert abs(st.res_norm - np.linalg.norm(R)) <= 1e-12 * np.linalg.norm(R) + 1e-300 or np.isclose(st.res_norm, np.linalg.norm(R), rtol=1e-10)
 
 
+def test_multigrid_linear_solve_matches_direct_with_large_coarse_level():
+    """The multigrid-preconditioned solve on the 128x64 cylinder (8,192 cells),
+    whose first coarse level is above the 1024 cells that one workgroup
+    smooths in one launch (so it runs one launch per colour, the smaller
+    levels one launch per sweep), against the direct sparse solve of the
+    CPU restatement's system.  The case is viscous with Sutherland's law,
+    whose pow is not bitwise equal on the two sides: the FD Jacobians agree
+    to about 1e-6, and so do the solutions."""
+    from oracle.oracle import Oracle
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.cylinder_case(128, 64, noise=0.01)
+    print("case cylinder 128x64 multigrid", flush=True)
+    o = Oracle(cs.raw, cs.opt)
+    R, _ = o.residual(cs.W0, with_jacobian=True)
+    dt, _ = o.time_step(A.DT_CFL_LOCAL, 10.0)
+    Amat = _assemble(o, dt)
+    dW_ref = spla.spsolve(Amat.tocsc(), R.reshape(-1)).reshape(-1, 5)
+    o.close()
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    opts = A.newton_opts(cfl=10.0, restart=60, max_iters=400, rtol=1e-12, abstol=0.0, precond=A.PC_MULTIGRID, sweeps=1)
+    st = ns.newton_step(opts)
+    dW = ns.state_host(include_ghosts=False) - cs.W0
+    err = (np.abs(dW - dW_ref) / (np.abs(dW_ref).max() + 1e-300)).max()
+    print("  iters %d rnorm %.3e/%.3e  max rel err %.3e" % (st.krylov_iters, st.krylov_rnorm, st.krylov_r0norm, err), flush=True)
+    assert st.krylov_converged
+    assert err < 1e-5
+    ns.close()
+
+
+def test_multigrid_level_launch_sweeps_match_per_colour_launches():
+    """The one-launch sweeps of the small coarse levels (all colours in one
+    workgroup, barriers between colours) give bit-identical Newton steps to
+    one launch per colour (FCFD_MG_NO_LEVEL_LAUNCH=1), over three steps with
+    both Krylov precisions."""
+    import os
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.cylinder_case(64, 32)
+    out = {}
+    for mode in ("per_colour", "level"):
+        if mode == "per_colour":
+            os.environ["FCFD_MG_NO_LEVEL_LAUNCH"] = "1"
+        else:
+            os.environ.pop("FCFD_MG_NO_LEVEL_LAUNCH", None)
+        ns = NavierStokes(cs.raw, cs.opt)
+        ns.set_state(cs.W0)
+        for fp32, jac in ((False, A.JAC_FD), (True, A.JAC_JVP_FULL), (False, A.JAC_FD)):
+            opts = A.newton_opts(cfl=100.0, jacobian=jac, fp32=fp32, restart=20, max_iters=40, rtol=1e-8,
+                                 precond=A.PC_MULTIGRID, sweeps=1)
+            ns.newton_step(opts)
+        out[mode] = ns.state_host(include_ghosts=False)
+        ns.close()
+    os.environ.pop("FCFD_MG_NO_LEVEL_LAUNCH", None)
+    print("  max |W_level - W_per_colour| = %.3e" % np.abs(out["level"] - out["per_colour"]).max(), flush=True)
+    assert np.array_equal(out["level"], out["per_colour"])
+
+
 def test_wedge_newton_converges():
     from freecfd_amd.ns import NavierStokes
     cs = cases.wedge_case(96, 48)
