@@ -31,6 +31,7 @@ def lib():
         "fcfd_cells_to_host": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp]),
         "fcfd_residual": (C.c_int, [_vp, _vp, _vp, _vp]),
         "fcfd_internal_buffers": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
+        "fcfd_gradients_limiters": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "fcfd_status": (C.c_int, [_vp, C.c_int]),
         "fcfd_profile": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
         "fcfd_fd_jacobian": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),

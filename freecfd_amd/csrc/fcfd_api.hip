@@ -576,6 +576,16 @@ int fcfd_profile(fcfd_ctx* c, int enable, double* stage_ms, long long* calls) {
     return FCFD_OK;
 }
 
+int fcfd_gradients_limiters(fcfd_ctx* c, double* W_dev, double* G_dev, double* PHI_dev, void* stream) {
+    if (!c || !W_dev) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    hipStream_t s = (hipStream_t)stream;
+    launch_ghosts(c->dg, c->P, W_dev, c->status, s);
+    launch_grad_lim(c->dg, c->P, W_dev, G_dev ? G_dev : c->G, PHI_dev ? PHI_dev : c->PHI, c->dref, s, nullptr);
+    HIPCHK(c, hipGetLastError());
+    return FCFD_OK;
+}
+
 int fcfd_internal_buffers(fcfd_ctx* c, const double** grad, const double** phi) {
     if (!c || !c->device) return FCFD_EINVAL;
     if (grad) *grad = c->G;

@@ -113,6 +113,21 @@ class NavierStokes(Grid):
     def state_host(self, include_ghosts=True):
         return self.to_host(self.W, self.nv, include_ghosts)
 
+    def gradients_limiters(self, W=None, G=None, PHI=None, stream=None):
+        """Boundary ghosts, gradients and limiters of W alone on the GPU
+        (update_boundaries + calc_cell_grads + calc_limiter) into G and PHI,
+        float64 device tensors in row pairs of 2*ceil(3nv/2)*ld and
+        2*ceil(nv/2)*ld entries (allocated when not given); returns (G, PHI)."""
+        W = self.W if W is None else W
+        ld, nv = self.info.ld_cell, self.nv
+        if G is None:
+            G = self.torch.zeros(2 * ((3 * nv + 1) // 2) * ld, dtype=self.torch.float64, device=self.device)
+        if PHI is None:
+            PHI = self.torch.zeros(2 * ((nv + 1) // 2) * ld, dtype=self.torch.float64, device=self.device)
+        s = self.torch.cuda.current_stream().cuda_stream if stream is None else stream
+        _check(self.h, lib().fcfd_gradients_limiters(self.h, W.data_ptr(), G.data_ptr(), PHI.data_ptr(), C.c_void_p(s)))
+        return G, PHI
+
     def gradients_limiters_host(self):
         """Gradients ((N+B) x 3nv) and limiters (N x nv) of the last residual."""
         n, nv = self.info.n_cells, self.nv

@@ -224,9 +224,22 @@ int  fcfd_cells_to_host(fcfd_ctx* ctx, const double* X_dev, int nvar, int includ
  * no host synchronisation. */
 int  fcfd_residual(fcfd_ctx* ctx, double* W_dev, double* R_dev, void* stream);
 
+/* Boundary ghosts, gradients and limiters of W_dev alone: replaces
+ * NavierStokes::update_boundaries (declared src/navier_stokes/ns.h:165),
+ * calc_cell_grads (ns.cc:301-336) and calc_limiter (ns_limiters.cc:26-248,
+ * with find_min_max ns.cc:400-435 for VK).  Writes the ghost slots of W_dev,
+ * the gradients (ghosts carry their parent's) into G_dev and the limiters of
+ * interior cells into PHI_dev, both in the row-pair layout below and sized
+ * 2*ceil(3nv/2)*ld_cell and 2*ceil(nv/2)*ld_cell doubles; a NULL G_dev or
+ * PHI_dev selects the context's own buffer.  PHI_dev is not written when no
+ * limiter runs (first order, or limiter NONE: phi is then the constant 0 or 1
+ * of ns.cc:282-287).  Stream-ordered; no host synchronisation. */
+int  fcfd_gradients_limiters(fcfd_ctx* ctx, double* W_dev, double* G_dev, double* PHI_dev, void* stream);
+
 /* Gradients (3nv components per cell: dp/dx,dp/dy,dp/dz, du/.., dv/.., dw/..,
  * dT/.. [, dnu~/..]; ghosts carry their parent's) and limiters (nv per cell)
- * of the last fcfd_residual call; device pointers owned by the context.
+ * of the last fcfd_residual call, or fcfd_gradients_limiters call into the
+ * context's own buffers; device pointers owned by the context.
  * Stored in row pairs: component k of cell i is at ((k/2)*ld_cell + i)*2 +
  * k%2 (fcfd_gradients_to_host returns them per cell in reference order). */
 int  fcfd_internal_buffers(fcfd_ctx* ctx, const double** grad, const double** phi);

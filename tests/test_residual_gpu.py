@@ -97,3 +97,59 @@ def test_plate_steger_warming():
     cs = cases.plate_case(12, 8, 3, noise=0.01, first=1e-3)
     cs.opt.convective_flux = A.FLUX_SW
     _compare(cs, label="SW")
+
+
+def _unpack_pairs(T, comps, ld, perm, rows):
+    """Row-pair device array -> [rows][comps] in reference numbering."""
+    a = T.cpu().numpy().reshape(-1, ld, 2)
+    return np.stack([a[c // 2, perm[:rows], c % 2] for c in range(comps)], axis=1)
+
+
+@pytest.mark.parametrize("name", ["wedge_vk", "first_order", "tetbox_lsqr", "plate_sa"])
+def test_gradients_limiters_alone(name):
+    """fcfd_gradients_limiters (ghosts, gradients and limiters of W, into the
+    caller's buffers) against the CPU restatement at the residual test's
+    tolerances, and bitwise against what the full residual leaves in the
+    context's buffers."""
+    from oracle.oracle import Oracle
+    from freecfd_amd.ns import NavierStokes
+    if name == "wedge_vk":
+        cs = cases.wedge_case(24, 12)
+        cs.opt.limiter = A.LIMITER_VK
+    elif name == "first_order":
+        cs = cases.wedge_case(24, 12)
+        cs.opt.order = A.ORDER_FIRST
+    elif name == "tetbox_lsqr":
+        cs = cases.tetbox_case(4)
+    else:
+        cs = cases.smooth_state(cases.plate_case(16, 10, 4, sa=True))
+    print("case", cs.name, name, flush=True)
+    o = Oracle(cs.raw, cs.opt)
+    o.residual(cs.W0)
+    Wo, Go, phio = o.fields()
+    o.close()
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    G, PHI = ns.gradients_limiters()
+    ns.status()
+    N, B, nv, ld = cs.n_cells, ns.info.n_ghosts, ns.nv, ns.info.ld_cell
+    perm = ns.metrics()["cell_perm"]
+    Gg = _unpack_pairs(G, 3 * nv, ld, perm, N + B)
+    Wg = ns.state_host(include_ghosts=True)
+    assert np.all(np.abs(Wg - Wo) <= 1e-14 * np.abs(Wo).max(axis=0) + 1e-300), "ghost state mismatch"
+    gerr = (np.abs(Gg - Go) / (np.abs(Go).max(axis=0) + 1e-300)).max()
+    limited = cs.opt.limiter != A.LIMITER_NONE and cs.opt.order == A.ORDER_SECOND
+    perr = 0.0
+    if limited:
+        phig = _unpack_pairs(PHI, nv, ld, perm, N)
+        perr = np.abs(phig - phio).max()
+    print("  gradients %.3e  limiters %.3e" % (gerr, perr), flush=True)
+    assert gerr <= 1e-11
+    assert perr <= 1e-9
+    # the residual computes the same fields into the context's buffers
+    ns.assemble_residual()
+    Gr, phir = ns.gradients_limiters_host()
+    assert np.array_equal(Gr, Gg)
+    if limited:
+        assert np.array_equal(phir, phig)
+    ns.close()
