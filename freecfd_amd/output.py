@@ -1,0 +1,148 @@
+"""Volume output in the reference's Tecplot FEPolyhedron layout, one rank
+(write_volume_output.cc:87-139 and write_tec_header :162-204,
+write_tec_nodes :206-228, write_tec_var :230-432, write_tec_face_node_counts
+:434-459, write_tec_face_nodes :461-490, write_tec_left :492-514,
+write_tec_right :516-542).
+
+./volume_output/volume_<step>_<g>.dat holds, in this order:
+- the header: variable names (vectors as name_x, name_y, name_z), zone,
+  counts, DATAPACKING=BLOCK, one DOUBLE per variable, the cell-centred
+  variable range;
+- x, y, z of every node (output id order: local node ids, first touch in
+  cell order, grid_create_elements.cc:35-59 and :774-812), each block ending
+  with a newline;
+- each cell variable (vector components one block each), with no newline
+  after the block;
+- "# node count per face", one count per face; "# face nodes", the face's
+  node output ids + 1 followed by a space each, one face per line;
+  "# left elements", parent + 1; "# right elements", neighbour + 1, or 0 on
+  a boundary face.
+Values are written as `file << scientific << setprecision(8)` writes them
+(printf's %.8e), ten to a line, TAB-separated.  A symbolic link
+./volume_latest_<g>.dat points at the file, as the reference's `ln -sf`.
+
+Variable names as in the reference's `volumevariables` list: p, T, rho, V,
+Mach, volume, dt, rank, gradp, gradu, gradv, gradw, gradT, limiterp, limiterV,
+limiterT, null; and nut for the build-only SA variable.
+"""
+import os
+
+import numpy as np
+
+_VEC = {"V", "gradp", "gradu", "gradv", "gradw", "gradT", "limiterV"}
+
+
+def local_nodes(raw):
+    """Raw ids of the local nodes in local (first touch in cell order) order."""
+    cp = np.asarray(raw["cell_ptr"])
+    cn = np.asarray(raw["cell_nodes"])[:cp[-1]]
+    _, first = np.unique(cn, return_index=True)
+    return cn[np.sort(first)]
+
+
+def _block(vals):
+    """The values, %.8e, a newline after every tenth and a TAB otherwise."""
+    out = []
+    for k, v in enumerate(vals):
+        out.append("%.8e" % v)
+        out.append("\n" if (k + 1) % 10 == 0 else "\t")
+    return "".join(out)
+
+
+def cell_fields(W, G=None, PHI=None, gamma=1.4, Mw=28.97, Pref=0.0, Tref=0.0, volume=None, dt=None):
+    """The named cell variables of the state W (n_cells x 5 or 6: p, u, v, w,
+    T[, nu~]), the gradients G (n x 3 nv) and limiters PHI (n x nv), in
+    reference cell order (material.cc:108-153 for rho and the sound speed)."""
+    n = W.shape[0]
+    R = 8314.47215 / Mw
+    p, V, T = W[:, 0], W[:, 1:4], W[:, 4]
+    rho = (p + Pref) / (R * (T + Tref))
+    a = np.sqrt(gamma * (p + Pref) / rho)
+    f = {"p": p, "T": T, "rho": rho, "V": V, "Mach": np.sqrt((V * V).sum(axis=1)) / a, "rank": np.zeros(n, dtype=int)}
+    if W.shape[1] == 6:
+        f["nut"] = W[:, 5]
+    if volume is not None:
+        f["volume"] = np.asarray(volume)[:n]
+    if dt is not None:
+        f["dt"] = np.asarray(dt)[:n]
+    if G is not None:
+        for k, name in enumerate(("gradp", "gradu", "gradv", "gradw", "gradT")):
+            f[name] = G[:n, 3 * k:3 * k + 3]
+    if PHI is not None:
+        f["limiterp"], f["limiterV"], f["limiterT"] = PHI[:n, 0], PHI[:n, 1:4], PHI[:n, 4]
+    return f
+
+
+def write_tecplot_volume_arrays(root, step, raw, metrics, fields, variables, gid=0):
+    """Write the file from the raw mesh, fcfd_metrics output (reference
+    numbering) and a dict of cell fields (cell_fields)."""
+    nodes = local_nodes(raw)
+    xyz = np.asarray(raw["xyz"], dtype=np.float64).reshape(-1, 3)[nodes]
+    fp, fn, fb = metrics["face_parent"], metrics["face_neighbor"], metrics["face_bc"]
+    nptr, fnodes = metrics["face_node_ptr"], metrics["face_nodes"]
+    F = len(fp)
+    n = len(raw["cell_ptr"]) - 1
+    names, nvars = [], 3
+    for v in variables:
+        if v == "null":
+            continue
+        if v not in fields:
+            raise ValueError("volume variable %r is not available" % v)
+        if v in _VEC:
+            names += [v + "_x", v + "_y", v + "_z"]
+            nvars += 3
+        else:
+            names.append(v)
+            nvars += 1
+    out = ['VARIABLES = "x", "y", "z" ', "".join(',"%s" ' % s for s in names), "\n",
+           'ZONE T="Grid_%d"\n' % (gid + 1),
+           "Nodes=%d, Faces=%d, Elements=%d, ZONETYPE=FEPolyhedron\n" % (len(nodes), F, n),
+           "DATAPACKING=BLOCK\n",
+           "TotalNumFaceNodes=%d, NumConnectedBoundaryFaces=0, TotalNumBoundaryConnections=0 \n" % nptr[F],
+           "DT=(" + "DOUBLE " * nvars + ")\n",
+           "VARLOCATION=([4]=CELLCENTERED)\n" if nvars == 4 else "VARLOCATION=([4-%d]=CELLCENTERED)\n" % nvars]
+    for i in range(3):
+        out.append(_block(xyz[:, i]) + "\n")
+    for v in variables:
+        if v == "null":
+            continue
+        a = np.asarray(fields[v])
+        if v in _VEC:
+            for i in range(3):
+                out.append(_block(a[:n, i]))
+        elif v == "rank":
+            out.append("".join("%d%s" % (x, "\n" if (k + 1) % 10 == 0 else "\t") for k, x in enumerate(a[:n])))
+        else:
+            out.append(_block(a[:n]))
+    out.append("\n# node count per face\n")
+    out.append("".join("%d\n" % (nptr[f + 1] - nptr[f]) for f in range(F)))
+    out.append("# face nodes\n")
+    out.append("".join("".join("%d " % (k + 1) for k in fnodes[nptr[f]:nptr[f + 1]]) + "\n" for f in range(F)))
+    out.append("# left elements\n")
+    out.append("".join("%d\n" % (c + 1) for c in fp))
+    out.append("# right elements\n")
+    out.append("".join("%d\n" % (0 if b >= 0 else c + 1) for c, b in zip(fn, fb)))
+    os.makedirs(os.path.join(root, "volume_output"), exist_ok=True)
+    name = "./volume_output/volume_%d_%d.dat" % (step, gid + 1)
+    path = os.path.join(root, name)
+    with open(path, "w") as f:
+        f.write("".join(out))
+    link = os.path.join(root, "volume_latest_%d.dat" % (gid + 1))
+    if os.path.lexists(link):
+        os.remove(link)
+    os.symlink(name, link)
+    return path
+
+
+def write_tecplot_volume(ns, raw, step, variables=("p", "V", "T"), root=".", gid=0, dt=None):
+    """write_volume_output(gid, step) with format tecplot for a NavierStokes
+    context: the current state and, when requested, the gradients and
+    limiters of the last residual evaluation."""
+    W = ns.state_host(include_ghosts=False)
+    G = PHI = None
+    if any(v.startswith("grad") or v.startswith("limiter") for v in variables):
+        G, PHI = ns.gradients_limiters_host()
+    m = ns.metrics()
+    o = ns._opt
+    f = cell_fields(W, G, PHI, gamma=o.gamma, Mw=o.Mw, Pref=o.Pref, Tref=o.Tref, volume=m["cell_volume"], dt=dt)
+    return write_tecplot_volume_arrays(root, step, raw, m, f, variables, gid)

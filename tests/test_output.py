@@ -1,0 +1,106 @@
+"""Tecplot FEPolyhedron volume output in the reference's layout
+(freecfd_amd/output.py; write_volume_output.cc:87-542): the file parsed back
+against the mesh metrics (node order pinned by the face centroids, which the
+reference computes as node averages, grid.cc:123-178), the header, the face
+connectivity and the cell variables; the number format against a compiled
+C++ ostream."""
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from freecfd_amd import cases
+from freecfd_amd import output as O
+
+
+def _parse(path, n_nodes, n_faces):
+    text = open(path).read()
+    head, tail = text.split("\n# node count per face\n")
+    lines = head.split("\n")
+    header, nums = lines[:7], np.array(" ".join(lines[7:]).split(), dtype=np.float64)
+    counts_txt, rest = tail.split("# face nodes\n")
+    nodes_txt, rest = rest.split("# left elements\n")
+    left_txt, right_txt = rest.split("# right elements\n")
+    counts = np.array(counts_txt.split(), dtype=int)
+    faces = [[int(t) for t in ln.split()] for ln in nodes_txt.rstrip("\n").split("\n")]
+    return (header, nums[:3 * n_nodes].reshape(3, n_nodes).T, nums[3 * n_nodes:], counts, faces,
+            np.array(left_txt.split(), dtype=int), np.array(right_txt.split(), dtype=int), text)
+
+
+@pytest.mark.parametrize("name", ["plate", "tetbox"])
+def test_volume_file_against_the_mesh(tmp_path, name):
+    from freecfd_amd.ns import Grid
+    cs = cases.plate_case(6, 4, 2, sa=True) if name == "plate" else cases.tetbox_case(2)
+    g = Grid(cs.raw, cs.opt)
+    m = g.metrics()
+    N, F = g.info.n_cells, g.info.n_faces
+    nv = cs.W0.shape[1]
+    rng = np.random.default_rng(1)
+    G, PHI = rng.standard_normal((N, 3 * nv)), rng.random((N, nv))
+    f = O.cell_fields(cs.W0, G, PHI, gamma=cs.opt.gamma, Mw=cs.opt.Mw, Pref=cs.opt.Pref, Tref=cs.opt.Tref,
+                      volume=m["cell_volume"])
+    variables = ["p", "V", "null", "T", "rho", "Mach", "volume", "gradu", "limiterT", "rank"] + (["nut"] if nv == 6 else [])
+    path = O.write_tecplot_volume_arrays(str(tmp_path), 3, cs.raw, m, f, variables)
+    n_nodes = len(O.local_nodes(cs.raw))
+    header, xyz, vals, counts, faces, left, right, text = _parse(path, n_nodes, F)
+    names = ["p", "V_x", "V_y", "V_z", "T", "rho", "Mach", "volume", "gradu_x", "gradu_y", "gradu_z", "limiterT", "rank"] + \
+        (["nut"] if nv == 6 else [])
+    nvars = 3 + len(names)
+    assert header[0] == 'VARIABLES = "x", "y", "z" ' + "".join(',"%s" ' % s for s in names)
+    assert header[1] == 'ZONE T="Grid_1"'
+    assert header[2] == "Nodes=%d, Faces=%d, Elements=%d, ZONETYPE=FEPolyhedron" % (n_nodes, F, N)
+    assert header[3] == "DATAPACKING=BLOCK"
+    assert header[4] == "TotalNumFaceNodes=%d, NumConnectedBoundaryFaces=0, TotalNumBoundaryConnections=0 " % counts.sum()
+    assert header[5] == "DT=(" + "DOUBLE " * nvars + ")"
+    assert header[6] == "VARLOCATION=([4-%d]=CELLCENTERED)" % nvars
+    # nodes in output-id order: every face centroid is the average of its nodes
+    assert counts.tolist() == np.diff(m["face_node_ptr"]).tolist()
+    fc = m["face_centroid"].reshape(-1, 3)
+    cen = np.array([xyz[np.array(fn) - 1].mean(axis=0) for fn in faces])
+    scale = np.abs(fc).max()
+    assert np.abs(cen - fc).max() <= 1e-7 * scale
+    assert left.tolist() == (m["face_parent"] + 1).tolist()
+    assert right.tolist() == np.where(m["face_bc"] >= 0, 0, m["face_neighbor"] + 1).tolist()
+    # the cell variables, block after block
+    want = [f["p"], f["V"][:, 0], f["V"][:, 1], f["V"][:, 2], f["T"], f["rho"], f["Mach"], f["volume"][:N],
+            G[:, 3], G[:, 4], G[:, 5], PHI[:, 4], np.zeros(N)] + ([cs.W0[:, 5]] if nv == 6 else [])
+    got = vals.reshape(len(want), N)
+    for w, x in zip(want, got):
+        assert np.allclose(x, w, rtol=1e-8, atol=0.0)
+    assert "\t" in text and text.count("\n") > 10
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_numbers_match_cpp_scientific_8(tmp_path):
+    vals = [101325.0, -3.5e-7, 0.0, 1.0 / 3.0, 6.02214076e23, 300.0, -0.0]
+    src = tmp_path / "fmt.cc"
+    src.write_text("#include <fstream>\n#include <iomanip>\nint main(int, char** argv) {\n  std::ofstream f(argv[1]);\n"
+                   "  f << std::scientific << std::setprecision(8);\n"
+                   + "".join("  f << %r << \"\\n\";\n" % v for v in vals) + "}\n")
+    exe = tmp_path / "fmt"
+    subprocess.run(["g++", "-O0", "-o", str(exe), str(src)], check=True)
+    out = tmp_path / "out.txt"
+    subprocess.run([str(exe), str(out)], check=True)
+    assert out.read_text() == "".join("%.8e\n" % v for v in vals)
+
+
+@pytest.mark.gpu
+def test_volume_output_of_a_context(tmp_path):
+    """Written from a live context: the p and gradient blocks parse back to the
+    state and the gradients of the last residual."""
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.cylinder_case(32, 16, noise=0.01)
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    ns.assemble_residual()
+    path = O.write_tecplot_volume(ns, cs.raw, 10, variables=("p", "gradp"), root=str(tmp_path))
+    N, F = ns.info.n_cells, ns.info.n_faces
+    G, _ = ns.gradients_limiters_host()
+    _, _, vals, *_ = _parse(path, len(O.local_nodes(cs.raw)), F)
+    got = vals.reshape(4, N)
+    assert np.allclose(got[0], cs.W0[:, 0], rtol=1e-8, atol=0)
+    for i in range(3):
+        assert np.allclose(got[1 + i], G[:N, i], rtol=1e-8, atol=1e-300)
+    assert (tmp_path / "volume_latest_1.dat").is_symlink()
+    ns.close()
