@@ -1,4 +1,7 @@
-"""Volume output in the reference's Tecplot FEPolyhedron layout, one rank
+"""Volume output in the reference's layouts, one rank: Tecplot FEPolyhedron
+(below) and VTK XML (write_vtk_volume_arrays).
+
+Tecplot FEPolyhedron
 (write_volume_output.cc:87-139 and write_tec_header :162-204,
 write_tec_nodes :206-228, write_tec_var :230-432, write_tec_face_node_counts
 :434-459, write_tec_face_nodes :461-490, write_tec_left :492-514,
@@ -146,3 +149,79 @@ def write_tecplot_volume(ns, raw, step, variables=("p", "V", "T"), root=".", gid
     o = ns._opt
     f = cell_fields(W, G, PHI, gamma=o.gamma, Mw=o.Mw, Pref=o.Pref, Tref=o.Tref, volume=m["cell_volume"], dt=dt)
     return write_tecplot_volume_arrays(root, step, raw, m, f, variables, gid)
+
+
+def _vtk_var_open(name, vec):
+    return '<DataArray Name="%s" %stype="Float32" format="ascii" >\n' % (name, 'Number of Components="3" ' if vec else "")
+
+
+def write_vtk_volume_arrays(root, step, raw, fields, variables, gid=0):
+    """Format `vtk` of write_volume_output (write_vtk_parallel
+    write_volume_output.cc:685-719 and write_vtk :555-683), one rank: the
+    .pvtu index and the .vtu piece.  Quirks kept as the reference writes
+    them: vectors carry the attribute `Number of Components="3"`, a `dt`
+    variable writes the cells' T, and "null" writes an empty array.  Points
+    are `setw(16) << setprecision(8) << scientific` (%16.8e), one coordinate
+    per line; cell values then keep the stream's scientific precision 8
+    (%.8e)."""
+    nodes = local_nodes(raw)
+    xyz = np.asarray(raw["xyz"], dtype=np.float64).reshape(-1, 3)[nodes]
+    g2l = np.full(int(np.asarray(raw["cell_nodes"]).max()) + 1, -1, dtype=np.int64)
+    g2l[nodes] = np.arange(len(nodes))
+    cp = np.asarray(raw["cell_ptr"])
+    cn = np.asarray(raw["cell_nodes"])
+    n = len(cp) - 1
+    d = os.path.join(root, "volume_output", "%d" % step)
+    os.makedirs(d, exist_ok=True)
+    pv = ['<?xml version="1.0"?>\n', '<VTKFile type="PUnstructuredGrid">\n', '<PUnstructuredGrid GhostLevel="0">\n', "<PPoints>\n",
+          '<DataArray NumberOfComponents="3" type="Float32" format="ascii" />\n', "</PPoints>\n", '<PCellData format="ascii">\n']
+    pv += [_vtk_var_open(v, v in _VEC) for v in variables]
+    pv += ["</PCellData>\n", '<Piece Source="grid_%d_proc_0.vtu" />\n' % (gid + 1), "</PUnstructuredGrid>\n", "</VTKFile>\n"]
+    with open(os.path.join(d, "grid_%d_volume_%d.pvtu" % (gid + 1, step)), "w") as f:
+        f.write("".join(pv))
+    out = ['<?xml version="1.0"?>\n', '<VTKFile type="UnstructuredGrid">\n', "<UnstructuredGrid>\n",
+           '<Piece NumberOfPoints="%d" NumberOfCells="%d">\n' % (len(nodes), n), "<Points>\n",
+           '<DataArray NumberOfComponents="3" type="Float32" format="ascii" >\n']
+    out.append("".join("%16.8e\n" % x for x in xyz.reshape(-1)))
+    out += ["</DataArray>\n", "</Points>\n", "<Cells>\n", '<DataArray Name="connectivity" type="Int32" format="ascii" >\n']
+    out.append("".join("".join("%d " % g2l[k] for k in cn[cp[c]:cp[c + 1]]) + "\n" for c in range(n)))
+    out += ["</DataArray>\n", '<DataArray Name="offsets" type="Int32" format="ascii" >\n']
+    out.append("".join("%d\n" % o for o in cp[1:]))
+    out += ["</DataArray>\n", '<DataArray Name="types" type="UInt8" format="ascii" >\n']
+    vtk_type = {4: "10", 8: "12", 6: "13", 5: "14"}
+    out.append("".join(vtk_type[int(k)] + "\n" for k in np.diff(cp)))
+    out += ["</DataArray>\n", "</Cells>\n", '<CellData format="ascii">\n']
+    for v in variables:
+        vec = v in _VEC
+        out.append(_vtk_var_open(v, vec))
+        if v == "null":
+            pass
+        elif v == "rank":
+            out.append("0\n" * n)
+        else:
+            src = "T" if v == "dt" else v
+            if src not in fields:
+                raise ValueError("volume variable %r is not available" % v)
+            a = np.asarray(fields[src])
+            if vec:
+                out.append("".join("%.8e %.8e %.8e \n" % tuple(a[c]) for c in range(n)))
+            else:
+                out.append("".join("%.8e\n" % x for x in a[:n]))
+        out.append("</DataArray>\n")
+    out += ["</CellData>\n", "</Piece>\n", "</UnstructuredGrid>\n", "</VTKFile>\n"]
+    path = os.path.join(d, "grid_%d_proc_0.vtu" % (gid + 1))
+    with open(path, "w") as f:
+        f.write("".join(out))
+    return path
+
+
+def write_vtk_volume(ns, raw, step, variables=("p", "V", "T"), root=".", gid=0):
+    """write_volume_output(gid, step) with format vtk for a NavierStokes context."""
+    W = ns.state_host(include_ghosts=False)
+    G = PHI = None
+    if any(v.startswith("grad") or v.startswith("limiter") for v in variables):
+        G, PHI = ns.gradients_limiters_host()
+    m = ns.metrics()
+    o = ns._opt
+    f = cell_fields(W, G, PHI, gamma=o.gamma, Mw=o.Mw, Pref=o.Pref, Tref=o.Tref, volume=m["cell_volume"])
+    return write_vtk_volume_arrays(root, step, raw, f, variables, gid)

@@ -104,3 +104,71 @@ def test_volume_output_of_a_context(tmp_path):
         assert np.allclose(got[1 + i], G[:N, i], rtol=1e-8, atol=1e-300)
     assert (tmp_path / "volume_latest_1.dat").is_symlink()
     ns.close()
+
+
+def _vtk_arrays(text):
+    """The DataArray bodies of a .vtu file in order: (opening tag, lines)."""
+    out, cur, tag = [], None, None
+    for ln in text.split("\n"):
+        if ln.startswith("<DataArray"):
+            tag, cur = ln, []
+        elif ln.startswith("</DataArray>"):
+            out.append((tag, cur))
+            cur = None
+        elif cur is not None:
+            cur.append(ln)
+    return out
+
+
+@pytest.mark.parametrize("name", ["plate", "tetbox"])
+def test_vtk_file_against_the_mesh(tmp_path, name):
+    from freecfd_amd.ns import Grid
+    cs = cases.plate_case(6, 4, 2) if name == "plate" else cases.tetbox_case(2)
+    g = Grid(cs.raw, cs.opt)
+    m = g.metrics()
+    N = g.info.n_cells
+    f = O.cell_fields(cs.W0, gamma=cs.opt.gamma, Mw=cs.opt.Mw, Pref=cs.opt.Pref, Tref=cs.opt.Tref)
+    variables = ["p", "V", "dt", "null", "rank"]
+    path = O.write_vtk_volume_arrays(str(tmp_path), 4, cs.raw, f, variables)
+    text = open(path).read()
+    nodes = O.local_nodes(cs.raw)
+    assert text.startswith('<?xml version="1.0"?>\n<VTKFile type="UnstructuredGrid">\n<UnstructuredGrid>\n'
+                           '<Piece NumberOfPoints="%d" NumberOfCells="%d">\n' % (len(nodes), N))
+    arrs = _vtk_arrays(text)
+    pts = np.array([float(x) for x in arrs[0][1]]).reshape(-1, 3)
+    assert np.allclose(pts, np.asarray(cs.raw["xyz"]).reshape(-1, 3)[nodes], rtol=1e-8, atol=1e-300)
+    conn = [[int(t) for t in ln.split()] for ln in arrs[1][1]]
+    cp, cn = np.asarray(cs.raw["cell_ptr"]), np.asarray(cs.raw["cell_nodes"])
+    assert [nodes[c].tolist() for c in conn] == [cn[cp[c]:cp[c + 1]].tolist() for c in range(N)]
+    assert [int(x) for x in arrs[2][1]] == cp[1:].tolist()
+    assert [int(x) for x in arrs[3][1]] == [{4: 10, 8: 12, 6: 13, 5: 14}[k] for k in np.diff(cp)]
+    tags = [t for t, _ in arrs[4:]]
+    assert tags == ['<DataArray Name="p" type="Float32" format="ascii" >',
+                    '<DataArray Name="V" Number of Components="3" type="Float32" format="ascii" >',
+                    '<DataArray Name="dt" type="Float32" format="ascii" >',
+                    '<DataArray Name="null" type="Float32" format="ascii" >',
+                    '<DataArray Name="rank" type="Float32" format="ascii" >']
+    p = np.array([float(x) for x in arrs[4][1]])
+    V = np.array([[float(x) for x in ln.split()] for ln in arrs[5][1]])
+    dt = np.array([float(x) for x in arrs[6][1]])
+    assert np.allclose(p, cs.W0[:, 0], rtol=1e-8) and np.allclose(V, cs.W0[:, 1:4], rtol=1e-8, atol=1e-300)
+    assert np.allclose(dt, cs.W0[:, 4], rtol=1e-8)          # the reference's dt array holds T
+    assert arrs[7][1] == [] and arrs[8][1] == ["0"] * N
+    pv = open(str(tmp_path / "volume_output" / "4" / "grid_1_volume_4.pvtu")).read()
+    assert '<Piece Source="grid_1_proc_0.vtu" />' in pv and pv.count("<DataArray") == 6
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_vtk_numbers_match_cpp_setw16(tmp_path):
+    """Points as `setw(16) << setprecision(8) << scientific << x`; later
+    values keep the stream's scientific precision 8."""
+    pts, vals = [0.5, -12.25, 1e-9], [101325.0, 0.1]
+    src = tmp_path / "fmt.cc"
+    src.write_text("#include <fstream>\n#include <iomanip>\nint main(int, char** argv) {\n  std::ofstream f(argv[1]);\n"
+                   + "".join("  f << std::setw(16) << std::setprecision(8) << std::scientific << %r << std::endl;\n" % v for v in pts)
+                   + "".join("  f << %r << std::endl;\n" % v for v in vals) + "}\n")
+    exe = tmp_path / "fmt"
+    subprocess.run(["g++", "-O0", "-o", str(exe), str(src)], check=True)
+    out = tmp_path / "out.txt"
+    subprocess.run([str(exe), str(out)], check=True)
+    assert out.read_text() == "".join("%16.8e\n" % v for v in pts) + "".join("%.8e\n" % v for v in vals)
