@@ -1,5 +1,6 @@
 """Volume output in the reference's layouts, one rank: Tecplot FEPolyhedron
-(below) and VTK XML (write_vtk_volume_arrays).
+(below), VTK XML (write_vtk_volume_arrays), legacy VTK
+(write_vtk_legacy_arrays); and the loads history files (write_loads_arrays).
 
 Tecplot FEPolyhedron
 (write_volume_output.cc:87-139 and write_tec_header :162-204,
@@ -225,3 +226,108 @@ def write_vtk_volume(ns, raw, step, variables=("p", "V", "T"), root=".", gid=0):
     o = ns._opt
     f = cell_fields(W, G, PHI, gamma=o.gamma, Mw=o.Mw, Pref=o.Pref, Tref=o.Tref, volume=m["cell_volume"])
     return write_vtk_volume_arrays(root, step, raw, f, variables, gid)
+
+
+# write_vtk_legacy's variable chain (write_volume_output.cc:742-786): the
+# names it writes values for; any other name gets its SCALARS header and no
+# values.  Of these, the ones without a source here are refused.
+_LEGACY_VALUES = {"p", "T", "rho", "dt", "rank", "V", "gradp", "gradu", "gradv", "gradw", "gradT", "limiterp",
+                  "limiterV", "limiterT", "Mach"}
+_LEGACY_ABSENT = {"mu", "lambda", "Cp", "resp", "resT", "resV", "k", "omega", "mu_t", "gradrho", "gradk", "gradomega"}
+
+
+def write_vtk_legacy_arrays(root, step, raw, fields, variables, gid=0):
+    """Format `vtklegacy` of write_volume_output, one rank: the .visit index
+    (write_visit_parallel, write_volume_output.cc:789-809, with its
+    ./volume_output/volume_latest_<g>.visit link) and the legacy ASCII piece
+    (write_vtk_legacy :701-787).  As the reference writes them: points one
+    coordinate a line as `setw(16) << setprecision(8) << scientific` (%16.8e),
+    later doubles in the stream's scientific precision 8 (%.8e); vector
+    variables (V, grad*, limiterV) as three SCALARS blocks name_x, name_y,
+    name_z; `dt` writes the cells' T; a name outside its variable chain
+    (null, volume, ...) writes its header with no values.  nu~ of the
+    build-only SA model is written under `nut`."""
+    nodes = local_nodes(raw)
+    xyz = np.asarray(raw["xyz"], dtype=np.float64).reshape(-1, 3)[nodes]
+    g2l = np.full(int(np.asarray(raw["cell_nodes"]).max()) + 1, -1, dtype=np.int64)
+    g2l[nodes] = np.arange(len(nodes))
+    cp = np.asarray(raw["cell_ptr"])
+    cn = np.asarray(raw["cell_nodes"])
+    n = len(cp) - 1
+    g = gid + 1
+    vo = os.path.join(root, "volume_output")
+    d = os.path.join(vo, "%d" % step)
+    os.makedirs(d, exist_ok=True)
+    visit = "volume_%d_%d.visit" % (step, g)
+    with open(os.path.join(vo, visit), "w") as f:
+        f.write("!NBLOCKS 1\n./%d/grid_%d_proc_0.vtk\n" % (step, g))
+    link = os.path.join(vo, "volume_latest_%d.visit" % g)
+    if os.path.lexists(link):
+        os.remove(link)
+    os.symlink("./" + visit, link)
+    out = ["# vtk DataFile Version 2.0\n", "Grid_%d\n" % g, "ASCII\n", "DATASET UNSTRUCTURED_GRID\n",
+           "POINTS %d double\n" % len(nodes), "".join("%16.8e\n" % x for x in xyz.reshape(-1)),
+           "CELLS %d %d\n" % (n, cp[n] + n)]
+    out.append("".join("%d" % (cp[c + 1] - cp[c]) + "".join(" %d" % g2l[k] for k in cn[cp[c]:cp[c + 1]]) + "\n"
+                       for c in range(n)))
+    out.append("CELL_TYPES %d\n" % n)
+    vtk_type = {4: "10\n", 8: "12\n", 6: "13\n", 5: "14\n"}
+    out.append("".join(vtk_type.get(int(k), "") for k in np.diff(cp)))
+    out.append("CELL_DATA %d\n" % n)
+    for v in variables:
+        if v in _LEGACY_ABSENT:
+            raise ValueError("volume variable %r is not available" % v)
+        vec = v == "V" or v.startswith("grad") or v == "limiterV"
+        src = "T" if v == "dt" else v
+        has = v in _LEGACY_VALUES or v == "nut"
+        if has and src not in fields:
+            raise ValueError("volume variable %r is not available" % v)
+        for i in range(3 if vec else 1):
+            out.append("SCALARS %s double\nLOOKUP_TABLE default\n" % (v + ("_x", "_y", "_z")[i] if vec else v))
+            if not has:
+                continue
+            a = np.asarray(fields[src])
+            if v == "rank":
+                out.append("0\n" * n)
+            else:
+                out.append("".join("%.8e\n" % x for x in (a[:n, i] if vec else a[:n])))
+    path = os.path.join(d, "grid_%d_proc_0.vtk" % g)
+    with open(path, "w") as f:
+        f.write("".join(out))
+    return path
+
+
+def write_vtk_legacy(ns, raw, step, variables=("p", "V", "T"), root=".", gid=0):
+    """write_volume_output(gid, step) with format vtklegacy for a NavierStokes context."""
+    W = ns.state_host(include_ghosts=False)
+    G = PHI = None
+    if any(v.startswith("grad") or v.startswith("limiter") for v in variables):
+        G, PHI = ns.gradients_limiters_host()
+    o = ns._opt
+    f = cell_fields(W, G, PHI, gamma=o.gamma, Mw=o.Mw, Pref=o.Pref, Tref=o.Tref)
+    return write_vtk_legacy_arrays(root, step, raw, f, variables, gid)
+
+
+def write_loads_arrays(root, step, time, loads, include_bcs, gid=0):
+    """write_loads (write_volume_output.cc:65-85): for each BC index b in
+    include_bcs, append `step<TAB>time<TAB>Fx<TAB>Fy<TAB>Fz<TAB>Mx<TAB>My<TAB>Mz`
+    to ./volume_output/loads_grid_<g>_BC_<b+1>.dat, the doubles as
+    `scientific << setprecision(8)` writes them (%.8e).  loads is (n_bc, 6):
+    forces then moments per BC."""
+    L = np.asarray(loads, dtype=np.float64)
+    vo = os.path.join(root, "volume_output")
+    os.makedirs(vo, exist_ok=True)
+    paths = []
+    for b in include_bcs:
+        path = os.path.join(vo, "loads_grid_%d_BC_%d.dat" % (gid + 1, b + 1))
+        with open(path, "a") as f:
+            f.write("%d\t%.8e\t" % (step, time) + "\t".join("%.8e" % x for x in L[b]) + "\n")
+        paths.append(path)
+    return paths
+
+
+def write_loads(ns, step, time, root=".", gid=0):
+    """write_loads(gid, step, time) for a NavierStokes context: the BCs with
+    include_in_loads, the loads of its current state (fcfd_loads)."""
+    inc = [b for b in range(int(ns._opt.n_bc)) if ns._opt.bc[b].include_in_loads]
+    return write_loads_arrays(root, step, time, ns.loads(), inc, gid)

@@ -4,6 +4,7 @@ against the mesh metrics (node order pinned by the face centroids, which the
 reference computes as node averages, grid.cc:123-178), the header, the face
 connectivity and the cell variables; the number format against a compiled
 C++ ostream."""
+import os
 import shutil
 import subprocess
 
@@ -172,3 +173,112 @@ def test_vtk_numbers_match_cpp_setw16(tmp_path):
     out = tmp_path / "out.txt"
     subprocess.run([str(exe), str(out)], check=True)
     assert out.read_text() == "".join("%16.8e\n" % v for v in pts) + "".join("%.8e\n" % v for v in vals)
+
+
+def _legacy_blocks(lines, start):
+    """SCALARS blocks from `start`: [(name, values as strings)]."""
+    out, i = [], start
+    while i < len(lines):
+        assert lines[i].startswith("SCALARS ") and lines[i].endswith(" double") and lines[i + 1] == "LOOKUP_TABLE default"
+        j = i + 2
+        while j < len(lines) and not lines[j].startswith("SCALARS "):
+            j += 1
+        out.append((lines[i].split()[1], lines[i + 2:j]))
+        i = j
+    return out
+
+
+@pytest.mark.parametrize("name", ["plate", "tetbox"])
+def test_vtk_legacy_file_against_the_mesh(tmp_path, name):
+    """Format vtklegacy (write_volume_output.cc:701-809) parsed back: points
+    in local order, cells in local node ids with their counts, cell types,
+    the SCALARS blocks (vectors by component, dt holding T, null and volume
+    with no values), and the .visit index with its latest link."""
+    cs = cases.plate_case(6, 4, 2, sa=True) if name == "plate" else cases.tetbox_case(2)
+    N = len(cs.raw["cell_ptr"]) - 1
+    f = O.cell_fields(cs.W0, gamma=cs.opt.gamma, Mw=cs.opt.Mw, Pref=cs.opt.Pref, Tref=cs.opt.Tref)
+    variables = ["p", "V", "dt", "null", "Mach", "volume", "rank"] + (["nut"] if cs.W0.shape[1] == 6 else [])
+    path = O.write_vtk_legacy_arrays(str(tmp_path), 8, cs.raw, f, variables)
+    assert path == str(tmp_path / "volume_output" / "8" / "grid_1_proc_0.vtk")
+    lines = open(path).read().split("\n")
+    assert lines[-1] == ""
+    lines = lines[:-1]
+    nodes = O.local_nodes(cs.raw)
+    P = len(nodes)
+    assert lines[:5] == ["# vtk DataFile Version 2.0", "Grid_1", "ASCII", "DATASET UNSTRUCTURED_GRID", "POINTS %d double" % P]
+    assert all(len(x) == 16 for x in lines[5:5 + 3 * P])
+    pts = np.array([float(x) for x in lines[5:5 + 3 * P]]).reshape(-1, 3)
+    assert np.allclose(pts, np.asarray(cs.raw["xyz"]).reshape(-1, 3)[nodes], rtol=1e-8, atol=1e-300)
+    cp, cn = np.asarray(cs.raw["cell_ptr"]), np.asarray(cs.raw["cell_nodes"])
+    k = 5 + 3 * P
+    assert lines[k] == "CELLS %d %d" % (N, cp[N] + N)
+    cells = [[int(t) for t in ln.split(" ")] for ln in lines[k + 1:k + 1 + N]]
+    assert [c[0] for c in cells] == np.diff(cp).tolist()
+    assert [nodes[c[1:]].tolist() for c in cells] == [cn[cp[c]:cp[c + 1]].tolist() for c in range(N)]
+    k += 1 + N
+    assert lines[k] == "CELL_TYPES %d" % N
+    assert [int(x) for x in lines[k + 1:k + 1 + N]] == [{4: 10, 8: 12, 6: 13, 5: 14}[n] for n in np.diff(cp)]
+    k += 1 + N
+    assert lines[k] == "CELL_DATA %d" % N
+    blocks = _legacy_blocks(lines, k + 1)
+    names = ["p", "V_x", "V_y", "V_z", "dt", "null", "Mach", "volume", "rank"] + (["nut"] if cs.W0.shape[1] == 6 else [])
+    assert [b[0] for b in blocks] == names
+    vals = dict(blocks)
+    want = {"p": cs.W0[:, 0], "V_x": cs.W0[:, 1], "V_y": cs.W0[:, 2], "V_z": cs.W0[:, 3], "dt": cs.W0[:, 4],
+            "Mach": f["Mach"]}
+    if "nut" in vals:
+        want["nut"] = cs.W0[:, 5]
+    for key, w in want.items():
+        assert all(x == "%.8e" % y for x, y in zip(vals[key], w)) and len(vals[key]) == N, key
+    assert vals["null"] == [] and vals["volume"] == [] and vals["rank"] == ["0"] * N
+    vo = tmp_path / "volume_output"
+    assert (vo / "volume_8_1.visit").read_text() == "!NBLOCKS 1\n./8/grid_1_proc_0.vtk\n"
+    assert os.readlink(str(vo / "volume_latest_1.visit")) == "./volume_8_1.visit"
+    with pytest.raises(ValueError):
+        O.write_vtk_legacy_arrays(str(tmp_path), 9, cs.raw, f, ["mu_t"])
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_loads_lines_match_a_cpp_ostream(tmp_path):
+    """write_loads appends `step \\t time \\t Fx .. Mz` per included BC in
+    `scientific << setprecision(8)`: the same statements compiled and run."""
+    L = np.array([[1.5, -2.25e-3, 0.0, 7.0, 1e-12, -3e5], [0.1, 0.2, 0.3, 0.4, 0.5, 0.6], [9.0, 8.0, 7.0, 6.0, 5.0, 4.0]])
+    rows = [(12, 0.125, 0), (12, 0.125, 2), (13, 2.5, 0)]
+    src = tmp_path / "fmt.cc"
+    body = ""
+    for step, time, b in rows:
+        body += ('  { std::ofstream f(std::string(argv[1]) + "/loads_grid_1_BC_%d.dat", std::ios::app);\n'
+                 '    f << std::scientific << std::setprecision(8);\n    f << %d << "\\t" << %r << "\\t";\n'
+                 '    f << %s << std::endl; }\n') % (b + 1, step, time, ' << "\\t" << '.join(repr(float(x)) for x in L[b]))
+    src.write_text("#include <fstream>\n#include <iomanip>\n#include <string>\nint main(int, char** argv) {\n" + body + "}\n")
+    exe = tmp_path / "fmt"
+    subprocess.run(["g++", "-O0", "-o", str(exe), str(src)], check=True)
+    cdir = tmp_path / "cpp"
+    cdir.mkdir()
+    subprocess.run([str(exe), str(cdir)], check=True)
+    O.write_loads_arrays(str(tmp_path), 12, 0.125, L, [0, 2])
+    O.write_loads_arrays(str(tmp_path), 13, 2.5, L, [0])
+    for b in (1, 3):
+        name = "loads_grid_1_BC_%d.dat" % b
+        assert (tmp_path / "volume_output" / name).read_text() == (cdir / name).read_text()
+    assert not (tmp_path / "volume_output" / "loads_grid_1_BC_2.dat").exists()
+
+
+@pytest.mark.gpu
+def test_loads_file_of_a_context(tmp_path):
+    """The loads file written from a live context holds fcfd_loads of the
+    wall (the only BC with include_in_loads), one line per call."""
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.cylinder_case(32, 16, noise=0.01)
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    F = ns.loads()
+    paths = O.write_loads(ns, 4, 0.5, root=str(tmp_path))
+    O.write_loads(ns, 5, 0.75, root=str(tmp_path))
+    assert paths == [str(tmp_path / "volume_output" / "loads_grid_1_BC_1.dat")]
+    rows = [ln.split("\t") for ln in open(paths[0]).read().rstrip("\n").split("\n")]
+    assert [r[:2] for r in rows] == [["4", "%.8e" % 0.5], ["5", "%.8e" % 0.75]]
+    for r in rows:
+        assert np.allclose([float(x) for x in r[2:]], F[0], rtol=1e-8, atol=1e-300)
+    assert abs(F[0, 0]) > 0
+    ns.close()
