@@ -1,6 +1,7 @@
 """Volume output in the reference's layouts, one rank: Tecplot FEPolyhedron
 (below), VTK XML (write_vtk_volume_arrays), legacy VTK
-(write_vtk_legacy_arrays); and the loads history files (write_loads_arrays).
+(write_vtk_legacy_arrays); the loads history files (write_loads_arrays); and
+Tecplot surface output (write_tecplot_surface_arrays).
 
 Tecplot FEPolyhedron
 (write_volume_output.cc:87-139 and write_tec_header :162-204,
@@ -331,3 +332,120 @@ def write_loads(ns, step, time, root=".", gid=0):
     include_in_loads, the loads of its current state (fcfd_loads)."""
     inc = [b for b in range(int(ns._opt.n_bc)) if ns._opt.bc[b].include_in_loads]
     return write_loads_arrays(root, step, time, ns.loads(), inc, gid)
+
+
+def face_values(metrics, Wg, bcs, gamma=1.4, Mw=28.97, Pref=0.0, Tref=0.0):
+    """p, T, rho, V and Mach on every face, as Variable::face gives them
+    (variable/variable.h:135-148): on a face of a BC where the variable is
+    fixed (fixedonBC), the BC's value; otherwise the face-average stencil
+    over cells and ghosts (fcfd_metrics avg_ptr/avg_idx/avg_w) applied to the
+    cell values, summed in stencil order.  Wg is the state with ghosts
+    ((N+B) x nv, reference order); bcs the options' BC records; Mach is
+    |V_f| / a(p_f, T_f) (write_surface_output.cc:327-332, material.cc)."""
+    ap, ai, aw = metrics["avg_ptr"], metrics["avg_idx"], metrics["avg_w"]
+    fb = metrics["face_bc"]
+    F = len(fb)
+    R = 8314.47215 / Mw
+    Wg = np.asarray(Wg, dtype=np.float64)
+    rho_c = (Wg[:, 0] + Pref) / (R * (Wg[:, 4] + Tref))
+    X = np.concatenate([Wg[:, :5], rho_c[:, None]], axis=1)       # p u v w T rho
+    out = np.zeros((F, 6))
+    cnt = np.diff(ap)
+    for k in range(int(cnt.max()) if F else 0):
+        sel = np.nonzero(cnt > k)[0]
+        out[sel] += aw[ap[sel] + k][:, None] * X[ai[ap[sel] + k]]
+    for b, bc in enumerate(bcs):
+        on = fb == b
+        if bc.p_fixed:
+            out[on, 0] = bc.p
+        if bc.V_fixed:
+            out[on, 1:4] = np.asarray(bc.V[:3])
+        if bc.T_fixed:
+            out[on, 4] = bc.T
+        if bc.rho_fixed:
+            out[on, 5] = bc.rho
+    p, V, T = out[:, 0], out[:, 1:4], out[:, 4]
+    a = np.sqrt(gamma * (p + Pref) / ((p + Pref) / (R * (T + Tref))))
+    return {"p": p, "V": V, "T": T, "rho": out[:, 5], "Mach": np.sqrt((V * V).sum(axis=1)) / a,
+            "rank": np.zeros(F, dtype=int)}
+
+
+def write_tecplot_surface_arrays(root, step, raw, metrics, fvals, variables, gid=0):
+    """write_surface_output with format tecplot (write_surface_output.cc:57-433),
+    one rank: ./surface_output/surface_<step>_<g>.dat and the
+    ./surface_latest_<g>.dat link.  One FEQUADRILATERAL zone per BC region;
+    the first zone carries the boundary nodes (local node ids that lie on a
+    boundary face, in increasing order: get_bc_output_ids,
+    grid_create_elements.cc:883-931), later zones share them
+    (VARSHARELIST).  Per zone, the face values of the BC's faces in reference
+    face order, one block per variable (vector components V, tau: one block
+    each), ten to a line and with no newline after a block; then one line
+    per face of its output node ids + 1, TAB after each, a triangle's third
+    node written twice.  fvals: face_values (p, T, rho, V, Mach, rank)."""
+    fb = metrics["face_bc"]
+    nptr, fnodes = metrics["face_node_ptr"], metrics["face_nodes"]
+    n_bc = int(np.asarray(raw["bc_ptr"]).size - 1)
+    xyz = np.asarray(raw["xyz"], dtype=np.float64).reshape(-1, 3)[local_nodes(raw)]
+    bface = np.nonzero(fb >= 0)[0]
+    bnodes = np.unique(np.concatenate([fnodes[nptr[f]:nptr[f + 1]] for f in bface])) if len(bface) else np.zeros(0, int)
+    out_id = np.full(len(xyz), -2, dtype=np.int64)
+    out_id[bnodes] = np.arange(len(bnodes))
+    vec = {"V", "tau"}
+    nvars = 3
+    for v in variables:
+        if v != "null" and v not in fvals:
+            raise ValueError("surface variable %r is not available" % v)
+        nvars += 3 if v in vec else (0 if v == "null" else 1)
+    out = []
+    for b in range(n_bc):
+        faces = np.nonzero(fb == b)[0]
+        if b == 0:
+            out.append('VARIABLES = "x", "y", "z" ' + "".join(
+                ',"%s_x" ,"%s_y" ,"%s_z" ' % (v, v, v) if v in vec else ('' if v == "null" else ',"%s" ' % v)
+                for v in variables) + "\n")
+        out.append('ZONE, T="BC_%d", ZONETYPE=FEQUADRILATERAL, DATAPACKING=BLOCK\n' % (b + 1))
+        if b == 0:
+            out.append("NODES=%d, " % len(bnodes))
+        out.append("ELEMENTS=%d\n" % len(faces))
+        out.append("VARLOCATION=([4]=CELLCENTERED)\n" if nvars == 4 else "VARLOCATION=([4-%d]=CELLCENTERED)\n" % nvars)
+        if b != 0:
+            out.append("VARSHARELIST=([1-3]=1)\n")
+        if b == 0:
+            for i in range(3):
+                out.append(_block(xyz[bnodes, i]) + "\n")
+        for v in variables:
+            if v == "null":
+                continue
+            a = np.asarray(fvals[v])
+            if v == "rank":
+                out.append("".join("%d%s" % (x, "\n" if (k + 1) % 10 == 0 else "\t") for k, x in enumerate(a[faces])))
+            elif v in vec:
+                for i in range(3):
+                    out.append(_block(a[faces, i]))
+            else:
+                out.append(_block(a[faces]))
+        for f in faces:
+            ids = out_id[fnodes[nptr[f]:nptr[f + 1]]] + 1
+            row = list(ids[:3]) + ([ids[3]] if len(ids) == 4 else [ids[2]])
+            out.append("".join("%d\t" % x for x in row) + "\n")
+    os.makedirs(os.path.join(root, "surface_output"), exist_ok=True)
+    name = "./surface_output/surface_%d_%d.dat" % (step, gid + 1)
+    path = os.path.join(root, name)
+    with open(path, "w") as f:
+        f.write("".join(out))
+    link = os.path.join(root, "surface_latest_%d.dat" % (gid + 1))
+    if os.path.lexists(link):
+        os.remove(link)
+    os.symlink(name, link)
+    return path
+
+
+def write_tecplot_surface(ns, raw, step, variables=("p", "V", "T"), root=".", gid=0):
+    """write_surface_output(gid, step) with format tecplot for a NavierStokes
+    context: face values from its current state and the ghosts of its last
+    residual evaluation (or gradients_limiters call)."""
+    m = ns.metrics()
+    o = ns._opt
+    fv = face_values(m, ns.state_host(include_ghosts=True), [o.bc[b] for b in range(int(o.n_bc))],
+                     gamma=o.gamma, Mw=o.Mw, Pref=o.Pref, Tref=o.Tref)
+    return write_tecplot_surface_arrays(root, step, raw, m, fv, variables, gid)

@@ -282,3 +282,130 @@ def test_loads_file_of_a_context(tmp_path):
         assert np.allclose([float(x) for x in r[2:]], F[0], rtol=1e-8, atol=1e-300)
     assert abs(F[0, 0]) > 0
     ns.close()
+
+
+def _parse_surface(text, n_bc, nvals):
+    """Zones of a surface file: [(header lines, coordinates or None, values
+    (nvals x faces), connectivity rows)]."""
+    chunks = text.split("ZONE, ")
+    assert chunks[0].startswith('VARIABLES = "x", "y", "z" ')
+    zones = []
+    for b, ch in enumerate(chunks[1:]):
+        lines = ch.split("\n")
+        nh = 3 if b == 0 else 4
+        head = lines[:nh]
+        toks = "\n".join(lines[nh:]).split()
+        nf = int(head[1].split("ELEMENTS=")[1])
+        k, xyz = 0, None
+        if b == 0:
+            nn = int(head[1].split("NODES=")[1].split(",")[0])
+            xyz = np.array(toks[:3 * nn], dtype=float).reshape(3, nn).T
+            k = 3 * nn
+        vals = toks[k:k + nvals * nf]
+        conn = np.array(toks[k + nvals * nf:], dtype=int).reshape(nf, 4) if nf else np.zeros((0, 4), int)
+        zones.append((head, xyz, vals, conn))
+    assert len(zones) == n_bc
+    return zones
+
+
+def test_surface_file_against_the_mesh(tmp_path):
+    """Tecplot surface output (write_surface_output.cc:57-433) parsed back:
+    zone headers, the boundary nodes (sorted local ids; every face centroid is
+    the average of its distinct nodes), per-zone face counts in reference
+    face order, the values of each block, and a triangle's repeated node."""
+    from freecfd_amd.ns import Grid
+    cs = cases.cylinder_case(16, 8)
+    g = Grid(cs.raw, cs.opt)
+    m = g.metrics()
+    N, B = g.info.n_cells, g.info.n_ghosts
+    rng = np.random.default_rng(4)
+    Wg = np.concatenate([cs.W0, np.tile(cs.W0[0], (B, 1))], axis=0) * (1 + 0.05 * rng.random((N + B, 5)))
+    bcs = [cs.opt.bc[b] for b in range(cs.opt.n_bc)]
+    fv = O.face_values(m, Wg, bcs, gamma=cs.opt.gamma, Mw=cs.opt.Mw)
+    variables = ["p", "V", "null", "T", "Mach", "rank"]
+    path = O.write_tecplot_surface_arrays(str(tmp_path), 6, cs.raw, m, fv, variables)
+    assert os.readlink(str(tmp_path / "surface_latest_1.dat")) == "./surface_output/surface_6_1.dat"
+    text = open(path).read()
+    assert text.split("\n")[0] == 'VARIABLES = "x", "y", "z" ,"p" ,"V_x" ,"V_y" ,"V_z" ,"T" ,"Mach" ,"rank" '
+    zones = _parse_surface(text, cs.opt.n_bc, 7)
+    fb, nptr, fnodes = m["face_bc"], m["face_node_ptr"], m["face_nodes"]
+    bnodes = np.unique(np.concatenate([fnodes[nptr[f]:nptr[f + 1]] for f in np.nonzero(fb >= 0)[0]]))
+    xyz_local = np.asarray(cs.raw["xyz"]).reshape(-1, 3)[O.local_nodes(cs.raw)]
+    head0, xyz, _, _ = zones[0]
+    assert head0 == ['T="BC_1", ZONETYPE=FEQUADRILATERAL, DATAPACKING=BLOCK',
+                     "NODES=%d, ELEMENTS=%d" % (len(bnodes), (fb == 0).sum()), "VARLOCATION=([4-10]=CELLCENTERED)"]
+    assert np.allclose(xyz, xyz_local[bnodes], rtol=1e-8, atol=1e-300)
+    fc = m["face_centroid"].reshape(-1, 3)
+    for b, (head, _, vals, conn) in enumerate(zones):
+        faces = np.nonzero(fb == b)[0]
+        if b:
+            assert head == ['T="BC_%d", ZONETYPE=FEQUADRILATERAL, DATAPACKING=BLOCK' % (b + 1),
+                            "ELEMENTS=%d" % len(faces), "VARLOCATION=([4-10]=CELLCENTERED)", "VARSHARELIST=([1-3]=1)"]
+        want = [fv["p"][faces], fv["V"][faces, 0], fv["V"][faces, 1], fv["V"][faces, 2], fv["T"][faces], fv["Mach"][faces]]
+        nf = len(faces)
+        for j, w in enumerate(want):
+            assert vals[j * nf:(j + 1) * nf] == ["%.8e" % x for x in w]
+        assert vals[6 * nf:] == ["0"] * nf
+        for f, row in zip(faces, conn):
+            k = nptr[f + 1] - nptr[f]
+            if k == 3:
+                assert row[3] == row[2]
+            cen = xyz[row[:k] - 1].mean(axis=0)
+            assert np.abs(cen - fc[f]).max() <= 1e-7 * np.abs(fc).max()      # 9 printed digits
+    g.close()
+
+
+def test_face_values_follow_fixed_bcs_and_stencils():
+    """Variable::face on a uniform state is that state off the fixed BCs and
+    the BC value on them; on any state a boundary face's unfixed value lies
+    between its parent's and its ghost's (a convex stencil)."""
+    from freecfd_amd.ns import Grid
+    cs = cases.cylinder_case(16, 8)
+    g = Grid(cs.raw, cs.opt)
+    m = g.metrics()
+    N, B = g.info.n_cells, g.info.n_ghosts
+    bcs = [cs.opt.bc[b] for b in range(cs.opt.n_bc)]
+    W = np.tile(np.array([9.0e4, 3.0, -2.0, 0.5, 280.0]), (N + B, 1))
+    fv = O.face_values(m, W, bcs)
+    fb = m["face_bc"]
+    assert np.allclose(fv["p"][fb != 2], 9.0e4, rtol=1e-14) and (fv["p"][fb == 2] == 101325.0).all()
+    assert np.allclose(fv["T"][fb != 1], 280.0, rtol=1e-14) and (fv["T"][fb == 1] == 300.0).all()
+    assert (fv["V"][fb == 0] == 0.0).all() and (fv["V"][fb == 1] == np.array(list(bcs[1].V))).all()
+    assert np.allclose(fv["V"][fb >= 3], [3.0, -2.0, 0.5], rtol=1e-14)
+    # rho is not fixed on the inlet: its face value averages the cells rho, not rho(p_f, T_f)
+    assert np.allclose(fv["rho"], 9.0e4 / (8314.47215 / 28.97 * 280.0), rtol=1e-14)
+    rng = np.random.default_rng(2)
+    W = W * (1 + 0.1 * rng.random(W.shape))
+    fv = O.face_values(m, W, bcs)
+    par = m["face_parent"]
+    bf = np.nonzero(fb == 3)[0]
+    ghost = N + np.array([m["avg_idx"][m["avg_ptr"][f]:m["avg_ptr"][f + 1]].max() - N for f in bf])
+    lo, hi = np.minimum(W[par[bf], 0], W[ghost, 0]), np.maximum(W[par[bf], 0], W[ghost, 0])
+    assert ((fv["p"][bf] >= lo * (1 - 1e-15)) & (fv["p"][bf] <= hi * (1 + 1e-15))).all()
+    g.close()
+
+
+@pytest.mark.gpu
+def test_surface_output_of_a_context(tmp_path):
+    """Written from a live context after a residual: the fixed inlet T and
+    outlet p, and the no-slip wall's zero velocity, in their zones."""
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.cylinder_case(32, 16, noise=0.01)
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    ns.assemble_residual()
+    path = O.write_tecplot_surface(ns, cs.raw, 3, variables=("p", "V", "T"), root=str(tmp_path))
+    zones = _parse_surface(open(path).read(), cs.opt.n_bc, 5)
+    m = ns.metrics()
+    fb = m["face_bc"]
+    for b, (_, _, vals, _) in enumerate(zones):
+        nf = int((fb == b).sum())
+        v = np.array(vals, dtype=float).reshape(5, nf)
+        if b == 0:
+            assert (v[1:4] == 0).all()
+        if b == 1:
+            assert (v[4] == 300.0).all()
+        if b == 2:
+            assert (v[0] == 101325.0).all()
+        assert (v[0] > 0).all() and (v[4] > 0).all()
+    ns.close()
