@@ -133,8 +133,6 @@ int validate_options(fcfd_ctx* c) {
             bc.specified == FCFD_SPEC_NONE)   // ns_apply_bcs.cc:78-81,123-126
             return fail(c, FCFD_EBC, "At least one of p,T,rho needs to be specified at Inlet at BC_" + std::to_string(b + 1));
     }
-    if (o.grad_hex == FCFD_GRAD_GREENGAUSS || o.grad_prism == FCFD_GRAD_GREENGAUSS || o.grad_other == FCFD_GRAD_GREENGAUSS)
-        return fail(c, FCFD_ENOTSUP, "Green-Gauss gradients are not supported by this build");
     if (o.turbulence != FCFD_TURB_NONE && o.turbulence != FCFD_TURB_SA) return fail(c, FCFD_EINVAL, "unknown turbulence model");
     if (o.turbulence == FCFD_TURB_SA) {
         if (!(o.visc_model == FCFD_VISC_SUTHERLAND || o.mu > 0.0))
@@ -175,6 +173,7 @@ void set_params(fcfd_ctx* c) {
         d.p_total = s.p_total; d.T_total = s.T_total; d.mdot = s.mdot; d.qdot = s.qdot;
         d.total_area = b < (int)c->hg.bc_area.size() ? c->hg.bc_area[b] : 0.0;
         d.nut = s.nut;
+        d.p_fixed = s.p_fixed; d.T_fixed = s.T_fixed; d.V_fixed = s.V_fixed;
     }
 }
 
@@ -267,6 +266,29 @@ int build_device(fcfd_ctx* c) {
         }
         int rc;
         if ((rc = upload(c, gs, &d.gslice)) || (rc = upload(c, ga, &d.gwa)) || (rc = upload(c, gb, &d.gwb))) return rc;
+    }
+    // Green-Gauss gradients (gradient_maps.cc:59-72 leaves a Green-Gauss
+    // cell's map empty, and Variable::cell_gradient, variable.h:193-216, then
+    // sums face values): the cells without a map, and the face-average
+    // weights of every face
+    {
+        std::vector<int> ggc(N, 0);
+        bool any = false;
+        for (int i = 0; i < N; ++i) {
+            const int r = g.iperm[i];
+            if (g.gptr[r + 1] == g.gptr[r]) { ggc[i] = 1; any = true; }
+        }
+        if (any) {
+            std::vector<double2> ggw(F);
+            for (int i = 0; i < F; ++i) {
+                const int f = g.ifperm[i];
+                if (g.aptr[f + 1] - g.aptr[f] != 2 || g.aidx[g.aptr[f]] != g.fparent[f] || g.aidx[g.aptr[f] + 1] != g.fneigh[f])
+                    return fail(c, FCFD_ENOTSUP, "Green-Gauss gradients on the device need two-cell face-average stencils");
+                ggw[i] = make_double2(g.aw[g.aptr[f]], g.aw[g.aptr[f] + 1]);
+            }
+            int rc;
+            if ((rc = upload(c, ggc, &d.ggc)) || (rc = upload(c, ggw, &d.ggw))) return rc;
+        }
     }
     // half-faces and flux blocks
     {
@@ -685,7 +707,17 @@ int fcfd_jacobian_to_host(fcfd_ctx* c, double* JL_host, double* JR_host, double*
     return FCFD_OK;
 }
 
+// The tangent and adjoint kernels differentiate the gradient-map gather;
+// Green-Gauss cells (no map) are not differentiated there, so fcfd_jvp,
+// fcfd_vjp, fcfd_adjoint_step and the matrix-free Newton step refuse a
+// context that has them.
+static int refuse_green_gauss(fcfd_ctx* c) {
+    return c->dg.ggc ? fail(c, FCFD_ENOTSUP, "the exact tangent and adjoint paths do not support Green-Gauss gradients")
+                     : FCFD_OK;
+}
+
 static int ensure_tangent(fcfd_ctx* c) {
+    if (int rc = refuse_green_gauss(c)) return rc;
     if (c->dG) return FCFD_OK;
     const DevGrid& g = c->dg;
     int rc;
@@ -769,7 +801,8 @@ static int ensure_adjoint(fcfd_ctx* c) {
 int fcfd_vjp(fcfd_ctx* c, double* W_dev, const double* Rbar_dev, const double* fx_weight_dev, double* Wbar_dev, void* stream) {
     if (!c || !W_dev || !Rbar_dev || !Wbar_dev) return FCFD_EINVAL;
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
-    int rc = ensure_adjoint(c);
+    int rc = refuse_green_gauss(c);
+    if (!rc) rc = ensure_adjoint(c);
     if (rc) return rc;
     launch_vjp(c->dg, c->ag, c->P, W_dev, c->G, c->PHI, c->dref, Rbar_dev, fx_weight_dev, c->aw, Wbar_dev, c->status,
                (hipStream_t)stream);
@@ -1098,7 +1131,8 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
     if (o->precond == FCFD_PC_BLOCK_SGS && o->sweeps < 1) return fail(c, FCFD_EINVAL, "block SGS needs sweeps >= 1");
     if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad Krylov options");
     const bool sgs = o->precond == FCFD_PC_BLOCK_SGS;
-    int rc = ensure_adjoint(c);
+    int rc = refuse_green_gauss(c);
+    if (!rc) rc = ensure_adjoint(c);
     if (!rc) rc = ensure_newton(c, o);
     if (!rc && sgs) rc = ensure_colors(c);
     if (rc) return rc;

@@ -37,6 +37,11 @@ struct DevGrid {
     const int4* fcell;
     const double2* fgeo;
     const double2* cc2;                 // [2][ld] cell centroid and volume packed: (x, y), (z, V)
+    // Green-Gauss gradients (NULL when every cell has a gradient map):
+    // ggc[i] = 1 for a cell without a map; ggw[f] = the face-average weights
+    // (parent, neighbour or ghost) of internal face f
+    const int* ggc;
+    const double2* ggw;
 };
 
 // Gradients ([3 nv] components per cell), limiters ([nv]) and the FD

@@ -63,6 +63,7 @@ struct BcDev {
     int type, kind, specified, thermal;
     double p, T, rho, V[3], p_total, T_total, mdot, qdot, total_area;
     double nut;      // SA nu~ prescribed at an inlet
+    int p_fixed, T_fixed, V_fixed;   // Variable::fixedonBC of p, T, V (face values in Green-Gauss gradients)
 };
 
 }  // namespace fcfd

@@ -7,7 +7,9 @@
 //   2. grad_lim_kernel   one thread per cell: gradient-map gather (sliced
 //                        ELL, 64-wide slices matching the wavefront) fused
 //                        with the limiter (and, with SA, the cell's V S
-//                        source); also copies gradients to the ghosts
+//                        source); also copies gradients to the ghosts.  Only
+//                        when some cells use Green-Gauss gradients (no map),
+//                        gg_kernel computes theirs just before it.
 //   3. face_flux_kernel  one thread per face, one wavefront per workgroup:
 //                        reconstruction, boundary condition, convective and
 //                        diffusive flux; D - C per face into FX [nv][F]
@@ -64,6 +66,56 @@ __global__ void __launch_bounds__(256) ghost_kernel(DevGrid g, Params P, double*
 // 2. gradients (calc_cell_grads, ns.cc:301-336 + variable.h:193-216) fused
 //    with the limiter (ns_limiters.cc:26-248)
 // --------------------------------------------------------------------------
+// Green-Gauss gradient of each cell without a gradient map
+// (Variable::cell_gradient, variable.h:193-216): over the cell's faces in
+// reference order, the face value times n A / V, negated where the cell is
+// not the face's parent.  The face value is the BC value where the variable
+// is fixed on the face's BC, else the face average w_p q_p + w_n q_n over the
+// parent and the neighbour or ghost (Variable::face_calculate,
+// variable.h:135-148).  Runs after the ghost kernel and before
+// grad_lim_kernel, which takes these gradients for its limiters and ghost
+// copies.
+template <int NV>
+__global__ void __launch_bounds__(256) gg_kernel(DevGrid g, const double* __restrict__ W, double* __restrict__ G) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N || !g.ggc[c]) return;
+    const size_t ld = g.ld;
+    const double vol = g.vol[c];
+    double gr[3 * NV];
+#pragma unroll
+    for (int k = 0; k < 3 * NV; ++k) gr[k] = 0.0;
+    for (int h = g.hf_ptr[c]; h < g.hf_ptr[c + 1]; ++h) {
+        const int fs = g.hf_face[h];
+        const int f = fs >= 0 ? fs : ~fs;
+        const double A = g.farea[f];
+        double a0 = g.fnx[f] * A / vol, a1 = g.fny[f] * A / vol, a2 = g.fnz[f] * A / vol;
+        if (fs < 0) { a0 = a0 * -1.; a1 = a1 * -1.; a2 = a2 * -1.; }
+        const int4 fc = g.fcell[f];
+        const double2 w = g.ggw[f];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            double fv;
+            bool fixed = false;
+            if (fc.z >= 0) {
+                const BcDev& bc = g.bc[fc.z];
+                if (v == 0 && bc.p_fixed) { fv = bc.p; fixed = true; }
+                if (v == 4 && bc.T_fixed) { fv = bc.T; fixed = true; }
+                if (v >= 1 && v <= 3 && bc.V_fixed) { fv = bc.V[v - 1]; fixed = true; }
+            }
+            if (!fixed) {
+                double t = 0.;
+                t += w.x * W[v * ld + fc.x];
+                t += w.y * W[v * ld + fc.y];
+                fv = t;
+            }
+            gr[3 * v] += fv * a0;
+            gr[3 * v + 1] += fv * a1;
+            gr[3 * v + 2] += fv * a2;
+        }
+    }
+    store_pairs<3 * NV>(G, ld, c, gr);
+}
+
 // (3 waves per SIMD: 168 VGPRs with a 12-byte spill; the compiler's own
 // choice, 174 VGPRs at 2 waves, is 6% slower)
 template <int LIM, int NV>
@@ -116,6 +168,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) g
             }
         }
     }
+    if (g.ggc && g.ggc[c]) load_pairs<3 * NV>(G, ld, c, gr);   // a Green-Gauss cell: gg_kernel's gradient
     store_pairs<3 * NV>(G, ld, c, gr);
     if constexpr (NV == 6) {
         // V_c S_c of the SA source (ns_sources.cc:25-30 has none; build-only model)
@@ -469,6 +522,7 @@ static void launch_grad_lim_t(const DevGrid& g, const Params& P, const double* W
                               hipStream_t s) {
     int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
     dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.ggc) hipLaunchKernelGGL(gg_kernel<NV>, grid, blk, 0, s, g, W, G);
     if (lim == FCFD_LIMITER_VK) {
         launch_vk_range(g, P, W, dref, s);
         hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_VK, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref, src);

@@ -214,6 +214,43 @@ def test_gradient_test_linear_exact(method, mesh, oracle_lib):
     assert err < 1e-9, err
 
 
+def test_green_gauss_exact_on_linear_data_in_a_uniform_box(oracle_lib):
+    """Green-Gauss (gradients ...method=greengauss leaves the map empty,
+    gradient_maps.cc:59-72; Variable::cell_gradient sums face values times
+    n A / V, variable.h:193-216).  On a uniform hex box whose cells and
+    mirrored ghosts hold p = a.x, each face average is the field at the face
+    centre, and the face sum over a closed cell is V times the gradient: a,
+    in every cell.  A value fixed on a BC replaces the face average there:
+    fixing p on x = L to a.x_L + delta adds delta / dx to d p / dx of the
+    cells along it, and nothing elsewhere."""
+    GG = A.GRAD_GREENGAUSS
+    nx, ny, nz = 4, 3, 3
+    raw = box_hex(nx, ny, nz)
+    a = np.array([[0.3, -1.2, 2.5], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0], [-0.7, 0.4, 0.9]])
+    bcs = [cases.bc_wall(slip=True)] * 6
+    o = _oracle(raw, cases.make_options(bcs, grad=(GG, GG, GG)))
+    cc = o.metrics()["cell_centroid"].reshape(-1, 3)
+    N = o.info.n_cells
+    o.set_full_state(cc @ a.T)
+    G = o.grads()[:N].reshape(-1, 5, 3)
+    assert np.abs(G - a[None]).max() < 1e-12
+    o.set_full_state(np.tile([2.0, 0.0, 0.0, 0.0, 300.0], (len(cc), 1)))
+    assert np.abs(o.grads()[:N]).max() < 1e-12
+    o.close()
+    delta = 0.25
+    bcs[1] = cases.bc_outlet_p(0.3 * 1.0 + delta)          # xmax: p fixed; the field is p = 0.3 x + ...
+    a2 = a.copy()
+    a2[0] = [0.3, 0.0, 0.0]
+    o = _oracle(raw, cases.make_options(bcs, grad=(GG, GG, GG)))
+    o.set_full_state(cc @ a2.T)
+    G = o.grads()[:N].reshape(-1, 5, 3)
+    edge = np.abs(cc[:N, 0] - (1.0 - 0.5 / nx)) < 1e-12
+    want = np.tile(a2[None], (N, 1, 1))
+    want[edge, 0, 0] += delta / (1.0 / nx)
+    assert np.abs(G - want).max() < 1e-12
+    o.close()
+
+
 def test_curvilinear_is_central_difference(oracle_lib):
     raw = box_hex(5, 4, 4, L=(5.0, 2.0, 4.0))
     opt = cases.make_options([cases.bc_wall(slip=True)] * 6, grad=(A.GRAD_CURVILINEAR,) * 3)

@@ -99,6 +99,49 @@ def test_plate_steger_warming():
     _compare(cs, label="SW")
 
 
+def _green_gauss_case(name):
+    GG = A.GRAD_GREENGAUSS
+    if name == "wedge":
+        cs = cases.wedge_case(24, 12)
+    elif name == "cylinder_fixed_bcs":            # no-slip wall (V fixed), T-fixed inlet, p-fixed outlet
+        cs = cases.cylinder_case(32, 16, noise=0.01)
+    elif name == "tetbox":
+        cs = cases.tetbox_case(4)
+    elif name == "plate_prisms_only":             # hexes keep their curvilinear maps
+        cs = cases.plate_case(16, 10, 4, noise=0.01, first=1e-3)
+        cs.opt.grad_prism = GG
+        return cs
+    else:
+        cs = cases.smooth_state(cases.plate_case(16, 10, 4, sa=True))
+    cs.opt.grad_hex, cs.opt.grad_prism, cs.opt.grad_other = GG, GG, GG
+    return cs
+
+
+@pytest.mark.parametrize("name", ["wedge", "cylinder_fixed_bcs", "tetbox", "plate_prisms_only", "plate_sa"])
+def test_green_gauss(name):
+    """Green-Gauss gradients (the gradients section's ...method=greengauss,
+    gradient_maps.cc:45-72; the face sum of Variable::cell_gradient,
+    variable.h:193-216) on hexes, tets and prisms, mixed with curvilinear
+    maps, with face values fixed by BCs, and with the SA variable: ghosts,
+    gradients, limiters and residual against the CPU restatement at the
+    residual bar."""
+    _compare(_green_gauss_case(name), label="green-gauss " + name)
+
+
+def test_green_gauss_refused_by_the_exact_tangent():
+    """The tangent and adjoint kernels do not differentiate Green-Gauss
+    gradients: the matrix-free Newton step and fcfd_jvp report ENOTSUP."""
+    from freecfd_amd.ns import NavierStokes
+    cs = _green_gauss_case("wedge")
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    with pytest.raises(RuntimeError, match="Green-Gauss"):
+        ns.newton_step(A.newton_opts(cfl=10.0, jacobian=A.JAC_JVP_FULL, fp32=True))
+    st = ns.newton_step(A.newton_opts(cfl=10.0))     # the FD Jacobian path runs
+    assert np.isfinite(st.res_norm)
+    ns.close()
+
+
 def _unpack_pairs(T, comps, ld, perm, rows):
     """Row-pair device array -> [rows][comps] in reference numbering."""
     a = T.cpu().numpy().reshape(-1, ld, 2)
