@@ -47,6 +47,7 @@ struct fcfd_ctx {
     double *JL = nullptr, *JR = nullptr, *Dg = nullptr, *zero_upd = nullptr;   // FD Jacobian (lazy)
     // Newton-Krylov workspace (lazy)
     double *R = nullptr, *dt = nullptr, *dW = nullptr, *upd = nullptr, *red_part = nullptr, *red_out = nullptr;
+    double* dt_red = nullptr;   // 2 + 256 doubles: CFL_MAX step, largest CFL, block partials (lazy)
     int gm_restart32 = 0, gm_restart64 = 0;   // restart length of the allocated FP32 / FP64 Krylov bases
     GmresWork<double> w64;
     GmresWork<float> w32;
@@ -641,13 +642,31 @@ int fcfd_status(fcfd_ctx* c, int clear) {
     return FCFD_OK;
 }
 
+static int check_dt(fcfd_ctx* c, int dt_type, double value) {
+    if (dt_type != FCFD_DT_FIXED && dt_type != FCFD_DT_CFL_MAX && dt_type != FCFD_DT_CFL_LOCAL)
+        return fail(c, FCFD_EINVAL, "unknown time step type");
+    if (!(value > 0.0)) return fail(c, FCFD_EINVAL, "the step size or CFL number must be positive");
+    if (!c->dt_red) return dalloc(c, 2 + 256, &c->dt_red);
+    return FCFD_OK;
+}
+
 int fcfd_time_step(fcfd_ctx* c, const double* W_dev, int dt_type, double cfl, double* dt_dev, double* max_cfl, void* stream) {
     if (!c || !W_dev || !dt_dev) return FCFD_EINVAL;
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
-    if (dt_type != FCFD_DT_CFL_LOCAL) return fail(c, FCFD_ENOTSUP, "only CFL_LOCAL time stepping is implemented");
-    launch_dt_local(c->dg, c->P, W_dev, cfl, dt_dev, (hipStream_t)stream);
+    if (int rc = check_dt(c, dt_type, cfl)) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    launch_time_step(c->dg, c->P, W_dev, dt_type, cfl, dt_dev, c->dt_red, s);
     HIPCHK(c, hipGetLastError());
-    if (max_cfl) *max_cfl = cfl;
+    if (max_cfl) {
+        if (dt_type == FCFD_DT_CFL_LOCAL) {
+            *max_cfl = cfl;   // time_step.cc:207-210
+        } else {
+            launch_max_cfl(c->dg, c->P, W_dev, dt_type == FCFD_DT_CFL_MAX ? c->dt_red : nullptr, cfl, c->dt_red, s);
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipMemcpyAsync(max_cfl, c->dt_red + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
+        }
+    }
     return FCFD_OK;
 }
 
@@ -972,7 +991,7 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     if (!c || !W_dev || !o) return FCFD_EINVAL;
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
     int rc0 = 0;
-    if (o->dt_type != FCFD_DT_CFL_LOCAL) return fail(c, FCFD_ENOTSUP, "only CFL_LOCAL time stepping is implemented");
+    if ((rc0 = check_dt(c, o->dt_type, o->cfl))) return rc0;
     if (o->jacobian != FCFD_JAC_FD && o->jacobian != FCFD_JAC_JVP_FULL && o->jacobian != FCFD_JAC_JVP_FROZEN)
         return fail(c, FCFD_EINVAL, "unknown Jacobian mode");
     const bool mfree = o->jacobian != FCFD_JAC_FD;
@@ -1008,7 +1027,7 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     S.res_norm = std::sqrt(rr);
     auto t1 = clk::now();
     // 2. local time step, FD Jacobian + time terms, block-Jacobi inverse
-    launch_dt_local(g, c->P, W_dev, o->cfl, c->dt, s);
+    launch_time_step(g, c->P, W_dev, o->dt_type, o->cfl, c->dt, c->dt_red, s);
     launch_fd_jacobian(g, c->P, W_dev, c->G, c->PHI, c->upd, c->dt, c->JL, c->JR, c->Dg, c->status, s);
     if (fp32) {
         dev_bj_invert<float>(g, c->Dg, c->Dinv32, c->status, s);
@@ -1124,7 +1143,7 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
                       fcfd_newton_stats* st, void* stream) {
     if (!c || !W_dev || !g_dev || !lam_dev || !o) return FCFD_EINVAL;
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
-    if (o->dt_type != FCFD_DT_CFL_LOCAL) return fail(c, FCFD_ENOTSUP, "only CFL_LOCAL time stepping is implemented");
+    if (int rc_dt = check_dt(c, o->dt_type, o->cfl)) return rc_dt;
     if (o->krylov_fp32) return fail(c, FCFD_ENOTSUP, "the adjoint step solves in FP64 (krylov_fp32 must be 0)");
     if (o->precond != FCFD_PC_BLOCK_JACOBI && o->precond != FCFD_PC_BLOCK_SGS)
         return fail(c, FCFD_ENOTSUP, "the adjoint step preconditions with block Jacobi or block SGS");
@@ -1157,7 +1176,7 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
     }
     // 2. local time step, FD blocks with the floor step, their diagonal
     //    inverses (transposed block Jacobi or transposed block SGS)
-    launch_dt_local(g, c->P, W_dev, o->cfl, c->dt, s);
+    launch_time_step(g, c->P, W_dev, o->dt_type, o->cfl, c->dt, c->dt_red, s);
     launch_fd_jacobian(g, c->P, W_dev, c->G, c->PHI, c->zero_upd, c->dt, c->JL, c->JR, c->Dg, c->status, s);
     dev_bj_invert<double>(g, c->Dg, c->Dinv64, c->status, s);
     HIPCHK(c, hipStreamSynchronize(s));

@@ -120,6 +120,14 @@ void launch_residual_sum(const DevGrid& g, const double* src, const double* FX, 
 void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double* dref, hipStream_t s);
 constexpr int kDrefSize = 8 + 10 * 1024;   // dref[nv] + VK min/max partials
 void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s);
+// dt of every cell for FCFD_DT_FIXED (value = the step size), FCFD_DT_CFL_MAX
+// (the minimum over the cells of value L / (|V| + a), kept in red[0]) or
+// FCFD_DT_CFL_LOCAL (value = the CFL number); red holds 2 + 256 doubles
+void launch_time_step(const DevGrid& g, const Params& P, const double* W, int dt_type, double value, double* dt, double* red,
+                      hipStream_t s);
+// red[1] = the largest (|V| + a) step / L over the cells, step = *step or step_val
+void launch_max_cfl(const DevGrid& g, const Params& P, const double* W, const double* step, double step_val, double* red,
+                    hipStream_t s);
 // brute-force wall distances (cells: T centroids; faces: with the SW dissipation factor)
 void launch_wall_distance(int T, const double* ccx, const double* ccy, const double* ccz, int nw, const double* wx, const double* wy,
                           const double* wz, double* wd, hipStream_t s);

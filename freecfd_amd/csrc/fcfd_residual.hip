@@ -569,6 +569,77 @@ void launch_residual_sum(const DevGrid& g, const double* src, const double* FX, 
     else hipLaunchKernelGGL((cell_sum_kernel<5, false>), grid, blk, 0, s, g, nullptr, FX, R);
 }
 
+// CFL_MAX and FIXED time steps and the largest CFL number
+// (update_time_step, time_step.cc:155-213).  Minima and maxima are exact, so
+// the two-pass reductions give the reference's values whatever their order.
+constexpr int kDtBlocks = 256;
+
+__global__ void __launch_bounds__(256) dt_reduce_partial_kernel(DevGrid g, Params P, const double* __restrict__ W,
+                                                                const double* __restrict__ dt, const double* __restrict__ cur,
+                                                                double cur_val, int want_max, double* __restrict__ part) {
+    // want_max = 0: min of dt[c]; 1: max of (|V| + a) cur / L with cur = *cur
+    // (or cur_val when cur is NULL)
+    __shared__ double sh[256];
+    const double step = cur ? *cur : cur_val;
+    double m = want_max ? -1. : 1.e20;
+    const size_t ld = g.ld;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < g.N; c += gridDim.x * 256) {
+        if (want_max) {
+            double p = W[c], T = W[4 * ld + c];
+            Vec<double> V = mk<double>(W[ld + c], W[2 * ld + c], W[3 * ld + c]);
+            m = fmax(m, (vmag(V) + m_a(P, p, T)) * step / g.lscale[c]);
+        } else {
+            m = fmin(m, dt[c]);
+        }
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] = want_max ? fmax(sh[threadIdx.x], sh[threadIdx.x + w]) : fmin(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) dt_reduce_final_kernel(const double* __restrict__ part, int want_max, double* __restrict__ out) {
+    __shared__ double sh[256];
+    sh[threadIdx.x] = part[threadIdx.x];   // kDtBlocks == blockDim.x partials
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] = want_max ? fmax(sh[threadIdx.x], sh[threadIdx.x + w]) : fmin(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+__global__ void __launch_bounds__(256) dt_fill_kernel(int N, const double* __restrict__ src, double val, double* __restrict__ dt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < N) dt[c] = src ? *src : val;
+}
+
+void launch_time_step(const DevGrid& g, const Params& P, const double* W, int dt_type, double value, double* dt, double* red,
+                      hipStream_t s) {
+    const dim3 grid((g.N + 255) / 256), blk(256);
+    if (dt_type == FCFD_DT_FIXED) {
+        hipLaunchKernelGGL(dt_fill_kernel, grid, blk, 0, s, g.N, (const double*)nullptr, value, dt);
+        return;
+    }
+    launch_dt_local(g, P, W, value, dt, s);
+    if (dt_type == FCFD_DT_CFL_MAX) {
+        hipLaunchKernelGGL(dt_reduce_partial_kernel, dim3(kDtBlocks), blk, 0, s, g, P, W, (const double*)dt, (const double*)nullptr,
+                           0.0, 0, red + 2);
+        hipLaunchKernelGGL(dt_reduce_final_kernel, dim3(1), blk, 0, s, (const double*)(red + 2), 0, red);
+        hipLaunchKernelGGL(dt_fill_kernel, grid, blk, 0, s, g.N, (const double*)red, 0.0, dt);
+    }
+}
+
+void launch_max_cfl(const DevGrid& g, const Params& P, const double* W, const double* step, double step_val, double* red,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(dt_reduce_partial_kernel, dim3(kDtBlocks), dim3(256), 0, s, g, P, W, (const double*)nullptr, step, step_val, 1,
+                       red + 2);
+    hipLaunchKernelGGL(dt_reduce_final_kernel, dim3(1), dim3(256), 0, s, (const double*)(red + 2), 1, red + 1);
+}
+
 void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s) {
     hipLaunchKernelGGL(dt_local_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, P, W, cfl, dt);
 }

@@ -297,5 +297,17 @@ class NavierStokes(Grid):
                                            C.c_void_p(s)))
         return self.to_host(dt.reshape(1, -1), 1)[:, 0]
 
+    def time_step(self, dt_type, value):
+        """update_time_step for the current state (fcfd_time_step):
+        FCFD_DT_FIXED (value = the step size), FCFD_DT_CFL_MAX or
+        FCFD_DT_CFL_LOCAL (value = the CFL number).  Returns the cells' dt in
+        reference order and the largest CFL number."""
+        dt = self.torch.zeros(self.ld, dtype=self.torch.float64, device=self.device)
+        mc = C.c_double()
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_time_step(self.h, self.W.data_ptr(), int(dt_type), float(value), dt.data_ptr(), C.byref(mc),
+                                           C.c_void_p(s)))
+        return self.to_host(dt.reshape(1, -1), 1)[:, 0], mc.value
+
     def status(self, clear=True):
         _check(self.h, lib().fcfd_status(self.h, int(clear)))

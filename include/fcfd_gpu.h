@@ -289,8 +289,9 @@ int  fcfd_jacobian_to_host(fcfd_ctx* ctx, double* JL_host, double* JR_host, doub
 #define FCFD_JAC_JVP_FULL    3   /* matrix-free exact tangent of the full residual    */
 
 typedef struct fcfd_newton_opts {
-    int dt_type;          /* FCFD_DT_CFL_LOCAL (update_time_step, time_step.cc:193-200) */
-    double cfl;
+    int dt_type;          /* FCFD_DT_CFL_LOCAL, FCFD_DT_CFL_MAX or FCFD_DT_FIXED
+                             (update_time_step, time_step.cc:155-200) */
+    double cfl;           /* the CFL number; the step size for FCFD_DT_FIXED */
     int jacobian;         /* FCFD_JAC_FD: assembled reference FD Jacobian; FCFD_JAC_JVP_*:
                              matrix-free tangent operator (P V/dt) x - J x.  In every mode the
                              block-Jacobi preconditioner inverts the diagonal blocks of the
@@ -342,8 +343,12 @@ int  fcfd_jvp(fcfd_ctx* ctx, double* W_dev, double* dW_dev, double* dR_dev, int 
  * [5][ld_cell], updated in place. */
 int  fcfd_newton_step(fcfd_ctx* ctx, double* W_dev, const fcfd_newton_opts* opts, fcfd_newton_stats* stats, void* stream);
 
-/* Local time step (update_time_step, src/time_step.cc:143-224) for
- * CFL_LOCAL or CFL_MAX; dt_dev is [ld_cell]; *max_cfl as the reference. */
+/* Time step of every cell (update_time_step, src/time_step.cc:143-224):
+ * FCFD_DT_CFL_LOCAL (cfl L / (|V| + a) per cell), FCFD_DT_CFL_MAX (the
+ * smallest of those for every cell) or FCFD_DT_FIXED (`cfl` is the step
+ * size).  dt_dev is [ld_cell].  *max_cfl (may be NULL) as the reference: the
+ * CFL number for CFL_LOCAL, else the largest (|V| + a) dt / L, which waits
+ * for the stream. */
 int  fcfd_time_step(fcfd_ctx* ctx, const double* W_dev, int dt_type, double cfl, double* dt_dev, double* max_cfl, void* stream);
 
 /* Adjoint (vector-Jacobian product) of the residual, the exact transpose of

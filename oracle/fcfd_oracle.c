@@ -1801,7 +1801,10 @@ int orc_get_loads(const orc_ctx* o, double* loads) { memcpy(loads, o->loads, siz
 /* update_time_step (src/time_step.cc:143-224), no ramp / schedule */
 int orc_time_step(orc_ctx* o, int dt_type, double cfl, double* dt, double* max_cfl) {
     double cur = 1.;
-    if (dt_type == FCFD_DT_CFL_MAX) {
+    if (dt_type == FCFD_DT_FIXED) {   /* time_step.cc:155-156: cfl carries the step size */
+        cur = cfl;
+        for (int c = 0; c < o->ncell; ++c) dt[c] = cur;
+    } else if (dt_type == FCFD_DT_CFL_MAX) {
         cur = 1.e20;
         for (int c = 0; c < o->ncell; ++c) {
             double a = m_a(o, o->p[c], o->T[c]);

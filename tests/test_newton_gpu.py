@@ -206,3 +206,51 @@ def test_relative_update_limit_scales_the_step():
             assert np.isclose(st.relax, limit / rel, rtol=1e-12)
             assert np.allclose(dW, st.relax * full, rtol=1e-9, atol=1e-12 * np.abs(cs.W0).max())
         ns.close()
+
+
+@pytest.mark.parametrize("dt_type", [A.DT_FIXED, A.DT_CFL_MAX, A.DT_CFL_LOCAL])
+def test_time_step_types_match_the_oracle(dt_type):
+    """fcfd_time_step for each update_time_step type (time_step.cc:155-213):
+    the cells' dt and the largest CFL number bitwise equal to the CPU
+    restatement's on the cylinder (minima and maxima are exact)."""
+    from oracle.oracle import Oracle
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.cylinder_case(32, 16, noise=0.05)
+    value = 3.0e-6 if dt_type == A.DT_FIXED else 7.5
+    o = Oracle(cs.raw, cs.opt)
+    o.set_state(cs.W0)
+    dt_o, mc_o = o.time_step(dt_type, value)
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    dt_g, mc_g = ns.time_step(dt_type, value)
+    print("  dt_type %d: max cfl %.17g / %.17g" % (dt_type, mc_g, mc_o), flush=True)
+    assert np.array_equal(dt_g, dt_o) and mc_g == mc_o
+    ns.close()
+    o.close()
+
+
+def test_newton_step_with_a_global_time_step():
+    """A first Newton step with CFL_MAX equals the first step with FIXED at
+    the step size CFL_MAX picks (the smallest local step), bitwise; each in a
+    fresh context, since a context carries the previous update into the
+    next FD Jacobian and initial guess."""
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.wedge_case(24, 12)
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    dt, _ = ns.time_step(A.DT_CFL_MAX, 50.0)
+    ns.close()
+    out = []
+    for dt_type, value in ((A.DT_CFL_MAX, 50.0), (A.DT_FIXED, float(dt[0]))):
+        ns = NavierStokes(cs.raw, cs.opt)
+        ns.set_state(cs.W0)
+        o = A.newton_opts(cfl=value, max_iters=30, rtol=1e-8)
+        o.dt_type = dt_type
+        st = ns.newton_step(o)
+        assert np.isfinite(st.res_norm)
+        out.append(ns.state_host(include_ghosts=False))
+        ns.close()
+    print("  max |W_cflmax - W_fixed| = %.3e, max |dW| = %.3e" % (np.abs(out[0] - out[1]).max(), np.abs(out[0] - cs.W0).max()),
+          flush=True)
+    assert not np.array_equal(out[0], cs.W0)
+    assert np.array_equal(out[0], out[1])

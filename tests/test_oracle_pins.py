@@ -465,6 +465,14 @@ def test_local_time_step_closed_form(oracle_lib):
     a = np.sqrt(g * R * W[:, 4])
     assert np.allclose(dt, 10.0 * ls / (np.linalg.norm(W[:, 1:4], axis=1) + a), rtol=1e-14)
     assert mc == 10.0
+    # CFL_MAX: one step, the smallest local one; the largest CFL is then the target
+    lam = np.linalg.norm(W[:, 1:4], axis=1) + a
+    dt, mc = o.time_step(A.DT_CFL_MAX, 10.0)
+    assert (dt == dt[0]).all() and np.isclose(dt[0], (10.0 * ls / lam).min(), rtol=1e-14)
+    assert np.isclose(mc, 10.0, rtol=1e-13)
+    # FIXED: the step size everywhere; the largest CFL from it (time_step.cc:155-156,200-206)
+    dt, mc = o.time_step(A.DT_FIXED, 2.5e-5)
+    assert (dt == 2.5e-5).all() and np.isclose(mc, (lam * 2.5e-5 / ls).max(), rtol=1e-14)
 
 
 # ---------------------------------------------------------------------------
