@@ -726,17 +726,14 @@ int fcfd_jacobian_to_host(fcfd_ctx* c, double* JL_host, double* JR_host, double*
     return FCFD_OK;
 }
 
-// The tangent and adjoint kernels differentiate the gradient-map gather;
-// Green-Gauss cells (no map) are not differentiated there, so fcfd_jvp,
-// fcfd_vjp, fcfd_adjoint_step and the matrix-free Newton step refuse a
-// context that has them.
+// The adjoint kernels transpose the gradient-map gather; Green-Gauss cells
+// (no map) are not transposed there, so fcfd_vjp and fcfd_adjoint_step
+// refuse a context that has them.
 static int refuse_green_gauss(fcfd_ctx* c) {
-    return c->dg.ggc ? fail(c, FCFD_ENOTSUP, "the exact tangent and adjoint paths do not support Green-Gauss gradients")
-                     : FCFD_OK;
+    return c->dg.ggc ? fail(c, FCFD_ENOTSUP, "the adjoint path does not support Green-Gauss gradients") : FCFD_OK;
 }
 
 static int ensure_tangent(fcfd_ctx* c) {
-    if (int rc = refuse_green_gauss(c)) return rc;
     if (c->dG) return FCFD_OK;
     const DevGrid& g = c->dg;
     int rc;

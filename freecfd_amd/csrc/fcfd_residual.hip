@@ -74,9 +74,11 @@ __global__ void __launch_bounds__(256) ghost_kernel(DevGrid g, Params P, double*
 // parent and the neighbour or ghost (Variable::face_calculate,
 // variable.h:135-148).  Runs after the ghost kernel and before
 // grad_lim_kernel, which takes these gradients for its limiters and ghost
-// copies.
+// copies.  The sum is linear in the cell values, so with tangent = 1 the
+// same kernel on a tangent dW gives the tangent gradient (a fixed BC value
+// contributes 0).
 template <int NV>
-__global__ void __launch_bounds__(256) gg_kernel(DevGrid g, const double* __restrict__ W, double* __restrict__ G) {
+__global__ void __launch_bounds__(256) gg_kernel(DevGrid g, const double* __restrict__ W, double* __restrict__ G, int tangent) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N || !g.ggc[c]) return;
     const size_t ld = g.ld;
@@ -101,6 +103,7 @@ __global__ void __launch_bounds__(256) gg_kernel(DevGrid g, const double* __rest
                 if (v == 0 && bc.p_fixed) { fv = bc.p; fixed = true; }
                 if (v == 4 && bc.T_fixed) { fv = bc.T; fixed = true; }
                 if (v >= 1 && v <= 3 && bc.V_fixed) { fv = bc.V[v - 1]; fixed = true; }
+                if (fixed && tangent) fv = 0.0;
             }
             if (!fixed) {
                 double t = 0.;
@@ -517,12 +520,19 @@ void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double*
     else launch_vk_range_t<5>(g, W, dref, s);
 }
 
+void launch_green_gauss(const DevGrid& g, const double* W, double* G, int tangent, hipStream_t s) {
+    if (!g.ggc) return;
+    const dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL(gg_kernel<6>, grid, blk, 0, s, g, W, G, tangent);
+    else hipLaunchKernelGGL(gg_kernel<5>, grid, blk, 0, s, g, W, G, tangent);
+}
+
 template <int NV>
 static void launch_grad_lim_t(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, double* src,
                               hipStream_t s) {
     int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
     dim3 grid((g.N + 255) / 256), blk(256);
-    if (g.ggc) hipLaunchKernelGGL(gg_kernel<NV>, grid, blk, 0, s, g, W, G);
+    launch_green_gauss(g, W, G, 0, s);
     if (lim == FCFD_LIMITER_VK) {
         launch_vk_range(g, P, W, dref, s);
         hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_VK, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref, src);

@@ -105,6 +105,12 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
     double gv[3 * NV], gd[3 * NV];
 #pragma unroll
     for (int k = 0; k < 3 * NV; ++k) { gv[k] = gr[k].v; gd[k] = gr[k].d; }
+    if (g.ggc && g.ggc[c]) {   // a Green-Gauss cell: launch_green_gauss's primal and tangent gradients
+        load_pairs<3 * NV>(G, ld, c, gv);
+        load_pairs<3 * NV>(dG, ld, c, gd);
+#pragma unroll
+        for (int k = 0; k < 3 * NV; ++k) gr[k] = Dual(gv[k], gd[k]);
+    }
     store_pairs<3 * NV>(G, ld, c, gv);
     store_pairs<3 * NV>(dG, ld, c, gd);
     Dual q0[NV];
@@ -267,6 +273,8 @@ static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, double* d
     dim3 grid((g.N + 255) / 256), blk(256);
     if (full) {
         if (g.B > 0) hipLaunchKernelGGL(ghost_t_kernel<NV>, dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
+        launch_green_gauss(g, W, G, 0, s);
+        launch_green_gauss(g, dW, dG, 1, s);
         if (lim == FCFD_LIMITER_VK) {
             launch_vk_range(g, P, W, dref, s);   // primal reference range (constant in the tangent)
             hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_VK, NV>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);

@@ -92,10 +92,10 @@ extern "C" {
 /* gradient methods (src/gradient_maps.cc:26-28) */
 #define FCFD_GRAD_CURVILINEAR 1
 #define FCFD_GRAD_LSQR        2
-#define FCFD_GRAD_GREENGAUSS  3   /* face sum of face values (variable.h:193-216); FP64 residual,
-                                     FD-Jacobian Newton and loads only: fcfd_jvp, fcfd_vjp,
-                                     fcfd_adjoint_step and the matrix-free Newton step return
-                                     FCFD_ENOTSUP for a context with Green-Gauss cells */
+#define FCFD_GRAD_GREENGAUSS  3   /* face sum of face values (variable.h:193-216): residual,
+                                     tangent and both Newton modes; fcfd_vjp and
+                                     fcfd_adjoint_step return FCFD_ENOTSUP for a context with
+                                     Green-Gauss cells */
 /* interpolation (src/interpolate/interpolate.h) */
 #define FCFD_INTERP_WTLI   1
 #define FCFD_INTERP_IDW    2
