@@ -70,7 +70,7 @@ def structured_hex(X, bcs, dimension=3, periodic_i=False):
 
 
 # ---------------------------------------------------------------------------
-# BASELINE meshes
+# benchmark meshes
 # ---------------------------------------------------------------------------
 
 def wedge(ni=96, nj=48, length=1.0, height=0.5, ramp_x=0.2, angle_deg=15.0, dz=0.01):

@@ -10,7 +10,7 @@
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
  * this code; the product library never links it.
  *
- * Decisions where the reference is undefined (documented in DESIGN.md):
+ * Decisions where the reference is undefined:
  *  D1  update_boundaries() has no source (declared ns.h:165).  Ghost values
  *      are set per boundary face by apply_bcs() with the parent cell-centre
  *      state as `left` and `right` pre-set to a copy of `left`; the ghost

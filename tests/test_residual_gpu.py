@@ -1,6 +1,6 @@
 """GPU parity of the FP64 residual path against the CPU restatement.
 
-Bar (north star): per cell and variable, |R_gpu - R_oracle| <= 1e-12 * S,
+Bar: per cell and variable, |R_gpu - R_oracle| <= 1e-12 * S,
 where S = sum over the cell's faces of |face contribution| (the summation
 scale; |R| itself cancels to ~0 near steady state).  Ghost states must match
 to 1e-14 relative, gradients to 1e-11 of their summation scale.
