@@ -295,14 +295,7 @@ int build_device(fcfd_ctx* c) {
     {
         std::vector<int> hp, hf;
         half_faces(g, hp, hf);
-        std::vector<int> blk{0};
-        int start = 0;
-        for (int i = 0; i < N; ++i) {
-            if (hp[i + 1] - hp[start] > 256) { blk.push_back(i); start = i; }
-        }
-        blk.push_back(N);
         d.nhf = (int)hf.size();
-        d.nblk = (int)blk.size() - 1;
         std::vector<int> hn(hf.size());
         for (int i = 0; i < N; ++i) {
             if (hp[i + 1] - hp[i] > kMaxCellFaces || hp[i + 1] == hp[i]) return fail(c, FCFD_EMESH, "cell with no or more than 6 faces");
@@ -312,9 +305,7 @@ int build_device(fcfd_ctx* c) {
             }
         }
         int rc;
-        if ((rc = upload(c, hp, &d.hf_ptr)) || (rc = upload(c, hf, &d.hf_face)) || (rc = upload(c, hn, &d.hf_nb)) ||
-            (rc = upload(c, blk, &d.blk_cell)))
-            return rc;
+        if ((rc = upload(c, hp, &d.hf_ptr)) || (rc = upload(c, hf, &d.hf_face)) || (rc = upload(c, hn, &d.hf_nb))) return rc;
     }
     // faces
     {
@@ -1055,7 +1046,7 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
         (void)hipMemcpyAsync(c->jv_in, x, n5 * sizeof(double), hipMemcpyDeviceToDevice, ss);
         launch_jvp(g, c->P, W_dev, c->jv_in, c->G, c->dG, c->PHI, c->dPHI, c->dref, c->dFX, c->jv_R,
                    o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss);
-        launch_newton_op(g, c->P, W_dev, c->dt, x, c->jv_R, y, ss);
+        launch_newton_op(g, c->P, W_dev, c->dt, c->jv_in, c->jv_R, y, ss);   // jv_in: x may alias y
     };
     // preconditioners: block Jacobi, block SGS, or the FP64 multigrid V-cycle
     // (FP32 Krylov vectors converted at its boundary)
@@ -1119,7 +1110,7 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     }
     double out[5];
     HIPCHK(c, dev_update(g, W_dev, c->dW, c->dt, c->red_part, c->red_out, out, s));
-    HIPCHK(c, hipMemcpyAsync(c->upd, c->dW, n5 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (!(out[3] > 0.0)) HIPCHK(c, hipMemcpyAsync(c->upd, c->dW, n5 * sizeof(double), hipMemcpyDeviceToDevice, s));
     HIPCHK(c, hipStreamSynchronize(s));
     auto t4 = clk::now();
     S.update_norms[0] = std::sqrt(out[0]); S.update_norms[1] = std::sqrt(out[1]); S.update_norms[2] = std::sqrt(out[2]);
@@ -1159,6 +1150,8 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
     using clk = std::chrono::steady_clock;
     auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
     fcfd_newton_stats S{};
+    // the step reports the pivot flags its own kernels raise
+    HIPCHK(c, hipMemsetAsync(c->status, 0, sizeof(int), s));
     auto t0 = clk::now();
     // 1. adjoint residual rho = J^T lam + g (the VJP also refreshes ghosts, G and PHI of W)
     launch_vjp(g, c->ag, c->P, W_dev, c->G, c->PHI, c->dref, lam_dev, nullptr, c->aw, c->R, c->status, s);
@@ -1221,7 +1214,13 @@ int fcfd_set_bc(fcfd_ctx* c, int index, const fcfd_bc* bc) {
         c->opt = saved;
         return rc;
     }
+    const int viscous_before = c->P.viscous;
     set_params(c);
+    if (c->P.viscous != viscous_before) {   // the kernels and adjoint workspace were chosen for the old flag
+        c->opt = saved;
+        set_params(c);
+        return fail(c, FCFD_EBC, "fcfd_set_bc cannot switch the diffusive flux on or off (a FIXED_Q wall's qdot between 0 and non-zero on an inviscid case)");
+    }
     if (c->device) {
         // the table is read by every kernel: replace it with no work in flight
         HIPCHK(c, hipDeviceSynchronize());

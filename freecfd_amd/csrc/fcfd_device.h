@@ -17,7 +17,7 @@ namespace fcfd {
 // half-faces of a cell are in ascending reference face order, so summing them
 // in order reproduces the reference's VecSetValues(ADD) sequence.
 struct DevGrid {
-    int N, B, F, ld, nhf, nblk;
+    int N, B, F, ld, nhf;
     int nv;                             // variables per cell (5, or 6 with SA)
     const double* wd;                   // [ld] closest wall distance of each cell (SA)
     int ref_first, ref_last;            // internal ids of reference cells 0 and N-1
@@ -26,7 +26,6 @@ struct DevGrid {
     const double2 *gwa, *gwb;           // per entry (w_x, w_y) and (w_z, neighbour index as int64 bits)
     const int *hf_ptr, *hf_face;        // N+1, nhf
     const int* hf_nb;                   // nhf: the cell across each half-face (neighbour or parent; a ghost for a boundary face)
-    const int* blk_cell;                // nblk+1 first cell of each flux block
     const int *fpar, *fnei, *fbc;
     const double *fnx, *fny, *fnz, *t1x, *t1y, *t1z, *farea, *fcx, *fcy, *fcz, *fwp, *fwn, *fwd, *fdiss;
     const int* gh_face;                 // B: boundary face of ghost N+i

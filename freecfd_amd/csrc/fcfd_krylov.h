@@ -57,7 +57,8 @@ void dev_block_invert(int nv, int n, size_t ld, const double* D, double* Dinv, i
 // max over interior cells of |dp|/(p+Pref) and |dT|/(T+Tref) for the update dW
 hipError_t dev_relmax(const DevGrid& g, double Pref, double Tref, const double* W, const double* dW, double* part, double* out_host,
                       hipStream_t s);
-// out_host: sum dp^2/dt^2, sum |dV|^2/dt^2, sum dT^2/dt^2, NaN/Inf count, sum dnu~^2/dt^2
+// out_host: sum dp^2/dt^2, sum |dV|^2/dt^2, sum dT^2/dt^2, NaN/Inf count, sum dnu~^2/dt^2;
+// W += dW (interior cells) only when the NaN/Inf count is 0, else W is left as it was
 hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
                       double out_host[5], hipStream_t s);
 template <typename T>

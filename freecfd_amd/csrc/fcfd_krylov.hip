@@ -136,12 +136,13 @@ __global__ void __launch_bounds__(kRedThreads) relmax_kernel(int N, size_t ld, d
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// Newton update W += dW over interior cells, with the reference's update norms
+// The reference's update norms of a Newton update dW over interior cells
 // (ns.cc:349-381): sums of dp^2/dt^2, |dV|^2/dt^2, dT^2/dt^2, a NaN/Inf count
-// and (SA model) dnu~^2/dt^2 -- partial sums per block, fixed-order tree.
+// and (SA model) dnu~^2/dt^2 -- partial sums per block, fixed-order tree.  W
+// is not touched here: apply_update_kernel adds dW only once the count is 0.
 template <int NV>
-__global__ void __launch_bounds__(kRedThreads) update_kernel(int N, size_t ld, double* __restrict__ W, const double* __restrict__ dW,
-                                                             const double* __restrict__ dt, double* __restrict__ part) {
+__global__ void __launch_bounds__(kRedThreads) update_norms_kernel(int N, size_t ld, const double* __restrict__ dW,
+                                                                   const double* __restrict__ dt, double* __restrict__ part) {
     __shared__ double sh[5][kRedThreads];
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int c = blockIdx.x * kRedThreads + threadIdx.x; c < N; c += kRedBlocks * kRedThreads) {
@@ -150,8 +151,6 @@ __global__ void __launch_bounds__(kRedThreads) update_kernel(int N, size_t ld, d
 #pragma unroll
         for (int v = 0; v < NV; ++v) { d[v] = dW[v * ld + c]; bad |= !isfinite(d[v]); }
         if (bad) { acc[3] += 1.0; continue; }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) W[v * ld + c] += d[v];
         double dt2 = dt[c] * dt[c];
         acc[0] += d[0] * d[0] / dt2;
         acc[1] += d[1] * d[1] / dt2 + d[2] * d[2] / dt2 + d[3] * d[3] / dt2;
@@ -167,6 +166,15 @@ __global__ void __launch_bounds__(kRedThreads) update_kernel(int N, size_t ld, d
     }
     if (threadIdx.x == 0)
         for (int q = 0; q < 5; ++q) part[q * kRedBlocks + blockIdx.x] = sh[q][0];
+}
+
+// W += dW over interior cells (ns.cc:340-348)
+template <int NV>
+__global__ void __launch_bounds__(256) apply_update_kernel(int N, size_t ld, double* __restrict__ W, const double* __restrict__ dW) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) W[v * ld + c] += dW[v * ld + c];
 }
 
 // ---------------------------------------------------------------------------
@@ -477,12 +485,17 @@ hipError_t dev_relmax(const DevGrid& g, double Pref, double Tref, const double* 
 
 hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const double* dt, double* part, double* out_dev,
                       double out_host[5], hipStream_t s) {
-    if (g.nv == 6) hipLaunchKernelGGL(update_kernel<6>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, W, dW, dt, part);
-    else hipLaunchKernelGGL(update_kernel<5>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, W, dW, dt, part);
+    if (g.nv == 6) hipLaunchKernelGGL(update_norms_kernel<6>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, dW, dt, part);
+    else hipLaunchKernelGGL(update_norms_kernel<5>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, dW, dt, part);
     hipLaunchKernelGGL(reduce_final_kernel, dim3(5), dim3(kRedThreads), 0, s, part, out_dev);
     hipError_t e = hipMemcpyAsync(out_host, out_dev, 5 * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
-    return hipStreamSynchronize(s);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    if (out_host[3] > 0.0) return hipSuccess;   // NaN/Inf in dW: W stays as it was
+    const dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL(apply_update_kernel<6>, grid, blk, 0, s, g.N, (size_t)g.ld, W, dW);
+    else hipLaunchKernelGGL(apply_update_kernel<5>, grid, blk, 0, s, g.N, (size_t)g.ld, W, dW);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -180,6 +180,9 @@ template <typename S> __device__ __forceinline__ int apply_bcs(const Params& P, 
         } else if (bc.kind == FCFD_KIND_MDOT) {
             S uNL = vdot(L.V, F.n);
             double mdotNR = -1. * bc.mdot / bc.total_area;
+            // the reference's test `!specified != BC_STATE` (ns_apply_bcs.cc:99-104)
+            // is true for every specification: a supersonic mass-flow inlet always stops
+            if (val(-uNL / L.a) >= 1.) flag = 1;
             if (bc.specified == FCFD_SPEC_STATE) { p = S(bc.p); T = S(bc.T); rho = S(bc.rho); }
             else if (bc.specified == FCFD_SPEC_P) { p = S(bc.p); T = L.T; rho = m_rho(P, p, T); }
             else if (bc.specified == FCFD_SPEC_T) { T = S(bc.T); p = L.p; rho = m_rho(P, p, T); }

@@ -210,9 +210,10 @@ const char* fcfd_last_error(const fcfd_ctx* ctx);
 int  fcfd_get_info(const fcfd_ctx* ctx, fcfd_info* info);
 int  fcfd_metrics(const fcfd_ctx* ctx, fcfd_metrics_out* out);
 
-/* Layout helpers.  W_host is N x 5 in reference numbering, AoS
- * (p,u,v,w,T per cell, the order of Variable p, V, T in ns.h:104-106).
- * W_dev is a device SoA array [5][ld_cell]. */
+/* Layout helpers.  W_host is N x nv in reference numbering, AoS
+ * (p,u,v,w,T per cell, the order of Variable p, V, T in ns.h:104-106, then
+ * nu~ with the SA model; nv = fcfd_info.nv).  W_dev is a device SoA array
+ * [nv][ld_cell]. */
 int  fcfd_state_to_device(fcfd_ctx* ctx, const double* W_host, double* W_dev);
 int  fcfd_cells_to_host(fcfd_ctx* ctx, const double* X_dev, int nvar, int include_ghosts, double* X_host);
 
@@ -269,7 +270,7 @@ int  fcfd_status(fcfd_ctx* ctx, int clear);
  * (get_jacobians + MatSetValues, ns_assemble_linear_system.cc:144-267) and,
  * when dt_dev is not NULL, the time terms P*V/dt (ns_time_terms.cc:147-163).
  * Recomputes ghosts, gradients and limiters of W_dev first.  dWprev_dev
- * ([5][ld_cell], the previous Newton update that sets the FD step,
+ * ([nv][ld_cell], the previous Newton update that sets the FD step,
  * :217-219) may be NULL (zero update).  The result stays in the context:
  * per face JL and JR (nv x nv entries per face: entry k = nv*var+eq of face f
  * at ((k/2)*F + f)*2 + k%2, in internal face order, entries stored in row
@@ -293,10 +294,15 @@ typedef struct fcfd_newton_opts {
                              (update_time_step, time_step.cc:155-200) */
     double cfl;           /* the CFL number; the step size for FCFD_DT_FIXED */
     int jacobian;         /* FCFD_JAC_FD: assembled reference FD Jacobian; FCFD_JAC_JVP_*:
-                             matrix-free tangent operator (P V/dt) x - J x.  In every mode the
-                             block-Jacobi preconditioner inverts the diagonal blocks of the
-                             FD Jacobian + time terms.                                 */
-    int krylov_fp32;      /* 0: FP64 GMRES + FP64 block-Jacobi; 1: FP32 for both         */
+                             matrix-free tangent operator (P V/dt) x - J x.  Every
+                             preconditioner is built from the blocks of the FD Jacobian +
+                             time terms (see precond).                                 */
+    int krylov_fp32;      /* 0: FP64 GMRES basis, FP64 preconditioner.  1: FP32 GMRES basis
+                             and Gram-Schmidt; block Jacobi and block SGS in FP32; the
+                             multigrid V-cycle stays FP64 (vectors converted at its
+                             boundary); with FCFD_JAC_FD the BSR matvec runs in FP32 on
+                             FP32 copies of the blocks, with FCFD_JAC_JVP_* the tangent
+                             runs in FP64 on the converted vector                      */
     int restart;          /* GMRES restart length (the reference uses FGMRES(300))       */
     int max_iters;        /* maximumiterations (default 10, read_inputs.cc:127)          */
     double rtol, abstol;  /* relativetolerance / absolutetolerance (read_inputs.cc:125-126) */
@@ -316,7 +322,7 @@ typedef struct fcfd_newton_opts {
 #define FCFD_PC_MULTIGRID    2
 
 typedef struct fcfd_newton_stats {
-    double res_norm;          /* ||R(W)||_2 over interior cells and the 5 equations, before the update */
+    double res_norm;          /* ||R(W)||_2 over interior cells and the nv equations, before the update */
     double update_norms[3];   /* sqrt(sum dp^2/dt^2), sqrt(sum |dV|^2/dt^2), sqrt(sum dT^2/dt^2) (ns.cc:377-394) */
     int krylov_iters, krylov_converged;
     double krylov_rnorm, krylov_r0norm;
@@ -331,7 +337,7 @@ typedef struct fcfd_newton_stats {
  * Jacobian of the second-order residual; the ghost slots [N,N+B) of W_dev
  * and dW_dev are written); FCFD_JAC_JVP_FROZEN holds gradients, limiters and
  * ghost centres fixed -- the linearisation the reference FD Jacobian
- * approximates.  Arrays [5][ld_cell].  No counterpart in the reference,
+ * approximates.  Arrays [nv][ld_cell].  No counterpart in the reference,
  * which only builds the FD Jacobian. */
 int  fcfd_jvp(fcfd_ctx* ctx, double* W_dev, double* dW_dev, double* dR_dev, int mode, void* stream);
 
@@ -340,7 +346,8 @@ int  fcfd_jvp(fcfd_ctx* ctx, double* W_dev, double* dW_dev, double* dR_dev, int 
  * tangent), time terms, GMRES with block-Jacobi, W += dW with the NaN/Inf
  * check (FCFD_EDIVERGED, ns.cc:350-355).  The update is kept in the context
  * as the previous update that sets the FD step of the next call.  W_dev is
- * [5][ld_cell], updated in place. */
+ * [nv][ld_cell], updated in place; it is left unchanged when the update holds a
+ * NaN/Inf (FCFD_EDIVERGED). */
 int  fcfd_newton_step(fcfd_ctx* ctx, double* W_dev, const fcfd_newton_opts* opts, fcfd_newton_stats* stats, void* stream);
 
 /* Time step of every cell (update_time_step, src/time_step.cc:143-224):
@@ -391,7 +398,9 @@ int  fcfd_adjoint_step(fcfd_ctx* ctx, double* W_dev, const double* g_dev, double
 /* Replace the values of BC `index` (p, T, rho, V, p_total, T_total, mdot,
  * qdot, nut) in a live context, the mesh and its set-up unchanged: type,
  * kind, specified and thermal must equal the ones given at create
- * (FCFD_EBC otherwise).  Synchronises the device first, so no work of the
+ * (FCFD_EBC otherwise), and the change may not switch the diffusive flux on
+ * or off (a FIXED_Q wall's qdot between 0 and non-zero on an otherwise
+ * inviscid case: FCFD_EBC).  Synchronises the device first, so no work of the
  * context may be in flight on another thread.  For parameter studies and
  * the parameter derivatives of an adjoint; the reference reads BC values
  * once from its input file (src/set_bcs.cc:35, ns_set_bcs.cc:25-180). */

@@ -978,6 +978,8 @@ static int apply_bcs(orc_ctx* o, cstate* L, cstate* R, const fstate* F) {
         } else if (bc->kind == FCFD_KIND_MDOT) {                       /* :90-135 */
             double uNL = vdot(L->V, F->normal);
             double mdotNR = -1. * bc->mdot / o->bc_total_area[F->bc];
+            /* :99-104: `!specified != BC_STATE` holds for every specification */
+            if (-uNL / L->a >= 1.) { seterr(o, FCFD_EBC, "mass-flow inlet became supersonic"); return FCFD_EBC; }
             if (bc->specified == FCFD_SPEC_STATE) { R->p = bc->p; R->T = bc->T; R->rho = bc->rho; }
             else if (bc->specified == FCFD_SPEC_P) { R->p = bc->p; R->T = L->T; R->rho = m_rho(o, R->p, R->T); }
             else if (bc->specified == FCFD_SPEC_T) { R->T = bc->T; R->p = L->p; R->rho = m_rho(o, R->p, R->T); }

@@ -45,3 +45,28 @@ def test_set_bc_validates_on_a_host_context():
         assert L.fcfd_set_bc(h, -1, ctypes.byref(bc)) == A.EINVAL
     finally:
         L.fcfd_destroy(h)
+
+
+def test_set_bc_refuses_switching_the_diffusive_flux():
+    """On an inviscid case a FIXED_Q wall with qdot = 0 leaves the diffusive
+    flux off; a new qdot != 0 would switch it on, which changes the kernels
+    and the adjoint workspace chosen at create: fcfd_set_bc refuses it and
+    keeps the old values."""
+    from freecfd_amd import _abi as A, cases
+    cs = cases.wedge_case(4, 2)
+    wall = [b for b in range(cs.opt.n_bc) if cs.opt.bc[b].type == A.BC_WALL][0]
+    cs.opt.bc[wall].thermal = A.THERMAL_FIXED_Q
+    cs.opt.bc[wall].qdot = 0.0
+    m = A.make_mesh(cs.raw)
+    h = ctypes.c_void_p()
+    L = _lib.lib()
+    assert L.fcfd_create_host(ctypes.byref(m), ctypes.byref(cs.opt), ctypes.byref(h)) == 0
+    try:
+        bc = A.BC.from_buffer_copy(cs.opt.bc[wall])
+        bc.qdot = 250.0
+        assert L.fcfd_set_bc(h, wall, ctypes.byref(bc)) == A.EBC
+        assert b"diffusive" in L.fcfd_last_error(h)
+        bc.qdot = 0.0     # the unchanged flag is accepted
+        assert L.fcfd_set_bc(h, wall, ctypes.byref(bc)) == 0
+    finally:
+        L.fcfd_destroy(h)
