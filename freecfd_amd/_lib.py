@@ -40,6 +40,7 @@ def lib():
         "fcfd_jvp": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
         "fcfd_newton_step": (C.c_int, [_vp, _vp, C.POINTER(A.NewtonOpts), C.POINTER(A.NewtonStats), _vp]),
         "fcfd_time_step": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _vp, C.POINTER(C.c_double), _vp]),
+        "fcfd_time_step_noalloc": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _vp, _vp, _vp]),
         "fcfd_vjp": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "fcfd_gradients_to_host": (C.c_int, [_vp, _vp, _vp]),
         "fcfd_loads": (C.c_int, [_vp, _vp, _vp, _vp]),

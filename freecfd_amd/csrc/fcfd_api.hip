@@ -20,6 +20,7 @@
 #include "fcfd_tangent.h"
 
 #include <chrono>
+#include <mutex>
 
 using namespace fcfd;
 
@@ -658,6 +659,64 @@ int fcfd_time_step(fcfd_ctx* c, const double* W_dev, int dt_type, double cfl, do
             HIPCHK(c, hipStreamSynchronize(s));
         }
     }
+    return FCFD_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Scratch of fcfd_time_step_noalloc: the CFL_MAX minimum (slot 0), the
+// largest CFL number (slot 1) and the 256 block partials of the two-pass
+// reductions (slots 2..257), the same layout as the context's lazily
+// allocated dt_red.  Its size does not depend on the mesh.  In host code the
+// array's name is no device address: kernels get the per-device address
+// from hipGetSymbolAddress, looked up once per device with that device
+// current.  Every call on one GPU shares that GPU's array.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kDtScratch = 2 + 256;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) double scratch[kDtScratch];
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(fcfd_ctx* c, double** out) {
+    int dev = -1;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(c, FCFD_ENOTSUP, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FCFD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int fcfd_time_step_noalloc(fcfd_ctx* c, const double* W_dev, int dt_type, double cfl, double* dt_dev, double* max_cfl_dev,
+                           void* stream) {
+    if (!c || !W_dev || !dt_dev) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    if (dt_type != FCFD_DT_FIXED && dt_type != FCFD_DT_CFL_MAX && dt_type != FCFD_DT_CFL_LOCAL)
+        return fail(c, FCFD_EINVAL, "unknown time step type");
+    if (!(cfl > 0.0)) return fail(c, FCFD_EINVAL, "the step size or CFL number must be positive");
+    double* red = nullptr;
+    if (int rc = scratch_for_current_device(c, &red)) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    launch_time_step(c->dg, c->P, W_dev, dt_type, cfl, dt_dev, red, s);
+    if (max_cfl_dev) {
+        if (dt_type == FCFD_DT_CFL_LOCAL) {
+            launch_store_scalar(nullptr, cfl, max_cfl_dev, s);   // time_step.cc:207-210
+        } else {
+            launch_max_cfl(c->dg, c->P, W_dev, dt_type == FCFD_DT_CFL_MAX ? red : nullptr, cfl, red, s);
+            launch_store_scalar(red + 1, 0.0, max_cfl_dev, s);
+        }
+    }
+    HIPCHK(c, hipGetLastError());
     return FCFD_OK;
 }
 

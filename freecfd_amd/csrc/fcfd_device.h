@@ -128,6 +128,8 @@ void launch_green_gauss(const DevGrid& g, const double* W, double* G, int tangen
 // FCFD_DT_CFL_LOCAL (value = the CFL number); red holds 2 + 256 doubles
 void launch_time_step(const DevGrid& g, const Params& P, const double* W, int dt_type, double value, double* dt, double* red,
                       hipStream_t s);
+// *dst = *src, or val when src is NULL (one thread, stream-ordered)
+void launch_store_scalar(const double* src, double val, double* dst, hipStream_t s);
 // red[1] = the largest (|V| + a) step / L over the cells, step = *step or step_val
 void launch_max_cfl(const DevGrid& g, const Params& P, const double* W, const double* step, double step_val, double* red,
                     hipStream_t s);

@@ -650,6 +650,10 @@ void launch_max_cfl(const DevGrid& g, const Params& P, const double* W, const do
     hipLaunchKernelGGL(dt_reduce_final_kernel, dim3(1), dim3(256), 0, s, (const double*)(red + 2), 1, red + 1);
 }
 
+void launch_store_scalar(const double* src, double val, double* dst, hipStream_t s) {
+    hipLaunchKernelGGL(dt_fill_kernel, dim3(1), dim3(64), 0, s, 1, src, val, dst);
+}
+
 void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s) {
     hipLaunchKernelGGL(dt_local_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, P, W, cfl, dt);
 }

@@ -266,6 +266,21 @@ int  fcfd_profile(fcfd_ctx* ctx, int enable, double* stage_ms, long long* calls)
  * ns_apply_bcs.cc:54-59 -> FCFD_EBC). */
 int  fcfd_status(fcfd_ctx* ctx, int clear);
 
+/* The same time step as fcfd_time_step, from an entry point that allocates
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc), where fcfd_time_step's first call
+ * allocates its reduction buffer.  Its scratch (the CFL_MAX minimum, the
+ * largest CFL number and 256 block partials: 258 doubles, whatever the mesh
+ * size, so no input is too large for it) is one static device array per GPU,
+ * shared by every call on that GPU: calls that overlap on one GPU race.  The
+ * largest CFL number goes to the device pointer max_cfl_dev (may be NULL),
+ * stream-ordered, with no host synchronisation; dt_dev is written as by
+ * fcfd_time_step.  Calls no hipMalloc/hipFree and no allocating library
+ * routine. */
+int  fcfd_time_step_noalloc(fcfd_ctx* ctx, const double* W_dev, int dt_type, double cfl, double* dt_dev, double* max_cfl_dev,
+                            void* stream);
+
 /* Reference-equivalent FP64 finite-difference Jacobian of the face loop
  * (get_jacobians + MatSetValues, ns_assemble_linear_system.cc:144-267) and,
  * when dt_dev is not NULL, the time terms P*V/dt (ns_time_terms.cc:147-163).
