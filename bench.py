@@ -23,6 +23,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the four timed stages of fcfd_residual (fcfd_profile) and their kernels
+STAGES = ["ghost", "grad_lim", "face_flux", "cell_sum"]
+KERNEL_NAMES = {"ghost": "ghost_kernel", "grad_lim": "grad_lim_kernel", "face_flux": "face_flux_kernel",
+                "cell_sum": "cell_sum_kernel"}
+FACE_PASS_KERNELS = ["face_flux", "cell_sum"]
 
 
 def make_case(name, scale=1):
@@ -56,21 +61,34 @@ WORKLOADS = {
 
 
 def model_bytes(info, n_grad, has_phi, sw=False):
-    """Algorithmic (compulsory) HBM bytes per launch of each kernel: every
-    array element the kernel needs, read or written once (nv = 5, or 6 with
-    the SA model)."""
+    """Algorithmic (compulsory) HBM bytes per residual evaluation by the
+    two-pass model of the residual path: every array element a pass needs,
+    read or written once, from the mesh's actual sizes (nv = 5, or 6 with the
+    SA model; 8 B per FP64, 4 B per int32).
+
+    pass 1 (cells: gradients + limiters): W (N+B) nv 8, centroids (N+B) 24,
+      CSR offsets 2 (N+1) 4, gradient-map entries n_g (4 + 24), cell->face
+      n_f 4, face centroid + parent/neighbour F 32, write grad W N 3nv 8,
+      write phi N nv 8
+    pass 2 (faces): face arrays F 104 (topology 8, n/area/t1 56, centroid 24,
+      face-average weights 16), gather W (N+B) nv 8, grad W N 3nv 8, phi
+      N nv 8, centroids (N+B) 24, write R N nv 8
+
+    The per-face D - C buffer the face pass writes and sums back is not
+    compulsory (the model's face pass writes R), so it counts as overhead, in
+    the measured traffic only.  The SA source adds V and d read and V S
+    written by pass 1 and read by pass 2 (N 3 8 and N 8).  The boundary-ghost
+    update is outside the two-pass model; its own bytes are listed apart."""
     N, B, F, H, nv = info.n_cells, info.n_ghosts, info.n_faces, info.n_halffaces, info.nv
-    face_arrays = F * (12 + 24 + 24 + 8 + 24 + 16 + (16 if sw else 0))   # parent/neighbour/bc, n, t1, area, centroid, weights
-    face_flux = (face_arrays + (N + B) * nv * 8 + N * 3 * nv * 8 + (N * nv * 8 if has_phi else 0) + (N + B) * 24
-                 + F * nv * 8)                                             # + write D-C per face
-    cell_sum = H * 4 + (N + 1) * 4 + F * nv * 8 + N * nv * 8               # half-faces, read D-C, write R
-    grad = ((N + B) * nv * 8 + n_grad * (4 + 24) + (N + 1) * 4 + H * 4 + F * (8 + 24) + N * 24
-            + (N + B) * 3 * nv * 8 + (N * nv * 8 if has_phi else 0))
-    if nv == 6:   # SA source V S: grad_lim reads V and d and writes it, cell_sum reads it
-        grad += N * 3 * 8
-        cell_sum += N * 8
+    phi = N * nv * 8 if has_phi else 0
+    pass1 = ((N + B) * nv * 8 + (N + B) * 24 + 2 * (N + 1) * 4 + n_grad * (4 + 24) + H * 4 + F * 32
+             + N * 3 * nv * 8 + phi)
+    pass2 = F * (104 + (16 if sw else 0)) + (N + B) * nv * 8 + N * 3 * nv * 8 + phi + (N + B) * 24 + N * nv * 8
+    if nv == 6:
+        pass1 += N * 3 * 8
+        pass2 += N * 8
     ghost = B * (4 + 4 + 4 + 24 + 24 + 8 + nv * 8 + nv * 8)
-    return {"ghost": ghost, "grad_lim": grad, "face_flux": face_flux, "cell_sum": cell_sum}
+    return {"pass1_gradients_limiters": pass1, "pass2_face_pass": pass2, "ghost": ghost}
 
 
 def measured_traffic(config, cells, kernel):
@@ -279,13 +297,19 @@ def main():
 
     ms_step = 1e3 * elapsed / args.steps
     evals_s = args.steps / elapsed
-    names = ["ghost", "grad_lim", "face_flux", "cell_sum"]
-    avg_ms = {names[k]: stage[k] / max(calls.value, 1) for k in range(4)}
+    avg_ms = {STAGES[k]: stage[k] / max(calls.value, 1) for k in range(4)}
     has_phi = cs.opt.limiter != A.LIMITER_NONE and cs.opt.order == A.ORDER_SECOND
     mb = model_bytes(info, info.n_gradmap, has_phi, sw=cs.opt.convective_flux == A.FLUX_SW)
-    dom = max(avg_ms, key=lambda k: avg_ms[k])
-    achieved = mb[dom] / (avg_ms[dom] * 1e-3) / 1e9
-    total_bytes = sum(mb.values())
+    # the two passes of the model and the kernels (fcfd_profile stages) that carry them
+    passes = {
+        "pass2_face_pass": (FACE_PASS_KERNELS, mb["pass2_face_pass"]),
+        "pass1_gradients_limiters": (["grad_lim"], mb["pass1_gradients_limiters"]),
+    }
+    pass_ms = {k: sum(avg_ms[x] for x in ks) for k, (ks, _) in passes.items()}
+    dom = max(pass_ms, key=lambda k: pass_ms[k])
+    achieved = passes[dom][1] / (pass_ms[dom] * 1e-3) / 1e9
+    model_total = mb["pass1_gradients_limiters"] + mb["pass2_face_pass"]
+    traffic = [measured_traffic(args.config, info.n_cells, k) for k in passes[dom][0]]
     line = {
         "metric": "cell-residual evals/s",
         "value": cs.n_cells * evals_s,
@@ -301,13 +325,22 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.config], "cells": info.n_cells, "faces": info.n_faces,
                    "boundary_faces": info.n_ghosts, "half_faces": info.n_halffaces,
+                   "gradmap_entries": info.n_gradmap,
                    "residual_evals_per_s": evals_s,
                    "stage_ms": {k: round(v, 5) for k, v in avg_ms.items()},
-                   "model_bytes_per_eval": total_bytes,
-                   "residual_hbm_gbs_model": total_bytes / (ms_step * 1e-3) / 1e9},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, info.n_cells, dom),
-                     "algorithmic_bytes_per_launch": mb[dom], "avg_launch_ms": avg_ms[dom]},
+                   "model_bytes": mb,
+                   "whole_residual": {"model_bytes_per_eval": model_total,
+                                      "achieved_gbs": model_total / (ms_step * 1e-3) / 1e9,
+                                      "frac": model_total / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                   "pass_roofline": {k: {"kernels": passes[k][0], "ms": pass_ms[k], "model_bytes": passes[k][1],
+                                         "achieved_gbs": passes[k][1] / (pass_ms[k] * 1e-3) / 1e9,
+                                         "frac": passes[k][1] / (pass_ms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                                     for k in passes}},
+        "roofline": {"bound": "hbm", "kernel": dom + " (" + " + ".join(KERNEL_NAMES[k] for k in passes[dom][0]) + ")",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": sum(traffic) if all(t is not None for t in traffic) else None,
+                     "algorithmic_bytes_per_launch": passes[dom][1], "avg_launch_ms": pass_ms[dom]},
     }
     if args.newton_steps > 0:
         line["config"]["newton"] = newton_compare(ns, cs, args.newton_steps)

@@ -126,6 +126,7 @@ struct orc_ctx {
     double* phi;                    /* ntot*NVM */
     double* upd;                    /* ncell*NVM */
     double *Rs, *Rabs;              /* ncell*NVM */
+    double *fC, *fD;                /* F*NVM: convective and diffusive flux of each face (last assemble) */
     double *JL, *JR, *Dg;           /* F*NVM^2 [var][eq], F*NVM^2, N*NVM^2 [row][col] */
     double* loads;                  /* nbc*6 */
     int err; char msg[256];
@@ -842,6 +843,8 @@ orc_ctx* orc_create(const fcfd_mesh* mesh, const fcfd_options* opt) {
     o->upd = (double*)calloc((size_t)o->ncell * NVM, sizeof(double));
     o->Rs = (double*)calloc((size_t)o->ncell * NVM, sizeof(double));
     o->Rabs = (double*)calloc((size_t)o->ncell * NVM, sizeof(double));
+    o->fC = (double*)calloc((size_t)o->nface * NVM, sizeof(double));
+    o->fD = (double*)calloc((size_t)o->nface * NVM, sizeof(double));
     o->loads = (double*)calloc((size_t)(o->nbc + 1) * 6, sizeof(double));
     return o;
 }
@@ -860,7 +863,7 @@ void orc_destroy(orc_ctx* o) {
     free(o->avg_ptr); free(o->avg_idx); free(o->avg_w); free(o->g_ptr); free(o->g_idx); free(o->g_w);
     free(o->p); free(o->u); free(o->v); free(o->w); free(o->T);
     free(o->gp); free(o->gu); free(o->gv); free(o->gw); free(o->gT); free(o->nut); free(o->gn);
-    free(o->phi); free(o->upd); free(o->Rs); free(o->Rabs); free(o->JL); free(o->JR); free(o->Dg); free(o->loads);
+    free(o->phi); free(o->upd); free(o->Rs); free(o->Rabs); free(o->fC); free(o->fD); free(o->JL); free(o->JR); free(o->Dg); free(o->loads);
     free(o);
 }
 
@@ -1673,6 +1676,7 @@ int orc_assemble(orc_ctx* o, int with_jac) {
         convective_face_flux(o, &L, &R, &F, fc);
         fd[0] = 0.;
         diffusive_face_flux(o, &F, fd);
+        for (int i = 0; i < NVM; ++i) { o->fC[NVM * f + i] = fc[i]; o->fD[NVM * f + i] = fd[i]; }
         if (F.bc >= 0 && o->opt.bc[F.bc].include_in_loads) {          /* :109-119 */
             V3 t = v3(fc[1] - fd[1], fc[2] - fd[2], fc[3] - fd[3]);
             double* ld = &o->loads[6 * F.bc];
@@ -1900,5 +1904,56 @@ int orc_sa_probe(double rho, double mu, double nut, const double* gvel, const do
     out[1] = P; out[2] = D;
     out[3] = sa_mut(rho, nut, mu);
     out[4] = sa_fn(rho, nut, mu);
+    return 0;
+}
+
+/* per-face fluxes of the last orc_assemble: C (convective) and D
+ * (diffusive), F x nv each, reference face order */
+int orc_get_face_flux(const orc_ctx* o, double* C, double* D) {
+    const int nv = o->nv;
+    for (int f = 0; f < o->nface; ++f)
+        for (int i = 0; i < nv; ++i) {
+            if (C) C[nv * f + i] = o->fC[NVM * f + i];
+            if (D) D[nv * f + i] = o->fD[NVM * f + i];
+        }
+    return 0;
+}
+
+/* probe of apply_bcs on BC `bc` of this context: the finished left face
+ * state from WL (p,u,v,w,T), its cell-centre values WLc, and the right
+ * state's centre values WRc (the ghost's); the right state starts as a copy
+ * of the left one, as in the face loop, and is finished after the BC.
+ * out: p, u, v, w, T, rho, a, p_c, u_c, v_c, w_c, T_c of the right state.
+ * Returns apply_bcs' status (FCFD_EBC where the reference exits). */
+int orc_bc_probe(orc_ctx* o, int bc, const double* WL, const double* WLc, const double* WRc, const double* n,
+                 const double* t1, double* out) {
+    fstate F;
+    memset(&F, 0, sizeof F);
+    F.normal = v3(n[0], n[1], n[2]);
+    F.tangent1 = v3(t1[0], t1[1], t1[2]);
+    F.tangent2 = vnorm(vcross(F.normal, F.tangent1));
+    F.area = 1.;
+    F.bc = bc;
+    cstate L, R;
+    memset(&L, 0, sizeof L);
+    L.p = WL[0]; L.V = v3(WL[1], WL[2], WL[3]); L.T = WL[4];
+    L.p_center = WLc[0]; L.V_center = v3(WLc[1], WLc[2], WLc[3]); L.T_center = WLc[4];
+    L.rho = m_rho(o, L.p, L.T);
+    finish_state(o, &L, &F);
+    R = L;
+    R.p_center = WRc[0]; R.V_center = v3(WRc[1], WRc[2], WRc[3]); R.T_center = WRc[4];
+    int rc = apply_bcs(o, &L, &R, &F);
+    if (rc) return rc;
+    double a_bc = R.a;
+    finish_state(o, &R, &F);
+    double v[12] = {R.p, R.V.c[0], R.V.c[1], R.V.c[2], R.T, R.rho, a_bc, R.p_center, R.V_center.c[0], R.V_center.c[1],
+                    R.V_center.c[2], R.T_center};
+    memcpy(out, v, sizeof v);
+    return 0;
+}
+
+/* AUSM+-up split functions at M (probe): out = M4+, M4-, P5+(alpha), P5-(alpha) */
+int orc_ausm_probe(double M, double alpha, double* out) {
+    out[0] = msp4(M); out[1] = msm4(M); out[2] = psp5(M, alpha); out[3] = psm5(M, alpha);
     return 0;
 }

@@ -51,6 +51,16 @@ int orc_get_loads(const orc_ctx* o, double* loads);
 int orc_time_step(orc_ctx* o, int dt_type, double cfl, double* dt, double* max_cfl);
 /* time_terms P*V/dt blocks [N][5][5] (ns_time_terms.cc:26-56,147-163) */
 int orc_time_terms(orc_ctx* o, const double* dt, double* blocks);
+/* per-face convective C and diffusive D fluxes of the last orc_assemble, F x nv each */
+int orc_get_face_flux(const orc_ctx* o, double* C, double* D);
+/* apply_bcs on BC `bc`: left face state WL and centre WLc, ghost centre WRc
+ * (p,u,v,w,T each), face normal n and tangent t1; out (12) = right state
+ * p,u,v,w,T,rho, the a apply_bcs set, centre p,u,v,w,T.  FCFD_EBC where the
+ * reference exits. */
+int orc_bc_probe(orc_ctx* o, int bc, const double* WL, const double* WLc, const double* WRc, const double* n,
+                 const double* t1, double* out);
+/* AUSM+-up split functions: out = M4+(M), M4-(M), P5+(M,alpha), P5-(M,alpha) */
+int orc_ausm_probe(double M, double alpha, double* out);
 /* SA model pieces at one point: out = source/volume, P, D, mu_t, fn */
 int orc_sa_probe(double rho, double mu, double nut, const double* gvel, const double* gn, double d, double* out);
 

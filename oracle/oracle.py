@@ -47,6 +47,9 @@ def lib():
         L.orc_flux_probe.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_double, C.c_void_p]
         L.orc_sa_probe.argtypes = [C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
+        L.orc_get_face_flux.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_bc_probe.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
+        L.orc_ausm_probe.argtypes = [C.c_double, C.c_double, C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -151,6 +154,24 @@ class Oracle:
         self._check(lib().orc_get_jacobian(self.h, _p(JL), _p(JR), _p(Dg)))
         return JL, JR, Dg
 
+    def face_fluxes(self):
+        """(C, D): convective and diffusive flux of every face of the last
+        residual evaluation, F x nv each, reference face order."""
+        F, nv = self.info.n_faces, self.info.nv
+        Cf = np.zeros((F, nv))
+        Df = np.zeros((F, nv))
+        self._check(lib().orc_get_face_flux(self.h, _p(Cf), _p(Df)))
+        return Cf, Df
+
+    def bc_probe(self, bc, WL, WLc, WRc, n, t1):
+        """apply_bcs of BC `bc` on one face: returns (status, dict of the
+        right state: p, V, T, rho, a (as the BC set it), p_c, V_c, T_c)."""
+        arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (WL, WLc, WRc, n, t1)]
+        out = np.zeros(12)
+        rc = lib().orc_bc_probe(self.h, int(bc), *[_p(a) for a in arr], _p(out))
+        return rc, dict(p=out[0], V=out[1:4].copy(), T=out[4], rho=out[5], a=out[6], p_c=out[7], V_c=out[8:11].copy(),
+                        T_c=out[11])
+
     def loads(self):
         out = np.zeros((self._opt.n_bc, 6))
         self._check(lib().orc_get_loads(self.h, _p(out)))
@@ -179,3 +200,10 @@ def sa_probe(rho, mu, nut, grad_vel, grad_nut, d):
     out = np.zeros(5)
     lib().orc_sa_probe(float(rho), float(mu), float(nut), _p(gv), _p(gn), float(d), _p(out))
     return dict(source=out[0], P=out[1], D=out[2], mu_t=out[3], fn=out[4])
+
+
+def ausm_split(M, alpha):
+    """AUSM+-up split functions (M4+, M4-, P5+, P5-) at Mach number M."""
+    out = np.zeros(4)
+    lib().orc_ausm_probe(float(M), float(alpha), _p(out))
+    return out

@@ -1,0 +1,641 @@
+"""More pins of the CPU restatement: every boundary-condition branch, the
+dissipation of the Riemann fluxes, the viscous flux and the conservation of
+the residual assembly.  Each check is a closed form or an identity that the
+mathematics fixes, written independently of the oracle's code:
+
+  - apply_bcs (ns_apply_bcs.cc:25-258), branch by branch: the outgoing
+    Riemann invariant u_n + 2a/(gamma-1) carried from the left state
+    (velocity and mass-flow inlets), isentropic extrapolation p/rho^gamma
+    (outlets, temperature inlet), total conditions (stagnation inlet), the
+    prescribed mass flux, mirror reflection (slip wall, symmetry), the
+    no-slip and isothermal walls, reverse-flow damping and blocking, the
+    forced-supersonic outlet, and where the reference exits;
+  - Roe (ns_roe.cc:25-98): equal to the symmetric five-wave textbook form
+    1/2 (F_L + F_R) - 1/2 sum_k |lambda_k| alpha_k r_k built from Roe
+    averages when the entropy fix is idle, and an entropy-fixed acoustic
+    eigenvalue that is continuous and C1 across the fix window;
+  - AUSM+-up (ns_ausm_plus_up.cc:35-152): M4+ + M4- = M, P5+ + P5- = 1, the
+    mirror symmetries, and continuity (C1) of the split polynomials at
+    |M| = 1;
+  - every flux: antisymmetry F(L, R, n) = -F(R, L, -n);
+  - diffusive flux (ns_diffusive_face_flux.cc:28-64) with the face-gradient
+    correction (ns_assemble_linear_system.cc:394-404): for linear velocity
+    and temperature fields the interior face flux is the exact
+    Newtonian stress tau.n A, the work (tau V).n A and the conduction
+    lambda grad T.n A; an adiabatic no-slip wall conducts no heat, a FIXED_Q
+    wall carries exactly qdot A, a FIXED_T wall conducts
+    lambda (T_w - T_P) / |l2r| A;
+  - assembly (ns_assemble_linear_system.cc:108-140): the cell residuals sum
+    to the boundary faces' contributions (interior faces telescope), and
+    their momentum components to minus the surface loads of every boundary.
+"""
+import numpy as np
+import pytest
+
+from freecfd_amd import _abi as A
+from freecfd_amd import cases, meshgen
+
+GAMMA = 1.4
+RGAS = cases.gas_R(28.97)
+
+
+def _oracle(raw, opt):
+    from oracle.oracle import Oracle
+    return Oracle(raw, opt)
+
+
+def box_hex(nx, ny, nz, L=(1.0, 1.0, 1.0), jitter=0.0, seed=0):
+    X = np.zeros((nx + 1, ny + 1, nz + 1, 3))
+    X[..., 0] = np.linspace(0, L[0], nx + 1)[:, None, None]
+    X[..., 1] = np.linspace(0, L[1], ny + 1)[None, :, None]
+    X[..., 2] = np.linspace(0, L[2], nz + 1)[None, None, :]
+    if jitter:
+        rng = np.random.default_rng(seed)
+        h = np.array([L[0] / nx, L[1] / ny, L[2] / nz])
+        d = rng.uniform(-jitter, jitter, X.shape) * h
+        d[0, :, :, 0] = d[-1, :, :, 0] = 0.0
+        d[:, 0, :, 1] = d[:, -1, :, 1] = 0.0
+        d[:, :, 0, 2] = d[:, :, -1, 2] = 0.0
+        X = X + d
+    bcs = [("xmin", lambda i, j, k: i == 0), ("xmax", lambda i, j, k: i == nx),
+           ("ymin", lambda i, j, k: j == 0), ("ymax", lambda i, j, k: j == ny),
+           ("zmin", lambda i, j, k: k == 0), ("zmax", lambda i, j, k: k == nz)]
+    return meshgen.structured_hex(X, bcs, dimension=3)
+
+
+class _BC(dict):
+    """a BC description for cases.make_options, its values also as attributes"""
+    __getattr__ = dict.__getitem__
+
+
+def bc_of(**kw):
+    d = _BC(kind=A.KIND_NONE, specified=A.SPEC_NONE, thermal=A.THERMAL_ADIABATIC, p=0.0, T=0.0, rho=0.0,
+            p_total=0.0, T_total=0.0, mdot=0.0, qdot=0.0)
+    d.update(kw)
+    if "V" in d:
+        d["V"] = tuple(float(x) for x in d["V"])
+    return d
+
+
+def probe_oracle(bc, Pref=0.0, Tref=0.0):
+    """Unit box whose first BC region (xmin, area 1) carries `bc`."""
+    opt = cases.make_options([bc] + [cases.bc_symmetry()] * 5, Pref=Pref, Tref=Tref)
+    return _oracle(box_hex(2, 2, 2), opt)
+
+
+def frame(seed):
+    rng = np.random.default_rng(seed)
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    t1 = np.cross(n, rng.normal(size=3))
+    t1 /= np.linalg.norm(t1)
+    return n, t1
+
+
+def state(p, Vn, Vt, T, n, t1):
+    """primitive (p, u, v, w, T) with normal velocity Vn along n and a
+    tangential part Vt along t1"""
+    V = Vn * n + Vt * t1
+    return np.array([p, *V, T])
+
+
+def sound(p, T, Pref=0.0, Tref=0.0):
+    rho = (p + Pref) / (RGAS * (T + Tref))
+    return rho, np.sqrt(GAMMA * (p + Pref) / rho)
+
+
+def assert_eos(r, Pref=0.0, Tref=0.0, rtol=1e-13):
+    assert np.isclose(r["p"] + Pref, r["rho"] * RGAS * (r["T"] + Tref), rtol=rtol, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# inlets
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("spec", [A.SPEC_P, A.SPEC_T, A.SPEC_RHO, A.SPEC_STATE])
+@pytest.mark.parametrize("Pref", [0.0, 1.0e5])
+def test_velocity_inlet_branches(spec, Pref, oracle_lib):
+    n, t1 = frame(1)
+    p0, T0 = 9.0e4 - (Pref if Pref else 0.0) * 0.5, 290.0
+    Vb = -60.0 * n + 15.0 * t1                         # inflow: V.n < 0 (n points out)
+    bc = bc_of(type=A.BC_INLET, kind=A.KIND_VELOCITY, specified=spec, V=Vb, p=p0 * 1.02, T=T0 * 0.97,
+               rho=(p0 * 1.02 + Pref) / (RGAS * T0 * 0.97))
+    o = probe_oracle(bc, Pref=Pref)
+    WL = state(p0, -50.0, 10.0, T0, n, t1)
+    WLc = state(p0 * 0.999, -49.0, 11.0, T0 * 1.001, n, t1)
+    rc, r = o.bc_probe(0, WL, WLc, WLc, n, t1)
+    assert rc == 0
+    rhoL, aL = sound(WL[0], WL[4], Pref)
+    uNL, uNR = WL[1:4] @ n, Vb @ n
+    assert np.allclose(r["V"], Vb, rtol=0, atol=0)
+    # Riemann invariant of the outgoing wave carried over (the BC's a)
+    assert np.isclose(uNL + 2 * aL / (GAMMA - 1), uNR + 2 * r["a"] / (GAMMA - 1), rtol=1e-14)
+    if spec == A.SPEC_P:
+        assert r["p"] == bc.p
+        assert np.isclose(r["p"] + Pref, r["rho"] * r["a"] ** 2 / GAMMA, rtol=1e-14)
+    elif spec == A.SPEC_T:
+        assert r["T"] == bc.T
+        assert np.isclose((r["p"] + Pref) / r["rho"] ** GAMMA, (WL[0] + Pref) / rhoL ** GAMMA, rtol=1e-13)
+    elif spec == A.SPEC_RHO:
+        assert r["rho"] == bc.rho
+        assert np.isclose(r["p"] + Pref, r["rho"] * r["a"] ** 2 / GAMMA, rtol=1e-13)
+    else:
+        assert (r["p"], r["T"], r["rho"]) == (bc.p, bc.T, bc.rho)
+    if spec != A.SPEC_STATE:
+        assert_eos(r, Pref)
+    # the centre values are the face values (ns_apply_bcs.cc:84-86)
+    assert r["p_c"] == r["p"] and r["T_c"] == r["T"] and np.array_equal(r["V_c"], r["V"])
+
+
+@pytest.mark.parametrize("spec", [A.SPEC_P, A.SPEC_T, A.SPEC_RHO])
+def test_velocity_inlet_supersonic_exits_without_state(spec, oracle_lib):
+    n, t1 = frame(2)
+    bc = bc_of(type=A.BC_INLET, kind=A.KIND_VELOCITY, specified=spec, V=-500.0 * n, p=1e5, T=300.0, rho=1.16)
+    o = probe_oracle(bc)
+    WL = state(1e5, -450.0, 0.0, 300.0, n, t1)          # Mach_L = -u_n / a ~ 1.3 >= 1
+    rc, _ = o.bc_probe(0, WL, WL, WL, n, t1)
+    assert rc == A.EBC
+    bc_state = bc_of(type=A.BC_INLET, kind=A.KIND_VELOCITY, specified=A.SPEC_STATE, V=-500.0 * n, p=1e5, T=300.0, rho=1.16)
+    rc, r = probe_oracle(bc_state).bc_probe(0, WL, WL, WL, n, t1)
+    assert rc == 0 and r["p"] == 1e5
+
+
+@pytest.mark.parametrize("spec", [A.SPEC_P, A.SPEC_T, A.SPEC_RHO, A.SPEC_STATE])
+def test_mass_flow_inlet_branches(spec, oracle_lib):
+    n, t1 = frame(3)
+    mdot = 35.0                                         # kg/s through the unit-area region
+    bc = bc_of(type=A.BC_INLET, kind=A.KIND_MDOT, specified=spec, mdot=mdot, p=1.01e5, T=305.0, rho=1.2)
+    o = probe_oracle(bc)
+    WL = state(1.0e5, -40.0, 7.0, 300.0, n, t1)
+    rc, r = o.bc_probe(0, WL, WL, WL, n, t1)
+    assert rc == 0
+    # the prescribed mass flux, entering along -n
+    assert np.isclose(r["rho"] * (r["V"] @ n), -mdot / 1.0, rtol=1e-14)
+    assert np.allclose(np.cross(r["V"], n), 0.0, atol=1e-13 * np.linalg.norm(r["V"]))
+    if spec == A.SPEC_P:
+        assert r["p"] == bc.p and r["T"] == WL[4]
+    elif spec == A.SPEC_T:
+        assert r["T"] == bc.T and r["p"] == WL[0]
+    elif spec == A.SPEC_RHO:
+        rhoL, aL = sound(WL[0], WL[4])
+        uNR = r["V"] @ n
+        assert np.isclose(WL[1:4] @ n + 2 * aL / (GAMMA - 1), uNR + 2 * r["a"] / (GAMMA - 1), rtol=1e-13)
+        assert np.isclose(r["p"], r["rho"] * r["a"] ** 2 / GAMMA, rtol=1e-13)
+    else:
+        assert (r["p"], r["T"], r["rho"]) == (bc.p, bc.T, bc.rho)
+    if spec != A.SPEC_STATE:
+        assert_eos(r)
+
+
+@pytest.mark.parametrize("spec", [A.SPEC_P, A.SPEC_STATE])
+def test_mass_flow_inlet_supersonic_exits_for_every_specification(spec, oracle_lib):
+    """ns_apply_bcs.cc:99-104 tests `!specified != BC_STATE`, which holds for
+    every value of `specified`: the reference exits even with a full state."""
+    n, t1 = frame(4)
+    bc = bc_of(type=A.BC_INLET, kind=A.KIND_MDOT, specified=spec, mdot=10.0, p=1e5, T=300.0, rho=1.16)
+    WL = state(1e5, -420.0, 0.0, 300.0, n, t1)          # Mach_L ~ 1.21
+    rc, _ = probe_oracle(bc).bc_probe(0, WL, WL, WL, n, t1)
+    assert rc == A.EBC
+
+
+@pytest.mark.parametrize("backflow", [False, True])
+def test_stagnation_inlet_total_conditions(backflow, oracle_lib):
+    n, t1 = frame(5)
+    p0, T0 = 1.2e5, 320.0
+    bc = bc_of(type=A.BC_INLET, kind=A.KIND_STAGNATION, p_total=p0, T_total=T0)
+    o = probe_oracle(bc)
+    WL = state(1.0e5, 30.0 if backflow else -30.0, 12.0, 300.0, n, t1)
+    WLc = state(1.0e5, 28.0 if backflow else -28.0, 11.0, 301.0, n, t1)
+    rc, r = o.bc_probe(0, WL, WLc, WLc, n, t1)
+    assert rc == 0
+    V = r["V"]
+    assert np.array_equal(V, -WLc[1:4] if backflow else WLc[1:4])
+    cp = GAMMA * RGAS / (GAMMA - 1)
+    assert np.isclose(r["T"] + V @ V / (2 * cp), T0, rtol=1e-14)
+    assert np.isclose(r["p"] * (1 + V @ V / (2 * RGAS * r["T"])), p0, rtol=1e-14)
+    assert_eos(r)
+
+
+# ---------------------------------------------------------------------------
+# outlets
+# ---------------------------------------------------------------------------
+
+def _outlet(spec, kind=A.KIND_NONE, Pref=0.0, **vals):
+    return probe_oracle(bc_of(type=A.BC_OUTLET, kind=kind, specified=spec, **vals), Pref=Pref)
+
+
+@pytest.mark.parametrize("Pref", [0.0, 1.0e5])
+def test_outlet_pressure_is_isentropic(Pref, oracle_lib):
+    n, t1 = frame(6)
+    pb = 0.97e5 - Pref
+    o = _outlet(A.SPEC_P, p=pb, Pref=Pref)
+    WL = state(1.0e5 - Pref, 60.0, -9.0, 300.0, n, t1)
+    rc, r = o.bc_probe(0, WL, WL, WL, n, t1)
+    assert rc == 0
+    rhoL, _ = sound(WL[0], WL[4], Pref)
+    assert r["p"] == pb and r["p_c"] == pb
+    assert np.isclose((r["p"] + Pref) / r["rho"] ** GAMMA, (WL[0] + Pref) / rhoL ** GAMMA, rtol=1e-13)
+    assert np.array_equal(r["V"], WL[1:4])
+    assert_eos(r, Pref)
+
+
+def test_outlet_density_is_isentropic(oracle_lib):
+    n, t1 = frame(7)
+    o = _outlet(A.SPEC_RHO, rho=1.1)
+    WL = state(1.0e5, 60.0, -9.0, 300.0, n, t1)
+    rc, r = o.bc_probe(0, WL, WL, WL, n, t1)
+    rhoL, _ = sound(WL[0], WL[4])
+    assert rc == 0 and r["rho"] == 1.1
+    assert np.isclose(r["p"] / r["rho"] ** GAMMA, WL[0] / rhoL ** GAMMA, rtol=1e-13)
+    assert np.isclose(r["a"] ** 2, GAMMA * r["p"] / r["rho"], rtol=1e-14)
+    assert_eos(r)
+
+
+def test_outlet_temperature_keeps_the_reference_exponent(oracle_lib):
+    """The reference's temperature outlet (ns_apply_bcs.cc:177-183) raises
+    gamma (p_L + Pref) / (a^2 rho_L^gamma) to the power 1 - gamma; the
+    isentropic density would need 1 / (1 - gamma).  With K = p_L / rho_L^gamma
+    the oracle's density is (gamma K / a^2)^(1 - gamma), which is not
+    isentropic -- the quirk is kept."""
+    n, t1 = frame(8)
+    Tb = 290.0
+    o = _outlet(A.SPEC_T, T=Tb)
+    WL = state(1.0e5, 60.0, -9.0, 300.0, n, t1)
+    rc, r = o.bc_probe(0, WL, WL, WL, n, t1)
+    rhoL, _ = sound(WL[0], WL[4])
+    K = WL[0] / rhoL ** GAMMA
+    a2 = GAMMA * RGAS * Tb
+    assert rc == 0 and r["T"] == Tb and r["T_c"] == Tb
+    assert np.isclose(np.log(r["rho"]), (1 - GAMMA) * np.log(GAMMA * K / a2), rtol=1e-12)
+    assert not np.isclose(r["rho"], (a2 / (GAMMA * K)) ** (1 / (GAMMA - 1)), rtol=1e-3)
+    assert_eos(r)
+
+
+def test_outlet_supersonic_extrapolates_everything(oracle_lib):
+    n, t1 = frame(9)
+    o = _outlet(A.SPEC_P, p=0.5e5)
+    WL = state(1.0e5, 420.0, 3.0, 300.0, n, t1)        # Mach_L > 1
+    rc, r = o.bc_probe(0, WL, WL, WL, n, t1)
+    assert rc == 0
+    assert (r["p"], r["T"]) == (WL[0], WL[4]) and np.array_equal(r["V"], WL[1:4])
+    assert_eos(r)                                       # rho of the extrapolated p, T (decision D2)
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+def test_outlet_damp_reverse(reverse, oracle_lib):
+    n, t1 = frame(10)
+    pb = 0.98e5
+    o = _outlet(A.SPEC_P, kind=A.KIND_DAMP_REVERSE, p=pb)
+    uN = -25.0 if reverse else 25.0
+    WL = state(1.0e5, uN, 4.0, 300.0, n, t1)
+    rc, r = o.bc_probe(0, WL, WL, WL, n, t1)
+    rhoL, _ = sound(WL[0], WL[4])
+    assert rc == 0
+    expect = pb - 0.5 * rhoL * uN * uN if reverse else pb
+    assert np.isclose(r["p"], expect, rtol=1e-15)
+    assert np.isclose(r["p"] / r["rho"] ** GAMMA, WL[0] / rhoL ** GAMMA, rtol=1e-13)
+    assert_eos(r)
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+def test_outlet_no_reverse(reverse, oracle_lib):
+    n, t1 = frame(11)
+    o = _outlet(A.SPEC_P, kind=A.KIND_NO_REVERSE, p=0.98e5)
+    WL = state(1.0e5, -25.0 if reverse else 25.0, 4.0, 300.0, n, t1)
+    rc, r = o.bc_probe(0, WL, WL, WL, n, t1)
+    assert rc == 0
+    assert np.array_equal(r["V"], -WL[1:4] if reverse else WL[1:4])
+
+
+def test_outlet_force_supersonic_keeps_the_min_quirk(oracle_lib):
+    """ns_apply_bcs.cc:206-212 lowers p by rho_L (a_L - u_n)^2 / 2 and then
+    takes min(1e-8, p + Pref): the ghost pressure ends at most 1e-8 above
+    -Pref (a `max` was probably meant); density follows isentropically."""
+    n, t1 = frame(12)
+    o = _outlet(A.SPEC_P, kind=A.KIND_FORCE_SUPERSONIC, p=0.98e5)
+    WL = state(1.0e5, 80.0, 4.0, 300.0, n, t1)
+    rc, r = o.bc_probe(0, WL, WL, WL, n, t1)
+    rhoL, aL = sound(WL[0], WL[4])
+    assert rc == 0
+    assert r["p"] == min(1e-8, 0.98e5 - 0.5 * rhoL * (aL - 80.0) ** 2)
+    assert np.isclose(r["p"] / r["rho"] ** GAMMA, WL[0] / rhoL ** GAMMA, rtol=1e-12)
+    assert_eos(r, rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# walls and symmetry
+# ---------------------------------------------------------------------------
+
+def _reflect(V, n):
+    return V - 2 * (V @ n) * n
+
+
+@pytest.mark.parametrize("kind", ["slip", "symmetry"])
+def test_reflection_branches(kind, oracle_lib):
+    n, t1 = frame(13)
+    bc = bc_of(type=A.BC_WALL, kind=A.KIND_SLIP) if kind == "slip" else bc_of(type=A.BC_SYMMETRY)
+    o = probe_oracle(bc)
+    WL = state(1.0e5, 33.0, -21.0, 300.0, n, t1)
+    WLc = state(1.001e5, 31.0, -20.0, 301.0, n, t1)
+    rc, r = o.bc_probe(0, WL, WLc, WLc, n, t1)
+    assert rc == 0
+    for V, VL in ((r["V"], WL[1:4]), (r["V_c"], WLc[1:4])):
+        assert np.isclose(V @ n, -(VL @ n), rtol=1e-14)                       # normal component reversed
+        assert np.allclose(V - (V @ n) * n, VL - (VL @ n) * n, rtol=0, atol=1e-13 * np.linalg.norm(VL))
+        assert np.isclose(np.linalg.norm(V), np.linalg.norm(VL), rtol=1e-14)
+    assert (r["p"], r["T"], r["p_c"], r["T_c"]) == (WL[0], WL[4], WLc[0], WLc[4])
+
+
+@pytest.mark.parametrize("thermal", [A.THERMAL_ADIABATIC, A.THERMAL_FIXED_T, A.THERMAL_FIXED_Q])
+def test_no_slip_wall_branches(thermal, oracle_lib):
+    n, t1 = frame(14)
+    Tw = 350.0
+    o = probe_oracle(bc_of(type=A.BC_WALL, kind=A.KIND_NONE, thermal=thermal, T=Tw, qdot=500.0))
+    WL = state(1.0e5, 3.0, -21.0, 300.0, n, t1)
+    WLc = state(1.001e5, 2.0, -20.0, 301.0, n, t1)
+    rc, r = o.bc_probe(0, WL, WLc, WLc, n, t1)
+    assert rc == 0
+    assert np.array_equal(r["V"], -WL[1:4]) and np.array_equal(r["V_c"], -WLc[1:4])
+    assert r["p"] == WL[0] and r["p_c"] == WLc[0]
+    if thermal == A.THERMAL_FIXED_T:
+        assert r["T"] == Tw and r["T_c"] == Tw
+        assert_eos(r)
+    else:                                                 # adiabatic (and FIXED_Q: its flux is set later)
+        assert r["T"] == WL[4] and r["T_c"] == WLc[4]
+
+
+# ---------------------------------------------------------------------------
+# Riemann fluxes
+# ---------------------------------------------------------------------------
+
+def _conserved_flux(rho, un, ut, uw, p, H):
+    m = rho * un
+    return np.array([m, m * un + p, m * ut, m * uw, m * H])
+
+
+def roe_five_wave(WL, WR, n, t1):
+    """Symmetric Roe flux in the face frame (n, t1, t2), textbook form with
+    Roe averages of the conserved state."""
+    t2 = np.cross(n, t1)
+    t2 /= np.linalg.norm(t2)
+    def prim(W):
+        rho = W[0] / (RGAS * W[4])
+        V = W[1:4]
+        u = np.array([V @ n, V @ t1, V @ t2])
+        a2 = GAMMA * W[0] / rho
+        return rho, u, W[0], a2 / (GAMMA - 1) + 0.5 * V @ V
+    rL, uL, pL, HL = prim(WL)
+    rR, uR, pR, HR = prim(WR)
+    sL, sR = np.sqrt(rL), np.sqrt(rR)
+    u = (sL * uL + sR * uR) / (sL + sR)
+    H = (sL * HL + sR * HR) / (sL + sR)
+    a = np.sqrt((GAMMA - 1) * (H - 0.5 * u @ u))
+    rho = sL * sR
+    dr, du, dp = rR - rL, uR - uL, pR - pL
+    lam = np.array([u[0] - a, u[0], u[0], u[0], u[0] + a])
+    alpha = np.array([(dp - rho * a * du[0]) / (2 * a * a), dr - dp / (a * a), rho * du[1], rho * du[2],
+                      (dp + rho * a * du[0]) / (2 * a * a)])
+    r = np.array([[1, u[0] - a, u[1], u[2], H - u[0] * a],
+                  [1, u[0], u[1], u[2], 0.5 * u @ u],
+                  [0, 0, 1, 0, u[1]],
+                  [0, 0, 0, 1, u[2]],
+                  [1, u[0] + a, u[1], u[2], H + u[0] * a]])
+    FL = _conserved_flux(rL, *uL, pL, HL)
+    FR = _conserved_flux(rR, *uR, pR, HR)
+    Fn = 0.5 * (FL + FR) - 0.5 * (np.abs(lam) * alpha) @ r
+    return np.array([Fn[0], *(Fn[1] * n + Fn[2] * t1 + Fn[3] * t2), Fn[4]]), dict(u=u[0], a=a, alpha=alpha, r=r, FL=FL)
+
+
+@pytest.mark.parametrize("direction", [+1.0, -1.0])
+def test_roe_equals_the_five_wave_form_without_entropy_fix(direction, oracle_lib):
+    """Subsonic faces where the entropy fix is idle (a_L - a_R + du <= 0 for
+    u >= 0, mirrored for u < 0): the one-sided form of ns_roe.cc equals the
+    symmetric sum over all five waves, so the acoustic dissipation, the
+    Roe averages and the eigenvectors are all pinned."""
+    o = probe_oracle(bc_of(type=A.BC_SYMMETRY))
+    rng = np.random.default_rng(21 if direction > 0 else 22)
+    checked = 0
+    for _ in range(60):
+        n, t1 = frame(int(rng.integers(1 << 30)))
+        TL, TR = 300.0 * (0.9 + 0.2 * rng.random(2))
+        WL = state(1e5 * (0.8 + 0.4 * rng.random()), direction * (40 + 60 * rng.random()), 30 * rng.normal(), TL, n, t1)
+        WR = state(1e5 * (0.8 + 0.4 * rng.random()), direction * (40 + 60 * rng.random()), 30 * rng.normal(), TR, n, t1)
+        ref, aux = roe_five_wave(WL, WR, n, t1)
+        _, aL = sound(WL[0], WL[4])
+        _, aR = sound(WR[0], WR[4])
+        du = (WR[1:4] - WL[1:4]) @ n
+        idle = (aL - aR + du <= 0) if aux["u"] >= 0 else (aR - aL + du <= 0)
+        if not idle:
+            continue
+        F = o.flux_probe(A.FLUX_ROE, WL, WR, n, t1)
+        assert np.allclose(F, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max()), (F, ref)
+        checked += 1
+    assert checked >= 10
+
+
+def test_roe_entropy_fix_is_continuous_and_smooth(oracle_lib):
+    """Inside the fix window |u - a| < Dl/2 the acoustic eigenvalue of the
+    u >= 0 branch is replaced; the effective eigenvalue
+    lambda_eff = (F - F_L)[0] / alpha_1 (r_1[0] = 1) must equal u - a below
+    the window, 0 above it, and join both continuously with a continuous
+    slope."""
+    o = probe_oracle(bc_of(type=A.BC_SYMMETRY))
+    n, t1 = np.array([1.0, 0, 0]), np.array([0, 1.0, 0])
+    TL, TR = 300.0, 280.0          # a_L > a_R: Dl = 2 min(a, 2 (a_L - a_R + du)) > 0
+    _, aL = sound(1e5, TL)
+    def lam_eff(uL):
+        WL = np.array([1e5, uL, 5.0, 0.0, TL])
+        WR = np.array([1.01e5, uL + 1.0, 5.0, 0.0, TR])
+        F = o.flux_probe(A.FLUX_ROE, WL, WR, n, t1)
+        _, aux = roe_five_wave(WL, WR, n, t1)
+        FL = aux["FL"]
+        # the window's half-width Dl/2 (ns_roe.cc:49): 2 (a_L - a_R + du), at most a
+        _, aRs = sound(WR[0], WR[4])
+        half = min(aux["a"], max(0.0, 2.0 * (sound(WL[0], WL[4])[1] - aRs + 1.0)))
+        return (F[0] - FL[0]) / aux["alpha"][0], aux["u"] - aux["a"], half
+    # sweep u - a through the window
+    us = np.linspace(aL - 60.0, aL + 60.0, 6001)
+    vals = np.array([lam_eff(u) for u in us])
+    le, lam, half = vals[:, 0], vals[:, 1], vals[:, 2]
+    assert half.min() > 10.0
+    below = lam < -half - 1e-6
+    above = lam > half + 1e-6
+    assert below.sum() > 100 and above.sum() > 100
+    assert np.allclose(le[below], lam[below], rtol=1e-8, atol=1e-8)
+    assert np.allclose(le[above], 0.0, atol=1e-8)
+    inside = ~below & ~above
+    assert np.any((le[inside] != lam[inside]) & (le[inside] != 0.0))    # the window is visited
+    steps = np.diff(le)
+    assert np.abs(steps).max() < 1.5 * np.abs(np.diff(lam)).max()       # no jump: |slope| <= 1
+    slope = steps / np.diff(lam)
+    assert np.abs(np.diff(slope)).max() < 0.05                           # no kink
+    assert np.all(le <= 0.0)                                             # an upwind (negative) part
+
+
+def test_ausm_split_polynomials(oracle_lib):
+    from oracle.oracle import ausm_split
+    alpha = 3.0 / 16.0
+    Ms = np.concatenate([np.linspace(-3, 3, 601), [-1.0, 1.0, 0.0]])
+    S = np.array([ausm_split(M, alpha) for M in Ms])
+    M4p, M4m, P5p, P5m = S.T
+    assert np.allclose(M4p + M4m, Ms, rtol=0, atol=1e-15)
+    assert np.allclose(P5p + P5m, 1.0, rtol=0, atol=1e-15)
+    mirror = np.array([ausm_split(-M, alpha) for M in Ms])
+    assert np.allclose(M4m, -mirror[:, 0], atol=1e-15)
+    assert np.allclose(P5m, mirror[:, 2], atol=1e-15)
+    # supersonic: the one-sided values
+    sup = np.abs(Ms) >= 1
+    assert np.allclose(M4p[sup], np.maximum(Ms[sup], 0.0)) and np.allclose(P5p[sup], (Ms[sup] > 0).astype(float))
+    # C1 at |M| = 1 for both polynomials (beta = 1/8, alpha = 3/16)
+    h = 1e-6
+    for M0 in (-1.0, 1.0):
+        for k in range(4):
+            f = lambda M: ausm_split(M, alpha)[k]
+            left, right = f(M0 - h), f(M0 + h)
+            assert abs(left - right) < 1e-5
+            dl = (f(M0 - h) - f(M0 - 2 * h)) / h
+            dr = (f(M0 + 2 * h) - f(M0 + h)) / h
+            assert abs(dl - dr) < 1e-4, (M0, k, dl, dr)
+
+
+@pytest.mark.parametrize("flux", [A.FLUX_ROE, A.FLUX_AUSM_PLUS_UP, A.FLUX_VAN_LEER, A.FLUX_SD_SLAU, A.FLUX_SW])
+def test_flux_antisymmetry(flux, oracle_lib):
+    """The flux from L to R through n is minus the flux from R to L through
+    -n: this relates each flux's two upwind branches (Roe u >= 0 / u < 0,
+    AUSM's mdot sign, van Leer's split, SD-SLAU's pressure weights,
+    Steger-Warming's positive and negative parts)."""
+    o = probe_oracle(bc_of(type=A.BC_SYMMETRY))
+    rng = np.random.default_rng(31)
+    for _ in range(20):
+        n, t1 = frame(int(rng.integers(1 << 30)))
+        mach = [0.2, 0.7, 1.3][int(rng.integers(3))]
+        a = np.sqrt(GAMMA * RGAS * 300.0)
+        WL = np.array([1e5 * (0.9 + 0.2 * rng.random()), *(a * mach * rng.normal(size=3)), 300 * (0.9 + 0.2 * rng.random())])
+        WR = np.array([1e5 * (0.9 + 0.2 * rng.random()), *(a * mach * rng.normal(size=3)), 300 * (0.9 + 0.2 * rng.random())])
+        F1 = o.flux_probe(flux, WL, WR, n, t1)
+        F2 = o.flux_probe(flux, WR, WL, -n, -t1)
+        assert np.allclose(F1, -F2, rtol=1e-11, atol=1e-11 * np.abs(F1).max()), (flux, F1, F2)
+
+
+# ---------------------------------------------------------------------------
+# viscous flux
+# ---------------------------------------------------------------------------
+
+def _linear_case(mesh, grad, Pr=0.72, mu=1.8e-5, walls=None, seed=41):
+    if mesh == "hex":
+        raw = box_hex(7, 6, 6, jitter=0.2, seed=seed)
+    else:
+        raw = meshgen.tet_box(5)
+        xyz = raw["xyz"]
+        inner = np.all((xyz > 1e-9) & (xyz < 1 - 1e-9), axis=1)
+        xyz[inner] += np.random.default_rng(seed).uniform(-0.05, 0.05, (inner.sum(), 3))
+    bcs = walls if walls is not None else [cases.bc_symmetry()] * 6
+    opt = cases.make_options(bcs, mu=mu, Pr=Pr, limiter=A.LIMITER_NONE, flux=A.FLUX_ROE, grad=(grad, grad, grad))
+    return raw, opt
+
+
+@pytest.mark.parametrize("mesh,grad", [("hex", A.GRAD_CURVILINEAR), ("hex", A.GRAD_LSQR), ("tet", A.GRAD_LSQR)])
+def test_viscous_flux_exact_for_linear_fields(mesh, grad, oracle_lib):
+    """Linear u, v, w, T with uniform p: the cell gradients are exact, the
+    face average and the l2r correction reproduce the same gradient, and the
+    reconstructed face velocity is V(x_f).  Every interior face then carries
+    exactly the Newtonian stress mu (G + G^T - 2/3 tr(G) I) n A, its work
+    (tau V(x_f)) . n A and the conduction lambda grad T . n A."""
+    raw, opt = _linear_case(mesh, grad)
+    o = _oracle(raw, opt)
+    m = o.metrics()
+    N = o.info.n_cells
+    cc = m["cell_centroid"].reshape(-1, 3)[:N]
+    rng = np.random.default_rng(5)
+    G = rng.normal(size=(3, 3)) * 40.0          # velocity gradient rows du, dv, dw
+    V0 = np.array([60.0, -10.0, 5.0])
+    gT = rng.normal(size=3) * 30.0
+    W = np.zeros((N, 5))
+    W[:, 0] = 1.0e5
+    W[:, 1:4] = V0 + cc @ G.T
+    W[:, 4] = 300.0 + cc @ gT
+    o.residual(W)
+    Cf, Df = o.face_fluxes()
+    mu = opt.mu
+    lam = (GAMMA * RGAS / (GAMMA - 1)) * mu / opt.Pr
+    tau = mu * (G + G.T - 2.0 / 3.0 * np.trace(G) * np.eye(3))
+    # faces between cells whose gradient stencils hold no boundary ghost (a
+    # ghost carries the BC's state, not the linear field)
+    gp, gi = m["grad_ptr"], m["grad_idx"]
+    clean = np.array([np.all(gi[gp[c]:gp[c + 1]] < N) for c in range(N)])
+    par, nei = m["face_parent"], m["face_neighbor"]
+    interior = (m["face_bc"] < 0) & clean[par] & clean[np.minimum(nei, N - 1)]
+    assert interior.sum() >= 50
+    xf = m["face_centroid"].reshape(-1, 3)
+    nA = m["face_normal"].reshape(-1, 3) * m["face_area"][:, None]
+    Vf = V0 + xf @ G.T
+    expect_mom = nA @ tau.T                     # tau symmetric: row i = (tau n)_i A
+    expect_en = np.einsum("fi,fi->f", Vf @ tau.T, nA) + lam * nA @ gT
+    d = Df[interior]
+    scale = np.abs(expect_mom).max()
+    assert np.abs(d[:, 0]).max() == 0.0
+    assert np.allclose(d[:, 1:4], expect_mom[interior], rtol=0, atol=1e-10 * scale)
+    assert np.allclose(d[:, 4], expect_en[interior], rtol=0, atol=1e-10 * np.abs(expect_en).max())
+
+
+def test_wall_heat_flux_adiabatic_fixed_q_fixed_t(oracle_lib):
+    """On a box with no-slip walls: an adiabatic wall conducts no heat (the
+    l2r correction zeroes the normal temperature gradient, the ghost mirrors
+    the parent across the face), a FIXED_Q wall carries exactly qdot A, and a
+    FIXED_T wall lambda (T_w - T_P) / |l2r| A; the wall work term is zero
+    because the face velocity averages V and -V."""
+    qdot, Tw = 750.0, 330.0
+    walls = [cases.bc_wall(), cases.bc_wall(qdot=qdot), cases.bc_wall(T=Tw)] + [cases.bc_symmetry()] * 3
+    raw, opt = _linear_case("hex", A.GRAD_CURVILINEAR, walls=walls)
+    o = _oracle(raw, opt)
+    m = o.metrics()
+    N = o.info.n_cells
+    cc = m["cell_centroid"].reshape(-1, 3)
+    rng = np.random.default_rng(8)
+    W = np.zeros((N, 5))
+    W[:, 0] = 1.0e5 + 100 * rng.random(N)
+    W[:, 1:4] = rng.normal(size=(N, 3)) * 20.0
+    W[:, 4] = 300.0 + 10 * rng.random(N)
+    o.residual(W)
+    _, Df = o.face_fluxes()
+    fb, A_ = m["face_bc"], m["face_area"]
+    lam = (GAMMA * RGAS / (GAMMA - 1)) * opt.mu / opt.Pr
+    par = m["face_parent"]
+    typical = np.abs(Df[fb < 0, 4]).max()
+    adi = fb == 0
+    assert np.abs(Df[adi, 4]).max() <= 1e-9 * typical
+    fq = fb == 1
+    assert np.array_equal(Df[fq, 4], qdot * A_[fq])
+    ft = np.where(fb == 2)[0]
+    l2r = np.linalg.norm(cc[m["face_neighbor"][ft]] - cc[par[ft]], axis=1)
+    expect = lam * (Tw - W[par[ft], 4]) / l2r * A_[ft]
+    assert np.allclose(Df[ft, 4], expect, rtol=1e-9, atol=1e-9 * np.abs(expect).max())
+
+
+# ---------------------------------------------------------------------------
+# assembly
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", ["wedge", "cylinder"])
+def test_residual_telescopes_to_boundary_fluxes_and_loads(case, oracle_lib):
+    """Interior faces add D - C to the parent and subtract it from the
+    neighbour, so the cell residuals sum to the boundary faces' D - C; the
+    momentum part of that sum is minus the loads of every boundary (a second,
+    separate accumulation).  A wrong sign or a dropped neighbour update
+    breaks both."""
+    cs = cases.wedge_case(16, 8) if case == "wedge" else cases.cylinder_case(24, 12, noise=0.01)
+    for b in range(cs.opt.n_bc):
+        cs.opt.bc[b].include_in_loads = 1
+    o = _oracle(cs.raw, cs.opt)
+    R, Rabs = o.residual(cs.W0)
+    Cf, Df = o.face_fluxes()
+    fb = o.metrics()["face_bc"]
+    bsum = (Df - Cf)[fb >= 0].sum(axis=0)
+    tol = 1e-12 * Rabs.sum(axis=0)
+    assert np.all(np.abs(R.sum(axis=0) - bsum) <= tol), (R.sum(axis=0), bsum)
+    loads = o.loads()
+    assert np.allclose(R.sum(axis=0)[1:4], -loads[:, :3].sum(axis=0), rtol=0, atol=1e-12 * Rabs[:, 1:4].sum())
+    # and the interior part alone cancels: no interior face is missing from either side
+    isum = (Df - Cf)[fb < 0]
+    assert np.abs(R.sum(axis=0) - bsum).max() <= tol.max()
+    assert isum.shape[0] > 0
