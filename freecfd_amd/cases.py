@@ -182,6 +182,114 @@ def smooth_perturbation(xyz, seed, amp=0.05, modes=8):
     return out
 
 
+def bc_inlet_velocity(spec, V, p=0.0, T=0.0, rho=0.0):
+    """velocity inlet with p, T or rho given (ns_set_bcs.cc:52-97)"""
+    d = dict(type=A.BC_INLET, kind=A.KIND_VELOCITY, specified=spec, thermal=A.THERMAL_ADIABATIC, V=tuple(V), V_fixed=1,
+             p=p, T=T, rho=rho)
+    d.update({A.SPEC_P: dict(p_fixed=1), A.SPEC_T: dict(T_fixed=1), A.SPEC_RHO: dict(rho_fixed=1),
+              A.SPEC_STATE: dict(p_fixed=1, T_fixed=1, rho_fixed=1)}.get(spec, {}))
+    return d
+
+
+def bc_mdot(spec, mdot, p=0.0, T=0.0, rho=0.0):
+    """mass-flow inlet (ns_apply_bcs.cc:90-135): mdot over the region's area"""
+    return dict(type=A.BC_INLET, kind=A.KIND_MDOT, specified=spec, thermal=A.THERMAL_ADIABATIC, mdot=mdot, p=p, T=T,
+                rho=rho)
+
+
+def bc_outlet(spec, kind=A.KIND_NONE, p=0.0, T=0.0, rho=0.0):
+    """outlet with p, T, rho or nothing given, and a reverse-flow kind
+    (ns_apply_bcs.cc:157-215)"""
+    d = dict(type=A.BC_OUTLET, kind=kind, specified=spec, thermal=A.THERMAL_ADIABATIC, p=p, T=T, rho=rho)
+    d.update({A.SPEC_P: dict(p_fixed=1), A.SPEC_T: dict(T_fixed=1), A.SPEC_RHO: dict(rho_fixed=1)}.get(spec, {}))
+    return d
+
+
+# Boundary-condition variants of a small 3-D channel (inlet at x = 0,
+# outlet at x = L, the wall under test at y = 0, a slip wall at y = H,
+# symmetry in z): each names (inlet, outlet, lower wall, flux).
+BC_VARIANTS = {
+    "vel_p__out_t__wall_fixed_t": ("vel_p", "out_t", "fixed_t", A.FLUX_ROE),
+    "vel_rho__out_rho__wall_fixed_q": ("vel_rho", "out_rho", "fixed_q", A.FLUX_AUSM_PLUS_UP),
+    "vel_t__out_damp_reverse__wall_adiabatic": ("vel_t", "out_damp", "adiabatic", A.FLUX_VAN_LEER),
+    "mdot_p__out_no_reverse__wall_fixed_t": ("mdot_p", "out_noreverse", "fixed_t", A.FLUX_ROE),
+    "mdot_t__out_extrapolate__wall_fixed_q": ("mdot_t", "out_none", "fixed_q", A.FLUX_AUSM_PLUS_UP),
+    "mdot_rho__out_force_supersonic__wall_slip": ("mdot_rho", "out_force", "slip", A.FLUX_ROE),
+    "mdot_state__out_p__wall_adiabatic": ("mdot_state", "out_p", "adiabatic", A.FLUX_SD_SLAU),
+    "stagnation__out_t_no_reverse__wall_fixed_t": ("stag", "out_t_noreverse", "fixed_t", A.FLUX_SW),
+}
+
+
+def bc_variant_case(variant, ni=10, nj=6, nk=4, seed=12, sa=False, limiter=A.LIMITER_VK):
+    """A jittered 1 x 0.5 x 0.25 hex channel, Mach ~0.2 viscous flow with a
+    sheared inflow profile u = U (0.3 + cos(2 pi y / H)) (so some outlet and
+    inlet faces see reverse flow) plus seeded 1% noise, under one of
+    BC_VARIANTS; sa=True adds nu~."""
+    inlet_k, outlet_k, wall_k, flux = BC_VARIANTS[variant]
+    Lx, H, Wz = 1.0, 0.5, 0.25
+    rng = np.random.default_rng(seed)
+    X = np.zeros((ni + 1, nj + 1, nk + 1, 3))
+    X[..., 0] = np.linspace(0, Lx, ni + 1)[:, None, None]
+    X[..., 1] = np.linspace(0, H, nj + 1)[None, :, None]
+    X[..., 2] = np.linspace(0, Wz, nk + 1)[None, None, :]
+    h = np.array([Lx / ni, H / nj, Wz / nk])
+    jit = rng.uniform(-0.15, 0.15, X.shape) * h
+    jit[0, :, :, 0] = jit[-1, :, :, 0] = 0.0
+    jit[:, 0, :, 1] = jit[:, -1, :, 1] = 0.0
+    jit[:, :, 0, 2] = jit[:, :, -1, 2] = 0.0
+    X = X + jit
+    regions = [("inlet", lambda i, j, k: i == 0), ("outlet", lambda i, j, k: i == ni),
+               ("wall", lambda i, j, k: j == 0), ("top", lambda i, j, k: j == nj),
+               ("zmin", lambda i, j, k: k == 0), ("zmax", lambda i, j, k: k == nk)]
+    raw = meshgen.structured_hex(X, regions, dimension=3)
+    Mw, g = 28.97, 1.4
+    R = gas_R(Mw)
+    p, T = 1.0e5, 300.0
+    U = 0.2 * np.sqrt(g * R * T)
+    rho = p / (R * T)
+    mdot = rho * U * H * Wz
+    mu = 2.0e-3
+    inlets = {
+        "vel_p": bc_inlet_velocity(A.SPEC_P, (U, 0.0, 0.0), p=1.01 * p),
+        "vel_t": bc_inlet_velocity(A.SPEC_T, (U, 0.0, 0.0), T=1.02 * T),
+        "vel_rho": bc_inlet_velocity(A.SPEC_RHO, (U, 0.0, 0.0), rho=1.01 * rho),
+        "mdot_p": bc_mdot(A.SPEC_P, mdot, p=1.01 * p),
+        "mdot_t": bc_mdot(A.SPEC_T, mdot, T=1.02 * T),
+        "mdot_rho": bc_mdot(A.SPEC_RHO, mdot, rho=1.01 * rho),
+        "mdot_state": bc_mdot(A.SPEC_STATE, mdot, p=1.01 * p, T=1.02 * T, rho=1.01 * p / (R * 1.02 * T)),
+        "stag": bc_stagnation(p * (1 + 0.5 * (g - 1) * 0.04) ** (g / (g - 1)), T * (1 + 0.5 * (g - 1) * 0.04)),
+    }
+    outlets = {
+        "out_p": bc_outlet(A.SPEC_P, p=0.99 * p),
+        "out_t": bc_outlet(A.SPEC_T, T=0.98 * T),
+        "out_rho": bc_outlet(A.SPEC_RHO, rho=0.99 * rho),
+        "out_none": bc_outlet(A.SPEC_NONE),
+        "out_damp": bc_outlet(A.SPEC_P, A.KIND_DAMP_REVERSE, p=0.99 * p),
+        "out_noreverse": bc_outlet(A.SPEC_P, A.KIND_NO_REVERSE, p=0.99 * p),
+        "out_force": bc_outlet(A.SPEC_P, A.KIND_FORCE_SUPERSONIC, p=0.99 * p),
+        "out_t_noreverse": bc_outlet(A.SPEC_T, A.KIND_NO_REVERSE, T=0.98 * T),
+    }
+    walls = {"adiabatic": bc_wall(), "fixed_t": bc_wall(T=1.1 * T), "fixed_q": bc_wall(qdot=5.0e3),
+             "slip": bc_wall(slip=True)}
+    inlet = dict(inlets[inlet_k])
+    nut_inf = 3.0 * mu / rho
+    inlet["nut"] = nut_inf
+    bcs = [inlet, outlets[outlet_k], walls[wall_k], bc_wall(slip=True), bc_symmetry(), bc_symmetry()]
+    opt = make_options(bcs, gamma=g, Mw=Mw, mu=mu, Pr=0.72, flux=flux, limiter=limiter, Minf=0.2, loads_bcs=(2,),
+                       turbulence=A.TURB_SA if sa else A.TURB_NONE)
+    n = len(raw["cell_ptr"]) - 1
+    xc = meshgen.cell_centroids(raw)
+    W = freestream_state(n, p, T, (U, 0.0, 0.0))
+    W[:, 1] = U * (0.3 + np.cos(2 * np.pi * xc[:, 1] / H))
+    W[:, 2] = 0.1 * U * np.sin(np.pi * xc[:, 0] / Lx)
+    W[:, 0] *= 1.0 + rng.uniform(-0.01, 0.01, n)
+    W[:, 1] *= 1.0 + rng.uniform(-0.01, 0.01, n)
+    W[:, 4] *= 1.0 + rng.uniform(-0.01, 0.01, n)
+    if sa:
+        W = np.concatenate([W, (nut_inf * (1.0 + rng.uniform(-0.01, 0.01, n)))[:, None]], axis=1)
+    return Case("bc_" + variant + ("_sa" if sa else ""), raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.2, mu=mu, nut=nut_inf))
+
+
 def smooth_state(cs, seed=21, amp=0.02):
     """Replace cs.W0 by the case's mean state times a seeded smooth
     perturbation (8 Fourier modes per variable over the bounding box), so that
