@@ -59,7 +59,8 @@ class Options(C.Structure):
 class Info(C.Structure):
     _fields_ = [("n_nodes", C.c_int), ("n_cells", C.c_int), ("n_ghosts", C.c_int), ("n_faces", C.c_int),
                 ("n_bfaces", C.c_int), ("ld_cell", C.c_int), ("ld_face", C.c_int), ("n_gradmap", C.c_int),
-                ("n_faceavg", C.c_int), ("n_nbr", C.c_int), ("n_halffaces", C.c_int), ("nv", C.c_int)]
+                ("n_faceavg", C.c_int), ("n_nbr", C.c_int), ("n_halffaces", C.c_int), ("nv", C.c_int),
+                ("n_nodeavg", C.c_int)]
 
 
 JAC_FD, JAC_JVP_FROZEN, JAC_JVP_FULL = 1, 2, 3
@@ -97,7 +98,8 @@ class MetricsOut(C.Structure):
                 ("cell_face_ptr", _ip), ("cell_faces", _ip), ("nbr_ptr", _ip), ("nbr", _ip),
                 ("grad_ptr", _ip), ("grad_idx", _ip), ("grad_w", _dp),
                 ("avg_ptr", _ip), ("avg_idx", _ip), ("avg_w", _dp),
-                ("cell_perm", _ip), ("grid_lengthscale", _dp)]
+                ("cell_perm", _ip), ("grid_lengthscale", _dp),
+                ("node_xyz", _dp), ("node_avg_ptr", _ip), ("node_avg_idx", _ip), ("node_avg_w", _dp)]
 
 
 def dptr(a):
@@ -139,7 +141,9 @@ def alloc_metrics(info, full=True):
         grad_w=np.zeros(3 * info.n_gradmap),
         avg_ptr=np.zeros(F + 1, np.int32), avg_idx=np.zeros(info.n_faceavg, np.int32),
         avg_w=np.zeros(info.n_faceavg),
-        cell_perm=np.zeros(T, np.int32), grid_lengthscale=np.zeros(1))
+        cell_perm=np.zeros(T, np.int32), grid_lengthscale=np.zeros(1),
+        node_xyz=np.zeros(3 * info.n_nodes), node_avg_ptr=np.zeros(info.n_nodes + 1, np.int32),
+        node_avg_idx=np.zeros(max(info.n_nodeavg, 1), np.int32), node_avg_w=np.zeros(max(info.n_nodeavg, 1)))
     m = MetricsOut()
     for k, a in arrs.items():
         setattr(m, k, iptr(a) if a.dtype == np.int32 else dptr(a))

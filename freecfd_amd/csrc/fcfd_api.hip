@@ -485,6 +485,7 @@ int fcfd_get_info(const fcfd_ctx* c, fcfd_info* info) {
     info->n_nbr = (int)g.nb.size();
     info->n_halffaces = (int)g.cfaces.size();
     info->nv = (c->opt.turbulence == FCFD_TURB_SA) ? 6 : 5;
+    info->n_nodeavg = (int)g.naidx.size();
     return FCFD_OK;
 }
 
@@ -506,6 +507,8 @@ int fcfd_metrics(const fcfd_ctx* c, fcfd_metrics_out* m) {
     cp_i(m->avg_ptr, g.aptr); cp_i(m->avg_idx, g.aidx); cp_d(m->avg_w, g.aw, g.aw.size());
     cp_i(m->cell_perm, g.perm);
     if (m->grid_lengthscale) m->grid_lengthscale[0] = g.glscale;
+    cp_d(m->node_xyz, g.node, 3 * (size_t)g.nnode);
+    cp_i(m->node_avg_ptr, g.naptr); cp_i(m->node_avg_idx, g.naidx); cp_d(m->node_avg_w, g.naw, g.naw.size());
     return FCFD_OK;
 }
 

@@ -29,6 +29,8 @@ struct HostGrid {
     std::vector<double> gw;                   // 3*entries
     std::vector<int> aptr, aidx;              // F+1, face-average keys (ascending)
     std::vector<double> aw;
+    std::vector<int> naptr, naidx;            // nnode+1, node-average keys (ascending cells, ghosts included)
+    std::vector<double> naw;
     std::vector<double> bc_area;              // per BC total area
     double total_volume = 0.0, glscale = 0.0;
     // internal numbering

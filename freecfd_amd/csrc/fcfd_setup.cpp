@@ -15,6 +15,7 @@
 // Geometry uses the reference formulas and operation order (grid.cc:123-178);
 // this file is compiled with -ffp-contract=off so it rounds like the reference.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -66,6 +67,187 @@ struct FaceKeyHash {
 };
 
 int lowest_bit(uint32_t m) { return m ? __builtin_ctz(m) : -1; }
+
+// Weighted tri-linear interpolation weights of a point from a stencil of
+// cell centroids (Interpolate, src/interpolate/interpolate.cc:36-351).  One
+// object serves one pass over faces or nodes: like the reference's members
+// weights3/weights4 it keeps the last small-system solution, which a
+// singular system (solve() leaves x alone) then reuses.
+class Wtli {
+public:
+    Wtli(int dim, int max_stencil, double skew_tol) : dim_(dim), maxs_(max_stencil), tol_(skew_tol) {}
+
+    // Reorders ids/pts as the reference's sort_stencil(is_internal = true)
+    // and truncates them; fills w.  Returns the number of weights: the
+    // stencil size for tetra or triangle fits, 2 for a line, 1 for a point.
+    int fit(V3 x, std::vector<int>& ids, std::vector<V3>& pts, std::vector<double>& w) {
+        keep_two_then_nearest(x, ids, pts);
+        const int m = (int)pts.size();
+        w.assign(m, 0.0);
+        if (tetra(x, pts, w) || tri(x, pts, w)) return m;
+        if (m >= 2) {   // interpolate_line (:320-343)
+            V3 e1 = pts[1] - pts[0];
+            V3 b1 = unit(e1);
+            V3 q = dot(x - pts[0], b1) * b1;
+            w[1] = dot(q, b1) / mag(e1);
+            w[0] = 1. - w[1];
+            return 2;
+        }
+        w[0] = 1.;      // interpolate_point (:345-351)
+        return 1;
+    }
+
+private:
+    int dim_, maxs_;
+    double tol_;
+    std::array<double, 4> x4_{};
+    std::array<double, 3> x3_{};
+
+    // the first two stay; the rest in ascending distance, an equal distance
+    // placed before the earlier entry (sort_stencil, :65-125)
+    void keep_two_then_nearest(V3 x, std::vector<int>& ids, std::vector<V3>& pts) const {
+        const int n = (int)pts.size(), m = std::min(n, maxs_);
+        std::vector<int> order(m, 0);
+        std::vector<double> best(m, 1.e20);
+        if (m > 1) order[1] = 1;
+        for (int d = 2; d < n; ++d) {
+            const double dd = mag(pts[d] - x);
+            for (int q = 2; q < m; ++q) {
+                if (dd <= best[q]) {
+                    for (int i = m - 1; i > q; --i) { order[i] = order[i - 1]; best[i] = best[i - 1]; }
+                    order[q] = d;
+                    best[q] = dd;
+                    break;
+                }
+            }
+        }
+        std::vector<int> ni(m);
+        std::vector<V3> np(m);
+        for (int i = 0; i < m; ++i) { ni[i] = ids[order[i]]; np[i] = pts[order[i]]; }
+        ids.swap(ni);
+        pts.swap(np);
+    }
+
+    // Gaussian elimination with a row swap only for a pivot below 1e-12
+    // (gelimd, src/utilities/utilities.cc:71-101); x unchanged when singular
+    template <int K>
+    static bool solve(double (&a)[K][K], double (&b)[K], std::array<double, K>& x) {
+        constexpr double kEps = 1e-12;
+        for (int i = 0; i < K; ++i) {
+            double pvt = a[i][i];
+            if (std::fabs(pvt) < kEps) {
+                int j = i + 1;
+                for (; j < K; ++j) { pvt = a[j][i]; if (std::fabs(pvt) >= kEps) break; }
+                if (std::fabs(pvt) < kEps) return false;
+                for (int c = 0; c < K; ++c) std::swap(a[i][c], a[j][c]);
+                std::swap(b[i], b[j]);
+            }
+            for (int k = i + 1; k < K; ++k) {
+                const double t = a[k][i] / pvt;
+                for (int c = i + 1; c < K; ++c) a[k][c] -= t * a[i][c];
+                b[k] -= t * b[i];
+            }
+        }
+        for (int i = K - 1; i >= 0; --i) {
+            double v = b[i];
+            for (int c = K - 1; c > i; --c) v -= a[i][c] * x[c];
+            x[i] = v / a[i][i];
+        }
+        return true;
+    }
+
+    static void normalise(std::vector<double>& w) {
+        double s = 0.;
+        for (double v : w) s += v;
+        for (double& v : w) v /= s;
+    }
+
+    // blended barycentric fits over every admissible tetrahedron (:127-243)
+    bool tetra(V3 x, const std::vector<V3>& P, std::vector<double>& w) {
+        const int m = (int)P.size();
+        if (dim_ != 3 || m < 4) return false;
+        bool ok = false;
+        for (int s1 = 0; s1 < m - 3; ++s1)
+            for (int s2 = s1 + 1; s2 < m - 2; ++s2)
+                for (int s3 = s2 + 1; s3 < m - 1; ++s3)
+                    for (int s4 = s3 + 1; s4 < m; ++s4) {
+                        const V3 e1 = P[s2] - P[s1], e2 = P[s3] - P[s1], e3 = P[s3] - P[s2];
+                        const V3 e4 = P[s4] - P[s1], e5 = P[s4] - P[s2], e6 = P[s4] - P[s3];
+                        const double edge = (mag(e1) + mag(e2) + mag(e3) + mag(e4) + mag(e5) + mag(e6)) / 6.;
+                        V3 nrm = cross(e1, e2);
+                        const double area = 0.5 * mag(nrm);
+                        nrm = unit(nrm);
+                        const double vol = area * std::fabs(dot(e4, nrm)) / 3.;
+                        const double regular = std::sqrt(2.) / 12. * edge * edge * edge;
+                        if (1. - vol / regular > tol_) continue;   // too skewed
+                        const V3 cen = (((P[s1] + P[s2]) + P[s3]) + P[s4]) / 4.;
+                        double dist = mag(x - cen) / edge;
+                        if (dist < 1.e-8) dist = 1.e-8;
+                        const double blend = 1. / (dist * dist);
+                        const int s[4] = {s1, s2, s3, s4};
+                        double a[4][4], b[4] = {x.x, x.y, x.z, 1.};
+                        for (int q = 0; q < 4; ++q) { a[0][q] = P[s[q]].x; a[1][q] = P[s[q]].y; a[2][q] = P[s[q]].z; a[3][q] = 1.; }
+                        solve<4>(a, b, x4_);
+                        const double sum = ((x4_[0] + x4_[1]) + x4_[2]) + x4_[3];
+                        if (std::fabs(sum - 1.) < 1.e-8) {
+                            for (int q = 0; q < 4; ++q) w[s[q]] += blend * x4_[q];
+                            ok = true;
+                        }
+                    }
+        normalise(w);
+        return ok;
+    }
+
+    // blended fits over every admissible triangle, in-plane coordinates with
+    // the point projected through the origin (:245-318)
+    bool tri(V3 x, const std::vector<V3>& P, std::vector<double>& w) {
+        const int m = (int)P.size();
+        if (dim_ < 2 || m < 3) return false;
+        std::fill(w.begin(), w.end(), 0.0);
+        bool ok = false;
+        for (int s1 = 0; s1 < m - 2; ++s1)
+            for (int s2 = s1 + 1; s2 < m - 1; ++s2)
+                for (int s3 = s2 + 1; s3 < m; ++s3) {
+                    const V3 e1 = P[s2] - P[s1], e2 = P[s3] - P[s1], e3 = P[s3] - P[s2];
+                    V3 nrm = cross(e1, e2);
+                    const double area = 0.5 * mag(nrm);
+                    nrm = unit(nrm);
+                    const double edge = (mag(e1) + mag(e2) + mag(e3)) / 3.;
+                    if (1. - area / (0.433 * edge * edge) > tol_) continue;   // too skewed
+                    V3 q = x - dot(x, nrm) * nrm;
+                    const V3 cen = ((P[s1] + P[s2]) + P[s3]) / 3.;
+                    double dist = mag(q - cen) / edge;
+                    if (dist < 1.e-2) dist = 1.e-2;
+                    const double blend = 1. / (dist * dist);
+                    const V3 b1 = unit(e1);
+                    const V3 b2 = (-1.) * unit(cross(b1, nrm));
+                    q = q - P[s1];
+                    double a[3][3] = {{0., dot(e1, b1), dot(e2, b1)}, {0., dot(e1, b2), dot(e2, b2)}, {1., 1., 1.}};
+                    double b[3] = {dot(q, b1), dot(q, b2), 1.};
+                    solve<3>(a, b, x3_);
+                    const double sum = (x3_[0] + x3_[1]) + x3_[2];
+                    if (std::fabs(sum - 1.) < 1.e-8) {
+                        w[s1] += blend * x3_[0]; w[s2] += blend * x3_[1]; w[s3] += blend * x3_[2];
+                        ok = true;
+                    }
+                }
+        normalise(w);
+        return ok;
+    }
+};
+
+// std::map<int, double>::insert of (ids[s], w[s]), s < n, appended to a CSR
+// row in ascending keys (a repeated key keeps its first weight)
+void append_map_row(const std::vector<int>& ids, const std::vector<double>& w, int n, std::vector<int>& idx, std::vector<double>& wt) {
+    std::vector<std::pair<int, double>> row;
+    for (int s = 0; s < n; ++s) {
+        bool dup = false;
+        for (auto& e : row) if (e.first == ids[s]) { dup = true; break; }
+        if (!dup) row.push_back({ids[s], w[s]});
+    }
+    std::stable_sort(row.begin(), row.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) { return a.first < b.first; });
+    for (auto& e : row) { idx.push_back(e.first); wt.push_back(e.second); }
+}
 
 }  // namespace
 
@@ -375,46 +557,89 @@ int build_host_grid(const fcfd_mesh* m, const fcfd_options* o, HostGrid& g, std:
             if (o->bc[b].type == FCFD_BC_SYMMETRY) { double s = std::sqrt(0.5 * g.bc_area[b]); if (g.glscale < s) g.glscale = s; }
     }
     // ---- face interpolation weights (face_interpolation_weights.cc:32-138) --
+    const int interp_method = o->interp_method;
+    int interp_max = o->stencil_size;
+    if (interp_max < 1) interp_max = (g.dim == 3) ? 12 : (g.dim == 2 ? 6 : 2);
+    if (interp_method == FCFD_INTERP_IDW) interp_max = 99;
     {
-        int method = o->interp_method;
-        int maxs = o->stencil_size;
-        if (maxs < 1) maxs = (g.dim == 3) ? 12 : (g.dim == 2 ? 6 : 2);
-        if (method == FCFD_INTERP_IDW) maxs = 99;
-        if (method == FCFD_INTERP_WTLI && maxs > 2) { err = "WTLI interpolation with stencil size > 2 is not supported"; return FCFD_ENOTSUP; }
+        Wtli wtli(g.dim, interp_max, o->skewness_tol);
         g.aptr.assign(F + 1, 0); g.aidx.clear(); g.aw.clear();
-        std::vector<int> st;
-        std::vector<std::pair<int, double>> ent;
+        std::vector<int> st, u;
+        std::vector<V3> pts;
+        std::vector<double> w;
         for (int f = 0; f < F; ++f) {
-            int p = g.fparent[f], n = g.fneigh[f];
-            ent.clear();
-            V3 pt = get3(g.fc, f);
-            if (method == FCFD_INTERP_WTLI) {
-                V3 e1 = get3(g.cc, n) - get3(g.cc, p);
-                V3 b1 = unit(e1);
-                V3 pp = pt - get3(g.cc, p);
-                pp = dot(pp, b1) * b1;
-                double w1 = dot(pp, b1) / mag(e1);
-                ent.push_back({p, 1. - w1});
-                ent.push_back({n, w1});
-            } else if (method == FCFD_INTERP_SIMPLE) {
-                ent.push_back({p, 0.5});
-                ent.push_back({n, 0.5});
-            } else {
-                st.clear();
-                st.push_back(p); st.push_back(n);
-                std::vector<int> u(g.nb.begin() + g.nbptr[p], g.nb.begin() + g.nbptr[p + 1]);
+            const int p = g.fparent[f], n = g.fneigh[f];
+            const V3 pt = get3(g.fc, f);
+            st.assign({p, n});
+            if (2 < interp_max) {   // the parent's (and an interior neighbour's) neighbour cells, as a set
+                u.assign(g.nb.begin() + g.nbptr[p], g.nb.begin() + g.nbptr[p + 1]);
                 if (g.fbc[f] == -1) u.insert(u.end(), g.nb.begin() + g.nbptr[n], g.nb.begin() + g.nbptr[n + 1]);
                 std::sort(u.begin(), u.end());
                 u.erase(std::unique(u.begin(), u.end()), u.end());
                 for (int x : u) if (x != p && x != n) st.push_back(x);
-                std::vector<double> w(st.size());
-                double sum = 0.0;
-                for (size_t s = 0; s < st.size(); ++s) { w[s] = mag(get3(g.fc, f) - get3(g.cc, st[s])); sum += w[s]; }
-                for (size_t s = 0; s < st.size(); ++s) ent.push_back({st[s], w[s] / sum});
             }
-            std::sort(ent.begin(), ent.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) { return a.first < b.first; });
-            for (auto& e : ent) { g.aidx.push_back(e.first); g.aw.push_back(e.second); }
+            pts.resize(st.size());
+            for (size_t q = 0; q < st.size(); ++q) pts[q] = get3(g.cc, st[q]);
+            int nw;
+            if (interp_method == FCFD_INTERP_WTLI) {
+                nw = wtli.fit(pt, st, pts, w);
+            } else if (interp_method == FCFD_INTERP_SIMPLE) {
+                w.assign({0.5, 0.5});
+                nw = 2;
+            } else {   // "idw": distance-proportional weights (:109-118)
+                w.resize(st.size());
+                double sum = 0.0;
+                for (size_t q = 0; q < st.size(); ++q) { w[q] = mag(pt - pts[q]); sum += w[q]; }
+                for (size_t q = 0; q < st.size(); ++q) w[q] = w[q] / sum;
+                nw = (int)st.size();
+            }
+            append_map_row(st, w, nw, g.aidx, g.aw);
             g.aptr[f + 1] = (int)g.aidx.size();
+        }
+    }
+    // ---- node interpolation weights (node_interpolation_weights.cc:32-118) --
+    // stencil: the cells around the node, boundary ghosts included, as a set
+    // (ascending); the reference calls calculate_weights with is_internal =
+    // true here too, so its two lowest cells always stay in a WTLI stencil
+    {
+        std::vector<int> ngptr(nn_local + 1, 0), nghost;
+        for (int q = 0; q < B; ++q) {
+            const int f = g.ghost_face[q];
+            for (int k = g.fnptr[f]; k < g.fnptr[f + 1]; ++k) ngptr[g.fnodes[k] + 1]++;
+        }
+        for (int n = 0; n < nn_local; ++n) ngptr[n + 1] += ngptr[n];
+        nghost.resize(ngptr[nn_local]);
+        std::vector<int> fill(ngptr.begin(), ngptr.end() - 1);
+        for (int q = 0; q < B; ++q) {
+            const int f = g.ghost_face[q];
+            for (int k = g.fnptr[f]; k < g.fnptr[f + 1]; ++k) nghost[fill[g.fnodes[k]]++] = N + q;
+        }
+        Wtli wtli(g.dim, interp_max, o->skewness_tol);
+        g.naptr.assign(nn_local + 1, 0); g.naidx.clear(); g.naw.clear();
+        std::vector<int> st;
+        std::vector<V3> pts;
+        std::vector<double> w;
+        for (int n = 0; n < nn_local; ++n) {
+            st.assign(ncell.begin() + ncptr[n], ncell.begin() + ncptr[n + 1]);   // ascending interior cells
+            st.insert(st.end(), nghost.begin() + ngptr[n], nghost.begin() + ngptr[n + 1]);   // then ascending ghosts
+            const V3 pt = get3(g.node, n);
+            pts.resize(st.size());
+            for (size_t q = 0; q < st.size(); ++q) pts[q] = get3(g.cc, st[q]);
+            int nw;
+            if (interp_method == FCFD_INTERP_WTLI) {
+                nw = wtli.fit(pt, st, pts, w);
+            } else if (interp_method == FCFD_INTERP_IDW) {
+                w.resize(st.size());
+                double sum = 0.0;
+                for (size_t q = 0; q < st.size(); ++q) { w[q] = mag(pt - pts[q]); sum += w[q]; }
+                for (size_t q = 0; q < st.size(); ++q) w[q] = w[q] / sum;
+                nw = (int)st.size();
+            } else {
+                w.assign(st.size(), 1. / double(st.size()));
+                nw = (int)st.size();
+            }
+            append_map_row(st, w, nw, g.naidx, g.naw);
+            g.naptr[n + 1] = (int)g.naidx.size();
         }
     }
     // ---- gradient maps (gradient_maps.cc:39-78) ----------------------------

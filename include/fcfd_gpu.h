@@ -175,7 +175,8 @@ typedef struct fcfd_info {
     int n_faceavg;       /* total face-average entries over faces          */
     int n_nbr;           /* total neighborCells entries over all cells     */
     int n_halffaces;     /* sum over interior cells of faces per cell      */
-    int nv;              /* equations per cell (5)                         */
+    int nv;              /* equations per cell (5, or 6 with the SA model) */
+    int n_nodeavg;       /* total node-average entries over nodes          */
 } fcfd_info;
 
 /* Metrics in reference numbering for bit-exact checks of the set-up.  Every
@@ -194,6 +195,12 @@ typedef struct fcfd_metrics_out {
     int* avg_ptr; int* avg_idx; double* avg_w;            /* F+1, A, A        */
     int* cell_perm;                                       /* N+B: internal index of reference cell */
     double* grid_lengthscale;                             /* 1                */
+    /* nodes in the reference's local order (first touch in cell order) and
+     * their interpolation weights, node.average of
+     * node_interpolation_weights (src/node_interpolation_weights.cc:32-118):
+     * keys are cells in reference numbering (ghosts included), ascending */
+    double* node_xyz;                                     /* 3 n_nodes        */
+    int* node_avg_ptr; int* node_avg_idx; double* node_avg_w; /* n_nodes+1, n_nodeavg, n_nodeavg */
 } fcfd_metrics_out;
 
 typedef struct fcfd_ctx fcfd_ctx;

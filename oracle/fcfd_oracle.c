@@ -116,6 +116,7 @@ struct orc_ctx {
     int nbc; unsigned char** bcflag; double* bc_area; double* bc_total_area;
     /* interpolation + gradient maps (CSR, ascending keys) */
     int *avg_ptr, *avg_idx; double* avg_w;
+    int *navg_ptr, *navg_idx; double* navg_w;   /* node averages (node_interpolation_weights) */
     int *g_ptr, *g_idx; V3* g_w;
     /* state */
     int nv;                         /* 5, or 6 with the SA model */
@@ -565,12 +566,264 @@ static void set_length_scales(orc_ctx* o) {
 /* ------------------------------------------------------------------ */
 static int cmp_int(const void* a, const void* b) { int x = *(const int*)a, y = *(const int*)b; return (x > y) - (x < y); }
 
+/* ------------------------------------------------------------------ */
+/* Interpolate (src/interpolate/interpolate.cc:36-351) and gelimd      */
+/* (src/utilities/utilities.cc:71-101)                                 */
+/* ------------------------------------------------------------------ */
+/* One Interpolate object: its weights3/weights4 members persist between
+ * calls, and gelimd leaves its solution untouched on a singular system, so
+ * a singular fit reuses the previous fit's solution (kept here). */
+typedef struct { int dim, maxs; double tol; double w4[4], w3[3]; } interp_t;
+
+#define GELIM_EPS 1e-12   /* EPS, src/utilities/utilities.h:26 */
+static int gelimd(int n, double a[4][4], double* b, double* x) {   /* utilities.cc:71-101 */
+    int i, j, k;
+    double tmp, pvt;
+    for (i = 0; i < n; i++) {
+        pvt = a[i][i];
+        if (fabs(pvt) < GELIM_EPS) {
+            for (j = i + 1; j < n; j++) { if (fabs(pvt = a[j][i]) >= GELIM_EPS) break; }
+            if (fabs(pvt) < GELIM_EPS) return 1;
+            for (k = 0; k < n; ++k) { tmp = a[i][k]; a[i][k] = a[j][k]; a[j][k] = tmp; }
+            tmp = b[j]; b[j] = b[i]; b[i] = tmp;
+        }
+        for (k = i + 1; k < n; k++) {
+            tmp = a[k][i] / pvt;
+            for (j = i + 1; j < n; j++) a[k][j] -= tmp * a[i][j];
+            b[k] -= tmp * b[i];
+        }
+    }
+    for (i = n - 1; i >= 0; i--) {
+        x[i] = b[i];
+        for (j = n - 1; j > i; j--) x[i] -= a[i][j] * x[j];
+        x[i] /= a[i][i];
+    }
+    return 0;
+}
+
+/* sort_stencil(is_internal = true) (interpolate.cc:65-125): the first two
+ * entries stay, the others follow by ascending distance to the point (an
+ * equal distance goes before the earlier entry), truncated to maxs.  Both
+ * callers pass is_internal = true.  Reorders ids/pts in place; returns the
+ * new size.  (With a single entry the reference writes order[1] past the
+ * end of its vector; that write is dropped here.) */
+static int sort_stencil_keep2(int n, int maxs, V3 point, int* ids, V3* pts) {
+    int m = n < maxs ? n : maxs;
+    int* order = (int*)malloc(sizeof(int) * (m + 1));
+    double* mind = (double*)malloc(sizeof(double) * (m + 1));
+    for (int i = 0; i < m; ++i) { order[i] = 0; mind[i] = 1.e20; }
+    order[0] = 0;
+    if (m > 1) order[1] = 1;
+    for (int d = 2; d < n; ++d) {
+        double dd = vmag(vsub(pts[d], point));
+        for (int q = 2; q < m; ++q) {
+            if (dd <= mind[q]) {
+                for (int i = m - 1; i > q; --i) order[i] = order[i - 1];
+                for (int i = m - 1; i > q; --i) mind[i] = mind[i - 1];
+                order[q] = d;
+                mind[q] = dd;
+                break;
+            }
+        }
+    }
+    int* ni = (int*)malloc(sizeof(int) * (m + 1));
+    V3* np_ = (V3*)malloc(sizeof(V3) * (m + 1));
+    for (int i = 0; i < m; ++i) { ni[i] = ids[order[i]]; np_[i] = pts[order[i]]; }
+    for (int i = 0; i < m; ++i) { ids[i] = ni[i]; pts[i] = np_[i]; }
+    free(order); free(mind); free(ni); free(np_);
+    return m;
+}
+
+static int interpolate_tetra(interp_t* it, int m, const V3* P, V3 point, double* w) {   /* :127-243 */
+    if (it->dim != 3 || m < 4) return 0;
+    for (int i = 0; i < m; ++i) w[i] = 0.;
+    int ok = 0;
+    for (int s1 = 0; s1 < m - 3; ++s1)
+        for (int s2 = s1 + 1; s2 < m - 2; ++s2)
+            for (int s3 = s2 + 1; s3 < m - 1; ++s3)
+                for (int s4 = s3 + 1; s4 < m; ++s4) {
+                    V3 e1 = vsub(P[s2], P[s1]), e2 = vsub(P[s3], P[s1]), e3 = vsub(P[s3], P[s2]);
+                    V3 e4 = vsub(P[s4], P[s1]), e5 = vsub(P[s4], P[s2]), e6 = vsub(P[s4], P[s3]);
+                    double ave = (vmag(e1) + vmag(e2) + vmag(e3) + vmag(e4) + vmag(e5) + vmag(e6)) / 6.;
+                    V3 pn = vcross(e1, e2);
+                    double area = 0.5 * vmag(pn);
+                    pn = vnorm(pn);
+                    double volume = area * fabs(vdot(e4, pn)) / 3.;
+                    double eqv = sqrt(2.) / 12. * ave * ave * ave;
+                    double skew = 1. - volume / eqv;
+                    if (skew > it->tol) continue;
+                    V3 cen = vdiv(vadd(vadd(vadd(P[s1], P[s2]), P[s3]), P[s4]), 4.);
+                    double dist = vmag(vsub(point, cen)) / ave;
+                    dist = dmax(dist, 1.e-8);
+                    double tw = 1. / (dist * dist);
+                    double a[4][4], b[4];
+                    const int s[4] = {s1, s2, s3, s4};
+                    for (int q = 0; q < 4; ++q) {
+                        a[0][q] = P[s[q]].c[0]; a[1][q] = P[s[q]].c[1]; a[2][q] = P[s[q]].c[2]; a[3][q] = 1.;
+                    }
+                    b[0] = point.c[0]; b[1] = point.c[1]; b[2] = point.c[2]; b[3] = 1.;
+                    gelimd(4, a, b, it->w4);
+                    double ws = 0.;
+                    for (int q = 0; q < 4; ++q) ws += it->w4[q];
+                    if (fabs(ws - 1.) < 1.e-8) {
+                        for (int q = 0; q < 4; ++q) w[s[q]] += tw * it->w4[q];
+                        ok = 1;
+                    }
+                }
+    double ws = 0.;
+    for (int i = 0; i < m; ++i) ws += w[i];
+    for (int i = 0; i < m; ++i) w[i] /= ws;
+    return ok;
+}
+
+static int interpolate_tri(interp_t* it, int m, const V3* P, V3 point, double* w) {   /* :245-318 */
+    if (it->dim < 2 || m < 3) return 0;
+    for (int i = 0; i < m; ++i) w[i] = 0.;
+    int ok = 0;
+    for (int s1 = 0; s1 < m - 2; ++s1)
+        for (int s2 = s1 + 1; s2 < m - 1; ++s2)
+            for (int s3 = s2 + 1; s3 < m; ++s3) {
+                V3 e1 = vsub(P[s2], P[s1]), e2 = vsub(P[s3], P[s1]), e3 = vsub(P[s3], P[s2]);
+                V3 pn = vcross(e1, e2);
+                double area = 0.5 * vmag(pn);
+                pn = vnorm(pn);
+                double ave = (vmag(e1) + vmag(e2) + vmag(e3)) / 3.;
+                double eqa = 0.433 * ave * ave;
+                double skew = 1. - area / eqa;
+                if (skew > it->tol) continue;
+                V3 pp = vsub(point, vsc(vdot(point, pn), pn));     /* projection through the origin (:271) */
+                V3 cen = vdiv(vadd(vadd(P[s1], P[s2]), P[s3]), 3.);
+                double dist = vmag(vsub(pp, cen)) / ave;
+                dist = dmax(dist, 1.e-2);
+                double tw = 1. / (dist * dist);
+                V3 b1 = vnorm(e1);
+                V3 b2 = vsc(-1., vnorm(vcross(b1, pn)));
+                pp = vsub(pp, P[s1]);
+                double a[4][4], b[4];
+                a[0][0] = 0.; a[0][1] = vdot(e1, b1); a[0][2] = vdot(e2, b1);
+                a[1][0] = 0.; a[1][1] = vdot(e1, b2); a[1][2] = vdot(e2, b2);
+                a[2][0] = 1.; a[2][1] = 1.; a[2][2] = 1.;
+                b[0] = vdot(pp, b1); b[1] = vdot(pp, b2); b[2] = 1.;
+                gelimd(3, a, b, it->w3);
+                double ws = 0.;
+                for (int q = 0; q < 3; ++q) ws += it->w3[q];
+                if (fabs(ws - 1.) < 1.e-8) {
+                    w[s1] += tw * it->w3[0]; w[s2] += tw * it->w3[1]; w[s3] += tw * it->w3[2];
+                    ok = 1;
+                }
+            }
+    double ws = 0.;
+    for (int i = 0; i < m; ++i) ws += w[i];
+    for (int i = 0; i < m; ++i) w[i] /= ws;
+    return ok;
+}
+
+/* calculate_weights (:36-50) after the stencil is in ids/pts (n entries):
+ * returns the number of weights (the stencil size for tetra/tri fits, 2 for
+ * a line, 1 for a point); ids/pts are reordered as sort_stencil leaves them */
+static int wtli_weights(interp_t* it, int n, V3 point, int* ids, V3* pts, double* w, int* kind) {
+    int m = sort_stencil_keep2(n, it->maxs, point, ids, pts);
+    for (int i = 0; i < m; ++i) w[i] = 0.;
+    if (interpolate_tetra(it, m, pts, point, w)) { *kind = 4; return m; }
+    if (interpolate_tri(it, m, pts, point, w)) { *kind = 3; return m; }
+    if (m >= 2) {                                                 /* interpolate_line :320-343 */
+        V3 e1 = vsub(pts[1], pts[0]);
+        V3 b1 = vnorm(e1);
+        V3 pp = vsub(point, pts[0]);
+        pp = vsc(vdot(pp, b1), b1);
+        w[1] = vdot(pp, b1) / vmag(e1);
+        w[0] = 1. - w[1];
+        *kind = 2;
+        return 2;
+    }
+    w[0] = 1.;                                                    /* interpolate_point :345-351 */
+    *kind = 1;
+    return 1;
+}
+
+/* std::map<int,double>::insert of (ids[s], w[s]) for s < nw into a CSR row
+ * kept in ascending keys (a repeated key keeps its first weight) */
+static void map_insert_row(int** idx, double** wt, int* cap, int* nent, const int* ids, const double* w, int nw) {
+    if (*nent + nw > *cap) {
+        *cap = 2 * *cap + nw;
+        *idx = (int*)realloc(*idx, sizeof(int) * *cap);
+        *wt = (double*)realloc(*wt, sizeof(double) * *cap);
+    }
+    int base = *nent;
+    for (int s = 0; s < nw; ++s) {
+        int key = ids[s], dup = 0;
+        for (int q = base; q < *nent; ++q) if ((*idx)[q] == key) { dup = 1; break; }
+        if (dup) continue;
+        int q = *nent;
+        while (q > base && (*idx)[q - 1] > key) { (*idx)[q] = (*idx)[q - 1]; (*wt)[q] = (*wt)[q - 1]; --q; }
+        (*idx)[q] = key; (*wt)[q] = w[s];
+        (*nent)++;
+    }
+}
+
+/* node_interpolation_weights (src/node_interpolation_weights.cc:32-118):
+ * stencil = the set of cells (ghosts included) around the node; WTLI with
+ * is_internal = true, IDW (distance-proportional, as for faces) or SIMPLE */
+static int node_interpolation_weights(orc_ctx* o) {
+    int method = o->opt.interp_method;
+    interp_t it;
+    memset(&it, 0, sizeof it);
+    it.dim = o->dim;
+    it.tol = o->opt.skewness_tol;
+    it.maxs = o->opt.stencil_size;
+    if (it.maxs < 1) it.maxs = (o->dim == 3) ? 12 : (o->dim == 2 ? 6 : 2);
+    if (method == FCFD_INTERP_IDW) it.maxs = 99;
+    o->navg_ptr = (int*)malloc(sizeof(int) * (o->nnode + 1));
+    int cap = o->nnode * 4 + 16, nent = 0;
+    o->navg_idx = (int*)malloc(sizeof(int) * cap);
+    o->navg_w = (double*)malloc(sizeof(double) * cap);
+    int scap = 64;
+    int* ids = (int*)malloc(sizeof(int) * scap);
+    V3* pts = (V3*)malloc(sizeof(V3) * scap);
+    double* w = (double*)malloc(sizeof(double) * scap);
+    for (int n = 0; n < o->nnode; ++n) {
+        o->navg_ptr[n] = nent;
+        const ivec* nc = &o->node_cells[n];
+        if (nc->n > scap) {
+            scap = 2 * nc->n;
+            ids = (int*)realloc(ids, sizeof(int) * scap); pts = (V3*)realloc(pts, sizeof(V3) * scap);
+            w = (double*)realloc(w, sizeof(double) * scap);
+        }
+        int ns = 0;                                                /* std::set<int>: ascending, unique */
+        for (int k = 0; k < nc->n; ++k) ids[ns++] = nc->a[k];
+        qsort(ids, ns, sizeof(int), cmp_int);
+        int u = 0;
+        for (int k = 0; k < ns; ++k) if (u == 0 || ids[k] != ids[u - 1]) ids[u++] = ids[k];
+        ns = u;
+        for (int k = 0; k < ns; ++k) pts[k] = o->cc[ids[k]];
+        V3 point = o->node[n];
+        int nw = 0, kind = 0;
+        if (method == FCFD_INTERP_WTLI) {
+            nw = wtli_weights(&it, ns, point, ids, pts, w, &kind);
+        } else if (method == FCFD_INTERP_IDW) {
+            double sum = 0.;
+            for (int s = 0; s < ns; ++s) { w[s] = vmag(vsub(point, pts[s])); sum += w[s]; }
+            for (int s = 0; s < ns; ++s) w[s] /= sum;
+            nw = ns;
+        } else {
+            for (int s = 0; s < ns; ++s) w[s] = 1. / (double)ns;
+            nw = ns;
+        }
+        map_insert_row(&o->navg_idx, &o->navg_w, &cap, &nent, ids, w, nw);
+    }
+    o->navg_ptr[o->nnode] = nent;
+    free(ids); free(pts); free(w);
+    return 0;
+}
+
 static int face_interpolation_weights(orc_ctx* o) {
     int method = o->opt.interp_method;
     int maxs = o->opt.stencil_size;
     if (maxs < 1) maxs = (o->dim == 3) ? 12 : (o->dim == 2 ? 6 : 2);
     if (method == FCFD_INTERP_IDW) maxs = 99;
-    if (method == FCFD_INTERP_WTLI && maxs > 2) { seterr(o, FCFD_ENOTSUP, "WTLI with stencil size > 2 is not supported"); return FCFD_ENOTSUP; }
+    interp_t it;
+    memset(&it, 0, sizeof it);
+    it.dim = o->dim; it.tol = o->opt.skewness_tol; it.maxs = maxs;
     o->avg_ptr = (int*)malloc(sizeof(int) * (o->nface + 1));
     int cap = o->nface * 2 + 16, nent = 0;
     o->avg_idx = (int*)malloc(sizeof(int) * cap);
@@ -602,15 +855,10 @@ static int face_interpolation_weights(orc_ctx* o) {
         double wts[4096];
         int nw = 0;
         if (method == FCFD_INTERP_WTLI) {
-            /* sort_stencil with is_internal=true keeps the first two; with
-             * max stencil 2 the stencil is [parent, neighbour] */
-            V3 edge1 = vsub(spts[1], spts[0]);
-            V3 basis1 = vnorm(edge1);
-            V3 pp = vsub(point, spts[0]);
-            pp = vsc(vdot(pp, basis1), basis1);
-            wts[1] = vdot(pp, basis1) / vmag(edge1);
-            wts[0] = 1. - wts[1];
-            nw = 2;
+            /* calculate_weights(is_internal = true): sort keeping parent and
+             * neighbour first, then tetra / tri / line / point fits */
+            int kind = 0;
+            nw = wtli_weights(&it, ns, point, sidx, spts, wts, &kind);
         } else if (method == FCFD_INTERP_IDW) {                        /* :109-118 */
             double sum = 0.;
             for (int s = 0; s < ns; ++s) { wts[s] = vmag(vsub(o->fc[f], spts[s])); sum += wts[s]; }
@@ -827,6 +1075,7 @@ orc_ctx* orc_create(const fcfd_mesh* mesh, const fcfd_options* opt) {
     set_bcs_grid(o);
     set_length_scales(o);
     rc = face_interpolation_weights(o);
+    if (!rc) rc = node_interpolation_weights(o);
     if (!rc) rc = gradient_maps(o);
     if (rc) { snprintf(g_err, sizeof g_err, "%s", o->msg); orc_destroy(o); return NULL; }
     int nt = o->ntot;
@@ -863,7 +1112,7 @@ void orc_destroy(orc_ctx* o) {
     free(o->avg_ptr); free(o->avg_idx); free(o->avg_w); free(o->g_ptr); free(o->g_idx); free(o->g_w);
     free(o->p); free(o->u); free(o->v); free(o->w); free(o->T);
     free(o->gp); free(o->gu); free(o->gv); free(o->gw); free(o->gT); free(o->nut); free(o->gn);
-    free(o->phi); free(o->upd); free(o->Rs); free(o->Rabs); free(o->fC); free(o->fD); free(o->JL); free(o->JR); free(o->Dg); free(o->loads);
+    free(o->phi); free(o->upd); free(o->Rs); free(o->Rabs); free(o->fC); free(o->fD); free(o->navg_ptr); free(o->navg_idx); free(o->navg_w); free(o->JL); free(o->JR); free(o->Dg); free(o->loads);
     free(o);
 }
 
@@ -878,6 +1127,7 @@ int orc_info(const orc_ctx* o, fcfd_info* info) {
     int hf = 0; for (int c = 0; c < o->ncell; ++c) hf += o->cfaces[c].n;
     info->n_halffaces = hf;
     info->nv = o->nv;
+    info->n_nodeavg = o->navg_ptr[o->nnode];
     return 0;
 }
 
@@ -936,7 +1186,25 @@ int orc_metrics(const orc_ctx* o, fcfd_metrics_out* m) {
         for (int k = 0; k < o->avg_ptr[F]; ++k) { if (m->avg_idx) m->avg_idx[k] = o->avg_idx[k]; if (m->avg_w) m->avg_w[k] = o->avg_w[k]; }
     }
     if (m->grid_lengthscale) m->grid_lengthscale[0] = o->glscale;
+    for (int n = 0; n < o->nnode; ++n)
+        if (m->node_xyz) for (int q = 0; q < 3; ++q) m->node_xyz[3 * n + q] = o->node[n].c[q];
+    if (m->node_avg_ptr) {
+        for (int n = 0; n <= o->nnode; ++n) m->node_avg_ptr[n] = o->navg_ptr[n];
+        for (int k = 0; k < o->navg_ptr[o->nnode]; ++k) {
+            if (m->node_avg_idx) m->node_avg_idx[k] = o->navg_idx[k];
+            if (m->node_avg_w) m->node_avg_w[k] = o->navg_w[k];
+        }
+    }
     return 0;
+}
+
+/* gelimd on an n x n system (n = 3 or 4), row-major a (n*n), b (n): the
+ * probe for the pin tests; returns gelimd's status, x untouched on 1 */
+int orc_gelimd_probe(int n, const double* a, const double* b, double* x) {
+    double A4[4][4], B4[4];
+    if (n < 1 || n > 4) return FCFD_EINVAL;
+    for (int i = 0; i < n; ++i) { B4[i] = b[i]; for (int j = 0; j < n; ++j) A4[i][j] = a[n * i + j]; }
+    return gelimd(n, A4, B4, x);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1956,4 +2224,25 @@ int orc_bc_probe(orc_ctx* o, int bc, const double* WL, const double* WLc, const 
 int orc_ausm_probe(double M, double alpha, double* out) {
     out[0] = msp4(M); out[1] = msm4(M); out[2] = psp5(M, alpha); out[3] = psm5(M, alpha);
     return 0;
+}
+
+/* Interpolate::calculate_weights on one stencil (probe for the pin tests):
+ * a fresh interpolation object of dimension dim, stencil size maxs and
+ * skewness tolerance tol; ids/pts (n entries, pts 3n) are sorted and
+ * truncated as sort_stencil(is_internal = true); out_ids/out_w get the kept
+ * ids and the weights; *kind = 4 tetra, 3 triangle, 2 line, 1 point.
+ * Returns the number of weights. */
+int orc_wtli_probe(int dim, int maxs, double tol, const double* point, int n, const int* ids, const double* pts,
+                   int* out_ids, double* out_w, int* kind) {
+    interp_t it;
+    memset(&it, 0, sizeof it);
+    it.dim = dim; it.maxs = maxs; it.tol = tol;
+    int* id = (int*)malloc(sizeof(int) * (n + 1));
+    V3* P = (V3*)malloc(sizeof(V3) * (n + 1));
+    double* w = (double*)malloc(sizeof(double) * (n + 1));
+    for (int i = 0; i < n; ++i) { id[i] = ids[i]; P[i] = v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]); }
+    int nw = wtli_weights(&it, n, v3(point[0], point[1], point[2]), id, P, w, kind);
+    for (int i = 0; i < nw; ++i) { out_ids[i] = id[i]; out_w[i] = w[i]; }
+    free(id); free(P); free(w);
+    return nw;
 }

@@ -59,6 +59,13 @@ int orc_get_face_flux(const orc_ctx* o, double* C, double* D);
  * reference exits. */
 int orc_bc_probe(orc_ctx* o, int bc, const double* WL, const double* WLc, const double* WRc, const double* n,
                  const double* t1, double* out);
+/* gelimd (utilities.cc:71-101) on an n x n row-major system, n <= 4 */
+int orc_gelimd_probe(int n, const double* a, const double* b, double* x);
+/* Interpolate::calculate_weights(is_internal = true) on one stencil (fresh
+ * object): kept ids and weights, *kind 4/3/2/1 = tetra/tri/line/point; returns
+ * the number of weights */
+int orc_wtli_probe(int dim, int maxs, double tol, const double* point, int n, const int* ids, const double* pts,
+                   int* out_ids, double* out_w, int* kind);
 /* AUSM+-up split functions: out = M4+(M), M4-(M), P5+(M,alpha), P5-(M,alpha) */
 int orc_ausm_probe(double M, double alpha, double* out);
 /* SA model pieces at one point: out = source/volume, P, D, mu_t, fn */

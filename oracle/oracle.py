@@ -50,6 +50,9 @@ def lib():
         L.orc_get_face_flux.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_bc_probe.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
         L.orc_ausm_probe.argtypes = [C.c_double, C.c_double, C.c_void_p]
+        L.orc_gelimd_probe.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_wtli_probe.argtypes = [C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
         _LIB = L
     return _LIB
 
@@ -207,3 +210,27 @@ def ausm_split(M, alpha):
     out = np.zeros(4)
     lib().orc_ausm_probe(float(M), float(alpha), _p(out))
     return out
+
+
+def gelimd(a, b, x0=None):
+    """the reference's Gaussian elimination on a small dense system: returns
+    (status, x); x keeps x0 (default zeros) where the system is singular"""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(len(b)) if x0 is None else np.array(x0, dtype=np.float64)
+    rc = lib().orc_gelimd_probe(len(b), _p(a), _p(b), _p(x))
+    return rc, x
+
+
+def wtli(point, ids, pts, dim=3, max_stencil=12, tol=0.99):
+    """Interpolate::calculate_weights(is_internal = true) on one stencil:
+    returns (kind, kept ids, weights); kind 4 tetra, 3 triangle, 2 line, 1 point"""
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    pt = np.ascontiguousarray(point, dtype=np.float64)
+    oi = np.zeros(len(ids) + 1, np.int32)
+    ow = np.zeros(len(ids) + 1)
+    kind = C.c_int(0)
+    nw = lib().orc_wtli_probe(int(dim), int(max_stencil), float(tol), _p(pt), len(ids), _p(ids), _p(pts), _p(oi), _p(ow),
+                              C.byref(kind))
+    return kind.value, oi[:nw].copy(), ow[:nw].copy()
