@@ -175,7 +175,7 @@ void set_params(fcfd_ctx* c) {
         d.p_total = s.p_total; d.T_total = s.T_total; d.mdot = s.mdot; d.qdot = s.qdot;
         d.total_area = b < (int)c->hg.bc_area.size() ? c->hg.bc_area[b] : 0.0;
         d.nut = s.nut;
-        d.p_fixed = s.p_fixed; d.T_fixed = s.T_fixed; d.V_fixed = s.V_fixed;
+        d.p_fixed = s.p_fixed; d.T_fixed = s.T_fixed; d.V_fixed = s.V_fixed; d.qdot_fixed = s.qdot_fixed;
     }
 }
 
@@ -893,6 +893,34 @@ int fcfd_loads(fcfd_ctx* c, double* W_dev, double* loads_host, void* stream) {
     if (c->opt.n_bc > 0) {
         HIPCHK(c, hipMemcpyAsync(loads_host, c->loads_out, sizeof(double) * 6 * c->opt.n_bc, hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));
+    }
+    return FCFD_OK;
+}
+
+int fcfd_surface(fcfd_ctx* c, double* W_dev, double* mdot_host, double* qdot_host, double* tau_host, void* stream) {
+    if (!c || !W_dev) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    const HostGrid& hg = c->hg;
+    const int F = hg.F;
+    double* out = nullptr;
+    int rc = dalloc(c, (size_t)5 * F, &out);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    launch_ghosts(c->dg, c->P, W_dev, c->status, s);
+    launch_grad_lim(c->dg, c->P, W_dev, c->G, c->PHI, c->dref, s);
+    launch_surface(c->dg, c->P, W_dev, c->G, c->PHI, out, c->status, s);
+    std::vector<double> h((size_t)5 * F);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dfree(c, out);
+    if (e != hipSuccess) return fail(c, FCFD_EHIP, std::string("fcfd_surface: ") + hipGetErrorString(e));
+    for (int i = 0; i < F; ++i) {
+        const int f = hg.ifperm[i];
+        if (mdot_host) mdot_host[f] = h[i];
+        if (qdot_host) qdot_host[f] = h[(size_t)F + i];
+        if (tau_host)
+            for (int k = 0; k < 3; ++k) tau_host[3 * (size_t)f + k] = h[(size_t)(2 + k) * F + i];
     }
     return FCFD_OK;
 }

@@ -110,6 +110,10 @@ void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double*
 // face fluxes D - C per face into FX ([nv][F]) ...
 void launch_flux(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* FX,
                  double* R, int* status, hipStream_t s);
+// surface quantities of the face loop into out [5][F]: mdot (every face),
+// qdot, tau_x, tau_y, tau_z (boundary faces; 0 on interior faces)
+void launch_surface(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* out, int* status,
+                    hipStream_t s);
 // ... and their ordered per-cell sums into R (no source: tangents)
 void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s);
 // the residual's per-cell sums, plus V S (src, from launch_grad_lim) in the

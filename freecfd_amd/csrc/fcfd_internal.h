@@ -66,6 +66,7 @@ struct BcDev {
     double p, T, rho, V[3], p_total, T_total, mdot, qdot, total_area;
     double nut;      // SA nu~ prescribed at an inlet
     int p_fixed, T_fixed, V_fixed;   // Variable::fixedonBC of p, T, V (face values in Green-Gauss gradients)
+    int qdot_fixed;                  // qdot.fixedonBC: the face loop keeps the given qdot (ns_assemble_linear_system.cc:125)
 };
 
 }  // namespace fcfd

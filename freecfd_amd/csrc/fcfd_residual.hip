@@ -375,6 +375,35 @@ __global__ void __launch_bounds__(64) face_flux_kernel(DevGrid g, Params P, cons
     for (int v = 0; v < NV; ++v) FX[v * Fs + f] = X.D[v] - X.C[v];
 }
 
+// Surface quantities of the face loop (ns_assemble_linear_system.cc:122-127):
+// mdot = (C - D)[0] / A on every face; on boundary faces qdot = (C - D)[4] / A
+// (the given value where qdot is fixed on the BC, which the loop does not
+// overwrite) and tau = -D[1..3] / A.  out: [5][F] rows mdot, qdot, tau_x,
+// tau_y, tau_z (interior faces: qdot and tau rows 0).
+template <int FLUX, int VISC, int NV>
+__global__ void __launch_bounds__(64) surface_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ G,
+                                                     const double* __restrict__ PHI, int has_phi, double phi_const,
+                                                     double* __restrict__ out, int* __restrict__ status) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= g.F) return;
+    FaceCtx<double, NV> X;
+    CellLoader<NV> ld_{W, G, PHI, has_phi != 0, phi_const, (size_t)g.ld};
+    face_base<FLUX, VISC, NV>(g, P, ld_, f, X, status);
+    const size_t Fs = g.F;
+    const double A = X.F.area;
+    out[f] = (X.C[0] - X.D[0]) / A;
+    double q = 0., tx = 0., ty = 0., tz = 0.;
+    if (X.F.bc >= 0) {
+        const BcDev& bc = g.bc[X.F.bc];
+        q = bc.qdot_fixed ? bc.qdot : (X.C[4] - X.D[4]) / A;
+        tx = -X.D[1] / A; ty = -X.D[2] / A; tz = -X.D[3] / A;
+    }
+    out[Fs + f] = q;
+    out[2 * Fs + f] = tx;
+    out[3 * Fs + f] = ty;
+    out[4 * Fs + f] = tz;
+}
+
 // R_c = sum over the faces of c in reference face order of (D-C)+0 (parent)
 // or -1*(D-C)+0 (neighbour) -- the VecSetValues(ADD) sequence of
 // ns_assemble_linear_system.cc:131-140 starting from VecSet(rhs, 0) -- then,
@@ -496,6 +525,29 @@ static void launch_flux_v(const DevGrid& g, const Params& P, const double* W, co
     if (g.nv == 6) launch_flux_t<FLUX, 1, 6>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);   // SA is always viscous
     else if (P.viscous) launch_flux_t<FLUX, 1, 5>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);
     else launch_flux_t<FLUX, 0, 5>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);
+}
+
+template <int FLUX>
+static void launch_surface_v(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, int has_phi,
+                             double phi_const, double* out, int* status, hipStream_t s) {
+    const dim3 grid((g.F + 63) / 64), blk(64);
+    if (g.nv == 6) hipLaunchKernelGGL((surface_kernel<FLUX, 1, 6>), grid, blk, 0, s, g, P, W, G, PHI, has_phi, phi_const, out, status);
+    else if (P.viscous) hipLaunchKernelGGL((surface_kernel<FLUX, 1, 5>), grid, blk, 0, s, g, P, W, G, PHI, has_phi, phi_const, out, status);
+    else hipLaunchKernelGGL((surface_kernel<FLUX, 0, 5>), grid, blk, 0, s, g, P, W, G, PHI, has_phi, phi_const, out, status);
+}
+
+void launch_surface(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, double* out, int* status,
+                    hipStream_t s) {
+    const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    const int has_phi = lim != FCFD_LIMITER_NONE;
+    const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
+    switch (P.flux) {
+    case FCFD_FLUX_ROE: launch_surface_v<FCFD_FLUX_ROE>(g, P, W, G, PHI, has_phi, phi_const, out, status, s); break;
+    case FCFD_FLUX_VAN_LEER: launch_surface_v<FCFD_FLUX_VAN_LEER>(g, P, W, G, PHI, has_phi, phi_const, out, status, s); break;
+    case FCFD_FLUX_SD_SLAU: launch_surface_v<FCFD_FLUX_SD_SLAU>(g, P, W, G, PHI, has_phi, phi_const, out, status, s); break;
+    case FCFD_FLUX_SW: launch_surface_v<FCFD_FLUX_SW>(g, P, W, G, PHI, has_phi, phi_const, out, status, s); break;
+    default: launch_surface_v<FCFD_FLUX_AUSM_PLUS_UP>(g, P, W, G, PHI, has_phi, phi_const, out, status, s); break;
+    }
 }
 
 void launch_ghosts(const DevGrid& g, const Params& P, double* W, int* status, hipStream_t s) {

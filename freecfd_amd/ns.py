@@ -183,6 +183,18 @@ class NavierStokes(Grid):
         _check(self.h, lib().fcfd_loads(self.h, W.data_ptr(), out.ctypes.data, C.c_void_p(s)))
         return out.reshape(n_bc, 6)[: int(self._opt.n_bc)]
 
+    def surface(self, W=None):
+        """Surface quantities of the face loop (ns_assemble_linear_system.cc:
+        122-127) in reference face order: (mdot[F], qdot[F], tau[F, 3]);
+        qdot and tau are set on boundary faces, 0 elsewhere."""
+        W = self.W if W is None else W
+        F = self.info.n_faces
+        mdot, qdot, tau = np.zeros(F), np.zeros(F), np.zeros(3 * F)
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_surface(self.h, W.data_ptr(), mdot.ctypes.data, qdot.ctypes.data, tau.ctypes.data,
+                                          C.c_void_p(s)))
+        return mdot, qdot, tau.reshape(F, 3)
+
     def to_device(self, X_host):
         """N x nv host array (reference order) -> [nv][ld] device tensor."""
         X_host = np.ascontiguousarray(X_host, dtype=np.float64)

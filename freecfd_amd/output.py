@@ -381,7 +381,8 @@ def write_tecplot_surface_arrays(root, step, raw, metrics, fvals, variables, gid
     face order, one block per variable (vector components V, tau: one block
     each), ten to a line and with no newline after a block; then one line
     per face of its output node ids + 1, TAB after each, a triangle's third
-    node written twice.  fvals: face_values (p, T, rho, V, Mach, rank)."""
+    node written twice.  fvals: face_values (p, T, rho, V, Mach, rank), and
+    the face loop's mdot, qdot, tau (per face, reference order) when given."""
     fb = metrics["face_bc"]
     nptr, fnodes = metrics["face_node_ptr"], metrics["face_nodes"]
     n_bc = int(np.asarray(raw["bc_ptr"]).size - 1)
@@ -443,9 +444,15 @@ def write_tecplot_surface_arrays(root, step, raw, metrics, fvals, variables, gid
 def write_tecplot_surface(ns, raw, step, variables=("p", "V", "T"), root=".", gid=0):
     """write_surface_output(gid, step) with format tecplot for a NavierStokes
     context: face values from its current state and the ghosts of its last
-    residual evaluation (or gradients_limiters call)."""
+    residual evaluation (or gradients_limiters call); mdot, qdot and tau
+    (write_surface_output.cc:266-290, 390) from the face loop's surface
+    quantities of the current state (fcfd_surface: mdot.face, qdot.bc,
+    tau.bc, ns_assemble_linear_system.cc:122-127)."""
     m = ns.metrics()
     o = ns._opt
     fv = face_values(m, ns.state_host(include_ghosts=True), [o.bc[b] for b in range(int(o.n_bc))],
                      gamma=o.gamma, Mw=o.Mw, Pref=o.Pref, Tref=o.Tref)
+    if {"mdot", "qdot", "tau"} & set(variables):
+        mdot, qdot, tau = ns.surface()
+        fv.update(mdot=mdot, qdot=qdot, tau=tau)
     return write_tecplot_surface_arrays(root, step, raw, m, fv, variables, gid)

@@ -401,6 +401,16 @@ int  fcfd_vjp(fcfd_ctx* ctx, double* W_dev, const double* Rbar_dev, const double
  * synchronising the stream.  The ghost slots of W_dev are written. */
 int  fcfd_loads(fcfd_ctx* ctx, double* W_dev, double* loads_host, void* stream);
 
+/* Surface quantities of the face loop at the state W_dev
+ * (ns_assemble_linear_system.cc:122-127), in reference face order on the
+ * host: mdot_host[F] = (C - D)[0] / A on every face (mdot.face); on boundary
+ * faces qdot_host[F] = (C - D)[4] / A, or the given qdot where qdot_fixed
+ * (qdot.bc), and tau_host[3F] = -D[1..3] / A (tau.bc); both 0 on interior
+ * faces.  The surface writer's mdot, qdot and tau (write_surface_output.cc:
+ * 266-290, 390).  Any output may be NULL.  Recomputes ghosts, gradients and
+ * limiters of W_dev (its ghost slots are written); synchronises the stream. */
+int  fcfd_surface(fcfd_ctx* ctx, double* W_dev, double* mdot_host, double* qdot_host, double* tau_host, void* stream);
+
 /* One pseudo-time step of the steady adjoint (dR/dW)^T lam = -g at the state
  * W_dev (g = dJ/dW of an output J, e.g. from fcfd_vjp with a load weight and
  * Rbar = 0): with rho = (dR/dW)^T lam + g, solves ((P V/dt) - dR/dW)^T delta

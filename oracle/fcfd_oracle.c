@@ -1704,7 +1704,6 @@ static void sw_flux(const orc_ctx* o, const cstate* L, const cstate* R, double d
     }
 }
 
-static double g_ausm_alpha = 0.;
 /* convective_face_flux (ns_convective_face_flux.cc:32-59) */
 static void convective_face_flux(orc_ctx* o, const cstate* L, const cstate* R, const fstate* F, double flux[NVM]) {
     double fn[5];
@@ -1713,7 +1712,7 @@ static void convective_face_flux(orc_ctx* o, const cstate* L, const cstate* R, c
     case FCFD_FLUX_VAN_LEER: vanleer_flux(L, R, fn, o->opt.gamma, o->opt.Pref); break;
     case FCFD_FLUX_SD_SLAU: slau_flux(L, R, fn, o->opt.Pref); break;
     case FCFD_FLUX_SW: sw_flux(o, L, R, o->fdiss[F->index], o->fwd[F->index], fn); break;
-    default: ausm_flux(L, R, fn, o->opt.Pref, o->opt.Minf, &g_ausm_alpha); break;
+    default: { double alpha = 0.; ausm_flux(L, R, fn, o->opt.Pref, o->opt.Minf, &alpha); break; }   /* the reference's global alpha is set before use on every call */
     }
     flux[0] = fn[0] * F->area;
     flux[1] = (fn[1] * F->normal.c[0] + fn[2] * F->tangent1.c[0] + fn[3] * F->tangent2.c[0]) * F->area;
@@ -1753,8 +1752,10 @@ static void diffusive_face_flux(const orc_ctx* o, const fstate* F, double flux[N
 /* update_boundaries: decision D1                                     */
 /* ------------------------------------------------------------------ */
 int orc_update_boundaries(orc_ctx* o) {
+    int err = 0;
+#pragma omp parallel for schedule(static) reduction(| : err)
     for (int f = 0; f < o->nface; ++f) {
-        if (o->fbc[f] < 0) continue;
+        if (o->fbc[f] < 0 || err) continue;
         fstate F;
         face_geom_update(o, &F, f);
         int c = F.parent;
@@ -1769,12 +1770,12 @@ int orc_update_boundaries(orc_ctx* o) {
         finish_state(o, &L, &F);
         cstate R = L;
         int rc = apply_bcs(o, &L, &R, &F);
-        if (rc) return rc;
+        if (rc) { err |= 1; continue; }
         int g = F.neighbor;
         o->p[g] = R.p_center; o->u[g] = R.V_center.c[0]; o->v[g] = R.V_center.c[1]; o->w[g] = R.V_center.c[2]; o->T[g] = R.T_center;
         o->nut[g] = R.nut_center;
     }
-    return 0;
+    return err ? FCFD_EBC : 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1797,6 +1798,7 @@ static double face_value_scalar(const orc_ctx* o, const double* q, int f, int wh
 int orc_calc_cell_grads(orc_ctx* o) {
     const double* q[6] = {o->p, o->u, o->v, o->w, o->T, o->nut};
     const int nv = o->nv;
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < o->ncell; ++c) {
         V3 g[6];
         for (int k = 0; k < 6; ++k) g[k] = vzero();
@@ -1859,6 +1861,7 @@ int orc_calc_limiter(orc_ctx* o) {
             deltaRef[5] = fabs(nmax - nmin);
         }
     }
+#pragma omp parallel for schedule(static)
     for (int c = 0; c < o->ncell; ++c) {
         double phi[6] = {1., 1., 1., 1., 1., 1.};
         if (lf == FCFD_LIMITER_MINMOD) {
@@ -1915,7 +1918,6 @@ int orc_calc_limiter(orc_ctx* o) {
 int orc_assemble(orc_ctx* o, int with_jac) {
     int N = o->ncell;
     const int nv = o->nv, B2 = NVM * NVM;
-    for (int i = 0; i < NVM * N; ++i) { o->Rs[i] = 0.; o->Rabs[i] = 0.; }
     for (int i = 0; i < 6 * o->nbc; ++i) o->loads[i] = 0.;
     if (with_jac) {
         if (!o->JL) { o->JL = (double*)malloc(sizeof(double) * B2 * o->nface); o->JR = (double*)malloc(sizeof(double) * B2 * o->nface); o->Dg = (double*)malloc(sizeof(double) * B2 * (size_t)N); }
@@ -1931,38 +1933,26 @@ int orc_assemble(orc_ctx* o, int with_jac) {
     double small_v[NVM];
     for (int i = 0; i < 5; ++i) small_v[i] = small_number;
     small_v[5] = small_number * sa_nut_scale(o);
-    double fc[NVM], fd[NVM], fcp[NVM], fdp[NVM];
+    /* 1. the face loop (:75-93, get_jacobians :197-267): every face is
+     *    independent, so the faces run in parallel and keep their fluxes
+     *    (and FD blocks); the residual insertion follows per cell */
+    int err = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : err)
     for (int f = 0; f < o->nface; ++f) {
+        if (err) continue;
+        double fc[NVM], fd[NVM], fcp[NVM], fdp[NVM];
         fstate F;
         cstate L, R;
         face_geom_update(o, &F, f);
         recon_state(o, &L, &F, F.parent);
         finish_state(o, &L, &F);
         int rc = right_state_update(o, &L, &R, &F);
-        if (rc) return rc;
+        if (rc) { err |= 1; continue; }
         face_state_update(o, &L, &R, &F);
         convective_face_flux(o, &L, &R, &F, fc);
         fd[0] = 0.;
         diffusive_face_flux(o, &F, fd);
         for (int i = 0; i < NVM; ++i) { o->fC[NVM * f + i] = fc[i]; o->fD[NVM * f + i] = fd[i]; }
-        if (F.bc >= 0 && o->opt.bc[F.bc].include_in_loads) {          /* :109-119 */
-            V3 t = v3(fc[1] - fd[1], fc[2] - fd[2], fc[3] - fd[3]);
-            double* ld = &o->loads[6 * F.bc];
-            ld[0] += t.c[0]; ld[1] += t.c[1]; ld[2] += t.c[2];
-            V3 mc = v3(o->opt.moment_center[0], o->opt.moment_center[1], o->opt.moment_center[2]);
-            V3 mm = vcross(vsub(o->fc[f], mc), t);
-            ld[3] += mm.c[0]; ld[4] += mm.c[1]; ld[5] += mm.c[2];
-        }
-        for (int i = 0; i < nv; ++i) {                                 /* :131-140 */
-            double val = fd[i] - fc[i] + 0.;
-            o->Rs[NVM * F.parent + i] += val;
-            o->Rabs[NVM * F.parent + i] += fabs(val);
-            if (F.bc == -1) {
-                double vn = -1. * (fd[i] - fc[i]) + 0.;
-                o->Rs[NVM * F.neighbor + i] += vn;
-                o->Rabs[NVM * F.neighbor + i] += fabs(vn);
-            }
-        }
         if (!with_jac) continue;
         for (int var = 0; var < nv; ++var) {                           /* get_jacobians (:197-267) */
             double jl[NVM], jr[NVM] = {0, 0, 0, 0, 0, 0};
@@ -1973,7 +1963,7 @@ int orc_assemble(orc_ctx* o, int with_jac) {
             else if (L.update[var] < -sn) eps = dmin(-sn, factor * L.update[var]);
             else eps = sn;
             state_perturb(o, &Lp, &F, var, eps);
-            if (F.bc >= 0) { rc = right_state_update(o, &Lp, &Rp, &F); if (rc) return rc; }
+            if (F.bc >= 0) { rc = right_state_update(o, &Lp, &Rp, &F); if (rc) { err |= 1; break; } }
             convective_face_flux(o, &Lp, &Rp, &F, fcp);
             face_state_adjust(o, &Lp, &Rp, &F, var);
             fdp[0] = 0.;
@@ -1995,14 +1985,50 @@ int orc_assemble(orc_ctx* o, int with_jac) {
                 o->JL[B2 * f + NVM * var + j] = jl[j];
                 o->JR[B2 * f + NVM * var + j] = jr[j];
             }
-            /* MatSetValues(ADD) of the diagonal blocks (:154-177) */
-            for (int j = 0; j < nv; ++j) {
-                o->Dg[B2 * F.parent + NVM * j + var] += -1. * jl[j];
-                if (F.bc == -1) o->Dg[B2 * F.neighbor + NVM * j + var] += jr[j];
-            }
         }
     }
+    if (err) return FCFD_EBC;
+    /* 2. loads (:109-119), in face order */
+    for (int f = 0; f < o->nface; ++f) {
+        int b = o->fbc[f];
+        if (b < 0 || !o->opt.bc[b].include_in_loads) continue;
+        const double *fc = &o->fC[NVM * f], *fd = &o->fD[NVM * f];
+        V3 t = v3(fc[1] - fd[1], fc[2] - fd[2], fc[3] - fd[3]);
+        double* ld = &o->loads[6 * b];
+        ld[0] += t.c[0]; ld[1] += t.c[1]; ld[2] += t.c[2];
+        V3 mc = v3(o->opt.moment_center[0], o->opt.moment_center[1], o->opt.moment_center[2]);
+        V3 mm = vcross(vsub(o->fc[f], mc), t);
+        ld[3] += mm.c[0]; ld[4] += mm.c[1]; ld[5] += mm.c[2];
+    }
+    /* 3. the residual insertion (:131-140) and the diagonal blocks of the
+     *    Jacobian (MatSetValues ADD, :154-177): each cell adds its faces'
+     *    contributions in ascending face order, the order of the reference's
+     *    face loop, so the sums round as there */
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < N; ++c) {
+        double r[NVM] = {0, 0, 0, 0, 0, 0}, ra[NVM] = {0, 0, 0, 0, 0, 0};
+        double* D = with_jac ? &o->Dg[B2 * (size_t)c] : NULL;
+        for (int k = 0; k < o->cfaces[c].n; ++k) {
+            int f = o->cfaces[c].a[k];
+            const double *fc = &o->fC[NVM * f], *fd = &o->fD[NVM * f];
+            int parent = o->fparent[f] == c;
+            for (int i = 0; i < nv; ++i) {
+                double val = parent ? fd[i] - fc[i] + 0. : -1. * (fd[i] - fc[i]) + 0.;
+                r[i] += val;
+                ra[i] += fabs(val);
+            }
+            if (D) {
+                for (int var = 0; var < nv; ++var)
+                    for (int j = 0; j < nv; ++j) {
+                        if (parent) D[NVM * j + var] += -1. * o->JL[B2 * f + NVM * var + j];
+                        else D[NVM * j + var] += o->JR[B2 * f + NVM * var + j];
+                    }
+            }
+        }
+        for (int i = 0; i < NVM; ++i) { o->Rs[NVM * c + i] = r[i]; o->Rabs[NVM * c + i] = ra[i]; }
+    }
     if (nv == 6) {   /* SA cell source, added after the face sums */
+#pragma omp parallel for schedule(static)
         for (int c = 0; c < N; ++c) {
             double rho = m_rho(o, o->p[c], o->T[c]);
             double S = o->vol[c] * sa_source(rho, m_visc(o, o->T[c]), o->nut[c], o->gu[c], o->gv[c], o->gw[c], o->gn[c], o->wdist[c], NULL, NULL);
@@ -2246,3 +2272,232 @@ int orc_wtli_probe(int dim, int maxs, double tol, const double* point, int n, co
     free(id); free(P); free(w);
     return nw;
 }
+
+/* surface quantities of the last orc_assemble (ns_assemble_linear_system.cc:
+ * 122-127): mdot.face = (C - D)[0] / A on every face; on boundary faces
+ * qdot.bc = (C - D)[4] / A unless qdot is fixed on the BC (then the given
+ * value) and tau.bc = -D[1..3] / A; interior faces get 0 for qdot and tau */
+int orc_get_surface(const orc_ctx* o, double* mdot, double* qdot, double* tau) {
+    for (int f = 0; f < o->nface; ++f) {
+        const double* C = &o->fC[NVM * f];
+        const double* D = &o->fD[NVM * f];
+        const double A = o->farea[f];
+        mdot[f] = (C[0] - D[0]) / A;
+        qdot[f] = 0.; tau[3 * f] = tau[3 * f + 1] = tau[3 * f + 2] = 0.;
+        int b = o->fbc[f];
+        if (b >= 0) {
+            qdot[f] = o->opt.bc[b].qdot_fixed ? o->opt.bc[b].qdot : (C[4] - D[4]) / A;
+            for (int i = 0; i < 3; ++i) tau[3 * f + i] = -D[i + 1] / A;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* One implicit step on the host cores: the CPU side of the Newton    */
+/* comparison (NavierStokes::solve, ns.cc:104-118, with the solve that */
+/* the reference leaves to PETSc, ns_petsc_functions.cc:73-110, done   */
+/* here as block-Jacobi right-preconditioned restarted GMRES)          */
+/* ------------------------------------------------------------------ */
+#include <omp.h>
+
+/* inverse of one nv x nv block (row-major, stride NVM) by Gauss-Jordan with
+ * partial pivoting; returns 1 when singular */
+static int block_inverse(int nv, const double* A, double* Ainv) {
+    double a[NVM][2 * NVM];
+    for (int i = 0; i < nv; ++i)
+        for (int j = 0; j < nv; ++j) { a[i][j] = A[NVM * i + j]; a[i][nv + j] = (i == j) ? 1. : 0.; }
+    for (int c = 0; c < nv; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < nv; ++r) if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
+        if (a[piv][c] == 0.) return 1;
+        if (piv != c) for (int j = 0; j < 2 * nv; ++j) { double t = a[c][j]; a[c][j] = a[piv][j]; a[piv][j] = t; }
+        double d = a[c][c];
+        for (int j = 0; j < 2 * nv; ++j) a[c][j] /= d;
+        for (int r = 0; r < nv; ++r) {
+            if (r == c) continue;
+            double m = a[r][c];
+            if (m != 0.) for (int j = 0; j < 2 * nv; ++j) a[r][j] -= m * a[c][j];
+        }
+    }
+    for (int i = 0; i < nv; ++i) for (int j = 0; j < nv; ++j) Ainv[NVM * i + j] = a[i][nv + j];
+    return 0;
+}
+
+/* y = M x with M = P V/dt - dR/dW: diagonal blocks D (time terms included)
+ * and per face -JR^T (row parent, column neighbour), +JL^T (row neighbour,
+ * column parent); x, y are N x nv */
+static void newton_matvec(const orc_ctx* o, const double* D, const double* x, double* y) {
+    const int nv = o->nv, B2 = NVM * NVM;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < o->ncell; ++c) {
+        double acc[NVM] = {0, 0, 0, 0, 0, 0};
+        const double* Dc = &D[B2 * (size_t)c];
+        for (int i = 0; i < nv; ++i) for (int j = 0; j < nv; ++j) acc[i] += Dc[NVM * i + j] * x[nv * c + j];
+        for (int k = 0; k < o->cfaces[c].n; ++k) {
+            int f = o->cfaces[c].a[k];
+            if (o->fbc[f] >= 0) continue;
+            int p = o->fparent[f], n = o->fneigh[f];
+            const double* Bk = (p == c) ? &o->JR[B2 * (size_t)f] : &o->JL[B2 * (size_t)f];   /* [var][eq] */
+            const double sg = (p == c) ? -1. : 1.;
+            const int other = (p == c) ? n : p;
+            for (int eq = 0; eq < nv; ++eq) {
+                double s = 0.;
+                for (int var = 0; var < nv; ++var) s += Bk[NVM * var + eq] * x[nv * other + var];
+                acc[eq] += sg * s;
+            }
+        }
+        for (int i = 0; i < nv; ++i) y[nv * c + i] = acc[i];
+    }
+}
+
+static void bj_apply(const orc_ctx* o, const double* Dinv, const double* x, double* y) {
+    const int nv = o->nv, B2 = NVM * NVM;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < o->ncell; ++c)
+        for (int i = 0; i < nv; ++i) {
+            double s = 0.;
+            for (int j = 0; j < nv; ++j) s += Dinv[B2 * (size_t)c + NVM * i + j] * x[nv * c + j];
+            y[nv * c + i] = s;
+        }
+}
+
+static double vdotn(size_t n, const double* a, const double* b) {
+    double s = 0.;
+#pragma omp parallel for schedule(static) reduction(+ : s)
+    for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+/* GMRES(m) for M x = b, right-preconditioned with block Jacobi, modified
+ * Gram-Schmidt, x0 = 0; stops at ||r|| <= rtol ||b|| or max_iters */
+static int gmres_bj(const orc_ctx* o, const double* D, const double* Dinv, const double* b, double* x, int m, int max_iters,
+                    double rtol, double* rnorm_out) {
+    const size_t n = (size_t)o->nv * o->ncell;
+    double* V = (double*)malloc(sizeof(double) * n * (m + 1));
+    double* z = (double*)malloc(sizeof(double) * n);
+    double* w = (double*)malloc(sizeof(double) * n);
+    double* H = (double*)calloc((size_t)(m + 1) * m, sizeof(double));
+    double *cs = (double*)malloc(sizeof(double) * m), *sn = (double*)malloc(sizeof(double) * m);
+    double *g = (double*)malloc(sizeof(double) * (m + 1)), *y = (double*)malloc(sizeof(double) * m);
+    memset(x, 0, sizeof(double) * n);
+    const double bnorm = sqrt(vdotn(n, b, b));
+    double rnorm = bnorm;
+    int its = 0;
+    while (its < max_iters && rnorm > rtol * bnorm && bnorm > 0.) {
+        /* r = b - M x */
+        newton_matvec(o, D, x, w);
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < n; ++i) w[i] = b[i] - w[i];
+        double beta = sqrt(vdotn(n, w, w));
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < n; ++i) V[i] = w[i] / beta;
+        for (int i = 0; i <= m; ++i) g[i] = 0.;
+        g[0] = beta;
+        int k = 0;
+        for (; k < m && its < max_iters; ++k, ++its) {
+            bj_apply(o, Dinv, &V[n * k], z);
+            newton_matvec(o, D, z, w);
+            for (int j = 0; j <= k; ++j) {
+                double h = vdotn(n, w, &V[n * j]);
+                H[(size_t)j * m + k] = h;
+#pragma omp parallel for schedule(static)
+                for (size_t i = 0; i < n; ++i) w[i] -= h * V[n * j + i];
+            }
+            double hn = sqrt(vdotn(n, w, w));
+            H[(size_t)(k + 1) * m + k] = hn;
+            if (hn > 0.) {
+#pragma omp parallel for schedule(static)
+                for (size_t i = 0; i < n; ++i) V[n * (k + 1) + i] = w[i] / hn;
+            }
+            for (int j = 0; j < k; ++j) {   /* previous Givens rotations */
+                double a = H[(size_t)j * m + k], bb = H[(size_t)(j + 1) * m + k];
+                H[(size_t)j * m + k] = cs[j] * a + sn[j] * bb;
+                H[(size_t)(j + 1) * m + k] = -sn[j] * a + cs[j] * bb;
+            }
+            double a = H[(size_t)k * m + k], bb = H[(size_t)(k + 1) * m + k];
+            double r = sqrt(a * a + bb * bb);
+            cs[k] = r > 0. ? a / r : 1.; sn[k] = r > 0. ? bb / r : 0.;
+            H[(size_t)k * m + k] = r; H[(size_t)(k + 1) * m + k] = 0.;
+            g[k + 1] = -sn[k] * g[k];
+            g[k] = cs[k] * g[k];
+            rnorm = fabs(g[k + 1]);
+            if (rnorm <= rtol * bnorm || hn == 0.) { ++k; ++its; break; }
+        }
+        for (int i = k - 1; i >= 0; --i) {   /* y = H^-1 g, then x += M^-1 V y */
+            double s = g[i];
+            for (int j = i + 1; j < k; ++j) s -= H[(size_t)i * m + j] * y[j];
+            y[i] = s / H[(size_t)i * m + i];
+        }
+        memset(w, 0, sizeof(double) * n);
+        for (int j = 0; j < k; ++j) {
+#pragma omp parallel for schedule(static)
+            for (size_t i = 0; i < n; ++i) w[i] += y[j] * V[n * j + i];
+        }
+        bj_apply(o, Dinv, w, z);
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < n; ++i) x[i] += z[i];
+    }
+    *rnorm_out = rnorm;
+    free(V); free(z); free(w); free(H); free(cs); free(sn); free(g); free(y);
+    return its;
+}
+
+/* One implicit step at CFL_LOCAL `cfl` on the state held by the context:
+ * residual + FD Jacobian (with the previous update setting the FD step),
+ * time terms P V/dt, block-Jacobi GMRES(restart) to rtol or max_iters,
+ * W += dW (kept as the next step's previous update).
+ * stats: ||R||_2 before the step, Krylov iterations, then wall milliseconds
+ * of residual + Jacobian, time terms + block inverses, Krylov, update. */
+int orc_newton_step(orc_ctx* o, double cfl, int restart, int max_iters, double rtol, double* stats) {
+    const int N = o->ncell, nv = o->nv, B2 = NVM * NVM;
+    const size_t n = (size_t)nv * N;
+    double t0 = omp_get_wtime();
+    int rc = orc_update_boundaries(o);
+    if (!rc) rc = orc_calc_cell_grads(o);
+    if (!rc) rc = orc_calc_limiter(o);
+    if (!rc) rc = orc_assemble(o, 1);
+    if (rc) return rc;
+    double* b = (double*)malloc(sizeof(double) * n);
+    double rr = 0.;
+    for (int c = 0; c < N; ++c) for (int i = 0; i < nv; ++i) { b[nv * c + i] = o->Rs[NVM * c + i]; rr += b[nv * c + i] * b[nv * c + i]; }
+    double t1 = omp_get_wtime();
+    double* dt = (double*)malloc(sizeof(double) * N);
+    double mc;
+    orc_time_step(o, FCFD_DT_CFL_LOCAL, cfl, dt, &mc);
+    double* T = (double*)malloc(sizeof(double) * (size_t)nv * nv * N);
+    orc_time_terms(o, dt, T);
+    double* D = (double*)malloc(sizeof(double) * B2 * (size_t)N);
+    double* Dinv = (double*)malloc(sizeof(double) * B2 * (size_t)N);
+    int sing = 0;
+#pragma omp parallel for schedule(static) reduction(| : sing)
+    for (int c = 0; c < N; ++c) {
+        double* Dc = &D[B2 * (size_t)c];
+        for (int i = 0; i < nv; ++i)
+            for (int j = 0; j < nv; ++j) Dc[NVM * i + j] = o->Dg[B2 * (size_t)c + NVM * i + j] + T[(size_t)nv * nv * c + nv * i + j];
+        sing |= block_inverse(nv, Dc, &Dinv[B2 * (size_t)c]);
+    }
+    double t2 = omp_get_wtime();
+    double* x = (double*)malloc(sizeof(double) * n);
+    double rn = 0.;
+    int its = sing ? 0 : gmres_bj(o, D, Dinv, b, x, restart, max_iters, rtol, &rn);
+    double t3 = omp_get_wtime();
+    int bad = 0;
+    for (size_t i = 0; i < n; ++i) bad |= !isfinite(x[i]);
+    if (!bad && !sing) {
+        double* q[6] = {o->p, o->u, o->v, o->w, o->T, o->nut};
+#pragma omp parallel for schedule(static)
+        for (int c = 0; c < N; ++c)
+            for (int i = 0; i < nv; ++i) { q[i][c] += x[nv * c + i]; o->upd[NVM * c + i] = x[nv * c + i]; }
+    }
+    double t4 = omp_get_wtime();
+    stats[0] = sqrt(rr); stats[1] = its;
+    stats[2] = 1e3 * (t1 - t0); stats[3] = 1e3 * (t2 - t1); stats[4] = 1e3 * (t3 - t2); stats[5] = 1e3 * (t4 - t3);
+    free(b); free(dt); free(T); free(D); free(Dinv); free(x);
+    if (sing) { seterr(o, FCFD_EDIVERGED, "singular diagonal block"); return FCFD_EDIVERGED; }
+    if (bad) { seterr(o, FCFD_EDIVERGED, "NaN/Inf in the update"); return FCFD_EDIVERGED; }
+    return 0;
+}
+
+/* threads the OpenMP loops use */
+int orc_num_threads(void) { return omp_get_max_threads(); }

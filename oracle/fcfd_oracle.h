@@ -53,6 +53,13 @@ int orc_time_step(orc_ctx* o, int dt_type, double cfl, double* dt, double* max_c
 int orc_time_terms(orc_ctx* o, const double* dt, double* blocks);
 /* per-face convective C and diffusive D fluxes of the last orc_assemble, F x nv each */
 int orc_get_face_flux(const orc_ctx* o, double* C, double* D);
+/* one implicit step at CFL_LOCAL on the host cores (FD Jacobian, time terms,
+ * block-Jacobi GMRES(restart), W += dW); stats[6] = ||R||_2, Krylov
+ * iterations, ms of residual+Jacobian, time terms+inverses, Krylov, update */
+int orc_newton_step(orc_ctx* o, double cfl, int restart, int max_iters, double rtol, double* stats);
+int orc_num_threads(void);
+/* mdot.face, qdot.bc, tau.bc of the last orc_assemble (F, F, 3F; reference face order) */
+int orc_get_surface(const orc_ctx* o, double* mdot, double* qdot, double* tau);
 /* apply_bcs on BC `bc`: left face state WL and centre WLc, ghost centre WRc
  * (p,u,v,w,T each), face normal n and tangent t1; out (12) = right state
  * p,u,v,w,T,rho, the a apply_bcs set, centre p,u,v,w,T.  FCFD_EBC where the

@@ -48,6 +48,9 @@ def lib():
                                      C.c_double, C.c_void_p]
         L.orc_sa_probe.argtypes = [C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
         L.orc_get_face_flux.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_get_surface.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_newton_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double, C.c_void_p]
+        L.orc_num_threads.restype = C.c_int
         L.orc_bc_probe.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
         L.orc_ausm_probe.argtypes = [C.c_double, C.c_double, C.c_void_p]
         L.orc_gelimd_probe.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -166,6 +169,26 @@ class Oracle:
         self._check(lib().orc_get_face_flux(self.h, _p(Cf), _p(Df)))
         return Cf, Df
 
+    def newton_step(self, cfl, restart=30, max_iters=30, rtol=1e-6):
+        """one implicit step on the host cores from the current state (set
+        with set_state); returns dict(res_norm, krylov_iters, ms_residual_jacobian,
+        ms_precond, ms_krylov, ms_update)"""
+        st = np.zeros(6)
+        self._check(lib().orc_newton_step(self.h, float(cfl), int(restart), int(max_iters), float(rtol), _p(st)))
+        return dict(res_norm=st[0], krylov_iters=int(st[1]), ms_residual_jacobian=st[2], ms_precond=st[3], ms_krylov=st[4],
+                    ms_update=st[5])
+
+    def state(self):
+        """interior state N x nv"""
+        return self.fields()[0][:self.info.n_cells].copy()
+
+    def surface(self):
+        """(mdot[F], qdot[F], tau[F, 3]) of the last residual evaluation"""
+        F = self.info.n_faces
+        mdot, qdot, tau = np.zeros(F), np.zeros(F), np.zeros((F, 3))
+        self._check(lib().orc_get_surface(self.h, _p(mdot), _p(qdot), _p(tau)))
+        return mdot, qdot, tau
+
     def bc_probe(self, bc, WL, WLc, WRc, n, t1):
         """apply_bcs of BC `bc` on one face: returns (status, dict of the
         right state: p, V, T, rho, a (as the BC set it), p_c, V_c, T_c)."""
@@ -234,3 +257,8 @@ def wtli(point, ids, pts, dim=3, max_stencil=12, tol=0.99):
     nw = lib().orc_wtli_probe(int(dim), int(max_stencil), float(tol), _p(pt), len(ids), _p(ids), _p(pts), _p(oi), _p(ow),
                               C.byref(kind))
     return kind.value, oi[:nw].copy(), ow[:nw].copy()
+
+
+def num_threads():
+    """threads the restatement's OpenMP loops use (OMP_NUM_THREADS or all cores)"""
+    return int(lib().orc_num_threads())

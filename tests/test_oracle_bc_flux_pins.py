@@ -639,3 +639,47 @@ def test_residual_telescopes_to_boundary_fluxes_and_loads(case, oracle_lib):
     isum = (Df - Cf)[fb < 0]
     assert np.abs(R.sum(axis=0) - bsum).max() <= tol.max()
     assert isum.shape[0] > 0
+
+
+# ---------------------------------------------------------------------------
+# surface quantities (ns_assemble_linear_system.cc:122-127)
+# ---------------------------------------------------------------------------
+
+def test_surface_quantities(oracle_lib):
+    """mdot per face reassembles the mass residual (R_c[0] = -sum of +-mdot A
+    over the cell's faces, a separate accumulation); at a no-slip adiabatic
+    wall the mirrored states carry no mass and no energy (mdot, qdot ~ 0);
+    a FIXED_Q wall reports its given qdot; tau = -D[1..3] / A on every
+    boundary face."""
+    walls = [cases.bc_wall(), cases.bc_wall(qdot=750.0), cases.bc_wall(T=330.0)] + [cases.bc_symmetry()] * 3
+    raw = box_hex(5, 4, 3, jitter=0.2, seed=41)
+    opt = cases.make_options(walls, mu=1.8e-5, Pr=0.72, flux=A.FLUX_ROE)
+    o = _oracle(raw, opt)
+    m = o.metrics()
+    N = o.info.n_cells
+    rng = np.random.default_rng(12)
+    W = np.zeros((N, 5))
+    W[:, 0] = 1.0e5 * (1 + 0.01 * rng.random(N))
+    W[:, 1:4] = rng.normal(size=(N, 3)) * 20.0
+    W[:, 4] = 300.0 * (1 + 0.01 * rng.random(N))
+    R, Rabs = o.residual(W)
+    Cf, Df = o.face_fluxes()
+    mdot, qdot, tau = o.surface()
+    fb, A_ = m["face_bc"], m["face_area"]
+    par, nei = m["face_parent"], m["face_neighbor"]
+    mass = np.zeros(N)
+    np.add.at(mass, par, -mdot * A_)
+    inner = fb < 0
+    np.add.at(mass, nei[inner], mdot[inner] * A_[inner])
+    assert np.allclose(mass, R[:, 0], rtol=0, atol=1e-12 * Rabs[:, 0].max())
+    scale = np.abs(mdot[inner]).max()
+    wall = fb == 0
+    assert np.abs(mdot[wall]).max() <= 1e-10 * scale
+    assert np.abs(qdot[wall]).max() <= 1e-9 * np.abs(Cf[inner, 4] / A_[inner]).max()
+    assert (qdot[fb == 1] == 750.0).all()
+    bnd = fb >= 0
+    assert np.array_equal(tau[bnd], -Df[bnd, 1:4] / A_[bnd, None])
+    assert (qdot[inner] == 0).all() and (tau[inner] == 0).all()
+    # the FIXED_T wall conducts: its qdot is the face's energy flux per area
+    ft = fb == 2
+    assert np.allclose(qdot[ft], (Cf[ft, 4] - Df[ft, 4]) / A_[ft], rtol=1e-15, atol=0)
