@@ -3,10 +3,14 @@
 A step is one FP64 residual evaluation R(W) over the whole mesh (boundary
 ghosts, gradients + limiters, face fluxes, ordered cell sums with the SA
 source), on a synthetic state resident in HBM.  Prints one JSON line:
-whole-job cell-residual evaluations per second, the roofline of the dominant
-kernel from HIP events recorded live around each kernel, and a CPU baseline
-(the CPU restatement on a bounded sample).  The default workload is the
-SA-RANS flat plate (6 equations per cell).
+whole-job cell-residual evaluations per second; the roofline of the dominant
+pass of the two-pass residual model from HIP events recorded live around each
+kernel, with the whole residual's fraction beside it; implicit step times of
+both Newton modes on the workload; the Newton race to a 1e-10 residual drop
+(laminar cylinder 256x128, FD/FP64 against JVP/FP32, loads against the CPU
+restatement); and the CPU baseline (the CPU restatement on all host cores, on
+a bounded sample: residual evaluations and implicit steps).  The default
+workload is the SA-RANS flat plate (6 equations per cell).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plate_sa]
 """
@@ -106,12 +110,29 @@ def measured_traffic(config, cells, kernel):
     return d["kernels"][kernel]["traffic_bytes"]
 
 
-def cpu_baseline(name, target_s=12.0):
-    """The CPU restatement (one core) on a bounded sample of the same workload
-    family: the same configuration at 1/8 of the mesh size per direction."""
-    from oracle.oracle import Oracle
-    cs = make_case(name, scale=8)
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(name, target_s=12.0, scale=2, newton_steps=2, cfl=5.0):
+    """The CPU restatement on the host cores (OpenMP over its face and cell
+    loops, every sum in the serial order) on a bounded sample of the same
+    workload: the configuration at 1/scale of the mesh per direction.
+    Residual evaluations for about target_s seconds, then implicit steps
+    (FD Jacobian, time terms, block-Jacobi GMRES(30) capped at 30
+    iterations, W += dW) from the same state."""
+    from oracle.oracle import Oracle, num_threads
+    cs = make_case(name, scale=scale)
+    t0 = time.perf_counter()
     o = Oracle(cs.raw, cs.opt)
+    setup = time.perf_counter() - t0
     o.residual(cs.W0)   # warm
     n = 0
     t0 = time.perf_counter()
@@ -121,9 +142,29 @@ def cpu_baseline(name, target_s=12.0):
         if time.perf_counter() - t0 > target_s:
             break
     dt = (time.perf_counter() - t0) / n
-    return {"value": cs.n_cells / dt, "unit": "cell-residual evals/s", "cores": 1, "kind": "port",
-            "sample": "%s at 1/8 scale per direction (%d cells), %d residual evaluations (update_boundaries + "
-                      "calc_cell_grads + calc_limiter + residual face loop), %.1f s" % (name, cs.n_cells, n, n * dt)}
+    o.set_state(cs.W0)
+    steps = []
+    for _ in range(newton_steps):
+        t1 = time.perf_counter()
+        st = o.newton_step(cfl=cfl, restart=30, max_iters=30, rtol=1e-6)
+        st["ms_per_step"] = 1e3 * (time.perf_counter() - t1)
+        steps.append(st)
+    o.close()
+    cores = num_threads()
+    out = {"value": cs.n_cells / dt, "unit": "cell-residual evals/s", "cores": cores, "kind": "port",
+           "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+           "sample": "%s at 1/%d scale per direction (%d cells), %d residual evaluations (update_boundaries + "
+                     "calc_cell_grads + calc_limiter + residual face loop) in %.1f s on %d OpenMP threads; "
+                     "set-up %.1f s" % (name, scale, cs.n_cells, n, n * dt, cores, setup),
+           "newton_step": {"cells": cs.n_cells, "cfl": cfl, "krylov": "block-Jacobi GMRES(30), at most 30 iterations",
+                           "ms_per_step": [round(x["ms_per_step"], 1) for x in steps],
+                           "ms_residual_jacobian": [round(x["ms_residual_jacobian"], 1) for x in steps],
+                           "ms_precond": [round(x["ms_precond"], 1) for x in steps],
+                           "ms_krylov": [round(x["ms_krylov"], 1) for x in steps],
+                           "krylov_iters": [x["krylov_iters"] for x in steps]}}
+    print("bench: cpu baseline %.3e cell-residual evals/s on %d threads, Newton step %s ms (%d cells)"
+          % (out["value"], cores, out["newton_step"]["ms_per_step"], cs.n_cells), file=sys.stderr, flush=True)
+    return out
 
 
 def newton_compare(ns, cs, steps, cfl=5.0, max_rel_update=0.2):
@@ -212,13 +253,19 @@ def race(name, cfl=1000.0, drop=1e-10, max_steps=200):
         print("bench: race %s %s: %s in %d steps, %.3f s, drop %.2e" % (name, mode, r["status"], r["steps"],
                                                                           r["total_ms"] / 1e3, r["drop"]),
               file=sys.stderr, flush=True)
+    # the reference loads: the CPU restatement's loads of the converged
+    # FD/FP64 state; each mode's GPU loads of its own converged state are
+    # compared with them (so the JVP/FP32 figure holds both the solution's
+    # and the evaluation's error)
     o = Oracle(cs.raw, cs.opt)
-    o.residual(states["jvp_fp32"])
+    o.residual(states["fd_fp64"])
     ref = o.loads()
+    o.close()
     wall = int(np.argmax(np.abs(ref).sum(axis=1)))
     for mode in modes:
         got = np.array(modes[mode]["loads"])
-        modes[mode]["load_err_vs_oracle"] = float(np.abs(got[wall, :3] - ref[wall, :3]).max() / np.abs(ref[wall, :3]).max())
+        modes[mode]["load_err_vs_oracle_fd_fp64"] = float(np.abs(got[wall, :3] - ref[wall, :3]).max() /
+                                                          np.abs(ref[wall, :3]).max())
     fd, jv = modes["fd_fp64"], modes["jvp_fp32"]
     both = fd["status"] == "converged" and jv["status"] == "converged"
     return {"metric": "wall time to a 1e-10 residual drop", "value": jv["seconds"], "unit": "s", "n_gpus": 1,
@@ -242,8 +289,11 @@ def main():
     ap.add_argument("--race", default=None, choices=sorted(RACES),
                     help="instead of the residual benchmark: Newton to a 1e-10 residual drop with FD/FP64 and "
                          "JVP/FP32 on this case, one JSON line")
-    ap.add_argument("--newton-steps", type=int, default=4,
-                    help="implicit steps timed per Jacobian mode (FD/FP64 and JVP/FP32) after the residual; 0 skips")
+    ap.add_argument("--newton-steps", type=int, default=3,
+                    help="implicit steps timed per Jacobian mode (FD/FP64 and JVP/FP32) on the workload after the residual "
+                         "(step times only: both modes are capped at 30 Krylov iterations); 0 skips")
+    ap.add_argument("--no-race", action="store_true",
+                    help="skip the Newton race to a 1e-10 drop (laminar cylinder 256x128, FD/FP64 vs JVP/FP32)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the residual path is measured on one MI355X); "
@@ -343,7 +393,13 @@ def main():
                      "algorithmic_bytes_per_launch": passes[dom][1], "avg_launch_ms": pass_ms[dom]},
     }
     if args.newton_steps > 0:
-        line["config"]["newton"] = newton_compare(ns, cs, args.newton_steps)
+        line["config"]["newton_step_times"] = newton_compare(ns, cs, args.newton_steps)
+    if not args.no_race:
+        r = race("cylinder256")
+        line["config"]["newton_race"] = {"workload": r["config"]["workload"], "cells": r["config"]["cells"],
+                                         "cfl": r["config"]["cfl"], "modes": r["config"]["modes"],
+                                         "speedup_fd_over_jvp": r["config"]["speedup_fd_over_jvp"],
+                                         "drag_lift_fd_vs_jvp": r["config"]["drag_lift_fd_vs_jvp"]}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.config)
     print(json.dumps(line), flush=True)
