@@ -80,6 +80,12 @@ class NewtonStats(C.Structure):
 
 
 PC_BLOCK_JACOBI, PC_BLOCK_SGS, PC_MULTIGRID = 0, 1, 2
+PSEUDO_PC_NONE, PSEUDO_PC_WS95 = 0, 1
+
+
+class PseudoOpts(C.Structure):
+    _fields_ = [("dtau_type", C.c_int), ("dtau", C.c_double), ("preconditioner", C.c_int), ("ps_step", C.c_int),
+                ("Wn_dev", C.c_void_p)]
 
 
 def newton_opts(cfl=10.0, jacobian=JAC_FD, fp32=False, restart=30, max_iters=10, rtol=1e-6, abstol=1e-12, knoll=True,

@@ -47,6 +47,7 @@ def lib():
         "fcfd_adjoint_step": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(A.NewtonOpts), C.POINTER(A.NewtonStats), _vp]),
         "fcfd_set_bc": (C.c_int, [_vp, C.c_int, C.POINTER(A.BC)]),
         "fcfd_surface": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+        "fcfd_pseudo_step": (C.c_int, [_vp, _vp, C.POINTER(A.NewtonOpts), C.POINTER(A.PseudoOpts), C.POINTER(A.NewtonStats), _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

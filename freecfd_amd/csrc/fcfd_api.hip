@@ -49,6 +49,10 @@ struct fcfd_ctx {
     // Newton-Krylov workspace (lazy)
     double *R = nullptr, *dt = nullptr, *dW = nullptr, *upd = nullptr, *red_part = nullptr, *red_out = nullptr;
     double* dt_red = nullptr;   // 2 + 256 doubles: CFL_MAX step, largest CFL, block partials (lazy)
+    // dual time stepping (lazy): pseudo step size, drho/dp of the pseudo
+    // block, the interior cells' neighbour cells (WS95)
+    double *dtau = nullptr, *drdp = nullptr;
+    const int *nb_ptr = nullptr, *nb_idx = nullptr;
     int gm_restart32 = 0, gm_restart64 = 0;   // restart length of the allocated FP32 / FP64 Krylov bases
     GmresWork<double> w64;
     GmresWork<float> w32;
@@ -1065,11 +1069,41 @@ static int ensure_colors(fcfd_ctx* c) {
     return FCFD_OK;
 }
 
-int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd_newton_stats* st, void* stream) {
+}  // extern "C"
+
+// the neighbour cells (ghosts included) of the interior cells in internal
+// numbering, for the WS95 pressure-difference floor
+static int ensure_neighbours(fcfd_ctx* c) {
+    if (c->nb_ptr) return FCFD_OK;
+    const HostGrid& g = c->hg;
+    std::vector<int> p(g.N + 1, 0), idx;
+    for (int i = 0; i < g.N; ++i) {
+        const int r = g.iperm[i];
+        for (int k = g.nbptr[r]; k < g.nbptr[r + 1]; ++k) idx.push_back(g.perm[g.nb[k]]);
+        p[i + 1] = (int)idx.size();
+    }
+    int rc;
+    if ((rc = upload(c, p, &c->nb_ptr)) || (rc = upload(c, idx, &c->nb_idx))) return rc;
+    return FCFD_OK;
+}
+
+static int newton_impl(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, const fcfd_pseudo_opts* ps, fcfd_newton_stats* st,
+                       void* stream) {
     if (!c || !W_dev || !o) return FCFD_EINVAL;
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
     int rc0 = 0;
     if ((rc0 = check_dt(c, o->dt_type, o->cfl))) return rc0;
+    if (ps) {
+        if (ps->dtau_type != FCFD_DT_FIXED && ps->dtau_type != FCFD_DT_CFL_MAX && ps->dtau_type != FCFD_DT_CFL_LOCAL)
+            return fail(c, FCFD_EINVAL, "unknown pseudo time step type");
+        if (!(ps->dtau > 0.0)) return fail(c, FCFD_EINVAL, "the pseudo step size or CFL number must be positive");
+        if (ps->preconditioner != FCFD_PSEUDO_PC_NONE && ps->preconditioner != FCFD_PSEUDO_PC_WS95)
+            return fail(c, FCFD_EINVAL, "unknown pseudo-time preconditioner");
+        if (ps->ps_step < 1) return fail(c, FCFD_EINVAL, "pseudo steps count from 1");
+        if (ps->ps_step > 1 && !ps->Wn_dev) return fail(c, FCFD_EINVAL, "a pseudo step after the first needs the state at time level n");
+        if (!c->dtau && ((rc0 = dalloc(c, (size_t)c->dg.ld, &c->dtau)) || (rc0 = dalloc(c, (size_t)c->dg.ld, &c->drdp)))) return rc0;
+        if (ps->preconditioner == FCFD_PSEUDO_PC_WS95 && (rc0 = ensure_neighbours(c))) return rc0;
+    }
     if (o->jacobian != FCFD_JAC_FD && o->jacobian != FCFD_JAC_JVP_FULL && o->jacobian != FCFD_JAC_JVP_FROZEN)
         return fail(c, FCFD_EINVAL, "unknown Jacobian mode");
     const bool mfree = o->jacobian != FCFD_JAC_FD;
@@ -1100,13 +1134,19 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     launch_grad_lim(g, c->P, W_dev, c->G, c->PHI, c->dref, s, c->SRC);
     launch_flux(g, c->P, W_dev, c->G, c->PHI, c->FX, c->R, c->status, s);
     launch_residual_sum(g, c->SRC, c->FX, c->R, s);
+    launch_time_step(g, c->P, W_dev, o->dt_type, o->cfl, c->dt, c->dt_red, s);
+    if (ps) {   // dual time: the pseudo step, its block and the right-hand side R - (P V/dt)(W - Wn)
+        launch_time_step(g, c->P, W_dev, ps->dtau_type, ps->dtau, c->dtau, c->dt_red, s);
+        launch_pseudo_rhs(g, c->P, W_dev, ps->Wn_dev, c->dt, c->nb_ptr, c->nb_idx, ps->preconditioner == FCFD_PSEUDO_PC_WS95,
+                          ps->ps_step > 1, c->R, c->drdp, s);
+    }
     double rr = 0.0;
     HIPCHK(c, dev_mdot(g, 1, c->R, n5, c->R, c->red_part, c->red_out, &rr, s));
     S.res_norm = std::sqrt(rr);
     auto t1 = clk::now();
-    // 2. local time step, FD Jacobian + time terms, block-Jacobi inverse
-    launch_time_step(g, c->P, W_dev, o->dt_type, o->cfl, c->dt, c->dt_red, s);
+    // 2. FD Jacobian + time terms (+ the pseudo-time block), block-Jacobi inverse
     launch_fd_jacobian(g, c->P, W_dev, c->G, c->PHI, c->upd, c->dt, c->JL, c->JR, c->Dg, c->status, s);
+    if (ps) launch_pseudo_diag(g, c->P, W_dev, c->dtau, c->drdp, c->Dg, s);
     if (fp32) {
         dev_bj_invert<float>(g, c->Dg, c->Dinv32, c->status, s);
         if (!mfree || sgs) {
@@ -1137,6 +1177,7 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
         launch_jvp(g, c->P, W_dev, c->jv_in, c->G, c->dG, c->PHI, c->dPHI, c->dref, c->dFX, c->jv_R,
                    o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss);
         launch_newton_op(g, c->P, W_dev, c->dt, c->jv_in, c->jv_R, y, ss);   // jv_in: x may alias y
+        if (ps) launch_pseudo_op(g, c->P, W_dev, c->dtau, c->drdp, c->jv_in, y, ss);
     };
     // preconditioners: block Jacobi, block SGS, or the FP64 multigrid V-cycle
     // (FP32 Krylov vectors converted at its boundary)
@@ -1215,6 +1256,18 @@ int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd
     if (bad & 1)   // the reference exits here (ns_apply_bcs.cc:54-59)
         return fail(c, FCFD_EBC, "an inlet became supersonic; the thermodynamic state must be specified (ns_apply_bcs.cc:54-59)");
     return FCFD_OK;
+}
+
+extern "C" {
+
+int fcfd_newton_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, fcfd_newton_stats* st, void* stream) {
+    return newton_impl(c, W_dev, o, nullptr, st, stream);
+}
+
+int fcfd_pseudo_step(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, const fcfd_pseudo_opts* ps, fcfd_newton_stats* st,
+                     void* stream) {
+    if (!ps) return fail(c, FCFD_EINVAL, "null pseudo-time options");
+    return newton_impl(c, W_dev, o, ps, st, stream);
 }
 
 int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* lam_dev, const fcfd_newton_opts* o,

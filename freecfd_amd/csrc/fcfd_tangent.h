@@ -11,6 +11,17 @@ void launch_jvp(const DevGrid& g, const Params& P, double* W, double* dW, double
 // y = (P V / dt) x - dR
 void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* dR,
                       double* y, hipStream_t s);
+// dual time stepping: drdp[c] = drho/dp of the pseudo block (cons2prim, or
+// WS95 with ws95; nbp/nbi the interior cells' neighbour cells, read with
+// ws95 only); with corr also R -= (P V/dt)(W - Wn)
+void launch_pseudo_rhs(const DevGrid& g, const Params& P, const double* W, const double* Wn, const double* dt, const int* nbp,
+                       const int* nbi, int ws95, int corr, double* R, double* drdp, hipStream_t s);
+// D += P_ps V/dtau (diagonal blocks in row pairs)
+void launch_pseudo_diag(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, double* D,
+                        hipStream_t s);
+// y += P_ps V/dtau x
+void launch_pseudo_op(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, const double* x,
+                      double* y, hipStream_t s);
 // y = (P V / dt)^T x - Jt   (Jt = (dR/dW)^T x)
 void launch_newton_op_t(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* Jt,
                         double* y, hipStream_t s);

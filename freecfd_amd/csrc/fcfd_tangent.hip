@@ -321,3 +321,148 @@ void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const 
 }
 
 }  // namespace fcfd
+
+namespace fcfd {
+
+// ---------------------------------------------------------------------------
+// dual (pseudo) time stepping (ns_time_terms.cc:58-186)
+// ---------------------------------------------------------------------------
+// drho/dp of the pseudo-time block of each interior cell: cons2prim's rho/p,
+// or with WS95 (preconditioner_ws95, :58-105) 1/Ur^2 - drho/dT / (rho c_p),
+// Ur = min(max(|V|, 1e-5 a), a), raised to mu / (rho L) when mu > 1e-8 and to
+// 1e-5 sqrt(dpmax / rho) (dpmax over the neighbour cells, ghosts included).
+// With corr, R -= (P V/dt)(W - Wn) (the pseudo right-hand side, :117-146,
+// 181-184); P is cons2prim at W.
+template <int NV>
+__global__ void __launch_bounds__(256) pseudo_rhs_kernel(DevGrid g, Params P, const double* __restrict__ W,
+                                                         const double* __restrict__ Wn, const double* __restrict__ dt,
+                                                         const int* __restrict__ nbp, const int* __restrict__ nbi, int ws95,
+                                                         int corr, double* __restrict__ R, double* __restrict__ drdp) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld;
+    const double p = W[c], u = W[ld + c], v = W[2 * ld + c], w = W[3 * ld + c], T = W[4 * ld + c];
+    const double rho = m_rho(P, p, T);
+    const double pressure = p + P.Pref, temperature = T + P.Tref;
+    const double drho_dT = -1. * rho / temperature, drho_dp = rho / pressure;
+    double d = drho_dp;
+    if (ws95) {
+        const double a = m_a(P, p, T);
+        double Ur = smax(sqrt(u * u + v * v + w * w), 1.e-5 * a);
+        Ur = smin(Ur, a);
+        const double mu = m_visc(P, T);
+        if (mu > 1.e-8) Ur = smax(Ur, mu / (rho * g.lscale[c]));
+        double dpmax = 0.;
+        for (int k = nbp[c]; k < nbp[c + 1]; ++k) dpmax = smax(dpmax, fabs(p - W[nbi[k]]));
+        Ur = smax(Ur, 1.e-5 * sqrt(dpmax / rho));
+        d = 1. / (Ur * Ur) - drho_dT / (rho * P.Cp);
+    }
+    drdp[c] = d;
+    if (!corr) return;
+    const double H = P.Cp * temperature + 0.5 * (u * u + v * v + w * w);
+    const double s = g.vol[c] / dt[c];
+    double x[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) x[k] = W[k * ld + c] - Wn[k * ld + c];
+    const double row[6][6] = {{drho_dp, 0., 0., 0., drho_dT, 0.},
+                              {drho_dp * u, rho, 0., 0., drho_dT * u, 0.},
+                              {drho_dp * v, 0., rho, 0., drho_dT * v, 0.},
+                              {drho_dp * w, 0., 0., rho, drho_dT * w, 0.},
+                              {drho_dp * H - 1., rho * u, rho * v, rho * w, drho_dT * H + rho * P.Cp, 0.},
+                              {NV == 6 ? drho_dp * W[5 * ld + c] : 0., 0., 0., 0., NV == 6 ? drho_dT * W[5 * ld + c] : 0., rho}};
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double corr_i = 0.;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) corr_i += (row[i][j] * s) * x[j];
+        R[i * ld + c] -= corr_i;
+    }
+}
+
+// the pseudo-time block P_ps V/dtau (P_ps: cons2prim with drho/dp = drdp[c])
+// of each interior cell, as a dense nv x nv array (row-major)
+template <int NV>
+__device__ __forceinline__ void pseudo_block(const DevGrid& g, const Params& P, const double* __restrict__ W, double dtau, double d,
+                                             int c, double (&B)[NV][NV]) {
+    const size_t ld = g.ld;
+    const double p = W[c], u = W[ld + c], v = W[2 * ld + c], w = W[3 * ld + c], T = W[4 * ld + c];
+    const double rho = m_rho(P, p, T);
+    const double temperature = T + P.Tref;
+    const double drho_dT = -1. * rho / temperature;
+    const double H = P.Cp * temperature + 0.5 * (u * u + v * v + w * w);
+    const double s = g.vol[c] / dtau;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) B[i][j] = 0.;
+    B[0][0] = d * s; B[0][4] = drho_dT * s;
+    B[1][0] = d * u * s; B[1][1] = rho * s; B[1][4] = drho_dT * u * s;
+    B[2][0] = d * v * s; B[2][2] = rho * s; B[2][4] = drho_dT * v * s;
+    B[3][0] = d * w * s; B[3][3] = rho * s; B[3][4] = drho_dT * w * s;
+    B[4][0] = (d * H - 1.) * s; B[4][1] = rho * u * s; B[4][2] = rho * v * s; B[4][3] = rho * w * s;
+    B[4][4] = (drho_dT * H + rho * P.Cp) * s;
+    if constexpr (NV == 6) {
+        const double nut = W[5 * ld + c];
+        B[5][0] = d * nut * s; B[5][4] = drho_dT * nut * s; B[5][5] = rho * s;
+    }
+}
+
+// D += P_ps V/dtau (diagonal blocks in row pairs, entry k = nv row + col)
+template <int NV>
+__global__ void __launch_bounds__(256) pseudo_diag_kernel(DevGrid g, Params P, const double* __restrict__ W,
+                                                          const double* __restrict__ dtau, const double* __restrict__ drdp,
+                                                          double* __restrict__ D) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    double B[NV][NV];
+    pseudo_block<NV>(g, P, W, dtau[c], drdp[c], c, B);
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) D[pidx(g.ld, NV * i + j, c)] += B[i][j];
+}
+
+// y += P_ps V/dtau x (the matrix-free operator's pseudo-time term)
+template <int NV>
+__global__ void __launch_bounds__(256) pseudo_op_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ dtau,
+                                                        const double* __restrict__ drdp, const double* __restrict__ x,
+                                                        double* __restrict__ y) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld;
+    double B[NV][NV];
+    pseudo_block<NV>(g, P, W, dtau[c], drdp[c], c, B);
+    double xv[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) xv[j] = x[j * ld + c];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double s = 0.;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) s += B[i][j] * xv[j];
+        y[i * ld + c] += s;
+    }
+}
+
+void launch_pseudo_rhs(const DevGrid& g, const Params& P, const double* W, const double* Wn, const double* dt, const int* nbp,
+                       const int* nbi, int ws95, int corr, double* R, double* drdp, hipStream_t s) {
+    const dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL(pseudo_rhs_kernel<6>, grid, blk, 0, s, g, P, W, Wn, dt, nbp, nbi, ws95, corr, R, drdp);
+    else hipLaunchKernelGGL(pseudo_rhs_kernel<5>, grid, blk, 0, s, g, P, W, Wn, dt, nbp, nbi, ws95, corr, R, drdp);
+}
+
+void launch_pseudo_diag(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, double* D,
+                        hipStream_t s) {
+    const dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL(pseudo_diag_kernel<6>, grid, blk, 0, s, g, P, W, dtau, drdp, D);
+    else hipLaunchKernelGGL(pseudo_diag_kernel<5>, grid, blk, 0, s, g, P, W, dtau, drdp, D);
+}
+
+void launch_pseudo_op(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, const double* x,
+                      double* y, hipStream_t s) {
+    const dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL(pseudo_op_kernel<6>, grid, blk, 0, s, g, P, W, dtau, drdp, x, y);
+    else hipLaunchKernelGGL(pseudo_op_kernel<5>, grid, blk, 0, s, g, P, W, dtau, drdp, x, y);
+}
+
+}  // namespace fcfd

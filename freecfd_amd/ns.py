@@ -183,6 +183,21 @@ class NavierStokes(Grid):
         _check(self.h, lib().fcfd_loads(self.h, W.data_ptr(), out.ctypes.data, C.c_void_p(s)))
         return out.reshape(n_bc, 6)[: int(self._opt.n_bc)]
 
+    def pseudo_step(self, opts, dtau_type, dtau, ps_step, Wn=None, ws95=False, W=None, stream=None, check=True):
+        """One pseudo step of dual time stepping (fcfd_pseudo_step): opts sets
+        the physical step and the Krylov solve, (dtau_type, dtau) the pseudo
+        step, Wn ([nv][ld] device tensor) the state at the time level."""
+        W = self.W if W is None else W
+        ps = A.PseudoOpts(dtau_type=dtau_type, dtau=dtau, preconditioner=A.PSEUDO_PC_WS95 if ws95 else A.PSEUDO_PC_NONE,
+                          ps_step=ps_step, Wn_dev=None if Wn is None else Wn.data_ptr())
+        st = A.NewtonStats()
+        s = self.torch.cuda.current_stream().cuda_stream if stream is None else stream
+        rc = lib().fcfd_pseudo_step(self.h, W.data_ptr(), C.byref(opts), C.byref(ps), C.byref(st), C.c_void_p(s))
+        if check:
+            _check(self.h, rc)
+            return st
+        return rc, st
+
     def surface(self, W=None):
         """Surface quantities of the face loop (ns_assemble_linear_system.cc:
         122-127) in reference face order: (mdot[F], qdot[F], tau[F, 3]);

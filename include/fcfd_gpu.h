@@ -353,6 +353,29 @@ typedef struct fcfd_newton_stats {
     double relax;             /* factor applied to the update (1 unless max_rel_update limited it) */
 } fcfd_newton_stats;
 
+/* ---- dual (pseudo) time stepping ---- */
+#define FCFD_PSEUDO_PC_NONE 0   /* pseudo-time block = cons2prim (ns_time_terms.cc:26-56) */
+#define FCFD_PSEUDO_PC_WS95 1   /* Weiss-Smith low-Mach preconditioner (ns_time_terms.cc:58-105) */
+
+typedef struct fcfd_pseudo_opts {
+    int dtau_type;          /* FCFD_DT_FIXED, FCFD_DT_CFL_MAX or FCFD_DT_CFL_LOCAL for the pseudo step
+                               (update_pseudo_time_step, src/time_step.cc:226-290)             */
+    double dtau;            /* its CFL number, or the step size for FCFD_DT_FIXED             */
+    int preconditioner;     /* FCFD_PSEUDO_PC_NONE or FCFD_PSEUDO_PC_WS95 (pseudotime.preconditioner) */
+    int ps_step;            /* pseudo step within the physical step, from 1                   */
+    const double* Wn_dev;   /* [nv][ld_cell] state at the physical time level n; read for ps_step > 1 */
+} fcfd_pseudo_opts;
+
+/* One pseudo step of dual time stepping (NavierStokes::solve inside the
+ * pseudo-time loop of main.cc:322-345, with time_terms for
+ * pseudotime.numberofsteps > 1, ns_time_terms.cc:107-186): as
+ * fcfd_newton_step, with the physical step from opts (dt_type, cfl) and the
+ * implicit operator P V/dt + P_ps V/dtau - dR/dW (P_ps: cons2prim or the
+ * WS95 matrix) and, for ps_step > 1, the right-hand side R - (P V/dt)(W - Wn).
+ * stats->res_norm is the norm of that right-hand side. */
+int  fcfd_pseudo_step(fcfd_ctx* ctx, double* W_dev, const fcfd_newton_opts* opts, const fcfd_pseudo_opts* pseudo,
+                      fcfd_newton_stats* stats, void* stream);
+
 /* Tangent (Jacobian-vector product) of the residual: dR = (dR/dW) dW, exact
  * derivatives by forward-mode dual numbers through the residual kernels.
  * mode FCFD_JAC_JVP_FULL differentiates ghosts, gradients and limiters (the

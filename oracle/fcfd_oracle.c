@@ -2501,3 +2501,66 @@ int orc_newton_step(orc_ctx* o, double cfl, int restart, int max_iters, double r
 
 /* threads the OpenMP loops use */
 int orc_num_threads(void) { return omp_get_max_threads(); }
+
+/* ------------------------------------------------------------------ */
+/* dual (pseudo) time stepping (ns_time_terms.cc:58-186,               */
+/* time_step.cc:226-290)                                               */
+/* ------------------------------------------------------------------ */
+/* cons2prim (:26-56), or with ws95 the Weiss-Smith preconditioned matrix
+ * (:58-105): drho/dp replaced by 1/Ur^2 - drho/dT / (rho c_p) with the
+ * reference velocity Ur = min(max(|V|, 1e-5 a), a), raised to the viscous
+ * velocity mu / (rho L) when mu > 1e-8 and to 1e-5 sqrt(dpmax / rho), dpmax
+ * the largest |p_c - p_n| over the cell's neighbour cells (ghosts included) */
+static void pseudo_block(const orc_ctx* o, int c, int ws95, double P[6][6]) {
+    for (int i = 0; i < 6; ++i) for (int j = 0; j < 6; ++j) P[i][j] = 0.;
+    double rho = m_rho(o, o->p[c], o->T[c]);
+    double pressure = o->p[c] + o->opt.Pref, temperature = o->T[c] + o->opt.Tref;
+    double drho_dT = -1. * rho / temperature, drho_dp = rho / pressure;
+    double c_p = o->Cp;
+    V3 V = cellV(o, c);
+    if (ws95) {
+        double a = m_a(o, o->p[c], o->T[c]);
+        double Ur = dmax(vmag(V), 1.e-5 * a);
+        Ur = dmin(Ur, a);
+        double mu = m_visc(o, o->T[c]);
+        if (mu > 1.e-8) Ur = dmax(Ur, mu / (rho * o->lscale[c]));
+        double dpmax = 0.;
+        for (int k = 0; k < o->cnbr[c].n; ++k) dpmax = dmax(dpmax, fabs(o->p[c] - o->p[o->cnbr[c].a[k]]));
+        Ur = dmax(Ur, 1.e-5 * sqrt(dpmax / rho));
+        drho_dp = 1. / (Ur * Ur) - drho_dT / (rho * c_p);
+    }
+    double H = c_p * temperature + 0.5 * vdot(V, V);
+    P[0][0] = drho_dp; P[0][4] = drho_dT;
+    P[1][0] = drho_dp * V.c[0]; P[1][1] = rho; P[1][4] = drho_dT * V.c[0];
+    P[2][0] = drho_dp * V.c[1]; P[2][2] = rho; P[2][4] = drho_dT * V.c[1];
+    P[3][0] = drho_dp * V.c[2]; P[3][3] = rho; P[3][4] = drho_dT * V.c[2];
+    P[4][0] = drho_dp * H - 1.; P[4][1] = rho * V.c[0]; P[4][2] = rho * V.c[1]; P[4][3] = rho * V.c[2];
+    P[4][4] = drho_dT * H + rho * c_p;
+    if (o->nv == 6) { P[5][0] = drho_dp * o->nut[c]; P[5][4] = drho_dT * o->nut[c]; P[5][5] = rho; }
+}
+
+/* The implicit system of one pseudo step (time_terms with
+ * pseudotime.numberofsteps > 1): blocks[N][nv][nv] = P V / dt + P_ps V / dtau
+ * (P_ps: cons2prim, or the WS95 matrix) and, from the residual of the last
+ * orc_assemble, rhs[N][nv] = R - (P V / dt)(W - Wn) for ps_step > 1 (R
+ * alone at ps_step 1, :117-146).  dt and dtau: N each. */
+int orc_pseudo_system(orc_ctx* o, const double* dt, const double* dtau, int ws95, int ps_step, const double* Wn, double* blocks,
+                      double* rhs) {
+    const int nv = o->nv;
+    const double* q[6] = {o->p, o->u, o->v, o->w, o->T, o->nut};
+    for (int c = 0; c < o->ncell; ++c) {
+        double P[6][6], Pp[6][6];
+        pseudo_block(o, c, 0, P);
+        pseudo_block(o, c, ws95, Pp);
+        for (int i = 0; i < nv; ++i) {
+            double corr = 0.;
+            for (int j = 0; j < nv; ++j) {
+                double value = P[i][j] * o->vol[c] / dt[c];
+                blocks[(size_t)nv * nv * c + nv * i + j] = value + Pp[i][j] * o->vol[c] / dtau[c];
+                if (ps_step > 1) corr += value * (q[j][c] - Wn[(size_t)nv * c + j]);
+            }
+            rhs[(size_t)nv * c + i] = o->Rs[NVM * c + i] - corr;
+        }
+    }
+    return 0;
+}

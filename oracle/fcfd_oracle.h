@@ -58,6 +58,11 @@ int orc_get_face_flux(const orc_ctx* o, double* C, double* D);
  * iterations, ms of residual+Jacobian, time terms+inverses, Krylov, update */
 int orc_newton_step(orc_ctx* o, double cfl, int restart, int max_iters, double rtol, double* stats);
 int orc_num_threads(void);
+/* one pseudo step's system after orc_assemble: blocks = P V/dt + P_ps V/dtau
+ * (P_ps = cons2prim, or WS95 with ws95), rhs = R - (P V/dt)(W - Wn) for
+ * ps_step > 1 (ns_time_terms.cc:58-186); dt, dtau N; Wn, rhs N x nv */
+int orc_pseudo_system(orc_ctx* o, const double* dt, const double* dtau, int ws95, int ps_step, const double* Wn, double* blocks,
+                      double* rhs);
 /* mdot.face, qdot.bc, tau.bc of the last orc_assemble (F, F, 3F; reference face order) */
 int orc_get_surface(const orc_ctx* o, double* mdot, double* qdot, double* tau);
 /* apply_bcs on BC `bc`: left face state WL and centre WLc, ghost centre WRc

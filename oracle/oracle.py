@@ -51,6 +51,8 @@ def lib():
         L.orc_get_surface.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_newton_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double, C.c_void_p]
         L.orc_num_threads.restype = C.c_int
+        L.orc_pseudo_system.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]
         L.orc_bc_probe.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
         L.orc_ausm_probe.argtypes = [C.c_double, C.c_double, C.c_void_p]
         L.orc_gelimd_probe.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -181,6 +183,16 @@ class Oracle:
     def state(self):
         """interior state N x nv"""
         return self.fields()[0][:self.info.n_cells].copy()
+
+    def pseudo_system(self, dt, dtau, ws95, ps_step, Wn):
+        """after residual(): (blocks N x nv x nv = P V/dt + P_ps V/dtau,
+        rhs N x nv = R - (P V/dt)(W - Wn) for ps_step > 1)"""
+        N, nv = self.info.n_cells, self.info.nv
+        blocks, rhs = np.zeros((N, nv, nv)), np.zeros((N, nv))
+        arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (dt, dtau, Wn)]
+        self._check(lib().orc_pseudo_system(self.h, _p(arr[0]), _p(arr[1]), int(ws95), int(ps_step), _p(arr[2]), _p(blocks),
+                                            _p(rhs)))
+        return blocks, rhs
 
     def surface(self):
         """(mdot[F], qdot[F], tau[F, 3]) of the last residual evaluation"""
