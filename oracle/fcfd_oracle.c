@@ -475,6 +475,121 @@ static void create_boundary_ghosts(orc_ctx* o) {
 /* ------------------------------------------------------------------ */
 /* set_bcs (src/set_bcs.cc:35-260), BC regions = mesh node sets       */
 /* ------------------------------------------------------------------ */
+/* closest wall face: the brute-force scan (set_bcs.cc:178-249): the
+ * smallest |x - x_w| over the wall face centroids, the first in face order
+ * on ties (*who; -1 without walls) */
+static int g_wall_brute = 0;
+static double wall_brute(const V3* wc, int n, V3 x, int* who) {
+    double best = 1.e20;
+    *who = -1;
+    for (int s = 0; s < n; ++s) {
+        double d = vmag(vsub(x, wc[s]));
+        if (best > d) { best = d; *who = s; }
+    }
+    return best;
+}
+
+/* uniform buckets over the wall centroids' bounding box, about two
+ * centroids per bucket, each bucket's centroids in face order */
+typedef struct { int n[3]; double lo[3], hi[3], h[3]; int *ptr, *idx; } wallgrid_t;
+
+static void wallgrid_build(wallgrid_t* g, const V3* wc, int nw) {
+    for (int a = 0; a < 3; ++a) { g->lo[a] = 1.e300; g->hi[a] = -1.e300; }
+    for (int s = 0; s < nw; ++s)
+        for (int a = 0; a < 3; ++a) { g->lo[a] = dmin(g->lo[a], wc[s].c[a]); g->hi[a] = dmax(g->hi[a], wc[s].c[a]); }
+    double ext[3], vol = 1.;
+    int nd = 0;
+    for (int a = 0; a < 3; ++a) { ext[a] = nw ? g->hi[a] - g->lo[a] : 0.; if (ext[a] > 0.) { vol *= ext[a]; nd++; } }
+    double target = nw > 2 ? nw / 2.0 : 1.0, side = nd ? pow(vol / target, 1.0 / nd) : 1.0;
+    int total = 1;
+    for (int a = 0; a < 3; ++a) {
+        g->n[a] = (ext[a] > 0. && side > 0.) ? (int)ceil(ext[a] / side) : 1;
+        if (g->n[a] < 1) g->n[a] = 1;
+        if (g->n[a] > 1024) g->n[a] = 1024;
+        g->h[a] = ext[a] > 0. ? ext[a] / g->n[a] : 1.;
+        total *= g->n[a];
+    }
+    g->ptr = (int*)calloc(total + 1, sizeof(int));
+    g->idx = (int*)malloc(sizeof(int) * (nw + 1));
+    int* cell = (int*)malloc(sizeof(int) * (nw + 1));
+    for (int s = 0; s < nw; ++s) {
+        int b[3];
+        for (int a = 0; a < 3; ++a) {
+            b[a] = (int)((wc[s].c[a] - g->lo[a]) / g->h[a]);
+            if (b[a] < 0) b[a] = 0;
+            if (b[a] >= g->n[a]) b[a] = g->n[a] - 1;
+        }
+        cell[s] = (b[2] * g->n[1] + b[1]) * g->n[0] + b[0];
+        g->ptr[cell[s] + 1]++;
+    }
+    for (int k = 0; k < total; ++k) g->ptr[k + 1] += g->ptr[k];
+    int* fill = (int*)malloc(sizeof(int) * (total + 1));
+    memcpy(fill, g->ptr, sizeof(int) * (total + 1));
+    for (int s = 0; s < nw; ++s) g->idx[fill[cell[s]]++] = s;   /* ascending face order per bucket */
+    free(fill); free(cell);
+}
+
+static void wallgrid_free(wallgrid_t* g) { free(g->ptr); free(g->idx); }
+
+/* the brute force's result by shells of buckets around the query's bucket:
+ * a bucket outside the searched box along axis a is at least
+ * sqrt(slab_a^2 + sum_{b != a} gap_b^2) away (slab: the distance to the box's
+ * side; gap: to the grid's extent), so the search stops once the best
+ * distance is below every such bound */
+static double wallgrid_nearest(const wallgrid_t* g, const V3* wc, V3 x, int* who) {
+    double best = 1.e20;
+    int bi = -1;
+    *who = -1;
+    if (g->ptr[g->n[0] * g->n[1] * g->n[2]] == 0) return best;
+    int q[3];
+    double gap[3];
+    for (int a = 0; a < 3; ++a) {
+        q[a] = (int)floor((x.c[a] - g->lo[a]) / g->h[a]);
+        if (q[a] < 0) q[a] = 0;
+        if (q[a] >= g->n[a]) q[a] = g->n[a] - 1;
+        gap[a] = dmax(0., dmax(g->lo[a] - x.c[a], x.c[a] - g->hi[a]));
+    }
+    for (int r = 0;; ++r) {
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) { lo[a] = q[a] - r; hi[a] = q[a] + r; if (lo[a] < 0) lo[a] = 0; if (hi[a] >= g->n[a]) hi[a] = g->n[a] - 1; }
+        for (int k = lo[2]; k <= hi[2]; ++k)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+                for (int i = lo[0]; i <= hi[0]; ++i) {
+                    /* only the new shell's buckets */
+                    if (r > 0 && i > q[0] - r && i < q[0] + r && j > q[1] - r && j < q[1] + r && k > q[2] - r && k < q[2] + r) continue;
+                    int b = (k * g->n[1] + j) * g->n[0] + i;
+                    for (int e = g->ptr[b]; e < g->ptr[b + 1]; ++e) {
+                        int s = g->idx[e];
+                        double d = vmag(vsub(x, wc[s]));
+                        if (d < best || (d == best && s < bi)) { best = d; bi = s; }
+                    }
+                }
+        /* lower bound on every bucket beyond the box */
+        double bound = 1.e300;
+        int beyond = 0;
+        double g2 = gap[0] * gap[0] + gap[1] * gap[1] + gap[2] * gap[2];
+        for (int a = 0; a < 3; ++a) {
+            double others = g2 - gap[a] * gap[a];
+            if (q[a] - r > 0) {
+                double slab = dmax(0., x.c[a] - (g->lo[a] + (q[a] - r) * g->h[a]));
+                bound = dmin(bound, sqrt(slab * slab + others));
+                beyond = 1;
+            }
+            if (q[a] + r < g->n[a] - 1) {
+                double slab = dmax(0., (g->lo[a] + (q[a] + r + 1) * g->h[a]) - x.c[a]);
+                bound = dmin(bound, sqrt(slab * slab + others));
+                beyond = 1;
+            }
+        }
+        if (!beyond || best < bound) break;
+    }
+    *who = bi;
+    return best;
+}
+
+/* the tests switch the exact bucketed search to the reference's scan */
+int orc_set_wall_brute_force(int on) { g_wall_brute = on; return 0; }
+
 static void set_bcs_grid(orc_ctx* o) {
     o->fsym = (int*)calloc(o->nface, sizeof(int));
     o->face2bc = (int*)calloc(o->nface, sizeof(int));
@@ -513,28 +628,33 @@ static void set_bcs_grid(orc_ctx* o) {
         int b = o->fbc[f];
         if (b >= 0 && o->opt.bc[b].type == FCFD_BC_WALL && o->opt.bc[b].kind != FCFD_KIND_SLIP) { wc[k] = o->fc[f]; wn[k] = o->fn[f]; k++; }
     }
+    /* The reference scans every wall face for every cell (and face); the
+     * minimum of the same distances, with the first closest face in face
+     * order on exact ties, is found here by a bucketed search, so the
+     * results are the brute force's bit for bit (brute_force = 1 runs the
+     * scan itself, for the tests) */
+    wallgrid_t wg;
+    wallgrid_build(&wg, wc, nsf);
     o->wdist = (double*)malloc(sizeof(double) * o->ncap);
+#pragma omp parallel for schedule(dynamic, 1024)
     for (int c = 0; c < o->ntot; ++c) {
-        o->wdist[c] = 1.e20;
-        for (int s = 0; s < nsf; ++s) {
-            double d = vmag(vsub(o->cc[c], wc[s]));
-            if (o->wdist[c] > d) o->wdist[c] = d;
-        }
+        int who;
+        o->wdist[c] = g_wall_brute ? wall_brute(wc, nsf, o->cc[c], &who) : wallgrid_nearest(&wg, wc, o->cc[c], &who);
     }
     o->fwd = (double*)malloc(sizeof(double) * o->nface);
     o->fdiss = (double*)malloc(sizeof(double) * o->nface);
+#pragma omp parallel for schedule(dynamic, 1024)
     for (int f = 0; f < o->nface; ++f) {
         int b = o->fbc[f];
         o->fwd[f] = 1.e20;
         if (b < 0 || o->opt.bc[b].type != FCFD_BC_WALL || o->opt.bc[b].kind == FCFD_KIND_SLIP) {
-            V3 fN = vzero();
-            for (int s = 0; s < nsf; ++s) {
-                double d = vmag(vsub(o->fc[f], wc[s]));
-                if (o->fwd[f] > d) { o->fwd[f] = d; fN = wn[s]; }
-            }
+            int who = -1;
+            o->fwd[f] = g_wall_brute ? wall_brute(wc, nsf, o->fc[f], &who) : wallgrid_nearest(&wg, wc, o->fc[f], &who);
+            V3 fN = who >= 0 ? wn[who] : vzero();
             o->fdiss[f] = 1. - fabs(fN.c[0] * o->fn[f].c[0] + fN.c[1] * o->fn[f].c[1] + fN.c[2] * o->fn[f].c[2]);
         } else { o->fwd[f] = 0.; o->fdiss[f] = 0.; }
     }
+    wallgrid_free(&wg);
     free(wc); free(wn);
 }
 

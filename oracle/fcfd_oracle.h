@@ -58,6 +58,9 @@ int orc_get_face_flux(const orc_ctx* o, double* C, double* D);
  * iterations, ms of residual+Jacobian, time terms+inverses, Krylov, update */
 int orc_newton_step(orc_ctx* o, double cfl, int restart, int max_iters, double rtol, double* stats);
 int orc_num_threads(void);
+/* 1: wall distances by the reference's brute-force scan in later orc_create
+ * calls; 0 (default): the exact bucketed search, bitwise the same */
+int orc_set_wall_brute_force(int on);
 /* one pseudo step's system after orc_assemble: blocks = P V/dt + P_ps V/dtau
  * (P_ps = cons2prim, or WS95 with ws95), rhs = R - (P V/dt)(W - Wn) for
  * ps_step > 1 (ns_time_terms.cc:58-186); dt, dtau N; Wn, rhs N x nv */

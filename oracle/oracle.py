@@ -51,6 +51,7 @@ def lib():
         L.orc_get_surface.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_newton_step.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double, C.c_void_p]
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_wall_brute_force.argtypes = [C.c_int]
         L.orc_pseudo_system.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
         L.orc_bc_probe.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 6
@@ -274,3 +275,9 @@ def wtli(point, ids, pts, dim=3, max_stencil=12, tol=0.99):
 def num_threads():
     """threads the restatement's OpenMP loops use (OMP_NUM_THREADS or all cores)"""
     return int(lib().orc_num_threads())
+
+
+def set_wall_brute_force(on):
+    """wall distances of later Oracle() set-ups by the reference's scan (on)
+    or the exact bucketed search (off, the default; bitwise the same)"""
+    lib().orc_set_wall_brute_force(int(bool(on)))

@@ -5,9 +5,12 @@
   the residual parity bar of test_residual_gpu (ghost states 1e-14,
   gradients 1e-11 of their scale, residual 1e-12 of the per-cell summation
   scale) on seeded perturbed states.  The restatement needs ~15 s per mesh.
-- The 5,046,272-cell SA-RANS plate: size-independent properties only, because
-  the restatement's brute-force wall distance (every cell against every wall
-  face, ~1.6e11 distance evaluations on one core) does not finish in a test:
+- The 5,046,272-cell SA-RANS plate (the benchmark workload) against the CPU
+  restatement, whose exact bucketed wall search gives the reference's
+  brute-force distances bit for bit in seconds: every set-up metric
+  bit-exact (the device's wall distances included), the residual parity bar
+  above, and the FD Jacobian on 4,000 sampled faces and 4,000 sampled cells
+  (1e-6 per block: the SA source's pow); plus size-independent properties:
   two residual evaluations are bitwise equal; the exact tangent is linear
   (J(a u + b v) = a Ju + b Jv to rounding); the adjoint is its transpose
   (<v, J u> = <J^T v, u> to 1e-11 of sum |v Ju|).
@@ -103,4 +106,53 @@ def test_sa_plate_full_size_properties():
     print("  tangent linearity: max |J(au+bv) - aJu - bJv| / max|aJu| = %.2e" % lin, flush=True)
     assert lin < 1e-10
     assert _dot(ns, N, 6, 11) < 1e-11
+    ns.close()
+
+
+def test_sa_plate_full_size_matches_oracle():
+    from oracle.oracle import Oracle
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.plate_case(sa=True, noise=0.01)
+    N = cs.n_cells
+    print("case", cs.name, "N", N, flush=True)
+    o = Oracle(cs.raw, cs.opt)
+    mo = o.metrics()
+    Ro, Rabs = o.residual(cs.W0, with_jacobian=True)
+    Wo, Go, phio = o.fields()
+    JLo, JRo, Do = o.jacobian()
+    o.close()
+    print("  oracle done", flush=True)
+    ns = NavierStokes(cs.raw, cs.opt)
+    mg = ns.metrics()
+    for key in ("face_parent", "face_neighbor", "face_bc", "face_centroid", "face_normal", "face_area", "face_tangent1",
+                "cell_centroid", "cell_volume", "cell_lengthscale", "wall_distance", "cell_face_ptr", "cell_faces",
+                "grad_ptr", "grad_idx", "grad_w", "avg_ptr", "avg_idx", "avg_w", "grid_lengthscale"):
+        assert np.array_equal(mg[key], mo[key]), key
+    print("  set-up metrics bit-exact (wall distance from the device)", flush=True)
+    ns.set_state(cs.W0)
+    ns.assemble_residual()
+    ns.status()
+    Rg = ns.residual_host()
+    Wg = ns.state_host(include_ghosts=True)
+    Gg, phig = ns.gradients_limiters_host()
+    gscale = np.abs(Wo).max(axis=0)
+    assert np.all(np.abs(Wg - Wo) <= 1e-14 * gscale + 1e-300), "ghost state mismatch"
+    gerr = (np.abs(Gg - Go) / (np.abs(Go).max(axis=0) + 1e-300)).max()
+    perr = np.abs(phig - phio).max()
+    err = (np.abs(Rg - Ro) / (Rabs + 1e-300)).max()
+    print("  gradients %.3e  limiters %.3e  max |dR|/S = %.3e" % (gerr, perr, err), flush=True)
+    assert gerr <= 1e-11 and perr <= 1e-9 and err <= 1e-12
+    ns.fd_jacobian()
+    ns.status()
+    JLg, JRg, Dg = ns.jacobian_host()
+    rng = np.random.default_rng(17)
+    faces = rng.choice(len(JLo), 4000, replace=False)
+    cells = rng.choice(N, 4000, replace=False)
+    def blockwise(a, b):
+        keep = np.abs(b).reshape(len(b), -1).max(axis=1) > 0
+        a, b = a[keep], b[keep]
+        return float((np.abs(a - b).reshape(len(b), -1).max(axis=1) / np.abs(b).reshape(len(b), -1).max(axis=1)).max())
+    e = (blockwise(JLg[faces], JLo[faces]), blockwise(JRg[faces], JRo[faces]), blockwise(Dg[cells], Do[cells]))
+    print("  FD Jacobian on 4000 faces / cells: JL %.2e  JR %.2e  D %.2e" % e, flush=True)
+    assert max(e) <= 1e-6
     ns.close()

@@ -48,3 +48,26 @@ def test_setup_lsqr_hex_and_curvilinear_tets(oracle_lib):
     cs = cases.tetbox_case(3)
     cs.opt.grad_other = A.GRAD_CURVILINEAR
     _check(cs)
+
+
+@pytest.mark.parametrize("flux", [A.FLUX_ROE, A.FLUX_SW])
+def test_oracle_bucketed_wall_distance_equals_the_scan(flux, oracle_lib):
+    """The oracle's exact bucketed wall search gives the reference's
+    brute-force distances bit for bit, and (Steger-Warming) the same first
+    closest wall face, whose normal sets the face dissipation factor: the
+    residual is bitwise equal."""
+    from oracle import oracle as O
+    cs = cases.plate_case(16, 10, 4, noise=0.01, first=1e-3, sa=True)
+    cs.opt.convective_flux = flux
+    try:
+        O.set_wall_brute_force(True)
+        a = O.Oracle(cs.raw, cs.opt)
+        O.set_wall_brute_force(False)
+        b = O.Oracle(cs.raw, cs.opt)
+    finally:
+        O.set_wall_brute_force(False)
+    wa, wb = a.metrics()["wall_distance"], b.metrics()["wall_distance"]
+    assert np.array_equal(wa, wb) and np.all(wa < 1e20)
+    Ra, _ = a.residual(cs.W0)
+    Rb, _ = b.residual(cs.W0)
+    assert np.array_equal(Ra, Rb)
