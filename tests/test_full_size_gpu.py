@@ -124,10 +124,10 @@ def test_sa_plate_full_size_matches_oracle():
     print("  oracle done", flush=True)
     ns = NavierStokes(cs.raw, cs.opt)
     mg = ns.metrics()
-    for key in ("face_parent", "face_neighbor", "face_bc", "face_centroid", "face_normal", "face_area", "face_tangent1",
-                "cell_centroid", "cell_volume", "cell_lengthscale", "wall_distance", "cell_face_ptr", "cell_faces",
-                "grad_ptr", "grad_idx", "grad_w", "avg_ptr", "avg_idx", "avg_w", "grid_lengthscale"):
-        assert np.array_equal(mg[key], mo[key]), key
+    for key, a in mo.items():
+        if key != "cell_perm":
+            assert a.shape == mg[key].shape and np.array_equal(a, mg[key]), key
+    del mo, mg
     print("  set-up metrics bit-exact (wall distance from the device)", flush=True)
     ns.set_state(cs.W0)
     ns.assemble_residual()
