@@ -60,6 +60,8 @@ struct fcfd_ctx {
     float *Dinv32 = nullptr, *D32 = nullptr, *JL32 = nullptr, *JR32 = nullptr, *b32 = nullptr, *x32 = nullptr;
     // tangent workspace (lazy)
     double *dG = nullptr, *dPHI = nullptr, *dFX = nullptr, *jv_in = nullptr, *jv_out = nullptr, *jv_R = nullptr;
+    // FP32 tangent workspace of the FP32 matrix-free Newton operator (lazy)
+    float *dG32 = nullptr, *dPHI32 = nullptr, *dFX32 = nullptr, *jv_in32 = nullptr, *jv_R32 = nullptr;
     // cell colouring for the block Gauss-Seidel preconditioner (lazy)
     Colors colors;
     // adjoint and loads (lazy)
@@ -802,6 +804,17 @@ static int ensure_tangent(fcfd_ctx* c) {
     return FCFD_OK;
 }
 
+static int ensure_tangent32(fcfd_ctx* c) {
+    if (c->dG32) return FCFD_OK;
+    const DevGrid& g = c->dg;
+    int rc;
+    const size_t nv = g.nv;
+    if ((rc = dalloc(c, 2 * pair_groups(3 * (int)nv) * g.ld, &c->dG32)) || (rc = dalloc(c, 2 * pair_groups((int)nv) * g.ld, &c->dPHI32)) ||
+        (rc = dalloc(c, nv * g.F, &c->dFX32)) || (rc = dalloc(c, nv * g.ld, &c->jv_in32)) || (rc = dalloc(c, nv * g.ld, &c->jv_R32)))
+        return rc;
+    return FCFD_OK;
+}
+
 // Reverse-sweep connectivity (half-face twins, ghost half-faces, transposed
 // gradient map), the per-BC load face lists and the adjoint workspace.
 static int ensure_adjoint(fcfd_ctx* c) {
@@ -1108,6 +1121,12 @@ static int newton_impl(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, co
         return fail(c, FCFD_EINVAL, "unknown Jacobian mode");
     const bool mfree = o->jacobian != FCFD_JAC_FD;
     if (mfree && (rc0 = ensure_tangent(c))) return rc0;
+    // FP32 Krylov + exact tangent: the tangent's derivative lane in FP32 too
+    // (FP64 lane behind conversions where Green-Gauss cells need it, or with
+    // FCFD_TANGENT_FP64=1, the comparison case of the tests)
+    const char* t64 = std::getenv("FCFD_TANGENT_FP64");
+    const bool tan32 = mfree && o->krylov_fp32 && !c->dg.ggc && !(t64 && t64[0] == '1');
+    if (tan32 && (rc0 = ensure_tangent32(c))) return rc0;
     if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad Krylov options");
     if (o->precond != FCFD_PC_BLOCK_JACOBI && o->precond != FCFD_PC_BLOCK_SGS && o->precond != FCFD_PC_MULTIGRID)
         return fail(c, FCFD_EINVAL, "unknown preconditioner");
@@ -1202,11 +1221,24 @@ static int newton_impl(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, co
             dev_bj_apply<float>(g, c->Dinv32, x, y, ss);
         }
     };
+    // the FP32 operator: y = (P V/dt) x - J x (+ the pseudo-time term) with the
+    // tangent's derivative lane in FP32 (values FP64, from W)
+    auto jvp_op32 = [&](const float* x, float* y, hipStream_t ss) {
+        (void)hipMemcpyAsync(c->jv_in32, x, n5 * sizeof(float), hipMemcpyDeviceToDevice, ss);
+        launch_jvp(g, c->P, W_dev, c->jv_in32, c->G, c->dG32, c->PHI, c->dPHI32, c->dref, c->dFX32, c->jv_R32,
+                   o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss);
+        launch_newton_op(g, c->P, W_dev, c->dt, c->jv_in32, c->jv_R32, y, ss);
+        if (ps) launch_pseudo_op(g, c->P, W_dev, c->dtau, c->drdp, c->jv_in32, y, ss);
+    };
     if (fp32 && mfree) {
         LinOp<float> A{[&](const float* x, float* y, hipStream_t ss) {
-            dev_convert<float, double>(n5, x, c->jv_out, ss);
-            jvp_op(c->jv_out, c->jv_out, ss);
-            dev_convert<double, float>(n5, c->jv_out, y, ss);
+            if (tan32) {
+                jvp_op32(x, y, ss);
+            } else {
+                dev_convert<float, double>(n5, x, c->jv_out, ss);
+                jvp_op(c->jv_out, c->jv_out, ss);
+                dev_convert<double, float>(n5, c->jv_out, y, ss);
+            }
         }};
         LinOp<float> M{prec32};
         rc = gmres<float>(g, A, M, c->b32, c->x32, go, c->w32, gs, s);

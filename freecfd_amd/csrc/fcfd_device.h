@@ -116,6 +116,7 @@ void launch_surface(const DevGrid& g, const Params& P, const double* W, const do
                     hipStream_t s);
 // ... and their ordered per-cell sums into R (no source: tangents)
 void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s);
+void launch_cell_sum(const DevGrid& g, const float* FX, float* R, hipStream_t s);   // FP32 tangent sums
 // the residual's per-cell sums, plus V S (src, from launch_grad_lim) in the
 // nu~ row with the SA model
 void launch_residual_sum(const DevGrid& g, const double* src, const double* FX, double* R, hipStream_t s);

@@ -55,36 +55,38 @@ struct CellLoader {
     }
 };
 
-// tangent cell loader: value = primal arrays, derivative = tangent arrays.
-// FULL = false: gradients, limiters and ghost states carry no tangent (the
-// reference FD Jacobian's frozen-gradient linearisation).
-template <int NV, bool FULL>
+// tangent cell loader: value = primal arrays, derivative = tangent arrays
+// (of the derivative lane's type TD).  FULL = false: gradients, limiters and
+// ghost states carry no tangent (the reference FD Jacobian's frozen-gradient
+// linearisation).
+template <int NV, bool FULL, typename TD = double>
 struct DualLoader {
+    using D = DualT<TD>;
     CellLoader<NV> p;
-    const double* __restrict__ dW;
-    const double* __restrict__ dG;
-    const double* __restrict__ dPHI;
+    const TD* __restrict__ dW;
+    const TD* __restrict__ dG;
+    const TD* __restrict__ dPHI;
     int N;
-    __device__ __forceinline__ void load(int c, bool ghost, Dual q[NV], Vec<Dual> gr[NV], Dual ph[NV]) const {
+    __device__ __forceinline__ void load(int c, bool ghost, D q[NV], Vec<D> gr[NV], D ph[NV]) const {
         double qv[NV], phv[NV];
         Vec<double> gv[NV];
         p.load(c, ghost, qv, gv, phv);
         const size_t ld = p.ld;
-        double dg[3 * NV], dp[NV];
+        TD dg[3 * NV], dp[NV];
         if (FULL) {
             load_pairs<3 * NV>(dG, ld, c, dg);
             if (p.has_phi && !ghost) load_pairs<NV>(dPHI, ld, c, dp);
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            double dq = (FULL || c < N) ? dW[v * ld + c] : 0.0;
-            q[v] = Dual(qv[v], dq);
+            TD dq = (FULL || c < N) ? dW[v * ld + c] : TD(0);
+            q[v] = D(qv[v], dq);
             if (FULL) {
-                gr[v] = mk<Dual>(Dual(gv[v].x, dg[3 * v]), Dual(gv[v].y, dg[3 * v + 1]), Dual(gv[v].z, dg[3 * v + 2]));
-                ph[v] = Dual(phv[v], (p.has_phi && !ghost) ? dp[v] : 0.0);
+                gr[v] = mk<D>(D(gv[v].x, dg[3 * v]), D(gv[v].y, dg[3 * v + 1]), D(gv[v].z, dg[3 * v + 2]));
+                ph[v] = D(phv[v], (p.has_phi && !ghost) ? dp[v] : TD(0));
             } else {
-                gr[v] = mk<Dual>(Dual(gv[v].x), Dual(gv[v].y), Dual(gv[v].z));
-                ph[v] = Dual(phv[v]);
+                gr[v] = mk<D>(D(gv[v].x), D(gv[v].y), D(gv[v].z));
+                ph[v] = D(phv[v]);
             }
         }
     }

@@ -409,9 +409,9 @@ __global__ void __launch_bounds__(64) surface_kernel(DevGrid g, Params P, const 
 // ns_assemble_linear_system.cc:131-140 starting from VecSet(rhs, 0) -- then,
 // with SRC (SA model), + V_c S_c (src[c], written by grad_lim_kernel) in the
 // nu~ row
-template <int NV, bool SRC>
-__global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, const double* __restrict__ src, const double* __restrict__ FX,
-                                                       double* __restrict__ Rout) {
+template <int NV, bool SRC, typename T>
+__global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, const double* __restrict__ src, const T* __restrict__ FX,
+                                                       T* __restrict__ Rout) {
     // all half-face entries in one round, then all their flux rows in one
     // round (entries past the cell's count repeat its last one and are
     // masked), then the sum in order
@@ -422,21 +422,21 @@ __global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, const double* 
     int fs[kMaxCellFaces];
 #pragma unroll
     for (int k = 0; k < kMaxCellFaces; ++k) fs[k] = g.hf_face[h0 + min(k, nh - 1)];
-    double x[kMaxCellFaces][NV];
+    T x[kMaxCellFaces][NV];
 #pragma unroll
     for (int k = 0; k < kMaxCellFaces; ++k) {
         const int f = fs[k] >= 0 ? fs[k] : ~fs[k];
 #pragma unroll
         for (int v = 0; v < NV; ++v) x[k][v] = FX[v * Fs + f];
     }
-    double s[NV];
+    T s[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) s[v] = 0.0;
+    for (int v = 0; v < NV; ++v) s[v] = T(0);
 #pragma unroll
     for (int k = 0; k < kMaxCellFaces; ++k) {
         if (k < nh) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) s[v] += (fs[k] >= 0 ? x[k][v] : -1. * x[k][v]) + 0.;
+            for (int v = 0; v < NV; ++v) s[v] += (fs[k] >= 0 ? x[k][v] : T(-1) * x[k][v]) + T(0);
         }
     }
     if constexpr (SRC && NV == 6) s[5] += src[c];
@@ -621,14 +621,20 @@ void launch_flux(const DevGrid& g, const Params& P, const double* W, const doubl
 
 void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s) {
     dim3 grid((g.N + 255) / 256), blk(256);
-    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, false>), grid, blk, 0, s, g, nullptr, FX, R);
-    else hipLaunchKernelGGL((cell_sum_kernel<5, false>), grid, blk, 0, s, g, nullptr, FX, R);
+    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, false, double>), grid, blk, 0, s, g, nullptr, FX, R);
+    else hipLaunchKernelGGL((cell_sum_kernel<5, false, double>), grid, blk, 0, s, g, nullptr, FX, R);
+}
+
+void launch_cell_sum(const DevGrid& g, const float* FX, float* R, hipStream_t s) {
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, false, float>), grid, blk, 0, s, g, nullptr, FX, R);
+    else hipLaunchKernelGGL((cell_sum_kernel<5, false, float>), grid, blk, 0, s, g, nullptr, FX, R);
 }
 
 void launch_residual_sum(const DevGrid& g, const double* src, const double* FX, double* R, hipStream_t s) {
     dim3 grid((g.N + 255) / 256), blk(256);
-    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, true>), grid, blk, 0, s, g, src, FX, R);
-    else hipLaunchKernelGGL((cell_sum_kernel<5, false>), grid, blk, 0, s, g, nullptr, FX, R);
+    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, true, double>), grid, blk, 0, s, g, src, FX, R);
+    else hipLaunchKernelGGL((cell_sum_kernel<5, false, double>), grid, blk, 0, s, g, nullptr, FX, R);
 }
 
 // CFL_MAX and FIXED time steps and the largest CFL number

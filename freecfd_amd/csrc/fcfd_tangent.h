@@ -8,9 +8,15 @@ namespace fcfd {
 // frozen-gradient linearisation.  dFX is [5][F] scratch.
 void launch_jvp(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
                 double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s);
+// the same with an FP32 derivative lane: FP64 values (the residual's), FP32
+// tangents dW, dG, dPHI, dFX, dR (not on meshes with Green-Gauss cells)
+void launch_jvp(const DevGrid& g, const Params& P, double* W, float* dW, double* G, float* dG, double* PHI, float* dPHI,
+                double* dref, float* dFX, float* dR, bool full, int* status, hipStream_t s);
 // y = (P V / dt) x - dR
 void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* dR,
                       double* y, hipStream_t s);
+void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const double* dt, const float* x, const float* dR,
+                      float* y, hipStream_t s);
 // dual time stepping: drdp[c] = drho/dp of the pseudo block (cons2prim, or
 // WS95 with ws95; nbp/nbi the interior cells' neighbour cells, read with
 // ws95 only); with corr also R -= (P V/dt)(W - Wn)
@@ -22,6 +28,8 @@ void launch_pseudo_diag(const DevGrid& g, const Params& P, const double* W, cons
 // y += P_ps V/dtau x
 void launch_pseudo_op(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, const double* x,
                       double* y, hipStream_t s);
+void launch_pseudo_op(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, const float* x,
+                      float* y, hipStream_t s);
 // y = (P V / dt)^T x - Jt   (Jt = (dR/dW)^T x)
 void launch_newton_op_t(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* Jt,
                         double* y, hipStream_t s);

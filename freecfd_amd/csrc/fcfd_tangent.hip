@@ -16,6 +16,8 @@
 // is templated on NV (5, or 6 with the SA model).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fcfd_face.h"
 #include "fcfd_limiter.h"
 #include "fcfd_tangent.h"
@@ -23,16 +25,18 @@
 namespace fcfd {
 
 // neighbour states with their tangents, for limiter_phi
+template <typename TD>
 struct TangentNb {
     const double* __restrict__ W;
-    const double* __restrict__ dW;
+    const TD* __restrict__ dW;
     size_t ld;
-    __device__ __forceinline__ Dual nb(int, int j, int v) const { return Dual(W[v * ld + j], dW[v * ld + j]); }
+    __device__ __forceinline__ DualT<TD> nb(int, int j, int v) const { return DualT<TD>(W[v * ld + j], dW[v * ld + j]); }
 };
 
-template <int NV>
-__global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, double* __restrict__ W, double* __restrict__ dW,
+template <int NV, typename TD>
+__global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, double* __restrict__ W, TD* __restrict__ dW,
                                                       int* __restrict__ status) {
+    using Dual = DualT<TD>;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.B) return;
     int f = g.gh_face[i];
@@ -58,11 +62,12 @@ __global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, doubl
     if constexpr (NV == 6) { W[5 * ld + gi] = R.nut_center.v; dW[5 * ld + gi] = R.nut_center.d; }
 }
 
-template <int LIM, int NV>
+template <int LIM, int NV, typename TD>
 __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, const double* __restrict__ W,
-                                                         const double* __restrict__ dW, double* __restrict__ G,
-                                                         double* __restrict__ dG, double* __restrict__ PHI,
-                                                         double* __restrict__ dPHI, const double* __restrict__ dref) {
+                                                         const TD* __restrict__ dW, double* __restrict__ G,
+                                                         TD* __restrict__ dG, double* __restrict__ PHI,
+                                                         TD* __restrict__ dPHI, const double* __restrict__ dref) {
+    using Dual = DualT<TD>;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
@@ -102,7 +107,8 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
         }
     }
     const int h0 = g.hf_ptr[c], nh = g.hf_ptr[c + 1] - h0;
-    double gv[3 * NV], gd[3 * NV];
+    double gv[3 * NV];
+    TD gd[3 * NV];
 #pragma unroll
     for (int k = 0; k < 3 * NV; ++k) { gv[k] = gr[k].v; gd[k] = gr[k].d; }
     if (g.ggc && g.ggc[c]) {   // a Green-Gauss cell: launch_green_gauss's primal and tangent gradients
@@ -117,9 +123,10 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
 #pragma unroll
     for (int v = 0; v < NV; ++v) q0[v] = Dual(W[v * ld + c], dW[v * ld + c]);
     Dual phi[NV];
-    limiter_phi<LIM, NV>(g, P, TangentNb{W, dW, ld}, c, q0, gr, dref, phi);
+    limiter_phi<LIM, NV>(g, P, TangentNb<TD>{W, dW, ld}, c, q0, gr, dref, phi);
     if (LIM != FCFD_LIMITER_NONE) {
-        double pv[NV], pd[NV];
+        double pv[NV];
+        TD pd[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) { pv[v] = phi[v].v; pd[v] = phi[v].d; }
         store_pairs<NV>(PHI, ld, c, pv);
@@ -137,12 +144,12 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
 }
 
 // one wavefront per workgroup, as the primal face kernel (3% faster here)
-template <int FLUX, int VISC, bool FULL, int NV>
-__global__ void __launch_bounds__(64) face_flux_t_kernel(DevGrid g, Params P, DualLoader<NV, FULL> L, double* __restrict__ dFX,
+template <int FLUX, int VISC, bool FULL, int NV, typename TD>
+__global__ void __launch_bounds__(64) face_flux_t_kernel(DevGrid g, Params P, DualLoader<NV, FULL, TD> L, TD* __restrict__ dFX,
                                                           int* __restrict__ status) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
-    FaceCtx<Dual, NV> X;
+    FaceCtx<DualT<TD>, NV> X;
     face_base<FLUX, VISC, NV>(g, P, L, f, X, status);
     const size_t Fs = g.F;
 #pragma unroll
@@ -151,31 +158,32 @@ __global__ void __launch_bounds__(64) face_flux_t_kernel(DevGrid g, Params P, Du
 
 // tangent of V_c S_c (SA source) added to the nu~ row; FULL carries the
 // gradient tangents, FROZEN only the cell state's
-template <bool FULL>
-__global__ void __launch_bounds__(256) sa_source_t_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ dW,
-                                                          const double* __restrict__ G, const double* __restrict__ dG,
-                                                          double* __restrict__ dR) {
+template <bool FULL, typename TD>
+__global__ void __launch_bounds__(256) sa_source_t_kernel(DevGrid g, Params P, const double* __restrict__ W, const TD* __restrict__ dW,
+                                                          const double* __restrict__ G, const TD* __restrict__ dG,
+                                                          TD* __restrict__ dR) {
+    using Dual = DualT<TD>;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
     Dual p(W[c], dW[c]), T(W[4 * ld + c], dW[4 * ld + c]), nut(W[5 * ld + c], dW[5 * ld + c]);
     auto gvec = [&](int v) {
         Vec<Dual> r;
-        r.x = Dual(G[pidx(ld, 3 * v, c)], FULL ? dG[pidx(ld, 3 * v, c)] : 0.0);
-        r.y = Dual(G[pidx(ld, 3 * v + 1, c)], FULL ? dG[pidx(ld, 3 * v + 1, c)] : 0.0);
-        r.z = Dual(G[pidx(ld, 3 * v + 2, c)], FULL ? dG[pidx(ld, 3 * v + 2, c)] : 0.0);
+        r.x = Dual(G[pidx(ld, 3 * v, c)], FULL ? dG[pidx(ld, 3 * v, c)] : TD(0));
+        r.y = Dual(G[pidx(ld, 3 * v + 1, c)], FULL ? dG[pidx(ld, 3 * v + 1, c)] : TD(0));
+        r.z = Dual(G[pidx(ld, 3 * v + 2, c)], FULL ? dG[pidx(ld, 3 * v + 2, c)] : TD(0));
         return r;
     };
     Dual S = sa_source<Dual>(m_rho(P, p, T), m_visc(P, T), nut, gvec(1), gvec(2), gvec(3), gvec(5), g.wd[c]);
-    dR[5 * ld + c] += g.vol[c] * S.d;
+    dR[5 * ld + c] += TD(g.vol[c]) * S.d;
 }
 
 // y = (P V/dt) x - dR  (the implicit operator applied matrix-free,
 // time_terms ns_time_terms.cc:26-56,147-163 minus the tangent)
-template <int NV>
+template <int NV, typename V>
 __global__ void __launch_bounds__(256) newton_op_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ dt,
-                                                        const double* __restrict__ x, const double* __restrict__ dR,
-                                                        double* __restrict__ y) {
+                                                        const V* __restrict__ x, const V* __restrict__ dR,
+                                                        V* __restrict__ y) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
@@ -193,15 +201,15 @@ __global__ void __launch_bounds__(256) newton_op_kernel(DevGrid g, Params P, con
     double r2 = drho_dp * v * xv[0] + rho * xv[2] + drho_dT * v * xv[4];
     double r3 = drho_dp * w * xv[0] + rho * xv[3] + drho_dT * w * xv[4];
     double r4 = (drho_dp * H - 1.) * xv[0] + rho * u * xv[1] + rho * v * xv[2] + rho * w * xv[3] + (drho_dT * H + rho * P.Cp) * xv[4];
-    y[c] = s * r0 - dR[c];
-    y[ld + c] = s * r1 - dR[ld + c];
-    y[2 * ld + c] = s * r2 - dR[2 * ld + c];
-    y[3 * ld + c] = s * r3 - dR[3 * ld + c];
-    y[4 * ld + c] = s * r4 - dR[4 * ld + c];
+    y[c] = V(s * r0 - dR[c]);
+    y[ld + c] = V(s * r1 - dR[ld + c]);
+    y[2 * ld + c] = V(s * r2 - dR[2 * ld + c]);
+    y[3 * ld + c] = V(s * r3 - dR[3 * ld + c]);
+    y[4 * ld + c] = V(s * r4 - dR[4 * ld + c]);
     if constexpr (NV == 6) {
         double nut = W[5 * ld + c];
         double r5 = drho_dp * nut * xv[0] + drho_dT * nut * xv[4] + rho * xv[5];
-        y[5 * ld + c] = s * r5 - dR[5 * ld + c];
+        y[5 * ld + c] = V(s * r5 - dR[5 * ld + c]);
     }
 }
 
@@ -242,68 +250,76 @@ __global__ void __launch_bounds__(256) newton_op_t_kernel(DevGrid g, Params P, c
 }
 
 // ---------------------------------------------------------------------------
-template <int FLUX, int VISC, bool FULL, int NV>
-static void launch_ft(const DevGrid& g, const Params& P, const DualLoader<NV, FULL>& L, double* dFX, int* status, hipStream_t s) {
-    hipLaunchKernelGGL((face_flux_t_kernel<FLUX, VISC, FULL, NV>), dim3((g.F + 63) / 64), dim3(64), 0, s, g, P, L, dFX, status);
+template <int FLUX, int VISC, bool FULL, int NV, typename TD>
+static void launch_ft(const DevGrid& g, const Params& P, const DualLoader<NV, FULL, TD>& L, TD* dFX, int* status, hipStream_t s) {
+    hipLaunchKernelGGL((face_flux_t_kernel<FLUX, VISC, FULL, NV, TD>), dim3((g.F + 63) / 64), dim3(64), 0, s, g, P, L, dFX, status);
 }
-template <int FLUX, bool FULL, int NV>
-static void launch_fv(const DevGrid& g, const Params& P, const DualLoader<NV, FULL>& L, double* dFX, int* status, hipStream_t s) {
-    if constexpr (NV == 6) launch_ft<FLUX, 1, FULL, 6>(g, P, L, dFX, status, s);   // SA is always viscous
-    else if (P.viscous) launch_ft<FLUX, 1, FULL, 5>(g, P, L, dFX, status, s);
-    else launch_ft<FLUX, 0, FULL, 5>(g, P, L, dFX, status, s);
+template <int FLUX, bool FULL, int NV, typename TD>
+static void launch_fv(const DevGrid& g, const Params& P, const DualLoader<NV, FULL, TD>& L, TD* dFX, int* status, hipStream_t s) {
+    if constexpr (NV == 6) launch_ft<FLUX, 1, FULL, 6, TD>(g, P, L, dFX, status, s);   // SA is always viscous
+    else if (P.viscous) launch_ft<FLUX, 1, FULL, 5, TD>(g, P, L, dFX, status, s);
+    else launch_ft<FLUX, 0, FULL, 5, TD>(g, P, L, dFX, status, s);
 }
-template <bool FULL, int NV>
-static void launch_ff(const DevGrid& g, const Params& P, const DualLoader<NV, FULL>& L, double* dFX, int* status, hipStream_t s) {
+template <bool FULL, int NV, typename TD>
+static void launch_ff(const DevGrid& g, const Params& P, const DualLoader<NV, FULL, TD>& L, TD* dFX, int* status, hipStream_t s) {
     switch (P.flux) {
-    case FCFD_FLUX_ROE: launch_fv<FCFD_FLUX_ROE, FULL, NV>(g, P, L, dFX, status, s); break;
-    case FCFD_FLUX_VAN_LEER: launch_fv<FCFD_FLUX_VAN_LEER, FULL, NV>(g, P, L, dFX, status, s); break;
-    case FCFD_FLUX_SD_SLAU: launch_fv<FCFD_FLUX_SD_SLAU, FULL, NV>(g, P, L, dFX, status, s); break;
-    case FCFD_FLUX_SW: launch_fv<FCFD_FLUX_SW, FULL, NV>(g, P, L, dFX, status, s); break;
-    default: launch_fv<FCFD_FLUX_AUSM_PLUS_UP, FULL, NV>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_ROE: launch_fv<FCFD_FLUX_ROE, FULL, NV, TD>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_VAN_LEER: launch_fv<FCFD_FLUX_VAN_LEER, FULL, NV, TD>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_SD_SLAU: launch_fv<FCFD_FLUX_SD_SLAU, FULL, NV, TD>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_SW: launch_fv<FCFD_FLUX_SW, FULL, NV, TD>(g, P, L, dFX, status, s); break;
+    default: launch_fv<FCFD_FLUX_AUSM_PLUS_UP, FULL, NV, TD>(g, P, L, dFX, status, s); break;
     }
 }
 
-template <int NV>
-static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
-                         double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s) {
+template <int NV, typename TD>
+static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, TD* dW, double* G, TD* dG, double* PHI, TD* dPHI,
+                         double* dref, TD* dFX, TD* dR, bool full, int* status, hipStream_t s) {
     const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
     const bool has_phi = lim != FCFD_LIMITER_NONE;
     const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
     CellLoader<NV> cl{W, G, PHI, has_phi, phi_const, (size_t)g.ld};
     dim3 grid((g.N + 255) / 256), blk(256);
     if (full) {
-        if (g.B > 0) hipLaunchKernelGGL(ghost_t_kernel<NV>, dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
-        launch_green_gauss(g, W, G, 0, s);
-        launch_green_gauss(g, dW, dG, 1, s);
+        if (g.B > 0) hipLaunchKernelGGL((ghost_t_kernel<NV, TD>), dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
+        if constexpr (std::is_same<TD, double>::value) {   // (the FP32 tangent is refused on meshes with Green-Gauss cells)
+            launch_green_gauss(g, W, G, 0, s);
+            launch_green_gauss(g, dW, dG, 1, s);
+        }
         if (lim == FCFD_LIMITER_VK) {
             launch_vk_range(g, P, W, dref, s);   // primal reference range (constant in the tangent)
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_VK, NV>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_VK, NV, TD>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
         } else if (lim == FCFD_LIMITER_BJ) {
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_BJ, NV>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_BJ, NV, TD>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
         } else if (lim == FCFD_LIMITER_MINMOD) {
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_MINMOD, NV>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_MINMOD, NV, TD>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
         } else {
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_NONE, NV>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_NONE, NV, TD>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
         }
-        DualLoader<NV, true> L{cl, dW, dG, dPHI, g.N};
-        launch_ff<true, NV>(g, P, L, dFX, status, s);
+        DualLoader<NV, true, TD> L{cl, dW, dG, dPHI, g.N};
+        launch_ff<true, NV, TD>(g, P, L, dFX, status, s);
     } else {
         launch_ghosts(g, P, W, status, s);
         launch_grad_lim(g, P, W, G, PHI, dref, s);
-        DualLoader<NV, false> L{cl, dW, nullptr, nullptr, g.N};
-        launch_ff<false, NV>(g, P, L, dFX, status, s);
+        DualLoader<NV, false, TD> L{cl, dW, nullptr, nullptr, g.N};
+        launch_ff<false, NV, TD>(g, P, L, dFX, status, s);
     }
     launch_cell_sum(g, dFX, dR, s);
     if constexpr (NV == 6) {
-        if (full) hipLaunchKernelGGL(sa_source_t_kernel<true>, grid, blk, 0, s, g, P, W, dW, G, dG, dR);
-        else hipLaunchKernelGGL(sa_source_t_kernel<false>, grid, blk, 0, s, g, P, W, dW, G, dG, dR);
+        if (full) hipLaunchKernelGGL((sa_source_t_kernel<true, TD>), grid, blk, 0, s, g, P, W, dW, G, dG, dR);
+        else hipLaunchKernelGGL((sa_source_t_kernel<false, TD>), grid, blk, 0, s, g, P, W, dW, G, dG, dR);
     }
 }
 
 void launch_jvp(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
                 double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s) {
-    if (g.nv == 6) launch_jvp_t<6>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
-    else launch_jvp_t<5>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
+    if (g.nv == 6) launch_jvp_t<6, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
+    else launch_jvp_t<5, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
+}
+
+void launch_jvp(const DevGrid& g, const Params& P, double* W, float* dW, double* G, float* dG, double* PHI, float* dPHI,
+                double* dref, float* dFX, float* dR, bool full, int* status, hipStream_t s) {
+    if (g.nv == 6) launch_jvp_t<6, float>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
+    else launch_jvp_t<5, float>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
 }
 
 void launch_newton_op_t(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* Jt,
@@ -313,11 +329,20 @@ void launch_newton_op_t(const DevGrid& g, const Params& P, const double* W, cons
     else hipLaunchKernelGGL(newton_op_t_kernel<5>, grid, blk, 0, s, g, P, W, dt, x, Jt, y);
 }
 
+template <typename T>
+static void launch_newton_op_v(const DevGrid& g, const Params& P, const double* W, const double* dt, const T* x, const T* dR, T* y,
+                               hipStream_t s) {
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL((newton_op_kernel<6, T>), grid, blk, 0, s, g, P, W, dt, x, dR, y);
+    else hipLaunchKernelGGL((newton_op_kernel<5, T>), grid, blk, 0, s, g, P, W, dt, x, dR, y);
+}
 void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* dR,
                       double* y, hipStream_t s) {
-    dim3 grid((g.N + 255) / 256), blk(256);
-    if (g.nv == 6) hipLaunchKernelGGL(newton_op_kernel<6>, grid, blk, 0, s, g, P, W, dt, x, dR, y);
-    else hipLaunchKernelGGL(newton_op_kernel<5>, grid, blk, 0, s, g, P, W, dt, x, dR, y);
+    launch_newton_op_v<double>(g, P, W, dt, x, dR, y, s);
+}
+void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const double* dt, const float* x, const float* dR,
+                      float* y, hipStream_t s) {
+    launch_newton_op_v<float>(g, P, W, dt, x, dR, y, s);
 }
 
 }  // namespace fcfd
@@ -423,10 +448,10 @@ __global__ void __launch_bounds__(256) pseudo_diag_kernel(DevGrid g, Params P, c
 }
 
 // y += P_ps V/dtau x (the matrix-free operator's pseudo-time term)
-template <int NV>
+template <int NV, typename T>
 __global__ void __launch_bounds__(256) pseudo_op_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ dtau,
-                                                        const double* __restrict__ drdp, const double* __restrict__ x,
-                                                        double* __restrict__ y) {
+                                                        const double* __restrict__ drdp, const T* __restrict__ x,
+                                                        T* __restrict__ y) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
@@ -440,7 +465,7 @@ __global__ void __launch_bounds__(256) pseudo_op_kernel(DevGrid g, Params P, con
         double s = 0.;
 #pragma unroll
         for (int j = 0; j < NV; ++j) s += B[i][j] * xv[j];
-        y[i * ld + c] += s;
+        y[i * ld + c] = T(y[i * ld + c] + s);
     }
 }
 
@@ -458,11 +483,20 @@ void launch_pseudo_diag(const DevGrid& g, const Params& P, const double* W, cons
     else hipLaunchKernelGGL(pseudo_diag_kernel<5>, grid, blk, 0, s, g, P, W, dtau, drdp, D);
 }
 
+template <typename T>
+static void launch_pseudo_op_v(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, const T* x,
+                               T* y, hipStream_t s) {
+    const dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL((pseudo_op_kernel<6, T>), grid, blk, 0, s, g, P, W, dtau, drdp, x, y);
+    else hipLaunchKernelGGL((pseudo_op_kernel<5, T>), grid, blk, 0, s, g, P, W, dtau, drdp, x, y);
+}
 void launch_pseudo_op(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, const double* x,
                       double* y, hipStream_t s) {
-    const dim3 grid((g.N + 255) / 256), blk(256);
-    if (g.nv == 6) hipLaunchKernelGGL(pseudo_op_kernel<6>, grid, blk, 0, s, g, P, W, dtau, drdp, x, y);
-    else hipLaunchKernelGGL(pseudo_op_kernel<5>, grid, blk, 0, s, g, P, W, dtau, drdp, x, y);
+    launch_pseudo_op_v<double>(g, P, W, dtau, drdp, x, y, s);
+}
+void launch_pseudo_op(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, const float* x,
+                      float* y, hipStream_t s) {
+    launch_pseudo_op_v<float>(g, P, W, dtau, drdp, x, y, s);
 }
 
 }  // namespace fcfd

@@ -182,3 +182,42 @@ def test_matrix_free_newton_steps(mode, fp32):
     hist = [ns.newton_step(opts).res_norm for _ in range(10)]
     print("  ||R||:", " ".join("%.3e" % h for h in hist), flush=True)
     assert np.all(np.isfinite(hist)) and hist[-1] < hist[0]
+
+
+@pytest.mark.parametrize("name", ["cylinder", "plate_sa"])
+def test_fp32_tangent_lane_newton_step_matches_fp64_lane(name):
+    """The FP32 matrix-free Newton operator carries the tangent's derivative
+    lane in FP32 (values stay FP64).  One Newton step with it against the
+    same step with the FP64 lane (FCFD_TANGENT_FP64=1, converted at the
+    operator's boundary), both FP32 GMRES to rtol 1e-5: the updates agree to
+    1e-3 of their size per variable (velocity components on the speed's
+    scale), the GMRES iteration counts within 2, and
+    the residual after the step within 1e-3 of the residual before it."""
+    import os
+    cs = cases.cylinder_case(64, 32) if name == "cylinder" else cases.plate_case(24, 12, 4, noise=0.01, sa=True)
+    print("case", cs.name, "N", cs.n_cells, flush=True)
+    opts = A.newton_opts(cfl=20.0, jacobian=A.JAC_JVP_FULL, fp32=True, restart=40, max_iters=80, rtol=1e-5)
+    out = {}
+    for lane in ("fp64", "fp32"):
+        if lane == "fp64":
+            os.environ["FCFD_TANGENT_FP64"] = "1"
+        else:
+            os.environ.pop("FCFD_TANGENT_FP64", None)
+        ns = _ns(cs)
+        st1 = ns.newton_step(opts)
+        W1 = ns.state_host(include_ghosts=False)
+        ns.assemble_residual()
+        R1 = float(np.linalg.norm(ns.residual_host()))
+        out[lane] = (W1, st1.krylov_iters, st1.res_norm, R1)
+        ns.close()
+    os.environ.pop("FCFD_TANGENT_FP64", None)
+    d64, d32 = out["fp64"][0] - cs.W0, out["fp32"][0] - cs.W0
+    scale = np.abs(d64).max(axis=0)
+    scale[1:4] = scale[1:4].max()        # velocity components on the speed's scale (w is rounding noise in 2-D)
+    rel = np.abs(d32 - d64).max(axis=0) / (scale + 1e-300)
+    print("  per-variable max |dW32 - dW64| / max|dW64|:", " ".join("%.2e" % e for e in rel), flush=True)
+    print("  iterations %d / %d, ||R|| after the step %.6e / %.6e" % (out["fp64"][1], out["fp32"][1], out["fp64"][3], out["fp32"][3]),
+          flush=True)
+    assert rel.max() < 1e-3
+    assert abs(out["fp64"][1] - out["fp32"][1]) <= 2
+    assert abs(out["fp32"][3] - out["fp64"][3]) <= 1e-3 * out["fp64"][2]
