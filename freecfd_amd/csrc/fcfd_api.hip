@@ -60,6 +60,10 @@ struct fcfd_ctx {
     float *Dinv32 = nullptr, *D32 = nullptr, *JL32 = nullptr, *JR32 = nullptr, *b32 = nullptr, *x32 = nullptr;
     // tangent workspace (lazy)
     double *dG = nullptr, *dPHI = nullptr, *dFX = nullptr, *jv_in = nullptr, *jv_out = nullptr, *jv_R = nullptr;
+    // the FD Jacobian's off-diagonal blocks re-laid for the streamed matvec (lazy)
+    SellBlocks sell;
+    double* sell64 = nullptr;
+    float* sell32 = nullptr;
     // FP32 tangent workspace of the FP32 matrix-free Newton operator (lazy)
     float *dG32 = nullptr, *dPHI32 = nullptr, *dFX32 = nullptr, *jv_in32 = nullptr, *jv_R32 = nullptr;
     // cell colouring for the block Gauss-Seidel preconditioner (lazy)
@@ -804,6 +808,59 @@ static int ensure_tangent(fcfd_ctx* c) {
     return FCFD_OK;
 }
 
+// Sliced-ELL index of the off-diagonal blocks (64-cell slices): per cell
+// its half-faces whose other cell is interior, in half-face (reference face)
+// order, and the block storage of the precision the Krylov solver needs.
+static int ensure_sell(fcfd_ctx* c, bool fp32) {
+    const DevGrid& g = c->dg;
+    int rc;
+    if (!c->sell.soff) {
+        const HostGrid& hg = c->hg;
+        const int N = g.N, ns = (N + 63) / 64;
+        std::vector<int> hp, hf;
+        half_faces(hg, hp, hf);
+        std::vector<int> soff(ns + 1, 0), sface, snb;
+        for (int sl = 0; sl < ns; ++sl) {
+            int width = 0;
+            for (int lane = 0; lane < 64; ++lane) {
+                const int i = 64 * sl + lane;
+                if (i >= N) break;
+                int k = 0;
+                for (int h = hp[i]; h < hp[i + 1]; ++h) {
+                    const int fs = hf[h], fr = hg.ifperm[fs >= 0 ? fs : ~fs];
+                    if (hg.perm[fs >= 0 ? hg.fneigh[fr] : hg.fparent[fr]] < N) ++k;
+                }
+                width = std::max(width, k);
+            }
+            const size_t base = sface.size();
+            sface.resize(base + (size_t)64 * width, kSellPad);
+            snb.resize(base + (size_t)64 * width, -1);
+            for (int lane = 0; lane < 64; ++lane) {
+                const int i = 64 * sl + lane;
+                if (i >= N) break;
+                int k = 0;
+                for (int h = hp[i]; h < hp[i + 1]; ++h) {
+                    const int fs = hf[h], fr = hg.ifperm[fs >= 0 ? fs : ~fs];
+                    const int o = hg.perm[fs >= 0 ? hg.fneigh[fr] : hg.fparent[fr]];
+                    if (o >= N) continue;
+                    sface[base + (size_t)64 * k + lane] = fs;
+                    snb[base + (size_t)64 * k + lane] = o;
+                    ++k;
+                }
+            }
+            soff[sl + 1] = soff[sl] + width;
+        }
+        c->sell.nslice = ns;
+        c->sell.nslot = (size_t)soff[ns];
+        if ((rc = upload(c, soff, &c->sell.soff)) || (rc = upload(c, sface, &c->sell.sface)) || (rc = upload(c, snb, &c->sell.snb)))
+            return rc;
+    }
+    const size_t n = c->sell.nslot * 64 * (size_t)(2 * pair_groups(g.nv * g.nv));
+    if (fp32 && !c->sell32 && (rc = dalloc(c, n, &c->sell32))) return rc;
+    if (!fp32 && !c->sell64 && (rc = dalloc(c, n, &c->sell64))) return rc;
+    return FCFD_OK;
+}
+
 static int ensure_tangent32(fcfd_ctx* c) {
     if (c->dG32) return FCFD_OK;
     const DevGrid& g = c->dg;
@@ -1177,6 +1234,16 @@ static int newton_impl(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, co
     } else if (!mgp) {
         dev_bj_invert<double>(g, c->Dg, c->Dinv64, c->status, s);
     }
+    // the assembled operator's matvec streams the off-diagonal blocks from
+    // the sliced layout (FCFD_BSR_MATVEC=1: from the per-face blocks, the
+    // comparison case of the tests)
+    const char* bsr = std::getenv("FCFD_BSR_MATVEC");
+    const bool sell = !mfree && !(bsr && bsr[0] == '1');
+    if (sell) {
+        if ((rc = ensure_sell(c, fp32))) return rc;
+        if (fp32) dev_sell_pack<float>(g, c->sell, c->JL32, c->JR32, c->sell32, s);
+        else dev_sell_pack<double>(g, c->sell, c->JL, c->JR, c->sell64, s);
+    }
     const bool use_graph = mgp && graphs_enabled();
     if (mgp) {   // FP64 V-cycle on the FD blocks: fine inverse + coarse levels
         if (!c->Dinv64 && (rc = dalloc(c, b2 * g.ld, &c->Dinv64))) return rc;
@@ -1248,12 +1315,18 @@ static int newton_impl(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, co
         LinOp<double> M{prec64};
         rc = gmres<double>(g, A, M, c->R, c->dW, go, c->w64, gs, s);
     } else if (fp32) {
-        LinOp<float> A{[&](const float* x, float* y, hipStream_t ss) { dev_bsr_matvec<float>(g, c->D32, c->JL32, c->JR32, x, y, ss); }};
+        LinOp<float> A{[&](const float* x, float* y, hipStream_t ss) {
+            if (sell) dev_sell_matvec<float>(g, c->sell, c->D32, c->sell32, x, y, ss);
+            else dev_bsr_matvec<float>(g, c->D32, c->JL32, c->JR32, x, y, ss);
+        }};
         LinOp<float> M{prec32};
         rc = gmres<float>(g, A, M, c->b32, c->x32, go, c->w32, gs, s);
         if (!rc) dev_convert<float, double>(n5, c->x32, c->dW, s);
     } else {
-        LinOp<double> A{[&](const double* x, double* y, hipStream_t ss) { dev_bsr_matvec<double>(g, c->Dg, c->JL, c->JR, x, y, ss); }};
+        LinOp<double> A{[&](const double* x, double* y, hipStream_t ss) {
+            if (sell) dev_sell_matvec<double>(g, c->sell, c->Dg, c->sell64, x, y, ss);
+            else dev_bsr_matvec<double>(g, c->Dg, c->JL, c->JR, x, y, ss);
+        }};
         LinOp<double> M{prec64};
         rc = gmres<double>(g, A, M, c->R, c->dW, go, c->w64, gs, s);
     }

@@ -34,6 +34,26 @@ template <typename T>
 hipError_t dev_mdot(const DevGrid& g, int k, const T* V, size_t stride, const T* w, double* part, double* out_dev,
                     double* out_host, hipStream_t s);
 template <typename T> void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s);
+// The off-diagonal blocks of the FD Jacobian re-laid for streaming (sliced
+// ELL, 64-cell slices as the wavefront): the entries of slice s are slots
+// [soff[s], soff[s+1]), 64 lanes each; slot k, lane l holds cell 64 s + l's
+// k-th half-face whose other cell is interior (reference face order, the
+// order of the matvec's sum), as the signed face id (hf_face's convention)
+// in sface and that cell in snb (-1: a pad).  The blocks follow in row
+// pairs, pair q of slot k at ((k * pairs + q) * 64 + l), already carrying the
+// row's sign (-JR where the cell is the face's parent, +JL where it is the
+// neighbour).
+constexpr int kSellPad = -2147483647 - 1;   // sface of a pad slot (no face id reaches it)
+struct SellBlocks {
+    int nslice = 0;
+    size_t nslot = 0;
+    const int *soff = nullptr, *sface = nullptr, *snb = nullptr;
+};
+// blk = the signed off-diagonal blocks of JL/JR ([pairs][F] row pairs) in the sliced layout
+template <typename T> void dev_sell_pack(const DevGrid& g, const SellBlocks& sb, const T* JL, const T* JR, T* blk, hipStream_t s);
+// y = D x + sum of the off-diagonal blocks times x: the same sums in the same
+// order as dev_bsr_matvec, bit for bit, with the blocks streamed
+template <typename T> void dev_sell_matvec(const DevGrid& g, const SellBlocks& sb, const T* D, const T* blk, const T* x, T* y, hipStream_t s);
 template <typename T> void dev_bj_invert(const DevGrid& g, const double* D, T* Dinv, int* status, hipStream_t s);
 template <typename T> void dev_bj_apply(const DevGrid& g, const T* Dinv, const T* x, T* y, hipStream_t s);
 void dev_bj_apply_t(const DevGrid& g, const double* Dinv, const double* x, double* y, hipStream_t s);   // y = D^{-T} x

@@ -244,6 +244,87 @@ __global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __r
     for (int j = 0; j < NV; ++j) y[j * ld + c] = acc[j];
 }
 
+// sliced-ELL off-diagonal blocks: one thread per (slot, lane) copies the
+// face's block with the row's sign (an exact negation), zeros for a pad
+template <typename T, int NV>
+__global__ void __launch_bounds__(256) sell_pack_kernel(size_t nent, size_t Fs, const int* __restrict__ sface, const T* __restrict__ JL,
+                                                        const T* __restrict__ JR, T* __restrict__ blk) {
+    using P2 = typename PairOf<T>::type;
+    constexpr int NP = pair_groups(NV * NV);
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nent) return;
+    const size_t slot = e >> 6, lane = e & 63;
+    const int fs = sface[e];
+    T b[NV * NV];
+    if (fs == kSellPad) {
+#pragma unroll
+        for (int q = 0; q < NV * NV; ++q) b[q] = T(0);
+    } else {
+        const int f = fs >= 0 ? fs : ~fs;
+        load_pairs<NV * NV>(fs >= 0 ? JR : JL, Fs, f, b);
+        const T sg = fs >= 0 ? T(-1) : T(1);
+#pragma unroll
+        for (int q = 0; q < NV * NV; ++q) b[q] = sg * b[q];
+    }
+    P2* B2 = reinterpret_cast<P2*>(blk);
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        P2 t;
+        t.x = b[2 * q];
+        t.y = 2 * q + 1 < NV * NV ? b[2 * q + 1] : T(0);
+        B2[(slot * NP + q) * 64 + lane] = t;
+    }
+}
+
+// y = D x + off-diagonal blocks x: per cell the diagonal product, then its
+// slots in order (acc[j] += b_ij x_i with b already signed: the products and
+// sums of bsr_matvec_kernel, which forms (sign b_ij) x_i)
+template <typename T, int NV>
+__global__ void __launch_bounds__(256) sell_matvec_kernel(DevGrid g, const int* __restrict__ soff, const int* __restrict__ snb,
+                                                          const T* __restrict__ D, const T* __restrict__ blk, const T* __restrict__ x,
+                                                          T* __restrict__ y) {
+    using P2 = typename PairOf<T>::type;
+    constexpr int NP = pair_groups(NV * NV);
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld;
+    const int s = c >> 6, lane = c & 63;
+    T xc[NV], acc[NV], b[NV * NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) xc[i] = x[i * ld + c];
+    load_pairs<NV * NV>(D, ld, c, b);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        T sum = 0;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) sum += b[NV * j + i] * xc[i];
+        acc[j] = sum;
+    }
+    const int k0 = soff[s], k1 = soff[s + 1];
+    const P2* B2 = reinterpret_cast<const P2*>(blk);
+#pragma unroll 2
+    for (int k = k0; k < k1; ++k) {
+        const int o = snb[(size_t)k * 64 + lane];
+        if (o >= 0) {
+            T xo[NV];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) xo[i] = x[i * ld + o];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                const P2 t = B2[((size_t)k * NP + q) * 64 + lane];
+                b[2 * q] = t.x;
+                if (2 * q + 1 < NV * NV) b[2 * q + 1] = t.y;
+            }
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+#pragma unroll
+                for (int j = 0; j < NV; ++j) acc[j] += b[NV * i + j] * xo[i];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j) y[j * ld + c] = acc[j];
+}
+
 // batched NV x NV inverse by Gauss-Jordan with partial pivoting (block-Jacobi);
 // PAIRED: blocks in row pairs with stride ld (the mesh level), else each
 // block contiguous, entry q of block c at c * block_stride(NV) + q (coarse levels)
@@ -409,6 +490,25 @@ void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, cons
 }
 template void dev_bsr_matvec<double>(const DevGrid&, const double*, const double*, const double*, const double*, double*, hipStream_t);
 template void dev_bsr_matvec<float>(const DevGrid&, const float*, const float*, const float*, const float*, float*, hipStream_t);
+
+template <typename T>
+void dev_sell_pack(const DevGrid& g, const SellBlocks& sb, const T* JL, const T* JR, T* blk, hipStream_t s) {
+    const size_t nent = sb.nslot * 64;
+    if (nent == 0) return;
+    const dim3 grid((unsigned)((nent + 255) / 256)), bk(256);
+    if (g.nv == 6) hipLaunchKernelGGL((sell_pack_kernel<T, 6>), grid, bk, 0, s, nent, (size_t)g.F, sb.sface, JL, JR, blk);
+    else hipLaunchKernelGGL((sell_pack_kernel<T, 5>), grid, bk, 0, s, nent, (size_t)g.F, sb.sface, JL, JR, blk);
+}
+template <typename T>
+void dev_sell_matvec(const DevGrid& g, const SellBlocks& sb, const T* D, const T* blk, const T* x, T* y, hipStream_t s) {
+    const dim3 grid((g.N + 255) / 256), bk(256);
+    if (g.nv == 6) hipLaunchKernelGGL((sell_matvec_kernel<T, 6>), grid, bk, 0, s, g, sb.soff, sb.snb, D, blk, x, y);
+    else hipLaunchKernelGGL((sell_matvec_kernel<T, 5>), grid, bk, 0, s, g, sb.soff, sb.snb, D, blk, x, y);
+}
+template void dev_sell_pack<double>(const DevGrid&, const SellBlocks&, const double*, const double*, double*, hipStream_t);
+template void dev_sell_pack<float>(const DevGrid&, const SellBlocks&, const float*, const float*, float*, hipStream_t);
+template void dev_sell_matvec<double>(const DevGrid&, const SellBlocks&, const double*, const double*, const double*, double*, hipStream_t);
+template void dev_sell_matvec<float>(const DevGrid&, const SellBlocks&, const float*, const float*, const float*, float*, hipStream_t);
 
 template <typename T>
 void dev_bj_invert(const DevGrid& g, const double* D, T* Dinv, int* status, hipStream_t s) {
