@@ -13,7 +13,8 @@ SPEC_NONE, SPEC_RHO, SPEC_P, SPEC_T, SPEC_STATE = 0, 1, 2, 3, 4
 LIMITER_NONE, LIMITER_VK, LIMITER_BJ, LIMITER_MINMOD = -1, 1, 2, 3
 ORDER_FIRST, ORDER_SECOND = 1, 2
 FLUX_ROE, FLUX_VAN_LEER, FLUX_AUSM_PLUS_UP, FLUX_SD_SLAU, FLUX_SW = 1, 2, 3, 4, 5
-TURB_NONE, TURB_SA = 0, 1
+TURB_NONE, TURB_SA, TURB_KW = 0, 1, 2
+RANS_KEPSILON, RANS_KOMEGA, RANS_BSL, RANS_SST = 1, 2, 3, 4
 GRAD_CURVILINEAR, GRAD_LSQR, GRAD_GREENGAUSS = 1, 2, 3
 INTERP_WTLI, INTERP_IDW, INTERP_SIMPLE = 1, 2, 3
 VISC_CONSTANT, VISC_SUTHERLAND = 1, 2
@@ -39,7 +40,8 @@ class BC(C.Structure):
                 ("qdot_fixed", C.c_int),
                 ("p", C.c_double), ("T", C.c_double), ("rho", C.c_double), ("V", C.c_double * 3),
                 ("p_total", C.c_double), ("T_total", C.c_double), ("mdot", C.c_double), ("qdot", C.c_double),
-                ("include_in_loads", C.c_int), ("nut", C.c_double)]
+                ("include_in_loads", C.c_int), ("nut", C.c_double),
+                ("turb_intensity", C.c_double), ("eddy_visc_ratio", C.c_double)]
 
 
 class Options(C.Structure):
@@ -53,7 +55,9 @@ class Options(C.Structure):
                 ("grad_hex", C.c_int), ("grad_prism", C.c_int), ("grad_other", C.c_int),
                 ("interp_method", C.c_int), ("stencil_size", C.c_int), ("skewness_tol", C.c_double),
                 ("moment_center", C.c_double * 3),
-                ("n_bc", C.c_int), ("bc", BC * MAX_BC), ("turbulence", C.c_int), ("Pr_t", C.c_double)]
+                ("n_bc", C.c_int), ("bc", BC * MAX_BC), ("turbulence", C.c_int), ("Pr_t", C.c_double),
+                ("rans_model", C.c_int), ("k_low", C.c_double), ("k_high", C.c_double), ("omega_low", C.c_double),
+                ("visc_ratio_limit", C.c_double)]
 
 
 class Info(C.Structure):

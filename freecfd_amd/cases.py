@@ -64,8 +64,9 @@ def make_options(bcs, gamma=1.4, Mw=28.97, Pref=0.0, Tref=0.0, mu=0.0, lam=0.0, 
                  order=A.ORDER_SECOND, jac_order=A.ORDER_FIRST, limiter=A.LIMITER_VK,
                  flux=A.FLUX_AUSM_PLUS_UP, limiter_threshold=0.0, Minf=1.0, wdiss=0.3, bl_height=0.0,
                  grad=(A.GRAD_CURVILINEAR, A.GRAD_CURVILINEAR, A.GRAD_LSQR), interp=A.INTERP_WTLI,
-                 stencil=2, loads_bcs=(), moment_center=(0.0, 0.0, 0.0), turbulence=A.TURB_NONE, Pr_t=0.9):
-    """Defaults are the reference's (src/read_inputs.cc:47-54,70-75,125-134)."""
+                 stencil=2, loads_bcs=(), moment_center=(0.0, 0.0, 0.0), turbulence=A.TURB_NONE, Pr_t=0.9,
+                 rans_model=A.RANS_SST, k_low=1e-10, k_high=1e8, omega_low=1e-2, visc_ratio_limit=1e8):
+    """Defaults are the reference's (src/read_inputs.cc:47-54,70-75,110-134)."""
     o = A.Options()
     o.gamma, o.Mw, o.Pref, o.Tref = gamma, Mw, Pref, Tref
     if sutherland is not None:
@@ -96,7 +97,10 @@ def make_options(bcs, gamma=1.4, Mw=28.97, Pref=0.0, Tref=0.0, mu=0.0, lam=0.0, 
             bc.V[q] = V[q]
         bc.include_in_loads = int(b in loads_bcs)
         bc.nut = float(d.get("nut", 0.0))
+        bc.turb_intensity = float(d.get("turb_intensity", 1e-4))
+        bc.eddy_visc_ratio = float(d.get("eddy_visc_ratio", 0.1))
     o.turbulence, o.Pr_t = turbulence, Pr_t
+    o.rans_model, o.k_low, o.k_high, o.omega_low, o.visc_ratio_limit = rans_model, k_low, k_high, omega_low, visc_ratio_limit
     return o
 
 
@@ -401,3 +405,66 @@ def plate_case(ni=512, nj=128, nk=64, seed=5, noise=0.0, first=5.5e-6, sa=False,
         if sa:
             W[:, 5] *= 1.0 + rng.uniform(-noise, noise, n)
     return Case("plate_sa" if sa else "plate", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.3, mu=mu, nut=nut_inf))
+
+
+# ---------------------------------------------------------------------------
+# segregated two-equation RANS (k-omega family)
+# ---------------------------------------------------------------------------
+
+def _kw_inflow(cs, intensity, visc_ratio):
+    """freestream k and omega of a case (the inlet's definition,
+    rans_set_bcs.cc:58-61, at its velocity and temperature)"""
+    m = cs.meta
+    R = m["R"]
+    rho = m["p"] / (R * m["T"])
+    k = max(1e-10, 1.5 * (intensity * m["U"]) ** 2)
+    return k, rho * k / (visc_ratio * m["mu"])
+
+
+def kw_state(cs, seed=31, amp=0.3, intensity=0.05, visc_ratio=10.0):
+    """k and omega per cell (N x 2): the freestream values times smooth
+    seeded variations (amplitude `amp`), so gradients, the blending and the
+    cross-diffusion term are all non-trivial"""
+    k0, w0 = _kw_inflow(cs, intensity, visc_ratio)
+    xc = meshgen.cell_centroids(cs.raw)
+    lo, hi = xc.min(axis=0), xc.max(axis=0)
+    s = (xc - lo) / np.maximum(hi - lo, 1e-30)
+    rng = np.random.default_rng(seed)
+    ph = rng.uniform(0, 2 * np.pi, 6)
+    fk = 1.0 + amp * np.sin(2 * np.pi * s[:, 0] + ph[0]) * np.cos(np.pi * s[:, 1] + ph[1]) + 0.3 * amp * np.sin(2 * np.pi * s[:, 2] + ph[2])
+    fw = 1.0 + amp * np.cos(2 * np.pi * s[:, 1] + ph[3]) * np.sin(np.pi * s[:, 0] + ph[4]) + 0.3 * amp * np.cos(2 * np.pi * s[:, 2] + ph[5])
+    return np.stack([k0 * fk, w0 * fw], axis=1)
+
+
+def kw_case(variant="vel_p__out_t__wall_fixed_t", model=A.RANS_SST, ni=10, nj=6, nk=4, seed=12, intensity=0.05,
+            visc_ratio=10.0, limiter=A.LIMITER_VK):
+    """bc_variant_case with the segregated two-equation model `model` (the
+    variant's flux, BCs and NS state); inlet turbulence intensity and eddy
+    viscosity ratio as given.  cs.K0 holds an N x 2 k, omega state."""
+    cs = bc_variant_case(variant, ni, nj, nk, seed=seed, limiter=limiter)
+    o = cs.opt
+    o.turbulence, o.rans_model = A.TURB_KW, model
+    for b in range(o.n_bc):
+        o.bc[b].turb_intensity, o.bc[b].eddy_visc_ratio = intensity, visc_ratio
+    cs.name = "kw_%d_%s" % (model, variant)
+    cs.K0 = kw_state(cs, intensity=intensity, visc_ratio=visc_ratio)
+    cs.meta.update(intensity=intensity, visc_ratio=visc_ratio)
+    return cs
+
+
+def plate_kw_case(ni=512, nj=128, nk=64, model=A.RANS_SST, noise=0.0, intensity=0.01, visc_ratio=10.0, seed=5,
+                  prism_frac=0.2):
+    """The turbulent flat plate (Mach 0.3, Re_L 5e6, viscous Roe, top 20% of
+    the layers split into prisms) with the segregated two-equation model:
+    5 NS equations plus k and omega; freestream k, omega from the inlet's
+    intensity and eddy viscosity ratio in cs.K0."""
+    cs = plate_case(ni, nj, nk, seed=seed, noise=noise, prism_frac=prism_frac)
+    o = cs.opt
+    o.turbulence, o.rans_model = A.TURB_KW, model
+    for b in range(o.n_bc):
+        o.bc[b].turb_intensity, o.bc[b].eddy_visc_ratio = intensity, visc_ratio
+    k0, w0 = _kw_inflow(cs, intensity, visc_ratio)
+    cs.name = "plate_kw"
+    cs.K0 = np.tile([k0, w0], (cs.n_cells, 1))
+    cs.meta.update(intensity=intensity, visc_ratio=visc_ratio)
+    return cs

@@ -110,9 +110,18 @@ extern "C" {
 #define FCFD_DT_CFL_MAX   2
 #define FCFD_DT_CFL_LOCAL 3
 
-/* turbulence model (build-only) */
+/* turbulence model */
 #define FCFD_TURB_NONE 0
-#define FCFD_TURB_SA   1
+#define FCFD_TURB_SA   1   /* build-only: Spalart-Allmaras nu~ as a coupled sixth equation   */
+#define FCFD_TURB_KW   2   /* the reference's segregated two-equation RANS (src/rans/): k and
+                              omega solved after each NS step (fcfd_rans_step), mu_t frozen
+                              in the NS face loop (ns_diffusive_face_flux.cc:36-39)          */
+/* two-equation model (src/rans/rans.h:44-47; "k-epsilon", "k-omega", "sst"
+ * in rans.cc:54-60; BSL has no input keyword there) */
+#define FCFD_RANS_KEPSILON 1
+#define FCFD_RANS_KOMEGA   2
+#define FCFD_RANS_BSL      3
+#define FCFD_RANS_SST      4
 
 #define FCFD_MAX_BC 32
 
@@ -143,6 +152,9 @@ typedef struct fcfd_bc {
     double p, T, rho, V[3], p_total, T_total, mdot, qdot;
     int include_in_loads;
     double nut;       /* SA working variable nu~ at an inlet (m^2/s); build-only */
+    /* two-equation RANS at an inlet: turbulenceintensity and eddyviscosityratio
+     * (read_inputs.cc:110-111, defaults 1e-4 and 0.1; rans_set_bcs.cc:33-64) */
+    double turb_intensity, eddy_visc_ratio;
 } fcfd_bc;
 
 typedef struct fcfd_options {
@@ -165,6 +177,11 @@ typedef struct fcfd_options {
      * k-omega/SST): FCFD_TURB_SA adds nu~ as a sixth variable, coupled */
     int turbulence;
     double Pr_t;      /* turbulent Prandtl number (the reference's turbulentPr) */
+    /* turbulence section of the two-equation RANS (read_inputs.cc:117-121):
+     * model (FCFD_RANS_*, default SST), klowlimit (1e-10), khighlimit (1e8),
+     * omegalowlimit (1e-2), viscosityratiolimit (1e8) */
+    int rans_model;
+    double k_low, k_high, omega_low, visc_ratio_limit;
 } fcfd_options;
 
 typedef struct fcfd_info {

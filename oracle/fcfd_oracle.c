@@ -130,6 +130,18 @@ struct orc_ctx {
     double *fC, *fD;                /* F*NVM: convective and diffusive flux of each face (last assemble) */
     double *JL, *JR, *Dg;           /* F*NVM^2 [var][eq], F*NVM^2, N*NVM^2 [row][col] */
     double* loads;                  /* nbc*6 */
+    double* wL;                     /* F: weightL of each face's convective flux (last orc_assemble) */
+    /* segregated two-equation RANS (src/rans/): k, omega, mu_t and the
+     * strain rate of every cell (ghosts included), the gradients of k and
+     * omega (interior cells; the ghosts' stay 0, rans.cc:172-178 never
+     * writes them), the inlet values of rans_set_bcs.cc and which BCs fix
+     * k and omega (inlets) and mu_t (inlets, no-slip walls) */
+    int kw;                         /* opt.turbulence == FCFD_TURB_KW */
+    double *kk, *om, *mut, *strain; /* ntot */
+    V3 *gk, *gom;                   /* ntot */
+    double *kw_kbc, *kw_obc, *kw_mbc;   /* nbc */
+    int *kw_kfix, *kw_mfix;             /* nbc */
+    double kw_lam_last;             /* laminar viscosity of the face loop's last face (terms() reads it in its cell loop) */
     int err; char msg[256];
 };
 
@@ -1215,6 +1227,20 @@ orc_ctx* orc_create(const fcfd_mesh* mesh, const fcfd_options* opt) {
     o->fC = (double*)calloc((size_t)o->nface * NVM, sizeof(double));
     o->fD = (double*)calloc((size_t)o->nface * NVM, sizeof(double));
     o->loads = (double*)calloc((size_t)(o->nbc + 1) * 6, sizeof(double));
+    o->wL = (double*)calloc((size_t)o->nface, sizeof(double));
+    o->kw = opt->turbulence == FCFD_TURB_KW;
+    o->kk = (double*)calloc(nt, sizeof(double)); o->om = (double*)calloc(nt, sizeof(double));
+    o->mut = (double*)calloc(nt, sizeof(double)); o->strain = (double*)calloc(nt, sizeof(double));
+    o->gk = (V3*)calloc(nt, sizeof(V3)); o->gom = (V3*)calloc(nt, sizeof(V3));
+    int nb1 = o->nbc + 1;
+    o->kw_kbc = (double*)calloc(nb1, sizeof(double)); o->kw_obc = (double*)calloc(nb1, sizeof(double));
+    o->kw_mbc = (double*)calloc(nb1, sizeof(double));
+    o->kw_kfix = (int*)calloc(nb1, sizeof(int)); o->kw_mfix = (int*)calloc(nb1, sizeof(int));
+    for (int b = 0; b < o->nbc; ++b) {   /* rans_set_bcs.cc:39-79 */
+        const fcfd_bc* bc = &opt->bc[b];
+        if (bc->type == FCFD_BC_INLET) { o->kw_kfix[b] = 1; o->kw_mfix[b] = 1; }
+        else if (bc->type == FCFD_BC_WALL && bc->kind != FCFD_KIND_SLIP) { o->kw_mfix[b] = 1; o->kw_mbc[b] = 0.; }
+    }
     return o;
 }
 
@@ -1233,6 +1259,8 @@ void orc_destroy(orc_ctx* o) {
     free(o->p); free(o->u); free(o->v); free(o->w); free(o->T);
     free(o->gp); free(o->gu); free(o->gv); free(o->gw); free(o->gT); free(o->nut); free(o->gn);
     free(o->phi); free(o->upd); free(o->Rs); free(o->Rabs); free(o->fC); free(o->fD); free(o->navg_ptr); free(o->navg_idx); free(o->navg_w); free(o->JL); free(o->JR); free(o->Dg); free(o->loads);
+    free(o->wL); free(o->kk); free(o->om); free(o->mut); free(o->strain); free(o->gk); free(o->gom);
+    free(o->kw_kbc); free(o->kw_obc); free(o->kw_mbc); free(o->kw_kfix); free(o->kw_mfix);
     free(o);
 }
 
@@ -1540,6 +1568,16 @@ static void sa_face_update(const orc_ctx* o, const cstate* L, const cstate* R, f
     F->mu_t = sa_mut(F->rho, F->nut, F->mu);
 }
 
+/* Variable::face_calculate (variable.h:134-147) of a RANS scalar: the BC
+ * value where it is fixed on the face's BC, else the face average */
+static double kw_face_value(const orc_ctx* o, const double* q, const int* fixed, const double* bcv, int f) {
+    int b = o->fbc[f];
+    if (b >= 0 && fixed[b]) return bcv[b];
+    double t = 0.;
+    for (int k = o->avg_ptr[f]; k < o->avg_ptr[f + 1]; ++k) t += o->avg_w[k] * q[o->avg_idx[k]];
+    return t;
+}
+
 /* face_state_update (:353-415) */
 static void face_state_update(const orc_ctx* o, const cstate* L, const cstate* R, fstate* F) {
     if (F->bc >= 0) {
@@ -1570,6 +1608,7 @@ static void face_state_update(const orc_ctx* o, const cstate* L, const cstate* R
     F->lambda = m_cond(o, F->T);
     F->gradn = vzero(); F->nut = 0.; F->rho = 0.; F->mu_t = 0.;
     if (o->nv == 6) sa_face_update(o, L, R, F, 1);
+    if (o->kw) F->mu_t = kw_face_value(o, o->mut, o->kw_mfix, o->kw_mbc, F->index);   /* rans[gid].mu_t.face (ns_diffusive_face_flux.cc:37) */
 }
 
 /* face_state_adjust (:460-500), including the case-1 fall-through */
@@ -1824,8 +1863,31 @@ static void sw_flux(const orc_ctx* o, const cstate* L, const cstate* R, double d
     }
 }
 
-/* convective_face_flux (ns_convective_face_flux.cc:32-59) */
-static void convective_face_flux(orc_ctx* o, const cstate* L, const cstate* R, const fstate* F, double flux[NVM]) {
+/* weightL.face, as each flux function leaves it (the segregated RANS
+ * convects k and omega with it, rans_terms.cc:79-84): Roe 0.5
+ * (ns_roe.cc:95); AUSM+-up and SD-SLAU 1 where the face mass flux is
+ * positive, else 0 (ns_ausm_plus_up.cc:93,99, ns_sd_slau.cc:84,90); van Leer
+ * 0 for M <= -1, 1 for M >= 1, else 0.5 (ns_vanleer.cc:39,48,62);
+ * Steger-Warming w = 0.5/(alpha^2 dp^2 + 1) of its second pass
+ * (ns_stegger_warming.cc:38-46).  fn0 is the normal mass flux. */
+static double weight_left(const orc_ctx* o, const cstate* L, const cstate* R, double fn0) {
+    switch (o->opt.convective_flux) {
+    case FCFD_FLUX_ROE: return 0.5;
+    case FCFD_FLUX_VAN_LEER: {
+        double M = 0.5 * (L->Vn.c[0] / L->a + R->Vn.c[0] / R->a);
+        return (M <= -1.) ? 0. : (M >= 1.) ? 1. : 0.5;
+    }
+    case FCFD_FLUX_SW: {
+        double deltap = fabs(L->p - R->p) / dmin(L->p, R->p);
+        return 0.5 / (6. * 6. * deltap * deltap + 1.);
+    }
+    default: return (fn0 > 0.) ? 1. : 0.;
+    }
+}
+
+/* convective_face_flux (ns_convective_face_flux.cc:32-59); wl (may be NULL)
+ * receives weightL */
+static void convective_face_flux(orc_ctx* o, const cstate* L, const cstate* R, const fstate* F, double flux[NVM], double* wl) {
     double fn[5];
     switch (o->opt.convective_flux) {
     case FCFD_FLUX_ROE: roe_flux(L, R, fn, o->opt.gamma); break;
@@ -1841,6 +1903,7 @@ static void convective_face_flux(orc_ctx* o, const cstate* L, const cstate* R, c
     flux[4] = fn[4] * F->area;
     /* SA: nu~ carried by the face mass flux, upwind face state */
     flux[5] = (o->nv == 6) ? flux[0] * ((flux[0] >= 0.) ? L->nut : R->nut) : 0.;
+    if (wl) *wl = weight_left(o, L, R, fn[0]);
 }
 
 /* diffusive_face_flux (ns_diffusive_face_flux.cc:28-64), laminar */
@@ -1861,7 +1924,7 @@ static void diffusive_face_flux(const orc_ctx* o, const fstate* F, double flux[N
     flux[2] = mu * vdot(ty, aV);
     flux[3] = mu * vdot(tz, aV);
     flux[4] = mu * (vdot(tx, F->V) * aV.c[0] + vdot(ty, F->V) * aV.c[1] + vdot(tz, F->V) * aV.c[2]);
-    double turb_cond = (o->nv == 6) ? o->Cp * F->mu_t / o->opt.Pr_t : 0.;   /* Cp(T_f) mu_t / Pr_t (:38) */
+    double turb_cond = (o->nv == 6 || o->kw) ? o->Cp * F->mu_t / o->opt.Pr_t : 0.;   /* Cp(T_f) mu_t / Pr_t (:38) */
     double qq = (F->lambda + turb_cond) * vdot(F->gradT, aV);
     if (F->bc >= 0 && o->opt.bc[F->bc].thermal == FCFD_THERMAL_FIXED_Q) qq = o->opt.bc[F->bc].qdot * F->area;
     flux[4] += qq;
@@ -2069,7 +2132,7 @@ int orc_assemble(orc_ctx* o, int with_jac) {
         int rc = right_state_update(o, &L, &R, &F);
         if (rc) { err |= 1; continue; }
         face_state_update(o, &L, &R, &F);
-        convective_face_flux(o, &L, &R, &F, fc);
+        convective_face_flux(o, &L, &R, &F, fc, &o->wL[f]);
         fd[0] = 0.;
         diffusive_face_flux(o, &F, fd);
         for (int i = 0; i < NVM; ++i) { o->fC[NVM * f + i] = fc[i]; o->fD[NVM * f + i] = fd[i]; }
@@ -2084,7 +2147,7 @@ int orc_assemble(orc_ctx* o, int with_jac) {
             else eps = sn;
             state_perturb(o, &Lp, &F, var, eps);
             if (F.bc >= 0) { rc = right_state_update(o, &Lp, &Rp, &F); if (rc) { err |= 1; break; } }
-            convective_face_flux(o, &Lp, &Rp, &F, fcp);
+            convective_face_flux(o, &Lp, &Rp, &F, fcp, NULL);
             face_state_adjust(o, &Lp, &Rp, &F, var);
             fdp[0] = 0.;
             diffusive_face_flux(o, &F, fdp);
@@ -2094,7 +2157,7 @@ int orc_assemble(orc_ctx* o, int with_jac) {
                 else if (R.update[var] < -sn) eps = dmin(-sn, factor * R.update[var]);
                 else eps = sn;
                 state_perturb(o, &Rp, &F, var, eps);
-                convective_face_flux(o, &L, &Rp, &F, fcp);
+                convective_face_flux(o, &L, &Rp, &F, fcp, NULL);
                 face_state_adjust(o, &L, &Rp, &F, var);
                 fdp[0] = 0.;
                 diffusive_face_flux(o, &F, fdp);
@@ -2302,7 +2365,7 @@ int orc_flux_probe(orc_ctx* o, int flux_type, const double* WL, const double* WR
     double wd0 = o->fwd ? o->fwd[0] : 0.;
     if (o->fdiss) { o->fdiss[0] = 1.; o->fwd[0] = 1.e20; }
     double fl[NVM];
-    convective_face_flux(o, &L, &R, &F, fl);
+    convective_face_flux(o, &L, &R, &F, fl, NULL);
     for (int i = 0; i < 5; ++i) out[i] = fl[i];
     if (o->fdiss) { o->fdiss[0] = diss0; o->fwd[0] = wd0; }
     o->opt.convective_flux = saved;
@@ -2682,5 +2745,357 @@ int orc_pseudo_system(orc_ctx* o, const double* dt, const double* dtau, int ws95
             rhs[(size_t)nv * c + i] = o->Rs[NVM * c + i] - corr;
         }
     }
+    return 0;
+}
+
+/* ================================================================== */
+/* segregated two-equation RANS (src/rans/): k-omega, k-epsilon in its */
+/* omega form, BSL and SST, solved after the NS step                   */
+/* ================================================================== */
+typedef struct { double sigma_k, sigma_omega, beta, beta_star, kappa, alpha; } kw_coef_t;
+static const kw_coef_t kw_komega = {0.85, 0.5, 0.075, 0.09, 0.41, 5. / 9.};   /* rans.cc:36-41 */
+static const kw_coef_t kw_keps = {1., 0.856, 0.0828, 0.09, 0.41, 0.44};       /* rans.cc:27-32 */
+
+static double ns_rho_cell(const orc_ctx* o, int c) { return m_rho(o, o->p[c], o->T[c]); }
+
+/* ns.rho.face (Variable::face_calculate): the BC's rho where fixed, else the face average */
+static double ns_face_rho(const orc_ctx* o, int f) {
+    int b = o->fbc[f];
+    if (b >= 0 && o->opt.bc[b].rho_fixed) return o->opt.bc[b].rho;
+    double t = 0.;
+    for (int k = o->avg_ptr[f]; k < o->avg_ptr[f + 1]; ++k) t += o->avg_w[k] * ns_rho_cell(o, o->avg_idx[k]);
+    return t;
+}
+
+/* get_blending (rans_terms.cc:316-328): the F1 blending function */
+static double kw_blending(double k, double omega, double rho, double y, double visc, V3 gk, V3 go) {
+    double cross_diff = gk.c[0] * go.c[0] + gk.c[1] * go.c[1] + gk.c[2] * go.c[2];
+    double CD = dmax(2. * rho * kw_keps.sigma_omega * cross_diff / omega, 1.e-10);
+    double arg1 = 2. * sqrt(k) / (kw_keps.beta_star * omega * y);
+    double arg2 = 500. * visc / (rho * y * y * omega);
+    double arg3 = 4. * rho * kw_keps.sigma_omega * k / (CD * y * y);
+    double arg = dmin(dmax(arg1, arg2), arg3);
+    return tanh(pow(arg, 4.));
+}
+
+/* the model constants blended with F1 (rans_terms.cc:89-93, 219-223) */
+static void kw_blend(double F1, kw_coef_t* c) {
+    c->sigma_omega = F1 * kw_komega.sigma_omega + (1. - F1) * kw_keps.sigma_omega;
+    c->sigma_k = F1 * kw_komega.sigma_k + (1. - F1) * kw_keps.sigma_k;
+    c->alpha = F1 * kw_komega.alpha + (1. - F1) * kw_keps.alpha;
+    c->beta = F1 * kw_komega.beta + (1. - F1) * kw_keps.beta;
+    c->beta_star = F1 * kw_komega.beta_star + (1. - F1) * kw_keps.beta_star;
+}
+
+/* sqrt(2 S_ij S_ij) from the velocity gradients (rans_terms.cc:242-246) */
+static double kw_strain(V3 gu, V3 gv, V3 gw) {
+    double dudx = gu.c[0], dudy = gu.c[1], dudz = gu.c[2];
+    double dvdx = gv.c[0], dvdy = gv.c[1], dvdz = gv.c[2];
+    double dwdx = gw.c[0], dwdy = gw.c[1], dwdz = gw.c[2];
+    return sqrt(2. * (dudx * dudx + dvdy * dvdy + dwdz * dwdz)
+                + (dudy + dvdx) * (dudy + dvdx) + (dudz + dwdx) * (dudz + dwdx)
+                + (dvdz + dwdy) * (dvdz + dwdy));
+}
+
+/* one cell of the source loop (rans_terms.cc:214-292): src[2] = the source
+ * terms times the volume, jac[3] = the destruction Jacobians times the
+ * volume, (k,k), (k,omega), (omega,omega); F1 = the blending (BSL, SST) or 1
+ * (k-omega) or 0 (k-epsilon); sr = the strain rate */
+static void kw_cell_source(int model, double F1, double rho, double k, double omega, double mu_t, double sr, double divU,
+                           V3 gk, V3 go, double vol, double src[2], double jac[3]) {
+    kw_coef_t m;
+    kw_blend(F1, &m);
+    double Prod_k = mu_t * (sr * sr - 2. / 3. * divU * divU) - 2. / 3. * rho * k * divU;
+    double Dest_k = m.beta_star * rho * omega * k;
+    Prod_k = dmin(Prod_k, 10. * Dest_k);
+    double cross_diffusion = 0.;
+    if (model != FCFD_RANS_KOMEGA) cross_diffusion = gk.c[0] * go.c[0] + gk.c[1] * go.c[1] + gk.c[2] * go.c[2];
+    double s0 = Prod_k;
+    s0 -= Dest_k;
+    double s1 = m.alpha * rho / mu_t * Prod_k;
+    s1 -= m.beta * rho * omega * omega;
+    s1 += 2. * (1. - F1) * rho * kw_komega.sigma_omega * cross_diffusion / omega;
+    src[0] = s0 * vol;
+    src[1] = s1 * vol;
+    jac[0] = m.beta_star * rho * omega * vol;
+    jac[1] = m.beta_star * rho * k * vol;
+    double v = 2. * m.beta * rho * omega * vol;
+    v += 2. * (1. - F1) * kw_komega.sigma_omega * kw_komega.sigma_omega * cross_diffusion / (omega * omega) * vol;
+    jac[2] = v;
+}
+
+/* update_boundaries of the RANS variables has no source (declared
+ * rans.h:99); as for the NS ghosts, a ghost takes the BC's value where the
+ * BC fixes k and omega (inlets, rans_set_bcs.cc:45) and its parent's
+ * otherwise */
+static void kw_update_boundaries(orc_ctx* o) {
+    for (int f = 0; f < o->nface; ++f) {
+        int b = o->fbc[f];
+        if (b < 0) continue;
+        int g = o->fneigh[f], p = o->fparent[f];
+        if (o->kw_kfix[b]) { o->kk[g] = o->kw_kbc[b]; o->om[g] = o->kw_obc[b]; }
+        else { o->kk[g] = o->kk[p]; o->om[g] = o->om[p]; }
+    }
+}
+
+/* update_eddy_viscosity (rans_update_eddy_viscosity.cc:25-59), every cell */
+static void kw_eddy_viscosity(orc_ctx* o) {
+    const double a1 = 0.31;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < o->ntot; ++c) {
+        double mu = m_visc(o, o->T[c]), rho = ns_rho_cell(o, c), d = o->wdist[c];
+        double mt;
+        if (o->opt.rans_model == FCFD_RANS_SST) {
+            double arg1 = 2. * sqrt(o->kk[c] + 1.e-15) / (kw_keps.beta_star * o->om[c] * d);
+            double arg2 = 500. * mu / (rho * o->om[c] * d * d);
+            double arg3 = dmax(arg1, arg2);
+            double F2 = tanh(arg3 * arg3);
+            mt = a1 * rho * o->kk[c] / dmax(a1 * o->om[c], o->strain[c] * F2);
+        } else {
+            mt = rho * o->kk[c] / o->om[c];
+        }
+        o->mut[c] = dmin(o->opt.visc_ratio_limit * mu, mt);
+    }
+}
+
+/* calc_cell_grads (rans.cc:172-178) with Variable::cell_gradient */
+static void kw_calc_cell_grads(orc_ctx* o) {
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < o->ncell; ++c) {
+        V3 a = vzero(), b = vzero();
+        if (o->g_ptr[c + 1] > o->g_ptr[c]) {
+            for (int e = o->g_ptr[c]; e < o->g_ptr[c + 1]; ++e) {
+                int j = o->g_idx[e];
+                for (int i = 0; i < 3; ++i) { a.c[i] += o->g_w[e].c[i] * o->kk[j]; b.c[i] += o->g_w[e].c[i] * o->om[j]; }
+            }
+        } else {   /* Green-Gauss */
+            for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
+                int f = o->cfaces[c].a[cf];
+                V3 aV = vdiv(vmul(o->fn[f], o->farea[f]), o->vol[c]);
+                if (o->fparent[f] != c) aV = vmul(aV, -1.);
+                double fk = kw_face_value(o, o->kk, o->kw_kfix, o->kw_kbc, f);
+                double fo = kw_face_value(o, o->om, o->kw_kfix, o->kw_obc, f);
+                for (int i = 0; i < 3; ++i) { a.c[i] += fk * aV.c[i]; b.c[i] += fo * aV.c[i]; }
+            }
+        }
+        o->gk[c] = a; o->gom[c] = b;
+    }
+}
+
+static void kw_refresh(orc_ctx* o) {
+    kw_update_boundaries(o);
+    kw_eddy_viscosity(o);
+    kw_calc_cell_grads(o);
+}
+
+int orc_kw_init(orc_ctx* o, double intensity, double visc_ratio) {
+    if (!o->kw) return FCFD_EINVAL;
+    /* apply_initial_conditions (rans.cc:120-170), one region over the whole domain */
+    for (int c = 0; c < o->ncell; ++c) {
+        double k = intensity * vmag(cellV(o, c));
+        k *= 1.5 * k;
+        k = dmax(o->opt.k_low, k);
+        o->kk[c] = k;
+        o->om[c] = ns_rho_cell(o, c) * k / (visc_ratio * m_visc(o, o->T[c]));
+        o->gk[c] = vzero(); o->gom[c] = vzero();
+    }
+    for (int c = 0; c < o->ntot; ++c) o->strain[c] = 0.;
+    /* set_bcs (rans_set_bcs.cc:39-73): inlet values from the NS face values
+     * of the BC's first face */
+    for (int b = 0; b < o->nbc; ++b) {
+        const fcfd_bc* bc = &o->opt.bc[b];
+        if (bc->type != FCFD_BC_INLET) continue;
+        int fid = 0;
+        for (int f = 0; f < o->nface; ++f) if (o->fbc[f] == b) { fid = f; break; }
+        double rho = ns_face_rho(o, fid);
+        double T = face_value_scalar(o, o->T, fid, 4);
+        V3 V = v3(face_value_scalar(o, o->u, fid, 1), face_value_scalar(o, o->v, fid, 2), face_value_scalar(o, o->w, fid, 3));
+        double kb = bc->turb_intensity * vmag(V);
+        kb *= 1.5 * kb;
+        kb = dmax(o->opt.k_low, kb);
+        o->kw_kbc[b] = kb;
+        o->kw_obc[b] = rho * kb / (bc->eddy_visc_ratio * m_visc(o, T));
+        o->kw_mbc[b] = bc->eddy_visc_ratio * m_visc(o, T);
+    }
+    kw_refresh(o);
+    return 0;
+}
+
+/* k, omega of the interior cells (N x 2), then ghosts, mu_t and gradients */
+int orc_kw_set(orc_ctx* o, const double* K) {
+    if (!o->kw) return FCFD_EINVAL;
+    for (int c = 0; c < o->ncell; ++c) { o->kk[c] = K[2 * c]; o->om[c] = K[2 * c + 1]; }
+    kw_refresh(o);
+    return 0;
+}
+
+/* the inlet values k, omega, mu_t of BC b (kbc[3]) */
+int orc_kw_bc_values(const orc_ctx* o, int b, double* out) {
+    if (b < 0 || b >= o->nbc) return FCFD_EINVAL;
+    out[0] = o->kw_kbc[b]; out[1] = o->kw_obc[b]; out[2] = o->kw_mbc[b];
+    return 0;
+}
+
+/* RANS::terms + time_terms (rans_terms.cc:39-292, rans_time_terms.cc:25-86
+ * without pseudo time) after orc_assemble at the same NS state: rhs[N][2],
+ * diagonal blocks diag[N][2][2] (row-major; every face's, then the sources',
+ * then rho V / dt), per face jf[F][6] = the face's contributions to its
+ * parent's rhs (diffusive - convective) for k and omega, jacL_k,
+ * jacL_omega, jacR_k, jacR_omega (jacR 0 on boundary faces; the operator's
+ * row P, column N entry is jacR, row N, column P is -jacL).  Sets the
+ * strain rate of every cell. */
+int orc_kw_terms(orc_ctx* o, const double* dt, double* rhs, double* diag, double* jf) {
+    if (!o->kw) return FCFD_EINVAL;
+    const int F = o->nface, N = o->ncell, model = o->opt.rans_model;
+    const int blended = model == FCFD_RANS_BSL || model == FCFD_RANS_SST;
+    double* fv = (double*)malloc(sizeof(double) * 6 * (size_t)F);
+#pragma omp parallel for schedule(static)
+    for (int f = 0; f < F; ++f) {
+        double blending = (model == FCFD_RANS_KOMEGA) ? 1. : 0.;
+        int parent = o->fparent[f], neighbor = o->fneigh[f], b = o->fbc[f];
+        double lam_visc = m_visc(o, face_value_scalar(o, o->T, f, 4));
+        double turb_visc = kw_face_value(o, o->mut, o->kw_mfix, o->kw_mbc, f);
+        double area = o->farea[f];
+        double mdot = (o->fC[NVM * f] - o->fD[NVM * f]) / area;   /* mdot.face (ns_assemble_linear_system.cc:122) */
+        double weightL = o->wL[f], weightR = 1. - weightL;
+        /* get_kOmega (rans_terms.cc:271-314) */
+        double leftK = o->kk[parent], leftOmega = o->om[parent], rightK = o->kk[neighbor], rightOmega = o->om[neighbor];
+        int extrapolated = 0;
+        V3 faceGradK, faceGradOmega;
+        if (b >= 0) {
+            int ty = o->opt.bc[b].type;
+            if (ty == FCFD_BC_SYMMETRY || ty == FCFD_BC_OUTLET) extrapolated = 1;
+            else if (ty == FCFD_BC_WALL && o->opt.bc[b].kind == FCFD_KIND_SLIP) extrapolated = 1;
+            faceGradK = o->gk[parent]; faceGradOmega = o->gom[parent];
+        } else {
+            faceGradK = vzero(); faceGradOmega = vzero();
+            for (int k = o->avg_ptr[f]; k < o->avg_ptr[f + 1]; ++k) {
+                faceGradK = vadd(faceGradK, vsc(o->avg_w[k], o->gk[o->avg_idx[k]]));
+                faceGradOmega = vadd(faceGradOmega, vsc(o->avg_w[k], o->gom[o->avg_idx[k]]));
+            }
+        }
+        double faceK = kw_face_value(o, o->kk, o->kw_kfix, o->kw_kbc, f);
+        double faceOmega = kw_face_value(o, o->om, o->kw_kfix, o->kw_obc, f);
+        double faceRho = ns_face_rho(o, f);
+        V3 left2right = vsub(o->cc[neighbor], o->cc[parent]);
+        V3 l2rn = vnorm(left2right);
+        double l2rmag = vmag(left2right);
+        faceGradK = vsub(faceGradK, vsc(vdot(faceGradK, l2rn), l2rn));
+        faceGradK = vadd(faceGradK, vsc((rightK - leftK) / l2rmag, l2rn));
+        faceGradOmega = vsub(faceGradOmega, vsc(vdot(faceGradOmega, l2rn), l2rn));
+        faceGradOmega = vadd(faceGradOmega, vsc((rightOmega - leftOmega) / l2rmag, l2rn));
+        double conv0 = mdot * (weightL * leftK + weightR * rightK) * area;
+        double conv1 = mdot * (weightL * leftOmega + weightR * rightOmega) * area;
+        if (blended) blending = kw_blending(faceK, faceOmega, faceRho, o->wdist[parent], lam_visc, faceGradK, faceGradOmega);
+        kw_coef_t m;
+        kw_blend(blending, &m);
+        double dn = vdot(faceGradK, o->fn[f]), dno = vdot(faceGradOmega, o->fn[f]);
+        double diff0 = (lam_visc + turb_visc * m.sigma_k) * dn * area;
+        double diff1 = (lam_visc + turb_visc * m.sigma_omega) * dno * area;
+        double AoverH = area * vdot(o->fn[f], vnorm(left2right)) / l2rmag;
+        double jl0 = weightL * mdot * area, jl1 = weightL * mdot * area;
+        if (extrapolated) { jl0 += weightR * mdot * area; jl1 += weightR * mdot * area; }
+        if (!extrapolated) { jl0 += (lam_visc + turb_visc * m.sigma_k) * AoverH; jl1 += (lam_visc + turb_visc * m.sigma_omega) * AoverH; }
+        double jr0 = 0., jr1 = 0.;
+        if (b < 0) {
+            jr0 = weightR * mdot * area;
+            jr0 -= (lam_visc + turb_visc * m.sigma_k) * AoverH;
+            jr1 = weightR * mdot * area;
+            jr1 -= (lam_visc + turb_visc * m.sigma_omega) * AoverH;
+        }
+        double* q = &fv[6 * (size_t)f];
+        q[0] = diff0 - conv0; q[1] = diff1 - conv1; q[2] = jl0; q[3] = jl1; q[4] = jr0; q[5] = jr1;
+        if (jf) for (int i = 0; i < 6; ++i) jf[6 * (size_t)f + i] = q[i];
+        if (f == F - 1) o->kw_lam_last = lam_visc;
+    }
+    /* the strain rate of every cell (rans_terms.cc:242-249) */
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < o->ntot; ++c) o->strain[c] = kw_strain(o->gu[c], o->gv[c], o->gw[c]);
+    /* per cell: faces in ascending order (the order of the face loop's
+     * ADD_VALUES), then the sources, then the time term */
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < N; ++c) {
+        double r0 = 0., r1 = 0., d00 = 0., d11 = 0.;
+        for (int k = 0; k < o->cfaces[c].n; ++k) {
+            int f = o->cfaces[c].a[k];
+            const double* q = &fv[6 * (size_t)f];
+            if (o->fparent[f] == c) { r0 += q[0]; r1 += q[1]; d00 += q[2]; d11 += q[3]; }
+            else { r0 += -1. * q[0]; r1 += -1. * q[1]; d00 += -q[4]; d11 += -q[5]; }
+        }
+        double rho = ns_rho_cell(o, c);
+        double F1 = (model == FCFD_RANS_KOMEGA) ? 1. : 0.;
+        if (blended) F1 = kw_blending(o->kk[c], o->om[c], rho, o->wdist[c], o->kw_lam_last, o->gk[c], o->gom[c]);
+        double divU = o->gu[c].c[0] + o->gv[c].c[1] + o->gw[c].c[2];
+        double src[2], jac[3];
+        kw_cell_source(model, F1, rho, o->kk[c], o->om[c], o->mut[c], o->strain[c], divU, o->gk[c], o->gom[c], o->vol[c], src, jac);
+        r0 += src[0]; r1 += src[1];
+        d00 += jac[0]; d11 += jac[2];
+        double tt = rho * o->vol[c] / dt[c];   /* rans_time_terms.cc:65-66 */
+        d00 += tt; d11 += tt;
+        rhs[2 * c] = r0; rhs[2 * c + 1] = r1;
+        diag[4 * c] = d00; diag[4 * c + 1] = jac[1]; diag[4 * c + 2] = 0.; diag[4 * c + 3] = d11;
+    }
+    free(fv);
+    return 0;
+}
+
+/* update_variables (rans.cc:180-252) with the update dU[N][2] and the NS
+ * time step dt[N]: the limits, k += dk, omega += domega, then the ghosts,
+ * mu_t (with the strain rate of the last orc_kw_terms) and the gradients.
+ * sums[2] = sum dk^2/dt^2, sum domega^2/dt^2.  FCFD_EDIVERGED on a
+ * NaN/Inf update (nothing changed). */
+int orc_kw_update(orc_ctx* o, const double* dU, const double* dt, double* sums) {
+    if (!o->kw) return FCFD_EINVAL;
+    for (int c = 0; c < o->ncell; ++c)
+        for (int i = 0; i < 2; ++i)
+            if (isnan(dU[2 * c + i]) || isinf(dU[2 * c + i])) return FCFD_EDIVERGED;
+    double s0 = 0., s1 = 0.;
+    for (int c = 0; c < o->ncell; ++c) {
+        double u0 = dU[2 * c], u1 = dU[2 * c + 1];
+        u0 = dmax(-1. * (o->kk[c] - o->opt.k_low), u0);
+        u0 = dmin((o->opt.k_high - o->kk[c]), u0);
+        u1 = dmax(-1. * (o->om[c] - o->opt.omega_low), u1);
+        o->kk[c] += u0;
+        o->om[c] += u1;
+        double dt2 = dt[c] * dt[c];
+        s0 += u0 * u0 / dt2;
+        s1 += u1 * u1 / dt2;
+    }
+    if (sums) { sums[0] = s0; sums[1] = s1; }
+    kw_refresh(o);
+    return 0;
+}
+
+/* K[ntot][2], mu_t[ntot], G[ntot][6] (grad k, grad omega), strain[ntot],
+ * mutf[F] (mu_t.face); any may be NULL */
+int orc_kw_get(const orc_ctx* o, double* K, double* mut, double* G, double* strain, double* mutf) {
+    for (int c = 0; c < o->ntot; ++c) {
+        if (K) { K[2 * c] = o->kk[c]; K[2 * c + 1] = o->om[c]; }
+        if (mut) mut[c] = o->mut[c];
+        if (strain) strain[c] = o->strain[c];
+        if (G) for (int i = 0; i < 3; ++i) { G[6 * c + i] = o->gk[c].c[i]; G[6 * c + 3 + i] = o->gom[c].c[i]; }
+    }
+    if (mutf) for (int f = 0; f < o->nface; ++f) mutf[f] = kw_face_value(o, o->mut, o->kw_mfix, o->kw_mbc, f);
+    return 0;
+}
+
+/* weightL of the faces of the last orc_assemble (F) */
+int orc_get_weight_left(const orc_ctx* o, double* wl) { memcpy(wl, o->wL, sizeof(double) * o->nface); return 0; }
+
+/* one cell's source terms for the pin tests: gvel = du, dv, dw gradients
+ * (9), gk, go (3 each); lam_visc enters the blending; out = source k,
+ * source omega, Jacobians (k,k), (k,omega), (omega,omega), F1, strain rate,
+ * SST/k-omega mu_t of the point (the strain rate's) */
+int orc_kw_source_probe(int model, double rho, double k, double omega, double mu_t, double lam_visc, const double* gvel,
+                        const double* gk, const double* go, double d, double vol, double* out) {
+    V3 gu = v3(gvel[0], gvel[1], gvel[2]), gv = v3(gvel[3], gvel[4], gvel[5]), gw = v3(gvel[6], gvel[7], gvel[8]);
+    V3 K = v3(gk[0], gk[1], gk[2]), O = v3(go[0], go[1], go[2]);
+    double F1 = (model == FCFD_RANS_KOMEGA) ? 1. : 0.;
+    if (model == FCFD_RANS_BSL || model == FCFD_RANS_SST) F1 = kw_blending(k, omega, rho, d, lam_visc, K, O);
+    double sr = kw_strain(gu, gv, gw);
+    double divU = gu.c[0] + gv.c[1] + gw.c[2];
+    double src[2], jac[3];
+    kw_cell_source(model, F1, rho, k, omega, mu_t, sr, divU, K, O, vol, src, jac);
+    out[0] = src[0]; out[1] = src[1]; out[2] = jac[0]; out[3] = jac[1]; out[4] = jac[2]; out[5] = F1; out[6] = sr;
     return 0;
 }

@@ -86,6 +86,20 @@ int orc_ausm_probe(double M, double alpha, double* out);
 /* SA model pieces at one point: out = source/volume, P, D, mu_t, fn */
 int orc_sa_probe(double rho, double mu, double nut, const double* gvel, const double* gn, double d, double* out);
 
+/* segregated two-equation RANS (src/rans/), for opt.turbulence = FCFD_TURB_KW.
+ * The NS state, its ghosts and gradients must be current (orc_set_state,
+ * orc_update_boundaries, orc_calc_cell_grads) before init/set; orc_assemble
+ * at the same state before orc_kw_terms (mdot and weightL of its faces). */
+int orc_kw_init(orc_ctx* o, double intensity, double visc_ratio);
+int orc_kw_set(orc_ctx* o, const double* K);
+int orc_kw_bc_values(const orc_ctx* o, int b, double* out);
+int orc_kw_terms(orc_ctx* o, const double* dt, double* rhs, double* diag, double* jf);
+int orc_kw_update(orc_ctx* o, const double* dU, const double* dt, double* sums);
+int orc_kw_get(const orc_ctx* o, double* K, double* mut, double* G, double* strain, double* mutf);
+int orc_get_weight_left(const orc_ctx* o, double* wl);
+int orc_kw_source_probe(int model, double rho, double k, double omega, double mu_t, double lam_visc, const double* gvel,
+                        const double* gk, const double* go, double d, double vol, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,14 @@ def lib():
         L.orc_gelimd_probe.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_wtli_probe.argtypes = [C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
+        L.orc_kw_init.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_kw_set.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_kw_bc_values.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_kw_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_kw_update.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_kw_get.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        L.orc_get_weight_left.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_kw_source_probe.argtypes = [C.c_int] + [C.c_double] * 5 + [C.c_void_p] * 3 + [C.c_double] * 2 + [C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -228,6 +236,64 @@ class Oracle:
         out = np.zeros((self.info.n_cells, nv, nv))
         self._check(lib().orc_time_terms(self.h, _p(dt), _p(out)))
         return out
+
+    # ---- segregated two-equation RANS (src/rans/) ----
+    def kw_init(self, intensity=1e-4, visc_ratio=0.1):
+        """apply_initial_conditions over the whole domain + set_bcs +
+        ghosts, mu_t and gradients, at the current NS state (after
+        residual() or prepare())"""
+        self._check(lib().orc_kw_init(self.h, float(intensity), float(visc_ratio)))
+
+    def kw_set(self, K):
+        K = np.ascontiguousarray(K, dtype=np.float64)
+        self._check(lib().orc_kw_set(self.h, _p(K)))
+
+    def kw_bc_values(self, b):
+        out = np.zeros(3)
+        self._check(lib().orc_kw_bc_values(self.h, int(b), _p(out)))
+        return out
+
+    def kw_terms(self, dt):
+        """after residual() at the same NS state: (rhs N x 2, diag N x 2 x 2,
+        jf F x 6 = parent-side flux contribution k, omega, jacL_k, jacL_omega,
+        jacR_k, jacR_omega)"""
+        N, F = self.info.n_cells, self.info.n_faces
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        rhs, diag, jf = np.zeros((N, 2)), np.zeros((N, 2, 2)), np.zeros((F, 6))
+        self._check(lib().orc_kw_terms(self.h, _p(dt), _p(rhs), _p(diag), _p(jf)))
+        return rhs, diag, jf
+
+    def kw_update(self, dU, dt):
+        dU = np.ascontiguousarray(dU, dtype=np.float64)
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        sums = np.zeros(2)
+        self._check(lib().orc_kw_update(self.h, _p(dU), _p(dt), _p(sums)))
+        return sums
+
+    def kw_fields(self):
+        """dict: K (N+B) x 2, mut (N+B), G (N+B) x 6 (grad k, grad omega),
+        strain (N+B), mutf F"""
+        T, F = self.info.n_cells + self.info.n_ghosts, self.info.n_faces
+        d = dict(K=np.zeros((T, 2)), mut=np.zeros(T), G=np.zeros((T, 6)), strain=np.zeros(T), mutf=np.zeros(F))
+        self._check(lib().orc_kw_get(self.h, _p(d["K"]), _p(d["mut"]), _p(d["G"]), _p(d["strain"]), _p(d["mutf"])))
+        return d
+
+    def weight_left(self):
+        wl = np.zeros(self.info.n_faces)
+        self._check(lib().orc_get_weight_left(self.h, _p(wl)))
+        return wl
+
+
+def kw_source_probe(model, rho, k, omega, mu_t, lam_visc, grad_vel, gk, go, d, vol=1.0):
+    """one cell of the two-equation source loop: dict(src (2), jac (k,k),
+    (k,omega), (omega,omega), F1, strain)"""
+    gv = np.ascontiguousarray(grad_vel, dtype=np.float64).reshape(9)
+    a = np.ascontiguousarray(gk, dtype=np.float64).reshape(3)
+    b = np.ascontiguousarray(go, dtype=np.float64).reshape(3)
+    out = np.zeros(7)
+    lib().orc_kw_source_probe(int(model), float(rho), float(k), float(omega), float(mu_t), float(lam_visc), _p(gv), _p(a),
+                              _p(b), float(d), float(vol), _p(out))
+    return dict(src=out[0:2].copy(), jac=out[2:5].copy(), F1=out[5], strain=out[6])
 
 
 def sa_probe(rho, mu, nut, grad_vel, grad_nut, d):
