@@ -80,7 +80,22 @@ class NewtonStats(C.Structure):
     _fields_ = [("res_norm", C.c_double), ("update_norms", C.c_double * 3), ("krylov_iters", C.c_int),
                 ("krylov_converged", C.c_int), ("krylov_rnorm", C.c_double), ("krylov_r0norm", C.c_double),
                 ("ms_residual", C.c_double), ("ms_jacobian", C.c_double), ("ms_krylov", C.c_double),
-                ("ms_update", C.c_double), ("nut_norm", C.c_double), ("relax", C.c_double)]
+                ("ms_update", C.c_double), ("nut_norm", C.c_double), ("relax", C.c_double),
+                ("rans_iters", C.c_int), ("rans_norms", C.c_double * 2)]
+
+
+class RansOpts(C.Structure):
+    _fields_ = [("restart", C.c_int), ("max_iters", C.c_int), ("rtol", C.c_double), ("abstol", C.c_double)]
+
+
+class RansStats(C.Structure):
+    _fields_ = [("krylov_iters", C.c_int), ("krylov_converged", C.c_int), ("krylov_rnorm", C.c_double),
+                ("krylov_r0norm", C.c_double), ("norms", C.c_double * 2)]
+
+
+def rans_opts(restart=30, max_iters=10, rtol=1e-6, abstol=1e-12):
+    """Defaults follow the reference's turbulence section (read_inputs.cc:113-115)."""
+    return RansOpts(restart=restart, max_iters=max_iters, rtol=rtol, abstol=abstol)
 
 
 PC_BLOCK_JACOBI, PC_BLOCK_SGS, PC_MULTIGRID = 0, 1, 2

@@ -48,6 +48,11 @@ def lib():
         "fcfd_set_bc": (C.c_int, [_vp, C.c_int, C.POINTER(A.BC)]),
         "fcfd_surface": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "fcfd_pseudo_step": (C.c_int, [_vp, _vp, C.POINTER(A.NewtonOpts), C.POINTER(A.PseudoOpts), C.POINTER(A.NewtonStats), _vp]),
+        "fcfd_rans_init": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.POINTER(A.RansOpts), _vp]),
+        "fcfd_rans_set": (C.c_int, [_vp, _vp, _vp, _vp]),
+        "fcfd_rans_get": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+        "fcfd_rans_terms": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "fcfd_rans_step": (C.c_int, [_vp, _vp, _vp, C.POINTER(A.RansOpts), C.POINTER(A.RansStats), _vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -24,7 +24,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 
 HIP_SOURCES = ["fcfd_residual.hip", "fcfd_jacobian.hip", "fcfd_tangent.hip", "fcfd_adjoint.hip", "fcfd_krylov.hip", "fcfd_mg.hip",
-               "fcfd_api.hip"]
+               "fcfd_rans.hip", "fcfd_api.hip"]
 CXX_SOURCES = ["fcfd_setup.cpp"]
 
 

@@ -77,6 +77,7 @@ __global__ void __launch_bounds__(256) face_adj_kernel(DevGrid g, Params P, Cell
         if (bnd && k >= Zr::R && k < Zr::Rc) { Z[k * Fs + f] = 0.0; continue; }
         FaceCtx<Dual, NV> X;
         X.F = X0.F; X.p = p; X.n = n;
+        if constexpr (VISC == kViscRans) X.mut_face = X0.mut_face;
         X.L.p = seed(X0.L.p, k, Zr::L);
         X.L.V = mk<Dual>(seed(X0.L.V.x, k, Zr::L + 1), seed(X0.L.V.y, k, Zr::L + 2), seed(X0.L.V.z, k, Zr::L + 3));
         X.L.T = seed(X0.L.T, k, Zr::L + 4);
@@ -387,6 +388,7 @@ struct AdjLaunch {
     static void face_v(const DevGrid& g, const Params& P, const CellLoader<NV>& cl, const double* Rbar, const double* fxw, double* Z,
                        int* status, hipStream_t s) {
         if constexpr (NV == 6) face<FLUX, 1>(g, P, cl, Rbar, fxw, Z, status, s);
+        else if (P.kw) face<FLUX, kViscRans>(g, P, cl, Rbar, fxw, Z, status, s);
         else if (P.viscous) face<FLUX, 1>(g, P, cl, Rbar, fxw, Z, status, s);
         else face<FLUX, 0>(g, P, cl, Rbar, fxw, Z, status, s);
     }
@@ -407,6 +409,8 @@ struct AdjLaunch {
         dim3 grid((nlf + 255) / 256), blk(256);
         if constexpr (NV == 6)
             hipLaunchKernelGGL((loads_face_kernel<FLUX, 1, NV>), grid, blk, 0, s, g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status);
+        else if (P.kw)
+            hipLaunchKernelGGL((loads_face_kernel<FLUX, kViscRans, NV>), grid, blk, 0, s, g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status);
         else if (P.viscous)
             hipLaunchKernelGGL((loads_face_kernel<FLUX, 1, NV>), grid, blk, 0, s, g, a, P, cl, mcx, mcy, mcz, tmp, nlf, status);
         else

@@ -17,6 +17,7 @@
 #include "fcfd_jacobian.h"
 #include "fcfd_krylov.h"
 #include "fcfd_mg.h"
+#include "fcfd_rans.h"
 #include "fcfd_tangent.h"
 
 #include <chrono>
@@ -74,6 +75,17 @@ struct fcfd_ctx {
     AdjWork aw{};
     int n_loadfaces = 0;
     double *loads_tmp = nullptr, *loads_out = nullptr;
+    // segregated two-equation RANS (opt.turbulence = FCFD_TURB_KW): fields,
+    // terms, Krylov workspace at nv = 2, the inlets' k, omega, mu_t
+    bool kw_ready = false;            // fcfd_rans_init or fcfd_rans_set has run
+    KwDev kw{};
+    KwTerms kwt{};
+    double *kw_dU = nullptr, *kw_part = nullptr, *kw_out = nullptr, *kw_inl = nullptr, *kw_Dinv = nullptr;
+    const int* kw_f0 = nullptr;       // per BC: internal id of its first face in reference order (-1: none)
+    std::vector<double> kw_bcv;       // 3 per BC: k, omega, mu_t (inlets)
+    GmresWork<double> kw_w;
+    int kw_restart = 0;
+    fcfd_rans_opts kw_opts{30, 10, 1e-6, 1e-12};   // used inside fcfd_newton_step (the reference's defaults)
     std::string err;
     // optional per-stage timing: a ring of event quadruples recorded on the
     // launch stream around the three kernels, folded in lazily so that
@@ -145,7 +157,15 @@ int validate_options(fcfd_ctx* c) {
             bc.specified == FCFD_SPEC_NONE)   // ns_apply_bcs.cc:78-81,123-126
             return fail(c, FCFD_EBC, "At least one of p,T,rho needs to be specified at Inlet at BC_" + std::to_string(b + 1));
     }
-    if (o.turbulence != FCFD_TURB_NONE && o.turbulence != FCFD_TURB_SA) return fail(c, FCFD_EINVAL, "unknown turbulence model");
+    if (o.turbulence != FCFD_TURB_NONE && o.turbulence != FCFD_TURB_SA && o.turbulence != FCFD_TURB_KW)
+        return fail(c, FCFD_EINVAL, "unknown turbulence model");
+    if (o.turbulence == FCFD_TURB_KW) {
+        if (o.rans_model < FCFD_RANS_KEPSILON || o.rans_model > FCFD_RANS_SST) return fail(c, FCFD_EINVAL, "unknown two-equation RANS model");
+        if (!(o.visc_model == FCFD_VISC_SUTHERLAND || o.mu > 0.0))
+            return fail(c, FCFD_EINVAL, "the two-equation RANS model needs a positive molecular viscosity");
+        if (!(o.Pr_t > 0.0)) return fail(c, FCFD_EINVAL, "the two-equation RANS model needs a positive turbulent Prandtl number");
+        if (!(o.omega_low > 0.0) || !(o.k_high > o.k_low)) return fail(c, FCFD_EINVAL, "two-equation RANS limits: omega_low > 0, k_high > k_low");
+    }
     if (o.turbulence == FCFD_TURB_SA) {
         if (!(o.visc_model == FCFD_VISC_SUTHERLAND || o.mu > 0.0))
             return fail(c, FCFD_EINVAL, "the Spalart-Allmaras model needs a positive molecular viscosity");
@@ -173,6 +193,9 @@ void set_params(fcfd_ctx* c) {
     P.viscous = (inviscid && !fixedq) ? 0 : 1;
     P.nv = (o.turbulence == FCFD_TURB_SA) ? 6 : 5;
     P.Pr_t = o.Pr_t;
+    P.kw = o.turbulence == FCFD_TURB_KW;
+    P.rans_model = o.rans_model;
+    P.k_low = o.k_low; P.k_high = o.k_high; P.omega_low = o.omega_low; P.visc_ratio_limit = o.visc_ratio_limit;
     double nut_scale = 0.0;
     for (int b = 0; b < o.n_bc; ++b) nut_scale = std::max(nut_scale, std::fabs(o.bc[b].nut));
     P.nut_eps = 10. * 1.4901161193847656e-08 * (nut_scale > 0.0 ? nut_scale : 1.0);
@@ -186,6 +209,16 @@ void set_params(fcfd_ctx* c) {
         d.total_area = b < (int)c->hg.bc_area.size() ? c->hg.bc_area[b] : 0.0;
         d.nut = s.nut;
         d.p_fixed = s.p_fixed; d.T_fixed = s.T_fixed; d.V_fixed = s.V_fixed; d.qdot_fixed = s.qdot_fixed;
+        d.rho_fixed = s.rho_fixed;
+        // two-equation RANS (rans_set_bcs.cc:39-79): inlets fix k, omega and
+        // mu_t (values from fcfd_rans_init), no-slip walls fix mu_t = 0
+        d.kw_kfix = s.type == FCFD_BC_INLET;
+        d.kw_mfix = s.type == FCFD_BC_INLET || (s.type == FCFD_BC_WALL && s.kind != FCFD_KIND_SLIP);
+        const bool have = (int)c->kw_bcv.size() >= 3 * (b + 1);
+        d.kw_k = (d.kw_kfix && have) ? c->kw_bcv[3 * b] : 0.0;
+        d.kw_om = (d.kw_kfix && have) ? c->kw_bcv[3 * b + 1] : 0.0;
+        d.kw_mut = (s.type == FCFD_BC_INLET && have) ? c->kw_bcv[3 * b + 2] : 0.0;
+        d.kw_turb_intensity = s.turb_intensity; d.kw_eddy_visc_ratio = s.eddy_visc_ratio;
     }
 }
 
@@ -406,6 +439,24 @@ int build_device(fcfd_ctx* c) {
         for (int i = 0; i < T; ++i) g.wdist[g.iperm[i]] = tw[i];
         if (faces)
             for (int i = 0; i < F; ++i) { g.fwd[g.ifperm[i]] = tf[i]; g.fdiss[g.ifperm[i]] = td[i]; }
+    }
+    if (c->P.kw) {   // the two-equation model's fields (mu_t read by every VISC = kViscRans face kernel)
+        int rc;
+        KwDev& k = c->kw;
+        KwTerms& t = c->kwt;
+        const size_t b2 = 2 * (size_t)pair_groups(4);
+        if ((rc = dalloc(c, 2 * (size_t)ld, &k.K)) || (rc = dalloc(c, (size_t)ld, &k.mut)) || (rc = dalloc(c, 6 * (size_t)ld, &k.GK)) ||
+            (rc = dalloc(c, (size_t)ld, &k.strain)) || (rc = dalloc(c, 6 * (size_t)F, &t.FV)) || (rc = dalloc(c, b2 * F, &t.JL)) ||
+            (rc = dalloc(c, b2 * F, &t.JR)) || (rc = dalloc(c, 2 * (size_t)ld, &t.rhs)) || (rc = dalloc(c, b2 * ld, &t.D)) ||
+            (rc = dalloc(c, 1, &t.lam_last)) || (rc = dalloc(c, 2 * (size_t)ld, &c->kw_dU)) || (rc = dalloc(c, 3 * 256, &c->kw_part)) ||
+            (rc = dalloc(c, 4, &c->kw_out)) || (rc = dalloc(c, 3 * (size_t)std::max(c->opt.n_bc, 1), &c->kw_inl)) ||
+            (rc = dalloc(c, b2 * ld, &c->kw_Dinv)))
+            return rc;
+        d.kw_mut = k.mut;
+        std::vector<int> f0(std::max(c->opt.n_bc, 1), -1);
+        for (int f = g.F - 1; f >= 0; --f)
+            if (g.fbc[f] >= 0) f0[g.fbc[f]] = g.fperm[f];
+        if ((rc = upload(c, f0, &c->kw_f0))) return rc;
     }
     c->device = true;
     return FCFD_OK;
@@ -1141,6 +1192,212 @@ static int ensure_colors(fcfd_ctx* c) {
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// segregated two-equation RANS
+// ---------------------------------------------------------------------------
+namespace {
+
+int kw_check(fcfd_ctx* c) {
+    if (!c) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    if (!c->P.kw) return fail(c, FCFD_EINVAL, "the context has no two-equation RANS model (turbulence != FCFD_TURB_KW)");
+    return FCFD_OK;
+}
+
+int kw_opts_check(fcfd_ctx* c, const fcfd_rans_opts* o) {
+    if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad RANS Krylov options");
+    return FCFD_OK;
+}
+
+// the nv = 2 view of the grid for the Krylov kernels
+DevGrid kw_grid(const fcfd_ctx* c) {
+    DevGrid g2 = c->dg;
+    g2.nv = 2;
+    return g2;
+}
+
+int kw_ensure_krylov(fcfd_ctx* c, int restart) {
+    if (c->kw_restart == restart) return FCFD_OK;
+    for (void* p : {(void*)c->kw_w.V, (void*)c->kw_w.z, (void*)c->kw_w.r, (void*)c->kw_w.part, (void*)c->kw_w.out_dev,
+                    (void*)c->kw_w.coef_dev})
+        dfree(c, p);
+    const size_t n2 = 2 * (size_t)c->dg.ld, m1 = (size_t)restart + 1;
+    int rc;
+    if ((rc = dalloc(c, m1 * n2, &c->kw_w.V)) || (rc = dalloc(c, n2, &c->kw_w.z)) || (rc = dalloc(c, n2, &c->kw_w.r)) ||
+        (rc = dalloc(c, m1 * 512, &c->kw_w.part)) || (rc = dalloc(c, m1, &c->kw_w.out_dev)) || (rc = dalloc(c, m1, &c->kw_w.coef_dev)))
+        return rc;
+    c->kw_restart = restart;
+    return FCFD_OK;
+}
+
+// NS ghosts, gradients and limiters of W (the RANS face loop reads them)
+void kw_ns_prepare(fcfd_ctx* c, double* W, hipStream_t s) {
+    launch_ghosts(c->dg, c->P, W, c->status, s);
+    launch_grad_lim(c->dg, c->P, W, c->G, c->PHI, c->dref, s);
+}
+
+// RANS::solve at the NS state W (ghosts, gradients and limiters current)
+int kw_solve(fcfd_ctx* c, double* W, const double* dt, const fcfd_rans_opts* o, fcfd_rans_stats* st, hipStream_t s) {
+    int rc;
+    if ((rc = kw_opts_check(c, o)) || (rc = kw_ensure_krylov(c, o->restart))) return rc;
+    const DevGrid g2 = kw_grid(c);
+    kw_launch_terms(c->dg, c->P, W, c->G, c->PHI, dt, c->kw, c->kwt, c->hg.fperm[c->hg.F - 1], c->status, s);
+    dev_bj_invert<double>(g2, c->kwt.D, c->kw_Dinv, c->status, s);
+    LinOp<double> A{[&](const double* x, double* y, hipStream_t ss) { dev_bsr_matvec<double>(g2, c->kwt.D, c->kwt.JL, c->kwt.JR, x, y, ss); }};
+    LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) { dev_bj_apply<double>(g2, c->kw_Dinv, x, y, ss); }};
+    GmresOpts go;
+    go.restart = o->restart; go.max_iters = o->max_iters; go.rtol = o->rtol; go.abstol = o->abstol;
+    go.knoll = 1;   // KSPSetInitialGuessKnoll (rans_petsc_functions.cc:76)
+    GmresStats gs;
+    rc = gmres<double>(g2, A, M, c->kwt.rhs, c->kw_dU, go, c->kw_w, gs, s);
+    if (rc) return fail(c, rc, "RANS GMRES failed (HIP error)");
+    double out[3];
+    HIPCHK(c, kw_update(c->dg, c->P, c->kw, c->kw_dU, dt, c->kw_part, c->kw_out, out, s));
+    if (st) {
+        st->krylov_iters = gs.iters; st->krylov_converged = gs.converged; st->krylov_rnorm = gs.rnorm; st->krylov_r0norm = gs.r0norm;
+        st->norms[0] = std::sqrt(out[0]); st->norms[1] = std::sqrt(out[1]);
+    }
+    if (out[2] > 0.0) return fail(c, FCFD_EDIVERGED, "Divergence detected in RANS (rans.cc:195-200)");
+    kw_launch_refresh(c->dg, c->P, W, c->kw, s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(s));
+    int bad = 0;
+    HIPCHK(c, hipMemcpy(&bad, c->status, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad & 2) return fail(c, FCFD_EDIVERGED, "singular diagonal block in the RANS block-Jacobi preconditioner");
+    return FCFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fcfd_rans_init(fcfd_ctx* c, double* W_dev, double intensity, double visc_ratio, const fcfd_rans_opts* opts, void* stream) {
+    int rc = kw_check(c);
+    if (rc) return rc;
+    if (!W_dev) return FCFD_EINVAL;
+    if (opts && (rc = kw_opts_check(c, opts))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const int nbc = c->opt.n_bc;
+    launch_ghosts(c->dg, c->P, W_dev, c->status, s);
+    kw_launch_init(c->dg, c->P, W_dev, intensity, visc_ratio, c->kw, s);
+    kw_launch_inlets(c->dg, c->P, nbc, W_dev, c->kw_f0, c->kw_inl, s);
+    std::vector<double> h(3 * (size_t)std::max(nbc, 1), 0.0);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->kw_inl, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->kw_bcv = h;
+    set_params(c);
+    // the BC table is read by every kernel: replace it with no work in flight
+    HIPCHK(c, hipDeviceSynchronize());
+    HIPCHK(c, hipMemcpy((void*)c->dg.bc, c->bch.data(), sizeof(BcDev) * c->bch.size(), hipMemcpyHostToDevice));
+    kw_launch_refresh(c->dg, c->P, W_dev, c->kw, s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (opts) c->kw_opts = *opts;
+    c->kw_ready = true;
+    return FCFD_OK;
+}
+
+int fcfd_rans_set(fcfd_ctx* c, double* W_dev, const double* K_host, void* stream) {
+    int rc = kw_check(c);
+    if (rc) return rc;
+    if (!W_dev || !K_host) return FCFD_EINVAL;
+    if (c->kw_bcv.empty()) return fail(c, FCFD_EINVAL, "fcfd_rans_set needs fcfd_rans_init first (the inlet values)");
+    const HostGrid& hg = c->hg;
+    const size_t ld = c->dg.ld;
+    std::vector<double> h(2 * ld, 0.0);
+    HIPCHK(c, hipDeviceSynchronize());
+    HIPCHK(c, hipMemcpy(h.data(), c->kw.K, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int r = 0; r < hg.N; ++r) {
+        const int i = hg.perm[r];
+        h[i] = K_host[2 * (size_t)r];
+        h[ld + i] = K_host[2 * (size_t)r + 1];
+    }
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(c, hipMemcpyAsync(c->kw.K, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    launch_ghosts(c->dg, c->P, W_dev, c->status, s);
+    kw_launch_refresh(c->dg, c->P, W_dev, c->kw, s);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->kw_ready = true;
+    return FCFD_OK;
+}
+
+int fcfd_rans_get(fcfd_ctx* c, double* K_host, double* mut_host, double* G_host, double* strain_host, double* mutf_host) {
+    int rc = kw_check(c);
+    if (rc) return rc;
+    const HostGrid& hg = c->hg;
+    const size_t ld = c->dg.ld;
+    const int T = hg.N + hg.B;
+    HIPCHK(c, hipDeviceSynchronize());
+    std::vector<double> K(2 * ld), mut(ld), G(6 * ld), st(ld);
+    HIPCHK(c, hipMemcpy(K.data(), c->kw.K, K.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(mut.data(), c->kw.mut, mut.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(G.data(), c->kw.GK, G.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(st.data(), c->kw.strain, st.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int r = 0; r < T; ++r) {
+        const int i = hg.perm[r];
+        if (K_host) { K_host[2 * (size_t)r] = K[i]; K_host[2 * (size_t)r + 1] = K[ld + i]; }
+        if (mut_host) mut_host[r] = mut[i];
+        if (strain_host) strain_host[r] = st[i];
+        if (G_host && r < hg.N)
+            for (int q = 0; q < 6; ++q) G_host[6 * (size_t)r + q] = G[q * ld + i];
+    }
+    if (mutf_host) {   // mu_t.face: the BC's value where fixed, else the face average (Variable::face_calculate)
+        for (int f = 0; f < hg.F; ++f) {
+            const int b = hg.fbc[f];
+            if (b >= 0 && c->bch[b].kw_mfix) { mutf_host[f] = c->bch[b].kw_mut; continue; }
+            double t = 0.;
+            for (int k = hg.aptr[f]; k < hg.aptr[f + 1]; ++k) t += hg.aw[k] * mut[hg.perm[hg.aidx[k]]];
+            mutf_host[f] = t;
+        }
+    }
+    return FCFD_OK;
+}
+
+int fcfd_rans_terms(fcfd_ctx* c, double* W_dev, const double* dt_dev, double* rhs_host, double* diag_host, double* jf_host,
+                    void* stream) {
+    int rc = kw_check(c);
+    if (rc) return rc;
+    if (!W_dev || !dt_dev) return FCFD_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    kw_ns_prepare(c, W_dev, s);
+    kw_launch_terms(c->dg, c->P, W_dev, c->G, c->PHI, dt_dev, c->kw, c->kwt, c->hg.fperm[c->hg.F - 1], c->status, s);
+    HIPCHK(c, hipGetLastError());
+    const HostGrid& hg = c->hg;
+    const size_t ld = c->dg.ld, F = hg.F;
+    std::vector<double> rhs(2 * ld), D(4 * ld), FV(6 * F);
+    HIPCHK(c, hipMemcpyAsync(rhs.data(), c->kwt.rhs, rhs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(D.data(), c->kwt.D, D.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(FV.data(), c->kwt.FV, FV.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    for (int r = 0; r < hg.N; ++r) {
+        const int i = hg.perm[r];
+        if (rhs_host) { rhs_host[2 * (size_t)r] = rhs[i]; rhs_host[2 * (size_t)r + 1] = rhs[ld + i]; }
+        if (diag_host)
+            for (int q = 0; q < 4; ++q) diag_host[4 * (size_t)r + q] = D[pidx(ld, q, i)];
+    }
+    if (jf_host)
+        for (int f = 0; f < hg.F; ++f) {
+            const int i = hg.fperm[f];
+            for (int q = 0; q < 6; ++q) jf_host[6 * (size_t)f + q] = FV[q * F + i];
+        }
+    return FCFD_OK;
+}
+
+int fcfd_rans_step(fcfd_ctx* c, double* W_dev, const double* dt_dev, const fcfd_rans_opts* o, fcfd_rans_stats* st, void* stream) {
+    int rc = kw_check(c);
+    if (rc) return rc;
+    if (!W_dev || !dt_dev || !o) return FCFD_EINVAL;
+    if (!c->kw_ready) return fail(c, FCFD_EINVAL, "fcfd_rans_step needs fcfd_rans_init or fcfd_rans_set first");
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(c, hipMemsetAsync(c->status, 0, sizeof(int), s));
+    kw_ns_prepare(c, W_dev, s);
+    return kw_solve(c, W_dev, dt_dev, o, st, s);
+}
+
+}  // extern "C"
+
 // the neighbour cells (ghosts included) of the interior cells in internal
 // numbering, for the WS95 pressure-difference floor
 static int ensure_neighbours(fcfd_ctx* c) {
@@ -1171,6 +1428,8 @@ static int newton_impl(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, co
             return fail(c, FCFD_EINVAL, "unknown pseudo-time preconditioner");
         if (ps->ps_step < 1) return fail(c, FCFD_EINVAL, "pseudo steps count from 1");
         if (ps->ps_step > 1 && !ps->Wn_dev) return fail(c, FCFD_EINVAL, "a pseudo step after the first needs the state at time level n");
+        if (c->P.kw && c->kw_ready)
+            return fail(c, FCFD_ENOTSUP, "dual time stepping with the segregated two-equation RANS model is not supported");
         if (!c->dtau && ((rc0 = dalloc(c, (size_t)c->dg.ld, &c->dtau)) || (rc0 = dalloc(c, (size_t)c->dg.ld, &c->drdp)))) return rc0;
         if (ps->preconditioner == FCFD_PSEUDO_PC_WS95 && (rc0 = ensure_neighbours(c))) return rc0;
     }
@@ -1332,6 +1591,18 @@ static int newton_impl(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, co
     }
     if (rc) return fail(c, rc, "GMRES failed (HIP error)");
     HIPCHK(c, hipStreamSynchronize(s));
+    // 3b. the segregated two-equation model's solve at the same NS state
+    //     (NavierStokes::solve calls RANS::solve after petsc_solve and before
+    //     update_variables, ns.cc:106-110)
+    if (c->P.kw && c->kw_ready) {
+        fcfd_rans_stats rs{};
+        rc = kw_solve(c, W_dev, c->dt, &c->kw_opts, &rs, s);
+        S.rans_iters = rs.krylov_iters; S.rans_norms[0] = rs.norms[0]; S.rans_norms[1] = rs.norms[1];
+        if (rc) {
+            if (st) *st = S;
+            return rc;
+        }
+    }
     auto t3 = clk::now();
     // 4. update W (relaxed if it would change p or T by more than
     //    max_rel_update), norms, NaN check; keep dW as the previous update

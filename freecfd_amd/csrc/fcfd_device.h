@@ -41,6 +41,10 @@ struct DevGrid {
     // (parent, neighbour or ghost) of internal face f
     const int* ggc;
     const double2* ggw;
+    // eddy viscosity of the segregated two-equation model per cell ([ld],
+    // interior and ghosts; NULL without it), read by the VISC = kViscRans face
+    // kernels
+    const double* kw_mut;
 };
 
 // Gradients ([3 nv] components per cell), limiters ([nv]) and the FD

@@ -115,6 +115,7 @@ struct FaceCtx {
     FState<S> fs;
     S C[NV], D[NV];
     int p, n;
+    double mut_face;   // VISC = kViscRans: mu_t.face of the two-equation model (frozen: no tangent)
 };
 
 // Stage 1 of a face-loop iteration: geometry, the two cell loads and the
@@ -171,6 +172,20 @@ __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_,
             if (F.bc >= 0) X.fs.gn = gp[5];
             else X.fs.gn = vadd(vsc(wp, gp[5]), vsc(wn, gn[5]));
         }
+        if constexpr (VISC == kViscRans) {
+            // rans.mu_t.face (ns_diffusive_face_flux.cc:37; Variable::face_calculate,
+            // variable.h:134-147): the BC's value where mu_t is fixed on it
+            // (inlets, no-slip walls), else the two-cell face average in key order
+            const BcDev* bcp = F.bc >= 0 ? &g.bc[F.bc] : nullptr;
+            if (bcp && bcp->kw_mfix) {
+                X.mut_face = bcp->kw_mut;
+            } else {
+                double t = 0.;
+                t += wp * g.kw_mut[p];
+                t += wn * g.kw_mut[n];
+                X.mut_face = t;
+            }
+        }
     }
 }
 
@@ -200,7 +215,8 @@ __device__ __forceinline__ void face_tail(const DevGrid& g, const Params& P, int
     convective_flux<FLUX, NV>(P, X.L, X.R, F, FLUX == FCFD_FLUX_SW ? g.fdiss[f] : 0.0, FLUX == FCFD_FLUX_SW ? g.fwd[f] : 0.0, X.C);
     if (VISC) {
         face_state<NV>(P, X.L, X.R, F, X.fs);
-        diffusive_flux<NV>(P, X.fs, F, g.bc, X.D);
+        if constexpr (VISC == kViscRans) X.fs.mu_t = S(X.mut_face);
+        diffusive_flux<NV, VISC == kViscRans>(P, X.fs, F, g.bc, X.D);
     } else {
 #pragma unroll
         for (int v = 0; v < NV; ++v) X.D[v] = S(0.0);

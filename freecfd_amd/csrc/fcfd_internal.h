@@ -58,7 +58,15 @@ struct Params {
     int nv;          // variables per cell: 5, or 6 with the SA model (nu~ last)
     double Pr_t;     // turbulent Prandtl number (SA)
     double nut_eps;  // FD step floor for nu~ (10 sqrt(eps) times the largest inlet nu~)
+    // segregated two-equation RANS (turbulence = FCFD_TURB_KW): the NS face
+    // loop takes mu_t from the model; model constants and limits
+    int kw, rans_model;
+    double k_low, k_high, omega_low, visc_ratio_limit;
 };
+
+// the face kernels' VISC template argument: 0 inviscid, 1 viscous (laminar or
+// SA), 2 viscous with the eddy viscosity of the segregated two-equation model
+constexpr int kViscRans = 2;
 
 // BC table entry on the device (uniform BC values).
 struct BcDev {
@@ -67,6 +75,12 @@ struct BcDev {
     double nut;      // SA nu~ prescribed at an inlet
     int p_fixed, T_fixed, V_fixed;   // Variable::fixedonBC of p, T, V (face values in Green-Gauss gradients)
     int qdot_fixed;                  // qdot.fixedonBC: the face loop keeps the given qdot (ns_assemble_linear_system.cc:125)
+    int rho_fixed;                   // rho.fixedonBC (ns.rho.face of the two-equation model's face loop)
+    // two-equation RANS face values (rans_set_bcs.cc:39-79): k and omega fixed
+    // at inlets, mu_t fixed at inlets and no-slip walls (0 there)
+    int kw_kfix, kw_mfix;
+    double kw_k, kw_om, kw_mut;
+    double kw_turb_intensity, kw_eddy_visc_ratio;   // the inlet's turbulenceintensity, eddyviscosityratio
 };
 
 }  // namespace fcfd

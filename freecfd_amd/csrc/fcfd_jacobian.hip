@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(128) fdjac_kernel(DevGrid g, Params P, const d
         convective_flux<FLUX, NV>(P, Lp, Rp, F, diss, cwd, Cp);
         if (VISC) {
             face_state_adjust<NV>(P, Lp, Rp, F, X.fs, var);
-            diffusive_flux<NV>(P, X.fs, F, g.bc, Dp);
+            diffusive_flux<NV, VISC == kViscRans>(P, X.fs, F, g.bc, Dp);
         } else {
 #pragma unroll
             for (int j = 0; j < NV; ++j) Dp[j] = 0.0;
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(128) fdjac_kernel(DevGrid g, Params P, const d
             convective_flux<FLUX, NV>(P, X.L, Rp, F, diss, cwd, Cp);
             if (VISC) {
                 face_state_adjust<NV>(P, X.L, Rp, F, X.fs, var);
-                diffusive_flux<NV>(P, X.fs, F, g.bc, Dp);
+                diffusive_flux<NV, VISC == kViscRans>(P, X.fs, F, g.bc, Dp);
             }
 #pragma unroll
             for (int j = 0; j < NV; ++j) JR[pidx(Fs, NV * var + j, f)] = (Dp[j] - X.D[j] - Cp[j] + X.C[j]) / eps2;
@@ -226,6 +226,7 @@ template <int FLUX>
 static void launch_fdjac_v(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI, int has_phi,
                            double phi_const, const double* UPD, double* JL, double* JR, int* status, hipStream_t s) {
     if (g.nv == 6) launch_fdjac_t<FLUX, 1, 6>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s);
+    else if (P.kw) launch_fdjac_t<FLUX, kViscRans, 5>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s);
     else if (P.viscous) launch_fdjac_t<FLUX, 1, 5>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s);
     else launch_fdjac_t<FLUX, 0, 5>(g, P, W, G, PHI, has_phi, phi_const, UPD, JL, JR, status, s);
 }

@@ -458,6 +458,8 @@ hipError_t dev_mdot(const DevGrid& g, int k, const T* V, size_t stride, const T*
                     double* out_host, hipStream_t s) {
     if (g.nv == 6)
         hipLaunchKernelGGL((mdot_partial_kernel<T, 6>), dim3(kRedBlocks, (k + kDotGroup - 1) / kDotGroup), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, k, V, stride, w, part);
+    else if (g.nv == 2)   // the segregated two-equation model's system
+        hipLaunchKernelGGL((mdot_partial_kernel<T, 2>), dim3(kRedBlocks, (k + kDotGroup - 1) / kDotGroup), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, k, V, stride, w, part);
     else
         hipLaunchKernelGGL((mdot_partial_kernel<T, 5>), dim3(kRedBlocks, (k + kDotGroup - 1) / kDotGroup), dim3(kRedThreads), 0, s, g.N, (size_t)g.ld, k, V, stride, w, part);
     hipLaunchKernelGGL(reduce_final_kernel, dim3(k), dim3(kRedThreads), 0, s, part, out_dev);
@@ -486,6 +488,7 @@ template void dev_lincomb<float>(const DevGrid&, int, const float*, size_t, cons
 template <typename T>
 void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s) {
     if (g.nv == 6) hipLaunchKernelGGL((bsr_matvec_kernel<T, 6>), dim3((g.N + 255) / 256), dim3(256), 0, s, g, D, JL, JR, x, y);
+    else if (g.nv == 2) hipLaunchKernelGGL((bsr_matvec_kernel<T, 2>), dim3((g.N + 255) / 256), dim3(256), 0, s, g, D, JL, JR, x, y);
     else hipLaunchKernelGGL((bsr_matvec_kernel<T, 5>), dim3((g.N + 255) / 256), dim3(256), 0, s, g, D, JL, JR, x, y);
 }
 template void dev_bsr_matvec<double>(const DevGrid&, const double*, const double*, const double*, const double*, double*, hipStream_t);
@@ -513,11 +516,13 @@ template void dev_sell_matvec<float>(const DevGrid&, const SellBlocks&, const fl
 template <typename T>
 void dev_bj_invert(const DevGrid& g, const double* D, T* Dinv, int* status, hipStream_t s) {
     if (g.nv == 6) hipLaunchKernelGGL((bj_invert_kernel<T, 6>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
+    else if (g.nv == 2) hipLaunchKernelGGL((bj_invert_kernel<T, 2>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
     else hipLaunchKernelGGL((bj_invert_kernel<T, 5>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
 }
 template <typename T>
 void dev_bj_apply(const DevGrid& g, const T* Dinv, const T* x, T* y, hipStream_t s) {
     if (g.nv == 6) hipLaunchKernelGGL((bj_apply_kernel<T, 6>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
+    else if (g.nv == 2) hipLaunchKernelGGL((bj_apply_kernel<T, 2>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
     else hipLaunchKernelGGL((bj_apply_kernel<T, 5>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
 }
 void dev_bj_apply_t(const DevGrid& g, const double* Dinv, const double* x, double* y, hipStream_t s) {

@@ -540,8 +540,9 @@ __device__ __forceinline__ void face_state(const Params& P, const CState<S>& L, 
 
 // diffusive_face_flux (ns_diffusive_face_flux.cc:28-64): mu + turb_visc and
 // lambda + Cp turb_visc / Pr_t (:36-39), turb_visc = mu_t(face) of the SA
-// model (0 without it); plus the SA diffusion of nu~
-template <int NV, typename S>
+// model or (RANS) of the segregated two-equation model, 0 without either;
+// plus the SA diffusion of nu~
+template <int NV, bool RANS = false, typename S>
 __device__ __forceinline__ void diffusive_flux(const Params& P, const FState<S>& f, const FaceGeo& F, const BcDev* bc, S flux[NV]) {
     Vec<double> aV = mk<double>(F.n.x * F.area, F.n.y * F.area, F.n.z * F.area);
     Vec<S> tx, ty, tz;
@@ -561,7 +562,7 @@ __device__ __forceinline__ void diffusive_flux(const Params& P, const FState<S>&
     flux[3] = mu * vdot(tz, aV);
     flux[4] = mu * (vdot(tx, f.V) * aV.x + vdot(ty, f.V) * aV.y + vdot(tz, f.V) * aV.z);
     S tc = S(0.);
-    if constexpr (NV == 6) tc = P.Cp * f.mu_t / P.Pr_t;
+    if constexpr (NV == 6 || RANS) tc = P.Cp * f.mu_t / P.Pr_t;
     S qq = (f.lambda + tc) * vdot(f.gT, aV);
     if (F.bc >= 0 && bc[F.bc].thermal == FCFD_THERMAL_FIXED_Q) qq = S(bc[F.bc].qdot * F.area);
     flux[4] = flux[4] + qq;

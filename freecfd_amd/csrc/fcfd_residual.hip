@@ -523,6 +523,7 @@ template <int FLUX>
 static void launch_flux_v(const DevGrid& g, const Params& P, const double* W, const double* G, const double* PHI,
                           int has_phi, double phi_const, double* FX, int* status, hipStream_t s) {
     if (g.nv == 6) launch_flux_t<FLUX, 1, 6>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);   // SA is always viscous
+    else if (P.kw) launch_flux_t<FLUX, kViscRans, 5>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);
     else if (P.viscous) launch_flux_t<FLUX, 1, 5>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);
     else launch_flux_t<FLUX, 0, 5>(g, P, W, G, PHI, has_phi, phi_const, FX, status, s);
 }
@@ -532,6 +533,7 @@ static void launch_surface_v(const DevGrid& g, const Params& P, const double* W,
                              double phi_const, double* out, int* status, hipStream_t s) {
     const dim3 grid((g.F + 63) / 64), blk(64);
     if (g.nv == 6) hipLaunchKernelGGL((surface_kernel<FLUX, 1, 6>), grid, blk, 0, s, g, P, W, G, PHI, has_phi, phi_const, out, status);
+    else if (P.kw) hipLaunchKernelGGL((surface_kernel<FLUX, kViscRans, 5>), grid, blk, 0, s, g, P, W, G, PHI, has_phi, phi_const, out, status);
     else if (P.viscous) hipLaunchKernelGGL((surface_kernel<FLUX, 1, 5>), grid, blk, 0, s, g, P, W, G, PHI, has_phi, phi_const, out, status);
     else hipLaunchKernelGGL((surface_kernel<FLUX, 0, 5>), grid, blk, 0, s, g, P, W, G, PHI, has_phi, phi_const, out, status);
 }

@@ -257,6 +257,7 @@ static void launch_ft(const DevGrid& g, const Params& P, const DualLoader<NV, FU
 template <int FLUX, bool FULL, int NV, typename TD>
 static void launch_fv(const DevGrid& g, const Params& P, const DualLoader<NV, FULL, TD>& L, TD* dFX, int* status, hipStream_t s) {
     if constexpr (NV == 6) launch_ft<FLUX, 1, FULL, 6, TD>(g, P, L, dFX, status, s);   // SA is always viscous
+    else if (P.kw) launch_ft<FLUX, kViscRans, FULL, 5, TD>(g, P, L, dFX, status, s);
     else if (P.viscous) launch_ft<FLUX, 1, FULL, 5, TD>(g, P, L, dFX, status, s);
     else launch_ft<FLUX, 0, FULL, 5, TD>(g, P, L, dFX, status, s);
 }

@@ -338,3 +338,66 @@ class NavierStokes(Grid):
 
     def status(self, clear=True):
         _check(self.h, lib().fcfd_status(self.h, int(clear)))
+
+    # ---- segregated two-equation RANS (the reference's RANS object, src/rans/) ----
+    def rans_initialize(self, intensity, visc_ratio, opts=None, W=None):
+        """RANS::initialize (rans.cc:70-81) at the NS state: initial k,
+        omega over the domain from the turbulence intensity and eddy
+        viscosity ratio, the inlets' values, ghosts, mu_t, gradients.  opts
+        (RansOpts) are used by the solve inside newton_step."""
+        W = self.W if W is None else W
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_rans_init(self.h, W.data_ptr(), float(intensity), float(visc_ratio),
+                                            None if opts is None else C.byref(opts), C.c_void_p(s)))
+
+    def rans_set(self, K, W=None):
+        """k, omega of the interior cells (N x 2, reference order), then
+        ghosts, mu_t and gradients"""
+        W = self.W if W is None else W
+        K = np.ascontiguousarray(K, dtype=np.float64)
+        assert K.shape == (self.info.n_cells, 2)
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_rans_set(self.h, W.data_ptr(), K.ctypes.data, C.c_void_p(s)))
+
+    def rans_fields(self):
+        """dict in reference order: K (N+B) x 2, mut N+B, G N x 6 (grad k,
+        grad omega), strain N+B, mutf F (mu_t.face)"""
+        N, B, F = self.info.n_cells, self.info.n_ghosts, self.info.n_faces
+        d = dict(K=np.zeros((N + B, 2)), mut=np.zeros(N + B), G=np.zeros((N, 6)), strain=np.zeros(N + B), mutf=np.zeros(F))
+        _check(self.h, lib().fcfd_rans_get(self.h, d["K"].ctypes.data, d["mut"].ctypes.data, d["G"].ctypes.data,
+                                           d["strain"].ctypes.data, d["mutf"].ctypes.data))
+        return d
+
+    def rans_terms(self, dt, W=None):
+        """RANS::terms + time_terms with the NS time step dt ([ld] device
+        tensor): (rhs N x 2, diag N x 2 x 2, jf F x 6) in reference order"""
+        W = self.W if W is None else W
+        N, F = self.info.n_cells, self.info.n_faces
+        rhs, diag, jf = np.zeros((N, 2)), np.zeros((N, 2, 2)), np.zeros((F, 6))
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_rans_terms(self.h, W.data_ptr(), dt.data_ptr(), rhs.ctypes.data, diag.ctypes.data,
+                                             jf.ctypes.data, C.c_void_p(s)))
+        return rhs, diag, jf
+
+    def rans_solve(self, dt, opts=None, W=None, check=True):
+        """RANS::solve (rans.cc:85-98) at the NS state with the NS time step
+        dt ([ld] device tensor); returns RansStats"""
+        W = self.W if W is None else W
+        opts = A.rans_opts() if opts is None else opts
+        st = A.RansStats()
+        s = self.torch.cuda.current_stream().cuda_stream
+        rc = lib().fcfd_rans_step(self.h, W.data_ptr(), dt.data_ptr(), C.byref(opts), C.byref(st), C.c_void_p(s))
+        if check:
+            _check(self.h, rc)
+            return st
+        return rc, st
+
+    def time_step_dev(self, dt_type, value, W=None):
+        """fcfd_time_step into a fresh [ld] device tensor"""
+        W = self.W if W is None else W
+        dt = self.torch.zeros(self.ld, dtype=self.torch.float64, device=self.device)
+        mc = C.c_double()
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_time_step(self.h, W.data_ptr(), int(dt_type), float(value), dt.data_ptr(), C.byref(mc),
+                                           C.c_void_p(s)))
+        return dt

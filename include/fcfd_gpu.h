@@ -368,7 +368,25 @@ typedef struct fcfd_newton_stats {
     double ms_residual, ms_jacobian, ms_krylov, ms_update;
     double nut_norm;          /* sqrt(sum dnu~^2/dt^2) with the SA model, else 0 */
     double relax;             /* factor applied to the update (1 unless max_rel_update limited it) */
+    /* the segregated two-equation model's solve inside the step (FCFD_TURB_KW
+     * after fcfd_rans_init / fcfd_rans_set; else 0): its Krylov iterations
+     * and sqrt(sum dk^2/dt^2), sqrt(sum domega^2/dt^2) (rans.cc:206-222) */
+    int rans_iters;
+    double rans_norms[2];
 } fcfd_newton_stats;
+
+/* ---- segregated two-equation RANS (opt.turbulence = FCFD_TURB_KW) ---- */
+typedef struct fcfd_rans_opts {
+    int restart;          /* GMRES restart length (the reference uses FGMRES(300))                    */
+    int max_iters;        /* turbulence maximumiterations (default 10, read_inputs.cc:115)         */
+    double rtol, abstol;  /* turbulence relativetolerance / absolutetolerance (1e-6 / 1e-12,
+                             read_inputs.cc:113-114)                                               */
+} fcfd_rans_opts;
+typedef struct fcfd_rans_stats {
+    int krylov_iters, krylov_converged;
+    double krylov_rnorm, krylov_r0norm;
+    double norms[2];      /* sqrt(sum dk^2/dt^2), sqrt(sum domega^2/dt^2) of the limited update   */
+} fcfd_rans_stats;
 
 /* ---- dual (pseudo) time stepping ---- */
 #define FCFD_PSEUDO_PC_NONE 0   /* pseudo-time block = cons2prim (ns_time_terms.cc:26-56) */
@@ -477,6 +495,55 @@ int  fcfd_adjoint_step(fcfd_ctx* ctx, double* W_dev, const double* g_dev, double
  * the parameter derivatives of an adjoint; the reference reads BC values
  * once from its input file (src/set_bcs.cc:35, ns_set_bcs.cc:25-180). */
 int  fcfd_set_bc(fcfd_ctx* ctx, int index, const fcfd_bc* bc);
+
+/* ---- segregated two-equation RANS (src/rans/), FCFD_TURB_KW ----
+ * The context holds k, omega, mu_t and their gradients on the device (the
+ * RANS object of the reference); the NS face loop of every entry point takes
+ * its mu_t from them (rans.mu_t.face, ns_diffusive_face_flux.cc:36-39), frozen
+ * in the NS Jacobian and tangents as in the reference.  Until fcfd_rans_init
+ * or fcfd_rans_set runs, mu_t is 0.  Every call below returns FCFD_EINVAL on
+ * a context without the model. */
+
+/* RANS::initialize (rans.cc:70-81): apply_initial_conditions over the whole
+ * domain with the given turbulence intensity and eddy viscosity ratio
+ * (rans.cc:120-170), set_bcs (the inlets' k, omega, mu_t from the NS face
+ * values of each inlet's first face, rans_set_bcs.cc:39-79), ghosts, eddy
+ * viscosity (rans_update_eddy_viscosity.cc:25-59) and gradients
+ * (rans.cc:172-178), at the NS state W_dev (its ghost slots are written).
+ * opts (may be NULL: the reference's defaults, restart 30) are kept for the
+ * solve that fcfd_newton_step runs.  Synchronises the device. */
+int  fcfd_rans_init(fcfd_ctx* ctx, double* W_dev, double intensity, double visc_ratio, const fcfd_rans_opts* opts, void* stream);
+/* k, omega of the interior cells from the host (N x 2, reference numbering),
+ * then ghosts, eddy viscosity and gradients at W_dev (a restart's
+ * read_restart, rans_read_restart.cc).  Needs fcfd_rans_init first (the inlet
+ * values).  Synchronises the stream. */
+int  fcfd_rans_set(fcfd_ctx* ctx, double* W_dev, const double* K_host, void* stream);
+/* To the host in reference numbering (any may be NULL): K_host (N+B) x 2 =
+ * k, omega; mut_host N+B; G_host N x 6 = grad k, grad omega; strain_host N+B
+ * (the strain rate of the last terms evaluation); mutf_host F = mu_t.face.
+ * Synchronises the device. */
+int  fcfd_rans_get(fcfd_ctx* ctx, double* K_host, double* mut_host, double* G_host, double* strain_host, double* mutf_host);
+/* RANS::terms + time_terms (rans_terms.cc:39-292, rans_time_terms.cc:25-86,
+ * no pseudo time) at the NS state W_dev with the NS time step dt_dev ([ld]),
+ * the NS face loop's mdot and weightL recomputed at W_dev: to the host in
+ * reference numbering, rhs_host N x 2, diag_host N x 4 (row-major 2 x 2
+ * diagonal blocks: faces, sources, rho V / dt), jf_host F x 6 (per face: its
+ * rhs contributions to the parent, k and omega, then jacL_k, jacL_omega,
+ * jacR_k, jacR_omega; the operator's row P, column N entry is jacR, row N,
+ * column P is -jacL).  Any output may be NULL.  Recomputes ghosts, gradients
+ * and limiters of W_dev; sets the strain rate.  Synchronises the stream. */
+int  fcfd_rans_terms(fcfd_ctx* ctx, double* W_dev, const double* dt_dev, double* rhs_host, double* diag_host, double* jf_host,
+                     void* stream);
+/* RANS::solve (rans.cc:85-98) at the NS state W_dev with the NS time step
+ * dt_dev: terms and time terms, GMRES with 2 x 2 block-Jacobi preconditioning
+ * (PETSc FGMRES in the reference, rans_petsc_functions.cc:25-122),
+ * update_variables with its limits (rans.cc:180-252), then ghosts, eddy
+ * viscosity and gradients.  W_dev is not changed (its ghost slots are
+ * written).  FCFD_EDIVERGED on a NaN/Inf update (k, omega unchanged; the
+ * reference exits).  fcfd_newton_step runs the same solve between its
+ * Krylov solve and its update, as NavierStokes::solve does (ns.cc:110). */
+int  fcfd_rans_step(fcfd_ctx* ctx, double* W_dev, const double* dt_dev, const fcfd_rans_opts* opts, fcfd_rans_stats* stats,
+                    void* stream);
 
 #ifdef __cplusplus
 }

@@ -70,3 +70,35 @@ def test_set_bc_refuses_switching_the_diffusive_flux():
         assert L.fcfd_set_bc(h, wall, ctypes.byref(bc)) == 0
     finally:
         L.fcfd_destroy(h)
+
+
+def test_two_equation_rans_validation_on_a_host_context():
+    """FCFD_TURB_KW contexts are created on the host; an unknown model or a
+    zero molecular viscosity is refused at create, and the model's entry
+    points refuse a context without device data or without the model."""
+    from freecfd_amd import _abi as A, cases
+    cs = cases.kw_case(model=A.RANS_SST)
+    L = _lib.lib()
+    m = A.make_mesh(cs.raw)
+    h = ctypes.c_void_p()
+    assert L.fcfd_create_host(ctypes.byref(m), ctypes.byref(cs.opt), ctypes.byref(h)) == 0
+    try:
+        assert L.fcfd_rans_init(h, None, 0.05, 10.0, None, None) == A.EINVAL
+        assert L.fcfd_rans_get(h, None, None, None, None, None) == A.EINVAL
+        assert b"device" in L.fcfd_last_error(h)
+    finally:
+        L.fcfd_destroy(h)
+    bad = cases.kw_case(model=A.RANS_SST)
+    bad.opt.rans_model = 7
+    assert L.fcfd_create_host(ctypes.byref(A.make_mesh(bad.raw)), ctypes.byref(bad.opt), ctypes.byref(h)) == A.EINVAL
+    assert b"RANS" in L.fcfd_last_error(None)
+    inv = cases.kw_case(model=A.RANS_SST)
+    inv.opt.visc_model, inv.opt.mu = A.VISC_CONSTANT, 0.0
+    assert L.fcfd_create_host(ctypes.byref(A.make_mesh(inv.raw)), ctypes.byref(inv.opt), ctypes.byref(h)) == A.EINVAL
+    lam = cases.wedge_case(4, 2)
+    hl = ctypes.c_void_p()
+    assert L.fcfd_create_host(ctypes.byref(A.make_mesh(lam.raw)), ctypes.byref(lam.opt), ctypes.byref(hl)) == 0
+    try:
+        assert L.fcfd_rans_get(hl, None, None, None, None, None) == A.EINVAL
+    finally:
+        L.fcfd_destroy(hl)

@@ -1,0 +1,272 @@
+"""GPU parity of the segregated two-equation RANS (src/rans/) against the CPU
+restatement, through the C-ABI (fcfd_rans_*):
+
+  - initial and inlet values, ghosts, mu_t and gradients (RANS::initialize,
+    rans.cc:70-81; rans_set_bcs.cc:39-79; rans_update_eddy_viscosity.cc:25-59;
+    rans.cc:172-178) for k-epsilon, k-omega, BSL and SST on channel variants
+    that run all five convective fluxes and every inlet, outlet and wall kind;
+  - the terms (rans_terms.cc:39-292 + rans_time_terms.cc:25-86): right-hand
+    side, diagonal blocks and every face's contributions, with the NS face
+    loop's mdot and weightL;
+  - the NS residual and FD Jacobian with mu_t from the model
+    (ns_diffusive_face_flux.cc:36-39);
+  - RANS::solve: the GPU's GMRES update equals the direct sparse solve of the
+    oracle's system, followed by the oracle's update_variables (limits) and
+    refresh;
+  - fcfd_newton_step runs the same solve at the pre-update NS state
+    (ns.cc:106-110).
+Tolerances: the model uses tanh and pow (F1, F2) and Sutherland's pow,
+whose device and host libraries may round differently in the last place, so
+values are compared to 1e-12 relative (1e-11 for sums with cancellation);
+the FD Jacobian to 1e-6 per block as in test_jacobian_gpu.py.
+"""
+import numpy as np
+import pytest
+
+from freecfd_amd import _abi as A
+from freecfd_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+MODELS = [A.RANS_KEPSILON, A.RANS_KOMEGA, A.RANS_BSL, A.RANS_SST]
+VARIANTS = sorted(cases.BC_VARIANTS)
+INTENSITY, RATIO = 0.05, 10.0
+
+
+def _rel(a, b, floor=0.0):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    scale = np.maximum(np.abs(b).max(), floor) if b.size else 1.0
+    return float(np.abs(a - b).max() / scale) if b.size else 0.0
+
+
+def _colrel(a, b):
+    """max over columns of max |a - b| / max |b| of that column"""
+    a, b = np.asarray(a, float).reshape(len(a), -1), np.asarray(b, float).reshape(len(b), -1)
+    out = 0.0
+    for j in range(b.shape[1]):
+        s = np.abs(b[:, j]).max()
+        if s > 0:
+            out = max(out, float(np.abs(a[:, j] - b[:, j]).max() / s))
+        else:
+            assert np.abs(a[:, j]).max() == 0.0
+    return out
+
+
+def _pair(cs):
+    """(GPU NavierStokes, oracle) at the case's NS state, both initialised"""
+    from freecfd_amd.ns import NavierStokes
+    from oracle.oracle import Oracle
+    print("case", cs.name, flush=True)
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    ns.rans_initialize(INTENSITY, RATIO)
+    o = Oracle(cs.raw, cs.opt)
+    o.residual(cs.W0)
+    o.kw_init(INTENSITY, RATIO)
+    return ns, o
+
+
+@pytest.mark.parametrize("model", MODELS)
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_kw_initialize_parity(variant, model):
+    cs = cases.kw_case(variant, model=model)
+    ns, o = _pair(cs)
+    g, r = ns.rans_fields(), o.kw_fields()
+    for b in range(cs.opt.n_bc):
+        if cs.opt.bc[b].type == A.BC_INLET:
+            want = o.kw_bc_values(b)
+            inl = np.where(ns.metrics()["face_bc"] == b)[0]
+            np.testing.assert_allclose(g["mutf"][inl], want[2], rtol=1e-13)
+    r["G"] = r["G"][:cs.n_cells]
+    for key in ("K", "mut", "G", "mutf"):
+        err = _colrel(g[key], r[key])
+        print("  %s: %.2e" % (key, err), flush=True)
+        assert err <= 1e-13, key
+
+
+def _dt(ns, cfl=5.0):
+    dt = ns.time_step_dev(A.DT_CFL_LOCAL, cfl)
+    return dt, ns.to_host(dt.reshape(1, -1), 1)[:, 0]
+
+
+@pytest.mark.parametrize("model", MODELS)
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_kw_terms_parity(variant, model):
+    cs = cases.kw_case(variant, model=model)
+    ns, o = _pair(cs)
+    ns.rans_set(cs.K0)
+    o.kw_set(cs.K0)
+    dt, dth = _dt(ns)
+    rhs_g, diag_g, jf_g = ns.rans_terms(dt)
+    rhs_o, diag_o, jf_o = o.kw_terms(dth)
+    e_jf = _colrel(jf_g, jf_o)
+    e_rhs = _colrel(rhs_g, rhs_o)
+    e_diag = _colrel(diag_g.reshape(-1, 4), diag_o.reshape(-1, 4))
+    e_strain = _rel(ns.rans_fields()["strain"], o.kw_fields()["strain"])
+    print("  jf %.2e rhs %.2e diag %.2e strain %.2e" % (e_jf, e_rhs, e_diag, e_strain), flush=True)
+    assert e_jf <= 1e-12 and e_diag <= 1e-12 and e_strain <= 1e-13
+    assert e_rhs <= 1e-11
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_ns_residual_and_jacobian_with_model_mu_t(variant):
+    """the NS face loop adds the model's mu_t.face to mu and Cp mu_t / Pr_t
+    to lambda (ns_diffusive_face_flux.cc:36-39), frozen in the Jacobian"""
+    cs = cases.kw_case(variant, model=A.RANS_SST)
+    ns, o = _pair(cs)
+    ns.rans_set(cs.K0)
+    o.kw_set(cs.K0)
+    Ro, Rabs = o.residual(cs.W0, with_jacobian=True)
+    ns.assemble_residual()
+    ns.status()
+    Rg = ns.residual_host()
+    err = float((np.abs(Rg - Ro) / (Rabs + 1e-300)).max())
+    print("  residual %.2e" % err, flush=True)
+    assert err <= 1e-12
+    # the mu_t really enters: a laminar evaluation differs
+    from oracle.oracle import Oracle
+    import copy
+    opt = copy.deepcopy(cs.opt)
+    opt.turbulence = A.TURB_NONE
+    Rlam, _ = Oracle(cs.raw, opt).residual(cs.W0)
+    assert float((np.abs(Rlam - Ro) / (Rabs + 1e-300)).max()) > 1e-6
+    ns.fd_jacobian()
+    JLg, JRg, Dg = ns.jacobian_host()
+    JLo, JRo, Do = o.jacobian()
+    for a, b in ((JLg, JLo), (JRg, JRo)):
+        keep = np.abs(b).reshape(len(b), -1).max(axis=1) > 0
+        scale = np.abs(b[keep]).reshape(keep.sum(), -1).max(axis=1)
+        e = float((np.abs(a[keep] - b[keep]).reshape(keep.sum(), -1).max(axis=1) / scale).max())
+        print("  FD Jacobian blocks %.2e" % e, flush=True)
+        assert e <= 1e-6
+
+
+def _direct_update(o, cs, dth):
+    """the oracle's system (rhs, diagonal blocks, face couplings) solved
+    directly, then the oracle's update_variables"""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    rhs, diag, jf = o.kw_terms(dth)
+    met = o.metrics()
+    n = cs.n_cells
+    rows, cols, vals = [], [], []
+    for c in range(n):
+        for i in range(2):
+            for j in range(2):
+                rows.append(2 * c + i); cols.append(2 * c + j); vals.append(diag[c, i, j])
+    P, Nb = met["face_parent"], met["face_neighbor"]
+    for f in np.where(met["face_bc"] < 0)[0]:
+        p, q = P[f], Nb[f]
+        for i in range(2):
+            rows.append(2 * p + i); cols.append(2 * q + i); vals.append(jf[f, 4 + i])
+            rows.append(2 * q + i); cols.append(2 * p + i); vals.append(-jf[f, 2 + i])
+    Amat = sp.csr_matrix((vals, (rows, cols)), shape=(2 * n, 2 * n))
+    dU = spla.spsolve(Amat.tocsc(), rhs.reshape(-1)).reshape(n, 2)
+    sums = o.kw_update(dU, dth)
+    return dU, sums
+
+
+@pytest.mark.parametrize("model", MODELS)
+@pytest.mark.parametrize("variant", VARIANTS[:4])
+def test_kw_solve_vs_direct_solve(variant, model):
+    cs = cases.kw_case(variant, model=model)
+    ns, o = _pair(cs)
+    ns.rans_set(cs.K0)
+    o.kw_set(cs.K0)
+    dt, dth = _dt(ns, cfl=20.0)
+    st = ns.rans_solve(dt, A.rans_opts(restart=60, max_iters=400, rtol=1e-14, abstol=1e-300))
+    print("  GMRES %d iterations, converged %d, %.2e -> %.2e" % (st.krylov_iters, st.krylov_converged, st.krylov_r0norm,
+                                                                 st.krylov_rnorm), flush=True)
+    _, sums = _direct_update(o, cs, dth)
+    g, r = ns.rans_fields(), o.kw_fields()
+    r["G"] = r["G"][:cs.n_cells]
+    for key in ("K", "mut", "G"):
+        err = _colrel(g[key], r[key])
+        print("  %s after the step: %.2e" % (key, err), flush=True)
+        assert err <= 1e-9, key
+    np.testing.assert_allclose(np.array(st.norms) ** 2, sums, rtol=1e-8)
+
+
+def test_kw_update_limits_on_the_gpu():
+    """a large time step on a state with tiny k and omega in some cells
+    drives them to their floors: the GPU update applies the reference's
+    limits as the oracle does"""
+    cs = cases.kw_case(VARIANTS[0], model=A.RANS_KOMEGA)
+    ns, o = _pair(cs)
+    K = cs.K0.copy()
+    K[::3, 1] *= 1e-3
+    K[1::3, 0] *= 1e-6
+    ns.rans_set(K)
+    o.kw_set(K)
+    dt, dth = _dt(ns, cfl=1e3)
+    st = ns.rans_solve(dt, A.rans_opts(restart=100, max_iters=1000, rtol=1e-14, abstol=1e-300))
+    print("  GMRES %d iterations, converged %d, %.2e -> %.2e" % (st.krylov_iters, st.krylov_converged, st.krylov_r0norm,
+                                                                 st.krylov_rnorm), flush=True)
+    assert st.krylov_converged
+    _direct_update(o, cs, dth)
+    g, r = ns.rans_fields(), o.kw_fields()
+    n = cs.n_cells
+    # k + max(-(k - k_low), dk) equals k_low up to the rounding of k - k_low
+    # (a few ulp of k), in the reference's arithmetic as in the oracle's
+    ulp = 4 * np.finfo(float).eps * np.abs(K).max(axis=0)
+    assert g["K"][:n, 0].min() >= cs.opt.k_low - ulp[0] and g["K"][:n, 1].min() >= cs.opt.omega_low - ulp[1]
+    at_floor = np.sum(np.abs(g["K"][:n, 0] - cs.opt.k_low) <= ulp[0]) + np.sum(np.abs(g["K"][:n, 1] - cs.opt.omega_low) <= ulp[1])
+    print("  cells at a floor: %d" % at_floor, flush=True)
+    assert at_floor > 0
+    assert _colrel(g["K"], r["K"]) <= 1e-9
+
+
+def test_newton_step_runs_the_rans_solve_before_the_ns_update():
+    """NavierStokes::solve calls RANS::solve between petsc_solve and
+    update_variables (ns.cc:106-110): the model's update inside
+    fcfd_newton_step equals a separate fcfd_rans_step at the pre-update
+    state with the step's time step, bit for bit"""
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.kw_case(VARIANTS[0], model=A.RANS_SST)
+    ro = A.rans_opts(restart=30, max_iters=20)
+    a = NavierStokes(cs.raw, cs.opt)
+    a.set_state(cs.W0)
+    a.rans_initialize(INTENSITY, RATIO, opts=ro)
+    a.rans_set(cs.K0)
+    b = NavierStokes(cs.raw, cs.opt)
+    b.set_state(cs.W0)
+    b.rans_initialize(INTENSITY, RATIO, opts=ro)
+    b.rans_set(cs.K0)
+    opts = A.newton_opts(cfl=5.0, max_iters=20)
+    st = a.newton_step(opts)
+    assert st.rans_iters > 0
+    dt = b.time_step_dev(A.DT_CFL_LOCAL, 5.0)
+    rs = b.rans_solve(dt, ro)
+    ka, kb = a.rans_fields()["K"], b.rans_fields()["K"]
+    assert np.array_equal(ka, kb)
+    assert rs.krylov_iters == st.rans_iters
+    assert list(rs.norms) == list(st.rans_norms)
+    # and the NS update happened: the state moved
+    assert not np.array_equal(a.state_host(False), cs.W0)
+
+
+def test_kw_plate_steps_stay_bounded_and_settle():
+    """a small turbulent plate with SST and AUSM+up from the freestream: 64
+    implicit NS steps, each with the model's solve inside, keep k and omega
+    finite and within their limits, mu_t positive, and the NS residual falls
+    from its start-up peak by more than a factor of 4"""
+    from freecfd_amd.ns import NavierStokes
+    cs = cases.plate_kw_case(24, 16, 2, model=A.RANS_SST)
+    cs.opt.convective_flux = A.FLUX_AUSM_PLUS_UP
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    ns.rans_initialize(cs.meta["intensity"], cs.meta["visc_ratio"], opts=A.rans_opts(max_iters=30))
+    opts = A.newton_opts(cfl=50.0, max_iters=30, restart=30)
+    norms, rans_iters = [], []
+    for k in range(64):
+        st = ns.newton_step(opts)
+        norms.append(st.res_norm)
+        rans_iters.append(st.rans_iters)
+    f = ns.rans_fields()
+    n = cs.n_cells
+    print("  residual %.3e, peak %.3e -> %.3e; RANS iterations per step %d..%d" % (norms[0], max(norms), norms[-1],
+                                                                                  min(rans_iters), max(rans_iters)), flush=True)
+    assert min(rans_iters) > 0
+    assert np.all(np.isfinite(f["K"])) and np.all(np.isfinite(f["mut"]))
+    assert f["K"][:n, 0].min() >= cs.opt.k_low and f["K"][:n, 1].min() >= cs.opt.omega_low and f["mut"][:n].min() > 0
+    assert norms[-1] < 0.25 * max(norms)
