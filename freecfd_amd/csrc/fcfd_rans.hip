@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(256) kw_grad_kernel(DevGrid g, KwDev k) {
 // JR (2 x 2 blocks, row pairs): -diag(jacL), -diag(jacR), so that the NS
 // operator convention gives row P, column N = jacR and row N, column P =
 // -jacL.  lam_last: the laminar viscosity of the reference's last face
-// (the cell loop reads the face loop's last lam_visc, rans_terms.cc:195).
+// (the cell loop reads the face loop's last lam_visc, rans_terms.cc:183-185).
 // ---------------------------------------------------------------------------
 template <int FLUX>
 __global__ void __launch_bounds__(64) kw_face_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ G,
