@@ -275,3 +275,17 @@ def test_shipped_example_mesh(name):
     assert np.bincount(fb[fb >= 0], minlength=len(names)).tolist() == want
     assert (m["cell_volume"][:g.info.n_cells] > 0).all()
     g.close()
+
+
+def test_wedge_example_fixture():
+    """tests/golden/wedge_example_mesh.npz (the mesh of the shipped wedge
+    example for its GPU test) has the example's sizes and BC names, and,
+    where the reference tree is present, equals a fresh read of wedge.cgns"""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "wedge_example_mesh.npz")
+    d = np.load(path, allow_pickle=False)
+    assert len(d["xyz"]) // 3 == 24442 and len(d["cell_ptr"]) - 1 == 12000 and int(d["dimension"]) == 2
+    assert [str(x) for x in d["bc_names"]] == ["inlet", "outlet", "symmetry", "wall"]
+    if os.path.isdir(EX):
+        raw = CG.read_cgns(os.path.join(EX, "wedge/wedge.cgns"), dimension=2)
+        for k in ("xyz", "cell_ptr", "cell_nodes", "bc_ptr", "bc_nodes"):
+            assert np.array_equal(np.asarray(raw[k]), d[k]), k
