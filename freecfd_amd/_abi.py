@@ -84,6 +84,10 @@ class NewtonStats(C.Structure):
                 ("rans_iters", C.c_int), ("rans_norms", C.c_double * 2)]
 
 
+# the BC values of fcfd_vjp_params, in its order
+BC_PARAMS = ("p", "T", "rho", "Vx", "Vy", "Vz", "p_total", "T_total", "mdot", "qdot", "nut")
+
+
 class RansOpts(C.Structure):
     _fields_ = [("restart", C.c_int), ("max_iters", C.c_int), ("rtol", C.c_double), ("abstol", C.c_double)]
 

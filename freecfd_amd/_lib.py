@@ -48,6 +48,8 @@ def lib():
         "fcfd_set_bc": (C.c_int, [_vp, C.c_int, C.POINTER(A.BC)]),
         "fcfd_surface": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "fcfd_pseudo_step": (C.c_int, [_vp, _vp, C.POINTER(A.NewtonOpts), C.POINTER(A.PseudoOpts), C.POINTER(A.NewtonStats), _vp]),
+        "fcfd_bc_tangent": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(A.BC), _vp, _vp, _vp]),
+        "fcfd_vjp_params": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "fcfd_rans_init": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.POINTER(A.RansOpts), _vp]),
         "fcfd_rans_set": (C.c_int, [_vp, _vp, _vp, _vp]),
         "fcfd_rans_get": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),

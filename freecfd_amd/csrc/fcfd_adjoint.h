@@ -38,4 +38,10 @@ void launch_vjp(const DevGrid& g, const AdjGrid& a, const Params& P, double* W, 
 // scratch.  G/PHI must hold W's primal gradients and limiters.
 void launch_loads(const DevGrid& g, const AdjGrid& a, const Params& P, const double* W, const double* G, const double* PHI,
                   double mcx, double mcy, double mcz, double* tmp, int nlf, double* out, int* status, hipStream_t s);
+// the loads' tangent from the faces' d(D - C) (the same ordered sums)
+void launch_loads_tangent(const DevGrid& g, const AdjGrid& a, const double* dFX, double mcx, double mcy, double mcz, double* tmp,
+                          int nlf, double* out, hipStream_t s);
+// out[0] = sum over boundary faces of sum_{v<5} w[5 bc + v] dFX[v][f]
+// (part: 256 doubles)
+void launch_bface_dot(const DevGrid& g, const double* w, const double* dFX, double* part, double* out, hipStream_t s);
 }  // namespace fcfd

@@ -465,6 +465,68 @@ void launch_vjp(const DevGrid& g, const AdjGrid& a, const Params& P, double* W, 
     else AdjLaunch<5>::vjp(g, a, P, W, G, PHI, dref, Rbar, fxw, w, Wbar, status, s);
 }
 
+// the loads' tangent from the faces' d(D - C) (dFX [nv][F]): per load face
+// dt = -dFX[1..3], dm = (x_f - x_m) x dt, then the same ordered per-BC sums
+__global__ void __launch_bounds__(256) loads_tangent_face_kernel(DevGrid g, AdjGrid a, const double* __restrict__ dFX, double mcx,
+                                                                 double mcy, double mcz, double* __restrict__ tmp, int nlf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlf) return;
+    const int f = a.lf_face[i];
+    const size_t Fs = g.F;
+    Vec<double> t = mk<double>(-dFX[Fs + f], -dFX[2 * Fs + f], -dFX[3 * Fs + f]);
+    const Vec<double> xf = face_centroid(g, f);
+    Vec<double> r = mk<double>(xf.x - mcx, xf.y - mcy, xf.z - mcz);
+    Vec<double> mm = vcross(r, t);
+    const size_t s = nlf;
+    tmp[i] = t.x; tmp[s + i] = t.y; tmp[2 * s + i] = t.z;
+    tmp[3 * s + i] = mm.x; tmp[4 * s + i] = mm.y; tmp[5 * s + i] = mm.z;
+}
+
+void launch_loads_tangent(const DevGrid& g, const AdjGrid& a, const double* dFX, double mcx, double mcy, double mcz, double* tmp,
+                          int nlf, double* out, hipStream_t s) {
+    if (nlf > 0) hipLaunchKernelGGL(loads_tangent_face_kernel, dim3((nlf + 255) / 256), dim3(256), 0, s, g, a, dFX, mcx, mcy, mcz, tmp, nlf);
+    if (a.n_bc > 0) hipLaunchKernelGGL(loads_sum_kernel, dim3((6 * a.n_bc + 63) / 64), dim3(64), 0, s, a, tmp, nlf, out);
+}
+
+// sum over boundary faces of sum_v w[5 bc + v] dFX[v][f] (v < 5): the
+// boundary-flux part of an output's tangent; partials per block, then a
+// fixed-order tree (part: 256 doubles, out: 1)
+constexpr int kBfBlocks = 256;
+__global__ void __launch_bounds__(256) bface_dot_kernel(DevGrid g, const double* __restrict__ w, const double* __restrict__ dFX,
+                                                        double* __restrict__ part) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    const size_t Fs = g.F;
+    for (int f = blockIdx.x * 256 + threadIdx.x; f < g.F; f += kBfBlocks * 256) {
+        const int b = g.fcell[f].z;
+        if (b < 0) continue;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) acc += w[5 * b + v] * dFX[v * Fs + f];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int q = 128; q > 0; q >>= 1) {
+        if (threadIdx.x < q) sh[threadIdx.x] += sh[threadIdx.x + q];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void __launch_bounds__(256) bface_dot_final_kernel(const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double sh[256];
+    sh[threadIdx.x] = part[threadIdx.x];
+    __syncthreads();
+    for (int q = 128; q > 0; q >>= 1) {
+        if (threadIdx.x < q) sh[threadIdx.x] += sh[threadIdx.x + q];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+void launch_bface_dot(const DevGrid& g, const double* w, const double* dFX, double* part, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(bface_dot_kernel, dim3(kBfBlocks), dim3(256), 0, s, g, w, dFX, part);
+    hipLaunchKernelGGL(bface_dot_final_kernel, dim3(1), dim3(256), 0, s, part, out);
+}
+
 void launch_loads(const DevGrid& g, const AdjGrid& a, const Params& P, const double* W, const double* G, const double* PHI,
                   double mcx, double mcy, double mcz, double* tmp, int nlf, double* out, int* status, hipStream_t s) {
     if (g.nv == 6) AdjLaunch<6>::loads(g, a, P, W, G, PHI, mcx, mcy, mcz, tmp, nlf, status, s);

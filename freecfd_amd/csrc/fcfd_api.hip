@@ -194,6 +194,8 @@ void set_params(fcfd_ctx* c) {
     P.nv = (o.turbulence == FCFD_TURB_SA) ? 6 : 5;
     P.Pr_t = o.Pr_t;
     P.kw = o.turbulence == FCFD_TURB_KW;
+    P.seed = BcSeed{};
+    P.seed.index = -1;
     P.rans_model = o.rans_model;
     P.k_low = o.k_low; P.k_high = o.k_high; P.omega_low = o.omega_low; P.visc_ratio_limit = o.visc_ratio_limit;
     double nut_scale = 0.0;
@@ -1047,6 +1049,82 @@ int fcfd_surface(fcfd_ctx* c, double* W_dev, double* mdot_host, double* qdot_hos
         if (tau_host)
             for (int k = 0; k < 3; ++k) tau_host[3 * (size_t)f + k] = h[(size_t)(2 + k) * F + i];
     }
+    return FCFD_OK;
+}
+
+// the values of one BC as a seed direction (the fields of fcfd_bc that enter
+// the residual: p, T, rho, V, p_total, T_total, mdot, qdot, nut)
+static BcSeed make_seed(int index, const fcfd_bc& d) {
+    BcSeed q{};
+    q.index = index;
+    q.p = d.p; q.T = d.T; q.rho = d.rho; q.V[0] = d.V[0]; q.V[1] = d.V[1]; q.V[2] = d.V[2];
+    q.p_total = d.p_total; q.T_total = d.T_total; q.mdot = d.mdot; q.qdot = d.qdot; q.nut = d.nut;
+    return q;
+}
+
+static int param_tangent_prepare(fcfd_ctx* c) {
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    if (c->dg.ggc) return fail(c, FCFD_ENOTSUP, "BC-value derivatives do not support Green-Gauss gradients");
+    int rc = ensure_tangent(c);
+    if (!rc) rc = ensure_adjoint(c);   // the load faces and the reduction buffers
+    return rc;
+}
+
+// dR, dFX at fixed W in the direction of BC `index`'s values
+static int param_tangent(fcfd_ctx* c, double* W_dev, int index, const fcfd_bc& d, double* dR_dev, hipStream_t s) {
+    Params P = c->P;
+    P.seed = make_seed(index, d);
+    HIPCHK(c, hipMemsetAsync(c->jv_in, 0, sizeof(double) * (size_t)c->dg.nv * c->dg.ld, s));
+    launch_param_tangent(c->dg, P, W_dev, c->jv_in, c->G, c->dG, c->PHI, c->dPHI, c->dref, c->dFX, dR_dev, c->status, s);
+    HIPCHK(c, hipGetLastError());
+    return FCFD_OK;
+}
+
+int fcfd_bc_tangent(fcfd_ctx* c, double* W_dev, int index, const fcfd_bc* dbc, double* dR_dev, double* dloads_host, void* stream) {
+    if (!c || !W_dev || !dbc || !dR_dev) return FCFD_EINVAL;
+    if (index < 0 || index >= c->opt.n_bc) return fail(c, FCFD_EINVAL, "BC index out of range");
+    int rc = param_tangent_prepare(c);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    launch_ghosts(c->dg, c->P, W_dev, c->status, s);   // the primal ghosts of W
+    if ((rc = param_tangent(c, W_dev, index, *dbc, dR_dev, s))) return rc;
+    if (dloads_host) {
+        launch_loads_tangent(c->dg, c->ag, c->dFX, c->opt.moment_center[0], c->opt.moment_center[1], c->opt.moment_center[2],
+                             c->loads_tmp, c->n_loadfaces, c->loads_out, s);
+        HIPCHK(c, hipGetLastError());
+        if (c->opt.n_bc > 0)
+            HIPCHK(c, hipMemcpyAsync(dloads_host, c->loads_out, sizeof(double) * 6 * c->opt.n_bc, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+    }
+    return FCFD_OK;
+}
+
+int fcfd_vjp_params(fcfd_ctx* c, double* W_dev, const double* Rbar_dev, const double* fx_weight_dev, double* pbar_host, void* stream) {
+    if (!c || !W_dev || !Rbar_dev || !pbar_host) return FCFD_EINVAL;
+    int rc = param_tangent_prepare(c);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    launch_ghosts(c->dg, c->P, W_dev, c->status, s);
+    const DevGrid& g = c->dg;
+    const size_t n5 = (size_t)g.nv * g.ld;
+    if (!c->red_part && ((rc = dalloc(c, (size_t)512 * 8, &c->red_part)) || (rc = dalloc(c, 64, &c->red_out)))) return rc;
+    // the 11 values of each BC, one unit direction each
+    for (int b = 0; b < c->opt.n_bc; ++b)
+        for (int q = 0; q < FCFD_BC_NPARAMS; ++q) {
+            fcfd_bc d{};
+            double* slot[FCFD_BC_NPARAMS] = {&d.p, &d.T, &d.rho, &d.V[0], &d.V[1], &d.V[2], &d.p_total, &d.T_total, &d.mdot, &d.qdot, &d.nut};
+            *slot[q] = 1.0;
+            if ((rc = param_tangent(c, W_dev, b, d, c->jv_out, s))) return rc;
+            double dot = 0.0, bf = 0.0;
+            HIPCHK(c, dev_mdot(g, 1, c->jv_out, n5, Rbar_dev, c->red_part, c->red_out, &dot, s));
+            if (fx_weight_dev) {
+                launch_bface_dot(g, fx_weight_dev, c->dFX, c->red_part, c->red_out, s);
+                HIPCHK(c, hipGetLastError());
+                HIPCHK(c, hipMemcpyAsync(&bf, c->red_out, sizeof(double), hipMemcpyDeviceToHost, s));
+                HIPCHK(c, hipStreamSynchronize(s));
+            }
+            pbar_host[(size_t)FCFD_BC_NPARAMS * b + q] = dot + bf;
+        }
     return FCFD_OK;
 }
 

@@ -192,8 +192,9 @@ __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_,
 // Stage 2: finish the states, apply the boundary condition (right state of
 // a boundary face = a copy of the finished left state with the ghost's
 // centre values, then apply_bcs; right_state_update :298-303) and evaluate
-// the convective and diffusive fluxes.
-template <int FLUX, int VISC, int NV, typename S>
+// the convective and diffusive fluxes.  PSEED (dual S only): the values of
+// BC P.seed.index carry the parameter tangent's direction.
+template <int FLUX, int VISC, int NV, typename S, bool PSEED = false>
 __device__ __forceinline__ void face_tail(const DevGrid& g, const Params& P, int f, FaceCtx<S, NV>& X, int* status) {
     FaceGeo& F = X.F;
     finish_state(P, X.L, F);
@@ -205,7 +206,14 @@ __device__ __forceinline__ void face_tail(const DevGrid& g, const Params& P, int
             Rb.V_center = X.R.V_center;
             Rb.T_center = X.R.T_center;
             Rb.nut_center = X.R.nut_center;
-            if (apply_bcs(P, g.bc[F.bc], X.L, Rb, F)) atomicOr(status, 1);
+            int flag;
+            if constexpr (PSEED) {
+                if (F.bc == P.seed.index) flag = apply_bcs(P, bc_val<S>(g.bc[F.bc], P.seed), X.L, Rb, F);
+                else flag = apply_bcs(P, g.bc[F.bc], X.L, Rb, F);
+            } else {
+                flag = apply_bcs(P, g.bc[F.bc], X.L, Rb, F);
+            }
+            if (flag) atomicOr(status, 1);
         } else {
             Rb = X.R;
         }
@@ -217,16 +225,20 @@ __device__ __forceinline__ void face_tail(const DevGrid& g, const Params& P, int
         face_state<NV>(P, X.L, X.R, F, X.fs);
         if constexpr (VISC == kViscRans) X.fs.mu_t = S(X.mut_face);
         diffusive_flux<NV, VISC == kViscRans>(P, X.fs, F, g.bc, X.D);
+        if constexpr (PSEED) {   // a FIXED_Q wall's heat flux q A (ns_diffusive_face_flux.cc:58-60)
+            if (F.bc >= 0 && F.bc == P.seed.index && g.bc[F.bc].thermal == FCFD_THERMAL_FIXED_Q)
+                X.D[4] = X.D[4] + S(0., P.seed.qdot * F.area);
+        }
     } else {
 #pragma unroll
         for (int v = 0; v < NV; ++v) X.D[v] = S(0.0);
     }
 }
 
-template <int FLUX, int VISC, int NV, typename S, typename Loader>
+template <int FLUX, int VISC, int NV, typename S, typename Loader, bool PSEED = false>
 __device__ __forceinline__ void face_base(const DevGrid& g, const Params& P, const Loader& ld_, int f, FaceCtx<S, NV>& X, int* status) {
     face_inputs<VISC, NV>(g, ld_, P, f, X);
-    face_tail<FLUX, VISC, NV>(g, P, f, X, status);
+    face_tail<FLUX, VISC, NV, S, PSEED>(g, P, f, X, status);
 }
 
 }  // namespace fcfd

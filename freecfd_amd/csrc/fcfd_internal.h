@@ -46,6 +46,13 @@ int build_host_grid(const fcfd_mesh* m, const fcfd_options* o, HostGrid& g, std:
 constexpr int kOrderRcm = 0, kOrderSfc = 1;
 void renumber(HostGrid& g, int cell_order);
 
+// A direction in the values of one BC (index; -1: none), for the exact
+// derivatives of the residual and the loads with respect to BC values.
+struct BcSeed {
+    int index;
+    double p, T, rho, V[3], p_total, T_total, mdot, qdot, nut;
+};
+
 // Material/numerics constants passed by value to kernels.
 struct Params {
     double gamma, R, Cp, Pref, Tref;
@@ -62,6 +69,7 @@ struct Params {
     // loop takes mu_t from the model; model constants and limits
     int kw, rans_model;
     double k_low, k_high, omega_low, visc_ratio_limit;
+    BcSeed seed;     // the parameter tangent's direction (index -1 elsewhere)
 };
 
 // the face kernels' VISC template argument: 0 inviscid, 1 viscous (laminar or

@@ -151,12 +151,32 @@ template <int NV, typename S> __device__ __forceinline__ void recon(const Params
     }
 }
 
+// A BC's values as the scalar type S: the parameter tangent seeds them
+// (BcSeed), so that the derivatives of the residual with respect to the BC
+// values follow from the same apply_bcs.  The kinds, flags and the BC area
+// are carried as they are.
+template <typename S> struct BcVal {
+    int type, kind, specified, thermal;
+    S p, T, rho, V[3], p_total, T_total, mdot, nut;
+    double total_area;
+};
+template <typename S> __device__ __forceinline__ BcVal<S> bc_val(const BcDev& b, const BcSeed& d) {
+    BcVal<S> r;
+    r.type = b.type; r.kind = b.kind; r.specified = b.specified; r.thermal = b.thermal;
+    r.p = S(b.p, d.p); r.T = S(b.T, d.T); r.rho = S(b.rho, d.rho);
+    r.V[0] = S(b.V[0], d.V[0]); r.V[1] = S(b.V[1], d.V[1]); r.V[2] = S(b.V[2], d.V[2]);
+    r.p_total = S(b.p_total, d.p_total); r.T_total = S(b.T_total, d.T_total); r.mdot = S(b.mdot, d.mdot); r.nut = S(b.nut, d.nut);
+    r.total_area = b.total_area;
+    return r;
+}
+
 // Boundary conditions (src/navier_stokes/ns_apply_bcs.cc:25-258).  Returns a
 // nonzero flag where the reference would exit (supersonic velocity inlet
 // without a complete state).  The right state is carried in SSA locals and
 // written back whole, so that no branch stores to an individual field of the
 // state struct (divergent field stores would pin the struct in scratch).
-template <typename S> __device__ __forceinline__ int apply_bcs(const Params& P, const BcDev& bc, const CState<S>& L,
+// B: BcDev (the BC's values as constants) or BcVal<S> (seeded values).
+template <typename S, typename B> __device__ __forceinline__ int apply_bcs(const Params& P, const B& bc, const CState<S>& L,
                                                       CState<S>& Rout, const FaceGeo& F) {
     const double g = P.gamma, Pref = P.Pref, Tref = P.Tref;
     int flag = 0;
@@ -179,7 +199,7 @@ template <typename S> __device__ __forceinline__ int apply_bcs(const Params& P, 
             pc = p; Vc = V; Tc = T;
         } else if (bc.kind == FCFD_KIND_MDOT) {
             S uNL = vdot(L.V, F.n);
-            double mdotNR = -1. * bc.mdot / bc.total_area;
+            const auto mdotNR = -1. * bc.mdot / bc.total_area;
             // the reference's test `!specified != BC_STATE` (ns_apply_bcs.cc:99-104)
             // is true for every specification: a supersonic mass-flow inlet always stops
             if (val(-uNL / L.a) >= 1.) flag = 1;

@@ -33,4 +33,9 @@ void launch_pseudo_op(const DevGrid& g, const Params& P, const double* W, const 
 // y = (P V / dt)^T x - Jt   (Jt = (dR/dW)^T x)
 void launch_newton_op_t(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* Jt,
                         double* y, hipStream_t s);
+// the parameter tangent: dR (and the faces' d(D - C) in dFX) at fixed W in
+// the direction P.seed of one BC's values; dW [nv][ld] must be 0 on interior
+// cells (its ghost rows, G, dG, PHI, dPHI are written)
+void launch_param_tangent(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
+                          double* dref, double* dFX, double* dR, int* status, hipStream_t s);
 }

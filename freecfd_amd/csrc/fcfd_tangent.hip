@@ -33,7 +33,8 @@ struct TangentNb {
     __device__ __forceinline__ DualT<TD> nb(int, int j, int v) const { return DualT<TD>(W[v * ld + j], dW[v * ld + j]); }
 };
 
-template <int NV, typename TD>
+// PSEED: the values of BC P.seed.index carry the parameter tangent's direction
+template <int NV, typename TD, bool PSEED = false>
 __global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, double* __restrict__ W, TD* __restrict__ dW,
                                                       int* __restrict__ status) {
     using Dual = DualT<TD>;
@@ -52,7 +53,14 @@ __global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, doubl
     L.rho = m_rho(P, L.p, L.T);
     finish_state(P, L, F);
     CState<Dual> R = L;
-    if (apply_bcs(P, g.bc[F.bc], L, R, F)) atomicOr(status, 1);
+    int flag;
+    if constexpr (PSEED) {
+        if (F.bc == P.seed.index) flag = apply_bcs(P, bc_val<Dual>(g.bc[F.bc], P.seed), L, R, F);
+        else flag = apply_bcs(P, g.bc[F.bc], L, R, F);
+    } else {
+        flag = apply_bcs(P, g.bc[F.bc], L, R, F);
+    }
+    if (flag) atomicOr(status, 1);
     int gi = g.N + i;
     W[gi] = R.p_center.v;          dW[gi] = R.p_center.d;
     W[ld + gi] = R.V_center.x.v;   dW[ld + gi] = R.V_center.x.d;
@@ -144,13 +152,13 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
 }
 
 // one wavefront per workgroup, as the primal face kernel (3% faster here)
-template <int FLUX, int VISC, bool FULL, int NV, typename TD>
+template <int FLUX, int VISC, bool FULL, int NV, typename TD, bool PSEED = false>
 __global__ void __launch_bounds__(64) face_flux_t_kernel(DevGrid g, Params P, DualLoader<NV, FULL, TD> L, TD* __restrict__ dFX,
                                                           int* __restrict__ status) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
     FaceCtx<DualT<TD>, NV> X;
-    face_base<FLUX, VISC, NV>(g, P, L, f, X, status);
+    face_base<FLUX, VISC, NV, DualT<TD>, DualLoader<NV, FULL, TD>, PSEED>(g, P, L, f, X, status);
     const size_t Fs = g.F;
 #pragma unroll
     for (int v = 0; v < NV; ++v) dFX[v * Fs + f] = (X.D[v] - X.C[v]).d;
@@ -498,6 +506,63 @@ void launch_pseudo_op(const DevGrid& g, const Params& P, const double* W, const 
 void launch_pseudo_op(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, const float* x,
                       float* y, hipStream_t s) {
     launch_pseudo_op_v<float>(g, P, W, dtau, drdp, x, y, s);
+}
+
+
+// ---------------------------------------------------------------------------
+// the parameter tangent: dR and dFX at fixed W in the direction P.seed of the
+// values of one BC (every path from a BC value: ghost centres, then
+// gradients and limiters, the boundary faces' right states, a FIXED_Q heat
+// flux, the SA inlet nu~).  dW must be 0 on interior cells.
+// ---------------------------------------------------------------------------
+template <int FLUX, int NV>
+static void launch_fp(const DevGrid& g, const Params& P, const DualLoader<NV, true, double>& L, double* dFX, int* status, hipStream_t s) {
+    const dim3 grid((g.F + 63) / 64), blk(64);
+    if constexpr (NV == 6)
+        hipLaunchKernelGGL((face_flux_t_kernel<FLUX, 1, true, 6, double, true>), grid, blk, 0, s, g, P, L, dFX, status);
+    else if (P.kw)
+        hipLaunchKernelGGL((face_flux_t_kernel<FLUX, kViscRans, true, 5, double, true>), grid, blk, 0, s, g, P, L, dFX, status);
+    else if (P.viscous)
+        hipLaunchKernelGGL((face_flux_t_kernel<FLUX, 1, true, 5, double, true>), grid, blk, 0, s, g, P, L, dFX, status);
+    else
+        hipLaunchKernelGGL((face_flux_t_kernel<FLUX, 0, true, 5, double, true>), grid, blk, 0, s, g, P, L, dFX, status);
+}
+
+template <int NV>
+static void launch_param_t(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
+                           double* dref, double* dFX, double* dR, int* status, hipStream_t s) {
+    const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    const bool has_phi = lim != FCFD_LIMITER_NONE;
+    const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
+    CellLoader<NV> cl{W, G, PHI, has_phi, phi_const, (size_t)g.ld};
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.B > 0) hipLaunchKernelGGL((ghost_t_kernel<NV, double, true>), dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
+    if (lim == FCFD_LIMITER_VK) {
+        launch_vk_range(g, P, W, dref, s);
+        hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_VK, NV, double>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+    } else if (lim == FCFD_LIMITER_BJ) {
+        hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_BJ, NV, double>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+    } else if (lim == FCFD_LIMITER_MINMOD) {
+        hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_MINMOD, NV, double>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+    } else {
+        hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_NONE, NV, double>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+    }
+    DualLoader<NV, true, double> L{cl, dW, dG, dPHI, g.N};
+    switch (P.flux) {
+    case FCFD_FLUX_ROE: launch_fp<FCFD_FLUX_ROE, NV>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_VAN_LEER: launch_fp<FCFD_FLUX_VAN_LEER, NV>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_SD_SLAU: launch_fp<FCFD_FLUX_SD_SLAU, NV>(g, P, L, dFX, status, s); break;
+    case FCFD_FLUX_SW: launch_fp<FCFD_FLUX_SW, NV>(g, P, L, dFX, status, s); break;
+    default: launch_fp<FCFD_FLUX_AUSM_PLUS_UP, NV>(g, P, L, dFX, status, s); break;
+    }
+    launch_cell_sum(g, dFX, dR, s);
+    if constexpr (NV == 6) hipLaunchKernelGGL((sa_source_t_kernel<true, double>), grid, blk, 0, s, g, P, W, dW, G, dG, dR);
+}
+
+void launch_param_tangent(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
+                          double* dref, double* dFX, double* dR, int* status, hipStream_t s) {
+    if (g.nv == 6) launch_param_t<6>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, status, s);
+    else launch_param_t<5>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, status, s);
 }
 
 }  // namespace fcfd

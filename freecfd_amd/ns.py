@@ -173,6 +173,41 @@ class NavierStokes(Grid):
                                       out.data_ptr(), C.c_void_p(s)))
         return out
 
+    def bc_tangent(self, index, W=None, out=None, **direction):
+        """Exact derivative at the fixed state W of the residual and of the
+        loads with respect to the values of BC `index`, in the direction
+        given by keywords (p, T, rho, V, p_total, T_total, mdot, qdot, nut):
+        (dR [nv][ld] device tensor, dloads (n_bc, 6))."""
+        W = self.W if W is None else W
+        out = self.torch.zeros_like(self.W) if out is None else out
+        d = A.BC()
+        for k, v in direction.items():
+            if k == "V":
+                for i in range(3):
+                    d.V[i] = float(v[i])
+            else:
+                setattr(d, k, float(v))
+        n_bc = max(int(self._opt.n_bc), 1)
+        dl = np.zeros(6 * n_bc)
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_bc_tangent(self.h, W.data_ptr(), int(index), C.byref(d), out.data_ptr(), dl.ctypes.data,
+                                             C.c_void_p(s)))
+        return out, dl.reshape(n_bc, 6)[: int(self._opt.n_bc)]
+
+    def vjp_params(self, Rbar, fx_weight=None, W=None):
+        """pbar (n_bc, 11): <Rbar, dR/dv> + d/dv sum_f fx_weight[bc(f)] . (D - C)_f
+        for every BC value v (A.BC_PARAMS order), exact (fcfd_vjp_params)."""
+        W = self.W if W is None else W
+        n_bc = int(self._opt.n_bc)
+        out = np.zeros(len(A.BC_PARAMS) * max(n_bc, 1))
+        fw = None
+        if fx_weight is not None:
+            fw = self.torch.as_tensor(np.asarray(fx_weight, dtype=np.float64).reshape(-1), device=self.W.device).contiguous()
+        s = self.torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_vjp_params(self.h, W.data_ptr(), Rbar.data_ptr(), None if fw is None else fw.data_ptr(),
+                                             out.ctypes.data, C.c_void_p(s)))
+        return out.reshape(max(n_bc, 1), len(A.BC_PARAMS))[:n_bc]
+
     def loads(self, W=None):
         """(n_bc, 6) surface loads (Fx Fy Fz Mx My Mz) of the state, reference
         face-order sums over the BCs with include_in_loads."""

@@ -13,11 +13,12 @@ coefficient form,
 
 with F_x the x force on the wall BC from the face loop (loads.h:33-39,
 ns_assemble_linear_system.cc:109-119).  dJ/dW comes from the exact adjoint
-kernels (fcfd_vjp with a load weight), lam from fcfd_adjoint_step.  The two
-partials at fixed W are central differences in the parameter of the GPU
-residual and of the GPU loads: R and J depend on a parameter only through BC
-values (and q_inf), so these differences are of smooth functions, with an
-error of order h^2 plus rounding / h.
+kernels (fcfd_vjp with a load weight), lam from fcfd_adjoint_step.  R and
+J depend on a parameter only through BC values (and q_inf); the two partials
+at fixed W are exact, from the tangent kernels with the BC values as dual
+numbers (fcfd_bc_tangent: dR/dalpha and dF_x/dalpha through the ghosts,
+gradients, limiters and boundary fluxes), plus the closed form of
+dq_inf/dalpha.
 
 Parameters are inlet BC values: the Mach number (inlet velocity = M a_inf
 along x at fixed T_inf, so q_inf moves with it) and the inlet nu~ of the SA
@@ -44,6 +45,13 @@ class InletMach:
         u = value * self.a_inf
         return 0.5 * self.rho_inf * u * u
 
+    def direction(self):
+        """(BC index, d(BC values)/d alpha)"""
+        return self.inlet, {"V": (self.a_inf, 0.0, 0.0)}
+
+    def dq_inf(self, value, base_q):
+        return self.rho_inf * self.a_inf * self.a_inf * value
+
 
 class InletNut:
     """alpha = nu~ at the inlet (SA model)."""
@@ -58,6 +66,12 @@ class InletNut:
 
     def q_inf(self, value, base_q):
         return base_q
+
+    def direction(self):
+        return self.inlet, {"nut": 1.0}
+
+    def dq_inf(self, value, base_q):
+        return 0.0
 
 
 def wall_area(ns, wall):
@@ -76,12 +90,25 @@ def drag_weight(n_bc, wall, q_inf, area):
     return w
 
 
-def adjoint_sensitivities(ns, wall, params, values, q_inf, rel_step=1e-6, adjoint_opts=None, adjoint_steps=200,
+def partials(ns, wall, prm, val, q_inf, area, lam=None):
+    """The exact partials at the fixed state ns.W: (lam . dR/dalpha or the
+    dR/dalpha tensor when lam is None, dC_D/dalpha|_W)"""
+    index, d = prm.direction()
+    dR, dl = ns.bc_tangent(index, **d)
+    n = ns.info.n_cells
+    J = drag_coefficient(ns, wall, q_inf, area)
+    explicit = float(dl[wall, 0]) / (q_inf * area) - J * prm.dq_inf(val, q_inf) / q_inf
+    if lam is None:
+        return dR, explicit
+    return float(ns.torch.sum(lam[:, :n] * dR[:, :n])), explicit
+
+
+def adjoint_sensitivities(ns, wall, params, values, q_inf, adjoint_opts=None, adjoint_steps=200,
                           adjoint_drop=1e-11, cfl_initial=10.0, cfl_growth=2.0, cfl_target=1e6):
-    """d C_D / d alpha for each parameter (objects with apply/q_inf, current
-    values `values`) at the converged state held in ns.W.  Returns a dict with
-    C_D, the adjoint solve's history and, per parameter, the total
-    derivative and its two parts."""
+    """d C_D / d alpha for each parameter (objects with apply/q_inf/
+    direction/dq_inf, current values `values`) at the converged state held
+    in ns.W.  Returns a dict with C_D, the adjoint solve's history and, per
+    parameter, the total derivative and its two parts."""
     torch = ns.torch
     area = wall_area(ns, wall)
     n_bc = int(ns._opt.n_bc)
@@ -93,22 +120,9 @@ def adjoint_sensitivities(ns, wall, params, values, q_inf, rel_step=1e-6, adjoin
     lam, hist = ns.adjoint_solve(g, opts, max_steps=adjoint_steps, drop=adjoint_drop, cfl_initial=cfl_initial,
                                  cfl_growth=cfl_growth, cfl_target=cfl_target)
     out = {"C_D": J0, "area": area, "adjoint": hist, "params": {}}
-    Rp = torch.zeros_like(ns.W)
-    Rm = torch.zeros_like(ns.W)
     for prm, val in zip(params, values):
-        h = rel_step * abs(val) if val != 0.0 else rel_step
-        prm.apply(ns, val + h)
-        ns.assemble_residual(R=Rp)
-        Jp = drag_coefficient(ns, wall, prm.q_inf(val + h, q_inf), area)
-        prm.apply(ns, val - h)
-        ns.assemble_residual(R=Rm)
-        Jm = drag_coefficient(ns, wall, prm.q_inf(val - h, q_inf), area)
-        prm.apply(ns, val)
-        dRda = (Rp - Rm) / (2.0 * h)
-        dJda = (Jp - Jm) / (2.0 * h)
-        lam_term = float(torch.sum(lam * dRda))
-        out["params"][prm.name] = {"value": val, "total": dJda + lam_term, "explicit": dJda, "adjoint_term": lam_term,
-                                   "step": h}
+        lam_term, dJda = partials(ns, wall, prm, val, q_inf, area, lam)
+        out["params"][prm.name] = {"value": val, "total": dJda + lam_term, "explicit": dJda, "adjoint_term": lam_term}
     ns.assemble_residual()
     return out
 

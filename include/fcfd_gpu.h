@@ -451,6 +451,33 @@ int  fcfd_time_step(fcfd_ctx* ctx, const double* W_dev, int dt_type, double cfl,
  * reference, which has no adjoint. */
 int  fcfd_vjp(fcfd_ctx* ctx, double* W_dev, const double* Rbar_dev, const double* fx_weight_dev, double* Wbar_dev, void* stream);
 
+/* The exact derivatives with respect to BC values (no counterpart in the
+ * reference: its BC values are fixed inputs, src/set_bcs.cc:35,
+ * ns_set_bcs.cc:25-180).  A BC's values enter the residual through its
+ * ghosts' centre states (then the gradients and limiters of the cells next
+ * to it), the right states of its faces (apply_bcs), a FIXED_Q wall's heat
+ * flux and the SA inlet nu~; these derivatives follow every path, by the
+ * tangent kernels with the BC's values as dual numbers.  Green-Gauss
+ * gradients: FCFD_ENOTSUP.
+ *
+ * fcfd_bc_tangent: dR_dev [nv][ld_cell] = dR/d(values of BC `index`) . dbc at
+ * the fixed state W_dev, and (dloads_host may be NULL) the loads' derivative
+ * in the layout of fcfd_loads (6 per BC).  dbc holds the direction in its
+ * fields p, T, rho, V, p_total, T_total, mdot, qdot, nut (the others are
+ * ignored).  The ghost slots of W_dev are written. */
+int  fcfd_bc_tangent(fcfd_ctx* ctx, double* W_dev, int index, const fcfd_bc* dbc, double* dR_dev, double* dloads_host, void* stream);
+/* The parameter part of the VJP (pbar): for every BC b and each of its
+ * FCFD_BC_NPARAMS values (p, T, rho, V[0], V[1], V[2], p_total, T_total,
+ * mdot, qdot, nut, in this order), pbar_host[FCFD_BC_NPARAMS b + q] =
+ * <Rbar, dR/dv> + the derivative of sum over boundary faces of
+ * fx_weight[bc][0..4] . (D - C) (fcfd_vjp's output term; fx_weight_dev may be
+ * NULL).  With Rbar = the adjoint lam and fx_weight an output's load weight
+ * this is lam . dR/dv + dJ/dv|_W, the adjoint sensitivity to each BC value.
+ * Exact; forward mode, one tangent evaluation per BC value. */
+#define FCFD_BC_NPARAMS 11
+int  fcfd_vjp_params(fcfd_ctx* ctx, double* W_dev, const double* Rbar_dev, const double* fx_weight_dev, double* pbar_host,
+                     void* stream);
+
 /* Surface loads of the state W_dev (ns_assemble_linear_system.cc:109-119,
  * loads.h:33-39): for every BC with include_in_loads, force = sum over its
  * faces of (C - D)[1..3] and moment = sum (x_f - moment_center) x force_f,

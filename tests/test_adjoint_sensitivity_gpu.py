@@ -6,7 +6,9 @@ The flow (SA-RANS plate 16x12x2, Mach 0.3, Re_L 5e6, AUSM+up, unlimited second
 order) is converged with the exact-tangent Newton solver; the adjoint
 (dR/dW)^T lam = -dC_D/dW is solved with fcfd_adjoint_step (exact transpose
 operator, transposed block-Jacobi); dC_D/dalpha = dC_D/dalpha|_W +
-lam . dR/dalpha|_W.  The check: re-converge the flow at alpha (1 +- 1e-3) and
+lam . dR/dalpha|_W, with both partials exact (fcfd_bc_tangent: the inlet's
+BC values as dual numbers through ghosts, gradients, limiters and boundary
+fluxes).  The check: re-converge the flow at alpha (1 +- 1e-3) and
 difference C_D.  Agreement is limited by how far the primal converges (its
 residual stalls near 3e-9 absolute from 2.5 at the start of the transient),
 which bounds the difference quotient's error at about 1e-5 relative.
