@@ -148,6 +148,30 @@ def wedge_case(ni=96, nj=48, seed=1, flux=A.FLUX_ROE):
     return Case("wedge", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=2.0))
 
 
+def pie_case(nr=6, nt=12, nz=3, seed=4, jitter=0.1, viscous=True):
+    """Axial flow through a cylinder whose innermost ring of hexahedra touches
+    the axis (hexahedra with two repeated nodes, which the set-up turns into
+    prisms): Mach 0.3 velocity inlet at the bottom, pressure outlet at the
+    top, slip outer wall; Roe, curvilinear/lsqr gradients, VK; the
+    freestream with seeded 1% noise."""
+    raw = meshgen.pie(nr, nt, nz, jitter=jitter, seed=seed)
+    Mw, g = 28.97, 1.4
+    R = gas_R(Mw)
+    p, T = 101325.0, 300.0
+    U = 0.3 * np.sqrt(g * R * T)
+    mu = 1.0e-3 if viscous else 0.0
+    bcs = [bc_wall(slip=True), bc_inlet_velocity(A.SPEC_P, (0.0, 0.0, U), p=p), bc_outlet(A.SPEC_P, p=p)]
+    opt = make_options(bcs, gamma=g, Mw=Mw, mu=mu, Pr=0.72 if viscous else None, flux=A.FLUX_ROE, limiter=A.LIMITER_VK,
+                       Minf=0.3, loads_bcs=(0,))
+    n = len(raw["cell_ptr"]) - 1
+    rng = np.random.default_rng(seed)
+    W = freestream_state(n, p, T, (0.0, 0.0, U))
+    W[:, 0] *= 1.0 + rng.uniform(-0.01, 0.01, n)
+    W[:, 3] *= 1.0 + rng.uniform(-0.01, 0.01, n)
+    W[:, 4] *= 1.0 + rng.uniform(-0.01, 0.01, n)
+    return Case("pie", raw, opt, W, dict(p=p, T=T, U=U, R=R, mach=0.3, mu=mu))
+
+
 def cylinder_case(nt=256, nr=128, seed=2, noise=0.0):
     """Laminar compressible NS past a unit cylinder, Mach 0.2, Re_D = 40, Pr 0.72,
     Sutherland viscosity; AUSM+up, lsqr gradients for every cell type."""

@@ -280,12 +280,17 @@ int build_host_grid(const fcfd_mesh* m, const fcfd_options* o, HostGrid& g, std:
             g.cell_nodes.push_back(g2l[ng]);
         }
         g.cell_nptr[c + 1] = (int)g.cell_nodes.size();
-        for (int a = 0; a < nn; ++a)
+    }
+    // cells with repeated nodes (collapsed edges): their faces keep each node
+    // once, and a hexahedron with two repeated nodes becomes a prism after the
+    // faces are built (grid_create_elements.cc:274-294, 426-467)
+    std::vector<char> repeated(N, 0);
+    for (int c = 0; c < N; ++c) {
+        const int* cn = &g.cell_nodes[g.cell_nptr[c]];
+        const int nn = g.cell_nptr[c + 1] - g.cell_nptr[c];
+        for (int a = 0; a < nn && !repeated[c]; ++a)
             for (int b = 0; b < a; ++b)
-                if (g.cell_nodes[g.cell_nptr[c] + a] == g.cell_nodes[g.cell_nptr[c] + b]) {
-                    err = "degenerate cell (repeated nodes) not supported";
-                    return FCFD_EMESH;
-                }
+                if (cn[a] == cn[b]) { repeated[c] = 1; break; }
     }
     g.nnode = nn_local;
     // BC membership bitmask per local node
@@ -296,15 +301,22 @@ int build_host_grid(const fcfd_mesh* m, const fcfd_options* o, HostGrid& g, std:
             if (ng >= 0 && ng < m->n_nodes && g2l[ng] >= 0) bcmask[g2l[ng]] |= (1u << b);
         }
     // ---- node -> cells CSR (ascending cells) ----------------------------------
+    // (a repeated node lists its cell once, grid_create_elements.cc:126-142)
+    auto first_in_cell = [&](int c, int k) {
+        for (int j = g.cell_nptr[c]; j < k; ++j) if (g.cell_nodes[j] == g.cell_nodes[k]) return false;
+        return true;
+    };
     std::vector<int> ncptr(nn_local + 1, 0), ncell;
     for (int c = 0; c < N; ++c)
-        for (int k = g.cell_nptr[c]; k < g.cell_nptr[c + 1]; ++k) ncptr[g.cell_nodes[k] + 1]++;
+        for (int k = g.cell_nptr[c]; k < g.cell_nptr[c + 1]; ++k)
+            if (!repeated[c] || first_in_cell(c, k)) ncptr[g.cell_nodes[k] + 1]++;
     for (int n = 0; n < nn_local; ++n) ncptr[n + 1] += ncptr[n];
     ncell.resize(ncptr[nn_local]);
     {
         std::vector<int> fill(ncptr.begin(), ncptr.end() - 1);
         for (int c = 0; c < N; ++c)
-            for (int k = g.cell_nptr[c]; k < g.cell_nptr[c + 1]; ++k) ncell[fill[g.cell_nodes[k]]++] = c;
+            for (int k = g.cell_nptr[c]; k < g.cell_nptr[c + 1]; ++k)
+                if (!repeated[c] || first_in_cell(c, k)) ncell[fill[g.cell_nodes[k]]++] = c;
     }
     // ---- faces by hashing sorted node tuples --------------------------------
     std::unordered_map<FaceKey, int, FaceKeyHash> fmap;
@@ -317,6 +329,16 @@ int build_host_grid(const fcfd_mesh* m, const fcfd_options* o, HostGrid& g, std:
         for (int cf = 0; cf < faces_of(nn); ++cf) {
             int fnod[4];
             int k = face_nodes(nn, cf, cn, fnod);
+            if (repeated[c]) {   // keep each node once; a face left with two nodes is not a face
+                int u = 0;
+                for (int q = 0; q < k; ++q) {
+                    bool seen = false;
+                    for (int j = 0; j < q; ++j) if (fnod[j] == fnod[q]) { seen = true; break; }
+                    if (!seen) fnod[u++] = fnod[q];
+                }
+                k = u;
+                if (k == 2) continue;
+            }
             FaceKey key{{-1, -1, -1, -1}};
             for (int q = 0; q < k; ++q) key.n[q] = fnod[q];
             std::sort(key.n, key.n + k);
@@ -371,6 +393,47 @@ int build_host_grid(const fcfd_mesh* m, const fcfd_options* o, HostGrid& g, std:
         }
         if (bc == -2) { err = "boundary condition could not be found for a face"; return FCFD_EBC; }
         g.fbc[f] = bc;
+    }
+    // ---- hexahedra with two repeated nodes become prisms (:426-467): the node
+    // list is read off the cell's triangular faces, in its face order, each
+    // rotated to start at a repeated node.  The node-sharing neighbours of
+    // interior cells keep the original lists (they are built before the faces
+    // in the reference, create_nodes_cells).
+    std::vector<int> orig_nptr, orig_nodes;
+    {
+        bool any = false;
+        for (int c = 0; c < N; ++c) any |= repeated[c] != 0;
+        if (any) {
+            orig_nptr = g.cell_nptr;
+            orig_nodes = g.cell_nodes;
+            std::vector<int> np(N + 1, 0), nodes;
+            nodes.reserve(g.cell_nodes.size());
+            for (int c = 0; c < N; ++c) {
+                const int* cn = &orig_nodes[orig_nptr[c]];
+                const int nn = orig_nptr[c + 1] - orig_nptr[c];
+                std::vector<int> rep;
+                if (repeated[c])
+                    for (int a = 0; a < nn; ++a)
+                        for (int b = 0; b < a; ++b)
+                            if (cn[a] == cn[b]) rep.push_back(cn[a]);
+                if (nn == 8 && rep.size() == 2) {
+                    for (int k = g.cfptr[c]; k < g.cfptr[c + 1]; ++k) {
+                        const int f = g.cfaces[k];
+                        if (g.fnptr[f + 1] - g.fnptr[f] != 3) continue;
+                        const int* fn3 = &g.fnodes[g.fnptr[f]];
+                        int q = 0;
+                        for (; q < 3; ++q) if (fn3[q] == rep[0] || fn3[q] == rep[1]) break;
+                        if (q == 3) continue;
+                        for (int j = 0; j < 3; ++j) nodes.push_back(fn3[(q + j) % 3]);
+                    }
+                } else {
+                    nodes.insert(nodes.end(), cn, cn + nn);
+                }
+                np[c + 1] = (int)nodes.size();
+            }
+            g.cell_nptr.swap(np);
+            g.cell_nodes.swap(nodes);
+        }
     }
     // ---- areas, normals, centroids, volumes (grid.cc:123-178) ----------------
     g.fc.assign(3 * F, 0.0); g.fn.assign(3 * F, 0.0); g.ft1.assign(3 * F, 0.0); g.farea.assign(F, 0.0);
@@ -462,10 +525,12 @@ int build_host_grid(const fcfd_mesh* m, const fcfd_options* o, HostGrid& g, std:
         }
         std::vector<std::vector<int>> lists(T);
         std::vector<int> stamp(T, -1);
+        const std::vector<int>& onp = orig_nptr.empty() ? g.cell_nptr : orig_nptr;
+        const std::vector<int>& onode = orig_nptr.empty() ? g.cell_nodes : orig_nodes;
         for (int c = 0; c < N; ++c) {
             auto& L = lists[c];
-            for (int k = g.cell_nptr[c]; k < g.cell_nptr[c + 1]; ++k) {
-                int n = g.cell_nodes[k];
+            for (int k = onp[c]; k < onp[c + 1]; ++k) {
+                int n = onode[k];
                 for (int j = ncptr[n]; j < ncptr[n + 1]; ++j) {
                     int c2 = ncell[j];
                     if (stamp[c2] != c) { stamp[c2] = c; L.push_back(c2); }

@@ -69,6 +69,47 @@ def structured_hex(X, bcs, dimension=3, periodic_i=False):
                 bc_nodes=bnodes, dimension=dimension, shape=(ni, nj, nk))
 
 
+def pie(nr=6, nt=12, nz=3, radius=1.0, height=0.5, jitter=0.0, seed=0):
+    """A cylinder of nr x nt x nz hexahedra in (r, theta, z) whose innermost
+    ring touches the axis: those cells are hexahedra with two repeated nodes
+    (the axis node at the bottom and at the top, positions 0 = 3 and 4 = 7 of
+    the CGNS HEXA_8), the collapsed-edge cells a mesh generator writes for
+    prisms.  BCs: "outer" (r = radius), "bottom" (z = 0), "top" (z = height)."""
+    rng = np.random.default_rng(seed)
+    rs = np.linspace(0.0, radius, nr + 1)
+    ts = np.linspace(0.0, 2 * np.pi, nt + 1)[:-1]
+    zs = np.linspace(0.0, height, nz + 1)
+    # nodes: per z level the axis node, then rings i = 1..nr, nt nodes each
+    per = 1 + nr * nt
+    xyz = np.zeros(((nz + 1) * per, 3))
+    for k, z in enumerate(zs):
+        xyz[k * per] = (0.0, 0.0, z)
+        for i in range(1, nr + 1):
+            for j, t in enumerate(ts):
+                r = rs[i]
+                if jitter and i < nr:
+                    r = r + jitter * (rs[1] - rs[0]) * rng.uniform(-1, 1)
+                xyz[k * per + 1 + (i - 1) * nt + j] = (r * np.cos(t), r * np.sin(t), z)
+
+    def nid(i, j, k):
+        return k * per if i == 0 else k * per + 1 + (i - 1) * nt + (j % nt)
+
+    cells = []
+    for k in range(nz):
+        for i in range(nr):
+            for j in range(nt):
+                cells.append([nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k), nid(i, j + 1, k),
+                              nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)])
+    cells = np.array(cells, dtype=np.int32)
+    outer = [nid(nr, j, k) for k in range(nz + 1) for j in range(nt)]
+    bottom = [nid(i, j, 0) for i in range(nr + 1) for j in range(nt)]
+    top = [nid(i, j, nz) for i in range(nr + 1) for j in range(nt)]
+    names, bptr, bnodes = _bc_pack([("outer", np.unique(outer)), ("bottom", np.unique(bottom)), ("top", np.unique(top))])
+    cptr = np.arange(0, 8 * len(cells) + 1, 8, dtype=np.int32)
+    return dict(xyz=xyz, cell_ptr=cptr, cell_nodes=cells.reshape(-1), bc_names=names, bc_ptr=bptr, bc_nodes=bnodes,
+                dimension=3)
+
+
 # ---------------------------------------------------------------------------
 # benchmark meshes
 # ---------------------------------------------------------------------------

@@ -22,8 +22,10 @@
  *      (curvilinear_grad_map.cc:57-78); here the intended greedy pairing is
  *      used: take the lowest remaining face slot, pair it with the most
  *      opposite remaining later slot (first minimum wins), remove both.
- *  D4  degenerate cells (repeated nodes) are rejected (FCFD_EMESH).
- *  D5  WTLI with stencil size > 2 (tetra/tri fits) is rejected (FCFD_ENOTSUP).
+ *  D4  (none: repeated-node cells follow the reference, including its
+ *      hexahedron -> prism node rebuild, grid_create_elements.cc:426-467)
+ *  D5  (none: WTLI fits of any stencil size are restated; the device path
+ *      refuses stencils wider than two cells)
  */
 #include "fcfd_oracle.h"
 #include <math.h>
@@ -323,8 +325,10 @@ static void face_grow(orc_ctx* o) {
 
 static int create_faces(orc_ctx* o) {
     o->nface = 0;
+    unsigned char* repeated_cell = (unsigned char*)calloc(o->ncell > 0 ? o->ncell : 1, 1);
     for (int c = 0; c < o->ncell; ++c) {
         int nn = o->cnodes[c].n;
+        int degenerate_face_count = 0;
         for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
             int fnn;
             switch (nn) {
@@ -342,9 +346,20 @@ static int create_faces(orc_ctx* o) {
                 default: tn[fn] = o->cnodes[c].a[hexaFaces[cf][fn]]; break;
                 }
             }
-            for (int fn = 1; fn < fnn; ++fn)                         /* :274-294 */
-                for (int i = 0; i < fn; ++i)
-                    if (tn[fn] == tn[i]) { seterr(o, FCFD_EMESH, "degenerate cell (repeated nodes) not supported"); return FCFD_EMESH; }
+            /* repeated nodes (:274-294): keep the first appearance of each; a
+             * face left with two nodes is degenerate and not created */
+            int un[4], nu = 0, degenerate = 0;
+            for (int fn = 0; fn < fnn; ++fn) {
+                int skip = 0;
+                for (int i = 0; i < fn; ++i) if (tn[fn] == tn[i]) { skip = 1; break; }
+                if (!skip) un[nu++] = tn[fn];
+            }
+            if (nu != fnn) {
+                repeated_cell[c] = 1;
+                if (nu == 2) { degenerate = 1; degenerate_face_count++; }
+                fnn = nu;
+                for (int fn = 0; fn < fnn; ++fn) tn[fn] = un[fn];
+            }
             int internal = 0, unique = 1, neighbor = -1;              /* :296-314 */
             int n0 = tn[0];
             for (int nc = 0; nc < o->node_cells[n0].n; ++nc) {
@@ -354,7 +369,7 @@ static int create_faces(orc_ctx* o) {
                     else unique = 0;
                 }
             }
-            if (!unique) continue;
+            if (!unique || degenerate) continue;
             int bcv = -1;  /* INTERNAL_FACE */
             if (!internal) {                                          /* :318-363 */
                 bcv = -2;  /* UNASSIGNED_FACE */
@@ -383,7 +398,32 @@ static int create_faces(orc_ctx* o) {
             for (int i = 0; i < fnn; ++i) iv_push(&o->fnodes[f], tn[i]);
             o->nface++;
         }
+        o->cfaces[c].n -= degenerate_face_count;                      /* :424 */
     }
+    /* cells with repeated nodes (:426-467): a hexahedron with two repeated
+     * nodes becomes a prism whose nodes are read off its triangular faces,
+     * each rotated to start at a repeated node, in the cell's face order */
+    for (int c = 0; c < o->ncell; ++c) {
+        if (!repeated_cell[c]) continue;
+        int rep[64], nrep = 0;
+        for (int cn = 0; cn < o->cnodes[c].n; ++cn)
+            for (int cn2 = 0; cn2 < cn; ++cn2)
+                if (o->cnodes[c].a[cn] == o->cnodes[c].a[cn2] && nrep < 64) rep[nrep++] = o->cnodes[c].a[cn];
+        if (o->cnodes[c].n != 8 || nrep != 2) continue;
+        o->cnodes[c].n = 0;
+        for (int cf = 0; cf < o->cfaces[c].n; ++cf) {
+            const ivec* fnod = &o->fnodes[o->cfaces[c].a[cf]];
+            if (fnod->n != 3) continue;
+            int fn;
+            for (fn = 0; fn < 3; ++fn) {
+                if (fnod->a[fn] == rep[0]) break;
+                if (fnod->a[fn] == rep[1]) break;
+            }
+            if (fn == 3) continue;
+            for (int k = 0; k < 3; ++k) iv_push(&o->cnodes[c], fnod->a[(fn + k) % 3]);
+        }
+    }
+    free(repeated_cell);
     for (int f = 0; f < o->nface; ++f) if (o->fbc[f] == -2) { seterr(o, FCFD_EBC, "boundary condition could not be found for a face"); return FCFD_EBC; } /* set_bcs.cc:147-150 */
     return 0;
 }

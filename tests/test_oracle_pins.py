@@ -125,6 +125,41 @@ def test_kuhn_tets_volume(oracle_lib):
     assert np.isclose(m["cell_volume"][:N].sum(), 1.0, rtol=1e-13)
 
 
+def test_collapsed_hexahedra_become_prisms(oracle_lib):
+    """Hexahedra with two repeated nodes (grid_create_elements.cc:274-294,
+    426-467): the face through the collapsed edges has two distinct nodes and
+    is dropped, the two faces on the collapsed edges become triangles, and
+    the cell (5 faces) has the geometry of the prism with the same nodes:
+    its volume, face areas and centroid equal those of the mesh written
+    with PENTA_6 cells, and the pie's total volume is the polygon's area
+    times the height, (nt/2) sin(2 pi/nt) R^2 H."""
+    nr, nt, nz = 3, 10, 2
+    raw = meshgen.pie(nr, nt, nz, radius=1.0, height=0.5)
+    opt = sym_options(n_bc=3)
+    o = _oracle(raw, opt)
+    m = o.metrics()
+    N = o.info.n_cells
+    nf = np.diff(m["cell_face_ptr"])
+    inner = np.zeros(N, bool)
+    inner[np.arange(nz)[:, None] * nr * nt + np.arange(nt)[None, :]] = True   # cell = (k nr + i) nt + j, i = 0
+    assert (nf[inner] == 5).all() and (nf[~inner] == 6).all()
+    assert np.isclose(m["cell_volume"][:N].sum(), 0.5 * nt * np.sin(2 * np.pi / nt) * 0.5, rtol=1e-14)
+    # the same cells written as prisms: [axis, B_j, B_j+1] bottom, the same on top
+    cp, cn = np.asarray(raw["cell_ptr"]), np.asarray(raw["cell_nodes"])
+    cells = []
+    for c in range(N):
+        h = cn[cp[c]:cp[c + 1]]
+        cells.append([h[0], h[1], h[2], h[4], h[5], h[6]] if inner[c] else list(h))
+    ptr = np.zeros(N + 1, np.int32)
+    ptr[1:] = np.cumsum([len(x) for x in cells])
+    raw2 = dict(raw, cell_ptr=ptr, cell_nodes=np.concatenate(cells).astype(np.int32))
+    m2 = _oracle(raw2, opt).metrics()
+    np.testing.assert_allclose(m["cell_volume"][:N], m2["cell_volume"][:N], rtol=1e-13)
+    np.testing.assert_allclose(m["cell_centroid"][:3 * N], m2["cell_centroid"][:3 * N], rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(np.sort(m["face_area"]), np.sort(m2["face_area"]), rtol=1e-13)
+    assert len(m["face_area"]) == len(m2["face_area"])
+
+
 def test_ghost_centroids(oracle_lib):
     raw = box_hex(2, 1, 1, L=(2.0, 1.0, 1.0))
     bcs = [cases.bc_inlet_state(1e5, 300.0, (10.0, 0, 0), cases.gas_R(28.97)), cases.bc_outlet_p(1e5)] + \

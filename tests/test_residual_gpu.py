@@ -93,6 +93,12 @@ def test_plate_prisms_viscous():
     _compare(cases.plate_case(16, 10, 4, noise=0.01, first=1e-3), label="16x10x4")
 
 
+def test_pie_collapsed_hexahedra():
+    """hexahedra with two repeated nodes at the axis, which the set-up turns
+    into prisms (grid_create_elements.cc:274-294, 426-467)"""
+    _compare(cases.pie_case(6, 12, 3), label="collapsed hexahedra")
+
+
 def test_plate_steger_warming():
     cs = cases.plate_case(12, 8, 3, noise=0.01, first=1e-3)
     cs.opt.convective_flux = A.FLUX_SW

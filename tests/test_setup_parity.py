@@ -29,6 +29,7 @@ def _check(cs):
     ("tetbox", lambda: cases.tetbox_case(5)),
     ("bump", lambda: cases.bump_case(16, 8, 4)),
     ("plate_prisms", lambda: cases.plate_case(16, 10, 4)),
+    ("pie_collapsed_hexahedra", lambda: cases.pie_case()),
 ])
 def test_setup_bit_exact(name, make, oracle_lib):
     _check(make())
