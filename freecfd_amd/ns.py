@@ -25,6 +25,30 @@ def _check(h, rc):
         raise RuntimeError("libfcfd error %d: %s" % (rc, msg.decode() if msg else ""))
 
 
+def read_cfl_schedule(path):
+    """The (step, CFL) pairs of a CFL schedule file (timemarching
+    schedulefile, time_step.cc:63-77): whitespace-separated integers and
+    numbers, read in pairs until the end of the file."""
+    vals = open(path).read().split()
+    return [(int(vals[i]), float(vals[i + 1])) for i in range(0, len(vals) - 1, 2)]
+
+
+def scheduled_cfl(schedule, step, cfl):
+    """update_time_step's schedule (time_step.cc:160-175) at time step `step`
+    (from 1): the CFL of the last pair whose step is below `step`, read in
+    file order up to the first pair at or beyond it.  Returns (CFL, whether a
+    pair applied; the ramp stops once one has).  After the file's last pair
+    the CFL stays at its value (the reference's loop reads once more at the
+    end of the file; that read changes nothing here)."""
+    used = False
+    for s, v in schedule:
+        if step > s:
+            cfl, used = v, True
+        else:
+            break
+    return cfl, used
+
+
 class Grid:
     """Host-only set-up (no GPU needed): connectivity, metrics, maps."""
 
@@ -265,22 +289,32 @@ class NavierStokes(Grid):
         return st if check else (rc, st)
 
     def solve(self, opts=None, max_steps=500, drop=1e-10, cfl_initial=None, cfl_growth=1.0, cfl_target=None,
-              W=None, callback=None):
+              W=None, callback=None, schedule=None):
         """Implicit pseudo-time iterations to a residual drop (the step loop of
         main.cc/NavierStokes::solve with update_time_step's CFL ramp,
         time_step.cc:143-158: CFL_1 = cfl_initial, CFL_k = min(CFL_{k-1} *
-        growth, target)).  Stops when ||R_k||_2 <= drop * ||R_0||_2 (R before
-        step k), on divergence, or after max_steps.  Returns a dict with the
-        residual history, per-step wall times (ms) and the final drop."""
+        growth, target)).  schedule: the (step, CFL) pairs of a CFL schedule
+        file (read_cfl_schedule); the step type is then CFL_MAX and the
+        schedule takes over from the ramp (time_step.cc:160-175).  Stops when
+        ||R_k||_2 <= drop * ||R_0||_2 (R before step k), on divergence, or
+        after max_steps.  Returns a dict with the residual history, per-step
+        wall times (ms) and the final drop."""
         import time as _time
         opts = A.newton_opts() if opts is None else opts
         cfl = opts.cfl if cfl_initial is None else cfl_initial
         target = cfl if cfl_target is None else cfl_target
+        ramp = True
+        if schedule is not None:
+            opts.dt_type = A.DT_CFL_MAX
         hist, ms, iters = [], [], []
         status = "max_steps"
         for k in range(max_steps):
-            if k > 0:
+            if ramp and k > 0:
                 cfl = min(cfl * cfl_growth, target)
+            if schedule is not None:
+                c, used = scheduled_cfl(schedule, k + 1, cfl)
+                if used:
+                    cfl, ramp = c, False
             opts.cfl = cfl
             self.torch.cuda.synchronize()
             t0 = _time.perf_counter()
