@@ -60,13 +60,13 @@ struct fcfd_ctx {
     double* Dinv64 = nullptr;
     float *Dinv32 = nullptr, *D32 = nullptr, *JL32 = nullptr, *JR32 = nullptr, *b32 = nullptr, *x32 = nullptr;
     // tangent workspace (lazy)
-    double *dG = nullptr, *dPHI = nullptr, *dFX = nullptr, *jv_in = nullptr, *jv_out = nullptr, *jv_R = nullptr;
+    double *dG = nullptr, *dPHI = nullptr, *dFX = nullptr, *jv_in = nullptr, *jv_out = nullptr, *jv_R = nullptr, *dSRC = nullptr;
     // the FD Jacobian's off-diagonal blocks re-laid for the streamed matvec (lazy)
     SellBlocks sell;
     double* sell64 = nullptr;
     float* sell32 = nullptr;
     // FP32 tangent workspace of the FP32 matrix-free Newton operator (lazy)
-    float *dG32 = nullptr, *dPHI32 = nullptr, *dFX32 = nullptr, *jv_in32 = nullptr, *jv_R32 = nullptr;
+    float *dG32 = nullptr, *dPHI32 = nullptr, *dFX32 = nullptr, *jv_in32 = nullptr, *jv_R32 = nullptr, *dSRC32 = nullptr;
     // cell colouring for the block Gauss-Seidel preconditioner (lazy)
     Colors colors;
     // adjoint and loads (lazy)
@@ -856,7 +856,8 @@ static int ensure_tangent(fcfd_ctx* c) {
     const size_t nv = g.nv;
     if ((rc = dalloc(c, 2 * pair_groups(3 * (int)nv) * g.ld, &c->dG)) || (rc = dalloc(c, 2 * pair_groups((int)nv) * g.ld, &c->dPHI)) ||
         (rc = dalloc(c, nv * g.F, &c->dFX)) || (rc = dalloc(c, nv * g.ld, &c->jv_in)) ||
-        (rc = dalloc(c, nv * g.ld, &c->jv_out)) || (rc = dalloc(c, nv * g.ld, &c->jv_R)))
+        (rc = dalloc(c, nv * g.ld, &c->jv_out)) || (rc = dalloc(c, nv * g.ld, &c->jv_R)) ||
+        (nv == 6 && (rc = dalloc(c, g.ld, &c->dSRC))))
         return rc;
     return FCFD_OK;
 }
@@ -920,7 +921,8 @@ static int ensure_tangent32(fcfd_ctx* c) {
     int rc;
     const size_t nv = g.nv;
     if ((rc = dalloc(c, 2 * pair_groups(3 * (int)nv) * g.ld, &c->dG32)) || (rc = dalloc(c, 2 * pair_groups((int)nv) * g.ld, &c->dPHI32)) ||
-        (rc = dalloc(c, nv * g.F, &c->dFX32)) || (rc = dalloc(c, nv * g.ld, &c->jv_in32)) || (rc = dalloc(c, nv * g.ld, &c->jv_R32)))
+        (rc = dalloc(c, nv * g.F, &c->dFX32)) || (rc = dalloc(c, nv * g.ld, &c->jv_in32)) || (rc = dalloc(c, nv * g.ld, &c->jv_R32)) ||
+        (nv == 6 && (rc = dalloc(c, g.ld, &c->dSRC32))))
         return rc;
     return FCFD_OK;
 }
@@ -1135,7 +1137,7 @@ int fcfd_jvp(fcfd_ctx* c, double* W_dev, double* dW_dev, double* dR_dev, int mod
     int rc = ensure_tangent(c);
     if (rc) return rc;
     launch_jvp(c->dg, c->P, W_dev, dW_dev, c->G, c->dG, c->PHI, c->dPHI, c->dref, c->dFX, dR_dev, mode == FCFD_JAC_JVP_FULL,
-               c->status, (hipStream_t)stream);
+               c->status, (hipStream_t)stream, false, c->dSRC);
     HIPCHK(c, hipGetLastError());
     return FCFD_OK;
 }
@@ -1594,11 +1596,16 @@ static int newton_impl(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, co
     go.restart = o->restart; go.max_iters = o->max_iters; go.rtol = o->rtol; go.abstol = o->abstol; go.knoll = o->knoll_guess;
     GmresStats gs;
     // matrix-free operator: y = (P V/dt) x - J x with the tangent of W (the
-    // ghost slots of W are rewritten with the same values by each tangent call)
+    // ghost slots of W are rewritten with the same values by each tangent
+    // call).  The primal ghosts, gradients and limiters at W are those of
+    // step 1, so the tangent reads them (FCFD_JVP_RECOMPUTE=1: recomputes
+    // them in every call, the comparison case of the tests).
+    const char* jrc = std::getenv("FCFD_JVP_RECOMPUTE");
+    const bool known = !(jrc && jrc[0] == '1');
     auto jvp_op = [&](const double* x, double* y, hipStream_t ss) {
         (void)hipMemcpyAsync(c->jv_in, x, n5 * sizeof(double), hipMemcpyDeviceToDevice, ss);
         launch_jvp(g, c->P, W_dev, c->jv_in, c->G, c->dG, c->PHI, c->dPHI, c->dref, c->dFX, c->jv_R,
-                   o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss);
+                   o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss, known, c->dSRC);
         launch_newton_op(g, c->P, W_dev, c->dt, c->jv_in, c->jv_R, y, ss);   // jv_in: x may alias y
         if (ps) launch_pseudo_op(g, c->P, W_dev, c->dtau, c->drdp, c->jv_in, y, ss);
     };
@@ -1630,7 +1637,7 @@ static int newton_impl(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, co
     auto jvp_op32 = [&](const float* x, float* y, hipStream_t ss) {
         (void)hipMemcpyAsync(c->jv_in32, x, n5 * sizeof(float), hipMemcpyDeviceToDevice, ss);
         launch_jvp(g, c->P, W_dev, c->jv_in32, c->G, c->dG32, c->PHI, c->dPHI32, c->dref, c->dFX32, c->jv_R32,
-                   o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss);
+                   o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss, known, c->dSRC32);
         launch_newton_op(g, c->P, W_dev, c->dt, c->jv_in32, c->jv_R32, y, ss);
         if (ps) launch_pseudo_op(g, c->P, W_dev, c->dtau, c->drdp, c->jv_in32, y, ss);
     };

@@ -122,8 +122,9 @@ void launch_surface(const DevGrid& g, const Params& P, const double* W, const do
 void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s);
 void launch_cell_sum(const DevGrid& g, const float* FX, float* R, hipStream_t s);   // FP32 tangent sums
 // the residual's per-cell sums, plus V S (src, from launch_grad_lim) in the
-// nu~ row with the SA model
+// nu~ row with the SA model; the FP32 form adds the FP32 tangent's source
 void launch_residual_sum(const DevGrid& g, const double* src, const double* FX, double* R, hipStream_t s);
+void launch_residual_sum(const DevGrid& g, const float* src, const float* FX, float* R, hipStream_t s);
 // VK reference range only (find_min_max), when the limiter threshold needs it
 void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double* dref, hipStream_t s);
 constexpr int kDrefSize = 8 + 10 * 1024;   // dref[nv] + VK min/max partials

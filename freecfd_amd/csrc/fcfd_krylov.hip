@@ -61,6 +61,55 @@ __global__ void lincomb_kernel(size_t n, int k, const T* __restrict__ V, size_t 
     }
 }
 
+// The same two updates on 16-byte packs (4 floats per lane), the
+// basis vectors' packs loaded four at a time before they are used: one
+// element at a time, a thread waited on each basis load before it issued the
+// next.  Each element is still updated by the same operations in the same
+// order (q ascending), so the results are bit-identical.  Needs n, stride
+// and the pointers on 16-byte pack boundaries (the launchers check).
+template <typename T>
+struct alignas(16) Pack16 {
+    static constexpr int L = 16 / sizeof(T);
+    T v[L];
+};
+template <typename T, bool UPDATE>
+__global__ void __launch_bounds__(256) gs_pack_kernel(size_t np, int k, const Pack16<T>* __restrict__ V, size_t pstride,
+                                                      const double* __restrict__ h, T scale, Pack16<T>* __restrict__ y) {
+    constexpr int L = Pack16<T>::L;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < np; i += (size_t)gridDim.x * blockDim.x) {
+        Pack16<T> acc = y[i];
+        int q = 0;
+        for (; q + 4 <= k; q += 4) {
+            Pack16<T> a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] = V[(size_t)(q + u) * pstride + i];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const T c = (T)h[q + u];
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                    if (UPDATE) acc.v[l] -= c * a[u].v[l];
+                    else acc.v[l] += c * a[u].v[l];
+                }
+            }
+        }
+        for (; q < k; ++q) {
+            const Pack16<T> a = V[(size_t)q * pstride + i];
+            const T c = (T)h[q];
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                if (UPDATE) acc.v[l] -= c * a.v[l];
+                else acc.v[l] += c * a.v[l];
+            }
+        }
+        if (UPDATE) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) acc.v[l] = acc.v[l] * scale;
+        }
+        y[i] = acc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // deterministic multi-dot: out[q] = <V_q, w> for q < k over interior cells,
 // accumulated in FP64; partials [k][kRedBlocks] then a fixed-order tree.
@@ -470,15 +519,35 @@ hipError_t dev_mdot(const DevGrid& g, int k, const T* V, size_t stride, const T*
 template hipError_t dev_mdot<double>(const DevGrid&, int, const double*, size_t, const double*, double*, double*, double*, hipStream_t);
 template hipError_t dev_mdot<float>(const DevGrid&, int, const float*, size_t, const float*, double*, double*, double*, hipStream_t);
 
+// (FP32 vectors only: with FP64 vectors the packed kernel measured 4% slower
+// than the element-wise one on the 5M-cell plate, 0.885 against 0.849 ms)
+template <typename T>
+static bool on_packs(size_t n, size_t stride, const T* V, const T* y) {
+    constexpr size_t L = Pack16<T>::L;
+    return sizeof(T) == 4 && n % L == 0 && stride % L == 0 && reinterpret_cast<uintptr_t>(V) % 16 == 0 &&
+           reinterpret_cast<uintptr_t>(y) % 16 == 0;
+}
 template <typename T>
 void dev_mgs_update(const DevGrid& g, int k, const T* V, size_t stride, const double* h_dev, T scale, T* y, hipStream_t s) {
     size_t n = (size_t)g.nv * g.ld;
-    hipLaunchKernelGGL(mgs_update_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, h_dev, scale, y);
+    if (on_packs(n, stride, V, y)) {
+        constexpr size_t L = Pack16<T>::L;
+        hipLaunchKernelGGL((gs_pack_kernel<T, true>), ew_grid(n / L), dim3(256), 0, s, n / L, k,
+                           reinterpret_cast<const Pack16<T>*>(V), stride / L, h_dev, scale, reinterpret_cast<Pack16<T>*>(y));
+    } else {
+        hipLaunchKernelGGL(mgs_update_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, h_dev, scale, y);
+    }
 }
 template <typename T>
 void dev_lincomb(const DevGrid& g, int k, const T* V, size_t stride, const double* c_dev, T* x, hipStream_t s) {
     size_t n = (size_t)g.nv * g.ld;
-    hipLaunchKernelGGL(lincomb_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, c_dev, x);
+    if (on_packs(n, stride, V, x)) {
+        constexpr size_t L = Pack16<T>::L;
+        hipLaunchKernelGGL((gs_pack_kernel<T, false>), ew_grid(n / L), dim3(256), 0, s, n / L, k,
+                           reinterpret_cast<const Pack16<T>*>(V), stride / L, c_dev, T(1), reinterpret_cast<Pack16<T>*>(x));
+    } else {
+        hipLaunchKernelGGL(lincomb_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, c_dev, x);
+    }
 }
 template void dev_mgs_update<double>(const DevGrid&, int, const double*, size_t, const double*, double, double*, hipStream_t);
 template void dev_mgs_update<float>(const DevGrid&, int, const float*, size_t, const double*, float, float*, hipStream_t);

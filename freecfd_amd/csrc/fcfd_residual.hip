@@ -407,10 +407,10 @@ __global__ void __launch_bounds__(64) surface_kernel(DevGrid g, Params P, const 
 // R_c = sum over the faces of c in reference face order of (D-C)+0 (parent)
 // or -1*(D-C)+0 (neighbour) -- the VecSetValues(ADD) sequence of
 // ns_assemble_linear_system.cc:131-140 starting from VecSet(rhs, 0) -- then,
-// with SRC (SA model), + V_c S_c (src[c], written by grad_lim_kernel) in the
-// nu~ row
+// with SRC (SA model), + V_c S_c (src[c], written by grad_lim_kernel, or its
+// tangent by grad_lim_t_kernel) in the nu~ row
 template <int NV, bool SRC, typename T>
-__global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, const double* __restrict__ src, const T* __restrict__ FX,
+__global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, const T* __restrict__ src, const T* __restrict__ FX,
                                                        T* __restrict__ Rout) {
     // all half-face entries in one round, then all their flux rows in one
     // round (entries past the cell's count repeat its last one and are
@@ -630,6 +630,12 @@ void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t 
 void launch_cell_sum(const DevGrid& g, const float* FX, float* R, hipStream_t s) {
     dim3 grid((g.N + 255) / 256), blk(256);
     if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, false, float>), grid, blk, 0, s, g, nullptr, FX, R);
+    else hipLaunchKernelGGL((cell_sum_kernel<5, false, float>), grid, blk, 0, s, g, nullptr, FX, R);
+}
+
+void launch_residual_sum(const DevGrid& g, const float* src, const float* FX, float* R, hipStream_t s) {
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, true, float>), grid, blk, 0, s, g, src, FX, R);
     else hipLaunchKernelGGL((cell_sum_kernel<5, false, float>), grid, blk, 0, s, g, nullptr, FX, R);
 }
 

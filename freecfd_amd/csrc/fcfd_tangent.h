@@ -5,13 +5,22 @@
 namespace fcfd {
 // dR = (dR/dW) dW.  full: differentiate ghosts, gradients and limiters too
 // (writes the ghost slots of W and dW, G/dG and PHI/dPHI); otherwise the
-// frozen-gradient linearisation.  dFX is [5][F] scratch.
+// frozen-gradient linearisation.  dFX is [5][F] scratch.  primal_known: the
+// ghost slots of W, G, PHI (and dref) already hold the residual pass's values
+// at this W, so the primal gradients and limiters are read, not recomputed
+// (the result is the same bit for bit; the Newton operator's case, where W
+// is fixed over the Krylov solve).  dsrc (N entries, SA model, full): the
+// source term's tangent is formed in the gradient kernel, kept there and
+// added by the ordered cell sum (the same result as the separate source
+// kernel used when dsrc is null).
 void launch_jvp(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
-                double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s);
+                double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s, bool primal_known = false,
+                double* dsrc = nullptr);
 // the same with an FP32 derivative lane: FP64 values (the residual's), FP32
 // tangents dW, dG, dPHI, dFX, dR (not on meshes with Green-Gauss cells)
 void launch_jvp(const DevGrid& g, const Params& P, double* W, float* dW, double* G, float* dG, double* PHI, float* dPHI,
-                double* dref, float* dFX, float* dR, bool full, int* status, hipStream_t s);
+                double* dref, float* dFX, float* dR, bool full, int* status, hipStream_t s, bool primal_known = false,
+                float* dsrc = nullptr);
 // y = (P V / dt) x - dR
 void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* dR,
                       double* y, hipStream_t s);

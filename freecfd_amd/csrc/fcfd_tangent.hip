@@ -70,18 +70,24 @@ __global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, doubl
     if constexpr (NV == 6) { W[5 * ld + gi] = R.nut_center.v; dW[5 * ld + gi] = R.nut_center.d; }
 }
 
-template <int LIM, int NV, typename TD>
+// KNOWN: G and PHI already hold the primal gradients and limiters of W (the
+// residual pass at the same state wrote them, with the same arithmetic in
+// the same order), so the primal gradient is loaded, not re-gathered, and
+// neither G nor PHI is written.  The derivative lane is the same either way.
+template <int LIM, int NV, typename TD, bool KNOWN>
 __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, const double* __restrict__ W,
                                                          const TD* __restrict__ dW, double* __restrict__ G,
                                                          TD* __restrict__ dG, double* __restrict__ PHI,
-                                                         TD* __restrict__ dPHI, const double* __restrict__ dref) {
+                                                         TD* __restrict__ dPHI, const double* __restrict__ dref,
+                                                         TD* __restrict__ dSRC) {
     using Dual = DualT<TD>;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
-    Dual gr[3 * NV];
+    double gv[3 * NV];
+    TD gd[3 * NV];
 #pragma unroll
-    for (int k = 0; k < 3 * NV; ++k) gr[k] = Dual(0.0);
+    for (int k = 0; k < 3 * NV; ++k) { gv[k] = 0.0; gd[k] = TD(0); }
     {   // map entries in chunks: their weights in one round, their states in the next
         constexpr int kChunk = 3;
         const int s = c >> 6, lane = c & 63;
@@ -99,34 +105,54 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
             for (int u = 0; u < kChunk; ++u) {
                 const int j = (int)__double_as_longlong(b[u].y);
 #pragma unroll
-                for (int v = 0; v < NV; ++v) q[u][v] = Dual(W[v * ld + j], dW[v * ld + j]);
+                for (int v = 0; v < NV; ++v) q[u][v] = Dual(KNOWN ? 0.0 : W[v * ld + j], dW[v * ld + j]);
             }
 #pragma unroll
             for (int u = 0; u < kChunk; ++u) {
                 if (k0 + u < K) {
 #pragma unroll
                     for (int v = 0; v < NV; ++v) {
-                        gr[3 * v] += a[u].x * q[u][v];
-                        gr[3 * v + 1] += a[u].y * q[u][v];
-                        gr[3 * v + 2] += b[u].x * q[u][v];
+                        // the dual sums gr += w q: value and derivative lanes
+                        // apart (the value lane is dropped when KNOWN)
+                        const Dual x = a[u].x * q[u][v], y = a[u].y * q[u][v], z = b[u].x * q[u][v];
+                        if constexpr (!KNOWN) {
+                            gv[3 * v] += x.v;
+                            gv[3 * v + 1] += y.v;
+                            gv[3 * v + 2] += z.v;
+                        }
+                        gd[3 * v] += x.d;
+                        gd[3 * v + 1] += y.d;
+                        gd[3 * v + 2] += z.d;
                     }
                 }
             }
         }
     }
     const int h0 = g.hf_ptr[c], nh = g.hf_ptr[c + 1] - h0;
-    double gv[3 * NV];
-    TD gd[3 * NV];
+    if (KNOWN || (g.ggc && g.ggc[c])) load_pairs<3 * NV>(G, ld, c, gv);
+    if (g.ggc && g.ggc[c]) load_pairs<3 * NV>(dG, ld, c, gd);   // a Green-Gauss cell: launch_green_gauss's tangent
+    Dual gr[3 * NV];
 #pragma unroll
-    for (int k = 0; k < 3 * NV; ++k) { gv[k] = gr[k].v; gd[k] = gr[k].d; }
-    if (g.ggc && g.ggc[c]) {   // a Green-Gauss cell: launch_green_gauss's primal and tangent gradients
-        load_pairs<3 * NV>(G, ld, c, gv);
-        load_pairs<3 * NV>(dG, ld, c, gd);
-#pragma unroll
-        for (int k = 0; k < 3 * NV; ++k) gr[k] = Dual(gv[k], gd[k]);
-    }
-    store_pairs<3 * NV>(G, ld, c, gv);
+    for (int k = 0; k < 3 * NV; ++k) gr[k] = Dual(gv[k], gd[k]);
+    if constexpr (!KNOWN) store_pairs<3 * NV>(G, ld, c, gv);
     store_pairs<3 * NV>(dG, ld, c, gd);
+    if constexpr (NV == 6) {
+        // the tangent of V_c S_c (SA source), summed into the nu~ row by
+        // cell_sum after the faces: sa_source_t_kernel's arithmetic on the
+        // same duals
+        if (dSRC) {
+            const Dual p(W[c], dW[c]), T(W[4 * ld + c], dW[4 * ld + c]), nut(W[5 * ld + c], dW[5 * ld + c]);
+            auto gvec = [&](int v) {
+                Vec<Dual> r;
+                r.x = gr[3 * v];
+                r.y = gr[3 * v + 1];
+                r.z = gr[3 * v + 2];
+                return r;
+            };
+            const Dual S = sa_source<Dual>(m_rho(P, p, T), m_visc(P, T), nut, gvec(1), gvec(2), gvec(3), gvec(5), g.wd[c]);
+            dSRC[c] = TD(g.vol[c]) * S.d;
+        }
+    }
     Dual q0[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) q0[v] = Dual(W[v * ld + c], dW[v * ld + c]);
@@ -137,7 +163,7 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
         TD pd[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) { pv[v] = phi[v].v; pd[v] = phi[v].d; }
-        store_pairs<NV>(PHI, ld, c, pv);
+        if constexpr (!KNOWN) store_pairs<NV>(PHI, ld, c, pv);
         store_pairs<NV>(dPHI, ld, c, pd);
     }
     // boundary ghosts take their parent's gradient
@@ -145,7 +171,7 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
     for (int k = 0; k < kMaxCellFaces; ++k) {
         const int h = h0 + min(k, nh - 1), fs = g.hf_face[h], gi = g.hf_nb[h];
         if (k < nh && fs >= 0 && gi >= g.N) {
-            store_pairs<3 * NV>(G, ld, gi, gv);
+            if constexpr (!KNOWN) store_pairs<3 * NV>(G, ld, gi, gv);
             store_pairs<3 * NV>(dG, ld, gi, gd);
         }
     }
@@ -280,9 +306,18 @@ static void launch_ff(const DevGrid& g, const Params& P, const DualLoader<NV, FU
     }
 }
 
+template <int LIM, int NV, typename TD>
+static void launch_glt(const DevGrid& g, const Params& P, double* W, TD* dW, double* G, TD* dG, double* PHI, TD* dPHI,
+                       double* dref, bool known, TD* dsrc, hipStream_t s) {
+    dim3 grid((g.N + 255) / 256), blk(256);
+    if (known) hipLaunchKernelGGL((grad_lim_t_kernel<LIM, NV, TD, true>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref, dsrc);
+    else hipLaunchKernelGGL((grad_lim_t_kernel<LIM, NV, TD, false>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref, dsrc);
+}
+
 template <int NV, typename TD>
 static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, TD* dW, double* G, TD* dG, double* PHI, TD* dPHI,
-                         double* dref, TD* dFX, TD* dR, bool full, int* status, hipStream_t s) {
+                         double* dref, TD* dFX, TD* dR, bool full, int* status, hipStream_t s, bool known, TD* dsrc) {
+    if (NV != 6 || !full) dsrc = nullptr;
     const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
     const bool has_phi = lim != FCFD_LIMITER_NONE;
     const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
@@ -291,26 +326,32 @@ static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, TD* dW, d
     if (full) {
         if (g.B > 0) hipLaunchKernelGGL((ghost_t_kernel<NV, TD>), dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
         if constexpr (std::is_same<TD, double>::value) {   // (the FP32 tangent is refused on meshes with Green-Gauss cells)
-            launch_green_gauss(g, W, G, 0, s);
+            if (!known) launch_green_gauss(g, W, G, 0, s);
             launch_green_gauss(g, dW, dG, 1, s);
         }
         if (lim == FCFD_LIMITER_VK) {
-            launch_vk_range(g, P, W, dref, s);   // primal reference range (constant in the tangent)
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_VK, NV, TD>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+            if (!known) launch_vk_range(g, P, W, dref, s);   // primal reference range (constant in the tangent)
+            launch_glt<FCFD_LIMITER_VK, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s);
         } else if (lim == FCFD_LIMITER_BJ) {
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_BJ, NV, TD>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+            launch_glt<FCFD_LIMITER_BJ, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s);
         } else if (lim == FCFD_LIMITER_MINMOD) {
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_MINMOD, NV, TD>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+            launch_glt<FCFD_LIMITER_MINMOD, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s);
         } else {
-            hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_NONE, NV, TD>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+            launch_glt<FCFD_LIMITER_NONE, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s);
         }
         DualLoader<NV, true, TD> L{cl, dW, dG, dPHI, g.N};
         launch_ff<true, NV, TD>(g, P, L, dFX, status, s);
     } else {
-        launch_ghosts(g, P, W, status, s);
-        launch_grad_lim(g, P, W, G, PHI, dref, s);
+        if (!known) {
+            launch_ghosts(g, P, W, status, s);
+            launch_grad_lim(g, P, W, G, PHI, dref, s);
+        }
         DualLoader<NV, false, TD> L{cl, dW, nullptr, nullptr, g.N};
         launch_ff<false, NV, TD>(g, P, L, dFX, status, s);
+    }
+    if (dsrc) {   // the SA source tangent came from grad_lim_t_kernel
+        launch_residual_sum(g, dsrc, dFX, dR, s);
+        return;
     }
     launch_cell_sum(g, dFX, dR, s);
     if constexpr (NV == 6) {
@@ -320,15 +361,15 @@ static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, TD* dW, d
 }
 
 void launch_jvp(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
-                double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s) {
-    if (g.nv == 6) launch_jvp_t<6, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
-    else launch_jvp_t<5, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
+                double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s, bool primal_known, double* dsrc) {
+    if (g.nv == 6) launch_jvp_t<6, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc);
+    else launch_jvp_t<5, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc);
 }
 
 void launch_jvp(const DevGrid& g, const Params& P, double* W, float* dW, double* G, float* dG, double* PHI, float* dPHI,
-                double* dref, float* dFX, float* dR, bool full, int* status, hipStream_t s) {
-    if (g.nv == 6) launch_jvp_t<6, float>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
-    else launch_jvp_t<5, float>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s);
+                double* dref, float* dFX, float* dR, bool full, int* status, hipStream_t s, bool primal_known, float* dsrc) {
+    if (g.nv == 6) launch_jvp_t<6, float>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc);
+    else launch_jvp_t<5, float>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc);
 }
 
 void launch_newton_op_t(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* Jt,
@@ -539,13 +580,13 @@ static void launch_param_t(const DevGrid& g, const Params& P, double* W, double*
     if (g.B > 0) hipLaunchKernelGGL((ghost_t_kernel<NV, double, true>), dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
     if (lim == FCFD_LIMITER_VK) {
         launch_vk_range(g, P, W, dref, s);
-        hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_VK, NV, double>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        launch_glt<FCFD_LIMITER_VK, NV, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, false, nullptr, s);
     } else if (lim == FCFD_LIMITER_BJ) {
-        hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_BJ, NV, double>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        launch_glt<FCFD_LIMITER_BJ, NV, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, false, nullptr, s);
     } else if (lim == FCFD_LIMITER_MINMOD) {
-        hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_MINMOD, NV, double>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        launch_glt<FCFD_LIMITER_MINMOD, NV, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, false, nullptr, s);
     } else {
-        hipLaunchKernelGGL((grad_lim_t_kernel<FCFD_LIMITER_NONE, NV, double>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref);
+        launch_glt<FCFD_LIMITER_NONE, NV, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, false, nullptr, s);
     }
     DualLoader<NV, true, double> L{cl, dW, dG, dPHI, g.N};
     switch (P.flux) {

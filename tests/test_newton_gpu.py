@@ -284,3 +284,36 @@ def test_sliced_matvec_newton_steps_match_the_face_block_matvec(fp32):
         os.environ.pop("FCFD_BSR_MATVEC", None)
         print("  max |W_sliced - W_face_blocks| = %.3e" % np.abs(out["sliced"] - out["face_blocks"]).max(), flush=True)
         assert np.array_equal(out["sliced"], out["face_blocks"])
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("jac", ["full", "frozen"])
+def test_jvp_operator_reading_the_step_gradients_matches_recomputing_them(jac, fp32):
+    """Inside a Newton step W is fixed, so the matrix-free operator reads the
+    primal gradients, limiters and ghosts that the step's residual wrote
+    instead of recomputing them in every product; they are the same numbers,
+    so two JVP Newton steps give bit-identical states with and without
+    recomputation (FCFD_JVP_RECOMPUTE=1), FP64 and FP32 Krylov, on the SA
+    plate (VK limiter, hexes and prisms), the cylinder (unlimited lsqr) and
+    the wedge (minmod)."""
+    import os
+    from freecfd_amd.ns import NavierStokes
+    mode = A.JAC_JVP_FULL if jac == "full" else A.JAC_JVP_FROZEN
+    for cs in (cases.plate_case(20, 10, 4, noise=0.01, sa=True), cases.cylinder_case(48, 24), cases.wedge_case(24, 12)):
+        print("case", cs.name, jac, "fp32", fp32, flush=True)
+        out = {}
+        for variant in ("recompute", "read"):
+            if variant == "recompute":
+                os.environ["FCFD_JVP_RECOMPUTE"] = "1"
+            else:
+                os.environ.pop("FCFD_JVP_RECOMPUTE", None)
+            ns = NavierStokes(cs.raw, cs.opt)
+            ns.set_state(cs.W0)
+            opts = A.newton_opts(cfl=20.0, jacobian=mode, fp32=fp32, restart=20, max_iters=20)
+            for _ in range(2):
+                ns.newton_step(opts)
+            out[variant] = ns.state_host(include_ghosts=False)
+            ns.close()
+        os.environ.pop("FCFD_JVP_RECOMPUTE", None)
+        print("  max |W_read - W_recompute| = %.3e" % np.abs(out["read"] - out["recompute"]).max(), flush=True)
+        assert np.array_equal(out["read"], out["recompute"])
