@@ -43,6 +43,7 @@ def lib():
         "fcfd_time_step_noalloc": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _vp, _vp, _vp]),
         "fcfd_vjp": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
         "fcfd_gradients_to_host": (C.c_int, [_vp, _vp, _vp]),
+        "fcfd_convective_flux": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "fcfd_loads": (C.c_int, [_vp, _vp, _vp, _vp]),
         "fcfd_adjoint_step": (C.c_int, [_vp, _vp, _vp, _vp, C.POINTER(A.NewtonOpts), C.POINTER(A.NewtonStats), _vp]),
         "fcfd_set_bc": (C.c_int, [_vp, C.c_int, C.POINTER(A.BC)]),

@@ -1026,6 +1026,22 @@ int fcfd_loads(fcfd_ctx* c, double* W_dev, double* loads_host, void* stream) {
     return FCFD_OK;
 }
 
+int fcfd_convective_flux(fcfd_ctx* c, int flux, int n, const double* WL_dev, const double* WR_dev, const double* normal_dev,
+                         const double* tangent1_dev, const double* area_dev, const double* diss_dev, const double* wall_dist_dev,
+                         double* flux_dev, double* weightL_dev, void* stream) {
+    if (!c) return FCFD_EINVAL;
+    if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
+    if (flux < FCFD_FLUX_ROE || flux > FCFD_FLUX_SW) return fail(c, FCFD_EINVAL, "unknown convective flux");
+    if (n < 0) return fail(c, FCFD_EINVAL, "negative face count");
+    if (n == 0) return FCFD_OK;
+    if (!WL_dev || !WR_dev || !normal_dev || !tangent1_dev || !area_dev || !flux_dev)
+        return fail(c, FCFD_EINVAL, "fcfd_convective_flux: a required array is NULL");
+    launch_flux_leaf(c->P, flux, n, WL_dev, WR_dev, normal_dev, tangent1_dev, area_dev, diss_dev, wall_dist_dev, flux_dev,
+                     weightL_dev, (hipStream_t)stream);
+    HIPCHK(c, hipGetLastError());
+    return FCFD_OK;
+}
+
 int fcfd_surface(fcfd_ctx* c, double* W_dev, double* mdot_host, double* qdot_host, double* tau_host, void* stream) {
     if (!c || !W_dev) return FCFD_EINVAL;
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");

@@ -125,6 +125,11 @@ void launch_cell_sum(const DevGrid& g, const float* FX, float* R, hipStream_t s)
 // nu~ row with the SA model; the FP32 form adds the FP32 tangent's source
 void launch_residual_sum(const DevGrid& g, const double* src, const double* FX, double* R, hipStream_t s);
 void launch_residual_sum(const DevGrid& g, const float* src, const float* FX, float* R, hipStream_t s);
+// the convective flux of a batch of faces from given left/right primitive
+// states (SoA [5][n]), normals and first tangents ([3][n]), areas; diss and
+// wdist (Steger-Warming only) may be null: 1 and 1e20
+void launch_flux_leaf(const Params& P, int flux, int n, const double* WL, const double* WR, const double* nrm, const double* tan1,
+                      const double* area, const double* diss, const double* wdist, double* out, double* wl, hipStream_t s);
 // VK reference range only (find_min_max), when the limiter threshold needs it
 void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double* dref, hipStream_t s);
 constexpr int kDrefSize = 8 + 10 * 1024;   // dref[nv] + VK min/max partials

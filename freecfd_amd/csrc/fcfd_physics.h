@@ -505,6 +505,25 @@ template <typename S> __device__ __forceinline__ void sw_flux(const Params& P, c
     }
 }
 
+// weightL.face as each flux function leaves it (ns_roe.cc:95,
+// ns_ausm_plus_up.cc:93,99, ns_sd_slau.cc:84,90, ns_vanleer.cc:39,48,62,
+// ns_stegger_warming.cc:38-46): Roe 0.5; van Leer 0 / 1 / 0.5 by the mean
+// normal Mach number; Steger-Warming the pressure-switch weight of its second
+// pass; AUSM+-up and SD-SLAU 1 where the face mass flux c0 is positive, else 0
+template <int FLUX>
+__device__ __forceinline__ double weight_left(const CState<double>& L, const CState<double>& R, double c0) {
+    if (FLUX == FCFD_FLUX_ROE) return 0.5;
+    if (FLUX == FCFD_FLUX_VAN_LEER) {
+        const double M = 0.5 * (L.Vn.x / L.a + R.Vn.x / R.a);
+        return (M <= -1.) ? 0. : (M >= 1.) ? 1. : 0.5;
+    }
+    if (FLUX == FCFD_FLUX_SW) {
+        const double deltap = fabs(L.p - R.p) / smin(L.p, R.p);
+        return 0.5 / (6. * 6. * deltap * deltap + 1.);
+    }
+    return (c0 > 0.) ? 1. : 0.;
+}
+
 // dispatch + rotation back to xyz (ns_convective_face_flux.cc:32-59)
 template <int FLUX, int NV, typename S>
 __device__ __forceinline__ void convective_flux(const Params& P, const CState<S>& L, const CState<S>& R, const FaceGeo& F, double diss,

@@ -245,21 +245,8 @@ __global__ void __launch_bounds__(64) kw_face_kernel(DevGrid g, Params P, const 
     const int parent = X.p, neighbor = X.n, b = F.bc;
     const double area = F.area;
     const double mdot = (X.C[0] - X.D[0]) / area;   // mdot.face (ns_assemble_linear_system.cc:122)
-    // weightL.face as the flux function leaves it (ns_roe.cc:95,
-    // ns_ausm_plus_up.cc:93,99, ns_sd_slau.cc:84,90, ns_vanleer.cc:39,48,62,
-    // ns_stegger_warming.cc:38-46)
-    double weightL;
-    if (FLUX == FCFD_FLUX_ROE) {
-        weightL = 0.5;
-    } else if (FLUX == FCFD_FLUX_VAN_LEER) {
-        const double M = 0.5 * (X.L.Vn.x / X.L.a + X.R.Vn.x / X.R.a);
-        weightL = (M <= -1.) ? 0. : (M >= 1.) ? 1. : 0.5;
-    } else if (FLUX == FCFD_FLUX_SW) {
-        const double deltap = fabs(X.L.p - X.R.p) / smin(X.L.p, X.R.p);
-        weightL = 0.5 / (6. * 6. * deltap * deltap + 1.);
-    } else {
-        weightL = (X.C[0] > 0.) ? 1. : 0.;
-    }
+    // weightL.face as the flux function leaves it
+    const double weightL = weight_left<FLUX>(X.L, X.R, X.C[0]);
     const double weightR = 1. - weightL;
     const double2 wgt = g.fgeo[(size_t)5 * Fs + f];
     const double lam_visc = m_visc(P, (b >= 0 && g.bc[b].T_fixed) ? g.bc[b].T : face_avg(wgt.x, wgt.y, W[4 * ld + parent], W[4 * ld + neighbor]));

@@ -445,6 +445,70 @@ __global__ void __launch_bounds__(256) cell_sum_kernel(DevGrid g, const T* __res
 }
 
 // --------------------------------------------------------------------------
+// the convective flux alone, over a batch of faces (the reference's leaf
+// functions roe_flux, AUSMplusUP_flux, vanLeer_flux, SD_SLAU_flux,
+// Stegger_Warming_flux with convective_face_flux's rotation and area,
+// ns_convective_face_flux.cc:25-59): each side's state from p, u, v, w, T
+// as the face loop finishes it (rho by the EOS, a, H, Vn in the face frame
+// n, t1, t2 = norm(n x t1)); flux [5][n], weightL [n]
+// --------------------------------------------------------------------------
+template <int FLUX>
+__global__ void __launch_bounds__(256) flux_leaf_kernel(Params P, int n, const double* __restrict__ WL, const double* __restrict__ WR,
+                                                        const double* __restrict__ nrm, const double* __restrict__ tan1,
+                                                        const double* __restrict__ area, const double* __restrict__ diss,
+                                                        const double* __restrict__ wdist, double* __restrict__ out,
+                                                        double* __restrict__ wl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t m = (size_t)n;
+    FaceGeo F;
+    F.n = mk<double>(nrm[i], nrm[m + i], nrm[2 * m + i]);
+    F.t1 = mk<double>(tan1[i], tan1[m + i], tan1[2 * m + i]);
+    F.t2 = vnorm(vcross(F.n, F.t1));
+    F.l2r = mk<double>(0., 0., 0.);
+    F.area = area[i];
+    F.bc = -1;
+    CState<double> S[2];
+    const double* W[2] = {WL, WR};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        S[k].p = S[k].p_center = W[k][i];
+        S[k].V = S[k].V_center = mk<double>(W[k][m + i], W[k][2 * m + i], W[k][3 * m + i]);
+        S[k].T = S[k].T_center = W[k][4 * m + i];
+        S[k].nut = S[k].nut_center = 0.;
+        S[k].rho = m_rho(P, S[k].p, S[k].T);
+        finish_state(P, S[k], F);
+    }
+    double fl[5];
+    convective_flux<FLUX, 5, double>(P, S[0], S[1], F, diss ? diss[i] : 1., wdist ? wdist[i] : 1.e20, fl);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v * m + i] = fl[v];
+    if (wl) wl[i] = weight_left<FLUX>(S[0], S[1], fl[0]);
+}
+
+void launch_flux_leaf(const Params& P, int flux, int n, const double* WL, const double* WR, const double* nrm, const double* tan1,
+                      const double* area, const double* diss, const double* wdist, double* out, double* wl, hipStream_t s) {
+    const dim3 grid((n + 255) / 256), blk(256);
+    switch (flux) {
+    case FCFD_FLUX_ROE:
+        hipLaunchKernelGGL(flux_leaf_kernel<FCFD_FLUX_ROE>, grid, blk, 0, s, P, n, WL, WR, nrm, tan1, area, diss, wdist, out, wl);
+        break;
+    case FCFD_FLUX_VAN_LEER:
+        hipLaunchKernelGGL(flux_leaf_kernel<FCFD_FLUX_VAN_LEER>, grid, blk, 0, s, P, n, WL, WR, nrm, tan1, area, diss, wdist, out, wl);
+        break;
+    case FCFD_FLUX_SD_SLAU:
+        hipLaunchKernelGGL(flux_leaf_kernel<FCFD_FLUX_SD_SLAU>, grid, blk, 0, s, P, n, WL, WR, nrm, tan1, area, diss, wdist, out, wl);
+        break;
+    case FCFD_FLUX_SW:
+        hipLaunchKernelGGL(flux_leaf_kernel<FCFD_FLUX_SW>, grid, blk, 0, s, P, n, WL, WR, nrm, tan1, area, diss, wdist, out, wl);
+        break;
+    default:
+        hipLaunchKernelGGL(flux_leaf_kernel<FCFD_FLUX_AUSM_PLUS_UP>, grid, blk, 0, s, P, n, WL, WR, nrm, tan1, area, diss, wdist, out, wl);
+        break;
+    }
+}
+
+// --------------------------------------------------------------------------
 // local time step (time_step.cc:193-220) for CFL_LOCAL; CFL_MAX uses a
 // deterministic min-reduction
 // --------------------------------------------------------------------------

@@ -183,6 +183,32 @@ class NavierStokes(Grid):
         _check(self.h, lib().fcfd_jvp(self.h, W.data_ptr(), dW.data_ptr(), out.data_ptr(), int(mode), C.c_void_p(s)))
         return out
 
+    def convective_flux(self, flux, WL, WR, normal, tangent1, area, diss=None, wall_dist=None):
+        """The convective flux of a batch of faces from given states
+        (fcfd_convective_flux: roe_flux and its siblings with
+        convective_face_flux's rotation and area).  WL, WR: (5, n) p, u, v,
+        w, T; normal, tangent1: (3, n); area, diss, wall_dist: (n,); host
+        arrays or device tensors.  Returns (flux (5, n), weightL (n,)) as
+        device tensors."""
+        torch = self.torch
+        dev = lambda x: None if x is None else torch.as_tensor(np.asarray(x, dtype=np.float64) if not torch.is_tensor(x) else x,
+                                                                dtype=torch.float64, device=self.device).contiguous()
+        WL, WR, normal, tangent1, area = dev(WL), dev(WR), dev(normal), dev(tangent1), dev(area)
+        diss, wall_dist = dev(diss), dev(wall_dist)
+        n = int(area.numel())
+        if WL.shape != (5, n) or WR.shape != (5, n) or normal.shape != (3, n) or tangent1.shape != (3, n):
+            raise ValueError("convective_flux: WL/WR must be (5, n) and normal/tangent1 (3, n) with n = len(area)")
+        for x in (diss, wall_dist):
+            if x is not None and x.numel() != n:
+                raise ValueError("convective_flux: diss and wall_dist must have n entries")
+        out = torch.empty((5, n), dtype=torch.float64, device=self.device)
+        wl = torch.empty((n,), dtype=torch.float64, device=self.device)
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        s = torch.cuda.current_stream().cuda_stream
+        _check(self.h, lib().fcfd_convective_flux(self.h, int(flux), n, ptr(WL), ptr(WR), ptr(normal), ptr(tangent1), ptr(area),
+                                                  ptr(diss), ptr(wall_dist), ptr(out), ptr(wl), C.c_void_p(s)))
+        return out, wl
+
     def vjp(self, Rbar, fx_weight=None, W=None, out=None):
         """Wbar = (dR/dW)^T Rbar [+ d/dW sum_f fx_weight[bc(f)] . (D - C)_f] on
         the GPU; Rbar a [5][ld] float64 device tensor, fx_weight an optional

@@ -496,6 +496,25 @@ int  fcfd_loads(fcfd_ctx* ctx, double* W_dev, double* loads_host, void* stream);
  * limiters of W_dev (its ghost slots are written); synchronises the stream. */
 int  fcfd_surface(fcfd_ctx* ctx, double* W_dev, double* mdot_host, double* qdot_host, double* tau_host, void* stream);
 
+/* The convective flux of a batch of n faces from given states: the leaf
+ * functions roe_flux, AUSMplusUP_flux, vanLeer_flux, SD_SLAU_flux and
+ * Stegger_Warming_flux with convective_face_flux's rotation to xyz and
+ * area (ns_convective_face_flux.cc:25-59; ns_roe.cc:25-98,
+ * ns_ausm_plus_up.cc:35-152, ns_vanleer.cc:25-66, ns_sd_slau.cc:37-94,
+ * ns_stegger_warming.cc:23-145).  Device arrays, SoA: WL_dev/WR_dev [5][n]
+ * primitive p, u, v, w, T of the left and right states; normal_dev and
+ * tangent1_dev [3][n] (unit vectors; t2 = norm(n x t1)); area_dev [n];
+ * diss_dev and wall_dist_dev [n] enter Steger-Warming only and may be NULL
+ * (then 1 and 1e20).  Each state is finished as the face loop does it (rho
+ * by the EOS, a, H, Vn in the face frame) with the context's material.
+ * Writes flux_dev [5][n] (area-weighted, xyz) and, if not NULL,
+ * weightL_dev [n] (weightL.face as the flux function leaves it).  flux is
+ * an FCFD_FLUX_* value, independent of the context's own scheme.  Stream
+ * ordered; n = 0 does nothing. */
+int  fcfd_convective_flux(fcfd_ctx* ctx, int flux, int n, const double* WL_dev, const double* WR_dev, const double* normal_dev,
+                          const double* tangent1_dev, const double* area_dev, const double* diss_dev,
+                          const double* wall_dist_dev, double* flux_dev, double* weightL_dev, void* stream);
+
 /* One pseudo-time step of the steady adjoint (dR/dW)^T lam = -g at the state
  * W_dev (g = dJ/dW of an output J, e.g. from fcfd_vjp with a load weight and
  * Rbar = 0): with rho = (dR/dW)^T lam + g, solves ((P V/dt) - dR/dW)^T delta

@@ -2376,7 +2376,8 @@ int orc_time_terms(orc_ctx* o, const double* dt, double* blocks) {
 }
 
 /* probe: convective flux between two primitive states across a face with
- * normal n and tangent t1 (t2 = norm(n x t1)), as convective_face_flux. */
+ * normal n and tangent t1 (t2 = norm(n x t1)), as convective_face_flux;
+ * out[0..4] the flux, out[5] weightL (weight_left). */
 int orc_flux_probe(orc_ctx* o, int flux_type, const double* WL, const double* WR, const double* n, const double* t1,
                    double area, double* out) {
     fstate F;
@@ -2404,9 +2405,10 @@ int orc_flux_probe(orc_ctx* o, int flux_type, const double* WL, const double* WR
     double diss0 = o->fdiss ? o->fdiss[0] : 0.;
     double wd0 = o->fwd ? o->fwd[0] : 0.;
     if (o->fdiss) { o->fdiss[0] = 1.; o->fwd[0] = 1.e20; }
-    double fl[NVM];
-    convective_face_flux(o, &L, &R, &F, fl, NULL);
+    double fl[NVM], wl = 0.;
+    convective_face_flux(o, &L, &R, &F, fl, &wl);
     for (int i = 0; i < 5; ++i) out[i] = fl[i];
+    out[5] = wl;
     if (o->fdiss) { o->fdiss[0] = diss0; o->fwd[0] = wd0; }
     o->opt.convective_flux = saved;
     return 0;

@@ -130,12 +130,14 @@ class Oracle:
         self._check(lib().orc_calc_limiter(self.h))
         return self.fields()[2]
 
-    def flux_probe(self, flux_type, WL, WR, n, t1, area=1.0):
-        out = np.zeros(5)
+    def flux_probe(self, flux_type, WL, WR, n, t1, area=1.0, weight=False):
+        """convective_face_flux of one face: the flux (5,), and with weight
+        also weightL"""
+        out = np.zeros(6)
         arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (WL, WR, n, t1)]
         self._check(lib().orc_flux_probe(self.h, flux_type, _p(arr[0]), _p(arr[1]), _p(arr[2]), _p(arr[3]),
                                          float(area), _p(out)))
-        return out
+        return (out[:5], out[5]) if weight else out[:5]
 
     def prepare(self):
         """update_boundaries -> calc_cell_grads -> calc_limiter (ns.cc:112-116)."""

@@ -102,3 +102,20 @@ def test_two_equation_rans_validation_on_a_host_context():
         assert L.fcfd_rans_get(hl, None, None, None, None, None) == A.EINVAL
     finally:
         L.fcfd_destroy(hl)
+
+
+def test_convective_flux_validates_on_a_host_context():
+    """fcfd_convective_flux needs device data: a host-only context refuses it
+    with FCFD_EINVAL (and a NULL context too), before any launch."""
+    from freecfd_amd import _abi as A, cases
+    cs = cases.wedge_case(4, 2)
+    m = A.make_mesh(cs.raw)
+    h = ctypes.c_void_p()
+    L = _lib.lib()
+    assert L.fcfd_create_host(ctypes.byref(m), ctypes.byref(cs.opt), ctypes.byref(h)) == 0
+    try:
+        assert L.fcfd_convective_flux(h, A.FLUX_ROE, 0, None, None, None, None, None, None, None, None, None, None) == A.EINVAL
+        assert b"device" in L.fcfd_last_error(h)
+        assert L.fcfd_convective_flux(None, A.FLUX_ROE, 0, None, None, None, None, None, None, None, None, None, None) == A.EINVAL
+    finally:
+        L.fcfd_destroy(h)

@@ -517,6 +517,46 @@ def test_flux_antisymmetry(flux, oracle_lib):
         assert np.allclose(F1, -F2, rtol=1e-11, atol=1e-11 * np.abs(F1).max()), (flux, F1, F2)
 
 
+@pytest.mark.parametrize("flux", [A.FLUX_ROE, A.FLUX_AUSM_PLUS_UP, A.FLUX_VAN_LEER, A.FLUX_SD_SLAU, A.FLUX_SW])
+def test_weight_left_closed_forms(flux, oracle_lib):
+    """weightL.face as each flux function leaves it, from the two states alone
+    (ns_roe.cc:95, ns_ausm_plus_up.cc:93,99, ns_sd_slau.cc:84,90,
+    ns_vanleer.cc:39,48,62, ns_stegger_warming.cc:38-46): Roe 1/2; AUSM+-up
+    and SD-SLAU 1 where the face mass flux is positive, else 0; van Leer by
+    the mean normal Mach number M = (M_L + M_R)/2: 0 for M <= -1, 1 for
+    M >= 1, else 1/2; Steger-Warming 1/2 / (36 dp^2 + 1) with
+    dp = |p_L - p_R| / min(p_L, p_R)."""
+    o = probe_oracle(bc_of(type=A.BC_SYMMETRY))
+    rng = np.random.default_rng(37)
+    seen = set()
+    for k in range(60):
+        n, t1 = frame(int(rng.integers(1 << 30)))
+        mach = [0.2, 0.7, 1.3, 2.5][k % 4]
+        TL, TR = 300 * (0.9 + 0.2 * rng.random()), 300 * (0.9 + 0.2 * rng.random())
+        aL, aR = np.sqrt(GAMMA * RGAS * TL), np.sqrt(GAMMA * RGAS * TR)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        WL = np.array([1e5 * (0.9 + 0.2 * rng.random()), *(aL * mach * d), TL])
+        WR = np.array([1e5 * (0.9 + 0.2 * rng.random()), *(aR * mach * (d + 0.3 * rng.normal(size=3))), TR])
+        F, w = o.flux_probe(flux, WL, WR, n, t1, weight=True)
+        if flux == A.FLUX_ROE:
+            want = 0.5
+        elif flux in (A.FLUX_AUSM_PLUS_UP, A.FLUX_SD_SLAU):
+            want = 1.0 if F[0] > 0 else 0.0
+        elif flux == A.FLUX_VAN_LEER:
+            M = 0.5 * (np.dot(WL[1:4], n) / aL + np.dot(WR[1:4], n) / aR)
+            want = 0.0 if M <= -1 else (1.0 if M >= 1 else 0.5)
+        else:
+            dp = abs(WL[0] - WR[0]) / min(WL[0], WR[0])
+            want = 0.5 / (36.0 * dp * dp + 1.0)
+        assert w == pytest.approx(want, rel=1e-14, abs=0.0), (flux, k, w, want)
+        seen.add(round(want, 6))
+    if flux in (A.FLUX_AUSM_PLUS_UP, A.FLUX_SD_SLAU):
+        assert seen == {0.0, 1.0}
+    if flux == A.FLUX_VAN_LEER:
+        assert seen == {0.0, 0.5, 1.0}
+
+
 # ---------------------------------------------------------------------------
 # viscous flux
 # ---------------------------------------------------------------------------
