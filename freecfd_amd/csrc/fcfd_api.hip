@@ -357,6 +357,8 @@ int build_device(fcfd_ctx* c) {
     {
         std::vector<int> par(F), nei(F), bc(F);
         std::vector<double> nx(F), ny(F), nz(F), tx(F), ty(F), tz(F), ar(F), cx(F), cy(F), cz(F), wp(F, 0.5), wn(F, 0.5), wd(F), ds(F);
+        std::vector<int> wslot(F, -1), wptr(1, 0), widx;   // wider face-average stencils
+        std::vector<double> ww;
         for (int i = 0; i < F; ++i) {
             int f = g.ifperm[i];
             par[i] = g.perm[g.fparent[f]];
@@ -368,11 +370,32 @@ int build_device(fcfd_ctx* c) {
             cx[i] = g.fc[3 * f]; cy[i] = g.fc[3 * f + 1]; cz[i] = g.fc[3 * f + 2];
             wd[i] = g.fwd[f]; ds[i] = g.fdiss[f];
             if (g.fbc[f] < 0) {
-                if (g.aptr[f + 1] - g.aptr[f] != 2 || g.aidx[g.aptr[f]] != g.fparent[f] || g.aidx[g.aptr[f] + 1] != g.fneigh[f])
-                    return fail(c, FCFD_ENOTSUP, "device path supports two-cell face-average stencils only (WTLI stencil 2 or SIMPLE)");
-                wp[i] = g.aw[g.aptr[f]];
-                wn[i] = g.aw[g.aptr[f] + 1];
+                if (g.aptr[f + 1] - g.aptr[f] == 2 && g.aidx[g.aptr[f]] == g.fparent[f] && g.aidx[g.aptr[f] + 1] == g.fneigh[f]) {
+                    wp[i] = g.aw[g.aptr[f]];
+                    wn[i] = g.aw[g.aptr[f] + 1];
+                } else {
+                    wslot[i] = (int)wptr.size() - 1;
+                    for (int k = g.aptr[f]; k < g.aptr[f + 1]; ++k) {
+                        widx.push_back(g.perm[g.aidx[k]]);
+                        ww.push_back(g.aw[k]);
+                    }
+                    wptr.push_back((int)widx.size());
+                }
             }
+        }
+        d.wslot = nullptr; d.nwide = 0; d.wptr = nullptr; d.widx = nullptr; d.ww = nullptr;
+        d.wgrad = nullptr; d.wgrad_d = nullptr; d.wgrad_f = nullptr;
+        const int nwide = (int)wptr.size() - 1;
+        if (nwide > 0 && c->P.viscous) {   // only the viscous face gradients read the face averages
+            if (c->P.kw)
+                return fail(c, FCFD_ENOTSUP, "the segregated two-equation RANS model on the device needs two-cell face-average stencils");
+            const size_t rows = 3 * ((size_t)d.nv - 1);
+            int rc;
+            if ((rc = upload(c, wslot, &d.wslot)) || (rc = upload(c, wptr, &d.wptr)) || (rc = upload(c, widx, &d.widx)) ||
+                (rc = upload(c, ww, &d.ww)) || (rc = dalloc(c, rows * nwide, &d.wgrad)) || (rc = dalloc(c, rows * nwide, &d.wgrad_d)) ||
+                (rc = dalloc(c, rows * nwide, &d.wgrad_f)))
+                return rc;
+            d.nwide = nwide;
         }
         std::vector<int> gh(B);
         for (int i = 0; i < B; ++i) gh[i] = g.fperm[g.ghost_face[g.iperm[N + i] - N]];
@@ -842,10 +865,12 @@ int fcfd_jacobian_to_host(fcfd_ctx* c, double* JL_host, double* JR_host, double*
     return FCFD_OK;
 }
 
-// The adjoint kernels transpose the gradient-map gather; Green-Gauss cells
-// (no map) are not transposed there, so fcfd_vjp and fcfd_adjoint_step
-// refuse a context that has them.
+// The adjoint kernels transpose the gradient-map gather and the two-cell
+// face average of the viscous face gradients; Green-Gauss cells (no map) and
+// wider face-average stencils are not transposed there, so fcfd_vjp and
+// fcfd_adjoint_step refuse a context that has them.
 static int refuse_green_gauss(fcfd_ctx* c) {
+    if (c->dg.wslot) return fail(c, FCFD_ENOTSUP, "the adjoint path supports two-cell face-average stencils only");
     return c->dg.ggc ? fail(c, FCFD_ENOTSUP, "the adjoint path does not support Green-Gauss gradients") : FCFD_OK;
 }
 

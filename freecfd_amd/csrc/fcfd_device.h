@@ -45,6 +45,22 @@ struct DevGrid {
     // interior and ghosts; NULL without it), read by the VISC = kViscRans face
     // kernels
     const double* kw_mut;
+    // face-average stencils wider than the face's two cells (WTLI or IDW with
+    // a stencil above 2, viscous cases; wslot NULL when every interior face
+    // averages over its parent and neighbour).  wslot[f] = the slot of
+    // internal face f (-1: two-cell); slot s averages entries wptr[s] ..
+    // wptr[s+1] (cells as internal ids, weights) in the reference's key order.
+    // Its face gradients of u, v, w, T (and nu~), 3 (nv-1) rows [.][nwide],
+    // are formed once per gradient pass (launch_wide_grad): wgrad from G,
+    // wgrad_d / wgrad_f the FP64 / FP32 tangent lanes' from dG.
+    const int* wslot;
+    int nwide;
+    const int* wptr;
+    const int* widx;
+    const double* ww;
+    double* wgrad;
+    double* wgrad_d;
+    float* wgrad_f;
 };
 
 // Gradients ([3 nv] components per cell), limiters ([nv]) and the FD
@@ -130,6 +146,11 @@ void launch_residual_sum(const DevGrid& g, const float* src, const float* FX, fl
 // wdist (Steger-Warming only) may be null: 1 and 1e20
 void launch_flux_leaf(const Params& P, int flux, int n, const double* WL, const double* WR, const double* nrm, const double* tan1,
                       const double* area, const double* diss, const double* wdist, double* out, double* wl, hipStream_t s);
+// the wide face-average stencils' face gradients (g.wslot set): out[k][s] =
+// sum over the entries of slot s, from 0 in key order, of w * G row 3 + k
+// (u, v, w, T, nu~ gradient components) of the entry's cell
+void launch_wide_grad(const DevGrid& g, const double* G, double* out, hipStream_t s);
+void launch_wide_grad(const DevGrid& g, const float* G, float* out, hipStream_t s);
 // VK reference range only (find_min_max), when the limiter threshold needs it
 void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double* dref, hipStream_t s);
 constexpr int kDrefSize = 8 + 10 * 1024;   // dref[nv] + VK min/max partials

@@ -6,6 +6,7 @@
 // on a cell loader, so that the tangent kernels run the same code on dual
 // numbers.
 #pragma once
+#include <type_traits>
 #include "fcfd_device.h"
 #include "fcfd_dual.h"
 #include "fcfd_physics.h"
@@ -53,6 +54,11 @@ struct CellLoader {
             ph[v] = (has_phi && !ghost) ? pl[v] : (ghost ? 0.0 : phi_const);
         }
     }
+    // face gradient of variable 1 + v (u, v, w, T, nu~) of wide-stencil slot sl
+    __device__ __forceinline__ Vec<double> face_grad(const DevGrid& g, int sl, int v) const {
+        const size_t m = (size_t)g.nwide;
+        return mk<double>(g.wgrad[(3 * v) * m + sl], g.wgrad[(3 * v + 1) * m + sl], g.wgrad[(3 * v + 2) * m + sl]);
+    }
 };
 
 // tangent cell loader: value = primal arrays, derivative = tangent arrays
@@ -89,6 +95,17 @@ struct DualLoader {
                 ph[v] = D(phv[v]);
             }
         }
+    }
+    // wide-stencil face gradient of variable 1 + v: the value from the
+    // gradient pass, the tangent (FULL) from the tangent's pass over dG
+    __device__ __forceinline__ Vec<D> face_grad(const DevGrid& g, int sl, int v) const {
+        const Vec<double> a = p.face_grad(g, sl, v);
+        if (!FULL) return mk<D>(D(a.x), D(a.y), D(a.z));
+        const size_t m = (size_t)g.nwide;
+        const TD* dg;
+        if constexpr (std::is_same<TD, float>::value) dg = g.wgrad_f;
+        else dg = g.wgrad_d;
+        return mk<D>(D(a.x, dg[(3 * v) * m + sl]), D(a.y, dg[(3 * v + 1) * m + sl]), D(a.z, dg[(3 * v + 2) * m + sl]));
     }
 };
 
@@ -160,8 +177,16 @@ __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_,
     }
     if (VISC) {
         const double wp = rec.a[5].x, wn = rec.a[5].y;
+        const int sl = g.wslot ? g.wslot[f] : -1;   // a wider face-average stencil's slot
         if (F.bc >= 0) {
             X.fs.gu = gp[1]; X.fs.gv = gp[2]; X.fs.gw = gp[3]; X.fs.gT = gp[4];
+        } else if (sl >= 0) {
+            // face average (variable.h:134-147) over a wider stencil (WTLI or
+            // IDW above 2 cells), formed by the gradient pass
+            X.fs.gu = ld_.face_grad(g, sl, 0);
+            X.fs.gv = ld_.face_grad(g, sl, 1);
+            X.fs.gw = ld_.face_grad(g, sl, 2);
+            X.fs.gT = ld_.face_grad(g, sl, 3);
         } else {   // face average (variable.h:134-147), two-entry stencil in key order
             X.fs.gu = vadd(vsc(wp, gp[1]), vsc(wn, gn[1]));
             X.fs.gv = vadd(vsc(wp, gp[2]), vsc(wn, gn[2]));
@@ -170,6 +195,7 @@ __device__ __forceinline__ void face_inputs(const DevGrid& g, const Loader& ld_,
         }
         if constexpr (NV == 6) {
             if (F.bc >= 0) X.fs.gn = gp[5];
+            else if (sl >= 0) X.fs.gn = ld_.face_grad(g, sl, 4);
             else X.fs.gn = vadd(vsc(wp, gp[5]), vsc(wn, gn[5]));
         }
         if constexpr (VISC == kViscRans) {

@@ -645,6 +645,38 @@ void launch_green_gauss(const DevGrid& g, const double* W, double* G, int tangen
     else hipLaunchKernelGGL(gg_kernel<5>, grid, blk, 0, s, g, W, G, tangent);
 }
 
+// face gradients of the wide face-average stencils (Variable<Vec3D>::
+// face_calculate, variable.h:134-147): per slot, from 0, entry by entry in
+// key order, w times the cell's gradient of u, v, w, T (and nu~); T is the
+// value type (the gradients, or a tangent lane's)
+template <int NV, typename T>
+__global__ void __launch_bounds__(256) wide_grad_kernel(DevGrid g, const T* __restrict__ G, T* __restrict__ out) {
+    const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= g.nwide) return;
+    const size_t ld = g.ld, m = (size_t)g.nwide;
+    T acc[3 * (NV - 1)];
+#pragma unroll
+    for (int k = 0; k < 3 * (NV - 1); ++k) acc[k] = T(0);
+    for (int e = g.wptr[sl]; e < g.wptr[sl + 1]; ++e) {
+        const int j = g.widx[e];
+        const T w = (T)g.ww[e];
+#pragma unroll
+        for (int k = 0; k < 3 * (NV - 1); ++k) acc[k] = acc[k] + w * G[pidx(ld, 3 + k, j)];
+    }
+#pragma unroll
+    for (int k = 0; k < 3 * (NV - 1); ++k) out[k * m + sl] = acc[k];
+}
+
+template <typename T>
+static void launch_wide_grad_t(const DevGrid& g, const T* G, T* out, hipStream_t s) {
+    if (!g.wslot || g.nwide == 0) return;
+    const dim3 grid((g.nwide + 255) / 256), blk(256);
+    if (g.nv == 6) hipLaunchKernelGGL((wide_grad_kernel<6, T>), grid, blk, 0, s, g, G, out);
+    else hipLaunchKernelGGL((wide_grad_kernel<5, T>), grid, blk, 0, s, g, G, out);
+}
+void launch_wide_grad(const DevGrid& g, const double* G, double* out, hipStream_t s) { launch_wide_grad_t<double>(g, G, out, s); }
+void launch_wide_grad(const DevGrid& g, const float* G, float* out, hipStream_t s) { launch_wide_grad_t<float>(g, G, out, s); }
+
 template <int NV>
 static void launch_grad_lim_t(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, double* src,
                               hipStream_t s) {
@@ -661,6 +693,7 @@ static void launch_grad_lim_t(const DevGrid& g, const Params& P, const double* W
     } else {
         hipLaunchKernelGGL((grad_lim_kernel<FCFD_LIMITER_NONE, NV>), grid, blk, 0, s, g, P, W, G, PHI, dref, src);
     }
+    launch_wide_grad(g, G, g.wgrad, s);
 }
 
 void launch_grad_lim(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, hipStream_t s,

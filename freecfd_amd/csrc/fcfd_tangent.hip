@@ -339,6 +339,9 @@ static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, TD* dW, d
         } else {
             launch_glt<FCFD_LIMITER_NONE, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s);
         }
+        if (!known) launch_wide_grad(g, G, g.wgrad, s);   // the primal face gradients of W (rewritten with G)
+        if constexpr (std::is_same<TD, float>::value) launch_wide_grad(g, dG, g.wgrad_f, s);
+        else launch_wide_grad(g, dG, g.wgrad_d, s);
         DualLoader<NV, true, TD> L{cl, dW, dG, dPHI, g.N};
         launch_ff<true, NV, TD>(g, P, L, dFX, status, s);
     } else {
@@ -588,6 +591,8 @@ static void launch_param_t(const DevGrid& g, const Params& P, double* W, double*
     } else {
         launch_glt<FCFD_LIMITER_NONE, NV, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, false, nullptr, s);
     }
+    launch_wide_grad(g, G, g.wgrad, s);
+    launch_wide_grad(g, dG, g.wgrad_d, s);
     DualLoader<NV, true, double> L{cl, dW, dG, dPHI, g.N};
     switch (P.flux) {
     case FCFD_FLUX_ROE: launch_fp<FCFD_FLUX_ROE, NV>(g, P, L, dFX, status, s); break;
