@@ -556,9 +556,9 @@ template void dev_lincomb<float>(const DevGrid&, int, const float*, size_t, cons
 
 template <typename T>
 void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s) {
-    if (g.nv == 6) hipLaunchKernelGGL((bsr_matvec_kernel<T, 6>), dim3((g.N + 255) / 256), dim3(256), 0, s, g, D, JL, JR, x, y);
-    else if (g.nv == 2) hipLaunchKernelGGL((bsr_matvec_kernel<T, 2>), dim3((g.N + 255) / 256), dim3(256), 0, s, g, D, JL, JR, x, y);
-    else hipLaunchKernelGGL((bsr_matvec_kernel<T, 5>), dim3((g.N + 255) / 256), dim3(256), 0, s, g, D, JL, JR, x, y);
+    if (g.nv == 6) hipLaunchKernelGGL((bsr_matvec_kernel<T, 6>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g, D, JL, JR, x, y);
+    else if (g.nv == 2) hipLaunchKernelGGL((bsr_matvec_kernel<T, 2>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g, D, JL, JR, x, y);
+    else hipLaunchKernelGGL((bsr_matvec_kernel<T, 5>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g, D, JL, JR, x, y);
 }
 template void dev_bsr_matvec<double>(const DevGrid&, const double*, const double*, const double*, const double*, double*, hipStream_t);
 template void dev_bsr_matvec<float>(const DevGrid&, const float*, const float*, const float*, const float*, float*, hipStream_t);
@@ -573,7 +573,7 @@ void dev_sell_pack(const DevGrid& g, const SellBlocks& sb, const T* JL, const T*
 }
 template <typename T>
 void dev_sell_matvec(const DevGrid& g, const SellBlocks& sb, const T* D, const T* blk, const T* x, T* y, hipStream_t s) {
-    const dim3 grid((g.N + 255) / 256), bk(256);
+    const dim3 grid = cell_grid(g.N), bk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL((sell_matvec_kernel<T, 6>), grid, bk, 0, s, g, sb.soff, sb.snb, D, blk, x, y);
     else hipLaunchKernelGGL((sell_matvec_kernel<T, 5>), grid, bk, 0, s, g, sb.soff, sb.snb, D, blk, x, y);
 }
@@ -584,21 +584,21 @@ template void dev_sell_matvec<float>(const DevGrid&, const SellBlocks&, const fl
 
 template <typename T>
 void dev_bj_invert(const DevGrid& g, const double* D, T* Dinv, int* status, hipStream_t s) {
-    if (g.nv == 6) hipLaunchKernelGGL((bj_invert_kernel<T, 6>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
-    else if (g.nv == 2) hipLaunchKernelGGL((bj_invert_kernel<T, 2>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
-    else hipLaunchKernelGGL((bj_invert_kernel<T, 5>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
+    if (g.nv == 6) hipLaunchKernelGGL((bj_invert_kernel<T, 6>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
+    else if (g.nv == 2) hipLaunchKernelGGL((bj_invert_kernel<T, 2>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
+    else hipLaunchKernelGGL((bj_invert_kernel<T, 5>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g.N, (size_t)g.ld, D, Dinv, status);
 }
 template <typename T>
 void dev_bj_apply(const DevGrid& g, const T* Dinv, const T* x, T* y, hipStream_t s) {
-    if (g.nv == 6) hipLaunchKernelGGL((bj_apply_kernel<T, 6>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
-    else if (g.nv == 2) hipLaunchKernelGGL((bj_apply_kernel<T, 2>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
-    else hipLaunchKernelGGL((bj_apply_kernel<T, 5>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
+    if (g.nv == 6) hipLaunchKernelGGL((bj_apply_kernel<T, 6>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
+    else if (g.nv == 2) hipLaunchKernelGGL((bj_apply_kernel<T, 2>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
+    else hipLaunchKernelGGL((bj_apply_kernel<T, 5>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
 }
 void dev_bj_apply_t(const DevGrid& g, const double* Dinv, const double* x, double* y, hipStream_t s) {
     if (g.nv == 6)
-        hipLaunchKernelGGL((bj_apply_kernel<double, 6, true>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
+        hipLaunchKernelGGL((bj_apply_kernel<double, 6, true>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
     else
-        hipLaunchKernelGGL((bj_apply_kernel<double, 5, true>), dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
+        hipLaunchKernelGGL((bj_apply_kernel<double, 5, true>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g.N, (size_t)g.ld, Dinv, x, y);
 }
 template void dev_bj_invert<double>(const DevGrid&, const double*, double*, int*, hipStream_t);
 template void dev_bj_invert<float>(const DevGrid&, const double*, float*, int*, hipStream_t);
@@ -626,7 +626,7 @@ void dev_bsgs_apply(const DevGrid& g, const Colors& col, const T* Dinv, const T*
         const int cnt = col.ptr[k + 1] - col.ptr[k];
         if (cnt <= 0) return;
         const int* cells = col.cells + col.ptr[k];
-        const dim3 grid((cnt + 255) / 256), blk(256);
+        const dim3 grid = cell_grid(cnt), blk(cell_threads(cnt));
         if (g.nv == 6 && trans) hipLaunchKernelGGL((bsgs_color_kernel<T, 6, true>), grid, blk, 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
         else if (g.nv == 6) hipLaunchKernelGGL((bsgs_color_kernel<T, 6>), grid, blk, 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
         else if (trans) hipLaunchKernelGGL((bsgs_color_kernel<T, 5, true>), grid, blk, 0, s, g, Dinv, JL, JR, b, x, cells, cnt);
@@ -666,7 +666,7 @@ hipError_t dev_update(const DevGrid& g, double* W, const double* dW, const doubl
     if (e != hipSuccess) return e;
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
     if (out_host[3] > 0.0) return hipSuccess;   // NaN/Inf in dW: W stays as it was
-    const dim3 grid((g.N + 255) / 256), blk(256);
+    const dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL(apply_update_kernel<6>, grid, blk, 0, s, g.N, (size_t)g.ld, W, dW);
     else hipLaunchKernelGGL(apply_update_kernel<5>, grid, blk, 0, s, g.N, (size_t)g.ld, W, dW);
     return hipGetLastError();

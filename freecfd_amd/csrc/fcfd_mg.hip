@@ -494,7 +494,7 @@ static void coarse_sgs(const MgLevel& L, int sweeps, hipStream_t s, int dir) {
     auto one = [&](int c) {
         const int cnt = L.colors.ptr[c + 1] - L.colors.ptr[c];
         if (cnt <= 0) return;
-        hipLaunchKernelGGL(mg_sgs_color_kernel<NV>, dim3((cnt + 255) / 256), dim3(256), 0, s, L.colors.cells + L.colors.ptr[c], cnt,
+        hipLaunchKernelGGL(mg_sgs_color_kernel<NV>, cell_grid(cnt), dim3(cell_threads(cnt)), 0, s, L.colors.cells + L.colors.ptr[c], cnt,
                            (size_t)L.ld, L.rptr, L.col, L.A, L.Dinv, L.b, L.x);
     };
     for (int sw = 0; sw < sweeps; ++sw) {
@@ -515,13 +515,13 @@ static void level_cycle(Multigrid& mg, size_t l, int sweeps, hipStream_t s) {
         return;
     }
     coarse_sgs<NV>(L, sweeps, s, 1);   // pre-smoothing: forward sweeps
-    hipLaunchKernelGGL(mg_residual_kernel<NV>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, (size_t)L.ld, L.rptr, L.col, L.D, L.A,
+    hipLaunchKernelGGL(mg_residual_kernel<NV>, cell_grid(L.n), dim3(cell_threads(L.n)), 0, s, L.n, (size_t)L.ld, L.rptr, L.col, L.D, L.A,
                        L.b, L.x, L.r);
     MgLevel& C = mg.lv[l + 1];
-    hipLaunchKernelGGL(mg_restrict_kernel<NV>, dim3((C.n + 255) / 256), dim3(256), 0, s, C.n, C.mptr, C.mem, L.r, (size_t)L.ld, C.b,
+    hipLaunchKernelGGL(mg_restrict_kernel<NV>, cell_grid(C.n), dim3(cell_threads(C.n)), 0, s, C.n, C.mptr, C.mem, L.r, (size_t)L.ld, C.b,
                        (size_t)C.ld);
     level_cycle<NV>(mg, l + 1, sweeps, s);
-    hipLaunchKernelGGL(mg_prolong_kernel<NV>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, C.agg, C.x, (size_t)C.ld, L.x,
+    hipLaunchKernelGGL(mg_prolong_kernel<NV>, cell_grid(L.n), dim3(cell_threads(L.n)), 0, s, L.n, C.agg, C.x, (size_t)C.ld, L.x,
                        (size_t)L.ld);
     coarse_sgs<NV>(L, sweeps, s, 2);   // post-smoothing: backward sweeps
 }
@@ -534,10 +534,10 @@ static void vcycle_t(const DevGrid& g, const Colors& fc, Multigrid& mg, const do
     dev_bsr_matvec<double>(g, D, JL, JR, x, mg.fine_r, s);
     dev_xmy<double>(n5, b, mg.fine_r, s);   // r = b - A x
     MgLevel& C = mg.lv[1];
-    hipLaunchKernelGGL(mg_restrict_kernel<NV>, dim3((C.n + 255) / 256), dim3(256), 0, s, C.n, C.mptr, C.mem, mg.fine_r, (size_t)g.ld,
+    hipLaunchKernelGGL(mg_restrict_kernel<NV>, cell_grid(C.n), dim3(cell_threads(C.n)), 0, s, C.n, C.mptr, C.mem, mg.fine_r, (size_t)g.ld,
                        C.b, (size_t)C.ld);
     level_cycle<NV>(mg, 1, sweeps, s);
-    hipLaunchKernelGGL(mg_prolong_kernel<NV>, dim3((g.N + 255) / 256), dim3(256), 0, s, g.N, C.agg, C.x, (size_t)C.ld, x, (size_t)g.ld);
+    hipLaunchKernelGGL(mg_prolong_kernel<NV>, cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g.N, C.agg, C.x, (size_t)C.ld, x, (size_t)g.ld);
     dev_bsgs_apply<double>(g, fc, Dinv, JL, JR, b, x, sweeps, s, false, 2);   // backward post-smoothing
 }
 

@@ -640,7 +640,7 @@ void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double*
 
 void launch_green_gauss(const DevGrid& g, const double* W, double* G, int tangent, hipStream_t s) {
     if (!g.ggc) return;
-    const dim3 grid((g.N + 255) / 256), blk(256);
+    const dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL(gg_kernel<6>, grid, blk, 0, s, g, W, G, tangent);
     else hipLaunchKernelGGL(gg_kernel<5>, grid, blk, 0, s, g, W, G, tangent);
 }
@@ -681,7 +681,7 @@ template <int NV>
 static void launch_grad_lim_t(const DevGrid& g, const Params& P, const double* W, double* G, double* PHI, double* dref, double* src,
                               hipStream_t s) {
     int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
-    dim3 grid((g.N + 255) / 256), blk(256);
+    dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     launch_green_gauss(g, W, G, 0, s);
     if (lim == FCFD_LIMITER_VK) {
         launch_vk_range(g, P, W, dref, s);
@@ -719,25 +719,25 @@ void launch_flux(const DevGrid& g, const Params& P, const double* W, const doubl
 }
 
 void launch_cell_sum(const DevGrid& g, const double* FX, double* R, hipStream_t s) {
-    dim3 grid((g.N + 255) / 256), blk(256);
+    dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, false, double>), grid, blk, 0, s, g, nullptr, FX, R);
     else hipLaunchKernelGGL((cell_sum_kernel<5, false, double>), grid, blk, 0, s, g, nullptr, FX, R);
 }
 
 void launch_cell_sum(const DevGrid& g, const float* FX, float* R, hipStream_t s) {
-    dim3 grid((g.N + 255) / 256), blk(256);
+    dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, false, float>), grid, blk, 0, s, g, nullptr, FX, R);
     else hipLaunchKernelGGL((cell_sum_kernel<5, false, float>), grid, blk, 0, s, g, nullptr, FX, R);
 }
 
 void launch_residual_sum(const DevGrid& g, const float* src, const float* FX, float* R, hipStream_t s) {
-    dim3 grid((g.N + 255) / 256), blk(256);
+    dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, true, float>), grid, blk, 0, s, g, src, FX, R);
     else hipLaunchKernelGGL((cell_sum_kernel<5, false, float>), grid, blk, 0, s, g, nullptr, FX, R);
 }
 
 void launch_residual_sum(const DevGrid& g, const double* src, const double* FX, double* R, hipStream_t s) {
-    dim3 grid((g.N + 255) / 256), blk(256);
+    dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL((cell_sum_kernel<6, true, double>), grid, blk, 0, s, g, src, FX, R);
     else hipLaunchKernelGGL((cell_sum_kernel<5, false, double>), grid, blk, 0, s, g, nullptr, FX, R);
 }
@@ -818,7 +818,7 @@ void launch_store_scalar(const double* src, double val, double* dst, hipStream_t
 }
 
 void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double cfl, double* dt, hipStream_t s) {
-    hipLaunchKernelGGL(dt_local_kernel, dim3((g.N + 255) / 256), dim3(256), 0, s, g, P, W, cfl, dt);
+    hipLaunchKernelGGL(dt_local_kernel, cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g, P, W, cfl, dt);
 }
 
 void launch_wall_distance(int T, const double* ccx, const double* ccy, const double* ccz, int nw, const double* wx, const double* wy,

@@ -309,7 +309,7 @@ static void launch_ff(const DevGrid& g, const Params& P, const DualLoader<NV, FU
 template <int LIM, int NV, typename TD>
 static void launch_glt(const DevGrid& g, const Params& P, double* W, TD* dW, double* G, TD* dG, double* PHI, TD* dPHI,
                        double* dref, bool known, TD* dsrc, hipStream_t s) {
-    dim3 grid((g.N + 255) / 256), blk(256);
+    dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (known) hipLaunchKernelGGL((grad_lim_t_kernel<LIM, NV, TD, true>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref, dsrc);
     else hipLaunchKernelGGL((grad_lim_t_kernel<LIM, NV, TD, false>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref, dsrc);
 }
@@ -322,7 +322,7 @@ static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, TD* dW, d
     const bool has_phi = lim != FCFD_LIMITER_NONE;
     const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
     CellLoader<NV> cl{W, G, PHI, has_phi, phi_const, (size_t)g.ld};
-    dim3 grid((g.N + 255) / 256), blk(256);
+    dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (full) {
         if (g.B > 0) hipLaunchKernelGGL((ghost_t_kernel<NV, TD>), dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
         if constexpr (std::is_same<TD, double>::value) {   // (the FP32 tangent is refused on meshes with Green-Gauss cells)
@@ -377,7 +377,7 @@ void launch_jvp(const DevGrid& g, const Params& P, double* W, float* dW, double*
 
 void launch_newton_op_t(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* Jt,
                         double* y, hipStream_t s) {
-    dim3 grid((g.N + 255) / 256), blk(256);
+    dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL(newton_op_t_kernel<6>, grid, blk, 0, s, g, P, W, dt, x, Jt, y);
     else hipLaunchKernelGGL(newton_op_t_kernel<5>, grid, blk, 0, s, g, P, W, dt, x, Jt, y);
 }
@@ -385,7 +385,7 @@ void launch_newton_op_t(const DevGrid& g, const Params& P, const double* W, cons
 template <typename T>
 static void launch_newton_op_v(const DevGrid& g, const Params& P, const double* W, const double* dt, const T* x, const T* dR, T* y,
                                hipStream_t s) {
-    dim3 grid((g.N + 255) / 256), blk(256);
+    dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL((newton_op_kernel<6, T>), grid, blk, 0, s, g, P, W, dt, x, dR, y);
     else hipLaunchKernelGGL((newton_op_kernel<5, T>), grid, blk, 0, s, g, P, W, dt, x, dR, y);
 }
@@ -524,14 +524,14 @@ __global__ void __launch_bounds__(256) pseudo_op_kernel(DevGrid g, Params P, con
 
 void launch_pseudo_rhs(const DevGrid& g, const Params& P, const double* W, const double* Wn, const double* dt, const int* nbp,
                        const int* nbi, int ws95, int corr, double* R, double* drdp, hipStream_t s) {
-    const dim3 grid((g.N + 255) / 256), blk(256);
+    const dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL(pseudo_rhs_kernel<6>, grid, blk, 0, s, g, P, W, Wn, dt, nbp, nbi, ws95, corr, R, drdp);
     else hipLaunchKernelGGL(pseudo_rhs_kernel<5>, grid, blk, 0, s, g, P, W, Wn, dt, nbp, nbi, ws95, corr, R, drdp);
 }
 
 void launch_pseudo_diag(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, double* D,
                         hipStream_t s) {
-    const dim3 grid((g.N + 255) / 256), blk(256);
+    const dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL(pseudo_diag_kernel<6>, grid, blk, 0, s, g, P, W, dtau, drdp, D);
     else hipLaunchKernelGGL(pseudo_diag_kernel<5>, grid, blk, 0, s, g, P, W, dtau, drdp, D);
 }
@@ -539,7 +539,7 @@ void launch_pseudo_diag(const DevGrid& g, const Params& P, const double* W, cons
 template <typename T>
 static void launch_pseudo_op_v(const DevGrid& g, const Params& P, const double* W, const double* dtau, const double* drdp, const T* x,
                                T* y, hipStream_t s) {
-    const dim3 grid((g.N + 255) / 256), blk(256);
+    const dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.nv == 6) hipLaunchKernelGGL((pseudo_op_kernel<6, T>), grid, blk, 0, s, g, P, W, dtau, drdp, x, y);
     else hipLaunchKernelGGL((pseudo_op_kernel<5, T>), grid, blk, 0, s, g, P, W, dtau, drdp, x, y);
 }
@@ -579,7 +579,7 @@ static void launch_param_t(const DevGrid& g, const Params& P, double* W, double*
     const bool has_phi = lim != FCFD_LIMITER_NONE;
     const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
     CellLoader<NV> cl{W, G, PHI, has_phi, phi_const, (size_t)g.ld};
-    dim3 grid((g.N + 255) / 256), blk(256);
+    dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.B > 0) hipLaunchKernelGGL((ghost_t_kernel<NV, double, true>), dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
     if (lim == FCFD_LIMITER_VK) {
         launch_vk_range(g, P, W, dref, s);
