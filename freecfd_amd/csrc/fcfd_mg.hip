@@ -129,27 +129,62 @@ __global__ void __launch_bounds__(256) mg_sgs_color_kernel(const int* __restrict
 }
 
 // all the colours of `sweeps` sweeps on a level of at most kSmallLevel cells
-// in one workgroup, a workgroup barrier between colours (the x a colour
-// writes is visible to the next one: the workgroup shares one CU's cache);
-// the same updates in the same colour order as the per-colour launches, in
-// one launch instead of one per colour.  dir as in coarse_sgs.
+// in one workgroup, a workgroup barrier between colours; the same updates in
+// the same colour order as the per-colour launches (sgs_row's operations), in
+// one launch instead of one per colour.  The level's x stays in LDS for the
+// whole call (one gather round per colour fewer; the blocks, b and the
+// row structure are read from memory).  dir as in coarse_sgs.
 constexpr int kSmallLevel = 1024;
 template <int NV>
-__global__ void __launch_bounds__(kSmallLevel) mg_sgs_level_kernel(int ncol, const int* __restrict__ cptr, const int* __restrict__ cells,
-                                                                   size_t ld, const int* __restrict__ rptr, const int* __restrict__ col,
+__global__ void __launch_bounds__(kSmallLevel) mg_sgs_level_kernel(int n, int ncol, const int* __restrict__ cptr,
+                                                                   const int* __restrict__ cells, size_t ld,
+                                                                   const int* __restrict__ rptr, const int* __restrict__ col,
                                                                    const double* __restrict__ A, const double* __restrict__ Dinv,
                                                                    const double* __restrict__ b, double* __restrict__ x, int sweeps,
                                                                    int dir) {
+    __shared__ double xs[NV * kSmallLevel];
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) xs[v * kSmallLevel + i] = x[v * ld + i];
+    __syncthreads();
     for (int sw = 0; sw < sweeps; ++sw) {
         for (int pass = 0; pass < 2; ++pass) {
             if ((pass == 0 && dir == 2) || (pass == 1 && dir == 1)) continue;
             for (int q = 0; q < ncol; ++q) {
                 const int k = pass == 0 ? q : ncol - 1 - q;
-                for (int t = cptr[k] + threadIdx.x; t < cptr[k + 1]; t += blockDim.x) sgs_row<NV>(cells[t], ld, rptr, col, A, Dinv, b, x);
+                for (int t = cptr[k] + threadIdx.x; t < cptr[k + 1]; t += blockDim.x) {
+                    const int I = cells[t];
+                    double acc[NV];
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) acc[j] = b[j * ld + I];
+                    for (int e = rptr[I]; e < rptr[I + 1]; ++e) {
+                        const int J = col[e];
+                        double xj[NV], a[NV * NV];
+                        load_block<NV>(A, e, a);
+#pragma unroll
+                        for (int i = 0; i < NV; ++i) xj[i] = xs[i * kSmallLevel + J];
+#pragma unroll
+                        for (int j = 0; j < NV; ++j)
+#pragma unroll
+                            for (int i = 0; i < NV; ++i) acc[j] -= a[NV * j + i] * xj[i];
+                    }
+                    double di[NV * NV];
+                    load_block<NV>(Dinv, I, di);
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) {
+                        double s = 0.0;
+#pragma unroll
+                        for (int i = 0; i < NV; ++i) s += di[NV * j + i] * acc[i];
+                        xs[j * kSmallLevel + I] = s;
+                    }
+                }
                 __syncthreads();
             }
         }
     }
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) x[v * ld + i] = xs[v * kSmallLevel + i];
 }
 
 // r = b - A x on a coarse level
@@ -487,7 +522,7 @@ void mg_setup(const DevGrid& g, Multigrid& mg, const double* D, const double* JL
 template <int NV>
 static void coarse_sgs(const MgLevel& L, int sweeps, hipStream_t s, int dir) {
     if (L.one_launch) {   // small level: every colour in one workgroup launch
-        hipLaunchKernelGGL(mg_sgs_level_kernel<NV>, dim3(1), dim3(kSmallLevel), 0, s, L.colors.n, L.cptr_dev, L.colors.cells,
+        hipLaunchKernelGGL(mg_sgs_level_kernel<NV>, dim3(1), dim3(kSmallLevel), 0, s, L.n, L.colors.n, L.cptr_dev, L.colors.cells,
                            (size_t)L.ld, L.rptr, L.col, L.A, L.Dinv, L.b, L.x, sweeps, dir);
         return;
     }
