@@ -40,13 +40,23 @@ template <typename T>
 __global__ void xmy_kernel(size_t n, const T* __restrict__ b, T* __restrict__ y) {   // y = b - y
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = b[i] - y[i];
 }
+// Gram-Schmidt coefficients handed to the update kernels by value (the
+// kernel-argument segment), so that the host's coefficients need no copy to
+// the device and no wait for it; more than kArgCoefs of them go through
+// device memory
+constexpr int kArgCoefs = 256;
+struct ArgCoefs {
+    double c[kArgCoefs];
+};
+__device__ __forceinline__ double coef(const double* h, int q) { return h[q]; }
+__device__ __forceinline__ double coef(const ArgCoefs& h, int q) { return h.c[q]; }
+
 // y = (y - sum_k h[k] * V_k) * scale  (V_k at V + k*stride)
-template <typename T>
-__global__ void mgs_update_kernel(size_t n, int k, const T* __restrict__ V, size_t stride, const double* __restrict__ h,
-                                  T scale, T* __restrict__ y) {
+template <typename T, typename CS>
+__global__ void mgs_update_kernel(size_t n, int k, const T* __restrict__ V, size_t stride, CS h, T scale, T* __restrict__ y) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         T acc = y[i];
-        for (int q = 0; q < k; ++q) acc -= (T)h[q] * V[q * stride + i];
+        for (int q = 0; q < k; ++q) acc -= (T)coef(h, q) * V[q * stride + i];
         y[i] = acc * scale;
     }
 }
@@ -72,9 +82,9 @@ struct alignas(16) Pack16 {
     static constexpr int L = 16 / sizeof(T);
     T v[L];
 };
-template <typename T, bool UPDATE>
-__global__ void __launch_bounds__(256) gs_pack_kernel(size_t np, int k, const Pack16<T>* __restrict__ V, size_t pstride,
-                                                      const double* __restrict__ h, T scale, Pack16<T>* __restrict__ y) {
+template <typename T, bool UPDATE, typename CS>
+__global__ void __launch_bounds__(256) gs_pack_kernel(size_t np, int k, const Pack16<T>* __restrict__ V, size_t pstride, CS h,
+                                                      T scale, Pack16<T>* __restrict__ y) {
     constexpr int L = Pack16<T>::L;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < np; i += (size_t)gridDim.x * blockDim.x) {
         Pack16<T> acc = y[i];
@@ -85,7 +95,7 @@ __global__ void __launch_bounds__(256) gs_pack_kernel(size_t np, int k, const Pa
             for (int u = 0; u < 4; ++u) a[u] = V[(size_t)(q + u) * pstride + i];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const T c = (T)h[q + u];
+                const T c = (T)coef(h, q + u);
 #pragma unroll
                 for (int l = 0; l < L; ++l) {
                     if (UPDATE) acc.v[l] -= c * a[u].v[l];
@@ -95,7 +105,7 @@ __global__ void __launch_bounds__(256) gs_pack_kernel(size_t np, int k, const Pa
         }
         for (; q < k; ++q) {
             const Pack16<T> a = V[(size_t)q * pstride + i];
-            const T c = (T)h[q];
+            const T c = (T)coef(h, q);
 #pragma unroll
             for (int l = 0; l < L; ++l) {
                 if (UPDATE) acc.v[l] -= c * a.v[l];
@@ -527,15 +537,27 @@ static bool on_packs(size_t n, size_t stride, const T* V, const T* y) {
     return sizeof(T) == 4 && n % L == 0 && stride % L == 0 && reinterpret_cast<uintptr_t>(V) % 16 == 0 &&
            reinterpret_cast<uintptr_t>(y) % 16 == 0;
 }
-template <typename T>
-void dev_mgs_update(const DevGrid& g, int k, const T* V, size_t stride, const double* h_dev, T scale, T* y, hipStream_t s) {
+template <typename T, typename CS>
+static void mgs_update_cs(const DevGrid& g, int k, const T* V, size_t stride, CS h, T scale, T* y, hipStream_t s) {
     size_t n = (size_t)g.nv * g.ld;
     if (on_packs(n, stride, V, y)) {
         constexpr size_t L = Pack16<T>::L;
-        hipLaunchKernelGGL((gs_pack_kernel<T, true>), ew_grid(n / L), dim3(256), 0, s, n / L, k,
-                           reinterpret_cast<const Pack16<T>*>(V), stride / L, h_dev, scale, reinterpret_cast<Pack16<T>*>(y));
+        hipLaunchKernelGGL((gs_pack_kernel<T, true, CS>), ew_grid(n / L), dim3(256), 0, s, n / L, k,
+                           reinterpret_cast<const Pack16<T>*>(V), stride / L, h, scale, reinterpret_cast<Pack16<T>*>(y));
     } else {
-        hipLaunchKernelGGL(mgs_update_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, h_dev, scale, y);
+        hipLaunchKernelGGL((mgs_update_kernel<T, CS>), ew_grid(n), dim3(256), 0, s, n, k, V, stride, h, scale, y);
+    }
+}
+// coefficients from the host by value (h_host, k <= kArgCoefs) or from h_dev
+template <typename T>
+void dev_mgs_update(const DevGrid& g, int k, const T* V, size_t stride, const double* h_dev, const double* h_host, T scale, T* y,
+                    hipStream_t s) {
+    if (h_host && k <= kArgCoefs) {
+        ArgCoefs a{};
+        for (int q = 0; q < k; ++q) a.c[q] = h_host[q];
+        mgs_update_cs<T, ArgCoefs>(g, k, V, stride, a, scale, y, s);
+    } else {
+        mgs_update_cs<T, const double*>(g, k, V, stride, h_dev, scale, y, s);
     }
 }
 template <typename T>
@@ -543,14 +565,16 @@ void dev_lincomb(const DevGrid& g, int k, const T* V, size_t stride, const doubl
     size_t n = (size_t)g.nv * g.ld;
     if (on_packs(n, stride, V, x)) {
         constexpr size_t L = Pack16<T>::L;
-        hipLaunchKernelGGL((gs_pack_kernel<T, false>), ew_grid(n / L), dim3(256), 0, s, n / L, k,
+        hipLaunchKernelGGL((gs_pack_kernel<T, false, const double*>), ew_grid(n / L), dim3(256), 0, s, n / L, k,
                            reinterpret_cast<const Pack16<T>*>(V), stride / L, c_dev, T(1), reinterpret_cast<Pack16<T>*>(x));
     } else {
         hipLaunchKernelGGL(lincomb_kernel<T>, ew_grid(n), dim3(256), 0, s, n, k, V, stride, c_dev, x);
     }
 }
-template void dev_mgs_update<double>(const DevGrid&, int, const double*, size_t, const double*, double, double*, hipStream_t);
-template void dev_mgs_update<float>(const DevGrid&, int, const float*, size_t, const double*, float, float*, hipStream_t);
+template void dev_mgs_update<double>(const DevGrid&, int, const double*, size_t, const double*, const double*, double, double*,
+                                      hipStream_t);
+template void dev_mgs_update<float>(const DevGrid&, int, const float*, size_t, const double*, const double*, float, float*,
+                                     hipStream_t);
 template void dev_lincomb<double>(const DevGrid&, int, const double*, size_t, const double*, double*, hipStream_t);
 template void dev_lincomb<float>(const DevGrid&, int, const float*, size_t, const double*, float*, hipStream_t);
 
@@ -728,10 +752,12 @@ int gmres(const DevGrid& g, const LinOp<T>& A, const LinOp<T>& M, const T* b, T*
                 const double est = nn - hs;
                 const bool again = pass == 0 && !(est > 0.5 * nn);
                 hn = (!again && est > 0.0) ? std::sqrt(est) : 0.0;
-                if (hipMemcpyAsync(w.coef_dev, h.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
-                    hipStreamSynchronize(s) != hipSuccess)
+                // up to kArgCoefs coefficients ride in the kernel's arguments
+                const bool by_value = j + 1 <= kArgCoefs;
+                if (!by_value && (hipMemcpyAsync(w.coef_dev, h.data(), (j + 1) * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+                                  hipStreamSynchronize(s) != hipSuccess))
                     return FCFD_EHIP;
-                dev_mgs_update(g, j + 1, V, stride, w.coef_dev, (T)(hn > 0.0 ? 1.0 / hn : 1.0), vn, s);
+                dev_mgs_update(g, j + 1, V, stride, w.coef_dev, by_value ? h.data() : nullptr, (T)(hn > 0.0 ? 1.0 / hn : 1.0), vn, s);
                 for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] += h[q];
                 if (!again) break;
             }
