@@ -67,6 +67,7 @@ struct fcfd_ctx {
     float* sell32 = nullptr;
     // FP32 tangent workspace of the FP32 matrix-free Newton operator (lazy)
     float *dG32 = nullptr, *dPHI32 = nullptr, *dFX32 = nullptr, *jv_in32 = nullptr, *jv_R32 = nullptr, *dSRC32 = nullptr;
+    LimLin lin32{nullptr, nullptr, nullptr};   // the limiters' linearisation for the FP32 operator (limited cases)
     // cell colouring for the block Gauss-Seidel preconditioner (lazy)
     Colors colors;
     // adjoint and loads (lazy)
@@ -949,6 +950,10 @@ static int ensure_tangent32(fcfd_ctx* c) {
         (rc = dalloc(c, nv * g.F, &c->dFX32)) || (rc = dalloc(c, nv * g.ld, &c->jv_in32)) || (rc = dalloc(c, nv * g.ld, &c->jv_R32)) ||
         (nv == 6 && (rc = dalloc(c, g.ld, &c->dSRC32))))
         return rc;
+    if (c->P.order != FCFD_ORDER_FIRST && c->P.limiter != FCFD_LIMITER_NONE &&
+        ((rc = dalloc(c, nv * g.ld, &c->lin32.nb)) || (rc = dalloc(c, nv * g.ld, &c->lin32.a)) ||
+         (rc = dalloc(c, 3 * nv * g.ld, &c->lin32.gv))))
+        return rc;
     return FCFD_OK;
 }
 
@@ -1643,6 +1648,8 @@ static int newton_impl(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, co
     // them in every call, the comparison case of the tests).
     const char* jrc = std::getenv("FCFD_JVP_RECOMPUTE");
     const bool known = !(jrc && jrc[0] == '1');
+    // the FP32 operator's limiter linearisation at this W (read by every product)
+    if (tan32 && known && o->jacobian == FCFD_JAC_JVP_FULL && c->lin32.nb) launch_lim_lin(g, c->P, W_dev, c->G, c->dref, c->lin32, s);
     auto jvp_op = [&](const double* x, double* y, hipStream_t ss) {
         (void)hipMemcpyAsync(c->jv_in, x, n5 * sizeof(double), hipMemcpyDeviceToDevice, ss);
         launch_jvp(g, c->P, W_dev, c->jv_in, c->G, c->dG, c->PHI, c->dPHI, c->dref, c->dFX, c->jv_R,
@@ -1678,7 +1685,7 @@ static int newton_impl(fcfd_ctx* c, double* W_dev, const fcfd_newton_opts* o, co
     auto jvp_op32 = [&](const float* x, float* y, hipStream_t ss) {
         (void)hipMemcpyAsync(c->jv_in32, x, n5 * sizeof(float), hipMemcpyDeviceToDevice, ss);
         launch_jvp(g, c->P, W_dev, c->jv_in32, c->G, c->dG32, c->PHI, c->dPHI32, c->dref, c->dFX32, c->jv_R32,
-                   o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss, known, c->dSRC32);
+                   o->jacobian == FCFD_JAC_JVP_FULL, c->status, ss, known, c->dSRC32, c->lin32.nb ? &c->lin32 : nullptr);
         launch_newton_op(g, c->P, W_dev, c->dt, c->jv_in32, c->jv_R32, y, ss);
         if (ps) launch_pseudo_op(g, c->P, W_dev, c->dtau, c->drdp, c->jv_in32, y, ss);
     };

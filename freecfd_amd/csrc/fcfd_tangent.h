@@ -3,6 +3,20 @@
 #include "fcfd_device.h"
 
 namespace fcfd {
+// The limiters' linearisation at a fixed state, for the FP32 operator of a
+// Newton step (W fixed over the Krylov solve): per cell c and variable v,
+// phi = min(1, min over c's faces of T_f) has one active term (or none), and
+//   dphi_v = a (dq_nb - dq_c) + gv . dgrad_v
+// with nb the neighbour whose state enters it (-1: none).  nb, a: [nv][ld];
+// gv: [3 nv][ld].  Formed once per step (launch_lim_lin) by the operations
+// and branch choices of the limiter itself, in FP64, stored in FP32.
+struct LimLin {
+    int* nb;
+    float* a;
+    float* gv;
+};
+void launch_lim_lin(const DevGrid& g, const Params& P, const double* W, const double* G, const double* dref, const LimLin& L,
+                    hipStream_t s);
 // dR = (dR/dW) dW.  full: differentiate ghosts, gradients and limiters too
 // (writes the ghost slots of W and dW, G/dG and PHI/dPHI); otherwise the
 // frozen-gradient linearisation.  dFX is [5][F] scratch.  primal_known: the
@@ -18,9 +32,12 @@ void launch_jvp(const DevGrid& g, const Params& P, double* W, double* dW, double
                 double* dsrc = nullptr);
 // the same with an FP32 derivative lane: FP64 values (the residual's), FP32
 // tangents dW, dG, dPHI, dFX, dR (not on meshes with Green-Gauss cells)
+// lin (full, limited second order): the limiters' linearisation at W
+// replaces the dual-number limiter in the gradient kernel; with primal_known
+// it must be that of the step's W, otherwise it is recomputed here.
 void launch_jvp(const DevGrid& g, const Params& P, double* W, float* dW, double* G, float* dG, double* PHI, float* dPHI,
                 double* dref, float* dFX, float* dR, bool full, int* status, hipStream_t s, bool primal_known = false,
-                float* dsrc = nullptr);
+                float* dsrc = nullptr, const LimLin* lin = nullptr);
 // y = (P V / dt) x - dR
 void launch_newton_op(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* dR,
                       double* y, hipStream_t s);

@@ -70,16 +70,151 @@ __global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, doubl
     if constexpr (NV == 6) { W[5 * ld + gi] = R.nut_center.v; dW[5 * ld + gi] = R.nut_center.d; }
 }
 
+// The limiters' linearisation at W (LimLin): the FP64 limiter of
+// limiter_phi (the same operations, neighbour order and tie rules: smin keeps
+// the earlier value, smax likewise), tracking which face's term is active
+// and, for it, the partial derivatives with respect to its two arguments:
+//   BJ  T = dP / dM,                       dP = u_max|min - q_c, dM = grad.c2f
+//   VK  T = (dP^2 + e^2 + 2 dM dP) / (dP^2 + 2 dM^2 + dM dP + e^2)
+//   minmod T = (mm + s_mm small) / (dM + s_dM small), mm = 0, dU or dM
+// dP's neighbour is the cell that set u_max (u_min), -1 when q_c itself did.
+template <int LIM, int NV>
+__global__ void __launch_bounds__(256) lim_lin_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ G,
+                                                      const double* __restrict__ dref, LimLin L) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.N) return;
+    const size_t ld = g.ld;
+    double q0[NV], gr[3 * NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) q0[v] = W[v * ld + c];
+    load_pairs<3 * NV>(G, ld, c, gr);
+    double phi[NV], a[NV], gx[NV], gy[NV], gz[NV];
+    int nbA[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { phi[v] = 1.; a[v] = 0.; gx[v] = gy[v] = gz[v] = 0.; nbA[v] = -1; }
+    const int h0 = g.hf_ptr[c], h1 = g.hf_ptr[c + 1];
+    const Vec<double> xc = centroid(g, c);
+    if (LIM == FCFD_LIMITER_MINMOD) {
+        const double small = 1.e-12;
+        for (int h = h0; h < h1; ++h) {
+            const int fs = g.hf_face[h], f = fs >= 0 ? fs : ~fs, nb = g.hf_nb[h];
+            const Vec<double> c2f = vsub(face_centroid(g, f), xc);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double dU = W[v * ld + nb] - q0[v];
+                const double dUg = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
+                int pick;   // 0: mm = 0, 1: mm = dU, 2: mm = dUg
+                double mm;
+                if (dU * dUg < 0.) { pick = 0; mm = 0.; }
+                else if (dU > 0.) { pick = (dUg < dU) ? 2 : 1; mm = pick == 2 ? dUg : dU; }   // smin(dU, dUg)
+                else { pick = (dU < dUg) ? 2 : 1; mm = pick == 2 ? dUg : dU; }                 // smax(dU, dUg)
+                const double sU = (double)((mm > 0) - (mm < 0)), sG = (double)((dUg > 0) - (dUg < 0));
+                const double den = dUg + sG * small;
+                const double T = (mm + sU * small) / den;
+                if (T < phi[v]) {
+                    phi[v] = T;
+                    double cm;
+                    if (pick == 1) { nbA[v] = nb; a[v] = 1. / den; cm = -T / den; }
+                    else { nbA[v] = -1; a[v] = 0.; cm = pick == 2 ? (1. - T) / den : -T / den; }
+                    gx[v] = cm * c2f.x; gy[v] = cm * c2f.y; gz[v] = cm * c2f.z;
+                }
+            }
+        }
+    } else if (LIM == FCFD_LIMITER_BJ || LIM == FCFD_LIMITER_VK) {
+        double umax[NV], umin[NV];
+        int imax[NV], imin[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { umax[v] = umin[v] = q0[v]; imax[v] = imin[v] = -1; }
+        for (int h = h0; h < h1; ++h) {
+            const int nb = g.hf_nb[h];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                const double q = W[v * ld + nb];
+                if (umax[v] < q) { umax[v] = q; imax[v] = nb; }
+                if (q < umin[v]) { umin[v] = q; imin[v] = nb; }
+            }
+        }
+        for (int h = h0; h < h1; ++h) {
+            const int fs = g.hf_face[h], f = fs >= 0 ? fs : ~fs;
+            const Vec<double> c2f = vsub(face_centroid(g, f), xc);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                double dM = gr[3 * v] * c2f.x + gr[3 * v + 1] * c2f.y + gr[3 * v + 2] * c2f.z;
+                double dP = q0[v];
+                int nbc = -1;
+                if (LIM == FCFD_LIMITER_BJ) {
+                    bool flat = false;
+                    if (dM > 0.) { dP = umax[v] - dP; nbc = imax[v]; }
+                    else if (dM < 0.) { dP = umin[v] - dP; nbc = imin[v]; }
+                    else { dP = 1.; dM = 1.; flat = true; }
+                    const double T = dP / dM;
+                    if (T < phi[v]) {
+                        phi[v] = T;
+                        const double cm = flat ? 0. : -T / dM;
+                        nbA[v] = flat ? -1 : nbc;
+                        a[v] = (flat || nbc < 0) ? 0. : 1. / dM;
+                        gx[v] = cm * c2f.x; gy[v] = cm * c2f.y; gz[v] = cm * c2f.z;
+                    }
+                } else {
+                    const double eps = dref[v] / P.glscale;
+                    if (dM > 0.) { dP = umax[v] - dP; nbc = imax[v]; }
+                    else if (dM < 0.) { dP = umin[v] - dP; nbc = imin[v]; }
+                    const double dP2 = dP * dP, dM2 = dM * dM;
+                    const double e = 1.e-6 * P.limiter_threshold * eps;
+                    const double eps2 = e * e;
+                    if (dM != 0.) {
+                        const double D = dP2 + 2. * dM2 + dM * dP + eps2;
+                        const double T = ((dP2 + eps2) + 2. * dM * dP) / D;
+                        if (T < phi[v]) {
+                            phi[v] = T;
+                            const double cp = ((2. * dP + 2. * dM) - T * (2. * dP + dM)) / D;
+                            const double cm = (2. * dP - T * (4. * dM + dP)) / D;
+                            nbA[v] = nbc;
+                            a[v] = nbc >= 0 ? cp : 0.;
+                            gx[v] = cm * c2f.x; gy[v] = cm * c2f.y; gz[v] = cm * c2f.z;
+                        }
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        L.nb[v * ld + c] = nbA[v];
+        L.a[v * ld + c] = (float)a[v];
+        L.gv[(3 * v) * ld + c] = (float)gx[v];
+        L.gv[(3 * v + 1) * ld + c] = (float)gy[v];
+        L.gv[(3 * v + 2) * ld + c] = (float)gz[v];
+    }
+}
+
+void launch_lim_lin(const DevGrid& g, const Params& P, const double* W, const double* G, const double* dref, const LimLin& L,
+                    hipStream_t s) {
+    const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    const dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
+#define FCFD_LIM_LIN(LM)                                                                                                     \
+    do {                                                                                                                     \
+        if (g.nv == 6) hipLaunchKernelGGL((lim_lin_kernel<LM, 6>), grid, blk, 0, s, g, P, W, G, dref, L);                    \
+        else hipLaunchKernelGGL((lim_lin_kernel<LM, 5>), grid, blk, 0, s, g, P, W, G, dref, L);                              \
+    } while (0)
+    if (lim == FCFD_LIMITER_VK) FCFD_LIM_LIN(FCFD_LIMITER_VK);
+    else if (lim == FCFD_LIMITER_BJ) FCFD_LIM_LIN(FCFD_LIMITER_BJ);
+    else if (lim == FCFD_LIMITER_MINMOD) FCFD_LIM_LIN(FCFD_LIMITER_MINMOD);
+#undef FCFD_LIM_LIN
+}
+
 // KNOWN: G and PHI already hold the primal gradients and limiters of W (the
 // residual pass at the same state wrote them, with the same arithmetic in
 // the same order), so the primal gradient is loaded, not re-gathered, and
 // neither G nor PHI is written.  The derivative lane is the same either way.
-template <int LIM, int NV, typename TD, bool KNOWN>
+// LIN (with KNOWN, FP32 lanes): the limiters' derivative from their
+// linearisation at W (LimLin) instead of the limiter on dual numbers.
+template <int LIM, int NV, typename TD, bool KNOWN, bool LIN = false>
 __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, const double* __restrict__ W,
                                                          const TD* __restrict__ dW, double* __restrict__ G,
                                                          TD* __restrict__ dG, double* __restrict__ PHI,
                                                          TD* __restrict__ dPHI, const double* __restrict__ dref,
-                                                         TD* __restrict__ dSRC) {
+                                                         TD* __restrict__ dSRC, LimLin lin) {
     using Dual = DualT<TD>;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
@@ -156,15 +291,32 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
     Dual q0[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) q0[v] = Dual(W[v * ld + c], dW[v * ld + c]);
-    Dual phi[NV];
-    limiter_phi<LIM, NV>(g, P, TangentNb<TD>{W, dW, ld}, c, q0, gr, dref, phi);
-    if (LIM != FCFD_LIMITER_NONE) {
-        double pv[NV];
-        TD pd[NV];
+    if constexpr (LIN) {
+        static_assert(KNOWN, "the linearised limiter belongs to the known-state product");
+        if (LIM != FCFD_LIMITER_NONE) {
+            TD pd[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) { pv[v] = phi[v].v; pd[v] = phi[v].d; }
-        if constexpr (!KNOWN) store_pairs<NV>(PHI, ld, c, pv);
-        store_pairs<NV>(dPHI, ld, c, pd);
+            for (int v = 0; v < NV; ++v) {
+                const int j = lin.nb[v * ld + c];
+                TD d = TD(0);
+                if (j >= 0) d = TD(lin.a[v * ld + c]) * (dW[v * ld + j] - q0[v].d);
+                d = d + (TD(lin.gv[(3 * v) * ld + c]) * gd[3 * v] + TD(lin.gv[(3 * v + 1) * ld + c]) * gd[3 * v + 1] +
+                         TD(lin.gv[(3 * v + 2) * ld + c]) * gd[3 * v + 2]);
+                pd[v] = d;
+            }
+            store_pairs<NV>(dPHI, ld, c, pd);
+        }
+    } else {
+        Dual phi[NV];
+        limiter_phi<LIM, NV>(g, P, TangentNb<TD>{W, dW, ld}, c, q0, gr, dref, phi);
+        if (LIM != FCFD_LIMITER_NONE) {
+            double pv[NV];
+            TD pd[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { pv[v] = phi[v].v; pd[v] = phi[v].d; }
+            if constexpr (!KNOWN) store_pairs<NV>(PHI, ld, c, pv);
+            store_pairs<NV>(dPHI, ld, c, pd);
+        }
     }
     // boundary ghosts take their parent's gradient
 #pragma unroll
@@ -308,17 +460,26 @@ static void launch_ff(const DevGrid& g, const Params& P, const DualLoader<NV, FU
 
 template <int LIM, int NV, typename TD>
 static void launch_glt(const DevGrid& g, const Params& P, double* W, TD* dW, double* G, TD* dG, double* PHI, TD* dPHI,
-                       double* dref, bool known, TD* dsrc, hipStream_t s) {
+                       double* dref, bool known, TD* dsrc, hipStream_t s, const LimLin* lin = nullptr) {
     dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
-    if (known) hipLaunchKernelGGL((grad_lim_t_kernel<LIM, NV, TD, true>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref, dsrc);
-    else hipLaunchKernelGGL((grad_lim_t_kernel<LIM, NV, TD, false>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref, dsrc);
+    const LimLin L = lin ? *lin : LimLin{nullptr, nullptr, nullptr};
+    if constexpr (std::is_same<TD, float>::value && LIM != FCFD_LIMITER_NONE) {
+        if (lin) {
+            hipLaunchKernelGGL((grad_lim_t_kernel<LIM, NV, TD, true, true>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref, dsrc, L);
+            return;
+        }
+    }
+    if (known) hipLaunchKernelGGL((grad_lim_t_kernel<LIM, NV, TD, true>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref, dsrc, L);
+    else hipLaunchKernelGGL((grad_lim_t_kernel<LIM, NV, TD, false>), grid, blk, 0, s, g, P, W, dW, G, dG, PHI, dPHI, dref, dsrc, L);
 }
 
 template <int NV, typename TD>
 static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, TD* dW, double* G, TD* dG, double* PHI, TD* dPHI,
-                         double* dref, TD* dFX, TD* dR, bool full, int* status, hipStream_t s, bool known, TD* dsrc) {
+                         double* dref, TD* dFX, TD* dR, bool full, int* status, hipStream_t s, bool known, TD* dsrc,
+                         const LimLin* lin) {
     if (NV != 6 || !full) dsrc = nullptr;
     const int lim = (P.order == FCFD_ORDER_FIRST) ? FCFD_LIMITER_NONE : P.limiter;
+    if (lim == FCFD_LIMITER_NONE || !full) lin = nullptr;
     const bool has_phi = lim != FCFD_LIMITER_NONE;
     const double phi_const = (P.order == FCFD_ORDER_FIRST) ? 0.0 : 1.0;
     CellLoader<NV> cl{W, G, PHI, has_phi, phi_const, (size_t)g.ld};
@@ -329,13 +490,18 @@ static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, TD* dW, d
             if (!known) launch_green_gauss(g, W, G, 0, s);
             launch_green_gauss(g, dW, dG, 1, s);
         }
+        if (lin && !known) {   // the primal gradients, limiters and their linearisation at this W
+            launch_grad_lim(g, P, W, G, PHI, dref, s);
+            launch_lim_lin(g, P, W, G, dref, *lin, s);
+            known = true;
+        }
         if (lim == FCFD_LIMITER_VK) {
             if (!known) launch_vk_range(g, P, W, dref, s);   // primal reference range (constant in the tangent)
-            launch_glt<FCFD_LIMITER_VK, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s);
+            launch_glt<FCFD_LIMITER_VK, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s, lin);
         } else if (lim == FCFD_LIMITER_BJ) {
-            launch_glt<FCFD_LIMITER_BJ, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s);
+            launch_glt<FCFD_LIMITER_BJ, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s, lin);
         } else if (lim == FCFD_LIMITER_MINMOD) {
-            launch_glt<FCFD_LIMITER_MINMOD, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s);
+            launch_glt<FCFD_LIMITER_MINMOD, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s, lin);
         } else {
             launch_glt<FCFD_LIMITER_NONE, NV, TD>(g, P, W, dW, G, dG, PHI, dPHI, dref, known, dsrc, s);
         }
@@ -365,14 +531,15 @@ static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, TD* dW, d
 
 void launch_jvp(const DevGrid& g, const Params& P, double* W, double* dW, double* G, double* dG, double* PHI, double* dPHI,
                 double* dref, double* dFX, double* dR, bool full, int* status, hipStream_t s, bool primal_known, double* dsrc) {
-    if (g.nv == 6) launch_jvp_t<6, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc);
-    else launch_jvp_t<5, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc);
+    if (g.nv == 6) launch_jvp_t<6, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc, nullptr);
+    else launch_jvp_t<5, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc, nullptr);
 }
 
 void launch_jvp(const DevGrid& g, const Params& P, double* W, float* dW, double* G, float* dG, double* PHI, float* dPHI,
-                double* dref, float* dFX, float* dR, bool full, int* status, hipStream_t s, bool primal_known, float* dsrc) {
-    if (g.nv == 6) launch_jvp_t<6, float>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc);
-    else launch_jvp_t<5, float>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc);
+                double* dref, float* dFX, float* dR, bool full, int* status, hipStream_t s, bool primal_known, float* dsrc,
+                const LimLin* lin) {
+    if (g.nv == 6) launch_jvp_t<6, float>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc, lin);
+    else launch_jvp_t<5, float>(g, P, W, dW, G, dG, PHI, dPHI, dref, dFX, dR, full, status, s, primal_known, dsrc, lin);
 }
 
 void launch_newton_op_t(const DevGrid& g, const Params& P, const double* W, const double* dt, const double* x, const double* Jt,

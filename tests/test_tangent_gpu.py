@@ -184,17 +184,35 @@ def test_matrix_free_newton_steps(mode, fp32):
     assert np.all(np.isfinite(hist)) and hist[-1] < hist[0]
 
 
-@pytest.mark.parametrize("name", ["cylinder", "plate_sa"])
+def _lane_case(name):
+    if name == "cylinder":
+        return cases.cylinder_case(64, 32)
+    if name == "cylinder_bj":
+        cs = cases.cylinder_case(64, 32, noise=0.01)
+        cs.opt.limiter = A.LIMITER_BJ
+        return cs
+    if name == "wedge_minmod":
+        return cases.wedge_case(48, 24)
+    return cases.plate_case(24, 12, 4, noise=0.01, sa=True)
+
+
+@pytest.mark.parametrize("name", ["cylinder", "plate_sa", "cylinder_bj", "wedge_minmod"])
 def test_fp32_tangent_lane_newton_step_matches_fp64_lane(name):
     """The FP32 matrix-free Newton operator carries the tangent's derivative
-    lane in FP32 (values stay FP64).  One Newton step with it against the
-    same step with the FP64 lane (FCFD_TANGENT_FP64=1, converted at the
-    operator's boundary), both FP32 GMRES to rtol 1e-5: the updates agree to
-    1e-3 of their size per variable (velocity components on the speed's
-    scale), the GMRES iteration counts within 2, and
-    the residual after the step within 1e-3 of the residual before it."""
+    lane in FP32 (values stay FP64), and takes the limiters' derivative from
+    their linearisation at the step's state (VK on the cylinder and the
+    plate, Barth-Jespersen, minmod on the wedge).  One Newton step with it
+    against the same step with the FP64 lane (FCFD_TANGENT_FP64=1: the dual-
+    number limiter, converted at the operator's boundary), both FP32 GMRES
+    to rtol 1e-5: the updates agree to 1e-3 of their size per variable
+    (velocity components on the speed's scale), the GMRES iteration counts
+    within 2, and the residual after the step within 1e-3 of the residual
+    before it (1e-2 on the Mach 2 wedge, whose minmod limiter switches
+    branch within the update's rounding and whose GMRES stops at the
+    iteration cap: there the FP32 lane with the dual-number limiter lands
+    1.2e-2 away, the linearised one 4.5e-3)."""
     import os
-    cs = cases.cylinder_case(64, 32) if name == "cylinder" else cases.plate_case(24, 12, 4, noise=0.01, sa=True)
+    cs = _lane_case(name)
     print("case", cs.name, "N", cs.n_cells, flush=True)
     opts = A.newton_opts(cfl=20.0, jacobian=A.JAC_JVP_FULL, fp32=True, restart=40, max_iters=80, rtol=1e-5)
     out = {}
@@ -220,4 +238,5 @@ def test_fp32_tangent_lane_newton_step_matches_fp64_lane(name):
           flush=True)
     assert rel.max() < 1e-3
     assert abs(out["fp64"][1] - out["fp32"][1]) <= 2
-    assert abs(out["fp32"][3] - out["fp64"][3]) <= 1e-3 * out["fp64"][2]
+    bar = 1e-2 if name == "wedge_minmod" else 1e-3
+    assert abs(out["fp32"][3] - out["fp64"][3]) <= bar * out["fp64"][2]
