@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(128) fdjac_kernel(DevGrid g, Params P, const d
                                                     const double* __restrict__ PHI, int has_phi, double phi_const,
                                                     const double* __restrict__ UPD, double* __restrict__ JL,
                                                     double* __restrict__ JR, int* __restrict__ status) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
     FaceCtx<double, NV> X;
     CellLoader<NV> ld_{W, G, PHI, has_phi != 0, phi_const, (size_t)g.ld};
@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(256) diag_kernel(DevGrid g, Params P, const do
                                                    const double* __restrict__ UPD, const double* __restrict__ JL,
                                                    const double* __restrict__ JR, const double* __restrict__ dt,
                                                    double* __restrict__ D) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld, Fs = g.F;
     constexpr int B2 = NV * NV;

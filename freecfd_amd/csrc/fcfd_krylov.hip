@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(kRedThreads) update_norms_kernel(int N, size_t
 // W += dW over interior cells (ns.cc:340-348)
 template <int NV>
 __global__ void __launch_bounds__(256) apply_update_kernel(int N, size_t ld, double* __restrict__ W, const double* __restrict__ dW) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= N) return;
 #pragma unroll
     for (int v = 0; v < NV; ++v) W[v * ld + c] += dW[v * ld + c];
@@ -284,7 +284,7 @@ __device__ __forceinline__ void offdiag_accumulate(const DevGrid& g, const T* __
 template <typename T, int NV>
 __global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __restrict__ D, const T* __restrict__ JL,
                                                          const T* __restrict__ JR, const T* __restrict__ x, T* __restrict__ y) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
     T xc[NV], acc[NV], blk[NV * NV];

@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) ghost_t_kernel(DevGrid g, Params P, doubl
 template <int LIM, int NV>
 __global__ void __launch_bounds__(256) lim_lin_kernel(DevGrid g, Params P, const double* __restrict__ W, const double* __restrict__ G,
                                                       const double* __restrict__ dref, LimLin L) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
     double q0[NV], gr[3 * NV];
@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
                                                          TD* __restrict__ dPHI, const double* __restrict__ dref,
                                                          TD* __restrict__ dSRC, LimLin lin) {
     using Dual = DualT<TD>;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= g.N) return;
     const size_t ld = g.ld;
     double gv[3 * NV];
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(256) grad_lim_t_kernel(DevGrid g, Params P, co
 template <int FLUX, int VISC, bool FULL, int NV, typename TD, bool PSEED = false>
 __global__ void __launch_bounds__(64) face_flux_t_kernel(DevGrid g, Params P, DualLoader<NV, FULL, TD> L, TD* __restrict__ dFX,
                                                           int* __restrict__ status) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= g.F) return;
     FaceCtx<DualT<TD>, NV> X;
     face_base<FLUX, VISC, NV, DualT<TD>, DualLoader<NV, FULL, TD>, PSEED>(g, P, L, f, X, status);
