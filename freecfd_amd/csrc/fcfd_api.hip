@@ -1785,15 +1785,16 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
     if (int rc_dt = check_dt(c, o->dt_type, o->cfl)) return rc_dt;
     if (o->krylov_fp32) return fail(c, FCFD_ENOTSUP, "the adjoint step solves in FP64 (krylov_fp32 must be 0)");
-    if (o->precond != FCFD_PC_BLOCK_JACOBI && o->precond != FCFD_PC_BLOCK_SGS)
-        return fail(c, FCFD_ENOTSUP, "the adjoint step preconditions with block Jacobi or block SGS");
+    if (o->precond != FCFD_PC_BLOCK_JACOBI && o->precond != FCFD_PC_BLOCK_SGS && o->precond != FCFD_PC_MULTIGRID)
+        return fail(c, FCFD_EINVAL, "unknown preconditioner");
     if (o->precond == FCFD_PC_BLOCK_SGS && o->sweeps < 1) return fail(c, FCFD_EINVAL, "block SGS needs sweeps >= 1");
     if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad Krylov options");
-    const bool sgs = o->precond == FCFD_PC_BLOCK_SGS;
+    const bool sgs = o->precond == FCFD_PC_BLOCK_SGS, mgp = o->precond == FCFD_PC_MULTIGRID;
     int rc = refuse_green_gauss(c);
     if (!rc) rc = ensure_adjoint(c);
     if (!rc) rc = ensure_newton(c, o);
-    if (!rc && sgs) rc = ensure_colors(c);
+    if (!rc && (sgs || mgp)) rc = ensure_colors(c);
+    if (!rc && mgp) rc = ensure_mg(c);
     if (rc) return rc;
     const DevGrid& g = c->dg;
     hipStream_t s = (hipStream_t)stream;
@@ -1817,10 +1818,12 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
         return fail(c, FCFD_EDIVERGED, "Divergence detected: NaN/Inf in the adjoint residual");
     }
     // 2. local time step, FD blocks with the floor step, their diagonal
-    //    inverses (transposed block Jacobi or transposed block SGS)
+    //    inverses (transposed block Jacobi, transposed block SGS, or the
+    //    multigrid hierarchy's transposed V-cycle)
     launch_time_step(g, c->P, W_dev, o->dt_type, o->cfl, c->dt, c->dt_red, s);
     launch_fd_jacobian(g, c->P, W_dev, c->G, c->PHI, c->zero_upd, c->dt, c->JL, c->JR, c->Dg, c->status, s);
     dev_bj_invert<double>(g, c->Dg, c->Dinv64, c->status, s);
+    if (mgp) mg_setup(g, c->mg, c->Dg, c->JL, c->JR, c->Dinv64, c->status, s);   // the hierarchy's blocks (used transposed)
     HIPCHK(c, hipStreamSynchronize(s));
     auto t2 = clk::now();
     // 3. GMRES on ((P V/dt) - J)^T delta = rho
@@ -1832,7 +1835,8 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
         launch_newton_op_t(g, c->P, W_dev, c->dt, x, c->adj_v, y, ss);
     }};
     LinOp<double> M{[&](const double* x, double* y, hipStream_t ss) {
-        if (sgs) dev_bsgs_apply<double>(g, c->colors, c->Dinv64, c->JL, c->JR, x, y, o->sweeps, ss, true, 0, true);
+        if (mgp) mg_vcycle(g, c->colors, c->mg, c->Dg, c->JL, c->JR, c->Dinv64, x, y, std::max(o->sweeps, 1), ss, true);
+        else if (sgs) dev_bsgs_apply<double>(g, c->colors, c->Dinv64, c->JL, c->JR, x, y, o->sweeps, ss, true, 0, true);
         else dev_bj_apply_t(g, c->Dinv64, x, y, ss);
     }};
     rc = gmres<double>(g, A, M, c->R, c->dW, go, c->w64, gs, s);

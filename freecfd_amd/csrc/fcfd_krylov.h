@@ -33,7 +33,8 @@ template <typename T> void dev_xmy(size_t n, const T* b, T* y, hipStream_t s);
 template <typename T>
 hipError_t dev_mdot(const DevGrid& g, int k, const T* V, size_t stride, const T* w, double* part, double* out_dev,
                     double* out_host, hipStream_t s);
-template <typename T> void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s);
+template <typename T> void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s,
+                                          bool trans = false);
 // The off-diagonal blocks of the FD Jacobian re-laid for streaming (sliced
 // ELL, 64-cell slices as the wavefront): the entries of slice s are slots
 // [soff[s], soff[s+1]), 64 lanes each; slot k, lane l holds cell 64 s + l's

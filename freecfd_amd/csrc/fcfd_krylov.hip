@@ -281,7 +281,8 @@ __device__ __forceinline__ void offdiag_accumulate(const DevGrid& g, const T* __
     }
 }
 
-template <typename T, int NV>
+// y = A x, or with TRANS y = A^T x (every block transposed)
+template <typename T, int NV, bool TRANS = false>
 __global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __restrict__ D, const T* __restrict__ JL,
                                                          const T* __restrict__ JR, const T* __restrict__ x, T* __restrict__ y) {
     const int c = xcd_block() * blockDim.x + threadIdx.x;
@@ -295,10 +296,10 @@ __global__ void __launch_bounds__(256) bsr_matvec_kernel(DevGrid g, const T* __r
     for (int j = 0; j < NV; ++j) {
         T s = 0;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) s += blk[NV * j + i] * xc[i];
+        for (int i = 0; i < NV; ++i) s += (TRANS ? blk[NV * i + j] : blk[NV * j + i]) * xc[i];
         acc[j] = s;
     }
-    offdiag_accumulate<T, NV, false>(g, JL, JR, x, c, acc);
+    offdiag_accumulate<T, NV, false, TRANS>(g, JL, JR, x, c, acc);
 #pragma unroll
     for (int j = 0; j < NV; ++j) y[j * ld + c] = acc[j];
 }
@@ -579,13 +580,20 @@ template void dev_lincomb<double>(const DevGrid&, int, const double*, size_t, co
 template void dev_lincomb<float>(const DevGrid&, int, const float*, size_t, const double*, float*, hipStream_t);
 
 template <typename T>
-void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s) {
+void dev_bsr_matvec(const DevGrid& g, const T* D, const T* JL, const T* JR, const T* x, T* y, hipStream_t s, bool trans) {
+    if (trans) {   // the transposed product (the adjoint's multigrid residual)
+        if (g.nv == 6) hipLaunchKernelGGL((bsr_matvec_kernel<T, 6, true>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g, D, JL, JR, x, y);
+        else hipLaunchKernelGGL((bsr_matvec_kernel<T, 5, true>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g, D, JL, JR, x, y);
+        return;
+    }
     if (g.nv == 6) hipLaunchKernelGGL((bsr_matvec_kernel<T, 6>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g, D, JL, JR, x, y);
     else if (g.nv == 2) hipLaunchKernelGGL((bsr_matvec_kernel<T, 2>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g, D, JL, JR, x, y);
     else hipLaunchKernelGGL((bsr_matvec_kernel<T, 5>), cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g, D, JL, JR, x, y);
 }
-template void dev_bsr_matvec<double>(const DevGrid&, const double*, const double*, const double*, const double*, double*, hipStream_t);
-template void dev_bsr_matvec<float>(const DevGrid&, const float*, const float*, const float*, const float*, float*, hipStream_t);
+template void dev_bsr_matvec<double>(const DevGrid&, const double*, const double*, const double*, const double*, double*, hipStream_t,
+                                     bool);
+template void dev_bsr_matvec<float>(const DevGrid&, const float*, const float*, const float*, const float*, float*, hipStream_t,
+                                    bool);
 
 template <typename T>
 void dev_sell_pack(const DevGrid& g, const SellBlocks& sb, const T* JL, const T* JR, T* blk, hipStream_t s) {

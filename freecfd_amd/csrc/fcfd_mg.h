@@ -32,6 +32,9 @@ struct MgLevel {
     // block rows without the diagonal: columns of row I are col[rptr[I] .. rptr[I+1])
     const int *rptr = nullptr, *col = nullptr;
     std::vector<int> rptr_h;
+    // the transposed pattern: row I of A^T holds, for J in tcol[tptr[I] ..
+    // tptr[I+1]) ascending, the block A(J, I) stored at tidx (transposed on use)
+    const int *tptr = nullptr, *tcol = nullptr, *tidx = nullptr;
     // Galerkin assembly lists: diagonal block I sums dsrc[dptr[I] .. dptr[I+1]),
     // off-diagonal block k sums osrc[optr[k] .. optr[k+1]) (codes: see fcfd_mg.hip)
     const int *dptr = nullptr, *dsrc = nullptr, *optr = nullptr, *osrc = nullptr;
@@ -64,8 +67,11 @@ void mg_free(Multigrid& mg);
 // diagonal blocks of all levels (fine_Dinv: FP64 fine-level inverse)
 void mg_setup(const DevGrid& g, Multigrid& mg, const double* D, const double* JL, const double* JR, double* fine_Dinv, int* status,
               hipStream_t s);
-// x = M^{-1} b by one V-cycle (b, x: [nv][ld] FP64 fine vectors)
+// x = M^{-1} b by one V-cycle (b, x: [nv][ld] FP64 fine vectors); trans: the
+// V-cycle of the transposed operator on the same hierarchy (every level's
+// blocks transposed: the Galerkin coarse operators of A^T are the transposes
+// of A's, since restriction and prolongation are each other's transposes)
 void mg_vcycle(const DevGrid& g, const Colors& fine_colors, Multigrid& mg, const double* D, const double* JL, const double* JR,
-               const double* fine_Dinv, const double* b, double* x, int sweeps, hipStream_t s);
+               const double* fine_Dinv, const double* b, double* x, int sweeps, hipStream_t s, bool trans = false);
 
 }  // namespace fcfd

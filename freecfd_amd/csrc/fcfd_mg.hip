@@ -86,19 +86,36 @@ __global__ void __launch_bounds__(256) mg_assemble_kernel(int nblk, const int* _
     store_block<NV>(out, k, acc);
 }
 
+// block k, transposed with TRANS (the transposed operator's block)
+template <int NV, bool TRANS>
+__device__ __forceinline__ void load_block_t(const double* __restrict__ A, size_t k, double (&b)[NV * NV]) {
+    if constexpr (TRANS) {
+        double t[NV * NV];
+        load_block<NV>(A, k, t);
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) b[NV * j + i] = t[NV * i + j];
+    } else {
+        load_block<NV>(A, k, b);
+    }
+}
+
 // block Gauss-Seidel update of coarse row I: x_I = D_I^{-1} (b_I - sum of
-// the row's off-diagonal blocks times x)
-template <int NV>
+// the row's off-diagonal blocks times x).  TRANS: the transposed operator,
+// whose row I holds the blocks A(J, I)^T (rptr/col/idx: the transposed
+// pattern, idx the block of A each entry transposes) and D_I^{-T}.
+template <int NV, bool TRANS = false>
 __device__ __forceinline__ void sgs_row(int I, size_t ld, const int* __restrict__ rptr, const int* __restrict__ col,
-                                        const double* __restrict__ A, const double* __restrict__ Dinv, const double* __restrict__ b,
-                                        double* __restrict__ x) {
+                                        const int* __restrict__ idx, const double* __restrict__ A, const double* __restrict__ Dinv,
+                                        const double* __restrict__ b, double* __restrict__ x) {
     double acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; ++j) acc[j] = b[j * ld + I];
     for (int k = rptr[I]; k < rptr[I + 1]; ++k) {
         const int J = col[k];
         double xj[NV], a[NV * NV];
-        load_block<NV>(A, k, a);
+        load_block_t<NV, TRANS>(A, TRANS ? idx[k] : k, a);
 #pragma unroll
         for (int i = 0; i < NV; ++i) xj[i] = x[i * ld + J];
 #pragma unroll
@@ -107,7 +124,7 @@ __device__ __forceinline__ void sgs_row(int I, size_t ld, const int* __restrict_
             for (int i = 0; i < NV; ++i) acc[j] -= a[NV * j + i] * xj[i];
     }
     double di[NV * NV];
-    load_block<NV>(Dinv, I, di);
+    load_block_t<NV, TRANS>(Dinv, I, di);
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
         double s = 0.0;
@@ -118,14 +135,15 @@ __device__ __forceinline__ void sgs_row(int I, size_t ld, const int* __restrict_
 }
 
 // one colour of a block Gauss-Seidel sweep on a coarse level
-template <int NV>
+template <int NV, bool TRANS = false>
 __global__ void __launch_bounds__(256) mg_sgs_color_kernel(const int* __restrict__ cells, int count, size_t ld,
                                                            const int* __restrict__ rptr, const int* __restrict__ col,
-                                                           const double* __restrict__ A, const double* __restrict__ Dinv,
-                                                           const double* __restrict__ b, double* __restrict__ x) {
+                                                           const int* __restrict__ idx, const double* __restrict__ A,
+                                                           const double* __restrict__ Dinv, const double* __restrict__ b,
+                                                           double* __restrict__ x) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
-    sgs_row<NV>(cells[t], ld, rptr, col, A, Dinv, b, x);
+    sgs_row<NV, TRANS>(cells[t], ld, rptr, col, idx, A, Dinv, b, x);
 }
 
 // all the colours of `sweeps` sweeps on a level of at most kSmallLevel cells
@@ -135,10 +153,11 @@ __global__ void __launch_bounds__(256) mg_sgs_color_kernel(const int* __restrict
 // whole call (one gather round per colour fewer; the blocks, b and the
 // row structure are read from memory).  dir as in coarse_sgs.
 constexpr int kSmallLevel = 1024;
-template <int NV>
+template <int NV, bool TRANS = false>
 __global__ void __launch_bounds__(kSmallLevel) mg_sgs_level_kernel(int n, int ncol, const int* __restrict__ cptr,
                                                                    const int* __restrict__ cells, size_t ld,
                                                                    const int* __restrict__ rptr, const int* __restrict__ col,
+                                                                   const int* __restrict__ idx,
                                                                    const double* __restrict__ A, const double* __restrict__ Dinv,
                                                                    const double* __restrict__ b, double* __restrict__ x, int sweeps,
                                                                    int dir) {
@@ -160,7 +179,7 @@ __global__ void __launch_bounds__(kSmallLevel) mg_sgs_level_kernel(int n, int nc
                     for (int e = rptr[I]; e < rptr[I + 1]; ++e) {
                         const int J = col[e];
                         double xj[NV], a[NV * NV];
-                        load_block<NV>(A, e, a);
+                        load_block_t<NV, TRANS>(A, TRANS ? idx[e] : e, a);
 #pragma unroll
                         for (int i = 0; i < NV; ++i) xj[i] = xs[i * kSmallLevel + J];
 #pragma unroll
@@ -169,7 +188,7 @@ __global__ void __launch_bounds__(kSmallLevel) mg_sgs_level_kernel(int n, int nc
                             for (int i = 0; i < NV; ++i) acc[j] -= a[NV * j + i] * xj[i];
                     }
                     double di[NV * NV];
-                    load_block<NV>(Dinv, I, di);
+                    load_block_t<NV, TRANS>(Dinv, I, di);
 #pragma unroll
                     for (int j = 0; j < NV; ++j) {
                         double s = 0.0;
@@ -188,16 +207,17 @@ __global__ void __launch_bounds__(kSmallLevel) mg_sgs_level_kernel(int n, int nc
 }
 
 // r = b - A x on a coarse level
-template <int NV>
+template <int NV, bool TRANS = false>
 __global__ void __launch_bounds__(256) mg_residual_kernel(int n, size_t ld, const int* __restrict__ rptr,
-                                                          const int* __restrict__ col, const double* __restrict__ D,
-                                                          const double* __restrict__ A, const double* __restrict__ b,
-                                                          const double* __restrict__ x, double* __restrict__ r) {
+                                                          const int* __restrict__ col, const int* __restrict__ idx,
+                                                          const double* __restrict__ D, const double* __restrict__ A,
+                                                          const double* __restrict__ b, const double* __restrict__ x,
+                                                          double* __restrict__ r) {
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= n) return;
     double acc[NV], xi[NV];
     double d[NV * NV];
-    load_block<NV>(D, I, d);
+    load_block_t<NV, TRANS>(D, I, d);
 #pragma unroll
     for (int j = 0; j < NV; ++j) { acc[j] = b[j * ld + I]; xi[j] = x[j * ld + I]; }
 #pragma unroll
@@ -207,7 +227,7 @@ __global__ void __launch_bounds__(256) mg_residual_kernel(int n, size_t ld, cons
     for (int k = rptr[I]; k < rptr[I + 1]; ++k) {
         const int J = col[k];
         double xj[NV], a[NV * NV];
-        load_block<NV>(A, k, a);
+        load_block_t<NV, TRANS>(A, TRANS ? idx[k] : k, a);
 #pragma unroll
         for (int i = 0; i < NV; ++i) xj[i] = x[i * ld + J];
 #pragma unroll
@@ -449,6 +469,21 @@ int mg_build(const DevGrid& g, const std::vector<int>& par, const std::vector<in
         int ncol = 0;
         color_graph(nc, rptr, col, cptr, cells, ncol);
         L.rptr_h = rptr;
+        // transposed pattern: entry (J, k) of row I for each block k = A(J, I),
+        // rows filled in ascending J
+        std::vector<int> tptr(nc + 1, 0), tcol(L.nnz), tidx(L.nnz);
+        for (int k = 0; k < L.nnz; ++k) tptr[col[k] + 1]++;
+        for (int I = 0; I < nc; ++I) tptr[I + 1] += tptr[I];
+        {
+            std::vector<int> next(tptr.begin(), tptr.end() - 1);
+            for (int J = 0; J < nc; ++J)
+                for (int k = rptr[J]; k < rptr[J + 1]; ++k) {
+                    const int q = next[col[k]]++;
+                    tcol[q] = J;
+                    tidx[q] = k;
+                }
+        }
+        if ((rc = mg_upload(mg, tptr, &L.tptr)) || (rc = mg_upload(mg, tcol, &L.tcol)) || (rc = mg_upload(mg, tidx, &L.tidx))) return rc;
         if ((rc = mg_upload(mg, agg, &L.agg)) || (rc = mg_upload(mg, mptr, &L.mptr)) || (rc = mg_upload(mg, mem, &L.mem)) ||
             (rc = mg_upload(mg, rptr, &L.rptr)) || (rc = mg_upload(mg, col, &L.col)) || (rc = mg_upload(mg, dptr, &L.dptr)) ||
             (rc = mg_upload(mg, dsrc, &L.dsrc)) || (rc = mg_upload(mg, optr, &L.optr)) || (rc = mg_upload(mg, osrc, &L.osrc)) ||
@@ -519,18 +554,21 @@ void mg_setup(const DevGrid& g, Multigrid& mg, const double* D, const double* JL
     else setup_t<5>(g, mg, D, JL, JR, status, s);
 }
 
-template <int NV>
+template <int NV, bool TRANS>
 static void coarse_sgs(const MgLevel& L, int sweeps, hipStream_t s, int dir) {
+    const int* rp = TRANS ? L.tptr : L.rptr;
+    const int* cl = TRANS ? L.tcol : L.col;
+    const int* ix = TRANS ? L.tidx : nullptr;
     if (L.one_launch) {   // small level: every colour in one workgroup launch
-        hipLaunchKernelGGL(mg_sgs_level_kernel<NV>, dim3(1), dim3(kSmallLevel), 0, s, L.n, L.colors.n, L.cptr_dev, L.colors.cells,
-                           (size_t)L.ld, L.rptr, L.col, L.A, L.Dinv, L.b, L.x, sweeps, dir);
+        hipLaunchKernelGGL((mg_sgs_level_kernel<NV, TRANS>), dim3(1), dim3(kSmallLevel), 0, s, L.n, L.colors.n, L.cptr_dev,
+                           L.colors.cells, (size_t)L.ld, rp, cl, ix, L.A, L.Dinv, L.b, L.x, sweeps, dir);
         return;
     }
     auto one = [&](int c) {
         const int cnt = L.colors.ptr[c + 1] - L.colors.ptr[c];
         if (cnt <= 0) return;
-        hipLaunchKernelGGL(mg_sgs_color_kernel<NV>, cell_grid(cnt), dim3(cell_threads(cnt)), 0, s, L.colors.cells + L.colors.ptr[c], cnt,
-                           (size_t)L.ld, L.rptr, L.col, L.A, L.Dinv, L.b, L.x);
+        hipLaunchKernelGGL((mg_sgs_color_kernel<NV, TRANS>), cell_grid(cnt), dim3(cell_threads(cnt)), 0, s,
+                           L.colors.cells + L.colors.ptr[c], cnt, (size_t)L.ld, rp, cl, ix, L.A, L.Dinv, L.b, L.x);
     };
     for (int sw = 0; sw < sweeps; ++sw) {
         if (dir != 2)
@@ -540,46 +578,49 @@ static void coarse_sgs(const MgLevel& L, int sweeps, hipStream_t s, int dir) {
     }
 }
 
-template <int NV>
+template <int NV, bool TRANS>
 static void level_cycle(Multigrid& mg, size_t l, int sweeps, hipStream_t s) {
     MgLevel& L = mg.lv[l];
     const size_t nvec = (size_t)NV * L.ld;
     hipLaunchKernelGGL(mg_zero_kernel, dim3((unsigned)std::min<size_t>((nvec + 255) / 256, 4096)), dim3(256), 0, s, nvec, L.x);
     if (l + 1 == mg.lv.size()) {   // coarsest: smoothing only
-        coarse_sgs<NV>(L, 2 * sweeps, s, 0);
+        coarse_sgs<NV, TRANS>(L, 2 * sweeps, s, 0);
         return;
     }
-    coarse_sgs<NV>(L, sweeps, s, 1);   // pre-smoothing: forward sweeps
-    hipLaunchKernelGGL(mg_residual_kernel<NV>, cell_grid(L.n), dim3(cell_threads(L.n)), 0, s, L.n, (size_t)L.ld, L.rptr, L.col, L.D, L.A,
-                       L.b, L.x, L.r);
+    coarse_sgs<NV, TRANS>(L, sweeps, s, 1);   // pre-smoothing: forward sweeps
+    hipLaunchKernelGGL((mg_residual_kernel<NV, TRANS>), cell_grid(L.n), dim3(cell_threads(L.n)), 0, s, L.n, (size_t)L.ld,
+                       TRANS ? L.tptr : L.rptr, TRANS ? L.tcol : L.col, TRANS ? L.tidx : nullptr, L.D, L.A, L.b, L.x, L.r);
     MgLevel& C = mg.lv[l + 1];
     hipLaunchKernelGGL(mg_restrict_kernel<NV>, cell_grid(C.n), dim3(cell_threads(C.n)), 0, s, C.n, C.mptr, C.mem, L.r, (size_t)L.ld, C.b,
                        (size_t)C.ld);
-    level_cycle<NV>(mg, l + 1, sweeps, s);
+    level_cycle<NV, TRANS>(mg, l + 1, sweeps, s);
     hipLaunchKernelGGL(mg_prolong_kernel<NV>, cell_grid(L.n), dim3(cell_threads(L.n)), 0, s, L.n, C.agg, C.x, (size_t)C.ld, L.x,
                        (size_t)L.ld);
-    coarse_sgs<NV>(L, sweeps, s, 2);   // post-smoothing: backward sweeps
+    coarse_sgs<NV, TRANS>(L, sweeps, s, 2);   // post-smoothing: backward sweeps
 }
 
-template <int NV>
+template <int NV, bool TRANS>
 static void vcycle_t(const DevGrid& g, const Colors& fc, Multigrid& mg, const double* D, const double* JL, const double* JR,
                      const double* Dinv, const double* b, double* x, int sweeps, hipStream_t s) {
     const size_t n5 = (size_t)NV * g.ld;
-    dev_bsgs_apply<double>(g, fc, Dinv, JL, JR, b, x, sweeps, s, true, 1);   // forward pre-smoothing
-    dev_bsr_matvec<double>(g, D, JL, JR, x, mg.fine_r, s);
+    dev_bsgs_apply<double>(g, fc, Dinv, JL, JR, b, x, sweeps, s, true, 1, TRANS);   // forward pre-smoothing
+    dev_bsr_matvec<double>(g, D, JL, JR, x, mg.fine_r, s, TRANS);
     dev_xmy<double>(n5, b, mg.fine_r, s);   // r = b - A x
     MgLevel& C = mg.lv[1];
-    hipLaunchKernelGGL(mg_restrict_kernel<NV>, cell_grid(C.n), dim3(cell_threads(C.n)), 0, s, C.n, C.mptr, C.mem, mg.fine_r, (size_t)g.ld,
-                       C.b, (size_t)C.ld);
-    level_cycle<NV>(mg, 1, sweeps, s);
-    hipLaunchKernelGGL(mg_prolong_kernel<NV>, cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g.N, C.agg, C.x, (size_t)C.ld, x, (size_t)g.ld);
-    dev_bsgs_apply<double>(g, fc, Dinv, JL, JR, b, x, sweeps, s, false, 2);   // backward post-smoothing
+    hipLaunchKernelGGL(mg_restrict_kernel<NV>, cell_grid(C.n), dim3(cell_threads(C.n)), 0, s, C.n, C.mptr, C.mem, mg.fine_r,
+                       (size_t)g.ld, C.b, (size_t)C.ld);
+    level_cycle<NV, TRANS>(mg, 1, sweeps, s);
+    hipLaunchKernelGGL(mg_prolong_kernel<NV>, cell_grid(g.N), dim3(cell_threads(g.N)), 0, s, g.N, C.agg, C.x, (size_t)C.ld, x,
+                       (size_t)g.ld);
+    dev_bsgs_apply<double>(g, fc, Dinv, JL, JR, b, x, sweeps, s, false, 2, TRANS);   // backward post-smoothing
 }
 
 void mg_vcycle(const DevGrid& g, const Colors& fine_colors, Multigrid& mg, const double* D, const double* JL, const double* JR,
-               const double* fine_Dinv, const double* b, double* x, int sweeps, hipStream_t s) {
-    if (g.nv == 6) vcycle_t<6>(g, fine_colors, mg, D, JL, JR, fine_Dinv, b, x, sweeps, s);
-    else vcycle_t<5>(g, fine_colors, mg, D, JL, JR, fine_Dinv, b, x, sweeps, s);
+               const double* fine_Dinv, const double* b, double* x, int sweeps, hipStream_t s, bool trans) {
+    if (g.nv == 6 && trans) vcycle_t<6, true>(g, fine_colors, mg, D, JL, JR, fine_Dinv, b, x, sweeps, s);
+    else if (g.nv == 6) vcycle_t<6, false>(g, fine_colors, mg, D, JL, JR, fine_Dinv, b, x, sweeps, s);
+    else if (trans) vcycle_t<5, true>(g, fine_colors, mg, D, JL, JR, fine_Dinv, b, x, sweeps, s);
+    else vcycle_t<5, false>(g, fine_colors, mg, D, JL, JR, fine_Dinv, b, x, sweeps, s);
 }
 
 }  // namespace fcfd

@@ -520,9 +520,10 @@ int  fcfd_convective_flux(fcfd_ctx* ctx, int flux, int n, const double* WL_dev, 
  * Rbar = 0): with rho = (dR/dW)^T lam + g, solves ((P V/dt) - dR/dW)^T delta
  * = rho by FP64 GMRES with the exact transpose (fcfd_vjp) as the operator and,
  * as the preconditioner, the transposed FD block Jacobi (FCFD_PC_BLOCK_JACOBI)
- * or `sweeps` symmetric multicolour block Gauss-Seidel sweeps on the
- * transposed FD operator (FCFD_PC_BLOCK_SGS) of the same step, then
- * lam += delta.  As CFL grows it tends to the Newton solve of the adjoint
+ * `sweeps` symmetric multicolour block Gauss-Seidel sweeps on the
+ * transposed FD operator (FCFD_PC_BLOCK_SGS), or one V-cycle of the
+ * aggregation multigrid on the transposed FD operator (FCFD_PC_MULTIGRID:
+ * every level's blocks transposed) of the same step, then lam += delta.  As CFL grows it tends to the Newton solve of the adjoint
  * system, which is linear: lam converges at the rate of the Krylov solves.
  * opts: cfl, restart, max_iters, rtol, abstol, knoll_guess, precond, sweeps;
  * krylov_fp32 must be 0.  stats->res_norm = ||rho||_2 before

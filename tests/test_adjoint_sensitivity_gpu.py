@@ -77,18 +77,19 @@ def test_adjoint_converges_and_matches_fd(converged):
         assert rel < 1e-4
 
 
-def test_adjoint_step_block_sgs_matches_block_jacobi(converged):
+def test_adjoint_step_block_sgs_and_multigrid_match_block_jacobi(converged):
     """One adjoint step from lam = 0 preconditioned by the transposed block SGS
-    sweep reaches the same update as with the transposed block Jacobi (the
-    preconditioner changes the Krylov iterations, not the step's linear
-    system), in fewer iterations: a sweep over the wrong transpose would still
-    converge, but no faster than block Jacobi."""
+    sweep, and by the transposed multigrid V-cycle, reaches the same update as
+    with the transposed block Jacobi (the preconditioner changes the Krylov
+    iterations, not the step's linear system), in fewer iterations: a sweep or
+    cycle over the wrong transpose would still converge, but no faster than
+    block Jacobi."""
     ns, cs, q, params, vals = converged
     torch = ns.torch
     area = S.wall_area(ns, WALL)
     g = ns.vjp(torch.zeros_like(ns.W), fx_weight=S.drag_weight(int(ns._opt.n_bc), WALL, q, area)).clone()
     out = {}
-    for pc in (A.PC_BLOCK_JACOBI, A.PC_BLOCK_SGS):
+    for pc in (A.PC_BLOCK_JACOBI, A.PC_BLOCK_SGS, A.PC_MULTIGRID):
         lam = torch.zeros_like(ns.W)
         opts = A.newton_opts(cfl=20.0, jacobian=A.JAC_FD, fp32=False, restart=120, max_iters=480, rtol=1e-12, abstol=0.0,
                              knoll=False, precond=pc, sweeps=1)
@@ -98,11 +99,13 @@ def test_adjoint_step_block_sgs_matches_block_jacobi(converged):
                                                                                  st.krylov_rnorm, st.krylov_r0norm), flush=True)
     lj, ij, cj = out[A.PC_BLOCK_JACOBI]
     ls, isgs, csgs = out[A.PC_BLOCK_SGS]
+    lm, img, cmg = out[A.PC_MULTIGRID]
     err = float((ls - lj).abs().max() / lj.abs().max())
-    print("  max |lam_SGS - lam_BJ| / max |lam_BJ| = %.3e" % err, flush=True)
-    assert cj and csgs
-    assert err < 1e-6
-    assert isgs < ij
+    errm = float((lm - lj).abs().max() / lj.abs().max())
+    print("  max |lam_SGS - lam_BJ| / max |lam_BJ| = %.3e, multigrid %.3e" % (err, errm), flush=True)
+    assert cj and csgs and cmg
+    assert err < 1e-6 and errm < 1e-6
+    assert isgs < ij and img < ij
 
 
 def test_set_bc_roundtrip(converged):
