@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(256) cell_adj_kernel(DevGrid g, Params P, cons
             gb[3 * v + 1] += s * c2f.y;
             gb[3 * v + 2] += s * c2f.z;
         }
-        if (VISC) {   // face gradient = w_P grad_P + w_N grad_N (interior), grad_P (boundary)
+        if (VISC && !(g.wslot && g.wslot[f] >= 0)) {   // face gradient = w_P grad_P + w_N grad_N (interior), grad_P (boundary)
             const double2 wpn = face_pair(g, 5, f);
             const double w = g.fbc[f] >= 0 ? 1.0 : (left ? wpn.x : wpn.y);
 #pragma unroll
@@ -183,6 +183,14 @@ __global__ void __launch_bounds__(256) cell_adj_kernel(DevGrid g, Params P, cons
 #pragma unroll
                 for (int m = 0; m < 3; ++m) gb[15 + m] += w * Z[(Zr::G + 12 + m) * Fs + f];
             }
+        }
+    }
+    if (VISC && g.wtptr) {   // wide stencils: this cell's weight in each of their face gradients
+        for (int e = g.wtptr[c]; e < g.wtptr[c + 1]; ++e) {
+            const int f = g.wtface[e];
+            const double w = g.wtw[e];
+#pragma unroll
+            for (int m = 0; m < Zr::ngrad; ++m) gb[3 + m] += w * Z[(Zr::G + m) * Fs + f];
         }
     }
     if constexpr (NV == 6) {   // SA source V S(p, T, nu~, grad u, grad v, grad w, grad nu~)

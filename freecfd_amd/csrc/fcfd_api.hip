@@ -386,6 +386,7 @@ int build_device(fcfd_ctx* c) {
         }
         d.wslot = nullptr; d.nwide = 0; d.wptr = nullptr; d.widx = nullptr; d.ww = nullptr;
         d.wgrad = nullptr; d.wgrad_d = nullptr; d.wgrad_f = nullptr;
+        d.wtptr = nullptr; d.wtface = nullptr; d.wtw = nullptr;
         const int nwide = (int)wptr.size() - 1;
         if (nwide > 0 && c->P.viscous) {   // only the viscous face gradients read the face averages
             if (c->P.kw)
@@ -397,6 +398,25 @@ int build_device(fcfd_ctx* c) {
                 (rc = dalloc(c, rows * nwide, &d.wgrad_f)))
                 return rc;
             d.nwide = nwide;
+            // transposed: per interior cell, its wide faces in ascending internal
+            // face order.  A stencil may hold a boundary ghost, whose gradient is
+            // its parent's copy: that entry belongs to the parent.
+            auto owner = [&](int j) { return j < N ? j : g.perm[g.fparent[g.ghost_face[g.iperm[j] - N]]]; };
+            std::vector<int> wtptr(N + 1, 0), wtface(widx.size());
+            std::vector<double> wtw(widx.size());
+            for (size_t k = 0; k < widx.size(); ++k) wtptr[owner(widx[k]) + 1]++;
+            for (int i = 0; i < N; ++i) wtptr[i + 1] += wtptr[i];
+            std::vector<int> next(wtptr.begin(), wtptr.end() - 1);
+            for (int i = 0; i < F; ++i) {
+                const int sl = wslot[i];
+                if (sl < 0) continue;
+                for (int k = wptr[sl]; k < wptr[sl + 1]; ++k) {
+                    const int q = next[owner(widx[k])]++;
+                    wtface[q] = i;
+                    wtw[q] = ww[k];
+                }
+            }
+            if ((rc = upload(c, wtptr, &d.wtptr)) || (rc = upload(c, wtface, &d.wtface)) || (rc = upload(c, wtw, &d.wtw))) return rc;
         }
         std::vector<int> gh(B);
         for (int i = 0; i < B; ++i) gh[i] = g.fperm[g.ghost_face[g.iperm[N + i] - N]];
@@ -866,12 +886,11 @@ int fcfd_jacobian_to_host(fcfd_ctx* c, double* JL_host, double* JR_host, double*
     return FCFD_OK;
 }
 
-// The adjoint kernels transpose the gradient-map gather and the two-cell
-// face average of the viscous face gradients; Green-Gauss cells (no map) and
-// wider face-average stencils are not transposed there, so fcfd_vjp and
-// fcfd_adjoint_step refuse a context that has them.
+// The adjoint kernels transpose the gradient-map gather and the face average
+// of the viscous face gradients (two-cell or wider); Green-Gauss cells (no
+// map) are not transposed there, so fcfd_vjp and fcfd_adjoint_step refuse a
+// context that has them.
 static int refuse_green_gauss(fcfd_ctx* c) {
-    if (c->dg.wslot) return fail(c, FCFD_ENOTSUP, "the adjoint path supports two-cell face-average stencils only");
     return c->dg.ggc ? fail(c, FCFD_ENOTSUP, "the adjoint path does not support Green-Gauss gradients") : FCFD_OK;
 }
 

@@ -61,6 +61,12 @@ struct DevGrid {
     double* wgrad;
     double* wgrad_d;
     float* wgrad_f;
+    // the same stencils transposed, for the adjoint: interior cell c appears
+    // in the wide faces wtface[wtptr[c] .. wtptr[c+1]) (internal ids, ascending)
+    // with the weights wtw
+    const int* wtptr;
+    const int* wtface;
+    const double* wtw;
 };
 
 // Gradients ([3 nv] components per cell), limiters ([nv]) and the FD

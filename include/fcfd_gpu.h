@@ -99,9 +99,10 @@ extern "C" {
 /* interpolation (src/interpolate/interpolate.h).  A stencil above 2 (WTLI
  * fits, or IDW over the extended stencil) gives faces whose viscous face
  * gradients average more than their two cells: residual, FD Jacobian,
- * tangent and both Newton modes; fcfd_vjp, fcfd_vjp_params and
- * fcfd_adjoint_step return FCFD_ENOTSUP for such a viscous context, and
- * fcfd_create with the segregated two-equation model */
+ * tangent, both Newton modes and the adjoint (fcfd_vjp transposes the
+ * average; a ghost in a stencil passes its weight to its parent, whose
+ * gradient it copies).  fcfd_create returns FCFD_ENOTSUP for such a viscous
+ * context with the segregated two-equation model */
 #define FCFD_INTERP_WTLI   1
 #define FCFD_INTERP_IDW    2
 #define FCFD_INTERP_SIMPLE 3

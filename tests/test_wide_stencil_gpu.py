@@ -9,8 +9,7 @@ the stencil's cell gradients in the reference's key order
   fits) and the SA plate (IDW 8, hexes and prisms);
 - the FD Jacobian against the oracle (1e-6 per block: Sutherland's pow);
 - the exact tangent against central differences of the GPU residual;
-- the two-cell stencil gives the same residual as before the wider ones were
-  supported (the wide path is not taken), and the adjoint refuses them.
+- the adjoint (dot-product test against the tangent).
 """
 import numpy as np
 import pytest
@@ -109,13 +108,25 @@ def test_tangent_with_wide_stencils_vs_central_differences(name, make):
     ns.close()
 
 
-def test_adjoint_refuses_wide_stencils():
+@pytest.mark.parametrize("name,make", CASES)
+def test_adjoint_with_wide_stencils_dot_product(name, make):
+    """The adjoint transposes the wide face averages (each stencil cell takes
+    its weight of the face-gradient adjoint): <v, J u> = <J^T v, u> to 1e-11
+    of sum |v J u|, with u, v random and J the exact tangent."""
     from freecfd_amd.ns import NavierStokes
-    cs = CASES[1][1]()
+    cs = make()
     ns = NavierStokes(cs.raw, cs.opt)
     ns.set_state(cs.W0)
-    with pytest.raises(RuntimeError, match="two-cell"):
-        ns.vjp(ns.torch.zeros_like(ns.W))
+    N, nv = cs.n_cells, ns.nv
+    rng = np.random.default_rng(11)
+    u = rng.normal(size=(N, nv)) * np.array([1e5, 100.0, 100.0, 100.0, 300.0, 1e-4])[:nv]
+    v = rng.normal(size=(N, nv))
+    Ju = ns.to_host(ns.jvp(ns.to_device(u), mode=A.JAC_JVP_FULL), nv)
+    JTv = ns.to_host(ns.vjp(ns.to_device(v)), nv)
+    lhs, rhs = float(np.sum(v * Ju)), float(np.sum(JTv * u))
+    rel = abs(lhs - rhs) / float(np.sum(np.abs(v * Ju)))
+    print("case", name, "<v,Ju> %.15e <J^T v,u> %.15e rel %.2e" % (lhs, rhs, rel), flush=True)
+    assert rel < 1e-11
     ns.close()
 
 
