@@ -11,6 +11,7 @@ struct AdjGrid {
     const int* tg_ptr;    // N+B+1: transposed gradient map, by stencil member j ...
     const int* tg_cell;   // ... the cells c whose gradient reads j, ascending
     const double *tgx, *tgy, *tgz;
+    const int* tgm;       // per entry, the variables a BC holds fixed on that face (bit v); null without Green-Gauss cells
     // surface loads: per BC, the boundary faces in reference face order
     const int* lf_ptr;    // n_bc+1
     const int* lf_face;   // internal face ids

@@ -275,8 +275,10 @@ __global__ void __launch_bounds__(256) gradt_kernel(DevGrid g, AdjGrid a, int ha
     for (int e = a.tg_ptr[j]; e < a.tg_ptr[j + 1]; ++e) {
         const int c = a.tg_cell[e];
         const double wx = a.tgx[e], wy = a.tgy[e], wz = a.tgz[e];
+        const int fixed = a.tgm ? a.tgm[e] : 0;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] += wx * Gb[(3 * v) * ld + c] + wy * Gb[(3 * v + 1) * ld + c] + wz * Gb[(3 * v + 2) * ld + c];
+        for (int v = 0; v < NV; ++v)
+            if (!((fixed >> v) & 1)) acc[v] += wx * Gb[(3 * v) * ld + c] + wy * Gb[(3 * v + 1) * ld + c] + wz * Gb[(3 * v + 2) * ld + c];
     }
     if (has_lim) {
         if (j < g.N) {

@@ -886,14 +886,6 @@ int fcfd_jacobian_to_host(fcfd_ctx* c, double* JL_host, double* JR_host, double*
     return FCFD_OK;
 }
 
-// The adjoint kernels transpose the gradient-map gather and the face average
-// of the viscous face gradients (two-cell or wider); Green-Gauss cells (no
-// map) are not transposed there, so fcfd_vjp and fcfd_adjoint_step refuse a
-// context that has them.
-static int refuse_green_gauss(fcfd_ctx* c) {
-    return c->dg.ggc ? fail(c, FCFD_ENOTSUP, "the adjoint path does not support Green-Gauss gradients") : FCFD_OK;
-}
-
 static int ensure_tangent(fcfd_ctx* c) {
     if (c->dG) return FCFD_OK;
     const DevGrid& g = c->dg;
@@ -999,24 +991,45 @@ static int ensure_adjoint(fcfd_ctx* c) {
         if ((rc = upload(c, twin, &c->ag.hf_twin)) || (rc = upload(c, ghh, &c->ag.gh_hf))) return rc;
     }
     {
-        std::vector<int> tp(T + 1, 0);
-        for (int i = 0; i < N; ++i) {
-            int r = g.iperm[i];
-            for (int e = g.gptr[r]; e < g.gptr[r + 1]; ++e) tp[g.perm[g.gidx[e]] + 1]++;
-        }
-        for (int j = 0; j < T; ++j) tp[j + 1] += tp[j];
-        std::vector<int> next(tp.begin(), tp.end() - 1), tc(tp[T]);
-        std::vector<double> wx(tp[T]), wy(tp[T]), wz(tp[T]);
-        for (int i = 0; i < N; ++i) {   // ascending internal cell: each list ends up in ascending c
-            int r = g.iperm[i];
-            for (int e = g.gptr[r]; e < g.gptr[r + 1]; ++e) {
-                int pos = next[g.perm[g.gidx[e]]]++;
-                tc[pos] = i;
-                wx[pos] = g.gw[3 * e]; wy[pos] = g.gw[3 * e + 1]; wz[pos] = g.gw[3 * e + 2];
+        // Transposed gradient stencils.  A mapped cell's gradient is sum_k w_k W_k
+        // over its map; a Green-Gauss cell's is sum_faces (n A / V) times the
+        // face value, w_p W_parent + w_n W_neighbour (or ghost), except for the
+        // variables the face's BC holds fixed, whose face value is a constant.
+        // Each cell visits its entries (j, 3 weights, fixed-variable bits).
+        const bool gg = d.ggc != nullptr;
+        auto visit = [&](int i, auto&& emit) {
+            const int r = g.iperm[i];
+            for (int e = g.gptr[r]; e < g.gptr[r + 1]; ++e) emit(g.perm[g.gidx[e]], g.gw[3 * e], g.gw[3 * e + 1], g.gw[3 * e + 2], 0);
+            if (!gg || g.gptr[r + 1] != g.gptr[r]) return;
+            for (int k = g.cfptr[r]; k < g.cfptr[r + 1]; ++k) {
+                const int f = g.cfaces[k];
+                const double sgn = g.fparent[f] == r ? 1.0 : -1.0, s = sgn * g.farea[f] / g.vol[r];
+                const double ax = g.fn[3 * f] * s, ay = g.fn[3 * f + 1] * s, az = g.fn[3 * f + 2] * s;
+                int fixed = 0;
+                if (g.fbc[f] >= 0) {
+                    const BcDev& b = c->bch[g.fbc[f]];
+                    fixed = (b.p_fixed ? 1 : 0) | (b.V_fixed ? 14 : 0) | (b.T_fixed ? 16 : 0);
+                }
+                const double wp = g.aw[g.aptr[f]], wn = g.aw[g.aptr[f] + 1];
+                emit(g.perm[g.fparent[f]], ax * wp, ay * wp, az * wp, fixed);
+                emit(g.perm[g.fneigh[f]], ax * wn, ay * wn, az * wn, fixed);
             }
-        }
+        };
+        std::vector<int> tp(T + 1, 0);
+        for (int i = 0; i < N; ++i) visit(i, [&](int j, double, double, double, int) { tp[j + 1]++; });
+        for (int j = 0; j < T; ++j) tp[j + 1] += tp[j];
+        std::vector<int> next(tp.begin(), tp.end() - 1), tc(tp[T]), tm(gg ? tp[T] : 0);
+        std::vector<double> wx(tp[T]), wy(tp[T]), wz(tp[T]);
+        for (int i = 0; i < N; ++i)   // ascending internal cell: each list ends up in ascending c
+            visit(i, [&](int j, double x, double y, double z, int fixed) {
+                const int pos = next[j]++;
+                tc[pos] = i;
+                wx[pos] = x; wy[pos] = y; wz[pos] = z;
+                if (gg) tm[pos] = fixed;
+            });
+        c->ag.tgm = nullptr;
         if ((rc = upload(c, tp, &c->ag.tg_ptr)) || (rc = upload(c, tc, &c->ag.tg_cell)) || (rc = upload(c, wx, &c->ag.tgx)) ||
-            (rc = upload(c, wy, &c->ag.tgy)) || (rc = upload(c, wz, &c->ag.tgz)))
+            (rc = upload(c, wy, &c->ag.tgy)) || (rc = upload(c, wz, &c->ag.tgz)) || (gg && (rc = upload(c, tm, &c->ag.tgm))))
             return rc;
     }
     {
@@ -1048,8 +1061,7 @@ static int ensure_adjoint(fcfd_ctx* c) {
 int fcfd_vjp(fcfd_ctx* c, double* W_dev, const double* Rbar_dev, const double* fx_weight_dev, double* Wbar_dev, void* stream) {
     if (!c || !W_dev || !Rbar_dev || !Wbar_dev) return FCFD_EINVAL;
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
-    int rc = refuse_green_gauss(c);
-    if (!rc) rc = ensure_adjoint(c);
+    int rc = ensure_adjoint(c);
     if (rc) return rc;
     launch_vjp(c->dg, c->ag, c->P, W_dev, c->G, c->PHI, c->dref, Rbar_dev, fx_weight_dev, c->aw, Wbar_dev, c->status,
                (hipStream_t)stream);
@@ -1131,7 +1143,6 @@ static BcSeed make_seed(int index, const fcfd_bc& d) {
 
 static int param_tangent_prepare(fcfd_ctx* c) {
     if (!c->device) return fail(c, FCFD_EINVAL, "context has no device data");
-    if (c->dg.ggc) return fail(c, FCFD_ENOTSUP, "BC-value derivatives do not support Green-Gauss gradients");
     int rc = ensure_tangent(c);
     if (!rc) rc = ensure_adjoint(c);   // the load faces and the reduction buffers
     return rc;
@@ -1809,8 +1820,7 @@ int fcfd_adjoint_step(fcfd_ctx* c, double* W_dev, const double* g_dev, double* l
     if (o->precond == FCFD_PC_BLOCK_SGS && o->sweeps < 1) return fail(c, FCFD_EINVAL, "block SGS needs sweeps >= 1");
     if (o->restart < 1 || o->restart > 500 || o->max_iters < 0) return fail(c, FCFD_EINVAL, "bad Krylov options");
     const bool sgs = o->precond == FCFD_PC_BLOCK_SGS, mgp = o->precond == FCFD_PC_MULTIGRID;
-    int rc = refuse_green_gauss(c);
-    if (!rc) rc = ensure_adjoint(c);
+    int rc = ensure_adjoint(c);
     if (!rc) rc = ensure_newton(c, o);
     if (!rc && (sgs || mgp)) rc = ensure_colors(c);
     if (!rc && mgp) rc = ensure_mg(c);

@@ -174,7 +174,7 @@ void launch_dt_local(const DevGrid& g, const Params& P, const double* W, double 
 // Green-Gauss gradients of the cells without a gradient map into G (row
 // pairs); tangent = 1 for a tangent state (BC-fixed face values give 0).
 // No-op when every cell has a map.
-void launch_green_gauss(const DevGrid& g, const double* W, double* G, int tangent, hipStream_t s);
+void launch_green_gauss(const DevGrid& g, const double* W, double* G, int tangent, hipStream_t s, const BcSeed* seed = nullptr);
 // dt of every cell for FCFD_DT_FIXED (value = the step size), FCFD_DT_CFL_MAX
 // (the minimum over the cells of value L / (|V| + a), kept in red[0]) or
 // FCFD_DT_CFL_LOCAL (value = the CFL number); red holds 2 + 256 doubles

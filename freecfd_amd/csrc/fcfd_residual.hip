@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256) ghost_kernel(DevGrid g, Params P, double*
 // same kernel on a tangent dW gives the tangent gradient (a fixed BC value
 // contributes 0).
 template <int NV>
-__global__ void __launch_bounds__(256) gg_kernel(DevGrid g, const double* __restrict__ W, double* __restrict__ G, int tangent) {
+__global__ void __launch_bounds__(256) gg_kernel(DevGrid g, const double* __restrict__ W, double* __restrict__ G, int tangent, BcSeed seed) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= g.N || !g.ggc[c]) return;
     const size_t ld = g.ld;
@@ -103,7 +103,9 @@ __global__ void __launch_bounds__(256) gg_kernel(DevGrid g, const double* __rest
                 if (v == 0 && bc.p_fixed) { fv = bc.p; fixed = true; }
                 if (v == 4 && bc.T_fixed) { fv = bc.T; fixed = true; }
                 if (v >= 1 && v <= 3 && bc.V_fixed) { fv = bc.V[v - 1]; fixed = true; }
-                if (fixed && tangent) fv = 0.0;
+                // the tangent of a fixed face value: 0, or the seeded BC value's direction
+                if (fixed && tangent)
+                    fv = fc.z != seed.index ? 0.0 : v == 0 ? seed.p : v == 4 ? seed.T : seed.V[v - 1];
             }
             if (!fixed) {
                 double t = 0.;
@@ -638,11 +640,13 @@ void launch_vk_range(const DevGrid& g, const Params& P, const double* W, double*
     else launch_vk_range_t<5>(g, W, dref, s);
 }
 
-void launch_green_gauss(const DevGrid& g, const double* W, double* G, int tangent, hipStream_t s) {
+void launch_green_gauss(const DevGrid& g, const double* W, double* G, int tangent, hipStream_t s, const BcSeed* seed) {
     if (!g.ggc) return;
+    BcSeed none{};
+    none.index = -1;
     const dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
-    if (g.nv == 6) hipLaunchKernelGGL(gg_kernel<6>, grid, blk, 0, s, g, W, G, tangent);
-    else hipLaunchKernelGGL(gg_kernel<5>, grid, blk, 0, s, g, W, G, tangent);
+    if (g.nv == 6) hipLaunchKernelGGL(gg_kernel<6>, grid, blk, 0, s, g, W, G, tangent, seed ? *seed : none);
+    else hipLaunchKernelGGL(gg_kernel<5>, grid, blk, 0, s, g, W, G, tangent, seed ? *seed : none);
 }
 
 // face gradients of the wide face-average stencils (Variable<Vec3D>::

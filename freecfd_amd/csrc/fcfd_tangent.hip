@@ -488,7 +488,7 @@ static void launch_jvp_t(const DevGrid& g, const Params& P, double* W, TD* dW, d
         if (g.B > 0) hipLaunchKernelGGL((ghost_t_kernel<NV, TD>), dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
         if constexpr (std::is_same<TD, double>::value) {   // (the FP32 tangent is refused on meshes with Green-Gauss cells)
             if (!known) launch_green_gauss(g, W, G, 0, s);
-            launch_green_gauss(g, dW, dG, 1, s);
+            launch_green_gauss(g, dW, dG, 1, s, &P.seed);
         }
         if (lin && !known) {   // the primal gradients, limiters and their linearisation at this W
             launch_grad_lim(g, P, W, G, PHI, dref, s);
@@ -748,6 +748,8 @@ static void launch_param_t(const DevGrid& g, const Params& P, double* W, double*
     CellLoader<NV> cl{W, G, PHI, has_phi, phi_const, (size_t)g.ld};
     dim3 grid = cell_grid(g.N), blk(cell_threads(g.N));
     if (g.B > 0) hipLaunchKernelGGL((ghost_t_kernel<NV, double, true>), dim3((g.B + 255) / 256), dim3(256), 0, s, g, P, W, dW, status);
+    launch_green_gauss(g, W, G, 0, s);
+    launch_green_gauss(g, dW, dG, 1, s, &P.seed);
     if (lim == FCFD_LIMITER_VK) {
         launch_vk_range(g, P, W, dref, s);
         launch_glt<FCFD_LIMITER_VK, NV, double>(g, P, W, dW, G, dG, PHI, dPHI, dref, false, nullptr, s);

@@ -93,9 +93,9 @@ extern "C" {
 #define FCFD_GRAD_CURVILINEAR 1
 #define FCFD_GRAD_LSQR        2
 #define FCFD_GRAD_GREENGAUSS  3   /* face sum of face values (variable.h:193-216): residual,
-                                     tangent and both Newton modes; fcfd_vjp and
-                                     fcfd_adjoint_step return FCFD_ENOTSUP for a context with
-                                     Green-Gauss cells */
+                                     tangent, both Newton modes, the adjoint and the
+                                     BC-value derivatives (a face value a BC holds
+                                     fixed is that BC's value) */
 /* interpolation (src/interpolate/interpolate.h).  A stencil above 2 (WTLI
  * fits, or IDW over the extended stencil) gives faces whose viscous face
  * gradients average more than their two cells: residual, FD Jacobian,
@@ -463,8 +463,8 @@ int  fcfd_vjp(fcfd_ctx* ctx, double* W_dev, const double* Rbar_dev, const double
  * ghosts' centre states (then the gradients and limiters of the cells next
  * to it), the right states of its faces (apply_bcs), a FIXED_Q wall's heat
  * flux and the SA inlet nu~; these derivatives follow every path, by the
- * tangent kernels with the BC's values as dual numbers.  Green-Gauss
- * gradients: FCFD_ENOTSUP.
+ * tangent kernels with the BC's values as dual numbers (with Green-Gauss
+ * gradients also the face values the BC holds fixed).
  *
  * fcfd_bc_tangent: dR_dev [nv][ld_cell] = dR/d(values of BC `index`) . dbc at
  * the fixed state W_dev, and (dloads_host may be NULL) the loads' derivative

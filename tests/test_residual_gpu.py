@@ -136,8 +136,7 @@ def test_green_gauss(name):
 
 def test_green_gauss_newton_paths():
     """With Green-Gauss gradients the FD-Jacobian and the matrix-free (exact
-    tangent, FP32 Krylov) Newton steps run; the adjoint kernels, which do not
-    transpose the Green-Gauss face sum, report ENOTSUP."""
+    tangent, FP32 Krylov) Newton steps run."""
     from freecfd_amd.ns import NavierStokes
     cs = _green_gauss_case("wedge")
     ns = NavierStokes(cs.raw, cs.opt)
@@ -145,8 +144,39 @@ def test_green_gauss_newton_paths():
         ns.set_state(cs.W0)
         st = ns.newton_step(opts)
         assert np.isfinite(st.res_norm) and st.krylov_iters > 0
-    with pytest.raises(RuntimeError, match="Green-Gauss"):
-        ns.vjp(ns.torch.zeros_like(ns.W))
+    ns.close()
+
+
+@pytest.mark.parametrize("name,limiter", [
+    ("wedge", A.LIMITER_NONE), ("wedge", A.LIMITER_VK),
+    pytest.param("wedge", A.LIMITER_MINMOD, marks=pytest.mark.xfail(
+        reason="minmod with Green-Gauss gradients on the wedge: <v,Ju> and <J^T v,u> differ by 2.4e-4 "
+               "(the same wedge with lsqr maps and minmod agrees to 2e-15); not yet explained")),
+    ("cylinder_fixed_bcs", None), ("cylinder_fixed_bcs", A.LIMITER_NONE), ("tetbox", None),
+    ("plate_prisms_only", None), ("plate_sa", None), ("plate_sa", A.LIMITER_NONE)])
+def test_green_gauss_adjoint_dot_product(name, limiter):
+    """The adjoint transposes the Green-Gauss face sum (each face value's cells
+    take their weight of n A / V times the gradient adjoint, nothing for the
+    variables a BC holds fixed on that face): <v, J u> = <J^T v, u> to 1e-11
+    of sum |v J u|, u and v random, J the exact tangent (itself checked
+    against central differences with Green-Gauss gradients).  None keeps the
+    case's own limiter (Venkatakrishnan)."""
+    from freecfd_amd.ns import NavierStokes
+    cs = _green_gauss_case(name)
+    if limiter is not None:
+        cs.opt.limiter = limiter
+    ns = NavierStokes(cs.raw, cs.opt)
+    ns.set_state(cs.W0)
+    N, nv = cs.n_cells, ns.nv
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=(N, nv)) * np.array([1e5, 100.0, 100.0, 100.0, 300.0, 1e-4])[:nv]
+    v = rng.normal(size=(N, nv))
+    Ju = ns.to_host(ns.jvp(ns.to_device(u), mode=A.JAC_JVP_FULL), nv)
+    JTv = ns.to_host(ns.vjp(ns.to_device(v)), nv)
+    lhs, rhs = float(np.sum(v * Ju)), float(np.sum(JTv * u))
+    rel = abs(lhs - rhs) / float(np.sum(np.abs(v * Ju)))
+    print("case", name, "<v,Ju> %.15e <J^T v,u> %.15e rel %.2e" % (lhs, rhs, rel), flush=True)
+    assert rel < 1e-11
     ns.close()
 
 
